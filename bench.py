@@ -1,0 +1,248 @@
+#!/usr/bin/env python3
+"""NICAM tracer-transport + vertical-implicit step on one MI355X.
+
+A step is one pass of the hot path over the glevel-7 workload: 40 regions of
+66x66 cells (1-cell halo), 98 levels (96 layers, dz stretched 100 m -> 2 km),
+plus the two pole regions, FP64 throughout:
+
+    horizontal_flux -> horizontal_limiter_thuburn -> vertical_limiter_thuburn
+    -> vi_rhow_solver
+
+All inputs are seeded synthetic fields already resident in HBM when the timed
+region starts.  Prints one JSON line (metric, value, roofline, cpu_baseline...).
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import platform
+import sys
+import time
+from pathlib import Path
+
+REPO = Path(__file__).resolve().parent
+sys.path.insert(0, str(REPO))
+
+METRIC = "NICAM adv+limiter+VI step: cell-levels/s & %HBM peak; adjoint/fwd ratio"
+KERNELS = ("horizontal_flux", "horizontal_limiter_thuburn", "vertical_limiter_thuburn", "vi_rhow_solver")
+DT_LARGE, DT_SMALL = 360.0, 60.0  # SET_dt_large / SET_dt_small (problem_size.inc:136-137)
+
+
+def log(msg: str) -> None:
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--seed", type=int, default=2024)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-regions", type=int, default=40, help="regions in the CPU baseline sample")
+    return ap.parse_args()
+
+
+def cpu_baseline(spec, case, regions: int):
+    """The reference's own Fortran kernels (oracle/_ref, one thread) on a
+    sample of the same workload; falls back to the C restatement."""
+    import numpy as np
+
+    sys.path.insert(0, str(REPO / "tests"))
+    from oracle import binding
+    from parity import region_slice
+
+    from icoatmosbenchmark_v1_amd import synth
+
+    sub = synth.GridSpec(spec.gall_1d, spec.kall, regions, seed=spec.seed)
+    pick = lambda d: region_slice(d, list(range(regions)), pole=True)  # noqa: E731
+    geo = pick(case["geo"])
+    flux, hlim, vlim, vi = pick(case["flux"]), pick(case["hlim"]), pick(case["vlim"]), pick(case["vi"])
+    if not binding.ORACLE_SO.exists():
+        binding.build(with_ref=False)
+    orc = binding.Oracle()
+    fac = orc.grd_setup(case["vi"]["gz"], case["vi"]["gzh"])  # GRD_setup, outside the timed part
+    kind = "reference" if binding.RefLibs.available() else "port"
+    if kind == "reference":
+        impl, cores = binding.RefLibs(), 1
+    else:
+        impl, cores = orc, orc.num_threads()
+    times = {}
+    t = time.perf_counter()
+    impl.horizontal_flux(sub, geo, flux, DT_LARGE)
+    times["horizontal_flux"] = time.perf_counter() - t
+    t = time.perf_counter()
+    impl.horizontal_limiter_thuburn(sub, geo["have_sgp"], hlim)
+    times["horizontal_limiter_thuburn"] = time.perf_counter() - t
+    t = time.perf_counter()
+    impl.vertical_limiter_thuburn(sub, vlim)
+    times["vertical_limiter_thuburn"] = time.perf_counter() - t
+    t = time.perf_counter()
+    impl.vi_rhow_solver(sub, vi, DT_SMALL, fac["rdgzh"], fac["afact"], fac["bfact"])
+    times["vi_rhow_solver"] = time.perf_counter() - t
+    total = sum(times.values())
+    cpu_model = ""
+    try:
+        for ln in open("/proc/cpuinfo"):
+            if ln.startswith("model name"):
+                cpu_model = ln.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return dict(value=sub.inner_cell_levels() / total, unit="cell-levels/s", cores=cores, kind=kind,
+                sample=(f"{regions} of 40 regions (66x66 cells, 98 levels) + poles, one pass of the four kernels, "
+                        f"{total:.2f} s wall; {'reference Fortran built with flang -O2, 1 thread' if kind == 'reference' else 'C restatement, OpenMP'}"
+                        f"; {cpu_model}"),
+                seconds={k: round(v, 4) for k, v in times.items()})
+
+
+def main() -> int:
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported (the hot path is built and "
+              f"measured on a single MI355X; multi-GPU sharding is a design note only)", file=sys.stderr)
+        return 2
+    import numpy as np
+    import torch
+
+    from icoatmosbenchmark_v1_amd import drivers, perfmodel, runtime, synth
+
+    if not torch.cuda.is_available():
+        print("bench.py needs a GPU", file=sys.stderr)
+        return 2
+    if not runtime.LIB_PATH.exists():
+        from icoatmosbenchmark_v1_amd import _build
+
+        _build.build_library()
+    dev = torch.device("cuda:0")
+    spec = synth.GridSpec(66, 98, 40, seed=args.seed)
+    t0 = time.perf_counter()
+    case = synth.make_case(spec)
+    log(f"inputs generated in {time.perf_counter() - t0:.1f} s")
+    geo, vi = case["geo"], case["vi"]
+    ctx = drivers.setup_context(spec, geo, vi)
+    stream = torch.cuda.current_stream(dev)
+    ctx.set_stream(stream)
+
+    def up(d, keys):
+        return {k: torch.from_numpy(np.ascontiguousarray(d[k])).to(dev) for k in keys}
+
+    F = up(case["flux"], drivers.FLUX_IN)
+    F["flx_h"] = torch.zeros((6,) + spec.shape3, dtype=torch.float64, device=dev)
+    F["flx_h_pl"] = torch.zeros(spec.shape_pl, dtype=torch.float64, device=dev)
+    F["GRD_xc"] = torch.zeros((3, 3) + spec.shape3, dtype=torch.float64, device=dev)
+    F["GRD_xc_pl"] = torch.zeros((3,) + spec.shape_pl, dtype=torch.float64, device=dev)
+    H = up(case["hlim"], drivers.HLIM_IN + ("q_a", "q_a_pl"))
+    H["Qout_prev"] = torch.zeros((2,) + spec.shape3, dtype=torch.float64, device=dev)
+    H["Qout_prev_pl"] = torch.zeros((2,) + spec.shape_pl, dtype=torch.float64, device=dev)
+    V = up(case["vlim"], drivers.VLIM_IN + ("q_h", "q_h_pl"))
+    vkeys = [k for k in drivers.VI_IN] + [k + "_pl" for k in drivers.VI_IN] + ["rhogw", "rhogw_pl"]
+    W = up(vi, vkeys)
+    torch.cuda.synchronize()
+    log(f"device memory in use: {torch.cuda.memory_allocated() / 1e9:.2f} GB + context {ctx.device_bytes() / 1e9:.2f} GB")
+
+    def k_flux():
+        ctx.horizontal_flux(F["flx_h"], F["flx_h_pl"], F["GRD_xc"], F["GRD_xc_pl"], *[F[k] for k in drivers.FLUX_IN],
+                            DT_LARGE)
+
+    def k_hlim():
+        ctx.horizontal_limiter_thuburn(H["q_a"], H["q_a_pl"], H["q"], H["q_pl"], H["d"], H["d_pl"], H["ch"],
+                                       H["ch_pl"], H["cmask"], H["cmask_pl"], H["Qout_prev"], H["Qout_prev_pl"],
+                                       H["Qout_post"], H["Qout_post_pl"])
+
+    def k_vlim():
+        ctx.vertical_limiter_thuburn(V["q_h"], V["q_h_pl"], V["q"], V["q_pl"], V["d"], V["d_pl"], V["ck"],
+                                     V["ck_pl"])
+
+    def k_vi():
+        args = []
+        for k in drivers.VI_IN:
+            args += [W[k], W[k + "_pl"]]
+        ctx.vi_rhow_solver(W["rhogw"], W["rhogw_pl"], *args, DT_SMALL)
+
+    calls = (k_flux, k_hlim, k_vlim, k_vi)
+    for _ in range(args.warmup):
+        for f in calls:
+            f()
+    torch.cuda.synchronize()
+
+    ev = [[torch.cuda.Event(enable_timing=True) for _ in range(len(calls) + 1)] for _ in range(args.steps)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for s in range(args.steps):
+        ev[s][0].record(stream)
+        for i, f in enumerate(calls):
+            f()
+            ev[s][i + 1].record(stream)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    ms_step = (t1 - t0) * 1e3 / args.steps
+    kms = {name: float(np.mean([ev[s][i].elapsed_time(ev[s][i + 1]) for s in range(args.steps)]))
+           for i, name in enumerate(KERNELS)}
+    log("per-kernel ms: " + ", ".join(f"{k}={v:.4f}" for k, v in kms.items()))
+
+    inner = spec.inner_cell_levels()
+    kb = perfmodel.kernel_bytes(spec)
+    step_bytes = sum(kb.values())
+    kernels = {}
+    for name in KERNELS:
+        gbs = kb[name] / (kms[name] * 1e-3) / 1e9
+        kernels[name] = dict(ms=round(kms[name], 5), alg_bytes=kb[name], gbs=round(gbs, 1),
+                             frac=round(gbs / perfmodel.HBM_PEAK_GBS, 4))
+    dom = max(KERNELS, key=lambda k: kms[k])
+    traffic = None
+    tfile = REPO / "profiles" / "traffic_gl7_fp64.json"
+    if tfile.exists():
+        try:
+            traffic = json.loads(tfile.read_text()).get(dom)
+        except (ValueError, OSError):
+            traffic = None
+    roof = dict(bound="hbm", kernel=dom, achieved=kernels[dom]["gbs"], peak=perfmodel.HBM_PEAK_GBS, unit="GB/s",
+                frac=kernels[dom]["frac"], traffic=traffic)
+    step_gbs = step_bytes / (ms_step * 1e-3) / 1e9
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        try:
+            log(f"CPU baseline on {args.cpu_regions} regions ...")
+            cpu = cpu_baseline(spec, case, args.cpu_regions)
+            log(f"CPU baseline: {cpu['value']:.4g} cell-levels/s ({cpu['kind']}, {cpu['cores']} core(s))")
+        except Exception as e:  # reported, never fatal
+            log(f"CPU baseline failed: {e!r}")
+            cpu = dict(value=None, unit="cell-levels/s", cores=None, kind=None, sample=f"failed: {e!r}")
+
+    out = dict(
+        metric=METRIC,
+        value=inner / (ms_step * 1e-3),
+        unit="cell-levels/s",
+        n_gpus=1,
+        steps=args.steps,
+        warmup=args.warmup,
+        ms_per_step=ms_step,
+        higher_is_better=True,
+        scaling="weak",
+        vs_baseline=None,
+        dtype="f64",
+        data="synthetic (seeded fields and metrics)",
+        config=dict(workload="NICAM gl7 rl1: 40 regions x 66x66 cells x 98 levels + 2 poles, FP64, "
+                             "horizontal_flux + horizontal_limiter_thuburn + vertical_limiter_thuburn + vi_rhow_solver",
+                    gall_1d=66, kall=98, lall=40, inner_cell_levels=inner, array_cell_levels=spec.array_cell_levels(),
+                    tracers=1),
+        hbm_frac_step=round(step_gbs / perfmodel.HBM_PEAK_GBS, 4),
+        step_alg_bytes=step_bytes,
+        adjoint_fwd_ratio=None,
+        kernels=kernels,
+        roofline=roof,
+        cpu_baseline=cpu,
+        host=platform.node(),
+    )
+    print(json.dumps(out), flush=True)
+    ctx.close()
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,534 @@
+// Context, set-up calls, pointer-mode staging and the exported C entry points.
+#include <hip/hip_runtime.h>
+
+#include <cfloat>
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "internal.h"
+
+namespace nicam {
+
+static thread_local std::string g_last_error;
+
+int fail(int code, const std::string &msg)
+{
+    g_last_error = msg;
+    return code;
+}
+
+int hip_fail(hipError_t e, const char *what)
+{
+    std::string m = std::string(what) + ": " + hipGetErrorName(e) + " (" + hipGetErrorString(e) + ")";
+    if (e == hipErrorOutOfMemory) return fail(NICAM_ERR_NOMEM, m);
+    return fail(NICAM_ERR_HIP, m);
+}
+
+static int dev_alloc(nicam_ctx c, double **p, size_t n)
+{
+    if (*p) return NICAM_OK;
+    void *q = nullptr;
+    hipError_t e = hipMalloc(&q, n * sizeof(double));
+    if (e != hipSuccess) return hip_fail(e, "hipMalloc");
+    *p = static_cast<double *>(q);
+    c->device_bytes += n * sizeof(double);
+    return NICAM_OK;
+}
+
+static void dev_free(double *&p)
+{
+    if (p) (void)hipFree(p);
+    p = nullptr;
+}
+
+// Copies n doubles from a host or device pointer (unified addressing decides).
+static int copy_in(nicam_ctx c, double *dst, const double *src, size_t n)
+{
+    NICAM_HIP(hipMemcpyAsync(dst, src, n * sizeof(double), hipMemcpyDefault, c->stream));
+    return NICAM_OK;
+}
+
+// One argument of an entry point in NICAM_PTR_HOST mode.
+struct HostArg {
+    const void *host;  // may be null (absent pole array)
+    size_t bytes;
+    bool out;          // copied back after the call
+};
+
+// Maps host arguments onto per-slot device staging buffers; every argument is
+// copied in (outputs too, since the kernels leave parts of them untouched).
+static int stage_in(nicam_ctx c, const std::vector<HostArg> &args, std::vector<void *> &dev)
+{
+    if (c->stage.size() < args.size()) {
+        c->stage.resize(args.size(), nullptr);
+        c->stage_bytes.resize(args.size(), 0);
+    }
+    dev.assign(args.size(), nullptr);
+    for (size_t i = 0; i < args.size(); ++i) {
+        if (!args[i].host) continue;
+        if (c->stage_bytes[i] < args[i].bytes) {
+            if (c->stage[i]) {
+                (void)hipFree(c->stage[i]);
+                c->device_bytes -= c->stage_bytes[i];
+            }
+            c->stage[i] = nullptr;
+            c->stage_bytes[i] = 0;
+            NICAM_HIP(hipMalloc(&c->stage[i], args[i].bytes));
+            c->stage_bytes[i] = args[i].bytes;
+            c->device_bytes += args[i].bytes;
+        }
+        dev[i] = c->stage[i];
+        NICAM_HIP(hipMemcpyAsync(dev[i], args[i].host, args[i].bytes, hipMemcpyHostToDevice, c->stream));
+    }
+    return NICAM_OK;
+}
+
+static int stage_out(nicam_ctx c, const std::vector<HostArg> &args, const std::vector<void *> &dev)
+{
+    for (size_t i = 0; i < args.size(); ++i)
+        if (args[i].host && args[i].out)
+            NICAM_HIP(hipMemcpyAsync(const_cast<void *>(args[i].host), dev[i], args[i].bytes,
+                                     hipMemcpyDeviceToHost, c->stream));
+    NICAM_HIP(hipStreamSynchronize(c->stream));
+    return NICAM_OK;
+}
+
+static int check_ctx(nicam_ctx c)
+{
+    if (!c) return fail(NICAM_ERR_ARG, "null context");
+    NICAM_HIP(hipSetDevice(c->device));
+    return NICAM_OK;
+}
+
+static int need(const void *p, const char *name)
+{
+    if (!p) return fail(NICAM_ERR_ARG, std::string("null pointer argument: ") + name);
+    return NICAM_OK;
+}
+
+#define NICAM_TRY(x)                  \
+    do {                              \
+        int rc_ = (x);                \
+        if (rc_ != NICAM_OK) return rc_; \
+    } while (0)
+
+}  // namespace nicam
+
+using namespace nicam;
+
+extern "C" {
+
+const char *nicam_version(void) { return "nicam_tt 0.1 gfx950"; }
+
+const char *nicam_last_error(void) { return nicam::g_last_error.c_str(); }
+
+int nicam_ctx_create(nicam_ctx *out, int gall_1d, int kall, int lall, int gall_pl, int have_pl,
+                     const int *have_sgp)
+{
+    if (!out) return fail(NICAM_ERR_ARG, "null output handle");
+    *out = nullptr;
+    if (gall_1d < 4 || kall < 4 || lall < 1 || gall_pl < 3 || gall_pl > 15)
+        return fail(NICAM_ERR_ARG, "sizes out of range: need gall_1d>=4, kall>=4, lall>=1, 3<=gall_pl<=15");
+    if ((size_t)gall_1d * gall_1d * kall * lall * 12 > ((size_t)1 << 40))
+        return fail(NICAM_ERR_ARG, "grid too large");
+    if (!have_sgp) return fail(NICAM_ERR_ARG, "null have_sgp");
+    auto *c = new (std::nothrow) nicam_ctx_s();
+    if (!c) return fail(NICAM_ERR_NOMEM, "host allocation failed");
+    c->d.gall_1d = gall_1d;
+    c->d.gall = gall_1d * gall_1d;
+    c->d.kall = kall;
+    c->d.lall = lall;
+    c->d.gall_pl = gall_pl;
+    c->d.have_pl = have_pl ? 1 : 0;
+    c->have_sgp_h.assign(have_sgp, have_sgp + lall);
+    for (int &v : c->have_sgp_h) v = v ? 1 : 0;
+    c->eps = DBL_EPSILON;
+    c->big = DBL_MAX;
+    hipError_t e = hipGetDevice(&c->device);
+    if (e == hipSuccess) e = hipMalloc(&c->have_sgp, sizeof(int) * lall);
+    if (e == hipSuccess)
+        e = hipMemcpy(c->have_sgp, c->have_sgp_h.data(), sizeof(int) * lall, hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        int rc = hip_fail(e, "context set-up");
+        nicam_ctx_destroy(c);
+        return rc;
+    }
+    *out = c;
+    return NICAM_OK;
+}
+
+int nicam_ctx_destroy(nicam_ctx c)
+{
+    if (!c) return NICAM_OK;
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    else (void)hipDeviceSynchronize();
+    double **all[] = {&c->GRD_xr, &c->GRD_xr_pl, &c->GMTR_p, &c->GMTR_p_pl, &c->GMTR_t,
+                      &c->GMTR_t_pl, &c->GMTR_a, &c->GMTR_a_pl, &c->RGAM, &c->RGAM_pl,
+                      &c->RGAMH, &c->RGAMH_pl, &c->RGSGAM2, &c->RGSGAM2_pl, &c->RGSGAM2H,
+                      &c->RGSGAM2H_pl, &c->GSGAM2H, &c->GSGAM2H_pl, &c->rdgzh, &c->afact,
+                      &c->bfact, &c->Mc, &c->Mc_pl, &c->Ml, &c->Ml_pl, &c->Mu, &c->Mu_pl};
+    for (double **p : all) dev_free(*p);
+    for (void *p : c->stage)
+        if (p) (void)hipFree(p);
+    if (c->have_sgp) (void)hipFree(c->have_sgp);
+    delete c;
+    return NICAM_OK;
+}
+
+int nicam_ctx_set_stream(nicam_ctx c, void *s)
+{
+    NICAM_TRY(check_ctx(c));
+    c->stream = static_cast<hipStream_t>(s);
+    return NICAM_OK;
+}
+
+int nicam_ctx_set_pointer_mode(nicam_ctx c, int mode)
+{
+    NICAM_TRY(check_ctx(c));
+    if (mode != NICAM_PTR_DEVICE && mode != NICAM_PTR_HOST)
+        return fail(NICAM_ERR_ARG, "pointer mode must be NICAM_PTR_DEVICE or NICAM_PTR_HOST");
+    c->ptr_mode = mode;
+    return NICAM_OK;
+}
+
+size_t nicam_ctx_device_bytes(nicam_ctx c) { return c ? c->device_bytes : 0; }
+
+int nicam_const_setup(nicam_ctx c)
+{
+    NICAM_TRY(check_ctx(c));
+    c->eps = DBL_EPSILON;
+    c->big = DBL_MAX;
+    return NICAM_OK;
+}
+
+int nicam_set_const(nicam_ctx c, double eps, double huge_val)
+{
+    NICAM_TRY(check_ctx(c));
+    if (!(eps >= 0.0) || !(huge_val > 0.0)) return fail(NICAM_ERR_ARG, "eps must be >= 0 and huge > 0");
+    c->eps = eps;
+    c->big = huge_val;
+    return NICAM_OK;
+}
+
+int nicam_set_phys_const(nicam_ctx c, double grav, double rdry, double cvdry, int non_hydro_alpha)
+{
+    NICAM_TRY(check_ctx(c));
+    if (!(rdry != 0.0)) return fail(NICAM_ERR_ARG, "Rdry must be nonzero");
+    c->grav = grav;
+    c->rdry = rdry;
+    c->cvdry = cvdry;
+    c->alpha = non_hydro_alpha;
+    return NICAM_OK;
+}
+
+// ---- vertical grid ---------------------------------------------------------
+
+static uint32_t be32(const unsigned char *p)
+{
+    return (uint32_t)p[0] << 24 | (uint32_t)p[1] << 16 | (uint32_t)p[2] << 8 | (uint32_t)p[3];
+}
+
+static double be64d(const unsigned char *p)
+{
+    uint64_t u = 0;
+    for (int i = 0; i < 8; ++i) u = u << 8 | p[i];
+    double v;
+    std::memcpy(&v, &u, 8);
+    return v;
+}
+
+// One Fortran sequential unformatted record with 4-byte big-endian markers.
+static int read_record(FILE *f, std::vector<unsigned char> &buf, const char *fname)
+{
+    unsigned char m[4], m2[4];
+    if (std::fread(m, 1, 4, f) != 4) return fail(NICAM_ERR_IO, std::string("short vgrid file: ") + fname);
+    uint32_t n = be32(m);
+    if (n > (1u << 26)) return fail(NICAM_ERR_IO, std::string("bad record marker in ") + fname);
+    buf.resize(n);
+    if (std::fread(buf.data(), 1, n, f) != n || std::fread(m2, 1, 4, f) != 4 || be32(m2) != n)
+        return fail(NICAM_ERR_IO, std::string("truncated record in ") + fname);
+    return NICAM_OK;
+}
+
+int nicam_grd_input_vgrid(const char *fname, int kall, double *gz, double *gzh)
+{
+    if (!fname || !gz || !gzh) return fail(NICAM_ERR_ARG, "null argument");
+    if (kall < 4) return fail(NICAM_ERR_ARG, "kall must be >= 4");
+    FILE *f = std::fopen(fname, "rb");
+    if (!f) return fail(NICAM_ERR_IO, std::string("No vertical grid file : ") + fname);
+    std::vector<unsigned char> r;
+    int rc = read_record(f, r, fname);
+    if (rc == NICAM_OK && r.size() != 4) rc = fail(NICAM_ERR_IO, "first vgrid record is not one int32");
+    if (rc == NICAM_OK && (int)be32(r.data()) != kall - 2)
+        rc = fail(NICAM_ERR_IO, "inconsistency in number of vertical layers");
+    for (int rec = 0; rec < 2 && rc == NICAM_OK; ++rec) {
+        rc = read_record(f, r, fname);
+        if (rc == NICAM_OK && r.size() != (size_t)kall * 8) rc = fail(NICAM_ERR_IO, "vgrid record size != 8*kall");
+        if (rc == NICAM_OK)
+            for (int k = 0; k < kall; ++k) (rec == 0 ? gz : gzh)[k] = be64d(&r[(size_t)k * 8]);
+    }
+    std::fclose(f);
+    return rc;
+}
+
+int nicam_grd_setup(nicam_ctx c, const double *gz, const double *gzh, double *rdgzh_out,
+                    double *afact_out, double *bfact_out)
+{
+    NICAM_TRY(check_ctx(c));
+    NICAM_TRY(need(gz, "gz"));
+    NICAM_TRY(need(gzh, "gzh"));
+    const int kall = c->d.kall, kmin = 1, kmax = kall - 2;  // 0-based kmin/kmax
+    std::vector<double> dgzh(kall), rdgzh(kall), afact(kall), bfact(kall);
+    // mod_misc.f90:486-490 and :493-496 (cell-wall intervals and their inverse)
+    for (int k = kmin; k <= kmax + 1; ++k) dgzh[k] = gz[k] - gz[k - 1];
+    dgzh[kmin - 1] = dgzh[kmin];
+    for (int k = 0; k < kall; ++k) rdgzh[k] = 1.0 / dgzh[k];
+    // mod_misc.f90:505-510 (cell-centre -> cell-wall interpolation factors)
+    for (int k = kmin; k <= kmax + 1; ++k) afact[k] = (gzh[k] - gz[k - 1]) / (gz[k] - gz[k - 1]);
+    afact[kmin - 1] = 1.0;
+    for (int k = 0; k < kall; ++k) bfact[k] = 1.0 - afact[k];
+    for (int k = 0; k < kall; ++k) {
+        if (!std::isfinite(rdgzh[k]) || !std::isfinite(afact[k]))
+            return fail(NICAM_ERR_ARG, "vertical grid has coincident levels");
+    }
+    if (rdgzh_out) std::memcpy(rdgzh_out, rdgzh.data(), sizeof(double) * kall);
+    if (afact_out) std::memcpy(afact_out, afact.data(), sizeof(double) * kall);
+    if (bfact_out) std::memcpy(bfact_out, bfact.data(), sizeof(double) * kall);
+    return nicam_set_vgrid_factors(c, rdgzh.data(), afact.data(), bfact.data());
+}
+
+int nicam_set_vgrid_factors(nicam_ctx c, const double *rdgzh, const double *afact, const double *bfact)
+{
+    NICAM_TRY(check_ctx(c));
+    NICAM_TRY(need(rdgzh, "rdgzh"));
+    NICAM_TRY(need(afact, "afact"));
+    NICAM_TRY(need(bfact, "bfact"));
+    const size_t k = c->d.kall;
+    NICAM_TRY(dev_alloc(c, &c->rdgzh, k));
+    NICAM_TRY(dev_alloc(c, &c->afact, k));
+    NICAM_TRY(dev_alloc(c, &c->bfact, k));
+    NICAM_TRY(copy_in(c, c->rdgzh, rdgzh, k));
+    NICAM_TRY(copy_in(c, c->afact, afact, k));
+    NICAM_TRY(copy_in(c, c->bfact, bfact, k));
+    NICAM_HIP(hipStreamSynchronize(c->stream));
+    c->have_vgrid = true;
+    return NICAM_OK;
+}
+
+// ---- metrics ------------------------------------------------------------------
+
+int nicam_set_hmetrics(nicam_ctx c, const double *xr, const double *xr_pl, const double *gp,
+                       const double *gp_pl, const double *gt, const double *gt_pl,
+                       const double *ga, const double *ga_pl)
+{
+    NICAM_TRY(check_ctx(c));
+    NICAM_TRY(need(xr, "GRD_xr"));
+    NICAM_TRY(need(gp, "GMTR_p"));
+    NICAM_TRY(need(gt, "GMTR_t"));
+    NICAM_TRY(need(ga, "GMTR_a"));
+    const size_t g = (size_t)c->d.gall * c->d.lall, p = (size_t)c->d.gall_pl * kLallPl;
+    NICAM_TRY(dev_alloc(c, &c->GRD_xr, g * 9));
+    NICAM_TRY(dev_alloc(c, &c->GMTR_p, g * kGmtrPNmax));
+    NICAM_TRY(dev_alloc(c, &c->GMTR_t, g * 2 * kGmtrTNmax));
+    NICAM_TRY(dev_alloc(c, &c->GMTR_a, g * 3 * kGmtrANmax));
+    NICAM_TRY(copy_in(c, c->GRD_xr, xr, g * 9));
+    NICAM_TRY(copy_in(c, c->GMTR_p, gp, g * kGmtrPNmax));
+    NICAM_TRY(copy_in(c, c->GMTR_t, gt, g * 2 * kGmtrTNmax));
+    NICAM_TRY(copy_in(c, c->GMTR_a, ga, g * 3 * kGmtrANmax));
+    if (c->d.have_pl) {
+        NICAM_TRY(need(xr_pl, "GRD_xr_pl"));
+        NICAM_TRY(need(gp_pl, "GMTR_p_pl"));
+        NICAM_TRY(need(gt_pl, "GMTR_t_pl"));
+        NICAM_TRY(need(ga_pl, "GMTR_a_pl"));
+        NICAM_TRY(dev_alloc(c, &c->GRD_xr_pl, p * 3));
+        NICAM_TRY(dev_alloc(c, &c->GMTR_p_pl, p * kGmtrPNmax));
+        NICAM_TRY(dev_alloc(c, &c->GMTR_t_pl, p * kGmtrTNmax));
+        NICAM_TRY(dev_alloc(c, &c->GMTR_a_pl, p * kGmtrANmaxPl));
+        NICAM_TRY(copy_in(c, c->GRD_xr_pl, xr_pl, p * 3));
+        NICAM_TRY(copy_in(c, c->GMTR_p_pl, gp_pl, p * kGmtrPNmax));
+        NICAM_TRY(copy_in(c, c->GMTR_t_pl, gt_pl, p * kGmtrTNmax));
+        NICAM_TRY(copy_in(c, c->GMTR_a_pl, ga_pl, p * kGmtrANmaxPl));
+    }
+    NICAM_HIP(hipStreamSynchronize(c->stream));
+    c->have_hmetrics = true;
+    return NICAM_OK;
+}
+
+static int set_pair(nicam_ctx c, double **dst, double **dst_pl, const double *src, const double *src_pl,
+                    const char *name)
+{
+    NICAM_TRY(need(src, name));
+    NICAM_TRY(dev_alloc(c, dst, c->d.nreg()));
+    NICAM_TRY(copy_in(c, *dst, src, c->d.nreg()));
+    if (c->d.have_pl) {
+        NICAM_TRY(need(src_pl, name));
+        NICAM_TRY(dev_alloc(c, dst_pl, c->d.npl()));
+        NICAM_TRY(copy_in(c, *dst_pl, src_pl, c->d.npl()));
+    }
+    return NICAM_OK;
+}
+
+int nicam_set_vmetrics(nicam_ctx c, const double *RGAM, const double *RGAM_pl, const double *RGAMH,
+                       const double *RGAMH_pl, const double *RGSGAM2, const double *RGSGAM2_pl,
+                       const double *RGSGAM2H, const double *RGSGAM2H_pl, const double *GSGAM2H,
+                       const double *GSGAM2H_pl)
+{
+    NICAM_TRY(check_ctx(c));
+    NICAM_TRY(set_pair(c, &c->RGAM, &c->RGAM_pl, RGAM, RGAM_pl, "VMTR_RGAM"));
+    NICAM_TRY(set_pair(c, &c->RGAMH, &c->RGAMH_pl, RGAMH, RGAMH_pl, "VMTR_RGAMH"));
+    NICAM_TRY(set_pair(c, &c->RGSGAM2, &c->RGSGAM2_pl, RGSGAM2, RGSGAM2_pl, "VMTR_RGSGAM2"));
+    NICAM_TRY(set_pair(c, &c->RGSGAM2H, &c->RGSGAM2H_pl, RGSGAM2H, RGSGAM2H_pl, "VMTR_RGSGAM2H"));
+    NICAM_TRY(set_pair(c, &c->GSGAM2H, &c->GSGAM2H_pl, GSGAM2H, GSGAM2H_pl, "VMTR_GSGAM2H"));
+    NICAM_HIP(hipStreamSynchronize(c->stream));
+    c->have_vmetrics = true;
+    return NICAM_OK;
+}
+
+int nicam_vi_rhow_solver_putcoef(nicam_ctx c, const double *Mc, const double *Mc_pl, const double *Ml,
+                                 const double *Ml_pl, const double *Mu, const double *Mu_pl)
+{
+    NICAM_TRY(check_ctx(c));
+    NICAM_TRY(set_pair(c, &c->Mc, &c->Mc_pl, Mc, Mc_pl, "Mc"));
+    NICAM_TRY(set_pair(c, &c->Ml, &c->Ml_pl, Ml, Ml_pl, "Ml"));
+    NICAM_TRY(set_pair(c, &c->Mu, &c->Mu_pl, Mu, Mu_pl, "Mu"));
+    NICAM_HIP(hipStreamSynchronize(c->stream));
+    c->have_coef = true;
+    return NICAM_OK;
+}
+
+int nicam_ctx_synchronize(nicam_ctx c)
+{
+    NICAM_TRY(check_ctx(c));
+    NICAM_HIP(hipStreamSynchronize(c->stream));
+    return NICAM_OK;
+}
+
+// ---- hot path entry points ---------------------------------------------------
+
+#define PL(p) (c->d.have_pl ? (p) : nullptr)
+
+int nicam_horizontal_flux(nicam_ctx c, double *flx_h, double *flx_h_pl, double *xc, double *xc_pl,
+                          const double *rho, const double *rho_pl, const double *rvx,
+                          const double *rvx_pl, const double *rvy, const double *rvy_pl,
+                          const double *rvz, const double *rvz_pl, double dt)
+{
+    NICAM_TRY(check_ctx(c));
+    if (!c->have_hmetrics) return fail(NICAM_ERR_STATE, "horizontal metrics not set (nicam_set_hmetrics)");
+    const void *req[] = {flx_h, xc, rho, rvx, rvy, rvz};
+    const char *nm[] = {"flx_h", "GRD_xc", "rho", "rhovx", "rhovy", "rhovz"};
+    for (int i = 0; i < 6; ++i) NICAM_TRY(need(req[i], nm[i]));
+    if (c->d.have_pl) {
+        const void *rp[] = {flx_h_pl, xc_pl, rho_pl, rvx_pl, rvy_pl, rvz_pl};
+        for (int i = 0; i < 6; ++i) NICAM_TRY(need(rp[i], "pole array"));
+    }
+    if (c->ptr_mode == NICAM_PTR_DEVICE)
+        return launch_hflux(c, flx_h, PL(flx_h_pl), xc, PL(xc_pl), rho, PL(rho_pl), rvx, PL(rvx_pl),
+                            rvy, PL(rvy_pl), rvz, PL(rvz_pl), dt);
+    const size_t R = c->d.nreg() * 8, P = c->d.npl() * 8;
+    std::vector<HostArg> a = {{flx_h, R * 6, true},  {PL(flx_h_pl), P, true}, {xc, R * 9, true},
+                              {PL(xc_pl), P * 3, true}, {rho, R, false},       {PL(rho_pl), P, false},
+                              {rvx, R, false},         {PL(rvx_pl), P, false}, {rvy, R, false},
+                              {PL(rvy_pl), P, false},  {rvz, R, false},        {PL(rvz_pl), P, false}};
+    std::vector<void *> v;
+    NICAM_TRY(stage_in(c, a, v));
+    auto D = [&](int i) { return static_cast<double *>(v[i]); };
+    NICAM_TRY(launch_hflux(c, D(0), D(1), D(2), D(3), D(4), D(5), D(6), D(7), D(8), D(9), D(10), D(11), dt));
+    return stage_out(c, a, v);
+}
+
+int nicam_horizontal_limiter_thuburn(nicam_ctx c, double *q_a, double *q_a_pl, const double *q,
+                                     const double *q_pl, const double *d, const double *d_pl,
+                                     const double *ch, const double *ch_pl, const double *cmask,
+                                     const double *cmask_pl, double *Qprev, double *Qprev_pl,
+                                     const double *Qpost, const double *Qpost_pl)
+{
+    NICAM_TRY(check_ctx(c));
+    const void *req[] = {q_a, q, d, ch, cmask, Qprev, Qpost};
+    const char *nm[] = {"q_a", "q", "d", "ch", "cmask", "Qout_prev", "Qout_post"};
+    for (int i = 0; i < 7; ++i) NICAM_TRY(need(req[i], nm[i]));
+    if (c->d.have_pl) {
+        const void *rp[] = {q_a_pl, q_pl, d_pl, ch_pl, cmask_pl, Qprev_pl, Qpost_pl};
+        for (int i = 0; i < 7; ++i) NICAM_TRY(need(rp[i], "pole array"));
+    }
+    if (c->ptr_mode == NICAM_PTR_DEVICE)
+        return launch_hlim(c, q_a, PL(q_a_pl), q, PL(q_pl), d, PL(d_pl), ch, PL(ch_pl), cmask,
+                           PL(cmask_pl), Qprev, Qprev_pl, Qpost, PL(Qpost_pl));
+    const size_t R = c->d.nreg() * 8, P = c->d.npl() * 8;
+    std::vector<HostArg> a = {{q_a, R * 6, true},       {PL(q_a_pl), P, true},   {q, R, false},
+                              {PL(q_pl), P, false},     {d, R, false},           {PL(d_pl), P, false},
+                              {ch, R * 6, false},       {PL(ch_pl), P, false},   {cmask, R * 6, false},
+                              {PL(cmask_pl), P, false}, {Qprev, R * 2, true},    {Qprev_pl, P * 2, true},
+                              {Qpost, R * 2, false},    {PL(Qpost_pl), P * 2, false}};
+    std::vector<void *> v;
+    NICAM_TRY(stage_in(c, a, v));
+    auto D = [&](int i) { return static_cast<double *>(v[i]); };
+    NICAM_TRY(launch_hlim(c, D(0), D(1), D(2), D(3), D(4), D(5), D(6), D(7), D(8), D(9), D(10), D(11),
+                          D(12), D(13)));
+    return stage_out(c, a, v);
+}
+
+int nicam_vertical_limiter_thuburn(nicam_ctx c, double *q_h, double *q_h_pl, const double *q,
+                                   const double *q_pl, const double *d, const double *d_pl,
+                                   const double *ck, const double *ck_pl)
+{
+    NICAM_TRY(check_ctx(c));
+    const void *req[] = {q_h, q, d, ck};
+    const char *nm[] = {"q_h", "q", "d", "ck"};
+    for (int i = 0; i < 4; ++i) NICAM_TRY(need(req[i], nm[i]));
+    if (c->d.have_pl) {
+        const void *rp[] = {q_h_pl, q_pl, d_pl, ck_pl};
+        for (int i = 0; i < 4; ++i) NICAM_TRY(need(rp[i], "pole array"));
+    }
+    if (c->ptr_mode == NICAM_PTR_DEVICE)
+        return launch_vlim(c, q_h, PL(q_h_pl), q, PL(q_pl), d, PL(d_pl), ck, PL(ck_pl));
+    const size_t R = c->d.nreg() * 8, P = c->d.npl() * 8;
+    std::vector<HostArg> a = {{q_h, R, true},  {PL(q_h_pl), P, true}, {q, R, false},      {PL(q_pl), P, false},
+                              {d, R, false},   {PL(d_pl), P, false},  {ck, R * 2, false}, {PL(ck_pl), P * 2, false}};
+    std::vector<void *> v;
+    NICAM_TRY(stage_in(c, a, v));
+    auto D = [&](int i) { return static_cast<double *>(v[i]); };
+    NICAM_TRY(launch_vlim(c, D(0), D(1), D(2), D(3), D(4), D(5), D(6), D(7)));
+    return stage_out(c, a, v);
+}
+
+int nicam_vi_rhow_solver(nicam_ctx c, double *w, double *w_pl, const double *w0, const double *w0_pl,
+                         const double *p0, const double *p0_pl, const double *r0, const double *r0_pl,
+                         const double *Sr, const double *Sr_pl, const double *Sw, const double *Sw_pl,
+                         const double *Sp, const double *Sp_pl, double dt)
+{
+    NICAM_TRY(check_ctx(c));
+    if (!c->have_coef) return fail(NICAM_ERR_STATE, "tridiagonal coefficients not set (nicam_vi_rhow_solver_putcoef)");
+    if (!c->have_vmetrics) return fail(NICAM_ERR_STATE, "vertical metrics not set (nicam_set_vmetrics)");
+    if (!c->have_vgrid) return fail(NICAM_ERR_STATE, "vertical grid not set (nicam_grd_setup)");
+    if (!(dt != 0.0)) return fail(NICAM_ERR_ARG, "dt must be nonzero");
+    const void *req[] = {w, w0, p0, r0, Sr, Sw, Sp};
+    const char *nm[] = {"rhogw", "rhogw0", "preg0", "rhog0", "Srho", "Sw", "Spre"};
+    for (int i = 0; i < 7; ++i) NICAM_TRY(need(req[i], nm[i]));
+    if (c->d.have_pl) {
+        const void *rp[] = {w_pl, w0_pl, p0_pl, r0_pl, Sr_pl, Sw_pl, Sp_pl};
+        for (int i = 0; i < 7; ++i) NICAM_TRY(need(rp[i], "pole array"));
+    }
+    if (c->ptr_mode == NICAM_PTR_DEVICE)
+        return launch_vi(c, w, PL(w_pl), w0, PL(w0_pl), p0, PL(p0_pl), r0, PL(r0_pl), Sr, PL(Sr_pl), Sw,
+                         PL(Sw_pl), Sp, PL(Sp_pl), dt);
+    const size_t R = c->d.nreg() * 8, P = c->d.npl() * 8;
+    std::vector<HostArg> a = {{w, R, true},  {PL(w_pl), P, true},   {w0, R, false}, {PL(w0_pl), P, false},
+                              {p0, R, false}, {PL(p0_pl), P, false}, {r0, R, false}, {PL(r0_pl), P, false},
+                              {Sr, R, false}, {PL(Sr_pl), P, false}, {Sw, R, false}, {PL(Sw_pl), P, false},
+                              {Sp, R, false}, {PL(Sp_pl), P, false}};
+    std::vector<void *> v;
+    NICAM_TRY(stage_in(c, a, v));
+    auto D = [&](int i) { return static_cast<double *>(v[i]); };
+    NICAM_TRY(launch_vi(c, D(0), D(1), D(2), D(3), D(4), D(5), D(6), D(7), D(8), D(9), D(10), D(11), D(12),
+                        D(13), dt));
+    return stage_out(c, a, v);
+}
+
+#undef PL
+
+}  // extern "C"

@@ -1,0 +1,109 @@
+// Internal declarations shared by the nicam_tt translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <string>
+#include <vector>
+
+#include "nicam_tt.h"
+
+namespace nicam {
+
+// Fixed indices of the reference's data contract
+// (dyn_vi_rhow_solver/src/problem_size.inc:14-23, 38-41, 80-132).
+constexpr int kLallPl = 2;     // ADM_lall_pl
+constexpr int kGmtrPNmax = 8;  // GMTR_p_nmax
+constexpr int kGmtrTNmax = 5;  // GMTR_t_nmax
+constexpr int kGmtrANmax = 12; // GMTR_a_nmax
+constexpr int kGmtrANmaxPl = 18;
+
+// Sizes of one grid, as the kernels see them.  All indices inside the
+// kernels are 0-based: g = j*gall_1d + i, halo at i or j = 0 and gall_1d-1.
+struct Dims {
+    int gall_1d;
+    int gall;  // gall_1d^2
+    int kall;
+    int lall;
+    int gall_pl;
+    int have_pl;
+    size_t nreg() const { return (size_t)gall * kall * lall; }  // one regular field
+    size_t npl() const { return (size_t)gall_pl * kall * kLallPl; }
+};
+
+}  // namespace nicam
+
+struct nicam_ctx_s {
+    nicam::Dims d{};
+    std::vector<int> have_sgp_h;
+    int *have_sgp = nullptr;  // device [lall]
+
+    double eps = 0, big = 0;
+    double grav = 9.80616, rdry = 287.0, cvdry = 717.5;
+    int alpha = 1;
+
+    hipStream_t stream = nullptr;
+    int ptr_mode = NICAM_PTR_DEVICE;
+    int device = 0;
+
+    // horizontal metrics (device copies of the mod_misc module arrays)
+    double *GRD_xr = nullptr, *GRD_xr_pl = nullptr;
+    double *GMTR_p = nullptr, *GMTR_p_pl = nullptr;
+    double *GMTR_t = nullptr, *GMTR_t_pl = nullptr;
+    double *GMTR_a = nullptr, *GMTR_a_pl = nullptr;
+    bool have_hmetrics = false;
+
+    // vertical metrics
+    double *RGAM = nullptr, *RGAM_pl = nullptr;
+    double *RGAMH = nullptr, *RGAMH_pl = nullptr;
+    double *RGSGAM2 = nullptr, *RGSGAM2_pl = nullptr;
+    double *RGSGAM2H = nullptr, *RGSGAM2H_pl = nullptr;
+    double *GSGAM2H = nullptr, *GSGAM2H_pl = nullptr;
+    bool have_vmetrics = false;
+
+    // vertical grid factors (kall each)
+    double *rdgzh = nullptr, *afact = nullptr, *bfact = nullptr;
+    bool have_vgrid = false;
+
+    // tridiagonal coefficients (module copies of mod_vi.f90:47-52)
+    double *Mc = nullptr, *Mc_pl = nullptr;
+    double *Ml = nullptr, *Ml_pl = nullptr;
+    double *Mu = nullptr, *Mu_pl = nullptr;
+    bool have_coef = false;
+
+    // staging buffers for NICAM_PTR_HOST, one per argument slot
+    std::vector<void *> stage;
+    std::vector<size_t> stage_bytes;
+
+    size_t device_bytes = 0;
+};
+
+namespace nicam {
+
+// error reporting (thread-local last message)
+int fail(int code, const std::string &msg);
+int hip_fail(hipError_t e, const char *what);
+#define NICAM_HIP(call)                                          \
+    do {                                                         \
+        hipError_t e_ = (call);                                  \
+        if (e_ != hipSuccess) return ::nicam::hip_fail(e_, #call); \
+    } while (0)
+
+// launchers (asynchronous on ctx->stream, device pointers)
+int launch_hflux(nicam_ctx c, double *flx_h, double *flx_h_pl, double *xc, double *xc_pl,
+                 const double *rho, const double *rho_pl, const double *rvx, const double *rvx_pl,
+                 const double *rvy, const double *rvy_pl, const double *rvz, const double *rvz_pl,
+                 double dt);
+int launch_hlim(nicam_ctx c, double *q_a, double *q_a_pl, const double *q, const double *q_pl,
+                const double *d, const double *d_pl, const double *ch, const double *ch_pl,
+                const double *cmask, const double *cmask_pl, double *Qprev, double *Qprev_pl,
+                const double *Qpost, const double *Qpost_pl);
+int launch_vlim(nicam_ctx c, double *q_h, double *q_h_pl, const double *q, const double *q_pl,
+                const double *d, const double *d_pl, const double *ck, const double *ck_pl);
+int launch_vi(nicam_ctx c, double *w, double *w_pl, const double *w0, const double *w0_pl,
+              const double *p0, const double *p0_pl, const double *r0, const double *r0_pl,
+              const double *Sr, const double *Sr_pl, const double *Sw, const double *Sw_pl,
+              const double *Sp, const double *Sp_pl, double dt);
+
+}  // namespace nicam

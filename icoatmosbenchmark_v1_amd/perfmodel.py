@@ -1,0 +1,71 @@
+"""Algorithmic HBM traffic of the four kernels (bytes per launch).
+
+"Algorithmic" counts every array element the kernel must read or write at
+least once -- exactly the elements the reference loops touch -- and nothing
+else: re-reads from cache, scratch traffic and unused planes of an argument
+(cmask faces 4..6, q_a faces 4..6 on input) are excluded.  The counts follow
+the loop bounds of the reference (paths relative to kernels/NICAM/):
+
+horizontal_flux (dyn_horiz_adv_flux/src/mod_src_tracer.f90:137-320)
+    reads  rho, rhovx, rhovy, rhovz at every cell of a slab      4 x gall
+    writes flx_h (6 faces) and GRD_xc (3 arcs x 3) at every cell 15 x gall
+    reads  25 K-invariant metrics per cell once per region
+           (GMTR_t W1..W3 x TI/TJ, GMTR_a HN x 3 arcs, GMTR_p RAREA, GRD_xr)
+horizontal_limiter_thuburn (dyn_horiz_adv_limiter/src/mod_src_tracer.f90:139-410)
+    reads  q (all cells), Qout_post (2, all cells), d and ch (6) on the
+           interior, cmask and q_a faces 1..3 on the arc origins
+           i,j in [gmin-1, gmax]
+    writes Qout_prev (2, all cells), q_a faces 1..3 on the arc origins and
+           faces 4..6 on their neighbours
+vertical_limiter_thuburn (dyn_vert_adv_limiter/src/mod_src_tracer.f90:99-186)
+    per column: q (kall), d (kall-2), ck lower (kall-1), ck upper (kall-2),
+    q_h read and written on kmin+1..kmax
+vi_rhow_solver (dyn_vi_rhow_solver/src/mod_vi.f90:192-246)
+    per column: rhogw0, Sw, RGAMH, Mc, Ml, Mu on kmin+1..kmax; preg0, Spre,
+    RGSGAM2, rhog0, Srho, RGAM on kmin..kmax; RGSGAM2H and rhogw at kmin,
+    kmax+1; GSGAM2H and the written rhogw on kmin..kmax+1.
+Pole regions are included; they are below 0.01 % of the total.
+"""
+from __future__ import annotations
+
+from typing import Dict
+
+from .synth import GridSpec
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, 8 TB/s
+
+
+def kernel_bytes(spec: GridSpec, word: int = 8) -> Dict[str, int]:
+    n, kall, lall, npl = spec.gall_1d, spec.kall, spec.lall, spec.gall_pl
+    gall = n * n
+    inner = (n - 2) ** 2
+    arcs = (n - 1) ** 2  # arc origins i,j in [gmin-1, gmax]
+    ncol = gall * lall
+    pl = 2 if spec.have_pl else 0
+    ring = npl - 1
+
+    flux = (19 * gall * kall * lall + 25 * gall * lall) * word
+    flux += pl * (4 * npl * kall + 4 * ring * kall) * word
+
+    hlim_r = (gall * (1 + 2) + inner * (1 + 6) + arcs * (3 + 3)) * kall * lall
+    hlim_w = (gall * 2 + arcs * 6) * kall * lall
+    hlim = (hlim_r + hlim_w) * word
+    hlim += pl * (4 * npl * kall + 2 * 2 * npl * kall + ring * kall * 2) * word
+
+    vcol = kall + (kall - 2) + (kall - 1) + (kall - 2) + 2 * (kall - 3)
+    vlim = vcol * (ncol + pl * npl) * word
+
+    nk3, nk2, nk1 = kall - 3, kall - 2, kall - 1
+    vicol = 6 * nk3 + 6 * nk2 + 2 * 2 + 2 * nk1
+    vi = vicol * (ncol + pl * npl) * word
+    return dict(horizontal_flux=flux, horizontal_limiter_thuburn=hlim, vertical_limiter_thuburn=vlim,
+                vi_rhow_solver=vi)
+
+
+def kernel_flops(spec: GridSpec) -> Dict[str, int]:
+    """Floating-point operations per launch (reported, not bounding): flux
+    ~125, h-limiter ~170 (min/max included), v-limiter ~45, VI ~45 per
+    array cell-level."""
+    cl = spec.array_cell_levels()
+    return dict(horizontal_flux=125 * cl, horizontal_limiter_thuburn=170 * cl, vertical_limiter_thuburn=45 * cl,
+                vi_rhow_solver=45 * cl)

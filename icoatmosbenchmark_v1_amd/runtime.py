@@ -1,0 +1,344 @@
+"""ctypes binding of libnicam_tt.so and a thin Python mirror of the reference
+interface.
+
+The reference exposes Fortran module procedures (paths relative to
+kernels/NICAM/ in the reference tree):
+
+* ``horizontal_flux``            dyn_horiz_adv_flux/src/mod_src_tracer.f90:42-379
+* ``horizontal_limiter_thuburn`` dyn_horiz_adv_limiter/src/mod_src_tracer.f90:42-430
+* ``vertical_limiter_thuburn``   dyn_vert_adv_limiter/src/mod_src_tracer.f90:41-243
+* ``vi_rhow_solver_putcoef``     dyn_vi_rhow_solver/src/mod_vi.f90:58-89
+* ``vi_rhow_solver``             dyn_vi_rhow_solver/src/mod_vi.f90:93-306
+
+:class:`Context` carries what the reference keeps in module globals and has
+one method per procedure, with the same argument order.  Arrays are numpy
+arrays (host pointer mode) or torch tensors on the GPU (device pointer mode),
+float64 and C-contiguous with the Fortran dimensions reversed: the Fortran
+array ``flx_h(gall,kall,lall,6)`` is the C array ``flx_h[6][lall][kall][gall]``.
+
+There is no CPU fallback: every call goes to the HIP library, and a missing
+library raises :class:`LibraryNotBuilt`.
+"""
+from __future__ import annotations
+
+import ctypes
+from ctypes import POINTER, c_char_p, c_double, c_int, c_size_t, c_void_p
+from pathlib import Path
+from typing import Optional, Sequence
+
+import numpy as np
+
+LIB_PATH = Path(__file__).resolve().parent / "libnicam_tt.so"
+
+NICAM_OK = 0
+NICAM_ERR_ARG = 1
+NICAM_ERR_STATE = 2
+NICAM_ERR_HIP = 3
+NICAM_ERR_IO = 4
+NICAM_ERR_NOMEM = 5
+PTR_DEVICE = 0
+PTR_HOST = 1
+
+STATUS_NAMES = {
+    NICAM_ERR_ARG: "NICAM_ERR_ARG",
+    NICAM_ERR_STATE: "NICAM_ERR_STATE",
+    NICAM_ERR_HIP: "NICAM_ERR_HIP",
+    NICAM_ERR_IO: "NICAM_ERR_IO",
+    NICAM_ERR_NOMEM: "NICAM_ERR_NOMEM",
+}
+
+# Every symbol declared in include/nicam_tt.h: (name, restype, argtypes)
+_P = c_void_p
+SIGNATURES = {
+    "nicam_version": (c_char_p, []),
+    "nicam_last_error": (c_char_p, []),
+    "nicam_ctx_create": (c_int, [POINTER(c_void_p), c_int, c_int, c_int, c_int, c_int, POINTER(c_int)]),
+    "nicam_ctx_destroy": (c_int, [_P]),
+    "nicam_ctx_set_stream": (c_int, [_P, _P]),
+    "nicam_ctx_set_pointer_mode": (c_int, [_P, c_int]),
+    "nicam_ctx_device_bytes": (c_size_t, [_P]),
+    "nicam_const_setup": (c_int, [_P]),
+    "nicam_set_const": (c_int, [_P, c_double, c_double]),
+    "nicam_set_phys_const": (c_int, [_P, c_double, c_double, c_double, c_int]),
+    "nicam_grd_input_vgrid": (c_int, [c_char_p, c_int, _P, _P]),
+    "nicam_grd_setup": (c_int, [_P, _P, _P, _P, _P, _P]),
+    "nicam_set_vgrid_factors": (c_int, [_P, _P, _P, _P]),
+    "nicam_set_hmetrics": (c_int, [_P] + [_P] * 8),
+    "nicam_set_vmetrics": (c_int, [_P] + [_P] * 10),
+    "nicam_horizontal_flux": (c_int, [_P] + [_P] * 12 + [c_double]),
+    "nicam_horizontal_limiter_thuburn": (c_int, [_P] + [_P] * 14),
+    "nicam_vertical_limiter_thuburn": (c_int, [_P] + [_P] * 8),
+    "nicam_vi_rhow_solver_putcoef": (c_int, [_P] + [_P] * 6),
+    "nicam_vi_rhow_solver": (c_int, [_P] + [_P] * 14 + [c_double]),
+    "nicam_ctx_synchronize": (c_int, [_P]),
+}
+
+
+class LibraryNotBuilt(RuntimeError):
+    pass
+
+
+class NicamError(RuntimeError):
+    def __init__(self, status: int, func: str, msg: str):
+        super().__init__(f"{func} failed with {STATUS_NAMES.get(status, status)}: {msg}")
+        self.status = status
+
+
+_lib: Optional[ctypes.CDLL] = None
+
+
+def load_library(path: Optional[Path] = None) -> ctypes.CDLL:
+    """Loads libnicam_tt.so (once) and declares every exported signature."""
+    global _lib
+    if _lib is not None and path is None:
+        return _lib
+    p = Path(path) if path else LIB_PATH
+    if not p.exists():
+        raise LibraryNotBuilt(f"{p} is missing: run __graft_entry__.build() (hipcc, gfx950) first")
+    lib = ctypes.CDLL(str(p))
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    if path is None:
+        _lib = lib
+    return lib
+
+
+def _check(rc: int, func: str) -> None:
+    if rc != NICAM_OK:
+        msg = load_library().nicam_last_error().decode(errors="replace")
+        raise NicamError(rc, func, msg)
+
+
+def _ptr(x, mode: int, name: str, numel: Optional[int] = None):
+    """Raw pointer of a numpy array (host mode) or torch tensor (device mode)."""
+    if x is None:
+        return None
+    if isinstance(x, np.ndarray):
+        if mode != PTR_HOST:
+            raise TypeError(f"{name}: numpy array passed in device pointer mode")
+        if x.dtype != np.float64 or not x.flags.c_contiguous:
+            raise TypeError(f"{name}: need a C-contiguous float64 array")
+        if numel is not None and x.size != numel:
+            raise ValueError(f"{name}: has {x.size} elements, expected {numel}")
+        return x.ctypes.data
+    # torch tensor
+    try:
+        import torch
+    except ImportError as e:  # pragma: no cover
+        raise TypeError(f"{name}: unsupported array type {type(x)}") from e
+    if not isinstance(x, torch.Tensor):
+        raise TypeError(f"{name}: unsupported array type {type(x)}")
+    if mode != PTR_DEVICE or not x.is_cuda:
+        raise TypeError(f"{name}: torch tensors must live on the GPU and use device pointer mode")
+    if x.dtype != torch.float64 or not x.is_contiguous():
+        raise TypeError(f"{name}: need a contiguous float64 tensor")
+    if numel is not None and x.numel() != numel:
+        raise ValueError(f"{name}: has {x.numel()} elements, expected {numel}")
+    return x.data_ptr()
+
+
+def _setup_ptr(x, name: str, numel: Optional[int] = None):
+    """Set-up calls accept host (numpy) or device (torch) pointers in any mode."""
+    if x is None:
+        return None
+    if isinstance(x, np.ndarray):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        if numel is not None and x.size != numel:
+            raise ValueError(f"{name}: has {x.size} elements, expected {numel}")
+        return x.ctypes.data, x
+    return _ptr(x, PTR_DEVICE, name, numel), x
+
+
+class Context:
+    """One grid: sizes, singular-point flags, constants, metrics, coefficients.
+
+    Mirrors the module state of the reference's mod_misc (problem_size.inc:28-59,
+    mod_misc.f90:78-152) and mod_vi (mod_vi.f90:47-52).
+    """
+
+    def __init__(self, gall_1d: int, kall: int, lall: int, gall_pl: int = 6, have_pl: bool = True,
+                 have_sgp: Optional[Sequence[int]] = None, pointer_mode: int = PTR_DEVICE):
+        self.lib = load_library()
+        if have_sgp is None:
+            have_sgp = [0] * lall
+        sgp = (c_int * lall)(*[1 if v else 0 for v in have_sgp])
+        h = c_void_p()
+        _check(self.lib.nicam_ctx_create(ctypes.byref(h), gall_1d, kall, lall, gall_pl, int(bool(have_pl)), sgp),
+               "nicam_ctx_create")
+        self.h = h
+        self.gall_1d, self.gall, self.kall, self.lall = gall_1d, gall_1d * gall_1d, kall, lall
+        self.gall_pl, self.have_pl = gall_pl, bool(have_pl)
+        self.have_sgp = list(have_sgp)
+        self.mode = PTR_DEVICE
+        self._keep = []  # host copies kept alive during asynchronous set-up copies
+        self.set_pointer_mode(pointer_mode)
+
+    # -- lifetime ---------------------------------------------------------
+    def close(self) -> None:
+        if getattr(self, "h", None):
+            self.lib.nicam_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):  # pragma: no cover - best effort
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    # -- sizes -------------------------------------------------------------
+    @property
+    def nreg(self) -> int:
+        return self.gall * self.kall * self.lall
+
+    @property
+    def npl(self) -> int:
+        return self.gall_pl * self.kall * 2
+
+    # -- configuration -------------------------------------------------------
+    def set_pointer_mode(self, mode: int) -> None:
+        _check(self.lib.nicam_ctx_set_pointer_mode(self.h, mode), "nicam_ctx_set_pointer_mode")
+        self.mode = mode
+
+    def set_stream(self, stream) -> None:
+        """stream: a torch.cuda.Stream, a raw hipStream_t integer, or None."""
+        raw = getattr(stream, "cuda_stream", stream)
+        _check(self.lib.nicam_ctx_set_stream(self.h, c_void_p(raw) if raw else None), "nicam_ctx_set_stream")
+
+    def const_setup(self) -> None:
+        _check(self.lib.nicam_const_setup(self.h), "nicam_const_setup")
+
+    def set_const(self, eps: float, huge: float) -> None:
+        _check(self.lib.nicam_set_const(self.h, eps, huge), "nicam_set_const")
+
+    def set_phys_const(self, grav=9.80616, rdry=287.0, cvdry=717.5, non_hydro_alpha=1) -> None:
+        _check(self.lib.nicam_set_phys_const(self.h, grav, rdry, cvdry, non_hydro_alpha), "nicam_set_phys_const")
+
+    def device_bytes(self) -> int:
+        return int(self.lib.nicam_ctx_device_bytes(self.h))
+
+    def synchronize(self) -> None:
+        _check(self.lib.nicam_ctx_synchronize(self.h), "nicam_ctx_synchronize")
+
+    # -- set-up routines -----------------------------------------------------
+    def grd_setup(self, gz: np.ndarray, gzh: np.ndarray):
+        """GRD_setup from gz/gzh; returns (rdgzh, afact, bfact)."""
+        gz = np.ascontiguousarray(gz, dtype=np.float64)
+        gzh = np.ascontiguousarray(gzh, dtype=np.float64)
+        if gz.size != self.kall or gzh.size != self.kall:
+            raise ValueError("gz/gzh need kall elements")
+        out = [np.empty(self.kall) for _ in range(3)]
+        _check(self.lib.nicam_grd_setup(self.h, gz.ctypes.data, gzh.ctypes.data, *[o.ctypes.data for o in out]),
+               "nicam_grd_setup")
+        return tuple(out)
+
+    def set_vgrid_factors(self, rdgzh, afact, bfact) -> None:
+        a = [np.ascontiguousarray(x, dtype=np.float64) for x in (rdgzh, afact, bfact)]
+        if any(x.size != self.kall for x in a):
+            raise ValueError("vertical factors need kall elements")
+        _check(self.lib.nicam_set_vgrid_factors(self.h, *[x.ctypes.data for x in a]), "nicam_set_vgrid_factors")
+
+    def set_hmetrics(self, GRD_xr, GRD_xr_pl, GMTR_p, GMTR_p_pl, GMTR_t, GMTR_t_pl, GMTR_a, GMTR_a_pl) -> None:
+        g, p = self.gall * self.lall, self.gall_pl * 2
+        spec = [(GRD_xr, g * 9), (GRD_xr_pl, p * 3), (GMTR_p, g * 8), (GMTR_p_pl, p * 8),
+                (GMTR_t, g * 10), (GMTR_t_pl, p * 5), (GMTR_a, g * 36), (GMTR_a_pl, p * 18)]
+        names = ["GRD_xr", "GRD_xr_pl", "GMTR_p", "GMTR_p_pl", "GMTR_t", "GMTR_t_pl", "GMTR_a", "GMTR_a_pl"]
+        ptrs = []
+        for (x, n), nm in zip(spec, names):
+            r = _setup_ptr(x if (self.have_pl or not nm.endswith("_pl")) else None, nm, n)
+            ptrs.append(r[0] if r else None)
+            if r:
+                self._keep.append(r[1])
+        _check(self.lib.nicam_set_hmetrics(self.h, *ptrs), "nicam_set_hmetrics")
+        self._keep.clear()
+
+    def _pairs(self, arrays, names):
+        ptrs = []
+        for x, nm in zip(arrays, names):
+            pole = nm.endswith("_pl")
+            if pole and not self.have_pl:
+                ptrs.append(None)
+                continue
+            r = _setup_ptr(x, nm, self.npl if pole else self.nreg)
+            ptrs.append(r[0])
+            self._keep.append(r[1])
+        return ptrs
+
+    def set_vmetrics(self, RGAM, RGAM_pl, RGAMH, RGAMH_pl, RGSGAM2, RGSGAM2_pl, RGSGAM2H, RGSGAM2H_pl,
+                     GSGAM2H, GSGAM2H_pl) -> None:
+        names = ["RGAM", "RGAM_pl", "RGAMH", "RGAMH_pl", "RGSGAM2", "RGSGAM2_pl", "RGSGAM2H", "RGSGAM2H_pl",
+                 "GSGAM2H", "GSGAM2H_pl"]
+        ptrs = self._pairs([RGAM, RGAM_pl, RGAMH, RGAMH_pl, RGSGAM2, RGSGAM2_pl, RGSGAM2H, RGSGAM2H_pl,
+                            GSGAM2H, GSGAM2H_pl], names)
+        _check(self.lib.nicam_set_vmetrics(self.h, *ptrs), "nicam_set_vmetrics")
+        self._keep.clear()
+
+    def vi_rhow_solver_putcoef(self, Mc, Mc_pl, Ml, Ml_pl, Mu, Mu_pl) -> None:
+        ptrs = self._pairs([Mc, Mc_pl, Ml, Ml_pl, Mu, Mu_pl], ["Mc", "Mc_pl", "Ml", "Ml_pl", "Mu", "Mu_pl"])
+        _check(self.lib.nicam_vi_rhow_solver_putcoef(self.h, *ptrs), "nicam_vi_rhow_solver_putcoef")
+        self._keep.clear()
+
+    # -- hot path --------------------------------------------------------------
+    def _args(self, spec):
+        """spec: list of (array, name, numel)."""
+        out = []
+        for x, nm, n in spec:
+            if nm.endswith("_pl") and not self.have_pl and nm != "Qout_prev_pl":
+                out.append(None)
+            else:
+                out.append(_ptr(x, self.mode, nm, n))
+        return out
+
+    def horizontal_flux(self, flx_h, flx_h_pl, GRD_xc, GRD_xc_pl, rho, rho_pl, rhovx, rhovx_pl,
+                        rhovy, rhovy_pl, rhovz, rhovz_pl, dt: float) -> None:
+        R, P = self.nreg, self.npl
+        a = self._args([(flx_h, "flx_h", 6 * R), (flx_h_pl, "flx_h_pl", P), (GRD_xc, "GRD_xc", 9 * R),
+                        (GRD_xc_pl, "GRD_xc_pl", 3 * P), (rho, "rho", R), (rho_pl, "rho_pl", P),
+                        (rhovx, "rhovx", R), (rhovx_pl, "rhovx_pl", P), (rhovy, "rhovy", R),
+                        (rhovy_pl, "rhovy_pl", P), (rhovz, "rhovz", R), (rhovz_pl, "rhovz_pl", P)])
+        _check(self.lib.nicam_horizontal_flux(self.h, *a, float(dt)), "nicam_horizontal_flux")
+
+    def horizontal_limiter_thuburn(self, q_a, q_a_pl, q, q_pl, d, d_pl, ch, ch_pl, cmask, cmask_pl,
+                                   Qout_prev, Qout_prev_pl, Qout_post, Qout_post_pl) -> None:
+        R, P = self.nreg, self.npl
+        a = self._args([(q_a, "q_a", 6 * R), (q_a_pl, "q_a_pl", P), (q, "q", R), (q_pl, "q_pl", P),
+                        (d, "d", R), (d_pl, "d_pl", P), (ch, "ch", 6 * R), (ch_pl, "ch_pl", P),
+                        (cmask, "cmask", 6 * R), (cmask_pl, "cmask_pl", P), (Qout_prev, "Qout_prev", 2 * R),
+                        (Qout_prev_pl, "Qout_prev_pl", 2 * P), (Qout_post, "Qout_post", 2 * R),
+                        (Qout_post_pl, "Qout_post_pl", 2 * P)])
+        _check(self.lib.nicam_horizontal_limiter_thuburn(self.h, *a), "nicam_horizontal_limiter_thuburn")
+
+    def vertical_limiter_thuburn(self, q_h, q_h_pl, q, q_pl, d, d_pl, ck, ck_pl) -> None:
+        R, P = self.nreg, self.npl
+        a = self._args([(q_h, "q_h", R), (q_h_pl, "q_h_pl", P), (q, "q", R), (q_pl, "q_pl", P), (d, "d", R),
+                        (d_pl, "d_pl", P), (ck, "ck", 2 * R), (ck_pl, "ck_pl", 2 * P)])
+        _check(self.lib.nicam_vertical_limiter_thuburn(self.h, *a), "nicam_vertical_limiter_thuburn")
+
+    def vi_rhow_solver(self, rhogw, rhogw_pl, rhogw0, rhogw0_pl, preg0, preg0_pl, rhog0, rhog0_pl,
+                       Srho, Srho_pl, Sw, Sw_pl, Spre, Spre_pl, dt: float) -> None:
+        R, P = self.nreg, self.npl
+        names = ["rhogw", "rhogw0", "preg0", "rhog0", "Srho", "Sw", "Spre"]
+        arrs = [rhogw, rhogw_pl, rhogw0, rhogw0_pl, preg0, preg0_pl, rhog0, rhog0_pl, Srho, Srho_pl, Sw, Sw_pl,
+                Spre, Spre_pl]
+        spec = []
+        for i, nm in enumerate(names):
+            spec.append((arrs[2 * i], nm, R))
+            spec.append((arrs[2 * i + 1], nm + "_pl", P))
+        a = self._args(spec)
+        _check(self.lib.nicam_vi_rhow_solver(self.h, *a, float(dt)), "nicam_vi_rhow_solver")
+
+
+def grd_input_vgrid(fname: str, kall: int):
+    """GRD_input_vgrid: (gz, gzh) from a Fortran unformatted big-endian file."""
+    lib = load_library()
+    gz, gzh = np.empty(kall), np.empty(kall)
+    _check(lib.nicam_grd_input_vgrid(str(fname).encode(), kall, gz.ctypes.data, gzh.ctypes.data),
+           "nicam_grd_input_vgrid")
+    return gz, gzh

@@ -1,0 +1,193 @@
+/*
+ * nicam_tt.h -- C ABI of the MI355X (gfx950) NICAM tracer-transport and
+ * vertical-implicit kernels.
+ *
+ * Drop-in for four Fortran module procedures of IcoAtmosBenchmark v1 (paths
+ * relative to kernels/NICAM/ in the reference tree):
+ *   horizontal_flux             dyn_horiz_adv_flux/src/mod_src_tracer.f90:42-379
+ *   horizontal_limiter_thuburn  dyn_horiz_adv_limiter/src/mod_src_tracer.f90:42-430
+ *   vertical_limiter_thuburn    dyn_vert_adv_limiter/src/mod_src_tracer.f90:41-243
+ *   vi_rhow_solver_putcoef      dyn_vi_rhow_solver/src/mod_vi.f90:58-89
+ *   vi_rhow_solver              dyn_vi_rhow_solver/src/mod_vi.f90:93-306
+ * and their set-up routines CONST_setup / GRD_setup / GRD_input_vgrid
+ * (dyn_vi_rhow_solver/src/mod_misc.f90:447-570).
+ *
+ * Arrays keep the reference's Fortran column-major layout:
+ *   regular x(gall, kall, lall[, n])   with gall = gall_1d*gall_1d and
+ *                                       g = (j-1)*gall_1d + i fastest,
+ *   pole    x_pl(gall_pl, kall, 2[, n]).
+ * Argument order is the reference's, always in (regular, _pl) pairs.  What the
+ * reference reads from module globals (sizes, singular-point flags, constants,
+ * GMTR_*, GRD_*, VMTR_* metrics, the Mc/Ml/Mu module copies) lives in a
+ * context created once per grid.
+ *
+ * Pointer mode: NICAM_PTR_DEVICE (default) -- every field argument is a
+ * device pointer and the call is asynchronous on the context's stream;
+ * NICAM_PTR_HOST -- fields are host pointers, the library stages them through
+ * device buffers it owns, and the call returns after the results are back.
+ * Set-up calls (metrics, coefficients) accept host or device pointers in
+ * either mode and copy the data.
+ *
+ * Every function returns NICAM_OK (0) or a nonzero status and never aborts;
+ * nicam_last_error() gives the message of the last failure on this thread.
+ * A context is not thread-safe: use one per host thread, or serialise.
+ */
+#ifndef NICAM_TT_H
+#define NICAM_TT_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct nicam_ctx_s *nicam_ctx;
+
+enum nicam_status {
+    NICAM_OK = 0,
+    NICAM_ERR_ARG = 1,     /* bad size, null pointer, inconsistent argument */
+    NICAM_ERR_STATE = 2,   /* a required set-up call has not been made       */
+    NICAM_ERR_HIP = 3,     /* HIP runtime error (message has the details)    */
+    NICAM_ERR_IO = 4,      /* file missing, short or malformed               */
+    NICAM_ERR_NOMEM = 5    /* device or host allocation failed               */
+};
+
+enum nicam_ptr_mode { NICAM_PTR_DEVICE = 0, NICAM_PTR_HOST = 1 };
+
+/* Library version string, e.g. "nicam_tt 0.1 gfx950". */
+const char *nicam_version(void);
+/* Message of the last failing call on the calling thread ("" if none). */
+const char *nicam_last_error(void);
+
+/* ---- context: replaces the ADM_* size globals
+ *      (dyn_vi_rhow_solver/src/problem_size.inc:28-50) ----
+ * gall_1d >= 4, kall >= 4, lall >= 1, gall_pl >= 3 (6 on the icosahedral
+ * grid); have_sgp[lall] is ADM_have_sgp (nonzero = region holds a singular
+ * point); have_pl is ADM_have_pl.  The context starts with CONST_setup's
+ * values (EPS = DBL_EPSILON, HUGE = DBL_MAX) and the physical constants of
+ * problem_size.inc:56-59 (GRAV 9.80616, Rdry 287, CVdry 717.5, alpha 1). */
+int nicam_ctx_create(nicam_ctx *ctx, int gall_1d, int kall, int lall, int gall_pl,
+                     int have_pl, const int *have_sgp);
+int nicam_ctx_destroy(nicam_ctx ctx);
+/* Stream for all work of this context (a hipStream_t; NULL = null stream). */
+int nicam_ctx_set_stream(nicam_ctx ctx, void *hip_stream);
+int nicam_ctx_set_pointer_mode(nicam_ctx ctx, int mode);
+/* Device memory currently held by the context, in bytes. */
+size_t nicam_ctx_device_bytes(nicam_ctx ctx);
+
+/* CONST_setup (mod_misc.f90:447-461): EPS = epsilon(1d0), HUGE = huge(1d0). */
+int nicam_const_setup(nicam_ctx ctx);
+/* Direct assignment of CONST_EPS / CONST_HUGE (problem_size.inc:54-55). */
+int nicam_set_const(nicam_ctx ctx, double eps, double huge_val);
+/* CONST_GRAV, CONST_Rdry, CONST_CVdry, NON_HYDRO_ALPHA (problem_size.inc:56-59). */
+int nicam_set_phys_const(nicam_ctx ctx, double grav, double rdry, double cvdry,
+                         int non_hydro_alpha);
+
+/* GRD_input_vgrid (mod_misc.f90:527-570): reads a Fortran unformatted
+ * sequential big-endian file with 4-byte record markers: nlayer (int32),
+ * gz(kall), gzh(kall) (float64).  nlayer must equal kall-2. */
+int nicam_grd_input_vgrid(const char *fname, int kall, double *gz, double *gzh);
+/* GRD_setup (mod_misc.f90:464-524) from gz/gzh(kall): computes and stores
+ * GRD_rdgzh, GRD_afact, GRD_bfact in the context.  Any of the optional
+ * outputs (kall doubles each) may be NULL. */
+int nicam_grd_setup(nicam_ctx ctx, const double *gz, const double *gzh,
+                    double *rdgzh_out, double *afact_out, double *bfact_out);
+/* Direct assignment of GRD_rdgzh, GRD_afact, GRD_bfact (kall each). */
+int nicam_set_vgrid_factors(nicam_ctx ctx, const double *rdgzh, const double *afact,
+                            const double *bfact);
+
+/* Horizontal metrics, mod_misc.f90 module arrays (shapes as in
+ * dyn_horiz_adv_flux/src/main.f90:93-100):
+ *   GRD_xr(gall,1,lall,3,3)  GRD_xr_pl(gall_pl,1,2,3)
+ *   GMTR_p(gall,1,lall,8)    GMTR_p_pl(gall_pl,1,2,8)
+ *   GMTR_t(gall,1,lall,2,5)  GMTR_t_pl(gall_pl,1,2,5)
+ *   GMTR_a(gall,1,lall,3,12) GMTR_a_pl(gall_pl,1,2,18)
+ * Pole pointers may be NULL when have_pl == 0. */
+int nicam_set_hmetrics(nicam_ctx ctx,
+                       const double *GRD_xr, const double *GRD_xr_pl,
+                       const double *GMTR_p, const double *GMTR_p_pl,
+                       const double *GMTR_t, const double *GMTR_t_pl,
+                       const double *GMTR_a, const double *GMTR_a_pl);
+
+/* Vertical metrics VMTR_RGAM, _RGAMH, _RGSGAM2, _RGSGAM2H, _GSGAM2H
+ * (gall,kall,lall) and (gall_pl,kall,2) (mod_misc.f90:101-112). */
+int nicam_set_vmetrics(nicam_ctx ctx,
+                       const double *RGAM, const double *RGAM_pl,
+                       const double *RGAMH, const double *RGAMH_pl,
+                       const double *RGSGAM2, const double *RGSGAM2_pl,
+                       const double *RGSGAM2H, const double *RGSGAM2H_pl,
+                       const double *GSGAM2H, const double *GSGAM2H_pl);
+
+/* ---- hot path ---------------------------------------------------------- */
+
+/* horizontal_flux (dyn_horiz_adv_flux/src/mod_src_tracer.f90:42-379).
+ *   out flx_h(gall,kall,lall,6), flx_h_pl(gall_pl,kall,2)
+ *   out GRD_xc(gall,kall,lall,3,3), GRD_xc_pl(gall_pl,kall,2,3)
+ *   in  rho, rhovx, rhovy, rhovz (gall,kall,lall) and _pl
+ *   dt  (the driver passes SET_dt_large = 360 s)
+ * Point 1 of flx_h_pl and GRD_xc_pl is not written (as in the reference). */
+int nicam_horizontal_flux(nicam_ctx ctx,
+                          double *flx_h, double *flx_h_pl,
+                          double *GRD_xc, double *GRD_xc_pl,
+                          const double *rho, const double *rho_pl,
+                          const double *rhovx, const double *rhovx_pl,
+                          const double *rhovy, const double *rhovy_pl,
+                          const double *rhovz, const double *rhovz_pl,
+                          double dt);
+
+/* horizontal_limiter_thuburn (dyn_horiz_adv_limiter/src/mod_src_tracer.f90:42-430).
+ *   inout q_a(gall,kall,lall,6), q_a_pl(gall_pl,kall,2)
+ *   in    q, d (gall,kall,lall); ch, cmask (gall,kall,lall,6); and _pl
+ *   out   Qout_prev(gall,kall,lall,2), Qout_prev_pl(gall_pl,kall,2,2)
+ *   in    Qout_post(gall,kall,lall,2), Qout_post_pl(gall_pl,kall,2,2)
+ * Qout_prev is the allowable range before the halo exchange; the clamp uses
+ * Qout_post, the range after it, exactly as the reference kernel does
+ * (:365-375). */
+int nicam_horizontal_limiter_thuburn(nicam_ctx ctx,
+                                     double *q_a, double *q_a_pl,
+                                     const double *q, const double *q_pl,
+                                     const double *d, const double *d_pl,
+                                     const double *ch, const double *ch_pl,
+                                     const double *cmask, const double *cmask_pl,
+                                     double *Qout_prev, double *Qout_prev_pl,
+                                     const double *Qout_post, const double *Qout_post_pl);
+
+/* vertical_limiter_thuburn (dyn_vert_adv_limiter/src/mod_src_tracer.f90:41-243).
+ *   inout q_h(gall,kall,lall), q_h_pl
+ *   in    q, d (gall,kall,lall), ck(gall,kall,lall,2), and _pl */
+int nicam_vertical_limiter_thuburn(nicam_ctx ctx,
+                                   double *q_h, double *q_h_pl,
+                                   const double *q, const double *q_pl,
+                                   const double *d, const double *d_pl,
+                                   const double *ck, const double *ck_pl);
+
+/* vi_rhow_solver_putcoef (dyn_vi_rhow_solver/src/mod_vi.f90:58-89): keeps a
+ * device copy of Mc, Ml, Mu (gall,kall,lall) and _pl in the context. */
+int nicam_vi_rhow_solver_putcoef(nicam_ctx ctx,
+                                 const double *Mc, const double *Mc_pl,
+                                 const double *Ml, const double *Ml_pl,
+                                 const double *Mu, const double *Mu_pl);
+
+/* vi_rhow_solver (dyn_vi_rhow_solver/src/mod_vi.f90:93-306).
+ *   inout rhogw(gall,kall,lall), rhogw_pl (levels kmin and kmax+1 are read)
+ *   in    rhogw0, preg0, rhog0, Srho, Sw, Spre (gall,kall,lall) and _pl
+ *   dt    (the driver passes SET_dt_small = 60 s)
+ * Needs nicam_vi_rhow_solver_putcoef, nicam_set_vmetrics and
+ * nicam_grd_setup (or nicam_set_vgrid_factors) first. */
+int nicam_vi_rhow_solver(nicam_ctx ctx,
+                         double *rhogw, double *rhogw_pl,
+                         const double *rhogw0, const double *rhogw0_pl,
+                         const double *preg0, const double *preg0_pl,
+                         const double *rhog0, const double *rhog0_pl,
+                         const double *Srho, const double *Srho_pl,
+                         const double *Sw, const double *Sw_pl,
+                         const double *Spre, const double *Spre_pl,
+                         double dt);
+
+/* Blocks until all work queued on the context's stream has finished. */
+int nicam_ctx_synchronize(nicam_ctx ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* NICAM_TT_H */

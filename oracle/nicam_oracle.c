@@ -1,0 +1,653 @@
+/*
+ * CPU oracle: plain-C restatement of the NICAM hot-path kernels.
+ *
+ * TEST INFRASTRUCTURE ONLY (checker for tests, smoke and the CPU baseline);
+ * the product library never links or calls this file.
+ *
+ * Every loop follows the reference's loop bounds, write set and floating-point
+ * evaluation order (left-to-right, no re-association).  Build with
+ * -ffp-contract=off so that no multiply-add is fused: the reference builds
+ * for x86-64 without -march, where no FMA instruction exists.
+ *
+ * Paths below are relative to the reference tree:
+ *   FLX  = kernels/NICAM/dyn_horiz_adv_flux/src
+ *   HLIM = kernels/NICAM/dyn_horiz_adv_limiter/src
+ *   VLIM = kernels/NICAM/dyn_vert_adv_limiter/src
+ *   VI   = kernels/NICAM/dyn_vi_rhow_solver/src
+ */
+#include "nicam_oracle.h"
+
+#include <float.h>
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#define LALL_PL 2
+#define GSLF_PL 1
+#define GMIN_PL 2
+#define K0 1
+
+/* Fortran MIN/MAX of two reals, first argument kept on ties. */
+static inline double mn(double a, double b) { return (b < a) ? b : a; }
+static inline double mx(double a, double b) { return (b > a) ? b : a; }
+/* Fortran SIGN(a, b) = |a| carrying the sign bit of b (so SIGN(0.5,-0.0) = -0.5). */
+static inline double fsign(double a, double b) { return copysign(fabs(a), b); }
+
+int orc_num_threads(void)
+{
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
+}
+
+/* VI/src/mod_misc.f90:447-461 */
+void orc_const_setup(double *eps, double *huge_val)
+{
+    *eps = DBL_EPSILON;
+    *huge_val = DBL_MAX;
+}
+
+/* VI/src/mod_misc.f90:464-524.  Arrays are 1-based in the comments, 0-based here. */
+void orc_grd_setup(int kall, const double *gz, const double *gzh,
+                   double *dgz, double *dgzh, double *rdgz, double *rdgzh,
+                   double *afact, double *bfact, double *cfact, double *dfact)
+{
+    const int kmin = 2, kmax = kall - 1;
+#define V1(a, k) a[(k) - 1]
+    for (int k = kmin - 1; k <= kmax; ++k)
+        V1(dgz, k) = V1(gzh, k + 1) - V1(gzh, k);
+    V1(dgz, kmax + 1) = V1(dgz, kmax);
+
+    for (int k = kmin; k <= kmax + 1; ++k)
+        V1(dgzh, k) = V1(gz, k) - V1(gz, k - 1);
+    V1(dgzh, kmin - 1) = V1(dgzh, kmin);
+
+    for (int k = 1; k <= kall; ++k) {
+        V1(rdgz, k) = 1.0 / V1(dgz, k);
+        V1(rdgzh, k) = 1.0 / V1(dgzh, k);
+    }
+
+    for (int k = kmin; k <= kmax + 1; ++k)
+        V1(afact, k) = (V1(gzh, k) - V1(gz, k - 1)) / (V1(gz, k) - V1(gz, k - 1));
+    V1(afact, kmin - 1) = 1.0;
+    for (int k = 1; k <= kall; ++k) V1(bfact, k) = 1.0 - V1(afact, k);
+
+    for (int k = kmin; k <= kmax; ++k)
+        V1(cfact, k) = (V1(gz, k) - V1(gzh, k)) / (V1(gzh, k + 1) - V1(gzh, k));
+    V1(cfact, kmin - 1) = 1.0;
+    V1(cfact, kmax + 1) = 0.0;
+    for (int k = 1; k <= kall; ++k) V1(dfact, k) = 1.0 - V1(cfact, k);
+#undef V1
+}
+
+/* ---------------------------------------------------------------------------
+ * Accessors (1-based indices, Fortran order)
+ * ------------------------------------------------------------------------- */
+#define F3(x, g, k, l) (x)[((size_t)((l) - 1) * kall + ((k) - 1)) * gall + ((g) - 1)]
+#define F4(x, g, k, l, n) \
+    (x)[(((size_t)((n) - 1) * lall + ((l) - 1)) * kall + ((k) - 1)) * gall + ((g) - 1)]
+#define F5(x, g, k, l, a, n)                                                              \
+    (x)[((((size_t)((n) - 1) * 3 + ((a) - 1)) * lall + ((l) - 1)) * kall + ((k) - 1)) * \
+            gall + ((g) - 1)]
+#define P3(x, v, k, l) (x)[((size_t)((l) - 1) * kall + ((k) - 1)) * gall_pl + ((v) - 1)]
+#define P4(x, v, k, l, n) \
+    (x)[(((size_t)((n) - 1) * LALL_PL + ((l) - 1)) * kall + ((k) - 1)) * gall_pl + ((v) - 1)]
+/* metrics: K0 dimension has extent 1 */
+#define MT(g, l, t, w) GMTR_t[(((size_t)((w) - 1) * 2 + ((t) - 1)) * lall + ((l) - 1)) * gall + ((g) - 1)]
+#define MA(g, l, a, c) GMTR_a[(((size_t)((c) - 1) * 3 + ((a) - 1)) * lall + ((l) - 1)) * gall + ((g) - 1)]
+#define MP(g, l, c) GMTR_p[((size_t)((c) - 1) * lall + ((l) - 1)) * gall + ((g) - 1)]
+#define XR(g, l, a, d) GRD_xr[(((size_t)((d) - 1) * 3 + ((a) - 1)) * lall + ((l) - 1)) * gall + ((g) - 1)]
+#define MT_PL(v, l, w) GMTR_t_pl[((size_t)((w) - 1) * LALL_PL + ((l) - 1)) * gall_pl + ((v) - 1)]
+#define MA_PL(v, l, c) GMTR_a_pl[((size_t)((c) - 1) * LALL_PL + ((l) - 1)) * gall_pl + ((v) - 1)]
+#define MP_PL(v, l, c) GMTR_p_pl[((size_t)((c) - 1) * LALL_PL + ((l) - 1)) * gall_pl + ((v) - 1)]
+#define XR_PL(v, l, d) GRD_xr_pl[((size_t)((d) - 1) * LALL_PL + ((l) - 1)) * gall_pl + ((v) - 1)]
+
+enum { TI = 1, TJ = 2 };
+enum { AI = 1, AIJ = 2, AJ = 3 };
+enum { W1 = 3, W2 = 4, W3 = 5 };
+enum { HNX = 1, HNY = 2, HNZ = 3 };
+enum { P_RAREA = 2 };
+enum { XDIR = 1, YDIR = 2, ZDIR = 3 };
+enum { I_MIN = 1, I_MAX = 2 };
+
+/* ===========================================================================
+ * horizontal_flux: FLX/mod_src_tracer.f90:42-379
+ * ======================================================================== */
+void orc_horizontal_flux(const orc_dims *D, const int *have_sgp,
+                         double *flx_h, double *flx_h_pl,
+                         double *GRD_xc, double *GRD_xc_pl,
+                         const double *rho, const double *rho_pl,
+                         const double *rhovx, const double *rhovx_pl,
+                         const double *rhovy, const double *rhovy_pl,
+                         const double *rhovz, const double *rhovz_pl,
+                         double dt,
+                         const double *GRD_xr, const double *GRD_xr_pl,
+                         const double *GMTR_p, const double *GMTR_p_pl,
+                         const double *GMTR_t, const double *GMTR_t_pl,
+                         const double *GMTR_a, const double *GMTR_a_pl)
+{
+    const int iall = D->gall_1d, gall = iall * iall, kall = D->kall, lall = D->lall;
+    const int gall_pl = D->gall_pl, gmax_pl = gall_pl;
+    const int gmin = 2, gmax = iall - 1;
+    const double EPS = D->eps;
+
+    for (int l = 1; l <= lall; ++l) {
+        /* FLX:130-137 -- the reference parallelises inside the k loop; here the
+         * k iterations are independent, so each thread owns whole k slabs. */
+#pragma omp parallel for schedule(static)
+        for (int k = 1; k <= kall; ++k) {
+            double *rhot_TI = (double *)malloc(sizeof(double) * 8 * gall);
+            double *rhot_TJ = rhot_TI + gall, *rhovxt_TI = rhot_TI + 2 * gall,
+                   *rhovxt_TJ = rhot_TI + 3 * gall, *rhovyt_TI = rhot_TI + 4 * gall,
+                   *rhovyt_TJ = rhot_TI + 5 * gall, *rhovzt_TI = rhot_TI + 6 * gall,
+                   *rhovzt_TJ = rhot_TI + 7 * gall;
+#define T1(a, g) (a)[(g) - 1]
+            /* FLX:139-161  (i,j),(i+1,j) */
+            for (int j = gmin - 1; j <= gmax; ++j)
+                for (int i = gmin - 1; i <= gmax; ++i) {
+                    const int ij = (j - 1) * iall + i, ip1j = ij + 1;
+                    T1(rhot_TI, ij) = F3(rho, ij, k, l) * MT(ij, l, TI, W1) + F3(rho, ip1j, k, l) * MT(ij, l, TI, W2);
+                    T1(rhovxt_TI, ij) = F3(rhovx, ij, k, l) * MT(ij, l, TI, W1) + F3(rhovx, ip1j, k, l) * MT(ij, l, TI, W2);
+                    T1(rhovyt_TI, ij) = F3(rhovy, ij, k, l) * MT(ij, l, TI, W1) + F3(rhovy, ip1j, k, l) * MT(ij, l, TI, W2);
+                    T1(rhovzt_TI, ij) = F3(rhovz, ij, k, l) * MT(ij, l, TI, W1) + F3(rhovz, ip1j, k, l) * MT(ij, l, TI, W2);
+                    T1(rhot_TJ, ij) = F3(rho, ij, k, l) * MT(ij, l, TJ, W1);
+                    T1(rhovxt_TJ, ij) = F3(rhovx, ij, k, l) * MT(ij, l, TJ, W1);
+                    T1(rhovyt_TJ, ij) = F3(rhovy, ij, k, l) * MT(ij, l, TJ, W1);
+                    T1(rhovzt_TJ, ij) = F3(rhovz, ij, k, l) * MT(ij, l, TJ, W1);
+                }
+            /* FLX:163-186  (i,j+1),(i+1,j+1) */
+            for (int j = gmin - 1; j <= gmax; ++j)
+                for (int i = gmin - 1; i <= gmax; ++i) {
+                    const int ij = (j - 1) * iall + i, ijp1 = ij + iall, ip1jp1 = ij + iall + 1;
+                    T1(rhot_TI, ij) = T1(rhot_TI, ij) + F3(rho, ip1jp1, k, l) * MT(ij, l, TI, W3);
+                    T1(rhovxt_TI, ij) = T1(rhovxt_TI, ij) + F3(rhovx, ip1jp1, k, l) * MT(ij, l, TI, W3);
+                    T1(rhovyt_TI, ij) = T1(rhovyt_TI, ij) + F3(rhovy, ip1jp1, k, l) * MT(ij, l, TI, W3);
+                    T1(rhovzt_TI, ij) = T1(rhovzt_TI, ij) + F3(rhovz, ip1jp1, k, l) * MT(ij, l, TI, W3);
+                    T1(rhot_TJ, ij) = T1(rhot_TJ, ij) + F3(rho, ip1jp1, k, l) * MT(ij, l, TJ, W2) + F3(rho, ijp1, k, l) * MT(ij, l, TJ, W3);
+                    T1(rhovxt_TJ, ij) = T1(rhovxt_TJ, ij) + F3(rhovx, ip1jp1, k, l) * MT(ij, l, TJ, W2) + F3(rhovx, ijp1, k, l) * MT(ij, l, TJ, W3);
+                    T1(rhovyt_TJ, ij) = T1(rhovyt_TJ, ij) + F3(rhovy, ip1jp1, k, l) * MT(ij, l, TJ, W2) + F3(rhovy, ijp1, k, l) * MT(ij, l, TJ, W3);
+                    T1(rhovzt_TJ, ij) = T1(rhovzt_TJ, ij) + F3(rhovz, ip1jp1, k, l) * MT(ij, l, TJ, W2) + F3(rhovz, ijp1, k, l) * MT(ij, l, TJ, W3);
+                }
+            /* FLX:188-201  singular point */
+            if (have_sgp[l - 1]) {
+                const int ij = (gmin - 1 - 1) * iall + (gmin - 1), ip1j = ij + 1;
+                T1(rhot_TI, ij) = T1(rhot_TJ, ip1j);
+                T1(rhovxt_TI, ij) = T1(rhovxt_TJ, ip1j);
+                T1(rhovyt_TI, ij) = T1(rhovyt_TJ, ip1j);
+                T1(rhovzt_TI, ij) = T1(rhovzt_TJ, ip1j);
+            }
+            /* FLX:205-232  zero the halo */
+            for (int j = 1; j <= iall; ++j)
+                for (int i = 1; i <= iall; ++i)
+                    if (i < gmin || i > gmax || j < gmin || j > gmax) {
+                        const int ij = (j - 1) * iall + i;
+                        for (int m = 1; m <= 6; ++m) F4(flx_h, ij, k, l, m) = 0.0;
+                        for (int a = AI; a <= AJ; ++a)
+                            for (int dd = XDIR; dd <= ZDIR; ++dd) F5(GRD_xc, ij, k, l, a, dd) = 0.0;
+                    }
+            /* FLX:234-258  arc AI */
+            for (int j = gmin; j <= gmax; ++j)
+                for (int i = gmin - 1; i <= gmax; ++i) {
+                    const int ij = (j - 1) * iall + i, ip1j = ij + 1, ijm1 = ij - iall;
+                    const double rrhoa2 = 1.0 / mx(T1(rhot_TJ, ijm1) + T1(rhot_TI, ij), EPS);
+                    const double rhovxt2 = T1(rhovxt_TJ, ijm1) + T1(rhovxt_TI, ij);
+                    const double rhovyt2 = T1(rhovyt_TJ, ijm1) + T1(rhovyt_TI, ij);
+                    const double rhovzt2 = T1(rhovzt_TJ, ijm1) + T1(rhovzt_TI, ij);
+                    const double flux = 0.5 * (rhovxt2 * MA(ij, l, AI, HNX) + rhovyt2 * MA(ij, l, AI, HNY) + rhovzt2 * MA(ij, l, AI, HNZ));
+                    F4(flx_h, ij, k, l, 1) = flux * MP(ij, l, P_RAREA) * dt;
+                    F4(flx_h, ip1j, k, l, 4) = -flux * MP(ip1j, l, P_RAREA) * dt;
+                    F5(GRD_xc, ij, k, l, AI, XDIR) = XR(ij, l, AI, XDIR) - rhovxt2 * rrhoa2 * dt * 0.5;
+                    F5(GRD_xc, ij, k, l, AI, YDIR) = XR(ij, l, AI, YDIR) - rhovyt2 * rrhoa2 * dt * 0.5;
+                    F5(GRD_xc, ij, k, l, AI, ZDIR) = XR(ij, l, AI, ZDIR) - rhovzt2 * rrhoa2 * dt * 0.5;
+                }
+            /* FLX:260-283  arc AIJ */
+            for (int j = gmin - 1; j <= gmax; ++j)
+                for (int i = gmin - 1; i <= gmax; ++i) {
+                    const int ij = (j - 1) * iall + i, ip1jp1 = ij + iall + 1;
+                    const double rrhoa2 = 1.0 / mx(T1(rhot_TI, ij) + T1(rhot_TJ, ij), EPS);
+                    const double rhovxt2 = T1(rhovxt_TI, ij) + T1(rhovxt_TJ, ij);
+                    const double rhovyt2 = T1(rhovyt_TI, ij) + T1(rhovyt_TJ, ij);
+                    const double rhovzt2 = T1(rhovzt_TI, ij) + T1(rhovzt_TJ, ij);
+                    const double flux = 0.5 * (rhovxt2 * MA(ij, l, AIJ, HNX) + rhovyt2 * MA(ij, l, AIJ, HNY) + rhovzt2 * MA(ij, l, AIJ, HNZ));
+                    F4(flx_h, ij, k, l, 2) = flux * MP(ij, l, P_RAREA) * dt;
+                    F4(flx_h, ip1jp1, k, l, 5) = -flux * MP(ip1jp1, l, P_RAREA) * dt;
+                    F5(GRD_xc, ij, k, l, AIJ, XDIR) = XR(ij, l, AIJ, XDIR) - rhovxt2 * rrhoa2 * dt * 0.5;
+                    F5(GRD_xc, ij, k, l, AIJ, YDIR) = XR(ij, l, AIJ, YDIR) - rhovyt2 * rrhoa2 * dt * 0.5;
+                    F5(GRD_xc, ij, k, l, AIJ, ZDIR) = XR(ij, l, AIJ, ZDIR) - rhovzt2 * rrhoa2 * dt * 0.5;
+                }
+            /* FLX:285-309  arc AJ */
+            for (int j = gmin - 1; j <= gmax; ++j)
+                for (int i = gmin; i <= gmax; ++i) {
+                    const int ij = (j - 1) * iall + i, ijp1 = ij + iall, im1j = ij - 1;
+                    const double rrhoa2 = 1.0 / mx(T1(rhot_TJ, ij) + T1(rhot_TI, im1j), EPS);
+                    const double rhovxt2 = T1(rhovxt_TJ, ij) + T1(rhovxt_TI, im1j);
+                    const double rhovyt2 = T1(rhovyt_TJ, ij) + T1(rhovyt_TI, im1j);
+                    const double rhovzt2 = T1(rhovzt_TJ, ij) + T1(rhovzt_TI, im1j);
+                    const double flux = 0.5 * (rhovxt2 * MA(ij, l, AJ, HNX) + rhovyt2 * MA(ij, l, AJ, HNY) + rhovzt2 * MA(ij, l, AJ, HNZ));
+                    F4(flx_h, ij, k, l, 3) = flux * MP(ij, l, P_RAREA) * dt;
+                    F4(flx_h, ijp1, k, l, 6) = -flux * MP(ijp1, l, P_RAREA) * dt;
+                    F5(GRD_xc, ij, k, l, AJ, XDIR) = XR(ij, l, AJ, XDIR) - rhovxt2 * rrhoa2 * dt * 0.5;
+                    F5(GRD_xc, ij, k, l, AJ, YDIR) = XR(ij, l, AJ, YDIR) - rhovyt2 * rrhoa2 * dt * 0.5;
+                    F5(GRD_xc, ij, k, l, AJ, ZDIR) = XR(ij, l, AJ, ZDIR) - rhovzt2 * rrhoa2 * dt * 0.5;
+                }
+            /* FLX:311-320 */
+            if (have_sgp[l - 1]) {
+                const int ij = (gmin - 1) * iall + gmin;
+                F4(flx_h, ij, k, l, 6) = 0.0;
+            }
+#undef T1
+            free(rhot_TI);
+        }
+    }
+
+    /* FLX:326-374  pole regions */
+    if (D->have_pl) {
+        const int n = GSLF_PL;
+        double rhot_pl[16], rhovxt_pl[16], rhovyt_pl[16], rhovzt_pl[16];
+        for (int l = 1; l <= LALL_PL; ++l)
+            for (int k = 1; k <= kall; ++k) {
+                for (int v = GMIN_PL; v <= gmax_pl; ++v) {
+                    const int ij = v;
+                    int ijp1 = v + 1;
+                    if (ijp1 == gmax_pl + 1) ijp1 = GMIN_PL;
+                    rhot_pl[v] = P3(rho_pl, n, k, l) * MT_PL(ij, l, W1) + P3(rho_pl, ij, k, l) * MT_PL(ij, l, W2) + P3(rho_pl, ijp1, k, l) * MT_PL(ij, l, W3);
+                    rhovxt_pl[v] = P3(rhovx_pl, n, k, l) * MT_PL(ij, l, W1) + P3(rhovx_pl, ij, k, l) * MT_PL(ij, l, W2) + P3(rhovx_pl, ijp1, k, l) * MT_PL(ij, l, W3);
+                    rhovyt_pl[v] = P3(rhovy_pl, n, k, l) * MT_PL(ij, l, W1) + P3(rhovy_pl, ij, k, l) * MT_PL(ij, l, W2) + P3(rhovy_pl, ijp1, k, l) * MT_PL(ij, l, W3);
+                    rhovzt_pl[v] = P3(rhovz_pl, n, k, l) * MT_PL(ij, l, W1) + P3(rhovz_pl, ij, k, l) * MT_PL(ij, l, W2) + P3(rhovz_pl, ijp1, k, l) * MT_PL(ij, l, W3);
+                }
+                for (int v = GMIN_PL; v <= gmax_pl; ++v) {
+                    const int ij = v;
+                    int ijm1 = v - 1;
+                    if (ijm1 == GMIN_PL - 1) ijm1 = gmax_pl;
+                    const double rrhoa2 = 1.0 / mx(rhot_pl[ijm1] + rhot_pl[ij], EPS);
+                    const double rhovxt2 = rhovxt_pl[ijm1] + rhovxt_pl[ij];
+                    const double rhovyt2 = rhovyt_pl[ijm1] + rhovyt_pl[ij];
+                    const double rhovzt2 = rhovzt_pl[ijm1] + rhovzt_pl[ij];
+                    const double flux = 0.5 * (rhovxt2 * MA_PL(ij, l, HNX) + rhovyt2 * MA_PL(ij, l, HNY) + rhovzt2 * MA_PL(ij, l, HNZ));
+                    P3(flx_h_pl, v, k, l) = flux * MP_PL(n, l, P_RAREA) * dt;
+                    P4(GRD_xc_pl, v, k, l, XDIR) = XR_PL(v, l, XDIR) - rhovxt2 * rrhoa2 * dt * 0.5;
+                    P4(GRD_xc_pl, v, k, l, YDIR) = XR_PL(v, l, YDIR) - rhovyt2 * rrhoa2 * dt * 0.5;
+                    P4(GRD_xc_pl, v, k, l, ZDIR) = XR_PL(v, l, ZDIR) - rhovzt2 * rrhoa2 * dt * 0.5;
+                }
+            }
+    }
+}
+
+/* ===========================================================================
+ * horizontal_limiter_thuburn: HLIM/mod_src_tracer.f90:42-430
+ * The automatic arrays Qin/Qout of the reference (:98-101) are allocated per
+ * (l, k) slab here; only one slab is live at a time in each thread.
+ * ======================================================================== */
+void orc_horizontal_limiter_thuburn(const orc_dims *D, const int *have_sgp,
+                                    double *q_a, double *q_a_pl,
+                                    const double *q, const double *q_pl,
+                                    const double *d, const double *d_pl,
+                                    const double *ch, const double *ch_pl,
+                                    const double *cmask, const double *cmask_pl,
+                                    double *Qout_prev, double *Qout_prev_pl,
+                                    const double *Qout_post, const double *Qout_post_pl)
+{
+    const int iall = D->gall_1d, gall = iall * iall, kall = D->kall, lall = D->lall;
+    const int gall_pl = D->gall_pl, gmax_pl = gall_pl;
+    const int gmin = 2, gmax = iall - 1;
+    const double EPS = D->eps, BIG = D->big;
+
+    /* Qin_pl(gall_pl,kall,2,2,2) and Qout_pl(gall_pl,kall,2,2); :99,101 */
+    const size_t npl = (size_t)gall_pl * kall * LALL_PL;
+    double *Qin_pl = (double *)calloc(npl * 4, sizeof(double));
+    double *Qout_pl = (double *)calloc(npl * 2, sizeof(double));
+#define QINPL(v, k, l, mm, f) Qin_pl[((((f) - 1) * 2 + ((mm) - 1)) * npl) + (((size_t)((l) - 1) * kall + ((k) - 1)) * gall_pl + ((v) - 1))]
+#define QOUTPL(v, k, l, mm) Qout_pl[(((mm) - 1) * npl) + (((size_t)((l) - 1) * kall + ((k) - 1)) * gall_pl + ((v) - 1))]
+
+    for (int l = 1; l <= lall; ++l) {
+#pragma omp parallel for schedule(static)
+        for (int k = 1; k <= kall; ++k) {
+            /* per-slab Qin(gall,2,6) and Qout(gall,2) */
+            double *Qin = (double *)malloc(sizeof(double) * gall * 12);
+            double *Qout = (double *)malloc(sizeof(double) * gall * 2);
+#define QIN(g, mm, f) Qin[(((f) - 1) * 2 + ((mm) - 1)) * (size_t)gall + ((g) - 1)]
+#define QOUT(g, mm) Qout[((mm) - 1) * (size_t)gall + ((g) - 1)]
+            /* HLIM:140-190  inflow bounds */
+            for (int j = gmin - 1; j <= gmax; ++j)
+                for (int i = gmin - 1; i <= gmax; ++i) {
+                    const int ij = (j - 1) * iall + i, ip1j = ij + 1, ip1jp1 = ij + iall + 1, ijp1 = ij + iall;
+                    int im1j = ij - 1, ijm1 = ij - iall;
+                    im1j = im1j > 1 ? im1j : 1; /* :152-153 */
+                    ijm1 = ijm1 > 1 ? ijm1 : 1;
+                    const double qij = F3(q, ij, k, l);
+                    const double q_min_AI = mn(mn(mn(qij, F3(q, ijm1, k, l)), F3(q, ip1j, k, l)), F3(q, ip1jp1, k, l));
+                    const double q_max_AI = mx(mx(mx(qij, F3(q, ijm1, k, l)), F3(q, ip1j, k, l)), F3(q, ip1jp1, k, l));
+                    const double q_min_AIJ = mn(mn(mn(qij, F3(q, ip1j, k, l)), F3(q, ip1jp1, k, l)), F3(q, ijp1, k, l));
+                    const double q_max_AIJ = mx(mx(mx(qij, F3(q, ip1j, k, l)), F3(q, ip1jp1, k, l)), F3(q, ijp1, k, l));
+                    const double q_min_AJ = mn(mn(mn(qij, F3(q, ip1jp1, k, l)), F3(q, ijp1, k, l)), F3(q, im1j, k, l));
+                    const double q_max_AJ = mx(mx(mx(qij, F3(q, ip1jp1, k, l)), F3(q, ijp1, k, l)), F3(q, im1j, k, l));
+                    const double c1 = F4(cmask, ij, k, l, 1), c2 = F4(cmask, ij, k, l, 2), c3 = F4(cmask, ij, k, l, 3);
+
+                    QIN(ij, I_MIN, 1) = (c1)*q_min_AI + (1.0 - c1) * BIG;
+                    QIN(ip1j, I_MIN, 4) = (c1)*BIG + (1.0 - c1) * q_min_AI;
+                    QIN(ij, I_MAX, 1) = (c1)*q_max_AI + (1.0 - c1) * (-BIG);
+                    QIN(ip1j, I_MAX, 4) = (c1) * (-BIG) + (1.0 - c1) * q_max_AI;
+
+                    QIN(ij, I_MIN, 2) = (c2)*q_min_AIJ + (1.0 - c2) * BIG;
+                    QIN(ip1jp1, I_MIN, 5) = (c2)*BIG + (1.0 - c2) * q_min_AIJ;
+                    QIN(ij, I_MAX, 2) = (c2)*q_max_AIJ + (1.0 - c2) * (-BIG);
+                    QIN(ip1jp1, I_MAX, 5) = (c2) * (-BIG) + (1.0 - c2) * q_max_AIJ;
+
+                    QIN(ij, I_MIN, 3) = (c3)*q_min_AJ + (1.0 - c3) * BIG;
+                    QIN(ijp1, I_MIN, 6) = (c3)*BIG + (1.0 - c3) * q_min_AJ;
+                    QIN(ij, I_MAX, 3) = (c3)*q_max_AJ + (1.0 - c3) * (-BIG);
+                    QIN(ijp1, I_MAX, 6) = (c3) * (-BIG) + (1.0 - c3) * q_max_AJ;
+                }
+            /* HLIM:192-214  singular point */
+            if (have_sgp[l - 1]) {
+                const int ij = (gmin - 1 - 1) * iall + (gmin - 1);
+                const int ijp1 = ij + iall, ip1jp1 = ij + iall + 1, ip2jp1 = ij + iall + 2;
+                const double q_min_AIJ = mn(mn(mn(F3(q, ij, k, l), F3(q, ip1jp1, k, l)), F3(q, ip2jp1, k, l)), F3(q, ijp1, k, l));
+                const double q_max_AIJ = mx(mx(mx(F3(q, ij, k, l), F3(q, ip1jp1, k, l)), F3(q, ip2jp1, k, l)), F3(q, ijp1, k, l));
+                const double c2 = F4(cmask, ij, k, l, 2);
+                QIN(ij, I_MIN, 2) = (c2)*q_min_AIJ + (1.0 - c2) * BIG;
+                QIN(ip1jp1, I_MIN, 5) = (c2)*BIG + (1.0 - c2) * q_min_AIJ;
+                QIN(ij, I_MAX, 2) = (c2)*q_max_AIJ + (1.0 - c2) * (-BIG);
+                QIN(ip1jp1, I_MAX, 5) = (c2) * (-BIG) + (1.0 - c2) * q_max_AIJ;
+            }
+            /* HLIM:216-284  allowable range */
+            for (int j = gmin; j <= gmax; ++j)
+                for (int i = gmin; i <= gmax; ++i) {
+                    const int ij = (j - 1) * iall + i;
+                    const double qij = F3(q, ij, k, l);
+                    double qnext_min = qij, qnext_max = qij;
+                    for (int f = 1; f <= 6; ++f) qnext_min = mn(qnext_min, QIN(ij, I_MIN, f));
+                    for (int f = 1; f <= 6; ++f) qnext_max = mx(qnext_max, QIN(ij, I_MAX, f));
+                    double chv[7], chm[7];
+                    for (int f = 1; f <= 6; ++f) {
+                        chv[f] = F4(ch, ij, k, l, f);
+                        chm[f] = mn(chv[f], 0.0);
+                    }
+                    const double Cin_sum = chm[1] + chm[2] + chm[3] + chm[4] + chm[5] + chm[6];
+                    const double Cout_sum = chv[1] - chm[1] + chv[2] - chm[2] + chv[3] - chm[3] + chv[4] - chm[4] + chv[5] - chm[5] + chv[6] - chm[6];
+                    const double CQin_min_sum = chm[1] * QIN(ij, I_MIN, 1) + chm[2] * QIN(ij, I_MIN, 2) + chm[3] * QIN(ij, I_MIN, 3) + chm[4] * QIN(ij, I_MIN, 4) + chm[5] * QIN(ij, I_MIN, 5) + chm[6] * QIN(ij, I_MIN, 6);
+                    const double CQin_max_sum = chm[1] * QIN(ij, I_MAX, 1) + chm[2] * QIN(ij, I_MAX, 2) + chm[3] * QIN(ij, I_MAX, 3) + chm[4] * QIN(ij, I_MAX, 4) + chm[5] * QIN(ij, I_MAX, 5) + chm[6] * QIN(ij, I_MAX, 6);
+                    const double zerosw = 0.5 - fsign(0.5, fabs(Cout_sum) - EPS);
+                    const double dij = F3(d, ij, k, l);
+                    QOUT(ij, I_MIN) = (qij - CQin_max_sum - qnext_max * (1.0 - Cin_sum - Cout_sum + dij)) / (Cout_sum + zerosw) * (1.0 - zerosw) + qij * zerosw;
+                    QOUT(ij, I_MAX) = (qij - CQin_min_sum - qnext_min * (1.0 - Cin_sum - Cout_sum + dij)) / (Cout_sum + zerosw) * (1.0 - zerosw) + qij * zerosw;
+                }
+            /* HLIM:286-301  halo: Qout = q */
+            for (int j = 1; j <= iall; ++j)
+                for (int i = 1; i <= iall; ++i)
+                    if (i < gmin || i > gmax || j < gmin || j > gmax) {
+                        const int ij = (j - 1) * iall + i;
+                        QOUT(ij, I_MIN) = F3(q, ij, k, l);
+                        QOUT(ij, I_MAX) = F3(q, ij, k, l);
+                    }
+            /* HLIM:369 Qout_prev = Qout;  :372 Qout = Qout_post */
+            for (int g = 1; g <= gall; ++g) {
+                F4(Qout_prev, g, k, l, I_MIN) = QOUT(g, I_MIN);
+                F4(Qout_prev, g, k, l, I_MAX) = QOUT(g, I_MAX);
+                QOUT(g, I_MIN) = F4(Qout_post, g, k, l, I_MIN);
+                QOUT(g, I_MAX) = F4(Qout_post, g, k, l, I_MAX);
+            }
+            /* HLIM:377-410  apply inflow/outflow limiter */
+            for (int j = gmin - 1; j <= gmax; ++j)
+                for (int i = gmin - 1; i <= gmax; ++i) {
+                    const int ij = (j - 1) * iall + i, ip1j = ij + 1, ip1jp1 = ij + iall + 1, ijp1 = ij + iall;
+                    const int nb[4] = {0, ip1j, ip1jp1, ijp1};
+                    for (int m = 1; m <= 3; ++m) {
+                        const double c = F4(cmask, ij, k, l, m);
+                        double qa = F4(q_a, ij, k, l, m);
+                        qa = (c)*mn(mx(qa, QIN(ij, I_MIN, m)), QIN(ij, I_MAX, m)) + (1.0 - c) * mn(mx(qa, QIN(nb[m], I_MIN, m + 3)), QIN(nb[m], I_MAX, m + 3));
+                        qa = (c)*mx(mn(qa, QOUT(nb[m], I_MAX)), QOUT(nb[m], I_MIN)) + (1.0 - c) * mx(mn(qa, QOUT(ij, I_MAX)), QOUT(ij, I_MIN));
+                        F4(q_a, ij, k, l, m) = qa;
+                        F4(q_a, nb[m], k, l, m + 3) = qa;
+                    }
+                }
+#undef QIN
+#undef QOUT
+            free(Qin);
+            free(Qout);
+        }
+    }
+
+    if (D->have_pl) {
+        const int n = GSLF_PL;
+        /* HLIM:307-363 */
+        for (int l = 1; l <= LALL_PL; ++l)
+            for (int k = 1; k <= kall; ++k) {
+                for (int v = GMIN_PL; v <= gmax_pl; ++v) {
+                    const int ij = v;
+                    int ijp1 = v + 1, ijm1 = v - 1;
+                    if (ijp1 == gmax_pl + 1) ijp1 = GMIN_PL;
+                    if (ijm1 == GMIN_PL - 1) ijm1 = gmax_pl;
+                    const double q_min_pl = mn(mn(mn(P3(q_pl, n, k, l), P3(q_pl, ij, k, l)), P3(q_pl, ijm1, k, l)), P3(q_pl, ijp1, k, l));
+                    const double q_max_pl = mx(mx(mx(P3(q_pl, n, k, l), P3(q_pl, ij, k, l)), P3(q_pl, ijm1, k, l)), P3(q_pl, ijp1, k, l));
+                    const double c = P3(cmask_pl, ij, k, l);
+                    QINPL(ij, k, l, I_MIN, 1) = (c)*q_min_pl + (1.0 - c) * BIG;
+                    QINPL(ij, k, l, I_MIN, 2) = (c)*BIG + (1.0 - c) * q_min_pl;
+                    QINPL(ij, k, l, I_MAX, 1) = (c)*q_max_pl + (1.0 - c) * (-BIG);
+                    QINPL(ij, k, l, I_MAX, 2) = (c) * (-BIG) + (1.0 - c) * q_max_pl;
+                }
+                double qnext_min_pl = P3(q_pl, n, k, l), qnext_max_pl = P3(q_pl, n, k, l);
+                for (int v = GMIN_PL; v <= gmax_pl; ++v) {
+                    qnext_min_pl = mn(qnext_min_pl, QINPL(v, k, l, I_MIN, 1));
+                    qnext_max_pl = mx(qnext_max_pl, QINPL(v, k, l, I_MAX, 1));
+                }
+                double Cin_sum_pl = 0.0, Cout_sum_pl = 0.0, CQin_max_sum_pl = 0.0, CQin_min_sum_pl = 0.0;
+                for (int v = GMIN_PL; v <= gmax_pl; ++v) {
+                    const double ch_masked = P3(cmask_pl, v, k, l) * P3(ch_pl, v, k, l);
+                    Cin_sum_pl = Cin_sum_pl + ch_masked;
+                    Cout_sum_pl = Cout_sum_pl - ch_masked + P3(ch_pl, v, k, l);
+                    CQin_min_sum_pl = CQin_min_sum_pl + ch_masked * QINPL(v, k, l, I_MIN, 1);
+                    CQin_max_sum_pl = CQin_max_sum_pl + ch_masked * QINPL(v, k, l, I_MAX, 1);
+                }
+                const double zerosw = 0.5 - fsign(0.5, fabs(Cout_sum_pl) - EPS);
+                const double qn = P3(q_pl, n, k, l), dn = P3(d_pl, n, k, l);
+                QOUTPL(n, k, l, I_MIN) = (qn - CQin_max_sum_pl - qnext_max_pl * (1.0 - Cin_sum_pl - Cout_sum_pl + dn)) / (Cout_sum_pl + zerosw) * (1.0 - zerosw) + qn * zerosw;
+                QOUTPL(n, k, l, I_MAX) = (qn - CQin_min_sum_pl - qnext_min_pl * (1.0 - Cin_sum_pl - Cout_sum_pl + dn)) / (Cout_sum_pl + zerosw) * (1.0 - zerosw) + qn * zerosw;
+            }
+        /* HLIM:367-373  ring zeroed, copied out, replaced by the post-exchange values */
+        for (int l = 1; l <= LALL_PL; ++l)
+            for (int k = 1; k <= kall; ++k)
+                for (int v = 1; v <= gall_pl; ++v)
+                    for (int mm = I_MIN; mm <= I_MAX; ++mm) {
+                        if (v >= GMIN_PL) QOUTPL(v, k, l, mm) = 0.0;
+                        P4(Qout_prev_pl, v, k, l, mm) = QOUTPL(v, k, l, mm);
+                        QOUTPL(v, k, l, mm) = P4(Qout_post_pl, v, k, l, mm);
+                    }
+        /* HLIM:412-425 */
+        for (int l = 1; l <= LALL_PL; ++l)
+            for (int k = 1; k <= kall; ++k)
+                for (int v = GMIN_PL; v <= gmax_pl; ++v) {
+                    const double c = P3(cmask_pl, v, k, l);
+                    double qa = P3(q_a_pl, v, k, l);
+                    qa = (c)*mn(mx(qa, QINPL(v, k, l, I_MIN, 1)), QINPL(v, k, l, I_MAX, 1)) + (1.0 - c) * mn(mx(qa, QINPL(v, k, l, I_MIN, 2)), QINPL(v, k, l, I_MAX, 2));
+                    qa = (c)*mx(mn(qa, QOUTPL(v, k, l, I_MAX)), QOUTPL(v, k, l, I_MIN)) + (1.0 - c) * mx(mn(qa, QOUTPL(n, k, l, I_MAX)), QOUTPL(n, k, l, I_MIN));
+                    P3(q_a_pl, v, k, l) = qa;
+                }
+    } else {
+        /* HLIM:367-370 run without a pole region too: the ring of Qout_prev_pl
+         * is zero and its centre point is left undefined by the reference. */
+        for (int l = 1; l <= LALL_PL; ++l)
+            for (int k = 1; k <= kall; ++k)
+                for (int v = GMIN_PL; v <= gall_pl; ++v)
+                    for (int mm = I_MIN; mm <= I_MAX; ++mm) P4(Qout_prev_pl, v, k, l, mm) = 0.0;
+    }
+#undef QINPL
+#undef QOUTPL
+    free(Qin_pl);
+    free(Qout_pl);
+}
+
+/* ===========================================================================
+ * vertical_limiter_thuburn: VLIM/mod_src_tracer.f90:41-243
+ * ======================================================================== */
+void orc_vertical_limiter_thuburn(const orc_dims *D,
+                                  double *q_h, double *q_h_pl,
+                                  const double *q, const double *q_pl,
+                                  const double *d, const double *d_pl,
+                                  const double *ck, const double *ck_pl)
+{
+    const int iall = D->gall_1d, gall = iall * iall, kall = D->kall, lall = D->lall;
+    const int gall_pl = D->gall_pl;
+    const int kmin = 2, kmax = kall - 1;
+    const double EPS = D->eps, BIG = D->big;
+
+    for (int l = 1; l <= lall; ++l) {
+        /* The reference parallelises over g inside the k loop (:145-185); the
+         * columns are independent, so threads own whole columns here. */
+#pragma omp parallel for schedule(static)
+        for (int g = 1; g <= gall; ++g) {
+            double Qout_min_km1 = 0.0, Qout_max_km1 = 0.0;
+            for (int k = kmin; k <= kmax; ++k) {
+                const double inflagL = 0.5 - fsign(0.5, F4(ck, g, k, l, 1));
+                const double inflagU = 0.5 + fsign(0.5, F4(ck, g, k + 1, l, 1));
+                const double qk = F3(q, g, k, l);
+                const double Qin_minL = mn(qk, F3(q, g, k - 1, l)) + (1.0 - inflagL) * BIG;
+                const double Qin_minU = mn(qk, F3(q, g, k + 1, l)) + (1.0 - inflagU) * BIG;
+                const double Qin_maxL = mx(qk, F3(q, g, k - 1, l)) - (1.0 - inflagL) * BIG;
+                const double Qin_maxU = mx(qk, F3(q, g, k + 1, l)) - (1.0 - inflagU) * BIG;
+                const double qnext_min = mn(mn(Qin_minL, Qin_minU), qk);
+                const double qnext_max = mx(mx(Qin_maxL, Qin_maxU), qk);
+                const double ck1 = F4(ck, g, k, l, 1), ck2 = F4(ck, g, k, l, 2);
+                const double Cin = (inflagL)*ck1 + (inflagU)*ck2;
+                const double Cout = (1.0 - inflagL) * ck1 + (1.0 - inflagU) * ck2;
+                const double CQin_min = (inflagL)*ck1 * Qin_minL + (inflagU)*ck2 * Qin_minU;
+                const double CQin_max = (inflagL)*ck1 * Qin_maxL + (inflagU)*ck2 * Qin_maxU;
+                const double zerosw = 0.5 - fsign(0.5, fabs(Cout) - EPS);
+                const double dk = F3(d, g, k, l);
+                const double Qout_min_k = ((qk - qnext_max) + qnext_max * (Cin + Cout - dk) - CQin_max) / (Cout + zerosw) * (1.0 - zerosw) + qk * zerosw;
+                const double Qout_max_k = ((qk - qnext_min) + qnext_min * (Cin + Cout - dk) - CQin_min) / (Cout + zerosw) * (1.0 - zerosw) + qk * zerosw;
+                if (k > kmin) { /* :179-180; the kmin level is peeled (:106-143) */
+                    const double qh = F3(q_h, g, k, l);
+                    F3(q_h, g, k, l) = (inflagL)*mx(mn(qh, Qout_max_km1), Qout_min_km1) + (1.0 - inflagL) * mx(mn(qh, Qout_max_k), Qout_min_k);
+                }
+                Qout_min_km1 = Qout_min_k;
+                Qout_max_km1 = Qout_max_k;
+            }
+        }
+    }
+
+    if (D->have_pl) {
+        double *Qmin = (double *)malloc(sizeof(double) * gall_pl * kall);
+        double *Qmax = (double *)malloc(sizeof(double) * gall_pl * kall);
+#define QP(a, g, k) a[((k) - 1) * gall_pl + ((g) - 1)]
+        for (int l = 1; l <= LALL_PL; ++l) {
+            /* VLIM:194-226 (note the different product grouping of CQin here) */
+            for (int k = kmin; k <= kmax; ++k)
+                for (int g = 1; g <= gall_pl; ++g) {
+                    const double inflagL = 0.5 - fsign(0.5, P4(ck_pl, g, k, l, 1));
+                    const double inflagU = 0.5 + fsign(0.5, P4(ck_pl, g, k + 1, l, 1));
+                    const double qk = P3(q_pl, g, k, l);
+                    const double Qin_minL = mn(qk, P3(q_pl, g, k - 1, l)) + (1.0 - inflagL) * BIG;
+                    const double Qin_minU = mn(qk, P3(q_pl, g, k + 1, l)) + (1.0 - inflagU) * BIG;
+                    const double Qin_maxL = mx(qk, P3(q_pl, g, k - 1, l)) - (1.0 - inflagL) * BIG;
+                    const double Qin_maxU = mx(qk, P3(q_pl, g, k + 1, l)) - (1.0 - inflagU) * BIG;
+                    const double qnext_min = mn(mn(Qin_minL, Qin_minU), qk);
+                    const double qnext_max = mx(mx(Qin_maxL, Qin_maxU), qk);
+                    const double ck1 = P4(ck_pl, g, k, l, 1), ck2 = P4(ck_pl, g, k, l, 2);
+                    const double Cin = (inflagL) * (ck1) + (inflagU) * (ck2);
+                    const double Cout = (1.0 - inflagL) * (ck1) + (1.0 - inflagU) * (ck2);
+                    const double CQin_max = (inflagL) * (ck1 * Qin_maxL) + (inflagU) * (ck2 * Qin_maxU);
+                    const double CQin_min = (inflagL) * (ck1 * Qin_minL) + (inflagU) * (ck2 * Qin_minU);
+                    const double zerosw = 0.5 - fsign(0.5, fabs(Cout) - EPS);
+                    const double dk = P3(d_pl, g, k, l);
+                    QP(Qmin, g, k) = ((qk - qnext_max) + qnext_max * (Cin + Cout - dk) - CQin_max) / (Cout + zerosw) * (1.0 - zerosw) + qk * zerosw;
+                    QP(Qmax, g, k) = ((qk - qnext_min) + qnext_min * (Cin + Cout - dk) - CQin_min) / (Cout + zerosw) * (1.0 - zerosw) + qk * zerosw;
+                }
+            /* VLIM:228-235 */
+            for (int k = kmin + 1; k <= kmax; ++k)
+                for (int g = 1; g <= gall_pl; ++g) {
+                    const double inflagL = 0.5 - fsign(0.5, P4(ck_pl, g, k, l, 1));
+                    const double qh = P3(q_h_pl, g, k, l);
+                    P3(q_h_pl, g, k, l) = (inflagL)*mx(mn(qh, QP(Qmax, g, k - 1)), QP(Qmin, g, k - 1)) + (1.0 - inflagL) * mx(mn(qh, QP(Qmax, g, k)), QP(Qmin, g, k));
+                }
+        }
+#undef QP
+        free(Qmin);
+        free(Qmax);
+    }
+}
+
+/* ===========================================================================
+ * vi_rhow_solver: VI/mod_vi.f90:93-306
+ * ======================================================================== */
+void orc_vi_rhow_solver(const orc_dims *D,
+                        double *rhogw, double *rhogw_pl,
+                        const double *rhogw0, const double *rhogw0_pl,
+                        const double *preg0, const double *preg0_pl,
+                        const double *rhog0, const double *rhog0_pl,
+                        const double *Srho, const double *Srho_pl,
+                        const double *Sw, const double *Sw_pl,
+                        const double *Spre, const double *Spre_pl,
+                        double dt,
+                        const double *Mc, const double *Mc_pl,
+                        const double *Ml, const double *Ml_pl,
+                        const double *Mu, const double *Mu_pl,
+                        const double *RGAM, const double *RGAM_pl,
+                        const double *RGAMH, const double *RGAMH_pl,
+                        const double *RGSGAM2, const double *RGSGAM2_pl,
+                        const double *RGSGAM2H, const double *RGSGAM2H_pl,
+                        const double *GSGAM2H, const double *GSGAM2H_pl,
+                        const double *rdgzh, const double *afact, const double *bfact,
+                        double grav, double rdry, double cvdry, int non_hydro_alpha)
+{
+    const int iall = D->gall_1d, gall = iall * iall, kall = D->kall, lall = D->lall;
+    const int gall_pl = D->gall_pl;
+    const int kmin = 2, kmax = kall - 1;
+    const double CVovRt2 = cvdry / rdry / (dt * dt); /* :178 */
+    const double alpha = (double)non_hydro_alpha;     /* :179 */
+
+    /* One column (g, l) at a time; the reference's k-outer/g-inner sweeps
+     * (:194-246) touch each column independently, so the order of columns
+     * does not change any value. */
+#define VI_COLUMN(G, L, NG, W, W0, PR, RH, SR, SWW, SP, MCC, MLL, MUU, RG, RGH, RGS, RGSH, GSH, A3)        \
+    do {                                                                                                   \
+        double Sall[kall + 1], gam[kall + 1];                                                              \
+        for (int k = kmin + 1; k <= kmax; ++k) {                                                           \
+            Sall[k] = ((A3(W0, G, k, L) * alpha + dt * A3(SWW, G, k, L)) * (A3(RGH, G, k, L) * A3(RGH, G, k, L)) \
+                       - ((A3(PR, G, k, L) + dt * A3(SP, G, k, L)) * A3(RGS, G, k, L)                      \
+                          - (A3(PR, G, k - 1, L) + dt * A3(SP, G, k - 1, L)) * A3(RGS, G, k - 1, L))       \
+                             * dt * rdgzh[k - 1]                                                           \
+                       - ((A3(RH, G, k, L) + dt * A3(SR, G, k, L)) * (A3(RG, G, k, L) * A3(RG, G, k, L)) * afact[k - 1] \
+                          + (A3(RH, G, k - 1, L) + dt * A3(SR, G, k - 1, L)) * (A3(RG, G, k - 1, L) * A3(RG, G, k - 1, L)) * bfact[k - 1]) \
+                             * dt * grav)                                                                  \
+                      * CVovRt2;                                                                           \
+        }                                                                                                  \
+        A3(W, G, kmin, L) = A3(W, G, kmin, L) * A3(RGSH, G, kmin, L);                                     \
+        A3(W, G, kmax + 1, L) = A3(W, G, kmax + 1, L) * A3(RGSH, G, kmax + 1, L);                         \
+        Sall[kmin + 1] = Sall[kmin + 1] - A3(MLL, G, kmin + 1, L) * A3(W, G, kmin, L);                     \
+        Sall[kmax] = Sall[kmax] - A3(MUU, G, kmax, L) * A3(W, G, kmax + 1, L);                             \
+        double beta = A3(MCC, G, kmin + 1, L);                                                             \
+        A3(W, G, kmin + 1, L) = Sall[kmin + 1] / beta;                                                     \
+        for (int k = kmin + 2; k <= kmax; ++k) {                                                           \
+            gam[k] = A3(MUU, G, k - 1, L) / beta;                                                          \
+            beta = A3(MCC, G, k, L) - A3(MLL, G, k, L) * gam[k];                                           \
+            A3(W, G, k, L) = (Sall[k] - A3(MLL, G, k, L) * A3(W, G, k - 1, L)) / beta;                     \
+        }                                                                                                  \
+        for (int k = kmax - 1; k >= kmin + 1; --k) {                                                       \
+            A3(W, G, k, L) = A3(W, G, k, L) - gam[k + 1] * A3(W, G, k + 1, L);                             \
+            A3(W, G, k + 1, L) = A3(W, G, k + 1, L) * A3(GSH, G, k + 1, L);                                \
+        }                                                                                                  \
+        A3(W, G, kmin, L) = A3(W, G, kmin, L) * A3(GSH, G, kmin, L);                                       \
+        A3(W, G, kmin + 1, L) = A3(W, G, kmin + 1, L) * A3(GSH, G, kmin + 1, L);                           \
+        A3(W, G, kmax + 1, L) = A3(W, G, kmax + 1, L) * A3(GSH, G, kmax + 1, L);                           \
+    } while (0)
+
+    for (int l = 1; l <= lall; ++l) {
+#pragma omp parallel for schedule(static)
+        for (int g = 1; g <= gall; ++g)
+            VI_COLUMN(g, l, gall, rhogw, rhogw0, preg0, rhog0, Srho, Sw, Spre, Mc, Ml, Mu,
+                      RGAM, RGAMH, RGSGAM2, RGSGAM2H, GSGAM2H, F3);
+    }
+    if (D->have_pl) { /* :250-301 */
+        for (int l = 1; l <= LALL_PL; ++l)
+            for (int g = 1; g <= gall_pl; ++g)
+                VI_COLUMN(g, l, gall_pl, rhogw_pl, rhogw0_pl, preg0_pl, rhog0_pl, Srho_pl, Sw_pl, Spre_pl,
+                          Mc_pl, Ml_pl, Mu_pl, RGAM_pl, RGAMH_pl, RGSGAM2_pl, RGSGAM2H_pl, GSGAM2H_pl, P3);
+    }
+#undef VI_COLUMN
+}

@@ -1,0 +1,105 @@
+/*
+ * CPU oracle for the NICAM tracer-transport + vertical-implicit hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This is a plain-C restatement of the reference
+ * Fortran kernels, used by the test suite, the package's smoke check and the
+ * CPU-baseline leg of the benchmark as the *checker*.  Nothing in the product
+ * library links, calls or falls back to it.
+ *
+ * Array layout follows the reference exactly (Fortran column-major):
+ *   regular field  x(gall, kall, lall[, n])  -> x[((n*lall + l)*kall + k)*gall + g]
+ *   pole field     x(gall_pl, kall, lall_pl[, n])
+ *   metric GMTR_t  (gall, 1, lall, 2, 5)     -> [((w*2 + t)*lall + l)*gall + g]
+ *   metric GMTR_a  (gall, 1, lall, 3, 12)    -> [((c*3 + a)*lall + l)*gall + g]
+ *   metric GMTR_p  (gall, 1, lall, 8)        -> [(c*lall + l)*gall + g]
+ *   GRD_xr         (gall, 1, lall, 3, 3)     -> [((d*3 + a)*lall + l)*gall + g]
+ *   pole metrics   GMTR_t_pl(gall_pl,1,2,5), GMTR_a_pl(gall_pl,1,2,18),
+ *                  GMTR_p_pl(gall_pl,1,2,8), GRD_xr_pl(gall_pl,1,2,3)
+ * Indices in the restated loops are kept 1-based, as in the reference, and
+ * converted to 0-based offsets only inside the accessor macros.
+ */
+#ifndef NICAM_ORACLE_H
+#define NICAM_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct {
+    int gall_1d;   /* ADM_gall_1d                                   */
+    int kall;      /* ADM_kall                                      */
+    int lall;      /* ADM_lall                                      */
+    int gall_pl;   /* ADM_gall_pl (6 on an icosahedral grid)         */
+    int have_pl;   /* ADM_have_pl                                   */
+    double eps;    /* CONST_EPS                                     */
+    double big;    /* CONST_HUGE                                    */
+} orc_dims;
+
+/* VI/src/mod_misc.f90:447-461 (CONST_setup): EPS = epsilon, HUGE = huge. */
+void orc_const_setup(double *eps, double *huge_val);
+
+/* VI/src/mod_misc.f90:464-524 (GRD_setup): vertical factors from gz/gzh. */
+void orc_grd_setup(int kall, const double *gz, const double *gzh,
+                   double *dgz, double *dgzh, double *rdgz, double *rdgzh,
+                   double *afact, double *bfact, double *cfact, double *dfact);
+
+/* FLX/src/mod_src_tracer.f90:42-379 */
+void orc_horizontal_flux(const orc_dims *D, const int *have_sgp,
+                         double *flx_h, double *flx_h_pl,
+                         double *GRD_xc, double *GRD_xc_pl,
+                         const double *rho, const double *rho_pl,
+                         const double *rhovx, const double *rhovx_pl,
+                         const double *rhovy, const double *rhovy_pl,
+                         const double *rhovz, const double *rhovz_pl,
+                         double dt,
+                         const double *GRD_xr, const double *GRD_xr_pl,
+                         const double *GMTR_p, const double *GMTR_p_pl,
+                         const double *GMTR_t, const double *GMTR_t_pl,
+                         const double *GMTR_a, const double *GMTR_a_pl);
+
+/* HLIM/src/mod_src_tracer.f90:42-430 */
+void orc_horizontal_limiter_thuburn(const orc_dims *D, const int *have_sgp,
+                                    double *q_a, double *q_a_pl,
+                                    const double *q, const double *q_pl,
+                                    const double *d, const double *d_pl,
+                                    const double *ch, const double *ch_pl,
+                                    const double *cmask, const double *cmask_pl,
+                                    double *Qout_prev, double *Qout_prev_pl,
+                                    const double *Qout_post, const double *Qout_post_pl);
+
+/* VLIM/src/mod_src_tracer.f90:41-243 */
+void orc_vertical_limiter_thuburn(const orc_dims *D,
+                                  double *q_h, double *q_h_pl,
+                                  const double *q, const double *q_pl,
+                                  const double *d, const double *d_pl,
+                                  const double *ck, const double *ck_pl);
+
+/* VI/src/mod_vi.f90:93-306; the coefficients Mc/Ml/Mu that the reference
+ * keeps as module state after vi_rhow_solver_putcoef (:58-89) are passed in. */
+void orc_vi_rhow_solver(const orc_dims *D,
+                        double *rhogw, double *rhogw_pl,
+                        const double *rhogw0, const double *rhogw0_pl,
+                        const double *preg0, const double *preg0_pl,
+                        const double *rhog0, const double *rhog0_pl,
+                        const double *Srho, const double *Srho_pl,
+                        const double *Sw, const double *Sw_pl,
+                        const double *Spre, const double *Spre_pl,
+                        double dt,
+                        const double *Mc, const double *Mc_pl,
+                        const double *Ml, const double *Ml_pl,
+                        const double *Mu, const double *Mu_pl,
+                        const double *RGAM, const double *RGAM_pl,
+                        const double *RGAMH, const double *RGAMH_pl,
+                        const double *RGSGAM2, const double *RGSGAM2_pl,
+                        const double *RGSGAM2H, const double *RGSGAM2H_pl,
+                        const double *GSGAM2H, const double *GSGAM2H_pl,
+                        const double *rdgzh, const double *afact, const double *bfact,
+                        double grav, double rdry, double cvdry, int non_hydro_alpha);
+
+/* Number of OpenMP threads the oracle runs with (1 when built without it). */
+int orc_num_threads(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
