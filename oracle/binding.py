@@ -154,8 +154,39 @@ class Oracle:
         return out
 
 
+def _big_stack(fn, *args):
+    """Runs fn(*args) on a thread with a 1 GiB stack.
+
+    The reference keeps its work arrays as automatic (stack) arrays, e.g.
+    Qin(gall,kall,lall,2,6) in the horizontal limiter
+    (dyn_horiz_adv_limiter/src/mod_src_tracer.f90:98-101): 41 MB for one
+    66x66x98 region, beyond the usual 8 MB main-thread stack."""
+    import threading
+
+    box = {}
+
+    def run():
+        try:
+            box["out"] = fn(*args)
+        except BaseException as e:  # re-raised in the caller
+            box["err"] = e
+
+    old = threading.stack_size()
+    threading.stack_size(1 << 30)
+    try:
+        th = threading.Thread(target=run)
+        th.start()
+        th.join()
+    finally:
+        threading.stack_size(old)
+    if "err" in box:
+        raise box["err"]
+    return box["out"]
+
+
 class RefLibs:
-    """The reference Fortran kernels (oracle/_ref), one region per call."""
+    """The reference Fortran kernels (oracle/_ref), one region per call, each
+    call on a large-stack thread."""
 
     @staticmethod
     def available() -> bool:
@@ -195,7 +226,7 @@ class RefLibs:
         g = {k: _f64(geo[k]) for k in ("GRD_xr", "GRD_xr_pl", "GMTR_p", "GMTR_p_pl", "GMTR_t", "GMTR_t_pl",
                                        "GMTR_a", "GMTR_a_pl")}
         sgp = np.ascontiguousarray(geo["have_sgp"], dtype=np.int32)
-        self.flux.ref_horizontal_flux(spec.gall_1d, spec.kall, spec.lall, spec.gall_pl, int(spec.have_pl),
+        _big_stack(self.flux.ref_horizontal_flux, spec.gall_1d, spec.kall, spec.lall, spec.gall_pl, int(spec.have_pl),
                                       sgp.ctypes.data, _p(out["flx_h"]), _p(out["flx_h_pl"]), _p(out["GRD_xc"]),
                                       _p(out["GRD_xc_pl"]), _p(ins["rho"]), _p(ins["rho_pl"]), _p(ins["rhovx"]),
                                       _p(ins["rhovx_pl"]), _p(ins["rhovy"]), _p(ins["rhovy_pl"]), _p(ins["rhovz"]),
@@ -210,7 +241,7 @@ class RefLibs:
         ins = {k: _f64(f[k]) for k in ("q", "q_pl", "d", "d_pl", "ch", "ch_pl", "cmask", "cmask_pl", "Qout_post",
                                        "Qout_post_pl")}
         sgp = np.ascontiguousarray(have_sgp, dtype=np.int32)
-        self.hlim.ref_horizontal_limiter_thuburn(spec.gall_1d, spec.kall, spec.lall, spec.gall_pl, int(spec.have_pl),
+        _big_stack(self.hlim.ref_horizontal_limiter_thuburn, spec.gall_1d, spec.kall, spec.lall, spec.gall_pl, int(spec.have_pl),
                                                  sgp.ctypes.data, _p(out["q_a"]), _p(out["q_a_pl"]), _p(ins["q"]),
                                                  _p(ins["q_pl"]), _p(ins["d"]), _p(ins["d_pl"]), _p(ins["ch"]),
                                                  _p(ins["ch_pl"]), _p(ins["cmask"]), _p(ins["cmask_pl"]),
@@ -221,7 +252,7 @@ class RefLibs:
     def vertical_limiter_thuburn(self, spec, f):
         out = dict(q_h=_f64(f["q_h"]), q_h_pl=_f64(f["q_h_pl"]))
         ins = {k: _f64(f[k]) for k in ("q", "q_pl", "d", "d_pl", "ck", "ck_pl")}
-        self.vlim.ref_vertical_limiter_thuburn(spec.gall_1d, spec.kall, spec.lall, spec.gall_pl, int(spec.have_pl),
+        _big_stack(self.vlim.ref_vertical_limiter_thuburn, spec.gall_1d, spec.kall, spec.lall, spec.gall_pl, int(spec.have_pl),
                                                _p(out["q_h"]), _p(out["q_h_pl"]), _p(ins["q"]), _p(ins["q_pl"]),
                                                _p(ins["d"]), _p(ins["d_pl"]), _p(ins["ck"]), _p(ins["ck_pl"]))
         return out
@@ -241,7 +272,7 @@ class RefLibs:
         for n in names[6:]:
             a2 += [_p(ins[n]), _p(ins[n + "_pl"])]
         v = [_f64(rdgzh), _f64(afact), _f64(bfact)]
-        self.vi.ref_vi_rhow_solver(spec.gall_1d, spec.kall, spec.lall, spec.gall_pl, int(spec.have_pl),
+        _big_stack(self.vi.ref_vi_rhow_solver, spec.gall_1d, spec.kall, spec.lall, spec.gall_pl, int(spec.have_pl),
                                    _p(out["rhogw"]), _p(out["rhogw_pl"]), *a1, float(dt), *a2,
                                    *[x.ctypes.data for x in v])
         return out
