@@ -171,7 +171,7 @@ int nicam_ctx_destroy(nicam_ctx c)
                       &c->GMTR_t_pl, &c->GMTR_a, &c->GMTR_a_pl, &c->RGAM, &c->RGAM_pl,
                       &c->RGAMH, &c->RGAMH_pl, &c->RGSGAM2, &c->RGSGAM2_pl, &c->RGSGAM2H,
                       &c->RGSGAM2H_pl, &c->GSGAM2H, &c->GSGAM2H_pl, &c->rdgzh, &c->afact,
-                      &c->bfact, &c->Mc, &c->Mc_pl, &c->Ml, &c->Ml_pl, &c->Mu, &c->Mu_pl};
+                      &c->bfact, &c->Mc, &c->Mc_pl, &c->Ml, &c->Ml_pl, &c->Mu, &c->Mu_pl, &c->vi_gamma};
     for (double **p : all) dev_free(*p);
     for (void *p : c->stage)
         if (p) (void)hipFree(p);
@@ -193,6 +193,23 @@ int nicam_ctx_set_pointer_mode(nicam_ctx c, int mode)
     if (mode != NICAM_PTR_DEVICE && mode != NICAM_PTR_HOST)
         return fail(NICAM_ERR_ARG, "pointer mode must be NICAM_PTR_DEVICE or NICAM_PTR_HOST");
     c->ptr_mode = mode;
+    return NICAM_OK;
+}
+
+int nicam_ctx_set_tuning(nicam_ctx c, int param, int value)
+{
+    NICAM_TRY(check_ctx(c));
+    if (value < 0) return fail(NICAM_ERR_ARG, "tuning values must be >= 0");
+    switch (param) {
+    case NICAM_TUNE_HFLUX_KPER: c->tune.hflux_kper = value; break;
+    case NICAM_TUNE_HLIM_KPER: c->tune.hlim_kper = value; break;
+    case NICAM_TUNE_VLIM_KCHUNK: c->tune.vlim_kchunk = value; break;
+    case NICAM_TUNE_VI_VARIANT:
+        if (value > 3) return fail(NICAM_ERR_ARG, "NICAM_TUNE_VI_VARIANT must be 0..3");
+        c->tune.vi_variant = value;
+        break;
+    default: return fail(NICAM_ERR_ARG, "unknown tuning parameter");
+    }
     return NICAM_OK;
 }
 

@@ -10,208 +10,235 @@
 // neighbour's arc instead of being scattered, so every store has exactly one
 // producer and the stores of a wavefront are contiguous in g.
 //
-// Each work-group covers CHUNK consecutive cells g of one (k, l) slab.  The
-// triangle-vertex values TI/TJ needed by the chunk (cells g0-gall_1d-1 ..
-// g0+CHUNK-1) are computed once into LDS from the rho/rhov values, which are
-// themselves staged in LDS; the 3 arcs of every cell then read only LDS.
-// Floating-point operations follow the reference's order term by term (the
-// library is compiled with -ffp-contract=off), so the FP64 results are the
-// reference's to the last bit.
+// A work-group owns CHUNK consecutive cells g of one region and marches up a
+// range of levels.  The K-invariant metrics of its cells (arc normals, arc
+// midpoints, cell areas, triangle weights) are loaded once into registers.
+// At each level the group stages rho/rhov of cells [g0-gall_1d-1,
+// g0+CHUNK+gall_1d+1) in LDS, builds TI/TJ of cells [g0-gall_1d-1, g0+CHUNK)
+// in LDS, and every work-item then forms its 6 arcs from LDS.  The next
+// level's rho/rhov are loaded into registers while the current level is
+// computed.  Floating-point operations follow the reference's order term by
+// term (the library is compiled with -ffp-contract=off), so the FP64 results
+// are the reference's to the last bit.
 #include "internal.h"
 
 namespace nicam {
 namespace {
 
-constexpr int CHUNK = 256;          // output cells per work-group
-constexpr int MAXI = 160;           // largest gall_1d the LDS tiles are sized for
+constexpr int CHUNK = 256;  // output cells per work-group
+constexpr int MAXI = 160;   // largest gall_1d supported (gl8/rl1 regions are 130)
+constexpr int NST = 3;      // staged cells per work-item: CHUNK*NST >= CHUNK + 2*(MAXI+1)
+constexpr int NTP = 2;      // TI/TJ points per work-item: CHUNK*NTP >= CHUNK + MAXI + 1
 
 struct HfluxArgs {
-    double *flx;       // (gall,kall,lall,6)
-    double *xc;        // (gall,kall,lall,3,3)
+    double *flx;         // (gall,kall,lall,6)
+    double *xc;          // (gall,kall,lall,3,3)
     const double *f[4];  // rho, rhovx, rhovy, rhovz (gall,kall,lall)
     const double *xr, *gp, *gt, *ga;
     const int *have_sgp;
     int iall, gall, kall, lall;
+    int nchunk, nkr, kper;  // chunks per slab, level ranges, levels per range
     double dt, eps;
 };
 
 __device__ __forceinline__ double dmax(double a, double b) { return (b > a) ? b : a; }
 
-// One arc: flux through it and the centroid shift, in the reference's order
-// (FLX:241-255 for AI; :266-280 AIJ; :292-306 AJ).
-struct Arc {
-    double flux, x2, y2, z2, rr2;
-};
-
-__device__ __forceinline__ Arc make_arc(const double (&ta)[4], const double (&tb)[4], double hnx,
-                                        double hny, double hnz, double eps)
-{
-    Arc a;
-    a.rr2 = 1.0 / dmax(ta[0] + tb[0], eps);
-    a.x2 = ta[1] + tb[1];
-    a.y2 = ta[2] + tb[2];
-    a.z2 = ta[3] + tb[3];
-    a.flux = 0.5 * (a.x2 * hnx + a.y2 * hny + a.z2 * hnz);
-    return a;
-}
-
 __global__ void __launch_bounds__(CHUNK) hflux_kernel(HfluxArgs A)
 {
-    // LDS: rho/rhov for cells [g0 - iall - 1, g0 + CHUNK + iall + 1)
-    //      TI/TJ (4 fields each) for cells [g0 - iall - 1, g0 + CHUNK)
-    constexpr int SPAN_IN = CHUNK + 2 * (MAXI + 1);
-    constexpr int SPAN_T = CHUNK + MAXI + 1;
-    __shared__ double s_in[4][SPAN_IN];
-    __shared__ double s_ti[4][SPAN_T];
-    __shared__ double s_tj[4][SPAN_T];
-
+    extern __shared__ double lds[];
     const int iall = A.iall, gall = A.gall, kall = A.kall, lall = A.lall;
-    const int nchunk = (gall + CHUNK - 1) / CHUNK;
-    // blocks of one region are consecutive, and consecutive blocks land on
-    // different XCDs; group a region's (k, chunk) blocks onto one XCD so its
-    // metrics stay in that XCD's L2.
-    const int nb = gridDim.x;
-    int b = blockIdx.x;
-    {
-        const int nxcd = 8;
-        const int per = nb / nxcd;
-        if (per > 0 && b < per * nxcd) b = (b % nxcd) * per + b / nxcd;
-    }
-    const int chunk = b % nchunk;
-    const int kl = b / nchunk;
-    const int k = kl % kall;
-    const int l = kl / kall;
-    if (l >= lall) return;
-    const int g0 = chunk * CHUNK;
-    const int lo_in = g0 - iall - 1;  // first cell staged
     const int n_in = CHUNK + 2 * (iall + 1);
     const int n_t = CHUNK + iall + 1;
-    const size_t fb = ((size_t)l * kall + k) * gall;
-    const size_t mb = (size_t)l * gall;
+    double *s_in = lds;             // [4][n_in]
+    double *s_t = lds + 4 * n_in;   // [8][n_t]: TI fields 0..3, TJ fields 4..7
+
+    // Logical block -> (region, level range, chunk); the chunks of one region
+    // and level range are consecutive and are kept on one XCD (hardware
+    // dispatch sends block b to XCD b % 8) so their shared halo rows hit L2.
+    int b = blockIdx.x;
+    {
+        const int nxcd = 8, per = gridDim.x / nxcd;
+        if (per > 0 && b < per * nxcd) b = (b % nxcd) * per + b / nxcd;
+    }
+    const int chunk = b % A.nchunk;
+    const int kr = (b / A.nchunk) % A.nkr;
+    const int l = b / (A.nchunk * A.nkr);
+    if (l >= lall) return;
+    const int k0 = kr * A.kper;
+    const int k1 = min(kall, k0 + A.kper);
+    if (k0 >= k1) return;
+
+    const int g0 = chunk * CHUNK;
+    const int lo = g0 - iall - 1;  // first staged / TI-TJ cell
     const int tid = threadIdx.x;
-
-    // stage inputs
-    for (int s = tid; s < n_in; s += CHUNK) {
-        const int g = lo_in + s;
-        const bool ok = (g >= 0 && g < gall);
-#pragma unroll
-        for (int f = 0; f < 4; ++f) s_in[f][s] = ok ? A.f[f][fb + g] : 0.0;
-    }
-    __syncthreads();
-
-    // triangle-vertex values at points p = lo_in + s (valid where i,j <= iall-2)
-    const size_t tstride = (size_t)lall * gall;  // between GMTR_t(…,t,w) planes
-    for (int s = tid; s < n_t; s += CHUNK) {
-        const int p = lo_in + s;
-        if (p < 0 || p >= gall) continue;
-        const int i = p % iall, j = p / iall;
-        if (i > iall - 2 || j > iall - 2) continue;
-        // GMTR_t(g,1,l,t,w): index ((w*2 + t)*lall + l)*gall + g, w = 2,3,4 for W1..W3
-        const double wi1 = A.gt[(2 * 2 + 0) * tstride + mb + p];
-        const double wi2 = A.gt[(3 * 2 + 0) * tstride + mb + p];
-        const double wi3 = A.gt[(4 * 2 + 0) * tstride + mb + p];
-        const double wj1 = A.gt[(2 * 2 + 1) * tstride + mb + p];
-        const double wj2 = A.gt[(3 * 2 + 1) * tstride + mb + p];
-        const double wj3 = A.gt[(4 * 2 + 1) * tstride + mb + p];
-        const int s00 = s, s10 = s + 1, s11 = s + iall + 1, s01 = s + iall;
-#pragma unroll
-        for (int f = 0; f < 4; ++f) {
-            // FLX:146-153 then :171-174
-            double ti = s_in[f][s00] * wi1 + s_in[f][s10] * wi2;
-            ti = ti + s_in[f][s11] * wi3;
-            // FLX:155-158 then :176-183
-            double tj = s_in[f][s00] * wj1;
-            tj = tj + s_in[f][s11] * wj2 + s_in[f][s01] * wj3;
-            s_ti[f][s] = ti;
-            s_tj[f][s] = tj;
-        }
-    }
-    __syncthreads();
-    // singular point: TI(gmin-1,gmin-1) := TJ(gmin,gmin-1)  (FLX:188-201)
-    if (A.have_sgp[l] && lo_in <= 0 && 1 - lo_in < n_t && tid < 4) {
-        s_ti[tid][0 - lo_in] = s_tj[tid][1 - lo_in];
-    }
-    __syncthreads();
-
-    const int g = g0 + tid;
-    if (g >= gall) return;
-    const int i = g % iall, j = g / iall;
-    const int gmx = iall - 2;  // last index of the TI/TJ range (0-based gmax-1... = iall-2)
-    const int s = g - lo_in;   // LDS slot of this cell
+    const size_t slab = (size_t)gall;
+    const size_t mb = (size_t)l * gall;
+    const size_t mstr = (size_t)lall * gall;  // between metric planes
+    const bool sgp = A.have_sgp[l] != 0;
     const double dt = A.dt, eps = A.eps;
 
-    double face[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-    double xcv[3][3] = {{0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}};
-
-    auto T = [&](double (&dst)[4], const double (*src)[SPAN_T], int slot) {
+    // ---- K-invariant metrics into registers ----
+    // triangle weights of my TI/TJ points (GMTR_t(g,1,l,t,w): ((w*2+t)*lall+l)*gall+g)
+    double wt[NTP][6];
+    int tp[NTP];
 #pragma unroll
-        for (int f = 0; f < 4; ++f) dst[f] = src[f][slot];
-    };
-    const size_t astride = (size_t)lall * gall;
-    auto HN = [&](int arc, int c, int p) { return A.ga[((size_t)c * 3 + arc) * astride + mb + p]; };
-    auto XR = [&](int arc, int d) { return A.xr[((size_t)d * 3 + arc) * astride + mb + g]; };
-    const double rarea = A.gp[1 * astride + mb + g];  // GMTR_p(:,:,:,P_RAREA)
+    for (int r = 0; r < NTP; ++r) {
+        const int s = tid + r * CHUNK;
+        const int p = lo + s;
+        tp[r] = -1;
+        if (s < n_t && p >= 0 && p < gall && (p % iall) <= iall - 2 && (p / iall) <= iall - 2) tp[r] = p;
+#pragma unroll
+        for (int q = 0; q < 6; ++q) {
+            const int w = 2 + (q % 3), t = q / 3;
+            wt[r][q] = tp[r] >= 0 ? A.gt[(size_t)(w * 2 + t) * mstr + mb + p] : 0.0;
+        }
+    }
+    const int g = g0 + tid;
+    const bool active = g < gall;
+    const int i = active ? g % iall : 0, j = active ? g / iall : 0;
+    const int gmx = iall - 2;
+    // which of the 6 arcs touching this cell exist (FLX:234-309 loop bounds)
+    const bool a_ai = active && i <= gmx && j >= 1 && j <= gmx;
+    const bool a_aij = active && i <= gmx && j <= gmx;
+    const bool a_aj = active && i >= 1 && i <= gmx && j <= gmx;
+    const bool a_f4 = active && i >= 1 && i - 1 <= gmx && j >= 1 && j <= gmx;
+    const bool a_f5 = active && i >= 1 && j >= 1 && i - 1 <= gmx && j - 1 <= gmx;
+    const bool a_f6 = active && i >= 1 && i <= gmx && j >= 1 && j - 1 <= gmx;
+    // GMTR_a(g,1,l,a,c): ((c*3+a)*lall+l)*gall+g; GRD_xr(g,1,l,a,d): ((d*3+a)*lall+l)*gall+g
+    double hn[6][3], xr[3][3], rarea = 0.0;
+    {
+        const bool have[6] = {a_ai, a_aij, a_aj, a_f4, a_f5, a_f6};
+        const int org[6] = {g, g, g, g - 1, g - 1 - iall, g - iall};
+#pragma unroll
+        for (int a = 0; a < 6; ++a)
+#pragma unroll
+            for (int c = 0; c < 3; ++c)
+                hn[a][c] = have[a] ? A.ga[(size_t)(c * 3 + (a % 3)) * mstr + mb + org[a]] : 0.0;
+#pragma unroll
+        for (int a = 0; a < 3; ++a)
+#pragma unroll
+            for (int d = 0; d < 3; ++d) xr[a][d] = active ? A.xr[(size_t)(d * 3 + a) * mstr + mb + g] : 0.0;
+        if (active) rarea = A.gp[1 * mstr + mb + g];
+    }
 
-    double ta[4], tb[4];
-    // own arcs (origin = this cell)
-    if (i <= gmx && j >= 1 && j <= gmx) {  // AI: TJ(ijm1) + TI(ij)
-        T(ta, s_tj, s - iall);
-        T(tb, s_ti, s);
-        const Arc a = make_arc(ta, tb, HN(0, 0, g), HN(0, 1, g), HN(0, 2, g), eps);
-        face[0] = a.flux * rarea * dt;
-        xcv[0][0] = XR(0, 0) - a.x2 * a.rr2 * dt * 0.5;
-        xcv[0][1] = XR(0, 1) - a.y2 * a.rr2 * dt * 0.5;
-        xcv[0][2] = XR(0, 2) - a.z2 * a.rr2 * dt * 0.5;
-    }
-    if (i <= gmx && j <= gmx) {  // AIJ: TI(ij) + TJ(ij)
-        T(ta, s_ti, s);
-        T(tb, s_tj, s);
-        const Arc a = make_arc(ta, tb, HN(1, 0, g), HN(1, 1, g), HN(1, 2, g), eps);
-        face[1] = a.flux * rarea * dt;
-        xcv[1][0] = XR(1, 0) - a.x2 * a.rr2 * dt * 0.5;
-        xcv[1][1] = XR(1, 1) - a.y2 * a.rr2 * dt * 0.5;
-        xcv[1][2] = XR(1, 2) - a.z2 * a.rr2 * dt * 0.5;
-    }
-    if (i >= 1 && i <= gmx && j <= gmx) {  // AJ: TJ(ij) + TI(im1j)
-        T(ta, s_tj, s);
-        T(tb, s_ti, s - 1);
-        const Arc a = make_arc(ta, tb, HN(2, 0, g), HN(2, 1, g), HN(2, 2, g), eps);
-        face[2] = a.flux * rarea * dt;
-        xcv[2][0] = XR(2, 0) - a.x2 * a.rr2 * dt * 0.5;
-        xcv[2][1] = XR(2, 1) - a.y2 * a.rr2 * dt * 0.5;
-        xcv[2][2] = XR(2, 2) - a.z2 * a.rr2 * dt * 0.5;
-    }
-    // mirrored faces (origin = a neighbour)
-    if (i >= 1 && i - 1 <= gmx && j >= 1 && j <= gmx) {  // face 4 <- AI at (i-1, j)
-        const int o = g - 1;
-        T(ta, s_tj, s - 1 - iall);
-        T(tb, s_ti, s - 1);
-        const Arc a = make_arc(ta, tb, HN(0, 0, o), HN(0, 1, o), HN(0, 2, o), eps);
-        face[3] = -a.flux * rarea * dt;
-    }
-    if (i >= 1 && j >= 1 && i - 1 <= gmx && j - 1 <= gmx) {  // face 5 <- AIJ at (i-1, j-1)
-        const int o = g - 1 - iall;
-        T(ta, s_ti, s - 1 - iall);
-        T(tb, s_tj, s - 1 - iall);
-        const Arc a = make_arc(ta, tb, HN(1, 0, o), HN(1, 1, o), HN(1, 2, o), eps);
-        face[4] = -a.flux * rarea * dt;
-    }
-    if (i >= 1 && i <= gmx && j >= 1 && j - 1 <= gmx) {  // face 6 <- AJ at (i, j-1)
-        const int o = g - iall;
-        T(ta, s_tj, s - iall);
-        T(tb, s_ti, s - iall - 1);
-        const Arc a = make_arc(ta, tb, HN(2, 0, o), HN(2, 1, o), HN(2, 2, o), eps);
-        face[5] = -a.flux * rarea * dt;
-    }
-    if (A.have_sgp[l] && i == 1 && j == 1) face[5] = 0.0;  // FLX:311-320
+    // ---- level loop ----
+    double nxt[NST][4];
+    auto load_level = [&](int k) {
+        const size_t fb = ((size_t)l * kall + k) * slab;
+#pragma unroll
+        for (int r = 0; r < NST; ++r) {
+            const int s = tid + r * CHUNK;
+            const int p = lo + s;
+            const bool ok = s < n_in && p >= 0 && p < gall;
+#pragma unroll
+            for (int f = 0; f < 4; ++f) nxt[r][f] = ok ? A.f[f][fb + p] : 0.0;
+        }
+    };
+    auto store_level = [&]() {
+#pragma unroll
+        for (int r = 0; r < NST; ++r) {
+            const int s = tid + r * CHUNK;
+            if (s < n_in) {
+#pragma unroll
+                for (int f = 0; f < 4; ++f) s_in[f * n_in + s] = nxt[r][f];
+            }
+        }
+    };
+
+    load_level(k0);
+    store_level();
+    __syncthreads();
+    if (k0 + 1 < k1) load_level(k0 + 1);
 
     const size_t plane = (size_t)lall * kall * gall;
+    for (int k = k0; k < k1; ++k) {
+        // TI/TJ of my points (FLX:146-158 then :171-183)
 #pragma unroll
-    for (int m = 0; m < 6; ++m) A.flx[m * plane + fb + g] = face[m];
+        for (int r = 0; r < NTP; ++r) {
+            if (tp[r] < 0) continue;
+            const int s = tid + r * CHUNK;
+            const int s10 = s + 1, s11 = s + iall + 1, s01 = s + iall;
 #pragma unroll
-    for (int d = 0; d < 3; ++d)
+            for (int f = 0; f < 4; ++f) {
+                const double *in = s_in + f * n_in;
+                double ti = in[s] * wt[r][0] + in[s10] * wt[r][1];
+                ti = ti + in[s11] * wt[r][2];
+                double tj = in[s] * wt[r][3];
+                tj = tj + in[s11] * wt[r][4] + in[s01] * wt[r][5];
+                s_t[f * n_t + s] = ti;
+                s_t[(4 + f) * n_t + s] = tj;
+            }
+            if (sgp && tp[r] == 0) {
+                // FLX:188-201: TI(gmin-1,gmin-1) := TJ(gmin,gmin-1), i.e. TJ of point 1
+                const double w1 = A.gt[(size_t)(2 * 2 + 1) * mstr + mb + 1];
+                const double w2 = A.gt[(size_t)(3 * 2 + 1) * mstr + mb + 1];
+                const double w3 = A.gt[(size_t)(4 * 2 + 1) * mstr + mb + 1];
 #pragma unroll
-        for (int a = 0; a < 3; ++a) A.xc[((size_t)d * 3 + a) * plane + fb + g] = xcv[a][d];
+                for (int f = 0; f < 4; ++f) {
+                    const double *in = s_in + f * n_in;
+                    double tj = in[s + 1] * w1;
+                    tj = tj + in[s + 1 + iall + 1] * w2 + in[s + 1 + iall] * w3;
+                    s_t[f * n_t + s] = tj;
+                }
+            }
+        }
+        __syncthreads();
+        // the input tile is free: put the next level in it and fetch the one after
+        if (k + 1 < k1) {
+            store_level();
+            if (k + 2 < k1) load_level(k + 2);
+        }
+
+        if (active) {
+            const int s = g - lo;
+            double face[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+            double xcv[3][3] = {{0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}};
+            // arc a between TI/TJ points (sa, ta) and (sb, tb); ta/tb = 0 for TI, 4 for TJ
+            auto arc = [&](int sa, int ta, int sb, int tb, const double (&h)[3], double &x2, double &y2, double &z2,
+                           double &rr2) {
+                rr2 = 1.0 / dmax(s_t[ta * n_t + sa] + s_t[tb * n_t + sb], eps);
+                x2 = s_t[(ta + 1) * n_t + sa] + s_t[(tb + 1) * n_t + sb];
+                y2 = s_t[(ta + 2) * n_t + sa] + s_t[(tb + 2) * n_t + sb];
+                z2 = s_t[(ta + 3) * n_t + sa] + s_t[(tb + 3) * n_t + sb];
+                return 0.5 * (x2 * h[0] + y2 * h[1] + z2 * h[2]);
+            };
+            double x2, y2, z2, rr2;
+            if (a_ai) {  // AI: TJ(ijm1) + TI(ij)   FLX:241-255
+                const double flux = arc(s - iall, 4, s, 0, hn[0], x2, y2, z2, rr2);
+                face[0] = flux * rarea * dt;
+                xcv[0][0] = xr[0][0] - x2 * rr2 * dt * 0.5;
+                xcv[0][1] = xr[0][1] - y2 * rr2 * dt * 0.5;
+                xcv[0][2] = xr[0][2] - z2 * rr2 * dt * 0.5;
+            }
+            if (a_aij) {  // AIJ: TI(ij) + TJ(ij)   FLX:266-280
+                const double flux = arc(s, 0, s, 4, hn[1], x2, y2, z2, rr2);
+                face[1] = flux * rarea * dt;
+                xcv[1][0] = xr[1][0] - x2 * rr2 * dt * 0.5;
+                xcv[1][1] = xr[1][1] - y2 * rr2 * dt * 0.5;
+                xcv[1][2] = xr[1][2] - z2 * rr2 * dt * 0.5;
+            }
+            if (a_aj) {  // AJ: TJ(ij) + TI(im1j)   FLX:292-306
+                const double flux = arc(s, 4, s - 1, 0, hn[2], x2, y2, z2, rr2);
+                face[2] = flux * rarea * dt;
+                xcv[2][0] = xr[2][0] - x2 * rr2 * dt * 0.5;
+                xcv[2][1] = xr[2][1] - y2 * rr2 * dt * 0.5;
+                xcv[2][2] = xr[2][2] - z2 * rr2 * dt * 0.5;
+            }
+            if (a_f4) face[3] = -arc(s - 1 - iall, 4, s - 1, 0, hn[3], x2, y2, z2, rr2) * rarea * dt;
+            if (a_f5) face[4] = -arc(s - 1 - iall, 0, s - 1 - iall, 4, hn[4], x2, y2, z2, rr2) * rarea * dt;
+            if (a_f6) face[5] = -arc(s - iall, 4, s - iall - 1, 0, hn[5], x2, y2, z2, rr2) * rarea * dt;
+            if (sgp && i == 1 && j == 1) face[5] = 0.0;  // FLX:311-320
+
+            const size_t fb = ((size_t)l * kall + k) * slab + g;
+#pragma unroll
+            for (int m = 0; m < 6; ++m) A.flx[m * plane + fb] = face[m];
+#pragma unroll
+            for (int d = 0; d < 3; ++d)
+#pragma unroll
+                for (int a = 0; a < 3; ++a) A.xc[(size_t)(d * 3 + a) * plane + fb] = xcv[a][d];
+        }
+        __syncthreads();
+    }
 }
 
 // Pole regions: one work-item per (k, l) walks the ring (FLX:326-374).
@@ -263,7 +290,7 @@ int launch_hflux(nicam_ctx c, double *flx_h, double *flx_h_pl, double *xc, doubl
                  double dt)
 {
     const Dims &D = c->d;
-    if (D.gall_1d > MAXI) return fail(NICAM_ERR_ARG, "gall_1d larger than the LDS tiles (160)");
+    if (D.gall_1d > MAXI) return fail(NICAM_ERR_ARG, "gall_1d larger than the supported 160");
     HfluxArgs A;
     A.flx = flx_h;
     A.xc = xc;
@@ -282,10 +309,20 @@ int launch_hflux(nicam_ctx c, double *flx_h, double *flx_h_pl, double *xc, doubl
     A.lall = D.lall;
     A.dt = dt;
     A.eps = c->eps;
-    const long nchunk = (D.gall + CHUNK - 1) / CHUNK;
-    const long nblk = nchunk * D.kall * D.lall;
+    A.nchunk = (D.gall + CHUNK - 1) / CHUNK;
+    // enough work-groups to fill 256 CUs several times over
+    const long base = (long)A.nchunk * D.lall;
+    int nkr = (int)((4096 + base - 1) / base);
+    nkr = std::max(1, std::min(nkr, D.kall));
+    if (c->tune.hflux_kper > 0) nkr = (D.kall + c->tune.hflux_kper - 1) / c->tune.hflux_kper;
+    A.kper = (D.kall + nkr - 1) / nkr;
+    A.nkr = (D.kall + A.kper - 1) / A.kper;
+    const long nblk = base * A.nkr;
     if (nblk > 0x7fffffffL) return fail(NICAM_ERR_ARG, "grid too large for one launch");
-    hipLaunchKernelGGL(hflux_kernel, dim3((unsigned)nblk), dim3(CHUNK), 0, c->stream, A);
+    const size_t lds = sizeof(double) * (4 * (CHUNK + 2 * (D.gall_1d + 1)) + 8 * (CHUNK + D.gall_1d + 1));
+    if (lds > 64 * 1024)
+        NICAM_HIP(hipFuncSetAttribute((const void *)hflux_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    hipLaunchKernelGGL(hflux_kernel, dim3((unsigned)nblk), dim3(CHUNK), lds, c->stream, A);
     NICAM_HIP(hipGetLastError());
     if (D.have_pl) {
         HfluxPoleArgs P;

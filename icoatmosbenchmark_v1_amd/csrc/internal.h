@@ -3,6 +3,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstddef>
 #include <string>
 #include <vector>
@@ -34,8 +35,17 @@ struct Dims {
 
 }  // namespace nicam
 
+// Launch-shape choices (0 = the library's default); set with nicam_ctx_set_tuning.
+struct nicam_tune {
+    int hflux_kper = 0;   // levels per work-group in horizontal_flux
+    int hlim_kper = 0;    // levels per work-group in horizontal_limiter_thuburn
+    int vlim_kchunk = 0;  // levels per work-item in vertical_limiter_thuburn
+    int vi_variant = 0;   // elimination-factor storage in vi_rhow_solver
+};
+
 struct nicam_ctx_s {
     nicam::Dims d{};
+    nicam_tune tune{};
     std::vector<int> have_sgp_h;
     int *have_sgp = nullptr;  // device [lall]
 
@@ -71,6 +81,10 @@ struct nicam_ctx_s {
     double *Ml = nullptr, *Ml_pl = nullptr;
     double *Mu = nullptr, *Mu_pl = nullptr;
     bool have_coef = false;
+
+    // vi_rhow_solver work array for the elimination factors (variant "global")
+    double *vi_gamma = nullptr;
+    size_t vi_gamma_n = 0;
 
     // staging buffers for NICAM_PTR_HOST, one per argument slot
     std::vector<void *> stage;

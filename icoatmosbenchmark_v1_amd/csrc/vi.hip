@@ -6,17 +6,22 @@
 // the boundary conditions, then runs a Thomas elimination per column, all with
 // g innermost.  Here one work-item owns one column (g, l): the source term is
 // formed level by level inside the forward sweep (the level k-1 products are
-// carried in registers, so no Sall array exists), the elimination factors
-// gamma(k) are kept in LDS (one slot per lane and level), and the forward
-// solution is written to rhogw and read back by the back substitution.  A
-// wavefront covers 64 consecutive g at one level, so all global traffic is
-// contiguous.  The pole columns are appended as the last work-items.
+// carried in registers, so no Sall array exists), and a wavefront covers 64
+// consecutive g at one level, so all global traffic is contiguous.  The pole
+// columns are appended as the last work-items.
+//
+// The back substitution needs the elimination factors gamma(k) and the
+// forward solution of every level.  Three storage variants are provided:
+//   1  gamma in LDS (one slot per lane and level), forward solution in rhogw
+//   2  gamma in a device work array laid out [level][column], forward
+//      solution in rhogw
+//   3  gamma and the forward solution in registers (fully unrolled levels;
+//      built for kall = 42 only: larger columns spill)
+// All three perform the same operations in the same order.
 #include "internal.h"
 
 namespace nicam {
 namespace {
-
-constexpr int BLOCK = 64;
 
 struct ViField {
     double *w;
@@ -28,14 +33,42 @@ struct ViField {
 struct ViArgs {
     ViField reg, pl;
     const double *rdgzh, *afact, *bfact;
+    double *gwork;  // variant 2: [kall][ncol]
+    long ncol;      // columns including the pole columns
     int gall, kall, lall, gall_pl, have_pl;
     double dt, CVovRt2, alpha, grav;
 };
 
-__device__ void vi_column(const ViField &F, size_t base, size_t s, int kall, const double *rdgzh,
-                          const double *afact, const double *bfact, double dt, double CVovRt2,
-                          double alpha, double grav, double *gam /* LDS, stride BLOCK */)
+// gamma(k) / forward-solution storage policies
+struct LdsGamma {
+    double *p;  // this lane's slot 0; stride = block size
+    int stride;
+    __device__ void set(int k, double v) { p[k * stride] = v; }
+    __device__ double get(int k) const { return p[k * stride]; }
+};
+struct GlobalGamma {
+    double *p;  // work array at this column; stride = ncol
+    long stride;
+    __device__ void set(int k, double v) { p[k * stride] = v; }
+    __device__ double get(int k) const { return p[k * stride]; }
+};
+template <int N>
+struct RegArray {
+    double v[N];
+    __device__ void set(int k, double x) { v[k] = x; }
+    __device__ double get(int k) const { return v[k]; }
+};
+
+// One column.  KC > 0 fixes kall at compile time (loops fully unrolled, W
+// may then hold the forward solution in registers); KC == 0 reads kall at run
+// time and keeps the forward solution in rhogw.
+template <int KC, bool WREG, class G, class WS>
+__device__ __forceinline__ void vi_column(const ViField &F, size_t base, size_t s, int kall_rt,
+                                          const double *__restrict__ rdgzh, const double *__restrict__ afact,
+                                          const double *__restrict__ bfact, double dt, double CVovRt2,
+                                          double alpha, double grav, G &gam, WS &wf)
 {
+    const int kall = KC > 0 ? KC : kall_rt;
     const int kmin = 1, kmax = kall - 2;  // 0-based
     auto at = [&](int k) { return base + (size_t)k * s; };
 
@@ -52,6 +85,7 @@ __device__ void vi_column(const ViField &F, size_t base, size_t s, int kall, con
         R_km1 = (F.r0[o] + dt * F.Sr[o]) * (rg * rg);
     }
     double beta = 0.0, w_km1 = 0.0, Mu_km1 = 0.0;
+#pragma unroll
     for (int k = kmin + 1; k <= kmax; ++k) {
         const size_t o = at(k);
         const double rgh = F.RGAMH[o], rg = F.RGAM[o];
@@ -69,11 +103,12 @@ __device__ void vi_column(const ViField &F, size_t base, size_t s, int kall, con
             w = S / beta;
         } else {  // mod_vi.f90:225-231
             const double gk = Mu_km1 / beta;
-            gam[k * BLOCK] = gk;
+            gam.set(k, gk);
             beta = Mc - Ml * gk;
             w = (S - Ml * w_km1) / beta;
         }
-        F.w[o] = w;  // forward value, replaced by the back substitution
+        if constexpr (WREG) wf.set(k, w);
+        else F.w[o] = w;  // forward value, replaced by the back substitution
         w_km1 = w;
         Mu_km1 = Mu;
         P_km1 = P_k;
@@ -81,8 +116,12 @@ __device__ void vi_column(const ViField &F, size_t base, size_t s, int kall, con
     }
     // mod_vi.f90:234-239
     double wn = w_km1;  // w(kmax), not yet scaled
+#pragma unroll
     for (int k = kmax - 1; k >= kmin + 1; --k) {
-        const double wk = F.w[at(k)] - gam[(k + 1) * BLOCK] * wn;
+        double wfk;
+        if constexpr (WREG) wfk = wf.get(k);
+        else wfk = F.w[at(k)];
+        const double wk = wfk - gam.get(k + 1) * wn;
         F.w[at(k + 1)] = wn * F.GSGAM2H[at(k + 1)];
         wn = wk;
     }
@@ -92,21 +131,72 @@ __device__ void vi_column(const ViField &F, size_t base, size_t s, int kall, con
     F.w[at(kmax + 1)] = w_kmax1 * F.GSGAM2H[at(kmax + 1)];
 }
 
-__global__ void __launch_bounds__(BLOCK) vi_kernel(ViArgs A)
+// work-item t -> (field set, column base, level stride); false if idle
+__device__ __forceinline__ bool vi_map(const ViArgs &A, long t, const ViField *&F, size_t &base, size_t &s)
 {
-    extern __shared__ double s_gam[];
-    double *gam = s_gam + threadIdx.x;
-    const long t = (long)blockIdx.x * BLOCK + threadIdx.x;
     const long nreg = (long)A.gall * A.lall;
     if (t < nreg) {
         const int g = (int)(t % A.gall), l = (int)(t / A.gall);
-        vi_column(A.reg, (size_t)l * A.kall * A.gall + g, A.gall, A.kall, A.rdgzh, A.afact, A.bfact, A.dt,
-                  A.CVovRt2, A.alpha, A.grav, gam);
-    } else if (A.have_pl && t < nreg + (long)A.gall_pl * kLallPl) {
+        F = &A.reg;
+        base = (size_t)l * A.kall * A.gall + g;
+        s = A.gall;
+        return true;
+    }
+    if (A.have_pl && t < nreg + (long)A.gall_pl * kLallPl) {
         const int p = (int)(t - nreg);
         const int g = p % A.gall_pl, l = p / A.gall_pl;
-        vi_column(A.pl, (size_t)l * A.kall * A.gall_pl + g, A.gall_pl, A.kall, A.rdgzh, A.afact, A.bfact,
-                  A.dt, A.CVovRt2, A.alpha, A.grav, gam);
+        F = &A.pl;
+        base = (size_t)l * A.kall * A.gall_pl + g;
+        s = A.gall_pl;
+        return true;
+    }
+    return false;
+}
+
+constexpr int BLOCK_LDS = 64;
+constexpr int BLOCK_GLB = 256;
+constexpr int BLOCK_REG = 64;
+
+__global__ void __launch_bounds__(BLOCK_LDS) vi_kernel_lds(ViArgs A)
+{
+    extern __shared__ double s_gam[];
+    const long t = (long)blockIdx.x * BLOCK_LDS + threadIdx.x;
+    const ViField *F;
+    size_t base, s;
+    if (!vi_map(A, t, F, base, s)) return;
+    LdsGamma gam{s_gam + threadIdx.x, BLOCK_LDS};
+    int unused = 0;
+    vi_column<0, false>(*F, base, s, A.kall, A.rdgzh, A.afact, A.bfact, A.dt, A.CVovRt2, A.alpha, A.grav, gam,
+                        unused);
+}
+
+__global__ void __launch_bounds__(BLOCK_GLB) vi_kernel_glb(ViArgs A)
+{
+    const long t = (long)blockIdx.x * BLOCK_GLB + threadIdx.x;
+    const ViField *F;
+    size_t base, s;
+    if (!vi_map(A, t, F, base, s)) return;
+    GlobalGamma gam{A.gwork + t, A.ncol};
+    int unused = 0;
+    vi_column<0, false>(*F, base, s, A.kall, A.rdgzh, A.afact, A.bfact, A.dt, A.CVovRt2, A.alpha, A.grav, gam,
+                        unused);
+}
+
+template <int KC, bool WREG>
+__global__ void __launch_bounds__(BLOCK_REG) vi_kernel_reg(ViArgs A)
+{
+    const long t = (long)blockIdx.x * BLOCK_REG + threadIdx.x;
+    const ViField *F;
+    size_t base, s;
+    if (!vi_map(A, t, F, base, s)) return;
+    RegArray<KC> gam;
+    if constexpr (WREG) {
+        RegArray<KC> wf;
+        vi_column<KC, true>(*F, base, s, KC, A.rdgzh, A.afact, A.bfact, A.dt, A.CVovRt2, A.alpha, A.grav, gam, wf);
+    } else {
+        int unused = 0;
+        vi_column<KC, false>(*F, base, s, KC, A.rdgzh, A.afact, A.bfact, A.dt, A.CVovRt2, A.alpha, A.grav, gam,
+                             unused);
     }
 }
 
@@ -118,8 +208,6 @@ int launch_vi(nicam_ctx c, double *w, double *w_pl, const double *w0, const doub
               const double *Sp, const double *Sp_pl, double dt)
 {
     const Dims &D = c->d;
-    const size_t lds = sizeof(double) * BLOCK * D.kall;
-    if (lds > 160 * 1024) return fail(NICAM_ERR_ARG, "kall too large for the LDS elimination factors");
     ViArgs A;
     A.reg = ViField{w, w0, p0, r0, Sr, Sw, Sp, c->Mc, c->Ml, c->Mu, c->RGAM, c->RGAMH, c->RGSGAM2,
                     c->RGSGAM2H, c->GSGAM2H};
@@ -137,11 +225,45 @@ int launch_vi(nicam_ctx c, double *w, double *w_pl, const double *w0, const doub
     A.CVovRt2 = c->cvdry / c->rdry / (dt * dt);  // mod_vi.f90:178
     A.alpha = (double)c->alpha;                   // mod_vi.f90:179
     A.grav = c->grav;
-    const long n = (long)D.gall * D.lall + (D.have_pl ? (long)D.gall_pl * kLallPl : 0);
-    const long nblk = (n + BLOCK - 1) / BLOCK;
-    if (lds > 64 * 1024) NICAM_HIP(hipFuncSetAttribute((const void *)vi_kernel,
-                                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    hipLaunchKernelGGL(vi_kernel, dim3((unsigned)nblk), dim3(BLOCK), lds, c->stream, A);
+    A.ncol = (long)D.gall * D.lall + (D.have_pl ? (long)D.gall_pl * kLallPl : 0);
+    A.gwork = nullptr;
+
+    int variant = c->tune.vi_variant;
+    const bool reg_ok = (D.kall == 42);
+    if (variant == 3 && !reg_ok) variant = 0;
+    if (variant == 0) variant = 2;
+
+    if (variant == 1) {
+        const size_t lds = sizeof(double) * BLOCK_LDS * D.kall;
+        if (lds > 160 * 1024) return fail(NICAM_ERR_ARG, "kall too large for the LDS elimination factors");
+        if (lds > 64 * 1024)
+            NICAM_HIP(hipFuncSetAttribute((const void *)vi_kernel_lds, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                          (int)lds));
+        hipLaunchKernelGGL(vi_kernel_lds, dim3((unsigned)((A.ncol + BLOCK_LDS - 1) / BLOCK_LDS)), dim3(BLOCK_LDS), lds,
+                           c->stream, A);
+    } else if (variant == 2) {
+        const size_t need = (size_t)A.ncol * D.kall;
+        if (c->vi_gamma_n < need) {
+            if (c->vi_gamma) {
+                NICAM_HIP(hipStreamSynchronize(c->stream));
+                (void)hipFree(c->vi_gamma);
+                c->device_bytes -= c->vi_gamma_n * sizeof(double);
+                c->vi_gamma = nullptr;
+                c->vi_gamma_n = 0;
+            }
+            NICAM_HIP(hipMalloc(&c->vi_gamma, need * sizeof(double)));
+            c->vi_gamma_n = need;
+            c->device_bytes += need * sizeof(double);
+        }
+        A.gwork = c->vi_gamma;
+        hipLaunchKernelGGL(vi_kernel_glb, dim3((unsigned)((A.ncol + BLOCK_GLB - 1) / BLOCK_GLB)), dim3(BLOCK_GLB), 0,
+                           c->stream, A);
+    } else {
+        const dim3 grid((unsigned)((A.ncol + BLOCK_REG - 1) / BLOCK_REG));
+        // kall = 42: gamma and the forward solution fit in registers (252
+        // VGPRs); at kall = 98 even gamma alone spills, so that size is not built
+        hipLaunchKernelGGL((vi_kernel_reg<42, true>), grid, dim3(BLOCK_REG), 0, c->stream, A);
+    }
     NICAM_HIP(hipGetLastError());
     return NICAM_OK;
 }

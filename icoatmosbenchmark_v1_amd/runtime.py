@@ -38,6 +38,10 @@ NICAM_ERR_IO = 4
 NICAM_ERR_NOMEM = 5
 PTR_DEVICE = 0
 PTR_HOST = 1
+TUNE_HFLUX_KPER = 1
+TUNE_HLIM_KPER = 2
+TUNE_VLIM_KCHUNK = 3
+TUNE_VI_VARIANT = 4
 
 STATUS_NAMES = {
     NICAM_ERR_ARG: "NICAM_ERR_ARG",
@@ -56,6 +60,7 @@ SIGNATURES = {
     "nicam_ctx_destroy": (c_int, [_P]),
     "nicam_ctx_set_stream": (c_int, [_P, _P]),
     "nicam_ctx_set_pointer_mode": (c_int, [_P, c_int]),
+    "nicam_ctx_set_tuning": (c_int, [_P, c_int, c_int]),
     "nicam_ctx_device_bytes": (c_size_t, [_P]),
     "nicam_const_setup": (c_int, [_P]),
     "nicam_set_const": (c_int, [_P, c_double, c_double]),
@@ -206,6 +211,10 @@ class Context:
     def set_pointer_mode(self, mode: int) -> None:
         _check(self.lib.nicam_ctx_set_pointer_mode(self.h, mode), "nicam_ctx_set_pointer_mode")
         self.mode = mode
+
+    def set_tuning(self, param: int, value: int) -> None:
+        """Launch-shape choice (TUNE_* constants); results do not depend on it."""
+        _check(self.lib.nicam_ctx_set_tuning(self.h, int(param), int(value)), "nicam_ctx_set_tuning")
 
     def set_stream(self, stream) -> None:
         """stream: a torch.cuda.Stream, a raw hipStream_t integer, or None."""

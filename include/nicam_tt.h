@@ -72,6 +72,23 @@ int nicam_ctx_destroy(nicam_ctx ctx);
 /* Stream for all work of this context (a hipStream_t; NULL = null stream). */
 int nicam_ctx_set_stream(nicam_ctx ctx, void *hip_stream);
 int nicam_ctx_set_pointer_mode(nicam_ctx ctx, int mode);
+/* Launch-shape parameters (all default to 0 = library's choice):
+ *   NICAM_TUNE_HFLUX_KPER   levels per work-group of horizontal_flux
+ *   NICAM_TUNE_HLIM_KPER    levels per work-group of the horizontal limiter
+ *   NICAM_TUNE_VLIM_KCHUNK  levels per work-item of the vertical limiter
+ *   NICAM_TUNE_VI_VARIANT   storage of the Thomas elimination factors:
+ *                           1 = LDS, 2 = device work array, 3 = registers
+ *                           (kall = 42 only, else automatic), 0 = automatic
+ * Results do not depend on these settings (every variant computes the same
+ * floating-point operations in the same order). */
+enum nicam_tuning {
+    NICAM_TUNE_HFLUX_KPER = 1,
+    NICAM_TUNE_HLIM_KPER = 2,
+    NICAM_TUNE_VLIM_KCHUNK = 3,
+    NICAM_TUNE_VI_VARIANT = 4
+};
+int nicam_ctx_set_tuning(nicam_ctx ctx, int param, int value);
+
 /* Device memory currently held by the context, in bytes. */
 size_t nicam_ctx_device_bytes(nicam_ctx ctx);
 

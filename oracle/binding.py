@@ -213,8 +213,26 @@ class RefLibs:
         return e.value, b.value
 
     def grd_setup(self, fname: str, kall: int):
+        # The reference reads the file big-endian (sysdep/Makedef.*: -fconvert=big-endian).
+        # flang applies -fconvert only in a Fortran main program, which this harness
+        # library has none of, so the reference is handed a native-order copy.
+        import tempfile
+
+        raw = Path(fname).read_bytes()
+        recs, pos = [], 0
+        while pos < len(raw):
+            n = int.from_bytes(raw[pos:pos + 4], "big")
+            recs.append(raw[pos + 4:pos + 4 + n])
+            pos += 8 + n
+        le = bytearray()
+        for i, r in enumerate(recs):
+            body = np.frombuffer(r, dtype=">i4" if i == 0 else ">f8").astype("<i4" if i == 0 else "<f8").tobytes()
+            le += len(body).to_bytes(4, "little") + body + len(body).to_bytes(4, "little")
+        tmp = tempfile.NamedTemporaryFile(suffix=".vgrid", delete=False)
+        tmp.write(bytes(le))
+        tmp.close()
         outs = [np.zeros(kall) for _ in range(10)]
-        b = str(fname).encode()
+        b = tmp.name.encode()
         self.vi.ref_grd_setup(kall, ctypes.create_string_buffer(b, len(b)), len(b), *[o.ctypes.data for o in outs])
         return dict(zip(["gz", "gzh", "dgz", "dgzh", "rdgz", "rdgzh", "afact", "bfact", "cfact", "dfact"], outs))
 

@@ -16,7 +16,8 @@ def assert_parity(got: dict, want: dict, rtol: float = RTOL_FP64, exact: bool = 
             continue
         a, b = np.asarray(got[k]), np.asarray(want[k])
         assert a.shape == b.shape, f"{k}: shape {a.shape} != {b.shape}"
-        bad = ~(np.abs(a - b) <= rtol * np.abs(b))
+        with np.errstate(invalid="ignore", over="ignore"):  # equal infinities are equal
+            bad = ~(np.abs(a - b) <= rtol * np.abs(b))
         bad &= ~((a == b) | (np.isnan(a) & np.isnan(b)))
         if bad.any():
             idx = np.argwhere(bad)[:5]
