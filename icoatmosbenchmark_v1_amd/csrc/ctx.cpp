@@ -202,7 +202,6 @@ int nicam_ctx_set_tuning(nicam_ctx c, int param, int value)
     if (value < 0) return fail(NICAM_ERR_ARG, "tuning values must be >= 0");
     switch (param) {
     case NICAM_TUNE_HFLUX_KPER: c->tune.hflux_kper = value; break;
-    case NICAM_TUNE_HLIM_KPER: c->tune.hlim_kper = value; break;
     case NICAM_TUNE_VLIM_KCHUNK: c->tune.vlim_kchunk = value; break;
     case NICAM_TUNE_VI_VARIANT:
         if (value > 3) return fail(NICAM_ERR_ARG, "NICAM_TUNE_VI_VARIANT must be 0..3");

@@ -38,7 +38,6 @@ struct Dims {
 // Launch-shape choices (0 = the library's default); set with nicam_ctx_set_tuning.
 struct nicam_tune {
     int hflux_kper = 0;   // levels per work-group in horizontal_flux
-    int hlim_kper = 0;    // levels per work-group in horizontal_limiter_thuburn
     int vlim_kchunk = 0;  // levels per work-item in vertical_limiter_thuburn
     int vi_variant = 0;   // elimination-factor storage in vi_rhow_solver
 };

@@ -84,46 +84,81 @@ __device__ __forceinline__ void vi_column(const ViField &F, size_t base, size_t 
         P_km1 = (F.p0[o] + dt * F.Sp[o]) * F.RGSGAM2[o];
         R_km1 = (F.r0[o] + dt * F.Sr[o]) * (rg * rg);
     }
+    // the inputs of one level; the next level's are loaded before this
+    // level's stores are issued, so the loads overlap the elimination
+    struct Lvl {
+        double rgh, rg, p0, Sp, rgs2, r0, Sr, w0, Sw, Mc, Ml, Mu;
+    };
+    auto load = [&](int k) {
+        const size_t o = at(k);
+        Lvl v;
+        v.rgh = F.RGAMH[o];
+        v.rg = F.RGAM[o];
+        v.p0 = F.p0[o];
+        v.Sp = F.Sp[o];
+        v.rgs2 = F.RGSGAM2[o];
+        v.r0 = F.r0[o];
+        v.Sr = F.Sr[o];
+        v.w0 = F.w0[o];
+        v.Sw = F.Sw[o];
+        v.Mc = F.Mc[o];
+        v.Ml = F.Ml[o];
+        v.Mu = F.Mu[o];
+        return v;
+    };
     double beta = 0.0, w_km1 = 0.0, Mu_km1 = 0.0;
-#pragma unroll
+    constexpr int UNROLL = KC > 0 ? KC : 1;  // full unroll only at compile-time kall
+    Lvl cur = load(kmin + 1);
+#pragma unroll UNROLL
     for (int k = kmin + 1; k <= kmax; ++k) {
         const size_t o = at(k);
-        const double rgh = F.RGAMH[o], rg = F.RGAM[o];
-        const double P_k = (F.p0[o] + dt * F.Sp[o]) * F.RGSGAM2[o];
-        const double R_k = (F.r0[o] + dt * F.Sr[o]) * (rg * rg);
-        double S = ((F.w0[o] * alpha + dt * F.Sw[o]) * (rgh * rgh) - (P_k - P_km1) * dt * rdgzh[k] -
+        Lvl nxt = cur;
+        if (k < kmax) nxt = load(k + 1);
+        const double P_k = (cur.p0 + dt * cur.Sp) * cur.rgs2;
+        const double R_k = (cur.r0 + dt * cur.Sr) * (cur.rg * cur.rg);
+        double S = ((cur.w0 * alpha + dt * cur.Sw) * (cur.rgh * cur.rgh) - (P_k - P_km1) * dt * rdgzh[k] -
                     (R_k * afact[k] + R_km1 * bfact[k]) * dt * grav) *
                    CVovRt2;
-        const double Mc = F.Mc[o], Ml = F.Ml[o], Mu = F.Mu[o];
-        if (k == kmin + 1) S = S - Ml * w_kmin;  // mod_vi.f90:211
-        if (k == kmax) S = S - Mu * w_kmax1;     // mod_vi.f90:212
+        if (k == kmin + 1) S = S - cur.Ml * w_kmin;  // mod_vi.f90:211
+        if (k == kmax) S = S - cur.Mu * w_kmax1;     // mod_vi.f90:212
         double w;
         if (k == kmin + 1) {  // mod_vi.f90:218-222
-            beta = Mc;
+            beta = cur.Mc;
             w = S / beta;
         } else {  // mod_vi.f90:225-231
             const double gk = Mu_km1 / beta;
             gam.set(k, gk);
-            beta = Mc - Ml * gk;
-            w = (S - Ml * w_km1) / beta;
+            beta = cur.Mc - cur.Ml * gk;
+            w = (S - cur.Ml * w_km1) / beta;
         }
         if constexpr (WREG) wf.set(k, w);
         else F.w[o] = w;  // forward value, replaced by the back substitution
         w_km1 = w;
-        Mu_km1 = Mu;
+        Mu_km1 = cur.Mu;
         P_km1 = P_k;
         R_km1 = R_k;
+        cur = nxt;
     }
     // mod_vi.f90:234-239
     double wn = w_km1;  // w(kmax), not yet scaled
-#pragma unroll
+    auto load_b = [&](int k, double &wf_k, double &g_k1, double &gs_k1) {
+        if constexpr (WREG) wf_k = wf.get(k);
+        else wf_k = F.w[at(k)];
+        g_k1 = gam.get(k + 1);
+        gs_k1 = F.GSGAM2H[at(k + 1)];
+    };
+    double wf_c = 0.0, g_c = 0.0, gs_c = 0.0;
+    if (kmax - 1 >= kmin + 1) load_b(kmax - 1, wf_c, g_c, gs_c);
+#pragma unroll UNROLL
     for (int k = kmax - 1; k >= kmin + 1; --k) {
-        double wfk;
-        if constexpr (WREG) wfk = wf.get(k);
-        else wfk = F.w[at(k)];
-        const double wk = wfk - gam.get(k + 1) * wn;
-        F.w[at(k + 1)] = wn * F.GSGAM2H[at(k + 1)];
+        double wf_n = 0.0, g_n = 0.0, gs_n = 0.0;
+        if (k - 1 >= kmin + 1) load_b(k - 1, wf_n, g_n, gs_n);
+        const double wk = wf_c - g_c * wn;
+        F.w[at(k + 1)] = wn * gs_c;
         wn = wk;
+        wf_c = wf_n;
+        g_c = g_n;
+        gs_c = gs_n;
     }
     // mod_vi.f90:242-246
     F.w[at(kmin)] = w_kmin * F.GSGAM2H[at(kmin)];

@@ -39,7 +39,6 @@ NICAM_ERR_NOMEM = 5
 PTR_DEVICE = 0
 PTR_HOST = 1
 TUNE_HFLUX_KPER = 1
-TUNE_HLIM_KPER = 2
 TUNE_VLIM_KCHUNK = 3
 TUNE_VI_VARIANT = 4
 

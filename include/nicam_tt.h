@@ -74,7 +74,6 @@ int nicam_ctx_set_stream(nicam_ctx ctx, void *hip_stream);
 int nicam_ctx_set_pointer_mode(nicam_ctx ctx, int mode);
 /* Launch-shape parameters (all default to 0 = library's choice):
  *   NICAM_TUNE_HFLUX_KPER   levels per work-group of horizontal_flux
- *   NICAM_TUNE_HLIM_KPER    levels per work-group of the horizontal limiter
  *   NICAM_TUNE_VLIM_KCHUNK  levels per work-item of the vertical limiter
  *   NICAM_TUNE_VI_VARIANT   storage of the Thomas elimination factors:
  *                           1 = LDS, 2 = device work array, 3 = registers
@@ -83,7 +82,6 @@ int nicam_ctx_set_pointer_mode(nicam_ctx ctx, int mode);
  * floating-point operations in the same order). */
 enum nicam_tuning {
     NICAM_TUNE_HFLUX_KPER = 1,
-    NICAM_TUNE_HLIM_KPER = 2,
     NICAM_TUNE_VLIM_KCHUNK = 3,
     NICAM_TUNE_VI_VARIANT = 4
 };
