@@ -16,7 +16,7 @@
 //   2  gamma in a device work array laid out [level][column], forward
 //      solution in rhogw
 //   3  gamma and the forward solution in registers (fully unrolled levels;
-//      built for kall = 42 only: larger columns spill)
+//      built for kall = 42 and 98)
 // All three perform the same operations in the same order.
 #include "internal.h"
 
@@ -138,6 +138,9 @@ __device__ __forceinline__ void vi_column(const ViField &F, size_t base, size_t 
         P_km1 = P_k;
         R_km1 = R_k;
         cur = nxt;
+        // with deep columns in registers, keep the scheduler from hoisting
+        // later levels' loads (it would run out of registers)
+        if constexpr (KC > 64) __builtin_amdgcn_sched_barrier(0);
     }
     // mod_vi.f90:234-239
     double wn = w_km1;  // w(kmax), not yet scaled
@@ -159,6 +162,7 @@ __device__ __forceinline__ void vi_column(const ViField &F, size_t base, size_t 
         wf_c = wf_n;
         g_c = g_n;
         gs_c = gs_n;
+        if constexpr (KC > 64) __builtin_amdgcn_sched_barrier(0);
     }
     // mod_vi.f90:242-246
     F.w[at(kmin)] = w_kmin * F.GSGAM2H[at(kmin)];
@@ -264,9 +268,10 @@ int launch_vi(nicam_ctx c, double *w, double *w_pl, const double *w0, const doub
     A.gwork = nullptr;
 
     int variant = c->tune.vi_variant;
-    const bool reg_ok = (D.kall == 42);
+    const bool reg_ok = (D.kall == 42 || D.kall == 98);
     if (variant == 3 && !reg_ok) variant = 0;
-    if (variant == 0) variant = 2;
+    // measured at gl7 (40 x 66x66 x 98): registers 407 us, work array 524 us, LDS 606 us
+    if (variant == 0) variant = reg_ok ? 3 : 2;
 
     if (variant == 1) {
         const size_t lds = sizeof(double) * BLOCK_LDS * D.kall;
@@ -295,9 +300,11 @@ int launch_vi(nicam_ctx c, double *w, double *w_pl, const double *w0, const doub
                            c->stream, A);
     } else {
         const dim3 grid((unsigned)((A.ncol + BLOCK_REG - 1) / BLOCK_REG));
-        // kall = 42: gamma and the forward solution fit in registers (252
-        // VGPRs); at kall = 98 even gamma alone spills, so that size is not built
-        hipLaunchKernelGGL((vi_kernel_reg<42, true>), grid, dim3(BLOCK_REG), 0, c->stream, A);
+        // gamma and the forward solution of a whole column in registers: 252
+        // VGPRs at kall = 42, 256 VGPRs + 256 AGPRs (one wavefront per SIMD,
+        // no spills) at kall = 98
+        if (D.kall == 42) hipLaunchKernelGGL((vi_kernel_reg<42, true>), grid, dim3(BLOCK_REG), 0, c->stream, A);
+        else hipLaunchKernelGGL((vi_kernel_reg<98, true>), grid, dim3(BLOCK_REG), 0, c->stream, A);
     }
     NICAM_HIP(hipGetLastError());
     return NICAM_OK;

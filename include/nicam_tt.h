@@ -77,7 +77,7 @@ int nicam_ctx_set_pointer_mode(nicam_ctx ctx, int mode);
  *   NICAM_TUNE_VLIM_KCHUNK  levels per work-item of the vertical limiter
  *   NICAM_TUNE_VI_VARIANT   storage of the Thomas elimination factors:
  *                           1 = LDS, 2 = device work array, 3 = registers
- *                           (kall = 42 only, else automatic), 0 = automatic
+ *                           (kall = 42 or 98, else automatic), 0 = automatic
  * Results do not depend on these settings (every variant computes the same
  * floating-point operations in the same order). */
 enum nicam_tuning {
