@@ -20,7 +20,7 @@ LIB_NAME = "libnicam_tt.so"
 LIB_PATH = PKG_DIR / LIB_NAME
 OBJ_DIR = PKG_DIR / "_build"
 
-SOURCES = ["ctx.cpp", "hflux.hip", "hlim.hip", "vlim.hip", "vi.hip"]
+SOURCES = ["ctx.cpp", "comm.cpp", "hflux.hip", "hlim.hip", "vlim.hip", "vi.hip", "comm.hip"]
 
 HIPCC_FLAGS = [
     "-x", "hip",
@@ -49,7 +49,7 @@ def build_library(verbose: bool = False, jobs: int = 4) -> Path:
     objs = []
     procs = []
     for src in SOURCES:
-        obj = OBJ_DIR / (Path(src).stem + ".o")
+        obj = OBJ_DIR / (src.replace(".", "_") + ".o")  # comm.cpp and comm.hip both exist
         cmd = [hipcc, *HIPCC_FLAGS, f"-I{INCLUDE}", f"-I{CSRC}", "-c", str(CSRC / src), "-o", str(obj)]
         if verbose:
             print(" ".join(cmd), flush=True)

@@ -171,11 +171,14 @@ int nicam_ctx_destroy(nicam_ctx c)
                       &c->GMTR_t_pl, &c->GMTR_a, &c->GMTR_a_pl, &c->RGAM, &c->RGAM_pl,
                       &c->RGAMH, &c->RGAMH_pl, &c->RGSGAM2, &c->RGSGAM2_pl, &c->RGSGAM2H,
                       &c->RGSGAM2H_pl, &c->GSGAM2H, &c->GSGAM2H_pl, &c->rdgzh, &c->afact,
-                      &c->bfact, &c->Mc, &c->Mc_pl, &c->Ml, &c->Ml_pl, &c->Mu, &c->Mu_pl, &c->vi_gamma};
+                      &c->bfact, &c->Mc, &c->Mc_pl, &c->Ml, &c->Ml_pl, &c->Mu, &c->Mu_pl, &c->vi_gamma,
+                      &c->qpost, &c->qpost_pl};
     for (double **p : all) dev_free(*p);
     for (void *p : c->stage)
         if (p) (void)hipFree(p);
     if (c->have_sgp) (void)hipFree(c->have_sgp);
+    if (c->halo_dst) (void)hipFree(c->halo_dst);
+    if (c->halo_src) (void)hipFree(c->halo_src);
     delete c;
     return NICAM_OK;
 }
@@ -424,6 +427,40 @@ int nicam_ctx_synchronize(nicam_ctx c)
     return NICAM_OK;
 }
 
+int nicam_comm_setup(nicam_ctx c, int rlevel)
+{
+    NICAM_TRY(check_ctx(c));
+    return comm_setup(c, rlevel);
+}
+
+// COMM_data_transfer / COMM_var on var(gall,kmax,lall,vmax), var_pl(gall_pl,kmax,2,vmax)
+static int halo_entry(nicam_ctx c, double *var, double *var_pl, int kmax, int vmax, bool pole_fill)
+{
+    NICAM_TRY(check_ctx(c));
+    if (kmax < 1 || vmax < 1) return fail(NICAM_ERR_ARG, "kmax and vmax must be positive");
+    NICAM_TRY(need(var, "var"));
+    NICAM_TRY(need(var_pl, "var_pl"));
+    if (!c->halo_n) return fail(NICAM_ERR_STATE, "halo map not set (nicam_comm_setup)");
+    if (c->ptr_mode == NICAM_PTR_DEVICE) return launch_halo_fill(c, var, var_pl, kmax, vmax, pole_fill);
+    const size_t n = (size_t)kmax * vmax * 8;
+    const size_t R = (size_t)c->d.gall * c->d.lall * n, P = (size_t)c->d.gall_pl * kLallPl * n;
+    std::vector<HostArg> a = {{var, R, true}, {var_pl, P, true}};
+    std::vector<void *> v;
+    NICAM_TRY(stage_in(c, a, v));
+    NICAM_TRY(launch_halo_fill(c, static_cast<double *>(v[0]), static_cast<double *>(v[1]), kmax, vmax, pole_fill));
+    return stage_out(c, a, v);
+}
+
+int nicam_comm_data_transfer(nicam_ctx c, double *var, double *var_pl, int kmax, int vmax)
+{
+    return halo_entry(c, var, var_pl, kmax, vmax, false);
+}
+
+int nicam_comm_var(nicam_ctx c, double *var, double *var_pl, int kmax, int vmax)
+{
+    return halo_entry(c, var, var_pl, kmax, vmax, true);
+}
+
 // ---- hot path entry points ---------------------------------------------------
 
 #define PL(p) (c->d.have_pl ? (p) : nullptr)
@@ -464,13 +501,17 @@ int nicam_horizontal_limiter_thuburn(nicam_ctx c, double *q_a, double *q_a_pl, c
                                      const double *Qpost, const double *Qpost_pl)
 {
     NICAM_TRY(check_ctx(c));
-    const void *req[] = {q_a, q, d, ch, cmask, Qprev, Qpost};
-    const char *nm[] = {"q_a", "q", "d", "ch", "cmask", "Qout_prev", "Qout_post"};
-    for (int i = 0; i < 7; ++i) NICAM_TRY(need(req[i], nm[i]));
+    // Qout_post NULL: the library exchanges Qout_prev itself (nicam_comm_setup
+    // must have run); Qout_post_pl is then ignored.
+    const void *req[] = {q_a, q, d, ch, cmask, Qprev};
+    const char *nm[] = {"q_a", "q", "d", "ch", "cmask", "Qout_prev"};
+    for (int i = 0; i < 6; ++i) NICAM_TRY(need(req[i], nm[i]));
     if (c->d.have_pl) {
-        const void *rp[] = {q_a_pl, q_pl, d_pl, ch_pl, cmask_pl, Qprev_pl, Qpost_pl};
-        for (int i = 0; i < 7; ++i) NICAM_TRY(need(rp[i], "pole array"));
+        const void *rp[] = {q_a_pl, q_pl, d_pl, ch_pl, cmask_pl, Qprev_pl};
+        for (int i = 0; i < 6; ++i) NICAM_TRY(need(rp[i], "pole array"));
+        if (Qpost) NICAM_TRY(need(Qpost_pl, "Qout_post_pl"));
     }
+    if (!Qpost) Qpost_pl = nullptr;
     if (c->ptr_mode == NICAM_PTR_DEVICE)
         return launch_hlim(c, q_a, PL(q_a_pl), q, PL(q_pl), d, PL(d_pl), ch, PL(ch_pl), cmask,
                            PL(cmask_pl), Qprev, Qprev_pl, Qpost, PL(Qpost_pl));

@@ -34,6 +34,7 @@ struct HlimArgs {
     const double *Qpost;  // (gall,kall,lall,2)
     const int *have_sgp;
     int iall, gall, kall, lall;
+    int phases;  // 1: allowable range only, 2: clamp only, 3: both
     double eps, big;
 };
 
@@ -111,7 +112,7 @@ __global__ void __launch_bounds__(CHUNK) hlim_kernel(HlimArgs A)
     // ---- phase 1: allowable range of the cell (HLIM:216-301) ----
     double qout_min, qout_max;
     const double qc = A.q[fb + g];
-    if (i >= 1 && i <= iall - 2 && j >= 1 && j <= iall - 2) {
+    if ((A.phases & 1) && i >= 1 && i <= iall - 2 && j >= 1 && j <= iall - 2) {
         double qinmn[6], qinmx[6];
 #pragma unroll
         for (int m = 0; m < 3; ++m) {  // faces 1..3: own side of own arcs (:162-187)
@@ -157,11 +158,13 @@ __global__ void __launch_bounds__(CHUNK) hlim_kernel(HlimArgs A)
         qout_min = qc;
         qout_max = qc;
     }
-    A.Qprev[fb + g] = qout_min;
-    A.Qprev[plane + fb + g] = qout_max;
+    if (A.phases & 1) {
+        A.Qprev[fb + g] = qout_min;
+        A.Qprev[plane + fb + g] = qout_max;
+    }
 
     // ---- phase 2: clamp the face values of the own arcs (HLIM:377-410) ----
-    if (own) {
+    if (own && (A.phases & 2)) {
         const int nb[3] = {g + 1, g + iall + 1, g + iall};
         const double qo_min_c = A.Qpost[fb + g], qo_max_c = A.Qpost[plane + fb + g];
 #pragma unroll
@@ -188,6 +191,7 @@ struct HlimPoleArgs {
     double *Qprev;
     const double *Qpost;
     int gall_pl, kall, have_pl;
+    int phases;
     double eps, big;
 };
 
@@ -200,6 +204,7 @@ __global__ void hlim_pole_kernel(HlimPoleArgs A)
     const size_t fb = ((size_t)l * kall + k) * npl;
     const size_t plane = (size_t)kLallPl * kall * npl;
     if (!A.have_pl) {  // only the ring zeroing of :367-370 happens
+        if (!(A.phases & 1)) return;
         for (int v = 1; v < npl; ++v) {
             A.Qprev[fb + v] = 0.0;
             A.Qprev[plane + fb + v] = 0.0;
@@ -239,12 +244,15 @@ __global__ void hlim_pole_kernel(HlimPoleArgs A)
     const double dn = A.d[fb + 0];
     const double qo_min = (qn - CQmax - qnext_max * (1.0 - Cin - Cout + dn)) / (Cout + zerosw) * (1.0 - zerosw) + qn * zerosw;
     const double qo_max = (qn - CQmin - qnext_min * (1.0 - Cin - Cout + dn)) / (Cout + zerosw) * (1.0 - zerosw) + qn * zerosw;
-    A.Qprev[fb + 0] = qo_min;
-    A.Qprev[plane + fb + 0] = qo_max;
-    for (int v = 1; v < npl; ++v) {
-        A.Qprev[fb + v] = 0.0;
-        A.Qprev[plane + fb + v] = 0.0;
+    if (A.phases & 1) {
+        A.Qprev[fb + 0] = qo_min;
+        A.Qprev[plane + fb + 0] = qo_max;
+        for (int v = 1; v < npl; ++v) {
+            A.Qprev[fb + v] = 0.0;
+            A.Qprev[plane + fb + v] = 0.0;
+        }
     }
+    if (!(A.phases & 2)) return;
     const double pn_min = A.Qpost[fb + 0], pn_max = A.Qpost[plane + fb + 0];
     for (int v = 1; v < npl; ++v) {
         const double c = A.cmask[fb + v];
@@ -281,27 +289,56 @@ int launch_hlim(nicam_ctx c, double *q_a, double *q_a_pl, const double *q, const
     A.big = c->big;
     const long nblk = (long)((D.gall + CHUNK - 1) / CHUNK) * D.kall * D.lall;
     if (nblk > 0x7fffffffL) return fail(NICAM_ERR_ARG, "grid too large for one launch");
-    hipLaunchKernelGGL(hlim_kernel, dim3((unsigned)nblk), dim3(CHUNK), 0, c->stream, A);
-    NICAM_HIP(hipGetLastError());
-    if (Qprev_pl) {
-        HlimPoleArgs P;
-        P.q_a = q_a_pl;
-        P.q = q_pl;
-        P.d = d_pl;
-        P.ch = ch_pl;
-        P.cmask = cmask_pl;
-        P.Qprev = Qprev_pl;
-        P.Qpost = Qpost_pl;
-        P.gall_pl = D.gall_pl;
-        P.kall = D.kall;
-        P.have_pl = D.have_pl;
-        P.eps = c->eps;
-        P.big = c->big;
-        const int n = D.kall * kLallPl;
-        hipLaunchKernelGGL(hlim_pole_kernel, dim3((n + 63) / 64), dim3(64), 0, c->stream, P);
+    HlimPoleArgs P;
+    P.q_a = q_a_pl;
+    P.q = q_pl;
+    P.d = d_pl;
+    P.ch = ch_pl;
+    P.cmask = cmask_pl;
+    P.Qprev = Qprev_pl;
+    P.Qpost = Qpost_pl;
+    P.gall_pl = D.gall_pl;
+    P.kall = D.kall;
+    P.have_pl = D.have_pl;
+    P.eps = c->eps;
+    P.big = c->big;
+    const int npole = D.kall * kLallPl;
+    auto run = [&](int phases) -> int {
+        A.phases = phases;
+        P.phases = phases;
+        hipLaunchKernelGGL(hlim_kernel, dim3((unsigned)nblk), dim3(CHUNK), 0, c->stream, A);
         NICAM_HIP(hipGetLastError());
+        if (P.Qprev) {
+            hipLaunchKernelGGL(hlim_pole_kernel, dim3((npole + 63) / 64), dim3(64), 0, c->stream, P);
+            NICAM_HIP(hipGetLastError());
+        }
+        return NICAM_OK;
+    };
+    if (Qpost) return run(3);  // the reference's kernel: Qout_post supplied
+
+    // No Qout_post: do the halo exchange the reference cut out (HLIM:365-375,
+    // the commented COMM_data_transfer of Qout) on the device.  Qout_prev gets
+    // the pre-exchange range; the clamp uses a context-owned exchanged copy.
+    if (!c->halo_n) return fail(NICAM_ERR_STATE, "Qout_post is NULL but no halo map is set (nicam_comm_setup)");
+    if (!Qprev_pl) return fail(NICAM_ERR_ARG, "Qout_prev_pl is needed to exchange Qout with the poles");
+    const size_t nq = 2 * D.nreg(), nqp = 2 * D.npl();
+    if (!c->qpost) {
+        NICAM_HIP(hipMalloc(&c->qpost, nq * sizeof(double)));
+        c->device_bytes += nq * sizeof(double);
     }
-    return NICAM_OK;
+    if (!c->qpost_pl) {
+        NICAM_HIP(hipMalloc(&c->qpost_pl, nqp * sizeof(double)));
+        c->device_bytes += nqp * sizeof(double);
+    }
+    int rc = run(1);
+    if (rc != NICAM_OK) return rc;
+    NICAM_HIP(hipMemcpyAsync(c->qpost, Qprev, nq * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+    NICAM_HIP(hipMemcpyAsync(c->qpost_pl, Qprev_pl, nqp * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+    rc = launch_halo_fill(c, c->qpost, c->qpost_pl, D.kall, 2, false);
+    if (rc != NICAM_OK) return rc;
+    A.Qpost = c->qpost;
+    P.Qpost = c->qpost_pl;
+    return run(2);
 }
 
 }  // namespace nicam

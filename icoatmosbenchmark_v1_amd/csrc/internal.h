@@ -81,6 +81,14 @@ struct nicam_ctx_s {
     double *Mu = nullptr, *Mu_pl = nullptr;
     bool have_coef = false;
 
+    // on-device halo fill (nicam_comm_setup): encoded destination/source
+    // points, the two COMM_var pole-centre entries first, then halo_n entries
+    int *halo_dst = nullptr, *halo_src = nullptr;
+    int halo_n = 0;
+    int rlevel = -1;
+    // post-exchange allowable range built by the limiter when Qout_post is NULL
+    double *qpost = nullptr, *qpost_pl = nullptr;
+
     // vi_rhow_solver work array for the elimination factors (variant "global")
     double *vi_gamma = nullptr;
     size_t vi_gamma_n = 0;
@@ -114,6 +122,23 @@ int launch_hlim(nicam_ctx c, double *q_a, double *q_a_pl, const double *q, const
                 const double *Qpost, const double *Qpost_pl);
 int launch_vlim(nicam_ctx c, double *q_h, double *q_h_pl, const double *q, const double *q_pl,
                 const double *d, const double *d_pl, const double *ck, const double *ck_pl);
+// halo map (comm.cpp) and on-device halo fill (comm.hip)
+// A halo point: kind 0 = regular (l, g), kind 1 = pole (l_pl, v); 0-based.
+struct HaloPoint {
+    int kind, l, g;
+    bool operator<(const HaloPoint &o) const
+    {
+        return kind != o.kind ? kind < o.kind : l != o.l ? l < o.l : g < o.g;
+    }
+};
+// dst/src: the exchange (later singular copies already composed); pole_src:
+// the point each pole centre (north, south) takes before COMM_var's exchange.
+int build_halo_map(int rlevel, int gall_1d, int gall_pl, std::vector<HaloPoint> &dst, std::vector<HaloPoint> &src,
+                   std::vector<int> &have_sgp, HaloPoint pole_src[2]);
+int comm_setup(nicam_ctx c, int rlevel);
+// var(gall,kmax,lall,nvar), var_pl(gall_pl,kmax,2,nvar); pole_fill: COMM_var
+int launch_halo_fill(nicam_ctx c, double *var, double *var_pl, int kmax, int nvar, bool pole_fill);
+
 int launch_vi(nicam_ctx c, double *w, double *w_pl, const double *w0, const double *w0_pl,
               const double *p0, const double *p0_pl, const double *r0, const double *r0_pl,
               const double *Sr, const double *Sr_pl, const double *Sw, const double *Sw_pl,

@@ -72,13 +72,16 @@ def run_flux(ctx: Context, spec: GridSpec, f: Dict[str, np.ndarray], dt: float, 
 
 
 def run_hlim(ctx: Context, spec: GridSpec, f: Dict[str, np.ndarray], fill: float = 0.0,
-             device="cuda:0") -> Dict[str, np.ndarray]:
+             device="cuda:0", exchange: bool = False) -> Dict[str, np.ndarray]:
+    """exchange=True passes Qout_post = NULL: the library exchanges Qout itself
+    (needs ctx.comm_setup)."""
     outs = dict(q_a=f["q_a"], q_a_pl=f["q_a_pl"], Qout_prev=np.full((2,) + spec.shape3, fill),
                 Qout_prev_pl=np.full((2,) + spec.shape_pl, fill))
-    a = _prep(ctx, {**outs, **{k: f[k] for k in HLIM_IN}}, device)
+    ins = [k for k in HLIM_IN if not (exchange and k.startswith("Qout_post"))]
+    a = _prep(ctx, {**outs, **{k: f[k] for k in ins}}, device)
     ctx.horizontal_limiter_thuburn(a["q_a"], a["q_a_pl"], a["q"], a["q_pl"], a["d"], a["d_pl"], a["ch"], a["ch_pl"],
-                                   a["cmask"], a["cmask_pl"], a["Qout_prev"], a["Qout_prev_pl"], a["Qout_post"],
-                                   a["Qout_post_pl"])
+                                   a["cmask"], a["cmask_pl"], a["Qout_prev"], a["Qout_prev_pl"], a.get("Qout_post"),
+                                   a.get("Qout_post_pl"))
     return _back(ctx, {k: a[k] for k in outs})
 
 

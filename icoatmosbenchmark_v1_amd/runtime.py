@@ -75,6 +75,11 @@ SIGNATURES = {
     "nicam_vi_rhow_solver_putcoef": (c_int, [_P] + [_P] * 6),
     "nicam_vi_rhow_solver": (c_int, [_P] + [_P] * 14 + [c_double]),
     "nicam_ctx_synchronize": (c_int, [_P]),
+    "nicam_halo_map": (c_int, [c_int, c_int, c_int, c_int, _P, _P, _P, _P, _P, _P, _P, _P]),
+    "nicam_comm_setup": (c_int, [_P, c_int]),
+    "nicam_pole_fill_map": (c_int, [c_int, c_int, _P, _P]),
+    "nicam_comm_data_transfer": (c_int, [_P, _P, _P, c_int, c_int]),
+    "nicam_comm_var": (c_int, [_P, _P, _P, c_int, c_int]),
 }
 
 
@@ -235,6 +240,26 @@ class Context:
     def synchronize(self) -> None:
         _check(self.lib.nicam_ctx_synchronize(self.h), "nicam_ctx_synchronize")
 
+    # -- halo exchange ---------------------------------------------------------
+    def comm_setup(self, rlevel: int) -> None:
+        """COMM_setup for a context that holds all 10*4^rlevel regions and the poles."""
+        _check(self.lib.nicam_comm_setup(self.h, int(rlevel)), "nicam_comm_setup")
+
+    def _halo_args(self, var, var_pl, kmax, vmax):
+        kmax = self.kall if kmax is None else int(kmax)
+        n = kmax * int(vmax)
+        return [_ptr(var, self.mode, "var", self.gall * self.lall * n),
+                _ptr(var_pl, self.mode, "var_pl", self.gall_pl * 2 * n), kmax, int(vmax)]
+
+    def comm_data_transfer(self, var, var_pl, vmax: int = 1, kmax: Optional[int] = None) -> None:
+        """COMM_data_transfer(var(gall,kmax,lall,vmax), var_pl(gall_pl,kmax,2,vmax)); kmax defaults to kall."""
+        _check(self.lib.nicam_comm_data_transfer(self.h, *self._halo_args(var, var_pl, kmax, vmax)),
+               "nicam_comm_data_transfer")
+
+    def comm_var(self, var, var_pl, vmax: int = 1, kmax: Optional[int] = None) -> None:
+        """COMM_var: pole centres from their regions, then COMM_data_transfer."""
+        _check(self.lib.nicam_comm_var(self.h, *self._halo_args(var, var_pl, kmax, vmax)), "nicam_comm_var")
+
     # -- set-up routines -----------------------------------------------------
     def grd_setup(self, gz: np.ndarray, gzh: np.ndarray):
         """GRD_setup from gz/gzh; returns (rdgzh, afact, bfact)."""
@@ -350,3 +375,25 @@ def grd_input_vgrid(fname: str, kall: int):
     _check(lib.nicam_grd_input_vgrid(str(fname).encode(), kall, gz.ctypes.data, gzh.ctypes.data),
            "nicam_grd_input_vgrid")
     return gz, gzh
+
+
+def halo_map(rlevel: int, gall_1d: int, gall_pl: int = 6):
+    """The icosahedral halo map (nicam_halo_map): a dict of int32 arrays
+    dst_kind, dst_l, dst_g, src_kind, src_l, src_g (kind 0 regular, 1 pole;
+    all indices 0-based), have_sgp (one flag per region), and pole_fill_l,
+    pole_fill_g (the regular point each pole centre takes in COMM_var)."""
+    lib = load_library()
+    n = c_int(0)
+    _check(lib.nicam_halo_map(rlevel, gall_1d, gall_pl, 0, ctypes.byref(n), *([None] * 7)), "nicam_halo_map")
+    keys = ["dst_kind", "dst_l", "dst_g", "src_kind", "src_l", "src_g"]
+    out = {k: np.empty(n.value, dtype=np.int32) for k in keys}
+    out["have_sgp"] = np.empty(10 * 4 ** rlevel, dtype=np.int32)
+    m = c_int(0)
+    _check(lib.nicam_halo_map(rlevel, gall_1d, gall_pl, n.value, ctypes.byref(m),
+                              *[out[k].ctypes.data for k in keys + ["have_sgp"]]), "nicam_halo_map")
+    if m.value != n.value:
+        raise RuntimeError("nicam_halo_map changed size between calls")
+    src_l, src_g = np.empty(2, dtype=np.int32), np.empty(2, dtype=np.int32)
+    _check(lib.nicam_pole_fill_map(rlevel, gall_1d, src_l.ctypes.data, src_g.ctypes.data), "nicam_pole_fill_map")
+    out["pole_fill_l"], out["pole_fill_g"] = src_l, src_g
+    return out

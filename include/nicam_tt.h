@@ -157,7 +157,9 @@ int nicam_horizontal_flux(nicam_ctx ctx,
  *   in    Qout_post(gall,kall,lall,2), Qout_post_pl(gall_pl,kall,2,2)
  * Qout_prev is the allowable range before the halo exchange; the clamp uses
  * Qout_post, the range after it, exactly as the reference kernel does
- * (:365-375). */
+ * (:365-375).  Qout_post = NULL asks the library to do that exchange itself
+ * (the COMM_data_transfer the kernel cut out), on a context-owned copy of
+ * Qout_prev; it needs nicam_comm_setup, and Qout_post_pl is then ignored. */
 int nicam_horizontal_limiter_thuburn(nicam_ctx ctx,
                                      double *q_a, double *q_a_pl,
                                      const double *q, const double *q_pl,
@@ -201,6 +203,47 @@ int nicam_vi_rhow_solver(nicam_ctx ctx,
 
 /* Blocks until all work queued on the context's stream has finished. */
 int nicam_ctx_synchronize(nicam_ctx ctx);
+
+/* ---- halo exchange on one GPU ------------------------------------------ */
+
+/* The halo map of the icosahedral grid at region level rlevel, as the
+ * reference builds it (communication/src/mod_adm.f90:629-894 region topology,
+ * communication/src/mod_comm.f90:213-609 region-to-region lists, :924-1380
+ * pole lists, :1384-1502 singular points).  Host-only; needs no context.
+ * Entry e copies point (src_kind, src_l, src_g) into (dst_kind, dst_l, dst_g):
+ * kind 0 is a regular point, l the region (0..10*4^rlevel-1) and g the 0-based
+ * point (j*gall_1d + i); kind 1 is a pole point, l 0 = north, 1 = south, g
+ * 0..gall_pl-1.  With capacity < 1 only *n_out is set.  have_sgp (may be
+ * NULL) receives the 10*4^rlevel singular-point flags ADM_have_sgp: 1 for the
+ * regions whose west vertex joins only three regions (mod_adm.f90:257-264). */
+int nicam_halo_map(int rlevel, int gall_1d, int gall_pl, int capacity, int *n_out,
+                   int *dst_kind, int *dst_l, int *dst_g,
+                   int *src_kind, int *src_l, int *src_g, int *have_sgp);
+
+/* COMM_setup (communication/src/mod_comm.f90:147-209) for a context holding
+ * every region of the level-rlevel grid (lall = 10*4^rlevel, have_pl = 1):
+ * builds the halo map and keeps it on the device.  The context's have_sgp
+ * must agree with the topology. */
+int nicam_comm_setup(nicam_ctx ctx, int rlevel);
+
+/* The point whose value COMM_var copies into each pole centre before its
+ * exchange (mod_comm.f90:1895-2101 with RGNMNG_rgn4pl, mod_adm.f90:393-405):
+ * pole p (0 north, 1 south) takes regular point src_g[p] of region src_l[p]
+ * (0-based).  Host-only. */
+int nicam_pole_fill_map(int rlevel, int gall_1d, int *src_l, int *src_g);
+
+/* COMM_data_transfer (communication/src/mod_comm.f90:1506-1891) on the
+ * device: fills the halo of var(gall,kmax,lall,vmax) and the pole rings of
+ * var_pl(gall_pl,kmax,2,vmax) from their owners, then applies the singular
+ * point copies, all in one gather launch.  kmax and vmax are the extents
+ * the reference takes from the assumed-shape arguments (kmax need not be
+ * kall).  Needs nicam_comm_setup. */
+int nicam_comm_data_transfer(nicam_ctx ctx, double *var, double *var_pl, int kmax, int vmax);
+
+/* COMM_var (communication/src/mod_comm.f90:1895-2101): each pole centre takes
+ * its value from the point nicam_pole_fill_map names, then the exchange of
+ * nicam_comm_data_transfer runs.  Same arguments. */
+int nicam_comm_var(nicam_ctx ctx, double *var, double *var_pl, int kmax, int vmax);
 
 #ifdef __cplusplus
 }
