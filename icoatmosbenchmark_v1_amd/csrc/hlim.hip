@@ -317,27 +317,18 @@ int launch_hlim(nicam_ctx c, double *q_a, double *q_a_pl, const double *q, const
     if (Qpost) return run(3);  // the reference's kernel: Qout_post supplied
 
     // No Qout_post: do the halo exchange the reference cut out (HLIM:365-375,
-    // the commented COMM_data_transfer of Qout) on the device.  Qout_prev gets
-    // the pre-exchange range; the clamp uses a context-owned exchanged copy.
+    // the commented COMM_data_transfer of Qout) on the device, in place on
+    // Qout_prev as NICAM itself exchanges Qout: sources are inner points and
+    // pole centres, destinations halo points and pole rings, so the gather
+    // reads nothing it writes.  The clamp then reads the exchanged range.
     if (!c->halo_n) return fail(NICAM_ERR_STATE, "Qout_post is NULL but no halo map is set (nicam_comm_setup)");
     if (!Qprev_pl) return fail(NICAM_ERR_ARG, "Qout_prev_pl is needed to exchange Qout with the poles");
-    const size_t nq = 2 * D.nreg(), nqp = 2 * D.npl();
-    if (!c->qpost) {
-        NICAM_HIP(hipMalloc(&c->qpost, nq * sizeof(double)));
-        c->device_bytes += nq * sizeof(double);
-    }
-    if (!c->qpost_pl) {
-        NICAM_HIP(hipMalloc(&c->qpost_pl, nqp * sizeof(double)));
-        c->device_bytes += nqp * sizeof(double);
-    }
     int rc = run(1);
     if (rc != NICAM_OK) return rc;
-    NICAM_HIP(hipMemcpyAsync(c->qpost, Qprev, nq * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
-    NICAM_HIP(hipMemcpyAsync(c->qpost_pl, Qprev_pl, nqp * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
-    rc = launch_halo_fill(c, c->qpost, c->qpost_pl, D.kall, 2, false);
+    rc = launch_halo_fill(c, Qprev, Qprev_pl, D.kall, 2, false);
     if (rc != NICAM_OK) return rc;
-    A.Qpost = c->qpost;
-    P.Qpost = c->qpost_pl;
+    A.Qpost = Qprev;
+    P.Qpost = Qprev_pl;
     return run(2);
 }
 

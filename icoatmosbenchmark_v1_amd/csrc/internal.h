@@ -86,8 +86,6 @@ struct nicam_ctx_s {
     int *halo_dst = nullptr, *halo_src = nullptr;
     int halo_n = 0;
     int rlevel = -1;
-    // post-exchange allowable range built by the limiter when Qout_post is NULL
-    double *qpost = nullptr, *qpost_pl = nullptr;
 
     // vi_rhow_solver work array for the elimination factors (variant "global")
     double *vi_gamma = nullptr;

@@ -158,8 +158,9 @@ int nicam_horizontal_flux(nicam_ctx ctx,
  * Qout_prev is the allowable range before the halo exchange; the clamp uses
  * Qout_post, the range after it, exactly as the reference kernel does
  * (:365-375).  Qout_post = NULL asks the library to do that exchange itself
- * (the COMM_data_transfer the kernel cut out), on a context-owned copy of
- * Qout_prev; it needs nicam_comm_setup, and Qout_post_pl is then ignored. */
+ * (the COMM_data_transfer of Qout the kernel cut out), in place on Qout_prev
+ * as NICAM does: Qout_prev then returns the range after the exchange.  It
+ * needs nicam_comm_setup; Qout_post_pl is then ignored. */
 int nicam_horizontal_limiter_thuburn(nicam_ctx ctx,
                                      double *q_a, double *q_a_pl,
                                      const double *q, const double *q_pl,

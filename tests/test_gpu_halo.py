@@ -108,8 +108,9 @@ def test_setup_errors(gpu):
 
 @pytest.mark.parametrize("rlevel,gall_1d,kall", [(0, 6, 5), (0, 18, 8), (1, 10, 6)])
 def test_limiter_with_library_exchange(gpu, oracle, rlevel, gall_1d, kall):
-    """Qout_post = NULL: Qout_prev as with Qout_post given, and q_a as the
-    oracle gives it for Qout_post = the exchanged Qout_prev."""
+    """Qout_post = NULL: the exchange runs in place on Qout_prev, which ends
+    as the exchanged range, and q_a is what the oracle gives for Qout_post =
+    that exchanged range."""
     m = runtime.halo_map(rlevel, gall_1d)
     sgp = m["have_sgp"].tolist()
     spec = synth.GridSpec(gall_1d, kall, 10 * 4 ** rlevel, have_pl=True, seed=50 + rlevel)
@@ -123,7 +124,7 @@ def test_limiter_with_library_exchange(gpu, oracle, rlevel, gall_1d, kall):
     want = oracle.horizontal_limiter_thuburn(spec, sgp, {**case["hlim"], "Qout_post": post,
                                                          "Qout_post_pl": post_pl})
     assert np.array_equal(want["Qout_prev"], qprev)
+    want["Qout_prev"], want["Qout_prev_pl"] = post, post_pl
     assert_parity(got, want)
-    # a second call reuses the context's exchange buffers
-    got2 = drivers.run_hlim(ctx, spec, case["hlim"], exchange=True)
+    got2 = drivers.run_hlim(ctx, spec, case["hlim"], exchange=True)  # repeatable
     assert_parity(got2, want)
