@@ -30,10 +30,13 @@ class OrcDims(ctypes.Structure):
 
 
 def build(with_ref: bool = True, quiet: bool = True) -> None:
-    """Runs oracle/Makefile (the reference part only if its tree is present)."""
+    """Runs oracle/Makefile (the reference part only if its tree is present;
+    the shim-linked drivers also need the built libnicam_tt.so)."""
     targets = ["oracle"]
     if with_ref and Path("/root/reference/kernels/NICAM").is_dir():
         targets.append("ref")
+        if (HERE.parent / "icoatmosbenchmark_v1_amd" / "libnicam_tt.so").exists():
+            targets.append("drivers")
     out = subprocess.run(["make", "-C", str(HERE), *targets], stdout=subprocess.PIPE, stderr=subprocess.STDOUT,
                          text=True)
     if out.returncode != 0:
