@@ -130,6 +130,46 @@ Topology build_topology(int rlevel)
 
 }  // namespace
 
+// Which corners of the 10 diamonds are the same icosahedron vertex.  Two
+// diamonds that share an edge share its two end points; the halo copy rules
+// of mod_comm.f90:259-348 (read at rlevel 0, where a region is a diamond) say
+// which ends meet: across a SW edge seen as the neighbour's NE edge, our W is
+// its N and our S its E; seen as its SE edge, our W is its E and our S its S;
+// across a NW edge seen as its SE edge, our W is its S and our N its E; seen
+// as its NE edge, our W is its E and our N its N.  cls[d][c] (corners W, N,
+// E, S) gets a class id 0..n-1; returns n (12 on the icosahedron).
+int diamond_corner_classes(int cls[10][4])
+{
+    const Topology T = build_topology(0);
+    enum { W = 0, N = 1, E = 2, S = 3 };
+    int parent[40];
+    for (int i = 0; i < 40; ++i) parent[i] = i;
+    auto find = [&](int x) {
+        while (parent[x] != x) x = parent[x] = parent[parent[x]];
+        return x;
+    };
+    auto unite = [&](int d1, int c1, int d2, int c2) {
+        const int a = find(d1 * 4 + c1), b = find(d2 * 4 + c2);
+        if (a != b) parent[a] = b;
+    };
+    for (int d = 1; d <= 10; ++d) {
+        const int rs = T.edge_rgn[T.E(d, SW)], ds = T.edge_dir[T.E(d, SW)];
+        if (ds == NE) { unite(d - 1, W, rs - 1, N); unite(d - 1, S, rs - 1, E); }
+        else { unite(d - 1, W, rs - 1, E); unite(d - 1, S, rs - 1, S); }  // SE
+        const int rn = T.edge_rgn[T.E(d, NW)], dn = T.edge_dir[T.E(d, NW)];
+        if (dn == SE) { unite(d - 1, W, rn - 1, S); unite(d - 1, N, rn - 1, E); }
+        else { unite(d - 1, W, rn - 1, E); unite(d - 1, N, rn - 1, N); }  // NE
+    }
+    int id[40], n = 0;
+    for (int i = 0; i < 40; ++i) id[i] = -1;
+    for (int i = 0; i < 40; ++i) {
+        const int r = find(i);
+        if (id[r] < 0) id[r] = n++;
+        cls[i / 4][i % 4] = id[r];
+    }
+    return n;
+}
+
 int build_halo_map(int rlevel, int gall_1d, int gall_pl, std::vector<HaloPoint> &dst, std::vector<HaloPoint> &src,
                    std::vector<int> &have_sgp, HaloPoint pole_src[2])
 {

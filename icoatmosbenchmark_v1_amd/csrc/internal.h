@@ -134,6 +134,8 @@ struct HaloPoint {
 int build_halo_map(int rlevel, int gall_1d, int gall_pl, std::vector<HaloPoint> &dst, std::vector<HaloPoint> &src,
                    std::vector<int> &have_sgp, HaloPoint pole_src[2]);
 int comm_setup(nicam_ctx c, int rlevel);
+// corners W, N, E, S of the 10 diamonds -> icosahedron vertex class; returns the class count
+int diamond_corner_classes(int cls[10][4]);
 // var(gall,kmax,lall,nvar), var_pl(gall_pl,kmax,2,nvar); pole_fill: COMM_var
 int launch_halo_fill(nicam_ctx c, double *var, double *var_pl, int kmax, int nvar, bool pole_fill);
 

@@ -80,6 +80,7 @@ SIGNATURES = {
     "nicam_pole_fill_map": (c_int, [c_int, c_int, _P, _P]),
     "nicam_comm_data_transfer": (c_int, [_P, _P, _P, c_int, c_int]),
     "nicam_comm_var": (c_int, [_P, _P, _P, c_int, c_int]),
+    "nicam_grid_icosahedral": (c_int, [c_int, c_int, c_double, _P, _P]),
 }
 
 
@@ -397,3 +398,15 @@ def halo_map(rlevel: int, gall_1d: int, gall_pl: int = 6):
     _check(lib.nicam_pole_fill_map(rlevel, gall_1d, src_l.ctypes.data, src_g.ctypes.data), "nicam_pole_fill_map")
     out["pole_fill_l"], out["pole_fill_g"] = src_l, src_g
     return out
+
+
+def grid_icosahedral(glevel: int, rlevel: int, radius: float = 6.37122e6):
+    """nicam_grid_icosahedral: (GRD_x[3][lall][1][gall], GRD_x_pl[3][2][1][6])."""
+    lib = load_library()
+    n1 = 2 ** (glevel - rlevel) + 2
+    lall = 10 * 4 ** rlevel
+    x = np.empty((3, lall, 1, n1 * n1))
+    x_pl = np.empty((3, 2, 1, 6))
+    _check(lib.nicam_grid_icosahedral(glevel, rlevel, float(radius), x.ctypes.data, x_pl.ctypes.data),
+           "nicam_grid_icosahedral")
+    return x, x_pl

@@ -241,6 +241,15 @@ int nicam_pole_fill_map(int rlevel, int gall_1d, int *src_l, int *src_g);
  * kall).  Needs nicam_comm_setup. */
 int nicam_comm_data_transfer(nicam_ctx ctx, double *var, double *var_pl, int kmax, int vmax);
 
+/* An icosahedral grid (host-only): the positions GRD_x(gall,1,lall,3) and
+ * GRD_x_pl(6,1,2,3) that the metrics kernel reads from its snapshot
+ * (dyn_metrics/src/main.f90:217-218), for gall_1d = 2^(glevel-rlevel)+2 and
+ * lall = 10*4^rlevel, on a sphere of the given radius.  Inner points come from
+ * the 10 diamonds of the icosahedron refined glevel times by great-circle
+ * midpoints; halo points and pole rings are then filled with the halo map, as
+ * COMM_data_transfer(GRD_x, GRD_x_pl) does.  Halo points outside the map are 0. */
+int nicam_grid_icosahedral(int glevel, int rlevel, double radius, double *GRD_x, double *GRD_x_pl);
+
 /* COMM_var (communication/src/mod_comm.f90:1895-2101): each pole centre takes
  * its value from the point nicam_pole_fill_map names, then the exchange of
  * nicam_comm_data_transfer runs.  Same arguments. */

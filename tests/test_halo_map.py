@@ -226,3 +226,38 @@ def test_map_rejects_bad_arguments(lib):
     assert lib.nicam_halo_map(0, 10, 6, 0, None, *([None] * 7)) == runtime.NICAM_ERR_ARG
     assert lib.nicam_halo_map(0, 10, 6, 0, runtime.ctypes.byref(n), *([None] * 7)) == runtime.NICAM_OK
     assert lib.nicam_halo_map(0, 10, 6, n.value - 1, runtime.ctypes.byref(n), *([None] * 7)) == runtime.NICAM_ERR_ARG
+
+
+@pytest.mark.parametrize("glevel,rlevel", [(3, 0), (4, 1), (5, 2), (5, 3)])
+def test_icosahedral_grid_pins_the_map_geometrically(lib, glevel, rlevel):
+    """nicam_grid_icosahedral places 10*4^glevel + 2 distinct points on the
+    sphere and fills halos and pole rings with the halo map.  The map is then
+    right exactly when every point's stencil -- its six lattice neighbours
+    (i+-1, j), (i, j+-1), (i+1, j+1), (i-1, j-1) read through the halo, five
+    distinct ones at a pentagon -- is the set of its nearest grid points, and
+    each pole centre's ring is its five nearest points.  A wrong source
+    anywhere in the map puts a far point into some stencil."""
+    from scipy.spatial import cKDTree
+
+    x, x_pl = runtime.grid_icosahedral(glevel, rlevel, 1.0)
+    n1, lall = 2 ** (glevel - rlevel) + 2, 10 * 4 ** rlevel
+    X = x[:, :, 0, :].reshape(3, lall, n1, n1)  # [c][l][j][i]
+    pts = np.vstack([X[:, :, 1:-1, 1:-1].reshape(3, -1).T, x_pl[:, :, 0, 0].T])
+    assert len(pts) == 10 * 4 ** glevel + 2
+    assert np.allclose(np.linalg.norm(pts, axis=1), 1.0, rtol=0, atol=4e-16)
+    tree = cKDTree(pts)
+    d, _ = tree.query(pts, 2)
+    assert d[:, 1].min() > 0.5 * d[:, 1].max()  # all distinct, near-uniform spacing
+    m = runtime.halo_map(rlevel, n1)
+    for l in range(lall):
+        for j in range(1, n1 - 1):
+            for i in range(1, n1 - 1):
+                nb = X[:, l, [j, j + 1, j + 1, j, j - 1, j - 1], [i + 1, i + 1, i, i - 1, i - 1, i]].T
+                uniq = {tuple(v) for v in np.round(nb, 13)}
+                pentagon = i == 1 and j == 1 and m["have_sgp"][l]
+                assert len(uniq) == (5 if pentagon else 6), (l, i, j)
+                _, idx = tree.query(X[:, l, j, i], len(uniq) + 1)
+                assert uniq == {tuple(v) for v in np.round(pts[idx[1:]], 13)}, (l, i, j)
+    for p in range(2):
+        _, idx = tree.query(x_pl[:, p, 0, 0], 6)
+        assert {tuple(v) for v in np.round(x_pl[:, p, 0, 1:].T, 13)} == {tuple(v) for v in np.round(pts[idx[1:]], 13)}
