@@ -231,11 +231,11 @@ __global__ void __launch_bounds__(CHUNK) hflux_kernel(HfluxArgs A)
 
             const size_t fb = ((size_t)l * kall + k) * slab + g;
 #pragma unroll
-            for (int m = 0; m < 6; ++m) A.flx[m * plane + fb] = face[m];
+            for (int m = 0; m < 6; ++m) stream_store(&A.flx[m * plane + fb], face[m]);
 #pragma unroll
             for (int d = 0; d < 3; ++d)
 #pragma unroll
-                for (int a = 0; a < 3; ++a) A.xc[(size_t)(d * 3 + a) * plane + fb] = xcv[a][d];
+                for (int a = 0; a < 3; ++a) stream_store(&A.xc[(size_t)(d * 3 + a) * plane + fb], xcv[a][d]);
         }
         __syncthreads();
     }

@@ -105,7 +105,7 @@ __global__ void __launch_bounds__(CHUNK) hlim_kernel(HlimArgs A)
 #pragma unroll
         for (int m = 0; m < 3; ++m) {
             arc_minmax(m, g, omn[m], omx[m]);
-            ocm[m] = A.cmask[(size_t)m * plane + fb + g];
+            ocm[m] = stream_load(&A.cmask[(size_t)m * plane + fb + g]);
         }
     }
 
@@ -138,7 +138,7 @@ __global__ void __launch_bounds__(CHUNK) hlim_kernel(HlimArgs A)
         double chv[6], chm[6];
 #pragma unroll
         for (int f = 0; f < 6; ++f) {
-            chv[f] = A.ch[(size_t)f * plane + fb + g];
+            chv[f] = stream_load(&A.ch[(size_t)f * plane + fb + g]);
             chm[f] = dmin(chv[f], 0.0);
         }
         const double Cin_sum = chm[0] + chm[1] + chm[2] + chm[3] + chm[4] + chm[5];
@@ -159,8 +159,8 @@ __global__ void __launch_bounds__(CHUNK) hlim_kernel(HlimArgs A)
         qout_max = qc;
     }
     if (A.phases & 1) {
-        A.Qprev[fb + g] = qout_min;
-        A.Qprev[plane + fb + g] = qout_max;
+        stream_store(&A.Qprev[fb + g], qout_min);
+        stream_store(&A.Qprev[plane + fb + g], qout_max);
     }
 
     // ---- phase 2: clamp the face values of the own arcs (HLIM:377-410) ----
@@ -178,8 +178,8 @@ __global__ void __launch_bounds__(CHUNK) hlim_kernel(HlimArgs A)
             double qa = A.q_a[(size_t)m * plane + fb + g];
             qa = (c)*dmin(dmax(qa, in_mn_s), in_mx_s) + (1.0 - c) * dmin(dmax(qa, in_mn_n), in_mx_n);
             qa = (c)*dmax(dmin(qa, qo_max_n), qo_min_n) + (1.0 - c) * dmax(dmin(qa, qo_max_c), qo_min_c);
-            A.q_a[(size_t)m * plane + fb + g] = qa;
-            A.q_a[(size_t)(m + 3) * plane + fb + nb[m]] = qa;
+            stream_store(&A.q_a[(size_t)m * plane + fb + g], qa);
+            stream_store(&A.q_a[(size_t)(m + 3) * plane + fb + nb[m]], qa);
         }
     }
 }

@@ -109,6 +109,32 @@ int hip_fail(hipError_t e, const char *what);
         if (e_ != hipSuccess) return ::nicam::hip_fail(e_, #call); \
     } while (0)
 
+// Stores of outputs that no work-item of the same launch reads back, and
+// loads of inputs read once: with NICAM_STREAMING set they carry the
+// nontemporal hint, so streams of hundreds of MB do not evict the halo rows
+// and metrics that neighbouring work-groups do re-read from L2.
+#ifndef NICAM_STREAMING
+#define NICAM_STREAMING 0
+#endif
+template <class T>
+__device__ __forceinline__ void stream_store(T *p, T v)
+{
+#if NICAM_STREAMING & 1
+    __builtin_nontemporal_store(v, p);
+#else
+    *p = v;
+#endif
+}
+template <class T>
+__device__ __forceinline__ T stream_load(const T *p)
+{
+#if NICAM_STREAMING & 2
+    return __builtin_nontemporal_load(p);
+#else
+    return *p;
+#endif
+}
+
 // launchers (asynchronous on ctx->stream, device pointers)
 int launch_hflux(nicam_ctx c, double *flx_h, double *flx_h_pl, double *xc, double *xc_pl,
                  const double *rho, const double *rho_pl, const double *rvx, const double *rvx_pl,

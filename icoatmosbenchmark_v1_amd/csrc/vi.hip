@@ -157,7 +157,7 @@ __device__ __forceinline__ void vi_column(const ViField &F, size_t base, size_t 
         double wf_n = 0.0, g_n = 0.0, gs_n = 0.0;
         if (k - 1 >= kmin + 1) load_b(k - 1, wf_n, g_n, gs_n);
         const double wk = wf_c - g_c * wn;
-        F.w[at(k + 1)] = wn * gs_c;
+        stream_store(&F.w[at(k + 1)], wn * gs_c);
         wn = wk;
         wf_c = wf_n;
         g_c = g_n;

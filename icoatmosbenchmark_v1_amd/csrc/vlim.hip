@@ -77,7 +77,7 @@ __device__ __forceinline__ void vlim_run(double *__restrict__ q_h, const double 
                                   (1.0 - zerosw) + qk * zerosw;
         if (k >= kb) {  // VLIM:179-180
             const double qh = q_h[o];
-            q_h[o] = (inflagL)*dmax(dmin(qh, Qmax_km1), Qmin_km1) + (1.0 - inflagL) * dmax(dmin(qh, Qmax_k), Qmin_k);
+            stream_store(&q_h[o], (inflagL)*dmax(dmin(qh, Qmax_km1), Qmin_km1) + (1.0 - inflagL) * dmax(dmin(qh, Qmax_k), Qmin_k));
         }
         Qmin_km1 = Qmin_k;
         Qmax_km1 = Qmax_k;
