@@ -205,6 +205,7 @@ int nicam_ctx_set_tuning(nicam_ctx c, int param, int value)
     switch (param) {
     case NICAM_TUNE_HFLUX_KPER: c->tune.hflux_kper = value; break;
     case NICAM_TUNE_VLIM_KCHUNK: c->tune.vlim_kchunk = value; break;
+    case NICAM_TUNE_DIFF_KPER: c->tune.diff_kper = value; break;
     case NICAM_TUNE_VI_VARIANT:
         if (value > 3) return fail(NICAM_ERR_ARG, "NICAM_TUNE_VI_VARIANT must be 0..3");
         c->tune.vi_variant = value;
@@ -549,6 +550,36 @@ int nicam_vertical_limiter_thuburn(nicam_ctx c, double *q_h, double *q_h_pl, con
     NICAM_TRY(stage_in(c, a, v));
     auto D = [&](int i) { return static_cast<double *>(v[i]); };
     NICAM_TRY(launch_vlim(c, D(0), D(1), D(2), D(3), D(4), D(5), D(6), D(7)));
+    return stage_out(c, a, v);
+}
+
+int nicam_oprt_diffusion(nicam_ctx c, double *dscl, double *dscl_pl, const double *scl, const double *scl_pl,
+                         const double *kh, const double *kh_pl, const double *coef_intp, const double *coef_intp_pl,
+                         const double *coef_diff, const double *coef_diff_pl)
+{
+    NICAM_TRY(check_ctx(c));
+    const void *req[] = {dscl, scl, kh, coef_intp, coef_diff};
+    const char *nm[] = {"dscl", "scl", "kh", "coef_intp", "coef_diff"};
+    for (int i = 0; i < 5; ++i) NICAM_TRY(need(req[i], nm[i]));
+    if (c->d.have_pl) {
+        const void *rp[] = {dscl_pl, scl_pl, kh_pl, coef_intp_pl, coef_diff_pl};
+        for (int i = 0; i < 5; ++i) NICAM_TRY(need(rp[i], "pole array"));
+    }
+    // dscl_pl is zeroed without pole regions, as the reference does, if given
+    if (c->ptr_mode == NICAM_PTR_DEVICE)
+        return launch_diffusion(c, dscl, dscl_pl, scl, PL(scl_pl), kh, PL(kh_pl), coef_intp, PL(coef_intp_pl),
+                                coef_diff, PL(coef_diff_pl));
+    const size_t R = c->d.nreg() * 8, P = c->d.npl() * 8;
+    const size_t CI = (size_t)c->d.gall * c->d.lall * 18 * 8, CIP = (size_t)c->d.gall_pl * 18 * 8;
+    const size_t CD = (size_t)c->d.gall * c->d.lall * 18 * 8, CDP = (size_t)5 * 3 * 2 * 8;
+    std::vector<HostArg> a = {{dscl, R, true},         {dscl_pl, P, true},      {scl, R, false},
+                              {PL(scl_pl), P, false},  {kh, R, false},          {PL(kh_pl), P, false},
+                              {coef_intp, CI, false},  {PL(coef_intp_pl), CIP, false},
+                              {coef_diff, CD, false},  {PL(coef_diff_pl), CDP, false}};
+    std::vector<void *> v;
+    NICAM_TRY(stage_in(c, a, v));
+    auto D = [&](int i) { return static_cast<double *>(v[i]); };
+    NICAM_TRY(launch_diffusion(c, D(0), D(1), D(2), D(3), D(4), D(5), D(6), D(7), D(8), D(9)));
     return stage_out(c, a, v);
 }
 

@@ -40,6 +40,7 @@ struct nicam_tune {
     int hflux_kper = 0;   // levels per work-group in horizontal_flux
     int vlim_kchunk = 0;  // levels per work-item in vertical_limiter_thuburn
     int vi_variant = 0;   // elimination-factor storage in vi_rhow_solver
+    int diff_kper = 0;    // levels per work-group in OPRT_diffusion
 };
 
 struct nicam_ctx_s {
@@ -165,6 +166,9 @@ int diamond_corner_classes(int cls[10][4]);
 // var(gall,kmax,lall,nvar), var_pl(gall_pl,kmax,2,nvar); pole_fill: COMM_var
 int launch_halo_fill(nicam_ctx c, double *var, double *var_pl, int kmax, int nvar, bool pole_fill);
 
+int launch_diffusion(nicam_ctx c, double *dscl, double *dscl_pl, const double *scl, const double *scl_pl,
+                     const double *kh, const double *kh_pl, const double *coef_intp, const double *coef_intp_pl,
+                     const double *coef_diff, const double *coef_diff_pl);
 int launch_vi(nicam_ctx c, double *w, double *w_pl, const double *w0, const double *w0_pl,
               const double *p0, const double *p0_pl, const double *r0, const double *r0_pl,
               const double *Sr, const double *Sr_pl, const double *Sw, const double *Sw_pl,

@@ -105,3 +105,14 @@ def run_vi(ctx: Context, spec: GridSpec, f: Dict[str, np.ndarray], dt: float,
         args += [a[k], a[k + "_pl"]]
     ctx.vi_rhow_solver(a["rhogw"], a["rhogw_pl"], *args, dt)
     return _back(ctx, {k: a[k] for k in outs})
+
+
+DIFF_IN = ("scl", "scl_pl", "kh", "kh_pl", "coef_intp", "coef_intp_pl", "coef_diff", "coef_diff_pl")
+
+
+def run_diffusion(ctx: Context, spec: GridSpec, f: Dict[str, np.ndarray], fill: float = 0.0,
+                  device="cuda:0") -> Dict[str, np.ndarray]:
+    outs = dict(dscl=np.full(spec.shape3, fill), dscl_pl=np.full(spec.shape_pl, fill))
+    a = _prep(ctx, {**outs, **{k: f[k] for k in DIFF_IN}}, device)
+    ctx.oprt_diffusion(a["dscl"], a["dscl_pl"], *[a[k] for k in DIFF_IN])
+    return _back(ctx, {k: a[k] for k in outs})

@@ -22,6 +22,7 @@ module nicam_tt_f
   integer(c_int), parameter, public :: NICAM_TUNE_HFLUX_KPER  = 1
   integer(c_int), parameter, public :: NICAM_TUNE_VLIM_KCHUNK = 3
   integer(c_int), parameter, public :: NICAM_TUNE_VI_VARIANT  = 4
+  integer(c_int), parameter, public :: NICAM_TUNE_DIFF_KPER   = 5
 
   public :: nicam_check
 
@@ -221,6 +222,22 @@ module nicam_tt_f
        integer(c_int) :: nicam_comm_var
      end function nicam_comm_var
 
+     function nicam_oprt_diffusion(ctx, dscl, dscl_pl, scl, scl_pl, kh, kh_pl, coef_intp, coef_intp_pl, &
+          coef_diff, coef_diff_pl) bind(C, name='nicam_oprt_diffusion')
+       import :: c_ptr, c_int
+       type(c_ptr), value :: ctx, dscl, dscl_pl, scl, scl_pl, kh, kh_pl
+       type(c_ptr), value :: coef_intp, coef_intp_pl, coef_diff, coef_diff_pl
+       integer(c_int) :: nicam_oprt_diffusion
+     end function nicam_oprt_diffusion
+
+     function nicam_grid_icosahedral(glevel, rlevel, radius, GRD_x, GRD_x_pl) bind(C, name='nicam_grid_icosahedral')
+       import :: c_int, c_double
+       integer(c_int), value :: glevel, rlevel
+       real(c_double), value :: radius
+       real(c_double), intent(out) :: GRD_x(*), GRD_x_pl(*)
+       integer(c_int) :: nicam_grid_icosahedral
+     end function nicam_grid_icosahedral
+
      function c_strlen(s) bind(C, name='strlen')
        import :: c_ptr, c_size_t
        type(c_ptr), value :: s
@@ -234,7 +251,8 @@ module nicam_tt_f
   public :: nicam_set_vgrid_factors, nicam_set_hmetrics, nicam_set_vmetrics, nicam_horizontal_flux
   public :: nicam_horizontal_limiter_thuburn, nicam_vertical_limiter_thuburn, nicam_vi_rhow_solver_putcoef
   public :: nicam_vi_rhow_solver, nicam_ctx_synchronize, nicam_halo_map, nicam_pole_fill_map
-  public :: nicam_comm_setup, nicam_comm_data_transfer, nicam_comm_var
+  public :: nicam_comm_setup, nicam_comm_data_transfer, nicam_comm_var, nicam_oprt_diffusion
+  public :: nicam_grid_icosahedral
 
 contains
 

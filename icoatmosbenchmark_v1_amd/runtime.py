@@ -41,6 +41,7 @@ PTR_HOST = 1
 TUNE_HFLUX_KPER = 1
 TUNE_VLIM_KCHUNK = 3
 TUNE_VI_VARIANT = 4
+TUNE_DIFF_KPER = 5
 
 STATUS_NAMES = {
     NICAM_ERR_ARG: "NICAM_ERR_ARG",
@@ -81,6 +82,7 @@ SIGNATURES = {
     "nicam_comm_data_transfer": (c_int, [_P, _P, _P, c_int, c_int]),
     "nicam_comm_var": (c_int, [_P, _P, _P, c_int, c_int]),
     "nicam_grid_icosahedral": (c_int, [c_int, c_int, c_double, _P, _P]),
+    "nicam_oprt_diffusion": (c_int, [_P] + [_P] * 10),
 }
 
 
@@ -354,6 +356,16 @@ class Context:
         a = self._args([(q_h, "q_h", R), (q_h_pl, "q_h_pl", P), (q, "q", R), (q_pl, "q_pl", P), (d, "d", R),
                         (d_pl, "d_pl", P), (ck, "ck", 2 * R), (ck_pl, "ck_pl", 2 * P)])
         _check(self.lib.nicam_vertical_limiter_thuburn(self.h, *a), "nicam_vertical_limiter_thuburn")
+
+    def oprt_diffusion(self, dscl, dscl_pl, scl, scl_pl, kh, kh_pl, coef_intp, coef_intp_pl,
+                       coef_diff, coef_diff_pl) -> None:
+        """OPRT_diffusion (dyn_diffusion/src/mod_oprt.f90:41-296)."""
+        R, P, G = self.nreg, self.npl, self.gall * self.lall
+        a = [_ptr(dscl, self.mode, "dscl", R), _ptr(dscl_pl, self.mode, "dscl_pl", P)]
+        a += self._args([(scl, "scl", R), (scl_pl, "scl_pl", P), (kh, "kh", R), (kh_pl, "kh_pl", P),
+                         (coef_intp, "coef_intp", 18 * G), (coef_intp_pl, "coef_intp_pl", 18 * self.gall_pl),
+                         (coef_diff, "coef_diff", 18 * G), (coef_diff_pl, "coef_diff_pl", 30)])
+        _check(self.lib.nicam_oprt_diffusion(self.h, *a), "nicam_oprt_diffusion")
 
     def vi_rhow_solver(self, rhogw, rhogw_pl, rhogw0, rhogw0_pl, preg0, preg0_pl, rhog0, rhog0_pl,
                        Srho, Srho_pl, Sw, Sw_pl, Spre, Spre_pl, dt: float) -> None:

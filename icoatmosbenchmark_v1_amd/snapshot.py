@@ -10,6 +10,7 @@ order the driver reads them (paths relative to kernels/NICAM/):
 * dyn_horiz_adv_limiter/src/main.f90:132-147
 * dyn_vert_adv_limiter/src/main.f90:96-105
 * dyn_vi_rhow_solver/src/main.f90:198-229
+* dyn_diffusion/src/main.f90:96-105
 
 Fields named like the kernel's outputs (``flx_h``, ``q_a``, ``Qout_prev``,
 ``q_h``, ``rhogw``) hold the outputs the driver checks its result against
@@ -27,7 +28,8 @@ from typing import Dict, List, Tuple
 
 import numpy as np
 
-# extents: "R" = gall*kall*lall, "P" = gall_pl*kall*2, "R1"/"P1" = one level (K0)
+# extents: "R" = gall*kall*lall, "P" = gall_pl*kall*2, "R1"/"P1" = one level (K0),
+# "V" = the two pole regions alone (coef_diff_pl(vlink,3,2))
 _R, _P, _R1, _P1 = "R", "P", "R1", "P1"
 
 LAYOUTS: Dict[str, List[Tuple[str, str, int]]] = {
@@ -58,6 +60,10 @@ LAYOUTS: Dict[str, List[Tuple[str, str, int]]] = {
         ("RGAMH", _R, 1), ("RGAMH_pl", _P, 1), ("RGAM", _R, 1), ("RGAM_pl", _P, 1),
         ("GSGAM2H", _R, 1), ("GSGAM2H_pl", _P, 1),
     ],
+    "dyn_diffusion": [
+        ("dscl", _R, 1), ("dscl_pl", _P, 1), ("scl", _R, 1), ("scl_pl", _P, 1), ("kh", _R, 1), ("kh_pl", _P, 1),
+        ("coef_intp", _R1, 18), ("coef_intp_pl", _P1, 9), ("coef_diff", _R1, 18), ("coef_diff_pl", "V", 15),
+    ],
 }
 
 KERNELS = tuple(LAYOUTS)
@@ -70,7 +76,7 @@ def snapshot_name(kernel: str, rank: int = 0) -> str:
 
 def field_sizes(kernel: str, gall_1d: int, kall: int, lall: int, gall_pl: int = 6) -> List[Tuple[str, int]]:
     ext = {_R: gall_1d * gall_1d * kall * lall, _P: gall_pl * kall * 2,
-           _R1: gall_1d * gall_1d * lall, _P1: gall_pl * 2}
+           _R1: gall_1d * gall_1d * lall, _P1: gall_pl * 2, "V": 2}
     return [(name, ext[kind] * n) for name, kind, n in LAYOUTS[kernel]]
 
 

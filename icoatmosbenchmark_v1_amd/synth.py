@@ -331,6 +331,29 @@ def vi_inputs(spec: GridSpec, dt: float = 60.0, vgrid=None) -> Dict[str, np.ndar
     return out
 
 
+def diffusion_inputs(spec: GridSpec, geo: Dict[str, np.ndarray]) -> Dict[str, np.ndarray]:
+    """Scalar, diffusion coefficient and the stencil coefficients of
+    OPRT_diffusion (dyn_diffusion/src/mod_oprt.f90:41-60):
+    coef_intp(gall,3,3,2,lall) -> [lall][2][3][3][gall], coef_intp_pl(gall_pl,3,3,2),
+    coef_diff(gall,6,3,lall) -> [lall][3][6][gall], coef_diff_pl(5,3,2).
+    The coefficients are seeded values of the size 1/dx that OPRT_setup
+    gives them (a gradient of a triangle-vertex interpolation); parity does
+    not depend on their geometry."""
+    rng = np.random.default_rng(spec.seed * 7919 + 61)
+    lall, kall, gall, npl = spec.lall, spec.kall, spec.gall, spec.gall_pl
+    q2 = _tracer(geo["points"], rng)
+    prof = 0.5 + 0.5 * np.cos(np.linspace(0.0, np.pi, kall))
+    scl = q2[:, None, :] * prof[None, :, None] + 0.01 * rng.standard_normal((lall, kall, gall))
+    kh = 1e5 * (1.0 + 0.5 * rng.uniform(0.0, 1.0, size=(lall, kall, gall)))
+    inv_dx = 1.0 / 1e5
+    return dict(scl=scl, scl_pl=rng.uniform(0.0, 1.0, size=(2, kall, npl)),
+                kh=kh, kh_pl=1e5 * (1.0 + 0.5 * rng.uniform(0.0, 1.0, size=(2, kall, npl))),
+                coef_intp=inv_dx * rng.uniform(-1.0, 1.0, size=(lall, 2, 3, 3, gall)),
+                coef_intp_pl=inv_dx * rng.uniform(-1.0, 1.0, size=(2, 3, 3, npl)),
+                coef_diff=inv_dx * rng.uniform(-1.0, 1.0, size=(lall, 3, 6, gall)),
+                coef_diff_pl=inv_dx * rng.uniform(-1.0, 1.0, size=(2, 3, 5)))
+
+
 def make_case(spec: GridSpec, have_sgp=None, vgrid=None) -> Dict[str, Dict[str, np.ndarray]]:
     """All inputs of the four kernels for one grid."""
     geo = make_geometry(spec, have_sgp)

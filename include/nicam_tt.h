@@ -78,12 +78,14 @@ int nicam_ctx_set_pointer_mode(nicam_ctx ctx, int mode);
  *   NICAM_TUNE_VI_VARIANT   storage of the Thomas elimination factors:
  *                           1 = LDS, 2 = device work array, 3 = registers
  *                           (kall = 42 or 98, else automatic), 0 = automatic
+ *   NICAM_TUNE_DIFF_KPER    levels per work-group of nicam_oprt_diffusion
  * Results do not depend on these settings (every variant computes the same
  * floating-point operations in the same order). */
 enum nicam_tuning {
     NICAM_TUNE_HFLUX_KPER = 1,
     NICAM_TUNE_VLIM_KCHUNK = 3,
-    NICAM_TUNE_VI_VARIANT = 4
+    NICAM_TUNE_VI_VARIANT = 4,
+    NICAM_TUNE_DIFF_KPER = 5
 };
 int nicam_ctx_set_tuning(nicam_ctx ctx, int param, int value);
 
@@ -201,6 +203,23 @@ int nicam_vi_rhow_solver(nicam_ctx ctx,
                          const double *Sw, const double *Sw_pl,
                          const double *Spre, const double *Spre_pl,
                          double dt);
+
+/* ---- neighbouring operators on the same grid --------------------------- */
+
+/* OPRT_diffusion (dyn_diffusion/src/mod_oprt.f90:41-296).
+ *   out dscl(gall,kall,lall), dscl_pl(gall_pl,kall,2)
+ *   in  scl, kh (gall,kall,lall) and _pl
+ *   in  coef_intp(gall,3,3,2,lall), coef_intp_pl(gall_pl,3,3,2)
+ *   in  coef_diff(gall,6,3,lall),   coef_diff_pl(5,3,2)
+ * As in the reference, the Y and Z passes read the X component of vt as the
+ * first point of each pair (:170-214), and dscl_pl is zeroed when the context
+ * has no pole regions (dscl_pl may then be NULL). */
+int nicam_oprt_diffusion(nicam_ctx ctx,
+                         double *dscl, double *dscl_pl,
+                         const double *scl, const double *scl_pl,
+                         const double *kh, const double *kh_pl,
+                         const double *coef_intp, const double *coef_intp_pl,
+                         const double *coef_diff, const double *coef_diff_pl);
 
 /* Blocks until all work queued on the context's stream has finished. */
 int nicam_ctx_synchronize(nicam_ctx ctx);

@@ -21,7 +21,11 @@ import numpy as np
 HERE = Path(__file__).resolve().parent
 ORACLE_SO = HERE / "liboracle.so"
 REF_DIR = HERE / "_ref"
-REF_LIBS = {k: REF_DIR / f"libref_{k}.so" for k in ("flux", "hlim", "vlim", "vi")}
+REF_LIBS = {k: REF_DIR / f"libref_{k}.so" for k in ("flux", "hlim", "vlim", "vi", "diff")}
+
+
+# inputs of OPRT_diffusion in argument order (dyn_diffusion/src/mod_oprt.f90:41-46)
+DIFF_IN = ("scl", "scl_pl", "kh", "kh_pl", "coef_intp", "coef_intp_pl", "coef_diff", "coef_diff_pl")
 
 
 class OrcDims(ctypes.Structure):
@@ -73,6 +77,7 @@ class Oracle:
         L.orc_vertical_limiter_thuburn.argtypes = [POINTER(OrcDims)] + [P] * 8
         L.orc_vi_rhow_solver.argtypes = ([POINTER(OrcDims)] + [P] * 14 + [c_double] + [P] * 16 + [P] * 3 +
                                          [c_double, c_double, c_double, c_int])
+        L.orc_oprt_diffusion.argtypes = [POINTER(OrcDims), P] + [P] * 10
         L.orc_num_threads.restype = c_int
 
     def num_threads(self) -> int:
@@ -88,6 +93,15 @@ class Oracle:
         e, b = c_double(), c_double()
         self.lib.orc_const_setup(ctypes.byref(e), ctypes.byref(b))
         return e.value, b.value
+
+    def oprt_diffusion(self, spec, have_sgp, f, fill=0.0) -> Dict[str, np.ndarray]:
+        D = self.dims(spec)
+        out = dict(dscl=_init_out(spec.shape3, fill), dscl_pl=_init_out(spec.shape_pl, fill))
+        ins = {k: _f64(f[k]) for k in DIFF_IN}
+        sgp = np.ascontiguousarray(have_sgp, dtype=np.int32)
+        self.lib.orc_oprt_diffusion(ctypes.byref(D), sgp.ctypes.data, _p(out["dscl"]), _p(out["dscl_pl"]),
+                                    *[_p(ins[k]) for k in DIFF_IN])
+        return out
 
     def grd_setup(self, gz, gzh):
         kall = len(gz)
@@ -209,11 +223,22 @@ class RefLibs:
         self.vi.ref_vi_rhow_solver.argtypes = [c_int] * 5 + [P] * 14 + [c_double] + [P] * 16 + [P] * 3
         self.vi.ref_grd_setup.argtypes = [c_int, POINTER(c_char), c_int] + [P] * 10
         self.vi.ref_const_setup.argtypes = [POINTER(c_double), POINTER(c_double)]
+        self.diff = ctypes.CDLL(str(REF_LIBS["diff"]))
+        self.diff.ref_oprt_diffusion.argtypes = [c_int] * 5 + [P] * 11
 
     def const_setup(self):
         e, b = c_double(), c_double()
         self.vi.ref_const_setup(ctypes.byref(e), ctypes.byref(b))
         return e.value, b.value
+
+    def oprt_diffusion(self, spec, have_sgp, f, fill=0.0):
+        out = dict(dscl=_init_out(spec.shape3, fill), dscl_pl=_init_out(spec.shape_pl, fill))
+        ins = {k: _f64(f[k]) for k in DIFF_IN}
+        sgp = np.ascontiguousarray(have_sgp, dtype=np.int32)
+        _big_stack(self.diff.ref_oprt_diffusion, spec.gall_1d, spec.kall, spec.lall, spec.gall_pl,
+                   int(spec.have_pl), sgp.ctypes.data, _p(out["dscl"]), _p(out["dscl_pl"]),
+                   *[_p(ins[k]) for k in DIFF_IN])
+        return out
 
     def grd_setup(self, fname: str, kall: int):
         # The reference reads the file big-endian (sysdep/Makedef.*: -fconvert=big-endian).

@@ -651,3 +651,110 @@ void orc_vi_rhow_solver(const orc_dims *D,
     }
 #undef VI_COLUMN
 }
+
+/* ------------------------------------------------------------------------- */
+/* dyn_diffusion/src/mod_oprt.f90:41-296                                     */
+/* ------------------------------------------------------------------------- */
+void orc_oprt_diffusion(const orc_dims *D, const int *have_sgp,
+                        double *dscl, double *dscl_pl,
+                        const double *scl, const double *scl_pl,
+                        const double *kh, const double *kh_pl,
+                        const double *coef_intp, const double *coef_intp_pl,
+                        const double *coef_diff, const double *coef_diff_pl)
+{
+    const int iall = D->gall_1d, gall = iall * iall, kall = D->kall, lall = D->lall;
+    const int gall_pl = D->gall_pl;
+    /* 1-based flattened bounds (:85-93) */
+    const int gmin = (2 - 1) * iall + 2, gmax = (iall - 1 - 1) * iall + iall - 1;
+    const int gminm1 = (2 - 1 - 1) * iall + 2 - 1;
+#define CI(g, c, d, t, l) coef_intp[((((size_t)((l) - 1) * 2 + (t)) * 3 + ((d) - 1)) * 3 + ((c) - 1)) * gall + ((g) - 1)]
+#define CD(g, m, d, l) coef_diff[(((size_t)((l) - 1) * 3 + ((d) - 1)) * 6 + ((m) - 1)) * gall + ((g) - 1)]
+#define CI_PL(v, c, d, l) coef_intp_pl[((((size_t)((l) - 1)) * 3 + ((d) - 1)) * 3 + ((c) - 1)) * gall_pl + ((v) - 1)]
+#define CD_PL(m, d, l) coef_diff_pl[(((size_t)((l) - 1)) * 3 + ((d) - 1)) * 5 + ((m) - 1)]
+
+    for (int l = 1; l <= lall; ++l) {
+#pragma omp parallel for schedule(static)
+        for (int k = 1; k <= kall; ++k) {
+            double *vt = (double *)malloc(sizeof(double) * (size_t)gall * 6); /* vt(g, d, t) */
+#define VT(g, d, t) vt[((size_t)(t) * 3 + ((d) - 1)) * gall + ((g) - 1)]
+            for (int d = 1; d <= 3; ++d) {
+                for (int g = gminm1; g <= gmax; ++g) { /* :97-110 */
+                    const int ij = g, ip1j = g + 1, ip1jp1 = g + iall + 1;
+                    const double c1 = CI(g, 1, d, 0, l), c2 = CI(g, 2, d, 0, l), c3 = CI(g, 3, d, 0, l);
+                    VT(g, d, 0) = ((+2.0 * c1 - 1.0 * c2 - 1.0 * c3) * F3(scl, ij, k, l)
+                                 + (-1.0 * c1 + 2.0 * c2 - 1.0 * c3) * F3(scl, ip1j, k, l)
+                                 + (-1.0 * c1 - 1.0 * c2 + 2.0 * c3) * F3(scl, ip1jp1, k, l)) / 3.0;
+                }
+                for (int g = gminm1; g <= gmax; ++g) { /* :112-125 */
+                    const int ij = g, ip1jp1 = g + iall + 1, ijp1 = g + iall;
+                    const double c1 = CI(g, 1, d, 1, l), c2 = CI(g, 2, d, 1, l), c3 = CI(g, 3, d, 1, l);
+                    VT(g, d, 1) = ((+2.0 * c1 - 1.0 * c2 - 1.0 * c3) * F3(scl, ij, k, l)
+                                 + (-1.0 * c1 + 2.0 * c2 - 1.0 * c3) * F3(scl, ip1jp1, k, l)
+                                 + (-1.0 * c1 - 1.0 * c2 + 2.0 * c3) * F3(scl, ijp1, k, l)) / 3.0;
+                }
+            }
+            if (have_sgp[l - 1]) /* :128-132 */
+                for (int d = 1; d <= 3; ++d) VT(gminm1, d, 0) = VT(gminm1 + 1, d, 1);
+            for (int g = 1; g <= gmin - 1; ++g) F3(dscl, g, k, l) = 0.0;
+            /* :138-216: the three passes; the Y and Z passes read the X
+             * component at the first point of each pair, as the reference does */
+            for (int d = 1; d <= 3; ++d)
+                for (int g = gmin; g <= gmax; ++g) {
+                    const int ij = g, ip1j = g + 1, ip1jp1 = g + iall + 1, ijp1 = g + iall;
+                    const int im1j = g - 1, im1jm1 = g - iall - 1, ijm1 = g - iall;
+                    const double kf1 = 0.5 * (F3(kh, ij, k, l) + F3(kh, ip1jp1, k, l));
+                    const double kf2 = 0.5 * (F3(kh, ij, k, l) + F3(kh, ijp1, k, l));
+                    const double kf3 = 0.5 * (F3(kh, im1j, k, l) + F3(kh, ij, k, l));
+                    const double kf4 = 0.5 * (F3(kh, im1jm1, k, l) + F3(kh, ij, k, l));
+                    const double kf5 = 0.5 * (F3(kh, ijm1, k, l) + F3(kh, ij, k, l));
+                    const double kf6 = 0.5 * (F3(kh, ij, k, l) + F3(kh, ip1j, k, l));
+                    const double s = kf1 * CD(g, 1, d, l) * (VT(ij, 1, 0) + VT(ij, d, 1))
+                                   + kf2 * CD(g, 2, d, l) * (VT(ij, 1, 1) + VT(im1j, d, 0))
+                                   + kf3 * CD(g, 3, d, l) * (VT(im1j, 1, 0) + VT(im1jm1, d, 1))
+                                   + kf4 * CD(g, 4, d, l) * (VT(im1jm1, 1, 1) + VT(im1jm1, d, 0))
+                                   + kf5 * CD(g, 5, d, l) * (VT(im1jm1, 1, 0) + VT(ijm1, d, 1))
+                                   + kf6 * CD(g, 6, d, l) * (VT(ijm1, 1, 1) + VT(ij, d, 0));
+                    if (d == 1) F3(dscl, g, k, l) = s;
+                    else F3(dscl, g, k, l) = F3(dscl, g, k, l) + s;
+                }
+            for (int g = gmax + 1; g <= gall; ++g) F3(dscl, g, k, l) = 0.0;
+#undef VT
+            free(vt);
+        }
+    }
+
+    if (D->have_pl) { /* :220-291 */
+        const int n = GSLF_PL, gmin_pl = GMIN_PL, gmax_pl = gall_pl;
+        double vt_pl[16][3];
+        for (int l = 1; l <= LALL_PL; ++l)
+            for (int k = 1; k <= kall; ++k) {
+                for (int d = 1; d <= 3; ++d)
+                    for (int v = gmin_pl; v <= gmax_pl; ++v) {
+                        const int ij = v;
+                        int ijp1 = v + 1;
+                        if (ijp1 == gmax_pl + 1) ijp1 = gmin_pl;
+                        const double c1 = CI_PL(v, 1, d, l), c2 = CI_PL(v, 2, d, l), c3 = CI_PL(v, 3, d, l);
+                        vt_pl[ij][d - 1] = ((+2.0 * c1 - 1.0 * c2 - 1.0 * c3) * P3(scl_pl, n, k, l)
+                                          + (-1.0 * c1 + 2.0 * c2 - 1.0 * c3) * P3(scl_pl, ij, k, l)
+                                          + (-1.0 * c1 - 1.0 * c2 + 2.0 * c3) * P3(scl_pl, ijp1, k, l)) / 3.0;
+                    }
+                for (int v = 1; v <= gall_pl; ++v) P3(dscl_pl, v, k, l) = 0.0;
+                for (int v = gmin_pl; v <= gmax_pl; ++v) {
+                    const int ij = v;
+                    int ijm1 = v - 1;
+                    if (ijm1 == gmin_pl - 1) ijm1 = gmax_pl;
+                    P3(dscl_pl, n, k, l) = P3(dscl_pl, n, k, l)
+                        + (CD_PL(v - 1, 1, l) * (vt_pl[ijm1][0] + vt_pl[ij][0])
+                         + CD_PL(v - 1, 2, l) * (vt_pl[ijm1][1] + vt_pl[ij][1])
+                         + CD_PL(v - 1, 3, l) * (vt_pl[ijm1][2] + vt_pl[ij][2]))
+                        * 0.5 * (P3(kh_pl, n, k, l) + P3(kh_pl, ij, k, l));
+                }
+            }
+    } else {
+        for (size_t i = 0; i < (size_t)gall_pl * kall * LALL_PL; ++i) dscl_pl[i] = 0.0;
+    }
+#undef CI
+#undef CD
+#undef CI_PL
+#undef CD_PL
+}

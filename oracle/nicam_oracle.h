@@ -96,6 +96,16 @@ void orc_vi_rhow_solver(const orc_dims *D,
                         const double *rdgzh, const double *afact, const double *bfact,
                         double grav, double rdry, double cvdry, int non_hydro_alpha);
 
+/* dyn_diffusion/src/mod_oprt.f90:41-296 (OPRT_diffusion).
+ *   coef_intp(gall,3,3,2,lall)  coef_intp_pl(gall_pl,3,3,2)
+ *   coef_diff(gall,6,3,lall)    coef_diff_pl(5,3,2) */
+void orc_oprt_diffusion(const orc_dims *D, const int *have_sgp,
+                        double *dscl, double *dscl_pl,
+                        const double *scl, const double *scl_pl,
+                        const double *kh, const double *kh_pl,
+                        const double *coef_intp, const double *coef_intp_pl,
+                        const double *coef_diff, const double *coef_diff_pl);
+
 /* Number of OpenMP threads the oracle runs with (1 when built without it). */
 int orc_num_threads(void);
 

@@ -33,7 +33,7 @@ REPO = Path(__file__).resolve().parent.parent
 REF = REPO / "oracle" / "_ref"
 VGRID = REPO / "tests" / "golden" / "vgrid40_600m_24km.dat"
 KERNELS = {"flux": "dyn_horiz_adv_flux", "hlim": "dyn_horiz_adv_limiter", "vlim": "dyn_vert_adv_limiter",
-           "vi": "dyn_vi_rhow_solver"}
+           "vi": "dyn_vi_rhow_solver", "diff": "dyn_diffusion"}
 N1, KALL, LALL = 130, 42, 1  # problem_size.inc:28-45 of every driver
 CHECK = re.compile(r"\+check\[(.{16})\] max=(.*),min=(.*),sum=(.*)$")
 
@@ -92,6 +92,9 @@ def make_snapshot(short, oracle, workdir: Path):
         f = case["vlim"]
         out = oracle.vertical_limiter_thuburn(spec, f)
         fields = {**f, **out, "q_h_prev": f["q_h"], "q_h_prev_pl": f["q_h_pl"]}
+    elif short == "diff":
+        f = synth.diffusion_inputs(spec, geo)
+        fields = {**f, **oracle.oprt_diffusion(spec, [1], f)}
     else:
         f = case["vi"]
         g = oracle.grd_setup(gz, gzh)
