@@ -171,7 +171,8 @@ int nicam_ctx_destroy(nicam_ctx c)
                       &c->GMTR_t_pl, &c->GMTR_a, &c->GMTR_a_pl, &c->RGAM, &c->RGAM_pl,
                       &c->RGAMH, &c->RGAMH_pl, &c->RGSGAM2, &c->RGSGAM2_pl, &c->RGSGAM2H,
                       &c->RGSGAM2H_pl, &c->GSGAM2H, &c->GSGAM2H_pl, &c->rdgzh, &c->afact,
-                      &c->bfact, &c->Mc, &c->Mc_pl, &c->Ml, &c->Ml_pl, &c->Mu, &c->Mu_pl, &c->vi_gamma};
+                      &c->bfact, &c->Mc, &c->Mc_pl, &c->Ml, &c->Ml_pl, &c->Mu, &c->Mu_pl, &c->vi_gamma,
+                      &c->RGSQRTH, &c->RGSQRTH_pl, &c->C2WfactGz, &c->C2WfactGz_pl, &c->rdgz};
     for (double **p : all) dev_free(*p);
     for (void *p : c->stage)
         if (p) (void)hipFree(p);
@@ -206,6 +207,7 @@ int nicam_ctx_set_tuning(nicam_ctx c, int param, int value)
     case NICAM_TUNE_HFLUX_KPER: c->tune.hflux_kper = value; break;
     case NICAM_TUNE_VLIM_KCHUNK: c->tune.vlim_kchunk = value; break;
     case NICAM_TUNE_DIFF_KPER: c->tune.diff_kper = value; break;
+    case NICAM_TUNE_DDMP_KPER: c->tune.ddmp_kper = value; break;
     case NICAM_TUNE_VI_VARIANT:
         if (value > 3) return fail(NICAM_ERR_ARG, "NICAM_TUNE_VI_VARIANT must be 0..3");
         c->tune.vi_variant = value;
@@ -311,14 +313,31 @@ int nicam_grd_setup(nicam_ctx c, const double *gz, const double *gzh, double *rd
     for (int k = kmin; k <= kmax + 1; ++k) afact[k] = (gzh[k] - gz[k - 1]) / (gz[k] - gz[k - 1]);
     afact[kmin - 1] = 1.0;
     for (int k = 0; k < kall; ++k) bfact[k] = 1.0 - afact[k];
+    // mod_misc.f90:481-484 (cell-centre intervals) for GRD_rdgz
+    std::vector<double> dgz(kall), rdgz(kall);
+    for (int k = kmin - 1; k <= kmax; ++k) dgz[k] = gzh[k + 1] - gzh[k];
+    dgz[kmax + 1] = dgz[kmax];
+    for (int k = 0; k < kall; ++k) rdgz[k] = 1.0 / dgz[k];
     for (int k = 0; k < kall; ++k) {
-        if (!std::isfinite(rdgzh[k]) || !std::isfinite(afact[k]))
+        if (!std::isfinite(rdgzh[k]) || !std::isfinite(afact[k]) || !std::isfinite(rdgz[k]))
             return fail(NICAM_ERR_ARG, "vertical grid has coincident levels");
     }
+    NICAM_TRY(nicam_set_rdgz(c, rdgz.data()));
     if (rdgzh_out) std::memcpy(rdgzh_out, rdgzh.data(), sizeof(double) * kall);
     if (afact_out) std::memcpy(afact_out, afact.data(), sizeof(double) * kall);
     if (bfact_out) std::memcpy(bfact_out, bfact.data(), sizeof(double) * kall);
     return nicam_set_vgrid_factors(c, rdgzh.data(), afact.data(), bfact.data());
+}
+
+int nicam_set_rdgz(nicam_ctx c, const double *rdgz)
+{
+    NICAM_TRY(check_ctx(c));
+    NICAM_TRY(need(rdgz, "rdgz"));
+    NICAM_TRY(dev_alloc(c, &c->rdgz, c->d.kall));
+    NICAM_TRY(copy_in(c, c->rdgz, rdgz, c->d.kall));
+    NICAM_HIP(hipStreamSynchronize(c->stream));
+    c->have_rdgz = true;
+    return NICAM_OK;
 }
 
 int nicam_set_vgrid_factors(nicam_ctx c, const double *rdgzh, const double *afact, const double *bfact)
@@ -405,6 +424,27 @@ int nicam_set_vmetrics(nicam_ctx c, const double *RGAM, const double *RGAM_pl, c
     NICAM_TRY(set_pair(c, &c->GSGAM2H, &c->GSGAM2H_pl, GSGAM2H, GSGAM2H_pl, "VMTR_GSGAM2H"));
     NICAM_HIP(hipStreamSynchronize(c->stream));
     c->have_vmetrics = true;
+    return NICAM_OK;
+}
+
+int nicam_set_vmetrics3d(nicam_ctx c, const double *RGSQRTH, const double *RGSQRTH_pl, const double *RGAM,
+                         const double *RGAM_pl, const double *RGAMH, const double *RGAMH_pl, const double *C2WfactGz,
+                         const double *C2WfactGz_pl)
+{
+    NICAM_TRY(check_ctx(c));
+    NICAM_TRY(set_pair(c, &c->RGSQRTH, &c->RGSQRTH_pl, RGSQRTH, RGSQRTH_pl, "VMTR_RGSQRTH"));
+    NICAM_TRY(set_pair(c, &c->RGAM, &c->RGAM_pl, RGAM, RGAM_pl, "VMTR_RGAM"));
+    NICAM_TRY(set_pair(c, &c->RGAMH, &c->RGAMH_pl, RGAMH, RGAMH_pl, "VMTR_RGAMH"));
+    NICAM_TRY(need(C2WfactGz, "VMTR_C2WfactGz"));
+    NICAM_TRY(dev_alloc(c, &c->C2WfactGz, c->d.nreg() * 6));
+    NICAM_TRY(copy_in(c, c->C2WfactGz, C2WfactGz, c->d.nreg() * 6));
+    if (c->d.have_pl) {
+        NICAM_TRY(need(C2WfactGz_pl, "VMTR_C2WfactGz_pl"));
+        NICAM_TRY(dev_alloc(c, &c->C2WfactGz_pl, c->d.npl() * 6));
+        NICAM_TRY(copy_in(c, c->C2WfactGz_pl, C2WfactGz_pl, c->d.npl() * 6));
+    }
+    NICAM_HIP(hipStreamSynchronize(c->stream));
+    c->have_vmetrics3d = true;
     return NICAM_OK;
 }
 
@@ -580,6 +620,55 @@ int nicam_oprt_diffusion(nicam_ctx c, double *dscl, double *dscl_pl, const doubl
     NICAM_TRY(stage_in(c, a, v));
     auto D = [&](int i) { return static_cast<double *>(v[i]); };
     NICAM_TRY(launch_diffusion(c, D(0), D(1), D(2), D(3), D(4), D(5), D(6), D(7), D(8), D(9)));
+    return stage_out(c, a, v);
+}
+
+int nicam_oprt3d_divdamp(nicam_ctx c, double *ddivdx, double *ddivdx_pl, double *ddivdy, double *ddivdy_pl,
+                         double *ddivdz, double *ddivdz_pl, const double *rhogvx, const double *rhogvx_pl,
+                         const double *rhogvy, const double *rhogvy_pl, const double *rhogvz,
+                         const double *rhogvz_pl, const double *rhogw, const double *rhogw_pl,
+                         const double *coef_intp, const double *coef_intp_pl, const double *coef_diff,
+                         const double *coef_diff_pl)
+{
+    NICAM_TRY(check_ctx(c));
+    if (!c->have_vmetrics3d) return fail(NICAM_ERR_STATE, "vertical metrics not set (nicam_set_vmetrics3d)");
+    if (!c->have_rdgz) return fail(NICAM_ERR_STATE, "GRD_rdgz not set (nicam_grd_setup or nicam_set_rdgz)");
+    const void *req[] = {ddivdx, ddivdy, ddivdz, rhogvx, rhogvy, rhogvz, rhogw, coef_intp, coef_diff};
+    const char *nm[] = {"ddivdx", "ddivdy", "ddivdz", "rhogvx", "rhogvy", "rhogvz", "rhogw", "coef_intp", "coef_diff"};
+    for (int i = 0; i < 9; ++i) NICAM_TRY(need(req[i], nm[i]));
+    if (c->d.have_pl) {
+        const void *rp[] = {ddivdx_pl, ddivdy_pl, ddivdz_pl, rhogvx_pl, rhogvy_pl, rhogvz_pl, rhogw_pl,
+                            coef_intp_pl, coef_diff_pl};
+        for (int i = 0; i < 9; ++i) NICAM_TRY(need(rp[i], "pole array"));
+    }
+    if ((ddivdx_pl == nullptr) != (ddivdy_pl == nullptr) || (ddivdx_pl == nullptr) != (ddivdz_pl == nullptr))
+        return fail(NICAM_ERR_ARG, "give all three or none of ddivdx_pl, ddivdy_pl, ddivdz_pl");
+    auto run = [&](double *const *o, double *const *opl, const double *const *v, const double *const *vpl,
+                   const double *w, const double *wpl, const double *ci, const double *cipl, const double *cd,
+                   const double *cdpl) {
+        return launch_divdamp(c, o, opl, v, vpl, w, wpl, ci, cipl, cd, cdpl, c->RGSQRTH, c->RGSQRTH_pl, c->RGAM,
+                              c->RGAM_pl, c->RGAMH, c->RGAMH_pl, c->C2WfactGz, c->C2WfactGz_pl, c->rdgz);
+    };
+    if (c->ptr_mode == NICAM_PTR_DEVICE) {
+        double *o[3] = {ddivdx, ddivdy, ddivdz}, *opl[3] = {ddivdx_pl, ddivdy_pl, ddivdz_pl};
+        const double *v[3] = {rhogvx, rhogvy, rhogvz}, *vpl[3] = {PL(rhogvx_pl), PL(rhogvy_pl), PL(rhogvz_pl)};
+        return run(o, opl, v, vpl, rhogw, PL(rhogw_pl), coef_intp, PL(coef_intp_pl), coef_diff, PL(coef_diff_pl));
+    }
+    const size_t R = c->d.nreg() * 8, P = c->d.npl() * 8;
+    const size_t CI = (size_t)c->d.gall * c->d.lall * 18 * 8, CIP = (size_t)c->d.gall_pl * 18 * 8;
+    const size_t CD = CI, CDP = (size_t)5 * 3 * 2 * 8;
+    std::vector<HostArg> a = {{ddivdx, R, true},          {ddivdx_pl, P, true},           {ddivdy, R, true},
+                              {ddivdy_pl, P, true},       {ddivdz, R, true},              {ddivdz_pl, P, true},
+                              {rhogvx, R, false},         {PL(rhogvx_pl), P, false},      {rhogvy, R, false},
+                              {PL(rhogvy_pl), P, false},  {rhogvz, R, false},             {PL(rhogvz_pl), P, false},
+                              {rhogw, R, false},          {PL(rhogw_pl), P, false},       {coef_intp, CI, false},
+                              {PL(coef_intp_pl), CIP, false}, {coef_diff, CD, false},     {PL(coef_diff_pl), CDP, false}};
+    std::vector<void *> v;
+    NICAM_TRY(stage_in(c, a, v));
+    auto D = [&](int i) { return static_cast<double *>(v[i]); };
+    double *o[3] = {D(0), D(2), D(4)}, *opl[3] = {D(1), D(3), D(5)};
+    const double *vv[3] = {D(6), D(8), D(10)}, *vpl[3] = {D(7), D(9), D(11)};
+    NICAM_TRY(run(o, opl, vv, vpl, D(12), D(13), D(14), D(15), D(16), D(17)));
     return stage_out(c, a, v);
 }
 

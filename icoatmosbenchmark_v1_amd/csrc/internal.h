@@ -41,6 +41,7 @@ struct nicam_tune {
     int vlim_kchunk = 0;  // levels per work-item in vertical_limiter_thuburn
     int vi_variant = 0;   // elimination-factor storage in vi_rhow_solver
     int diff_kper = 0;    // levels per work-group in OPRT_diffusion
+    int ddmp_kper = 0;    // levels per work-group in OPRT3D_divdamp
 };
 
 struct nicam_ctx_s {
@@ -71,6 +72,12 @@ struct nicam_ctx_s {
     double *RGSGAM2H = nullptr, *RGSGAM2H_pl = nullptr;
     double *GSGAM2H = nullptr, *GSGAM2H_pl = nullptr;
     bool have_vmetrics = false;
+    // vertical metrics of OPRT3D_divdamp (RGAM/RGAMH above are shared)
+    double *RGSQRTH = nullptr, *RGSQRTH_pl = nullptr;
+    double *C2WfactGz = nullptr, *C2WfactGz_pl = nullptr;  // (gall,kall,6,lall), (gall_pl,kall,6,2)
+    bool have_vmetrics3d = false;
+    double *rdgz = nullptr;  // GRD_rdgz (kall)
+    bool have_rdgz = false;
 
     // vertical grid factors (kall each)
     double *rdgzh = nullptr, *afact = nullptr, *bfact = nullptr;
@@ -169,6 +176,12 @@ int launch_halo_fill(nicam_ctx c, double *var, double *var_pl, int kmax, int nva
 int launch_diffusion(nicam_ctx c, double *dscl, double *dscl_pl, const double *scl, const double *scl_pl,
                      const double *kh, const double *kh_pl, const double *coef_intp, const double *coef_intp_pl,
                      const double *coef_diff, const double *coef_diff_pl);
+int launch_divdamp(nicam_ctx c, double *const out[3], double *const out_pl[3], const double *const v[3],
+                   const double *const v_pl[3], const double *w, const double *w_pl, const double *coef_intp,
+                   const double *coef_intp_pl, const double *coef_diff, const double *coef_diff_pl,
+                   const double *rgsqrth, const double *rgsqrth_pl, const double *rgam, const double *rgam_pl,
+                   const double *rgamh, const double *rgamh_pl, const double *c2w, const double *c2w_pl,
+                   const double *rdgz_dev);
 int launch_vi(nicam_ctx c, double *w, double *w_pl, const double *w0, const double *w0_pl,
               const double *p0, const double *p0_pl, const double *r0, const double *r0_pl,
               const double *Sr, const double *Sr_pl, const double *Sw, const double *Sw_pl,

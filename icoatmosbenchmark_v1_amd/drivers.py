@@ -116,3 +116,23 @@ def run_diffusion(ctx: Context, spec: GridSpec, f: Dict[str, np.ndarray], fill: 
     a = _prep(ctx, {**outs, **{k: f[k] for k in DIFF_IN}}, device)
     ctx.oprt_diffusion(a["dscl"], a["dscl_pl"], *[a[k] for k in DIFF_IN])
     return _back(ctx, {k: a[k] for k in outs})
+
+
+DDMP_IN = ("rhogvx", "rhogvx_pl", "rhogvy", "rhogvy_pl", "rhogvz", "rhogvz_pl", "rhogw", "rhogw_pl",
+           "coef_intp", "coef_intp_pl", "coef_diff", "coef_diff_pl")
+DDMP_OUT = ("ddivdx", "ddivdx_pl", "ddivdy", "ddivdy_pl", "ddivdz", "ddivdz_pl")
+
+
+def setup_divdamp(ctx: Context, f: Dict[str, np.ndarray]) -> None:
+    """The module state OPRT3D_divdamp reads: VMTR_* and GRD_rdgz."""
+    ctx.set_vmetrics3d(f["RGSQRTH"], f["RGSQRTH_pl"], f["RGAM"], f["RGAM_pl"], f["RGAMH"], f["RGAMH_pl"],
+                       f["C2WfactGz"], f["C2WfactGz_pl"])
+    ctx.set_rdgz(f["rdgz"])
+
+
+def run_divdamp(ctx: Context, spec: GridSpec, f: Dict[str, np.ndarray], fill: float = 0.0,
+                device="cuda:0") -> Dict[str, np.ndarray]:
+    outs = {k: np.full(spec.shape_pl if k.endswith("_pl") else spec.shape3, fill) for k in DDMP_OUT}
+    a = _prep(ctx, {**outs, **{k: f[k] for k in DDMP_IN}}, device)
+    ctx.oprt3d_divdamp(*[a[k] for k in DDMP_OUT], *[a[k] for k in DDMP_IN])
+    return _back(ctx, {k: a[k] for k in outs})

@@ -23,6 +23,7 @@ module nicam_tt_f
   integer(c_int), parameter, public :: NICAM_TUNE_VLIM_KCHUNK = 3
   integer(c_int), parameter, public :: NICAM_TUNE_VI_VARIANT  = 4
   integer(c_int), parameter, public :: NICAM_TUNE_DIFF_KPER   = 5
+  integer(c_int), parameter, public :: NICAM_TUNE_DDMP_KPER   = 6
 
   public :: nicam_check
 
@@ -238,6 +239,32 @@ module nicam_tt_f
        integer(c_int) :: nicam_grid_icosahedral
      end function nicam_grid_icosahedral
 
+     function nicam_set_rdgz(ctx, rdgz) bind(C, name='nicam_set_rdgz')
+       import :: c_ptr, c_int, c_double
+       type(c_ptr), value :: ctx
+       real(c_double), intent(in) :: rdgz(*)
+       integer(c_int) :: nicam_set_rdgz
+     end function nicam_set_rdgz
+
+     function nicam_set_vmetrics3d(ctx, RGSQRTH, RGSQRTH_pl, RGAM, RGAM_pl, RGAMH, RGAMH_pl, &
+          C2WfactGz, C2WfactGz_pl) bind(C, name='nicam_set_vmetrics3d')
+       import :: c_ptr, c_int, c_double
+       type(c_ptr), value :: ctx
+       real(c_double), intent(in) :: RGSQRTH(*), RGSQRTH_pl(*), RGAM(*), RGAM_pl(*), RGAMH(*), RGAMH_pl(*)
+       real(c_double), intent(in) :: C2WfactGz(*), C2WfactGz_pl(*)
+       integer(c_int) :: nicam_set_vmetrics3d
+     end function nicam_set_vmetrics3d
+
+     function nicam_oprt3d_divdamp(ctx, ddivdx, ddivdx_pl, ddivdy, ddivdy_pl, ddivdz, ddivdz_pl, &
+          rhogvx, rhogvx_pl, rhogvy, rhogvy_pl, rhogvz, rhogvz_pl, rhogw, rhogw_pl, coef_intp, coef_intp_pl, &
+          coef_diff, coef_diff_pl) bind(C, name='nicam_oprt3d_divdamp')
+       import :: c_ptr, c_int
+       type(c_ptr), value :: ctx, ddivdx, ddivdx_pl, ddivdy, ddivdy_pl, ddivdz, ddivdz_pl
+       type(c_ptr), value :: rhogvx, rhogvx_pl, rhogvy, rhogvy_pl, rhogvz, rhogvz_pl, rhogw, rhogw_pl
+       type(c_ptr), value :: coef_intp, coef_intp_pl, coef_diff, coef_diff_pl
+       integer(c_int) :: nicam_oprt3d_divdamp
+     end function nicam_oprt3d_divdamp
+
      function c_strlen(s) bind(C, name='strlen')
        import :: c_ptr, c_size_t
        type(c_ptr), value :: s
@@ -252,7 +279,7 @@ module nicam_tt_f
   public :: nicam_horizontal_limiter_thuburn, nicam_vertical_limiter_thuburn, nicam_vi_rhow_solver_putcoef
   public :: nicam_vi_rhow_solver, nicam_ctx_synchronize, nicam_halo_map, nicam_pole_fill_map
   public :: nicam_comm_setup, nicam_comm_data_transfer, nicam_comm_var, nicam_oprt_diffusion
-  public :: nicam_grid_icosahedral
+  public :: nicam_grid_icosahedral, nicam_set_rdgz, nicam_set_vmetrics3d, nicam_oprt3d_divdamp
 
 contains
 

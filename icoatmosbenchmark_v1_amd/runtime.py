@@ -42,6 +42,7 @@ TUNE_HFLUX_KPER = 1
 TUNE_VLIM_KCHUNK = 3
 TUNE_VI_VARIANT = 4
 TUNE_DIFF_KPER = 5
+TUNE_DDMP_KPER = 6
 
 STATUS_NAMES = {
     NICAM_ERR_ARG: "NICAM_ERR_ARG",
@@ -83,6 +84,9 @@ SIGNATURES = {
     "nicam_comm_var": (c_int, [_P, _P, _P, c_int, c_int]),
     "nicam_grid_icosahedral": (c_int, [c_int, c_int, c_double, _P, _P]),
     "nicam_oprt_diffusion": (c_int, [_P] + [_P] * 10),
+    "nicam_set_rdgz": (c_int, [_P, _P]),
+    "nicam_set_vmetrics3d": (c_int, [_P] + [_P] * 8),
+    "nicam_oprt3d_divdamp": (c_int, [_P] + [_P] * 18),
 }
 
 
@@ -316,6 +320,21 @@ class Context:
         _check(self.lib.nicam_set_vmetrics(self.h, *ptrs), "nicam_set_vmetrics")
         self._keep.clear()
 
+    def set_rdgz(self, rdgz) -> None:
+        a = np.ascontiguousarray(rdgz, dtype=np.float64)
+        if a.size != self.kall:
+            raise ValueError("rdgz needs kall elements")
+        _check(self.lib.nicam_set_rdgz(self.h, a.ctypes.data), "nicam_set_rdgz")
+
+    def set_vmetrics3d(self, RGSQRTH, RGSQRTH_pl, RGAM, RGAM_pl, RGAMH, RGAMH_pl, C2WfactGz, C2WfactGz_pl) -> None:
+        ptrs = self._pairs([RGSQRTH, RGSQRTH_pl, RGAM, RGAM_pl, RGAMH, RGAMH_pl],
+                           ["RGSQRTH", "RGSQRTH_pl", "RGAM", "RGAM_pl", "RGAMH", "RGAMH_pl"])
+        c = _setup_ptr(C2WfactGz, "C2WfactGz", 6 * self.nreg)
+        cp = _setup_ptr(C2WfactGz_pl, "C2WfactGz_pl", 6 * self.npl) if self.have_pl else None
+        self._keep += [c[1]] + ([cp[1]] if cp else [])
+        _check(self.lib.nicam_set_vmetrics3d(self.h, *ptrs, c[0], cp[0] if cp else None), "nicam_set_vmetrics3d")
+        self._keep.clear()
+
     def vi_rhow_solver_putcoef(self, Mc, Mc_pl, Ml, Ml_pl, Mu, Mu_pl) -> None:
         ptrs = self._pairs([Mc, Mc_pl, Ml, Ml_pl, Mu, Mu_pl], ["Mc", "Mc_pl", "Ml", "Ml_pl", "Mu", "Mu_pl"])
         _check(self.lib.nicam_vi_rhow_solver_putcoef(self.h, *ptrs), "nicam_vi_rhow_solver_putcoef")
@@ -366,6 +385,21 @@ class Context:
                          (coef_intp, "coef_intp", 18 * G), (coef_intp_pl, "coef_intp_pl", 18 * self.gall_pl),
                          (coef_diff, "coef_diff", 18 * G), (coef_diff_pl, "coef_diff_pl", 30)])
         _check(self.lib.nicam_oprt_diffusion(self.h, *a), "nicam_oprt_diffusion")
+
+    def oprt3d_divdamp(self, ddivdx, ddivdx_pl, ddivdy, ddivdy_pl, ddivdz, ddivdz_pl, rhogvx, rhogvx_pl,
+                       rhogvy, rhogvy_pl, rhogvz, rhogvz_pl, rhogw, rhogw_pl, coef_intp, coef_intp_pl,
+                       coef_diff, coef_diff_pl) -> None:
+        """OPRT3D_divdamp (dyn_divdamp/src/mod_oprt3d.f90:41-402)."""
+        R, P, G = self.nreg, self.npl, self.gall * self.lall
+        a = [_ptr(x, self.mode, nm, n) for x, nm, n in
+             [(ddivdx, "ddivdx", R), (ddivdx_pl, "ddivdx_pl", P), (ddivdy, "ddivdy", R), (ddivdy_pl, "ddivdy_pl", P),
+              (ddivdz, "ddivdz", R), (ddivdz_pl, "ddivdz_pl", P)]]
+        a += self._args([(rhogvx, "rhogvx", R), (rhogvx_pl, "rhogvx_pl", P), (rhogvy, "rhogvy", R),
+                         (rhogvy_pl, "rhogvy_pl", P), (rhogvz, "rhogvz", R), (rhogvz_pl, "rhogvz_pl", P),
+                         (rhogw, "rhogw", R), (rhogw_pl, "rhogw_pl", P), (coef_intp, "coef_intp", 18 * G),
+                         (coef_intp_pl, "coef_intp_pl", 18 * self.gall_pl), (coef_diff, "coef_diff", 18 * G),
+                         (coef_diff_pl, "coef_diff_pl", 30)])
+        _check(self.lib.nicam_oprt3d_divdamp(self.h, *a), "nicam_oprt3d_divdamp")
 
     def vi_rhow_solver(self, rhogw, rhogw_pl, rhogw0, rhogw0_pl, preg0, preg0_pl, rhog0, rhog0_pl,
                        Srho, Srho_pl, Sw, Sw_pl, Spre, Spre_pl, dt: float) -> None:

@@ -11,6 +11,7 @@ order the driver reads them (paths relative to kernels/NICAM/):
 * dyn_vert_adv_limiter/src/main.f90:96-105
 * dyn_vi_rhow_solver/src/main.f90:198-229
 * dyn_diffusion/src/main.f90:96-105
+* dyn_divdamp/src/main.f90:181-206
 
 Fields named like the kernel's outputs (``flx_h``, ``q_a``, ``Qout_prev``,
 ``q_h``, ``rhogw``) hold the outputs the driver checks its result against
@@ -63,6 +64,15 @@ LAYOUTS: Dict[str, List[Tuple[str, str, int]]] = {
     "dyn_diffusion": [
         ("dscl", _R, 1), ("dscl_pl", _P, 1), ("scl", _R, 1), ("scl_pl", _P, 1), ("kh", _R, 1), ("kh_pl", _P, 1),
         ("coef_intp", _R1, 18), ("coef_intp_pl", _P1, 9), ("coef_diff", _R1, 18), ("coef_diff_pl", "V", 15),
+    ],
+    "dyn_divdamp": [
+        ("ddivdx", _R, 1), ("ddivdx_pl", _P, 1), ("ddivdy", _R, 1), ("ddivdy_pl", _P, 1),
+        ("ddivdz", _R, 1), ("ddivdz_pl", _P, 1), ("rhogvx", _R, 1), ("rhogvx_pl", _P, 1),
+        ("rhogvy", _R, 1), ("rhogvy_pl", _P, 1), ("rhogvz", _R, 1), ("rhogvz_pl", _P, 1),
+        ("rhogw", _R, 1), ("rhogw_pl", _P, 1),
+        ("coef_intp", _R1, 18), ("coef_intp_pl", _P1, 9), ("coef_diff", _R1, 18), ("coef_diff_pl", "V", 15),
+        ("RGSQRTH", _R, 1), ("RGSQRTH_pl", _P, 1), ("RGAM", _R, 1), ("RGAM_pl", _P, 1),
+        ("RGAMH", _R, 1), ("RGAMH_pl", _P, 1), ("C2WfactGz", _R, 6), ("C2WfactGz_pl", _P, 6),
     ],
 }
 

@@ -354,6 +354,42 @@ def diffusion_inputs(spec: GridSpec, geo: Dict[str, np.ndarray]) -> Dict[str, np
                 coef_diff_pl=inv_dx * rng.uniform(-1.0, 1.0, size=(2, 3, 5)))
 
 
+def divdamp_inputs(spec: GridSpec, geo: Dict[str, np.ndarray]) -> Dict[str, np.ndarray]:
+    """Momentum, the stencil coefficients and the vertical metrics that
+    OPRT3D_divdamp reads (dyn_divdamp/src/mod_oprt3d.f90:41-84):
+    VMTR_C2WfactGz(gall,kall,6,lall) -> [lall][6][kall][gall] and
+    _pl(gall_pl,kall,6,2), GRD_rdgz(kall), the rest as diffusion_inputs."""
+    rng = np.random.default_rng(spec.seed * 7919 + 67)
+    lall, kall, gall, npl = spec.lall, spec.kall, spec.gall, spec.gall_pl
+    base = diffusion_inputs(spec, geo)
+    gz, _ = stretched_vgrid(kall)
+    dz = np.diff(gz, append=gz[-1] + (gz[-1] - gz[-2]))
+    x = geo["points"]
+    v = np.cross(_unit(rng.normal(size=3)) * (20.0 / RADIUS), x)  # solid body, m/s
+    prof = np.exp(-np.maximum(gz, 0.0) / 8000.0)
+
+    def reg(c):
+        return prof[None, :, None] * v[:, None, :, c] * (1.0 + 0.05 * rng.uniform(-1, 1, size=(lall, kall, gall)))
+
+    def pol(scale=1.0):
+        return scale * rng.uniform(-1.0, 1.0, size=(2, kall, npl))
+
+    def metric(shape):
+        return 1.0 + 0.01 * rng.uniform(-1.0, 1.0, size=shape)
+
+    return dict(rhogvx=reg(0), rhogvx_pl=pol(10.0), rhogvy=reg(1), rhogvy_pl=pol(10.0),
+                rhogvz=reg(2), rhogvz_pl=pol(10.0),
+                rhogw=0.1 * rng.standard_normal((lall, kall, gall)), rhogw_pl=pol(0.1),
+                coef_intp=base["coef_intp"], coef_intp_pl=base["coef_intp_pl"],
+                coef_diff=base["coef_diff"], coef_diff_pl=base["coef_diff_pl"],
+                RGSQRTH=metric((lall, kall, gall)), RGSQRTH_pl=metric((2, kall, npl)),
+                RGAM=metric((lall, kall, gall)), RGAM_pl=metric((2, kall, npl)),
+                RGAMH=metric((lall, kall, gall)), RGAMH_pl=metric((2, kall, npl)),
+                C2WfactGz=0.5 * rng.uniform(-1.0, 1.0, size=(lall, 6, kall, gall)) / dz[None, None, :, None],
+                C2WfactGz_pl=0.5 * rng.uniform(-1.0, 1.0, size=(2, 6, kall, npl)) / dz[None, None, :, None],
+                rdgz=1.0 / dz)
+
+
 def make_case(spec: GridSpec, have_sgp=None, vgrid=None) -> Dict[str, Dict[str, np.ndarray]]:
     """All inputs of the four kernels for one grid."""
     geo = make_geometry(spec, have_sgp)

@@ -79,13 +79,15 @@ int nicam_ctx_set_pointer_mode(nicam_ctx ctx, int mode);
  *                           1 = LDS, 2 = device work array, 3 = registers
  *                           (kall = 42 or 98, else automatic), 0 = automatic
  *   NICAM_TUNE_DIFF_KPER    levels per work-group of nicam_oprt_diffusion
+ *   NICAM_TUNE_DDMP_KPER    levels per work-group of nicam_oprt3d_divdamp
  * Results do not depend on these settings (every variant computes the same
  * floating-point operations in the same order). */
 enum nicam_tuning {
     NICAM_TUNE_HFLUX_KPER = 1,
     NICAM_TUNE_VLIM_KCHUNK = 3,
     NICAM_TUNE_VI_VARIANT = 4,
-    NICAM_TUNE_DIFF_KPER = 5
+    NICAM_TUNE_DIFF_KPER = 5,
+    NICAM_TUNE_DDMP_KPER = 6
 };
 int nicam_ctx_set_tuning(nicam_ctx ctx, int param, int value);
 
@@ -109,6 +111,9 @@ int nicam_grd_input_vgrid(const char *fname, int kall, double *gz, double *gzh);
  * outputs (kall doubles each) may be NULL. */
 int nicam_grd_setup(nicam_ctx ctx, const double *gz, const double *gzh,
                     double *rdgzh_out, double *afact_out, double *bfact_out);
+/* Direct assignment of GRD_rdgz (kall), which OPRT3D_divdamp reads;
+ * nicam_grd_setup also sets it (dyn_divdamp/src/mod_misc.f90:480-496). */
+int nicam_set_rdgz(nicam_ctx ctx, const double *rdgz);
 /* Direct assignment of GRD_rdgzh, GRD_afact, GRD_bfact (kall each). */
 int nicam_set_vgrid_factors(nicam_ctx ctx, const double *rdgzh, const double *afact,
                             const double *bfact);
@@ -134,6 +139,16 @@ int nicam_set_vmetrics(nicam_ctx ctx,
                        const double *RGSGAM2, const double *RGSGAM2_pl,
                        const double *RGSGAM2H, const double *RGSGAM2H_pl,
                        const double *GSGAM2H, const double *GSGAM2H_pl);
+
+/* Vertical metrics that OPRT3D_divdamp reads (dyn_divdamp/src/mod_misc.f90,
+ * module arrays VMTR_*): RGSQRTH, RGAM, RGAMH (gall,kall,lall) and _pl,
+ * C2WfactGz(gall,kall,6,lall) and C2WfactGz_pl(gall_pl,kall,6,2).  RGAM and
+ * RGAMH are the same context copies nicam_set_vmetrics sets. */
+int nicam_set_vmetrics3d(nicam_ctx ctx,
+                         const double *RGSQRTH, const double *RGSQRTH_pl,
+                         const double *RGAM, const double *RGAM_pl,
+                         const double *RGAMH, const double *RGAMH_pl,
+                         const double *C2WfactGz, const double *C2WfactGz_pl);
 
 /* ---- hot path ---------------------------------------------------------- */
 
@@ -218,6 +233,24 @@ int nicam_oprt_diffusion(nicam_ctx ctx,
                          double *dscl, double *dscl_pl,
                          const double *scl, const double *scl_pl,
                          const double *kh, const double *kh_pl,
+                         const double *coef_intp, const double *coef_intp_pl,
+                         const double *coef_diff, const double *coef_diff_pl);
+
+/* OPRT3D_divdamp (dyn_divdamp/src/mod_oprt3d.f90:41-402).
+ *   out ddivdx, ddivdy, ddivdz (gall,kall,lall) and _pl (gall_pl,kall,2)
+ *   in  rhogvx, rhogvy, rhogvz, rhogw (gall,kall,lall) and _pl
+ *   in  coef_intp, coef_diff and _pl as for nicam_oprt_diffusion
+ * Needs nicam_set_vmetrics3d and GRD_rdgz (nicam_grd_setup or
+ * nicam_set_rdgz).  The outputs' pole arrays are zeroed when the context has
+ * no pole regions (they may then be NULL, all three together). */
+int nicam_oprt3d_divdamp(nicam_ctx ctx,
+                         double *ddivdx, double *ddivdx_pl,
+                         double *ddivdy, double *ddivdy_pl,
+                         double *ddivdz, double *ddivdz_pl,
+                         const double *rhogvx, const double *rhogvx_pl,
+                         const double *rhogvy, const double *rhogvy_pl,
+                         const double *rhogvz, const double *rhogvz_pl,
+                         const double *rhogw, const double *rhogw_pl,
                          const double *coef_intp, const double *coef_intp_pl,
                          const double *coef_diff, const double *coef_diff_pl);
 

@@ -21,11 +21,17 @@ import numpy as np
 HERE = Path(__file__).resolve().parent
 ORACLE_SO = HERE / "liboracle.so"
 REF_DIR = HERE / "_ref"
-REF_LIBS = {k: REF_DIR / f"libref_{k}.so" for k in ("flux", "hlim", "vlim", "vi", "diff")}
+REF_LIBS = {k: REF_DIR / f"libref_{k}.so" for k in ("flux", "hlim", "vlim", "vi", "diff", "ddmp")}
 
 
 # inputs of OPRT_diffusion in argument order (dyn_diffusion/src/mod_oprt.f90:41-46)
 DIFF_IN = ("scl", "scl_pl", "kh", "kh_pl", "coef_intp", "coef_intp_pl", "coef_diff", "coef_diff_pl")
+# inputs of OPRT3D_divdamp (dyn_divdamp/src/mod_oprt3d.f90:41-50) followed by the
+# module arrays it reads (mod_oprt3d.f90:71-84)
+DDMP_IN = ("rhogvx", "rhogvx_pl", "rhogvy", "rhogvy_pl", "rhogvz", "rhogvz_pl", "rhogw", "rhogw_pl",
+           "coef_intp", "coef_intp_pl", "coef_diff", "coef_diff_pl", "RGSQRTH", "RGSQRTH_pl", "RGAM", "RGAM_pl",
+           "RGAMH", "RGAMH_pl", "C2WfactGz", "C2WfactGz_pl", "rdgz")
+DDMP_OUT = ("ddivdx", "ddivdx_pl", "ddivdy", "ddivdy_pl", "ddivdz", "ddivdz_pl")
 
 
 class OrcDims(ctypes.Structure):
@@ -78,6 +84,7 @@ class Oracle:
         L.orc_vi_rhow_solver.argtypes = ([POINTER(OrcDims)] + [P] * 14 + [c_double] + [P] * 16 + [P] * 3 +
                                          [c_double, c_double, c_double, c_int])
         L.orc_oprt_diffusion.argtypes = [POINTER(OrcDims), P] + [P] * 10
+        L.orc_oprt3d_divdamp.argtypes = [POINTER(OrcDims), P] + [P] * 27
         L.orc_num_threads.restype = c_int
 
     def num_threads(self) -> int:
@@ -101,6 +108,15 @@ class Oracle:
         sgp = np.ascontiguousarray(have_sgp, dtype=np.int32)
         self.lib.orc_oprt_diffusion(ctypes.byref(D), sgp.ctypes.data, _p(out["dscl"]), _p(out["dscl_pl"]),
                                     *[_p(ins[k]) for k in DIFF_IN])
+        return out
+
+    def oprt3d_divdamp(self, spec, have_sgp, f, fill=0.0) -> Dict[str, np.ndarray]:
+        D = self.dims(spec)
+        out = {k: _init_out(spec.shape_pl if k.endswith("_pl") else spec.shape3, fill) for k in DDMP_OUT}
+        ins = {k: _f64(f[k]) for k in DDMP_IN}
+        sgp = np.ascontiguousarray(have_sgp, dtype=np.int32)
+        self.lib.orc_oprt3d_divdamp(ctypes.byref(D), sgp.ctypes.data, *[_p(out[k]) for k in DDMP_OUT],
+                                    *[_p(ins[k]) for k in DDMP_IN])
         return out
 
     def grd_setup(self, gz, gzh):
@@ -225,11 +241,22 @@ class RefLibs:
         self.vi.ref_const_setup.argtypes = [POINTER(c_double), POINTER(c_double)]
         self.diff = ctypes.CDLL(str(REF_LIBS["diff"]))
         self.diff.ref_oprt_diffusion.argtypes = [c_int] * 5 + [P] * 11
+        self.ddmp = ctypes.CDLL(str(REF_LIBS["ddmp"]))
+        self.ddmp.ref_oprt3d_divdamp.argtypes = [c_int] * 5 + [P] * 28
 
     def const_setup(self):
         e, b = c_double(), c_double()
         self.vi.ref_const_setup(ctypes.byref(e), ctypes.byref(b))
         return e.value, b.value
+
+    def oprt3d_divdamp(self, spec, have_sgp, f, fill=0.0):
+        out = {k: _init_out(spec.shape_pl if k.endswith("_pl") else spec.shape3, fill) for k in DDMP_OUT}
+        ins = {k: _f64(f[k]) for k in DDMP_IN}
+        sgp = np.ascontiguousarray(have_sgp, dtype=np.int32)
+        _big_stack(self.ddmp.ref_oprt3d_divdamp, spec.gall_1d, spec.kall, spec.lall, spec.gall_pl,
+                   int(spec.have_pl), sgp.ctypes.data, *[_p(out[k]) for k in DDMP_OUT],
+                   *[_p(ins[k]) for k in DDMP_IN])
+        return out
 
     def oprt_diffusion(self, spec, have_sgp, f, fill=0.0):
         out = dict(dscl=_init_out(spec.shape3, fill), dscl_pl=_init_out(spec.shape_pl, fill))

@@ -758,3 +758,182 @@ void orc_oprt_diffusion(const orc_dims *D, const int *have_sgp,
 #undef CI_PL
 #undef CD_PL
 }
+
+/* ------------------------------------------------------------------------- */
+/* dyn_divdamp/src/mod_oprt3d.f90:41-402                                     */
+/* ------------------------------------------------------------------------- */
+void orc_oprt3d_divdamp(const orc_dims *D, const int *have_sgp,
+                        double *ddivdx, double *ddivdx_pl,
+                        double *ddivdy, double *ddivdy_pl,
+                        double *ddivdz, double *ddivdz_pl,
+                        const double *rhogvx, const double *rhogvx_pl,
+                        const double *rhogvy, const double *rhogvy_pl,
+                        const double *rhogvz, const double *rhogvz_pl,
+                        const double *rhogw, const double *rhogw_pl,
+                        const double *coef_intp, const double *coef_intp_pl,
+                        const double *coef_diff, const double *coef_diff_pl,
+                        const double *RGSQRTH, const double *RGSQRTH_pl,
+                        const double *RGAM, const double *RGAM_pl,
+                        const double *RGAMH, const double *RGAMH_pl,
+                        const double *C2WfactGz, const double *C2WfactGz_pl,
+                        const double *rdgz)
+{
+    const int iall = D->gall_1d, gall = iall * iall, kall = D->kall, lall = D->lall;
+    const int gall_pl = D->gall_pl;
+    const int kmin = 2, kmax = kall - 1;
+    const int gmin = (2 - 1) * iall + 2, gmax = (iall - 1 - 1) * iall + iall - 1;
+    const int gminm1 = (2 - 1 - 1) * iall + 2 - 1;
+#define CI(g, c, d, t, l) coef_intp[((((size_t)((l) - 1) * 2 + (t)) * 3 + ((d) - 1)) * 3 + ((c) - 1)) * gall + ((g) - 1)]
+#define CD(g, m, d, l) coef_diff[(((size_t)((l) - 1) * 3 + ((d) - 1)) * 6 + ((m) - 1)) * gall + ((g) - 1)]
+#define CI_PL(v, c, d, l) coef_intp_pl[((((size_t)((l) - 1)) * 3 + ((d) - 1)) * 3 + ((c) - 1)) * gall_pl + ((v) - 1)]
+#define CD_PL(m, d, l) coef_diff_pl[(((size_t)((l) - 1)) * 3 + ((d) - 1)) * 5 + ((m) - 1)]
+#define C2W(g, k, m, l) C2WfactGz[(((size_t)((l) - 1) * 6 + ((m) - 1)) * kall + ((k) - 1)) * gall + ((g) - 1)]
+#define C2W_PL(g, k, m, l) C2WfactGz_pl[(((size_t)((l) - 1) * 6 + ((m) - 1)) * kall + ((k) - 1)) * gall_pl + ((g) - 1)]
+
+    double *wvm = (double *)malloc(sizeof(double) * (size_t)gall * kall); /* rhogw_vm of one region */
+#define WVM(g, k) wvm[((size_t)((k) - 1)) * gall + ((g) - 1)]
+    for (int l = 1; l <= lall; ++l) {
+        for (int k = kmin + 1; k <= kmax; ++k) /* :143-157 */
+            for (int g = 1; g <= gall; ++g)
+                WVM(g, k) = (C2W(g, k, 1, l) * F3(rhogvx, g, k, l) + C2W(g, k, 2, l) * F3(rhogvx, g, k - 1, l)
+                           + C2W(g, k, 3, l) * F3(rhogvy, g, k, l) + C2W(g, k, 4, l) * F3(rhogvy, g, k - 1, l)
+                           + C2W(g, k, 5, l) * F3(rhogvz, g, k, l) + C2W(g, k, 6, l) * F3(rhogvz, g, k - 1, l))
+                          * F3(RGAMH, g, k, l)
+                          + F3(rhogw, g, k, l) * F3(RGSQRTH, g, k, l);
+        for (int g = 1; g <= gall; ++g) {
+            WVM(g, kmin) = 0.0;
+            WVM(g, kmax + 1) = 0.0;
+        }
+#pragma omp parallel for schedule(static)
+        for (int k = kmin; k <= kmax; ++k) {
+            double *buf = (double *)malloc(sizeof(double) * (size_t)gall * 5);
+            double *vxm = buf, *vym = buf + gall, *vzm = buf + 2 * (size_t)gall, *st = buf + 3 * (size_t)gall;
+#define ST(g, t) st[(size_t)(t) * gall + ((g) - 1)]
+            for (int g = 1; g <= gall; ++g) { /* :176-181 */
+                vxm[g - 1] = F3(rhogvx, g, k, l) * F3(RGAM, g, k, l);
+                vym[g - 1] = F3(rhogvy, g, k, l) * F3(RGAM, g, k, l);
+                vzm[g - 1] = F3(rhogvz, g, k, l) * F3(RGAM, g, k, l);
+            }
+            for (int g = gminm1; g <= gmax; ++g) { /* :183-203 */
+                const int ij = g, ip1j = g + 1, ip1jp1 = g + iall + 1;
+                const double sw = ((WVM(ij, k + 1) + WVM(ip1j, k + 1) + WVM(ip1jp1, k + 1))
+                                 - (WVM(ij, k) + WVM(ip1j, k) + WVM(ip1jp1, k))) / 3.0 * rdgz[k - 1];
+                ST(g, 0) = CI(g, 1, 1, 0, l) * vxm[ij - 1] + CI(g, 2, 1, 0, l) * vxm[ip1j - 1]
+                         + CI(g, 3, 1, 0, l) * vxm[ip1jp1 - 1] + CI(g, 1, 2, 0, l) * vym[ij - 1]
+                         + CI(g, 2, 2, 0, l) * vym[ip1j - 1] + CI(g, 3, 2, 0, l) * vym[ip1jp1 - 1]
+                         + CI(g, 1, 3, 0, l) * vzm[ij - 1] + CI(g, 2, 3, 0, l) * vzm[ip1j - 1]
+                         + CI(g, 3, 3, 0, l) * vzm[ip1jp1 - 1] + sw;
+            }
+            for (int g = gminm1; g <= gmax; ++g) { /* :205-225 */
+                const int ij = g, ip1jp1 = g + iall + 1, ijp1 = g + iall;
+                const double sw = ((WVM(ij, k + 1) + WVM(ip1jp1, k + 1) + WVM(ijp1, k + 1))
+                                 - (WVM(ij, k) + WVM(ip1jp1, k) + WVM(ijp1, k))) / 3.0 * rdgz[k - 1];
+                ST(g, 1) = CI(g, 1, 1, 1, l) * vxm[ij - 1] + CI(g, 2, 1, 1, l) * vxm[ip1jp1 - 1]
+                         + CI(g, 3, 1, 1, l) * vxm[ijp1 - 1] + CI(g, 1, 2, 1, l) * vym[ij - 1]
+                         + CI(g, 2, 2, 1, l) * vym[ip1jp1 - 1] + CI(g, 3, 2, 1, l) * vym[ijp1 - 1]
+                         + CI(g, 1, 3, 1, l) * vzm[ij - 1] + CI(g, 2, 3, 1, l) * vzm[ip1jp1 - 1]
+                         + CI(g, 3, 3, 1, l) * vzm[ijp1 - 1] + sw;
+            }
+            if (have_sgp[l - 1]) ST(gminm1, 0) = ST(gminm1 + 1, 1); /* :227-229 */
+            double *out[3] = {ddivdx, ddivdy, ddivdz};
+            for (int d = 0; d < 3; ++d) {
+                for (int g = 1; g <= gmin - 1; ++g) F3(out[d], g, k, l) = 0.0;
+                for (int g = gmin; g <= gmax; ++g) { /* :237-282 */
+                    const int ij = g, im1j = g - 1, im1jm1 = g - iall - 1, ijm1 = g - iall;
+                    F3(out[d], g, k, l) = CD(g, 1, d + 1, l) * (ST(ij, 0) + ST(ij, 1))
+                                        + CD(g, 2, d + 1, l) * (ST(ij, 1) + ST(im1j, 0))
+                                        + CD(g, 3, d + 1, l) * (ST(im1j, 0) + ST(im1jm1, 1))
+                                        + CD(g, 4, d + 1, l) * (ST(im1jm1, 1) + ST(im1jm1, 0))
+                                        + CD(g, 5, d + 1, l) * (ST(im1jm1, 0) + ST(ijm1, 1))
+                                        + CD(g, 6, d + 1, l) * (ST(ijm1, 1) + ST(ij, 0));
+                }
+                for (int g = gmax + 1; g <= gall; ++g) F3(out[d], g, k, l) = 0.0;
+            }
+#undef ST
+            free(buf);
+        }
+        for (int g = 1; g <= gall; ++g) { /* :303-310 */
+            F3(ddivdx, g, kmin - 1, l) = 0.0;
+            F3(ddivdy, g, kmin - 1, l) = 0.0;
+            F3(ddivdz, g, kmin - 1, l) = 0.0;
+            F3(ddivdx, g, kmax + 1, l) = 0.0;
+            F3(ddivdy, g, kmax + 1, l) = 0.0;
+            F3(ddivdz, g, kmax + 1, l) = 0.0;
+        }
+    }
+#undef WVM
+    free(wvm);
+
+    if (D->have_pl) { /* :313-392 */
+        const int n = GSLF_PL, gmin_pl = GMIN_PL, gmax_pl = gall_pl;
+        double *wpl = (double *)malloc(sizeof(double) * (size_t)gall_pl * kall);
+#define WPL(g, k) wpl[((size_t)((k) - 1)) * gall_pl + ((g) - 1)]
+        for (int l = 1; l <= LALL_PL; ++l) {
+            for (int k = kmin + 1; k <= kmax; ++k)
+                for (int g = 1; g <= gall_pl; ++g)
+                    WPL(g, k) = (C2W_PL(g, k, 1, l) * P3(rhogvx_pl, g, k, l) + C2W_PL(g, k, 2, l) * P3(rhogvx_pl, g, k - 1, l)
+                               + C2W_PL(g, k, 3, l) * P3(rhogvy_pl, g, k, l) + C2W_PL(g, k, 4, l) * P3(rhogvy_pl, g, k - 1, l)
+                               + C2W_PL(g, k, 5, l) * P3(rhogvz_pl, g, k, l) + C2W_PL(g, k, 6, l) * P3(rhogvz_pl, g, k - 1, l))
+                              * P3(RGAMH_pl, g, k, l)
+                              + P3(rhogw_pl, g, k, l) * P3(RGSQRTH_pl, g, k, l);
+            for (int g = 1; g <= gall_pl; ++g) {
+                WPL(g, kmin) = 0.0;
+                WPL(g, kmax + 1) = 0.0;
+            }
+            for (int k = kmin; k <= kmax; ++k) {
+                double vx[16], vy[16], vz[16], sp[16];
+                for (int v = 1; v <= gall_pl; ++v) {
+                    vx[v] = P3(rhogvx_pl, v, k, l) * P3(RGAM_pl, v, k, l);
+                    vy[v] = P3(rhogvy_pl, v, k, l) * P3(RGAM_pl, v, k, l);
+                    vz[v] = P3(rhogvz_pl, v, k, l) * P3(RGAM_pl, v, k, l);
+                }
+                for (int v = gmin_pl; v <= gmax_pl; ++v) {
+                    const int ij = v;
+                    int ijp1 = v + 1;
+                    if (ijp1 == gmax_pl + 1) ijp1 = gmin_pl;
+                    const double sw = ((WPL(n, k + 1) + WPL(ij, k + 1) + WPL(ijp1, k + 1))
+                                     - (WPL(n, k) + WPL(ij, k) + WPL(ijp1, k))) / 3.0 * rdgz[k - 1];
+                    sp[ij] = CI_PL(v, 1, 1, l) * vx[n] + CI_PL(v, 2, 1, l) * vx[ij] + CI_PL(v, 3, 1, l) * vx[ijp1]
+                           + CI_PL(v, 1, 2, l) * vy[n] + CI_PL(v, 2, 2, l) * vy[ij] + CI_PL(v, 3, 2, l) * vy[ijp1]
+                           + CI_PL(v, 1, 3, l) * vz[n] + CI_PL(v, 2, 3, l) * vz[ij] + CI_PL(v, 3, 3, l) * vz[ijp1]
+                           + sw;
+                }
+                for (int v = 1; v <= gall_pl; ++v) {
+                    P3(ddivdx_pl, v, k, l) = 0.0;
+                    P3(ddivdy_pl, v, k, l) = 0.0;
+                    P3(ddivdz_pl, v, k, l) = 0.0;
+                }
+                for (int v = gmin_pl; v <= gmax_pl; ++v) {
+                    const int ij = v;
+                    int ijm1 = v - 1;
+                    if (ijm1 == gmin_pl - 1) ijm1 = gmax_pl;
+                    P3(ddivdx_pl, n, k, l) = P3(ddivdx_pl, n, k, l) + CD_PL(v - 1, 1, l) * (sp[ijm1] + sp[ij]);
+                    P3(ddivdy_pl, n, k, l) = P3(ddivdy_pl, n, k, l) + CD_PL(v - 1, 2, l) * (sp[ijm1] + sp[ij]);
+                    P3(ddivdz_pl, n, k, l) = P3(ddivdz_pl, n, k, l) + CD_PL(v - 1, 3, l) * (sp[ijm1] + sp[ij]);
+                }
+            }
+            for (int v = 1; v <= gall_pl; ++v) {
+                P3(ddivdx_pl, v, kmin - 1, l) = 0.0;
+                P3(ddivdx_pl, v, kmax + 1, l) = 0.0;
+                P3(ddivdy_pl, v, kmin - 1, l) = 0.0;
+                P3(ddivdy_pl, v, kmax + 1, l) = 0.0;
+                P3(ddivdz_pl, v, kmin - 1, l) = 0.0;
+                P3(ddivdz_pl, v, kmax + 1, l) = 0.0;
+            }
+        }
+#undef WPL
+        free(wpl);
+    } else {
+        for (size_t i = 0; i < (size_t)gall_pl * kall * LALL_PL; ++i) {
+            ddivdx_pl[i] = 0.0;
+            ddivdy_pl[i] = 0.0;
+            ddivdz_pl[i] = 0.0;
+        }
+    }
+#undef CI
+#undef CD
+#undef CI_PL
+#undef CD_PL
+#undef C2W
+#undef C2W_PL
+}

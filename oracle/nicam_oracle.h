@@ -106,6 +106,26 @@ void orc_oprt_diffusion(const orc_dims *D, const int *have_sgp,
                         const double *coef_intp, const double *coef_intp_pl,
                         const double *coef_diff, const double *coef_diff_pl);
 
+/* dyn_divdamp/src/mod_oprt3d.f90:41-402 (OPRT3D_divdamp).  The module
+ * arrays it reads are passed in: VMTR_RGSQRTH, VMTR_RGAM, VMTR_RGAMH
+ * (gall,kall,lall) and _pl, VMTR_C2WfactGz(gall,kall,6,lall) and
+ * _pl(gall_pl,kall,6,2), GRD_rdgz(kall). */
+void orc_oprt3d_divdamp(const orc_dims *D, const int *have_sgp,
+                        double *ddivdx, double *ddivdx_pl,
+                        double *ddivdy, double *ddivdy_pl,
+                        double *ddivdz, double *ddivdz_pl,
+                        const double *rhogvx, const double *rhogvx_pl,
+                        const double *rhogvy, const double *rhogvy_pl,
+                        const double *rhogvz, const double *rhogvz_pl,
+                        const double *rhogw, const double *rhogw_pl,
+                        const double *coef_intp, const double *coef_intp_pl,
+                        const double *coef_diff, const double *coef_diff_pl,
+                        const double *RGSQRTH, const double *RGSQRTH_pl,
+                        const double *RGAM, const double *RGAM_pl,
+                        const double *RGAMH, const double *RGAMH_pl,
+                        const double *C2WfactGz, const double *C2WfactGz_pl,
+                        const double *rdgz);
+
 /* Number of OpenMP threads the oracle runs with (1 when built without it). */
 int orc_num_threads(void);
 

@@ -33,7 +33,7 @@ REPO = Path(__file__).resolve().parent.parent
 REF = REPO / "oracle" / "_ref"
 VGRID = REPO / "tests" / "golden" / "vgrid40_600m_24km.dat"
 KERNELS = {"flux": "dyn_horiz_adv_flux", "hlim": "dyn_horiz_adv_limiter", "vlim": "dyn_vert_adv_limiter",
-           "vi": "dyn_vi_rhow_solver", "diff": "dyn_diffusion"}
+           "vi": "dyn_vi_rhow_solver", "diff": "dyn_diffusion", "ddmp": "dyn_divdamp"}
 N1, KALL, LALL = 130, 42, 1  # problem_size.inc:28-45 of every driver
 CHECK = re.compile(r"\+check\[(.{16})\] max=(.*),min=(.*),sum=(.*)$")
 
@@ -95,6 +95,10 @@ def make_snapshot(short, oracle, workdir: Path):
     elif short == "diff":
         f = synth.diffusion_inputs(spec, geo)
         fields = {**f, **oracle.oprt_diffusion(spec, [1], f)}
+    elif short == "ddmp":  # the driver's GRD_setup reads GRD_rdgz from the vertical grid file
+        f = dict(synth.divdamp_inputs(spec, geo), rdgz=oracle.grd_setup(gz, gzh)["rdgz"])
+        fields = {**f, **oracle.oprt3d_divdamp(spec, [1], f)}
+        shutil.copy(VGRID, workdir / VGRID.name)
     else:
         f = case["vi"]
         g = oracle.grd_setup(gz, gzh)
