@@ -43,6 +43,8 @@ def parse():
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-regions", type=int, default=40, help="regions in the CPU baseline sample")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="skip timing the neighbouring operators, the halo fill and the limiter's exchange mode")
     return ap.parse_args()
 
 
@@ -106,6 +108,74 @@ def cpu_baseline(spec, case, regions: int, reps: int = 4):
                 seconds={k: round(v, 4) for k, v in times.items()})
 
 
+def time_extras(ctx, spec, case, stream, steps):
+    """Per-launch times of what sits beside the step on the same gl7 grid:
+    the limiter with its own Qout exchange (Qout_post = NULL), the halo fill
+    of one field, OPRT_diffusion and OPRT3D_divdamp.  Reported beside the
+    headline, not part of it."""
+    import numpy as np
+    import torch
+
+    from icoatmosbenchmark_v1_amd import drivers, perfmodel, synth
+
+    dev = torch.device("cuda:0")
+
+    def up(d, keys):
+        return {k: torch.from_numpy(np.ascontiguousarray(d[k])).to(dev) for k in keys}
+
+    def z(shape):
+        return torch.zeros(shape, dtype=torch.float64, device=dev)
+
+    def timed(fn):
+        for _ in range(2):
+            fn()
+        e = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+        e[0].record(stream)
+        for _ in range(steps):
+            fn()
+        e[1].record(stream)
+        torch.cuda.synchronize()
+        return e[0].elapsed_time(e[1]) / steps
+
+    out = {}
+    t0 = time.perf_counter()
+    ctx.comm_setup(1)
+    H = up(case["hlim"], drivers.HLIM_IN + ("q_a", "q_a_pl"))
+    H["Qout_prev"], H["Qout_prev_pl"] = z((2,) + spec.shape3), z((2,) + spec.shape_pl)
+    ms = timed(lambda: ctx.horizontal_limiter_thuburn(H["q_a"], H["q_a_pl"], H["q"], H["q_pl"], H["d"], H["d_pl"],
+                                                       H["ch"], H["ch_pl"], H["cmask"], H["cmask_pl"],
+                                                       H["Qout_prev"], H["Qout_prev_pl"], None, None))
+    out["horizontal_limiter_thuburn_exchange"] = dict(ms=round(ms, 5))
+    del H
+    var, var_pl = z(spec.shape3), z(spec.shape_pl)
+    ms = timed(lambda: ctx.comm_data_transfer(var, var_pl, 1))
+    out["comm_data_transfer_1var"] = dict(ms=round(ms, 5))
+    del var, var_pl
+    ob = perfmodel.operator_bytes(spec)
+    f = synth.diffusion_inputs(spec, case["geo"])
+    D = up(f, drivers.DIFF_IN)
+    D["dscl"], D["dscl_pl"] = z(spec.shape3), z(spec.shape_pl)
+    ms = timed(lambda: ctx.oprt_diffusion(D["dscl"], D["dscl_pl"], *[D[k] for k in drivers.DIFF_IN]))
+    gbs = ob["oprt_diffusion"] / (ms * 1e-3) / 1e9
+    out["oprt_diffusion"] = dict(ms=round(ms, 5), alg_bytes=ob["oprt_diffusion"], gbs=round(gbs, 1),
+                                 frac=round(gbs / perfmodel.HBM_PEAK_GBS, 4))
+    del D, f
+    f = synth.divdamp_inputs(spec, case["geo"])
+    drivers.setup_divdamp(ctx, f)
+    G = up(f, drivers.DDMP_IN)
+    for k in drivers.DDMP_OUT:
+        G[k] = z(spec.shape_pl if k.endswith("_pl") else spec.shape3)
+    ms = timed(lambda: ctx.oprt3d_divdamp(*[G[k] for k in drivers.DDMP_OUT], *[G[k] for k in drivers.DDMP_IN]))
+    gbs = ob["oprt3d_divdamp"] / (ms * 1e-3) / 1e9
+    out["oprt3d_divdamp"] = dict(ms=round(ms, 5), alg_bytes=ob["oprt3d_divdamp"], gbs=round(gbs, 1),
+                                 frac=round(gbs / perfmodel.HBM_PEAK_GBS, 4))
+    del G, f
+    torch.cuda.synchronize()
+    log("extras: " + ", ".join(f"{k}={v['ms']:.4f} ms" for k, v in out.items())
+        + f" ({time.perf_counter() - t0:.1f} s)")
+    return out
+
+
 def main() -> int:
     args = parse()
     if args.gpus != 1:
@@ -127,7 +197,8 @@ def main() -> int:
     dev = torch.device("cuda:0")
     spec = synth.GridSpec(66, 98, 40, seed=args.seed)
     t0 = time.perf_counter()
-    case = synth.make_case(spec)
+    sgp = runtime.halo_map(1, spec.gall_1d)["have_sgp"].tolist()  # the rlevel-1 icosahedral regions
+    case = synth.make_case(spec, have_sgp=sgp)
     log(f"inputs generated in {time.perf_counter() - t0:.1f} s")
     geo, vi = case["geo"], case["vi"]
     ctx = drivers.setup_context(spec, geo, vi)
@@ -211,6 +282,8 @@ def main() -> int:
                 frac=kernels[dom]["frac"], traffic=traffic)
     step_gbs = step_bytes / (ms_step * 1e-3) / 1e9
 
+    extras = None if args.no_extras else time_extras(ctx, spec, case, stream, args.steps)
+
     cpu = None
     if not args.no_cpu_baseline:
         try:
@@ -242,6 +315,7 @@ def main() -> int:
         step_alg_bytes=step_bytes,
         adjoint_fwd_ratio=None,
         kernels=kernels,
+        extras=extras,
         roofline=roof,
         cpu_baseline=cpu,
         host=platform.node(),

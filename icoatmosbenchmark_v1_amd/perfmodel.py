@@ -69,3 +69,24 @@ def kernel_flops(spec: GridSpec) -> Dict[str, int]:
     cl = spec.array_cell_levels()
     return dict(horizontal_flux=125 * cl, horizontal_limiter_thuburn=170 * cl, vertical_limiter_thuburn=45 * cl,
                 vi_rhow_solver=45 * cl)
+
+
+def operator_bytes(spec: GridSpec, word: int = 8) -> Dict[str, int]:
+    """Algorithmic bytes of the neighbouring operators and the halo fill.
+
+    OPRT_diffusion (dyn_diffusion/src/mod_oprt.f90:97-216): scl and kh read
+    and dscl written on every cell of every level; coef_intp (18) and
+    coef_diff (18) per cell, K-invariant, once per region.
+    OPRT3D_divdamp (dyn_divdamp/src/mod_oprt3d.f90:143-310): per column the
+    momenta and RGAM on kmin..kmax, rhogw, RGSQRTH, RGAMH and C2WfactGz (6)
+    on kmin+1..kmax, the three outputs on all levels; the same 36
+    coefficients per cell.
+    halo fill (nicam_comm_data_transfer, one variable): every map entry
+    reads one value and writes one per level."""
+    n, kall, lall, npl = spec.gall_1d, spec.kall, spec.lall, spec.gall_pl
+    gall = n * n
+    pl = 2 if spec.have_pl else 0
+    diff = (3 * gall * kall + 36 * gall) * lall * word + pl * (3 * npl * kall + 9 * npl + 15) * word
+    dcol = 4 * (kall - 2) + 9 * (kall - 3) + 3 * kall
+    ddmp = (dcol * gall + 36 * gall) * lall * word + pl * (dcol * npl + 9 * npl + 15) * word
+    return dict(oprt_diffusion=diff, oprt3d_divdamp=ddmp)
