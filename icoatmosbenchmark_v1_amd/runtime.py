@@ -83,6 +83,8 @@ SIGNATURES = {
     "nicam_comm_data_transfer": (c_int, [_P, _P, _P, c_int, c_int]),
     "nicam_comm_var": (c_int, [_P, _P, _P, c_int, c_int]),
     "nicam_grid_icosahedral": (c_int, [c_int, c_int, c_double, _P, _P]),
+    "nicam_grid_vertices": (c_int, [c_int, c_int, c_int, c_int, _P, c_double] + [_P] * 8),
+    "nicam_gmtr_setup": (c_int, [c_int, c_int, c_int, c_int, _P, c_double] + [_P] * 12),
     "nicam_oprt_diffusion": (c_int, [_P] + [_P] * 10),
     "nicam_set_rdgz": (c_int, [_P, _P]),
     "nicam_set_vmetrics3d": (c_int, [_P] + [_P] * 8),
@@ -456,3 +458,49 @@ def grid_icosahedral(glevel: int, rlevel: int, radius: float = 6.37122e6):
     _check(lib.nicam_grid_icosahedral(glevel, rlevel, float(radius), x.ctypes.data, x_pl.ctypes.data),
            "nicam_grid_icosahedral")
     return x, x_pl
+
+
+def grid_vertices(x, x_pl, have_sgp, radius: float = 6.37122e6):
+    """nicam_grid_vertices for GRD_x[3][lall][1][gall] and GRD_x_pl[3][2][1][gall_pl]:
+    a dict xt[3][2][lall][1][gall], xt_pl[3][2][1][gall_pl], s[2][lall][1][gall],
+    s_pl[2][2][1][gall_pl], xr[3][3][lall][1][gall], xr_pl[3][2][1][gall_pl]
+    (the Fortran shapes reversed)."""
+    lib = load_library()
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    x_pl = np.ascontiguousarray(x_pl, dtype=np.float64)
+    _, lall, _, gall = x.shape
+    gall_pl = x_pl.shape[-1]
+    n1 = int(round(gall ** 0.5))
+    sgp = np.ascontiguousarray(have_sgp, dtype=np.int32)
+    if sgp.size != lall or n1 * n1 != gall:
+        raise ValueError("have_sgp needs one flag per region and gall must be a square")
+    out = {"xt": np.empty((3, 2, lall, 1, gall)), "xt_pl": np.empty((3, 2, 1, gall_pl)),
+           "s": np.empty((2, lall, 1, gall)), "s_pl": np.empty((2, 2, 1, gall_pl)),
+           "xr": np.empty((3, 3, lall, 1, gall)), "xr_pl": np.empty((3, 2, 1, gall_pl))}
+    _check(lib.nicam_grid_vertices(n1, lall, gall_pl, 1, sgp.ctypes.data, float(radius), x.ctypes.data,
+                                   x_pl.ctypes.data, *[out[k].ctypes.data for k in
+                                                       ("xt", "xt_pl", "s", "s_pl", "xr", "xr_pl")]),
+           "nicam_grid_vertices")
+    return out
+
+
+def gmtr_setup(x, x_pl, xt, xt_pl, s, s_pl, have_sgp, radius: float = 6.37122e6, have_pl: bool = True):
+    """nicam_gmtr_setup: a dict p[8][lall][1][gall], p_pl[8][2][1][gall_pl],
+    t[5][2][lall][1][gall], t_pl[5][2][1][gall_pl], a[12][3][lall][1][gall],
+    a_pl[18][2][1][gall_pl] (the Fortran shapes reversed)."""
+    lib = load_library()
+    arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (x, x_pl, xt, xt_pl, s, s_pl)]
+    _, lall, _, gall = arrs[0].shape
+    gall_pl = arrs[1].shape[-1]
+    n1 = int(round(gall ** 0.5))
+    sgp = np.ascontiguousarray(have_sgp, dtype=np.int32)
+    if sgp.size != lall or n1 * n1 != gall:
+        raise ValueError("have_sgp needs one flag per region and gall must be a square")
+    out = {"p": np.empty((8, lall, 1, gall)), "p_pl": np.empty((8, 2, 1, gall_pl)),
+           "t": np.empty((5, 2, lall, 1, gall)), "t_pl": np.empty((5, 2, 1, gall_pl)),
+           "a": np.empty((12, 3, lall, 1, gall)), "a_pl": np.empty((18, 2, 1, gall_pl))}
+    _check(lib.nicam_gmtr_setup(n1, lall, gall_pl, int(have_pl), sgp.ctypes.data, float(radius),
+                                *[a.ctypes.data for a in arrs],
+                                *[out[k].ctypes.data for k in ("p", "p_pl", "t", "t_pl", "a", "a_pl")]),
+           "nicam_gmtr_setup")
+    return out

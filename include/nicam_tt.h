@@ -307,6 +307,37 @@ int nicam_grid_icosahedral(int glevel, int rlevel, double radius, double *GRD_x,
  * nicam_comm_data_transfer runs.  Same arguments. */
 int nicam_comm_var(nicam_ctx ctx, double *var, double *var_pl, int kmax, int vmax);
 
+/* The triangle vertices, latitude/longitude and arc midpoints of a grid whose
+ * GRD_x halo is filled (host-only): GRD_xt(gall,1,lall,2,3) (TI, TJ),
+ * GRD_xt_pl(gall_pl,1,2,3), GRD_s(gall,1,lall,2) (lat, lon),
+ * GRD_s_pl(gall_pl,1,2,2), GRD_xr(gall,1,lall,3,3) (AI, AIJ, AJ) and
+ * GRD_xr_pl(gall_pl,1,2,3).  The metrics kernel reads GRD_xt and GRD_s from
+ * its snapshot (dyn_metrics/src/main.f90:219-222) and the flux kernel reads
+ * GRD_xr (dyn_horiz_adv_flux/src/main.f90:157-158); the reference has no
+ * generator for them.  TI/TJ are the normalised centroids of the triangles
+ * (ij, ip1j, ip1jp1) and (ij, ip1jp1, ijp1), the pole ring's of (pole, v, v+1);
+ * lat/lon as VECTR_xyz2latlon (dyn_metrics/src/mod_vector.f90:56-113); the
+ * arc midpoints are the normalised midpoints of the hexagon edges whose ends
+ * GMTR_a_setup uses for HN/HT (dyn_metrics/src/mod_gmtr.f90:561-650).  Every
+ * point lies on the sphere of the given radius.  have_sgp has lall entries. */
+int nicam_grid_vertices(int gall_1d, int lall, int gall_pl, int have_pl, const int *have_sgp, double radius,
+                        const double *GRD_x, const double *GRD_x_pl, double *GRD_xt, double *GRD_xt_pl,
+                        double *GRD_s, double *GRD_s_pl, double *GRD_xr, double *GRD_xr_pl);
+
+/* GMTR_p_setup, GMTR_t_setup and GMTR_a_setup
+ * (dyn_metrics/src/mod_gmtr.f90:46-741) on the sphere with spherical
+ * polygons, the reference's configuration (dyn_metrics/src/problem_size.inc:69-78),
+ * host-only.  Inputs as the metrics kernel reads them
+ * (dyn_metrics/src/main.f90:217-222), outputs GMTR_p(gall,1,lall,8),
+ * GMTR_p_pl(gall_pl,1,2,8), GMTR_t(gall,1,lall,2,5), GMTR_t_pl(gall_pl,1,2,5),
+ * GMTR_a(gall,1,lall,3,12) and GMTR_a_pl(gall_pl,1,2,18), in the operand order
+ * of the reference, so they match its compiled kernel.  The _pl arguments are
+ * ignored when have_pl is 0. */
+int nicam_gmtr_setup(int gall_1d, int lall, int gall_pl, int have_pl, const int *have_sgp, double radius,
+                     const double *GRD_x, const double *GRD_x_pl, const double *GRD_xt, const double *GRD_xt_pl,
+                     const double *GRD_s, const double *GRD_s_pl, double *GMTR_p, double *GMTR_p_pl,
+                     double *GMTR_t, double *GMTR_t_pl, double *GMTR_a, double *GMTR_a_pl);
+
 #ifdef __cplusplus
 }
 #endif
