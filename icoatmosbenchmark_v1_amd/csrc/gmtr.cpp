@@ -473,6 +473,10 @@ extern "C" int nicam_grid_vertices(int gall_1d, int lall, int gall_pl, int have_
                 put_xt(ij, 0, unit3(X(ij, l), X(suf(i + 1, j), l), X(suf(i + 1, j + 1), l)));
                 put_xt(ij, 1, unit3(X(ij, l), X(suf(i + 1, j + 1), l), X(suf(i, j + 1), l)));
             }
+        // At a pentagon the triangle TI(gmin-1,gmin-1) is TJ(gmin,gmin-1), as
+        // GMTR_t_setup treats it (mod_gmtr.f90:312-314): the fifth wall of the
+        // pentagon then runs from xt(gmin-1,gmin-1,TJ) to xt(gmin,gmin-1,TJ).
+        if (have_sgp[l - 1]) put_xt(suf(gmin - 1, gmin - 1), 0, XT(suf(gmin, gmin - 1), 1));
         for (int g = 1; g <= (int)gall; ++g) {
             double lat, lon;
             latlon(X(g, l), lat, lon);

@@ -85,6 +85,7 @@ SIGNATURES = {
     "nicam_grid_icosahedral": (c_int, [c_int, c_int, c_double, _P, _P]),
     "nicam_grid_vertices": (c_int, [c_int, c_int, c_int, c_int, _P, c_double] + [_P] * 8),
     "nicam_gmtr_setup": (c_int, [c_int, c_int, c_int, c_int, _P, c_double] + [_P] * 12),
+    "nicam_oprt_diffusion_setup": (c_int, [c_int, c_int, c_int, c_int, _P] + [_P] * 10),
     "nicam_oprt_diffusion": (c_int, [_P] + [_P] * 10),
     "nicam_set_rdgz": (c_int, [_P, _P]),
     "nicam_set_vmetrics3d": (c_int, [_P] + [_P] * 8),
@@ -503,4 +504,26 @@ def gmtr_setup(x, x_pl, xt, xt_pl, s, s_pl, have_sgp, radius: float = 6.37122e6,
                                 *[a.ctypes.data for a in arrs],
                                 *[out[k].ctypes.data for k in ("p", "p_pl", "t", "t_pl", "a", "a_pl")]),
            "nicam_gmtr_setup")
+    return out
+
+
+def oprt_diffusion_setup(m, have_sgp, have_pl: bool = True):
+    """nicam_oprt_diffusion_setup on the metrics dict of gmtr_setup: a dict
+    coef_intp[lall][2][3][3][gall], coef_intp_pl[2][3][3][gall_pl],
+    coef_diff[lall][3][6][gall], coef_diff_pl[2][3][gall_pl-1]."""
+    lib = load_library()
+    ins = [np.ascontiguousarray(m[k], dtype=np.float64) for k in ("p", "p_pl", "t", "t_pl", "a", "a_pl")]
+    _, lall, _, gall = ins[0].shape
+    gall_pl = ins[1].shape[-1]
+    n1 = int(round(gall ** 0.5))
+    sgp = np.ascontiguousarray(have_sgp, dtype=np.int32)
+    if sgp.size != lall or n1 * n1 != gall:
+        raise ValueError("have_sgp needs one flag per region and gall must be a square")
+    out = {"coef_intp": np.empty((lall, 2, 3, 3, gall)), "coef_intp_pl": np.empty((2, 3, 3, gall_pl)),
+           "coef_diff": np.empty((lall, 3, 6, gall)), "coef_diff_pl": np.empty((2, 3, gall_pl - 1))}
+    _check(lib.nicam_oprt_diffusion_setup(n1, lall, gall_pl, int(have_pl), sgp.ctypes.data,
+                                          *[a.ctypes.data for a in ins],
+                                          *[out[k].ctypes.data for k in ("coef_intp", "coef_intp_pl", "coef_diff",
+                                                                         "coef_diff_pl")]),
+           "nicam_oprt_diffusion_setup")
     return out

@@ -338,6 +338,18 @@ int nicam_gmtr_setup(int gall_1d, int lall, int gall_pl, int have_pl, const int 
                      const double *GRD_s, const double *GRD_s_pl, double *GMTR_p, double *GMTR_p_pl,
                      double *GMTR_t, double *GMTR_t_pl, double *GMTR_a, double *GMTR_a_pl);
 
+/* OPRT_diffusion_setup (dyn_metrics/src/mod_oprt.f90:1235-1390), host-only:
+ * from the metrics of nicam_gmtr_setup, the coefficients that
+ * nicam_oprt_diffusion and nicam_oprt3d_divdamp read,
+ * coef_intp(gall,3,3,2,lall), coef_intp_pl(gall_pl,3,3,2),
+ * coef_diff(gall,6,3,lall) and coef_diff_pl(gall_pl-1,3,2), bit for bit as the
+ * reference computes them (halo entries included).  The _pl arguments are
+ * ignored when have_pl is 0. */
+int nicam_oprt_diffusion_setup(int gall_1d, int lall, int gall_pl, int have_pl, const int *have_sgp,
+                               const double *GMTR_p, const double *GMTR_p_pl, const double *GMTR_t,
+                               const double *GMTR_t_pl, const double *GMTR_a, const double *GMTR_a_pl,
+                               double *coef_intp, double *coef_intp_pl, double *coef_diff, double *coef_diff_pl);
+
 #ifdef __cplusplus
 }
 #endif

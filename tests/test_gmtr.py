@@ -170,3 +170,64 @@ def test_argument_errors(lib):
     v = runtime.grid_vertices(x, xpl, sgp, R)
     with pytest.raises(RuntimeError, match="radius"):
         runtime.gmtr_setup(x, xpl, v["xt"], v["xt_pl"], v["s"], v["s_pl"], sgp, -1.0)
+
+
+# ---------------------------------------------------------------------------
+# OPRT_diffusion_setup (dyn_metrics/src/mod_oprt.f90:1235-1390)
+# ---------------------------------------------------------------------------
+def _coefs(gl, rl):
+    n1, x, xpl, sgp, v, m = _grid(gl, rl)
+    return n1, x, sgp, v, m, runtime.oprt_diffusion_setup(m, sgp)
+
+
+@pytest.mark.ref
+@pytest.mark.parametrize("gl,rl", GRIDS)
+def test_diffusion_setup_matches_reference(lib, reflibs, gl, rl):
+    n1, x, sgp, v, m, c = _coefs(gl, rl)
+    want = reflibs.oprt_diffusion_setup(m, sgp)
+    for k in want:
+        assert np.array_equal(c[k], want[k]), k
+
+
+@pytest.mark.parametrize("gl,rl", [(4, 1), (5, 0), (6, 1)])
+def test_coef_intp_is_a_triangle_gradient(lib, gl, rl):
+    """sum_m coef_intp(m) f(x_m) over the three corners of TI/TJ is the
+    gradient of f: for the linear f = c.x/r its tangential part is c minus its
+    normal component at the triangle, to first order in the mesh size (the
+    error halves with each refinement)."""
+    n1, x, sgp, v, m, c = _coefs(gl, rl)
+    L = len(sgp)
+    s = slice(1, -1)
+    X = _sq(x[:, :, 0], n1) / R
+    cv = np.random.default_rng(1).standard_normal(3)
+    f = np.einsum("d,dlji->lji", cv, X)
+    CI = c["coef_intp"].reshape(L, 2, 3, 3, n1, n1)
+    h = 2.0 ** -gl
+    for t, corners in ((0, (f[:, s, s], f[:, s, 2:], f[:, 2:, 2:])), (1, (f[:, s, s], f[:, 2:, 2:], f[:, 2:, s]))):
+        g = np.einsum("ldmji,mlji->dlji", CI[:, t, :, :, s, s], np.stack(corners)) * R
+        n = _sq(v["xt"][:, t, :, 0], n1)[:, :, s, s] / R
+        g_tan = g - (g * n).sum(0) * n
+        c_tan = cv[:, None, None, None] - (cv[:, None, None, None] * n).sum(0) * n
+        assert np.abs(g_tan - c_tan).max() < 0.1 * h * np.abs(cv).max(), t
+
+
+@pytest.mark.parametrize("gl,rl", [(4, 1), (5, 0), (6, 1)])
+def test_coef_diff_is_a_cell_divergence(lib, gl, rl):
+    """2 sum_m coef_diff(m) is (1/A) times the sum of the outward wall normals
+    of the cell, whose tangential parts cancel: on a sphere of radius r it is
+    -2 n / r, at every hexagon, every pentagon (wall 5 is zero there) and at
+    both pole cells."""
+    n1, x, sgp, v, m, c = _coefs(gl, rl)
+    L = len(sgp)
+    s = slice(1, -1)
+    h = 2.0 ** -gl
+    cd = c["coef_diff"].reshape(L, 3, 6, n1, n1)
+    err = cd[:, :, :, s, s].sum(2) * R + (_sq(x[:, :, 0], n1)[:, :, s, s] / R).transpose(1, 0, 2, 3)
+    assert np.abs(err).max() < 0.1 * h
+    gmin = n1 + 1  # 0-based flat index of (gmin, gmin)
+    for l in np.flatnonzero(sgp):
+        assert np.all(c["coef_diff"][l, :, 4, gmin] == 0.0)
+        walls = c["coef_diff"][l, :, :, gmin]  # [3][6]
+        assert np.all(np.linalg.norm(walls[:, [0, 1, 2, 3, 5]], axis=0) > 0.0)
+    pole = c["coef_diff_pl"].sum(2) * R  # [2][3]
+    assert np.abs(pole - np.array([[0, 0, -1.0], [0, 0, 1.0]])).max() < 0.1 * h
