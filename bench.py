@@ -8,8 +8,9 @@ plus the two pole regions, FP64 throughout:
     horizontal_flux -> horizontal_limiter_thuburn -> vertical_limiter_thuburn
     -> vi_rhow_solver
 
-All inputs are seeded synthetic fields already resident in HBM when the timed
-region starts.  Prints one JSON line (metric, value, roofline, cpu_baseline...).
+The grid is the generated glevel-7 icosahedral grid with its GMTR metrics
+(the reference's dyn_metrics formulas); the fields are seeded synthetic
+fields, all resident in HBM when the timed region starts.  Prints one JSON line (metric, value, roofline, cpu_baseline...).
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline]
 """
@@ -108,7 +109,7 @@ def cpu_baseline(spec, case, regions: int, reps: int = 4):
                 seconds={k: round(v, 4) for k, v in times.items()})
 
 
-def time_extras(ctx, spec, case, stream, steps):
+def time_extras(ctx, spec, case, coef, stream, steps):
     """Per-launch times of what sits beside the step on the same gl7 grid:
     the limiter with its own Qout exchange (Qout_post = NULL), the halo fill
     of one field, OPRT_diffusion and OPRT3D_divdamp.  Reported beside the
@@ -152,7 +153,7 @@ def time_extras(ctx, spec, case, stream, steps):
     out["comm_data_transfer_1var"] = dict(ms=round(ms, 5))
     del var, var_pl
     ob = perfmodel.operator_bytes(spec)
-    f = synth.diffusion_inputs(spec, case["geo"])
+    f = synth.diffusion_inputs(spec, case["geo"], coef)
     D = up(f, drivers.DIFF_IN)
     D["dscl"], D["dscl_pl"] = z(spec.shape3), z(spec.shape_pl)
     ms = timed(lambda: ctx.oprt_diffusion(D["dscl"], D["dscl_pl"], *[D[k] for k in drivers.DIFF_IN]))
@@ -160,7 +161,7 @@ def time_extras(ctx, spec, case, stream, steps):
     out["oprt_diffusion"] = dict(ms=round(ms, 5), alg_bytes=ob["oprt_diffusion"], gbs=round(gbs, 1),
                                  frac=round(gbs / perfmodel.HBM_PEAK_GBS, 4))
     del D, f
-    f = synth.divdamp_inputs(spec, case["geo"])
+    f = synth.divdamp_inputs(spec, case["geo"], coef)
     drivers.setup_divdamp(ctx, f)
     G = up(f, drivers.DDMP_IN)
     for k in drivers.DDMP_OUT:
@@ -197,8 +198,8 @@ def main() -> int:
     dev = torch.device("cuda:0")
     spec = synth.GridSpec(66, 98, 40, seed=args.seed)
     t0 = time.perf_counter()
-    sgp = runtime.halo_map(1, spec.gall_1d)["have_sgp"].tolist()  # the rlevel-1 icosahedral regions
-    case = synth.make_case(spec, have_sgp=sgp)
+    geo, coef = synth.icosahedral_geometry(7, 1)  # the grid, its GMTR metrics and operator coefficients
+    case = synth.make_case(spec, geo=geo)
     log(f"inputs generated in {time.perf_counter() - t0:.1f} s")
     geo, vi = case["geo"], case["vi"]
     ctx = drivers.setup_context(spec, geo, vi)
@@ -282,7 +283,7 @@ def main() -> int:
                 frac=kernels[dom]["frac"], traffic=traffic)
     step_gbs = step_bytes / (ms_step * 1e-3) / 1e9
 
-    extras = None if args.no_extras else time_extras(ctx, spec, case, stream, args.steps)
+    extras = None if args.no_extras else time_extras(ctx, spec, case, coef, stream, args.steps)
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -306,7 +307,7 @@ def main() -> int:
         scaling="weak",
         vs_baseline=None,
         dtype="f64",
-        data="synthetic (seeded fields and metrics)",
+        data="synthetic (seeded fields on the generated gl7 icosahedral grid and its metrics)",
         config=dict(workload="NICAM gl7 rl1: 40 regions x 66x66 cells x 98 levels + 2 poles, FP64, "
                              "horizontal_flux + horizontal_limiter_thuburn + vertical_limiter_thuburn + vi_rhow_solver",
                     gall_1d=66, kall=98, lall=40, inner_cell_levels=inner, array_cell_levels=spec.array_cell_levels(),

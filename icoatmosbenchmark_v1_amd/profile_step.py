@@ -44,7 +44,9 @@ def main() -> int:
     del a, b
 
     spec = synth.GridSpec(args.gall_1d, args.kall, args.lall, seed=args.seed)
-    case = synth.make_case(spec)
+    lv = synth.icosahedral_levels(spec.gall_1d, spec.lall)
+    geo = synth.icosahedral_geometry(*lv)[0] if lv else None
+    case = synth.make_case(spec, geo=geo)
     ctx = drivers.setup_context(spec, case["geo"], case["vi"])
     ctx.set_stream(torch.cuda.current_stream(dev))
 

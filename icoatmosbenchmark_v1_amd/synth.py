@@ -167,6 +167,45 @@ def make_geometry(spec: GridSpec, have_sgp=None) -> Dict[str, np.ndarray]:
                 have_sgp=np.asarray(have_sgp, dtype=np.int32), points=p00)
 
 
+def icosahedral_levels(gall_1d: int, lall: int):
+    """(glevel, rlevel) when gall_1d = 2^(glevel-rlevel)+2 and lall = 10*4^rlevel
+    (all regions of an icosahedral grid on one process), else None."""
+    r = 0
+    while 10 * 4 ** r < lall:
+        r += 1
+    n = gall_1d - 2
+    if 10 * 4 ** r != lall or n < 2 or n & (n - 1):
+        return None
+    return n.bit_length() - 1 + r, r
+
+
+def icosahedral_geometry(glevel: int, rlevel: int, radius: float = RADIUS):
+    """The metrics of the real icosahedral grid, from the library's host
+    routines: positions (nicam_grid_icosahedral), triangle vertices and arc
+    midpoints (nicam_grid_vertices), GMTR_p/t/a (nicam_gmtr_setup, the
+    reference's dyn_metrics formulas) and the OPRT_diffusion coefficients
+    (nicam_oprt_diffusion_setup).  Returns (geo, coef): geo in the layout of
+    make_geometry, coef the coefficient dict of runtime.oprt_diffusion_setup.
+    For gall_1d = 2^(glevel-rlevel)+2 and lall = 10*4^rlevel."""
+    from . import runtime
+
+    n1 = 2 ** (glevel - rlevel) + 2
+    x, x_pl = runtime.grid_icosahedral(glevel, rlevel, radius)
+    sgp = runtime.halo_map(rlevel, n1)["have_sgp"]
+    v = runtime.grid_vertices(x, x_pl, sgp, radius)
+    m = runtime.gmtr_setup(x, x_pl, v["xt"], v["xt_pl"], v["s"], v["s_pl"], sgp, radius)
+    coef = runtime.oprt_diffusion_setup(m, sgp)
+    points = np.ascontiguousarray(np.moveaxis(x[:, :, 0, :], 0, -1))  # (lall, gall, 3)
+    # the unused halo corners hold no point: give them the first inner point
+    empty = np.linalg.norm(points, axis=-1) == 0.0
+    first = points[:, n1 + 1, :]
+    points = np.where(empty[..., None], first[:, None, :], points)
+    geo = dict(GRD_xr=v["xr"], GRD_xr_pl=v["xr_pl"], GMTR_p=m["p"], GMTR_p_pl=m["p_pl"], GMTR_t=m["t"],
+               GMTR_t_pl=m["t_pl"], GMTR_a=m["a"], GMTR_a_pl=m["a_pl"], have_sgp=np.asarray(sgp, dtype=np.int32),
+               points=points)
+    return geo, coef
+
+
 # ----------------------------------------------------------------------------
 # fields
 # ----------------------------------------------------------------------------
@@ -331,14 +370,14 @@ def vi_inputs(spec: GridSpec, dt: float = 60.0, vgrid=None) -> Dict[str, np.ndar
     return out
 
 
-def diffusion_inputs(spec: GridSpec, geo: Dict[str, np.ndarray]) -> Dict[str, np.ndarray]:
+def diffusion_inputs(spec: GridSpec, geo: Dict[str, np.ndarray], coef=None) -> Dict[str, np.ndarray]:
     """Scalar, diffusion coefficient and the stencil coefficients of
     OPRT_diffusion (dyn_diffusion/src/mod_oprt.f90:41-60):
     coef_intp(gall,3,3,2,lall) -> [lall][2][3][3][gall], coef_intp_pl(gall_pl,3,3,2),
     coef_diff(gall,6,3,lall) -> [lall][3][6][gall], coef_diff_pl(5,3,2).
-    The coefficients are seeded values of the size 1/dx that OPRT_setup
-    gives them (a gradient of a triangle-vertex interpolation); parity does
-    not depend on their geometry."""
+    The coefficients are those of coef (icosahedral_geometry's) when given,
+    else seeded values of the size 1/dx that OPRT_diffusion_setup gives them;
+    parity does not depend on their geometry."""
     rng = np.random.default_rng(spec.seed * 7919 + 61)
     lall, kall, gall, npl = spec.lall, spec.kall, spec.gall, spec.gall_pl
     q2 = _tracer(geo["points"], rng)
@@ -346,22 +385,28 @@ def diffusion_inputs(spec: GridSpec, geo: Dict[str, np.ndarray]) -> Dict[str, np
     scl = q2[:, None, :] * prof[None, :, None] + 0.01 * rng.standard_normal((lall, kall, gall))
     kh = 1e5 * (1.0 + 0.5 * rng.uniform(0.0, 1.0, size=(lall, kall, gall)))
     inv_dx = 1.0 / 1e5
-    return dict(scl=scl, scl_pl=rng.uniform(0.0, 1.0, size=(2, kall, npl)),
-                kh=kh, kh_pl=1e5 * (1.0 + 0.5 * rng.uniform(0.0, 1.0, size=(2, kall, npl))),
-                coef_intp=inv_dx * rng.uniform(-1.0, 1.0, size=(lall, 2, 3, 3, gall)),
-                coef_intp_pl=inv_dx * rng.uniform(-1.0, 1.0, size=(2, 3, 3, npl)),
-                coef_diff=inv_dx * rng.uniform(-1.0, 1.0, size=(lall, 3, 6, gall)),
-                coef_diff_pl=inv_dx * rng.uniform(-1.0, 1.0, size=(2, 3, 5)))
+    out = dict(scl=scl, scl_pl=rng.uniform(0.0, 1.0, size=(2, kall, npl)),
+               kh=kh, kh_pl=1e5 * (1.0 + 0.5 * rng.uniform(0.0, 1.0, size=(2, kall, npl))),
+               coef_intp=inv_dx * rng.uniform(-1.0, 1.0, size=(lall, 2, 3, 3, gall)),
+               coef_intp_pl=inv_dx * rng.uniform(-1.0, 1.0, size=(2, 3, 3, npl)),
+               coef_diff=inv_dx * rng.uniform(-1.0, 1.0, size=(lall, 3, 6, gall)),
+               coef_diff_pl=inv_dx * rng.uniform(-1.0, 1.0, size=(2, 3, 5)))
+    if coef is not None:
+        for k in ("coef_intp", "coef_intp_pl", "coef_diff", "coef_diff_pl"):
+            if np.shape(coef[k]) != out[k].shape:
+                raise ValueError(f"{k}: shape {np.shape(coef[k])}, the grid needs {out[k].shape}")
+            out[k] = np.ascontiguousarray(coef[k], dtype=np.float64)
+    return out
 
 
-def divdamp_inputs(spec: GridSpec, geo: Dict[str, np.ndarray]) -> Dict[str, np.ndarray]:
+def divdamp_inputs(spec: GridSpec, geo: Dict[str, np.ndarray], coef=None) -> Dict[str, np.ndarray]:
     """Momentum, the stencil coefficients and the vertical metrics that
     OPRT3D_divdamp reads (dyn_divdamp/src/mod_oprt3d.f90:41-84):
     VMTR_C2WfactGz(gall,kall,6,lall) -> [lall][6][kall][gall] and
     _pl(gall_pl,kall,6,2), GRD_rdgz(kall), the rest as diffusion_inputs."""
     rng = np.random.default_rng(spec.seed * 7919 + 67)
     lall, kall, gall, npl = spec.lall, spec.kall, spec.gall, spec.gall_pl
-    base = diffusion_inputs(spec, geo)
+    base = diffusion_inputs(spec, geo, coef)
     gz, _ = stretched_vgrid(kall)
     dz = np.diff(gz, append=gz[-1] + (gz[-1] - gz[-2]))
     x = geo["points"]
@@ -390,8 +435,10 @@ def divdamp_inputs(spec: GridSpec, geo: Dict[str, np.ndarray]) -> Dict[str, np.n
                 rdgz=1.0 / dz)
 
 
-def make_case(spec: GridSpec, have_sgp=None, vgrid=None) -> Dict[str, Dict[str, np.ndarray]]:
-    """All inputs of the four kernels for one grid."""
-    geo = make_geometry(spec, have_sgp)
+def make_case(spec: GridSpec, have_sgp=None, vgrid=None, geo=None) -> Dict[str, Dict[str, np.ndarray]]:
+    """All inputs of the four kernels for one grid (on the given geometry, else
+    on make_geometry's seeded patches)."""
+    if geo is None:
+        geo = make_geometry(spec, have_sgp)
     return dict(geo=geo, flux=flux_inputs(spec, geo), hlim=hlim_inputs(spec, geo),
                 vlim=vlim_inputs(spec, geo), vi=vi_inputs(spec, vgrid=vgrid))

@@ -103,11 +103,12 @@ def test_host_pointer_mode(gpu, oracle):
 
 
 def test_full_gl7_shape_sampled_regions(gpu, oracle):
-    """The benchmark shape (40 regions of 66x66, 98 levels) on the GPU; three
-    regions (with and without a singular point) and the poles are checked
-    against the oracle run on just those regions."""
+    """The benchmark workload (the gl7 icosahedral grid: 40 regions of 66x66,
+    98 levels, with its GMTR metrics) on the GPU; three regions (with and
+    without a singular point) and the poles are checked against the oracle run
+    on just those regions."""
     spec = synth.GridSpec(66, 98, 40, seed=2024)
-    case = synth.make_case(spec)
+    case = synth.make_case(spec, geo=synth.icosahedral_geometry(7, 1)[0])
     regions = [0, 17, 39]
     sub = synth.GridSpec(66, 98, len(regions), seed=2024)
     ctx = drivers.setup_context(spec, case["geo"], case["vi"])
@@ -134,6 +135,30 @@ def test_full_gl7_shape_sampled_regions(gpu, oracle):
     want = oracle.vi_rhow_solver(sub, {**pick(vi), "gz": vi["gz"], "gzh": vi["gzh"]}, 60.0, g["rdgzh"], g["afact"],
                                  g["bfact"])
     assert_parity(pick(got), want)
+
+
+def test_icosahedral_gl5_all_regions(gpu, oracle):
+    """Every region of the gl5 icosahedral grid (10 regions of 34x34, 40
+    layers, metrics from the grid) through the four kernels and both
+    operators, against the oracle on the whole grid."""
+    spec = synth.GridSpec(34, 42, 10, seed=31)
+    geo, coef = synth.icosahedral_geometry(5, 0)
+    case = synth.make_case(spec, geo=geo, vgrid=synth.uniform_vgrid(42))
+    sgp = geo["have_sgp"]
+    ctx = drivers.setup_context(spec, geo, case["vi"])
+    assert_parity(drivers.run_flux(ctx, spec, case["flux"], 360.0),
+                  oracle.horizontal_flux(spec, geo, case["flux"], 360.0))
+    assert_parity(drivers.run_hlim(ctx, spec, case["hlim"]), oracle.horizontal_limiter_thuburn(spec, sgp, case["hlim"]))
+    assert_parity(drivers.run_vlim(ctx, spec, case["vlim"]), oracle.vertical_limiter_thuburn(spec, case["vlim"]))
+    vi = case["vi"]
+    g = oracle.grd_setup(vi["gz"], vi["gzh"])
+    assert_parity(drivers.run_vi(ctx, spec, vi, 60.0),
+                  oracle.vi_rhow_solver(spec, vi, 60.0, g["rdgzh"], g["afact"], g["bfact"]))
+    f = synth.diffusion_inputs(spec, geo, coef)
+    assert_parity(drivers.run_diffusion(ctx, spec, f), oracle.oprt_diffusion(spec, sgp, f))
+    f = synth.divdamp_inputs(spec, geo, coef)
+    drivers.setup_divdamp(ctx, f)
+    assert_parity(drivers.run_divdamp(ctx, spec, f), oracle.oprt3d_divdamp(spec, sgp, f))
 
 
 # ---- every launch-shape variant computes the same bits --------------------
