@@ -86,6 +86,10 @@ SIGNATURES = {
     "nicam_grid_vertices": (c_int, [c_int, c_int, c_int, c_int, _P, c_double] + [_P] * 8),
     "nicam_gmtr_setup": (c_int, [c_int, c_int, c_int, c_int, _P, c_double] + [_P] * 12),
     "nicam_oprt_diffusion_setup": (c_int, [c_int, c_int, c_int, c_int, _P] + [_P] * 10),
+    "nicam_oprt_divergence_setup": (c_int, [c_int, c_int, c_int, c_int, _P] + [_P] * 8),
+    "nicam_oprt_rotation_setup": (c_int, [c_int, c_int, c_int, c_int, _P] + [_P] * 8),
+    "nicam_oprt_gradient_setup": (c_int, [c_int, c_int, c_int, c_int, _P] + [_P] * 8),
+    "nicam_oprt_laplacian_setup": (c_int, [c_int, c_int, c_int, c_int, _P] + [_P] * 8),
     "nicam_oprt_diffusion": (c_int, [_P] + [_P] * 10),
     "nicam_set_rdgz": (c_int, [_P, _P]),
     "nicam_set_vmetrics3d": (c_int, [_P] + [_P] * 8),
@@ -527,3 +531,30 @@ def oprt_diffusion_setup(m, have_sgp, have_pl: bool = True):
                                                                          "coef_diff_pl")]),
            "nicam_oprt_diffusion_setup")
     return out
+
+
+def oprt_coef_setup(which: str, m, have_sgp, have_pl: bool = True):
+    """nicam_oprt_{divergence,rotation,gradient,laplacian}_setup ("div", "rot",
+    "grad", "lap") on the metrics dict of gmtr_setup: (coef, coef_pl) as
+    coef[lall][3][7][gall], coef_pl[2][3][gall_pl] (coef_lap[lall][7][gall],
+    coef_lap_pl[2][gall_pl] for "lap")."""
+    names = {"div": "nicam_oprt_divergence_setup", "rot": "nicam_oprt_rotation_setup",
+             "grad": "nicam_oprt_gradient_setup", "lap": "nicam_oprt_laplacian_setup"}
+    if which not in names:
+        raise ValueError(f"which must be one of {sorted(names)}")
+    lib = load_library()
+    ins = [np.ascontiguousarray(m[k], dtype=np.float64) for k in ("p", "p_pl", "t", "t_pl", "a", "a_pl")]
+    _, lall, _, gall = ins[0].shape
+    gall_pl = ins[1].shape[-1]
+    n1 = int(round(gall ** 0.5))
+    sgp = np.ascontiguousarray(have_sgp, dtype=np.int32)
+    if sgp.size != lall or n1 * n1 != gall:
+        raise ValueError("have_sgp needs one flag per region and gall must be a square")
+    if which == "lap":
+        coef, coef_pl = np.empty((lall, 7, gall)), np.empty((2, gall_pl))
+    else:
+        coef, coef_pl = np.empty((lall, 3, 7, gall)), np.empty((2, 3, gall_pl))
+    _check(getattr(lib, names[which])(n1, lall, gall_pl, int(have_pl), sgp.ctypes.data,
+                                      *[a.ctypes.data for a in ins], coef.ctypes.data, coef_pl.ctypes.data),
+           names[which])
+    return coef, coef_pl

@@ -350,6 +350,31 @@ int nicam_oprt_diffusion_setup(int gall_1d, int lall, int gall_pl, int have_pl, 
                                const double *GMTR_t_pl, const double *GMTR_a, const double *GMTR_a_pl,
                                double *coef_intp, double *coef_intp_pl, double *coef_diff, double *coef_diff_pl);
 
+/* OPRT_divergence_setup, OPRT_rotation_setup, OPRT_gradient_setup
+ * (dyn_metrics/src/mod_oprt.f90:45-269, 272-496, 499-734), host-only: the
+ * 7-point stencil coefficients coef(gall,0:6,3,lall) and
+ * coef_pl(0:gall_pl-1,3,2) of the divergence (arc normals HN), rotation (arc
+ * tangents HT) and gradient operators, from the metrics of nicam_gmtr_setup,
+ * bit for bit as the reference computes them. */
+int nicam_oprt_divergence_setup(int gall_1d, int lall, int gall_pl, int have_pl, const int *have_sgp,
+                                const double *GMTR_p, const double *GMTR_p_pl, const double *GMTR_t,
+                                const double *GMTR_t_pl, const double *GMTR_a, const double *GMTR_a_pl,
+                                double *coef_div, double *coef_div_pl);
+int nicam_oprt_rotation_setup(int gall_1d, int lall, int gall_pl, int have_pl, const int *have_sgp,
+                              const double *GMTR_p, const double *GMTR_p_pl, const double *GMTR_t,
+                              const double *GMTR_t_pl, const double *GMTR_a, const double *GMTR_a_pl,
+                              double *coef_rot, double *coef_rot_pl);
+int nicam_oprt_gradient_setup(int gall_1d, int lall, int gall_pl, int have_pl, const int *have_sgp,
+                              const double *GMTR_p, const double *GMTR_p_pl, const double *GMTR_t,
+                              const double *GMTR_t_pl, const double *GMTR_a, const double *GMTR_a_pl,
+                              double *coef_grad, double *coef_grad_pl);
+/* OPRT_laplacian_setup (dyn_metrics/src/mod_oprt.f90:737-1232), host-only:
+ * coef_lap(gall,0:6,lall) and coef_lap_pl(0:gall_pl-1,2). */
+int nicam_oprt_laplacian_setup(int gall_1d, int lall, int gall_pl, int have_pl, const int *have_sgp,
+                               const double *GMTR_p, const double *GMTR_p_pl, const double *GMTR_t,
+                               const double *GMTR_t_pl, const double *GMTR_a, const double *GMTR_a_pl,
+                               double *coef_lap, double *coef_lap_pl);
+
 #ifdef __cplusplus
 }
 #endif

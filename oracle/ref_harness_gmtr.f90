@@ -13,12 +13,78 @@ module ref_harness_gmtr
   use mod_misc
   use ref_harness_common
   use mod_gmtr, only: GMTR_p_setup, GMTR_t_setup, GMTR_a_setup
-  use mod_oprt, only: OPRT_diffusion_setup
+  use mod_oprt, only: OPRT_diffusion_setup, OPRT_divergence_setup, OPRT_rotation_setup, &
+                      OPRT_gradient_setup, OPRT_laplacian_setup
   implicit none
   private
   public :: ref_gmtr_setup
   public :: ref_oprt_diffusion_setup
+  public :: ref_oprt_vector_setup
+  public :: ref_oprt_laplacian_setup
 contains
+
+  !> OPRT_divergence_setup (kind 1), OPRT_rotation_setup (2) or
+  !> OPRT_gradient_setup (3) (dyn_metrics/src/mod_oprt.f90:45-734), one
+  !> region per call, the poles kept from the first region's call.
+  subroutine ref_oprt_vector_setup( kind, gall_1d, lall, gall_pl, have_pl, have_sgp, &
+       GMTR_p, GMTR_p_pl, GMTR_t, GMTR_t_pl, GMTR_a, GMTR_a_pl, coef, coef_pl )     &
+       bind(C, name='ref_oprt_vector_setup')
+    integer(c_int), value       :: kind, gall_1d, lall, gall_pl, have_pl
+    integer(c_int), intent(in)  :: have_sgp(lall)
+    real(c_double), intent(in)  :: GMTR_p(gall_1d*gall_1d,1,lall,8),    GMTR_p_pl(gall_pl,1,2,8)
+    real(c_double), intent(in)  :: GMTR_t(gall_1d*gall_1d,1,lall,2,5),  GMTR_t_pl(gall_pl,1,2,5)
+    real(c_double), intent(in)  :: GMTR_a(gall_1d*gall_1d,1,lall,3,12), GMTR_a_pl(gall_pl,1,2,18)
+    real(c_double), intent(out) :: coef(gall_1d*gall_1d,0:6,3,lall), coef_pl(0:gall_pl-1,3,2)
+
+    real(RP), allocatable :: c_pl(:,:,:)
+    integer :: l
+    logical :: pl_here
+
+    allocate( c_pl(0:gall_pl-1,3,2) )
+    coef_pl(:,:,:) = 0.0_RP
+    do l = 1, lall
+       pl_here = ( have_pl /= 0 .and. l == 1 )
+       call harness_set_sizes( gall_1d, 4, gall_pl, pl_here, have_sgp(l) /= 0 )
+       select case ( kind )
+       case ( 1 )
+          call OPRT_divergence_setup( GMTR_p(:,:,l:l,:), GMTR_p_pl, GMTR_t(:,:,l:l,:,:), GMTR_t_pl, &
+                                      GMTR_a(:,:,l:l,:,:), GMTR_a_pl, coef(:,:,:,l:l), c_pl )
+       case ( 2 )
+          call OPRT_rotation_setup( GMTR_p(:,:,l:l,:), GMTR_p_pl, GMTR_t(:,:,l:l,:,:), GMTR_t_pl, &
+                                    GMTR_a(:,:,l:l,:,:), GMTR_a_pl, coef(:,:,:,l:l), c_pl )
+       case default
+          call OPRT_gradient_setup( GMTR_p(:,:,l:l,:), GMTR_p_pl, GMTR_t(:,:,l:l,:,:), GMTR_t_pl, &
+                                    GMTR_a(:,:,l:l,:,:), GMTR_a_pl, coef(:,:,:,l:l), c_pl )
+       end select
+       if ( pl_here ) coef_pl(:,:,:) = c_pl(:,:,:)
+    enddo
+  end subroutine ref_oprt_vector_setup
+
+  !> OPRT_laplacian_setup (dyn_metrics/src/mod_oprt.f90:737-1232), as above.
+  subroutine ref_oprt_laplacian_setup( gall_1d, lall, gall_pl, have_pl, have_sgp, &
+       GMTR_p, GMTR_p_pl, GMTR_t, GMTR_t_pl, GMTR_a, GMTR_a_pl, coef, coef_pl ) &
+       bind(C, name='ref_oprt_laplacian_setup')
+    integer(c_int), value       :: gall_1d, lall, gall_pl, have_pl
+    integer(c_int), intent(in)  :: have_sgp(lall)
+    real(c_double), intent(in)  :: GMTR_p(gall_1d*gall_1d,1,lall,8),    GMTR_p_pl(gall_pl,1,2,8)
+    real(c_double), intent(in)  :: GMTR_t(gall_1d*gall_1d,1,lall,2,5),  GMTR_t_pl(gall_pl,1,2,5)
+    real(c_double), intent(in)  :: GMTR_a(gall_1d*gall_1d,1,lall,3,12), GMTR_a_pl(gall_pl,1,2,18)
+    real(c_double), intent(out) :: coef(gall_1d*gall_1d,0:6,lall), coef_pl(0:gall_pl-1,2)
+
+    real(RP), allocatable :: c_pl(:,:)
+    integer :: l
+    logical :: pl_here
+
+    allocate( c_pl(0:gall_pl-1,2) )
+    coef_pl(:,:) = 0.0_RP
+    do l = 1, lall
+       pl_here = ( have_pl /= 0 .and. l == 1 )
+       call harness_set_sizes( gall_1d, 4, gall_pl, pl_here, have_sgp(l) /= 0 )
+       call OPRT_laplacian_setup( GMTR_p(:,:,l:l,:), GMTR_p_pl, GMTR_t(:,:,l:l,:,:), GMTR_t_pl, &
+                                  GMTR_a(:,:,l:l,:,:), GMTR_a_pl, coef(:,:,l:l), c_pl )
+       if ( pl_here ) coef_pl(:,:) = c_pl(:,:)
+    enddo
+  end subroutine ref_oprt_laplacian_setup
 
   !> OPRT_diffusion_setup (dyn_metrics/src/mod_oprt.f90:1235-1390), one region
   !> per call; it zeroes its pole output on entry, so the poles are kept from

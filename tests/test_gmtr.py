@@ -257,3 +257,92 @@ def test_kernels_on_the_generated_grid_match_reference(lib, oracle, reflibs):
     want, got = reflibs.oprt3d_divdamp(spec, sgp, f), oracle.oprt3d_divdamp(spec, sgp, f)
     for k in want:
         assert np.array_equal(got[k], want[k]), k
+
+
+# ---------------------------------------------------------------------------
+# OPRT_{divergence,rotation,gradient,laplacian}_setup (mod_oprt.f90:45-1232)
+# ---------------------------------------------------------------------------
+@pytest.mark.ref
+@pytest.mark.parametrize("gl,rl", GRIDS)
+def test_stencil_setups_match_reference(lib, reflibs, gl, rl):
+    n1, x, xpl, sgp, v, m = _grid(gl, rl)
+    for which in ("div", "rot", "grad", "lap"):
+        got, want = runtime.oprt_coef_setup(which, m, sgp), reflibs.oprt_coef_setup(which, m, sgp)
+        for g_, w_, part in zip(got, want, ("regular", "pole")):
+            assert np.array_equal(g_, w_), (which, part)
+
+
+@pytest.mark.ref
+def test_stencil_setups_structure_on_exact_metrics(lib, reflibs):
+    """With small-integer metrics every product and sum is exact, so any
+    difference would be a missing, extra or misplaced term, not rounding."""
+    rng = np.random.default_rng(0)
+    n1, lall = 6, 3
+    gall = n1 * n1
+    sgp = np.array([1, 0, 1], dtype=np.int32)
+    m = {"p": np.zeros((8, lall, 1, gall)), "p_pl": np.zeros((8, 2, 1, 6)),
+         "t": np.zeros((5, 2, lall, 1, gall)), "t_pl": np.zeros((5, 2, 1, 6)),
+         "a": rng.integers(-8, 9, size=(12, 3, lall, 1, gall)).astype(np.float64),
+         "a_pl": rng.integers(-8, 9, size=(18, 2, 1, 6)).astype(np.float64)}
+    m["p"][1], m["p_pl"][1] = 12.0, 12.0
+    m["t"][1] = rng.integers(1, 4, size=(2, lall, 1, gall))
+    m["t_pl"][1] = rng.integers(1, 4, size=(2, 1, 6))
+    m["t"][2:5] = rng.integers(-3, 4, size=(3, 2, lall, 1, gall))
+    m["t_pl"][2:5] = rng.integers(-3, 4, size=(3, 2, 1, 6))
+    for which in ("div", "rot", "grad", "lap"):
+        got, want = runtime.oprt_coef_setup(which, m, sgp), reflibs.oprt_coef_setup(which, m, sgp)
+        for g_, w_ in zip(got, want):
+            assert np.array_equal(g_, w_), which
+    got, want = runtime.oprt_diffusion_setup(m, sgp), reflibs.oprt_diffusion_setup(m, sgp)
+    for k in want:
+        assert np.array_equal(got[k], want[k]), k
+
+
+@pytest.mark.parametrize("gl,rl", [(4, 1), (5, 0), (6, 1)])
+def test_stencils_are_consistent(lib, gl, rl):
+    """The 7-point stencils (ij, ip1j, ip1jp1, ijp1, im1j, im1jm1, ijm1) on
+    fields whose answers are known on a sphere of radius r, with first-order
+    error in the mesh size h = 2^-glevel:
+      gradient of f = c.x/r is c minus its normal part;
+      divergence of the tangential part of a constant c is -2 c.n / r;
+      rotation of the solid-body field w x x/r is -2 w.n / r (the arc
+      tangents HT turn clockwise);
+      laplacian of f = c.x/r is -2 f / r^2 (an l = 1 spherical harmonic),
+      away from the 12 pentagons, where the reference's stencil is not
+      consistent for this field."""
+    n1, x, xpl, sgp, v, m = _grid(gl, rl)
+    L = len(sgp)
+    s = slice(1, -1)
+    X = _sq(x[:, :, 0], n1) / R  # unit positions [d][l][j][i]
+    nbrs = [(0, 0), (0, 1), (1, 1), (1, 0), (0, -1), (-1, -1), (-1, 0)]
+
+    def at(a, k):
+        dj, di = nbrs[k]
+        return a[..., 1 + dj:n1 - 1 + dj, 1 + di:n1 - 1 + di]
+
+    cv = np.random.default_rng(3).standard_normal(3)
+    c4 = cv[:, None, None, None]
+    f = np.einsum("d,dlji->lji", cv, X)
+    n = X[:, :, s, s]
+    h = 2.0 ** -gl
+    off_pent = np.ones((L, n1 - 2, n1 - 2), dtype=bool)
+    for l in np.flatnonzero(sgp):
+        off_pent[l, 0:3, 0:3] = False
+
+    cg = runtime.oprt_coef_setup("grad", m, sgp)[0].reshape(L, 3, 7, n1, n1)[..., s, s]
+    grad = sum(cg[:, :, k] * at(f, k)[:, None] for k in range(7)) * R
+    assert np.abs(grad.transpose(1, 0, 2, 3) - (c4 - (c4 * n).sum(0) * n)).max() < h
+
+    vt = c4 - (c4 * X).sum(0) * X
+    cd = runtime.oprt_coef_setup("div", m, sgp)[0].reshape(L, 3, 7, n1, n1)[..., s, s]
+    div = sum((cd[:, :, k] * at(vt, k).transpose(1, 0, 2, 3)).sum(1) for k in range(7)) * R
+    assert np.abs(div + 2.0 * (c4 * n).sum(0)).max() < 1.2 * h
+
+    vs = np.cross(cv, X.transpose(1, 2, 3, 0)).transpose(3, 0, 1, 2)
+    cr = runtime.oprt_coef_setup("rot", m, sgp)[0].reshape(L, 3, 7, n1, n1)[..., s, s]
+    rot = sum((cr[:, :, k] * at(vs, k).transpose(1, 0, 2, 3)).sum(1) for k in range(7)) * R
+    assert np.abs(rot + 2.0 * (c4 * n).sum(0))[off_pent].max() < h
+
+    cl = runtime.oprt_coef_setup("lap", m, sgp)[0].reshape(L, 7, n1, n1)[..., s, s]
+    lap = sum(cl[:, k] * at(f, k) for k in range(7)) * R * R
+    assert np.abs(lap + 2.0 * f[:, s, s])[off_pent].max() < h
