@@ -270,6 +270,69 @@ module nicam_tt_f
        type(c_ptr), value :: s
        integer(c_size_t) :: c_strlen
      end function c_strlen
+     function nicam_gmtr_setup(gall_1d, lall, gall_pl, have_pl, have_sgp, radius, GRD_x, GRD_x_pl, GRD_xt, &
+          GRD_xt_pl, GRD_s, GRD_s_pl, GMTR_p, GMTR_p_pl, GMTR_t, GMTR_t_pl, GMTR_a, GMTR_a_pl) &
+          bind(C, name='nicam_gmtr_setup')
+       import :: c_int, c_double
+       integer(c_int), value :: gall_1d, lall, gall_pl, have_pl
+       integer(c_int), intent(in) :: have_sgp(*)
+       real(c_double), value :: radius
+       real(c_double), intent(in) :: GRD_x(*), GRD_x_pl(*), GRD_xt(*), GRD_xt_pl(*), GRD_s(*), GRD_s_pl(*)
+       real(c_double), intent(out) :: GMTR_p(*), GMTR_p_pl(*), GMTR_t(*), GMTR_t_pl(*), GMTR_a(*), GMTR_a_pl(*)
+       integer(c_int) :: nicam_gmtr_setup
+     end function nicam_gmtr_setup
+
+     function nicam_oprt_diffusion_setup(gall_1d, lall, gall_pl, have_pl, have_sgp, GMTR_p, GMTR_p_pl, &
+          GMTR_t, GMTR_t_pl, GMTR_a, GMTR_a_pl, coef_intp, coef_intp_pl, coef_diff, coef_diff_pl) &
+          bind(C, name='nicam_oprt_diffusion_setup')
+       import :: c_int, c_double
+       integer(c_int), value :: gall_1d, lall, gall_pl, have_pl
+       integer(c_int), intent(in) :: have_sgp(*)
+       real(c_double), intent(in) :: GMTR_p(*), GMTR_p_pl(*), GMTR_t(*), GMTR_t_pl(*), GMTR_a(*), GMTR_a_pl(*)
+       real(c_double), intent(out) :: coef_intp(*), coef_intp_pl(*), coef_diff(*), coef_diff_pl(*)
+       integer(c_int) :: nicam_oprt_diffusion_setup
+     end function nicam_oprt_diffusion_setup
+
+     function nicam_oprt_divergence_setup(gall_1d, lall, gall_pl, have_pl, have_sgp, GMTR_p, GMTR_p_pl, &
+          GMTR_t, GMTR_t_pl, GMTR_a, GMTR_a_pl, coef, coef_pl) bind(C, name='nicam_oprt_divergence_setup')
+       import :: c_int, c_double
+       integer(c_int), value :: gall_1d, lall, gall_pl, have_pl
+       integer(c_int), intent(in) :: have_sgp(*)
+       real(c_double), intent(in) :: GMTR_p(*), GMTR_p_pl(*), GMTR_t(*), GMTR_t_pl(*), GMTR_a(*), GMTR_a_pl(*)
+       real(c_double), intent(out) :: coef(*), coef_pl(*)
+       integer(c_int) :: nicam_oprt_divergence_setup
+     end function nicam_oprt_divergence_setup
+
+     function nicam_oprt_rotation_setup(gall_1d, lall, gall_pl, have_pl, have_sgp, GMTR_p, GMTR_p_pl, &
+          GMTR_t, GMTR_t_pl, GMTR_a, GMTR_a_pl, coef, coef_pl) bind(C, name='nicam_oprt_rotation_setup')
+       import :: c_int, c_double
+       integer(c_int), value :: gall_1d, lall, gall_pl, have_pl
+       integer(c_int), intent(in) :: have_sgp(*)
+       real(c_double), intent(in) :: GMTR_p(*), GMTR_p_pl(*), GMTR_t(*), GMTR_t_pl(*), GMTR_a(*), GMTR_a_pl(*)
+       real(c_double), intent(out) :: coef(*), coef_pl(*)
+       integer(c_int) :: nicam_oprt_rotation_setup
+     end function nicam_oprt_rotation_setup
+
+     function nicam_oprt_gradient_setup(gall_1d, lall, gall_pl, have_pl, have_sgp, GMTR_p, GMTR_p_pl, &
+          GMTR_t, GMTR_t_pl, GMTR_a, GMTR_a_pl, coef, coef_pl) bind(C, name='nicam_oprt_gradient_setup')
+       import :: c_int, c_double
+       integer(c_int), value :: gall_1d, lall, gall_pl, have_pl
+       integer(c_int), intent(in) :: have_sgp(*)
+       real(c_double), intent(in) :: GMTR_p(*), GMTR_p_pl(*), GMTR_t(*), GMTR_t_pl(*), GMTR_a(*), GMTR_a_pl(*)
+       real(c_double), intent(out) :: coef(*), coef_pl(*)
+       integer(c_int) :: nicam_oprt_gradient_setup
+     end function nicam_oprt_gradient_setup
+
+     function nicam_oprt_laplacian_setup(gall_1d, lall, gall_pl, have_pl, have_sgp, GMTR_p, GMTR_p_pl, &
+          GMTR_t, GMTR_t_pl, GMTR_a, GMTR_a_pl, coef, coef_pl) bind(C, name='nicam_oprt_laplacian_setup')
+       import :: c_int, c_double
+       integer(c_int), value :: gall_1d, lall, gall_pl, have_pl
+       integer(c_int), intent(in) :: have_sgp(*)
+       real(c_double), intent(in) :: GMTR_p(*), GMTR_p_pl(*), GMTR_t(*), GMTR_t_pl(*), GMTR_a(*), GMTR_a_pl(*)
+       real(c_double), intent(out) :: coef(*), coef_pl(*)
+       integer(c_int) :: nicam_oprt_laplacian_setup
+     end function nicam_oprt_laplacian_setup
+
   end interface
 
   public :: nicam_version, nicam_last_error, nicam_ctx_create, nicam_ctx_destroy, nicam_ctx_set_stream
@@ -280,6 +343,8 @@ module nicam_tt_f
   public :: nicam_vi_rhow_solver, nicam_ctx_synchronize, nicam_halo_map, nicam_pole_fill_map
   public :: nicam_comm_setup, nicam_comm_data_transfer, nicam_comm_var, nicam_oprt_diffusion
   public :: nicam_grid_icosahedral, nicam_set_rdgz, nicam_set_vmetrics3d, nicam_oprt3d_divdamp
+  public :: nicam_gmtr_setup, nicam_oprt_diffusion_setup, nicam_oprt_divergence_setup
+  public :: nicam_oprt_rotation_setup, nicam_oprt_gradient_setup, nicam_oprt_laplacian_setup
 
 contains
 

@@ -12,6 +12,7 @@ order the driver reads them (paths relative to kernels/NICAM/):
 * dyn_vi_rhow_solver/src/main.f90:198-229
 * dyn_diffusion/src/main.f90:96-105
 * dyn_divdamp/src/main.f90:181-206
+* dyn_metrics/src/main.f90:217-240
 
 Fields named like the kernel's outputs (``flx_h``, ``q_a``, ``Qout_prev``,
 ``q_h``, ``rhogw``) hold the outputs the driver checks its result against
@@ -73,6 +74,16 @@ LAYOUTS: Dict[str, List[Tuple[str, str, int]]] = {
         ("coef_intp", _R1, 18), ("coef_intp_pl", _P1, 9), ("coef_diff", _R1, 18), ("coef_diff_pl", "V", 15),
         ("RGSQRTH", _R, 1), ("RGSQRTH_pl", _P, 1), ("RGAM", _R, 1), ("RGAM_pl", _P, 1),
         ("RGAMH", _R, 1), ("RGAMH_pl", _P, 1), ("C2WfactGz", _R, 6), ("C2WfactGz_pl", _P, 6),
+    ],
+    # one level (k0); the pole coefficients (0:vlink, ...) span gall_pl = vlink+1 points
+    "dyn_metrics": [
+        ("GRD_x", _R1, 3), ("GRD_x_pl", _P1, 3), ("GRD_xt", _R1, 6), ("GRD_xt_pl", _P1, 3),
+        ("GRD_s", _R1, 2), ("GRD_s_pl", _P1, 2), ("GMTR_p", _R1, 8), ("GMTR_p_pl", _P1, 8),
+        ("GMTR_t", _R1, 10), ("GMTR_t_pl", _P1, 5), ("GMTR_a", _R1, 36), ("GMTR_a_pl", _P1, 18),
+        ("OPRT_coef_div", _R1, 21), ("OPRT_coef_div_pl", _P1, 3), ("OPRT_coef_rot", _R1, 21),
+        ("OPRT_coef_rot_pl", _P1, 3), ("OPRT_coef_grad", _R1, 21), ("OPRT_coef_grad_pl", _P1, 3),
+        ("OPRT_coef_lap", _R1, 7), ("OPRT_coef_lap_pl", _P1, 1), ("OPRT_coef_intp", _R1, 18),
+        ("OPRT_coef_intp_pl", _P1, 9), ("OPRT_coef_diff", _R1, 18), ("OPRT_coef_diff_pl", "V", 15),
     ],
 }
 
