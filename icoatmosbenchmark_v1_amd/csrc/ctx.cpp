@@ -705,6 +705,108 @@ int nicam_vi_rhow_solver(nicam_ctx c, double *w, double *w_pl, const double *w0,
     return stage_out(c, a, v);
 }
 
+// ---- step glue (NICAM forms; not in the reference) ----
+int nicam_oprt_gradient(nicam_ctx c, double *grad, double *grad_pl, const double *scl, const double *scl_pl,
+                        const double *coef_grad, const double *coef_grad_pl)
+{
+    NICAM_TRY(check_ctx(c));
+    NICAM_TRY(need(grad, "grad"));
+    NICAM_TRY(need(scl, "scl"));
+    NICAM_TRY(need(coef_grad, "coef_grad"));
+    if (c->d.have_pl) {
+        const void *rp[] = {grad_pl, scl_pl, coef_grad_pl};
+        for (int i = 0; i < 3; ++i) NICAM_TRY(need(rp[i], "pole array"));
+    }
+    if (c->ptr_mode == NICAM_PTR_DEVICE)
+        return launch_gradient(c, grad, PL(grad_pl), scl, PL(scl_pl), coef_grad, PL(coef_grad_pl));
+    const size_t R = c->d.nreg() * 8, P = c->d.npl() * 8;
+    const size_t CG = (size_t)c->d.gall * c->d.lall * 21 * 8, CGP = (size_t)c->d.gall_pl * 3 * 2 * 8;
+    std::vector<HostArg> a = {{grad, 3 * R, true},          {PL(grad_pl), 3 * P, true}, {scl, R, false},
+                              {PL(scl_pl), P, false},       {coef_grad, CG, false},     {PL(coef_grad_pl), CGP, false}};
+    std::vector<void *> v;
+    NICAM_TRY(stage_in(c, a, v));
+    auto D = [&](int i) { return static_cast<double *>(v[i]); };
+    NICAM_TRY(launch_gradient(c, D(0), D(1), D(2), D(3), D(4), D(5)));
+    return stage_out(c, a, v);
+}
+
+int nicam_tracer_courant(nicam_ctx c, double *ch, double *ch_pl, double *cmask, double *cmask_pl,
+                         const double *flx_h, const double *flx_h_pl, const double *rhog, const double *rhog_pl)
+{
+    NICAM_TRY(check_ctx(c));
+    const void *req[] = {ch, cmask, flx_h, rhog};
+    const char *nm[] = {"ch", "cmask", "flx_h", "rhog"};
+    for (int i = 0; i < 4; ++i) NICAM_TRY(need(req[i], nm[i]));
+    if (c->d.have_pl) {
+        const void *rp[] = {ch_pl, cmask_pl, flx_h_pl, rhog_pl};
+        for (int i = 0; i < 4; ++i) NICAM_TRY(need(rp[i], "pole array"));
+    }
+    if ((ch_pl == nullptr) != (cmask_pl == nullptr)) return fail(NICAM_ERR_ARG, "give both or none of ch_pl, cmask_pl");
+    if (c->ptr_mode == NICAM_PTR_DEVICE)
+        return launch_courant(c, ch, ch_pl, cmask, cmask_pl, flx_h, PL(flx_h_pl), rhog, PL(rhog_pl));
+    const size_t R = c->d.nreg() * 8, P = c->d.npl() * 8;
+    std::vector<HostArg> a = {{ch, 6 * R, true},    {ch_pl, P, true},         {cmask, 6 * R, true},
+                              {cmask_pl, P, true},  {flx_h, 6 * R, false},    {PL(flx_h_pl), P, false},
+                              {rhog, R, false},     {PL(rhog_pl), P, false}};
+    std::vector<void *> v;
+    NICAM_TRY(stage_in(c, a, v));
+    auto D = [&](int i) { return static_cast<double *>(v[i]); };
+    NICAM_TRY(launch_courant(c, D(0), D(1), D(2), D(3), D(4), D(5), D(6), D(7)));
+    return stage_out(c, a, v);
+}
+
+int nicam_horizontal_remap(nicam_ctx c, double *q_a, double *q_a_pl, const double *q, const double *q_pl,
+                           const double *grad, const double *grad_pl, const double *cmask, const double *cmask_pl,
+                           const double *GRD_xc, const double *GRD_xc_pl, const double *GRD_x,
+                           const double *GRD_x_pl)
+{
+    NICAM_TRY(check_ctx(c));
+    const void *req[] = {q_a, q, grad, cmask, GRD_xc, GRD_x};
+    const char *nm[] = {"q_a", "q", "grad", "cmask", "GRD_xc", "GRD_x"};
+    for (int i = 0; i < 6; ++i) NICAM_TRY(need(req[i], nm[i]));
+    if (c->d.have_pl) {
+        const void *rp[] = {q_a_pl, q_pl, grad_pl, cmask_pl, GRD_xc_pl, GRD_x_pl};
+        for (int i = 0; i < 6; ++i) NICAM_TRY(need(rp[i], "pole array"));
+    }
+    if (c->ptr_mode == NICAM_PTR_DEVICE)
+        return launch_remap(c, q_a, PL(q_a_pl), q, PL(q_pl), grad, PL(grad_pl), cmask, PL(cmask_pl), GRD_xc,
+                            PL(GRD_xc_pl), GRD_x, PL(GRD_x_pl));
+    const size_t R = c->d.nreg() * 8, P = c->d.npl() * 8;
+    const size_t X = (size_t)c->d.gall * c->d.lall * 3 * 8, XP = (size_t)c->d.gall_pl * 2 * 3 * 8;
+    std::vector<HostArg> a = {{q_a, 6 * R, true},          {PL(q_a_pl), P, true},      {q, R, false},
+                              {PL(q_pl), P, false},        {grad, 3 * R, false},       {PL(grad_pl), 3 * P, false},
+                              {cmask, 6 * R, false},       {PL(cmask_pl), P, false},   {GRD_xc, 9 * R, false},
+                              {PL(GRD_xc_pl), 3 * P, false}, {GRD_x, X, false},       {PL(GRD_x_pl), XP, false}};
+    std::vector<void *> v;
+    NICAM_TRY(stage_in(c, a, v));
+    auto D = [&](int i) { return static_cast<double *>(v[i]); };
+    NICAM_TRY(launch_remap(c, D(0), D(1), D(2), D(3), D(4), D(5), D(6), D(7), D(8), D(9), D(10), D(11)));
+    return stage_out(c, a, v);
+}
+
+int nicam_tracer_update(nicam_ctx c, double *rhogq, double *rhogq_pl, const double *flx_h, const double *flx_h_pl,
+                        const double *q_a, const double *q_a_pl)
+{
+    NICAM_TRY(check_ctx(c));
+    const void *req[] = {rhogq, flx_h, q_a};
+    const char *nm[] = {"rhogq", "flx_h", "q_a"};
+    for (int i = 0; i < 3; ++i) NICAM_TRY(need(req[i], nm[i]));
+    if (c->d.have_pl) {
+        const void *rp[] = {rhogq_pl, flx_h_pl, q_a_pl};
+        for (int i = 0; i < 3; ++i) NICAM_TRY(need(rp[i], "pole array"));
+    }
+    if (c->ptr_mode == NICAM_PTR_DEVICE)
+        return launch_update(c, rhogq, PL(rhogq_pl), flx_h, PL(flx_h_pl), q_a, PL(q_a_pl));
+    const size_t R = c->d.nreg() * 8, P = c->d.npl() * 8;
+    std::vector<HostArg> a = {{rhogq, R, true},        {PL(rhogq_pl), P, true}, {flx_h, 6 * R, false},
+                              {PL(flx_h_pl), P, false}, {q_a, 6 * R, false},     {PL(q_a_pl), P, false}};
+    std::vector<void *> v;
+    NICAM_TRY(stage_in(c, a, v));
+    auto D = [&](int i) { return static_cast<double *>(v[i]); };
+    NICAM_TRY(launch_update(c, D(0), D(1), D(2), D(3), D(4), D(5)));
+    return stage_out(c, a, v);
+}
+
 #undef PL
 
 }  // extern "C"

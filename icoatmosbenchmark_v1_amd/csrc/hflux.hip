@@ -38,10 +38,54 @@ struct HfluxArgs {
     const int *have_sgp;
     int iall, gall, kall, lall;
     int nchunk, nkr, kper;  // chunks per slab, level ranges, levels per range
+    int nmain;              // work-groups of the regular regions; the rest do the poles
     double dt, eps;
+    // pole regions (FLX:326-374)
+    double *flx_pl, *xc_pl;
+    const double *f_pl[4];
+    const double *xr_pl, *gp_pl, *gt_pl, *ga_pl;
+    int gall_pl;
 };
 
 __device__ __forceinline__ double dmax(double a, double b) { return (b > a) ? b : a; }
+
+// Pole regions: one work-item per (k, l) walks the ring (FLX:326-374).  Ring
+// point v takes the TI value of v-1 (of the last point for v = 1), which is
+// carried from the previous step instead of stored: no per-lane array.
+__device__ void hflux_pole(const HfluxArgs &A, int t)
+{
+    const int kall = A.kall, npl = A.gall_pl;
+    if (t >= kall * kLallPl) return;
+    const int k = t % kall, l = t / kall;
+    const size_t fb = ((size_t)l * kall + k) * npl;
+    // GMTR_*_pl(v,1,l,c): index (c*2 + l)*gall_pl + v
+    auto M = [&](const double *m, int c, int v) { return m[((size_t)c * kLallPl + l) * npl + v]; };
+    auto tvp = [&](int v, double (&tv)[4]) {
+        const int vp1 = (v + 1 == npl) ? 1 : v + 1;
+        const double w1 = M(A.gt_pl, 2, v), w2 = M(A.gt_pl, 3, v), w3 = M(A.gt_pl, 4, v);
+#pragma unroll
+        for (int f = 0; f < 4; ++f)
+            tv[f] = A.f_pl[f][fb + 0] * w1 + A.f_pl[f][fb + v] * w2 + A.f_pl[f][fb + vp1] * w3;
+    };
+    double prev[4], cur[4];
+    tvp(npl - 1, prev);
+    const double rarea = M(A.gp_pl, 1, 0);
+    const size_t plane = (size_t)kLallPl * kall * npl;
+    for (int v = 1; v < npl; ++v) {
+        tvp(v, cur);
+        const double rr2 = 1.0 / dmax(prev[0] + cur[0], A.eps);
+        const double x2 = prev[1] + cur[1];
+        const double y2 = prev[2] + cur[2];
+        const double z2 = prev[3] + cur[3];
+        const double flux = 0.5 * (x2 * M(A.ga_pl, 0, v) + y2 * M(A.ga_pl, 1, v) + z2 * M(A.ga_pl, 2, v));
+        A.flx_pl[fb + v] = flux * rarea * A.dt;
+        A.xc_pl[0 * plane + fb + v] = M(A.xr_pl, 0, v) - x2 * rr2 * A.dt * 0.5;
+        A.xc_pl[1 * plane + fb + v] = M(A.xr_pl, 1, v) - y2 * rr2 * A.dt * 0.5;
+        A.xc_pl[2 * plane + fb + v] = M(A.xr_pl, 2, v) - z2 * rr2 * A.dt * 0.5;
+#pragma unroll
+        for (int f = 0; f < 4; ++f) prev[f] = cur[f];
+    }
+}
 
 __global__ void __launch_bounds__(CHUNK) hflux_kernel(HfluxArgs A)
 {
@@ -56,8 +100,12 @@ __global__ void __launch_bounds__(CHUNK) hflux_kernel(HfluxArgs A)
     // and level range are consecutive and are kept on one XCD (hardware
     // dispatch sends block b to XCD b % 8) so their shared halo rows hit L2.
     int b = blockIdx.x;
+    if (b >= A.nmain) {  // pole work-groups
+        hflux_pole(A, (b - A.nmain) * CHUNK + threadIdx.x);
+        return;
+    }
     {
-        const int nxcd = 8, per = gridDim.x / nxcd;
+        const int nxcd = 8, per = A.nmain / nxcd;
         if (per > 0 && b < per * nxcd) b = (b % nxcd) * per + b / nxcd;
     }
     const int chunk = b % A.nchunk;
@@ -231,54 +279,13 @@ __global__ void __launch_bounds__(CHUNK) hflux_kernel(HfluxArgs A)
 
             const size_t fb = ((size_t)l * kall + k) * slab + g;
 #pragma unroll
-            for (int m = 0; m < 6; ++m) stream_store(&A.flx[m * plane + fb], face[m]);
+            for (int m = 0; m < 6; ++m) A.flx[m * plane + fb] = face[m];
 #pragma unroll
             for (int d = 0; d < 3; ++d)
 #pragma unroll
-                for (int a = 0; a < 3; ++a) stream_store(&A.xc[(size_t)(d * 3 + a) * plane + fb], xcv[a][d]);
+                for (int a = 0; a < 3; ++a) A.xc[(size_t)(d * 3 + a) * plane + fb] = xcv[a][d];
         }
         __syncthreads();
-    }
-}
-
-// Pole regions: one work-item per (k, l) walks the ring (FLX:326-374).
-struct HfluxPoleArgs {
-    double *flx_pl, *xc_pl;
-    const double *f[4];
-    const double *xr, *gp, *gt, *ga;
-    int gall_pl, kall;
-    double dt, eps;
-};
-
-__global__ void hflux_pole_kernel(HfluxPoleArgs A)
-{
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    const int kall = A.kall, npl = A.gall_pl;
-    if (t >= kall * kLallPl) return;
-    const int k = t % kall, l = t / kall;
-    const size_t fb = ((size_t)l * kall + k) * npl;
-    // GMTR_*_pl(v,1,l,c): index (c*2 + l)*gall_pl + v
-    auto M = [&](const double *m, int c, int v) { return m[((size_t)c * kLallPl + l) * npl + v]; };
-    double tv[16][4];
-    for (int v = 1; v < npl; ++v) {
-        const int vp1 = (v + 1 == npl) ? 1 : v + 1;
-        const double w1 = M(A.gt, 2, v), w2 = M(A.gt, 3, v), w3 = M(A.gt, 4, v);
-        for (int f = 0; f < 4; ++f)
-            tv[v][f] = A.f[f][fb + 0] * w1 + A.f[f][fb + v] * w2 + A.f[f][fb + vp1] * w3;
-    }
-    const double rarea = M(A.gp, 1, 0);
-    const size_t plane = (size_t)kLallPl * kall * npl;
-    for (int v = 1; v < npl; ++v) {
-        const int vm1 = (v - 1 == 0) ? npl - 1 : v - 1;
-        const double rr2 = 1.0 / dmax(tv[vm1][0] + tv[v][0], A.eps);
-        const double x2 = tv[vm1][1] + tv[v][1];
-        const double y2 = tv[vm1][2] + tv[v][2];
-        const double z2 = tv[vm1][3] + tv[v][3];
-        const double flux = 0.5 * (x2 * M(A.ga, 0, v) + y2 * M(A.ga, 1, v) + z2 * M(A.ga, 2, v));
-        A.flx_pl[fb + v] = flux * rarea * A.dt;
-        A.xc_pl[0 * plane + fb + v] = M(A.xr, 0, v) - x2 * rr2 * A.dt * 0.5;
-        A.xc_pl[1 * plane + fb + v] = M(A.xr, 1, v) - y2 * rr2 * A.dt * 0.5;
-        A.xc_pl[2 * plane + fb + v] = M(A.xr, 2, v) - z2 * rr2 * A.dt * 0.5;
     }
 }
 
@@ -317,33 +324,26 @@ int launch_hflux(nicam_ctx c, double *flx_h, double *flx_h_pl, double *xc, doubl
     if (c->tune.hflux_kper > 0) nkr = (D.kall + c->tune.hflux_kper - 1) / c->tune.hflux_kper;
     A.kper = (D.kall + nkr - 1) / nkr;
     A.nkr = (D.kall + A.kper - 1) / A.kper;
-    const long nblk = base * A.nkr;
-    if (nblk > 0x7fffffffL) return fail(NICAM_ERR_ARG, "grid too large for one launch");
+    const long nmain = base * A.nkr;
+    const long npole = D.have_pl ? (D.kall * kLallPl + CHUNK - 1) / CHUNK : 0;
+    if (nmain + npole > 0x7fffffffL) return fail(NICAM_ERR_ARG, "grid too large for one launch");
+    A.nmain = (int)nmain;
+    A.flx_pl = flx_h_pl;
+    A.xc_pl = xc_pl;
+    A.f_pl[0] = rho_pl;
+    A.f_pl[1] = rvx_pl;
+    A.f_pl[2] = rvy_pl;
+    A.f_pl[3] = rvz_pl;
+    A.xr_pl = c->GRD_xr_pl;
+    A.gp_pl = c->GMTR_p_pl;
+    A.gt_pl = c->GMTR_t_pl;
+    A.ga_pl = c->GMTR_a_pl;
+    A.gall_pl = D.gall_pl;
     const size_t lds = sizeof(double) * (4 * (CHUNK + 2 * (D.gall_1d + 1)) + 8 * (CHUNK + D.gall_1d + 1));
     if (lds > 64 * 1024)
         NICAM_HIP(hipFuncSetAttribute((const void *)hflux_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    hipLaunchKernelGGL(hflux_kernel, dim3((unsigned)nblk), dim3(CHUNK), lds, c->stream, A);
+    hipLaunchKernelGGL(hflux_kernel, dim3((unsigned)(nmain + npole)), dim3(CHUNK), lds, c->stream, A);
     NICAM_HIP(hipGetLastError());
-    if (D.have_pl) {
-        HfluxPoleArgs P;
-        P.flx_pl = flx_h_pl;
-        P.xc_pl = xc_pl;
-        P.f[0] = rho_pl;
-        P.f[1] = rvx_pl;
-        P.f[2] = rvy_pl;
-        P.f[3] = rvz_pl;
-        P.xr = c->GRD_xr_pl;
-        P.gp = c->GMTR_p_pl;
-        P.gt = c->GMTR_t_pl;
-        P.ga = c->GMTR_a_pl;
-        P.gall_pl = D.gall_pl;
-        P.kall = D.kall;
-        P.dt = dt;
-        P.eps = c->eps;
-        const int n = D.kall * kLallPl;
-        hipLaunchKernelGGL(hflux_pole_kernel, dim3((n + 63) / 64), dim3(64), 0, c->stream, P);
-        NICAM_HIP(hipGetLastError());
-    }
     return NICAM_OK;
 }
 

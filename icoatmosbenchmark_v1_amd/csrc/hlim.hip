@@ -35,8 +35,15 @@ struct HlimArgs {
     const int *have_sgp;
     int iall, gall, kall, lall;
     int phases;  // 1: allowable range only, 2: clamp only, 3: both
+    int nmain;   // work-groups of the regular regions; the rest do the poles
     double eps, big;
+    // pole regions (HLIM:307-363, 367-373, 412-425)
+    double *q_a_pl, *Qprev_pl;
+    const double *q_pl, *d_pl, *ch_pl, *cmask_pl, *Qpost_pl;
+    int gall_pl, have_pl;
 };
+
+__device__ void hlim_pole(const HlimArgs &A, int t);
 
 __global__ void __launch_bounds__(CHUNK) hlim_kernel(HlimArgs A)
 {
@@ -47,8 +54,12 @@ __global__ void __launch_bounds__(CHUNK) hlim_kernel(HlimArgs A)
     const int iall = A.iall, gall = A.gall, kall = A.kall, lall = A.lall;
     const int nchunk = (gall + CHUNK - 1) / CHUNK;
     int b = blockIdx.x;
+    if (b >= A.nmain) {  // pole work-groups
+        hlim_pole(A, (b - A.nmain) * CHUNK + threadIdx.x);
+        return;
+    }
     {
-        const int nxcd = 8, per = gridDim.x / nxcd;
+        const int nxcd = 8, per = A.nmain / nxcd;
         if (per > 0 && b < per * nxcd) b = (b % nxcd) * per + b / nxcd;
     }
     const int chunk = b % nchunk;
@@ -105,7 +116,7 @@ __global__ void __launch_bounds__(CHUNK) hlim_kernel(HlimArgs A)
 #pragma unroll
         for (int m = 0; m < 3; ++m) {
             arc_minmax(m, g, omn[m], omx[m]);
-            ocm[m] = stream_load(&A.cmask[(size_t)m * plane + fb + g]);
+            ocm[m] = A.cmask[(size_t)m * plane + fb + g];
         }
     }
 
@@ -138,7 +149,7 @@ __global__ void __launch_bounds__(CHUNK) hlim_kernel(HlimArgs A)
         double chv[6], chm[6];
 #pragma unroll
         for (int f = 0; f < 6; ++f) {
-            chv[f] = stream_load(&A.ch[(size_t)f * plane + fb + g]);
+            chv[f] = A.ch[(size_t)f * plane + fb + g];
             chm[f] = dmin(chv[f], 0.0);
         }
         const double Cin_sum = chm[0] + chm[1] + chm[2] + chm[3] + chm[4] + chm[5];
@@ -185,19 +196,10 @@ __global__ void __launch_bounds__(CHUNK) hlim_kernel(HlimArgs A)
 }
 
 // Pole regions: one work-item per (k, l) (HLIM:307-363, 367-373, 412-425).
-struct HlimPoleArgs {
-    double *q_a;
-    const double *q, *d, *ch, *cmask;
-    double *Qprev;
-    const double *Qpost;
-    int gall_pl, kall, have_pl;
-    int phases;
-    double eps, big;
-};
-
-__global__ void hlim_pole_kernel(HlimPoleArgs A)
+// Qin of ring point v depends only on q around it and its cmask, so it is
+// recomputed in each of the three passes over the ring instead of stored.
+__device__ void hlim_pole(const HlimArgs &A, int t)
 {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
     const int kall = A.kall, npl = A.gall_pl;
     if (t >= kall * kLallPl) return;
     const int k = t % kall, l = t / kall;
@@ -206,60 +208,66 @@ __global__ void hlim_pole_kernel(HlimPoleArgs A)
     if (!A.have_pl) {  // only the ring zeroing of :367-370 happens
         if (!(A.phases & 1)) return;
         for (int v = 1; v < npl; ++v) {
-            A.Qprev[fb + v] = 0.0;
-            A.Qprev[plane + fb + v] = 0.0;
+            A.Qprev_pl[fb + v] = 0.0;
+            A.Qprev_pl[plane + fb + v] = 0.0;
         }
         return;
     }
     const double BIG = A.big;
-    double qin[16][2][2];  // [v][min,max][side 1,2]
-    for (int v = 1; v < npl; ++v) {
+    // [min,max][side 1,2] of ring point v
+    auto qin = [&](int v, double &mn1, double &mn2, double &mx1, double &mx2) {
         int vp1 = v + 1, vm1 = v - 1;
         if (vp1 == npl) vp1 = 1;
         if (vm1 == 0) vm1 = npl - 1;
-        const double a = A.q[fb + 0], b = A.q[fb + v], c2 = A.q[fb + vm1], d2 = A.q[fb + vp1];
+        const double a = A.q_pl[fb + 0], b = A.q_pl[fb + v], c2 = A.q_pl[fb + vm1], d2 = A.q_pl[fb + vp1];
         const double mn = dmin(dmin(dmin(a, b), c2), d2);
         const double mx = dmax(dmax(dmax(a, b), c2), d2);
-        const double c = A.cmask[fb + v];
-        qin[v][0][0] = (c)*mn + (1.0 - c) * BIG;
-        qin[v][0][1] = (c)*BIG + (1.0 - c) * mn;
-        qin[v][1][0] = (c)*mx + (1.0 - c) * (-BIG);
-        qin[v][1][1] = (c) * (-BIG) + (1.0 - c) * mx;
-    }
-    const double qn = A.q[fb + 0];
+        const double c = A.cmask_pl[fb + v];
+        mn1 = (c)*mn + (1.0 - c) * BIG;
+        mn2 = (c)*BIG + (1.0 - c) * mn;
+        mx1 = (c)*mx + (1.0 - c) * (-BIG);
+        mx2 = (c) * (-BIG) + (1.0 - c) * mx;
+    };
+    const double qn = A.q_pl[fb + 0];
     double qnext_min = qn, qnext_max = qn;
     for (int v = 1; v < npl; ++v) {
-        qnext_min = dmin(qnext_min, qin[v][0][0]);
-        qnext_max = dmax(qnext_max, qin[v][1][0]);
+        double mn1, mn2, mx1, mx2;
+        qin(v, mn1, mn2, mx1, mx2);
+        qnext_min = dmin(qnext_min, mn1);
+        qnext_max = dmax(qnext_max, mx1);
     }
     double Cin = 0.0, Cout = 0.0, CQmax = 0.0, CQmin = 0.0;
     for (int v = 1; v < npl; ++v) {
-        const double chm = A.cmask[fb + v] * A.ch[fb + v];
+        double mn1, mn2, mx1, mx2;
+        qin(v, mn1, mn2, mx1, mx2);
+        const double chm = A.cmask_pl[fb + v] * A.ch_pl[fb + v];
         Cin = Cin + chm;
-        Cout = Cout - chm + A.ch[fb + v];
-        CQmin = CQmin + chm * qin[v][0][0];
-        CQmax = CQmax + chm * qin[v][1][0];
+        Cout = Cout - chm + A.ch_pl[fb + v];
+        CQmin = CQmin + chm * mn1;
+        CQmax = CQmax + chm * mx1;
     }
     const double zerosw = 0.5 - sign_half(fabs(Cout) - A.eps);
-    const double dn = A.d[fb + 0];
+    const double dn = A.d_pl[fb + 0];
     const double qo_min = (qn - CQmax - qnext_max * (1.0 - Cin - Cout + dn)) / (Cout + zerosw) * (1.0 - zerosw) + qn * zerosw;
     const double qo_max = (qn - CQmin - qnext_min * (1.0 - Cin - Cout + dn)) / (Cout + zerosw) * (1.0 - zerosw) + qn * zerosw;
     if (A.phases & 1) {
-        A.Qprev[fb + 0] = qo_min;
-        A.Qprev[plane + fb + 0] = qo_max;
+        A.Qprev_pl[fb + 0] = qo_min;
+        A.Qprev_pl[plane + fb + 0] = qo_max;
         for (int v = 1; v < npl; ++v) {
-            A.Qprev[fb + v] = 0.0;
-            A.Qprev[plane + fb + v] = 0.0;
+            A.Qprev_pl[fb + v] = 0.0;
+            A.Qprev_pl[plane + fb + v] = 0.0;
         }
     }
     if (!(A.phases & 2)) return;
-    const double pn_min = A.Qpost[fb + 0], pn_max = A.Qpost[plane + fb + 0];
+    const double pn_min = A.Qpost_pl[fb + 0], pn_max = A.Qpost_pl[plane + fb + 0];
     for (int v = 1; v < npl; ++v) {
-        const double c = A.cmask[fb + v];
-        double qa = A.q_a[fb + v];
-        qa = (c)*dmin(dmax(qa, qin[v][0][0]), qin[v][1][0]) + (1.0 - c) * dmin(dmax(qa, qin[v][0][1]), qin[v][1][1]);
-        qa = (c)*dmax(dmin(qa, A.Qpost[plane + fb + v]), A.Qpost[fb + v]) + (1.0 - c) * dmax(dmin(qa, pn_max), pn_min);
-        A.q_a[fb + v] = qa;
+        double mn1, mn2, mx1, mx2;
+        qin(v, mn1, mn2, mx1, mx2);
+        const double c = A.cmask_pl[fb + v];
+        double qa = A.q_a_pl[fb + v];
+        qa = (c)*dmin(dmax(qa, mn1), mx1) + (1.0 - c) * dmin(dmax(qa, mn2), mx2);
+        qa = (c)*dmax(dmin(qa, A.Qpost_pl[plane + fb + v]), A.Qpost_pl[fb + v]) + (1.0 - c) * dmax(dmin(qa, pn_max), pn_min);
+        A.q_a_pl[fb + v] = qa;
     }
 }
 
@@ -287,31 +295,23 @@ int launch_hlim(nicam_ctx c, double *q_a, double *q_a_pl, const double *q, const
     A.lall = D.lall;
     A.eps = c->eps;
     A.big = c->big;
-    const long nblk = (long)((D.gall + CHUNK - 1) / CHUNK) * D.kall * D.lall;
-    if (nblk > 0x7fffffffL) return fail(NICAM_ERR_ARG, "grid too large for one launch");
-    HlimPoleArgs P;
-    P.q_a = q_a_pl;
-    P.q = q_pl;
-    P.d = d_pl;
-    P.ch = ch_pl;
-    P.cmask = cmask_pl;
-    P.Qprev = Qprev_pl;
-    P.Qpost = Qpost_pl;
-    P.gall_pl = D.gall_pl;
-    P.kall = D.kall;
-    P.have_pl = D.have_pl;
-    P.eps = c->eps;
-    P.big = c->big;
-    const int npole = D.kall * kLallPl;
+    const long nmain = (long)((D.gall + CHUNK - 1) / CHUNK) * D.kall * D.lall;
+    const long npole = Qprev_pl ? (D.kall * kLallPl + CHUNK - 1) / CHUNK : 0;
+    if (nmain + npole > 0x7fffffffL) return fail(NICAM_ERR_ARG, "grid too large for one launch");
+    A.nmain = (int)nmain;
+    A.q_a_pl = q_a_pl;
+    A.q_pl = q_pl;
+    A.d_pl = d_pl;
+    A.ch_pl = ch_pl;
+    A.cmask_pl = cmask_pl;
+    A.Qprev_pl = Qprev_pl;
+    A.Qpost_pl = Qpost_pl;
+    A.gall_pl = D.gall_pl;
+    A.have_pl = D.have_pl;
     auto run = [&](int phases) -> int {
         A.phases = phases;
-        P.phases = phases;
-        hipLaunchKernelGGL(hlim_kernel, dim3((unsigned)nblk), dim3(CHUNK), 0, c->stream, A);
+        hipLaunchKernelGGL(hlim_kernel, dim3((unsigned)(nmain + npole)), dim3(CHUNK), 0, c->stream, A);
         NICAM_HIP(hipGetLastError());
-        if (P.Qprev) {
-            hipLaunchKernelGGL(hlim_pole_kernel, dim3((npole + 63) / 64), dim3(64), 0, c->stream, P);
-            NICAM_HIP(hipGetLastError());
-        }
         return NICAM_OK;
     };
     if (Qpost) return run(3);  // the reference's kernel: Qout_post supplied
@@ -328,7 +328,7 @@ int launch_hlim(nicam_ctx c, double *q_a, double *q_a_pl, const double *q, const
     rc = launch_halo_fill(c, Qprev, Qprev_pl, D.kall, 2, false);
     if (rc != NICAM_OK) return rc;
     A.Qpost = Qprev;
-    P.Qpost = Qprev_pl;
+    A.Qpost_pl = Qprev_pl;
     return run(2);
 }
 

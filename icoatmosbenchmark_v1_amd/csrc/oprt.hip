@@ -8,10 +8,10 @@
 // hexagon in three passes, one per component (:138-216).  Here a work-group
 // owns CHUNK consecutive cells of one region and marches up a range of
 // levels.  Its coefficients (coef_intp of the vt points it forms, coef_diff
-// of its cells) are K-invariant and stay in registers.  At each level the
-// group stages scl and kh of the cells its stencils touch in LDS, forms vt
-// of cells [g0-gall_1d-1, g0+CHUNK) in LDS, and each work-item then produces
-// its cell's dscl.  The expressions keep the reference's operand order
+// of its cells) are K-invariant and are read once.  At each level the group
+// stages scl and kh of the cells its stencils touch in LDS, forms vt of cells
+// [g0-gall_1d-1, g0+CHUNK) in LDS, and each work-item then produces its
+// cell's dscl.  The expressions keep the reference's operand order
 // (including the X component that the Y and Z passes read as the first point
 // of each pair), so with -ffp-contract=off the FP64 results are the
 // reference's to the last bit.  The pole regions (:220-291) are extra
@@ -22,9 +22,12 @@ namespace nicam {
 namespace {
 
 constexpr int CHUNK = 256;  // output cells per work-group
-constexpr int MAXI = 255;   // largest gall_1d: CHUNK*NTP >= CHUNK + MAXI + 1
-constexpr int NTP = 2;      // vt points per work-item
-constexpr int NST = 3;      // staged cells per work-item: CHUNK*NST >= CHUNK + 2*MAXI + 2
+constexpr int MAXI = 255;   // largest gall_1d: the gall_1d+1 vt points below a chunk fit one slot
+constexpr int NPL = 6;      // pole ring: the centre and ADM_vlink = 5 points (problem_size.inc:41)
+#ifndef NICAM_DIFF_DEPTH
+#define NICAM_DIFF_DEPTH 4
+#endif
+constexpr int DEPTH = NICAM_DIFF_DEPTH;  // levels of OPRT_diffusion input in flight per work-item
 
 struct DiffArgs {
     double *dscl, *dscl_pl;
@@ -36,12 +39,25 @@ struct DiffArgs {
     int nchunk, nkr, kper, nmain;
 };
 
-// vt of one triangle vertex: the reference's expression, term by term
+// The three factors of a triangle vertex's vt (:97-125), from its coef_intp
+// c1..c3: the reference's expressions, evaluated once per column (they do not
+// depend on the level, and each evaluation gives the same bits).
+__device__ __forceinline__ void vt_factors(double c1, double c2, double c3, double &e0, double &e1, double &e2)
+{
+    e0 = +2.0 * c1 - 1.0 * c2 - 1.0 * c3;
+    e1 = -1.0 * c1 + 2.0 * c2 - 1.0 * c3;
+    e2 = -1.0 * c1 - 1.0 * c2 + 2.0 * c3;
+}
+// vt = (e0*s0 + e1*s1 + e2*s2) / 3
+__device__ __forceinline__ double vt_value(double e0, double e1, double e2, double s0, double s1, double s2)
+{
+    return div3(e0 * s0 + e1 * s1 + e2 * s2);
+}
 __device__ __forceinline__ double vt_point(double c1, double c2, double c3, double s0, double s1, double s2)
 {
-    return ((+2.0 * c1 - 1.0 * c2 - 1.0 * c3) * s0 + (-1.0 * c1 + 2.0 * c2 - 1.0 * c3) * s1 +
-            (-1.0 * c1 - 1.0 * c2 + 2.0 * c3) * s2) /
-           3.0;
+    double e0, e1, e2;
+    vt_factors(c1, c2, c3, e0, e1, e2);
+    return vt_value(e0, e1, e2, s0, s1, s2);
 }
 
 __device__ void diffusion_pole(const DiffArgs &A, int t)
@@ -55,34 +71,51 @@ __device__ void diffusion_pole(const DiffArgs &A, int t)
         for (int v = 0; v < npl; ++v) A.dscl_pl[fb + v] = 0.0;
         return;
     }
-    const int gmin_pl = 1, gmax_pl = npl - 1;  // 0-based ring 1..npl-1, centre 0
-    double vt[16][3];
+    // ring points 1..5 around the centre 0 (the launcher checks gall_pl == NPL)
+    double vt[NPL][3];
+#pragma unroll
     for (int d = 0; d < 3; ++d)
-        for (int v = gmin_pl; v <= gmax_pl; ++v) {
-            const int vp1 = (v + 1 == gmax_pl + 1) ? gmin_pl : v + 1;
-            const double *c = A.ci_pl + ((size_t)(l * 3 + d) * 3) * npl;
-            vt[v][d] = vt_point(c[0 * npl + v], c[1 * npl + v], c[2 * npl + v], A.scl_pl[fb + 0], A.scl_pl[fb + v],
+#pragma unroll
+        for (int v = 1; v < NPL; ++v) {
+            const int vp1 = (v + 1 == NPL) ? 1 : v + 1;
+            const double *c = A.ci_pl + ((size_t)(l * 3 + d) * 3) * NPL;
+            vt[v][d] = vt_point(c[0 * NPL + v], c[1 * NPL + v], c[2 * NPL + v], A.scl_pl[fb + 0], A.scl_pl[fb + v],
                                 A.scl_pl[fb + vp1]);
         }
     double acc = 0.0;
-    for (int v = gmin_pl; v <= gmax_pl; ++v) {
-        const int vm1 = (v - 1 == gmin_pl - 1) ? gmax_pl : v - 1;
-        const double *cd = A.cd_pl + (size_t)l * 3 * 5;
+    const double *cd = A.cd_pl + (size_t)l * 3 * (NPL - 1);
+#pragma unroll
+    for (int v = 1; v < NPL; ++v) {
+        const int vm1 = (v == 1) ? NPL - 1 : v - 1;
         acc = acc + (cd[0 * 5 + (v - 1)] * (vt[vm1][0] + vt[v][0]) + cd[1 * 5 + (v - 1)] * (vt[vm1][1] + vt[v][1]) +
                      cd[2 * 5 + (v - 1)] * (vt[vm1][2] + vt[v][2])) *
                         0.5 * (A.kh_pl[fb + 0] + A.kh_pl[fb + v]);
     }
     A.dscl_pl[fb + 0] = acc;
-    for (int v = 1; v < npl; ++v) A.dscl_pl[fb + v] = 0.0;
+#pragma unroll
+    for (int v = 1; v < NPL; ++v) A.dscl_pl[fb + v] = 0.0;
 }
 
+// A work-group owns CHUNK consecutive cells [g0, g0+CHUNK) of one region and
+// a range of levels.  Window index s names point lo + s, lo = g0 - gall_1d - 1:
+// the inputs are staged for s in [0, n_in), vt is formed for s in [0, n_t),
+// and work-item tid owns cell g0 + tid at s = n_x + tid.  The vt points below
+// the chunk (s < n_x) are formed by the first n_x work-items with their
+// coefficients read from LDS; the own point's coefficients and coef_diff stay
+// in registers.  The inputs of the next DEPTH levels are in flight in a
+// register ring while a level is computed, and the staging tiles alternate
+// between two LDS buffers, so a level needs two barriers.
+template <int NST>
 __global__ void __launch_bounds__(CHUNK) diffusion_kernel(DiffArgs A)
 {
     extern __shared__ double lds[];
     const int iall = A.iall, gall = A.gall, kall = A.kall, lall = A.lall;
-    const int n_in = CHUNK + 2 * iall + 2;  // staged scl/kh window
-    const int n_t = CHUNK + iall + 1;       // vt window
-    double *s_s = lds, *s_k = lds + n_in, *s_vt = lds + 2 * n_in;  // s_vt: [t*3+d][n_t]
+    const int n_x = iall + 1;               // vt points below the chunk
+    const int n_in = CHUNK + 2 * n_x;       // staged scl/kh window
+    const int n_t = CHUNK + n_x;            // vt window
+    double *s_in = lds;                     // [2 buffers][scl, kh][n_in]
+    double *s_vt = lds + 4 * n_in;          // [t*3+d][n_t]
+    double *s_cx = s_vt + 6 * n_t;          // [(t*3+d)*3+c][n_x]: coef_intp of the points below
 
     int b = blockIdx.x;
     if (b >= A.nmain) {  // pole work-groups
@@ -99,31 +132,44 @@ __global__ void __launch_bounds__(CHUNK) diffusion_kernel(DiffArgs A)
     const int k0 = kr * A.kper, k1 = min(kall, k0 + A.kper);
     if (l >= lall || k0 >= k1) return;
     const int tid = threadIdx.x;
-    const int g0 = chunk * CHUNK, lo = g0 - iall - 1;
+    const int g0 = chunk * CHUNK, lo = g0 - n_x;
     const int gmin = iall + 1, gmax = gall - iall - 2;  // 0-based (:85-86); vt runs over [0, gmax]
     const bool sgp = A.have_sgp[l] != 0;
 
     // ---- K-invariant coefficients ----
-    // vt points of this work-item; with a singular point, TI of point 0 is TJ of point 1 (:128-132)
-    double cv[NTP][2][3][3];
-    int vp[NTP], vsh[NTP];
+    const int g = g0 + tid, so = n_x + tid;  // own cell and its window index
+    const bool own_vt = g <= gmax;
+    // with a singular point, TI of point 0 is TJ of point 1 (:128-132)
+    const bool vsh = g == 0 && sgp;
+    double cv[2][3][3];  // vt factors e0..e2 of my TI, TJ per component
 #pragma unroll
-    for (int r = 0; r < NTP; ++r) {
-        const int s = tid + r * CHUNK, p = lo + s;
-        vp[r] = (s < n_t && p >= 0 && p <= gmax) ? p : -1;
-        vsh[r] = (vp[r] == 0 && sgp) ? 1 : 0;
+    for (int t = 0; t < 2; ++t) {
+        const int tt = vsh ? 1 : t, pp = vsh && t == 0 ? 1 : g;
 #pragma unroll
-        for (int t = 0; t < 2; ++t) {
-            const int tt = vsh[r] ? 1 : t, pp = vsh[r] && t == 0 ? 1 : p;
+        for (int d = 0; d < 3; ++d) {
+            double c[3];
 #pragma unroll
-            for (int d = 0; d < 3; ++d)
-#pragma unroll
-                for (int c = 0; c < 3; ++c)
-                    cv[r][t][d][c] =
-                        vp[r] >= 0 ? A.ci[((((size_t)l * 2 + tt) * 3 + d) * 3 + c) * gall + pp] : 0.0;
+            for (int q = 0; q < 3; ++q)
+                c[q] = own_vt ? A.ci[((((size_t)l * 2 + tt) * 3 + d) * 3 + q) * gall + pp] : 0.0;
+            vt_factors(c[0], c[1], c[2], cv[t][d][0], cv[t][d][1], cv[t][d][2]);
         }
     }
-    const int g = g0 + tid;
+    const int px = lo + tid;  // my point below the chunk, if tid < n_x
+    const bool x_vt = tid < n_x && px >= 0 && px <= gmax;
+    if (tid < n_x) {
+#pragma unroll
+        for (int t = 0; t < 2; ++t)
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+                double c[3], e[3];
+#pragma unroll
+                for (int q = 0; q < 3; ++q)
+                    c[q] = x_vt ? A.ci[((((size_t)l * 2 + t) * 3 + d) * 3 + q) * gall + px] : 0.0;
+                vt_factors(c[0], c[1], c[2], e[0], e[1], e[2]);
+#pragma unroll
+                for (int q = 0; q < 3; ++q) s_cx[((t * 3 + d) * 3 + q) * n_x + tid] = e[q];
+            }
+    }
     const bool active = g < gall;
     const bool inner = g >= gmin && g <= gmax;
     double cdv[3][6];
@@ -132,92 +178,89 @@ __global__ void __launch_bounds__(CHUNK) diffusion_kernel(DiffArgs A)
 #pragma unroll
         for (int m = 0; m < 6; ++m) cdv[d][m] = inner ? A.cd[(((size_t)l * 3 + d) * 6 + m) * gall + g] : 0.0;
 
-    // ---- level loop, the next level's window prefetched into registers ----
-    double nx[NST][2];
-    auto load_level = [&](int k) {
+    // ---- staging: slot 0 is the own cell, slots 1.. the 2*n_x cells below and above ----
+    auto slot_s = [&](int r) { return r == 0 ? so : ((r - 1) * CHUNK + tid < n_x ? (r - 1) * CHUNK + tid : (r - 1) * CHUNK + tid + CHUNK); };
+    auto slot_ok = [&](int r) { return r == 0 || (r - 1) * CHUNK + tid < 2 * n_x; };
+    double ring[DEPTH][NST][2];
+    auto load_level = [&](double (&dst)[NST][2], int k) {
         const size_t fb = ((size_t)l * kall + k) * gall;
 #pragma unroll
         for (int r = 0; r < NST; ++r) {
-            const int s = tid + r * CHUNK, p = lo + s;
-            const bool ok = s < n_in && p >= 0 && p < gall;
-            nx[r][0] = ok ? A.scl[fb + p] : 0.0;
-            nx[r][1] = ok ? A.kh[fb + p] : 0.0;
+            const int p = lo + slot_s(r);
+            const bool ok = slot_ok(r) && p >= 0 && p < gall;
+            dst[r][0] = ok ? A.scl[fb + p] : 0.0;
+            dst[r][1] = ok ? A.kh[fb + p] : 0.0;
         }
     };
-    auto store_level = [&]() {
 #pragma unroll
-        for (int r = 0; r < NST; ++r) {
-            const int s = tid + r * CHUNK;
-            if (s < n_in) {
-                s_s[s] = nx[r][0];
-                s_k[s] = nx[r][1];
-            }
-        }
-    };
-    load_level(k0);
-    store_level();
-    __syncthreads();
-    if (k0 + 1 < k1) load_level(k0 + 1);
+    for (int u = 0; u < DEPTH; ++u)
+        if (k0 + u < k1) load_level(ring[u], k0 + u);
 
-    for (int k = k0; k < k1; ++k) {
-        // vt of my points (:97-125)
+    for (int kb = k0; kb < k1; kb += DEPTH) {
 #pragma unroll
-        for (int r = 0; r < NTP; ++r) {
-            if (vp[r] < 0) continue;
-            const int s = tid + r * CHUNK;
-            const int sj = s + vsh[r];  // TJ stencil origin of the TI slot at a singular point
+        for (int u = 0; u < DEPTH; ++u) {
+            const int k = kb + u;
+            if (k >= k1) break;
+            double *s_s = s_in + (k & 1) * 2 * n_in, *s_k = s_s + n_in;
 #pragma unroll
-            for (int d = 0; d < 3; ++d) {
-                double ti;
-                if (vsh[r])
-                    ti = vt_point(cv[r][0][d][0], cv[r][0][d][1], cv[r][0][d][2], s_s[sj], s_s[sj + iall + 1],
-                                  s_s[sj + iall]);
-                else
-                    ti = vt_point(cv[r][0][d][0], cv[r][0][d][1], cv[r][0][d][2], s_s[s], s_s[s + 1],
-                                  s_s[s + iall + 1]);
-                const double tj = vt_point(cv[r][1][d][0], cv[r][1][d][1], cv[r][1][d][2], s_s[s],
-                                           s_s[s + iall + 1], s_s[s + iall]);
-                s_vt[(0 * 3 + d) * n_t + s] = ti;
-                s_vt[(1 * 3 + d) * n_t + s] = tj;
-            }
-        }
-        __syncthreads();
-        const double kc = s_k[g - lo];
-        double kf[6];
-        if (inner) {  // read kh before the window is overwritten
-            const int s = g - lo;
-            kf[0] = 0.5 * (kc + s_k[s + iall + 1]);
-            kf[1] = 0.5 * (kc + s_k[s + iall]);
-            kf[2] = 0.5 * (s_k[s - 1] + kc);
-            kf[3] = 0.5 * (s_k[s - iall - 1] + kc);
-            kf[4] = 0.5 * (s_k[s - iall] + kc);
-            kf[5] = 0.5 * (kc + s_k[s + 1]);
-        }
-        __syncthreads();
-        if (k + 1 < k1) {  // the staging tiles are free
-            store_level();
-            if (k + 2 < k1) load_level(k + 2);
-        }
-        if (active) {
-            double out = 0.0;
-            if (inner) {
-                // vt window slots of ij, im1j, im1jm1, ijm1
-                const int s0 = g - lo, s1 = s0 - 1, s2 = s0 - iall - 1, s3 = s0 - iall;
-                auto V = [&](int t, int d, int s) { return s_vt[(t * 3 + d) * n_t + s]; };
+            for (int r = 0; r < NST; ++r)
+                if (slot_ok(r)) {
+                    s_s[slot_s(r)] = ring[u][r][0];
+                    s_k[slot_s(r)] = ring[u][r][1];
+                }
+            if (k + DEPTH < k1) load_level(ring[u], k + DEPTH);
+            __syncthreads();
+
+            // vt (:97-125): the own point, then the point below the chunk
+            if (own_vt) {
+                const int sj = so + (vsh ? 1 : 0);  // TJ stencil origin of the TI slot at a singular point
 #pragma unroll
-                for (int d = 0; d < 3; ++d) {  // :138-216, X then Y then Z
-                    const double sum = kf[0] * cdv[d][0] * (V(0, 0, s0) + V(1, d, s0)) +
-                                       kf[1] * cdv[d][1] * (V(1, 0, s0) + V(0, d, s1)) +
-                                       kf[2] * cdv[d][2] * (V(0, 0, s1) + V(1, d, s2)) +
-                                       kf[3] * cdv[d][3] * (V(1, 0, s2) + V(0, d, s2)) +
-                                       kf[4] * cdv[d][4] * (V(0, 0, s2) + V(1, d, s3)) +
-                                       kf[5] * cdv[d][5] * (V(1, 0, s3) + V(0, d, s0));
-                    out = d == 0 ? sum : out + sum;
+                for (int d = 0; d < 3; ++d) {
+                    const double ti =
+                        vsh ? vt_value(cv[0][d][0], cv[0][d][1], cv[0][d][2], s_s[sj], s_s[sj + iall + 1], s_s[sj + iall])
+                            : vt_value(cv[0][d][0], cv[0][d][1], cv[0][d][2], s_s[so], s_s[so + 1], s_s[so + iall + 1]);
+                    const double tj = vt_value(cv[1][d][0], cv[1][d][1], cv[1][d][2], s_s[so], s_s[so + iall + 1],
+                                               s_s[so + iall]);
+                    s_vt[(0 * 3 + d) * n_t + so] = ti;
+                    s_vt[(1 * 3 + d) * n_t + so] = tj;
                 }
             }
-            stream_store(&A.dscl[((size_t)l * kall + k) * gall + g], out);
+            if (x_vt) {
+                const int s = tid;
+#pragma unroll
+                for (int d = 0; d < 3; ++d) {
+                    const double *c0 = s_cx + ((0 * 3 + d) * 3) * n_x + tid, *c1 = s_cx + ((1 * 3 + d) * 3) * n_x + tid;
+                    s_vt[(0 * 3 + d) * n_t + s] =
+                        vt_value(c0[0], c0[n_x], c0[2 * n_x], s_s[s], s_s[s + 1], s_s[s + iall + 1]);
+                    s_vt[(1 * 3 + d) * n_t + s] =
+                        vt_value(c1[0], c1[n_x], c1[2 * n_x], s_s[s], s_s[s + iall + 1], s_s[s + iall]);
+                }
+            }
+            __syncthreads();
+
+            if (active) {
+                double out = 0.0;
+                if (inner) {
+                    const int s0 = so, s1 = s0 - 1, s2 = s0 - iall - 1, s3 = s0 - iall;
+                    const double kc = s_k[s0];
+                    const double kf0 = 0.5 * (kc + s_k[s0 + iall + 1]), kf1 = 0.5 * (kc + s_k[s0 + iall]),
+                                 kf2 = 0.5 * (s_k[s0 - 1] + kc), kf3 = 0.5 * (s_k[s0 - iall - 1] + kc),
+                                 kf4 = 0.5 * (s_k[s0 - iall] + kc), kf5 = 0.5 * (kc + s_k[s0 + 1]);
+                    auto V = [&](int t, int d, int s) { return s_vt[(t * 3 + d) * n_t + s]; };
+#pragma unroll
+                    for (int d = 0; d < 3; ++d) {  // :138-216, X then Y then Z
+                        const double sum = kf0 * cdv[d][0] * (V(0, 0, s0) + V(1, d, s0)) +
+                                           kf1 * cdv[d][1] * (V(1, 0, s0) + V(0, d, s1)) +
+                                           kf2 * cdv[d][2] * (V(0, 0, s1) + V(1, d, s2)) +
+                                           kf3 * cdv[d][3] * (V(1, 0, s2) + V(0, d, s2)) +
+                                           kf4 * cdv[d][4] * (V(0, 0, s2) + V(1, d, s3)) +
+                                           kf5 * cdv[d][5] * (V(1, 0, s3) + V(0, d, s0));
+                        out = d == 0 ? sum : out + sum;
+                    }
+                }
+                stream_store(&A.dscl[((size_t)l * kall + k) * gall + g], out);
+            }
         }
-        __syncthreads();
     }
 }
 
@@ -259,54 +302,69 @@ __device__ void ddmp_pole(const DdmpArgs &A, int t)
             for (int v = 0; v < npl; ++v) A.out_pl[d][fb + v] = 0.0;
         return;
     }
+    // ring points 1..5 around the centre 0 (the launcher checks gall_pl == NPL)
     const int kmin = 1, kmax = kall - 2;
-    const size_t cstr = (size_t)kall * npl;           // between C2WfactGz_pl planes
+    const size_t cstr = (size_t)kall * NPL;  // between C2WfactGz_pl planes
     const double *c2w = A.c2w_pl + (size_t)l * 6 * cstr;
     auto W = [&](int p, int kk) -> double {
         if (kk == kmin || kk == kmax + 1) return 0.0;
-        const size_t f = ((size_t)l * kall + kk) * npl + p, fm = ((size_t)l * kall + kk - 1) * npl + p;
-        return (c2w[0 * cstr + (size_t)kk * npl + p] * A.v_pl[0][f] + c2w[1 * cstr + (size_t)kk * npl + p] * A.v_pl[0][fm] +
-                c2w[2 * cstr + (size_t)kk * npl + p] * A.v_pl[1][f] + c2w[3 * cstr + (size_t)kk * npl + p] * A.v_pl[1][fm] +
-                c2w[4 * cstr + (size_t)kk * npl + p] * A.v_pl[2][f] + c2w[5 * cstr + (size_t)kk * npl + p] * A.v_pl[2][fm]) *
+        const size_t f = ((size_t)l * kall + kk) * NPL + p, fm = ((size_t)l * kall + kk - 1) * NPL + p;
+        return (c2w[0 * cstr + (size_t)kk * NPL + p] * A.v_pl[0][f] + c2w[1 * cstr + (size_t)kk * NPL + p] * A.v_pl[0][fm] +
+                c2w[2 * cstr + (size_t)kk * NPL + p] * A.v_pl[1][f] + c2w[3 * cstr + (size_t)kk * NPL + p] * A.v_pl[1][fm] +
+                c2w[4 * cstr + (size_t)kk * NPL + p] * A.v_pl[2][f] + c2w[5 * cstr + (size_t)kk * NPL + p] * A.v_pl[2][fm]) *
                    A.rgamh_pl[f] +
                A.w_pl[f] * A.rgsqrth_pl[f];
     };
-    double vm[3][16], sp[16];
-    for (int v = 0; v < npl; ++v)
+    double vm[3][NPL], sp[NPL];
+#pragma unroll
+    for (int v = 0; v < NPL; ++v)
+#pragma unroll
         for (int d = 0; d < 3; ++d) vm[d][v] = A.v_pl[d][fb + v] * A.rgam_pl[fb + v];
     const double rdgz = A.rdgz[k];
-    for (int v = 1; v < npl; ++v) {
-        const int vp1 = (v + 1 == npl) ? 1 : v + 1;
-        const double sw = ((W(0, k + 1) + W(v, k + 1) + W(vp1, k + 1)) - (W(0, k) + W(v, k) + W(vp1, k))) / 3.0 * rdgz;
-        const double *c = A.ci_pl + (size_t)l * 9 * npl;  // coef_intp_pl(v,c,d,l): ((l*3+d)*3+c)*npl+v
-        sp[v] = c[(0 * 3 + 0) * npl + v] * vm[0][0] + c[(0 * 3 + 1) * npl + v] * vm[0][v] +
-                c[(0 * 3 + 2) * npl + v] * vm[0][vp1] + c[(1 * 3 + 0) * npl + v] * vm[1][0] +
-                c[(1 * 3 + 1) * npl + v] * vm[1][v] + c[(1 * 3 + 2) * npl + v] * vm[1][vp1] +
-                c[(2 * 3 + 0) * npl + v] * vm[2][0] + c[(2 * 3 + 1) * npl + v] * vm[2][v] +
-                c[(2 * 3 + 2) * npl + v] * vm[2][vp1] + sw;
+    const double w0 = W(0, k + 1), w0m = W(0, k);
+    sp[0] = 0.0;
+#pragma unroll
+    for (int v = 1; v < NPL; ++v) {
+        const int vp1 = (v + 1 == NPL) ? 1 : v + 1;
+        const double sw = div3((w0 + W(v, k + 1) + W(vp1, k + 1)) - (w0m + W(v, k) + W(vp1, k))) * rdgz;
+        const double *c = A.ci_pl + (size_t)l * 9 * NPL;  // coef_intp_pl(v,c,d,l): ((l*3+d)*3+c)*npl+v
+        sp[v] = c[(0 * 3 + 0) * NPL + v] * vm[0][0] + c[(0 * 3 + 1) * NPL + v] * vm[0][v] +
+                c[(0 * 3 + 2) * NPL + v] * vm[0][vp1] + c[(1 * 3 + 0) * NPL + v] * vm[1][0] +
+                c[(1 * 3 + 1) * NPL + v] * vm[1][v] + c[(1 * 3 + 2) * NPL + v] * vm[1][vp1] +
+                c[(2 * 3 + 0) * NPL + v] * vm[2][0] + c[(2 * 3 + 1) * NPL + v] * vm[2][v] +
+                c[(2 * 3 + 2) * NPL + v] * vm[2][vp1] + sw;
     }
-    (void)kmin;
+#pragma unroll
     for (int d = 0; d < 3; ++d) {
         const double *cd = A.cd_pl + (size_t)l * 15 + d * 5;
         double acc = 0.0;
-        for (int v = 1; v < npl; ++v) {
-            const int vm1 = (v == 1) ? npl - 1 : v - 1;
+#pragma unroll
+        for (int v = 1; v < NPL; ++v) {
+            const int vm1 = (v == 1) ? NPL - 1 : v - 1;
             acc = acc + cd[v - 1] * (sp[vm1] + sp[v]);
         }
         A.out_pl[d][fb + 0] = acc;
-        for (int v = 1; v < npl; ++v) A.out_pl[d][fb + v] = 0.0;
+#pragma unroll
+        for (int v = 1; v < NPL; ++v) A.out_pl[d][fb + v] = 0.0;
     }
 }
 
+// A work-group owns CHUNK consecutive cells [g0, g0+CHUNK) of one region and
+// marches up a range of levels; the window indexing is OPRT_diffusion's.  The
+// per-level record of a window point (rhogv x/y/z, RGAM, C2WfactGz(1:6),
+// RGAMH, rhogw, RGSQRTH) of the next level is in flight in registers while a
+// level is computed; rhogv of the level below and rhogv*RGAM of the current
+// level are carried in registers, rhogw_vm of two levels alternates in LDS.
 template <int NSTT>
 __global__ void __launch_bounds__(CHUNK) ddmp_kernel(DdmpArgs A)
 {
     extern __shared__ double lds[];
     const int iall = A.iall, gall = A.gall, kall = A.kall, lall = A.lall;
-    const int n_in = CHUNK + 2 * iall + 2, n_t = CHUNK + iall + 1;
+    const int n_x = iall + 1, n_in = CHUNK + 2 * n_x, n_t = CHUNK + n_x;
     double *s_vm = lds;                  // [3][n_in] rhogv*RGAM of level k
     double *s_w = lds + 3 * n_in;        // [2][n_in] rhogw_vm of levels k and k+1 (alternating)
     double *s_st = lds + 5 * n_in;       // [2][n_t]  sclt at TI, TJ
+    double *s_cx = s_st + 2 * n_t;       // [(t*3+d)*3+c][n_x]: coef_intp of the points below
 
     int b = blockIdx.x;
     if (b >= A.nmain) {
@@ -324,10 +382,10 @@ __global__ void __launch_bounds__(CHUNK) ddmp_kernel(DdmpArgs A)
     const int k0 = kmin + kr * A.kper, k1 = min(kmax + 1, k0 + A.kper);  // levels [k0, k1) of [kmin, kmax]
     if (l >= lall || k0 >= k1) return;
     const int tid = threadIdx.x;
-    const int g0 = chunk * CHUNK, lo = g0 - iall - 1;
+    const int g0 = chunk * CHUNK, lo = g0 - n_x;
     const int gmin = iall + 1, gmax = gall - iall - 2;
     const bool sgp = A.have_sgp[l] != 0;
-    const int g = g0 + tid;
+    const int g = g0 + tid, so = n_x + tid;
     const bool active = g < gall;
     const bool inner = g >= gmin && g <= gmax;
     const size_t slab = (size_t)gall, lk = (size_t)l * kall;
@@ -341,22 +399,29 @@ __global__ void __launch_bounds__(CHUNK) ddmp_kernel(DdmpArgs A)
         for (int d = 0; d < 3; ++d) A.out[d][(lk + kall - 1) * slab + g] = 0.0;
 
     // ---- K-invariant coefficients (TI of point 0 is TJ of point 1 at a singular point, :227-229) ----
-    double cv[NTP][2][3][3];
-    int vp[NTP], vsh[NTP];
+    const bool own_vt = g <= gmax;
+    const bool vsh = g == 0 && sgp;
+    double cv[2][3][3];
 #pragma unroll
-    for (int r = 0; r < NTP; ++r) {
-        const int s = tid + r * CHUNK, p = lo + s;
-        vp[r] = (s < n_t && p >= 0 && p <= gmax) ? p : -1;
-        vsh[r] = (vp[r] == 0 && sgp) ? 1 : 0;
+    for (int t = 0; t < 2; ++t) {
+        const int tt = vsh ? 1 : t, pp = vsh && t == 0 ? 1 : g;
 #pragma unroll
-        for (int t = 0; t < 2; ++t) {
-            const int tt = vsh[r] ? 1 : t, pp = vsh[r] && t == 0 ? 1 : p;
+        for (int d = 0; d < 3; ++d)
+#pragma unroll
+            for (int c = 0; c < 3; ++c)
+                cv[t][d][c] = own_vt ? A.ci[((((size_t)l * 2 + tt) * 3 + d) * 3 + c) * gall + pp] : 0.0;
+    }
+    const int px = lo + tid;
+    const bool x_vt = tid < n_x && px >= 0 && px <= gmax;
+    if (tid < n_x) {
+#pragma unroll
+        for (int t = 0; t < 2; ++t)
 #pragma unroll
             for (int d = 0; d < 3; ++d)
 #pragma unroll
                 for (int c = 0; c < 3; ++c)
-                    cv[r][t][d][c] = vp[r] >= 0 ? A.ci[((((size_t)l * 2 + tt) * 3 + d) * 3 + c) * gall + pp] : 0.0;
-        }
+                    s_cx[((t * 3 + d) * 3 + c) * n_x + tid] =
+                        x_vt ? A.ci[((((size_t)l * 2 + t) * 3 + d) * 3 + c) * gall + px] : 0.0;
     }
     double cdv[3][6];
 #pragma unroll
@@ -364,93 +429,120 @@ __global__ void __launch_bounds__(CHUNK) ddmp_kernel(DdmpArgs A)
 #pragma unroll
         for (int m = 0; m < 6; ++m) cdv[d][m] = inner ? A.cd[(((size_t)l * 3 + d) * 6 + m) * gall + g] : 0.0;
 
-    // ---- window state: raw rhogv of the current level in registers ----
-    double vr[NSTT][3];
-    auto in_win = [&](int r, int &p) {
-        const int s = tid + r * CHUNK;
-        p = lo + s;
-        return s < n_in && p >= 0 && p < gall;
+    // ---- window slots: 0 is the own cell, 1.. the 2*n_x cells below and above ----
+    auto slot_s = [&](int r) {
+        return r == 0 ? so : ((r - 1) * CHUNK + tid < n_x ? (r - 1) * CHUNK + tid : (r - 1) * CHUNK + tid + CHUNK);
     };
-    // prologue: rhogv(k0) and rhogw_vm(k0)
+    auto slot_ok = [&](int r) { return r == 0 || (r - 1) * CHUNK + tid < 2 * n_x; };
+    auto slot_p = [&](int r, int &p) {
+        p = lo + slot_s(r);
+        return slot_ok(r) && p >= 0 && p < gall;
+    };
+    // record of level kk: v[3], rgam, c2w[6], rgamh, w, rgsqrth
+    double nx[NSTT][13];
+    auto load_rec = [&](int kk) {
+#pragma unroll
+        for (int r = 0; r < NSTT; ++r) {
+            int p;
+            const bool ok = slot_p(r, p) && kk <= kall - 1;
+            const size_t f = (lk + kk) * slab + p, fk = (size_t)kk * slab + p;
+#pragma unroll
+            for (int d = 0; d < 3; ++d) nx[r][d] = ok ? A.v[d][f] : 0.0;
+            nx[r][3] = ok ? A.rgam[f] : 0.0;
+#pragma unroll
+            for (int j = 0; j < 6; ++j) nx[r][4 + j] = ok ? c2w[j * cstr + fk] : 0.0;
+            nx[r][10] = ok ? A.rgamh[f] : 0.0;
+            nx[r][11] = ok ? A.w[f] : 0.0;
+            nx[r][12] = ok ? A.rgsqrth[f] : 0.0;
+        }
+    };
+    // rhogw_vm of a level from its record and rhogv of the level below (:148-156)
+    auto wvm = [&](const double *rec, const double *vb) {
+        return (rec[4] * rec[0] + rec[5] * vb[0] + rec[6] * rec[1] + rec[7] * vb[1] + rec[8] * rec[2] +
+                rec[9] * vb[2]) *
+                   rec[10] +
+               rec[11] * rec[12];
+    };
+    // prologue: rhogv(k0-1) for rhogw_vm(k0); record of k0 gives rhogw_vm(k0) and rhogv*RGAM(k0)
+    double vb[NSTT][3], vmc[NSTT][3];
     int cur = 0;
 #pragma unroll
     for (int r = 0; r < NSTT; ++r) {
         int p;
-        const bool ok = in_win(r, p);
-        const size_t f = (lk + k0) * slab + p;
+        const bool ok = slot_p(r, p);
+        const size_t f = (lk + k0 - 1) * slab + p;
 #pragma unroll
-        for (int d = 0; d < 3; ++d) vr[r][d] = ok ? A.v[d][f] : 0.0;
-        const int s = tid + r * CHUNK;
-        if (s < n_in) {
-            double wv = 0.0;
-            if (ok && k0 > kmin) {  // :148-156 at level k0
-                const size_t fk = (size_t)k0 * slab + p, fm = f - slab;  // fk: within the region's C2W planes
-                wv = (c2w[0 * cstr + fk] * A.v[0][f] + c2w[1 * cstr + fk] * A.v[0][fm] +
-                      c2w[2 * cstr + fk] * A.v[1][f] + c2w[3 * cstr + fk] * A.v[1][fm] +
-                      c2w[4 * cstr + fk] * A.v[2][f] + c2w[5 * cstr + fk] * A.v[2][fm]) *
-                         A.rgamh[f] +
-                     A.w[f] * A.rgsqrth[f];
-            }
-            s_w[cur * n_in + s] = wv;
+        for (int d = 0; d < 3; ++d) vb[r][d] = ok ? A.v[d][f] : 0.0;
+    }
+    load_rec(k0);
+#pragma unroll
+    for (int r = 0; r < NSTT; ++r) {
+        if (!slot_ok(r)) continue;
+        int p;
+        const bool ok = slot_p(r, p);
+        s_w[cur * n_in + slot_s(r)] = (ok && k0 > kmin) ? wvm(nx[r], vb[r]) : 0.0;
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            vmc[r][d] = nx[r][d] * nx[r][3];
+            vb[r][d] = nx[r][d];
         }
     }
+    load_rec(k0 + 1);
 
     for (int k = k0; k < k1; ++k) {
         const int nxt = cur ^ 1;
-        // rhogw_vm(k+1) and rhogv*RGAM(k) of the window
+        // rhogw_vm(k+1) from the record of k+1, and rhogv*RGAM of level k
 #pragma unroll
         for (int r = 0; r < NSTT; ++r) {
+            if (!slot_ok(r)) continue;
             int p;
-            const bool ok = in_win(r, p);
-            const int s = tid + r * CHUNK;
-            if (s >= n_in) continue;
-            const size_t f = (lk + k) * slab + p, f1 = f + slab;
-            double v1[3] = {0.0, 0.0, 0.0};
-            double wv = 0.0;
-            if (ok && k + 1 <= kmax) {
-#pragma unroll
-                for (int d = 0; d < 3; ++d) v1[d] = A.v[d][f1];
-                const size_t fk = (size_t)(k + 1) * slab + p;  // within the region's C2W planes
-                wv = (c2w[0 * cstr + fk] * v1[0] + c2w[1 * cstr + fk] * vr[r][0] + c2w[2 * cstr + fk] * v1[1] +
-                      c2w[3 * cstr + fk] * vr[r][1] + c2w[4 * cstr + fk] * v1[2] + c2w[5 * cstr + fk] * vr[r][2]) *
-                         A.rgamh[f1] +
-                     A.w[f1] * A.rgsqrth[f1];
-            }
-            s_w[nxt * n_in + s] = wv;
-            const double rg = ok ? A.rgam[f] : 0.0;
+            const bool ok = slot_p(r, p);
+            const int s = slot_s(r);
+            s_w[nxt * n_in + s] = (ok && k + 1 <= kmax) ? wvm(nx[r], vb[r]) : 0.0;
 #pragma unroll
             for (int d = 0; d < 3; ++d) {
-                s_vm[d * n_in + s] = vr[r][d] * rg;
-                vr[r][d] = v1[d];
+                s_vm[d * n_in + s] = vmc[r][d];
+                vmc[r][d] = nx[r][d] * nx[r][3];
+                vb[r][d] = nx[r][d];
             }
         }
+        if (k + 1 < k1) load_rec(k + 2);
         __syncthreads();
         // sclt of my vertices (:183-225)
         const double rdgz = A.rdgz[k];
         const double *w0 = s_w + cur * n_in, *w1 = s_w + nxt * n_in;
-#pragma unroll
-        for (int r = 0; r < NTP; ++r) {
-            if (vp[r] < 0) continue;
-            const int s = tid + r * CHUNK;
+        const double *vx = s_vm, *vy = s_vm + n_in, *vz = s_vm + 2 * n_in;
+        if (own_vt) {
 #pragma unroll
             for (int t = 0; t < 2; ++t) {
                 // TI: (ij, ip1j, ip1jp1); TJ: (ij, ip1jp1, ijp1); the singular TI slot is TJ of point 1
-                const bool tj = t == 1 || vsh[r];
-                const int a = tj && t == 0 ? s + 1 : s;
+                const bool tj = t == 1 || vsh;
+                const int a = tj && t == 0 ? so + 1 : so;
                 const int b2 = tj ? a + iall + 1 : a + 1;
                 const int c3 = tj ? a + iall : a + iall + 1;
-                const double sw = ((w1[a] + w1[b2] + w1[c3]) - (w0[a] + w0[b2] + w0[c3])) / 3.0 * rdgz;
-                const double *vx = s_vm, *vy = s_vm + n_in, *vz = s_vm + 2 * n_in;
-                s_st[t * n_t + s] = cv[r][t][0][0] * vx[a] + cv[r][t][0][1] * vx[b2] + cv[r][t][0][2] * vx[c3] +
-                                    cv[r][t][1][0] * vy[a] + cv[r][t][1][1] * vy[b2] + cv[r][t][1][2] * vy[c3] +
-                                    cv[r][t][2][0] * vz[a] + cv[r][t][2][1] * vz[b2] + cv[r][t][2][2] * vz[c3] + sw;
+                const double sw = div3((w1[a] + w1[b2] + w1[c3]) - (w0[a] + w0[b2] + w0[c3])) * rdgz;
+                s_st[t * n_t + so] = cv[t][0][0] * vx[a] + cv[t][0][1] * vx[b2] + cv[t][0][2] * vx[c3] +
+                                     cv[t][1][0] * vy[a] + cv[t][1][1] * vy[b2] + cv[t][1][2] * vy[c3] +
+                                     cv[t][2][0] * vz[a] + cv[t][2][1] * vz[b2] + cv[t][2][2] * vz[c3] + sw;
+            }
+        }
+        if (x_vt) {
+            const int a0 = tid;
+#pragma unroll
+            for (int t = 0; t < 2; ++t) {
+                const int a = a0, b2 = t == 1 ? a + iall + 1 : a + 1, c3 = t == 1 ? a + iall : a + iall + 1;
+                const double sw = div3((w1[a] + w1[b2] + w1[c3]) - (w0[a] + w0[b2] + w0[c3])) * rdgz;
+                const double *c = s_cx + (t * 9) * n_x + tid;  // [(t*3+d)*3+c][n_x]
+                s_st[t * n_t + a0] = c[0 * n_x] * vx[a] + c[1 * n_x] * vx[b2] + c[2 * n_x] * vx[c3] +
+                                     c[3 * n_x] * vy[a] + c[4 * n_x] * vy[b2] + c[5 * n_x] * vy[c3] +
+                                     c[6 * n_x] * vz[a] + c[7 * n_x] * vz[b2] + c[8 * n_x] * vz[c3] + sw;
             }
         }
         __syncthreads();
         if (active) {
             const size_t fo = (lk + k) * slab + g;
             if (inner) {
-                const int s0 = g - lo, s1 = s0 - 1, s2 = s0 - iall - 1, s3 = s0 - iall;
+                const int s0 = so, s1 = s0 - 1, s2 = s0 - iall - 1, s3 = s0 - iall;
                 const double *ti = s_st, *tj = s_st + n_t;
                 const double e1 = ti[s0] + tj[s0], e2 = tj[s0] + ti[s1], e3 = ti[s1] + tj[s2];
                 const double e4 = tj[s2] + ti[s2], e5 = ti[s2] + tj[s3], e6 = tj[s3] + ti[s0];
@@ -463,7 +555,6 @@ __global__ void __launch_bounds__(CHUNK) ddmp_kernel(DdmpArgs A)
                 for (int d = 0; d < 3; ++d) stream_store(&A.out[d][fo], 0.0);
             }
         }
-        __syncthreads();
         cur = nxt;
     }
 }
@@ -476,6 +567,8 @@ int launch_diffusion(nicam_ctx c, double *dscl, double *dscl_pl, const double *s
 {
     const Dims &D = c->d;
     if (D.gall_1d > MAXI) return fail(NICAM_ERR_ARG, "gall_1d larger than the supported 255");
+    if (dscl_pl && D.have_pl && D.gall_pl != NPL)
+        return fail(NICAM_ERR_ARG, "OPRT_diffusion's pole regions need gall_pl = 6 (ADM_vlink = 5)");
     DiffArgs A;
     A.dscl = dscl;
     A.dscl_pl = dscl_pl;
@@ -505,11 +598,16 @@ int launch_diffusion(nicam_ctx c, double *dscl, double *dscl_pl, const double *s
     const long npole = dscl_pl ? (D.kall * kLallPl + CHUNK - 1) / CHUNK : 0;
     if (nmain + npole > 0x7fffffffL) return fail(NICAM_ERR_ARG, "grid too large for one launch");
     A.nmain = (int)nmain;
-    const size_t lds = sizeof(double) * (2 * (CHUNK + 2 * D.gall_1d + 2) + 6 * (CHUNK + D.gall_1d + 1));
+    const int n_x = D.gall_1d + 1;
+    const size_t lds = sizeof(double) * (4 * (CHUNK + 2 * n_x) + 6 * (CHUNK + n_x) + 18 * n_x);
+    const bool two = 2 * n_x <= CHUNK;  // the cells below and above the chunk fit one extra slot
+    const void *kern = two ? (const void *)diffusion_kernel<2> : (const void *)diffusion_kernel<3>;
     if (lds > 64 * 1024)
-        NICAM_HIP(hipFuncSetAttribute((const void *)diffusion_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                      (int)lds));
-    hipLaunchKernelGGL(diffusion_kernel, dim3((unsigned)(nmain + npole)), dim3(CHUNK), lds, c->stream, A);
+        NICAM_HIP(hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    if (two)
+        hipLaunchKernelGGL(diffusion_kernel<2>, dim3((unsigned)(nmain + npole)), dim3(CHUNK), lds, c->stream, A);
+    else
+        hipLaunchKernelGGL(diffusion_kernel<3>, dim3((unsigned)(nmain + npole)), dim3(CHUNK), lds, c->stream, A);
     NICAM_HIP(hipGetLastError());
     return NICAM_OK;
 }
@@ -524,6 +622,8 @@ int launch_divdamp(nicam_ctx c, double *const out[3], double *const out_pl[3], c
     const Dims &D = c->d;
     if (D.gall_1d > MAXI) return fail(NICAM_ERR_ARG, "gall_1d larger than the supported 255");
     if (D.kall < 3) return fail(NICAM_ERR_ARG, "OPRT3D_divdamp needs kall >= 3");
+    if (out_pl[0] && D.have_pl && D.gall_pl != NPL)
+        return fail(NICAM_ERR_ARG, "OPRT3D_divdamp's pole regions need gall_pl = 6 (ADM_vlink = 5)");
     DdmpArgs A;
     for (int d = 0; d < 3; ++d) {
         A.out[d] = out[d];
@@ -565,8 +665,9 @@ int launch_divdamp(nicam_ctx c, double *const out[3], double *const out_pl[3], c
     const long npole = out_pl[0] ? (D.kall * kLallPl + CHUNK - 1) / CHUNK : 0;
     if (nmain + npole > 0x7fffffffL) return fail(NICAM_ERR_ARG, "grid too large for one launch");
     A.nmain = (int)nmain;
-    const size_t lds = sizeof(double) * (5 * (CHUNK + 2 * D.gall_1d + 2) + 2 * (CHUNK + D.gall_1d + 1));
-    const bool small = CHUNK + 2 * D.gall_1d + 2 <= 2 * CHUNK;
+    const int n_x = D.gall_1d + 1;
+    const size_t lds = sizeof(double) * (5 * (CHUNK + 2 * n_x) + 2 * (CHUNK + n_x) + 18 * n_x);
+    const bool small = 2 * n_x <= CHUNK;  // the cells below and above the chunk fit one extra slot
     const void *fn = small ? (const void *)ddmp_kernel<2> : (const void *)ddmp_kernel<3>;
     if (lds > 64 * 1024) NICAM_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     if (small)

@@ -231,6 +231,35 @@ module nicam_tt_f
        integer(c_int) :: nicam_oprt_diffusion
      end function nicam_oprt_diffusion
 
+     function nicam_oprt_gradient(ctx, grad, grad_pl, scl, scl_pl, coef_grad, coef_grad_pl) &
+          bind(C, name='nicam_oprt_gradient')
+       import :: c_ptr, c_int
+       type(c_ptr), value :: ctx, grad, grad_pl, scl, scl_pl, coef_grad, coef_grad_pl
+       integer(c_int) :: nicam_oprt_gradient
+     end function nicam_oprt_gradient
+
+     function nicam_tracer_courant(ctx, ch, ch_pl, cmask, cmask_pl, flx_h, flx_h_pl, rhog, rhog_pl) &
+          bind(C, name='nicam_tracer_courant')
+       import :: c_ptr, c_int
+       type(c_ptr), value :: ctx, ch, ch_pl, cmask, cmask_pl, flx_h, flx_h_pl, rhog, rhog_pl
+       integer(c_int) :: nicam_tracer_courant
+     end function nicam_tracer_courant
+
+     function nicam_horizontal_remap(ctx, q_a, q_a_pl, q, q_pl, grad, grad_pl, cmask, cmask_pl, &
+          GRD_xc, GRD_xc_pl, GRD_x, GRD_x_pl) bind(C, name='nicam_horizontal_remap')
+       import :: c_ptr, c_int
+       type(c_ptr), value :: ctx, q_a, q_a_pl, q, q_pl, grad, grad_pl, cmask, cmask_pl
+       type(c_ptr), value :: GRD_xc, GRD_xc_pl, GRD_x, GRD_x_pl
+       integer(c_int) :: nicam_horizontal_remap
+     end function nicam_horizontal_remap
+
+     function nicam_tracer_update(ctx, rhogq, rhogq_pl, flx_h, flx_h_pl, q_a, q_a_pl) &
+          bind(C, name='nicam_tracer_update')
+       import :: c_ptr, c_int
+       type(c_ptr), value :: ctx, rhogq, rhogq_pl, flx_h, flx_h_pl, q_a, q_a_pl
+       integer(c_int) :: nicam_tracer_update
+     end function nicam_tracer_update
+
      function nicam_grid_icosahedral(glevel, rlevel, radius, GRD_x, GRD_x_pl) bind(C, name='nicam_grid_icosahedral')
        import :: c_int, c_double
        integer(c_int), value :: glevel, rlevel

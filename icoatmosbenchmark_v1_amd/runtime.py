@@ -90,6 +90,10 @@ SIGNATURES = {
     "nicam_oprt_rotation_setup": (c_int, [c_int, c_int, c_int, c_int, _P] + [_P] * 8),
     "nicam_oprt_gradient_setup": (c_int, [c_int, c_int, c_int, c_int, _P] + [_P] * 8),
     "nicam_oprt_laplacian_setup": (c_int, [c_int, c_int, c_int, c_int, _P] + [_P] * 8),
+    "nicam_oprt_gradient": (c_int, [_P] * 7),
+    "nicam_tracer_courant": (c_int, [_P] * 9),
+    "nicam_horizontal_remap": (c_int, [_P] * 13),
+    "nicam_tracer_update": (c_int, [_P] * 7),
     "nicam_oprt_diffusion": (c_int, [_P] + [_P] * 10),
     "nicam_set_rdgz": (c_int, [_P, _P]),
     "nicam_set_vmetrics3d": (c_int, [_P] + [_P] * 8),
@@ -392,6 +396,43 @@ class Context:
                          (coef_intp, "coef_intp", 18 * G), (coef_intp_pl, "coef_intp_pl", 18 * self.gall_pl),
                          (coef_diff, "coef_diff", 18 * G), (coef_diff_pl, "coef_diff_pl", 30)])
         _check(self.lib.nicam_oprt_diffusion(self.h, *a), "nicam_oprt_diffusion")
+
+    # ---- step glue (the inputs the reference's kernels take from "other subroutine") ----
+    def oprt_gradient(self, grad, grad_pl, scl, scl_pl, coef_grad, coef_grad_pl) -> None:
+        """nicam_oprt_gradient: grad(gall,kall,lall,3) of scl with OPRT_gradient_setup's coefficients."""
+        R, P, G = self.nreg, self.npl, self.gall * self.lall
+        a = [_ptr(grad, self.mode, "grad", 3 * R), _ptr(grad_pl, self.mode, "grad_pl", 3 * P)]
+        a += self._args([(scl, "scl", R), (scl_pl, "scl_pl", P), (coef_grad, "coef_grad", 21 * G),
+                         (coef_grad_pl, "coef_grad_pl", 6 * self.gall_pl)])
+        _check(self.lib.nicam_oprt_gradient(self.h, *a), "nicam_oprt_gradient")
+
+    def tracer_courant(self, ch, ch_pl, cmask, cmask_pl, flx_h, flx_h_pl, rhog, rhog_pl) -> None:
+        """nicam_tracer_courant: ch = flx_h/rhog and the upwind mask of the six faces."""
+        R, P = self.nreg, self.npl
+        a = [_ptr(ch, self.mode, "ch", 6 * R), _ptr(ch_pl, self.mode, "ch_pl", P),
+             _ptr(cmask, self.mode, "cmask", 6 * R), _ptr(cmask_pl, self.mode, "cmask_pl", P)]
+        a += self._args([(flx_h, "flx_h", 6 * R), (flx_h_pl, "flx_h_pl", P), (rhog, "rhog", R),
+                         (rhog_pl, "rhog_pl", P)])
+        _check(self.lib.nicam_tracer_courant(self.h, *a), "nicam_tracer_courant")
+
+    def horizontal_remap(self, q_a, q_a_pl, q, q_pl, grad, grad_pl, cmask, cmask_pl, GRD_xc, GRD_xc_pl,
+                         GRD_x, GRD_x_pl) -> None:
+        """nicam_horizontal_remap: upwind face values q + grad.(GRD_xc - GRD_x)."""
+        R, P, G = self.nreg, self.npl, self.gall * self.lall
+        a = [_ptr(q_a, self.mode, "q_a", 6 * R), _ptr(q_a_pl, self.mode, "q_a_pl", P)]
+        a += self._args([(q, "q", R), (q_pl, "q_pl", P), (grad, "grad", 3 * R), (grad_pl, "grad_pl", 3 * P),
+                         (cmask, "cmask", 6 * R), (cmask_pl, "cmask_pl", P), (GRD_xc, "GRD_xc", 9 * R),
+                         (GRD_xc_pl, "GRD_xc_pl", 3 * P), (GRD_x, "GRD_x", 3 * G),
+                         (GRD_x_pl, "GRD_x_pl", 6 * self.gall_pl)])
+        _check(self.lib.nicam_horizontal_remap(self.h, *a), "nicam_horizontal_remap")
+
+    def tracer_update(self, rhogq, rhogq_pl, flx_h, flx_h_pl, q_a, q_a_pl) -> None:
+        """nicam_tracer_update: rhogq -= sum of flx_h*q_a over the six faces."""
+        R, P = self.nreg, self.npl
+        a = [_ptr(rhogq, self.mode, "rhogq", R), _ptr(rhogq_pl, self.mode, "rhogq_pl", P)]
+        a += self._args([(flx_h, "flx_h", 6 * R), (flx_h_pl, "flx_h_pl", P), (q_a, "q_a", 6 * R),
+                         (q_a_pl, "q_a_pl", P)])
+        _check(self.lib.nicam_tracer_update(self.h, *a), "nicam_tracer_update")
 
     def oprt3d_divdamp(self, ddivdx, ddivdx_pl, ddivdy, ddivdy_pl, ddivdz, ddivdz_pl, rhogvx, rhogvx_pl,
                        rhogvy, rhogvy_pl, rhogvz, rhogvz_pl, rhogw, rhogw_pl, coef_intp, coef_intp_pl,

@@ -375,6 +375,47 @@ int nicam_oprt_laplacian_setup(int gall_1d, int lall, int gall_pl, int have_pl, 
                                const double *GMTR_t_pl, const double *GMTR_a, const double *GMTR_a_pl,
                                double *coef_lap, double *coef_lap_pl);
 
+/* ---- step glue: the inputs the reference's kernels take "computed in
+ * advance by other subroutine" (its kernels have no such code).  The NICAM
+ * forms of the Miura upwind scheme (docs/NICAM/src/20_intro_nicam.tex:368-438);
+ * no reference interface is replaced, the argument conventions follow the
+ * kernels' own (regular, _pl) pairs and layouts.  Pole arguments may be NULL
+ * when have_pl == 0. ---- */
+
+/* The gradient of a scalar with the coefficients of OPRT_gradient_setup
+ * (dyn_metrics/src/mod_oprt.f90:499-734; nicam_oprt_gradient_setup):
+ * grad(gall,kall,lall,3) = sum over ij, ip1j, ip1jp1, ijp1, im1j, im1jm1, ijm1
+ * of coef_grad(gall,0:6,3,lall) * scl, on the flat range [gmin, gmax], zero
+ * elsewhere; grad_pl(gall_pl,kall,2,3) at the pole centre from
+ * coef_grad_pl(0:gall_pl-1,3,2), zero on the ring.  The halo and pole-ring
+ * values come from nicam_comm_data_transfer(ctx, grad, grad_pl, kall, 3). */
+int nicam_oprt_gradient(nicam_ctx ctx, double *grad, double *grad_pl, const double *scl, const double *scl_pl,
+                        const double *coef_grad, const double *coef_grad_pl);
+
+/* Courant numbers and upwind masks of the six faces for the horizontal
+ * limiter: ch(gall,kall,lall,6) = flx_h / rhog, cmask = 0.5 - sign(0.5, ch - EPS)
+ * (1 on inflow faces); the pole ring against the centre's rhog_pl, the centre
+ * entry 0. */
+int nicam_tracer_courant(nicam_ctx ctx, double *ch, double *ch_pl, double *cmask, double *cmask_pl,
+                         const double *flx_h, const double *flx_h_pl, const double *rhog, const double *rhog_pl);
+
+/* Face values q_a(gall,kall,lall,6) for the horizontal limiter: for each arc
+ * AI, AIJ, AJ with origin in [gmin-1, gmax]^2, the upwind node's q plus its
+ * gradient dotted with GRD_xc(gall,kall,lall,3,3) - GRD_x(gall,1,lall,3) of
+ * that node.  Face a of the origin takes the side cmask(origin,a) picks and
+ * face a+3 of the neighbour the side cmask(neighbour,a+3) picks (each face
+ * by its owner's mask, as the limiter reads them); other faces are kept.
+ * The pole ring faces likewise (GRD_x_pl(gall_pl,1,2,3)). */
+int nicam_horizontal_remap(nicam_ctx ctx, double *q_a, double *q_a_pl, const double *q, const double *q_pl,
+                           const double *grad, const double *grad_pl, const double *cmask, const double *cmask_pl,
+                           const double *GRD_xc, const double *GRD_xc_pl, const double *GRD_x,
+                           const double *GRD_x_pl);
+
+/* The flux-form update of the tracer mass: rhogq -= sum over the six faces
+ * of flx_h * q_a, on the inner cells and the pole centres. */
+int nicam_tracer_update(nicam_ctx ctx, double *rhogq, double *rhogq_pl, const double *flx_h, const double *flx_h_pl,
+                        const double *q_a, const double *q_a_pl);
+
 #ifdef __cplusplus
 }
 #endif

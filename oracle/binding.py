@@ -85,6 +85,10 @@ class Oracle:
                                          [c_double, c_double, c_double, c_int])
         L.orc_oprt_diffusion.argtypes = [POINTER(OrcDims), P] + [P] * 10
         L.orc_oprt3d_divdamp.argtypes = [POINTER(OrcDims), P] + [P] * 27
+        L.orc_oprt_gradient.argtypes = [POINTER(OrcDims)] + [P] * 6
+        L.orc_tracer_courant.argtypes = [POINTER(OrcDims)] + [P] * 8
+        L.orc_horizontal_remap.argtypes = [POINTER(OrcDims)] + [P] * 12
+        L.orc_tracer_update.argtypes = [POINTER(OrcDims)] + [P] * 6
         L.orc_num_threads.restype = c_int
 
     def num_threads(self) -> int:
@@ -117,6 +121,38 @@ class Oracle:
         sgp = np.ascontiguousarray(have_sgp, dtype=np.int32)
         self.lib.orc_oprt3d_divdamp(ctypes.byref(D), sgp.ctypes.data, *[_p(out[k]) for k in DDMP_OUT],
                                     *[_p(ins[k]) for k in DDMP_IN])
+        return out
+
+    # ---- step glue ----
+    def oprt_gradient(self, spec, scl, scl_pl, coef_grad, coef_grad_pl):
+        D = self.dims(spec)
+        grad = np.zeros((3,) + spec.shape3)
+        grad_pl = np.zeros((3,) + spec.shape_pl)
+        ins = [_f64(a) for a in (scl, scl_pl, coef_grad, coef_grad_pl)]  # held for the call
+        self.lib.orc_oprt_gradient(ctypes.byref(D), _p(grad), _p(grad_pl), *[_p(a) for a in ins])
+        return dict(grad=grad, grad_pl=grad_pl)
+
+    def tracer_courant(self, spec, flx_h, flx_h_pl, rhog, rhog_pl, eps=None):
+        D = self.dims(spec, eps=eps)
+        out = dict(ch=np.zeros((6,) + spec.shape3), ch_pl=np.zeros(spec.shape_pl),
+                   cmask=np.zeros((6,) + spec.shape3), cmask_pl=np.zeros(spec.shape_pl))
+        ins = [_f64(a) for a in (flx_h, flx_h_pl, rhog, rhog_pl)]
+        self.lib.orc_tracer_courant(ctypes.byref(D), _p(out["ch"]), _p(out["ch_pl"]), _p(out["cmask"]),
+                                    _p(out["cmask_pl"]), *[_p(a) for a in ins])
+        return out
+
+    def horizontal_remap(self, spec, q_a, q_a_pl, q, q_pl, grad, grad_pl, cmask, cmask_pl, xc, xc_pl, x, x_pl):
+        D = self.dims(spec)
+        out = dict(q_a=_f64(q_a), q_a_pl=_f64(q_a_pl))
+        ins = [_f64(a) for a in (q, q_pl, grad, grad_pl, cmask, cmask_pl, xc, xc_pl, x, x_pl)]
+        self.lib.orc_horizontal_remap(ctypes.byref(D), _p(out["q_a"]), _p(out["q_a_pl"]), *[_p(a) for a in ins])
+        return out
+
+    def tracer_update(self, spec, rhogq, rhogq_pl, flx_h, flx_h_pl, q_a, q_a_pl):
+        D = self.dims(spec)
+        out = dict(rhogq=_f64(rhogq), rhogq_pl=_f64(rhogq_pl))
+        ins = [_f64(a) for a in (flx_h, flx_h_pl, q_a, q_a_pl)]
+        self.lib.orc_tracer_update(ctypes.byref(D), _p(out["rhogq"]), _p(out["rhogq_pl"]), *[_p(a) for a in ins])
         return out
 
     def grd_setup(self, gz, gzh):

@@ -937,3 +937,143 @@ void orc_oprt3d_divdamp(const orc_dims *D, const int *have_sgp,
 #undef C2W
 #undef C2W_PL
 }
+
+/* ---------------------------------------------------------------------------
+ * Step glue.  The reference has none of this code (its kernels take ch,
+ * cmask, q_a and ck as inputs, which its drivers describe as computed "in
+ * advance by other subroutine"); these are the NICAM forms of the Miura upwind scheme
+ * (docs/NICAM/src/20_intro_nicam.tex:368-438) that the library builds, and
+ * this restatement defines them for the parity tests.
+ * ------------------------------------------------------------------------- */
+#define CG(g, m, d, l) coef_grad[((((size_t)((l) - 1) * 3 + ((d) - 1)) * 7 + (m)) * gall + ((g) - 1))]
+#define CG_PL(m, d, l) coef_grad_pl[(((size_t)((l) - 1) * 3 + ((d) - 1)) * gall_pl + (m))]
+#define GX(g, l, d) GRD_x[((size_t)((d) - 1) * lall + ((l) - 1)) * gall + ((g) - 1)]
+#define GX_PL(v, l, d) GRD_x_pl[((size_t)((d) - 1) * LALL_PL + ((l) - 1)) * gall_pl + ((v) - 1)]
+
+void orc_oprt_gradient(const orc_dims *D, double *grad, double *grad_pl, const double *scl, const double *scl_pl,
+                       const double *coef_grad, const double *coef_grad_pl)
+{
+    const int iall = D->gall_1d, gall = iall * iall, kall = D->kall, lall = D->lall, gall_pl = D->gall_pl;
+    const int gmin = (2 - 1) * iall + 2, gmax = (iall - 1 - 1) * iall + iall - 1;
+    for (int l = 1; l <= lall; ++l)
+        for (int k = 1; k <= kall; ++k)
+            for (int d = 1; d <= 3; ++d) {
+                for (int g = 1; g <= gall; ++g) F4(grad, g, k, l, d) = 0.0;
+                for (int g = gmin; g <= gmax; ++g) {
+                    const int ij = g, ip1j = g + 1, ip1jp1 = g + iall + 1, ijp1 = g + iall;
+                    const int im1j = g - 1, im1jm1 = g - iall - 1, ijm1 = g - iall;
+                    F4(grad, g, k, l, d) = CG(g, 0, d, l) * F3(scl, ij, k, l) + CG(g, 1, d, l) * F3(scl, ip1j, k, l) +
+                                           CG(g, 2, d, l) * F3(scl, ip1jp1, k, l) + CG(g, 3, d, l) * F3(scl, ijp1, k, l) +
+                                           CG(g, 4, d, l) * F3(scl, im1j, k, l) + CG(g, 5, d, l) * F3(scl, im1jm1, k, l) +
+                                           CG(g, 6, d, l) * F3(scl, ijm1, k, l);
+                }
+            }
+    for (int l = 1; l <= LALL_PL; ++l)
+        for (int k = 1; k <= kall; ++k)
+            for (int d = 1; d <= 3; ++d) {
+                for (int v = 1; v <= gall_pl; ++v) P4(grad_pl, v, k, l, d) = 0.0;
+                if (!D->have_pl) continue;
+                double s = 0.0;
+                for (int v = 1; v <= gall_pl; ++v) s = s + CG_PL(v - 1, d, l) * P3(scl_pl, v, k, l);
+                P4(grad_pl, GSLF_PL, k, l, d) = s;
+            }
+}
+
+void orc_tracer_courant(const orc_dims *D, double *ch, double *ch_pl, double *cmask, double *cmask_pl,
+                        const double *flx_h, const double *flx_h_pl, const double *rhog, const double *rhog_pl)
+{
+    const int iall = D->gall_1d, gall = iall * iall, kall = D->kall, lall = D->lall, gall_pl = D->gall_pl;
+    const double eps = D->eps;
+    for (int m = 1; m <= 6; ++m)
+        for (int l = 1; l <= lall; ++l)
+            for (int k = 1; k <= kall; ++k)
+                for (int g = 1; g <= gall; ++g) {
+                    const double c = F4(flx_h, g, k, l, m) / F3(rhog, g, k, l);
+                    F4(ch, g, k, l, m) = c;
+                    F4(cmask, g, k, l, m) = 0.5 - copysign(0.5, c - eps);
+                }
+    for (int l = 1; l <= LALL_PL; ++l)
+        for (int k = 1; k <= kall; ++k) {
+            P3(ch_pl, GSLF_PL, k, l) = 0.0;
+            P3(cmask_pl, GSLF_PL, k, l) = 0.0;
+            for (int v = GMIN_PL; v <= gall_pl; ++v) {
+                const double c = D->have_pl ? P3(flx_h_pl, v, k, l) / P3(rhog_pl, GSLF_PL, k, l) : 0.0;
+                P3(ch_pl, v, k, l) = c;
+                P3(cmask_pl, v, k, l) = D->have_pl ? 0.5 - copysign(0.5, c - eps) : 0.0;
+            }
+        }
+}
+
+void orc_horizontal_remap(const orc_dims *D, double *q_a, double *q_a_pl, const double *q, const double *q_pl,
+                          const double *grad, const double *grad_pl, const double *cmask, const double *cmask_pl,
+                          const double *GRD_xc, const double *GRD_xc_pl, const double *GRD_x,
+                          const double *GRD_x_pl)
+{
+    const int iall = D->gall_1d, gall = iall * iall, kall = D->kall, lall = D->lall, gall_pl = D->gall_pl;
+    for (int l = 1; l <= lall; ++l)
+        for (int k = 1; k <= kall; ++k)
+            for (int j = 1; j <= iall - 1; ++j)
+                for (int i = 1; i <= iall - 1; ++i) {
+                    const int ij = (j - 1) * iall + i;
+                    const int nbs[3] = {ij + 1, ij + iall + 1, ij + iall};  /* AI, AIJ, AJ */
+                    for (int a = 1; a <= 3; ++a) {
+                        const int nb = nbs[a - 1];
+                        const double x1 = F5(GRD_xc, ij, k, l, a, 1), x2 = F5(GRD_xc, ij, k, l, a, 2),
+                                     x3 = F5(GRD_xc, ij, k, l, a, 3);
+                        const double qs = F3(q, ij, k, l) + (F4(grad, ij, k, l, 1) * (x1 - GX(ij, l, 1)) +
+                                                             F4(grad, ij, k, l, 2) * (x2 - GX(ij, l, 2)) +
+                                                             F4(grad, ij, k, l, 3) * (x3 - GX(ij, l, 3)));
+                        const double qn = F3(q, nb, k, l) + (F4(grad, nb, k, l, 1) * (x1 - GX(nb, l, 1)) +
+                                                             F4(grad, nb, k, l, 2) * (x2 - GX(nb, l, 2)) +
+                                                             F4(grad, nb, k, l, 3) * (x3 - GX(nb, l, 3)));
+                        /* each face with its owner's mask: the origin's face a, the neighbour's face a+3 */
+                        const double cs = F4(cmask, ij, k, l, a), cn = F4(cmask, nb, k, l, a + 3);
+                        F4(q_a, ij, k, l, a) = cs * qn + (1.0 - cs) * qs;
+                        F4(q_a, nb, k, l, a + 3) = cn * qs + (1.0 - cn) * qn;
+                    }
+                }
+    if (!D->have_pl) return;
+    for (int l = 1; l <= LALL_PL; ++l)
+        for (int k = 1; k <= kall; ++k)
+            for (int v = GMIN_PL; v <= gall_pl; ++v) {
+                const int n = GSLF_PL;
+                const double x1 = P4(GRD_xc_pl, v, k, l, 1), x2 = P4(GRD_xc_pl, v, k, l, 2),
+                             x3 = P4(GRD_xc_pl, v, k, l, 3);
+                const double qs = P3(q_pl, n, k, l) + (P4(grad_pl, n, k, l, 1) * (x1 - GX_PL(n, l, 1)) +
+                                                       P4(grad_pl, n, k, l, 2) * (x2 - GX_PL(n, l, 2)) +
+                                                       P4(grad_pl, n, k, l, 3) * (x3 - GX_PL(n, l, 3)));
+                const double qn = P3(q_pl, v, k, l) + (P4(grad_pl, v, k, l, 1) * (x1 - GX_PL(v, l, 1)) +
+                                                       P4(grad_pl, v, k, l, 2) * (x2 - GX_PL(v, l, 2)) +
+                                                       P4(grad_pl, v, k, l, 3) * (x3 - GX_PL(v, l, 3)));
+                const double c = P3(cmask_pl, v, k, l);
+                P3(q_a_pl, v, k, l) = c * qn + (1.0 - c) * qs;
+            }
+}
+
+void orc_tracer_update(const orc_dims *D, double *rhogq, double *rhogq_pl, const double *flx_h,
+                       const double *flx_h_pl, const double *q_a, const double *q_a_pl)
+{
+    const int iall = D->gall_1d, gall = iall * iall, kall = D->kall, lall = D->lall, gall_pl = D->gall_pl;
+    for (int l = 1; l <= lall; ++l)
+        for (int k = 1; k <= kall; ++k)
+            for (int j = 2; j <= iall - 1; ++j)
+                for (int i = 2; i <= iall - 1; ++i) {
+                    const int g = (j - 1) * iall + i;
+                    F3(rhogq, g, k, l) =
+                        F3(rhogq, g, k, l) -
+                        (F4(flx_h, g, k, l, 1) * F4(q_a, g, k, l, 1) + F4(flx_h, g, k, l, 2) * F4(q_a, g, k, l, 2) +
+                         F4(flx_h, g, k, l, 3) * F4(q_a, g, k, l, 3) + F4(flx_h, g, k, l, 4) * F4(q_a, g, k, l, 4) +
+                         F4(flx_h, g, k, l, 5) * F4(q_a, g, k, l, 5) + F4(flx_h, g, k, l, 6) * F4(q_a, g, k, l, 6));
+                }
+    if (!D->have_pl) return;
+    for (int l = 1; l <= LALL_PL; ++l)
+        for (int k = 1; k <= kall; ++k) {
+            double s = P3(flx_h_pl, GMIN_PL, k, l) * P3(q_a_pl, GMIN_PL, k, l);
+            for (int v = GMIN_PL + 1; v <= gall_pl; ++v) s = s + P3(flx_h_pl, v, k, l) * P3(q_a_pl, v, k, l);
+            P3(rhogq_pl, GSLF_PL, k, l) = P3(rhogq_pl, GSLF_PL, k, l) - s;
+        }
+}
+#undef CG
+#undef CG_PL
+#undef GX
+#undef GX_PL

@@ -127,6 +127,33 @@ void orc_oprt3d_divdamp(const orc_dims *D, const int *have_sgp,
                         const double *rdgz);
 
 /* Number of OpenMP threads the oracle runs with (1 when built without it). */
+/* Step glue (not in the reference; the NICAM forms of the Miura upwind
+ * scheme, docs/NICAM/src/20_intro_nicam.tex:368-438):
+ *   orc_oprt_gradient: grad(gall,kall,lall,3) = the 7-point stencil of
+ *     coef_grad(gall,0:6,3,lall) (ij, ip1j, ip1jp1, ijp1, im1j, im1jm1, ijm1)
+ *     on g in [gmin, gmax] of the flat index, zero elsewhere; the pole centre
+ *     sums coef_grad_pl(0:gall_pl-1,3,2) over the centre and its ring.
+ *   orc_tracer_courant: ch = flx_h/rhog, cmask = 0.5 - sign(0.5, ch - EPS)
+ *     (1 where the face is an inflow); the pole ring against the centre.
+ *   orc_horizontal_remap: for every arc AI/AIJ/AJ with origin in
+ *     [gmin-1, gmax]^2, the face value is the upwind node's q plus its
+ *     gradient dotted with (GRD_xc - GRD_x(node)), the upwind side chosen by
+ *     the mask of the face's owner: face a of the origin by cmask(origin,a),
+ *     face a+3 of the neighbour by cmask(neighbour,a+3); faces no arc
+ *     reaches are kept.
+ *   orc_tracer_update: rhogq -= sum over the six faces of flx_h*q_a, on the
+ *     inner cells and the pole centres. */
+void orc_oprt_gradient(const orc_dims *D, double *grad, double *grad_pl, const double *scl, const double *scl_pl,
+                       const double *coef_grad, const double *coef_grad_pl);
+void orc_tracer_courant(const orc_dims *D, double *ch, double *ch_pl, double *cmask, double *cmask_pl,
+                        const double *flx_h, const double *flx_h_pl, const double *rhog, const double *rhog_pl);
+void orc_horizontal_remap(const orc_dims *D, double *q_a, double *q_a_pl, const double *q, const double *q_pl,
+                          const double *grad, const double *grad_pl, const double *cmask, const double *cmask_pl,
+                          const double *GRD_xc, const double *GRD_xc_pl, const double *GRD_x,
+                          const double *GRD_x_pl);
+void orc_tracer_update(const orc_dims *D, double *rhogq, double *rhogq_pl, const double *flx_h,
+                       const double *flx_h_pl, const double *q_a, const double *q_a_pl);
+
 int orc_num_threads(void);
 
 #ifdef __cplusplus
