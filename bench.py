@@ -171,6 +171,31 @@ def time_extras(ctx, spec, case, coef, stream, steps):
     out["oprt3d_divdamp"] = dict(ms=round(ms, 5), alg_bytes=ob["oprt3d_divdamp"], gbs=round(gbs, 1),
                                  frac=round(gbs / perfmodel.HBM_PEAK_GBS, 4))
     del G, f
+    geo = case["geo"]
+    if "coef_grad" in geo:  # the step glue between the flux and the limiter, on the generated grid
+        F = up(case["flux"], drivers.FLUX_IN)
+        flx, flx_pl = z((6,) + spec.shape3), z(spec.shape_pl)
+        xc, xc_pl = z((3, 3) + spec.shape3), z((3,) + spec.shape_pl)
+        ctx.horizontal_flux(flx, flx_pl, xc, xc_pl, *[F[k] for k in drivers.FLUX_IN], 360.0)
+        Q = up(case["hlim"], ("q", "q_pl"))
+        M = up(geo, ("coef_grad", "coef_grad_pl", "GRD_x", "GRD_x_pl"))
+        ch, ch_pl, cm, cm_pl = z((6,) + spec.shape3), z(spec.shape_pl), z((6,) + spec.shape3), z(spec.shape_pl)
+        grad, grad_pl = z((3,) + spec.shape3), z((3,) + spec.shape_pl)
+        qa, qa_pl = z((6,) + spec.shape3), z(spec.shape_pl)
+        rq, rq_pl = F["rho"] * Q["q"], F["rho_pl"] * Q["q_pl"]
+        calls = dict(
+            tracer_courant=lambda: ctx.tracer_courant(ch, ch_pl, cm, cm_pl, flx, flx_pl, F["rho"], F["rho_pl"]),
+            oprt_gradient=lambda: ctx.oprt_gradient(grad, grad_pl, Q["q"], Q["q_pl"], M["coef_grad"],
+                                                    M["coef_grad_pl"]),
+            horizontal_remap=lambda: ctx.horizontal_remap(qa, qa_pl, Q["q"], Q["q_pl"], grad, grad_pl, cm, cm_pl, xc,
+                                                          xc_pl, M["GRD_x"], M["GRD_x_pl"]),
+            tracer_update=lambda: ctx.tracer_update(rq, rq_pl, flx, flx_pl, qa, qa_pl))
+        for name, fn in calls.items():
+            ms = timed(fn)
+            gbs = ob[name] / (ms * 1e-3) / 1e9
+            out[name] = dict(ms=round(ms, 5), alg_bytes=ob[name], gbs=round(gbs, 1),
+                             frac=round(gbs / perfmodel.HBM_PEAK_GBS, 4))
+        del F, Q, M, flx, xc, ch, cm, grad, qa, rq
     torch.cuda.synchronize()
     log("extras: " + ", ".join(f"{k}={v['ms']:.4f} ms" for k, v in out.items())
         + f" ({time.perf_counter() - t0:.1f} s)")

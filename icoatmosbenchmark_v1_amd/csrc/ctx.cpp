@@ -807,6 +807,103 @@ int nicam_tracer_update(nicam_ctx c, double *rhogq, double *rhogq_pl, const doub
     return stage_out(c, a, v);
 }
 
+int nicam_vertical_mass_flux(nicam_ctx c, double *flx_v, double *flx_v_pl, const double *rhogvx,
+                             const double *rhogvx_pl, const double *rhogvy, const double *rhogvy_pl,
+                             const double *rhogvz, const double *rhogvz_pl, const double *rhogw,
+                             const double *rhogw_pl, double dt)
+{
+    NICAM_TRY(check_ctx(c));
+    if (!c->have_vmetrics3d) return fail(NICAM_ERR_STATE, "vertical metrics not set (nicam_set_vmetrics3d)");
+    const void *req[] = {flx_v, rhogvx, rhogvy, rhogvz, rhogw};
+    const char *nm[] = {"flx_v", "rhogvx", "rhogvy", "rhogvz", "rhogw"};
+    for (int i = 0; i < 5; ++i) NICAM_TRY(need(req[i], nm[i]));
+    if (c->d.have_pl) {
+        const void *rp[] = {flx_v_pl, rhogvx_pl, rhogvy_pl, rhogvz_pl, rhogw_pl};
+        for (int i = 0; i < 5; ++i) NICAM_TRY(need(rp[i], "pole array"));
+    }
+    if (c->ptr_mode == NICAM_PTR_DEVICE)
+        return launch_vertical_mass_flux(c, flx_v, PL(flx_v_pl), rhogvx, PL(rhogvx_pl), rhogvy, PL(rhogvy_pl), rhogvz,
+                                         PL(rhogvz_pl), rhogw, PL(rhogw_pl), dt);
+    const size_t R = c->d.nreg() * 8, P = c->d.npl() * 8;
+    std::vector<HostArg> a = {{flx_v, R, true},   {PL(flx_v_pl), P, true},   {rhogvx, R, false},
+                              {PL(rhogvx_pl), P, false}, {rhogvy, R, false}, {PL(rhogvy_pl), P, false},
+                              {rhogvz, R, false}, {PL(rhogvz_pl), P, false}, {rhogw, R, false},
+                              {PL(rhogw_pl), P, false}};
+    std::vector<void *> v;
+    NICAM_TRY(stage_in(c, a, v));
+    auto D = [&](int i) { return static_cast<double *>(v[i]); };
+    NICAM_TRY(launch_vertical_mass_flux(c, D(0), D(1), D(2), D(3), D(4), D(5), D(6), D(7), D(8), D(9), dt));
+    return stage_out(c, a, v);
+}
+
+int nicam_vertical_courant(nicam_ctx c, double *ck, double *ck_pl, const double *flx_v, const double *flx_v_pl,
+                           const double *rhog, const double *rhog_pl)
+{
+    NICAM_TRY(check_ctx(c));
+    if (!c->have_rdgz) return fail(NICAM_ERR_STATE, "GRD_rdgz not set (nicam_grd_setup or nicam_set_rdgz)");
+    const void *req[] = {ck, flx_v, rhog};
+    const char *nm[] = {"ck", "flx_v", "rhog"};
+    for (int i = 0; i < 3; ++i) NICAM_TRY(need(req[i], nm[i]));
+    if (c->d.have_pl) {
+        const void *rp[] = {ck_pl, flx_v_pl, rhog_pl};
+        for (int i = 0; i < 3; ++i) NICAM_TRY(need(rp[i], "pole array"));
+    }
+    if (c->ptr_mode == NICAM_PTR_DEVICE)
+        return launch_vertical_courant(c, ck, PL(ck_pl), flx_v, PL(flx_v_pl), rhog, PL(rhog_pl));
+    const size_t R = c->d.nreg() * 8, P = c->d.npl() * 8;
+    std::vector<HostArg> a = {{ck, 2 * R, true},  {PL(ck_pl), 2 * P, true}, {flx_v, R, false},
+                              {PL(flx_v_pl), P, false}, {rhog, R, false}, {PL(rhog_pl), P, false}};
+    std::vector<void *> v;
+    NICAM_TRY(stage_in(c, a, v));
+    auto D = [&](int i) { return static_cast<double *>(v[i]); };
+    NICAM_TRY(launch_vertical_courant(c, D(0), D(1), D(2), D(3), D(4), D(5)));
+    return stage_out(c, a, v);
+}
+
+int nicam_vertical_face(nicam_ctx c, double *q_h, double *q_h_pl, const double *q, const double *q_pl)
+{
+    NICAM_TRY(check_ctx(c));
+    if (!c->have_vgrid) return fail(NICAM_ERR_STATE, "vertical grid not set (nicam_grd_setup)");
+    NICAM_TRY(need(q_h, "q_h"));
+    NICAM_TRY(need(q, "q"));
+    if (c->d.have_pl) {
+        NICAM_TRY(need(q_h_pl, "pole array"));
+        NICAM_TRY(need(q_pl, "pole array"));
+    }
+    if (c->ptr_mode == NICAM_PTR_DEVICE) return launch_vertical_face(c, q_h, PL(q_h_pl), q, PL(q_pl));
+    const size_t R = c->d.nreg() * 8, P = c->d.npl() * 8;
+    std::vector<HostArg> a = {{q_h, R, true}, {PL(q_h_pl), P, true}, {q, R, false}, {PL(q_pl), P, false}};
+    std::vector<void *> v;
+    NICAM_TRY(stage_in(c, a, v));
+    auto D = [&](int i) { return static_cast<double *>(v[i]); };
+    NICAM_TRY(launch_vertical_face(c, D(0), D(1), D(2), D(3)));
+    return stage_out(c, a, v);
+}
+
+int nicam_tracer_update_vertical(nicam_ctx c, double *rhogq, double *rhogq_pl, const double *flx_v,
+                                 const double *flx_v_pl, const double *q_h, const double *q_h_pl)
+{
+    NICAM_TRY(check_ctx(c));
+    if (!c->have_rdgz) return fail(NICAM_ERR_STATE, "GRD_rdgz not set (nicam_grd_setup or nicam_set_rdgz)");
+    const void *req[] = {rhogq, flx_v, q_h};
+    const char *nm[] = {"rhogq", "flx_v", "q_h"};
+    for (int i = 0; i < 3; ++i) NICAM_TRY(need(req[i], nm[i]));
+    if (c->d.have_pl) {
+        const void *rp[] = {rhogq_pl, flx_v_pl, q_h_pl};
+        for (int i = 0; i < 3; ++i) NICAM_TRY(need(rp[i], "pole array"));
+    }
+    if (c->ptr_mode == NICAM_PTR_DEVICE)
+        return launch_vertical_update(c, rhogq, PL(rhogq_pl), flx_v, PL(flx_v_pl), q_h, PL(q_h_pl));
+    const size_t R = c->d.nreg() * 8, P = c->d.npl() * 8;
+    std::vector<HostArg> a = {{rhogq, R, true},       {PL(rhogq_pl), P, true}, {flx_v, R, false},
+                              {PL(flx_v_pl), P, false}, {q_h, R, false},      {PL(q_h_pl), P, false}};
+    std::vector<void *> v;
+    NICAM_TRY(stage_in(c, a, v));
+    auto D = [&](int i) { return static_cast<double *>(v[i]); };
+    NICAM_TRY(launch_vertical_update(c, D(0), D(1), D(2), D(3), D(4), D(5)));
+    return stage_out(c, a, v);
+}
+
 #undef PL
 
 }  // extern "C"

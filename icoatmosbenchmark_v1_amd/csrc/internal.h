@@ -155,6 +155,14 @@ int launch_remap(nicam_ctx c, double *q_a, double *q_a_pl, const double *q, cons
                  const double *xc_pl, const double *x, const double *x_pl);
 int launch_update(nicam_ctx c, double *rhogq, double *rhogq_pl, const double *flx_h, const double *flx_h_pl,
                   const double *q_a, const double *q_a_pl);
+int launch_vertical_mass_flux(nicam_ctx c, double *flx_v, double *flx_v_pl, const double *vx, const double *vx_pl,
+                              const double *vy, const double *vy_pl, const double *vz, const double *vz_pl,
+                              const double *w, const double *w_pl, double dt);
+int launch_vertical_courant(nicam_ctx c, double *ck, double *ck_pl, const double *flx_v, const double *flx_v_pl,
+                            const double *rhog, const double *rhog_pl);
+int launch_vertical_face(nicam_ctx c, double *q_h, double *q_h_pl, const double *q, const double *q_pl);
+int launch_vertical_update(nicam_ctx c, double *rhogq, double *rhogq_pl, const double *flx_v, const double *flx_v_pl,
+                           const double *q_h, const double *q_h_pl);
 int launch_hflux(nicam_ctx c, double *flx_h, double *flx_h_pl, double *xc, double *xc_pl,
                  const double *rho, const double *rho_pl, const double *rvx, const double *rvx_pl,
                  const double *rvy, const double *rvy_pl, const double *rvz, const double *rvz_pl,

@@ -260,6 +260,35 @@ module nicam_tt_f
        integer(c_int) :: nicam_tracer_update
      end function nicam_tracer_update
 
+     function nicam_vertical_mass_flux(ctx, flx_v, flx_v_pl, rhogvx, rhogvx_pl, rhogvy, rhogvy_pl, &
+          rhogvz, rhogvz_pl, rhogw, rhogw_pl, dt) bind(C, name='nicam_vertical_mass_flux')
+       import :: c_ptr, c_int, c_double
+       type(c_ptr), value :: ctx, flx_v, flx_v_pl, rhogvx, rhogvx_pl, rhogvy, rhogvy_pl
+       type(c_ptr), value :: rhogvz, rhogvz_pl, rhogw, rhogw_pl
+       real(c_double), value :: dt
+       integer(c_int) :: nicam_vertical_mass_flux
+     end function nicam_vertical_mass_flux
+
+     function nicam_vertical_courant(ctx, ck, ck_pl, flx_v, flx_v_pl, rhog, rhog_pl) &
+          bind(C, name='nicam_vertical_courant')
+       import :: c_ptr, c_int
+       type(c_ptr), value :: ctx, ck, ck_pl, flx_v, flx_v_pl, rhog, rhog_pl
+       integer(c_int) :: nicam_vertical_courant
+     end function nicam_vertical_courant
+
+     function nicam_vertical_face(ctx, q_h, q_h_pl, q, q_pl) bind(C, name='nicam_vertical_face')
+       import :: c_ptr, c_int
+       type(c_ptr), value :: ctx, q_h, q_h_pl, q, q_pl
+       integer(c_int) :: nicam_vertical_face
+     end function nicam_vertical_face
+
+     function nicam_tracer_update_vertical(ctx, rhogq, rhogq_pl, flx_v, flx_v_pl, q_h, q_h_pl) &
+          bind(C, name='nicam_tracer_update_vertical')
+       import :: c_ptr, c_int
+       type(c_ptr), value :: ctx, rhogq, rhogq_pl, flx_v, flx_v_pl, q_h, q_h_pl
+       integer(c_int) :: nicam_tracer_update_vertical
+     end function nicam_tracer_update_vertical
+
      function nicam_grid_icosahedral(glevel, rlevel, radius, GRD_x, GRD_x_pl) bind(C, name='nicam_grid_icosahedral')
        import :: c_int, c_double
        integer(c_int), value :: glevel, rlevel

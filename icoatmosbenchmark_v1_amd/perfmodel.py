@@ -82,11 +82,19 @@ def operator_bytes(spec: GridSpec, word: int = 8) -> Dict[str, int]:
     on kmin+1..kmax, the three outputs on all levels; the same 36
     coefficients per cell.
     halo fill (nicam_comm_data_transfer, one variable): every map entry
-    reads one value and writes one per level."""
+    reads one value and writes one per level.
+    step glue, per cell-level of the array: courant reads flx_h (6) and rhog
+    and writes ch (6) and cmask (6); gradient reads q and writes grad (3),
+    plus 21 coef_grad per cell once; remap reads q, grad (3), cmask (6) and
+    GRD_xc (9) and writes q_a (6), plus GRD_x (3) per cell once; update reads
+    rhogq, flx_h (6) and q_a (6) and writes rhogq."""
     n, kall, lall, npl = spec.gall_1d, spec.kall, spec.lall, spec.gall_pl
     gall = n * n
     pl = 2 if spec.have_pl else 0
     diff = (3 * gall * kall + 36 * gall) * lall * word + pl * (3 * npl * kall + 9 * npl + 15) * word
     dcol = 4 * (kall - 2) + 9 * (kall - 3) + 3 * kall
     ddmp = (dcol * gall + 36 * gall) * lall * word + pl * (dcol * npl + 9 * npl + 15) * word
-    return dict(oprt_diffusion=diff, oprt3d_divdamp=ddmp)
+    cl = gall * kall * lall + pl * npl * kall
+    glue = dict(tracer_courant=19 * cl * word, oprt_gradient=(4 * cl + 21 * gall * lall) * word,
+                horizontal_remap=(25 * cl + 3 * gall * lall) * word, tracer_update=14 * cl * word)
+    return dict(oprt_diffusion=diff, oprt3d_divdamp=ddmp, **glue)

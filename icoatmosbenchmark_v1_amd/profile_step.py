@@ -26,6 +26,8 @@ def main() -> int:
     ap.add_argument("--kall", type=int, default=98)
     ap.add_argument("--lall", type=int, default=40)
     ap.add_argument("--seed", type=int, default=2024)
+    ap.add_argument("--extras", action="store_true",
+                    help="profile OPRT_diffusion, OPRT3D_divdamp and the step glue instead of the step")
     args = ap.parse_args()
 
     import numpy as np
@@ -45,7 +47,7 @@ def main() -> int:
 
     spec = synth.GridSpec(args.gall_1d, args.kall, args.lall, seed=args.seed)
     lv = synth.icosahedral_levels(spec.gall_1d, spec.lall)
-    geo = synth.icosahedral_geometry(*lv)[0] if lv else None
+    geo, coef = synth.icosahedral_geometry(*lv) if lv else (None, None)
     case = synth.make_case(spec, geo=geo)
     ctx = drivers.setup_context(spec, case["geo"], case["vi"])
     ctx.set_stream(torch.cuda.current_stream(dev))
@@ -62,6 +64,8 @@ def main() -> int:
     V = up(case["vlim"], drivers.VLIM_IN + ("q_h", "q_h_pl"))
     W = up(case["vi"], [k for k in drivers.VI_IN] + [k + "_pl" for k in drivers.VI_IN] + ["rhogw", "rhogw_pl"])
     torch.cuda.synchronize()
+    if args.extras:
+        return extras(ctx, spec, case, coef, args.steps, up, z)
     for s in range(args.steps):
         ctx.horizontal_flux(F["flx_h"], F["flx_h_pl"], F["GRD_xc"], F["GRD_xc_pl"], *[F[k] for k in drivers.FLUX_IN],
                             360.0)
@@ -75,6 +79,49 @@ def main() -> int:
         ctx.vi_rhow_solver(W["rhogw"], W["rhogw_pl"], *args_vi, 60.0)
         torch.cuda.synchronize()
         print(f"[profile_step] step {s + 1}/{args.steps} done", flush=True)
+    ctx.close()
+    return 0
+
+
+def extras(ctx, spec, case, coef, steps, up, z) -> int:
+    """OPRT_diffusion, OPRT3D_divdamp and the step glue (courant, gradient,
+    remap, update), each launched once per step."""
+    import torch
+
+    from icoatmosbenchmark_v1_amd import drivers, synth
+
+    if coef is None or "coef_grad" not in case["geo"]:
+        print("[profile_step] --extras needs an icosahedral grid size", file=sys.stderr)
+        return 2
+    geo = case["geo"]
+    D = up(synth.diffusion_inputs(spec, geo, coef), drivers.DIFF_IN)
+    D["dscl"], D["dscl_pl"] = z(spec.shape3), z(spec.shape_pl)
+    f = synth.divdamp_inputs(spec, geo, coef)
+    drivers.setup_divdamp(ctx, f)
+    G = up(f, drivers.DDMP_IN)
+    for k in drivers.DDMP_OUT:
+        G[k] = z(spec.shape_pl if k.endswith("_pl") else spec.shape3)
+    F = up(case["flux"], drivers.FLUX_IN)
+    flx, flx_pl = z((6,) + spec.shape3), z(spec.shape_pl)
+    xc, xc_pl = z((3, 3) + spec.shape3), z((3,) + spec.shape_pl)
+    ctx.horizontal_flux(flx, flx_pl, xc, xc_pl, *[F[k] for k in drivers.FLUX_IN], 360.0)
+    Q = up(case["hlim"], ("q", "q_pl"))
+    M = up(geo, ("coef_grad", "coef_grad_pl", "GRD_x", "GRD_x_pl"))
+    ch, ch_pl, cm, cm_pl = z((6,) + spec.shape3), z(spec.shape_pl), z((6,) + spec.shape3), z(spec.shape_pl)
+    grad, grad_pl = z((3,) + spec.shape3), z((3,) + spec.shape_pl)
+    qa, qa_pl = z((6,) + spec.shape3), z(spec.shape_pl)
+    rq, rq_pl = F["rho"] * Q["q"], F["rho_pl"] * Q["q_pl"]
+    torch.cuda.synchronize()
+    for s in range(steps):
+        ctx.oprt_diffusion(D["dscl"], D["dscl_pl"], *[D[k] for k in drivers.DIFF_IN])
+        ctx.oprt3d_divdamp(*[G[k] for k in drivers.DDMP_OUT], *[G[k] for k in drivers.DDMP_IN])
+        ctx.tracer_courant(ch, ch_pl, cm, cm_pl, flx, flx_pl, F["rho"], F["rho_pl"])
+        ctx.oprt_gradient(grad, grad_pl, Q["q"], Q["q_pl"], M["coef_grad"], M["coef_grad_pl"])
+        ctx.horizontal_remap(qa, qa_pl, Q["q"], Q["q_pl"], grad, grad_pl, cm, cm_pl, xc, xc_pl, M["GRD_x"],
+                             M["GRD_x_pl"])
+        ctx.tracer_update(rq, rq_pl, flx, flx_pl, qa, qa_pl)
+        torch.cuda.synchronize()
+        print(f"[profile_step] extras {s + 1}/{steps} done", flush=True)
     ctx.close()
     return 0
 

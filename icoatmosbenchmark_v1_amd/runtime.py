@@ -94,6 +94,10 @@ SIGNATURES = {
     "nicam_tracer_courant": (c_int, [_P] * 9),
     "nicam_horizontal_remap": (c_int, [_P] * 13),
     "nicam_tracer_update": (c_int, [_P] * 7),
+    "nicam_vertical_mass_flux": (c_int, [_P] * 11 + [c_double]),
+    "nicam_vertical_courant": (c_int, [_P] * 7),
+    "nicam_vertical_face": (c_int, [_P] * 5),
+    "nicam_tracer_update_vertical": (c_int, [_P] * 7),
     "nicam_oprt_diffusion": (c_int, [_P] + [_P] * 10),
     "nicam_set_rdgz": (c_int, [_P, _P]),
     "nicam_set_vmetrics3d": (c_int, [_P] + [_P] * 8),
@@ -433,6 +437,36 @@ class Context:
         a += self._args([(flx_h, "flx_h", 6 * R), (flx_h_pl, "flx_h_pl", P), (q_a, "q_a", 6 * R),
                          (q_a_pl, "q_a_pl", P)])
         _check(self.lib.nicam_tracer_update(self.h, *a), "nicam_tracer_update")
+
+    def vertical_mass_flux(self, flx_v, flx_v_pl, rhogvx, rhogvx_pl, rhogvy, rhogvy_pl, rhogvz, rhogvz_pl,
+                           rhogw, rhogw_pl, dt: float) -> None:
+        """nicam_vertical_mass_flux: flx_v = rhogw_vm*dt on the interior half levels."""
+        R, P = self.nreg, self.npl
+        a = self._args([(flx_v, "flx_v", R), (flx_v_pl, "flx_v_pl", P), (rhogvx, "rhogvx", R),
+                        (rhogvx_pl, "rhogvx_pl", P), (rhogvy, "rhogvy", R), (rhogvy_pl, "rhogvy_pl", P),
+                        (rhogvz, "rhogvz", R), (rhogvz_pl, "rhogvz_pl", P), (rhogw, "rhogw", R),
+                        (rhogw_pl, "rhogw_pl", P)])
+        _check(self.lib.nicam_vertical_mass_flux(self.h, *a, float(dt)), "nicam_vertical_mass_flux")
+
+    def vertical_courant(self, ck, ck_pl, flx_v, flx_v_pl, rhog, rhog_pl) -> None:
+        """nicam_vertical_courant: ck(...,2) of the lower and upper faces."""
+        R, P = self.nreg, self.npl
+        a = self._args([(ck, "ck", 2 * R), (ck_pl, "ck_pl", 2 * P), (flx_v, "flx_v", R), (flx_v_pl, "flx_v_pl", P),
+                        (rhog, "rhog", R), (rhog_pl, "rhog_pl", P)])
+        _check(self.lib.nicam_vertical_courant(self.h, *a), "nicam_vertical_courant")
+
+    def vertical_face(self, q_h, q_h_pl, q, q_pl) -> None:
+        """nicam_vertical_face: q_h = afact q(k) + bfact q(k-1) on kmin+1..kmax."""
+        R, P = self.nreg, self.npl
+        a = self._args([(q_h, "q_h", R), (q_h_pl, "q_h_pl", P), (q, "q", R), (q_pl, "q_pl", P)])
+        _check(self.lib.nicam_vertical_face(self.h, *a), "nicam_vertical_face")
+
+    def tracer_update_vertical(self, rhogq, rhogq_pl, flx_v, flx_v_pl, q_h, q_h_pl) -> None:
+        """nicam_tracer_update_vertical: rhogq -= the vertical flux divergence of flx_v*q_h."""
+        R, P = self.nreg, self.npl
+        a = self._args([(rhogq, "rhogq", R), (rhogq_pl, "rhogq_pl", P), (flx_v, "flx_v", R),
+                        (flx_v_pl, "flx_v_pl", P), (q_h, "q_h", R), (q_h_pl, "q_h_pl", P)])
+        _check(self.lib.nicam_tracer_update_vertical(self.h, *a), "nicam_tracer_update_vertical")
 
     def oprt3d_divdamp(self, ddivdx, ddivdx_pl, ddivdy, ddivdy_pl, ddivdz, ddivdz_pl, rhogvx, rhogvx_pl,
                        rhogvy, rhogvy_pl, rhogvz, rhogvz_pl, rhogw, rhogw_pl, coef_intp, coef_intp_pl,

@@ -185,7 +185,9 @@ def icosahedral_geometry(glevel: int, rlevel: int, radius: float = RADIUS):
     midpoints (nicam_grid_vertices), GMTR_p/t/a (nicam_gmtr_setup, the
     reference's dyn_metrics formulas) and the OPRT_diffusion coefficients
     (nicam_oprt_diffusion_setup).  Returns (geo, coef): geo in the layout of
-    make_geometry, coef the coefficient dict of runtime.oprt_diffusion_setup.
+    make_geometry plus the positions GRD_x/GRD_x_pl and the gradient
+    coefficients coef_grad/coef_grad_pl (nicam_oprt_gradient_setup) that the
+    step glue reads, coef the coefficient dict of runtime.oprt_diffusion_setup.
     For gall_1d = 2^(glevel-rlevel)+2 and lall = 10*4^rlevel."""
     from . import runtime
 
@@ -195,6 +197,7 @@ def icosahedral_geometry(glevel: int, rlevel: int, radius: float = RADIUS):
     v = runtime.grid_vertices(x, x_pl, sgp, radius)
     m = runtime.gmtr_setup(x, x_pl, v["xt"], v["xt_pl"], v["s"], v["s_pl"], sgp, radius)
     coef = runtime.oprt_diffusion_setup(m, sgp)
+    cg, cg_pl = runtime.oprt_coef_setup("grad", m, sgp)
     points = np.ascontiguousarray(np.moveaxis(x[:, :, 0, :], 0, -1))  # (lall, gall, 3)
     # the unused halo corners hold no point: give them the first inner point
     empty = np.linalg.norm(points, axis=-1) == 0.0
@@ -202,7 +205,7 @@ def icosahedral_geometry(glevel: int, rlevel: int, radius: float = RADIUS):
     points = np.where(empty[..., None], first[:, None, :], points)
     geo = dict(GRD_xr=v["xr"], GRD_xr_pl=v["xr_pl"], GMTR_p=m["p"], GMTR_p_pl=m["p_pl"], GMTR_t=m["t"],
                GMTR_t_pl=m["t_pl"], GMTR_a=m["a"], GMTR_a_pl=m["a_pl"], have_sgp=np.asarray(sgp, dtype=np.int32),
-               points=points)
+               points=points, GRD_x=x, GRD_x_pl=x_pl, coef_grad=cg, coef_grad_pl=cg_pl)
     return geo, coef
 
 

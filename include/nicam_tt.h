@@ -416,6 +416,41 @@ int nicam_horizontal_remap(nicam_ctx ctx, double *q_a, double *q_a_pl, const dou
 int nicam_tracer_update(nicam_ctx ctx, double *rhogq, double *rhogq_pl, const double *flx_h, const double *flx_h_pl,
                         const double *q_a, const double *q_a_pl);
 
+/* The vertical half of the step glue.  No reference counterpart: the
+ * reference's vertical limiter takes ck and q_h as inputs
+ * (dyn_vert_adv_limiter/src/main.f90), computed by code it does not hold.
+ * Arrays (gall,kall,lall), poles (gall_pl,kall,2); kmin = 2, kmax = kall-1. */
+
+/* Vertical mass flux through the half levels over dt:
+ * flx_v(k) = rhogw_vm(k) * dt on kmin+1..kmax, 0 on every other level, where
+ * rhogw_vm is the contravariant vertical mass flux OPRT3D_divdamp forms
+ * (dyn_divdamp/src/mod_oprt3d.f90:148-155) from rhogvx/vy/vz at k and k-1,
+ * rhogw and the metrics of nicam_set_vmetrics3d. */
+int nicam_vertical_mass_flux(nicam_ctx ctx, double *flx_v, double *flx_v_pl, const double *rhogvx,
+                             const double *rhogvx_pl, const double *rhogvy, const double *rhogvy_pl,
+                             const double *rhogvz, const double *rhogvz_pl, const double *rhogw,
+                             const double *rhogw_pl, double dt);
+
+/* Vertical Courant numbers for nicam_vertical_limiter_thuburn:
+ * ck(gall,kall,lall,2) with ck(k,1) = -flx_v(k)/rhog(k)*GRD_rdgz(k) (lower
+ * face) and ck(k,2) = flx_v(k+1)/rhog(k)*GRD_rdgz(k) (upper face) on the
+ * interior faces of kmin..kmax, 0 elsewhere; negative where the face is an
+ * inflow.  The closed faces hold +0 in ck(kmin,1) and -0 in ck(kmax+1,1):
+ * the limiter's flags (dyn_vert_adv_limiter/src/mod_src_tracer.f90:113-114)
+ * read both as outflow, so its bounds never take q at kmin-1 or kmax+1. */
+int nicam_vertical_courant(nicam_ctx ctx, double *ck, double *ck_pl, const double *flx_v, const double *flx_v_pl,
+                           const double *rhog, const double *rhog_pl);
+
+/* Half-level face values q_h(k) = GRD_afact(k) q(k) + GRD_bfact(k) q(k-1) on
+ * kmin+1..kmax (the factors of nicam_grd_setup); other levels kept. */
+int nicam_vertical_face(nicam_ctx ctx, double *q_h, double *q_h_pl, const double *q, const double *q_pl);
+
+/* Flux-form vertical update of the tracer mass on kmin..kmax:
+ * rhogq(k) -= (flx_v(k+1) q_h(k+1) - flx_v(k) q_h(k)) * GRD_rdgz(k), the
+ * boundary faces kmin and kmax+1 carrying no flux. */
+int nicam_tracer_update_vertical(nicam_ctx ctx, double *rhogq, double *rhogq_pl, const double *flx_v,
+                                 const double *flx_v_pl, const double *q_h, const double *q_h_pl);
+
 #ifdef __cplusplus
 }
 #endif

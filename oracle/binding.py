@@ -89,6 +89,10 @@ class Oracle:
         L.orc_tracer_courant.argtypes = [POINTER(OrcDims)] + [P] * 8
         L.orc_horizontal_remap.argtypes = [POINTER(OrcDims)] + [P] * 12
         L.orc_tracer_update.argtypes = [POINTER(OrcDims)] + [P] * 6
+        L.orc_vertical_mass_flux.argtypes = [POINTER(OrcDims)] + [P] * 16 + [c_double]
+        L.orc_vertical_courant.argtypes = [POINTER(OrcDims)] + [P] * 7
+        L.orc_vertical_face.argtypes = [POINTER(OrcDims)] + [P] * 6
+        L.orc_tracer_update_vertical.argtypes = [POINTER(OrcDims)] + [P] * 7
         L.orc_num_threads.restype = c_int
 
     def num_threads(self) -> int:
@@ -153,6 +157,41 @@ class Oracle:
         out = dict(rhogq=_f64(rhogq), rhogq_pl=_f64(rhogq_pl))
         ins = [_f64(a) for a in (flx_h, flx_h_pl, q_a, q_a_pl)]
         self.lib.orc_tracer_update(ctypes.byref(D), _p(out["rhogq"]), _p(out["rhogq_pl"]), *[_p(a) for a in ins])
+        return out
+
+    def vertical_mass_flux(self, spec, f, m, dt):
+        """f: rhogvx, rhogvy, rhogvz, rhogw (+ _pl); m: C2WfactGz, RGAMH, RGSQRTH (+ _pl)."""
+        D = self.dims(spec)
+        out = dict(flx_v=np.zeros(spec.shape3), flx_v_pl=np.zeros(spec.shape_pl))
+        ins = [_f64(f[k]) for k in ("rhogvx", "rhogvx_pl", "rhogvy", "rhogvy_pl", "rhogvz", "rhogvz_pl", "rhogw",
+                                    "rhogw_pl")]
+        ins += [_f64(m[k]) for k in ("C2WfactGz", "C2WfactGz_pl", "RGAMH", "RGAMH_pl", "RGSQRTH", "RGSQRTH_pl")]
+        self.lib.orc_vertical_mass_flux(ctypes.byref(D), _p(out["flx_v"]), _p(out["flx_v_pl"]),
+                                        *[_p(a) for a in ins], float(dt))
+        return out
+
+    def vertical_courant(self, spec, rdgz, flx_v, flx_v_pl, rhog, rhog_pl):
+        D = self.dims(spec)
+        out = dict(ck=np.zeros((2,) + spec.shape3), ck_pl=np.zeros((2,) + spec.shape_pl))
+        ins = [_f64(a) for a in (rdgz, flx_v, flx_v_pl, rhog, rhog_pl)]
+        self.lib.orc_vertical_courant(ctypes.byref(D), _p(ins[0]), _p(out["ck"]), _p(out["ck_pl"]),
+                                      *[_p(a) for a in ins[1:]])
+        return out
+
+    def vertical_face(self, spec, afact, bfact, q_h, q_h_pl, q, q_pl):
+        D = self.dims(spec)
+        out = dict(q_h=_f64(q_h), q_h_pl=_f64(q_h_pl))
+        ins = [_f64(a) for a in (afact, bfact, q, q_pl)]
+        self.lib.orc_vertical_face(ctypes.byref(D), _p(ins[0]), _p(ins[1]), _p(out["q_h"]), _p(out["q_h_pl"]),
+                                   _p(ins[2]), _p(ins[3]))
+        return out
+
+    def tracer_update_vertical(self, spec, rdgz, rhogq, rhogq_pl, flx_v, flx_v_pl, q_h, q_h_pl):
+        D = self.dims(spec)
+        out = dict(rhogq=_f64(rhogq), rhogq_pl=_f64(rhogq_pl))
+        ins = [_f64(a) for a in (rdgz, flx_v, flx_v_pl, q_h, q_h_pl)]
+        self.lib.orc_tracer_update_vertical(ctypes.byref(D), _p(ins[0]), _p(out["rhogq"]), _p(out["rhogq_pl"]),
+                                            *[_p(a) for a in ins[1:]])
         return out
 
     def grd_setup(self, gz, gzh):

@@ -1077,3 +1077,135 @@ void orc_tracer_update(const orc_dims *D, double *rhogq, double *rhogq_pl, const
 #undef CG_PL
 #undef GX
 #undef GX_PL
+
+/* ---------------------------------------------------------------------------
+ * Vertical step glue (no reference code; the reference's vertical limiter
+ * takes ck and q_h as inputs).  kmin = 2, kmax = kall-1.
+ * ------------------------------------------------------------------------- */
+#define C2W(g, k, c, l) C2WfactGz[((((size_t)(l) - 1) * 6 + ((c) - 1)) * kall + ((k) - 1)) * gall + ((g) - 1)]
+#define C2W_PL(v, k, c, l) \
+    C2WfactGz_pl[((((size_t)(l) - 1) * 6 + ((c) - 1)) * kall + ((k) - 1)) * gall_pl + ((v) - 1)]
+
+/* rhogw_vm as dyn_divdamp/src/mod_oprt3d.f90:148-155 forms it (and :325-332
+ * for the poles), times dt, on kmin+1..kmax; 0 on the other levels. */
+void orc_vertical_mass_flux(const orc_dims *D, double *flx_v, double *flx_v_pl, const double *vx, const double *vx_pl,
+                            const double *vy, const double *vy_pl, const double *vz, const double *vz_pl,
+                            const double *w, const double *w_pl, const double *C2WfactGz,
+                            const double *C2WfactGz_pl, const double *RGAMH, const double *RGAMH_pl,
+                            const double *RGSQRTH, const double *RGSQRTH_pl, double dt)
+{
+    const int iall = D->gall_1d, gall = iall * iall, kall = D->kall, lall = D->lall, gall_pl = D->gall_pl;
+    const int kmin = 2, kmax = kall - 1;
+    for (int l = 1; l <= lall; ++l)
+        for (int k = 1; k <= kall; ++k)
+            for (int g = 1; g <= gall; ++g) {
+                double f = 0.0;
+                if (k >= kmin + 1 && k <= kmax) {
+                    const double vm = (C2W(g, k, 1, l) * F3(vx, g, k, l) + C2W(g, k, 2, l) * F3(vx, g, k - 1, l) +
+                                       C2W(g, k, 3, l) * F3(vy, g, k, l) + C2W(g, k, 4, l) * F3(vy, g, k - 1, l) +
+                                       C2W(g, k, 5, l) * F3(vz, g, k, l) + C2W(g, k, 6, l) * F3(vz, g, k - 1, l)) *
+                                          F3(RGAMH, g, k, l) +
+                                      F3(w, g, k, l) * F3(RGSQRTH, g, k, l);
+                    f = vm * dt;
+                }
+                F3(flx_v, g, k, l) = f;
+            }
+    if (!D->have_pl) return;
+    for (int l = 1; l <= LALL_PL; ++l)
+        for (int k = 1; k <= kall; ++k)
+            for (int v = 1; v <= gall_pl; ++v) {
+                double f = 0.0;
+                if (k >= kmin + 1 && k <= kmax) {
+                    const double vm =
+                        (C2W_PL(v, k, 1, l) * P3(vx_pl, v, k, l) + C2W_PL(v, k, 2, l) * P3(vx_pl, v, k - 1, l) +
+                         C2W_PL(v, k, 3, l) * P3(vy_pl, v, k, l) + C2W_PL(v, k, 4, l) * P3(vy_pl, v, k - 1, l) +
+                         C2W_PL(v, k, 5, l) * P3(vz_pl, v, k, l) + C2W_PL(v, k, 6, l) * P3(vz_pl, v, k - 1, l)) *
+                            P3(RGAMH_pl, v, k, l) +
+                        P3(w_pl, v, k, l) * P3(RGSQRTH_pl, v, k, l);
+                    f = vm * dt;
+                }
+                P3(flx_v_pl, v, k, l) = f;
+            }
+}
+#undef C2W
+#undef C2W_PL
+
+/* ck(k,1) = -flx_v(k)/rhog(k)*rdgz(k), ck(k,2) = flx_v(k+1)/rhog(k)*rdgz(k)
+ * on the interior faces of kmin..kmax, 0 elsewhere; the closed faces carry
+ * +0 in ck(kmin,1) and -0 in ck(kmax+1,1), which the vertical limiter's flags
+ * (0.5 - sign(0.5, ck(k,1)) below, 0.5 + sign(0.5, ck(k+1,1)) above) read as
+ * outflow. */
+void orc_vertical_courant(const orc_dims *D, const double *rdgz, double *ck, double *ck_pl, const double *flx_v,
+                          const double *flx_v_pl, const double *rhog, const double *rhog_pl)
+{
+    const int iall = D->gall_1d, gall = iall * iall, kall = D->kall, lall = D->lall, gall_pl = D->gall_pl;
+    const int kmin = 2, kmax = kall - 1;
+    for (int l = 1; l <= lall; ++l)
+        for (int k = 1; k <= kall; ++k)
+            for (int g = 1; g <= gall; ++g) {
+                double lo = 0.0, up = 0.0;
+                if (k >= kmin && k <= kmax) {
+                    if (k > kmin) lo = -F3(flx_v, g, k, l) / F3(rhog, g, k, l) * rdgz[k - 1];
+                    if (k < kmax) up = F3(flx_v, g, k + 1, l) / F3(rhog, g, k, l) * rdgz[k - 1];
+                } else if (k == kmax + 1) {
+                    lo = -0.0;
+                }
+                F4(ck, g, k, l, 1) = lo;
+                F4(ck, g, k, l, 2) = up;
+            }
+    if (!D->have_pl) return;
+    for (int l = 1; l <= LALL_PL; ++l)
+        for (int k = 1; k <= kall; ++k)
+            for (int v = 1; v <= gall_pl; ++v) {
+                double lo = 0.0, up = 0.0;
+                if (k >= kmin && k <= kmax) {
+                    if (k > kmin) lo = -P3(flx_v_pl, v, k, l) / P3(rhog_pl, v, k, l) * rdgz[k - 1];
+                    if (k < kmax) up = P3(flx_v_pl, v, k + 1, l) / P3(rhog_pl, v, k, l) * rdgz[k - 1];
+                } else if (k == kmax + 1) {
+                    lo = -0.0;
+                }
+                P4(ck_pl, v, k, l, 1) = lo;
+                P4(ck_pl, v, k, l, 2) = up;
+            }
+}
+
+/* q_h(k) = afact(k) q(k) + bfact(k) q(k-1) on kmin+1..kmax. */
+void orc_vertical_face(const orc_dims *D, const double *afact, const double *bfact, double *q_h, double *q_h_pl,
+                       const double *q, const double *q_pl)
+{
+    const int iall = D->gall_1d, gall = iall * iall, kall = D->kall, lall = D->lall, gall_pl = D->gall_pl;
+    const int kmin = 2, kmax = kall - 1;
+    for (int l = 1; l <= lall; ++l)
+        for (int k = kmin + 1; k <= kmax; ++k)
+            for (int g = 1; g <= gall; ++g)
+                F3(q_h, g, k, l) = afact[k - 1] * F3(q, g, k, l) + bfact[k - 1] * F3(q, g, k - 1, l);
+    if (!D->have_pl) return;
+    for (int l = 1; l <= LALL_PL; ++l)
+        for (int k = kmin + 1; k <= kmax; ++k)
+            for (int v = 1; v <= gall_pl; ++v)
+                P3(q_h_pl, v, k, l) = afact[k - 1] * P3(q_pl, v, k, l) + bfact[k - 1] * P3(q_pl, v, k - 1, l);
+}
+
+/* rhogq(k) -= (flx_v(k+1) q_h(k+1) - flx_v(k) q_h(k)) rdgz(k) on kmin..kmax,
+ * no flux through the faces kmin and kmax+1. */
+void orc_tracer_update_vertical(const orc_dims *D, const double *rdgz, double *rhogq, double *rhogq_pl,
+                                const double *flx_v, const double *flx_v_pl, const double *q_h, const double *q_h_pl)
+{
+    const int iall = D->gall_1d, gall = iall * iall, kall = D->kall, lall = D->lall, gall_pl = D->gall_pl;
+    const int kmin = 2, kmax = kall - 1;
+    for (int l = 1; l <= lall; ++l)
+        for (int k = kmin; k <= kmax; ++k)
+            for (int g = 1; g <= gall; ++g) {
+                const double up = (k + 1 <= kmax) ? F3(flx_v, g, k + 1, l) * F3(q_h, g, k + 1, l) : 0.0;
+                const double lo = (k >= kmin + 1) ? F3(flx_v, g, k, l) * F3(q_h, g, k, l) : 0.0;
+                F3(rhogq, g, k, l) = F3(rhogq, g, k, l) - (up - lo) * rdgz[k - 1];
+            }
+    if (!D->have_pl) return;
+    for (int l = 1; l <= LALL_PL; ++l)
+        for (int k = kmin; k <= kmax; ++k)
+            for (int v = 1; v <= gall_pl; ++v) {
+                const double up = (k + 1 <= kmax) ? P3(flx_v_pl, v, k + 1, l) * P3(q_h_pl, v, k + 1, l) : 0.0;
+                const double lo = (k >= kmin + 1) ? P3(flx_v_pl, v, k, l) * P3(q_h_pl, v, k, l) : 0.0;
+                P3(rhogq_pl, v, k, l) = P3(rhogq_pl, v, k, l) - (up - lo) * rdgz[k - 1];
+            }
+}

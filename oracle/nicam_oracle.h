@@ -156,6 +156,27 @@ void orc_tracer_update(const orc_dims *D, double *rhogq, double *rhogq_pl, const
 
 int orc_num_threads(void);
 
+/* Vertical step glue (not in the reference), kmin = 2, kmax = kall-1:
+ *   orc_vertical_mass_flux: flx_v = rhogw_vm*dt on kmin+1..kmax (rhogw_vm as
+ *     dyn_divdamp/src/mod_oprt3d.f90:148-155 forms it), 0 elsewhere;
+ *   orc_vertical_courant: ck(k,1) = -flx_v(k)/rhog(k)*rdgz(k),
+ *     ck(k,2) = flx_v(k+1)/rhog(k)*rdgz(k) on the interior faces of
+ *     kmin..kmax, 0 elsewhere, +0 in ck(kmin,1) and -0 in ck(kmax+1,1);
+ *   orc_vertical_face: q_h(k) = afact(k) q(k) + bfact(k) q(k-1) on kmin+1..kmax;
+ *   orc_tracer_update_vertical: rhogq(k) -= (flx_v(k+1) q_h(k+1) - flx_v(k) q_h(k)) rdgz(k)
+ *     on kmin..kmax, the faces kmin and kmax+1 carrying nothing. */
+void orc_vertical_mass_flux(const orc_dims *D, double *flx_v, double *flx_v_pl, const double *vx, const double *vx_pl,
+                            const double *vy, const double *vy_pl, const double *vz, const double *vz_pl,
+                            const double *w, const double *w_pl, const double *C2WfactGz,
+                            const double *C2WfactGz_pl, const double *RGAMH, const double *RGAMH_pl,
+                            const double *RGSQRTH, const double *RGSQRTH_pl, double dt);
+void orc_vertical_courant(const orc_dims *D, const double *rdgz, double *ck, double *ck_pl, const double *flx_v,
+                          const double *flx_v_pl, const double *rhog, const double *rhog_pl);
+void orc_vertical_face(const orc_dims *D, const double *afact, const double *bfact, double *q_h, double *q_h_pl,
+                       const double *q, const double *q_pl);
+void orc_tracer_update_vertical(const orc_dims *D, const double *rdgz, double *rhogq, double *rhogq_pl,
+                                const double *flx_v, const double *flx_v_pl, const double *q_h, const double *q_h_pl);
+
 #ifdef __cplusplus
 }
 #endif
