@@ -51,6 +51,7 @@ __global__ void __launch_bounds__(BLK) gradient_kernel(GradArgs A)
         gradient_pole(A, (b - A.nmain) * BLK + threadIdx.x);
         return;
     }
+    b = (int)xcd_contiguous(b, A.nmain);  // a region's chunks share halo rows: one XCD
     const int iall = A.iall, gall = A.gall, kall = A.kall;
     const int chunk = b % A.nchunk, kr = (b / A.nchunk) % A.nkr, l = b / (A.nchunk * A.nkr);
     const int k0 = kr * A.kper, k1 = min(kall, k0 + A.kper);
@@ -122,7 +123,7 @@ struct RemapArgs {
     double *q_a, *q_a_pl;
     const double *q, *q_pl, *grad, *grad_pl, *cmask, *cmask_pl, *xc, *xc_pl, *x, *x_pl;
     int iall, gall, kall, lall, gall_pl, have_pl;
-    long nreg;
+    long nreg, nmain;  // cell-levels of the regular regions, their work-groups
 };
 
 // Face value on the arc a (0 AI, 1 AIJ, 2 AJ) from origin o to neighbour n,
@@ -155,10 +156,9 @@ __device__ __forceinline__ double face_value(const RemapArgs &A, size_t fb, size
 // im1jm1 and ijm1, each with its own face's mask.
 __global__ void __launch_bounds__(BLK) remap_kernel(RemapArgs A)
 {
-    const long t = (long)blockIdx.x * BLK + threadIdx.x;
     const int iall = A.iall, gall = A.gall, kall = A.kall;
-    if (t >= A.nreg) {
-        const long p = t - A.nreg;  // one work-item per (k, pole region)
+    if ((long)blockIdx.x >= A.nmain) {
+        const long p = ((long)blockIdx.x - A.nmain) * BLK + threadIdx.x;  // one work-item per (k, pole region)
         if (!A.have_pl || p >= (long)kall * kLallPl) return;
         const int npl = A.gall_pl, k = (int)(p % kall), l = (int)(p / kall);
         const size_t fb = ((size_t)l * kall + k) * npl, plane = (size_t)kLallPl * kall * npl;
@@ -177,6 +177,9 @@ __global__ void __launch_bounds__(BLK) remap_kernel(RemapArgs A)
         }
         return;
     }
+    // the rows above and below a cell lie in the neighbouring work-groups: keep those on one XCD
+    const long t = xcd_contiguous(blockIdx.x, A.nmain) * BLK + threadIdx.x;
+    if (t >= A.nreg) return;
     const int g = (int)(t % gall);
     const long kl = t / gall;
     const int k = (int)(kl % kall), l = (int)(kl / kall);
@@ -310,7 +313,8 @@ int launch_remap(nicam_ctx c, double *q_a, double *q_a_pl, const double *q, cons
     A.gall_pl = D.gall_pl;
     A.have_pl = D.have_pl && q_a_pl;
     A.nreg = (long)D.gall * D.kall * D.lall;
-    const long n = A.nreg + (A.have_pl ? (long)D.kall * kLallPl : 0), nb = (n + BLK - 1) / BLK;
+    A.nmain = (A.nreg + BLK - 1) / BLK;
+    const long nb = A.nmain + (A.have_pl ? ((long)D.kall * kLallPl + BLK - 1) / BLK : 0);
     if (nb > 0x7fffffffL) return fail(NICAM_ERR_ARG, "grid too large for one launch");
     hipLaunchKernelGGL(remap_kernel, dim3((unsigned)nb), dim3(BLK), 0, c->stream, A);
     NICAM_HIP(hipGetLastError());

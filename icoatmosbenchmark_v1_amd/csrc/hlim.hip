@@ -18,8 +18,24 @@
 namespace nicam {
 namespace {
 
-constexpr int CHUNK = 256;
+#ifndef NICAM_HLIM_CHUNK
+#define NICAM_HLIM_CHUNK 256
+#endif
+#ifndef NICAM_HLIM_NTLOAD
+#define NICAM_HLIM_NTLOAD 0
+#endif
+constexpr int CHUNK = NICAM_HLIM_CHUNK;  // cells (work-items) per work-group
 constexpr int MAXI = 160;
+
+// A load of a streamed input read once (optionally with the nontemporal hint).
+__device__ __forceinline__ double ld_once(const double *p)
+{
+#if NICAM_HLIM_NTLOAD
+    return __builtin_nontemporal_load(p);
+#else
+    return *p;
+#endif
+}
 
 __device__ __forceinline__ double dmin(double a, double b) { return (b < a) ? b : a; }
 __device__ __forceinline__ double dmax(double a, double b) { return (b > a) ? b : a; }
@@ -116,13 +132,13 @@ __global__ void __launch_bounds__(CHUNK) hlim_kernel(HlimArgs A)
 #pragma unroll
         for (int m = 0; m < 3; ++m) {
             arc_minmax(m, g, omn[m], omx[m]);
-            ocm[m] = A.cmask[(size_t)m * plane + fb + g];
+            ocm[m] = A.cmask[(size_t)m * plane + fb + g];  // re-read by the neighbours
         }
     }
 
     // ---- phase 1: allowable range of the cell (HLIM:216-301) ----
     double qout_min, qout_max;
-    const double qc = A.q[fb + g];
+    const double qc = Q(g);
     if ((A.phases & 1) && i >= 1 && i <= iall - 2 && j >= 1 && j <= iall - 2) {
         double qinmn[6], qinmx[6];
 #pragma unroll
@@ -149,7 +165,7 @@ __global__ void __launch_bounds__(CHUNK) hlim_kernel(HlimArgs A)
         double chv[6], chm[6];
 #pragma unroll
         for (int f = 0; f < 6; ++f) {
-            chv[f] = A.ch[(size_t)f * plane + fb + g];
+            chv[f] = ld_once(&A.ch[(size_t)f * plane + fb + g]);
             chm[f] = dmin(chv[f], 0.0);
         }
         const double Cin_sum = chm[0] + chm[1] + chm[2] + chm[3] + chm[4] + chm[5];
@@ -160,7 +176,7 @@ __global__ void __launch_bounds__(CHUNK) hlim_kernel(HlimArgs A)
         const double CQin_max_sum = chm[0] * qinmx[0] + chm[1] * qinmx[1] + chm[2] * qinmx[2] +
                                     chm[3] * qinmx[3] + chm[4] * qinmx[4] + chm[5] * qinmx[5];
         const double zerosw = 0.5 - sign_half(fabs(Cout_sum) - A.eps);
-        const double dc = A.d[fb + g];
+        const double dc = ld_once(&A.d[fb + g]);
         qout_min = (qc - CQin_max_sum - qnext_max * (1.0 - Cin_sum - Cout_sum + dc)) / (Cout_sum + zerosw) *
                        (1.0 - zerosw) + qc * zerosw;
         qout_max = (qc - CQin_min_sum - qnext_min * (1.0 - Cin_sum - Cout_sum + dc)) / (Cout_sum + zerosw) *
@@ -177,7 +193,7 @@ __global__ void __launch_bounds__(CHUNK) hlim_kernel(HlimArgs A)
     // ---- phase 2: clamp the face values of the own arcs (HLIM:377-410) ----
     if (own && (A.phases & 2)) {
         const int nb[3] = {g + 1, g + iall + 1, g + iall};
-        const double qo_min_c = A.Qpost[fb + g], qo_max_c = A.Qpost[plane + fb + g];
+        const double qo_min_c = A.Qpost[fb + g], qo_max_c = A.Qpost[plane + fb + g];  // re-read by neighbours
 #pragma unroll
         for (int m = 0; m < 3; ++m) {
             const double c = ocm[m];
@@ -186,7 +202,7 @@ __global__ void __launch_bounds__(CHUNK) hlim_kernel(HlimArgs A)
             const double in_mn_n = (c)*BIG + (1.0 - c) * omn[m];
             const double in_mx_n = (c) * (-BIG) + (1.0 - c) * omx[m];
             const double qo_min_n = A.Qpost[fb + nb[m]], qo_max_n = A.Qpost[plane + fb + nb[m]];
-            double qa = A.q_a[(size_t)m * plane + fb + g];
+            double qa = ld_once(&A.q_a[(size_t)m * plane + fb + g]);
             qa = (c)*dmin(dmax(qa, in_mn_s), in_mx_s) + (1.0 - c) * dmin(dmax(qa, in_mn_n), in_mx_n);
             qa = (c)*dmax(dmin(qa, qo_max_n), qo_min_n) + (1.0 - c) * dmax(dmin(qa, qo_max_c), qo_min_c);
             stream_store(&A.q_a[(size_t)m * plane + fb + g], qa);

@@ -117,6 +117,18 @@ int hip_fail(hipError_t e, const char *what);
         if (e_ != hipSuccess) return ::nicam::hip_fail(e_, #call); \
     } while (0)
 
+// Hardware dispatch sends work-group b to XCD b % 8, and each XCD has its own
+// L2.  For a launch of n work-groups whose neighbours in logical order share
+// data (halo rows, the level above and below), this maps the hardware index
+// to a logical one so that each XCD runs one contiguous logical range; the
+// last n % 8 work-groups keep their index.
+__device__ __forceinline__ long xcd_contiguous(long b, long n)
+{
+    constexpr long nxcd = 8;
+    const long per = n / nxcd;
+    return (per > 0 && b < per * nxcd) ? (b % nxcd) * per + b / nxcd : b;
+}
+
 // Stores of outputs that no work-item of the same launch reads back carry the
 // nontemporal hint, so output streams of hundreds of MB do not evict the halo
 // rows and metrics that neighbouring work-groups re-read from L2.  Measured at

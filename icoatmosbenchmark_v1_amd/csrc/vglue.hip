@@ -44,12 +44,16 @@ struct VCommon {
 
 enum { V_FLUX = 0, V_COURANT = 1, V_FACE = 2, V_UPDATE = 3 };
 
+// Work-groups [0, nR) cover the regular regions, the rest the poles.  The
+// levels above and below a cell are gall cells away, a few work-groups on:
+// the regular part keeps consecutive work-groups on one XCD.
 template <int OP>
 __global__ void __launch_bounds__(BLK) vglue_kernel(VPart R, VPart P, VCommon C)
 {
-    long t = (long)blockIdx.x * BLK + threadIdx.x;
-    const bool pole = t >= R.n;
-    if (pole) t -= R.n;
+    const long nR = (R.n + BLK - 1) / BLK;
+    const bool pole = (long)blockIdx.x >= nR;
+    const long t = pole ? ((long)blockIdx.x - nR) * BLK + threadIdx.x
+                        : xcd_contiguous(blockIdx.x, nR) * BLK + threadIdx.x;
     const VPart &A = pole ? P : R;
     if (t >= A.n) return;
     const int gall = A.gall, kall = C.kall;
@@ -123,7 +127,7 @@ int launch_v(nicam_ctx c, const VPart &R, const VPart &P, double dt)
     C.afact = c->afact;
     C.bfact = c->bfact;
     C.dt = dt;
-    const long n = R.n + P.n, nb = (n + BLK - 1) / BLK;
+    const long nb = (R.n + BLK - 1) / BLK + (P.n + BLK - 1) / BLK;
     if (nb == 0) return NICAM_OK;
     if (nb > 0x7fffffffL) return fail(NICAM_ERR_ARG, "grid too large for one launch");
     hipLaunchKernelGGL(vglue_kernel<OP>, dim3((unsigned)nb), dim3(BLK), 0, c->stream, R, P, C);

@@ -43,6 +43,12 @@ def region_slice(d: dict, regions, pole: bool, reg_axis_of=None):
         if k == "points":
             out[k] = v[list(regions)]
             continue
+        if k == "coef_grad":  # [lall][3][7][gall]
+            out[k] = np.asarray(v)[list(regions)]
+            continue
+        if k == "GRD_x":  # [3][lall][1][gall]
+            out[k] = np.asarray(v)[:, list(regions)]
+            continue
         v = np.asarray(v)
         if v.ndim < 3:  # per-level vectors (gz, gzh, ...)
             out[k] = v
