@@ -172,7 +172,8 @@ int nicam_ctx_destroy(nicam_ctx c)
                       &c->RGAMH, &c->RGAMH_pl, &c->RGSGAM2, &c->RGSGAM2_pl, &c->RGSGAM2H,
                       &c->RGSGAM2H_pl, &c->GSGAM2H, &c->GSGAM2H_pl, &c->rdgzh, &c->afact,
                       &c->bfact, &c->Mc, &c->Mc_pl, &c->Ml, &c->Ml_pl, &c->Mu, &c->Mu_pl, &c->vi_gamma,
-                      &c->RGSQRTH, &c->RGSQRTH_pl, &c->C2WfactGz, &c->C2WfactGz_pl, &c->rdgz};
+                      &c->RGSQRTH, &c->RGSQRTH_pl, &c->C2WfactGz, &c->C2WfactGz_pl, &c->rdgz,
+                      &c->cfact, &c->dfact};
     for (double **p : all) dev_free(*p);
     for (void *p : c->stage)
         if (p) (void)hipFree(p);
@@ -318,10 +319,22 @@ int nicam_grd_setup(nicam_ctx c, const double *gz, const double *gzh, double *rd
     for (int k = kmin - 1; k <= kmax; ++k) dgz[k] = gzh[k + 1] - gzh[k];
     dgz[kmax + 1] = dgz[kmax];
     for (int k = 0; k < kall; ++k) rdgz[k] = 1.0 / dgz[k];
+    // mod_misc.f90:513-521 (cell-wall -> cell-centre factors, for the VI matrix)
+    std::vector<double> cfact(kall), dfact(kall);
+    for (int k = kmin; k <= kmax; ++k) cfact[k] = (gz[k] - gzh[k]) / (gzh[k + 1] - gzh[k]);
+    cfact[kmin - 1] = 1.0;
+    cfact[kmax + 1] = 0.0;
+    for (int k = 0; k < kall; ++k) dfact[k] = 1.0 - cfact[k];
     for (int k = 0; k < kall; ++k) {
-        if (!std::isfinite(rdgzh[k]) || !std::isfinite(afact[k]) || !std::isfinite(rdgz[k]))
+        if (!std::isfinite(rdgzh[k]) || !std::isfinite(afact[k]) || !std::isfinite(rdgz[k]) ||
+            !std::isfinite(cfact[k]))
             return fail(NICAM_ERR_ARG, "vertical grid has coincident levels");
     }
+    NICAM_TRY(dev_alloc(c, &c->cfact, kall));
+    NICAM_TRY(dev_alloc(c, &c->dfact, kall));
+    NICAM_TRY(copy_in(c, c->cfact, cfact.data(), kall));
+    NICAM_TRY(copy_in(c, c->dfact, dfact.data(), kall));
+    c->have_cdfact = true;
     NICAM_TRY(nicam_set_rdgz(c, rdgz.data()));
     if (rdgzh_out) std::memcpy(rdgzh_out, rdgzh.data(), sizeof(double) * kall);
     if (afact_out) std::memcpy(afact_out, afact.data(), sizeof(double) * kall);
@@ -902,6 +915,69 @@ int nicam_tracer_update_vertical(nicam_ctx c, double *rhogq, double *rhogq_pl, c
     auto D = [&](int i) { return static_cast<double *>(v[i]); };
     NICAM_TRY(launch_vertical_update(c, D(0), D(1), D(2), D(3), D(4), D(5)));
     return stage_out(c, a, v);
+}
+
+int nicam_vi_rhow_update_matrix(nicam_ctx c, const double *eth, const double *eth_pl, const double *g_tilde,
+                                const double *g_tilde_pl, double dt)
+{
+    NICAM_TRY(check_ctx(c));
+    if (!c->have_vmetrics) return fail(NICAM_ERR_STATE, "vertical metrics not set (nicam_set_vmetrics)");
+    if (!c->have_cdfact || !c->have_rdgz || !c->have_vgrid)
+        return fail(NICAM_ERR_STATE, "vertical grid factors not set (nicam_grd_setup)");
+    if (!(dt > 0.0)) return fail(NICAM_ERR_ARG, "dt must be positive");
+    NICAM_TRY(need(eth, "eth"));
+    NICAM_TRY(need(g_tilde, "g_tilde"));
+    if (c->d.have_pl) {
+        NICAM_TRY(need(eth_pl, "pole array"));
+        NICAM_TRY(need(g_tilde_pl, "pole array"));
+    }
+    const size_t nr = c->d.nreg(), np = c->d.npl();
+    NICAM_TRY(dev_alloc(c, &c->Mc, nr));
+    NICAM_TRY(dev_alloc(c, &c->Ml, nr));
+    NICAM_TRY(dev_alloc(c, &c->Mu, nr));
+    if (c->d.have_pl) {
+        NICAM_TRY(dev_alloc(c, &c->Mc_pl, np));
+        NICAM_TRY(dev_alloc(c, &c->Ml_pl, np));
+        NICAM_TRY(dev_alloc(c, &c->Mu_pl, np));
+    }
+    if (c->ptr_mode == NICAM_PTR_DEVICE) {
+        NICAM_TRY(launch_vi_matrix(c, eth, PL(eth_pl), g_tilde, PL(g_tilde_pl), dt));
+    } else {
+        const size_t R = nr * 8, P = np * 8;
+        std::vector<HostArg> a = {{eth, R, false}, {PL(eth_pl), P, false}, {g_tilde, R, false},
+                                  {PL(g_tilde_pl), P, false}};
+        std::vector<void *> v;
+        NICAM_TRY(stage_in(c, a, v));
+        auto D = [&](int i) { return static_cast<const double *>(v[i]); };
+        NICAM_TRY(launch_vi_matrix(c, D(0), D(1), D(2), D(3), dt));
+        NICAM_TRY(stage_out(c, a, v));
+    }
+    NICAM_HIP(hipStreamSynchronize(c->stream));
+    c->have_coef = true;
+    return NICAM_OK;
+}
+
+int nicam_vi_rhow_getcoef(nicam_ctx c, double *Mc, double *Mc_pl, double *Ml, double *Ml_pl, double *Mu,
+                          double *Mu_pl)
+{
+    NICAM_TRY(check_ctx(c));
+    if (!c->have_coef) return fail(NICAM_ERR_STATE, "no coefficients (nicam_vi_rhow_solver_putcoef or _update_matrix)");
+    const size_t nr = c->d.nreg() * sizeof(double), np = c->d.npl() * sizeof(double);
+    const hipMemcpyKind kind = c->ptr_mode == NICAM_PTR_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
+    double *dst[] = {Mc, Ml, Mu};
+    double *src[] = {c->Mc, c->Ml, c->Mu};
+    double *dst_pl[] = {Mc_pl, Ml_pl, Mu_pl};
+    double *src_pl[] = {c->Mc_pl, c->Ml_pl, c->Mu_pl};
+    for (int i = 0; i < 3; ++i) {
+        NICAM_TRY(need(dst[i], "coefficient array"));
+        NICAM_HIP(hipMemcpyAsync(dst[i], src[i], nr, kind, c->stream));
+        if (c->d.have_pl) {
+            NICAM_TRY(need(dst_pl[i], "pole array"));
+            NICAM_HIP(hipMemcpyAsync(dst_pl[i], src_pl[i], np, kind, c->stream));
+        }
+    }
+    NICAM_HIP(hipStreamSynchronize(c->stream));
+    return NICAM_OK;
 }
 
 #undef PL

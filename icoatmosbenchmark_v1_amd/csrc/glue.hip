@@ -119,84 +119,151 @@ __global__ void __launch_bounds__(BLK) courant_kernel(CourantArgs A)
     A.cmask_pl[p] = m;
 }
 
+constexpr int RCH = 256;   // output cells per remap work-group
+constexpr int RMAXI = 160; // largest gall_1d the remap tiles hold
+constexpr int RNO = 2;     // arc origins per work-item: RCH*RNO >= RCH + RMAXI + 1
+
 struct RemapArgs {
     double *q_a, *q_a_pl;
     const double *q, *q_pl, *grad, *grad_pl, *cmask, *cmask_pl, *xc, *xc_pl, *x, *x_pl;
     int iall, gall, kall, lall, gall_pl, have_pl;
-    long nreg, nmain;  // cell-levels of the regular regions, their work-groups
+    int nchunk, nkr, kper, nmain;  // chunks per slab, level ranges, levels per range, regular work-groups
 };
 
-// Face value on the arc a (0 AI, 1 AIJ, 2 AJ) from origin o to neighbour n,
-// as seen by the cell that owns the face: the origin (face a+1, own = o) or
-// the neighbour (face a+4, own = n).  The owner's mask picks the other side's
-// reconstruction where the flux enters it (cmask = 1) and its own otherwise.
-__device__ __forceinline__ double face_value(const RemapArgs &A, size_t fb, size_t mb, size_t plane, int o, int n,
-                                             int a, bool at_origin)
+// Pole regions: one work-item per (k, pole region); each ring face takes the
+// upwind one of the centre's and the ring point's reconstructions.
+__device__ void remap_pole(const RemapArgs &A, long p)
 {
-    const size_t xplane = (size_t)A.lall * A.gall;  // between GRD_x components
-    double xc[3];
-#pragma unroll
-    for (int d = 0; d < 3; ++d) xc[d] = A.xc[(size_t)(d * 3 + a) * plane + fb + o];
-    const double qs = A.q[fb + o] + (A.grad[0 * plane + fb + o] * (xc[0] - A.x[0 * xplane + mb + o]) +
-                                     A.grad[1 * plane + fb + o] * (xc[1] - A.x[1 * xplane + mb + o]) +
-                                     A.grad[2 * plane + fb + o] * (xc[2] - A.x[2 * xplane + mb + o]));
-    const double qn = A.q[fb + n] + (A.grad[0 * plane + fb + n] * (xc[0] - A.x[0 * xplane + mb + n]) +
-                                     A.grad[1 * plane + fb + n] * (xc[1] - A.x[1 * xplane + mb + n]) +
-                                     A.grad[2 * plane + fb + n] * (xc[2] - A.x[2 * xplane + mb + n]));
-    if (at_origin) {
-        const double c = A.cmask[(size_t)a * plane + fb + o];
-        return c * qn + (1.0 - c) * qs;
+    const int kall = A.kall;
+    if (!A.have_pl || p >= (long)kall * kLallPl) return;
+    const int npl = A.gall_pl, k = (int)(p % kall), l = (int)(p / kall);
+    const size_t fb = ((size_t)l * kall + k) * npl, plane = (size_t)kLallPl * kall * npl;
+    const size_t xb = (size_t)l * npl, xplane = (size_t)kLallPl * npl;
+    for (int v = 1; v < npl; ++v) {
+        const double x0 = A.xc_pl[0 * plane + fb + v], x1 = A.xc_pl[1 * plane + fb + v],
+                     x2 = A.xc_pl[2 * plane + fb + v];
+        const double qs = A.q_pl[fb] + (A.grad_pl[0 * plane + fb] * (x0 - A.x_pl[0 * xplane + xb]) +
+                                        A.grad_pl[1 * plane + fb] * (x1 - A.x_pl[1 * xplane + xb]) +
+                                        A.grad_pl[2 * plane + fb] * (x2 - A.x_pl[2 * xplane + xb]));
+        const double qn = A.q_pl[fb + v] + (A.grad_pl[0 * plane + fb + v] * (x0 - A.x_pl[0 * xplane + xb + v]) +
+                                            A.grad_pl[1 * plane + fb + v] * (x1 - A.x_pl[1 * xplane + xb + v]) +
+                                            A.grad_pl[2 * plane + fb + v] * (x2 - A.x_pl[2 * xplane + xb + v]));
+        const double c = A.cmask_pl[fb + v];
+        A.q_a_pl[fb + v] = c * qn + (1.0 - c) * qs;
     }
-    const double c = A.cmask[(size_t)(a + 3) * plane + fb + n];
-    return c * qs + (1.0 - c) * qn;
 }
 
-// One work-item per cell and level writes the faces of its cell that an arc
-// reaches: faces 1-3 from its own arcs, faces 4-6 from the arcs of im1j,
-// im1jm1 and ijm1, each with its own face's mask.
-__global__ void __launch_bounds__(BLK) remap_kernel(RemapArgs A)
+// A work-group owns RCH consecutive cells of one region and marches up a
+// range of levels.  Positions of the cells [g0-iall-1, g0+RCH+iall+1) sit in
+// LDS for the whole range; at each level q and its gradient over the same
+// window are staged, then (phase 1) every arc with origin in
+// [g0-iall-1, g0+RCH) forms its two reconstructions once, qs from the origin
+// and qn from the far end, into LDS, and (phase 2) every cell picks each of
+// its six faces by that face's own mask: faces 1-3 from its own arcs, faces
+// 4-6 from the arcs of im1j (AI), im1jm1 (AIJ) and ijm1 (AJ).  A face no arc
+// reaches keeps its value.
+__global__ void __launch_bounds__(RCH) remap_kernel(RemapArgs A)
 {
-    const int iall = A.iall, gall = A.gall, kall = A.kall;
-    if ((long)blockIdx.x >= A.nmain) {
-        const long p = ((long)blockIdx.x - A.nmain) * BLK + threadIdx.x;  // one work-item per (k, pole region)
-        if (!A.have_pl || p >= (long)kall * kLallPl) return;
-        const int npl = A.gall_pl, k = (int)(p % kall), l = (int)(p / kall);
-        const size_t fb = ((size_t)l * kall + k) * npl, plane = (size_t)kLallPl * kall * npl;
-        const size_t xb = (size_t)l * npl, xplane = (size_t)kLallPl * npl;
-        for (int v = 1; v < npl; ++v) {
-            const double x0 = A.xc_pl[0 * plane + fb + v], x1 = A.xc_pl[1 * plane + fb + v],
-                         x2 = A.xc_pl[2 * plane + fb + v];
-            const double qs = A.q_pl[fb] + (A.grad_pl[0 * plane + fb] * (x0 - A.x_pl[0 * xplane + xb]) +
-                                            A.grad_pl[1 * plane + fb] * (x1 - A.x_pl[1 * xplane + xb]) +
-                                            A.grad_pl[2 * plane + fb] * (x2 - A.x_pl[2 * xplane + xb]));
-            const double qn = A.q_pl[fb + v] + (A.grad_pl[0 * plane + fb + v] * (x0 - A.x_pl[0 * xplane + xb + v]) +
-                                                A.grad_pl[1 * plane + fb + v] * (x1 - A.x_pl[1 * xplane + xb + v]) +
-                                                A.grad_pl[2 * plane + fb + v] * (x2 - A.x_pl[2 * xplane + xb + v]));
-            const double c = A.cmask_pl[fb + v];
-            A.q_a_pl[fb + v] = c * qn + (1.0 - c) * qs;
-        }
+    extern __shared__ double lds[];
+    int b = blockIdx.x;
+    if (b >= A.nmain) {
+        remap_pole(A, (long)(b - A.nmain) * RCH + threadIdx.x);
         return;
     }
-    // the rows above and below a cell lie in the neighbouring work-groups: keep those on one XCD
-    const long t = xcd_contiguous(blockIdx.x, A.nmain) * BLK + threadIdx.x;
-    if (t >= A.nreg) return;
-    const int g = (int)(t % gall);
-    const long kl = t / gall;
-    const int k = (int)(kl % kall), l = (int)(kl / kall);
-    const int i = g % iall, j = g / iall;
-    const size_t fb = ((size_t)l * kall + k) * gall, mb = (size_t)l * gall, plane = (size_t)A.lall * kall * gall;
-    const int last = iall - 2;  // arc origins run over [0, iall-2] in each direction
-    if (i <= last && j <= last) {
-        stream_store(&A.q_a[0 * plane + fb + g], face_value(A, fb, mb, plane, g, g + 1, 0, true));
-        stream_store(&A.q_a[1 * plane + fb + g], face_value(A, fb, mb, plane, g, g + iall + 1, 1, true));
-        stream_store(&A.q_a[2 * plane + fb + g], face_value(A, fb, mb, plane, g, g + iall, 2, true));
+    b = (int)xcd_contiguous(b, A.nmain);  // a region's chunks share halo rows: one XCD
+    const int iall = A.iall, gall = A.gall, kall = A.kall, lall = A.lall;
+    const int chunk = b % A.nchunk, kr = (b / A.nchunk) % A.nkr, l = b / (A.nchunk * A.nkr);
+    const int k0 = kr * A.kper, k1 = min(kall, k0 + A.kper);
+    if (l >= lall || k0 >= k1) return;
+    const int g0 = chunk * RCH, lo = g0 - iall - 1;
+    const int n_in = RCH + 2 * iall + 2;  // staged cells [lo, lo + n_in)
+    const int n_o = RCH + iall + 1;       // arc origins [lo, lo + n_o)
+    double *s_q = lds;                    // [n_in]
+    double *s_g = lds + n_in;             // [3][n_in]
+    double *s_x = lds + 4 * n_in;         // [3][n_in]
+    double *s_a = lds + 7 * n_in;         // [6][n_o]: qs of AI, AIJ, AJ, then qn of AI, AIJ, AJ
+    const int tid = threadIdx.x;
+    const size_t mb = (size_t)l * gall, xplane = (size_t)lall * gall;
+    const size_t plane = (size_t)lall * kall * gall;
+
+    for (int s = tid; s < n_in; s += RCH) {  // GRD_x(g,1,l,d), K-invariant
+        const int p = lo + s;
+        const bool ok = p >= 0 && p < gall;
+#pragma unroll
+        for (int d = 0; d < 3; ++d) s_x[d * n_in + s] = ok ? A.x[d * xplane + mb + p] : 0.0;
     }
-    if (i >= 1 && j <= last)
-        stream_store(&A.q_a[3 * plane + fb + g], face_value(A, fb, mb, plane, g - 1, g, 0, false));
-    if (i >= 1 && j >= 1)
-        stream_store(&A.q_a[4 * plane + fb + g], face_value(A, fb, mb, plane, g - iall - 1, g, 1, false));
-    if (i <= last && j >= 1)
-        stream_store(&A.q_a[5 * plane + fb + g], face_value(A, fb, mb, plane, g - iall, g, 2, false));
+    int org[RNO];  // my arc origins (cell index), -1 if none
+#pragma unroll
+    for (int r = 0; r < RNO; ++r) {
+        const int s = tid + r * RCH, p = lo + s;
+        org[r] = (s < n_o && p >= 0 && p < gall && p % iall <= iall - 2 && p / iall <= iall - 2) ? p : -1;
+    }
+    const int g = g0 + tid;
+    const bool active = g < gall;
+    const int i = active ? g % iall : 0, j = active ? g / iall : 0;
+    const int last = iall - 2;
+    const bool own = active && i <= last && j <= last;
+    const bool f4 = active && i >= 1 && j <= last, f5 = active && i >= 1 && j >= 1,
+               f6 = active && i <= last && j >= 1;
+
+    for (int k = k0; k < k1; ++k) {
+        const size_t fb = ((size_t)l * kall + k) * gall;
+        // loads that do not wait for the tile: my origins' arc centroids and my faces' masks
+        double xcv[RNO][9];
+#pragma unroll
+        for (int r = 0; r < RNO; ++r)
+#pragma unroll
+            for (int m = 0; m < 9; ++m) xcv[r][m] = org[r] >= 0 ? A.xc[(size_t)m * plane + fb + org[r]] : 0.0;
+        double cm[6];
+#pragma unroll
+        for (int f = 0; f < 6; ++f) cm[f] = active ? A.cmask[(size_t)f * plane + fb + g] : 0.0;
+        for (int s = tid; s < n_in; s += RCH) {
+            const int p = lo + s;
+            const bool ok = p >= 0 && p < gall;
+            s_q[s] = ok ? A.q[fb + p] : 0.0;
+#pragma unroll
+            for (int d = 0; d < 3; ++d) s_g[d * n_in + s] = ok ? A.grad[d * plane + fb + p] : 0.0;
+        }
+        __syncthreads();
+        // phase 1: the two reconstructions of each arc (GRD_xc(g,k,l,a,d) at (d*3+a)*plane)
+#pragma unroll
+        for (int r = 0; r < RNO; ++r) {
+            if (org[r] < 0) continue;
+            const int s = tid + r * RCH;
+            const int far[3] = {s + 1, s + iall + 1, s + iall};
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                const double x0 = xcv[r][0 * 3 + a], x1 = xcv[r][1 * 3 + a], x2 = xcv[r][2 * 3 + a];
+                const int n = far[a];
+                const double qs = s_q[s] + (s_g[0 * n_in + s] * (x0 - s_x[0 * n_in + s]) +
+                                            s_g[1 * n_in + s] * (x1 - s_x[1 * n_in + s]) +
+                                            s_g[2 * n_in + s] * (x2 - s_x[2 * n_in + s]));
+                const double qn = s_q[n] + (s_g[0 * n_in + n] * (x0 - s_x[0 * n_in + n]) +
+                                            s_g[1 * n_in + n] * (x1 - s_x[1 * n_in + n]) +
+                                            s_g[2 * n_in + n] * (x2 - s_x[2 * n_in + n]));
+                s_a[a * n_o + s] = qs;
+                s_a[(3 + a) * n_o + s] = qn;
+            }
+        }
+        __syncthreads();
+        // phase 2: my faces, each by its own mask
+        if (active) {
+            const int s = g - lo;
+            if (own) {
+#pragma unroll
+                for (int a = 0; a < 3; ++a) {
+                    const double c = cm[a];
+                    stream_store(&A.q_a[(size_t)a * plane + fb + g],
+                                 c * s_a[(3 + a) * n_o + s] + (1.0 - c) * s_a[a * n_o + s]);
+                }
+            }
+            const int o4 = s - 1, o5 = s - iall - 1, o6 = s - iall;  // origins of the arcs of faces 4-6
+            if (f4) stream_store(&A.q_a[3 * plane + fb + g], cm[3] * s_a[0 * n_o + o4] + (1.0 - cm[3]) * s_a[3 * n_o + o4]);
+            if (f5) stream_store(&A.q_a[4 * plane + fb + g], cm[4] * s_a[1 * n_o + o5] + (1.0 - cm[4]) * s_a[4 * n_o + o5]);
+            if (f6) stream_store(&A.q_a[5 * plane + fb + g], cm[5] * s_a[2 * n_o + o6] + (1.0 - cm[5]) * s_a[5 * n_o + o6]);
+        }
+        __syncthreads();  // the tiles are restaged for the next level
+    }
 }
 
 struct UpdateArgs {
@@ -293,6 +360,7 @@ int launch_remap(nicam_ctx c, double *q_a, double *q_a_pl, const double *q, cons
                  const double *xc_pl, const double *x, const double *x_pl)
 {
     const Dims &D = c->d;
+    if (D.gall_1d > RMAXI) return fail(NICAM_ERR_ARG, "gall_1d larger than the remap tiles (160)");
     RemapArgs A;
     A.q_a = q_a;
     A.q_a_pl = q_a_pl;
@@ -312,11 +380,19 @@ int launch_remap(nicam_ctx c, double *q_a, double *q_a_pl, const double *q, cons
     A.lall = D.lall;
     A.gall_pl = D.gall_pl;
     A.have_pl = D.have_pl && q_a_pl;
-    A.nreg = (long)D.gall * D.kall * D.lall;
-    A.nmain = (A.nreg + BLK - 1) / BLK;
-    const long nb = A.nmain + (A.have_pl ? ((long)D.kall * kLallPl + BLK - 1) / BLK : 0);
-    if (nb > 0x7fffffffL) return fail(NICAM_ERR_ARG, "grid too large for one launch");
-    hipLaunchKernelGGL(remap_kernel, dim3((unsigned)nb), dim3(BLK), 0, c->stream, A);
+    A.nchunk = (D.gall + RCH - 1) / RCH;
+    const long base = (long)A.nchunk * D.lall;
+    int nkr = (int)((4096 + base - 1) / base);  // enough work-groups to fill the chip
+    nkr = std::max(1, std::min(nkr, D.kall));
+    A.kper = (D.kall + nkr - 1) / nkr;
+    A.nkr = (D.kall + A.kper - 1) / A.kper;
+    const long nmain = base * A.nkr;
+    const long npole = A.have_pl ? ((long)D.kall * kLallPl + RCH - 1) / RCH : 0;
+    if (nmain + npole > 0x7fffffffL) return fail(NICAM_ERR_ARG, "grid too large for one launch");
+    A.nmain = (int)nmain;
+    const int n_in = RCH + 2 * D.gall_1d + 2, n_o = RCH + D.gall_1d + 1;
+    const size_t lds = sizeof(double) * (7 * (size_t)n_in + 6 * (size_t)n_o);
+    hipLaunchKernelGGL(remap_kernel, dim3((unsigned)(nmain + npole)), dim3(RCH), lds, c->stream, A);
     NICAM_HIP(hipGetLastError());
     return NICAM_OK;
 }

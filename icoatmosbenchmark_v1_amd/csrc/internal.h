@@ -82,6 +82,8 @@ struct nicam_ctx_s {
     // vertical grid factors (kall each)
     double *rdgzh = nullptr, *afact = nullptr, *bfact = nullptr;
     bool have_vgrid = false;
+    double *cfact = nullptr, *dfact = nullptr;  // cell wall -> cell centre (nicam_grd_setup only)
+    bool have_cdfact = false;
 
     // tridiagonal coefficients (module copies of mod_vi.f90:47-52)
     double *Mc = nullptr, *Mc_pl = nullptr;
@@ -173,6 +175,8 @@ int launch_vertical_mass_flux(nicam_ctx c, double *flx_v, double *flx_v_pl, cons
 int launch_vertical_courant(nicam_ctx c, double *ck, double *ck_pl, const double *flx_v, const double *flx_v_pl,
                             const double *rhog, const double *rhog_pl);
 int launch_vertical_face(nicam_ctx c, double *q_h, double *q_h_pl, const double *q, const double *q_pl);
+int launch_vi_matrix(nicam_ctx c, const double *eth, const double *eth_pl, const double *g_tilde,
+                     const double *g_tilde_pl, double dt);
 int launch_vertical_update(nicam_ctx c, double *rhogq, double *rhogq_pl, const double *flx_v, const double *flx_v_pl,
                            const double *q_h, const double *q_h_pl);
 int launch_hflux(nicam_ctx c, double *flx_h, double *flx_h_pl, double *xc, double *xc_pl,

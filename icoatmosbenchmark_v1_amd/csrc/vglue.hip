@@ -135,7 +135,90 @@ int launch_v(nicam_ctx c, const VPart &R, const VPart &P, double dt)
     return NICAM_OK;
 }
 
+// Coefficients of the tridiagonal system for rho*w (the vertically implicit
+// Helmholtz equation, docs/NICAM/src/20_intro_nicam.tex:480-600), collected
+// from its discretised form at the half level k between full levels k-1 and
+// k (0-based storage, the row Ml(k) W(k-1) + Mc(k) W(k) + Mu(k) W(k+1) of
+// vi_rhow_solver, dyn_vi_rhow_solver/src/mod_vi.f90:218-232):
+//   A^o = RGSGAM2 (full), A^i = gam2 h (half), B = g_tilde (half),
+//   C^o = (Cv/R) g / gam2 (half k), C^i = gam2 (half), D = Cv G^1/2 / (R dt^2) (half k)
+// with gam2 = 1/RGAMH^2 and G^1/2 = GSGAM2H / gam2;
+//   Mc = alpha D + (A^o(k) A^i(k) rdgz(k) + A^o(k-1) A^i(k) rdgz(k-1)) rdgzh(k)
+//        - (dfact(k) - cfact(k-1)) (B(k) + C^o C^i(k)) rdgzh(k)
+//   Mu = -A^o(k) A^i(k+1) rdgz(k) rdgzh(k) - cfact(k) (B(k+1) + C^o C^i(k+1)) rdgzh(k)
+//   Ml = -A^o(k-1) A^i(k-1) rdgz(k-1) rdgzh(k) + dfact(k-1) (B(k-1) + C^o C^i(k-1)) rdgzh(k)
+// on kmin+1..kmax, 0 on the other levels.
+struct MatPart {
+    double *Mc, *Ml, *Mu;
+    const double *eth, *gt, *rgsgam2, *rgamh, *gsgam2h;
+    int gall;
+    long n;
+};
+
+struct MatCommon {
+    int kall, kmin, kmax;
+    const double *rdgz, *rdgzh, *cfact, *dfact;
+    double cvovr, cvovrt2, grav, alpha;
+};
+
+__global__ void __launch_bounds__(BLK) vi_matrix_kernel(MatPart R, MatPart P, MatCommon C)
+{
+    const long nR = (R.n + BLK - 1) / BLK;
+    const bool pole = (long)blockIdx.x >= nR;
+    const long t = pole ? ((long)blockIdx.x - nR) * BLK + threadIdx.x
+                        : xcd_contiguous(blockIdx.x, nR) * BLK + threadIdx.x;
+    const MatPart &A = pole ? P : R;
+    if (t >= A.n) return;
+    const int k = (int)((t / A.gall) % C.kall);
+    const long dn = A.gall;
+    double mc = 0.0, ml = 0.0, mu = 0.0;
+    if (k >= C.kmin + 1 && k <= C.kmax) {
+        const double rh = A.rgamh[t];
+        const double g2m = 1.0 / (A.rgamh[t - dn] * A.rgamh[t - dn]), g2 = 1.0 / (rh * rh),
+                     g2p = 1.0 / (A.rgamh[t + dn] * A.rgamh[t + dn]);
+        const double aim = g2m * A.eth[t - dn], ai = g2 * A.eth[t], aip = g2p * A.eth[t + dn];
+        const double co = C.cvovr * C.grav * rh * rh;
+        const double d = C.cvovrt2 * A.gsgam2h[t] * rh * rh;
+        const double ao = A.rgsgam2[t], aom = A.rgsgam2[t - dn];
+        const double rzh = C.rdgzh[k];
+        mc = C.alpha * d + (ao * ai * C.rdgz[k] + aom * ai * C.rdgz[k - 1]) * rzh -
+             (C.dfact[k] - C.cfact[k - 1]) * (A.gt[t] + co * g2) * rzh;
+        mu = -ao * aip * C.rdgz[k] * rzh - C.cfact[k] * (A.gt[t + dn] + co * g2p) * rzh;
+        ml = -aom * aim * C.rdgz[k - 1] * rzh + C.dfact[k - 1] * (A.gt[t - dn] + co * g2m) * rzh;
+    }
+    A.Mc[t] = mc;
+    A.Ml[t] = ml;
+    A.Mu[t] = mu;
+}
+
 }  // namespace
+
+int launch_vi_matrix(nicam_ctx c, const double *eth, const double *eth_pl, const double *g_tilde,
+                     const double *g_tilde_pl, double dt)
+{
+    const Dims &D = c->d;
+    MatPart R{c->Mc, c->Ml, c->Mu, eth, g_tilde, c->RGSGAM2, c->RGAMH, c->GSGAM2H, D.gall,
+              (long)D.gall * D.kall * D.lall};
+    MatPart P{c->Mc_pl, c->Ml_pl, c->Mu_pl, eth_pl, g_tilde_pl, c->RGSGAM2_pl, c->RGAMH_pl, c->GSGAM2H_pl,
+              D.gall_pl, D.have_pl ? (long)D.gall_pl * D.kall * kLallPl : 0};
+    MatCommon C;
+    C.kall = D.kall;
+    C.kmin = 1;
+    C.kmax = D.kall - 2;
+    C.rdgz = c->rdgz;
+    C.rdgzh = c->rdgzh;
+    C.cfact = c->cfact;
+    C.dfact = c->dfact;
+    C.cvovr = c->cvdry / c->rdry;
+    C.cvovrt2 = c->cvdry / c->rdry / (dt * dt);
+    C.grav = c->grav;
+    C.alpha = (double)c->alpha;
+    const long nb = (R.n + BLK - 1) / BLK + (P.n + BLK - 1) / BLK;
+    if (nb > 0x7fffffffL) return fail(NICAM_ERR_ARG, "grid too large for one launch");
+    hipLaunchKernelGGL(vi_matrix_kernel, dim3((unsigned)nb), dim3(BLK), 0, c->stream, R, P, C);
+    NICAM_HIP(hipGetLastError());
+    return NICAM_OK;
+}
 
 int launch_vertical_mass_flux(nicam_ctx c, double *flx_v, double *flx_v_pl, const double *vx, const double *vx_pl,
                               const double *vy, const double *vy_pl, const double *vz, const double *vz_pl,

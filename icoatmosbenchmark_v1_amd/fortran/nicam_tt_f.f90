@@ -289,6 +289,20 @@ module nicam_tt_f
        integer(c_int) :: nicam_tracer_update_vertical
      end function nicam_tracer_update_vertical
 
+     function nicam_vi_rhow_update_matrix(ctx, eth, eth_pl, g_tilde, g_tilde_pl, dt) &
+          bind(C, name='nicam_vi_rhow_update_matrix')
+       import :: c_ptr, c_int, c_double
+       type(c_ptr), value :: ctx, eth, eth_pl, g_tilde, g_tilde_pl
+       real(c_double), value :: dt
+       integer(c_int) :: nicam_vi_rhow_update_matrix
+     end function nicam_vi_rhow_update_matrix
+
+     function nicam_vi_rhow_getcoef(ctx, Mc, Mc_pl, Ml, Ml_pl, Mu, Mu_pl) bind(C, name='nicam_vi_rhow_getcoef')
+       import :: c_ptr, c_int
+       type(c_ptr), value :: ctx, Mc, Mc_pl, Ml, Ml_pl, Mu, Mu_pl
+       integer(c_int) :: nicam_vi_rhow_getcoef
+     end function nicam_vi_rhow_getcoef
+
      function nicam_grid_icosahedral(glevel, rlevel, radius, GRD_x, GRD_x_pl) bind(C, name='nicam_grid_icosahedral')
        import :: c_int, c_double
        integer(c_int), value :: glevel, rlevel

@@ -98,6 +98,8 @@ SIGNATURES = {
     "nicam_vertical_courant": (c_int, [_P] * 7),
     "nicam_vertical_face": (c_int, [_P] * 5),
     "nicam_tracer_update_vertical": (c_int, [_P] * 7),
+    "nicam_vi_rhow_update_matrix": (c_int, [_P] * 5 + [c_double]),
+    "nicam_vi_rhow_getcoef": (c_int, [_P] * 7),
     "nicam_oprt_diffusion": (c_int, [_P] + [_P] * 10),
     "nicam_set_rdgz": (c_int, [_P, _P]),
     "nicam_set_vmetrics3d": (c_int, [_P] + [_P] * 8),
@@ -467,6 +469,20 @@ class Context:
         a = self._args([(rhogq, "rhogq", R), (rhogq_pl, "rhogq_pl", P), (flx_v, "flx_v", R),
                         (flx_v_pl, "flx_v_pl", P), (q_h, "q_h", R), (q_h_pl, "q_h_pl", P)])
         _check(self.lib.nicam_tracer_update_vertical(self.h, *a), "nicam_tracer_update_vertical")
+
+    def vi_rhow_update_matrix(self, eth, eth_pl, g_tilde, g_tilde_pl, dt: float) -> None:
+        """nicam_vi_rhow_update_matrix: Mc/Ml/Mu of the context from h and g~ at the half levels."""
+        R, P = self.nreg, self.npl
+        a = self._args([(eth, "eth", R), (eth_pl, "eth_pl", P), (g_tilde, "g_tilde", R),
+                        (g_tilde_pl, "g_tilde_pl", P)])
+        _check(self.lib.nicam_vi_rhow_update_matrix(self.h, *a, float(dt)), "nicam_vi_rhow_update_matrix")
+
+    def vi_rhow_getcoef(self, Mc, Mc_pl, Ml, Ml_pl, Mu, Mu_pl) -> None:
+        """nicam_vi_rhow_getcoef: copies the context's Mc/Ml/Mu out."""
+        R, P = self.nreg, self.npl
+        a = self._args([(Mc, "Mc", R), (Mc_pl, "Mc_pl", P), (Ml, "Ml", R), (Ml_pl, "Ml_pl", P), (Mu, "Mu", R),
+                        (Mu_pl, "Mu_pl", P)])
+        _check(self.lib.nicam_vi_rhow_getcoef(self.h, *a), "nicam_vi_rhow_getcoef")
 
     def oprt3d_divdamp(self, ddivdx, ddivdx_pl, ddivdy, ddivdy_pl, ddivdz, ddivdz_pl, rhogvx, rhogvx_pl,
                        rhogvy, rhogvy_pl, rhogvz, rhogvz_pl, rhogw, rhogw_pl, coef_intp, coef_intp_pl,

@@ -451,6 +451,27 @@ int nicam_vertical_face(nicam_ctx ctx, double *q_h, double *q_h_pl, const double
 int nicam_tracer_update_vertical(nicam_ctx ctx, double *rhogq, double *rhogq_pl, const double *flx_v,
                                  const double *flx_v_pl, const double *q_h, const double *q_h_pl);
 
+/* Coefficients of the tridiagonal rho*w system that nicam_vi_rhow_solver
+ * reads, computed on the device and installed in the context (what
+ * nicam_vi_rhow_solver_putcoef installs from arrays).  No reference
+ * counterpart: the reference reads Mc/Ml/Mu from its input file ("calculated
+ * in advance by other subroutine", docs/NICAM/src/33_vi_rhow_solver.tex:188-190).
+ * Collected from the discretised Helmholtz equation of
+ * docs/NICAM/src/20_intro_nicam.tex:480-600 at the half levels kmin+1..kmax
+ * (0 on the other levels), with A^o = VMTR_RGSGAM2, gam^2 = 1/VMTR_RGAMH^2,
+ * G^1/2 = VMTR_GSGAM2H/gam^2 (nicam_set_vmetrics), GRD_rdgz/rdgzh/cfact/dfact
+ * (nicam_grd_setup) and the constants of nicam_set_phys_const:
+ *   eth(gall,kall,lall)     h, the enthalpy of the large step at the half levels
+ *   g_tilde(gall,kall,lall) the effective gravity of the large step at the half levels
+ * The equation's scaling is the one vi_rhow_solver's Sall carries (the factor
+ * Cv/(R dt^2) of its source, dyn_vi_rhow_solver/src/mod_vi.f90:192-205). */
+int nicam_vi_rhow_update_matrix(nicam_ctx ctx, const double *eth, const double *eth_pl, const double *g_tilde,
+                                const double *g_tilde_pl, double dt);
+
+/* Copies the context's Mc/Ml/Mu (gall,kall,lall), _pl (gall_pl,kall,2) out. */
+int nicam_vi_rhow_getcoef(nicam_ctx ctx, double *Mc, double *Mc_pl, double *Ml, double *Ml_pl, double *Mu,
+                          double *Mu_pl);
+
 #ifdef __cplusplus
 }
 #endif

@@ -93,6 +93,8 @@ class Oracle:
         L.orc_vertical_courant.argtypes = [POINTER(OrcDims)] + [P] * 7
         L.orc_vertical_face.argtypes = [POINTER(OrcDims)] + [P] * 6
         L.orc_tracer_update_vertical.argtypes = [POINTER(OrcDims)] + [P] * 7
+        L.orc_vi_rhow_update_matrix.argtypes = ([POINTER(OrcDims)] + [P] * 20 +
+                                                [c_double, c_double, c_double, c_double, c_int])
         L.orc_num_threads.restype = c_int
 
     def num_threads(self) -> int:
@@ -192,6 +194,21 @@ class Oracle:
         ins = [_f64(a) for a in (rdgz, flx_v, flx_v_pl, q_h, q_h_pl)]
         self.lib.orc_tracer_update_vertical(ctypes.byref(D), _p(ins[0]), _p(out["rhogq"]), _p(out["rhogq_pl"]),
                                             *[_p(a) for a in ins[1:]])
+        return out
+
+    def vi_rhow_update_matrix(self, spec, eth, eth_pl, g_tilde, g_tilde_pl, vm, fac, dt, grav=9.80616, rdry=287.0,
+                              cvdry=717.5, alpha=1):
+        """vm: RGSGAM2, RGAMH, GSGAM2H (+ _pl); fac: grd_setup's rdgz, rdgzh, cfact, dfact."""
+        D = self.dims(spec)
+        out = {k: np.zeros(spec.shape3 if not k.endswith("_pl") else spec.shape_pl)
+               for k in ("Mc", "Mc_pl", "Ml", "Ml_pl", "Mu", "Mu_pl")}
+        ins = [_f64(a) for a in (eth, eth_pl, g_tilde, g_tilde_pl)]
+        ins += [_f64(vm[k]) for k in ("RGSGAM2", "RGSGAM2_pl", "RGAMH", "RGAMH_pl", "GSGAM2H", "GSGAM2H_pl")]
+        ins += [_f64(fac[k]) for k in ("rdgz", "rdgzh", "cfact", "dfact")]
+        self.lib.orc_vi_rhow_update_matrix(ctypes.byref(D), *[_p(out[k]) for k in ("Mc", "Mc_pl", "Ml", "Ml_pl", "Mu",
+                                                                                    "Mu_pl")],
+                                           *[_p(a) for a in ins], float(dt), float(grav), float(rdry), float(cvdry),
+                                           int(alpha))
         return out
 
     def grd_setup(self, gz, gzh):

@@ -1209,3 +1209,56 @@ void orc_tracer_update_vertical(const orc_dims *D, const double *rdgz, double *r
                 P3(rhogq_pl, v, k, l) = P3(rhogq_pl, v, k, l) - (up - lo) * rdgz[k - 1];
             }
 }
+
+/* ---------------------------------------------------------------------------
+ * Mc/Ml/Mu of the vertically implicit rho*w system (no reference code: the
+ * reference reads them from its input file).  Collected from the discretised
+ * Helmholtz equation, docs/NICAM/src/20_intro_nicam.tex:510-545, at the half
+ * level k (1-based kmin+1..kmax) between full levels k-1 and k; the docs'
+ * expressions for M^C and M^U (:585-596) agree with this collection, the
+ * docs' M^L (:597-599) writes A^o_{k+1}/dxi_{k+1} where collecting the terms
+ * of W_{k-1/2} gives A^o_k/dxi_k.
+ * ------------------------------------------------------------------------- */
+void orc_vi_rhow_update_matrix(const orc_dims *D, double *Mc, double *Mc_pl, double *Ml, double *Ml_pl, double *Mu,
+                               double *Mu_pl, const double *eth, const double *eth_pl, const double *g_tilde,
+                               const double *g_tilde_pl, const double *RGSGAM2, const double *RGSGAM2_pl,
+                               const double *RGAMH, const double *RGAMH_pl, const double *GSGAM2H,
+                               const double *GSGAM2H_pl, const double *rdgz, const double *rdgzh, const double *cfact,
+                               const double *dfact, double dt, double grav, double rdry, double cvdry, int alpha_i)
+{
+    const int iall = D->gall_1d, gall = iall * iall, kall = D->kall, lall = D->lall, gall_pl = D->gall_pl;
+    const int kmin = 2, kmax = kall - 1;
+    const double cvovr = cvdry / rdry, cvovrt2 = cvdry / rdry / (dt * dt), alpha = (double)alpha_i;
+    for (int part = 0; part < 2; ++part) {
+        if (part == 1 && !D->have_pl) break;
+        const int ng = part ? gall_pl : gall, nl = part ? LALL_PL : lall;
+        double *mc = part ? Mc_pl : Mc, *ml = part ? Ml_pl : Ml, *mu = part ? Mu_pl : Mu;
+        const double *h = part ? eth_pl : eth, *gt = part ? g_tilde_pl : g_tilde;
+        const double *ao_ = part ? RGSGAM2_pl : RGSGAM2, *rh_ = part ? RGAMH_pl : RGAMH;
+        const double *gs_ = part ? GSGAM2H_pl : GSGAM2H;
+#define IX(g, k, l) (((size_t)((l) - 1) * kall + ((k) - 1)) * ng + ((g) - 1))
+        for (int l = 1; l <= nl; ++l)
+            for (int k = 1; k <= kall; ++k)
+                for (int g = 1; g <= ng; ++g) {
+                    double c = 0.0, lo = 0.0, up = 0.0;
+                    if (k >= kmin + 1 && k <= kmax) {
+                        const double rhm = rh_[IX(g, k - 1, l)], rh = rh_[IX(g, k, l)], rhp = rh_[IX(g, k + 1, l)];
+                        const double g2m = 1.0 / (rhm * rhm), g2 = 1.0 / (rh * rh), g2p = 1.0 / (rhp * rhp);
+                        const double aim = g2m * h[IX(g, k - 1, l)], ai = g2 * h[IX(g, k, l)],
+                                     aip = g2p * h[IX(g, k + 1, l)];
+                        const double co = cvovr * grav * rh * rh;
+                        const double dd = cvovrt2 * gs_[IX(g, k, l)] * rh * rh;
+                        const double ao = ao_[IX(g, k, l)], aom = ao_[IX(g, k - 1, l)];
+                        const double rzh = rdgzh[k - 1];
+                        c = alpha * dd + (ao * ai * rdgz[k - 1] + aom * ai * rdgz[k - 2]) * rzh -
+                            (dfact[k - 1] - cfact[k - 2]) * (gt[IX(g, k, l)] + co * g2) * rzh;
+                        up = -ao * aip * rdgz[k - 1] * rzh - cfact[k - 1] * (gt[IX(g, k + 1, l)] + co * g2p) * rzh;
+                        lo = -aom * aim * rdgz[k - 2] * rzh + dfact[k - 2] * (gt[IX(g, k - 1, l)] + co * g2m) * rzh;
+                    }
+                    mc[IX(g, k, l)] = c;
+                    ml[IX(g, k, l)] = lo;
+                    mu[IX(g, k, l)] = up;
+                }
+#undef IX
+    }
+}

@@ -177,6 +177,15 @@ void orc_vertical_face(const orc_dims *D, const double *afact, const double *bfa
 void orc_tracer_update_vertical(const orc_dims *D, const double *rdgz, double *rhogq, double *rhogq_pl,
                                 const double *flx_v, const double *flx_v_pl, const double *q_h, const double *q_h_pl);
 
+/* Mc/Ml/Mu of the rho*w system from h (eth) and g~ at the half levels, the
+ * VMTR metrics and the GRD factors (no reference code). */
+void orc_vi_rhow_update_matrix(const orc_dims *D, double *Mc, double *Mc_pl, double *Ml, double *Ml_pl, double *Mu,
+                               double *Mu_pl, const double *eth, const double *eth_pl, const double *g_tilde,
+                               const double *g_tilde_pl, const double *RGSGAM2, const double *RGSGAM2_pl,
+                               const double *RGAMH, const double *RGAMH_pl, const double *GSGAM2H,
+                               const double *GSGAM2H_pl, const double *rdgz, const double *rdgzh, const double *cfact,
+                               const double *dfact, double dt, double grav, double rdry, double cvdry, int alpha_i);
+
 #ifdef __cplusplus
 }
 #endif
