@@ -27,8 +27,15 @@ namespace {
 
 constexpr int CHUNK = 256;  // output cells per work-group
 constexpr int MAXI = 160;   // largest gall_1d supported (gl8/rl1 regions are 130)
-constexpr int NST = 3;      // staged cells per work-item: CHUNK*NST >= CHUNK + 2*(MAXI+1)
 constexpr int NTP = 2;      // TI/TJ points per work-item: CHUNK*NTP >= CHUNK + MAXI + 1
+// Staged cells per work-item, NST: CHUNK*NST >= CHUNK + 2*(gall_1d+1); the
+// launcher picks 2 up to gall_1d = 127 and 3 above, to spare registers.
+#ifndef NICAM_HFLUX_WT_LDS
+#define NICAM_HFLUX_WT_LDS 0  // 1: triangle weights in LDS (154 VGPRs, 3 waves per SIMD) measured no faster at gl7
+#endif
+#ifndef NICAM_HFLUX_XR_GLOBAL
+#define NICAM_HFLUX_XR_GLOBAL 0  // 1: arc midpoints read per level from L2 rather than held (no fewer registers)
+#endif
 
 struct HfluxArgs {
     double *flx;         // (gall,kall,lall,6)
@@ -87,6 +94,7 @@ __device__ void hflux_pole(const HfluxArgs &A, int t)
     }
 }
 
+template <int NST>
 __global__ void __launch_bounds__(CHUNK) hflux_kernel(HfluxArgs A)
 {
     extern __shared__ double lds[];
@@ -95,6 +103,9 @@ __global__ void __launch_bounds__(CHUNK) hflux_kernel(HfluxArgs A)
     const int n_t = CHUNK + iall + 1;
     double *s_in = lds;             // [4][n_in]
     double *s_t = lds + 4 * n_in;   // [8][n_t]: TI fields 0..3, TJ fields 4..7
+#if NICAM_HFLUX_WT_LDS
+    double *s_w = s_t + 8 * n_t;    // [6][n_t]: GMTR_t W1..W3 of TI, then of TJ
+#endif
 
     // Logical block -> (region, level range, chunk); the chunks of one region
     // and level range are consecutive and are kept on one XCD (hardware
@@ -127,7 +138,9 @@ __global__ void __launch_bounds__(CHUNK) hflux_kernel(HfluxArgs A)
 
     // ---- K-invariant metrics into registers ----
     // triangle weights of my TI/TJ points (GMTR_t(g,1,l,t,w): ((w*2+t)*lall+l)*gall+g)
+#if !NICAM_HFLUX_WT_LDS
     double wt[NTP][6];
+#endif
     int tp[NTP];
 #pragma unroll
     for (int r = 0; r < NTP; ++r) {
@@ -138,7 +151,12 @@ __global__ void __launch_bounds__(CHUNK) hflux_kernel(HfluxArgs A)
 #pragma unroll
         for (int q = 0; q < 6; ++q) {
             const int w = 2 + (q % 3), t = q / 3;
-            wt[r][q] = tp[r] >= 0 ? A.gt[(size_t)(w * 2 + t) * mstr + mb + p] : 0.0;
+            const double v = tp[r] >= 0 ? A.gt[(size_t)(w * 2 + t) * mstr + mb + p] : 0.0;
+#if NICAM_HFLUX_WT_LDS
+            if (s < n_t) s_w[q * n_t + s] = v;  // read only by this work-item
+#else
+            wt[r][q] = v;
+#endif
         }
     }
     const int g = g0 + tid;
@@ -153,7 +171,13 @@ __global__ void __launch_bounds__(CHUNK) hflux_kernel(HfluxArgs A)
     const bool a_f5 = active && i >= 1 && j >= 1 && i - 1 <= gmx && j - 1 <= gmx;
     const bool a_f6 = active && i >= 1 && i <= gmx && j >= 1 && j - 1 <= gmx;
     // GMTR_a(g,1,l,a,c): ((c*3+a)*lall+l)*gall+g; GRD_xr(g,1,l,a,d): ((d*3+a)*lall+l)*gall+g
-    double hn[6][3], xr[3][3], rarea = 0.0;
+    double hn[6][3], rarea = 0.0;
+#if NICAM_HFLUX_XR_GLOBAL
+#define XR(a, d) A.xr[(size_t)((d) * 3 + (a)) * mstr + mb + g]
+#else
+    double xr[3][3];
+#define XR(a, d) xr[a][d]
+#endif
     {
         const bool have[6] = {a_ai, a_aij, a_aj, a_f4, a_f5, a_f6};
         const int org[6] = {g, g, g, g - 1, g - 1 - iall, g - iall};
@@ -165,7 +189,11 @@ __global__ void __launch_bounds__(CHUNK) hflux_kernel(HfluxArgs A)
 #pragma unroll
         for (int a = 0; a < 3; ++a)
 #pragma unroll
-            for (int d = 0; d < 3; ++d) xr[a][d] = active ? A.xr[(size_t)(d * 3 + a) * mstr + mb + g] : 0.0;
+            for (int d = 0; d < 3; ++d) {
+#if !NICAM_HFLUX_XR_GLOBAL
+                xr[a][d] = active ? A.xr[(size_t)(d * 3 + a) * mstr + mb + g] : 0.0;
+#endif
+            }
         if (active) rarea = A.gp[1 * mstr + mb + g];
     }
 
@@ -209,10 +237,18 @@ __global__ void __launch_bounds__(CHUNK) hflux_kernel(HfluxArgs A)
 #pragma unroll
             for (int f = 0; f < 4; ++f) {
                 const double *in = s_in + f * n_in;
+#if NICAM_HFLUX_WT_LDS
+                const double *w = s_w + s;
+                double ti = in[s] * w[0 * n_t] + in[s10] * w[1 * n_t];
+                ti = ti + in[s11] * w[2 * n_t];
+                double tj = in[s] * w[3 * n_t];
+                tj = tj + in[s11] * w[4 * n_t] + in[s01] * w[5 * n_t];
+#else
                 double ti = in[s] * wt[r][0] + in[s10] * wt[r][1];
                 ti = ti + in[s11] * wt[r][2];
                 double tj = in[s] * wt[r][3];
                 tj = tj + in[s11] * wt[r][4] + in[s01] * wt[r][5];
+#endif
                 s_t[f * n_t + s] = ti;
                 s_t[(4 + f) * n_t + s] = tj;
             }
@@ -239,8 +275,7 @@ __global__ void __launch_bounds__(CHUNK) hflux_kernel(HfluxArgs A)
 
         if (active) {
             const int s = g - lo;
-            double face[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-            double xcv[3][3] = {{0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}};
+            const size_t fb = ((size_t)l * kall + k) * slab + g;
             // arc a between TI/TJ points (sa, ta) and (sb, tb); ta/tb = 0 for TI, 4 for TJ
             auto arc = [&](int sa, int ta, int sb, int tb, const double (&h)[3], double &x2, double &y2, double &z2,
                            double &rr2) {
@@ -250,43 +285,38 @@ __global__ void __launch_bounds__(CHUNK) hflux_kernel(HfluxArgs A)
                 z2 = s_t[(ta + 3) * n_t + sa] + s_t[(tb + 3) * n_t + sb];
                 return 0.5 * (x2 * h[0] + y2 * h[1] + z2 * h[2]);
             };
+            // each own arc's face and mass centroid are stored as soon as they are formed
+            auto own_arc = [&](int m, bool have, int sa, int ta, int sb, int tb) {
+                double f = 0.0, c0 = 0.0, c1 = 0.0, c2 = 0.0;
+                if (have) {
+                    double x2, y2, z2, rr2;
+                    const double flux = arc(sa, ta, sb, tb, hn[m], x2, y2, z2, rr2);
+                    f = flux * rarea * dt;
+                    c0 = XR(m, 0) - x2 * rr2 * dt * 0.5;
+                    c1 = XR(m, 1) - y2 * rr2 * dt * 0.5;
+                    c2 = XR(m, 2) - z2 * rr2 * dt * 0.5;
+                }
+                A.flx[m * plane + fb] = f;
+                A.xc[(size_t)(0 * 3 + m) * plane + fb] = c0;
+                A.xc[(size_t)(1 * 3 + m) * plane + fb] = c1;
+                A.xc[(size_t)(2 * 3 + m) * plane + fb] = c2;
+            };
+            own_arc(0, a_ai, s - iall, 4, s, 0);   // AI: TJ(ijm1) + TI(ij)   FLX:241-255
+            own_arc(1, a_aij, s, 0, s, 4);         // AIJ: TI(ij) + TJ(ij)    FLX:266-280
+            own_arc(2, a_aj, s, 4, s - 1, 0);      // AJ: TJ(ij) + TI(im1j)   FLX:292-306
             double x2, y2, z2, rr2;
-            if (a_ai) {  // AI: TJ(ijm1) + TI(ij)   FLX:241-255
-                const double flux = arc(s - iall, 4, s, 0, hn[0], x2, y2, z2, rr2);
-                face[0] = flux * rarea * dt;
-                xcv[0][0] = xr[0][0] - x2 * rr2 * dt * 0.5;
-                xcv[0][1] = xr[0][1] - y2 * rr2 * dt * 0.5;
-                xcv[0][2] = xr[0][2] - z2 * rr2 * dt * 0.5;
-            }
-            if (a_aij) {  // AIJ: TI(ij) + TJ(ij)   FLX:266-280
-                const double flux = arc(s, 0, s, 4, hn[1], x2, y2, z2, rr2);
-                face[1] = flux * rarea * dt;
-                xcv[1][0] = xr[1][0] - x2 * rr2 * dt * 0.5;
-                xcv[1][1] = xr[1][1] - y2 * rr2 * dt * 0.5;
-                xcv[1][2] = xr[1][2] - z2 * rr2 * dt * 0.5;
-            }
-            if (a_aj) {  // AJ: TJ(ij) + TI(im1j)   FLX:292-306
-                const double flux = arc(s, 4, s - 1, 0, hn[2], x2, y2, z2, rr2);
-                face[2] = flux * rarea * dt;
-                xcv[2][0] = xr[2][0] - x2 * rr2 * dt * 0.5;
-                xcv[2][1] = xr[2][1] - y2 * rr2 * dt * 0.5;
-                xcv[2][2] = xr[2][2] - z2 * rr2 * dt * 0.5;
-            }
-            if (a_f4) face[3] = -arc(s - 1 - iall, 4, s - 1, 0, hn[3], x2, y2, z2, rr2) * rarea * dt;
-            if (a_f5) face[4] = -arc(s - 1 - iall, 0, s - 1 - iall, 4, hn[4], x2, y2, z2, rr2) * rarea * dt;
-            if (a_f6) face[5] = -arc(s - iall, 4, s - iall - 1, 0, hn[5], x2, y2, z2, rr2) * rarea * dt;
-            if (sgp && i == 1 && j == 1) face[5] = 0.0;  // FLX:311-320
-
-            const size_t fb = ((size_t)l * kall + k) * slab + g;
-#pragma unroll
-            for (int m = 0; m < 6; ++m) A.flx[m * plane + fb] = face[m];
-#pragma unroll
-            for (int d = 0; d < 3; ++d)
-#pragma unroll
-                for (int a = 0; a < 3; ++a) A.xc[(size_t)(d * 3 + a) * plane + fb] = xcv[a][d];
+            double f4 = 0.0, f5 = 0.0, f6 = 0.0;
+            if (a_f4) f4 = -arc(s - 1 - iall, 4, s - 1, 0, hn[3], x2, y2, z2, rr2) * rarea * dt;
+            if (a_f5) f5 = -arc(s - 1 - iall, 0, s - 1 - iall, 4, hn[4], x2, y2, z2, rr2) * rarea * dt;
+            if (a_f6) f6 = -arc(s - iall, 4, s - iall - 1, 0, hn[5], x2, y2, z2, rr2) * rarea * dt;
+            if (sgp && i == 1 && j == 1) f6 = 0.0;  // FLX:311-320
+            A.flx[3 * plane + fb] = f4;
+            A.flx[4 * plane + fb] = f5;
+            A.flx[5 * plane + fb] = f6;
         }
         __syncthreads();
     }
+#undef XR
 }
 
 }  // namespace
@@ -339,10 +369,15 @@ int launch_hflux(nicam_ctx c, double *flx_h, double *flx_h_pl, double *xc, doubl
     A.gt_pl = c->GMTR_t_pl;
     A.ga_pl = c->GMTR_a_pl;
     A.gall_pl = D.gall_pl;
-    const size_t lds = sizeof(double) * (4 * (CHUNK + 2 * (D.gall_1d + 1)) + 8 * (CHUNK + D.gall_1d + 1));
-    if (lds > 64 * 1024)
-        NICAM_HIP(hipFuncSetAttribute((const void *)hflux_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    hipLaunchKernelGGL(hflux_kernel, dim3((unsigned)(nmain + npole)), dim3(CHUNK), lds, c->stream, A);
+    const size_t n_t = CHUNK + D.gall_1d + 1;
+    const size_t lds = sizeof(double) * (4 * (CHUNK + 2 * (D.gall_1d + 1)) + 8 * n_t + (NICAM_HFLUX_WT_LDS ? 6 * n_t : 0));
+    const bool two = CHUNK * 2 >= CHUNK + 2 * (D.gall_1d + 1);  // two staged cells per work-item suffice
+    const void *fn = two ? (const void *)hflux_kernel<2> : (const void *)hflux_kernel<3>;
+    if (lds > 64 * 1024) NICAM_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    if (two)
+        hipLaunchKernelGGL(hflux_kernel<2>, dim3((unsigned)(nmain + npole)), dim3(CHUNK), lds, c->stream, A);
+    else
+        hipLaunchKernelGGL(hflux_kernel<3>, dim3((unsigned)(nmain + npole)), dim3(CHUNK), lds, c->stream, A);
     NICAM_HIP(hipGetLastError());
     return NICAM_OK;
 }

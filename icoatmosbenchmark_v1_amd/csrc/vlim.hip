@@ -87,15 +87,18 @@ __device__ __forceinline__ void vlim_run(double *__restrict__ q_h, const double 
     }
 }
 
+// A work-group is BLOCK consecutive columns at one run of levels.  The runs
+// of one column block are consecutive work-groups on one XCD: each run also
+// reads the two levels below it, which the run below has just brought into L2.
 __global__ void __launch_bounds__(BLOCK) vlim_kernel(VlimArgs A)
 {
-    // work-item -> (run, column); runs of one level band are contiguous in g
-    const long t = (long)blockIdx.x * BLOCK + threadIdx.x;
     const long ncol_reg = (long)A.gall * A.lall;
     const long ncol = ncol_reg + (A.have_pl ? (long)A.gall_pl * kLallPl : 0);
-    if (t >= ncol * A.nrun) return;
-    const int run = (int)(t / ncol);
-    const long col = t % ncol;
+    const long ncb = (ncol + BLOCK - 1) / BLOCK;  // column blocks
+    const long L = xcd_contiguous(blockIdx.x, ncb * A.nrun);
+    const int run = (int)(L % A.nrun);
+    const long col = (L / A.nrun) * BLOCK + threadIdx.x;
+    if (col >= ncol) return;
     const int kmin = 1, kmax = A.kall - 2;  // 0-based
     const int kb = kmin + 1 + run * A.kc;
     const int ke = min(kmax + 1, kb + A.kc);
@@ -144,8 +147,7 @@ int launch_vlim(nicam_ctx c, double *q_h, double *q_h_pl, const double *q, const
     A.kc = kc;
     A.nrun = (nlev + kc - 1) / kc;
     const long ncol = (long)D.gall * D.lall + (D.have_pl ? (long)D.gall_pl * kLallPl : 0);
-    const long n = ncol * A.nrun;
-    const long nblk = (n + BLOCK - 1) / BLOCK;
+    const long nblk = (ncol + BLOCK - 1) / BLOCK * A.nrun;
     if (nblk > 0x7fffffffL) return fail(NICAM_ERR_ARG, "grid too large for one launch");
     hipLaunchKernelGGL(vlim_kernel, dim3((unsigned)nblk), dim3(BLOCK), 0, c->stream, A);
     NICAM_HIP(hipGetLastError());
