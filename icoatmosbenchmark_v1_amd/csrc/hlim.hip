@@ -24,6 +24,9 @@ namespace {
 #ifndef NICAM_HLIM_NTLOAD
 #define NICAM_HLIM_NTLOAD 0
 #endif
+#ifndef NICAM_HLIM_STORE
+#define NICAM_HLIM_STORE 0  // 0: nontemporal stores (668 us at gl7), 1: plain mirrored-face stores (697), 2: plain throughout (710)
+#endif
 constexpr int CHUNK = NICAM_HLIM_CHUNK;  // cells (work-items) per work-group
 constexpr int MAXI = 160;
 
@@ -186,8 +189,13 @@ __global__ void __launch_bounds__(CHUNK) hlim_kernel(HlimArgs A)
         qout_max = qc;
     }
     if (A.phases & 1) {
+#if NICAM_HLIM_STORE == 2
+        A.Qprev[fb + g] = qout_min;
+        A.Qprev[plane + fb + g] = qout_max;
+#else
         stream_store(&A.Qprev[fb + g], qout_min);
         stream_store(&A.Qprev[plane + fb + g], qout_max);
+#endif
     }
 
     // ---- phase 2: clamp the face values of the own arcs (HLIM:377-410) ----
@@ -205,8 +213,16 @@ __global__ void __launch_bounds__(CHUNK) hlim_kernel(HlimArgs A)
             double qa = ld_once(&A.q_a[(size_t)m * plane + fb + g]);
             qa = (c)*dmin(dmax(qa, in_mn_s), in_mx_s) + (1.0 - c) * dmin(dmax(qa, in_mn_n), in_mx_n);
             qa = (c)*dmax(dmin(qa, qo_max_n), qo_min_n) + (1.0 - c) * dmax(dmin(qa, qo_max_c), qo_min_c);
+#if NICAM_HLIM_STORE == 2
+            A.q_a[(size_t)m * plane + fb + g] = qa;
+#else
             stream_store(&A.q_a[(size_t)m * plane + fb + g], qa);
+#endif
+#if NICAM_HLIM_STORE >= 1
+            A.q_a[(size_t)(m + 3) * plane + fb + nb[m]] = qa;  // one element off the wave's alignment
+#else
             stream_store(&A.q_a[(size_t)(m + 3) * plane + fb + nb[m]], qa);
+#endif
         }
     }
 }

@@ -288,6 +288,38 @@ def test_gpu_chained_step_matches_oracle(gpu, oracle):
     assert_parity(dict(q_a=qa.cpu().numpy(), q_a_pl=qa_pl.cpu().numpy()), dict(q_a=lim["q_a"], q_a_pl=lim["q_a_pl"]))
 
 
+@pytest.mark.gpu
+def test_gpu_glue_host_pointer_mode(gpu, oracle):
+    """The four entry points with host arrays (the library stages them over
+    PCIe): the same bits as the oracle."""
+    spec, geo, hm, x, x_pl, cg, cg_pl, case = _ico(4, 0, 5, 13)
+    ctx = drivers.setup_context(spec, geo, pointer_mode=PTR_HOST)
+    f = case["flux"]
+    fl = oracle.horizontal_flux(spec, geo, f, 360.0)
+    q = case["hlim"]["q"]
+    q, q_pl = (a[0] for a in apply_map(hm, q[None], np.zeros((1,) + spec.shape_pl)))
+    C = lambda a: np.ascontiguousarray(a, dtype=np.float64)  # noqa: E731
+    g, g_pl = np.full((3,) + spec.shape3, 5.0), np.full((3,) + spec.shape_pl, 5.0)
+    ctx.oprt_gradient(g, g_pl, C(q), C(q_pl), C(cg), C(cg_pl))
+    assert_parity(dict(grad=g, grad_pl=g_pl), oracle.oprt_gradient(spec, q, q_pl, cg, cg_pl))
+    want = oracle.tracer_courant(spec, fl["flx_h"], fl["flx_h_pl"], f["rho"], f["rho_pl"])
+    o = {k: np.full(v.shape, 5.0) for k, v in want.items()}
+    ctx.tracer_courant(o["ch"], o["ch_pl"], o["cmask"], o["cmask_pl"], C(fl["flx_h"]), C(fl["flx_h_pl"]),
+                       C(f["rho"]), C(f["rho_pl"]))
+    assert_parity(o, want)
+    grad, grad_pl = _exchange(hm, g, g_pl)
+    qa, qa_pl = np.zeros((6,) + spec.shape3), np.zeros(spec.shape_pl)
+    want = oracle.horizontal_remap(spec, qa, qa_pl, q, q_pl, grad, grad_pl, o["cmask"], o["cmask_pl"], fl["GRD_xc"],
+                                   fl["GRD_xc_pl"], x, x_pl)
+    ctx.horizontal_remap(qa, qa_pl, C(q), C(q_pl), C(grad), C(grad_pl), o["cmask"], o["cmask_pl"], C(fl["GRD_xc"]),
+                         C(fl["GRD_xc_pl"]), C(x), C(x_pl))
+    assert_parity(dict(q_a=qa, q_a_pl=qa_pl), want)
+    rq, rq_pl = C(f["rho"] * q), C(f["rho_pl"] * q_pl)
+    want = oracle.tracer_update(spec, rq, rq_pl, fl["flx_h"], fl["flx_h_pl"], qa, qa_pl)
+    ctx.tracer_update(rq, rq_pl, C(fl["flx_h"]), C(fl["flx_h_pl"]), qa, qa_pl)
+    assert_parity(dict(rhogq=rq, rhogq_pl=rq_pl), want)
+
+
 def test_host_pointer_mode_requires_nothing_on_cpu(lib):
     """The four entry points are exported and declared (no GPU call here)."""
     for name in ("nicam_oprt_gradient", "nicam_tracer_courant", "nicam_horizontal_remap", "nicam_tracer_update"):
