@@ -14,14 +14,16 @@ namespace nicam {
 
 namespace {
 
+template <class T>
 struct GatherArgs {
-    double *var;     // (gall,kmax,lall,nvar)
-    double *var_pl;  // (gall_pl,kmax,2,nvar)
+    T *var;     // (gall,kmax,lall,nvar)
+    T *var_pl;  // (gall_pl,kmax,2,nvar)
     const int *dst, *src;  // encoded points: l*gall + g, or lall*gall + l_pl*gall_pl + v
     int n, gall, kmax, lall, gall_pl, nvar;
 };
 
-__device__ __forceinline__ double *locate(const GatherArgs &A, int code, int k, int v)
+template <class T>
+__device__ __forceinline__ T *locate(const GatherArgs<T> &A, int code, int k, int v)
 {
     const int nreg = A.gall * A.lall;
     if (code < nreg) {
@@ -35,7 +37,8 @@ __device__ __forceinline__ double *locate(const GatherArgs &A, int code, int k, 
 
 // One work-item per (entry, level); entries vary fastest so that the
 // consecutive halo points of one edge share cache lines of their rows.
-__global__ void __launch_bounds__(256) halo_gather_kernel(GatherArgs A)
+template <class T>
+__global__ void __launch_bounds__(256) halo_gather_kernel(GatherArgs<T> A)
 {
     const long t = (long)blockIdx.x * 256 + threadIdx.x;
     if (t >= (long)A.n * A.kmax) return;
@@ -44,12 +47,13 @@ __global__ void __launch_bounds__(256) halo_gather_kernel(GatherArgs A)
     for (int v = 0; v < A.nvar; ++v) *locate(A, d, k, v) = *locate(A, s, k, v);
 }
 
-int gather(nicam_ctx c, GatherArgs A)
+template <class T>
+int gather(nicam_ctx c, GatherArgs<T> A)
 {
     const long n = (long)A.n * A.kmax;
     if (n == 0) return NICAM_OK;
     if ((n + 255) / 256 > 0x7fffffffL) return fail(NICAM_ERR_ARG, "halo fill too large for one launch");
-    hipLaunchKernelGGL(halo_gather_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, A);
+    hipLaunchKernelGGL(halo_gather_kernel<T>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, A);
     NICAM_HIP(hipGetLastError());
     return NICAM_OK;
 }
@@ -96,12 +100,13 @@ int comm_setup(nicam_ctx c, int rlevel)
     return NICAM_OK;
 }
 
-int launch_halo_fill(nicam_ctx c, double *var, double *var_pl, int kmax, int nvar, bool pole_fill)
+template <class T>
+int halo_fill_t(nicam_ctx c, T *var, T *var_pl, int kmax, int nvar, bool pole_fill)
 {
     if (!c->halo_n) return fail(NICAM_ERR_STATE, "halo map not set (nicam_comm_setup)");
     if (kmax < 1 || nvar < 1) return fail(NICAM_ERR_ARG, "kmax and vmax must be positive");
     const Dims &D = c->d;
-    GatherArgs A;
+    GatherArgs<T> A;
     A.var = var;
     A.var_pl = var_pl;
     A.gall = D.gall;
@@ -120,6 +125,16 @@ int launch_halo_fill(nicam_ctx c, double *var, double *var_pl, int kmax, int nva
     A.src = c->halo_src + 2;
     A.n = c->halo_n;
     return gather(c, A);
+}
+
+int launch_halo_fill(nicam_ctx c, double *var, double *var_pl, int kmax, int nvar, bool pole_fill)
+{
+    return halo_fill_t<double>(c, var, var_pl, kmax, nvar, pole_fill);
+}
+
+int launch_halo_fill_f32(nicam_ctx c, float *var, float *var_pl, int kmax, int nvar, bool pole_fill)
+{
+    return halo_fill_t<float>(c, var, var_pl, kmax, nvar, pole_fill);
 }
 
 }  // namespace nicam

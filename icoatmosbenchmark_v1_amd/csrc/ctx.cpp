@@ -39,6 +39,17 @@ static int dev_alloc(nicam_ctx c, double **p, size_t n)
     return NICAM_OK;
 }
 
+static int dev_alloc_f(nicam_ctx c, float **p, size_t n)
+{
+    if (*p) return NICAM_OK;
+    void *q = nullptr;
+    hipError_t e = hipMalloc(&q, n * sizeof(float));
+    if (e != hipSuccess) return hip_fail(e, "hipMalloc");
+    *p = static_cast<float *>(q);
+    c->device_bytes += n * sizeof(float);
+    return NICAM_OK;
+}
+
 static void dev_free(double *&p)
 {
     if (p) (void)hipFree(p);
@@ -175,6 +186,10 @@ int nicam_ctx_destroy(nicam_ctx c)
                       &c->RGSQRTH, &c->RGSQRTH_pl, &c->C2WfactGz, &c->C2WfactGz_pl, &c->rdgz,
                       &c->cfact, &c->dfact};
     for (double **p : all) dev_free(*p);
+    float *f32[] = {c->hm32.xr, c->hm32.gp, c->hm32.gt, c->hm32.ga,
+                    c->hm32.xr_pl, c->hm32.gp_pl, c->hm32.gt_pl, c->hm32.ga_pl};
+    for (float *p : f32)
+        if (p) (void)hipFree(p);
     for (void *p : c->stage)
         if (p) (void)hipFree(p);
     if (c->have_sgp) (void)hipFree(c->have_sgp);
@@ -404,6 +419,26 @@ int nicam_set_hmetrics(nicam_ctx c, const double *xr, const double *xr_pl, const
         NICAM_TRY(copy_in(c, c->GMTR_p_pl, gp_pl, p * kGmtrPNmax));
         NICAM_TRY(copy_in(c, c->GMTR_t_pl, gt_pl, p * kGmtrTNmax));
         NICAM_TRY(copy_in(c, c->GMTR_a_pl, ga_pl, p * kGmtrANmaxPl));
+    }
+    // FP32 copies for the _f32 entry points
+    nicam::HMetrics<float> &h = c->hm32;
+    NICAM_TRY(dev_alloc_f(c, &h.xr, g * 9));
+    NICAM_TRY(dev_alloc_f(c, &h.gp, g * kGmtrPNmax));
+    NICAM_TRY(dev_alloc_f(c, &h.gt, g * 2 * kGmtrTNmax));
+    NICAM_TRY(dev_alloc_f(c, &h.ga, g * 3 * kGmtrANmax));
+    NICAM_TRY(launch_to_float(c, c->GRD_xr, h.xr, g * 9));
+    NICAM_TRY(launch_to_float(c, c->GMTR_p, h.gp, g * kGmtrPNmax));
+    NICAM_TRY(launch_to_float(c, c->GMTR_t, h.gt, g * 2 * kGmtrTNmax));
+    NICAM_TRY(launch_to_float(c, c->GMTR_a, h.ga, g * 3 * kGmtrANmax));
+    if (c->d.have_pl) {
+        NICAM_TRY(dev_alloc_f(c, &h.xr_pl, p * 3));
+        NICAM_TRY(dev_alloc_f(c, &h.gp_pl, p * kGmtrPNmax));
+        NICAM_TRY(dev_alloc_f(c, &h.gt_pl, p * kGmtrTNmax));
+        NICAM_TRY(dev_alloc_f(c, &h.ga_pl, p * kGmtrANmaxPl));
+        NICAM_TRY(launch_to_float(c, c->GRD_xr_pl, h.xr_pl, p * 3));
+        NICAM_TRY(launch_to_float(c, c->GMTR_p_pl, h.gp_pl, p * kGmtrPNmax));
+        NICAM_TRY(launch_to_float(c, c->GMTR_t_pl, h.gt_pl, p * kGmtrTNmax));
+        NICAM_TRY(launch_to_float(c, c->GMTR_a_pl, h.ga_pl, p * kGmtrANmaxPl));
     }
     NICAM_HIP(hipStreamSynchronize(c->stream));
     c->have_hmetrics = true;
@@ -978,6 +1013,94 @@ int nicam_vi_rhow_getcoef(nicam_ctx c, double *Mc, double *Mc_pl, double *Ml, do
     }
     NICAM_HIP(hipStreamSynchronize(c->stream));
     return NICAM_OK;
+}
+
+int nicam_horizontal_flux_f32(nicam_ctx c, float *flx_h, float *flx_h_pl, float *xc, float *xc_pl,
+                          const float *rho, const float *rho_pl, const float *rvx,
+                          const float *rvx_pl, const float *rvy, const float *rvy_pl,
+                          const float *rvz, const float *rvz_pl, float dt)
+{
+    NICAM_TRY(check_ctx(c));
+    if (!c->have_hmetrics) return fail(NICAM_ERR_STATE, "horizontal metrics not set (nicam_set_hmetrics)");
+    const void *req[] = {flx_h, xc, rho, rvx, rvy, rvz};
+    const char *nm[] = {"flx_h", "GRD_xc", "rho", "rhovx", "rhovy", "rhovz"};
+    for (int i = 0; i < 6; ++i) NICAM_TRY(need(req[i], nm[i]));
+    if (c->d.have_pl) {
+        const void *rp[] = {flx_h_pl, xc_pl, rho_pl, rvx_pl, rvy_pl, rvz_pl};
+        for (int i = 0; i < 6; ++i) NICAM_TRY(need(rp[i], "pole array"));
+    }
+    if (c->ptr_mode == NICAM_PTR_DEVICE)
+        return launch_hflux_f32(c, flx_h, PL(flx_h_pl), xc, PL(xc_pl), rho, PL(rho_pl), rvx, PL(rvx_pl),
+                            rvy, PL(rvy_pl), rvz, PL(rvz_pl), dt);
+    const size_t R = c->d.nreg() * 4, P = c->d.npl() * 4;
+    std::vector<HostArg> a = {{flx_h, R * 6, true},  {PL(flx_h_pl), P, true}, {xc, R * 9, true},
+                              {PL(xc_pl), P * 3, true}, {rho, R, false},       {PL(rho_pl), P, false},
+                              {rvx, R, false},         {PL(rvx_pl), P, false}, {rvy, R, false},
+                              {PL(rvy_pl), P, false},  {rvz, R, false},        {PL(rvz_pl), P, false}};
+    std::vector<void *> v;
+    NICAM_TRY(stage_in(c, a, v));
+    auto D = [&](int i) { return static_cast<float *>(v[i]); };
+    NICAM_TRY(launch_hflux_f32(c, D(0), D(1), D(2), D(3), D(4), D(5), D(6), D(7), D(8), D(9), D(10), D(11), dt));
+    return stage_out(c, a, v);
+}
+
+int nicam_horizontal_limiter_thuburn_f32(nicam_ctx c, float *q_a, float *q_a_pl, const float *q,
+                                     const float *q_pl, const float *d, const float *d_pl,
+                                     const float *ch, const float *ch_pl, const float *cmask,
+                                     const float *cmask_pl, float *Qprev, float *Qprev_pl,
+                                     const float *Qpost, const float *Qpost_pl)
+{
+    NICAM_TRY(check_ctx(c));
+    // Qout_post NULL: the library exchanges Qout_prev itself (nicam_comm_setup
+    // must have run); Qout_post_pl is then ignored.
+    const void *req[] = {q_a, q, d, ch, cmask, Qprev};
+    const char *nm[] = {"q_a", "q", "d", "ch", "cmask", "Qout_prev"};
+    for (int i = 0; i < 6; ++i) NICAM_TRY(need(req[i], nm[i]));
+    if (c->d.have_pl) {
+        const void *rp[] = {q_a_pl, q_pl, d_pl, ch_pl, cmask_pl, Qprev_pl};
+        for (int i = 0; i < 6; ++i) NICAM_TRY(need(rp[i], "pole array"));
+        if (Qpost) NICAM_TRY(need(Qpost_pl, "Qout_post_pl"));
+    }
+    if (!Qpost) Qpost_pl = nullptr;
+    if (c->ptr_mode == NICAM_PTR_DEVICE)
+        return launch_hlim_f32(c, q_a, PL(q_a_pl), q, PL(q_pl), d, PL(d_pl), ch, PL(ch_pl), cmask,
+                           PL(cmask_pl), Qprev, Qprev_pl, Qpost, PL(Qpost_pl));
+    const size_t R = c->d.nreg() * 4, P = c->d.npl() * 4;
+    std::vector<HostArg> a = {{q_a, R * 6, true},       {PL(q_a_pl), P, true},   {q, R, false},
+                              {PL(q_pl), P, false},     {d, R, false},           {PL(d_pl), P, false},
+                              {ch, R * 6, false},       {PL(ch_pl), P, false},   {cmask, R * 6, false},
+                              {PL(cmask_pl), P, false}, {Qprev, R * 2, true},    {Qprev_pl, P * 2, true},
+                              {Qpost, R * 2, false},    {PL(Qpost_pl), P * 2, false}};
+    std::vector<void *> v;
+    NICAM_TRY(stage_in(c, a, v));
+    auto D = [&](int i) { return static_cast<float *>(v[i]); };
+    NICAM_TRY(launch_hlim_f32(c, D(0), D(1), D(2), D(3), D(4), D(5), D(6), D(7), D(8), D(9), D(10), D(11),
+                          D(12), D(13)));
+    return stage_out(c, a, v);
+}
+
+int nicam_vertical_limiter_thuburn_f32(nicam_ctx c, float *q_h, float *q_h_pl, const float *q,
+                                   const float *q_pl, const float *d, const float *d_pl,
+                                   const float *ck, const float *ck_pl)
+{
+    NICAM_TRY(check_ctx(c));
+    const void *req[] = {q_h, q, d, ck};
+    const char *nm[] = {"q_h", "q", "d", "ck"};
+    for (int i = 0; i < 4; ++i) NICAM_TRY(need(req[i], nm[i]));
+    if (c->d.have_pl) {
+        const void *rp[] = {q_h_pl, q_pl, d_pl, ck_pl};
+        for (int i = 0; i < 4; ++i) NICAM_TRY(need(rp[i], "pole array"));
+    }
+    if (c->ptr_mode == NICAM_PTR_DEVICE)
+        return launch_vlim_f32(c, q_h, PL(q_h_pl), q, PL(q_pl), d, PL(d_pl), ck, PL(ck_pl));
+    const size_t R = c->d.nreg() * 4, P = c->d.npl() * 4;
+    std::vector<HostArg> a = {{q_h, R, true},  {PL(q_h_pl), P, true}, {q, R, false},      {PL(q_pl), P, false},
+                              {d, R, false},   {PL(d_pl), P, false},  {ck, R * 2, false}, {PL(ck_pl), P * 2, false}};
+    std::vector<void *> v;
+    NICAM_TRY(stage_in(c, a, v));
+    auto D = [&](int i) { return static_cast<float *>(v[i]); };
+    NICAM_TRY(launch_vlim_f32(c, D(0), D(1), D(2), D(3), D(4), D(5), D(6), D(7)));
+    return stage_out(c, a, v);
 }
 
 #undef PL

@@ -37,74 +37,78 @@ constexpr int NTP = 2;      // TI/TJ points per work-item: CHUNK*NTP >= CHUNK + 
 #define NICAM_HFLUX_XR_GLOBAL 0  // 1: arc midpoints read per level from L2 rather than held (no fewer registers)
 #endif
 
+template <class T>
 struct HfluxArgs {
-    double *flx;         // (gall,kall,lall,6)
-    double *xc;          // (gall,kall,lall,3,3)
-    const double *f[4];  // rho, rhovx, rhovy, rhovz (gall,kall,lall)
-    const double *xr, *gp, *gt, *ga;
+    T *flx;         // (gall,kall,lall,6)
+    T *xc;          // (gall,kall,lall,3,3)
+    const T *f[4];  // rho, rhovx, rhovy, rhovz (gall,kall,lall)
+    const T *xr, *gp, *gt, *ga;
     const int *have_sgp;
     int iall, gall, kall, lall;
     int nchunk, nkr, kper;  // chunks per slab, level ranges, levels per range
     int nmain;              // work-groups of the regular regions; the rest do the poles
-    double dt, eps;
+    T dt, eps;
     // pole regions (FLX:326-374)
-    double *flx_pl, *xc_pl;
-    const double *f_pl[4];
-    const double *xr_pl, *gp_pl, *gt_pl, *ga_pl;
+    T *flx_pl, *xc_pl;
+    const T *f_pl[4];
+    const T *xr_pl, *gp_pl, *gt_pl, *ga_pl;
     int gall_pl;
 };
 
-__device__ __forceinline__ double dmax(double a, double b) { return (b > a) ? b : a; }
+template <class T>
+__device__ __forceinline__ T dmax(T a, T b) { return (b > a) ? b : a; }
 
 // Pole regions: one work-item per (k, l) walks the ring (FLX:326-374).  Ring
 // point v takes the TI value of v-1 (of the last point for v = 1), which is
 // carried from the previous step instead of stored: no per-lane array.
-__device__ void hflux_pole(const HfluxArgs &A, int t)
+template <class T>
+__device__ void hflux_pole(const HfluxArgs<T> &A, int t)
 {
     const int kall = A.kall, npl = A.gall_pl;
     if (t >= kall * kLallPl) return;
     const int k = t % kall, l = t / kall;
     const size_t fb = ((size_t)l * kall + k) * npl;
     // GMTR_*_pl(v,1,l,c): index (c*2 + l)*gall_pl + v
-    auto M = [&](const double *m, int c, int v) { return m[((size_t)c * kLallPl + l) * npl + v]; };
-    auto tvp = [&](int v, double (&tv)[4]) {
+    auto M = [&](const T *m, int c, int v) { return m[((size_t)c * kLallPl + l) * npl + v]; };
+    auto tvp = [&](int v, T (&tv)[4]) {
         const int vp1 = (v + 1 == npl) ? 1 : v + 1;
-        const double w1 = M(A.gt_pl, 2, v), w2 = M(A.gt_pl, 3, v), w3 = M(A.gt_pl, 4, v);
+        const T w1 = M(A.gt_pl, 2, v), w2 = M(A.gt_pl, 3, v), w3 = M(A.gt_pl, 4, v);
 #pragma unroll
         for (int f = 0; f < 4; ++f)
             tv[f] = A.f_pl[f][fb + 0] * w1 + A.f_pl[f][fb + v] * w2 + A.f_pl[f][fb + vp1] * w3;
     };
-    double prev[4], cur[4];
+    T prev[4], cur[4];
     tvp(npl - 1, prev);
-    const double rarea = M(A.gp_pl, 1, 0);
+    const T rarea = M(A.gp_pl, 1, 0);
     const size_t plane = (size_t)kLallPl * kall * npl;
     for (int v = 1; v < npl; ++v) {
         tvp(v, cur);
-        const double rr2 = 1.0 / dmax(prev[0] + cur[0], A.eps);
-        const double x2 = prev[1] + cur[1];
-        const double y2 = prev[2] + cur[2];
-        const double z2 = prev[3] + cur[3];
-        const double flux = 0.5 * (x2 * M(A.ga_pl, 0, v) + y2 * M(A.ga_pl, 1, v) + z2 * M(A.ga_pl, 2, v));
+        const T rr2 = T(1) / dmax(prev[0] + cur[0], A.eps);
+        const T x2 = prev[1] + cur[1];
+        const T y2 = prev[2] + cur[2];
+        const T z2 = prev[3] + cur[3];
+        const T flux = T(0.5) * (x2 * M(A.ga_pl, 0, v) + y2 * M(A.ga_pl, 1, v) + z2 * M(A.ga_pl, 2, v));
         A.flx_pl[fb + v] = flux * rarea * A.dt;
-        A.xc_pl[0 * plane + fb + v] = M(A.xr_pl, 0, v) - x2 * rr2 * A.dt * 0.5;
-        A.xc_pl[1 * plane + fb + v] = M(A.xr_pl, 1, v) - y2 * rr2 * A.dt * 0.5;
-        A.xc_pl[2 * plane + fb + v] = M(A.xr_pl, 2, v) - z2 * rr2 * A.dt * 0.5;
+        A.xc_pl[0 * plane + fb + v] = M(A.xr_pl, 0, v) - x2 * rr2 * A.dt * T(0.5);
+        A.xc_pl[1 * plane + fb + v] = M(A.xr_pl, 1, v) - y2 * rr2 * A.dt * T(0.5);
+        A.xc_pl[2 * plane + fb + v] = M(A.xr_pl, 2, v) - z2 * rr2 * A.dt * T(0.5);
 #pragma unroll
         for (int f = 0; f < 4; ++f) prev[f] = cur[f];
     }
 }
 
-template <int NST>
-__global__ void __launch_bounds__(CHUNK) hflux_kernel(HfluxArgs A)
+template <int NST, class T>
+__global__ void __launch_bounds__(CHUNK) hflux_kernel(HfluxArgs<T> A)
 {
-    extern __shared__ double lds[];
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    T *lds = reinterpret_cast<T *>(lds_raw);
     const int iall = A.iall, gall = A.gall, kall = A.kall, lall = A.lall;
     const int n_in = CHUNK + 2 * (iall + 1);
     const int n_t = CHUNK + iall + 1;
-    double *s_in = lds;             // [4][n_in]
-    double *s_t = lds + 4 * n_in;   // [8][n_t]: TI fields 0..3, TJ fields 4..7
+    T *s_in = lds;             // [4][n_in]
+    T *s_t = lds + 4 * n_in;   // [8][n_t]: TI fields 0..3, TJ fields 4..7
 #if NICAM_HFLUX_WT_LDS
-    double *s_w = s_t + 8 * n_t;    // [6][n_t]: GMTR_t W1..W3 of TI, then of TJ
+    T *s_w = s_t + 8 * n_t;    // [6][n_t]: GMTR_t W1..W3 of TI, then of TJ
 #endif
 
     // Logical block -> (region, level range, chunk); the chunks of one region
@@ -134,12 +138,12 @@ __global__ void __launch_bounds__(CHUNK) hflux_kernel(HfluxArgs A)
     const size_t mb = (size_t)l * gall;
     const size_t mstr = (size_t)lall * gall;  // between metric planes
     const bool sgp = A.have_sgp[l] != 0;
-    const double dt = A.dt, eps = A.eps;
+    const T dt = A.dt, eps = A.eps;
 
     // ---- K-invariant metrics into registers ----
     // triangle weights of my TI/TJ points (GMTR_t(g,1,l,t,w): ((w*2+t)*lall+l)*gall+g)
 #if !NICAM_HFLUX_WT_LDS
-    double wt[NTP][6];
+    T wt[NTP][6];
 #endif
     int tp[NTP];
 #pragma unroll
@@ -151,7 +155,7 @@ __global__ void __launch_bounds__(CHUNK) hflux_kernel(HfluxArgs A)
 #pragma unroll
         for (int q = 0; q < 6; ++q) {
             const int w = 2 + (q % 3), t = q / 3;
-            const double v = tp[r] >= 0 ? A.gt[(size_t)(w * 2 + t) * mstr + mb + p] : 0.0;
+            const T v = tp[r] >= 0 ? A.gt[(size_t)(w * 2 + t) * mstr + mb + p] : T(0);
 #if NICAM_HFLUX_WT_LDS
             if (s < n_t) s_w[q * n_t + s] = v;  // read only by this work-item
 #else
@@ -171,11 +175,11 @@ __global__ void __launch_bounds__(CHUNK) hflux_kernel(HfluxArgs A)
     const bool a_f5 = active && i >= 1 && j >= 1 && i - 1 <= gmx && j - 1 <= gmx;
     const bool a_f6 = active && i >= 1 && i <= gmx && j >= 1 && j - 1 <= gmx;
     // GMTR_a(g,1,l,a,c): ((c*3+a)*lall+l)*gall+g; GRD_xr(g,1,l,a,d): ((d*3+a)*lall+l)*gall+g
-    double hn[6][3], rarea = 0.0;
+    T hn[6][3], rarea = T(0);
 #if NICAM_HFLUX_XR_GLOBAL
 #define XR(a, d) A.xr[(size_t)((d) * 3 + (a)) * mstr + mb + g]
 #else
-    double xr[3][3];
+    T xr[3][3];
 #define XR(a, d) xr[a][d]
 #endif
     {
@@ -185,20 +189,20 @@ __global__ void __launch_bounds__(CHUNK) hflux_kernel(HfluxArgs A)
         for (int a = 0; a < 6; ++a)
 #pragma unroll
             for (int c = 0; c < 3; ++c)
-                hn[a][c] = have[a] ? A.ga[(size_t)(c * 3 + (a % 3)) * mstr + mb + org[a]] : 0.0;
+                hn[a][c] = have[a] ? A.ga[(size_t)(c * 3 + (a % 3)) * mstr + mb + org[a]] : T(0);
 #pragma unroll
         for (int a = 0; a < 3; ++a)
 #pragma unroll
             for (int d = 0; d < 3; ++d) {
 #if !NICAM_HFLUX_XR_GLOBAL
-                xr[a][d] = active ? A.xr[(size_t)(d * 3 + a) * mstr + mb + g] : 0.0;
+                xr[a][d] = active ? A.xr[(size_t)(d * 3 + a) * mstr + mb + g] : T(0);
 #endif
             }
         if (active) rarea = A.gp[1 * mstr + mb + g];
     }
 
     // ---- level loop ----
-    double nxt[NST][4];
+    T nxt[NST][4];
     auto load_level = [&](int k) {
         const size_t fb = ((size_t)l * kall + k) * slab;
 #pragma unroll
@@ -207,7 +211,7 @@ __global__ void __launch_bounds__(CHUNK) hflux_kernel(HfluxArgs A)
             const int p = lo + s;
             const bool ok = s < n_in && p >= 0 && p < gall;
 #pragma unroll
-            for (int f = 0; f < 4; ++f) nxt[r][f] = ok ? A.f[f][fb + p] : 0.0;
+            for (int f = 0; f < 4; ++f) nxt[r][f] = ok ? A.f[f][fb + p] : T(0);
         }
     };
     auto store_level = [&]() {
@@ -236,17 +240,17 @@ __global__ void __launch_bounds__(CHUNK) hflux_kernel(HfluxArgs A)
             const int s10 = s + 1, s11 = s + iall + 1, s01 = s + iall;
 #pragma unroll
             for (int f = 0; f < 4; ++f) {
-                const double *in = s_in + f * n_in;
+                const T *in = s_in + f * n_in;
 #if NICAM_HFLUX_WT_LDS
-                const double *w = s_w + s;
-                double ti = in[s] * w[0 * n_t] + in[s10] * w[1 * n_t];
+                const T *w = s_w + s;
+                T ti = in[s] * w[0 * n_t] + in[s10] * w[1 * n_t];
                 ti = ti + in[s11] * w[2 * n_t];
-                double tj = in[s] * w[3 * n_t];
+                T tj = in[s] * w[3 * n_t];
                 tj = tj + in[s11] * w[4 * n_t] + in[s01] * w[5 * n_t];
 #else
-                double ti = in[s] * wt[r][0] + in[s10] * wt[r][1];
+                T ti = in[s] * wt[r][0] + in[s10] * wt[r][1];
                 ti = ti + in[s11] * wt[r][2];
-                double tj = in[s] * wt[r][3];
+                T tj = in[s] * wt[r][3];
                 tj = tj + in[s11] * wt[r][4] + in[s01] * wt[r][5];
 #endif
                 s_t[f * n_t + s] = ti;
@@ -254,13 +258,13 @@ __global__ void __launch_bounds__(CHUNK) hflux_kernel(HfluxArgs A)
             }
             if (sgp && tp[r] == 0) {
                 // FLX:188-201: TI(gmin-1,gmin-1) := TJ(gmin,gmin-1), i.e. TJ of point 1
-                const double w1 = A.gt[(size_t)(2 * 2 + 1) * mstr + mb + 1];
-                const double w2 = A.gt[(size_t)(3 * 2 + 1) * mstr + mb + 1];
-                const double w3 = A.gt[(size_t)(4 * 2 + 1) * mstr + mb + 1];
+                const T w1 = A.gt[(size_t)(2 * 2 + 1) * mstr + mb + 1];
+                const T w2 = A.gt[(size_t)(3 * 2 + 1) * mstr + mb + 1];
+                const T w3 = A.gt[(size_t)(4 * 2 + 1) * mstr + mb + 1];
 #pragma unroll
                 for (int f = 0; f < 4; ++f) {
-                    const double *in = s_in + f * n_in;
-                    double tj = in[s + 1] * w1;
+                    const T *in = s_in + f * n_in;
+                    T tj = in[s + 1] * w1;
                     tj = tj + in[s + 1 + iall + 1] * w2 + in[s + 1 + iall] * w3;
                     s_t[f * n_t + s] = tj;
                 }
@@ -277,24 +281,24 @@ __global__ void __launch_bounds__(CHUNK) hflux_kernel(HfluxArgs A)
             const int s = g - lo;
             const size_t fb = ((size_t)l * kall + k) * slab + g;
             // arc a between TI/TJ points (sa, ta) and (sb, tb); ta/tb = 0 for TI, 4 for TJ
-            auto arc = [&](int sa, int ta, int sb, int tb, const double (&h)[3], double &x2, double &y2, double &z2,
-                           double &rr2) {
-                rr2 = 1.0 / dmax(s_t[ta * n_t + sa] + s_t[tb * n_t + sb], eps);
+            auto arc = [&](int sa, int ta, int sb, int tb, const T (&h)[3], T &x2, T &y2, T &z2,
+                           T &rr2) {
+                rr2 = T(1) / dmax(s_t[ta * n_t + sa] + s_t[tb * n_t + sb], eps);
                 x2 = s_t[(ta + 1) * n_t + sa] + s_t[(tb + 1) * n_t + sb];
                 y2 = s_t[(ta + 2) * n_t + sa] + s_t[(tb + 2) * n_t + sb];
                 z2 = s_t[(ta + 3) * n_t + sa] + s_t[(tb + 3) * n_t + sb];
-                return 0.5 * (x2 * h[0] + y2 * h[1] + z2 * h[2]);
+                return T(0.5) * (x2 * h[0] + y2 * h[1] + z2 * h[2]);
             };
             // each own arc's face and mass centroid are stored as soon as they are formed
             auto own_arc = [&](int m, bool have, int sa, int ta, int sb, int tb) {
-                double f = 0.0, c0 = 0.0, c1 = 0.0, c2 = 0.0;
+                T f = T(0), c0 = T(0), c1 = T(0), c2 = T(0);
                 if (have) {
-                    double x2, y2, z2, rr2;
-                    const double flux = arc(sa, ta, sb, tb, hn[m], x2, y2, z2, rr2);
+                    T x2, y2, z2, rr2;
+                    const T flux = arc(sa, ta, sb, tb, hn[m], x2, y2, z2, rr2);
                     f = flux * rarea * dt;
-                    c0 = XR(m, 0) - x2 * rr2 * dt * 0.5;
-                    c1 = XR(m, 1) - y2 * rr2 * dt * 0.5;
-                    c2 = XR(m, 2) - z2 * rr2 * dt * 0.5;
+                    c0 = XR(m, 0) - x2 * rr2 * dt * T(0.5);
+                    c1 = XR(m, 1) - y2 * rr2 * dt * T(0.5);
+                    c2 = XR(m, 2) - z2 * rr2 * dt * T(0.5);
                 }
                 A.flx[m * plane + fb] = f;
                 A.xc[(size_t)(0 * 3 + m) * plane + fb] = c0;
@@ -304,12 +308,12 @@ __global__ void __launch_bounds__(CHUNK) hflux_kernel(HfluxArgs A)
             own_arc(0, a_ai, s - iall, 4, s, 0);   // AI: TJ(ijm1) + TI(ij)   FLX:241-255
             own_arc(1, a_aij, s, 0, s, 4);         // AIJ: TI(ij) + TJ(ij)    FLX:266-280
             own_arc(2, a_aj, s, 4, s - 1, 0);      // AJ: TJ(ij) + TI(im1j)   FLX:292-306
-            double x2, y2, z2, rr2;
-            double f4 = 0.0, f5 = 0.0, f6 = 0.0;
+            T x2, y2, z2, rr2;
+            T f4 = T(0), f5 = T(0), f6 = T(0);
             if (a_f4) f4 = -arc(s - 1 - iall, 4, s - 1, 0, hn[3], x2, y2, z2, rr2) * rarea * dt;
             if (a_f5) f5 = -arc(s - 1 - iall, 0, s - 1 - iall, 4, hn[4], x2, y2, z2, rr2) * rarea * dt;
             if (a_f6) f6 = -arc(s - iall, 4, s - iall - 1, 0, hn[5], x2, y2, z2, rr2) * rarea * dt;
-            if (sgp && i == 1 && j == 1) f6 = 0.0;  // FLX:311-320
+            if (sgp && i == 1 && j == 1) f6 = T(0);  // FLX:311-320
             A.flx[3 * plane + fb] = f4;
             A.flx[4 * plane + fb] = f5;
             A.flx[5 * plane + fb] = f6;
@@ -319,33 +323,31 @@ __global__ void __launch_bounds__(CHUNK) hflux_kernel(HfluxArgs A)
 #undef XR
 }
 
-}  // namespace
-
-int launch_hflux(nicam_ctx c, double *flx_h, double *flx_h_pl, double *xc, double *xc_pl,
-                 const double *rho, const double *rho_pl, const double *rvx, const double *rvx_pl,
-                 const double *rvy, const double *rvy_pl, const double *rvz, const double *rvz_pl,
-                 double dt)
+template <class T>
+int launch_hflux_t(nicam_ctx c, T *flx_h, T *flx_h_pl, T *xc, T *xc_pl, const T *rho, const T *rho_pl,
+                   const T *rvx, const T *rvx_pl, const T *rvy, const T *rvy_pl, const T *rvz, const T *rvz_pl,
+                   T dt, T eps, const HMetrics<T> &M)
 {
     const Dims &D = c->d;
     if (D.gall_1d > MAXI) return fail(NICAM_ERR_ARG, "gall_1d larger than the supported 160");
-    HfluxArgs A;
+    HfluxArgs<T> A;
     A.flx = flx_h;
     A.xc = xc;
     A.f[0] = rho;
     A.f[1] = rvx;
     A.f[2] = rvy;
     A.f[3] = rvz;
-    A.xr = c->GRD_xr;
-    A.gp = c->GMTR_p;
-    A.gt = c->GMTR_t;
-    A.ga = c->GMTR_a;
+    A.xr = M.xr;
+    A.gp = M.gp;
+    A.gt = M.gt;
+    A.ga = M.ga;
     A.have_sgp = c->have_sgp;
     A.iall = D.gall_1d;
     A.gall = D.gall;
     A.kall = D.kall;
     A.lall = D.lall;
     A.dt = dt;
-    A.eps = c->eps;
+    A.eps = eps;
     A.nchunk = (D.gall + CHUNK - 1) / CHUNK;
     // enough work-groups to fill 256 CUs several times over
     const long base = (long)A.nchunk * D.lall;
@@ -364,22 +366,44 @@ int launch_hflux(nicam_ctx c, double *flx_h, double *flx_h_pl, double *xc, doubl
     A.f_pl[1] = rvx_pl;
     A.f_pl[2] = rvy_pl;
     A.f_pl[3] = rvz_pl;
-    A.xr_pl = c->GRD_xr_pl;
-    A.gp_pl = c->GMTR_p_pl;
-    A.gt_pl = c->GMTR_t_pl;
-    A.ga_pl = c->GMTR_a_pl;
+    A.xr_pl = M.xr_pl;
+    A.gp_pl = M.gp_pl;
+    A.gt_pl = M.gt_pl;
+    A.ga_pl = M.ga_pl;
     A.gall_pl = D.gall_pl;
     const size_t n_t = CHUNK + D.gall_1d + 1;
-    const size_t lds = sizeof(double) * (4 * (CHUNK + 2 * (D.gall_1d + 1)) + 8 * n_t + (NICAM_HFLUX_WT_LDS ? 6 * n_t : 0));
+    const size_t lds = sizeof(T) * (4 * (CHUNK + 2 * (D.gall_1d + 1)) + 8 * n_t + (NICAM_HFLUX_WT_LDS ? 6 * n_t : 0));
     const bool two = CHUNK * 2 >= CHUNK + 2 * (D.gall_1d + 1);  // two staged cells per work-item suffice
-    const void *fn = two ? (const void *)hflux_kernel<2> : (const void *)hflux_kernel<3>;
+    const void *fn = two ? (const void *)hflux_kernel<2, T> : (const void *)hflux_kernel<3, T>;
     if (lds > 64 * 1024) NICAM_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     if (two)
-        hipLaunchKernelGGL(hflux_kernel<2>, dim3((unsigned)(nmain + npole)), dim3(CHUNK), lds, c->stream, A);
+        hipLaunchKernelGGL((hflux_kernel<2, T>), dim3((unsigned)(nmain + npole)), dim3(CHUNK), lds, c->stream, A);
     else
-        hipLaunchKernelGGL(hflux_kernel<3>, dim3((unsigned)(nmain + npole)), dim3(CHUNK), lds, c->stream, A);
+        hipLaunchKernelGGL((hflux_kernel<3, T>), dim3((unsigned)(nmain + npole)), dim3(CHUNK), lds, c->stream, A);
     NICAM_HIP(hipGetLastError());
     return NICAM_OK;
+}
+
+}  // namespace
+
+int launch_hflux(nicam_ctx c, double *flx_h, double *flx_h_pl, double *xc, double *xc_pl,
+                 const double *rho, const double *rho_pl, const double *rvx, const double *rvx_pl,
+                 const double *rvy, const double *rvy_pl, const double *rvz, const double *rvz_pl,
+                 double dt)
+{
+    const HMetrics<double> M{c->GRD_xr, c->GMTR_p, c->GMTR_t, c->GMTR_a,
+                             c->GRD_xr_pl, c->GMTR_p_pl, c->GMTR_t_pl, c->GMTR_a_pl};
+    return launch_hflux_t<double>(c, flx_h, flx_h_pl, xc, xc_pl, rho, rho_pl, rvx, rvx_pl, rvy, rvy_pl, rvz, rvz_pl,
+                                  dt, c->eps, M);
+}
+
+int launch_hflux_f32(nicam_ctx c, float *flx_h, float *flx_h_pl, float *xc, float *xc_pl, const float *rho,
+                     const float *rho_pl, const float *rvx, const float *rvx_pl, const float *rvy,
+                     const float *rvy_pl, const float *rvz, const float *rvz_pl, float dt)
+{
+    // CONST_EPS of the reference built with -DSINGLE: epsilon(0._SP)
+    return launch_hflux_t<float>(c, flx_h, flx_h_pl, xc, xc_pl, rho, rho_pl, rvx, rvx_pl, rvy, rvy_pl, rvz, rvz_pl,
+                                 dt, kEpsF32, c->hm32);
 }
 
 }  // namespace nicam

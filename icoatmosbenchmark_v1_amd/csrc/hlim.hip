@@ -31,7 +31,8 @@ constexpr int CHUNK = NICAM_HLIM_CHUNK;  // cells (work-items) per work-group
 constexpr int MAXI = 160;
 
 // A load of a streamed input read once (optionally with the nontemporal hint).
-__device__ __forceinline__ double ld_once(const double *p)
+template <class T>
+__device__ __forceinline__ T ld_once(const T *p)
 {
 #if NICAM_HLIM_NTLOAD
     return __builtin_nontemporal_load(p);
@@ -40,35 +41,43 @@ __device__ __forceinline__ double ld_once(const double *p)
 #endif
 }
 
-__device__ __forceinline__ double dmin(double a, double b) { return (b < a) ? b : a; }
-__device__ __forceinline__ double dmax(double a, double b) { return (b > a) ? b : a; }
+template <class T>
+__device__ __forceinline__ T dmin(T a, T b) { return (b < a) ? b : a; }
+template <class T>
+__device__ __forceinline__ T dmax(T a, T b) { return (b > a) ? b : a; }
 // Fortran SIGN(0.5, x): 0.5 with the sign bit of x
 __device__ __forceinline__ double sign_half(double x) { return copysign(0.5, x); }
+__device__ __forceinline__ float sign_half(float x) { return copysignf(0.5f, x); }
+__device__ __forceinline__ double tabs(double x) { return fabs(x); }
+__device__ __forceinline__ float tabs(float x) { return fabsf(x); }
 
+template <class T>
 struct HlimArgs {
-    double *q_a;          // (gall,kall,lall,6) inout
-    const double *q, *d;  // (gall,kall,lall)
-    const double *ch, *cmask;  // (gall,kall,lall,6)
-    double *Qprev;        // (gall,kall,lall,2)
-    const double *Qpost;  // (gall,kall,lall,2)
+    T *q_a;          // (gall,kall,lall,6) inout
+    const T *q, *d;  // (gall,kall,lall)
+    const T *ch, *cmask;  // (gall,kall,lall,6)
+    T *Qprev;        // (gall,kall,lall,2)
+    const T *Qpost;  // (gall,kall,lall,2)
     const int *have_sgp;
     int iall, gall, kall, lall;
     int phases;  // 1: allowable range only, 2: clamp only, 3: both
     int nmain;   // work-groups of the regular regions; the rest do the poles
-    double eps, big;
+    T eps, big;
     // pole regions (HLIM:307-363, 367-373, 412-425)
-    double *q_a_pl, *Qprev_pl;
-    const double *q_pl, *d_pl, *ch_pl, *cmask_pl, *Qpost_pl;
+    T *q_a_pl, *Qprev_pl;
+    const T *q_pl, *d_pl, *ch_pl, *cmask_pl, *Qpost_pl;
     int gall_pl, have_pl;
 };
 
-__device__ void hlim_pole(const HlimArgs &A, int t);
+template <class T>
+__device__ void hlim_pole(const HlimArgs<T> &A, int t);
 
-__global__ void __launch_bounds__(CHUNK) hlim_kernel(HlimArgs A)
+template <class T>
+__global__ void __launch_bounds__(CHUNK) hlim_kernel(HlimArgs<T> A)
 {
     // q for cells [g0 - iall - 1, g0 + CHUNK + iall + 2)
     constexpr int SPAN = CHUNK + 2 * MAXI + 3;
-    __shared__ double s_q[SPAN];
+    __shared__ T s_q[SPAN];
 
     const int iall = A.iall, gall = A.gall, kall = A.kall, lall = A.lall;
     const int nchunk = (gall + CHUNK - 1) / CHUNK;
@@ -92,7 +101,7 @@ __global__ void __launch_bounds__(CHUNK) hlim_kernel(HlimArgs A)
     const int tid = threadIdx.x;
     for (int s = tid; s < n_in; s += CHUNK) {
         const int g = lo + s;
-        s_q[s] = (g >= 0 && g < gall) ? A.q[fb + g] : 0.0;
+        s_q[s] = (g >= 0 && g < gall) ? A.q[fb + g] : T(0);
     }
     __syncthreads();
 
@@ -101,7 +110,7 @@ __global__ void __launch_bounds__(CHUNK) hlim_kernel(HlimArgs A)
     const int i = g % iall, j = g / iall;
     const bool sgp = A.have_sgp[l] != 0;
     const size_t plane = (size_t)lall * kall * gall;
-    const double BIG = A.big;
+    const T BIG = A.big;
 
     // tracer at absolute cell index p (must lie in the staged window)
     auto Q = [&](int p) { return s_q[p - lo]; };
@@ -109,8 +118,8 @@ __global__ void __launch_bounds__(CHUNK) hlim_kernel(HlimArgs A)
     // min/max of the 4 tracer values around arc m (0=AI,1=AIJ,2=AJ) with
     // origin p; HLIM:145-160 with the index clamps of :152-153 and the
     // singular-point stencil of :197-203.
-    auto arc_minmax = [&](int m, int p, double &qmn, double &qmx) {
-        double a, b2, c2, d2;
+    auto arc_minmax = [&](int m, int p, T &qmn, T &qmx) {
+        T a, b2, c2, d2;
         if (m == 0) {
             const int ijm1 = max(p - iall, 0);
             a = Q(p); b2 = Q(ijm1); c2 = Q(p + 1); d2 = Q(p + iall + 1);
@@ -130,7 +139,7 @@ __global__ void __launch_bounds__(CHUNK) hlim_kernel(HlimArgs A)
 
     // own arcs, origin g (valid for i,j <= iall-2)
     const bool own = (i <= iall - 2 && j <= iall - 2);
-    double omn[3], omx[3], ocm[3];
+    T omn[3], omx[3], ocm[3];
     if (own) {
 #pragma unroll
         for (int m = 0; m < 3; ++m) {
@@ -140,50 +149,50 @@ __global__ void __launch_bounds__(CHUNK) hlim_kernel(HlimArgs A)
     }
 
     // ---- phase 1: allowable range of the cell (HLIM:216-301) ----
-    double qout_min, qout_max;
-    const double qc = Q(g);
+    T qout_min, qout_max;
+    const T qc = Q(g);
     if ((A.phases & 1) && i >= 1 && i <= iall - 2 && j >= 1 && j <= iall - 2) {
-        double qinmn[6], qinmx[6];
+        T qinmn[6], qinmx[6];
 #pragma unroll
         for (int m = 0; m < 3; ++m) {  // faces 1..3: own side of own arcs (:162-187)
-            const double c = ocm[m];
-            qinmn[m] = (c)*omn[m] + (1.0 - c) * BIG;
-            qinmx[m] = (c)*omx[m] + (1.0 - c) * (-BIG);
+            const T c = ocm[m];
+            qinmn[m] = (c)*omn[m] + (T(1) - c) * BIG;
+            qinmx[m] = (c)*omx[m] + (T(1) - c) * (-BIG);
         }
         // faces 4..6: neighbour side of the arcs at (i-1,j), (i-1,j-1), (i,j-1)
         const int org[3] = {g - 1, g - 1 - iall, g - iall};
 #pragma unroll
         for (int m = 0; m < 3; ++m) {
-            double mn, mx;
+            T mn, mx;
             arc_minmax(m, org[m], mn, mx);
-            const double c = A.cmask[(size_t)m * plane + fb + org[m]];
-            qinmn[m + 3] = (c)*BIG + (1.0 - c) * mn;
-            qinmx[m + 3] = (c) * (-BIG) + (1.0 - c) * mx;
+            const T c = A.cmask[(size_t)m * plane + fb + org[m]];
+            qinmn[m + 3] = (c)*BIG + (T(1) - c) * mn;
+            qinmx[m + 3] = (c) * (-BIG) + (T(1) - c) * mx;
         }
-        double qnext_min = qc, qnext_max = qc;
+        T qnext_min = qc, qnext_max = qc;
 #pragma unroll
         for (int f = 0; f < 6; ++f) qnext_min = dmin(qnext_min, qinmn[f]);
 #pragma unroll
         for (int f = 0; f < 6; ++f) qnext_max = dmax(qnext_max, qinmx[f]);
-        double chv[6], chm[6];
+        T chv[6], chm[6];
 #pragma unroll
         for (int f = 0; f < 6; ++f) {
             chv[f] = ld_once(&A.ch[(size_t)f * plane + fb + g]);
-            chm[f] = dmin(chv[f], 0.0);
+            chm[f] = dmin(chv[f], T(0));
         }
-        const double Cin_sum = chm[0] + chm[1] + chm[2] + chm[3] + chm[4] + chm[5];
-        const double Cout_sum = chv[0] - chm[0] + chv[1] - chm[1] + chv[2] - chm[2] + chv[3] - chm[3] +
+        const T Cin_sum = chm[0] + chm[1] + chm[2] + chm[3] + chm[4] + chm[5];
+        const T Cout_sum = chv[0] - chm[0] + chv[1] - chm[1] + chv[2] - chm[2] + chv[3] - chm[3] +
                                 chv[4] - chm[4] + chv[5] - chm[5];
-        const double CQin_min_sum = chm[0] * qinmn[0] + chm[1] * qinmn[1] + chm[2] * qinmn[2] +
+        const T CQin_min_sum = chm[0] * qinmn[0] + chm[1] * qinmn[1] + chm[2] * qinmn[2] +
                                     chm[3] * qinmn[3] + chm[4] * qinmn[4] + chm[5] * qinmn[5];
-        const double CQin_max_sum = chm[0] * qinmx[0] + chm[1] * qinmx[1] + chm[2] * qinmx[2] +
+        const T CQin_max_sum = chm[0] * qinmx[0] + chm[1] * qinmx[1] + chm[2] * qinmx[2] +
                                     chm[3] * qinmx[3] + chm[4] * qinmx[4] + chm[5] * qinmx[5];
-        const double zerosw = 0.5 - sign_half(fabs(Cout_sum) - A.eps);
-        const double dc = ld_once(&A.d[fb + g]);
-        qout_min = (qc - CQin_max_sum - qnext_max * (1.0 - Cin_sum - Cout_sum + dc)) / (Cout_sum + zerosw) *
-                       (1.0 - zerosw) + qc * zerosw;
-        qout_max = (qc - CQin_min_sum - qnext_min * (1.0 - Cin_sum - Cout_sum + dc)) / (Cout_sum + zerosw) *
-                       (1.0 - zerosw) + qc * zerosw;
+        const T zerosw = T(0.5) - sign_half(tabs(Cout_sum) - A.eps);
+        const T dc = ld_once(&A.d[fb + g]);
+        qout_min = (qc - CQin_max_sum - qnext_max * (T(1) - Cin_sum - Cout_sum + dc)) / (Cout_sum + zerosw) *
+                       (T(1) - zerosw) + qc * zerosw;
+        qout_max = (qc - CQin_min_sum - qnext_min * (T(1) - Cin_sum - Cout_sum + dc)) / (Cout_sum + zerosw) *
+                       (T(1) - zerosw) + qc * zerosw;
     } else {  // halo (:286-301)
         qout_min = qc;
         qout_max = qc;
@@ -201,18 +210,18 @@ __global__ void __launch_bounds__(CHUNK) hlim_kernel(HlimArgs A)
     // ---- phase 2: clamp the face values of the own arcs (HLIM:377-410) ----
     if (own && (A.phases & 2)) {
         const int nb[3] = {g + 1, g + iall + 1, g + iall};
-        const double qo_min_c = A.Qpost[fb + g], qo_max_c = A.Qpost[plane + fb + g];  // re-read by neighbours
+        const T qo_min_c = A.Qpost[fb + g], qo_max_c = A.Qpost[plane + fb + g];  // re-read by neighbours
 #pragma unroll
         for (int m = 0; m < 3; ++m) {
-            const double c = ocm[m];
-            const double in_mn_s = (c)*omn[m] + (1.0 - c) * BIG;
-            const double in_mx_s = (c)*omx[m] + (1.0 - c) * (-BIG);
-            const double in_mn_n = (c)*BIG + (1.0 - c) * omn[m];
-            const double in_mx_n = (c) * (-BIG) + (1.0 - c) * omx[m];
-            const double qo_min_n = A.Qpost[fb + nb[m]], qo_max_n = A.Qpost[plane + fb + nb[m]];
-            double qa = ld_once(&A.q_a[(size_t)m * plane + fb + g]);
-            qa = (c)*dmin(dmax(qa, in_mn_s), in_mx_s) + (1.0 - c) * dmin(dmax(qa, in_mn_n), in_mx_n);
-            qa = (c)*dmax(dmin(qa, qo_max_n), qo_min_n) + (1.0 - c) * dmax(dmin(qa, qo_max_c), qo_min_c);
+            const T c = ocm[m];
+            const T in_mn_s = (c)*omn[m] + (T(1) - c) * BIG;
+            const T in_mx_s = (c)*omx[m] + (T(1) - c) * (-BIG);
+            const T in_mn_n = (c)*BIG + (T(1) - c) * omn[m];
+            const T in_mx_n = (c) * (-BIG) + (T(1) - c) * omx[m];
+            const T qo_min_n = A.Qpost[fb + nb[m]], qo_max_n = A.Qpost[plane + fb + nb[m]];
+            T qa = ld_once(&A.q_a[(size_t)m * plane + fb + g]);
+            qa = (c)*dmin(dmax(qa, in_mn_s), in_mx_s) + (T(1) - c) * dmin(dmax(qa, in_mn_n), in_mx_n);
+            qa = (c)*dmax(dmin(qa, qo_max_n), qo_min_n) + (T(1) - c) * dmax(dmin(qa, qo_max_c), qo_min_c);
 #if NICAM_HLIM_STORE == 2
             A.q_a[(size_t)m * plane + fb + g] = qa;
 #else
@@ -230,7 +239,8 @@ __global__ void __launch_bounds__(CHUNK) hlim_kernel(HlimArgs A)
 // Pole regions: one work-item per (k, l) (HLIM:307-363, 367-373, 412-425).
 // Qin of ring point v depends only on q around it and its cmask, so it is
 // recomputed in each of the three passes over the ring instead of stored.
-__device__ void hlim_pole(const HlimArgs &A, int t)
+template <class T>
+__device__ void hlim_pole(const HlimArgs<T> &A, int t)
 {
     const int kall = A.kall, npl = A.gall_pl;
     if (t >= kall * kLallPl) return;
@@ -240,79 +250,90 @@ __device__ void hlim_pole(const HlimArgs &A, int t)
     if (!A.have_pl) {  // only the ring zeroing of :367-370 happens
         if (!(A.phases & 1)) return;
         for (int v = 1; v < npl; ++v) {
-            A.Qprev_pl[fb + v] = 0.0;
-            A.Qprev_pl[plane + fb + v] = 0.0;
+            A.Qprev_pl[fb + v] = T(0);
+            A.Qprev_pl[plane + fb + v] = T(0);
         }
         return;
     }
-    const double BIG = A.big;
+    const T BIG = A.big;
     // [min,max][side 1,2] of ring point v
-    auto qin = [&](int v, double &mn1, double &mn2, double &mx1, double &mx2) {
+    auto qin = [&](int v, T &mn1, T &mn2, T &mx1, T &mx2) {
         int vp1 = v + 1, vm1 = v - 1;
         if (vp1 == npl) vp1 = 1;
         if (vm1 == 0) vm1 = npl - 1;
-        const double a = A.q_pl[fb + 0], b = A.q_pl[fb + v], c2 = A.q_pl[fb + vm1], d2 = A.q_pl[fb + vp1];
-        const double mn = dmin(dmin(dmin(a, b), c2), d2);
-        const double mx = dmax(dmax(dmax(a, b), c2), d2);
-        const double c = A.cmask_pl[fb + v];
-        mn1 = (c)*mn + (1.0 - c) * BIG;
-        mn2 = (c)*BIG + (1.0 - c) * mn;
-        mx1 = (c)*mx + (1.0 - c) * (-BIG);
-        mx2 = (c) * (-BIG) + (1.0 - c) * mx;
+        const T a = A.q_pl[fb + 0], b = A.q_pl[fb + v], c2 = A.q_pl[fb + vm1], d2 = A.q_pl[fb + vp1];
+        const T mn = dmin(dmin(dmin(a, b), c2), d2);
+        const T mx = dmax(dmax(dmax(a, b), c2), d2);
+        const T c = A.cmask_pl[fb + v];
+        mn1 = (c)*mn + (T(1) - c) * BIG;
+        mn2 = (c)*BIG + (T(1) - c) * mn;
+        mx1 = (c)*mx + (T(1) - c) * (-BIG);
+        mx2 = (c) * (-BIG) + (T(1) - c) * mx;
     };
-    const double qn = A.q_pl[fb + 0];
-    double qnext_min = qn, qnext_max = qn;
+    const T qn = A.q_pl[fb + 0];
+    T qnext_min = qn, qnext_max = qn;
     for (int v = 1; v < npl; ++v) {
-        double mn1, mn2, mx1, mx2;
+        T mn1, mn2, mx1, mx2;
         qin(v, mn1, mn2, mx1, mx2);
         qnext_min = dmin(qnext_min, mn1);
         qnext_max = dmax(qnext_max, mx1);
     }
-    double Cin = 0.0, Cout = 0.0, CQmax = 0.0, CQmin = 0.0;
+    T Cin = T(0), Cout = T(0), CQmax = T(0), CQmin = T(0);
     for (int v = 1; v < npl; ++v) {
-        double mn1, mn2, mx1, mx2;
+        T mn1, mn2, mx1, mx2;
         qin(v, mn1, mn2, mx1, mx2);
-        const double chm = A.cmask_pl[fb + v] * A.ch_pl[fb + v];
+        const T chm = A.cmask_pl[fb + v] * A.ch_pl[fb + v];
         Cin = Cin + chm;
         Cout = Cout - chm + A.ch_pl[fb + v];
         CQmin = CQmin + chm * mn1;
         CQmax = CQmax + chm * mx1;
     }
-    const double zerosw = 0.5 - sign_half(fabs(Cout) - A.eps);
-    const double dn = A.d_pl[fb + 0];
-    const double qo_min = (qn - CQmax - qnext_max * (1.0 - Cin - Cout + dn)) / (Cout + zerosw) * (1.0 - zerosw) + qn * zerosw;
-    const double qo_max = (qn - CQmin - qnext_min * (1.0 - Cin - Cout + dn)) / (Cout + zerosw) * (1.0 - zerosw) + qn * zerosw;
+    const T zerosw = T(0.5) - sign_half(tabs(Cout) - A.eps);
+    const T dn = A.d_pl[fb + 0];
+    const T qo_min = (qn - CQmax - qnext_max * (T(1) - Cin - Cout + dn)) / (Cout + zerosw) * (T(1) - zerosw) + qn * zerosw;
+    const T qo_max = (qn - CQmin - qnext_min * (T(1) - Cin - Cout + dn)) / (Cout + zerosw) * (T(1) - zerosw) + qn * zerosw;
     if (A.phases & 1) {
         A.Qprev_pl[fb + 0] = qo_min;
         A.Qprev_pl[plane + fb + 0] = qo_max;
         for (int v = 1; v < npl; ++v) {
-            A.Qprev_pl[fb + v] = 0.0;
-            A.Qprev_pl[plane + fb + v] = 0.0;
+            A.Qprev_pl[fb + v] = T(0);
+            A.Qprev_pl[plane + fb + v] = T(0);
         }
     }
     if (!(A.phases & 2)) return;
-    const double pn_min = A.Qpost_pl[fb + 0], pn_max = A.Qpost_pl[plane + fb + 0];
+    const T pn_min = A.Qpost_pl[fb + 0], pn_max = A.Qpost_pl[plane + fb + 0];
     for (int v = 1; v < npl; ++v) {
-        double mn1, mn2, mx1, mx2;
+        T mn1, mn2, mx1, mx2;
         qin(v, mn1, mn2, mx1, mx2);
-        const double c = A.cmask_pl[fb + v];
-        double qa = A.q_a_pl[fb + v];
-        qa = (c)*dmin(dmax(qa, mn1), mx1) + (1.0 - c) * dmin(dmax(qa, mn2), mx2);
-        qa = (c)*dmax(dmin(qa, A.Qpost_pl[plane + fb + v]), A.Qpost_pl[fb + v]) + (1.0 - c) * dmax(dmin(qa, pn_max), pn_min);
+        const T c = A.cmask_pl[fb + v];
+        T qa = A.q_a_pl[fb + v];
+        qa = (c)*dmin(dmax(qa, mn1), mx1) + (T(1) - c) * dmin(dmax(qa, mn2), mx2);
+        qa = (c)*dmax(dmin(qa, A.Qpost_pl[plane + fb + v]), A.Qpost_pl[fb + v]) + (T(1) - c) * dmax(dmin(qa, pn_max), pn_min);
         A.q_a_pl[fb + v] = qa;
     }
 }
 
 }  // namespace
 
-int launch_hlim(nicam_ctx c, double *q_a, double *q_a_pl, const double *q, const double *q_pl,
-                const double *d, const double *d_pl, const double *ch, const double *ch_pl,
-                const double *cmask, const double *cmask_pl, double *Qprev, double *Qprev_pl,
-                const double *Qpost, const double *Qpost_pl)
+namespace {
+
+int halo_fill(nicam_ctx c, double *v, double *v_pl, int kmax, int nvar)
+{
+    return launch_halo_fill(c, v, v_pl, kmax, nvar, false);
+}
+int halo_fill(nicam_ctx c, float *v, float *v_pl, int kmax, int nvar)
+{
+    return launch_halo_fill_f32(c, v, v_pl, kmax, nvar, false);
+}
+
+template <class T>
+int launch_hlim_t(nicam_ctx c, T *q_a, T *q_a_pl, const T *q, const T *q_pl, const T *d, const T *d_pl,
+                  const T *ch, const T *ch_pl, const T *cmask, const T *cmask_pl, T *Qprev, T *Qprev_pl,
+                  const T *Qpost, const T *Qpost_pl, T eps, T big)
 {
     const Dims &D = c->d;
     if (D.gall_1d > MAXI) return fail(NICAM_ERR_ARG, "gall_1d larger than the LDS tiles (160)");
-    HlimArgs A;
+    HlimArgs<T> A;
     A.q_a = q_a;
     A.q = q;
     A.d = d;
@@ -325,8 +346,8 @@ int launch_hlim(nicam_ctx c, double *q_a, double *q_a_pl, const double *q, const
     A.gall = D.gall;
     A.kall = D.kall;
     A.lall = D.lall;
-    A.eps = c->eps;
-    A.big = c->big;
+    A.eps = eps;
+    A.big = big;
     const long nmain = (long)((D.gall + CHUNK - 1) / CHUNK) * D.kall * D.lall;
     const long npole = Qprev_pl ? (D.kall * kLallPl + CHUNK - 1) / CHUNK : 0;
     if (nmain + npole > 0x7fffffffL) return fail(NICAM_ERR_ARG, "grid too large for one launch");
@@ -342,7 +363,7 @@ int launch_hlim(nicam_ctx c, double *q_a, double *q_a_pl, const double *q, const
     A.have_pl = D.have_pl;
     auto run = [&](int phases) -> int {
         A.phases = phases;
-        hipLaunchKernelGGL(hlim_kernel, dim3((unsigned)(nmain + npole)), dim3(CHUNK), 0, c->stream, A);
+        hipLaunchKernelGGL(hlim_kernel<T>, dim3((unsigned)(nmain + npole)), dim3(CHUNK), 0, c->stream, A);
         NICAM_HIP(hipGetLastError());
         return NICAM_OK;
     };
@@ -357,11 +378,32 @@ int launch_hlim(nicam_ctx c, double *q_a, double *q_a_pl, const double *q, const
     if (!Qprev_pl) return fail(NICAM_ERR_ARG, "Qout_prev_pl is needed to exchange Qout with the poles");
     int rc = run(1);
     if (rc != NICAM_OK) return rc;
-    rc = launch_halo_fill(c, Qprev, Qprev_pl, D.kall, 2, false);
+    rc = halo_fill(c, Qprev, Qprev_pl, D.kall, 2);
     if (rc != NICAM_OK) return rc;
     A.Qpost = Qprev;
     A.Qpost_pl = Qprev_pl;
     return run(2);
+}
+
+}  // namespace
+
+int launch_hlim(nicam_ctx c, double *q_a, double *q_a_pl, const double *q, const double *q_pl,
+                const double *d, const double *d_pl, const double *ch, const double *ch_pl,
+                const double *cmask, const double *cmask_pl, double *Qprev, double *Qprev_pl,
+                const double *Qpost, const double *Qpost_pl)
+{
+    return launch_hlim_t<double>(c, q_a, q_a_pl, q, q_pl, d, d_pl, ch, ch_pl, cmask, cmask_pl, Qprev, Qprev_pl, Qpost,
+                                 Qpost_pl, c->eps, c->big);
+}
+
+int launch_hlim_f32(nicam_ctx c, float *q_a, float *q_a_pl, const float *q, const float *q_pl, const float *d,
+                    const float *d_pl, const float *ch, const float *ch_pl, const float *cmask,
+                    const float *cmask_pl, float *Qprev, float *Qprev_pl, const float *Qpost,
+                    const float *Qpost_pl)
+{
+    // CONST_EPS and CONST_HUGE of the reference built with -DSINGLE
+    return launch_hlim_t<float>(c, q_a, q_a_pl, q, q_pl, d, d_pl, ch, ch_pl, cmask, cmask_pl, Qprev, Qprev_pl, Qpost,
+                                Qpost_pl, kEpsF32, kHugeF32);
 }
 
 }  // namespace nicam

@@ -44,6 +44,18 @@ struct nicam_tune {
     int ddmp_kper = 0;    // levels per work-group in OPRT3D_divdamp
 };
 
+namespace nicam {
+// horizontal metrics in one precision (device pointers)
+template <class T>
+struct HMetrics {
+    T *xr = nullptr, *gp = nullptr, *gt = nullptr, *ga = nullptr;
+    T *xr_pl = nullptr, *gp_pl = nullptr, *gt_pl = nullptr, *ga_pl = nullptr;
+};
+// CONST_EPS and CONST_HUGE of the reference built with -DSINGLE
+constexpr float kEpsF32 = 1.1920928955078125e-07f;  // epsilon(0._SP) = 2^-23
+constexpr float kHugeF32 = 3.4028234663852886e+38f; // huge(0._SP)
+}  // namespace nicam
+
 struct nicam_ctx_s {
     nicam::Dims d{};
     nicam_tune tune{};
@@ -64,6 +76,7 @@ struct nicam_ctx_s {
     double *GMTR_t = nullptr, *GMTR_t_pl = nullptr;
     double *GMTR_a = nullptr, *GMTR_a_pl = nullptr;
     bool have_hmetrics = false;
+    nicam::HMetrics<float> hm32;  // the same metrics rounded to FP32, for the _f32 entry points
 
     // vertical metrics
     double *RGAM = nullptr, *RGAM_pl = nullptr;
@@ -179,6 +192,10 @@ int launch_vi_matrix(nicam_ctx c, const double *eth, const double *eth_pl, const
                      const double *g_tilde_pl, double dt);
 int launch_vertical_update(nicam_ctx c, double *rhogq, double *rhogq_pl, const double *flx_v, const double *flx_v_pl,
                            const double *q_h, const double *q_h_pl);
+int launch_to_float(nicam_ctx c, const double *src, float *dst, size_t n);
+int launch_hflux_f32(nicam_ctx c, float *flx_h, float *flx_h_pl, float *xc, float *xc_pl, const float *rho,
+                     const float *rho_pl, const float *rvx, const float *rvx_pl, const float *rvy,
+                     const float *rvy_pl, const float *rvz, const float *rvz_pl, float dt);
 int launch_hflux(nicam_ctx c, double *flx_h, double *flx_h_pl, double *xc, double *xc_pl,
                  const double *rho, const double *rho_pl, const double *rvx, const double *rvx_pl,
                  const double *rvy, const double *rvy_pl, const double *rvz, const double *rvz_pl,
@@ -207,6 +224,13 @@ int comm_setup(nicam_ctx c, int rlevel);
 int diamond_corner_classes(int cls[10][4]);
 // var(gall,kmax,lall,nvar), var_pl(gall_pl,kmax,2,nvar); pole_fill: COMM_var
 int launch_halo_fill(nicam_ctx c, double *var, double *var_pl, int kmax, int nvar, bool pole_fill);
+int launch_halo_fill_f32(nicam_ctx c, float *var, float *var_pl, int kmax, int nvar, bool pole_fill);
+int launch_hlim_f32(nicam_ctx c, float *q_a, float *q_a_pl, const float *q, const float *q_pl, const float *d,
+                    const float *d_pl, const float *ch, const float *ch_pl, const float *cmask,
+                    const float *cmask_pl, float *Qprev, float *Qprev_pl, const float *Qpost,
+                    const float *Qpost_pl);
+int launch_vlim_f32(nicam_ctx c, float *q_h, float *q_h_pl, const float *q, const float *q_pl, const float *d,
+                    const float *d_pl, const float *ck, const float *ck_pl);
 
 int launch_diffusion(nicam_ctx c, double *dscl, double *dscl_pl, const double *scl, const double *scl_pl,
                      const double *kh, const double *kh_pl, const double *coef_intp, const double *coef_intp_pl,

@@ -18,51 +18,57 @@ namespace {
 
 constexpr int BLOCK = 256;
 
-__device__ __forceinline__ double dmin(double a, double b) { return (b < a) ? b : a; }
-__device__ __forceinline__ double dmax(double a, double b) { return (b > a) ? b : a; }
+template <class T>
+__device__ __forceinline__ T dmin(T a, T b) { return (b < a) ? b : a; }
+template <class T>
+__device__ __forceinline__ T dmax(T a, T b) { return (b > a) ? b : a; }
 __device__ __forceinline__ double sign_half(double x) { return copysign(0.5, x); }
+__device__ __forceinline__ float sign_half(float x) { return copysignf(0.5f, x); }
+__device__ __forceinline__ double tabs(double x) { return fabs(x); }
+__device__ __forceinline__ float tabs(float x) { return fabsf(x); }
 
+template <class T>
 struct VlimArgs {
-    double *q_h;
-    const double *q, *d, *ck;
-    double *q_h_pl;
-    const double *q_pl, *d_pl, *ck_pl;
+    T *q_h;
+    const T *q, *d, *ck;
+    T *q_h_pl;
+    const T *q_pl, *d_pl, *ck_pl;
     int gall, kall, lall, gall_pl, have_pl;
     int kc, nrun;  // levels per work-item, runs per column
-    double eps, big;
+    T eps, big;
 };
 
 // Levels [kb, ke) of one column are limited (kb >= kmin+1, 0-based).
 // base = offset of level 0, s = stride between levels, ck2 = offset of the
 // upper-face plane of ck.  POLE selects the pole loop's product grouping of
 // CQin (VLIM:212-215 versus :126-129).
-template <bool POLE>
-__device__ __forceinline__ void vlim_run(double *__restrict__ q_h, const double *__restrict__ q,
-                                         const double *__restrict__ d, const double *__restrict__ ck, size_t base,
-                                         size_t s, size_t ck2, int kb, int ke, double EPS, double BIG)
+template <bool POLE, class T>
+__device__ __forceinline__ void vlim_run(T *__restrict__ q_h, const T *__restrict__ q, const T *__restrict__ d,
+                                         const T *__restrict__ ck, size_t base, size_t s, size_t ck2, int kb,
+                                         int ke, T EPS, T BIG)
 {
     int k = kb - 1;  // first level whose Qout is needed
-    double qkm1 = q[base + (size_t)(k - 1) * s];
-    double qk = q[base + (size_t)k * s];
-    double ck1k = ck[base + (size_t)k * s];
-    double Qmin_km1 = 0.0, Qmax_km1 = 0.0;
+    T qkm1 = q[base + (size_t)(k - 1) * s];
+    T qk = q[base + (size_t)k * s];
+    T ck1k = ck[base + (size_t)k * s];
+    T Qmin_km1 = T(0), Qmax_km1 = T(0);
     for (; k < ke; ++k) {
         const size_t o = base + (size_t)k * s;
-        const double qkp1 = q[o + s];
-        const double ck1kp1 = ck[o + s];
-        const double ck2k = ck[ck2 + o];
-        const double dk = d[o];
-        const double inflagL = 0.5 - sign_half(ck1k);
-        const double inflagU = 0.5 + sign_half(ck1kp1);
-        const double Qin_minL = dmin(qk, qkm1) + (1.0 - inflagL) * BIG;
-        const double Qin_minU = dmin(qk, qkp1) + (1.0 - inflagU) * BIG;
-        const double Qin_maxL = dmax(qk, qkm1) - (1.0 - inflagL) * BIG;
-        const double Qin_maxU = dmax(qk, qkp1) - (1.0 - inflagU) * BIG;
-        const double qnext_min = dmin(dmin(Qin_minL, Qin_minU), qk);
-        const double qnext_max = dmax(dmax(Qin_maxL, Qin_maxU), qk);
-        const double Cin = (inflagL)*ck1k + (inflagU)*ck2k;
-        const double Cout = (1.0 - inflagL) * ck1k + (1.0 - inflagU) * ck2k;
-        double CQin_min, CQin_max;
+        const T qkp1 = q[o + s];
+        const T ck1kp1 = ck[o + s];
+        const T ck2k = ck[ck2 + o];
+        const T dk = d[o];
+        const T inflagL = T(0.5) - sign_half(ck1k);
+        const T inflagU = T(0.5) + sign_half(ck1kp1);
+        const T Qin_minL = dmin(qk, qkm1) + (T(1) - inflagL) * BIG;
+        const T Qin_minU = dmin(qk, qkp1) + (T(1) - inflagU) * BIG;
+        const T Qin_maxL = dmax(qk, qkm1) - (T(1) - inflagL) * BIG;
+        const T Qin_maxU = dmax(qk, qkp1) - (T(1) - inflagU) * BIG;
+        const T qnext_min = dmin(dmin(Qin_minL, Qin_minU), qk);
+        const T qnext_max = dmax(dmax(Qin_maxL, Qin_maxU), qk);
+        const T Cin = (inflagL)*ck1k + (inflagU)*ck2k;
+        const T Cout = (T(1) - inflagL) * ck1k + (T(1) - inflagU) * ck2k;
+        T CQin_min, CQin_max;
         if constexpr (POLE) {
             CQin_max = (inflagL) * (ck1k * Qin_maxL) + (inflagU) * (ck2k * Qin_maxU);
             CQin_min = (inflagL) * (ck1k * Qin_minL) + (inflagU) * (ck2k * Qin_minU);
@@ -70,14 +76,14 @@ __device__ __forceinline__ void vlim_run(double *__restrict__ q_h, const double 
             CQin_min = (inflagL)*ck1k * Qin_minL + (inflagU)*ck2k * Qin_minU;
             CQin_max = (inflagL)*ck1k * Qin_maxL + (inflagU)*ck2k * Qin_maxU;
         }
-        const double zerosw = 0.5 - sign_half(fabs(Cout) - EPS);
-        const double Qmin_k = ((qk - qnext_max) + qnext_max * (Cin + Cout - dk) - CQin_max) / (Cout + zerosw) *
-                                  (1.0 - zerosw) + qk * zerosw;
-        const double Qmax_k = ((qk - qnext_min) + qnext_min * (Cin + Cout - dk) - CQin_min) / (Cout + zerosw) *
-                                  (1.0 - zerosw) + qk * zerosw;
+        const T zerosw = T(0.5) - sign_half(tabs(Cout) - EPS);
+        const T Qmin_k = ((qk - qnext_max) + qnext_max * (Cin + Cout - dk) - CQin_max) / (Cout + zerosw) *
+                                  (T(1) - zerosw) + qk * zerosw;
+        const T Qmax_k = ((qk - qnext_min) + qnext_min * (Cin + Cout - dk) - CQin_min) / (Cout + zerosw) *
+                                  (T(1) - zerosw) + qk * zerosw;
         if (k >= kb) {  // VLIM:179-180
-            const double qh = q_h[o];
-            stream_store(&q_h[o], (inflagL)*dmax(dmin(qh, Qmax_km1), Qmin_km1) + (1.0 - inflagL) * dmax(dmin(qh, Qmax_k), Qmin_k));
+            const T qh = q_h[o];
+            stream_store(&q_h[o], (inflagL)*dmax(dmin(qh, Qmax_km1), Qmin_km1) + (T(1) - inflagL) * dmax(dmin(qh, Qmax_k), Qmin_k));
         }
         Qmin_km1 = Qmin_k;
         Qmax_km1 = Qmax_k;
@@ -90,7 +96,8 @@ __device__ __forceinline__ void vlim_run(double *__restrict__ q_h, const double 
 // A work-group is BLOCK consecutive columns at one run of levels.  The runs
 // of one column block are consecutive work-groups on one XCD: each run also
 // reads the two levels below it, which the run below has just brought into L2.
-__global__ void __launch_bounds__(BLOCK) vlim_kernel(VlimArgs A)
+template <class T>
+__global__ void __launch_bounds__(BLOCK) vlim_kernel(VlimArgs<T> A)
 {
     const long ncol_reg = (long)A.gall * A.lall;
     const long ncol = ncol_reg + (A.have_pl ? (long)A.gall_pl * kLallPl : 0);
@@ -107,23 +114,26 @@ __global__ void __launch_bounds__(BLOCK) vlim_kernel(VlimArgs A)
         const int g = (int)(col % A.gall), l = (int)(col / A.gall);
         const size_t base = (size_t)l * A.kall * A.gall + g;
         const size_t ck2 = (size_t)A.gall * A.kall * A.lall;
-        vlim_run<false>(A.q_h, A.q, A.d, A.ck, base, A.gall, ck2, kb, ke, A.eps, A.big);
+        vlim_run<false, T>(A.q_h, A.q, A.d, A.ck, base, A.gall, ck2, kb, ke, A.eps, A.big);
     } else {
         const int p = (int)(col - ncol_reg);
         const int g = p % A.gall_pl, l = p / A.gall_pl;
         const size_t base = (size_t)l * A.kall * A.gall_pl + g;
         const size_t ck2 = (size_t)A.gall_pl * A.kall * kLallPl;
-        vlim_run<true>(A.q_h_pl, A.q_pl, A.d_pl, A.ck_pl, base, A.gall_pl, ck2, kb, ke, A.eps, A.big);
+        vlim_run<true, T>(A.q_h_pl, A.q_pl, A.d_pl, A.ck_pl, base, A.gall_pl, ck2, kb, ke, A.eps, A.big);
     }
 }
 
 }  // namespace
 
-int launch_vlim(nicam_ctx c, double *q_h, double *q_h_pl, const double *q, const double *q_pl,
-                const double *d, const double *d_pl, const double *ck, const double *ck_pl)
+namespace {
+
+template <class T>
+int launch_vlim_t(nicam_ctx c, T *q_h, T *q_h_pl, const T *q, const T *q_pl, const T *d, const T *d_pl,
+                  const T *ck, const T *ck_pl, T eps, T big)
 {
     const Dims &D = c->d;
-    VlimArgs A;
+    VlimArgs<T> A;
     A.q_h = q_h;
     A.q = q;
     A.d = d;
@@ -137,8 +147,8 @@ int launch_vlim(nicam_ctx c, double *q_h, double *q_h_pl, const double *q, const
     A.lall = D.lall;
     A.gall_pl = D.gall_pl;
     A.have_pl = D.have_pl;
-    A.eps = c->eps;
-    A.big = c->big;
+    A.eps = eps;
+    A.big = big;
     const int nlev = D.kall - 3;  // limited levels kmin+1..kmax
     if (nlev <= 0) return NICAM_OK;
     // short runs give more independent work-items (measured best at gl7: 4)
@@ -149,9 +159,24 @@ int launch_vlim(nicam_ctx c, double *q_h, double *q_h_pl, const double *q, const
     const long ncol = (long)D.gall * D.lall + (D.have_pl ? (long)D.gall_pl * kLallPl : 0);
     const long nblk = (ncol + BLOCK - 1) / BLOCK * A.nrun;
     if (nblk > 0x7fffffffL) return fail(NICAM_ERR_ARG, "grid too large for one launch");
-    hipLaunchKernelGGL(vlim_kernel, dim3((unsigned)nblk), dim3(BLOCK), 0, c->stream, A);
+    hipLaunchKernelGGL(vlim_kernel<T>, dim3((unsigned)nblk), dim3(BLOCK), 0, c->stream, A);
     NICAM_HIP(hipGetLastError());
     return NICAM_OK;
+}
+
+}  // namespace
+
+int launch_vlim(nicam_ctx c, double *q_h, double *q_h_pl, const double *q, const double *q_pl,
+                const double *d, const double *d_pl, const double *ck, const double *ck_pl)
+{
+    return launch_vlim_t<double>(c, q_h, q_h_pl, q, q_pl, d, d_pl, ck, ck_pl, c->eps, c->big);
+}
+
+int launch_vlim_f32(nicam_ctx c, float *q_h, float *q_h_pl, const float *q, const float *q_pl, const float *d,
+                    const float *d_pl, const float *ck, const float *ck_pl)
+{
+    // CONST_EPS and CONST_HUGE of the reference built with -DSINGLE
+    return launch_vlim_t<float>(c, q_h, q_h_pl, q, q_pl, d, d_pl, ck, ck_pl, kEpsF32, kHugeF32);
 }
 
 }  // namespace nicam

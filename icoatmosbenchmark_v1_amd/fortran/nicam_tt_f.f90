@@ -303,6 +303,31 @@ module nicam_tt_f
        integer(c_int) :: nicam_vi_rhow_getcoef
      end function nicam_vi_rhow_getcoef
 
+     function nicam_horizontal_flux_f32(ctx, flx_h, flx_h_pl, GRD_xc, GRD_xc_pl, rho, rho_pl, rhovx, rhovx_pl, &
+          rhovy, rhovy_pl, rhovz, rhovz_pl, dt) bind(C, name='nicam_horizontal_flux_f32')
+       import :: c_ptr, c_int, c_float
+       type(c_ptr), value :: ctx, flx_h, flx_h_pl, GRD_xc, GRD_xc_pl, rho, rho_pl
+       type(c_ptr), value :: rhovx, rhovx_pl, rhovy, rhovy_pl, rhovz, rhovz_pl
+       real(c_float), value :: dt
+       integer(c_int) :: nicam_horizontal_flux_f32
+     end function nicam_horizontal_flux_f32
+
+     function nicam_horizontal_limiter_thuburn_f32(ctx, q_a, q_a_pl, q, q_pl, d, d_pl, ch, ch_pl, cmask, &
+          cmask_pl, Qout_prev, Qout_prev_pl, Qout_post, Qout_post_pl) &
+          bind(C, name='nicam_horizontal_limiter_thuburn_f32')
+       import :: c_ptr, c_int
+       type(c_ptr), value :: ctx, q_a, q_a_pl, q, q_pl, d, d_pl, ch, ch_pl, cmask, cmask_pl
+       type(c_ptr), value :: Qout_prev, Qout_prev_pl, Qout_post, Qout_post_pl
+       integer(c_int) :: nicam_horizontal_limiter_thuburn_f32
+     end function nicam_horizontal_limiter_thuburn_f32
+
+     function nicam_vertical_limiter_thuburn_f32(ctx, q_h, q_h_pl, q, q_pl, d, d_pl, ck, ck_pl) &
+          bind(C, name='nicam_vertical_limiter_thuburn_f32')
+       import :: c_ptr, c_int
+       type(c_ptr), value :: ctx, q_h, q_h_pl, q, q_pl, d, d_pl, ck, ck_pl
+       integer(c_int) :: nicam_vertical_limiter_thuburn_f32
+     end function nicam_vertical_limiter_thuburn_f32
+
      function nicam_grid_icosahedral(glevel, rlevel, radius, GRD_x, GRD_x_pl) bind(C, name='nicam_grid_icosahedral')
        import :: c_int, c_double
        integer(c_int), value :: glevel, rlevel

@@ -22,7 +22,7 @@ library raises :class:`LibraryNotBuilt`.
 from __future__ import annotations
 
 import ctypes
-from ctypes import POINTER, c_char_p, c_double, c_int, c_size_t, c_void_p
+from ctypes import POINTER, c_char_p, c_double, c_float, c_int, c_size_t, c_void_p
 from pathlib import Path
 from typing import Optional, Sequence
 
@@ -99,6 +99,9 @@ SIGNATURES = {
     "nicam_vertical_face": (c_int, [_P] * 5),
     "nicam_tracer_update_vertical": (c_int, [_P] * 7),
     "nicam_vi_rhow_update_matrix": (c_int, [_P] * 5 + [c_double]),
+    "nicam_horizontal_flux_f32": (c_int, [_P] * 13 + [c_float]),
+    "nicam_horizontal_limiter_thuburn_f32": (c_int, [_P] * 15),
+    "nicam_vertical_limiter_thuburn_f32": (c_int, [_P] * 9),
     "nicam_vi_rhow_getcoef": (c_int, [_P] * 7),
     "nicam_oprt_diffusion": (c_int, [_P] + [_P] * 10),
     "nicam_set_rdgz": (c_int, [_P, _P]),
@@ -144,15 +147,17 @@ def _check(rc: int, func: str) -> None:
         raise NicamError(rc, func, msg)
 
 
-def _ptr(x, mode: int, name: str, numel: Optional[int] = None):
-    """Raw pointer of a numpy array (host mode) or torch tensor (device mode)."""
+def _ptr(x, mode: int, name: str, numel: Optional[int] = None, single: bool = False):
+    """Raw pointer of a numpy array (host mode) or torch tensor (device mode),
+    float64 (float32 with single=True)."""
     if x is None:
         return None
     if isinstance(x, np.ndarray):
         if mode != PTR_HOST:
             raise TypeError(f"{name}: numpy array passed in device pointer mode")
-        if x.dtype != np.float64 or not x.flags.c_contiguous:
-            raise TypeError(f"{name}: need a C-contiguous float64 array")
+        want = np.float32 if single else np.float64
+        if x.dtype != want or not x.flags.c_contiguous:
+            raise TypeError(f"{name}: need a C-contiguous {np.dtype(want).name} array")
         if numel is not None and x.size != numel:
             raise ValueError(f"{name}: has {x.size} elements, expected {numel}")
         return x.ctypes.data
@@ -165,8 +170,9 @@ def _ptr(x, mode: int, name: str, numel: Optional[int] = None):
         raise TypeError(f"{name}: unsupported array type {type(x)}")
     if mode != PTR_DEVICE or not x.is_cuda:
         raise TypeError(f"{name}: torch tensors must live on the GPU and use device pointer mode")
-    if x.dtype != torch.float64 or not x.is_contiguous():
-        raise TypeError(f"{name}: need a contiguous float64 tensor")
+    want = torch.float32 if single else torch.float64
+    if x.dtype != want or not x.is_contiguous():
+        raise TypeError(f"{name}: need a contiguous {want} tensor")
     if numel is not None and x.numel() != numel:
         raise ValueError(f"{name}: has {x.numel()} elements, expected {numel}")
     return x.data_ptr()
@@ -358,14 +364,14 @@ class Context:
         self._keep.clear()
 
     # -- hot path --------------------------------------------------------------
-    def _args(self, spec):
+    def _args(self, spec, single: bool = False):
         """spec: list of (array, name, numel)."""
         out = []
         for x, nm, n in spec:
             if nm.endswith("_pl") and not self.have_pl and nm != "Qout_prev_pl":
                 out.append(None)
             else:
-                out.append(_ptr(x, self.mode, nm, n))
+                out.append(_ptr(x, self.mode, nm, n, single))
         return out
 
     def horizontal_flux(self, flx_h, flx_h_pl, GRD_xc, GRD_xc_pl, rho, rho_pl, rhovx, rhovx_pl,
@@ -386,6 +392,32 @@ class Context:
                         (Qout_prev_pl, "Qout_prev_pl", 2 * P), (Qout_post, "Qout_post", 2 * R),
                         (Qout_post_pl, "Qout_post_pl", 2 * P)])
         _check(self.lib.nicam_horizontal_limiter_thuburn(self.h, *a), "nicam_horizontal_limiter_thuburn")
+
+    # -- FP32 variants (the reference built with -DSINGLE) ---------------------
+    def horizontal_flux_f32(self, flx_h, flx_h_pl, GRD_xc, GRD_xc_pl, rho, rho_pl, rhovx, rhovx_pl,
+                            rhovy, rhovy_pl, rhovz, rhovz_pl, dt: float) -> None:
+        R, P = self.nreg, self.npl
+        a = self._args([(flx_h, "flx_h", 6 * R), (flx_h_pl, "flx_h_pl", P), (GRD_xc, "GRD_xc", 9 * R),
+                        (GRD_xc_pl, "GRD_xc_pl", 3 * P), (rho, "rho", R), (rho_pl, "rho_pl", P),
+                        (rhovx, "rhovx", R), (rhovx_pl, "rhovx_pl", P), (rhovy, "rhovy", R),
+                        (rhovy_pl, "rhovy_pl", P), (rhovz, "rhovz", R), (rhovz_pl, "rhovz_pl", P)], single=True)
+        _check(self.lib.nicam_horizontal_flux_f32(self.h, *a, float(dt)), "nicam_horizontal_flux_f32")
+
+    def horizontal_limiter_thuburn_f32(self, q_a, q_a_pl, q, q_pl, d, d_pl, ch, ch_pl, cmask, cmask_pl,
+                                       Qout_prev, Qout_prev_pl, Qout_post, Qout_post_pl) -> None:
+        R, P = self.nreg, self.npl
+        a = self._args([(q_a, "q_a", 6 * R), (q_a_pl, "q_a_pl", P), (q, "q", R), (q_pl, "q_pl", P),
+                        (d, "d", R), (d_pl, "d_pl", P), (ch, "ch", 6 * R), (ch_pl, "ch_pl", P),
+                        (cmask, "cmask", 6 * R), (cmask_pl, "cmask_pl", P), (Qout_prev, "Qout_prev", 2 * R),
+                        (Qout_prev_pl, "Qout_prev_pl", 2 * P), (Qout_post, "Qout_post", 2 * R),
+                        (Qout_post_pl, "Qout_post_pl", 2 * P)], single=True)
+        _check(self.lib.nicam_horizontal_limiter_thuburn_f32(self.h, *a), "nicam_horizontal_limiter_thuburn_f32")
+
+    def vertical_limiter_thuburn_f32(self, q_h, q_h_pl, q, q_pl, d, d_pl, ck, ck_pl) -> None:
+        R, P = self.nreg, self.npl
+        a = self._args([(q_h, "q_h", R), (q_h_pl, "q_h_pl", P), (q, "q", R), (q_pl, "q_pl", P), (d, "d", R),
+                        (d_pl, "d_pl", P), (ck, "ck", 2 * R), (ck_pl, "ck_pl", 2 * P)], single=True)
+        _check(self.lib.nicam_vertical_limiter_thuburn_f32(self.h, *a), "nicam_vertical_limiter_thuburn_f32")
 
     def vertical_limiter_thuburn(self, q_h, q_h_pl, q, q_pl, d, d_pl, ck, ck_pl) -> None:
         R, P = self.nreg, self.npl

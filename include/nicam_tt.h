@@ -416,6 +416,26 @@ int nicam_horizontal_remap(nicam_ctx ctx, double *q_a, double *q_a_pl, const dou
 int nicam_tracer_update(nicam_ctx ctx, double *rhogq, double *rhogq_pl, const double *flx_h, const double *flx_h_pl,
                         const double *q_a, const double *q_a_pl);
 
+/* FP32 variants of the flux and the two limiters: the same arguments as the
+ * FP64 entry points above, every array and dt in single precision.  They
+ * reproduce, bit for bit, the reference's own kernels built with -DSINGLE
+ * (mod_precision.f90: RP = SP): the same operations in the same order in
+ * IEEE single precision, CONST_EPS = epsilon(0._SP), CONST_HUGE = huge(0._SP),
+ * and the horizontal metrics of nicam_set_hmetrics rounded to single
+ * precision, as real(x, kind=SP) rounds them.  The FP64 VI solve completes the
+ * mixed-precision step. */
+int nicam_horizontal_flux_f32(nicam_ctx ctx, float *flx_h, float *flx_h_pl, float *GRD_xc, float *GRD_xc_pl,
+                              const float *rho, const float *rho_pl, const float *rhovx, const float *rhovx_pl,
+                              const float *rhovy, const float *rhovy_pl, const float *rhovz, const float *rhovz_pl,
+                              float dt);
+int nicam_horizontal_limiter_thuburn_f32(nicam_ctx ctx, float *q_a, float *q_a_pl, const float *q,
+                                         const float *q_pl, const float *d, const float *d_pl, const float *ch,
+                                         const float *ch_pl, const float *cmask, const float *cmask_pl,
+                                         float *Qout_prev, float *Qout_prev_pl, const float *Qout_post,
+                                         const float *Qout_post_pl);
+int nicam_vertical_limiter_thuburn_f32(nicam_ctx ctx, float *q_h, float *q_h_pl, const float *q, const float *q_pl,
+                                       const float *d, const float *d_pl, const float *ck, const float *ck_pl);
+
 /* The vertical half of the step glue.  No reference counterpart: the
  * reference's vertical limiter takes ck and q_h as inputs
  * (dyn_vert_adv_limiter/src/main.f90), computed by code it does not hold.

@@ -22,6 +22,7 @@ HERE = Path(__file__).resolve().parent
 ORACLE_SO = HERE / "liboracle.so"
 REF_DIR = HERE / "_ref"
 REF_LIBS = {k: REF_DIR / f"libref_{k}.so" for k in ("flux", "hlim", "vlim", "vi", "diff", "ddmp", "gmtr")}
+REF_LIBS_SP = {k: REF_DIR / f"libref_{k}_sp.so" for k in ("flux", "hlim", "vlim")}
 
 
 # inputs of OPRT_diffusion in argument order (dyn_diffusion/src/mod_oprt.f90:41-46)
@@ -497,4 +498,74 @@ class RefLibs:
         _big_stack(self.vi.ref_vi_rhow_solver, spec.gall_1d, spec.kall, spec.lall, spec.gall_pl, int(spec.have_pl),
                                    _p(out["rhogw"]), _p(out["rhogw_pl"]), *a1, float(dt), *a2,
                                    *[x.ctypes.data for x in v])
+        return out
+
+
+def _f32(a) -> np.ndarray:
+    """float32 copy, rounded to nearest as Fortran's real(x, kind=SP) rounds."""
+    return np.ascontiguousarray(a, dtype=np.float64).astype(np.float32)
+
+
+class RefLibsSP:
+    """The reference's flux and limiters built with -DSINGLE (RP = SP): the
+    checkers of the _f32 entry points.  Inputs are rounded to float32 here."""
+
+    @staticmethod
+    def available() -> bool:
+        return all(p.exists() for p in REF_LIBS_SP.values())
+
+    def __init__(self):
+        if not self.available():
+            raise FileNotFoundError("oracle/_ref SINGLE builds missing (make -C oracle ref)")
+        P = c_void_p
+        self.flux = ctypes.CDLL(str(REF_LIBS_SP["flux"]))
+        self.hlim = ctypes.CDLL(str(REF_LIBS_SP["hlim"]))
+        self.vlim = ctypes.CDLL(str(REF_LIBS_SP["vlim"]))
+        self.flux.ref_horizontal_flux.argtypes = [c_int] * 5 + [P] * 13 + [ctypes.c_float] + [P] * 8
+        self.hlim.ref_horizontal_limiter_thuburn.argtypes = [c_int] * 5 + [P] * 15
+        self.vlim.ref_vertical_limiter_thuburn.argtypes = [c_int] * 5 + [P] * 8
+        self.hlim.ref_const_setup.argtypes = [POINTER(c_double), POINTER(c_double)]
+
+    def const_setup(self):
+        """CONST_EPS, CONST_HUGE of the SINGLE build."""
+        e, b = c_double(), c_double()
+        self.hlim.ref_const_setup(ctypes.byref(e), ctypes.byref(b))
+        return e.value, b.value
+
+    def horizontal_flux(self, spec, geo, f, dt, fill=0.0):
+        sh, shp = spec.shape3, spec.shape_pl
+        out = dict(flx_h=np.full((6,) + sh, fill, np.float32), flx_h_pl=np.full(shp, fill, np.float32),
+                   GRD_xc=np.full((3, 3) + sh, fill, np.float32), GRD_xc_pl=np.full((3,) + shp, fill, np.float32))
+        ins = {k: _f32(f[k]) for k in ("rho", "rho_pl", "rhovx", "rhovx_pl", "rhovy", "rhovy_pl", "rhovz", "rhovz_pl")}
+        g = {k: _f32(geo[k]) for k in ("GRD_xr", "GRD_xr_pl", "GMTR_p", "GMTR_p_pl", "GMTR_t", "GMTR_t_pl",
+                                       "GMTR_a", "GMTR_a_pl")}
+        sgp = np.ascontiguousarray(geo["have_sgp"], dtype=np.int32)
+        _big_stack(self.flux.ref_horizontal_flux, spec.gall_1d, spec.kall, spec.lall, spec.gall_pl,
+                   int(spec.have_pl), sgp.ctypes.data, *[_p(out[k]) for k in ("flx_h", "flx_h_pl", "GRD_xc",
+                                                                              "GRD_xc_pl")],
+                   *[_p(ins[k]) for k in ("rho", "rho_pl", "rhovx", "rhovx_pl", "rhovy", "rhovy_pl", "rhovz",
+                                          "rhovz_pl")], float(np.float32(dt)),
+                   *[_p(g[k]) for k in ("GRD_xr", "GRD_xr_pl", "GMTR_p", "GMTR_p_pl", "GMTR_t", "GMTR_t_pl",
+                                        "GMTR_a", "GMTR_a_pl")])
+        return out
+
+    def horizontal_limiter_thuburn(self, spec, have_sgp, f, fill=0.0):
+        out = dict(q_a=_f32(f["q_a"]), q_a_pl=_f32(f["q_a_pl"]),
+                   Qout_prev=np.full((2,) + spec.shape3, fill, np.float32),
+                   Qout_prev_pl=np.full((2,) + spec.shape_pl, fill, np.float32))
+        ins = {k: _f32(f[k]) for k in ("q", "q_pl", "d", "d_pl", "ch", "ch_pl", "cmask", "cmask_pl", "Qout_post",
+                                       "Qout_post_pl")}
+        sgp = np.ascontiguousarray(have_sgp, dtype=np.int32)
+        _big_stack(self.hlim.ref_horizontal_limiter_thuburn, spec.gall_1d, spec.kall, spec.lall, spec.gall_pl,
+                   int(spec.have_pl), sgp.ctypes.data, _p(out["q_a"]), _p(out["q_a_pl"]),
+                   *[_p(ins[k]) for k in ("q", "q_pl", "d", "d_pl", "ch", "ch_pl", "cmask", "cmask_pl")],
+                   _p(out["Qout_prev"]), _p(out["Qout_prev_pl"]), _p(ins["Qout_post"]), _p(ins["Qout_post_pl"]))
+        return out
+
+    def vertical_limiter_thuburn(self, spec, f):
+        out = dict(q_h=_f32(f["q_h"]), q_h_pl=_f32(f["q_h_pl"]))
+        ins = {k: _f32(f[k]) for k in ("q", "q_pl", "d", "d_pl", "ck", "ck_pl")}
+        _big_stack(self.vlim.ref_vertical_limiter_thuburn, spec.gall_1d, spec.kall, spec.lall, spec.gall_pl,
+                   int(spec.have_pl), _p(out["q_h"]), _p(out["q_h_pl"]),
+                   *[_p(ins[k]) for k in ("q", "q_pl", "d", "d_pl", "ck", "ck_pl")])
         return out

@@ -4,6 +4,13 @@
 ! TEST INFRASTRUCTURE ONLY.  Regions are run one at a time (lall = 1); the
 ! metrics of region l are copied into the mod_misc globals before each call.
 !-------------------------------------------------------------------------------
+! The interface kind follows the build: c_double, or c_float when the
+! reference is compiled with -DSINGLE (RP = SP, mod_precision.f90:30-34).
+#ifdef SINGLE
+#define HK c_float
+#else
+#define HK c_double
+#endif
 module ref_harness_flux
   use iso_c_binding
   use mod_precision
@@ -20,19 +27,19 @@ contains
        xr, xr_pl, gp, gp_pl, gt, gt_pl, ga, ga_pl ) bind(C, name='ref_horizontal_flux')
     integer(c_int), value         :: gall_1d, kall, lall, gall_pl, have_pl
     integer(c_int), intent(in)    :: have_sgp(lall)
-    real(c_double), intent(inout) :: flx_h (gall_1d*gall_1d,kall,lall,6)
-    real(c_double), intent(inout) :: flx_h_pl(gall_pl,kall,2)
-    real(c_double), intent(inout) :: xc    (gall_1d*gall_1d,kall,lall,3,3)
-    real(c_double), intent(inout) :: xc_pl (gall_pl,kall,2,3)
-    real(c_double), intent(in)    :: rho   (gall_1d*gall_1d,kall,lall), rho_pl(gall_pl,kall,2)
-    real(c_double), intent(in)    :: rvx   (gall_1d*gall_1d,kall,lall), rvx_pl(gall_pl,kall,2)
-    real(c_double), intent(in)    :: rvy   (gall_1d*gall_1d,kall,lall), rvy_pl(gall_pl,kall,2)
-    real(c_double), intent(in)    :: rvz   (gall_1d*gall_1d,kall,lall), rvz_pl(gall_pl,kall,2)
-    real(c_double), value         :: dt
-    real(c_double), intent(in)    :: xr(gall_1d*gall_1d,1,lall,3,3),  xr_pl(gall_pl,1,2,3)
-    real(c_double), intent(in)    :: gp(gall_1d*gall_1d,1,lall,8),    gp_pl(gall_pl,1,2,8)
-    real(c_double), intent(in)    :: gt(gall_1d*gall_1d,1,lall,2,5),  gt_pl(gall_pl,1,2,5)
-    real(c_double), intent(in)    :: ga(gall_1d*gall_1d,1,lall,3,12), ga_pl(gall_pl,1,2,18)
+    real(HK), intent(inout) :: flx_h (gall_1d*gall_1d,kall,lall,6)
+    real(HK), intent(inout) :: flx_h_pl(gall_pl,kall,2)
+    real(HK), intent(inout) :: xc    (gall_1d*gall_1d,kall,lall,3,3)
+    real(HK), intent(inout) :: xc_pl (gall_pl,kall,2,3)
+    real(HK), intent(in)    :: rho   (gall_1d*gall_1d,kall,lall), rho_pl(gall_pl,kall,2)
+    real(HK), intent(in)    :: rvx   (gall_1d*gall_1d,kall,lall), rvx_pl(gall_pl,kall,2)
+    real(HK), intent(in)    :: rvy   (gall_1d*gall_1d,kall,lall), rvy_pl(gall_pl,kall,2)
+    real(HK), intent(in)    :: rvz   (gall_1d*gall_1d,kall,lall), rvz_pl(gall_pl,kall,2)
+    real(HK), value         :: dt
+    real(HK), intent(in)    :: xr(gall_1d*gall_1d,1,lall,3,3),  xr_pl(gall_pl,1,2,3)
+    real(HK), intent(in)    :: gp(gall_1d*gall_1d,1,lall,8),    gp_pl(gall_pl,1,2,8)
+    real(HK), intent(in)    :: gt(gall_1d*gall_1d,1,lall,2,5),  gt_pl(gall_pl,1,2,5)
+    real(HK), intent(in)    :: ga(gall_1d*gall_1d,1,lall,3,12), ga_pl(gall_pl,1,2,18)
 
     real(RP), allocatable :: fpl(:,:,:), xcpl(:,:,:,:)
     real(RP) :: dt_rp

@@ -3,6 +3,13 @@
 ! (kernels/NICAM/dyn_horiz_adv_limiter/src/mod_src_tracer.f90:42-430).
 ! TEST INFRASTRUCTURE ONLY.  Regions are run one at a time (lall = 1).
 !-------------------------------------------------------------------------------
+! The interface kind follows the build: c_double, or c_float when the
+! reference is compiled with -DSINGLE (RP = SP, mod_precision.f90:30-34).
+#ifdef SINGLE
+#define HK c_float
+#else
+#define HK c_double
+#endif
 module ref_harness_hlim
   use iso_c_binding
   use mod_precision
@@ -19,13 +26,13 @@ contains
        Qout_prev, Qout_prev_pl, Qout_post, Qout_post_pl ) bind(C, name='ref_horizontal_limiter_thuburn')
     integer(c_int), value         :: gall_1d, kall, lall, gall_pl, have_pl
     integer(c_int), intent(in)    :: have_sgp(lall)
-    real(c_double), intent(inout) :: q_a      (gall_1d*gall_1d,kall,lall,6), q_a_pl(gall_pl,kall,2)
-    real(c_double), intent(in)    :: q        (gall_1d*gall_1d,kall,lall),   q_pl  (gall_pl,kall,2)
-    real(c_double), intent(in)    :: d        (gall_1d*gall_1d,kall,lall),   d_pl  (gall_pl,kall,2)
-    real(c_double), intent(in)    :: ch       (gall_1d*gall_1d,kall,lall,6), ch_pl (gall_pl,kall,2)
-    real(c_double), intent(in)    :: cmask    (gall_1d*gall_1d,kall,lall,6), cmask_pl(gall_pl,kall,2)
-    real(c_double), intent(inout) :: Qout_prev(gall_1d*gall_1d,kall,lall,2), Qout_prev_pl(gall_pl,kall,2,2)
-    real(c_double), intent(in)    :: Qout_post(gall_1d*gall_1d,kall,lall,2), Qout_post_pl(gall_pl,kall,2,2)
+    real(HK), intent(inout) :: q_a      (gall_1d*gall_1d,kall,lall,6), q_a_pl(gall_pl,kall,2)
+    real(HK), intent(in)    :: q        (gall_1d*gall_1d,kall,lall),   q_pl  (gall_pl,kall,2)
+    real(HK), intent(in)    :: d        (gall_1d*gall_1d,kall,lall),   d_pl  (gall_pl,kall,2)
+    real(HK), intent(in)    :: ch       (gall_1d*gall_1d,kall,lall,6), ch_pl (gall_pl,kall,2)
+    real(HK), intent(in)    :: cmask    (gall_1d*gall_1d,kall,lall,6), cmask_pl(gall_pl,kall,2)
+    real(HK), intent(inout) :: Qout_prev(gall_1d*gall_1d,kall,lall,2), Qout_prev_pl(gall_pl,kall,2,2)
+    real(HK), intent(in)    :: Qout_post(gall_1d*gall_1d,kall,lall,2), Qout_post_pl(gall_pl,kall,2,2)
 
     real(RP), allocatable :: qapl(:,:,:), qprevpl(:,:,:,:)
     integer :: l

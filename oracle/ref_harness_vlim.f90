@@ -3,6 +3,13 @@
 ! (kernels/NICAM/dyn_vert_adv_limiter/src/mod_src_tracer.f90:41-243).
 ! TEST INFRASTRUCTURE ONLY.  Regions are run one at a time (lall = 1).
 !-------------------------------------------------------------------------------
+! The interface kind follows the build: c_double, or c_float when the
+! reference is compiled with -DSINGLE (RP = SP, mod_precision.f90:30-34).
+#ifdef SINGLE
+#define HK c_float
+#else
+#define HK c_double
+#endif
 module ref_harness_vlim
   use iso_c_binding
   use mod_precision
@@ -17,10 +24,10 @@ contains
   subroutine ref_vertical_limiter_thuburn( gall_1d, kall, lall, gall_pl, have_pl, &
        q_h, q_h_pl, q, q_pl, d, d_pl, ck, ck_pl ) bind(C, name='ref_vertical_limiter_thuburn')
     integer(c_int), value         :: gall_1d, kall, lall, gall_pl, have_pl
-    real(c_double), intent(inout) :: q_h(gall_1d*gall_1d,kall,lall),   q_h_pl(gall_pl,kall,2)
-    real(c_double), intent(in)    :: q  (gall_1d*gall_1d,kall,lall),   q_pl  (gall_pl,kall,2)
-    real(c_double), intent(in)    :: d  (gall_1d*gall_1d,kall,lall),   d_pl  (gall_pl,kall,2)
-    real(c_double), intent(in)    :: ck (gall_1d*gall_1d,kall,lall,2), ck_pl (gall_pl,kall,2,2)
+    real(HK), intent(inout) :: q_h(gall_1d*gall_1d,kall,lall),   q_h_pl(gall_pl,kall,2)
+    real(HK), intent(in)    :: q  (gall_1d*gall_1d,kall,lall),   q_pl  (gall_pl,kall,2)
+    real(HK), intent(in)    :: d  (gall_1d*gall_1d,kall,lall),   d_pl  (gall_pl,kall,2)
+    real(HK), intent(in)    :: ck (gall_1d*gall_1d,kall,lall,2), ck_pl (gall_pl,kall,2,2)
 
     real(RP), allocatable :: qhpl(:,:,:)
     integer :: l
