@@ -153,6 +153,52 @@ def test_gpu_f32_host_pointer_mode(gpu, refsp):
     assert_parity(out, refsp.horizontal_flux(spec, case["geo"], case["flux"], 360.0), rtol=0.0)
 
 
+@pytest.mark.gpu
+def test_gpu_mixed_gl8_sampled_regions(gpu, refsp):
+    """Mixed precision at glevel 8: the generated gl8/rl1 grid (40 regions of
+    130x130, its GMTR metrics), 10 levels.  The FP32 flux and limiters run on
+    the whole grid; three regions are checked bit for bit against the SINGLE
+    reference run on just those regions, and the FP64 GPU path on the same
+    grid (bit-identical to the FP64 oracle elsewhere) gives the error of every
+    region and field."""
+    import torch
+
+    from parity import region_slice
+
+    spec = synth.GridSpec(130, 10, 40, seed=2025)
+    geo = synth.icosahedral_geometry(8, 1)[0]
+    case = synth.make_case(spec, geo=geo)
+    ctx = drivers.setup_context(spec, geo)
+    regions = [0, 21, 39]
+    sub = synth.GridSpec(130, 10, len(regions), seed=2025)
+    pick = lambda d: region_slice(d, regions, pole=True)  # noqa: E731
+    z = lambda shape: torch.zeros(shape, dtype=torch.float32, device=gpu)  # noqa: E731
+    f = _dev32(None, case["flux"], gpu)
+    out = dict(flx_h=z((6,) + spec.shape3), flx_h_pl=z(spec.shape_pl), GRD_xc=z((3, 3) + spec.shape3),
+               GRD_xc_pl=z((3,) + spec.shape_pl))
+    ctx.horizontal_flux_f32(out["flx_h"], out["flx_h_pl"], out["GRD_xc"], out["GRD_xc_pl"],
+                            *[f[k] for k in drivers.FLUX_IN], 360.0)
+    got = {k: v.cpu().numpy() for k, v in out.items()}
+    assert_parity(pick(got), refsp.horizontal_flux(sub, pick(geo), pick(case["flux"]), 360.0), rtol=0.0)
+    dp = drivers.run_flux(ctx, spec, case["flux"], 360.0)
+    assert _rel_err(got["flx_h"], dp["flx_h"]) < 5e-5 and _rel_err(got["GRD_xc"], dp["GRD_xc"]) < 1e-6
+    h = _dev32(None, {k: case["hlim"][k] for k in drivers.HLIM_IN + ("q_a", "q_a_pl")}, gpu)
+    h.update(Qout_prev=z((2,) + spec.shape3), Qout_prev_pl=z((2,) + spec.shape_pl))
+    ctx.horizontal_limiter_thuburn_f32(h["q_a"], h["q_a_pl"], h["q"], h["q_pl"], h["d"], h["d_pl"], h["ch"],
+                                       h["ch_pl"], h["cmask"], h["cmask_pl"], h["Qout_prev"], h["Qout_prev_pl"],
+                                       h["Qout_post"], h["Qout_post_pl"])
+    got = {k: h[k].cpu().numpy() for k in ("q_a", "q_a_pl", "Qout_prev", "Qout_prev_pl")}
+    assert_parity(pick(got), refsp.horizontal_limiter_thuburn(sub, pick(geo)["have_sgp"], pick(case["hlim"])),
+                  rtol=0.0)
+    dp = drivers.run_hlim(ctx, spec, case["hlim"])
+    assert _rel_err(got["q_a"], dp["q_a"]) < 1e-5
+    v = _dev32(None, {k: case["vlim"][k] for k in drivers.VLIM_IN + ("q_h", "q_h_pl")}, gpu)
+    ctx.vertical_limiter_thuburn_f32(v["q_h"], v["q_h_pl"], v["q"], v["q_pl"], v["d"], v["d_pl"], v["ck"],
+                                     v["ck_pl"])
+    got = {k: v[k].cpu().numpy() for k in ("q_h", "q_h_pl")}
+    assert_parity(pick(got), refsp.vertical_limiter_thuburn(sub, pick(case["vlim"])), rtol=0.0)
+
+
 def test_f32_symbols_and_dtype_checks():
     lib = runtime.load_library() if runtime.LIB_PATH.exists() else None
     if lib is None:
