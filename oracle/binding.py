@@ -82,6 +82,10 @@ class Oracle:
         L.orc_horizontal_flux.argtypes = [POINTER(OrcDims), P] + [P] * 12 + [c_double] + [P] * 8
         L.orc_horizontal_limiter_thuburn.argtypes = [POINTER(OrcDims), P] + [P] * 14
         L.orc_vertical_limiter_thuburn.argtypes = [POINTER(OrcDims)] + [P] * 8
+        L.orc_vi_rhow_solver_adjoint.argtypes = ([POINTER(OrcDims)] + [P] * 14 + [c_double] + [P] * 16 + [P] * 3 +
+                                                 [c_double, c_double, c_double, c_int])
+        L.orc_vertical_limiter_thuburn_tlm.argtypes = [POINTER(OrcDims)] + [P] * 16
+        L.orc_vertical_limiter_thuburn_adjoint.argtypes = [POINTER(OrcDims)] + [P] * 14
         L.orc_vi_rhow_solver.argtypes = ([POINTER(OrcDims)] + [P] * 14 + [c_double] + [P] * 16 + [P] * 3 +
                                          [c_double, c_double, c_double, c_int])
         L.orc_oprt_diffusion.argtypes = [POINTER(OrcDims), P] + [P] * 10
@@ -257,6 +261,57 @@ class Oracle:
         self.lib.orc_vertical_limiter_thuburn(ctypes.byref(D), _p(out["q_h"]), _p(out["q_h_pl"]), _p(ins["q"]),
                                               _p(ins["q_pl"]), _p(ins["d"]), _p(ins["d_pl"]), _p(ins["ck"]),
                                               _p(ins["ck_pl"]))
+        return out
+
+    def vertical_limiter_thuburn_lin(self, spec, f, q_h_out, mask, mask_pl, adjoint, a, eps=None, big=None):
+        """TLM (adjoint=False; a = dict(dq_h_in, dq_h_in_pl, dq, dq_pl) -> dq_h, dq_h_pl) or adjoint
+        (adjoint=True; a = dict(q_h_b, q_h_b_pl) -> q_h_b, q_h_b_pl, q_b, q_b_pl) of the vertical limiter
+        at the forward state f (its inputs), q_h_out (its output) and mask."""
+        D = self.dims(spec, eps=eps, big=big)
+        ins = {k: _f64(f[k]) for k in ("q", "q_pl", "d", "d_pl", "ck", "ck_pl")}
+        qh = {k: _f64(v) for k, v in q_h_out.items()}
+        m = np.ascontiguousarray(mask, dtype=np.uint32)
+        mp = np.ascontiguousarray(mask_pl, dtype=np.uint32)
+        fw = [_p(qh["q_h"]), _p(qh["q_h_pl"]), _p(ins["q"]), _p(ins["q_pl"]), _p(ins["d"]), _p(ins["d_pl"]),
+              _p(ins["ck"]), _p(ins["ck_pl"]), m.ctypes.data, mp.ctypes.data]
+        if not adjoint:
+            din = {k: _f64(v) for k, v in a.items()}
+            out = dict(dq_h=din["dq_h_in"].copy(), dq_h_pl=din["dq_h_in_pl"].copy())
+            self.lib.orc_vertical_limiter_thuburn_tlm(ctypes.byref(D), _p(out["dq_h"]), _p(out["dq_h_pl"]),
+                                                      _p(din["dq_h_in"]), _p(din["dq_h_in_pl"]), _p(din["dq"]),
+                                                      _p(din["dq_pl"]), *fw)
+            return out
+        out = dict(q_h_b=_f64(a["q_h_b"]), q_h_b_pl=_f64(a["q_h_b_pl"]), q_b=np.zeros_like(ins["q"]),
+                   q_b_pl=np.zeros_like(ins["q_pl"]))
+        self.lib.orc_vertical_limiter_thuburn_adjoint(ctypes.byref(D), _p(out["q_h_b"]), _p(out["q_h_b_pl"]),
+                                                      _p(out["q_b"]), _p(out["q_b_pl"]), *fw)
+        return out
+
+    def vi_rhow_solver_adjoint(self, spec, f, b, dt, rdgzh, afact, bfact, grav=9.80616, rdry=287.0, cvdry=717.5,
+                               alpha=1):
+        """Transpose of vi_rhow_solver's linear map at the coefficients and metrics of f: b holds rhogw_b
+        (output cotangent); returns the cotangents of rhogw, rhogw0, preg0, rhog0, Srho, Sw, Spre (+ _pl)."""
+        D = self.dims(spec)
+        outs = ["rhogw", "rhogw0", "preg0", "rhog0", "Srho", "Sw", "Spre"]
+        out = {}
+        for n in outs:
+            src = b[n] if n == "rhogw" else np.zeros_like(f[n])
+            srcp = b[n + "_pl"] if n == "rhogw" else np.zeros_like(f[n + "_pl"])
+            out[n] = _f64(src)
+            out[n + "_pl"] = _f64(srcp)
+        coef = {}
+        for n in ["Mc", "Ml", "Mu", "RGAM", "RGAMH", "RGSGAM2", "RGSGAM2H", "GSGAM2H"]:
+            coef[n] = _f64(f[n])
+            coef[n + "_pl"] = _f64(f[n + "_pl"])
+        v = [_f64(rdgzh), _f64(afact), _f64(bfact)]
+        a1 = []
+        for n in outs:
+            a1 += [_p(out[n]), _p(out[n + "_pl"])]
+        a2 = []
+        for n in ["Mc", "Ml", "Mu", "RGAM", "RGAMH", "RGSGAM2", "RGSGAM2H", "GSGAM2H"]:
+            a2 += [_p(coef[n]), _p(coef[n + "_pl"])]
+        self.lib.orc_vi_rhow_solver_adjoint(ctypes.byref(D), *a1, float(dt), *a2, *[x.ctypes.data for x in v],
+                                            grav, rdry, cvdry, int(alpha))
         return out
 
     def vi_rhow_solver(self, spec, f, dt, rdgzh, afact, bfact, grav=9.80616, rdry=287.0, cvdry=717.5, alpha=1):

@@ -1262,3 +1262,251 @@ void orc_vi_rhow_update_matrix(const orc_dims *D, double *Mc, double *Mc_pl, dou
 #undef IX
     }
 }
+
+/* ------------------------------------------------------------------------- */
+/* Adjoint of vi_rhow_solver (dyn_vi_rhow_solver/src/mod_vi.f90:93-306).     */
+/* ------------------------------------------------------------------------- */
+/* With the coefficients, metrics and dt fixed, the solver is a linear map of
+ * (rhogw at kmin and kmax+1, rhogw0, preg0, rhog0, Srho, Sw, Spre) to rhogw on
+ * kmin..kmax+1; rhogw(1) passes through and its interior input values are
+ * overwritten (:209-246).  This is the transpose of that map, column by
+ * column: the transposed tridiagonal system (diagonal Mc, sub-diagonal
+ * Mu(k-1), super-diagonal Ml(k+1)) solved by the same elimination, then the
+ * transpose of the source term Sall (:194-206) and of the boundary terms
+ * (:209-212).  wb holds the output cotangent on entry and the input cotangent
+ * on return; the six others are written on every level (zero where the
+ * forward map does not read them).  The level loop below is the order of
+ * operations the GPU kernel follows. */
+static void vi_adjoint_column(int kall, size_t base, size_t s, double *wb, double *w0b, double *p0b, double *r0b,
+                              double *Srb, double *Swb, double *Spb, const double *Mc, const double *Ml,
+                              const double *Mu, const double *RGAM, const double *RGAMH, const double *RGSGAM2,
+                              const double *RGSGAM2H, const double *GSGAM2H, const double *rdgzh,
+                              const double *afact, const double *bfact, double dt, double CVovRt2, double alpha,
+                              double grav)
+{
+    const int kmin = 2, kmax = kall - 1;
+#define AT(k) (base + (size_t)((k) - 1) * s)
+    double y[kall + 2], gam[kall + 2];
+    /* T^T y = GSGAM2H * wb on kmin+1..kmax */
+    double beta = Mc[AT(kmin + 1)];
+    y[kmin + 1] = (wb[AT(kmin + 1)] * GSGAM2H[AT(kmin + 1)]) / beta;
+    for (int k = kmin + 2; k <= kmax; ++k) {
+        gam[k] = Ml[AT(k)] / beta;
+        beta = Mc[AT(k)] - Mu[AT(k - 1)] * gam[k];
+        y[k] = (wb[AT(k)] * GSGAM2H[AT(k)] - Mu[AT(k - 1)] * y[k - 1]) / beta;
+    }
+    for (int k = kmax - 1; k >= kmin + 1; --k) y[k] = y[k] - gam[k + 1] * y[k + 1];
+    /* boundary values: w(kmin), w(kmax+1) scaled by RGSGAM2H, entered in Sall
+     * (:209-212) and scaled back by GSGAM2H (:242-246) */
+    const double bb = wb[AT(kmin)] * GSGAM2H[AT(kmin)] - Ml[AT(kmin + 1)] * y[kmin + 1];
+    const double bt = wb[AT(kmax + 1)] * GSGAM2H[AT(kmax + 1)] - Mu[AT(kmax)] * y[kmax];
+    for (int k = kall; k >= 1; --k) {
+        const int in_k = (k >= kmin + 1 && k <= kmax), in_n = (k + 1 >= kmin + 1 && k + 1 <= kmax);
+        const double sb = in_k ? y[k] * CVovRt2 : 0.0;
+        const double sbn = in_n ? y[k + 1] * CVovRt2 : 0.0;
+        double pb = 0.0, rb = 0.0;
+        if (in_k) {
+            pb = -(sb * dt * rdgzh[k - 1]);
+            rb = -(sb * dt * grav * afact[k - 1]);
+        }
+        if (in_n) {
+            pb = pb + sbn * dt * rdgzh[k];
+            rb = rb - sbn * dt * grav * bfact[k];
+        }
+        const size_t o = AT(k);
+        p0b[o] = pb * RGSGAM2[o];
+        Spb[o] = dt * p0b[o];
+        r0b[o] = rb * (RGAM[o] * RGAM[o]);
+        Srb[o] = dt * r0b[o];
+        const double sw = in_k ? sb * (RGAMH[o] * RGAMH[o]) : 0.0;
+        w0b[o] = sw * alpha;
+        Swb[o] = dt * sw;
+        if (k == kmin) wb[o] = bb * RGSGAM2H[o];
+        else if (k == kmax + 1) wb[o] = bt * RGSGAM2H[o];
+        else if (k > kmin) wb[o] = 0.0;
+    }
+#undef AT
+}
+
+void orc_vi_rhow_solver_adjoint(const orc_dims *D, double *wb, double *wb_pl, double *w0b, double *w0b_pl,
+                                double *p0b, double *p0b_pl, double *r0b, double *r0b_pl, double *Srb,
+                                double *Srb_pl, double *Swb, double *Swb_pl, double *Spb, double *Spb_pl, double dt,
+                                const double *Mc, const double *Mc_pl, const double *Ml, const double *Ml_pl,
+                                const double *Mu, const double *Mu_pl, const double *RGAM, const double *RGAM_pl,
+                                const double *RGAMH, const double *RGAMH_pl, const double *RGSGAM2,
+                                const double *RGSGAM2_pl, const double *RGSGAM2H, const double *RGSGAM2H_pl,
+                                const double *GSGAM2H, const double *GSGAM2H_pl, const double *rdgzh,
+                                const double *afact, const double *bfact, double grav, double rdry, double cvdry,
+                                int non_hydro_alpha)
+{
+    const int gall = D->gall_1d * D->gall_1d, kall = D->kall;
+    const double CVovRt2 = cvdry / rdry / (dt * dt), alpha = (double)non_hydro_alpha;
+    for (int l = 0; l < D->lall; ++l)
+#pragma omp parallel for schedule(static)
+        for (int g = 0; g < gall; ++g)
+            vi_adjoint_column(kall, (size_t)l * kall * gall + g, (size_t)gall, wb, w0b, p0b, r0b, Srb, Swb, Spb, Mc,
+                              Ml, Mu, RGAM, RGAMH, RGSGAM2, RGSGAM2H, GSGAM2H, rdgzh, afact, bfact, dt, CVovRt2,
+                              alpha, grav);
+    if (D->have_pl)
+        for (int l = 0; l < LALL_PL; ++l)
+            for (int g = 0; g < D->gall_pl; ++g)
+                vi_adjoint_column(kall, (size_t)l * kall * D->gall_pl + g, (size_t)D->gall_pl, wb_pl, w0b_pl, p0b_pl,
+                                  r0b_pl, Srb_pl, Swb_pl, Spb_pl, Mc_pl, Ml_pl, Mu_pl, RGAM_pl, RGAMH_pl,
+                                  RGSGAM2_pl, RGSGAM2H_pl, GSGAM2H_pl, rdgzh, afact, bfact, dt, CVovRt2, alpha, grav);
+}
+
+/* ------------------------------------------------------------------------- */
+/* Tangent-linear and adjoint of vertical_limiter_thuburn                    */
+/* (dyn_vert_adv_limiter/src/mod_src_tracer.f90:41-243), at fixed ck and d.  */
+/* ------------------------------------------------------------------------- */
+/* The limited face value is one of three things: the unlimited q_h, or the
+ * lower or upper bound Qout of the upwind level j (j = k-1 for inflow from
+ * below, else k).  The forward pass records one bit per face, set where the
+ * value was clipped (output != input); a clipped face sits on the lower bound
+ * if its value equals Qout_min(j), else on the upper bound.  Qout_min(j) and
+ * Qout_max(j) are piecewise linear in q(j-1), q(j), q(j+1): the min/max
+ * selections below follow the forward's own comparisons, mn(a,b) = b<a?b:a
+ * and mx(a,b) = b>a?b:a.  vlim_level gives the two bounds of level j and
+ * their rows of derivatives (index 0,1,2 = q(j-1), q(j), q(j+1)). */
+typedef struct {
+    double qmin, qmax;  /* Qout_min, Qout_max */
+    double rmin[3], rmax[3];
+} vlim_lvl;
+
+static void vlim_level(const double *q, const double *d, const double *ck, size_t o, size_t s, size_t ck2,
+                       double EPS, double BIG, int pole, vlim_lvl *r)
+{
+    const double ck1 = ck[o], ck1p = ck[o + s], ck2k = ck[ck2 + o];
+    const double inflagL = 0.5 - fsign(0.5, ck1), inflagU = 0.5 + fsign(0.5, ck1p);
+    const double qk = q[o], qkm1 = q[o - s], qkp1 = q[o + s];
+    const double Qin_minL = mn(qk, qkm1) + (1.0 - inflagL) * BIG;
+    const double Qin_minU = mn(qk, qkp1) + (1.0 - inflagU) * BIG;
+    const double Qin_maxL = mx(qk, qkm1) - (1.0 - inflagL) * BIG;
+    const double Qin_maxU = mx(qk, qkp1) - (1.0 - inflagU) * BIG;
+    const double tmin = mn(Qin_minL, Qin_minU), tmax = mx(Qin_maxL, Qin_maxU);
+    const double qnext_min = mn(tmin, qk), qnext_max = mx(tmax, qk);
+    const double Cin = inflagL * ck1 + inflagU * ck2k;
+    const double Cout = (1.0 - inflagL) * ck1 + (1.0 - inflagU) * ck2k;
+    double CQin_min, CQin_max;
+    if (pole) {
+        CQin_max = inflagL * (ck1 * Qin_maxL) + inflagU * (ck2k * Qin_maxU);
+        CQin_min = inflagL * (ck1 * Qin_minL) + inflagU * (ck2k * Qin_minU);
+    } else {
+        CQin_min = inflagL * ck1 * Qin_minL + inflagU * ck2k * Qin_minU;
+        CQin_max = inflagL * ck1 * Qin_maxL + inflagU * ck2k * Qin_maxU;
+    }
+    const double zerosw = 0.5 - fsign(0.5, fabs(Cout) - EPS);
+    const double dk = d[o];
+    r->qmin = ((qk - qnext_max) + qnext_max * (Cin + Cout - dk) - CQin_max) / (Cout + zerosw) * (1.0 - zerosw) + qk * zerosw;
+    r->qmax = ((qk - qnext_min) + qnext_min * (Cin + Cout - dk) - CQin_min) / (Cout + zerosw) * (1.0 - zerosw) + qk * zerosw;
+    /* one-hot rows of the selections */
+    const int maxL = (qkm1 > qk) ? 0 : 1, maxU = (qkp1 > qk) ? 2 : 1;  /* Qin_max* -> q index */
+    const int minL = (qkm1 < qk) ? 0 : 1, minU = (qkp1 < qk) ? 2 : 1;  /* Qin_min* -> q index */
+    const int nmax = (qk > tmax) ? 1 : ((Qin_maxU > Qin_maxL) ? maxU : maxL);
+    const int nmin = (qk < tmin) ? 1 : ((Qin_minU < Qin_minL) ? minU : minL);
+    const double fac = (1.0 - zerosw) / (Cout + zerosw);
+    const double e = (Cin + Cout - dk) - 1.0;
+    const double cL = inflagL * ck1, cU = inflagU * ck2k;
+    for (int m = 0; m < 3; ++m) {
+        const double dnx = (m == nmax) ? 1.0 : 0.0, dnn = (m == nmin) ? 1.0 : 0.0;
+        const double cqx = ((m == maxL) ? cL : 0.0) + ((m == maxU) ? cU : 0.0);
+        const double cqn = ((m == minL) ? cL : 0.0) + ((m == minU) ? cU : 0.0);
+        r->rmin[m] = fac * (e * dnx - cqx);
+        r->rmax[m] = fac * (e * dnn - cqn);
+    }
+    r->rmin[1] = (r->rmin[1] + fac) + zerosw;
+    r->rmax[1] = (r->rmax[1] + fac) + zerosw;
+}
+
+/* Row of the limited face k: coefficient of q_h_in (*cqh) and of q(k-2..k+1)
+ * (row[0..3]). */
+static void vlim_face_row(const vlim_lvl *lo, const vlim_lvl *hi, double inflagL, int clipped, double out,
+                          double *cqh, double row[4])
+{
+    row[0] = row[1] = row[2] = row[3] = 0.0;
+    *cqh = clipped ? 0.0 : 1.0;
+    if (!clipped) return;
+    const vlim_lvl *L = (inflagL > 0.0) ? lo : hi; /* level k-1 or k */
+    const int off = (inflagL > 0.0) ? 0 : 1;        /* row index of q(j-1) */
+    const double *r = (out == L->qmin) ? L->rmin : L->rmax;
+    for (int m = 0; m < 3; ++m) row[off + m] = r[m];
+}
+
+/* One column: TLM (dqh_out from dqh_in, dq) or adjoint (qhb, qb) */
+static void vlim_lin_column(int kall, size_t base, size_t s, size_t ck2, const uint32_t *mask, size_t mbase,
+                            size_t mstride, const double *q_h, const double *q, const double *d, const double *ck,
+                            double EPS, double BIG, int pole, int adjoint, double *a_qh, const double *a_dqh_in,
+                            double *a_q)
+{
+    const int kmin = 2, kmax = kall - 1;
+#define AT(k) (base + (size_t)((k) - 1) * s)
+    vlim_lvl lo, hi;
+    if (adjoint)
+        for (int k = 1; k <= kall; ++k) a_q[AT(k)] = 0.0;
+    vlim_level(q, d, ck, AT(kmin), s, ck2, EPS, BIG, pole, &lo);
+    for (int k = kmin + 1; k <= kmax; ++k) {
+        vlim_level(q, d, ck, AT(k), s, ck2, EPS, BIG, pole, &hi);
+        const int p = k - (kmin + 1);
+        const int clipped = (mask[mbase + (size_t)(p / 32) * mstride] >> (p % 32)) & 1u;
+        const double inflagL = 0.5 - fsign(0.5, ck[AT(k)]);
+        double cqh, row[4];
+        vlim_face_row(&lo, &hi, inflagL, clipped, q_h[AT(k)], &cqh, row);
+        if (adjoint) {
+            const double ob = a_qh[AT(k)];
+            a_qh[AT(k)] = cqh * ob;
+            for (int m = 0; m < 4; ++m) {
+                const int kk = k - 2 + m;
+                if (kk >= 1 && kk <= kall) a_q[AT(kk)] = a_q[AT(kk)] + row[m] * ob;
+            }
+        } else {
+            double v = cqh * a_dqh_in[AT(k)];
+            for (int m = 0; m < 4; ++m) {
+                const int kk = k - 2 + m;
+                if (kk >= 1 && kk <= kall) v = v + row[m] * a_q[AT(kk)];
+            }
+            a_qh[AT(k)] = v;
+        }
+        lo = hi;
+    }
+#undef AT
+}
+
+static void vlim_lin(const orc_dims *D, int adjoint, double *a_qh, double *a_qh_pl, const double *a_dqh_in,
+                     const double *a_dqh_in_pl, double *a_q, double *a_q_pl, const double *q_h, const double *q_h_pl,
+                     const double *q, const double *q_pl, const double *d, const double *d_pl, const double *ck,
+                     const double *ck_pl, const uint32_t *mask, const uint32_t *mask_pl)
+{
+    const int gall = D->gall_1d * D->gall_1d, kall = D->kall, nword = (kall - 3 + 31) / 32;
+    if (kall - 3 <= 0) return;
+    for (int l = 0; l < D->lall; ++l)
+#pragma omp parallel for schedule(static)
+        for (int g = 0; g < gall; ++g)
+            vlim_lin_column(kall, (size_t)l * kall * gall + g, gall, (size_t)gall * kall * D->lall, mask,
+                            (size_t)l * nword * gall + g, gall, q_h, q, d, ck, D->eps, D->big, 0, adjoint, a_qh,
+                            a_dqh_in, a_q);
+    if (D->have_pl)
+        for (int l = 0; l < LALL_PL; ++l)
+            for (int g = 0; g < D->gall_pl; ++g)
+                vlim_lin_column(kall, (size_t)l * kall * D->gall_pl + g, D->gall_pl,
+                                (size_t)D->gall_pl * kall * LALL_PL, mask_pl, (size_t)l * nword * D->gall_pl + g,
+                                D->gall_pl, q_h_pl, q_pl, d_pl, ck_pl, D->eps, D->big, 1, adjoint, a_qh_pl,
+                                a_dqh_in_pl, a_q_pl);
+}
+
+void orc_vertical_limiter_thuburn_tlm(const orc_dims *D, double *dqh, double *dqh_pl, const double *dqh_in,
+                                      const double *dqh_in_pl, const double *dq, const double *dq_pl,
+                                      const double *q_h, const double *q_h_pl, const double *q, const double *q_pl,
+                                      const double *d, const double *d_pl, const double *ck, const double *ck_pl,
+                                      const uint32_t *mask, const uint32_t *mask_pl)
+{
+    vlim_lin(D, 0, dqh, dqh_pl, dqh_in, dqh_in_pl, (double *)dq, (double *)dq_pl, q_h, q_h_pl, q, q_pl, d, d_pl,
+             ck, ck_pl, mask, mask_pl);
+}
+
+void orc_vertical_limiter_thuburn_adjoint(const orc_dims *D, double *qhb, double *qhb_pl, double *qb, double *qb_pl,
+                                          const double *q_h, const double *q_h_pl, const double *q,
+                                          const double *q_pl, const double *d, const double *d_pl, const double *ck,
+                                          const double *ck_pl, const uint32_t *mask, const uint32_t *mask_pl)
+{
+    vlim_lin(D, 1, qhb, qhb_pl, NULL, NULL, qb, qb_pl, q_h, q_h_pl, q, q_pl, d, d_pl, ck, ck_pl, mask, mask_pl);
+}

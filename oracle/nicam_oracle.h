@@ -19,6 +19,8 @@
  * converted to 0-based offsets only inside the accessor macros.
  */
 #ifndef NICAM_ORACLE_H
+#include <stdint.h>
+
 #define NICAM_ORACLE_H
 
 #ifdef __cplusplus
@@ -185,6 +187,35 @@ void orc_vi_rhow_update_matrix(const orc_dims *D, double *Mc, double *Mc_pl, dou
                                const double *RGAMH, const double *RGAMH_pl, const double *GSGAM2H,
                                const double *GSGAM2H_pl, const double *rdgz, const double *rdgzh, const double *cfact,
                                const double *dfact, double dt, double grav, double rdry, double cvdry, int alpha_i);
+
+/* Adjoint (transpose) of orc_vi_rhow_solver's linear map, coefficients fixed:
+ * wb in = output cotangent, out = input cotangent; the six others written. */
+void orc_vi_rhow_solver_adjoint(const orc_dims *D, double *wb, double *wb_pl, double *w0b, double *w0b_pl,
+                                double *p0b, double *p0b_pl, double *r0b, double *r0b_pl, double *Srb,
+                                double *Srb_pl, double *Swb, double *Swb_pl, double *Spb, double *Spb_pl, double dt,
+                                const double *Mc, const double *Mc_pl, const double *Ml, const double *Ml_pl,
+                                const double *Mu, const double *Mu_pl, const double *RGAM, const double *RGAM_pl,
+                                const double *RGAMH, const double *RGAMH_pl, const double *RGSGAM2,
+                                const double *RGSGAM2_pl, const double *RGSGAM2H, const double *RGSGAM2H_pl,
+                                const double *GSGAM2H, const double *GSGAM2H_pl, const double *rdgzh,
+                                const double *afact, const double *bfact, double grav, double rdry, double cvdry,
+                                int non_hydro_alpha);
+
+/* Tangent-linear and adjoint of orc_vertical_limiter_thuburn at fixed ck and
+ * d.  mask: one bit per limited face (set where the forward clipped it),
+ * word ((l*nword + p/32)*gall + g), bit p%32, p = k-(kmin+1), nword =
+ * ceil((kall-3)/32); q_h is the forward's output.  TLM: dqh from dqh_in and
+ * dq.  Adjoint: qhb in = output cotangent, out = cotangent of q_h's input;
+ * qb written on every level. */
+void orc_vertical_limiter_thuburn_tlm(const orc_dims *D, double *dqh, double *dqh_pl, const double *dqh_in,
+                                      const double *dqh_in_pl, const double *dq, const double *dq_pl,
+                                      const double *q_h, const double *q_h_pl, const double *q, const double *q_pl,
+                                      const double *d, const double *d_pl, const double *ck, const double *ck_pl,
+                                      const uint32_t *mask, const uint32_t *mask_pl);
+void orc_vertical_limiter_thuburn_adjoint(const orc_dims *D, double *qhb, double *qhb_pl, double *qb, double *qb_pl,
+                                          const double *q_h, const double *q_h_pl, const double *q,
+                                          const double *q_pl, const double *d, const double *d_pl, const double *ck,
+                                          const double *ck_pl, const uint32_t *mask, const uint32_t *mask_pl);
 
 #ifdef __cplusplus
 }
