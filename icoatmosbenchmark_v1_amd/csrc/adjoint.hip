@@ -119,7 +119,191 @@ __global__ void __launch_bounds__(BLOCK_VA) vi_adjoint_kernel(VaArgs A)
     }
 }
 
+// ---- vertical_limiter_thuburn (dyn_vert_adv_limiter/src/mod_src_tracer.f90:41-243)
+//
+// At fixed ck and d the limited face value is q_h's input (bit clear in the
+// forward's mask), or Qout_min or Qout_max of the upwind level j (bit set;
+// the lower bound when the stored output equals Qout_min(j)).  Both bounds
+// are piecewise linear in q(j-1), q(j), q(j+1); their rows follow the
+// forward's own comparisons (dmin(a,b) = b<a?b:a, dmax(a,b) = b>a?b:a).
+// One work-item per column marches up the faces and scatters each face's
+// cotangent into a window of four q levels held in registers; a level is
+// stored once no later face reaches it.
+struct VlimLvl {
+    double qmin, qmax, rmin[3], rmax[3];
+};
+
+__device__ __forceinline__ double sgn_half(double x) { return copysign(0.5, x); }
+__device__ __forceinline__ double dmn(double a, double b) { return (b < a) ? b : a; }
+__device__ __forceinline__ double dmx(double a, double b) { return (b > a) ? b : a; }
+
+template <bool POLE>
+__device__ __forceinline__ VlimLvl vlim_level(double qkm1, double qk, double qkp1, double ck1, double ck1p,
+                                              double ck2k, double dk, double EPS, double BIG)
+{
+    VlimLvl r;
+    const double inflagL = 0.5 - sgn_half(ck1), inflagU = 0.5 + sgn_half(ck1p);
+    const double Qin_minL = dmn(qk, qkm1) + (1.0 - inflagL) * BIG;
+    const double Qin_minU = dmn(qk, qkp1) + (1.0 - inflagU) * BIG;
+    const double Qin_maxL = dmx(qk, qkm1) - (1.0 - inflagL) * BIG;
+    const double Qin_maxU = dmx(qk, qkp1) - (1.0 - inflagU) * BIG;
+    const double tmin = dmn(Qin_minL, Qin_minU), tmax = dmx(Qin_maxL, Qin_maxU);
+    const double qnext_min = dmn(tmin, qk), qnext_max = dmx(tmax, qk);
+    const double Cin = inflagL * ck1 + inflagU * ck2k;
+    const double Cout = (1.0 - inflagL) * ck1 + (1.0 - inflagU) * ck2k;
+    double CQin_min, CQin_max;
+    if constexpr (POLE) {
+        CQin_max = inflagL * (ck1 * Qin_maxL) + inflagU * (ck2k * Qin_maxU);
+        CQin_min = inflagL * (ck1 * Qin_minL) + inflagU * (ck2k * Qin_minU);
+    } else {
+        CQin_min = inflagL * ck1 * Qin_minL + inflagU * ck2k * Qin_minU;
+        CQin_max = inflagL * ck1 * Qin_maxL + inflagU * ck2k * Qin_maxU;
+    }
+    const double zerosw = 0.5 - sgn_half(fabs(Cout) - EPS);
+    r.qmin = ((qk - qnext_max) + qnext_max * (Cin + Cout - dk) - CQin_max) / (Cout + zerosw) * (1.0 - zerosw) +
+             qk * zerosw;
+    r.qmax = ((qk - qnext_min) + qnext_min * (Cin + Cout - dk) - CQin_min) / (Cout + zerosw) * (1.0 - zerosw) +
+             qk * zerosw;
+    const int maxL = (qkm1 > qk) ? 0 : 1, maxU = (qkp1 > qk) ? 2 : 1;
+    const int minL = (qkm1 < qk) ? 0 : 1, minU = (qkp1 < qk) ? 2 : 1;
+    const int nmax = (qk > tmax) ? 1 : ((Qin_maxU > Qin_maxL) ? maxU : maxL);
+    const int nmin = (qk < tmin) ? 1 : ((Qin_minU < Qin_minL) ? minU : minL);
+    const double fac = (1.0 - zerosw) / (Cout + zerosw);
+    const double e = (Cin + Cout - dk) - 1.0;
+    const double cL = inflagL * ck1, cU = inflagU * ck2k;
+#pragma unroll
+    for (int m = 0; m < 3; ++m) {
+        const double dnx = (m == nmax) ? 1.0 : 0.0, dnn = (m == nmin) ? 1.0 : 0.0;
+        const double cqx = ((m == maxL) ? cL : 0.0) + ((m == maxU) ? cU : 0.0);
+        const double cqn = ((m == minL) ? cL : 0.0) + ((m == minU) ? cU : 0.0);
+        r.rmin[m] = fac * (e * dnx - cqx);
+        r.rmax[m] = fac * (e * dnn - cqn);
+    }
+    r.rmin[1] = (r.rmin[1] + fac) + zerosw;
+    r.rmax[1] = (r.rmax[1] + fac) + zerosw;
+    return r;
+}
+
+struct VlaField {
+    double *qhb, *qb;
+    const double *q_h, *q, *d, *ck;
+    const uint32_t *mask;
+};
+
+struct VlaArgs {
+    VlaField reg, pl;
+    int gall, kall, lall, gall_pl, have_pl, nword;
+    double eps, big;
+};
+
+template <bool POLE>
+__device__ __forceinline__ void vlim_adjoint_column(const VlaField &F, size_t base, size_t s, size_t ck2,
+                                                    const uint32_t *mw, size_t mstride, int kall, double EPS,
+                                                    double BIG)
+{
+    const int kmin = 1, kmax = kall - 2;  // 0-based
+    auto at = [&](int k) { return base + (size_t)k * s; };
+    // level kmin (the level below the first limited face)
+    double qm = F.q[at(kmin - 1)], q0 = F.q[at(kmin)], qp = F.q[at(kmin + 1)];
+    double c1 = F.ck[at(kmin)], c1p = F.ck[at(kmin + 1)];
+    VlimLvl lo = vlim_level<POLE>(qm, q0, qp, c1, c1p, F.ck[ck2 + at(kmin)], F.d[at(kmin)], EPS, BIG);
+    // q cotangents of levels k-2 .. k+1 while face k is processed
+    double w0 = 0.0, w1 = 0.0, w2 = 0.0, w3 = 0.0;
+    uint32_t word = 0;
+    for (int k = kmin + 1; k <= kmax; ++k) {
+        const int p = k - (kmin + 1);
+        if ((p & 31) == 0) word = mw[(size_t)(p >> 5) * mstride];
+        qm = q0;
+        q0 = qp;
+        qp = F.q[at(k + 1)];
+        c1 = c1p;
+        c1p = F.ck[at(k + 1)];
+        const VlimLvl hi = vlim_level<POLE>(qm, q0, qp, c1, c1p, F.ck[ck2 + at(k)], F.d[at(k)], EPS, BIG);
+        const bool clipped = (word >> (p & 31)) & 1u;
+        const double inflagL = 0.5 - sgn_half(c1);
+        const size_t o = at(k);
+        const double ob = F.qhb[o];
+        double row[4] = {0.0, 0.0, 0.0, 0.0};
+        double cqh = 1.0;
+        if (clipped) {
+            cqh = 0.0;
+            const double out = F.q_h[o];
+            const bool from_lo = inflagL > 0.0;
+            const VlimLvl &L = from_lo ? lo : hi;
+            const double *r = (out == L.qmin) ? L.rmin : L.rmax;
+            if (from_lo) {
+                row[0] = r[0];
+                row[1] = r[1];
+                row[2] = r[2];
+            } else {
+                row[1] = r[0];
+                row[2] = r[1];
+                row[3] = r[2];
+            }
+        }
+        F.qhb[o] = cqh * ob;
+        w0 = w0 + row[0] * ob;
+        w1 = w1 + row[1] * ob;
+        w2 = w2 + row[2] * ob;
+        w3 = w3 + row[3] * ob;
+        F.qb[at(k - 2)] = w0;  // no later face reaches level k-2
+        w0 = w1;
+        w1 = w2;
+        w2 = w3;
+        w3 = 0.0;
+        lo = hi;
+    }
+    F.qb[at(kmax - 1)] = w0;
+    F.qb[at(kmax)] = w1;
+    F.qb[at(kmax + 1)] = w2;
+}
+
+constexpr int BLOCK_VLA = 256;
+
+__global__ void __launch_bounds__(BLOCK_VLA) vlim_adjoint_kernel(VlaArgs A)
+{
+    const long t = (long)blockIdx.x * BLOCK_VLA + threadIdx.x;
+    const long nreg = (long)A.gall * A.lall;
+    if (t < nreg) {
+        const int g = (int)(t % A.gall), l = (int)(t / A.gall);
+        vlim_adjoint_column<false>(A.reg, (size_t)l * A.kall * A.gall + g, A.gall,
+                                   (size_t)A.gall * A.kall * A.lall, A.reg.mask + (size_t)l * A.nword * A.gall + g,
+                                   A.gall, A.kall, A.eps, A.big);
+    } else if (A.have_pl && t < nreg + (long)A.gall_pl * kLallPl) {
+        const int p = (int)(t - nreg);
+        const int g = p % A.gall_pl, l = p / A.gall_pl;
+        vlim_adjoint_column<true>(A.pl, (size_t)l * A.kall * A.gall_pl + g, A.gall_pl,
+                                  (size_t)A.gall_pl * A.kall * kLallPl,
+                                  A.pl.mask + (size_t)l * A.nword * A.gall_pl + g, A.gall_pl, A.kall, A.eps, A.big);
+    }
+}
+
 }  // namespace
+
+int launch_vlim_adjoint(nicam_ctx c, double *qhb, double *qhb_pl, double *qb, double *qb_pl, const double *q_h,
+                        const double *q_h_pl, const double *q, const double *q_pl, const double *d,
+                        const double *d_pl, const double *ck, const double *ck_pl, const uint32_t *mask,
+                        const uint32_t *mask_pl)
+{
+    const Dims &D = c->d;
+    if (D.kall - 3 <= 0) return NICAM_OK;
+    VlaArgs A;
+    A.reg = VlaField{qhb, qb, q_h, q, d, ck, mask};
+    A.pl = VlaField{qhb_pl, qb_pl, q_h_pl, q_pl, d_pl, ck_pl, mask_pl};
+    A.gall = D.gall;
+    A.kall = D.kall;
+    A.lall = D.lall;
+    A.gall_pl = D.gall_pl;
+    A.have_pl = D.have_pl;
+    A.nword = (D.kall - 3 + 31) / 32;
+    A.eps = c->eps;
+    A.big = c->big;
+    const long ncol = (long)D.gall * D.lall + (D.have_pl ? (long)D.gall_pl * kLallPl : 0);
+    hipLaunchKernelGGL(vlim_adjoint_kernel, dim3((unsigned)((ncol + BLOCK_VLA - 1) / BLOCK_VLA)), dim3(BLOCK_VLA), 0,
+                       c->stream, A);
+    NICAM_HIP(hipGetLastError());
+    return NICAM_OK;
+}
 
 int launch_vi_adjoint(nicam_ctx c, double *wb, double *wb_pl, double *w0b, double *w0b_pl, double *p0b,
                       double *p0b_pl, double *r0b, double *r0b_pl, double *Srb, double *Srb_pl, double *Swb,

@@ -753,6 +753,105 @@ int nicam_vi_rhow_solver(nicam_ctx c, double *w, double *w_pl, const double *w0,
     return stage_out(c, a, v);
 }
 
+// ---- adjoints ----
+int nicam_vi_rhow_solver_adjoint(nicam_ctx c, double *wb, double *wb_pl, double *w0b, double *w0b_pl, double *p0b,
+                                 double *p0b_pl, double *r0b, double *r0b_pl, double *Srb, double *Srb_pl,
+                                 double *Swb, double *Swb_pl, double *Spb, double *Spb_pl, double dt)
+{
+    NICAM_TRY(check_ctx(c));
+    if (!c->have_coef) return fail(NICAM_ERR_STATE, "tridiagonal coefficients not set (nicam_vi_rhow_solver_putcoef)");
+    if (!c->have_vmetrics) return fail(NICAM_ERR_STATE, "vertical metrics not set (nicam_set_vmetrics)");
+    if (!c->have_vgrid) return fail(NICAM_ERR_STATE, "vertical grid not set (nicam_grd_setup)");
+    if (!(dt != 0.0)) return fail(NICAM_ERR_ARG, "dt must be nonzero");
+    double *req[] = {wb, w0b, p0b, r0b, Srb, Swb, Spb};
+    const char *nm[] = {"rhogw_b", "rhogw0_b", "preg0_b", "rhog0_b", "Srho_b", "Sw_b", "Spre_b"};
+    for (int i = 0; i < 7; ++i) NICAM_TRY(need(req[i], nm[i]));
+    if (c->d.have_pl) {
+        const void *rp[] = {wb_pl, w0b_pl, p0b_pl, r0b_pl, Srb_pl, Swb_pl, Spb_pl};
+        for (int i = 0; i < 7; ++i) NICAM_TRY(need(rp[i], "pole array"));
+    }
+    if (c->ptr_mode == NICAM_PTR_DEVICE)
+        return launch_vi_adjoint(c, wb, PL(wb_pl), w0b, PL(w0b_pl), p0b, PL(p0b_pl), r0b, PL(r0b_pl), Srb,
+                                 PL(Srb_pl), Swb, PL(Swb_pl), Spb, PL(Spb_pl), dt);
+    const size_t R = c->d.nreg() * 8, P = c->d.npl() * 8;
+    std::vector<HostArg> a = {{wb, R, true},  {PL(wb_pl), P, true},  {w0b, R, true}, {PL(w0b_pl), P, true},
+                              {p0b, R, true}, {PL(p0b_pl), P, true}, {r0b, R, true}, {PL(r0b_pl), P, true},
+                              {Srb, R, true}, {PL(Srb_pl), P, true}, {Swb, R, true}, {PL(Swb_pl), P, true},
+                              {Spb, R, true}, {PL(Spb_pl), P, true}};
+    std::vector<void *> v;
+    NICAM_TRY(stage_in(c, a, v));
+    auto D = [&](int i) { return static_cast<double *>(v[i]); };
+    NICAM_TRY(launch_vi_adjoint(c, D(0), D(1), D(2), D(3), D(4), D(5), D(6), D(7), D(8), D(9), D(10), D(11), D(12),
+                                D(13), dt));
+    return stage_out(c, a, v);
+}
+
+static size_t vlim_mask_words(const Dims &d, bool pole)
+{
+    const size_t nword = d.kall > 3 ? (size_t)(d.kall - 3 + 31) / 32 : 0;
+    return pole ? nword * d.gall_pl * kLallPl : nword * d.gall * d.lall;
+}
+
+int nicam_vertical_limiter_thuburn_mask(nicam_ctx c, double *q_h, double *q_h_pl, const double *q, const double *q_pl,
+                                        const double *d, const double *d_pl, const double *ck, const double *ck_pl,
+                                        uint32_t *mask, uint32_t *mask_pl)
+{
+    NICAM_TRY(check_ctx(c));
+    const void *req[] = {q_h, q, d, ck, mask};
+    const char *nm[] = {"q_h", "q", "d", "ck", "mask"};
+    for (int i = 0; i < 5; ++i) NICAM_TRY(need(req[i], nm[i]));
+    if (c->d.have_pl) {
+        const void *rp[] = {q_h_pl, q_pl, d_pl, ck_pl, mask_pl};
+        for (int i = 0; i < 5; ++i) NICAM_TRY(need(rp[i], "pole array"));
+    }
+    if (c->ptr_mode == NICAM_PTR_DEVICE)
+        return launch_vlim_mask(c, q_h, PL(q_h_pl), q, PL(q_pl), d, PL(d_pl), ck, PL(ck_pl), mask, PL(mask_pl));
+    const size_t R = c->d.nreg() * 8, P = c->d.npl() * 8;
+    const size_t M = vlim_mask_words(c->d, false) * 4, MP = vlim_mask_words(c->d, true) * 4;
+    std::vector<HostArg> a = {{q_h, R, true},       {PL(q_h_pl), P, true}, {q, R, false},
+                              {PL(q_pl), P, false},  {d, R, false},
+                              {PL(d_pl), P, false},  {ck, 2 * R, false},
+                              {PL(ck_pl), 2 * P, false}, {mask, M, true},
+                              {PL(mask_pl), MP, true}};
+    std::vector<void *> v;
+    NICAM_TRY(stage_in(c, a, v));
+    auto D = [&](int i) { return static_cast<double *>(v[i]); };
+    NICAM_TRY(launch_vlim_mask(c, D(0), D(1), D(2), D(3), D(4), D(5), D(6), D(7), static_cast<uint32_t *>(v[8]),
+                               static_cast<uint32_t *>(v[9])));
+    return stage_out(c, a, v);
+}
+
+int nicam_vertical_limiter_thuburn_adjoint(nicam_ctx c, double *q_h_b, double *q_h_b_pl, double *q_b, double *q_b_pl,
+                                           const double *q_h, const double *q_h_pl, const double *q,
+                                           const double *q_pl, const double *d, const double *d_pl, const double *ck,
+                                           const double *ck_pl, const uint32_t *mask, const uint32_t *mask_pl)
+{
+    NICAM_TRY(check_ctx(c));
+    const void *req[] = {q_h_b, q_b, q_h, q, d, ck, mask};
+    const char *nm[] = {"q_h_b", "q_b", "q_h", "q", "d", "ck", "mask"};
+    for (int i = 0; i < 7; ++i) NICAM_TRY(need(req[i], nm[i]));
+    if (c->d.have_pl) {
+        const void *rp[] = {q_h_b_pl, q_b_pl, q_h_pl, q_pl, d_pl, ck_pl, mask_pl};
+        for (int i = 0; i < 7; ++i) NICAM_TRY(need(rp[i], "pole array"));
+    }
+    if (c->ptr_mode == NICAM_PTR_DEVICE)
+        return launch_vlim_adjoint(c, q_h_b, PL(q_h_b_pl), q_b, PL(q_b_pl), q_h, PL(q_h_pl), q, PL(q_pl), d,
+                                   PL(d_pl), ck, PL(ck_pl), mask, PL(mask_pl));
+    const size_t R = c->d.nreg() * 8, P = c->d.npl() * 8;
+    const size_t M = vlim_mask_words(c->d, false) * 4, MP = vlim_mask_words(c->d, true) * 4;
+    std::vector<HostArg> a = {{q_h_b, R, true},        {PL(q_h_b_pl), P, true},      {q_b, R, true},
+                              {PL(q_b_pl), P, true},   {q_h, R, false},          {PL(q_h_pl), P, false},
+                              {q, R, false},       {PL(q_pl), P, false},     {d, R, false},
+                              {PL(d_pl), P, false}, {ck, 2 * R, false},      {PL(ck_pl), 2 * P, false},
+                              {mask, M, false},    {PL(mask_pl), MP, false}};
+    std::vector<void *> v;
+    NICAM_TRY(stage_in(c, a, v));
+    auto D = [&](int i) { return static_cast<double *>(v[i]); };
+    NICAM_TRY(launch_vlim_adjoint(c, D(0), D(1), D(2), D(3), D(4), D(5), D(6), D(7), D(8), D(9), D(10), D(11),
+                                  static_cast<const uint32_t *>(v[12]), static_cast<const uint32_t *>(v[13])));
+    return stage_out(c, a, v);
+}
+
 // ---- step glue (NICAM forms; not in the reference) ----
 int nicam_oprt_gradient(nicam_ctx c, double *grad, double *grad_pl, const double *scl, const double *scl_pl,
                         const double *coef_grad, const double *coef_grad_pl)

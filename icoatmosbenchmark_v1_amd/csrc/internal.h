@@ -241,6 +241,16 @@ int launch_divdamp(nicam_ctx c, double *const out[3], double *const out_pl[3], c
                    const double *rgsqrth, const double *rgsqrth_pl, const double *rgam, const double *rgam_pl,
                    const double *rgamh, const double *rgamh_pl, const double *c2w, const double *c2w_pl,
                    const double *rdgz_dev);
+int launch_vlim_mask(nicam_ctx c, double *q_h, double *q_h_pl, const double *q, const double *q_pl,
+                     const double *d, const double *d_pl, const double *ck, const double *ck_pl, uint32_t *mask,
+                     uint32_t *mask_pl);
+int launch_vlim_adjoint(nicam_ctx c, double *qhb, double *qhb_pl, double *qb, double *qb_pl, const double *q_h,
+                        const double *q_h_pl, const double *q, const double *q_pl, const double *d,
+                        const double *d_pl, const double *ck, const double *ck_pl, const uint32_t *mask,
+                        const uint32_t *mask_pl);
+int launch_vi_adjoint(nicam_ctx c, double *wb, double *wb_pl, double *w0b, double *w0b_pl, double *p0b,
+                      double *p0b_pl, double *r0b, double *r0b_pl, double *Srb, double *Srb_pl, double *Swb,
+                      double *Swb_pl, double *Spb, double *Spb_pl, double dt);
 int launch_vi(nicam_ctx c, double *w, double *w_pl, const double *w0, const double *w0_pl,
               const double *p0, const double *p0_pl, const double *r0, const double *r0_pl,
               const double *Sr, const double *Sr_pl, const double *Sw, const double *Sw_pl,

@@ -36,17 +36,22 @@ struct VlimArgs {
     int gall, kall, lall, gall_pl, have_pl;
     int kc, nrun;  // levels per work-item, runs per column
     T eps, big;
+    uint32_t *mask, *mask_pl;  // clip bits (masked variant only; kc = 32)
 };
 
 // Levels [kb, ke) of one column are limited (kb >= kmin+1, 0-based).
 // base = offset of level 0, s = stride between levels, ck2 = offset of the
 // upper-face plane of ck.  POLE selects the pole loop's product grouping of
 // CQin (VLIM:212-215 versus :126-129).
-template <bool POLE, class T>
+// MASK: also record one bit per limited face, set where the limited value
+// differs from q_h's input, in the word at mask_word (the run is the 32
+// faces of that word).
+template <bool POLE, class T, bool MASK = false>
 __device__ __forceinline__ void vlim_run(T *__restrict__ q_h, const T *__restrict__ q, const T *__restrict__ d,
                                          const T *__restrict__ ck, size_t base, size_t s, size_t ck2, int kb,
-                                         int ke, T EPS, T BIG)
+                                         int ke, T EPS, T BIG, uint32_t *mask_word = nullptr)
 {
+    uint32_t bits = 0;
     int k = kb - 1;  // first level whose Qout is needed
     T qkm1 = q[base + (size_t)(k - 1) * s];
     T qk = q[base + (size_t)k * s];
@@ -83,7 +88,9 @@ __device__ __forceinline__ void vlim_run(T *__restrict__ q_h, const T *__restric
                                   (T(1) - zerosw) + qk * zerosw;
         if (k >= kb) {  // VLIM:179-180
             const T qh = q_h[o];
-            stream_store(&q_h[o], (inflagL)*dmax(dmin(qh, Qmax_km1), Qmin_km1) + (T(1) - inflagL) * dmax(dmin(qh, Qmax_k), Qmin_k));
+            const T out = (inflagL)*dmax(dmin(qh, Qmax_km1), Qmin_km1) + (T(1) - inflagL) * dmax(dmin(qh, Qmax_k), Qmin_k);
+            stream_store(&q_h[o], out);
+            if constexpr (MASK) bits |= (out != qh ? 1u : 0u) << (k - kb);
         }
         Qmin_km1 = Qmin_k;
         Qmax_km1 = Qmax_k;
@@ -91,12 +98,13 @@ __device__ __forceinline__ void vlim_run(T *__restrict__ q_h, const T *__restric
         qk = qkp1;
         ck1k = ck1kp1;
     }
+    if constexpr (MASK) *mask_word = bits;
 }
 
 // A work-group is BLOCK consecutive columns at one run of levels.  The runs
 // of one column block are consecutive work-groups on one XCD: each run also
 // reads the two levels below it, which the run below has just brought into L2.
-template <class T>
+template <class T, bool MASK = false>
 __global__ void __launch_bounds__(BLOCK) vlim_kernel(VlimArgs<T> A)
 {
     const long ncol_reg = (long)A.gall * A.lall;
@@ -114,13 +122,15 @@ __global__ void __launch_bounds__(BLOCK) vlim_kernel(VlimArgs<T> A)
         const int g = (int)(col % A.gall), l = (int)(col / A.gall);
         const size_t base = (size_t)l * A.kall * A.gall + g;
         const size_t ck2 = (size_t)A.gall * A.kall * A.lall;
-        vlim_run<false, T>(A.q_h, A.q, A.d, A.ck, base, A.gall, ck2, kb, ke, A.eps, A.big);
+        uint32_t *mw = MASK ? A.mask + ((size_t)l * A.nrun + run) * A.gall + g : nullptr;
+        vlim_run<false, T, MASK>(A.q_h, A.q, A.d, A.ck, base, A.gall, ck2, kb, ke, A.eps, A.big, mw);
     } else {
         const int p = (int)(col - ncol_reg);
         const int g = p % A.gall_pl, l = p / A.gall_pl;
         const size_t base = (size_t)l * A.kall * A.gall_pl + g;
         const size_t ck2 = (size_t)A.gall_pl * A.kall * kLallPl;
-        vlim_run<true, T>(A.q_h_pl, A.q_pl, A.d_pl, A.ck_pl, base, A.gall_pl, ck2, kb, ke, A.eps, A.big);
+        uint32_t *mw = MASK ? A.mask_pl + ((size_t)l * A.nrun + run) * A.gall_pl + g : nullptr;
+        vlim_run<true, T, MASK>(A.q_h_pl, A.q_pl, A.d_pl, A.ck_pl, base, A.gall_pl, ck2, kb, ke, A.eps, A.big, mw);
     }
 }
 
@@ -130,7 +140,7 @@ namespace {
 
 template <class T>
 int launch_vlim_t(nicam_ctx c, T *q_h, T *q_h_pl, const T *q, const T *q_pl, const T *d, const T *d_pl,
-                  const T *ck, const T *ck_pl, T eps, T big)
+                  const T *ck, const T *ck_pl, T eps, T big, uint32_t *mask = nullptr, uint32_t *mask_pl = nullptr)
 {
     const Dims &D = c->d;
     VlimArgs<T> A;
@@ -149,17 +159,21 @@ int launch_vlim_t(nicam_ctx c, T *q_h, T *q_h_pl, const T *q, const T *q_pl, con
     A.have_pl = D.have_pl;
     A.eps = eps;
     A.big = big;
+    A.mask = mask;
+    A.mask_pl = mask_pl;
     const int nlev = D.kall - 3;  // limited levels kmin+1..kmax
     if (nlev <= 0) return NICAM_OK;
     // short runs give more independent work-items (measured best at gl7: 4)
     int kc = c->tune.vlim_kchunk > 0 ? c->tune.vlim_kchunk : 4;
+    if (mask) kc = 32;  // one mask word per run
     kc = std::min(kc, nlev);
     A.kc = kc;
     A.nrun = (nlev + kc - 1) / kc;
     const long ncol = (long)D.gall * D.lall + (D.have_pl ? (long)D.gall_pl * kLallPl : 0);
     const long nblk = (ncol + BLOCK - 1) / BLOCK * A.nrun;
     if (nblk > 0x7fffffffL) return fail(NICAM_ERR_ARG, "grid too large for one launch");
-    hipLaunchKernelGGL(vlim_kernel<T>, dim3((unsigned)nblk), dim3(BLOCK), 0, c->stream, A);
+    if (mask) hipLaunchKernelGGL((vlim_kernel<T, true>), dim3((unsigned)nblk), dim3(BLOCK), 0, c->stream, A);
+    else hipLaunchKernelGGL((vlim_kernel<T, false>), dim3((unsigned)nblk), dim3(BLOCK), 0, c->stream, A);
     NICAM_HIP(hipGetLastError());
     return NICAM_OK;
 }
@@ -170,6 +184,13 @@ int launch_vlim(nicam_ctx c, double *q_h, double *q_h_pl, const double *q, const
                 const double *d, const double *d_pl, const double *ck, const double *ck_pl)
 {
     return launch_vlim_t<double>(c, q_h, q_h_pl, q, q_pl, d, d_pl, ck, ck_pl, c->eps, c->big);
+}
+
+int launch_vlim_mask(nicam_ctx c, double *q_h, double *q_h_pl, const double *q, const double *q_pl,
+                     const double *d, const double *d_pl, const double *ck, const double *ck_pl, uint32_t *mask,
+                     uint32_t *mask_pl)
+{
+    return launch_vlim_t<double>(c, q_h, q_h_pl, q, q_pl, d, d_pl, ck, ck_pl, c->eps, c->big, mask, mask_pl);
 }
 
 int launch_vlim_f32(nicam_ctx c, float *q_h, float *q_h_pl, const float *q, const float *q_pl, const float *d,

@@ -180,6 +180,31 @@ module nicam_tt_f
        integer(c_int) :: nicam_vi_rhow_solver
      end function nicam_vi_rhow_solver
 
+     function nicam_vi_rhow_solver_adjoint(ctx, rhogw_b, rhogw_b_pl, rhogw0_b, rhogw0_b_pl, preg0_b, preg0_b_pl, &
+          rhog0_b, rhog0_b_pl, Srho_b, Srho_b_pl, Sw_b, Sw_b_pl, Spre_b, Spre_b_pl, dt) &
+          bind(C, name='nicam_vi_rhow_solver_adjoint')
+       import :: c_ptr, c_int, c_double
+       type(c_ptr), value :: ctx, rhogw_b, rhogw_b_pl, rhogw0_b, rhogw0_b_pl, preg0_b, preg0_b_pl
+       type(c_ptr), value :: rhog0_b, rhog0_b_pl, Srho_b, Srho_b_pl, Sw_b, Sw_b_pl, Spre_b, Spre_b_pl
+       real(c_double), value :: dt
+       integer(c_int) :: nicam_vi_rhow_solver_adjoint
+     end function nicam_vi_rhow_solver_adjoint
+
+     function nicam_vertical_limiter_thuburn_mask(ctx, q_h, q_h_pl, q, q_pl, d, d_pl, ck, ck_pl, mask, mask_pl) &
+          bind(C, name='nicam_vertical_limiter_thuburn_mask')
+       import :: c_ptr, c_int
+       type(c_ptr), value :: ctx, q_h, q_h_pl, q, q_pl, d, d_pl, ck, ck_pl, mask, mask_pl
+       integer(c_int) :: nicam_vertical_limiter_thuburn_mask
+     end function nicam_vertical_limiter_thuburn_mask
+
+     function nicam_vertical_limiter_thuburn_adjoint(ctx, q_h_b, q_h_b_pl, q_b, q_b_pl, q_h, q_h_pl, q, q_pl, &
+          d, d_pl, ck, ck_pl, mask, mask_pl) bind(C, name='nicam_vertical_limiter_thuburn_adjoint')
+       import :: c_ptr, c_int
+       type(c_ptr), value :: ctx, q_h_b, q_h_b_pl, q_b, q_b_pl, q_h, q_h_pl, q, q_pl, d, d_pl, ck, ck_pl
+       type(c_ptr), value :: mask, mask_pl
+       integer(c_int) :: nicam_vertical_limiter_thuburn_adjoint
+     end function nicam_vertical_limiter_thuburn_adjoint
+
      function nicam_ctx_synchronize(ctx) bind(C, name='nicam_ctx_synchronize')
        import :: c_ptr, c_int
        type(c_ptr), value :: ctx

@@ -107,6 +107,9 @@ SIGNATURES = {
     "nicam_set_rdgz": (c_int, [_P, _P]),
     "nicam_set_vmetrics3d": (c_int, [_P] + [_P] * 8),
     "nicam_oprt3d_divdamp": (c_int, [_P] + [_P] * 18),
+    "nicam_vi_rhow_solver_adjoint": (c_int, [_P] + [_P] * 14 + [c_double]),
+    "nicam_vertical_limiter_thuburn_mask": (c_int, [_P] * 11),
+    "nicam_vertical_limiter_thuburn_adjoint": (c_int, [_P] * 15),
 }
 
 
@@ -173,6 +176,30 @@ def _ptr(x, mode: int, name: str, numel: Optional[int] = None, single: bool = Fa
     want = torch.float32 if single else torch.float64
     if x.dtype != want or not x.is_contiguous():
         raise TypeError(f"{name}: need a contiguous {want} tensor")
+    if numel is not None and x.numel() != numel:
+        raise ValueError(f"{name}: has {x.numel()} elements, expected {numel}")
+    return x.data_ptr()
+
+
+def _ptr_u32(x, mode: int, name: str, numel: Optional[int] = None):
+    """Raw pointer of a 32-bit mask array: numpy uint32 (host mode) or a
+    torch int32/uint32 tensor on the GPU (device mode)."""
+    if x is None:
+        return None
+    if isinstance(x, np.ndarray):
+        if mode != PTR_HOST:
+            raise TypeError(f"{name}: numpy array passed in device pointer mode")
+        if x.dtype != np.uint32 or not x.flags.c_contiguous:
+            raise TypeError(f"{name}: need a C-contiguous uint32 array")
+        if numel is not None and x.size != numel:
+            raise ValueError(f"{name}: has {x.size} elements, expected {numel}")
+        return x.ctypes.data
+    import torch
+
+    if not isinstance(x, torch.Tensor) or mode != PTR_DEVICE or not x.is_cuda:
+        raise TypeError(f"{name}: need a GPU tensor in device pointer mode")
+    if x.dtype not in (torch.int32, getattr(torch, "uint32", torch.int32)) or not x.is_contiguous():
+        raise TypeError(f"{name}: need a contiguous 32-bit integer tensor")
     if numel is not None and x.numel() != numel:
         raise ValueError(f"{name}: has {x.numel()} elements, expected {numel}")
     return x.data_ptr()
@@ -530,6 +557,48 @@ class Context:
                          (coef_intp_pl, "coef_intp_pl", 18 * self.gall_pl), (coef_diff, "coef_diff", 18 * G),
                          (coef_diff_pl, "coef_diff_pl", 30)])
         _check(self.lib.nicam_oprt3d_divdamp(self.h, *a), "nicam_oprt3d_divdamp")
+
+    def vlim_mask_words(self):
+        """uint32 words of the vertical limiter's branch mask: (regions, poles)."""
+        nword = (self.kall - 3 + 31) // 32 if self.kall > 3 else 0
+        return nword * self.gall * self.lall, nword * self.gall_pl * 2
+
+    def vertical_limiter_thuburn_mask(self, q_h, q_h_pl, q, q_pl, d, d_pl, ck, ck_pl, mask, mask_pl) -> None:
+        """nicam_vertical_limiter_thuburn_mask: the limiter plus its clip bits (uint32 arrays)."""
+        R, P = self.nreg, self.npl
+        M, MP = self.vlim_mask_words()
+        a = self._args([(q_h, "q_h", R), (q_h_pl, "q_h_pl", P), (q, "q", R), (q_pl, "q_pl", P), (d, "d", R),
+                        (d_pl, "d_pl", P), (ck, "ck", 2 * R), (ck_pl, "ck_pl", 2 * P)])
+        a += [_ptr_u32(mask, self.mode, "mask", M), None if not self.have_pl else _ptr_u32(mask_pl, self.mode,
+                                                                                           "mask_pl", MP)]
+        _check(self.lib.nicam_vertical_limiter_thuburn_mask(self.h, *a), "nicam_vertical_limiter_thuburn_mask")
+
+    def vertical_limiter_thuburn_adjoint(self, q_h_b, q_h_b_pl, q_b, q_b_pl, q_h, q_h_pl, q, q_pl, d, d_pl, ck,
+                                         ck_pl, mask, mask_pl) -> None:
+        """nicam_vertical_limiter_thuburn_adjoint: cotangents of q_h's input and of q."""
+        R, P = self.nreg, self.npl
+        M, MP = self.vlim_mask_words()
+        a = self._args([(q_h_b, "q_h_b", R), (q_h_b_pl, "q_h_b_pl", P), (q_b, "q_b", R), (q_b_pl, "q_b_pl", P),
+                        (q_h, "q_h", R), (q_h_pl, "q_h_pl", P), (q, "q", R), (q_pl, "q_pl", P), (d, "d", R),
+                        (d_pl, "d_pl", P), (ck, "ck", 2 * R), (ck_pl, "ck_pl", 2 * P)])
+        a += [_ptr_u32(mask, self.mode, "mask", M), None if not self.have_pl else _ptr_u32(mask_pl, self.mode,
+                                                                                           "mask_pl", MP)]
+        _check(self.lib.nicam_vertical_limiter_thuburn_adjoint(self.h, *a),
+               "nicam_vertical_limiter_thuburn_adjoint")
+
+    def vi_rhow_solver_adjoint(self, rhogw_b, rhogw_b_pl, rhogw0_b, rhogw0_b_pl, preg0_b, preg0_b_pl, rhog0_b,
+                               rhog0_b_pl, Srho_b, Srho_b_pl, Sw_b, Sw_b_pl, Spre_b, Spre_b_pl, dt: float) -> None:
+        """nicam_vi_rhow_solver_adjoint: rhogw_b in/out, the six source cotangents written."""
+        R, P = self.nreg, self.npl
+        names = ["rhogw_b", "rhogw0_b", "preg0_b", "rhog0_b", "Srho_b", "Sw_b", "Spre_b"]
+        arrs = [rhogw_b, rhogw_b_pl, rhogw0_b, rhogw0_b_pl, preg0_b, preg0_b_pl, rhog0_b, rhog0_b_pl, Srho_b,
+                Srho_b_pl, Sw_b, Sw_b_pl, Spre_b, Spre_b_pl]
+        spec = []
+        for i, nm in enumerate(names):
+            spec.append((arrs[2 * i], nm, R))
+            spec.append((arrs[2 * i + 1], nm + "_pl", P))
+        _check(self.lib.nicam_vi_rhow_solver_adjoint(self.h, *self._args(spec), float(dt)),
+               "nicam_vi_rhow_solver_adjoint")
 
     def vi_rhow_solver(self, rhogw, rhogw_pl, rhogw0, rhogw0_pl, preg0, preg0_pl, rhog0, rhog0_pl,
                        Srho, Srho_pl, Sw, Sw_pl, Spre, Spre_pl, dt: float) -> None:

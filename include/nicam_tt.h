@@ -36,6 +36,7 @@
 #define NICAM_TT_H
 
 #include <stddef.h>
+#include <stdint.h>
 
 #ifdef __cplusplus
 extern "C" {
@@ -491,6 +492,60 @@ int nicam_vi_rhow_update_matrix(nicam_ctx ctx, const double *eth, const double *
 /* Copies the context's Mc/Ml/Mu (gall,kall,lall), _pl (gall_pl,kall,2) out. */
 int nicam_vi_rhow_getcoef(nicam_ctx ctx, double *Mc, double *Mc_pl, double *Ml, double *Ml_pl, double *Mu,
                           double *Mu_pl);
+
+/* ---- Adjoints (hand-written reverse of the kernels above) ----------------
+ * Each computes the transpose of its forward kernel's derivative at fixed
+ * coefficients.  No counterpart exists in the reference: these replace no
+ * reference interface; they serve gradient computations through the
+ * forward kernels (data assimilation, parameter estimation).
+ *
+ * vi_rhow_solver adjoint (transpose of mod_vi.f90:93-306 as a linear map of
+ * rhogw(kmin), rhogw(kmax+1), rhogw0, preg0, rhog0, Srho, Sw, Spre, at the
+ * context's Mc/Ml/Mu, metrics and the given dt):
+ *   inout rhogw_b   in: cotangent of the solver's output rhogw;
+ *                   out: cotangent of its input rhogw (zero on the levels the
+ *                   solver overwrites without reading; level 1 passes through)
+ *   out   rhogw0_b, preg0_b, rhog0_b, Srho_b, Sw_b, Spre_b (every level written)
+ * Same set-up as nicam_vi_rhow_solver. */
+int nicam_vi_rhow_solver_adjoint(nicam_ctx ctx,
+                                 double *rhogw_b, double *rhogw_b_pl,
+                                 double *rhogw0_b, double *rhogw0_b_pl,
+                                 double *preg0_b, double *preg0_b_pl,
+                                 double *rhog0_b, double *rhog0_b_pl,
+                                 double *Srho_b, double *Srho_b_pl,
+                                 double *Sw_b, double *Sw_b_pl,
+                                 double *Spre_b, double *Spre_b_pl,
+                                 double dt);
+
+/* vertical_limiter_thuburn that also records its branch mask: the same
+ * arguments and results as nicam_vertical_limiter_thuburn, plus one bit per
+ * limited face (levels kmin+1..kmax), set where the limited value differs
+ * from q_h's input.  Layout: uint32 word ((l*nword + p/32)*gall + g), bit
+ * p%32, p = k-(kmin+1), nword = ceil((kall-3)/32); mask_pl the same over
+ * gall_pl and the 2 pole regions. */
+int nicam_vertical_limiter_thuburn_mask(nicam_ctx ctx,
+                                        double *q_h, double *q_h_pl,
+                                        const double *q, const double *q_pl,
+                                        const double *d, const double *d_pl,
+                                        const double *ck, const double *ck_pl,
+                                        uint32_t *mask, uint32_t *mask_pl);
+
+/* Adjoint of vertical_limiter_thuburn with respect to q_h's input and q, at
+ * fixed ck and d (dyn_vert_adv_limiter/src/mod_src_tracer.f90:41-243):
+ *   inout q_h_b  in: cotangent of the limited q_h; out: cotangent of q_h's
+ *                input (faces outside kmin+1..kmax pass through)
+ *   out   q_b    cotangent of q (every level written)
+ *   in    q_h    the forward's output; q, d, ck its inputs; mask its mask
+ * A clipped face takes the lower bound when its value equals Qout_min of the
+ * upwind level, else the upper bound. */
+int nicam_vertical_limiter_thuburn_adjoint(nicam_ctx ctx,
+                                           double *q_h_b, double *q_h_b_pl,
+                                           double *q_b, double *q_b_pl,
+                                           const double *q_h, const double *q_h_pl,
+                                           const double *q, const double *q_pl,
+                                           const double *d, const double *d_pl,
+                                           const double *ck, const double *ck_pl,
+                                           const uint32_t *mask, const uint32_t *mask_pl);
 
 #ifdef __cplusplus
 }
