@@ -85,6 +85,8 @@ class Oracle:
         L.orc_vi_rhow_solver_adjoint.argtypes = ([POINTER(OrcDims)] + [P] * 14 + [c_double] + [P] * 16 + [P] * 3 +
                                                  [c_double, c_double, c_double, c_int])
         L.orc_vertical_limiter_thuburn_tlm.argtypes = [POINTER(OrcDims)] + [P] * 16
+        L.orc_horizontal_limiter_thuburn_tlm.argtypes = [POINTER(OrcDims), P] + [P] * 22
+        L.orc_horizontal_limiter_thuburn_adjoint.argtypes = [POINTER(OrcDims), P] + [P] * 22
         L.orc_vertical_limiter_thuburn_adjoint.argtypes = [POINTER(OrcDims)] + [P] * 14
         L.orc_vi_rhow_solver.argtypes = ([POINTER(OrcDims)] + [P] * 14 + [c_double] + [P] * 16 + [P] * 3 +
                                          [c_double, c_double, c_double, c_int])
@@ -285,6 +287,40 @@ class Oracle:
                    q_b_pl=np.zeros_like(ins["q_pl"]))
         self.lib.orc_vertical_limiter_thuburn_adjoint(ctypes.byref(D), _p(out["q_h_b"]), _p(out["q_h_b_pl"]),
                                                       _p(out["q_b"]), _p(out["q_b_pl"]), *fw)
+        return out
+
+    def horizontal_limiter_thuburn_lin(self, spec, have_sgp, f, q_a_out, mask, mask_pl, adjoint, a):
+        """TLM (adjoint=False; a = dq_a_in, dq, dQpost (+_pl) -> dq_a, dQprev) or adjoint (adjoint=True;
+        a = q_a_b, Qprev_b (+_pl) -> q_a_b, q_b, Qpost_b) of the horizontal limiter with Qout_post
+        supplied, at the forward state f (inputs), q_a_out (its output) and its mask."""
+        D = self.dims(spec)
+        sgp_a = np.ascontiguousarray(have_sgp, dtype=np.int32)
+        sgp = sgp_a.ctypes.data
+        ins = {k: _f64(f[k]) for k in ("q", "q_pl", "d", "d_pl", "ch", "ch_pl", "cmask", "cmask_pl", "Qout_post",
+                                          "Qout_post_pl")}
+        qa = {k: _f64(v) for k, v in q_a_out.items()}
+        m = np.ascontiguousarray(mask, dtype=np.uint8)
+        mp = np.ascontiguousarray(mask_pl, dtype=np.uint8)
+        fw = [_p(qa["q_a"]), _p(qa["q_a_pl"])] + [_p(ins[k]) for k in ("q", "q_pl", "d", "d_pl", "ch", "ch_pl",
+                                                                        "cmask", "cmask_pl", "Qout_post",
+                                                                        "Qout_post_pl")] + [m.ctypes.data,
+                                                                                            mp.ctypes.data]
+        x = {k: _f64(v) for k, v in a.items()}
+        if not adjoint:
+            out = dict(dq_a=x["dq_a_in"].copy(), dq_a_pl=x["dq_a_in_pl"].copy(),
+                       dQprev=np.zeros_like(ins["Qout_post"]), dQprev_pl=np.zeros_like(ins["Qout_post_pl"]))
+            self.lib.orc_horizontal_limiter_thuburn_tlm(ctypes.byref(D), sgp, _p(out["dq_a"]), _p(out["dq_a_pl"]),
+                                                        _p(x["dq"]), _p(x["dq_pl"]), _p(out["dQprev"]),
+                                                        _p(out["dQprev_pl"]), _p(x["dQpost"]), _p(x["dQpost_pl"]),
+                                                        *fw)
+            return out
+        out = dict(q_a_b=x["q_a_b"].copy(), q_a_b_pl=x["q_a_b_pl"].copy(), q_b=np.zeros_like(ins["q"]),
+                   q_b_pl=np.zeros_like(ins["q_pl"]), Qpost_b=np.zeros_like(ins["Qout_post"]),
+                   Qpost_b_pl=np.zeros_like(ins["Qout_post_pl"]))
+        self.lib.orc_horizontal_limiter_thuburn_adjoint(ctypes.byref(D), sgp, _p(out["q_a_b"]), _p(out["q_a_b_pl"]),
+                                                        _p(out["q_b"]), _p(out["q_b_pl"]), _p(x["Qprev_b"]),
+                                                        _p(x["Qprev_b_pl"]), _p(out["Qpost_b"]),
+                                                        _p(out["Qpost_b_pl"]), *fw)
         return out
 
     def vi_rhow_solver_adjoint(self, spec, f, b, dt, rdgzh, afact, bfact, grav=9.80616, rdry=287.0, cvdry=717.5,

@@ -1510,3 +1510,312 @@ void orc_vertical_limiter_thuburn_adjoint(const orc_dims *D, double *qhb, double
 {
     vlim_lin(D, 1, qhb, qhb_pl, NULL, NULL, qb, qb_pl, q_h, q_h_pl, q, q_pl, d, d_pl, ck, ck_pl, mask, mask_pl);
 }
+
+/* ------------------------------------------------------------------------- */
+/* Tangent-linear and adjoint of horizontal_limiter_thuburn                  */
+/* (dyn_horiz_adv_limiter/src/mod_src_tracer.f90:42-430) with Qout_post       */
+/* supplied, at fixed d, ch and cmask.                                        */
+/* ------------------------------------------------------------------------- */
+/* Inputs that matter: q_a (faces 1-3 of cells [gmin-1,gmax]^2), q and
+ * Qout_post; outputs: q_a and Qout_prev.  Qin of an arc is the min (max) of
+ * its four tracer values, a selection; Qout of an inner cell is affine in q
+ * given the selections of qnext and of its six arcs; a halo cell's Qout is q.
+ * A face value is its input, a Qout_post bound of the downwind cell, or a Qin
+ * bound of its arc.  The forward records one bit per face (bit m-1 of the
+ * cell's mask byte: the value changed); a changed value is attributed, in
+ * this order, to Qout_post min, Qout_post max, Qin min, Qin max, by equality
+ * with the stored output.  Pole faces: bit 0 of the ring point's byte. */
+typedef struct {
+    int cell[4];
+} hl_arc;
+
+/* the 4 cells of arc m (1=AI, 2=AIJ, 3=AJ) of cell ij (HLIM:145-160, :197-203) */
+static hl_arc hl_arc_cells(int ij, int m, int iall, int sgp_cell)
+{
+    hl_arc a;
+    if (m == 1) {
+        const int ijm1 = (ij - iall) > 1 ? ij - iall : 1;
+        a.cell[0] = ij; a.cell[1] = ijm1; a.cell[2] = ij + 1; a.cell[3] = ij + iall + 1;
+    } else if (m == 2) {
+        if (sgp_cell) { a.cell[0] = ij; a.cell[1] = ij + iall + 1; a.cell[2] = ij + iall + 2; a.cell[3] = ij + iall; }
+        else { a.cell[0] = ij; a.cell[1] = ij + 1; a.cell[2] = ij + iall + 1; a.cell[3] = ij + iall; }
+    } else {
+        const int im1j = (ij - 1) > 1 ? ij - 1 : 1;
+        a.cell[0] = ij; a.cell[1] = ij + iall + 1; a.cell[2] = ij + iall; a.cell[3] = im1j;
+    }
+    return a;
+}
+
+/* cells holding the min and the max of the arc, following mn/mx's chains */
+static void hl_arc_sel(const double *qs, const hl_arc *a, int *cmin, int *cmax, double *vmin, double *vmax)
+{
+    double lo = qs[a->cell[0] - 1], hi = lo;
+    int il = a->cell[0], ih = a->cell[0];
+    for (int t = 1; t < 4; ++t) {
+        const double v = qs[a->cell[t] - 1];
+        if (v < lo) { lo = v; il = a->cell[t]; }
+        if (v > hi) { hi = v; ih = a->cell[t]; }
+    }
+    *cmin = il; *cmax = ih; *vmin = lo; *vmax = hi;
+}
+
+static int hl_off(int m, int iall) { return m == 1 ? 1 : (m == 2 ? iall + 1 : iall); }
+
+/* mode 0 = TLM, 1 = adjoint; one (k, l) slab of a regular region.
+ * TLM:     dqa (in dq_a_in, out dq_a), dq, dQpost -> dQprev
+ * adjoint: qab (in/out), Qprevb -> qb (+=), Qpostb (+=) */
+static void hl_slab(int iall, int sgp, const double *qs, const double *ds, const double *chs, const double *cms,
+                    size_t cplane, const double *qas_out, const double *Qpost_s, size_t qplane,
+                    const uint8_t *mk, double EPS, double BIG, int adjoint, double *xa, size_t aplane,
+                    double *xq, double *xQprev, double *xQpost)
+{
+    const int gall = iall * iall, gmin = 2, gmax = iall - 1;
+    const int sgp_cell = sgp ? (gmin - 1 - 1) * iall + (gmin - 1) : -1;
+#define CM(g, f) cms[(size_t)((f) - 1) * cplane + (g) - 1]
+#define CH(g, f) chs[(size_t)((f) - 1) * cplane + (g) - 1]
+    /* Qout of each cell: coefficient rows (up to 14 terms) */
+    for (int g = 1; g <= gall; ++g) {
+        const int i = (g - 1) % iall + 1, j = (g - 1) / iall + 1;
+        if (i < gmin || i > gmax || j < gmin || j > gmax) { /* halo: Qout = q (HLIM:286-301) */
+            for (int mm = 0; mm < 2; ++mm) {
+                if (adjoint) xq[g - 1] = xq[g - 1] + xQprev[(size_t)mm * qplane + g - 1];
+                else xQprev[(size_t)mm * qplane + g - 1] = xq[g - 1];
+            }
+            continue;
+        }
+        /* the six arcs of g: f=1..3 its own, f=4..6 the neighbours' (m = f-3) */
+        int cmn[7], cmx[7];
+        double vmn[7], vmx[7], cmin_c[7], cmax_c[7];
+        for (int f = 1; f <= 6; ++f) {
+            const int owner = f <= 3 ? g : g - hl_off(f - 3, iall);
+            const int m = f <= 3 ? f : f - 3;
+            hl_arc a = hl_arc_cells(owner, m, iall, owner == sgp_cell && m == 2);
+            hl_arc_sel(qs, &a, &cmn[f], &cmx[f], &vmn[f], &vmx[f]);
+            const double c = CM(owner, m);
+            cmin_c[f] = f <= 3 ? c : 1.0 - c; /* d QIN / d (arc extreme) */
+            cmax_c[f] = cmin_c[f];
+            /* the forward's values, with the sentinels */
+            if (f <= 3) { vmn[f] = c * vmn[f] + (1.0 - c) * BIG; vmx[f] = c * vmx[f] + (1.0 - c) * (-BIG); }
+            else { vmn[f] = c * BIG + (1.0 - c) * vmn[f]; vmx[f] = c * (-BIG) + (1.0 - c) * vmx[f]; }
+        }
+        const double qij = qs[g - 1];
+        int smin = 0, smax = 0;
+        double nmin = qij, nmax = qij;
+        for (int f = 1; f <= 6; ++f) {
+            if (vmn[f] < nmin) { nmin = vmn[f]; smin = f; }
+            if (vmx[f] > nmax) { nmax = vmx[f]; smax = f; }
+        }
+        double chv[7], chm[7];
+        for (int f = 1; f <= 6; ++f) { chv[f] = CH(g, f); chm[f] = mn(chv[f], 0.0); }
+        const double Cin = chm[1] + chm[2] + chm[3] + chm[4] + chm[5] + chm[6];
+        const double Cout = chv[1] - chm[1] + chv[2] - chm[2] + chv[3] - chm[3] + chv[4] - chm[4] + chv[5] - chm[5] + chv[6] - chm[6];
+        const double zerosw = 0.5 - fsign(0.5, fabs(Cout) - EPS);
+        const double E = 1.0 - Cin - Cout + ds[g - 1];
+        const double fac = (1.0 - zerosw) / (Cout + zerosw);
+        /* Qout min (mm=0) uses the max side, Qout max (mm=1) the min side */
+        for (int mm = 0; mm < 2; ++mm) {
+            const int sel = mm == 0 ? smax : smin;
+            const int *cx = mm == 0 ? cmx : cmn;
+            const double *cc = mm == 0 ? cmax_c : cmin_c;
+            int cells[14];
+            double coef[14];
+            int n = 0;
+            cells[n] = g; coef[n++] = fac + zerosw;
+            cells[n] = sel == 0 ? g : cx[sel]; coef[n++] = -(fac * E) * (sel == 0 ? 1.0 : cc[sel]);
+            for (int f = 1; f <= 6; ++f) { cells[n] = cx[f]; coef[n++] = -(fac * chm[f]) * cc[f]; }
+            const size_t o = (size_t)mm * qplane + g - 1;
+            if (adjoint) {
+                const double b = xQprev[o];
+                for (int t = 0; t < n; ++t) xq[cells[t] - 1] = xq[cells[t] - 1] + coef[t] * b;
+            } else {
+                double v = 0.0;
+                for (int t = 0; t < n; ++t) v = v + coef[t] * xq[cells[t] - 1];
+                xQprev[o] = v;
+            }
+        }
+    }
+    /* faces 1-3 of [gmin-1, gmax]^2 (HLIM:377-410) */
+    for (int j = gmin - 1; j <= gmax; ++j)
+        for (int i = gmin - 1; i <= gmax; ++i) {
+            const int ij = (j - 1) * iall + i;
+            for (int m = 1; m <= 3; ++m) {
+                const int nb = ij + hl_off(m, iall);
+                const size_t fo = (size_t)(m - 1) * aplane + ij - 1, fn = (size_t)(m + 2) * aplane + nb - 1;
+                const int clipped = (mk[ij - 1] >> (m - 1)) & 1;
+                const double c = CM(ij, m);
+                const int T = c > 0.0 ? nb : ij;
+                /* attribution: 0 input, 1 Qpost min, 2 Qpost max, 3 Qin min cell, 4 Qin max cell */
+                int kind = 0, cell = 0;
+                if (clipped) {
+                    const double out = qas_out[fo];
+                    hl_arc a = hl_arc_cells(ij, m, iall, ij == sgp_cell && m == 2);
+                    int amn, amx;
+                    double vmn, vmx;
+                    hl_arc_sel(qs, &a, &amn, &amx, &vmn, &vmx);
+                    if (out == Qpost_s[T - 1]) kind = 1;
+                    else if (out == Qpost_s[qplane + T - 1]) kind = 2;
+                    else if (out == vmn) { kind = 3; cell = amn; }
+                    else { kind = 4; cell = amx; }
+                }
+                if (adjoint) {
+                    const double ob = xa[fo] + xa[fn];
+                    xa[fn] = 0.0;
+                    xa[fo] = kind == 0 ? ob : 0.0;
+                    if (kind == 1) xQpost[T - 1] = xQpost[T - 1] + ob;
+                    else if (kind == 2) xQpost[qplane + T - 1] = xQpost[qplane + T - 1] + ob;
+                    else if (kind >= 3) xq[cell - 1] = xq[cell - 1] + ob;
+                } else {
+                    double v;
+                    if (kind == 0) v = xa[fo];
+                    else if (kind == 1) v = xQpost[T - 1];
+                    else if (kind == 2) v = xQpost[qplane + T - 1];
+                    else v = xq[cell - 1];
+                    xa[fo] = v;
+                    xa[fn] = v;
+                }
+            }
+        }
+#undef CM
+#undef CH
+}
+
+/* one (k, l) slab of a pole region (HLIM:307-363, 412-425); arrays at the slab */
+static void hl_pole_slab(int npl, const double *qs, double dn, const double *chs, const double *cms,
+                         const double *qas_out, const double *Qpost_s, size_t pplane, const uint8_t *mk, double EPS,
+                         double BIG, int adjoint, double *xa, double *xq, double *xQprev, double *xQpost)
+{
+    /* ring point v (1-based 2..npl): arc {centre, v, v-1, v+1} */
+    int amn[16], amx[16];
+    double vmn[16], vmx[16];
+    for (int v = 2; v <= npl; ++v) {
+        int vp1 = v + 1, vm1 = v - 1;
+        if (vp1 == npl + 1) vp1 = 2;
+        if (vm1 == 1) vm1 = npl;
+        hl_arc a = {{1, v, vm1, vp1}};
+        hl_arc_sel(qs, &a, &amn[v], &amx[v], &vmn[v], &vmx[v]);
+    }
+    const double qn = qs[0];
+    int smin = 0, smax = 0;
+    double nmin = qn, nmax = qn;
+    for (int v = 2; v <= npl; ++v) {
+        const double c = cms[v - 1];
+        const double mn1 = c * vmn[v] + (1.0 - c) * BIG, mx1 = c * vmx[v] + (1.0 - c) * (-BIG);
+        if (mn1 < nmin) { nmin = mn1; smin = v; }
+        if (mx1 > nmax) { nmax = mx1; smax = v; }
+    }
+    double Cin = 0.0, Cout = 0.0;
+    for (int v = 2; v <= npl; ++v) {
+        const double chm = cms[v - 1] * chs[v - 1];
+        Cin = Cin + chm;
+        Cout = Cout - chm + chs[v - 1];
+    }
+    const double zerosw = 0.5 - fsign(0.5, fabs(Cout) - EPS);
+    const double E = 1.0 - Cin - Cout + dn;
+    const double fac = (1.0 - zerosw) / (Cout + zerosw);
+    for (int mm = 0; mm < 2; ++mm) {
+        const int sel = mm == 0 ? smax : smin;
+        const int *cx = mm == 0 ? amx : amn;
+        int cells[20];
+        double coef[20];
+        int n = 0;
+        cells[n] = 1; coef[n++] = fac + zerosw;
+        cells[n] = sel == 0 ? 1 : cx[sel]; coef[n++] = -(fac * E) * (sel == 0 ? 1.0 : cms[sel - 1]);
+        for (int v = 2; v <= npl; ++v) { cells[n] = cx[v]; coef[n++] = -(fac * (cms[v - 1] * chs[v - 1])) * cms[v - 1]; }
+        const size_t o = (size_t)mm * pplane;
+        if (adjoint) {
+            const double b = xQprev[o];
+            for (int t = 0; t < n; ++t) xq[cells[t] - 1] = xq[cells[t] - 1] + coef[t] * b;
+        } else {
+            double v = 0.0;
+            for (int t = 0; t < n; ++t) v = v + coef[t] * xq[cells[t] - 1];
+            xQprev[o] = v;
+            for (int r = 2; r <= npl; ++r) xQprev[o + r - 1] = 0.0; /* the ring is zeroed (HLIM:367-370) */
+        }
+    }
+    for (int v = 2; v <= npl; ++v) {
+        const int clipped = mk[v - 1] & 1;
+        const double c = cms[v - 1];
+        const int T = c > 0.0 ? v : 1;
+        int kind = 0, cell = 0;
+        if (clipped) {
+            const double out = qas_out[v - 1];
+            if (out == Qpost_s[T - 1]) kind = 1;
+            else if (out == Qpost_s[pplane + T - 1]) kind = 2;
+            else if (out == vmn[v]) { kind = 3; cell = amn[v]; }
+            else { kind = 4; cell = amx[v]; }
+        }
+        if (adjoint) {
+            const double ob = xa[v - 1];
+            xa[v - 1] = kind == 0 ? ob : 0.0;
+            if (kind == 1) xQpost[T - 1] = xQpost[T - 1] + ob;
+            else if (kind == 2) xQpost[pplane + T - 1] = xQpost[pplane + T - 1] + ob;
+            else if (kind >= 3) xq[cell - 1] = xq[cell - 1] + ob;
+        } else {
+            xa[v - 1] = kind == 0 ? xa[v - 1] : (kind == 1 ? xQpost[T - 1] : (kind == 2 ? xQpost[pplane + T - 1] : xq[cell - 1]));
+        }
+    }
+}
+
+/* TLM: dq_a (in: dq_a_in, out: dq_a), dQprev written from dq and dQpost.
+ * Adjoint: q_a_b in/out, qb and Qpostb written (zeroed first), from
+ * Qprevb.  q_a is the forward's output, mask its mask (uint8 per cell-level,
+ * uint8 per pole point). */
+static void hlim_lin(const orc_dims *D, const int *have_sgp, int adjoint, double *xa, double *xa_pl, double *xq,
+                     double *xq_pl, double *xQprev, double *xQprev_pl, double *xQpost, double *xQpost_pl,
+                     const double *q_a, const double *q_a_pl, const double *q, const double *q_pl, const double *d,
+                     const double *d_pl, const double *ch, const double *ch_pl, const double *cmask,
+                     const double *cmask_pl, const double *Qpost, const double *Qpost_pl, const uint8_t *mask,
+                     const uint8_t *mask_pl)
+{
+    const int iall = D->gall_1d, gall = iall * iall, kall = D->kall, lall = D->lall, npl = D->gall_pl;
+    const size_t nreg = (size_t)gall * kall * lall, npole = (size_t)npl * kall * LALL_PL;
+    if (adjoint) {
+        memset(xq, 0, nreg * sizeof(double));
+        memset(xQpost, 0, 2 * nreg * sizeof(double));
+        if (D->have_pl) {
+            memset(xq_pl, 0, npole * sizeof(double));
+            memset(xQpost_pl, 0, 2 * npole * sizeof(double));
+        }
+    }
+    for (int l = 0; l < lall; ++l)
+#pragma omp parallel for schedule(static)
+        for (int k = 0; k < kall; ++k) {
+            const size_t s = ((size_t)l * kall + k) * gall;
+            hl_slab(iall, have_sgp[l], q + s, d + s, ch + s, cmask + s, nreg, q_a + s, Qpost + s, nreg, mask + s,
+                    D->eps, D->big, adjoint, xa + s, nreg, xq + s, xQprev + s, xQpost + s);
+        }
+    if (D->have_pl)
+        for (int l = 0; l < LALL_PL; ++l)
+            for (int k = 0; k < kall; ++k) {
+                const size_t s = ((size_t)l * kall + k) * npl;
+                hl_pole_slab(npl, q_pl + s, d_pl[s], ch_pl + s, cmask_pl + s, q_a_pl + s, Qpost_pl + s, npole,
+                             mask_pl + s, D->eps, D->big, adjoint, xa_pl + s, xq_pl + s, xQprev_pl + s,
+                             xQpost_pl + s);
+            }
+}
+
+void orc_horizontal_limiter_thuburn_tlm(const orc_dims *D, const int *have_sgp, double *dq_a, double *dq_a_pl,
+                                        const double *dq, const double *dq_pl, double *dQprev, double *dQprev_pl,
+                                        const double *dQpost, const double *dQpost_pl, const double *q_a,
+                                        const double *q_a_pl, const double *q, const double *q_pl, const double *d,
+                                        const double *d_pl, const double *ch, const double *ch_pl,
+                                        const double *cmask, const double *cmask_pl, const double *Qpost,
+                                        const double *Qpost_pl, const uint8_t *mask, const uint8_t *mask_pl)
+{
+    hlim_lin(D, have_sgp, 0, dq_a, dq_a_pl, (double *)dq, (double *)dq_pl, dQprev, dQprev_pl, (double *)dQpost,
+             (double *)dQpost_pl, q_a, q_a_pl, q, q_pl, d, d_pl, ch, ch_pl, cmask, cmask_pl, Qpost, Qpost_pl, mask,
+             mask_pl);
+}
+
+void orc_horizontal_limiter_thuburn_adjoint(const orc_dims *D, const int *have_sgp, double *q_a_b,
+                                            double *q_a_b_pl, double *q_b, double *q_b_pl, const double *Qprev_b,
+                                            const double *Qprev_b_pl, double *Qpost_b, double *Qpost_b_pl,
+                                            const double *q_a, const double *q_a_pl, const double *q,
+                                            const double *q_pl, const double *d, const double *d_pl,
+                                            const double *ch, const double *ch_pl, const double *cmask,
+                                            const double *cmask_pl, const double *Qpost, const double *Qpost_pl,
+                                            const uint8_t *mask, const uint8_t *mask_pl)
+{
+    hlim_lin(D, have_sgp, 1, q_a_b, q_a_b_pl, q_b, q_b_pl, (double *)Qprev_b, (double *)Qprev_b_pl, Qpost_b,
+             Qpost_b_pl, q_a, q_a_pl, q, q_pl, d, d_pl, ch, ch_pl, cmask, cmask_pl, Qpost, Qpost_pl, mask, mask_pl);
+}

@@ -217,6 +217,28 @@ void orc_vertical_limiter_thuburn_adjoint(const orc_dims *D, double *qhb, double
                                           const double *q_pl, const double *d, const double *d_pl, const double *ck,
                                           const double *ck_pl, const uint32_t *mask, const uint32_t *mask_pl);
 
+/* Tangent-linear and adjoint of orc_horizontal_limiter_thuburn with
+ * Qout_post supplied, at fixed d, ch and cmask.  mask: uint8 per cell-level,
+ * bit m-1 set where face m (1-3) of a cell in [gmin-1,gmax]^2 changed; pole:
+ * uint8 per point, bit 0 for ring faces.  q_a is the forward's output.
+ * TLM: dq_a in = dq_a_in, out = dq_a; dQprev written from dq and dQpost.
+ * Adjoint: q_a_b in/out; q_b, Qpost_b written from q_a_b and Qprev_b. */
+void orc_horizontal_limiter_thuburn_tlm(const orc_dims *D, const int *have_sgp, double *dq_a, double *dq_a_pl,
+                                        const double *dq, const double *dq_pl, double *dQprev, double *dQprev_pl,
+                                        const double *dQpost, const double *dQpost_pl, const double *q_a,
+                                        const double *q_a_pl, const double *q, const double *q_pl, const double *d,
+                                        const double *d_pl, const double *ch, const double *ch_pl,
+                                        const double *cmask, const double *cmask_pl, const double *Qpost,
+                                        const double *Qpost_pl, const uint8_t *mask, const uint8_t *mask_pl);
+void orc_horizontal_limiter_thuburn_adjoint(const orc_dims *D, const int *have_sgp, double *q_a_b,
+                                            double *q_a_b_pl, double *q_b, double *q_b_pl, const double *Qprev_b,
+                                            const double *Qprev_b_pl, double *Qpost_b, double *Qpost_b_pl,
+                                            const double *q_a, const double *q_a_pl, const double *q,
+                                            const double *q_pl, const double *d, const double *d_pl,
+                                            const double *ch, const double *ch_pl, const double *cmask,
+                                            const double *cmask_pl, const double *Qpost, const double *Qpost_pl,
+                                            const uint8_t *mask, const uint8_t *mask_pl);
+
 #ifdef __cplusplus
 }
 #endif

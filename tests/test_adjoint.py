@@ -304,3 +304,87 @@ def test_gpu_vlim_adjoint_host_pointer_mode(gpu, oracle):
     want = oracle.vertical_limiter_thuburn_lin(spec, f, out, wm, wmp, True, u)
     for k, got in (("q_h_b", qhb), ("q_h_b_pl", qhb_pl), ("q_b", qb), ("q_b_pl", qb_pl)):
         np.testing.assert_array_equal(got, want[k], err_msg=k)
+
+
+# ---- horizontal limiter ---------------------------------------------------------
+def pack_hlim_mask(spec, qa_in, qa_out):
+    """uint8 per cell-level: bit m-1 set where face m (1-3) of a cell in
+    [gmin-1, gmax]^2 changed; pole: uint8 per point, bit 0 for ring faces."""
+    n = spec.gall_1d
+    a_in = qa_in["q_a"].reshape((6, spec.lall, spec.kall, n, n))
+    a_out = qa_out["q_a"].reshape((6, spec.lall, spec.kall, n, n))
+    m = np.zeros((spec.lall, spec.kall, n, n), np.uint8)
+    for f in range(3):
+        ch = (a_out[f] != a_in[f])[..., : n - 1, : n - 1]
+        m[..., : n - 1, : n - 1] |= ch.astype(np.uint8) << np.uint8(f)
+    mp = (qa_out["q_a_pl"] != qa_in["q_a_pl"]).astype(np.uint8)
+    mp[..., 0] = 0
+    return m.reshape(spec.lall, spec.kall, n * n), mp
+
+
+HLIM_SPECS = [(6, 4, 2, 71, [1, 0]), (10, 5, 2, 72, [0, 1]), (18, 3 + 1, 1, 73, [1])]
+
+
+def _hlim_dx(rng, f):
+    return {"dq_a_in": rng.standard_normal(f["q_a"].shape), "dq_a_in_pl": rng.standard_normal(f["q_a_pl"].shape),
+            "dq": rng.standard_normal(f["q"].shape), "dq_pl": rng.standard_normal(f["q_pl"].shape),
+            "dQpost": rng.standard_normal(f["Qout_post"].shape),
+            "dQpost_pl": rng.standard_normal(f["Qout_post_pl"].shape)}
+
+
+@pytest.mark.parametrize("c", HLIM_SPECS, ids=[f"g{c[0]}k{c[1]}" for c in HLIM_SPECS])
+def test_oracle_hlim_adjoint_dot_product(oracle, c):
+    n, kall, lall, seed, sgp = c
+    spec = synth.GridSpec(n, kall, lall, have_pl=True, seed=seed)
+    case = synth.make_case(spec, have_sgp=sgp)
+    f = dict(case["hlim"])
+    out = oracle.horizontal_limiter_thuburn(spec, sgp, f)
+    m, mp = pack_hlim_mask(spec, f, out)
+    assert m.any() and mp.any()
+    rng = np.random.default_rng(seed)
+    dx = _hlim_dx(rng, f)
+    u = {"q_a_b": rng.standard_normal(f["q_a"].shape), "q_a_b_pl": rng.standard_normal(f["q_a_pl"].shape),
+         "Qprev_b": rng.standard_normal(f["Qout_post"].shape),
+         "Qprev_b_pl": rng.standard_normal(f["Qout_post_pl"].shape)}
+    u["Qprev_b_pl"][..., 1:] = 0.0  # the ring of Qout_prev_pl is a constant zero
+    t = oracle.horizontal_limiter_thuburn_lin(spec, sgp, f, out, m, mp, False, dx)
+    a = oracle.horizontal_limiter_thuburn_lin(spec, sgp, f, out, m, mp, True, u)
+    lhs, s1 = _dot({"a": t["dq_a"], "a_pl": t["dq_a_pl"], "b": t["dQprev"], "b_pl": t["dQprev_pl"]},
+                   {"a": u["q_a_b"], "a_pl": u["q_a_b_pl"], "b": u["Qprev_b"], "b_pl": u["Qprev_b_pl"]}, ["a", "b"])
+    rhs, s2 = _dot({"a": dx["dq_a_in"], "a_pl": dx["dq_a_in_pl"], "b": dx["dq"], "b_pl": dx["dq_pl"],
+                    "c": dx["dQpost"], "c_pl": dx["dQpost_pl"]},
+                   {"a": a["q_a_b"], "a_pl": a["q_a_b_pl"], "b": a["q_b"], "b_pl": a["q_b_pl"], "c": a["Qpost_b"],
+                    "c_pl": a["Qpost_b_pl"]}, ["a", "b", "c"])
+    assert abs(lhs - rhs) <= 1e-13 * max(s1, s2), (lhs, rhs)
+
+
+def test_oracle_hlim_tlm_matches_finite_differences(oracle):
+    spec = synth.GridSpec(12, 6, 2, have_pl=True, seed=81)
+    sgp = [1, 0]
+    case = synth.make_case(spec, have_sgp=sgp)
+    f = dict(case["hlim"])
+    rng = np.random.default_rng(81)
+    for k in ("q", "q_pl"):
+        f[k] = 0.5 + 0.3 * np.sin(rng.uniform(0, 6, size=f[k].shape)) + 1e-3 * rng.standard_normal(f[k].shape)
+    for k in ("q_a", "q_a_pl"):
+        f[k] = 0.5 + 0.4 * rng.standard_normal(f[k].shape)
+    for k in ("Qout_post", "Qout_post_pl"):
+        f[k] = 0.5 + 0.1 * rng.standard_normal(f[k].shape) * np.array([-1.0, 1.0]).reshape((2,) + (1,) * (f[k].ndim - 1)) \
+            + np.array([-0.1, 0.1]).reshape((2,) + (1,) * (f[k].ndim - 1))
+    out = oracle.horizontal_limiter_thuburn(spec, sgp, f)
+    m, mp = pack_hlim_mask(spec, f, out)
+    assert int(np.unpackbits(m).sum()) > 100
+    dx = _hlim_dx(rng, f)
+    t = oracle.horizontal_limiter_thuburn_lin(spec, sgp, f, out, m, mp, False, dx)
+    h = 1e-7
+    pert = lambda s: {**f, "q_a": f["q_a"] + s * h * dx["dq_a_in"], "q_a_pl": f["q_a_pl"] + s * h * dx["dq_a_in_pl"],  # noqa: E731
+                      "q": f["q"] + s * h * dx["dq"], "q_pl": f["q_pl"] + s * h * dx["dq_pl"],
+                      "Qout_post": f["Qout_post"] + s * h * dx["dQpost"],
+                      "Qout_post_pl": f["Qout_post_pl"] + s * h * dx["dQpost_pl"]}
+    op, om = oracle.horizontal_limiter_thuburn(spec, sgp, pert(1)), oracle.horizontal_limiter_thuburn(spec, sgp, pert(-1))
+    for k, tk in (("q_a", "dq_a"), ("Qout_prev", "dQprev"), ("q_a_pl", "dq_a_pl")):
+        fd = (op[k] - om[k]) / (2 * h)
+        err = np.abs(fd - t[tk])
+        bad = err > 1e-5 * (1.0 + np.abs(t[tk]))
+        assert bad.mean() < 0.02, (k, int(bad.sum()), bad.size)
+        assert np.median(err) < 1e-7, k
