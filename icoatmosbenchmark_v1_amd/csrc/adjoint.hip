@@ -278,7 +278,333 @@ __global__ void __launch_bounds__(BLOCK_VLA) vlim_adjoint_kernel(VlaArgs A)
     }
 }
 
+// ---- horizontal_limiter_thuburn (dyn_horiz_adv_limiter/src/mod_src_tracer.f90:42-430)
+//
+// With Qout_post supplied and d, ch, cmask fixed the limiter is piecewise
+// linear in (q_a faces 1-3, q, Qout_post).  Qout of an inner cell is affine in
+// q given the selections of qnext and of its six arcs (14 terms), a halo
+// cell's Qout is q, and a face value is its input, a Qout_post bound of the
+// downwind cell or a Qin bound of its arc (the forward's mask says whether it
+// changed; a changed value is attributed, in this order, to Qout_post min,
+// Qout_post max, Qin min, Qin max, by equality with the stored output).
+//
+// One work-group per CHUNK cells of one (k, l) slab, as in the forward: q of
+// the window [g0-iall-1, g0+CHUNK+iall+2) staged in LDS, and the cotangents of
+// q and Qout_post over the same window accumulated in LDS (every scatter
+// target lies in it), then added to global memory once per window cell.
+// Neighbouring windows overlap, so those adds are atomic and q_b, Qout_post_b
+// are cleared first; sums of a cell's few terms may therefore round in any
+// order.  Faces of q_a are owned by one work-item each.
+constexpr int HCHUNK = 256;
+constexpr int HMAXI = 160;
+
+struct HaArgs {
+    double *qab, *qb, *Qpostb;
+    const double *Qprevb;
+    const double *q_a, *q, *d, *ch, *cmask, *Qpost;
+    const uint8_t *mask;
+    const int *have_sgp;
+    int iall, gall, kall, lall, nmain;
+    double eps, big;
+    // poles
+    double *qab_pl, *qb_pl, *Qpostb_pl;
+    const double *Qprevb_pl;
+    const double *q_a_pl, *q_pl, *d_pl, *ch_pl, *cmask_pl, *Qpost_pl;
+    const uint8_t *mask_pl;
+    int gall_pl, have_pl;
+};
+
+// cells of arc m (0 = AI, 1 = AIJ, 2 = AJ) with origin p, in the forward's
+// min/max chain order (HLIM:145-160, the index clamps of :152-153 and the
+// singular-point stencil of :197-203)
+__device__ __forceinline__ void ha_arc(int m, int p, int iall, bool sgp, int c[4])
+{
+    c[0] = p;
+    if (m == 0) {
+        c[1] = max(p - iall, 0); c[2] = p + 1; c[3] = p + iall + 1;
+    } else if (m == 1) {
+        if (sgp && p == 0) { c[1] = p + iall + 1; c[2] = p + iall + 2; c[3] = p + iall; }
+        else { c[1] = p + 1; c[2] = p + iall + 1; c[3] = p + iall; }
+    } else {
+        c[1] = p + iall + 1; c[2] = p + iall; c[3] = max(p - 1, 0);
+    }
+}
+
+__global__ void __launch_bounds__(HCHUNK) hlim_adjoint_pole(HaArgs A);
+
+__global__ void __launch_bounds__(HCHUNK) hlim_adjoint_kernel(HaArgs A)
+{
+    constexpr int SPAN = HCHUNK + 2 * HMAXI + 3;
+    __shared__ double s_q[SPAN], s_b[SPAN], s_pn[SPAN], s_px[SPAN];
+    const int iall = A.iall, gall = A.gall, kall = A.kall, lall = A.lall;
+    const int nchunk = (gall + HCHUNK - 1) / HCHUNK;
+    const int b = blockIdx.x;
+    const int chunk = b % nchunk, kl = b / nchunk;
+    const int k = kl % kall, l = kl / kall;
+    const int g0 = chunk * HCHUNK, lo = g0 - iall - 1, n_in = HCHUNK + 2 * iall + 3;
+    const size_t fb = ((size_t)l * kall + k) * gall;
+    const size_t plane = (size_t)lall * kall * gall;
+    const int tid = threadIdx.x;
+    for (int s = tid; s < n_in; s += HCHUNK) {
+        const int gg = lo + s;
+        s_q[s] = (gg >= 0 && gg < gall) ? A.q[fb + gg] : 0.0;
+        s_b[s] = 0.0;
+        s_pn[s] = 0.0;
+        s_px[s] = 0.0;
+    }
+    __syncthreads();
+    auto Q = [&](int p) { return s_q[p - lo]; };
+    auto addq = [&](int p, double v) { atomicAdd(&s_b[p - lo], v); };
+    const int g = g0 + tid;
+    if (g < gall) {
+        const int i = g % iall, j = g / iall;
+        const bool sgp = A.have_sgp[l] != 0;
+        const double BIG = A.big;
+        if (i < 1 || i > iall - 2 || j < 1 || j > iall - 2) {  // halo: Qout = q (HLIM:286-301)
+            addq(g, A.Qprevb[fb + g]);
+            addq(g, A.Qprevb[plane + fb + g]);
+        } else {
+            int cmn[6], cmx[6];
+            double vmn[6], vmx[6], cc[6];
+#pragma unroll
+            for (int f = 0; f < 6; ++f) {
+                const int m = f < 3 ? f : f - 3;
+                const int owner = f < 3 ? g : g - (m == 0 ? 1 : (m == 1 ? iall + 1 : iall));
+                int c4[4];
+                ha_arc(m, owner, iall, sgp, c4);
+                double lo_v = Q(c4[0]), hi_v = lo_v;
+                int il = c4[0], ih = c4[0];
+#pragma unroll
+                for (int t = 1; t < 4; ++t) {
+                    const double v = Q(c4[t]);
+                    if (v < lo_v) { lo_v = v; il = c4[t]; }
+                    if (v > hi_v) { hi_v = v; ih = c4[t]; }
+                }
+                const double c = A.cmask[(size_t)m * plane + fb + owner];
+                cmn[f] = il;
+                cmx[f] = ih;
+                cc[f] = f < 3 ? c : 1.0 - c;
+                if (f < 3) {
+                    vmn[f] = c * lo_v + (1.0 - c) * BIG;
+                    vmx[f] = c * hi_v + (1.0 - c) * (-BIG);
+                } else {
+                    vmn[f] = c * BIG + (1.0 - c) * lo_v;
+                    vmx[f] = c * (-BIG) + (1.0 - c) * hi_v;
+                }
+            }
+            const double qij = Q(g);
+            int smin = -1, smax = -1;
+            double nmin = qij, nmax = qij;
+#pragma unroll
+            for (int f = 0; f < 6; ++f) {
+                if (vmn[f] < nmin) { nmin = vmn[f]; smin = f; }
+                if (vmx[f] > nmax) { nmax = vmx[f]; smax = f; }
+            }
+            double chv[6], chm[6];
+#pragma unroll
+            for (int f = 0; f < 6; ++f) {
+                chv[f] = A.ch[(size_t)f * plane + fb + g];
+                chm[f] = dmn(chv[f], 0.0);
+            }
+            const double Cin = chm[0] + chm[1] + chm[2] + chm[3] + chm[4] + chm[5];
+            const double Cout = chv[0] - chm[0] + chv[1] - chm[1] + chv[2] - chm[2] + chv[3] - chm[3] + chv[4] -
+                                chm[4] + chv[5] - chm[5];
+            const double zerosw = 0.5 - sgn_half(fabs(Cout) - A.eps);
+            const double E = 1.0 - Cin - Cout + A.d[fb + g];
+            const double fac = (1.0 - zerosw) / (Cout + zerosw);
+#pragma unroll
+            for (int mm = 0; mm < 2; ++mm) {
+                const double bq = A.Qprevb[(size_t)mm * plane + fb + g];
+                const int sel = mm == 0 ? smax : smin;
+                const int *cx = mm == 0 ? cmx : cmn;
+                addq(g, (fac + zerosw) * bq);
+                addq(sel < 0 ? g : cx[sel], (-(fac * E) * (sel < 0 ? 1.0 : cc[sel])) * bq);
+#pragma unroll
+                for (int f = 0; f < 6; ++f) addq(cx[f], (-(fac * chm[f]) * cc[f]) * bq);
+            }
+        }
+        // faces 1-3 of the own arcs (HLIM:377-410)
+        if (i <= iall - 2 && j <= iall - 2) {
+            const uint32_t bits = A.mask[fb + g];
+#pragma unroll
+            for (int m = 0; m < 3; ++m) {
+                const int nb = g + (m == 0 ? 1 : (m == 1 ? iall + 1 : iall));
+                const size_t fo = (size_t)m * plane + fb + g, fn = (size_t)(m + 3) * plane + fb + nb;
+                const double c = A.cmask[fo];
+                const int T = c > 0.0 ? nb : g;
+                const double ob = A.qab[fo] + A.qab[fn];
+                int kind = 0, cell = 0;
+                if ((bits >> m) & 1u) {
+                    const double out = A.q_a[fo];
+                    int c4[4];
+                    ha_arc(m, g, iall, sgp, c4);
+                    double lo_v = Q(c4[0]), hi_v = lo_v;
+                    int il = c4[0], ih = c4[0];
+#pragma unroll
+                    for (int t = 1; t < 4; ++t) {
+                        const double v = Q(c4[t]);
+                        if (v < lo_v) { lo_v = v; il = c4[t]; }
+                        if (v > hi_v) { hi_v = v; ih = c4[t]; }
+                    }
+                    if (out == A.Qpost[fb + T]) kind = 1;
+                    else if (out == A.Qpost[plane + fb + T]) kind = 2;
+                    else if (out == lo_v) { kind = 3; cell = il; }
+                    else { kind = 4; cell = ih; }
+                }
+                A.qab[fn] = 0.0;
+                A.qab[fo] = kind == 0 ? ob : 0.0;
+                if (kind == 1) atomicAdd(&s_pn[T - lo], ob);
+                else if (kind == 2) atomicAdd(&s_px[T - lo], ob);
+                else if (kind >= 3) addq(cell, ob);
+            }
+        }
+    }
+    __syncthreads();
+    for (int s = tid; s < n_in; s += HCHUNK) {
+        const int gg = lo + s;
+        if (gg < 0 || gg >= gall) continue;
+        if (s_b[s] != 0.0) atomicAdd(&A.qb[fb + gg], s_b[s]);
+        if (s_pn[s] != 0.0) atomicAdd(&A.Qpostb[fb + gg], s_pn[s]);
+        if (s_px[s] != 0.0) atomicAdd(&A.Qpostb[plane + fb + gg], s_px[s]);
+    }
+}
+
+// Poles: one work-item per (k, l) slab, sequential as the oracle (HLIM:307-363,
+// 412-425); it owns the slab's pole arrays, so no atomics.
+__global__ void __launch_bounds__(HCHUNK) hlim_adjoint_pole(HaArgs A)
+{
+    const int t = blockIdx.x * HCHUNK + threadIdx.x;
+    const int kall = A.kall, npl = A.gall_pl;
+    if (!A.have_pl || t >= kall * kLallPl) return;
+    const int k = t % kall, l = t / kall;
+    const size_t fb = ((size_t)l * kall + k) * npl;
+    const size_t pplane = (size_t)kLallPl * kall * npl;
+    const double *qs = A.q_pl + fb, *chs = A.ch_pl + fb, *cms = A.cmask_pl + fb;
+    double *xq = A.qb_pl + fb, *xa = A.qab_pl + fb, *xp = A.Qpostb_pl + fb;
+    for (int v = 0; v < npl; ++v) {
+        xq[v] = 0.0;
+        xp[v] = 0.0;
+        xp[pplane + v] = 0.0;
+    }
+    const double BIG = A.big;
+    auto arc = [&](int v, int &il, int &ih, double &lo_v, double &hi_v) {
+        int vp1 = v + 1, vm1 = v - 1;
+        if (vp1 == npl) vp1 = 1;
+        if (vm1 == 0) vm1 = npl - 1;
+        const int c4[4] = {0, v, vm1, vp1};
+        lo_v = qs[0];
+        hi_v = lo_v;
+        il = 0;
+        ih = 0;
+        for (int u = 1; u < 4; ++u) {
+            const double x = qs[c4[u]];
+            if (x < lo_v) { lo_v = x; il = c4[u]; }
+            if (x > hi_v) { hi_v = x; ih = c4[u]; }
+        }
+    };
+    const double qn = qs[0];
+    int smin = -1, smax = -1;
+    double nmin = qn, nmax = qn;
+    for (int v = 1; v < npl; ++v) {
+        int il, ih;
+        double lv, hv;
+        arc(v, il, ih, lv, hv);
+        const double c = cms[v];
+        const double mn1 = c * lv + (1.0 - c) * BIG, mx1 = c * hv + (1.0 - c) * (-BIG);
+        if (mn1 < nmin) { nmin = mn1; smin = v; }
+        if (mx1 > nmax) { nmax = mx1; smax = v; }
+    }
+    double Cin = 0.0, Cout = 0.0;
+    for (int v = 1; v < npl; ++v) {
+        const double chm = cms[v] * chs[v];
+        Cin = Cin + chm;
+        Cout = Cout - chm + chs[v];
+    }
+    const double zerosw = 0.5 - sgn_half(fabs(Cout) - A.eps);
+    const double E = 1.0 - Cin - Cout + A.d_pl[fb];
+    const double fac = (1.0 - zerosw) / (Cout + zerosw);
+    for (int mm = 0; mm < 2; ++mm) {
+        const double bq = A.Qprevb_pl[(size_t)mm * pplane + fb];
+        const int sel = mm == 0 ? smax : smin;
+        xq[0] = xq[0] + (fac + zerosw) * bq;
+        int selc = 0;
+        if (sel >= 0) {
+            int il, ih;
+            double lv, hv;
+            arc(sel, il, ih, lv, hv);
+            selc = mm == 0 ? ih : il;
+        }
+        xq[selc] = xq[selc] + (-(fac * E) * (sel < 0 ? 1.0 : cms[sel])) * bq;
+        for (int v = 1; v < npl; ++v) {
+            int il, ih;
+            double lv, hv;
+            arc(v, il, ih, lv, hv);
+            const int cx = mm == 0 ? ih : il;
+            xq[cx] = xq[cx] + (-(fac * (cms[v] * chs[v])) * cms[v]) * bq;
+        }
+    }
+    for (int v = 1; v < npl; ++v) {
+        const double c = cms[v];
+        const int T = c > 0.0 ? v : 0;
+        int kind = 0, cell = 0;
+        if (A.mask_pl[fb + v] & 1u) {
+            const double out = A.q_a_pl[fb + v];
+            int il, ih;
+            double lv, hv;
+            arc(v, il, ih, lv, hv);
+            if (out == A.Qpost_pl[fb + T]) kind = 1;
+            else if (out == A.Qpost_pl[pplane + fb + T]) kind = 2;
+            else if (out == lv) { kind = 3; cell = il; }
+            else { kind = 4; cell = ih; }
+        }
+        const double ob = xa[v];
+        xa[v] = kind == 0 ? ob : 0.0;
+        if (kind == 1) xp[T] = xp[T] + ob;
+        else if (kind == 2) xp[pplane + T] = xp[pplane + T] + ob;
+        else if (kind >= 3) xq[cell] = xq[cell] + ob;
+    }
+}
+
+__global__ void fill_zero_kernel(double *p, size_t n)
+{
+    for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (size_t)gridDim.x * blockDim.x)
+        p[t] = 0.0;
+}
+
 }  // namespace
+
+int launch_hlim_adjoint(nicam_ctx c, double *qab, double *qab_pl, double *qb, double *qb_pl, const double *Qprevb,
+                        const double *Qprevb_pl, double *Qpostb, double *Qpostb_pl, const double *q_a,
+                        const double *q_a_pl, const double *q, const double *q_pl, const double *d,
+                        const double *d_pl, const double *ch, const double *ch_pl, const double *cmask,
+                        const double *cmask_pl, const double *Qpost, const double *Qpost_pl, const uint8_t *mask,
+                        const uint8_t *mask_pl)
+{
+    const Dims &D = c->d;
+    if (D.gall_1d > HMAXI) return fail(NICAM_ERR_ARG, "gall_1d larger than the LDS tiles (160)");
+    HaArgs A;
+    A.qab = qab; A.qb = qb; A.Qpostb = Qpostb; A.Qprevb = Qprevb;
+    A.q_a = q_a; A.q = q; A.d = d; A.ch = ch; A.cmask = cmask; A.Qpost = Qpost; A.mask = mask;
+    A.have_sgp = c->have_sgp;
+    A.iall = D.gall_1d; A.gall = D.gall; A.kall = D.kall; A.lall = D.lall;
+    A.eps = c->eps; A.big = c->big;
+    A.qab_pl = qab_pl; A.qb_pl = qb_pl; A.Qpostb_pl = Qpostb_pl; A.Qprevb_pl = Qprevb_pl;
+    A.q_a_pl = q_a_pl; A.q_pl = q_pl; A.d_pl = d_pl; A.ch_pl = ch_pl; A.cmask_pl = cmask_pl;
+    A.Qpost_pl = Qpost_pl; A.mask_pl = mask_pl;
+    A.gall_pl = D.gall_pl; A.have_pl = D.have_pl;
+    const long nmain = (long)((D.gall + HCHUNK - 1) / HCHUNK) * D.kall * D.lall;
+    if (nmain > 0x7fffffffL) return fail(NICAM_ERR_ARG, "grid too large for one launch");
+    A.nmain = (int)nmain;
+    const size_t nreg = (size_t)D.gall * D.kall * D.lall;
+    hipLaunchKernelGGL(fill_zero_kernel, dim3(2048), dim3(256), 0, c->stream, qb, nreg);
+    hipLaunchKernelGGL(fill_zero_kernel, dim3(2048), dim3(256), 0, c->stream, Qpostb, 2 * nreg);
+    hipLaunchKernelGGL(hlim_adjoint_kernel, dim3((unsigned)nmain), dim3(HCHUNK), 0, c->stream, A);
+    if (D.have_pl)
+        hipLaunchKernelGGL(hlim_adjoint_pole, dim3((unsigned)((D.kall * kLallPl + HCHUNK - 1) / HCHUNK)),
+                           dim3(HCHUNK), 0, c->stream, A);
+    NICAM_HIP(hipGetLastError());
+    return NICAM_OK;
+}
 
 int launch_vlim_adjoint(nicam_ctx c, double *qhb, double *qhb_pl, double *qb, double *qb_pl, const double *q_h,
                         const double *q_h_pl, const double *q, const double *q_pl, const double *d,

@@ -786,6 +786,81 @@ int nicam_vi_rhow_solver_adjoint(nicam_ctx c, double *wb, double *wb_pl, double 
     return stage_out(c, a, v);
 }
 
+int nicam_horizontal_limiter_thuburn_mask(nicam_ctx c, double *q_a, double *q_a_pl, const double *q,
+                                          const double *q_pl, const double *d, const double *d_pl, const double *ch,
+                                          const double *ch_pl, const double *cmask, const double *cmask_pl,
+                                          double *Qprev, double *Qprev_pl, const double *Qpost,
+                                          const double *Qpost_pl, uint8_t *mask, uint8_t *mask_pl)
+{
+    NICAM_TRY(check_ctx(c));
+    const void *req[] = {q_a, q, d, ch, cmask, Qprev, mask};
+    const char *nm[] = {"q_a", "q", "d", "ch", "cmask", "Qout_prev", "mask"};
+    for (int i = 0; i < 7; ++i) NICAM_TRY(need(req[i], nm[i]));
+    if (c->d.have_pl) {
+        const void *rp[] = {q_a_pl, q_pl, d_pl, ch_pl, cmask_pl, Qprev_pl, mask_pl};
+        for (int i = 0; i < 7; ++i) NICAM_TRY(need(rp[i], "pole array"));
+        if (Qpost) NICAM_TRY(need(Qpost_pl, "Qout_post_pl"));
+    }
+    if (!Qpost) Qpost_pl = nullptr;
+    if (c->ptr_mode == NICAM_PTR_DEVICE)
+        return launch_hlim_mask(c, q_a, PL(q_a_pl), q, PL(q_pl), d, PL(d_pl), ch, PL(ch_pl), cmask, PL(cmask_pl),
+                                Qprev, Qprev_pl, Qpost, PL(Qpost_pl), mask, PL(mask_pl));
+    const size_t R = c->d.nreg() * 8, P = c->d.npl() * 8;
+    std::vector<HostArg> a = {{q_a, R * 6, true},       {PL(q_a_pl), P, true},       {q, R, false},
+                              {PL(q_pl), P, false},     {d, R, false},               {PL(d_pl), P, false},
+                              {ch, R * 6, false},       {PL(ch_pl), P, false},       {cmask, R * 6, false},
+                              {PL(cmask_pl), P, false}, {Qprev, R * 2, true},        {Qprev_pl, P * 2, true},
+                              {Qpost, R * 2, false},    {PL(Qpost_pl), P * 2, false}, {mask, R / 8, true},
+                              {PL(mask_pl), P / 8, true}};
+    std::vector<void *> v;
+    NICAM_TRY(stage_in(c, a, v));
+    auto D = [&](int i) { return static_cast<double *>(v[i]); };
+    NICAM_TRY(launch_hlim_mask(c, D(0), D(1), D(2), D(3), D(4), D(5), D(6), D(7), D(8), D(9), D(10), D(11), D(12),
+                               D(13), static_cast<uint8_t *>(v[14]), static_cast<uint8_t *>(v[15])));
+    return stage_out(c, a, v);
+}
+
+int nicam_horizontal_limiter_thuburn_adjoint(nicam_ctx c, double *q_a_b, double *q_a_b_pl, double *q_b,
+                                             double *q_b_pl, const double *Qprev_b, const double *Qprev_b_pl,
+                                             double *Qpost_b, double *Qpost_b_pl, const double *q_a,
+                                             const double *q_a_pl, const double *q, const double *q_pl,
+                                             const double *d, const double *d_pl, const double *ch,
+                                             const double *ch_pl, const double *cmask, const double *cmask_pl,
+                                             const double *Qpost, const double *Qpost_pl, const uint8_t *mask,
+                                             const uint8_t *mask_pl)
+{
+    NICAM_TRY(check_ctx(c));
+    const void *req[] = {q_a_b, q_b, Qprev_b, Qpost_b, q_a, q, d, ch, cmask, Qpost, mask};
+    const char *nm[] = {"q_a_b", "q_b", "Qout_prev_b", "Qout_post_b", "q_a", "q", "d", "ch", "cmask",
+                        "Qout_post", "mask"};
+    for (int i = 0; i < 11; ++i) NICAM_TRY(need(req[i], nm[i]));
+    if (c->d.have_pl) {
+        const void *rp[] = {q_a_b_pl, q_b_pl, Qprev_b_pl, Qpost_b_pl, q_a_pl, q_pl, d_pl, ch_pl, cmask_pl,
+                            Qpost_pl, mask_pl};
+        for (int i = 0; i < 11; ++i) NICAM_TRY(need(rp[i], "pole array"));
+    }
+    if (c->ptr_mode == NICAM_PTR_DEVICE)
+        return launch_hlim_adjoint(c, q_a_b, PL(q_a_b_pl), q_b, PL(q_b_pl), Qprev_b, PL(Qprev_b_pl), Qpost_b,
+                                   PL(Qpost_b_pl), q_a, PL(q_a_pl), q, PL(q_pl), d, PL(d_pl), ch, PL(ch_pl), cmask,
+                                   PL(cmask_pl), Qpost, PL(Qpost_pl), mask, PL(mask_pl));
+    const size_t R = c->d.nreg() * 8, P = c->d.npl() * 8;
+    std::vector<HostArg> a = {{q_a_b, R * 6, true},      {PL(q_a_b_pl), P, true},       {q_b, R, true},
+                              {PL(q_b_pl), P, true},     {Qprev_b, R * 2, false},       {PL(Qprev_b_pl), P * 2, false},
+                              {Qpost_b, R * 2, true},    {PL(Qpost_b_pl), P * 2, true}, {q_a, R * 6, false},
+                              {PL(q_a_pl), P, false},    {q, R, false},                 {PL(q_pl), P, false},
+                              {d, R, false},             {PL(d_pl), P, false},          {ch, R * 6, false},
+                              {PL(ch_pl), P, false},     {cmask, R * 6, false},         {PL(cmask_pl), P, false},
+                              {Qpost, R * 2, false},     {PL(Qpost_pl), P * 2, false},  {mask, R / 8, false},
+                              {PL(mask_pl), P / 8, false}};
+    std::vector<void *> v;
+    NICAM_TRY(stage_in(c, a, v));
+    auto D = [&](int i) { return static_cast<double *>(v[i]); };
+    NICAM_TRY(launch_hlim_adjoint(c, D(0), D(1), D(2), D(3), D(4), D(5), D(6), D(7), D(8), D(9), D(10), D(11),
+                                  D(12), D(13), D(14), D(15), D(16), D(17), D(18), D(19),
+                                  static_cast<const uint8_t *>(v[20]), static_cast<const uint8_t *>(v[21])));
+    return stage_out(c, a, v);
+}
+
 static size_t vlim_mask_words(const Dims &d, bool pole)
 {
     const size_t nword = d.kall > 3 ? (size_t)(d.kall - 3 + 31) / 32 : 0;

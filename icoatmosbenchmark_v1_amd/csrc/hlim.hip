@@ -67,6 +67,9 @@ struct HlimArgs {
     T *q_a_pl, *Qprev_pl;
     const T *q_pl, *d_pl, *ch_pl, *cmask_pl, *Qpost_pl;
     int gall_pl, have_pl;
+    // branch mask (masked variant): uint8 per cell-level, bit m set where face
+    // m+1 of an own arc changed; uint8 per pole point, bit 0 for ring faces
+    uint8_t *mask, *mask_pl;
 };
 
 template <class T>
@@ -208,7 +211,9 @@ __global__ void __launch_bounds__(CHUNK) hlim_kernel(HlimArgs<T> A)
     }
 
     // ---- phase 2: clamp the face values of the own arcs (HLIM:377-410) ----
+    if (A.mask && (A.phases & 2) && !own) A.mask[fb + g] = 0;
     if (own && (A.phases & 2)) {
+        uint32_t bits = 0;
         const int nb[3] = {g + 1, g + iall + 1, g + iall};
         const T qo_min_c = A.Qpost[fb + g], qo_max_c = A.Qpost[plane + fb + g];  // re-read by neighbours
 #pragma unroll
@@ -220,8 +225,10 @@ __global__ void __launch_bounds__(CHUNK) hlim_kernel(HlimArgs<T> A)
             const T in_mx_n = (c) * (-BIG) + (T(1) - c) * omx[m];
             const T qo_min_n = A.Qpost[fb + nb[m]], qo_max_n = A.Qpost[plane + fb + nb[m]];
             T qa = ld_once(&A.q_a[(size_t)m * plane + fb + g]);
+            const T qa_in = qa;
             qa = (c)*dmin(dmax(qa, in_mn_s), in_mx_s) + (T(1) - c) * dmin(dmax(qa, in_mn_n), in_mx_n);
             qa = (c)*dmax(dmin(qa, qo_max_n), qo_min_n) + (T(1) - c) * dmax(dmin(qa, qo_max_c), qo_min_c);
+            bits |= (qa != qa_in ? 1u : 0u) << m;
 #if NICAM_HLIM_STORE == 2
             A.q_a[(size_t)m * plane + fb + g] = qa;
 #else
@@ -233,6 +240,7 @@ __global__ void __launch_bounds__(CHUNK) hlim_kernel(HlimArgs<T> A)
             stream_store(&A.q_a[(size_t)(m + 3) * plane + fb + nb[m]], qa);
 #endif
         }
+        if (A.mask) A.mask[fb + g] = (uint8_t)bits;
     }
 }
 
@@ -302,14 +310,17 @@ __device__ void hlim_pole(const HlimArgs<T> &A, int t)
     }
     if (!(A.phases & 2)) return;
     const T pn_min = A.Qpost_pl[fb + 0], pn_max = A.Qpost_pl[plane + fb + 0];
+    if (A.mask_pl) A.mask_pl[fb + 0] = 0;
     for (int v = 1; v < npl; ++v) {
         T mn1, mn2, mx1, mx2;
         qin(v, mn1, mn2, mx1, mx2);
         const T c = A.cmask_pl[fb + v];
         T qa = A.q_a_pl[fb + v];
+        const T qa_in = qa;
         qa = (c)*dmin(dmax(qa, mn1), mx1) + (T(1) - c) * dmin(dmax(qa, mn2), mx2);
         qa = (c)*dmax(dmin(qa, A.Qpost_pl[plane + fb + v]), A.Qpost_pl[fb + v]) + (T(1) - c) * dmax(dmin(qa, pn_max), pn_min);
         A.q_a_pl[fb + v] = qa;
+        if (A.mask_pl) A.mask_pl[fb + v] = (qa != qa_in) ? 1 : 0;
     }
 }
 
@@ -329,11 +340,14 @@ int halo_fill(nicam_ctx c, float *v, float *v_pl, int kmax, int nvar)
 template <class T>
 int launch_hlim_t(nicam_ctx c, T *q_a, T *q_a_pl, const T *q, const T *q_pl, const T *d, const T *d_pl,
                   const T *ch, const T *ch_pl, const T *cmask, const T *cmask_pl, T *Qprev, T *Qprev_pl,
-                  const T *Qpost, const T *Qpost_pl, T eps, T big)
+                  const T *Qpost, const T *Qpost_pl, T eps, T big, uint8_t *mask = nullptr,
+                  uint8_t *mask_pl = nullptr)
 {
     const Dims &D = c->d;
     if (D.gall_1d > MAXI) return fail(NICAM_ERR_ARG, "gall_1d larger than the LDS tiles (160)");
     HlimArgs<T> A;
+    A.mask = mask;
+    A.mask_pl = mask_pl;
     A.q_a = q_a;
     A.q = q;
     A.d = d;
@@ -394,6 +408,15 @@ int launch_hlim(nicam_ctx c, double *q_a, double *q_a_pl, const double *q, const
 {
     return launch_hlim_t<double>(c, q_a, q_a_pl, q, q_pl, d, d_pl, ch, ch_pl, cmask, cmask_pl, Qprev, Qprev_pl, Qpost,
                                  Qpost_pl, c->eps, c->big);
+}
+
+int launch_hlim_mask(nicam_ctx c, double *q_a, double *q_a_pl, const double *q, const double *q_pl,
+                     const double *d, const double *d_pl, const double *ch, const double *ch_pl,
+                     const double *cmask, const double *cmask_pl, double *Qprev, double *Qprev_pl,
+                     const double *Qpost, const double *Qpost_pl, uint8_t *mask, uint8_t *mask_pl)
+{
+    return launch_hlim_t<double>(c, q_a, q_a_pl, q, q_pl, d, d_pl, ch, ch_pl, cmask, cmask_pl, Qprev, Qprev_pl, Qpost,
+                                 Qpost_pl, c->eps, c->big, mask, mask_pl);
 }
 
 int launch_hlim_f32(nicam_ctx c, float *q_a, float *q_a_pl, const float *q, const float *q_pl, const float *d,

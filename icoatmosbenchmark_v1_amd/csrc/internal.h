@@ -241,6 +241,16 @@ int launch_divdamp(nicam_ctx c, double *const out[3], double *const out_pl[3], c
                    const double *rgsqrth, const double *rgsqrth_pl, const double *rgam, const double *rgam_pl,
                    const double *rgamh, const double *rgamh_pl, const double *c2w, const double *c2w_pl,
                    const double *rdgz_dev);
+int launch_hlim_mask(nicam_ctx c, double *q_a, double *q_a_pl, const double *q, const double *q_pl,
+                     const double *d, const double *d_pl, const double *ch, const double *ch_pl,
+                     const double *cmask, const double *cmask_pl, double *Qprev, double *Qprev_pl,
+                     const double *Qpost, const double *Qpost_pl, uint8_t *mask, uint8_t *mask_pl);
+int launch_hlim_adjoint(nicam_ctx c, double *qab, double *qab_pl, double *qb, double *qb_pl, const double *Qprevb,
+                        const double *Qprevb_pl, double *Qpostb, double *Qpostb_pl, const double *q_a,
+                        const double *q_a_pl, const double *q, const double *q_pl, const double *d,
+                        const double *d_pl, const double *ch, const double *ch_pl, const double *cmask,
+                        const double *cmask_pl, const double *Qpost, const double *Qpost_pl, const uint8_t *mask,
+                        const uint8_t *mask_pl);
 int launch_vlim_mask(nicam_ctx c, double *q_h, double *q_h_pl, const double *q, const double *q_pl,
                      const double *d, const double *d_pl, const double *ck, const double *ck_pl, uint32_t *mask,
                      uint32_t *mask_pl);

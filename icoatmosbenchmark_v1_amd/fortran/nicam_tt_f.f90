@@ -190,6 +190,25 @@ module nicam_tt_f
        integer(c_int) :: nicam_vi_rhow_solver_adjoint
      end function nicam_vi_rhow_solver_adjoint
 
+     function nicam_horizontal_limiter_thuburn_mask(ctx, q_a, q_a_pl, q, q_pl, d, d_pl, ch, ch_pl, cmask, &
+          cmask_pl, Qout_prev, Qout_prev_pl, Qout_post, Qout_post_pl, mask, mask_pl) &
+          bind(C, name='nicam_horizontal_limiter_thuburn_mask')
+       import :: c_ptr, c_int
+       type(c_ptr), value :: ctx, q_a, q_a_pl, q, q_pl, d, d_pl, ch, ch_pl, cmask, cmask_pl
+       type(c_ptr), value :: Qout_prev, Qout_prev_pl, Qout_post, Qout_post_pl, mask, mask_pl
+       integer(c_int) :: nicam_horizontal_limiter_thuburn_mask
+     end function nicam_horizontal_limiter_thuburn_mask
+
+     function nicam_horizontal_limiter_thuburn_adjoint(ctx, q_a_b, q_a_b_pl, q_b, q_b_pl, Qout_prev_b, &
+          Qout_prev_b_pl, Qout_post_b, Qout_post_b_pl, q_a, q_a_pl, q, q_pl, d, d_pl, ch, ch_pl, cmask, &
+          cmask_pl, Qout_post, Qout_post_pl, mask, mask_pl) bind(C, name='nicam_horizontal_limiter_thuburn_adjoint')
+       import :: c_ptr, c_int
+       type(c_ptr), value :: ctx, q_a_b, q_a_b_pl, q_b, q_b_pl, Qout_prev_b, Qout_prev_b_pl, Qout_post_b
+       type(c_ptr), value :: Qout_post_b_pl, q_a, q_a_pl, q, q_pl, d, d_pl, ch, ch_pl, cmask, cmask_pl
+       type(c_ptr), value :: Qout_post, Qout_post_pl, mask, mask_pl
+       integer(c_int) :: nicam_horizontal_limiter_thuburn_adjoint
+     end function nicam_horizontal_limiter_thuburn_adjoint
+
      function nicam_vertical_limiter_thuburn_mask(ctx, q_h, q_h_pl, q, q_pl, d, d_pl, ck, ck_pl, mask, mask_pl) &
           bind(C, name='nicam_vertical_limiter_thuburn_mask')
        import :: c_ptr, c_int

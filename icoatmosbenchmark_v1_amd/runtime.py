@@ -109,6 +109,8 @@ SIGNATURES = {
     "nicam_oprt3d_divdamp": (c_int, [_P] + [_P] * 18),
     "nicam_vi_rhow_solver_adjoint": (c_int, [_P] + [_P] * 14 + [c_double]),
     "nicam_vertical_limiter_thuburn_mask": (c_int, [_P] * 11),
+    "nicam_horizontal_limiter_thuburn_mask": (c_int, [_P] * 17),
+    "nicam_horizontal_limiter_thuburn_adjoint": (c_int, [_P] * 23),
     "nicam_vertical_limiter_thuburn_adjoint": (c_int, [_P] * 15),
 }
 
@@ -200,6 +202,30 @@ def _ptr_u32(x, mode: int, name: str, numel: Optional[int] = None):
         raise TypeError(f"{name}: need a GPU tensor in device pointer mode")
     if x.dtype not in (torch.int32, getattr(torch, "uint32", torch.int32)) or not x.is_contiguous():
         raise TypeError(f"{name}: need a contiguous 32-bit integer tensor")
+    if numel is not None and x.numel() != numel:
+        raise ValueError(f"{name}: has {x.numel()} elements, expected {numel}")
+    return x.data_ptr()
+
+
+def _ptr_u8(x, mode: int, name: str, numel: Optional[int] = None):
+    """Raw pointer of a byte mask: numpy uint8 (host mode) or a torch uint8
+    tensor on the GPU (device mode)."""
+    if x is None:
+        return None
+    if isinstance(x, np.ndarray):
+        if mode != PTR_HOST:
+            raise TypeError(f"{name}: numpy array passed in device pointer mode")
+        if x.dtype != np.uint8 or not x.flags.c_contiguous:
+            raise TypeError(f"{name}: need a C-contiguous uint8 array")
+        if numel is not None and x.size != numel:
+            raise ValueError(f"{name}: has {x.size} elements, expected {numel}")
+        return x.ctypes.data
+    import torch
+
+    if not isinstance(x, torch.Tensor) or mode != PTR_DEVICE or not x.is_cuda:
+        raise TypeError(f"{name}: need a GPU tensor in device pointer mode")
+    if x.dtype != torch.uint8 or not x.is_contiguous():
+        raise TypeError(f"{name}: need a contiguous uint8 tensor")
     if numel is not None and x.numel() != numel:
         raise ValueError(f"{name}: has {x.numel()} elements, expected {numel}")
     return x.data_ptr()
@@ -557,6 +583,36 @@ class Context:
                          (coef_intp_pl, "coef_intp_pl", 18 * self.gall_pl), (coef_diff, "coef_diff", 18 * G),
                          (coef_diff_pl, "coef_diff_pl", 30)])
         _check(self.lib.nicam_oprt3d_divdamp(self.h, *a), "nicam_oprt3d_divdamp")
+
+    def horizontal_limiter_thuburn_mask(self, q_a, q_a_pl, q, q_pl, d, d_pl, ch, ch_pl, cmask, cmask_pl,
+                                        Qout_prev, Qout_prev_pl, Qout_post, Qout_post_pl, mask, mask_pl) -> None:
+        """nicam_horizontal_limiter_thuburn_mask: the limiter plus its uint8 change bits per cell-level."""
+        R, P = self.nreg, self.npl
+        a = self._args([(q_a, "q_a", 6 * R), (q_a_pl, "q_a_pl", P), (q, "q", R), (q_pl, "q_pl", P), (d, "d", R),
+                        (d_pl, "d_pl", P), (ch, "ch", 6 * R), (ch_pl, "ch_pl", P), (cmask, "cmask", 6 * R),
+                        (cmask_pl, "cmask_pl", P), (Qout_prev, "Qout_prev", 2 * R),
+                        (Qout_prev_pl, "Qout_prev_pl", 2 * P), (Qout_post, "Qout_post", 2 * R),
+                        (Qout_post_pl, "Qout_post_pl", 2 * P)])
+        a += [_ptr_u8(mask, self.mode, "mask", R), None if not self.have_pl else _ptr_u8(mask_pl, self.mode,
+                                                                                         "mask_pl", P)]
+        _check(self.lib.nicam_horizontal_limiter_thuburn_mask(self.h, *a), "nicam_horizontal_limiter_thuburn_mask")
+
+    def horizontal_limiter_thuburn_adjoint(self, q_a_b, q_a_b_pl, q_b, q_b_pl, Qout_prev_b, Qout_prev_b_pl,
+                                           Qout_post_b, Qout_post_b_pl, q_a, q_a_pl, q, q_pl, d, d_pl, ch, ch_pl,
+                                           cmask, cmask_pl, Qout_post, Qout_post_pl, mask, mask_pl) -> None:
+        """nicam_horizontal_limiter_thuburn_adjoint (Qout_post supplied)."""
+        R, P = self.nreg, self.npl
+        a = self._args([(q_a_b, "q_a_b", 6 * R), (q_a_b_pl, "q_a_b_pl", P), (q_b, "q_b", R), (q_b_pl, "q_b_pl", P),
+                        (Qout_prev_b, "Qout_prev_b", 2 * R), (Qout_prev_b_pl, "Qout_prev_b_pl", 2 * P),
+                        (Qout_post_b, "Qout_post_b", 2 * R), (Qout_post_b_pl, "Qout_post_b_pl", 2 * P),
+                        (q_a, "q_a", 6 * R), (q_a_pl, "q_a_pl", P), (q, "q", R), (q_pl, "q_pl", P), (d, "d", R),
+                        (d_pl, "d_pl", P), (ch, "ch", 6 * R), (ch_pl, "ch_pl", P), (cmask, "cmask", 6 * R),
+                        (cmask_pl, "cmask_pl", P), (Qout_post, "Qout_post", 2 * R),
+                        (Qout_post_pl, "Qout_post_pl", 2 * P)])
+        a += [_ptr_u8(mask, self.mode, "mask", R), None if not self.have_pl else _ptr_u8(mask_pl, self.mode,
+                                                                                         "mask_pl", P)]
+        _check(self.lib.nicam_horizontal_limiter_thuburn_adjoint(self.h, *a),
+               "nicam_horizontal_limiter_thuburn_adjoint")
 
     def vlim_mask_words(self):
         """uint32 words of the vertical limiter's branch mask: (regions, poles)."""

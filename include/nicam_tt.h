@@ -517,6 +517,49 @@ int nicam_vi_rhow_solver_adjoint(nicam_ctx ctx,
                                  double *Spre_b, double *Spre_b_pl,
                                  double dt);
 
+/* horizontal_limiter_thuburn that also records its branch mask: the same
+ * arguments and results as nicam_horizontal_limiter_thuburn, plus one uint8
+ * per cell-level (gall,kall,lall): bit m-1 set where face m (1-3) of a cell
+ * with i,j in [gmin-1,gmax] changed (0 elsewhere); mask_pl one uint8 per pole
+ * point (gall_pl,kall,2), bit 0 set where the ring face changed. */
+int nicam_horizontal_limiter_thuburn_mask(nicam_ctx ctx,
+                                          double *q_a, double *q_a_pl,
+                                          const double *q, const double *q_pl,
+                                          const double *d, const double *d_pl,
+                                          const double *ch, const double *ch_pl,
+                                          const double *cmask, const double *cmask_pl,
+                                          double *Qout_prev, double *Qout_prev_pl,
+                                          const double *Qout_post, const double *Qout_post_pl,
+                                          uint8_t *mask, uint8_t *mask_pl);
+
+/* Adjoint of horizontal_limiter_thuburn with Qout_post supplied, at fixed d,
+ * ch and cmask (dyn_horiz_adv_limiter/src/mod_src_tracer.f90:42-430):
+ *   inout q_a_b        in: cotangent of the limited q_a; out: cotangent of
+ *                      q_a's input (faces 4-6 the kernel overwrites get 0;
+ *                      faces it does not touch pass through)
+ *   out   q_b          cotangent of q
+ *   in    Qout_prev_b  cotangent of Qout_prev (the ring of Qout_prev_pl is a
+ *                      constant and is not read)
+ *   out   Qout_post_b  cotangent of Qout_post
+ *   in    q_a          the forward's output; q, d, ch, cmask, Qout_post its
+ *                      inputs; mask its mask
+ * A changed face value is attributed, in this order, to Qout_post min,
+ * Qout_post max, the arc's Qin min, Qin max, by equality with q_a.  The
+ * regions' q_b and Qout_post_b are sums of a few terms added atomically, so
+ * they may differ from a sequential sum by rounding. */
+int nicam_horizontal_limiter_thuburn_adjoint(nicam_ctx ctx,
+                                             double *q_a_b, double *q_a_b_pl,
+                                             double *q_b, double *q_b_pl,
+                                             const double *Qout_prev_b, const double *Qout_prev_b_pl,
+                                             double *Qout_post_b, double *Qout_post_b_pl,
+                                             const double *q_a, const double *q_a_pl,
+                                             const double *q, const double *q_pl,
+                                             const double *d, const double *d_pl,
+                                             const double *ch, const double *ch_pl,
+                                             const double *cmask, const double *cmask_pl,
+                                             const double *Qout_post, const double *Qout_post_pl,
+                                             const uint8_t *mask, const uint8_t *mask_pl);
+
 /* vertical_limiter_thuburn that also records its branch mask: the same
  * arguments and results as nicam_vertical_limiter_thuburn, plus one bit per
  * limited face (levels kmin+1..kmax), set where the limited value differs

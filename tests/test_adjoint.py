@@ -388,3 +388,54 @@ def test_oracle_hlim_tlm_matches_finite_differences(oracle):
         bad = err > 1e-5 * (1.0 + np.abs(t[tk]))
         assert bad.mean() < 0.02, (k, int(bad.sum()), bad.size)
         assert np.median(err) < 1e-7, k
+
+
+HLIM_GPU = [(6, 4, 2, 91, [1, 0]), (18, 6, 3, 92, [0, 1, 1]), (66, 12, 2, 93, [1, 0]), (130, 4, 1, 94, [0])]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("c", HLIM_GPU, ids=[f"g{c[0]}k{c[1]}" for c in HLIM_GPU])
+def test_gpu_hlim_mask_and_adjoint_match_oracle(gpu, oracle, c):
+    """Mask and q_a_b bit for bit; q_b and Qout_post_b (summed atomically on
+    the GPU) to 1e-13 of the largest value; the pole arrays bit for bit."""
+    import torch
+
+    n, kall, lall, seed, sgp = c
+    spec = synth.GridSpec(n, kall, lall, have_pl=True, seed=seed)
+    case = synth.make_case(spec, have_sgp=sgp)
+    f = dict(case["hlim"])
+    ctx = drivers.setup_context(spec, case["geo"])
+    T = lambda a: torch.from_numpy(np.ascontiguousarray(a, np.float64)).to(gpu)  # noqa: E731
+    dv = {k: T(v) for k, v in f.items()}
+    qp = torch.zeros((2,) + spec.shape3, dtype=torch.float64, device=gpu)
+    qp_pl = torch.zeros((2,) + spec.shape_pl, dtype=torch.float64, device=gpu)
+    m = torch.full((spec.lall * spec.kall * spec.gall,), 255, dtype=torch.uint8, device=gpu)
+    mp = torch.full((2 * spec.kall * spec.gall_pl,), 255, dtype=torch.uint8, device=gpu)
+    ctx.horizontal_limiter_thuburn_mask(dv["q_a"], dv["q_a_pl"], dv["q"], dv["q_pl"], dv["d"], dv["d_pl"], dv["ch"],
+                                        dv["ch_pl"], dv["cmask"], dv["cmask_pl"], qp, qp_pl, dv["Qout_post"],
+                                        dv["Qout_post_pl"], m, mp)
+    out = oracle.horizontal_limiter_thuburn(spec, sgp, f)
+    wm, wmp = pack_hlim_mask(spec, f, out)
+    np.testing.assert_array_equal(dv["q_a"].cpu().numpy(), out["q_a"])
+    np.testing.assert_array_equal(dv["q_a_pl"].cpu().numpy(), out["q_a_pl"])
+    np.testing.assert_array_equal(m.cpu().numpy(), wm.ravel())
+    np.testing.assert_array_equal(mp.cpu().numpy(), wmp.ravel())
+    rng = np.random.default_rng(seed)
+    u = {"q_a_b": rng.standard_normal(f["q_a"].shape), "q_a_b_pl": rng.standard_normal(f["q_a_pl"].shape),
+         "Qprev_b": rng.standard_normal(f["Qout_post"].shape),
+         "Qprev_b_pl": rng.standard_normal(f["Qout_post_pl"].shape)}
+    want = oracle.horizontal_limiter_thuburn_lin(spec, sgp, f, out, wm, wmp, True, u)
+    g = {k: T(v) for k, v in u.items()}
+    qb, qb_pl = torch.full_like(dv["q"], np.nan), torch.full_like(dv["q_pl"], np.nan)
+    pb, pb_pl = torch.full_like(qp, np.nan), torch.full_like(qp_pl, np.nan)
+    ctx.horizontal_limiter_thuburn_adjoint(g["q_a_b"], g["q_a_b_pl"], qb, qb_pl, g["Qprev_b"], g["Qprev_b_pl"], pb,
+                                           pb_pl, dv["q_a"], dv["q_a_pl"], dv["q"], dv["q_pl"], dv["d"], dv["d_pl"],
+                                           dv["ch"], dv["ch_pl"], dv["cmask"], dv["cmask_pl"], dv["Qout_post"],
+                                           dv["Qout_post_pl"], m, mp)
+    np.testing.assert_array_equal(g["q_a_b"].cpu().numpy(), want["q_a_b"])
+    np.testing.assert_array_equal(g["q_a_b_pl"].cpu().numpy(), want["q_a_b_pl"])
+    np.testing.assert_array_equal(qb_pl.cpu().numpy(), want["q_b_pl"])
+    np.testing.assert_array_equal(pb_pl.cpu().numpy(), want["Qpost_b_pl"])
+    for got, k in ((qb, "q_b"), (pb, "Qpost_b")):
+        w = want[k]
+        np.testing.assert_allclose(got.cpu().numpy(), w, rtol=0, atol=1e-13 * np.abs(w).max(), err_msg=k)
