@@ -130,7 +130,9 @@ __global__ void __launch_bounds__(BLOCK_VA) vi_adjoint_kernel(VaArgs A)
 // cotangent into a window of four q levels held in registers; a level is
 // stored once no later face reaches it.
 struct VlimLvl {
-    double qmin, qmax, rmin[3], rmax[3];
+    double qmin, qmax;
+    double n0, n1, n2;  // d Qout_min / d q(j-1), q(j), q(j+1)
+    double x0, x1, x2;  // d Qout_max / d q(j-1), q(j), q(j+1)
 };
 
 __device__ __forceinline__ double sgn_half(double x) { return copysign(0.5, x); }
@@ -171,16 +173,18 @@ __device__ __forceinline__ VlimLvl vlim_level(double qkm1, double qk, double qkp
     const double fac = (1.0 - zerosw) / (Cout + zerosw);
     const double e = (Cin + Cout - dk) - 1.0;
     const double cL = inflagL * ck1, cU = inflagU * ck2k;
-#pragma unroll
-    for (int m = 0; m < 3; ++m) {
+    auto rows = [&](int m, double &rn, double &rx) {
         const double dnx = (m == nmax) ? 1.0 : 0.0, dnn = (m == nmin) ? 1.0 : 0.0;
         const double cqx = ((m == maxL) ? cL : 0.0) + ((m == maxU) ? cU : 0.0);
         const double cqn = ((m == minL) ? cL : 0.0) + ((m == minU) ? cU : 0.0);
-        r.rmin[m] = fac * (e * dnx - cqx);
-        r.rmax[m] = fac * (e * dnn - cqn);
-    }
-    r.rmin[1] = (r.rmin[1] + fac) + zerosw;
-    r.rmax[1] = (r.rmax[1] + fac) + zerosw;
+        rn = fac * (e * dnx - cqx);
+        rx = fac * (e * dnn - cqn);
+    };
+    rows(0, r.n0, r.x0);
+    rows(1, r.n1, r.x1);
+    rows(2, r.n2, r.x2);
+    r.n1 = (r.n1 + fac) + zerosw;
+    r.x1 = (r.x1 + fac) + zerosw;
     return r;
 }
 
@@ -196,23 +200,42 @@ struct VlaArgs {
     double eps, big;
 };
 
+// A work-group covers VCOL columns; each column is split into VRUN runs of
+// faces.  The output cotangent of every face of the group's columns is first
+// staged in LDS, so that q_h_b can be rewritten in place while other runs
+// still read their neighbours' faces.  The work-item of run r writes q_h_b
+// for its faces and q_b for its levels; a level's cotangent gathers faces
+// m-1 .. m+2 in ascending order, the order of a sequential sweep, so the sums
+// are the same bits whichever run computes them.
+constexpr int VCOL = 32, VRUN = 8, BLOCK_VLA = VCOL * VRUN;
+
 template <bool POLE>
-__device__ __forceinline__ void vlim_adjoint_column(const VlaField &F, size_t base, size_t s, size_t ck2,
-                                                    const uint32_t *mw, size_t mstride, int kall, double EPS,
-                                                    double BIG)
+__device__ __forceinline__ void vlim_adjoint_run(const VlaField &F, size_t base, size_t s, size_t ck2,
+                                                 const uint32_t *mw, size_t mstride, int kall, double EPS,
+                                                 double BIG, const double *s_ob, int r)
 {
     const int kmin = 1, kmax = kall - 2;  // 0-based
+    const int nf = kmax - kmin, kc = (nf + VRUN - 1) / VRUN;
+    const int pb = min(r * kc, nf), pe = min(pb + kc, nf);
+    if (pb >= pe) return;
+    // levels written by this run; run 0 also takes the levels below its faces, the run with the last face
+    // those above
+    const int mb = (r == 0) ? 0 : kmin + 1 + pb;
+    const int me = (pe == nf) ? kall : kmin + 1 + pe;
+    const int kf = max(kmin + 1, mb - 1), kl = min(kmax, me + 1);  // faces that reach [mb, me)
     auto at = [&](int k) { return base + (size_t)k * s; };
-    // level kmin (the level below the first limited face)
-    double qm = F.q[at(kmin - 1)], q0 = F.q[at(kmin)], qp = F.q[at(kmin + 1)];
-    double c1 = F.ck[at(kmin)], c1p = F.ck[at(kmin + 1)];
-    VlimLvl lo = vlim_level<POLE>(qm, q0, qp, c1, c1p, F.ck[ck2 + at(kmin)], F.d[at(kmin)], EPS, BIG);
-    // q cotangents of levels k-2 .. k+1 while face k is processed
-    double w0 = 0.0, w1 = 0.0, w2 = 0.0, w3 = 0.0;
+    double qm = F.q[at(kf - 2)], q0 = F.q[at(kf - 1)], qp = F.q[at(kf)];
+    double c1 = F.ck[at(kf - 1)], c1p = F.ck[at(kf)];
+    VlimLvl lo = vlim_level<POLE>(qm, q0, qp, c1, c1p, F.ck[ck2 + at(kf - 1)], F.d[at(kf - 1)], EPS, BIG);
+    double w0 = 0.0, w1 = 0.0, w2 = 0.0, w3 = 0.0;  // levels k-2 .. k+1
+    int wi = -1;
     uint32_t word = 0;
-    for (int k = kmin + 1; k <= kmax; ++k) {
+    for (int k = kf; k <= kl; ++k) {
         const int p = k - (kmin + 1);
-        if ((p & 31) == 0) word = mw[(size_t)(p >> 5) * mstride];
+        if ((p >> 5) != wi) {
+            wi = p >> 5;
+            word = mw[(size_t)wi * mstride];
+        }
         qm = q0;
         q0 = qp;
         qp = F.q[at(k + 1)];
@@ -221,61 +244,80 @@ __device__ __forceinline__ void vlim_adjoint_column(const VlaField &F, size_t ba
         const VlimLvl hi = vlim_level<POLE>(qm, q0, qp, c1, c1p, F.ck[ck2 + at(k)], F.d[at(k)], EPS, BIG);
         const bool clipped = (word >> (p & 31)) & 1u;
         const double inflagL = 0.5 - sgn_half(c1);
-        const size_t o = at(k);
-        const double ob = F.qhb[o];
-        double row[4] = {0.0, 0.0, 0.0, 0.0};
+        const double ob = s_ob[(size_t)k * VCOL];
+        // row of the face: the selected bound's derivatives, placed at q(k-2..k) or q(k-1..k+1); written
+        // with selects only (a pointer to either level's arrays would put both in scratch memory)
+        double row0 = 0.0, row1 = 0.0, row2 = 0.0, row3 = 0.0;
         double cqh = 1.0;
         if (clipped) {
             cqh = 0.0;
-            const double out = F.q_h[o];
+            const double out = F.q_h[at(k)];
             const bool from_lo = inflagL > 0.0;
-            const VlimLvl &L = from_lo ? lo : hi;
-            const double *r = (out == L.qmin) ? L.rmin : L.rmax;
-            if (from_lo) {
-                row[0] = r[0];
-                row[1] = r[1];
-                row[2] = r[2];
-            } else {
-                row[1] = r[0];
-                row[2] = r[1];
-                row[3] = r[2];
-            }
+            const bool use_min = out == (from_lo ? lo.qmin : hi.qmin);
+            const double e0 = from_lo ? (use_min ? lo.n0 : lo.x0) : (use_min ? hi.n0 : hi.x0);
+            const double e1 = from_lo ? (use_min ? lo.n1 : lo.x1) : (use_min ? hi.n1 : hi.x1);
+            const double e2 = from_lo ? (use_min ? lo.n2 : lo.x2) : (use_min ? hi.n2 : hi.x2);
+            row0 = from_lo ? e0 : 0.0;
+            row1 = from_lo ? e1 : e0;
+            row2 = from_lo ? e2 : e1;
+            row3 = from_lo ? 0.0 : e2;
         }
-        F.qhb[o] = cqh * ob;
-        w0 = w0 + row[0] * ob;
-        w1 = w1 + row[1] * ob;
-        w2 = w2 + row[2] * ob;
-        w3 = w3 + row[3] * ob;
-        F.qb[at(k - 2)] = w0;  // no later face reaches level k-2
+        if (p >= pb && p < pe) F.qhb[at(k)] = cqh * ob;
+        w0 = w0 + row0 * ob;
+        w1 = w1 + row1 * ob;
+        w2 = w2 + row2 * ob;
+        w3 = w3 + row3 * ob;
+        if (k - 2 >= mb && k - 2 < me) F.qb[at(k - 2)] = w0;  // no later face reaches level k-2
         w0 = w1;
         w1 = w2;
         w2 = w3;
         w3 = 0.0;
         lo = hi;
     }
-    F.qb[at(kmax - 1)] = w0;
-    F.qb[at(kmax)] = w1;
-    F.qb[at(kmax + 1)] = w2;
+    if (kl - 1 >= mb && kl - 1 < me) F.qb[at(kl - 1)] = w0;
+    if (kl >= mb && kl < me) F.qb[at(kl)] = w1;
+    if (kl + 1 >= mb && kl + 1 < me) F.qb[at(kl + 1)] = w2;
 }
-
-constexpr int BLOCK_VLA = 256;
 
 __global__ void __launch_bounds__(BLOCK_VLA) vlim_adjoint_kernel(VlaArgs A)
 {
-    const long t = (long)blockIdx.x * BLOCK_VLA + threadIdx.x;
+    extern __shared__ double s_ob[];  // [kall][VCOL]
+    const int c = threadIdx.x % VCOL, r = threadIdx.x / VCOL;
+    const long t = (long)blockIdx.x * VCOL + c;
     const long nreg = (long)A.gall * A.lall;
+    const long ncol = nreg + (A.have_pl ? (long)A.gall_pl * kLallPl : 0);
+    VlaField F{};  // by value: a pointer into the kernel arguments would copy them to scratch memory
+    bool have = false;
+    size_t base = 0, s = 0, ck2 = 0, mstride = 0;
+    const uint32_t *mw = nullptr;
     if (t < nreg) {
         const int g = (int)(t % A.gall), l = (int)(t / A.gall);
-        vlim_adjoint_column<false>(A.reg, (size_t)l * A.kall * A.gall + g, A.gall,
-                                   (size_t)A.gall * A.kall * A.lall, A.reg.mask + (size_t)l * A.nword * A.gall + g,
-                                   A.gall, A.kall, A.eps, A.big);
-    } else if (A.have_pl && t < nreg + (long)A.gall_pl * kLallPl) {
+        F = A.reg;
+        have = true;
+        base = (size_t)l * A.kall * A.gall + g;
+        s = A.gall;
+        ck2 = (size_t)A.gall * A.kall * A.lall;
+        mw = A.reg.mask + (size_t)l * A.nword * A.gall + g;
+        mstride = A.gall;
+    } else if (t < ncol) {
         const int p = (int)(t - nreg);
         const int g = p % A.gall_pl, l = p / A.gall_pl;
-        vlim_adjoint_column<true>(A.pl, (size_t)l * A.kall * A.gall_pl + g, A.gall_pl,
-                                  (size_t)A.gall_pl * A.kall * kLallPl,
-                                  A.pl.mask + (size_t)l * A.nword * A.gall_pl + g, A.gall_pl, A.kall, A.eps, A.big);
+        F = A.pl;
+        have = true;
+        base = (size_t)l * A.kall * A.gall_pl + g;
+        s = A.gall_pl;
+        ck2 = (size_t)A.gall_pl * A.kall * kLallPl;
+        mw = A.pl.mask + (size_t)l * A.nword * A.gall_pl + g;
+        mstride = A.gall_pl;
     }
+    if (have)
+        for (int k = r; k < A.kall; k += VRUN) s_ob[(size_t)k * VCOL + c] = F.qhb[base + (size_t)k * s];
+    __syncthreads();
+    if (!have) return;
+    if (t < nreg)
+        vlim_adjoint_run<false>(F, base, s, ck2, mw, mstride, A.kall, A.eps, A.big, s_ob + c, r);
+    else
+        vlim_adjoint_run<true>(F, base, s, ck2, mw, mstride, A.kall, A.eps, A.big, s_ob + c, r);
 }
 
 // ---- horizontal_limiter_thuburn (dyn_horiz_adv_limiter/src/mod_src_tracer.f90:42-430)
@@ -625,7 +667,12 @@ int launch_vlim_adjoint(nicam_ctx c, double *qhb, double *qhb_pl, double *qb, do
     A.eps = c->eps;
     A.big = c->big;
     const long ncol = (long)D.gall * D.lall + (D.have_pl ? (long)D.gall_pl * kLallPl : 0);
-    hipLaunchKernelGGL(vlim_adjoint_kernel, dim3((unsigned)((ncol + BLOCK_VLA - 1) / BLOCK_VLA)), dim3(BLOCK_VLA), 0,
+    const size_t lds = sizeof(double) * VCOL * D.kall;
+    if (lds > 160 * 1024) return fail(NICAM_ERR_ARG, "kall too large for the staged cotangents");
+    if (lds > 64 * 1024)
+        NICAM_HIP(hipFuncSetAttribute((const void *)vlim_adjoint_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                      (int)lds));
+    hipLaunchKernelGGL(vlim_adjoint_kernel, dim3((unsigned)((ncol + VCOL - 1) / VCOL)), dim3(BLOCK_VLA), lds,
                        c->stream, A);
     NICAM_HIP(hipGetLastError());
     return NICAM_OK;

@@ -62,6 +62,37 @@ def kernel_bytes(spec: GridSpec, word: int = 8) -> Dict[str, int]:
                 vi_rhow_solver=vi)
 
 
+def adjoint_bytes(spec: GridSpec, word: int = 8) -> Dict[str, int]:
+    """Algorithmic bytes of the adjoint kernels: each array the kernel must
+    read or write counted once over the levels and points it touches (the
+    masks at their stored width; clearing q_b and Qout_post_b before the
+    atomic sums and the solver's reuse of its outputs as work space are not
+    counted)."""
+    n, kall, lall, npl = spec.gall_1d, spec.kall, spec.lall, spec.gall_pl
+    gall = n * n
+    inner = (n - 2) ** 2
+    arcs = (n - 1) ** 2
+    ncol = gall * lall
+    pl = 2 if spec.have_pl else 0
+    ring = npl - 1
+    # h-limiter: reads q, Qout_post (2), Qout_prev_b (2) on every cell; d and ch (6) on inner cells; q_a_b (6),
+    # q_a (3) and cmask (3) on the arcs; writes q_b, Qout_post_b (2) and q_a_b (6)
+    hr = (gall * (1 + 2 + 2) + inner * (1 + 6) + arcs * (6 + 3 + 3)) * word + gall
+    hw = (gall * (1 + 2) + arcs * 6) * word
+    hlim = (hr + hw) * kall * lall
+    hlim += pl * kall * ((npl * (1 + 2 + 2 + 1 + 2) + ring * 3) * word + npl)
+    # v-limiter per column: q, d, ck (2 planes), q_h, q_h_b, mask; writes q_h_b, q_b
+    nf = kall - 3
+    vcol = (kall + (kall - 2) + (kall - 1) + (kall - 2) + 2 * nf) * word + 4 * ((nf + 31) // 32)
+    vcol += (nf + kall) * word
+    vlim = vcol * (ncol + pl * npl)
+    # VI per column: rhogw_b, GSGAM2H (kall-1), Mc, Ml, Mu (kall-3 each), RGSGAM2H (2), RGSGAM2, RGAM,
+    # RGAMH (kall); writes the seven cotangents
+    vicol = (2 * (kall - 1) + 3 * (kall - 3) + 2 + 3 * kall + 6 * kall + (kall - 1)) * word
+    vi = vicol * (ncol + pl * npl)
+    return dict(horizontal_limiter_thuburn=hlim, vertical_limiter_thuburn=vlim, vi_rhow_solver=vi)
+
+
 def kernel_flops(spec: GridSpec) -> Dict[str, int]:
     """Floating-point operations per launch (reported, not bounding): flux
     ~125, h-limiter ~170 (min/max included), v-limiter ~45, VI ~45 per
