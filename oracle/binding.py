@@ -86,6 +86,8 @@ class Oracle:
                                                  [c_double, c_double, c_double, c_int])
         L.orc_vertical_limiter_thuburn_tlm.argtypes = [POINTER(OrcDims)] + [P] * 16
         L.orc_horizontal_limiter_thuburn_tlm.argtypes = [POINTER(OrcDims), P] + [P] * 22
+        L.orc_horizontal_flux_tlm.argtypes = [POINTER(OrcDims), P] + [P] * 20 + [c_double] + [P] * 6
+        L.orc_horizontal_flux_adjoint.argtypes = [POINTER(OrcDims), P] + [P] * 20 + [c_double] + [P] * 6
         L.orc_horizontal_limiter_thuburn_adjoint.argtypes = [POINTER(OrcDims), P] + [P] * 22
         L.orc_vertical_limiter_thuburn_adjoint.argtypes = [POINTER(OrcDims)] + [P] * 14
         L.orc_vi_rhow_solver.argtypes = ([POINTER(OrcDims)] + [P] * 14 + [c_double] + [P] * 16 + [P] * 3 +
@@ -287,6 +289,41 @@ class Oracle:
                    q_b_pl=np.zeros_like(ins["q_pl"]))
         self.lib.orc_vertical_limiter_thuburn_adjoint(ctypes.byref(D), _p(out["q_h_b"]), _p(out["q_h_b_pl"]),
                                                       _p(out["q_b"]), _p(out["q_b_pl"]), *fw)
+        return out
+
+    def horizontal_flux_lin(self, spec, geo, f, dt, adjoint, a):
+        """TLM (adjoint=False; a = drho, drhovx, drhovy, drhovz (+_pl) -> dflx_h, dGRD_xc (+_pl)) or adjoint
+        (adjoint=True; a = flx_h_b, GRD_xc_b (+_pl) -> rho_b .. rhovz_b (+_pl)) of the flux at f."""
+        D = self.dims(spec)
+        sgp_a = np.ascontiguousarray(geo["have_sgp"], dtype=np.int32)
+        names = ["rho", "rhovx", "rhovy", "rhovz"]
+        X = {}
+        for n in names:
+            X[n], X[n + "_pl"] = _f64(f[n]), _f64(f[n + "_pl"])
+        met = [_f64(geo[k]) for k in ("GMTR_t", "GMTR_t_pl", "GMTR_a", "GMTR_a_pl", "GMTR_p", "GMTR_p_pl")]
+        xs = []
+        for n in names:
+            xs += [_p(X[n]), _p(X[n + "_pl"])]
+        x = {k: _f64(v) for k, v in a.items()}
+        if not adjoint:
+            out = dict(dflx_h=np.zeros((6,) + spec.shape3), dflx_h_pl=np.zeros(spec.shape_pl),
+                       dGRD_xc=np.zeros((3, 3) + spec.shape3), dGRD_xc_pl=np.zeros((3,) + spec.shape_pl))
+            ds = []
+            for n in names:
+                ds += [_p(x["d" + n]), _p(x["d" + n + "_pl"])]
+            self.lib.orc_horizontal_flux_tlm(ctypes.byref(D), sgp_a.ctypes.data, _p(out["dflx_h"]),
+                                             _p(out["dflx_h_pl"]), _p(out["dGRD_xc"]), _p(out["dGRD_xc_pl"]), *xs,
+                                             *ds, float(dt), *[_p(m) for m in met])
+            return out
+        out = {}
+        for n in names:
+            out[n + "_b"], out[n + "_b_pl"] = np.zeros(spec.shape3), np.zeros(spec.shape_pl)
+        bs = []
+        for n in names:
+            bs += [_p(out[n + "_b"]), _p(out[n + "_b_pl"])]
+        self.lib.orc_horizontal_flux_adjoint(ctypes.byref(D), sgp_a.ctypes.data, _p(x["flx_h_b"]),
+                                             _p(x["flx_h_b_pl"]), _p(x["GRD_xc_b"]), _p(x["GRD_xc_b_pl"]), *bs, *xs,
+                                             float(dt), *[_p(m) for m in met])
         return out
 
     def horizontal_limiter_thuburn_lin(self, spec, have_sgp, f, q_a_out, mask, mask_pl, adjoint, a):

@@ -1819,3 +1819,275 @@ void orc_horizontal_limiter_thuburn_adjoint(const orc_dims *D, const int *have_s
     hlim_lin(D, have_sgp, 1, q_a_b, q_a_b_pl, q_b, q_b_pl, (double *)Qprev_b, (double *)Qprev_b_pl, Qpost_b,
              Qpost_b_pl, q_a, q_a_pl, q, q_pl, d, d_pl, ch, ch_pl, cmask, cmask_pl, Qpost, Qpost_pl, mask, mask_pl);
 }
+
+/* ------------------------------------------------------------------------- */
+/* Tangent-linear and adjoint of horizontal_flux                              */
+/* (dyn_horiz_adv_flux/src/mod_src_tracer.f90:42-379), metrics and dt fixed.  */
+/* ------------------------------------------------------------------------- */
+/* flx_h is linear in rho*v; GRD_xc = GRD_xr - (rho*v)_arc / max(rho_arc, EPS)
+ * * dt/2, whose derivative in rho_arc is (rho*v)_arc dt/2 / rho_arc^2 where
+ * rho_arc > EPS... precisely where max() takes rho_arc.  The outputs that
+ * carry a derivative are those the arc loops write last: faces 1-3 and
+ * GRD_xc of the arc origins, faces 4-6 of their neighbours, except face 6 of
+ * the cell the singular-point fix zeroes (FLX:311-320).  The triangle values
+ * are the forward's (same sums, same order). */
+#define FL_TW(g, t, w) GMTR_t[(((size_t)((w) - 1) * 2 + ((t) - 1)) * lall + l) * gall + (g)]
+#define FL_A(g, a, c) GMTR_a[(((size_t)((c) - 1) * 3 + ((a) - 1)) * lall + l) * gall + (g)]
+#define FL_RA(g) GMTR_p[((size_t)(P_RAREA - 1) * lall + l) * gall + (g)]
+
+static void flux_lin_slab(int iall, int sgp, int l, int lall, const double *X[4], double dt, double EPS,
+                          const double *GMTR_t, const double *GMTR_a, const double *GMTR_p, int adjoint,
+                          const double *dX[4], double *dflx, double *dxc, size_t plane, const double *flxb,
+                          const double *xcb, double *Xb[4])
+{
+    const int gall = iall * iall;
+    double *T = (double *)calloc((size_t)16 * gall, sizeof(double)); /* TI, TJ (forward) and their tangents */
+#define TIv(f, g) T[(size_t)(2 * (f)) * gall + (g)]
+#define TJv(f, g) T[(size_t)(2 * (f) + 1) * gall + (g)]
+#define TId(f, g) T[(size_t)(8 + 2 * (f)) * gall + (g)]
+#define TJd(f, g) T[(size_t)(8 + 2 * (f) + 1) * gall + (g)]
+    for (int f = 0; f < 4; ++f)
+        for (int j = 0; j <= iall - 2; ++j)
+            for (int i = 0; i <= iall - 2; ++i) {
+                const int g = j * iall + i;
+                const double *x = X[f];
+                TIv(f, g) = x[g] * FL_TW(g, TI, W1) + x[g + 1] * FL_TW(g, TI, W2) + x[g + iall + 1] * FL_TW(g, TI, W3);
+                TJv(f, g) = x[g] * FL_TW(g, TJ, W1) + x[g + iall + 1] * FL_TW(g, TJ, W2) + x[g + iall] * FL_TW(g, TJ, W3);
+                if (!adjoint) {
+                    const double *y = dX[f];
+                    TId(f, g) = y[g] * FL_TW(g, TI, W1) + y[g + 1] * FL_TW(g, TI, W2) + y[g + iall + 1] * FL_TW(g, TI, W3);
+                    TJd(f, g) = y[g] * FL_TW(g, TJ, W1) + y[g + iall + 1] * FL_TW(g, TJ, W2) + y[g + iall] * FL_TW(g, TJ, W3);
+                }
+            }
+    if (sgp)
+        for (int f = 0; f < 4; ++f) {
+            TIv(f, 0) = TJv(f, 1);
+            if (!adjoint) TId(f, 0) = TJd(f, 1);
+        }
+    const int sgp_face = sgp ? iall + 1 : -1; /* face 6 of this cell stays zero */
+    if (!adjoint)
+        for (size_t t = 0; t < 6 * plane; t += plane)
+            for (int g = 0; g < gall; ++g) dflx[t + g] = 0.0;
+    if (!adjoint)
+        for (size_t t = 0; t < 9 * plane; t += plane)
+            for (int g = 0; g < gall; ++g) dxc[t + g] = 0.0;
+    double *Tb = adjoint ? (double *)calloc((size_t)8 * gall, sizeof(double)) : NULL;
+#define TIb(f, g) Tb[(size_t)(2 * (f)) * gall + (g)]
+#define TJb(f, g) Tb[(size_t)(2 * (f) + 1) * gall + (g)]
+    for (int a = 1; a <= 3; ++a)
+        for (int j = (a == 1 ? 1 : 0); j <= iall - 2; ++j)
+            for (int i = (a == 3 ? 1 : 0); i <= iall - 2; ++i) {
+                const int g = j * iall + i;
+                const int nb = g + (a == 1 ? 1 : (a == 2 ? iall + 1 : iall));
+                /* the two triangles of the arc: (kind, cell), kind 0 = TI, 1 = TJ */
+                const int t1k = a == 1 ? 1 : (a == 2 ? 0 : 1), t1c = a == 1 ? g - iall : g;
+                const int t2k = a == 1 ? 0 : (a == 2 ? 1 : 0), t2c = a == 3 ? g - 1 : g;
+                double s2[4];
+                for (int f = 0; f < 4; ++f)
+                    s2[f] = (t1k ? TJv(f, t1c) : TIv(f, t1c)) + (t2k ? TJv(f, t2c) : TIv(f, t2c));
+                const double rr = 1.0 / mx(s2[0], EPS);
+                const int live_rho = !(EPS > s2[0]);
+                const double hn[3] = {FL_A(g, a, HNX), FL_A(g, a, HNY), FL_A(g, a, HNZ)};
+                const double ra = FL_RA(g) * dt, rn = FL_RA(nb) * dt;
+                if (!adjoint) {
+                    double d2[4];
+                    for (int f = 0; f < 4; ++f)
+                        d2[f] = (t1k ? TJd(f, t1c) : TId(f, t1c)) + (t2k ? TJd(f, t2c) : TId(f, t2c));
+                    const double dflux = 0.5 * (d2[1] * hn[0] + d2[2] * hn[1] + d2[3] * hn[2]);
+                    dflx[(size_t)(a - 1) * plane + g] = dflux * ra;
+                    dflx[(size_t)(a + 2) * plane + nb] = (a == 3 && nb == sgp_face) ? 0.0 : -dflux * rn;
+                    const double drr = live_rho ? -(d2[0] * rr * rr) : 0.0;
+                    for (int dd = 0; dd < 3; ++dd)
+                        dxc[((size_t)dd * 3 + (a - 1)) * plane + g] = -((d2[1 + dd] * rr + s2[1 + dd] * drr) * dt * 0.5);
+                } else {
+                    const double fb_nb = (a == 3 && nb == sgp_face) ? 0.0 : flxb[(size_t)(a + 2) * plane + nb];
+                    const double fluxb = flxb[(size_t)(a - 1) * plane + g] * ra - fb_nb * rn;
+                    double b[4];
+                    double r2b = 0.0;
+                    for (int dd = 0; dd < 3; ++dd) {
+                        const double xb = xcb[((size_t)dd * 3 + (a - 1)) * plane + g];
+                        b[1 + dd] = 0.5 * hn[dd] * fluxb - xb * rr * dt * 0.5;
+                        r2b = r2b + xb * s2[1 + dd] * dt * 0.5 * rr * rr;
+                    }
+                    b[0] = live_rho ? r2b : 0.0;
+                    for (int f = 0; f < 4; ++f) {
+                        if (t1k) TJb(f, t1c) += b[f]; else TIb(f, t1c) += b[f];
+                        if (t2k) TJb(f, t2c) += b[f]; else TIb(f, t2c) += b[f];
+                    }
+                }
+            }
+    if (adjoint) {
+        if (sgp)
+            for (int f = 0; f < 4; ++f) {
+                TJb(f, 1) += TIb(f, 0);
+                TIb(f, 0) = 0.0;
+            }
+        for (int f = 0; f < 4; ++f)
+            for (int j = 0; j <= iall - 2; ++j)
+                for (int i = 0; i <= iall - 2; ++i) {
+                    const int g = j * iall + i;
+                    double *y = Xb[f];
+                    const double bi = TIb(f, g), bj = TJb(f, g);
+                    y[g] += bi * FL_TW(g, TI, W1) + bj * FL_TW(g, TJ, W1);
+                    y[g + 1] += bi * FL_TW(g, TI, W2);
+                    y[g + iall + 1] += bi * FL_TW(g, TI, W3) + bj * FL_TW(g, TJ, W2);
+                    y[g + iall] += bj * FL_TW(g, TJ, W3);
+                }
+        free(Tb);
+    }
+#undef TIv
+#undef TJv
+#undef TId
+#undef TJd
+#undef TIb
+#undef TJb
+    free(T);
+}
+
+static void flux_lin_pole(int npl, int l, const double *X[4], double dt, double EPS, const double *GMTR_t_pl,
+                          const double *GMTR_a_pl, const double *GMTR_p_pl, int adjoint, const double *dX[4],
+                          double *dflx, double *dxc, size_t pplane, const double *flxb, const double *xcb,
+                          double *Xb[4])
+{
+    const int gall_pl = npl;
+#define PT(v, w) GMTR_t_pl[((size_t)((w) - 1) * LALL_PL + l) * gall_pl + (v)]
+#define PA(v, c) GMTR_a_pl[((size_t)((c) - 1) * LALL_PL + l) * gall_pl + (v)]
+    const double ran = GMTR_p_pl[((size_t)(P_RAREA - 1) * LALL_PL + l) * gall_pl + 0] * dt;
+    double Tv[4][16], Td[4][16], Tb[4][16];
+    for (int v = 1; v < npl; ++v) {
+        const int vp1 = (v + 1 == npl) ? 1 : v + 1;
+        for (int f = 0; f < 4; ++f) {
+            Tv[f][v] = X[f][0] * PT(v, W1) + X[f][v] * PT(v, W2) + X[f][vp1] * PT(v, W3);
+            if (!adjoint) Td[f][v] = dX[f][0] * PT(v, W1) + dX[f][v] * PT(v, W2) + dX[f][vp1] * PT(v, W3);
+            Tb[f][v] = 0.0;
+        }
+    }
+    if (!adjoint) {
+        for (int v = 0; v < npl; ++v) dflx[v] = 0.0;
+        for (int dd = 0; dd < 3; ++dd)
+            for (int v = 0; v < npl; ++v) dxc[(size_t)dd * pplane + v] = 0.0;
+    }
+    for (int v = 1; v < npl; ++v) {
+        const int vm1 = (v == 1) ? npl - 1 : v - 1;
+        double s2[4];
+        for (int f = 0; f < 4; ++f) s2[f] = Tv[f][vm1] + Tv[f][v];
+        const double rr = 1.0 / mx(s2[0], EPS);
+        const int live_rho = !(EPS > s2[0]);
+        const double hn[3] = {PA(v, HNX), PA(v, HNY), PA(v, HNZ)};
+        if (!adjoint) {
+            double d2[4];
+            for (int f = 0; f < 4; ++f) d2[f] = Td[f][vm1] + Td[f][v];
+            dflx[v] = 0.5 * (d2[1] * hn[0] + d2[2] * hn[1] + d2[3] * hn[2]) * ran;
+            const double drr = live_rho ? -(d2[0] * rr * rr) : 0.0;
+            for (int dd = 0; dd < 3; ++dd) dxc[(size_t)dd * pplane + v] = -((d2[1 + dd] * rr + s2[1 + dd] * drr) * dt * 0.5);
+        } else {
+            const double fluxb = flxb[v] * ran;
+            double b[4], r2b = 0.0;
+            for (int dd = 0; dd < 3; ++dd) {
+                const double xb = xcb[(size_t)dd * pplane + v];
+                b[1 + dd] = 0.5 * hn[dd] * fluxb - xb * rr * dt * 0.5;
+                r2b = r2b + xb * s2[1 + dd] * dt * 0.5 * rr * rr;
+            }
+            b[0] = live_rho ? r2b : 0.0;
+            for (int f = 0; f < 4; ++f) {
+                Tb[f][vm1] += b[f];
+                Tb[f][v] += b[f];
+            }
+        }
+    }
+    if (adjoint)
+        for (int v = 1; v < npl; ++v) {
+            const int vp1 = (v + 1 == npl) ? 1 : v + 1;
+            for (int f = 0; f < 4; ++f) {
+                Xb[f][0] += Tb[f][v] * PT(v, W1);
+                Xb[f][v] += Tb[f][v] * PT(v, W2);
+                Xb[f][vp1] += Tb[f][v] * PT(v, W3);
+            }
+        }
+#undef PT
+#undef PA
+}
+
+/* mode 0: TLM (dflx, dxc from drho, drhovx/y/z); mode 1: adjoint (rho_b...
+ * written from flx_h_b, GRD_xc_b). */
+static void flux_lin(const orc_dims *D, const int *have_sgp, int adjoint, double dt, const double *const X_reg[4],
+                     const double *const X_pl[4], const double *const dX_reg[4], const double *const dX_pl[4],
+                     double *dflx, double *dflx_pl, double *dxc, double *dxc_pl, const double *flxb,
+                     const double *flxb_pl, const double *xcb, const double *xcb_pl, double *const Xb_reg[4],
+                     double *const Xb_pl[4], const double *GMTR_t, const double *GMTR_t_pl, const double *GMTR_a,
+                     const double *GMTR_a_pl, const double *GMTR_p, const double *GMTR_p_pl)
+{
+    const int iall = D->gall_1d, gall = iall * iall, kall = D->kall, lall = D->lall, npl = D->gall_pl;
+    const size_t plane = (size_t)gall * kall * lall, pplane = (size_t)npl * kall * LALL_PL;
+    if (adjoint) {
+        for (int f = 0; f < 4; ++f) memset(Xb_reg[f], 0, plane * sizeof(double));
+        if (D->have_pl)
+            for (int f = 0; f < 4; ++f) memset(Xb_pl[f], 0, pplane * sizeof(double));
+    }
+    for (int l = 0; l < lall; ++l)
+#pragma omp parallel for schedule(static)
+        for (int k = 0; k < kall; ++k) {
+            const size_t s = ((size_t)l * kall + k) * gall;
+            const double *X[4], *dX[4] = {0, 0, 0, 0};
+            double *Xb[4] = {0, 0, 0, 0};
+            for (int f = 0; f < 4; ++f) {
+                X[f] = X_reg[f] + s;
+                if (!adjoint) dX[f] = dX_reg[f] + s;
+                else Xb[f] = Xb_reg[f] + s;
+            }
+            flux_lin_slab(iall, have_sgp[l], l, lall, X, dt, D->eps, GMTR_t, GMTR_a, GMTR_p, adjoint, dX,
+                          adjoint ? NULL : dflx + s, adjoint ? NULL : dxc + s, plane, adjoint ? flxb + s : NULL,
+                          adjoint ? xcb + s : NULL, Xb);
+        }
+    if (D->have_pl)
+        for (int l = 0; l < LALL_PL; ++l)
+            for (int k = 0; k < kall; ++k) {
+                const size_t s = ((size_t)l * kall + k) * npl;
+                const double *X[4], *dX[4] = {0, 0, 0, 0};
+                double *Xb[4] = {0, 0, 0, 0};
+                for (int f = 0; f < 4; ++f) {
+                    X[f] = X_pl[f] + s;
+                    if (!adjoint) dX[f] = dX_pl[f] + s;
+                    else Xb[f] = Xb_pl[f] + s;
+                }
+                flux_lin_pole(npl, l, X, dt, D->eps, GMTR_t_pl, GMTR_a_pl, GMTR_p_pl, adjoint, dX,
+                              adjoint ? NULL : dflx_pl + s, adjoint ? NULL : dxc_pl + s, pplane,
+                              adjoint ? flxb_pl + s : NULL, adjoint ? xcb_pl + s : NULL, Xb);
+            }
+}
+
+void orc_horizontal_flux_tlm(const orc_dims *D, const int *have_sgp, double *dflx, double *dflx_pl, double *dxc,
+                             double *dxc_pl, const double *rho, const double *rho_pl, const double *rhovx,
+                             const double *rhovx_pl, const double *rhovy, const double *rhovy_pl,
+                             const double *rhovz, const double *rhovz_pl, const double *drho, const double *drho_pl,
+                             const double *drhovx, const double *drhovx_pl, const double *drhovy,
+                             const double *drhovy_pl, const double *drhovz, const double *drhovz_pl, double dt,
+                             const double *GMTR_t, const double *GMTR_t_pl, const double *GMTR_a,
+                             const double *GMTR_a_pl, const double *GMTR_p, const double *GMTR_p_pl)
+{
+    const double *Xr[4] = {rho, rhovx, rhovy, rhovz}, *Xp[4] = {rho_pl, rhovx_pl, rhovy_pl, rhovz_pl};
+    const double *dXr[4] = {drho, drhovx, drhovy, drhovz}, *dXp[4] = {drho_pl, drhovx_pl, drhovy_pl, drhovz_pl};
+    double *none[4] = {0, 0, 0, 0};
+    flux_lin(D, have_sgp, 0, dt, Xr, Xp, dXr, dXp, dflx, dflx_pl, dxc, dxc_pl, NULL, NULL, NULL, NULL, none, none,
+             GMTR_t, GMTR_t_pl, GMTR_a, GMTR_a_pl, GMTR_p, GMTR_p_pl);
+}
+
+void orc_horizontal_flux_adjoint(const orc_dims *D, const int *have_sgp, const double *flx_h_b,
+                                 const double *flx_h_b_pl, const double *GRD_xc_b, const double *GRD_xc_b_pl,
+                                 double *rho_b, double *rho_b_pl, double *rhovx_b, double *rhovx_b_pl,
+                                 double *rhovy_b, double *rhovy_b_pl, double *rhovz_b, double *rhovz_b_pl,
+                                 const double *rho, const double *rho_pl, const double *rhovx, const double *rhovx_pl,
+                                 const double *rhovy, const double *rhovy_pl, const double *rhovz,
+                                 const double *rhovz_pl, double dt, const double *GMTR_t, const double *GMTR_t_pl,
+                                 const double *GMTR_a, const double *GMTR_a_pl, const double *GMTR_p,
+                                 const double *GMTR_p_pl)
+{
+    const double *Xr[4] = {rho, rhovx, rhovy, rhovz}, *Xp[4] = {rho_pl, rhovx_pl, rhovy_pl, rhovz_pl};
+    const double *none[4] = {0, 0, 0, 0};
+    double *Br[4] = {rho_b, rhovx_b, rhovy_b, rhovz_b}, *Bp[4] = {rho_b_pl, rhovx_b_pl, rhovy_b_pl, rhovz_b_pl};
+    flux_lin(D, have_sgp, 1, dt, Xr, Xp, none, none, NULL, NULL, NULL, NULL, flx_h_b, flx_h_b_pl, GRD_xc_b,
+             GRD_xc_b_pl, Br, Bp, GMTR_t, GMTR_t_pl, GMTR_a, GMTR_a_pl, GMTR_p, GMTR_p_pl);
+}
+#undef FL_TW
+#undef FL_A
+#undef FL_RA

@@ -239,6 +239,27 @@ void orc_horizontal_limiter_thuburn_adjoint(const orc_dims *D, const int *have_s
                                             const double *cmask_pl, const double *Qpost, const double *Qpost_pl,
                                             const uint8_t *mask, const uint8_t *mask_pl);
 
+/* Tangent-linear and adjoint of orc_horizontal_flux in (rho, rhovx, rhovy,
+ * rhovz), metrics and dt fixed.  TLM: dflx (6 faces), dxc (3 arcs x 3)
+ * written.  Adjoint: rho_b .. rhovz_b written from flx_h_b, GRD_xc_b. */
+void orc_horizontal_flux_tlm(const orc_dims *D, const int *have_sgp, double *dflx, double *dflx_pl, double *dxc,
+                             double *dxc_pl, const double *rho, const double *rho_pl, const double *rhovx,
+                             const double *rhovx_pl, const double *rhovy, const double *rhovy_pl,
+                             const double *rhovz, const double *rhovz_pl, const double *drho, const double *drho_pl,
+                             const double *drhovx, const double *drhovx_pl, const double *drhovy,
+                             const double *drhovy_pl, const double *drhovz, const double *drhovz_pl, double dt,
+                             const double *GMTR_t, const double *GMTR_t_pl, const double *GMTR_a,
+                             const double *GMTR_a_pl, const double *GMTR_p, const double *GMTR_p_pl);
+void orc_horizontal_flux_adjoint(const orc_dims *D, const int *have_sgp, const double *flx_h_b,
+                                 const double *flx_h_b_pl, const double *GRD_xc_b, const double *GRD_xc_b_pl,
+                                 double *rho_b, double *rho_b_pl, double *rhovx_b, double *rhovx_b_pl,
+                                 double *rhovy_b, double *rhovy_b_pl, double *rhovz_b, double *rhovz_b_pl,
+                                 const double *rho, const double *rho_pl, const double *rhovx, const double *rhovx_pl,
+                                 const double *rhovy, const double *rhovy_pl, const double *rhovz,
+                                 const double *rhovz_pl, double dt, const double *GMTR_t, const double *GMTR_t_pl,
+                                 const double *GMTR_a, const double *GMTR_a_pl, const double *GMTR_p,
+                                 const double *GMTR_p_pl);
+
 #ifdef __cplusplus
 }
 #endif

@@ -439,3 +439,58 @@ def test_gpu_hlim_mask_and_adjoint_match_oracle(gpu, oracle, c):
     for got, k in ((qb, "q_b"), (pb, "Qpost_b")):
         w = want[k]
         np.testing.assert_allclose(got.cpu().numpy(), w, rtol=0, atol=1e-13 * np.abs(w).max(), err_msg=k)
+
+
+# ---- horizontal flux ------------------------------------------------------------
+FLUX_NAMES = ["rho", "rhovx", "rhovy", "rhovz"]
+FLUX_SPECS = [(6, 3, 2, 101, [1, 0]), (10, 4, 2, 102, [0, 1]), (18, 3, 1, 103, [1])]
+
+
+def _flux_dirs(rng, f):
+    return {"d" + n + s: rng.standard_normal(f[n + s].shape) for n in FLUX_NAMES for s in ("", "_pl")}
+
+
+def _flux_cot(rng, spec):
+    return {"flx_h_b": rng.standard_normal((6,) + spec.shape3), "flx_h_b_pl": rng.standard_normal(spec.shape_pl),
+            "GRD_xc_b": rng.standard_normal((3, 3) + spec.shape3),
+            "GRD_xc_b_pl": rng.standard_normal((3,) + spec.shape_pl)}
+
+
+@pytest.mark.parametrize("c", FLUX_SPECS, ids=[f"g{c[0]}k{c[1]}" for c in FLUX_SPECS])
+def test_oracle_flux_adjoint_dot_product(oracle, c):
+    n, kall, lall, seed, sgp = c
+    spec = synth.GridSpec(n, kall, lall, have_pl=True, seed=seed)
+    case = synth.make_case(spec, have_sgp=sgp)
+    rng = np.random.default_rng(seed)
+    dx, u = _flux_dirs(rng, case["flux"]), _flux_cot(rng, spec)
+    t = oracle.horizontal_flux_lin(spec, case["geo"], case["flux"], 360.0, False, dx)
+    a = oracle.horizontal_flux_lin(spec, case["geo"], case["flux"], 360.0, True, u)
+    lhs, s1 = _dot({"f": t["dflx_h"], "f_pl": t["dflx_h_pl"], "x": t["dGRD_xc"], "x_pl": t["dGRD_xc_pl"]},
+                   {"f": u["flx_h_b"], "f_pl": u["flx_h_b_pl"], "x": u["GRD_xc_b"], "x_pl": u["GRD_xc_b_pl"]},
+                   ["f", "x"])
+    rhs, s2 = _dot({n + s: dx["d" + n + s] for n in FLUX_NAMES for s in ("", "_pl")},
+                   {n + s: a[n + "_b" + s] for n in FLUX_NAMES for s in ("", "_pl")}, FLUX_NAMES)
+    assert abs(lhs - rhs) <= 1e-13 * max(s1, s2), (lhs, rhs)
+
+
+def test_oracle_flux_tlm_matches_finite_differences(oracle):
+    """The flux is smooth wherever rho > EPS: central differences match the
+    TLM everywhere to the truncation error."""
+    spec = synth.GridSpec(12, 3, 2, have_pl=True, seed=111)
+    case = synth.make_case(spec, have_sgp=[1, 0])
+    f = case["flux"]
+    rng = np.random.default_rng(111)
+    dx = _flux_dirs(rng, f)
+    t = oracle.horizontal_flux_lin(spec, case["geo"], f, 360.0, False, dx)
+    h = 1e-6
+    pert = lambda s: {**f, **{n + q: f[n + q] + s * h * dx["d" + n + q] * np.abs(f[n + q]).max()  # noqa: E731
+                              for n in FLUX_NAMES for q in ("", "_pl")}}
+    scale = {n + q: np.abs(f[n + q]).max() for n in FLUX_NAMES for q in ("", "_pl")}
+    tl = oracle.horizontal_flux_lin(spec, case["geo"], f, 360.0, False,
+                                    {k: v * scale[k[1:]] for k, v in dx.items()})
+    op = oracle.horizontal_flux(spec, case["geo"], pert(1), 360.0)
+    om = oracle.horizontal_flux(spec, case["geo"], pert(-1), 360.0)
+    for k, tk in (("flx_h", "dflx_h"), ("GRD_xc", "dGRD_xc"), ("flx_h_pl", "dflx_h_pl"), ("GRD_xc_pl", "dGRD_xc_pl")):
+        fd = (op[k] - om[k]) / (2 * h)
+        np.testing.assert_allclose(fd, tl[tk], rtol=0, atol=1e-6 * max(np.abs(tl[tk]).max(), 1e-30), err_msg=k)
+    assert np.abs(t["dflx_h"]).max() > 0
