@@ -373,6 +373,14 @@ def time_adjoint(ctx, spec, case, stream, steps, fwd_ms):
         vm, vm_pl))
     del V, qhb, qhb_pl, vqb, vqb_pl
 
+    F = up(case["flux"], drivers.FLUX_IN)
+    fb = [rnd((6,) + spec.shape3), rnd(spec.shape_pl), rnd((3, 3) + spec.shape3), rnd((3,) + spec.shape_pl)]
+    xb = [torch.empty(spec.shape3 if i % 2 == 0 else spec.shape_pl, dtype=torch.float64, device=dev)
+          for i in range(8)]
+    out["horizontal_flux"] = timed(lambda: ctx.horizontal_flux_adjoint(*fb, *xb, *[F[k] for k in drivers.FLUX_IN],
+                                                                       DT_LARGE))
+    del F, fb, xb
+
     wb = {s: rnd(spec.shape3 if s == "" else spec.shape_pl) for s in ("", "_pl")}
     bufs = [torch.empty(spec.shape3 if i % 2 == 0 else spec.shape_pl, dtype=torch.float64, device=dev)
             for i in range(12)]

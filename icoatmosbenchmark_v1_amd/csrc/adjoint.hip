@@ -613,7 +613,207 @@ __global__ void fill_zero_kernel(double *p, size_t n)
         p[t] = 0.0;
 }
 
+// ---- horizontal_flux (dyn_horiz_adv_flux/src/mod_src_tracer.f90:42-379) -------
+//
+// Linear in rho*v; GRD_xc carries 1/max(rho_arc, EPS).  The arc cotangents
+// are gathered, not scattered: a work-group of FCHUNK cells of one slab
+// computes the cotangent of every arc whose triangles touch its cells
+// (origins [g0-iall-1, g0+FCHUNK+iall)) into LDS, then each work-item sums
+// its cell's six triangles in the order of a sequential sweep over triangle
+// origins, so the result is the oracle's, bit for bit, with no atomics.  The
+// tracer fields are read from HBM directly (neighbouring arcs hit the
+// caches), which keeps LDS to the arc cotangents: 2 work-groups per CU at
+// gl7.  (Staging them too, with 256-cell chunks, ran at 3.05 ms at gl7.)
+constexpr int FCHUNK = 512;
+
+struct FaArgs {
+    const double *flxb, *xcb;   // (gall,kall,lall,6), (gall,kall,lall,3,3)
+    double *Xb[4];              // rho_b, rhovx_b, rhovy_b, rhovz_b
+    const double *X[4];
+    const double *gt, *ga, *gp;
+    const int *have_sgp;
+    int iall, gall, kall, lall;
+    double dt, eps;
+    // poles
+    const double *flxb_pl, *xcb_pl;
+    double *Xb_pl[4];
+    const double *X_pl[4];
+    const double *gt_pl, *ga_pl, *gp_pl;
+    int gall_pl, have_pl;
+};
+
+__global__ void __launch_bounds__(FCHUNK) hflux_adjoint_kernel(FaArgs A)
+{
+    extern __shared__ double lds[];
+    const int iall = A.iall, gall = A.gall, kall = A.kall, lall = A.lall;
+    const int nchunk = (gall + FCHUNK - 1) / FCHUNK;
+    const int b = blockIdx.x;
+    const int chunk = b % nchunk, kl = b / nchunk;
+    const int k = kl % kall, l = kl / kall;
+    const int g0 = chunk * FCHUNK;
+    const int alo = g0 - iall - 1, na = FCHUNK + 2 * iall + 1;
+    double *s_a = lds;  // [3 arcs][4 fields][na]
+    const size_t fb = ((size_t)l * kall + k) * gall;
+    const size_t plane = (size_t)lall * kall * gall;
+    const size_t mstr = (size_t)lall * gall, mb = (size_t)l * gall;
+    const bool sgp = A.have_sgp[l] != 0;
+    const int tid = threadIdx.x;
+    auto W = [&](int c, int t, int w) { return A.gt[(size_t)(w * 2 + t) * mstr + mb + c]; };  // w: 2,3,4 = W1..W3
+    auto X = [&](int f, int p) { return A.X[f][fb + p]; };
+    auto tri = [&](int f, int t, int c) {  // forward triangle sums (FLX:139-201)
+        if (t == 0 && sgp && c == 0) t = 1, c = 1;
+        if (t == 0) return X(f, c) * W(c, 0, 2) + X(f, c + 1) * W(c, 0, 3) + X(f, c + iall + 1) * W(c, 0, 4);
+        return X(f, c) * W(c, 1, 2) + X(f, c + iall + 1) * W(c, 1, 3) + X(f, c + iall) * W(c, 1, 4);
+    };
+    const double dt = A.dt, EPS = A.eps;
+    for (int s = tid; s < na; s += FCHUNK) {
+        const int a0 = alo + s;
+        const int i = a0 >= 0 ? a0 % iall : -1, j = a0 >= 0 ? a0 / iall : -1;
+#pragma unroll
+        for (int a = 1; a <= 3; ++a) {
+            const bool valid = a0 >= 0 && a0 < gall && j >= (a == 1 ? 1 : 0) && j <= iall - 2 &&
+                               i >= (a == 3 ? 1 : 0) && i <= iall - 2;
+            double bv[4] = {0.0, 0.0, 0.0, 0.0};
+            if (valid) {
+                const int g = a0;
+                const int nb = g + (a == 1 ? 1 : (a == 2 ? iall + 1 : iall));
+                const int t1k = a == 1 ? 1 : (a == 2 ? 0 : 1), t1c = a == 1 ? g - iall : g;
+                const int t2k = a == 1 ? 0 : (a == 2 ? 1 : 0), t2c = a == 3 ? g - 1 : g;
+                double s2[4];
+#pragma unroll
+                for (int f = 0; f < 4; ++f) s2[f] = tri(f, t1k, t1c) + tri(f, t2k, t2c);
+                const double rr = 1.0 / dmx(s2[0], EPS);
+                const bool live_rho = !(EPS > s2[0]);
+                const double ra = A.gp[1 * mstr + mb + g] * dt, rn = A.gp[1 * mstr + mb + nb] * dt;
+                const bool sgp_face = sgp && a == 3 && nb == iall + 1;
+                const double fb_nb = sgp_face ? 0.0 : A.flxb[(size_t)(a + 2) * plane + fb + nb];
+                const double fluxb = A.flxb[(size_t)(a - 1) * plane + fb + g] * ra - fb_nb * rn;
+                double r2b = 0.0;
+#pragma unroll
+                for (int dd = 0; dd < 3; ++dd) {
+                    const double hn = A.ga[(size_t)(dd * 3 + (a - 1)) * mstr + mb + g];
+                    const double xb = A.xcb[(size_t)(dd * 3 + (a - 1)) * plane + fb + g];
+                    bv[1 + dd] = 0.5 * hn * fluxb - xb * rr * dt * 0.5;
+                    r2b = r2b + xb * s2[1 + dd] * dt * 0.5 * rr * rr;
+                }
+                bv[0] = live_rho ? r2b : 0.0;
+            }
+#pragma unroll
+            for (int f = 0; f < 4; ++f) s_a[((a - 1) * 4 + f) * na + s] = bv[f];
+        }
+    }
+    __syncthreads();
+    const int X0 = g0 + tid;
+    if (X0 >= gall) return;
+    auto AB = [&](int a, int f, int c) { return s_a[((a - 1) * 4 + f) * na + c - alo]; };
+    auto own = [&](int c) { return c >= 0 && c % iall <= iall - 2 && c / iall <= iall - 2; };
+    // triangle cotangents: TI(c) from AI(c), AIJ(c), AJ(c+1); TJ(c) from AI(c+iall), AIJ(c), AJ(c)
+    auto tib0 = [&](int f, int c) { return ((0.0 + AB(1, f, c)) + AB(2, f, c)) + AB(3, f, c + 1); };
+    auto tjb0 = [&](int f, int c) { return ((0.0 + AB(1, f, c + iall)) + AB(2, f, c)) + AB(3, f, c); };
+    auto tib = [&](int f, int c) { return (sgp && c == 0) ? 0.0 : tib0(f, c); };
+    auto tjb = [&](int f, int c) { return (sgp && c == 1) ? tjb0(f, 1) + tib0(f, 0) : tjb0(f, c); };
+#pragma unroll
+    for (int f = 0; f < 4; ++f) {
+        double y = 0.0;
+        int c = X0 - iall - 1;
+        if (own(c)) y = y + (tib(f, c) * W(c, 0, 4) + tjb(f, c) * W(c, 1, 3));
+        c = X0 - iall;
+        if (own(c)) y = y + tjb(f, c) * W(c, 1, 4);
+        c = X0 - 1;
+        if (own(c)) y = y + tib(f, c) * W(c, 0, 3);
+        c = X0;
+        if (own(c)) y = y + (tib(f, c) * W(c, 0, 2) + tjb(f, c) * W(c, 1, 2));
+        A.Xb[f][fb + X0] = y;
+    }
+}
+
+// Poles: one work-item per (k, l), in the oracle's order (FLX:326-374).
+__global__ void hflux_adjoint_pole(FaArgs A)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int kall = A.kall, npl = A.gall_pl;
+    if (!A.have_pl || t >= kall * kLallPl || npl > 16) return;
+    const int k = t % kall, l = t / kall;
+    const size_t fb = ((size_t)l * kall + k) * npl;
+    const size_t pplane = (size_t)kLallPl * kall * npl, mstr = (size_t)kLallPl * npl, mb = (size_t)l * npl;
+    const double dt = A.dt, EPS = A.eps;
+    auto PT = [&](int v, int w) { return A.gt_pl[(size_t)w * mstr + mb + v]; };  // w: 2,3,4 = W1..W3
+    const double ran = A.gp_pl[1 * mstr + mb + 0] * dt;
+    double Tv[4][16], Tb[4][16];
+    for (int v = 1; v < npl; ++v) {
+        const int vp1 = (v + 1 == npl) ? 1 : v + 1;
+        for (int f = 0; f < 4; ++f) {
+            const double *x = A.X_pl[f] + fb;
+            Tv[f][v] = x[0] * PT(v, 2) + x[v] * PT(v, 3) + x[vp1] * PT(v, 4);
+            Tb[f][v] = 0.0;
+        }
+    }
+    for (int v = 1; v < npl; ++v) {
+        const int vm1 = (v == 1) ? npl - 1 : v - 1;
+        double s2[4];
+        for (int f = 0; f < 4; ++f) s2[f] = Tv[f][vm1] + Tv[f][v];
+        const double rr = 1.0 / dmx(s2[0], EPS);
+        const bool live_rho = !(EPS > s2[0]);
+        const double fluxb = A.flxb_pl[fb + v] * ran;
+        double bv[4], r2b = 0.0;
+        for (int dd = 0; dd < 3; ++dd) {
+            const double hn = A.ga_pl[(size_t)dd * mstr + mb + v];
+            const double xb = A.xcb_pl[(size_t)dd * pplane + fb + v];
+            bv[1 + dd] = 0.5 * hn * fluxb - xb * rr * dt * 0.5;
+            r2b = r2b + xb * s2[1 + dd] * dt * 0.5 * rr * rr;
+        }
+        bv[0] = live_rho ? r2b : 0.0;
+        for (int f = 0; f < 4; ++f) {
+            Tb[f][vm1] += bv[f];
+            Tb[f][v] += bv[f];
+        }
+    }
+    for (int f = 0; f < 4; ++f) {
+        double *y = A.Xb_pl[f] + fb;
+        for (int v = 0; v < npl; ++v) y[v] = 0.0;
+        for (int v = 1; v < npl; ++v) {
+            const int vp1 = (v + 1 == npl) ? 1 : v + 1;
+            y[0] += Tb[f][v] * PT(v, 2);
+            y[v] += Tb[f][v] * PT(v, 3);
+            y[vp1] += Tb[f][v] * PT(v, 4);
+        }
+    }
+}
+
 }  // namespace
+
+int launch_hflux_adjoint(nicam_ctx c, const double *flxb, const double *flxb_pl, const double *xcb,
+                         const double *xcb_pl, double *const Xb[4], double *const Xb_pl[4], const double *const X[4],
+                         const double *const X_pl[4], double dt)
+{
+    const Dims &D = c->d;
+    if (D.gall_1d > 160) return fail(NICAM_ERR_ARG, "gall_1d larger than the LDS tiles (160)");
+    if (D.have_pl && D.gall_pl > 16) return fail(NICAM_ERR_ARG, "gall_pl larger than 16");
+    FaArgs A;
+    A.flxb = flxb; A.xcb = xcb; A.flxb_pl = flxb_pl; A.xcb_pl = xcb_pl;
+    for (int f = 0; f < 4; ++f) {
+        A.Xb[f] = Xb[f]; A.X[f] = X[f]; A.Xb_pl[f] = Xb_pl[f]; A.X_pl[f] = X_pl[f];
+    }
+    A.gt = c->GMTR_t; A.ga = c->GMTR_a; A.gp = c->GMTR_p;
+    A.gt_pl = c->GMTR_t_pl; A.ga_pl = c->GMTR_a_pl; A.gp_pl = c->GMTR_p_pl;
+    A.have_sgp = c->have_sgp;
+    A.iall = D.gall_1d; A.gall = D.gall; A.kall = D.kall; A.lall = D.lall;
+    A.dt = dt; A.eps = c->eps;
+    A.gall_pl = D.gall_pl; A.have_pl = D.have_pl;
+    const long nblk = (long)((D.gall + FCHUNK - 1) / FCHUNK) * D.kall * D.lall;
+    if (nblk > 0x7fffffffL) return fail(NICAM_ERR_ARG, "grid too large for one launch");
+    const int iall = D.gall_1d;
+    const size_t lds = sizeof(double) * 12 * (size_t)(FCHUNK + 2 * iall + 1);
+    if (lds > 64 * 1024)
+        NICAM_HIP(hipFuncSetAttribute((const void *)hflux_adjoint_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                      (int)lds));
+    hipLaunchKernelGGL(hflux_adjoint_kernel, dim3((unsigned)nblk), dim3(FCHUNK), lds, c->stream, A);
+    if (D.have_pl)
+        hipLaunchKernelGGL(hflux_adjoint_pole, dim3((unsigned)((D.kall * kLallPl + 63) / 64)), dim3(64), 0, c->stream,
+                           A);
+    NICAM_HIP(hipGetLastError());
+    return NICAM_OK;
+}
 
 int launch_hlim_adjoint(nicam_ctx c, double *qab, double *qab_pl, double *qb, double *qb_pl, const double *Qprevb,
                         const double *Qprevb_pl, double *Qpostb, double *Qpostb_pl, const double *q_a,

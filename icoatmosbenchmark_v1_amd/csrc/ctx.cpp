@@ -786,6 +786,50 @@ int nicam_vi_rhow_solver_adjoint(nicam_ctx c, double *wb, double *wb_pl, double 
     return stage_out(c, a, v);
 }
 
+int nicam_horizontal_flux_adjoint(nicam_ctx c, const double *flx_h_b, const double *flx_h_b_pl,
+                                  const double *GRD_xc_b, const double *GRD_xc_b_pl, double *rho_b, double *rho_b_pl,
+                                  double *rhovx_b, double *rhovx_b_pl, double *rhovy_b, double *rhovy_b_pl,
+                                  double *rhovz_b, double *rhovz_b_pl, const double *rho, const double *rho_pl,
+                                  const double *rhovx, const double *rhovx_pl, const double *rhovy,
+                                  const double *rhovy_pl, const double *rhovz, const double *rhovz_pl, double dt)
+{
+    NICAM_TRY(check_ctx(c));
+    if (!c->have_hmetrics) return fail(NICAM_ERR_STATE, "horizontal metrics not set (nicam_set_hmetrics)");
+    const void *req[] = {flx_h_b, GRD_xc_b, rho_b, rhovx_b, rhovy_b, rhovz_b, rho, rhovx, rhovy, rhovz};
+    const char *nm[] = {"flx_h_b", "GRD_xc_b", "rho_b", "rhovx_b", "rhovy_b", "rhovz_b", "rho", "rhovx", "rhovy",
+                        "rhovz"};
+    for (int i = 0; i < 10; ++i) NICAM_TRY(need(req[i], nm[i]));
+    if (c->d.have_pl) {
+        const void *rp[] = {flx_h_b_pl, GRD_xc_b_pl, rho_b_pl, rhovx_b_pl, rhovy_b_pl, rhovz_b_pl, rho_pl, rhovx_pl,
+                            rhovy_pl, rhovz_pl};
+        for (int i = 0; i < 10; ++i) NICAM_TRY(need(rp[i], "pole array"));
+    }
+    if (c->ptr_mode == NICAM_PTR_DEVICE) {
+        double *const Xb[4] = {rho_b, rhovx_b, rhovy_b, rhovz_b};
+        double *const Xb_pl[4] = {PL(rho_b_pl), PL(rhovx_b_pl), PL(rhovy_b_pl), PL(rhovz_b_pl)};
+        const double *const X[4] = {rho, rhovx, rhovy, rhovz};
+        const double *const X_pl[4] = {PL(rho_pl), PL(rhovx_pl), PL(rhovy_pl), PL(rhovz_pl)};
+        return launch_hflux_adjoint(c, flx_h_b, PL(flx_h_b_pl), GRD_xc_b, PL(GRD_xc_b_pl), Xb, Xb_pl, X, X_pl, dt);
+    }
+    const size_t R = c->d.nreg() * 8, P = c->d.npl() * 8;
+    std::vector<HostArg> a = {{flx_h_b, 6 * R, false},  {PL(flx_h_b_pl), P, false},  {GRD_xc_b, 9 * R, false},
+                              {PL(GRD_xc_b_pl), 3 * P, false}, {rho_b, R, true},      {PL(rho_b_pl), P, true},
+                              {rhovx_b, R, true},       {PL(rhovx_b_pl), P, true},   {rhovy_b, R, true},
+                              {PL(rhovy_b_pl), P, true}, {rhovz_b, R, true},         {PL(rhovz_b_pl), P, true},
+                              {rho, R, false},          {PL(rho_pl), P, false},      {rhovx, R, false},
+                              {PL(rhovx_pl), P, false}, {rhovy, R, false},           {PL(rhovy_pl), P, false},
+                              {rhovz, R, false},        {PL(rhovz_pl), P, false}};
+    std::vector<void *> v;
+    NICAM_TRY(stage_in(c, a, v));
+    auto D = [&](int i) { return static_cast<double *>(v[i]); };
+    double *const Xb[4] = {D(4), D(6), D(8), D(10)};
+    double *const Xb_pl[4] = {D(5), D(7), D(9), D(11)};
+    const double *const X[4] = {D(12), D(14), D(16), D(18)};
+    const double *const X_pl[4] = {D(13), D(15), D(17), D(19)};
+    NICAM_TRY(launch_hflux_adjoint(c, D(0), D(1), D(2), D(3), Xb, Xb_pl, X, X_pl, dt));
+    return stage_out(c, a, v);
+}
+
 int nicam_horizontal_limiter_thuburn_mask(nicam_ctx c, double *q_a, double *q_a_pl, const double *q,
                                           const double *q_pl, const double *d, const double *d_pl, const double *ch,
                                           const double *ch_pl, const double *cmask, const double *cmask_pl,

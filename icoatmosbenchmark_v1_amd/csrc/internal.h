@@ -241,6 +241,9 @@ int launch_divdamp(nicam_ctx c, double *const out[3], double *const out_pl[3], c
                    const double *rgsqrth, const double *rgsqrth_pl, const double *rgam, const double *rgam_pl,
                    const double *rgamh, const double *rgamh_pl, const double *c2w, const double *c2w_pl,
                    const double *rdgz_dev);
+int launch_hflux_adjoint(nicam_ctx c, const double *flxb, const double *flxb_pl, const double *xcb,
+                         const double *xcb_pl, double *const Xb[4], double *const Xb_pl[4], const double *const X[4],
+                         const double *const X_pl[4], double dt);
 int launch_hlim_mask(nicam_ctx c, double *q_a, double *q_a_pl, const double *q, const double *q_pl,
                      const double *d, const double *d_pl, const double *ch, const double *ch_pl,
                      const double *cmask, const double *cmask_pl, double *Qprev, double *Qprev_pl,

@@ -190,6 +190,17 @@ module nicam_tt_f
        integer(c_int) :: nicam_vi_rhow_solver_adjoint
      end function nicam_vi_rhow_solver_adjoint
 
+     function nicam_horizontal_flux_adjoint(ctx, flx_h_b, flx_h_b_pl, GRD_xc_b, GRD_xc_b_pl, rho_b, rho_b_pl, &
+          rhovx_b, rhovx_b_pl, rhovy_b, rhovy_b_pl, rhovz_b, rhovz_b_pl, rho, rho_pl, rhovx, rhovx_pl, rhovy, &
+          rhovy_pl, rhovz, rhovz_pl, dt) bind(C, name='nicam_horizontal_flux_adjoint')
+       import :: c_ptr, c_int, c_double
+       type(c_ptr), value :: ctx, flx_h_b, flx_h_b_pl, GRD_xc_b, GRD_xc_b_pl, rho_b, rho_b_pl, rhovx_b, rhovx_b_pl
+       type(c_ptr), value :: rhovy_b, rhovy_b_pl, rhovz_b, rhovz_b_pl, rho, rho_pl, rhovx, rhovx_pl, rhovy
+       type(c_ptr), value :: rhovy_pl, rhovz, rhovz_pl
+       real(c_double), value :: dt
+       integer(c_int) :: nicam_horizontal_flux_adjoint
+     end function nicam_horizontal_flux_adjoint
+
      function nicam_horizontal_limiter_thuburn_mask(ctx, q_a, q_a_pl, q, q_pl, d, d_pl, ch, ch_pl, cmask, &
           cmask_pl, Qout_prev, Qout_prev_pl, Qout_post, Qout_post_pl, mask, mask_pl) &
           bind(C, name='nicam_horizontal_limiter_thuburn_mask')

@@ -90,7 +90,12 @@ def adjoint_bytes(spec: GridSpec, word: int = 8) -> Dict[str, int]:
     # RGAMH (kall); writes the seven cotangents
     vicol = (2 * (kall - 1) + 3 * (kall - 3) + 2 + 3 * kall + 6 * kall + (kall - 1)) * word
     vi = vicol * (ncol + pl * npl)
-    return dict(horizontal_limiter_thuburn=hlim, vertical_limiter_thuburn=vlim, vi_rhow_solver=vi)
+    # flux: reads rho, rhovx/y/z on every cell, flx_h_b (6) and GRD_xc_b (9) on the arcs and faces; writes the
+    # four cotangents; metrics GMTR_t (6), GMTR_a (9), GMTR_p (1) once per column
+    flux = ((4 + 4) * gall + (6 + 9) * arcs) * kall * lall * word + 16 * gall * lall * word
+    flux += pl * kall * (npl * (4 + 4) + ring * (1 + 3)) * word
+    return dict(horizontal_flux=flux, horizontal_limiter_thuburn=hlim, vertical_limiter_thuburn=vlim,
+                vi_rhow_solver=vi)
 
 
 def kernel_flops(spec: GridSpec) -> Dict[str, int]:

@@ -110,6 +110,7 @@ SIGNATURES = {
     "nicam_vi_rhow_solver_adjoint": (c_int, [_P] + [_P] * 14 + [c_double]),
     "nicam_vertical_limiter_thuburn_mask": (c_int, [_P] * 11),
     "nicam_horizontal_limiter_thuburn_mask": (c_int, [_P] * 17),
+    "nicam_horizontal_flux_adjoint": (c_int, [_P] * 21 + [c_double]),
     "nicam_horizontal_limiter_thuburn_adjoint": (c_int, [_P] * 23),
     "nicam_vertical_limiter_thuburn_adjoint": (c_int, [_P] * 15),
 }
@@ -583,6 +584,20 @@ class Context:
                          (coef_intp_pl, "coef_intp_pl", 18 * self.gall_pl), (coef_diff, "coef_diff", 18 * G),
                          (coef_diff_pl, "coef_diff_pl", 30)])
         _check(self.lib.nicam_oprt3d_divdamp(self.h, *a), "nicam_oprt3d_divdamp")
+
+    def horizontal_flux_adjoint(self, flx_h_b, flx_h_b_pl, GRD_xc_b, GRD_xc_b_pl, rho_b, rho_b_pl, rhovx_b,
+                                rhovx_b_pl, rhovy_b, rhovy_b_pl, rhovz_b, rhovz_b_pl, rho, rho_pl, rhovx, rhovx_pl,
+                                rhovy, rhovy_pl, rhovz, rhovz_pl, dt: float) -> None:
+        """nicam_horizontal_flux_adjoint: cotangents of rho and rho*v from those of flx_h and GRD_xc."""
+        R, P = self.nreg, self.npl
+        a = self._args([(flx_h_b, "flx_h_b", 6 * R), (flx_h_b_pl, "flx_h_b_pl", P), (GRD_xc_b, "GRD_xc_b", 9 * R),
+                        (GRD_xc_b_pl, "GRD_xc_b_pl", 3 * P), (rho_b, "rho_b", R), (rho_b_pl, "rho_b_pl", P),
+                        (rhovx_b, "rhovx_b", R), (rhovx_b_pl, "rhovx_b_pl", P), (rhovy_b, "rhovy_b", R),
+                        (rhovy_b_pl, "rhovy_b_pl", P), (rhovz_b, "rhovz_b", R), (rhovz_b_pl, "rhovz_b_pl", P),
+                        (rho, "rho", R), (rho_pl, "rho_pl", P), (rhovx, "rhovx", R), (rhovx_pl, "rhovx_pl", P),
+                        (rhovy, "rhovy", R), (rhovy_pl, "rhovy_pl", P), (rhovz, "rhovz", R),
+                        (rhovz_pl, "rhovz_pl", P)])
+        _check(self.lib.nicam_horizontal_flux_adjoint(self.h, *a, float(dt)), "nicam_horizontal_flux_adjoint")
 
     def horizontal_limiter_thuburn_mask(self, q_a, q_a_pl, q, q_pl, d, d_pl, ch, ch_pl, cmask, cmask_pl,
                                         Qout_prev, Qout_prev_pl, Qout_post, Qout_post_pl, mask, mask_pl) -> None:

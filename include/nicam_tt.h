@@ -517,6 +517,28 @@ int nicam_vi_rhow_solver_adjoint(nicam_ctx ctx,
                                  double *Spre_b, double *Spre_b_pl,
                                  double dt);
 
+/* Adjoint of horizontal_flux in (rho, rhovx, rhovy, rhovz) at the context's
+ * metrics and the given dt (dyn_horiz_adv_flux/src/mod_src_tracer.f90:42-379):
+ *   in  flx_h_b (gall,kall,lall,6), GRD_xc_b (gall,kall,lall,3,3) and _pl:
+ *       cotangents of the outputs (entries the forward leaves at zero are
+ *       not read)
+ *   out rho_b, rhovx_b, rhovy_b, rhovz_b and _pl (every point written)
+ *   in  rho, rhovx, rhovy, rhovz and _pl: the forward's inputs
+ * The derivative of GRD_xc in rho is taken where max(rho_arc, EPS) selects
+ * rho_arc. */
+int nicam_horizontal_flux_adjoint(nicam_ctx ctx,
+                                  const double *flx_h_b, const double *flx_h_b_pl,
+                                  const double *GRD_xc_b, const double *GRD_xc_b_pl,
+                                  double *rho_b, double *rho_b_pl,
+                                  double *rhovx_b, double *rhovx_b_pl,
+                                  double *rhovy_b, double *rhovy_b_pl,
+                                  double *rhovz_b, double *rhovz_b_pl,
+                                  const double *rho, const double *rho_pl,
+                                  const double *rhovx, const double *rhovx_pl,
+                                  const double *rhovy, const double *rhovy_pl,
+                                  const double *rhovz, const double *rhovz_pl,
+                                  double dt);
+
 /* horizontal_limiter_thuburn that also records its branch mask: the same
  * arguments and results as nicam_horizontal_limiter_thuburn, plus one uint8
  * per cell-level (gall,kall,lall): bit m-1 set where face m (1-3) of a cell
