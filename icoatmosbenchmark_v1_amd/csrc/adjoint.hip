@@ -669,19 +669,27 @@ __global__ void __launch_bounds__(FCHUNK) hflux_adjoint_kernel(FaArgs A)
     for (int s = tid; s < na; s += FCHUNK) {
         const int a0 = alo + s;
         const int i = a0 >= 0 ? a0 % iall : -1, j = a0 >= 0 ? a0 / iall : -1;
+        const bool org = a0 >= 0 && a0 < gall && i <= iall - 2 && j <= iall - 2;
+        // the four triangles the origin's arcs use: TI(a0), TJ(a0), TJ(a0-iall) (AI), TI(a0-1) (AJ)
+        double tI[4], tJ[4], tJs[4], tIw[4];
+#pragma unroll
+        for (int f = 0; f < 4; ++f) {
+            tI[f] = org ? tri(f, 0, a0) : 0.0;
+            tJ[f] = org ? tri(f, 1, a0) : 0.0;
+            tJs[f] = (org && j >= 1) ? tri(f, 1, a0 - iall) : 0.0;
+            tIw[f] = (org && i >= 1) ? tri(f, 0, a0 - 1) : 0.0;
+        }
 #pragma unroll
         for (int a = 1; a <= 3; ++a) {
-            const bool valid = a0 >= 0 && a0 < gall && j >= (a == 1 ? 1 : 0) && j <= iall - 2 &&
-                               i >= (a == 3 ? 1 : 0) && i <= iall - 2;
+            const bool valid = org && j >= (a == 1 ? 1 : 0) && i >= (a == 3 ? 1 : 0);
             double bv[4] = {0.0, 0.0, 0.0, 0.0};
             if (valid) {
                 const int g = a0;
                 const int nb = g + (a == 1 ? 1 : (a == 2 ? iall + 1 : iall));
-                const int t1k = a == 1 ? 1 : (a == 2 ? 0 : 1), t1c = a == 1 ? g - iall : g;
-                const int t2k = a == 1 ? 0 : (a == 2 ? 1 : 0), t2c = a == 3 ? g - 1 : g;
                 double s2[4];
 #pragma unroll
-                for (int f = 0; f < 4; ++f) s2[f] = tri(f, t1k, t1c) + tri(f, t2k, t2c);
+                for (int f = 0; f < 4; ++f)  // FLX:234-309: the arc's two triangles, in the forward's order
+                    s2[f] = a == 1 ? tJs[f] + tI[f] : (a == 2 ? tI[f] + tJ[f] : tJ[f] + tIw[f]);
                 const double rr = 1.0 / dmx(s2[0], EPS);
                 const bool live_rho = !(EPS > s2[0]);
                 const double ra = A.gp[1 * mstr + mb + g] * dt, rn = A.gp[1 * mstr + mb + nb] * dt;
