@@ -28,6 +28,8 @@ def main() -> int:
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--extras", action="store_true",
                     help="profile OPRT_diffusion, OPRT3D_divdamp and the step glue instead of the step")
+    ap.add_argument("--adjoint", action="store_true",
+                    help="profile the masked forward limiters and the four adjoint kernels instead of the step")
     args = ap.parse_args()
 
     import numpy as np
@@ -66,6 +68,8 @@ def main() -> int:
     torch.cuda.synchronize()
     if args.extras:
         return extras(ctx, spec, case, coef, args.steps, up, z)
+    if args.adjoint:
+        return adjoint(ctx, spec, args.steps, F, H, V, W, z)
     for s in range(args.steps):
         ctx.horizontal_flux(F["flx_h"], F["flx_h_pl"], F["GRD_xc"], F["GRD_xc_pl"], *[F[k] for k in drivers.FLUX_IN],
                             360.0)
@@ -79,6 +83,47 @@ def main() -> int:
         ctx.vi_rhow_solver(W["rhogw"], W["rhogw_pl"], *args_vi, 60.0)
         torch.cuda.synchronize()
         print(f"[profile_step] step {s + 1}/{args.steps} done", flush=True)
+    ctx.close()
+    return 0
+
+
+def adjoint(ctx, spec, steps, F, H, V, W, z) -> int:
+    """One reverse pass per step: the masked limiters, then the adjoints of
+    the flux, both limiters and the solve, each on random cotangents."""
+    import torch
+
+    from icoatmosbenchmark_v1_amd import drivers
+
+    dev = F["rho"].device
+    R, P = ctx.nreg, ctx.npl
+    hm, hm_pl = torch.zeros(R, dtype=torch.uint8, device=dev), torch.zeros(P, dtype=torch.uint8, device=dev)
+    M, MP = ctx.vlim_mask_words()
+    vm = torch.zeros(M, dtype=torch.int32, device=dev)
+    vm_pl = torch.zeros(max(MP, 1), dtype=torch.int32, device=dev)[:MP]
+    r = lambda shape: torch.randn(shape, dtype=torch.float64, device=dev)  # noqa: E731
+    fb = [r((6,) + spec.shape3), r(spec.shape_pl), r((3, 3) + spec.shape3), r((3,) + spec.shape_pl)]
+    xb = [z(spec.shape3 if i % 2 == 0 else spec.shape_pl) for i in range(8)]
+    qab, qab_pl, qpb, qpb_pl = r((6,) + spec.shape3), r(spec.shape_pl), r((2,) + spec.shape3), r((2,) + spec.shape_pl)
+    qb, qb_pl, pob, pob_pl = z(spec.shape3), z(spec.shape_pl), z((2,) + spec.shape3), z((2,) + spec.shape_pl)
+    qhb, qhb_pl, vqb, vqb_pl = r(spec.shape3), r(spec.shape_pl), z(spec.shape3), z(spec.shape_pl)
+    wb, wb_pl = r(spec.shape3), r(spec.shape_pl)
+    bufs = [z(spec.shape3 if i % 2 == 0 else spec.shape_pl) for i in range(12)]
+    for s in range(steps):
+        ctx.horizontal_limiter_thuburn_mask(H["q_a"], H["q_a_pl"], H["q"], H["q_pl"], H["d"], H["d_pl"], H["ch"],
+                                            H["ch_pl"], H["cmask"], H["cmask_pl"], H["Qout_prev"], H["Qout_prev_pl"],
+                                            H["Qout_post"], H["Qout_post_pl"], hm, hm_pl)
+        ctx.vertical_limiter_thuburn_mask(V["q_h"], V["q_h_pl"], V["q"], V["q_pl"], V["d"], V["d_pl"], V["ck"],
+                                          V["ck_pl"], vm, vm_pl)
+        ctx.horizontal_flux_adjoint(*fb, *xb, *[F[k] for k in drivers.FLUX_IN], 360.0)
+        ctx.horizontal_limiter_thuburn_adjoint(qab, qab_pl, qb, qb_pl, qpb, qpb_pl, pob, pob_pl, H["q_a"],
+                                               H["q_a_pl"], H["q"], H["q_pl"], H["d"], H["d_pl"], H["ch"],
+                                               H["ch_pl"], H["cmask"], H["cmask_pl"], H["Qout_post"],
+                                               H["Qout_post_pl"], hm, hm_pl)
+        ctx.vertical_limiter_thuburn_adjoint(qhb, qhb_pl, vqb, vqb_pl, V["q_h"], V["q_h_pl"], V["q"], V["q_pl"],
+                                             V["d"], V["d_pl"], V["ck"], V["ck_pl"], vm, vm_pl)
+        ctx.vi_rhow_solver_adjoint(wb, wb_pl, *bufs, 60.0)
+        torch.cuda.synchronize()
+        print(f"[profile_step] adjoint {s + 1}/{steps} done", flush=True)
     ctx.close()
     return 0
 
