@@ -266,7 +266,19 @@ def time_mixed(ctx, spec, case, stream, steps, k_vi, fp64):
     H["q_a"].copy_(qa0)
     V["q_h"].copy_(qh0)
 
-    calls = (k_flux, k_hlim, k_vlim, k_vi)
+    # the solve with FP32 state and sources, FP64 rho*w and elimination
+    vkeys = list(drivers.VI_IN)
+    W32 = {k + s: torch.from_numpy(np.ascontiguousarray(case["vi"][k + s], dtype=np.float32)).to(dev)
+           for k in vkeys for s in ("", "_pl")}
+    w64 = {s: torch.from_numpy(np.ascontiguousarray(case["vi"]["rhogw" + s])).to(dev) for s in ("", "_pl")}
+
+    def k_vi_mixed():
+        a = []
+        for k in vkeys:
+            a += [W32[k], W32[k + "_pl"]]
+        ctx.vi_rhow_solver_mixed(w64[""], w64["_pl"], *a, DT_SMALL)
+
+    calls = (k_flux, k_hlim, k_vlim, k_vi_mixed)
     for _ in range(2):
         for f in calls:
             f()
@@ -283,7 +295,7 @@ def time_mixed(ctx, spec, case, stream, steps, k_vi, fp64):
     kms = {name: float(np.mean([ev[st][i].elapsed_time(ev[st][i + 1]) for st in range(steps)]))
            for i, name in enumerate(KERNELS)}
     b32, b64 = perfmodel.kernel_bytes(spec, word=4), perfmodel.kernel_bytes(spec)
-    kb = dict(b32, vi_rhow_solver=b64["vi_rhow_solver"])
+    kb = dict(b32, vi_rhow_solver=perfmodel.vi_mixed_bytes(spec))
     kernels = {}
     for name in KERNELS:
         gbs = kb[name] / (kms[name] * 1e-3) / 1e9
@@ -291,7 +303,7 @@ def time_mixed(ctx, spec, case, stream, steps, k_vi, fp64):
                              frac=round(gbs / perfmodel.HBM_PEAK_GBS, 4))
     step_gbs = sum(kb.values()) / (ms_step * 1e-3) / 1e9
     log(f"mixed step: {ms_step:.4f} ms; " + ", ".join(f"{k}={v:.4f}" for k, v in kms.items()))
-    return dict(dtype="f32 flux and limiters, f64 VI", ms_per_step=ms_step,
+    return dict(dtype="f32 flux, limiters and VI sources; f64 rho*w and tridiagonal elimination", ms_per_step=ms_step,
                 value=spec.inner_cell_levels() / (ms_step * 1e-3), unit="cell-levels/s",
                 hbm_frac_step=round(step_gbs / perfmodel.HBM_PEAK_GBS, 4), step_alg_bytes=sum(kb.values()),
                 kernels=kernels, rel_error_vs_f64=errors, share_above_1em5=share)
@@ -468,7 +480,19 @@ def main() -> int:
             args += [W[k], W[k + "_pl"]]
         ctx.vi_rhow_solver(W["rhogw"], W["rhogw_pl"], *args, DT_SMALL)
 
-    calls = (k_flux, k_hlim, k_vlim, k_vi)
+    # the solve with FP32 state and sources, FP64 rho*w and elimination
+    vkeys = list(drivers.VI_IN)
+    W32 = {k + s: torch.from_numpy(np.ascontiguousarray(case["vi"][k + s], dtype=np.float32)).to(dev)
+           for k in vkeys for s in ("", "_pl")}
+    w64 = {s: torch.from_numpy(np.ascontiguousarray(case["vi"]["rhogw" + s])).to(dev) for s in ("", "_pl")}
+
+    def k_vi_mixed():
+        a = []
+        for k in vkeys:
+            a += [W32[k], W32[k + "_pl"]]
+        ctx.vi_rhow_solver_mixed(w64[""], w64["_pl"], *a, DT_SMALL)
+
+    calls = (k_flux, k_hlim, k_vlim, k_vi_mixed)
     for _ in range(args.warmup):
         for f in calls:
             f()

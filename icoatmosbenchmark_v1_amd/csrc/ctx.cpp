@@ -187,7 +187,8 @@ int nicam_ctx_destroy(nicam_ctx c)
                       &c->cfact, &c->dfact};
     for (double **p : all) dev_free(*p);
     float *f32[] = {c->hm32.xr, c->hm32.gp, c->hm32.gt, c->hm32.ga,
-                    c->hm32.xr_pl, c->hm32.gp_pl, c->hm32.gt_pl, c->hm32.ga_pl};
+                    c->hm32.xr_pl, c->hm32.gp_pl, c->hm32.gt_pl, c->hm32.ga_pl,
+                    c->RGAM32, c->RGAM32_pl, c->RGAMH32, c->RGAMH32_pl, c->RGSGAM2_32, c->RGSGAM2_32_pl};
     for (float *p : f32)
         if (p) (void)hipFree(p);
     for (void *p : c->stage)
@@ -459,6 +460,29 @@ static int set_pair(nicam_ctx c, double **dst, double **dst_pl, const double *sr
     return NICAM_OK;
 }
 
+// FP32 copies of the metrics that enter only the solver's source term, kept
+// in step with the FP64 ones (both setters write RGAM and RGAMH).
+static int refresh_vmetrics32(nicam_ctx c)
+{
+    const size_t R = c->d.nreg(), P = c->d.npl();
+    struct {
+        double *src, *src_pl;
+        float **dst, **dst_pl;
+    } m[] = {{c->RGAM, c->RGAM_pl, &c->RGAM32, &c->RGAM32_pl},
+             {c->RGAMH, c->RGAMH_pl, &c->RGAMH32, &c->RGAMH32_pl},
+             {c->RGSGAM2, c->RGSGAM2_pl, &c->RGSGAM2_32, &c->RGSGAM2_32_pl}};
+    for (auto &e : m) {
+        if (!e.src) continue;
+        NICAM_TRY(dev_alloc_f(c, e.dst, R));
+        NICAM_TRY(launch_to_float(c, e.src, *e.dst, R));
+        if (c->d.have_pl && e.src_pl) {
+            NICAM_TRY(dev_alloc_f(c, e.dst_pl, P));
+            NICAM_TRY(launch_to_float(c, e.src_pl, *e.dst_pl, P));
+        }
+    }
+    return NICAM_OK;
+}
+
 int nicam_set_vmetrics(nicam_ctx c, const double *RGAM, const double *RGAM_pl, const double *RGAMH,
                        const double *RGAMH_pl, const double *RGSGAM2, const double *RGSGAM2_pl,
                        const double *RGSGAM2H, const double *RGSGAM2H_pl, const double *GSGAM2H,
@@ -470,6 +494,7 @@ int nicam_set_vmetrics(nicam_ctx c, const double *RGAM, const double *RGAM_pl, c
     NICAM_TRY(set_pair(c, &c->RGSGAM2, &c->RGSGAM2_pl, RGSGAM2, RGSGAM2_pl, "VMTR_RGSGAM2"));
     NICAM_TRY(set_pair(c, &c->RGSGAM2H, &c->RGSGAM2H_pl, RGSGAM2H, RGSGAM2H_pl, "VMTR_RGSGAM2H"));
     NICAM_TRY(set_pair(c, &c->GSGAM2H, &c->GSGAM2H_pl, GSGAM2H, GSGAM2H_pl, "VMTR_GSGAM2H"));
+    NICAM_TRY(refresh_vmetrics32(c));
     NICAM_HIP(hipStreamSynchronize(c->stream));
     c->have_vmetrics = true;
     return NICAM_OK;
@@ -491,6 +516,7 @@ int nicam_set_vmetrics3d(nicam_ctx c, const double *RGSQRTH, const double *RGSQR
         NICAM_TRY(dev_alloc(c, &c->C2WfactGz_pl, c->d.npl() * 6));
         NICAM_TRY(copy_in(c, c->C2WfactGz_pl, C2WfactGz_pl, c->d.npl() * 6));
     }
+    NICAM_TRY(refresh_vmetrics32(c));
     NICAM_HIP(hipStreamSynchronize(c->stream));
     c->have_vmetrics3d = true;
     return NICAM_OK;
@@ -750,6 +776,40 @@ int nicam_vi_rhow_solver(nicam_ctx c, double *w, double *w_pl, const double *w0,
     auto D = [&](int i) { return static_cast<double *>(v[i]); };
     NICAM_TRY(launch_vi(c, D(0), D(1), D(2), D(3), D(4), D(5), D(6), D(7), D(8), D(9), D(10), D(11), D(12),
                         D(13), dt));
+    return stage_out(c, a, v);
+}
+
+int nicam_vi_rhow_solver_mixed(nicam_ctx c, double *w, double *w_pl, const float *w0, const float *w0_pl,
+                               const float *p0, const float *p0_pl, const float *r0, const float *r0_pl,
+                               const float *Sr, const float *Sr_pl, const float *Sw, const float *Sw_pl,
+                               const float *Sp, const float *Sp_pl, double dt)
+{
+    NICAM_TRY(check_ctx(c));
+    if (!c->have_coef) return fail(NICAM_ERR_STATE, "tridiagonal coefficients not set (nicam_vi_rhow_solver_putcoef)");
+    if (!c->have_vmetrics) return fail(NICAM_ERR_STATE, "vertical metrics not set (nicam_set_vmetrics)");
+    if (!c->have_vgrid) return fail(NICAM_ERR_STATE, "vertical grid not set (nicam_grd_setup)");
+    if (!(dt != 0.0)) return fail(NICAM_ERR_ARG, "dt must be nonzero");
+    const void *req[] = {w, w0, p0, r0, Sr, Sw, Sp};
+    const char *nm[] = {"rhogw", "rhogw0", "preg0", "rhog0", "Srho", "Sw", "Spre"};
+    for (int i = 0; i < 7; ++i) NICAM_TRY(need(req[i], nm[i]));
+    if (c->d.have_pl) {
+        const void *rp[] = {w_pl, w0_pl, p0_pl, r0_pl, Sr_pl, Sw_pl, Sp_pl};
+        for (int i = 0; i < 7; ++i) NICAM_TRY(need(rp[i], "pole array"));
+    }
+    if (c->ptr_mode == NICAM_PTR_DEVICE)
+        return launch_vi_mixed(c, w, PL(w_pl), w0, PL(w0_pl), p0, PL(p0_pl), r0, PL(r0_pl), Sr, PL(Sr_pl), Sw,
+                               PL(Sw_pl), Sp, PL(Sp_pl), dt);
+    const size_t R = c->d.nreg(), P = c->d.npl();
+    std::vector<HostArg> a = {{w, R * 8, true},       {PL(w_pl), P * 8, true},   {w0, R * 4, false},
+                              {PL(w0_pl), P * 4, false}, {p0, R * 4, false},      {PL(p0_pl), P * 4, false},
+                              {r0, R * 4, false},     {PL(r0_pl), P * 4, false}, {Sr, R * 4, false},
+                              {PL(Sr_pl), P * 4, false}, {Sw, R * 4, false},      {PL(Sw_pl), P * 4, false},
+                              {Sp, R * 4, false},     {PL(Sp_pl), P * 4, false}};
+    std::vector<void *> v;
+    NICAM_TRY(stage_in(c, a, v));
+    auto F = [&](int i) { return static_cast<const float *>(v[i]); };
+    NICAM_TRY(launch_vi_mixed(c, static_cast<double *>(v[0]), static_cast<double *>(v[1]), F(2), F(3), F(4), F(5),
+                              F(6), F(7), F(8), F(9), F(10), F(11), F(12), F(13), dt));
     return stage_out(c, a, v);
 }
 

@@ -85,6 +85,9 @@ struct nicam_ctx_s {
     double *RGSGAM2H = nullptr, *RGSGAM2H_pl = nullptr;
     double *GSGAM2H = nullptr, *GSGAM2H_pl = nullptr;
     bool have_vmetrics = false;
+    // RGAM, RGAMH, RGSGAM2 rounded to FP32, for the mixed solver's source term
+    float *RGAM32 = nullptr, *RGAM32_pl = nullptr, *RGAMH32 = nullptr, *RGAMH32_pl = nullptr;
+    float *RGSGAM2_32 = nullptr, *RGSGAM2_32_pl = nullptr;
     // vertical metrics of OPRT3D_divdamp (RGAM/RGAMH above are shared)
     double *RGSQRTH = nullptr, *RGSQRTH_pl = nullptr;
     double *C2WfactGz = nullptr, *C2WfactGz_pl = nullptr;  // (gall,kall,6,lall), (gall_pl,kall,6,2)
@@ -264,6 +267,9 @@ int launch_vlim_adjoint(nicam_ctx c, double *qhb, double *qhb_pl, double *qb, do
 int launch_vi_adjoint(nicam_ctx c, double *wb, double *wb_pl, double *w0b, double *w0b_pl, double *p0b,
                       double *p0b_pl, double *r0b, double *r0b_pl, double *Srb, double *Srb_pl, double *Swb,
                       double *Swb_pl, double *Spb, double *Spb_pl, double dt);
+int launch_vi_mixed(nicam_ctx c, double *w, double *w_pl, const float *w0, const float *w0_pl, const float *p0,
+                    const float *p0_pl, const float *r0, const float *r0_pl, const float *Sr, const float *Sr_pl,
+                    const float *Sw, const float *Sw_pl, const float *Sp, const float *Sp_pl, double dt);
 int launch_vi(nicam_ctx c, double *w, double *w_pl, const double *w0, const double *w0_pl,
               const double *p0, const double *p0_pl, const double *r0, const double *r0_pl,
               const double *Sr, const double *Sr_pl, const double *Sw, const double *Sw_pl,

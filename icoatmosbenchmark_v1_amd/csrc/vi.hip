@@ -18,20 +18,28 @@
 //   3  gamma and the forward solution in registers (fully unrolled levels;
 //      built for kall = 42 and 98)
 // All three perform the same operations in the same order.
+#include <type_traits>
+
 #include "internal.h"
 
 namespace nicam {
 namespace {
 
-struct ViField {
+// Src is the type of the prognostic and source inputs and of the metrics that
+// only enter the source term (double, or float for the mixed solver: loads
+// are widened exactly, every operation stays FP64).
+template <class Src>
+struct ViFieldT {
     double *w;
-    const double *w0, *p0, *r0, *Sr, *Sw, *Sp;
+    const Src *w0, *p0, *r0, *Sr, *Sw, *Sp;
     const double *Mc, *Ml, *Mu;
-    const double *RGAM, *RGAMH, *RGSGAM2, *RGSGAM2H, *GSGAM2H;
+    const Src *RGAM, *RGAMH, *RGSGAM2;
+    const double *RGSGAM2H, *GSGAM2H;
 };
 
-struct ViArgs {
-    ViField reg, pl;
+template <class Src>
+struct ViArgsT {
+    ViFieldT<Src> reg, pl;
     const double *rdgzh, *afact, *bfact;
     double *gwork;  // variant 2: [kall][ncol]
     long ncol;      // columns including the pole columns
@@ -62,8 +70,8 @@ struct RegArray {
 // One column.  KC > 0 fixes kall at compile time (loops fully unrolled, W
 // may then hold the forward solution in registers); KC == 0 reads kall at run
 // time and keeps the forward solution in rhogw.
-template <int KC, bool WREG, class G, class WS>
-__device__ __forceinline__ void vi_column(const ViField &F, size_t base, size_t s, int kall_rt,
+template <int KC, bool WREG, class G, class WS, class Src>
+__device__ __forceinline__ void vi_column(const ViFieldT<Src> &F, size_t base, size_t s, int kall_rt,
                                           const double *__restrict__ rdgzh, const double *__restrict__ afact,
                                           const double *__restrict__ bfact, double dt, double CVovRt2,
                                           double alpha, double grav, G &gam, WS &wf)
@@ -171,7 +179,8 @@ __device__ __forceinline__ void vi_column(const ViField &F, size_t base, size_t 
 }
 
 // work-item t -> (field set, column base, level stride); false if idle
-__device__ __forceinline__ bool vi_map(const ViArgs &A, long t, const ViField *&F, size_t &base, size_t &s)
+template <class Src>
+__device__ __forceinline__ bool vi_map(const ViArgsT<Src> &A, long t, const ViFieldT<Src> *&F, size_t &base, size_t &s)
 {
     const long nreg = (long)A.gall * A.lall;
     if (t < nreg) {
@@ -196,11 +205,12 @@ constexpr int BLOCK_LDS = 64;
 constexpr int BLOCK_GLB = 256;
 constexpr int BLOCK_REG = 64;
 
-__global__ void __launch_bounds__(BLOCK_LDS) vi_kernel_lds(ViArgs A)
+template <class Src>
+__global__ void __launch_bounds__(BLOCK_LDS) vi_kernel_lds(ViArgsT<Src> A)
 {
     extern __shared__ double s_gam[];
     const long t = (long)blockIdx.x * BLOCK_LDS + threadIdx.x;
-    const ViField *F;
+    const ViFieldT<Src> *F;
     size_t base, s;
     if (!vi_map(A, t, F, base, s)) return;
     LdsGamma gam{s_gam + threadIdx.x, BLOCK_LDS};
@@ -209,10 +219,11 @@ __global__ void __launch_bounds__(BLOCK_LDS) vi_kernel_lds(ViArgs A)
                         unused);
 }
 
-__global__ void __launch_bounds__(BLOCK_GLB) vi_kernel_glb(ViArgs A)
+template <class Src>
+__global__ void __launch_bounds__(BLOCK_GLB) vi_kernel_glb(ViArgsT<Src> A)
 {
     const long t = (long)blockIdx.x * BLOCK_GLB + threadIdx.x;
-    const ViField *F;
+    const ViFieldT<Src> *F;
     size_t base, s;
     if (!vi_map(A, t, F, base, s)) return;
     GlobalGamma gam{A.gwork + t, A.ncol};
@@ -221,11 +232,11 @@ __global__ void __launch_bounds__(BLOCK_GLB) vi_kernel_glb(ViArgs A)
                         unused);
 }
 
-template <int KC, bool WREG>
-__global__ void __launch_bounds__(BLOCK_REG) vi_kernel_reg(ViArgs A)
+template <int KC, bool WREG, class Src>
+__global__ void __launch_bounds__(BLOCK_REG) vi_kernel_reg(ViArgsT<Src> A)
 {
     const long t = (long)blockIdx.x * BLOCK_REG + threadIdx.x;
-    const ViField *F;
+    const ViFieldT<Src> *F;
     size_t base, s;
     if (!vi_map(A, t, F, base, s)) return;
     RegArray<KC> gam;
@@ -239,19 +250,18 @@ __global__ void __launch_bounds__(BLOCK_REG) vi_kernel_reg(ViArgs A)
     }
 }
 
-}  // namespace
-
-int launch_vi(nicam_ctx c, double *w, double *w_pl, const double *w0, const double *w0_pl,
-              const double *p0, const double *p0_pl, const double *r0, const double *r0_pl,
-              const double *Sr, const double *Sr_pl, const double *Sw, const double *Sw_pl,
-              const double *Sp, const double *Sp_pl, double dt)
+template <class Src>
+int launch_vi_t(nicam_ctx c, double *w, double *w_pl, const Src *w0, const Src *w0_pl, const Src *p0, const Src *p0_pl,
+                const Src *r0, const Src *r0_pl, const Src *Sr, const Src *Sr_pl, const Src *Sw, const Src *Sw_pl, const Src *Sp,
+                const Src *Sp_pl, double dt, const Src *RGAM, const Src *RGAM_pl, const Src *RGAMH, const Src *RGAMH_pl,
+                const Src *RGSGAM2, const Src *RGSGAM2_pl)
 {
     const Dims &D = c->d;
-    ViArgs A;
-    A.reg = ViField{w, w0, p0, r0, Sr, Sw, Sp, c->Mc, c->Ml, c->Mu, c->RGAM, c->RGAMH, c->RGSGAM2,
-                    c->RGSGAM2H, c->GSGAM2H};
-    A.pl = ViField{w_pl, w0_pl, p0_pl, r0_pl, Sr_pl, Sw_pl, Sp_pl, c->Mc_pl, c->Ml_pl, c->Mu_pl,
-                   c->RGAM_pl, c->RGAMH_pl, c->RGSGAM2_pl, c->RGSGAM2H_pl, c->GSGAM2H_pl};
+    ViArgsT<Src> A;
+    A.reg = ViFieldT<Src>{w, w0, p0, r0, Sr, Sw, Sp, c->Mc, c->Ml, c->Mu, RGAM, RGAMH, RGSGAM2, c->RGSGAM2H,
+                        c->GSGAM2H};
+    A.pl = ViFieldT<Src>{w_pl, w0_pl, p0_pl, r0_pl, Sr_pl, Sw_pl, Sp_pl, c->Mc_pl, c->Ml_pl, c->Mu_pl, RGAM_pl,
+                       RGAMH_pl, RGSGAM2_pl, c->RGSGAM2H_pl, c->GSGAM2H_pl};
     A.rdgzh = c->rdgzh;
     A.afact = c->afact;
     A.bfact = c->bfact;
@@ -270,16 +280,18 @@ int launch_vi(nicam_ctx c, double *w, double *w_pl, const double *w0, const doub
     int variant = c->tune.vi_variant;
     const bool reg_ok = (D.kall == 42 || D.kall == 98);
     if (variant == 3 && !reg_ok) variant = 0;
-    // measured at gl7 (40 x 66x66 x 98): registers 407 us, work array 524 us, LDS 606 us
-    if (variant == 0) variant = reg_ok ? 3 : 2;
+    // measured at gl7 (40 x 66x66 x 98): FP64 inputs: registers 380-407 us, work array 525 us, LDS 600 us;
+    // FP32 inputs: work array 372 us, registers 562 us, LDS 566 us (the widening loads lengthen the
+    // register variant's one-wave-per-SIMD dependency chain)
+    if (variant == 0) variant = (reg_ok && !std::is_same<Src, float>::value) ? 3 : 2;
 
     if (variant == 1) {
         const size_t lds = sizeof(double) * BLOCK_LDS * D.kall;
         if (lds > 160 * 1024) return fail(NICAM_ERR_ARG, "kall too large for the LDS elimination factors");
         if (lds > 64 * 1024)
-            NICAM_HIP(hipFuncSetAttribute((const void *)vi_kernel_lds, hipFuncAttributeMaxDynamicSharedMemorySize,
+            NICAM_HIP(hipFuncSetAttribute((const void *)vi_kernel_lds<Src>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                           (int)lds));
-        hipLaunchKernelGGL(vi_kernel_lds, dim3((unsigned)((A.ncol + BLOCK_LDS - 1) / BLOCK_LDS)), dim3(BLOCK_LDS), lds,
+        hipLaunchKernelGGL(vi_kernel_lds<Src>, dim3((unsigned)((A.ncol + BLOCK_LDS - 1) / BLOCK_LDS)), dim3(BLOCK_LDS), lds,
                            c->stream, A);
     } else if (variant == 2) {
         const size_t need = (size_t)A.ncol * D.kall;
@@ -296,18 +308,37 @@ int launch_vi(nicam_ctx c, double *w, double *w_pl, const double *w0, const doub
             c->device_bytes += need * sizeof(double);
         }
         A.gwork = c->vi_gamma;
-        hipLaunchKernelGGL(vi_kernel_glb, dim3((unsigned)((A.ncol + BLOCK_GLB - 1) / BLOCK_GLB)), dim3(BLOCK_GLB), 0,
+        hipLaunchKernelGGL(vi_kernel_glb<Src>, dim3((unsigned)((A.ncol + BLOCK_GLB - 1) / BLOCK_GLB)), dim3(BLOCK_GLB), 0,
                            c->stream, A);
     } else {
         const dim3 grid((unsigned)((A.ncol + BLOCK_REG - 1) / BLOCK_REG));
         // gamma and the forward solution of a whole column in registers: 252
         // VGPRs at kall = 42, 256 VGPRs + 256 AGPRs (one wavefront per SIMD,
         // no spills) at kall = 98
-        if (D.kall == 42) hipLaunchKernelGGL((vi_kernel_reg<42, true>), grid, dim3(BLOCK_REG), 0, c->stream, A);
-        else hipLaunchKernelGGL((vi_kernel_reg<98, true>), grid, dim3(BLOCK_REG), 0, c->stream, A);
+        if (D.kall == 42) hipLaunchKernelGGL((vi_kernel_reg<42, true, Src>), grid, dim3(BLOCK_REG), 0, c->stream, A);
+        else hipLaunchKernelGGL((vi_kernel_reg<98, true, Src>), grid, dim3(BLOCK_REG), 0, c->stream, A);
     }
     NICAM_HIP(hipGetLastError());
     return NICAM_OK;
+}
+
+}  // namespace
+
+int launch_vi(nicam_ctx c, double *w, double *w_pl, const double *w0, const double *w0_pl,
+              const double *p0, const double *p0_pl, const double *r0, const double *r0_pl,
+              const double *Sr, const double *Sr_pl, const double *Sw, const double *Sw_pl,
+              const double *Sp, const double *Sp_pl, double dt)
+{
+    return launch_vi_t<double>(c, w, w_pl, w0, w0_pl, p0, p0_pl, r0, r0_pl, Sr, Sr_pl, Sw, Sw_pl, Sp, Sp_pl, dt,
+                               c->RGAM, c->RGAM_pl, c->RGAMH, c->RGAMH_pl, c->RGSGAM2, c->RGSGAM2_pl);
+}
+
+int launch_vi_mixed(nicam_ctx c, double *w, double *w_pl, const float *w0, const float *w0_pl, const float *p0,
+                    const float *p0_pl, const float *r0, const float *r0_pl, const float *Sr, const float *Sr_pl,
+                    const float *Sw, const float *Sw_pl, const float *Sp, const float *Sp_pl, double dt)
+{
+    return launch_vi_t<float>(c, w, w_pl, w0, w0_pl, p0, p0_pl, r0, r0_pl, Sr, Sr_pl, Sw, Sw_pl, Sp, Sp_pl, dt,
+                              c->RGAM32, c->RGAM32_pl, c->RGAMH32, c->RGAMH32_pl, c->RGSGAM2_32, c->RGSGAM2_32_pl);
 }
 
 }  // namespace nicam

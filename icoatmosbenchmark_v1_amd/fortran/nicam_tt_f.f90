@@ -180,6 +180,15 @@ module nicam_tt_f
        integer(c_int) :: nicam_vi_rhow_solver
      end function nicam_vi_rhow_solver
 
+     function nicam_vi_rhow_solver_mixed(ctx, rhogw, rhogw_pl, rhogw0, rhogw0_pl, preg0, preg0_pl, rhog0, &
+          rhog0_pl, Srho, Srho_pl, Sw, Sw_pl, Spre, Spre_pl, dt) bind(C, name='nicam_vi_rhow_solver_mixed')
+       import :: c_ptr, c_int, c_double
+       type(c_ptr), value :: ctx, rhogw, rhogw_pl, rhogw0, rhogw0_pl, preg0, preg0_pl, rhog0, rhog0_pl
+       type(c_ptr), value :: Srho, Srho_pl, Sw, Sw_pl, Spre, Spre_pl
+       real(c_double), value :: dt
+       integer(c_int) :: nicam_vi_rhow_solver_mixed
+     end function nicam_vi_rhow_solver_mixed
+
      function nicam_vi_rhow_solver_adjoint(ctx, rhogw_b, rhogw_b_pl, rhogw0_b, rhogw0_b_pl, preg0_b, preg0_b_pl, &
           rhog0_b, rhog0_b_pl, Srho_b, Srho_b_pl, Sw_b, Sw_b_pl, Spre_b, Spre_b_pl, dt) &
           bind(C, name='nicam_vi_rhow_solver_adjoint')

@@ -62,6 +62,17 @@ def kernel_bytes(spec: GridSpec, word: int = 8) -> Dict[str, int]:
                 vi_rhow_solver=vi)
 
 
+def vi_mixed_bytes(spec: GridSpec) -> int:
+    """Algorithmic bytes of nicam_vi_rhow_solver_mixed: the per-column streams
+    of kernel_bytes' VI with the state, the sources and RGAM, RGAMH, RGSGAM2
+    at 4 bytes; Mc/Ml/Mu, RGSGAM2H, GSGAM2H and rhogw at 8."""
+    kall, ncol = spec.kall, spec.gall * spec.lall + (2 if spec.have_pl else 0) * spec.gall_pl
+    nk3, nk2, nk1 = kall - 3, kall - 2, kall - 1
+    # levels kmin+1..kmax: rhogw0, Sw, RGAMH (4 B) and Mc, Ml, Mu (8 B); kmin..kmax: preg0, Spre, rhog0, Srho,
+    # RGSGAM2, RGAM (4 B); rhogw and RGSGAM2H at kmin and kmax+1; GSGAM2H and the rhogw result on kall-1 levels
+    return (nk3 * (3 * 4 + 3 * 8) + nk2 * 6 * 4 + 2 * 2 * 8 + nk1 * 2 * 8) * ncol
+
+
 def adjoint_bytes(spec: GridSpec, word: int = 8) -> Dict[str, int]:
     """Algorithmic bytes of the adjoint kernels: each array the kernel must
     read or write counted once over the levels and points it touches (the

@@ -108,6 +108,7 @@ SIGNATURES = {
     "nicam_set_vmetrics3d": (c_int, [_P] + [_P] * 8),
     "nicam_oprt3d_divdamp": (c_int, [_P] + [_P] * 18),
     "nicam_vi_rhow_solver_adjoint": (c_int, [_P] + [_P] * 14 + [c_double]),
+    "nicam_vi_rhow_solver_mixed": (c_int, [_P] + [_P] * 14 + [c_double]),
     "nicam_vertical_limiter_thuburn_mask": (c_int, [_P] * 11),
     "nicam_horizontal_limiter_thuburn_mask": (c_int, [_P] * 17),
     "nicam_horizontal_flux_adjoint": (c_int, [_P] * 21 + [c_double]),
@@ -656,6 +657,21 @@ class Context:
                                                                                            "mask_pl", MP)]
         _check(self.lib.nicam_vertical_limiter_thuburn_adjoint(self.h, *a),
                "nicam_vertical_limiter_thuburn_adjoint")
+
+    def vi_rhow_solver_mixed(self, rhogw, rhogw_pl, rhogw0, rhogw0_pl, preg0, preg0_pl, rhog0, rhog0_pl, Srho,
+                             Srho_pl, Sw, Sw_pl, Spre, Spre_pl, dt: float) -> None:
+        """nicam_vi_rhow_solver_mixed: float sources and state, double rhogw, FP64 elimination."""
+        R, P = self.nreg, self.npl
+        a = [_ptr(rhogw, self.mode, "rhogw", R), None if not self.have_pl else _ptr(rhogw_pl, self.mode,
+                                                                                    "rhogw_pl", P)]
+        names = ["rhogw0", "preg0", "rhog0", "Srho", "Sw", "Spre"]
+        arrs = [rhogw0, rhogw0_pl, preg0, preg0_pl, rhog0, rhog0_pl, Srho, Srho_pl, Sw, Sw_pl, Spre, Spre_pl]
+        spec = []
+        for i, nm in enumerate(names):
+            spec.append((arrs[2 * i], nm, R))
+            spec.append((arrs[2 * i + 1], nm + "_pl", P))
+        a += self._args(spec, single=True)
+        _check(self.lib.nicam_vi_rhow_solver_mixed(self.h, *a, float(dt)), "nicam_vi_rhow_solver_mixed")
 
     def vi_rhow_solver_adjoint(self, rhogw_b, rhogw_b_pl, rhogw0_b, rhogw0_b_pl, preg0_b, preg0_b_pl, rhog0_b,
                                rhog0_b_pl, Srho_b, Srho_b_pl, Sw_b, Sw_b_pl, Spre_b, Spre_b_pl, dt: float) -> None:

@@ -493,6 +493,23 @@ int nicam_vi_rhow_update_matrix(nicam_ctx ctx, const double *eth, const double *
 int nicam_vi_rhow_getcoef(nicam_ctx ctx, double *Mc, double *Mc_pl, double *Ml, double *Ml_pl, double *Mu,
                           double *Mu_pl);
 
+/* vi_rhow_solver with FP32 prognostic and source inputs (the mixed-precision
+ * step): rhogw0, preg0, rhog0, Srho, Sw, Spre are float arrays, rhogw stays
+ * double, and the tridiagonal elimination and every operation run in FP64
+ * on the exactly widened inputs.  The source term reads FP32 copies of
+ * VMTR_RGAM, VMTR_RGAMH, VMTR_RGSGAM2 kept by nicam_set_vmetrics; Mc/Ml/Mu,
+ * RGSGAM2H and GSGAM2H stay FP64.  The result equals nicam_vi_rhow_solver run
+ * on the same inputs and those three metrics rounded to FP32. */
+int nicam_vi_rhow_solver_mixed(nicam_ctx ctx,
+                               double *rhogw, double *rhogw_pl,
+                               const float *rhogw0, const float *rhogw0_pl,
+                               const float *preg0, const float *preg0_pl,
+                               const float *rhog0, const float *rhog0_pl,
+                               const float *Srho, const float *Srho_pl,
+                               const float *Sw, const float *Sw_pl,
+                               const float *Spre, const float *Spre_pl,
+                               double dt);
+
 /* ---- Adjoints (hand-written reverse of the kernels above) ----------------
  * Each computes the transpose of its forward kernel's derivative at fixed
  * coefficients.  No counterpart exists in the reference: these replace no

@@ -209,3 +209,37 @@ def test_f32_symbols_and_dtype_checks():
     with pytest.raises(TypeError):
         runtime._ptr(np.zeros(4), PTR_HOST, "x", 4, single=True)  # float64 where float32 is needed
     assert runtime._ptr(np.zeros(4, np.float32), PTR_HOST, "x", 4, single=True)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("c", [(10, 12, 2, 151, True), (18, 42, 1, 152, False), (66, 98, 1, 153, True)],
+                         ids=["g10k12", "g18k42-nopl", "g66k98"])
+def test_gpu_vi_mixed_is_fp64_on_rounded_inputs(gpu, oracle, c):
+    """nicam_vi_rhow_solver_mixed widens its FP32 inputs exactly and runs the
+    FP64 solver: it equals the FP64 oracle given the same inputs, and
+    RGAM, RGAMH, RGSGAM2 rounded to FP32, bit for bit."""
+    import torch
+
+    n, kall, lall, seed, pl = c
+    spec = synth.GridSpec(n, kall, lall, have_pl=pl, seed=seed)
+    vi = synth.make_case(spec)["vi"]
+    ctx = drivers.setup_context(spec, synth.make_geometry(spec), vi)
+    names = ["rhogw0", "preg0", "rhog0", "Srho", "Sw", "Spre"]
+    r32 = lambda a: np.asarray(a, np.float32).astype(np.float64)  # noqa: E731
+    rounded = dict(vi)
+    for k in names + ["RGAM", "RGAMH", "RGSGAM2"]:
+        rounded[k], rounded[k + "_pl"] = r32(vi[k]), r32(vi[k + "_pl"])
+    g = oracle.grd_setup(vi["gz"], vi["gzh"])
+    want = oracle.vi_rhow_solver(spec, rounded, 60.0, g["rdgzh"], g["afact"], g["bfact"])
+    T32 = lambda a: torch.from_numpy(np.ascontiguousarray(a, np.float32)).to(gpu)  # noqa: E731
+    w = torch.from_numpy(np.ascontiguousarray(vi["rhogw"])).to(gpu)
+    w_pl = torch.from_numpy(np.ascontiguousarray(vi["rhogw_pl"])).to(gpu)
+    args = []
+    for k in names:
+        args += [T32(vi[k]), T32(vi[k + "_pl"])]
+    ctx.vi_rhow_solver_mixed(w, w_pl, *args, 60.0)
+    np.testing.assert_array_equal(w.cpu().numpy(), want["rhogw"])
+    if pl:
+        np.testing.assert_array_equal(w_pl.cpu().numpy(), want["rhogw_pl"])
+    full = oracle.vi_rhow_solver(spec, vi, 60.0, g["rdgzh"], g["afact"], g["bfact"])
+    assert _rel_err(w.cpu().numpy(), full["rhogw"]) < 1e-5
