@@ -714,7 +714,16 @@ __global__ void __launch_bounds__(FCHUNK) hflux_adjoint_kernel(FaArgs A)
     const int X0 = g0 + tid;
     if (X0 >= gall) return;
     auto AB = [&](int a, int f, int c) { return s_a[((a - 1) * 4 + f) * na + c - alo]; };
-    auto own = [&](int c) { return c >= 0 && c % iall <= iall - 2 && c / iall <= iall - 2; };
+    // triangle origins of the cell's six triangles, and whether each is one (i, j in [0, iall-2])
+    const int i0 = X0 % iall, j0 = X0 / iall;
+    const bool o_sw = i0 >= 1 && j0 >= 1 && i0 - 1 <= iall - 2 && j0 - 1 <= iall - 2;  // X0-iall-1
+    const bool o_s = j0 >= 1 && i0 <= iall - 2 && j0 - 1 <= iall - 2;                  // X0-iall
+    const bool o_w = i0 >= 1 && i0 - 1 <= iall - 2 && j0 <= iall - 2;                  // X0-1
+    const bool o_c = i0 <= iall - 2 && j0 <= iall - 2;                                 // X0
+    const int c_sw = X0 - iall - 1, c_s = X0 - iall, c_w = X0 - 1;
+    const double w_sw_i = o_sw ? W(c_sw, 0, 4) : 0.0, w_sw_j = o_sw ? W(c_sw, 1, 3) : 0.0;
+    const double w_s_j = o_s ? W(c_s, 1, 4) : 0.0, w_w_i = o_w ? W(c_w, 0, 3) : 0.0;
+    const double w_c_i = o_c ? W(X0, 0, 2) : 0.0, w_c_j = o_c ? W(X0, 1, 2) : 0.0;
     // triangle cotangents: TI(c) from AI(c), AIJ(c), AJ(c+1); TJ(c) from AI(c+iall), AIJ(c), AJ(c)
     auto tib0 = [&](int f, int c) { return ((0.0 + AB(1, f, c)) + AB(2, f, c)) + AB(3, f, c + 1); };
     auto tjb0 = [&](int f, int c) { return ((0.0 + AB(1, f, c + iall)) + AB(2, f, c)) + AB(3, f, c); };
@@ -723,14 +732,10 @@ __global__ void __launch_bounds__(FCHUNK) hflux_adjoint_kernel(FaArgs A)
 #pragma unroll
     for (int f = 0; f < 4; ++f) {
         double y = 0.0;
-        int c = X0 - iall - 1;
-        if (own(c)) y = y + (tib(f, c) * W(c, 0, 4) + tjb(f, c) * W(c, 1, 3));
-        c = X0 - iall;
-        if (own(c)) y = y + tjb(f, c) * W(c, 1, 4);
-        c = X0 - 1;
-        if (own(c)) y = y + tib(f, c) * W(c, 0, 3);
-        c = X0;
-        if (own(c)) y = y + (tib(f, c) * W(c, 0, 2) + tjb(f, c) * W(c, 1, 2));
+        if (o_sw) y = y + (tib(f, c_sw) * w_sw_i + tjb(f, c_sw) * w_sw_j);
+        if (o_s) y = y + tjb(f, c_s) * w_s_j;
+        if (o_w) y = y + tib(f, c_w) * w_w_i;
+        if (o_c) y = y + (tib(f, X0) * w_c_i + tjb(f, X0) * w_c_j);
         A.Xb[f][fb + X0] = y;
     }
 }
