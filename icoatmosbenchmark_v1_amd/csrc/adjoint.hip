@@ -616,15 +616,18 @@ __global__ void fill_zero_kernel(double *p, size_t n)
 // ---- horizontal_flux (dyn_horiz_adv_flux/src/mod_src_tracer.f90:42-379) -------
 //
 // Linear in rho*v; GRD_xc carries 1/max(rho_arc, EPS).  The arc cotangents
-// are gathered, not scattered: a work-group of FCHUNK cells of one slab
+// are gathered, not scattered: a work-group of cells of one slab
 // computes the cotangent of every arc whose triangles touch its cells
-// (origins [g0-iall-1, g0+FCHUNK+iall)) into LDS, then each work-item sums
+// (origins [g0-iall-1, g0+cells+iall)) into LDS, then each work-item sums
 // its cell's six triangles in the order of a sequential sweep over triangle
 // origins, so the result is the oracle's, bit for bit, with no atomics.  The
 // tracer fields are read from HBM directly (neighbouring arcs hit the
-// caches), which keeps LDS to the arc cotangents: 2 work-groups per CU at
-// gl7.  (Staging them too, with 256-cell chunks, ran at 3.05 ms at gl7.)
-constexpr int FCHUNK = 512;
+// caches), which keeps LDS to the arc cotangents.  (Staging them too, with
+// 256-cell chunks, ran at 3.05 ms at gl7; 512-cell chunks with two passes
+// over the origins at 1.91 ms.)
+// A work-group of FBLOCK work-items covers FBLOCK - (2*iall+1) cells, so the
+// arc phase (FBLOCK origins) takes one pass.
+constexpr int FBLOCK = 1024;
 
 struct FaArgs {
     const double *flxb, *xcb;   // (gall,kall,lall,6), (gall,kall,lall,3,3)
@@ -632,7 +635,7 @@ struct FaArgs {
     const double *X[4];
     const double *gt, *ga, *gp;
     const int *have_sgp;
-    int iall, gall, kall, lall;
+    int iall, gall, kall, lall, cells;
     double dt, eps;
     // poles
     const double *flxb_pl, *xcb_pl;
@@ -642,16 +645,16 @@ struct FaArgs {
     int gall_pl, have_pl;
 };
 
-__global__ void __launch_bounds__(FCHUNK) hflux_adjoint_kernel(FaArgs A)
+__global__ void __launch_bounds__(FBLOCK) hflux_adjoint_kernel(FaArgs A)
 {
     extern __shared__ double lds[];
-    const int iall = A.iall, gall = A.gall, kall = A.kall, lall = A.lall;
-    const int nchunk = (gall + FCHUNK - 1) / FCHUNK;
+    const int iall = A.iall, gall = A.gall, kall = A.kall, lall = A.lall, cells = A.cells;
+    const int nchunk = (gall + cells - 1) / cells;
     const int b = blockIdx.x;
     const int chunk = b % nchunk, kl = b / nchunk;
     const int k = kl % kall, l = kl / kall;
-    const int g0 = chunk * FCHUNK;
-    const int alo = g0 - iall - 1, na = FCHUNK + 2 * iall + 1;
+    const int g0 = chunk * cells;
+    const int alo = g0 - iall - 1, na = cells + 2 * iall + 1;
     double *s_a = lds;  // [3 arcs][4 fields][na]
     const size_t fb = ((size_t)l * kall + k) * gall;
     const size_t plane = (size_t)lall * kall * gall;
@@ -666,7 +669,7 @@ __global__ void __launch_bounds__(FCHUNK) hflux_adjoint_kernel(FaArgs A)
         return X(f, c) * W(c, 1, 2) + X(f, c + iall + 1) * W(c, 1, 3) + X(f, c + iall) * W(c, 1, 4);
     };
     const double dt = A.dt, EPS = A.eps;
-    for (int s = tid; s < na; s += FCHUNK) {
+    for (int s = tid; s < na; s += FBLOCK) {
         const int a0 = alo + s;
         const int i = a0 >= 0 ? a0 % iall : -1, j = a0 >= 0 ? a0 / iall : -1;
         const bool org = a0 >= 0 && a0 < gall && i <= iall - 2 && j <= iall - 2;
@@ -712,7 +715,7 @@ __global__ void __launch_bounds__(FCHUNK) hflux_adjoint_kernel(FaArgs A)
     }
     __syncthreads();
     const int X0 = g0 + tid;
-    if (X0 >= gall) return;
+    if (tid >= cells || X0 >= gall) return;
     auto AB = [&](int a, int f, int c) { return s_a[((a - 1) * 4 + f) * na + c - alo]; };
     // triangle origins of the cell's six triangles, and whether each is one (i, j in [0, iall-2])
     const int i0 = X0 % iall, j0 = X0 / iall;
@@ -813,14 +816,15 @@ int launch_hflux_adjoint(nicam_ctx c, const double *flxb, const double *flxb_pl,
     A.iall = D.gall_1d; A.gall = D.gall; A.kall = D.kall; A.lall = D.lall;
     A.dt = dt; A.eps = c->eps;
     A.gall_pl = D.gall_pl; A.have_pl = D.have_pl;
-    const long nblk = (long)((D.gall + FCHUNK - 1) / FCHUNK) * D.kall * D.lall;
-    if (nblk > 0x7fffffffL) return fail(NICAM_ERR_ARG, "grid too large for one launch");
     const int iall = D.gall_1d;
-    const size_t lds = sizeof(double) * 12 * (size_t)(FCHUNK + 2 * iall + 1);
+    A.cells = FBLOCK - (2 * iall + 1);
+    const long nblk = (long)((D.gall + A.cells - 1) / A.cells) * D.kall * D.lall;
+    if (nblk > 0x7fffffffL) return fail(NICAM_ERR_ARG, "grid too large for one launch");
+    const size_t lds = sizeof(double) * 12 * (size_t)FBLOCK;
     if (lds > 64 * 1024)
         NICAM_HIP(hipFuncSetAttribute((const void *)hflux_adjoint_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
                                       (int)lds));
-    hipLaunchKernelGGL(hflux_adjoint_kernel, dim3((unsigned)nblk), dim3(FCHUNK), lds, c->stream, A);
+    hipLaunchKernelGGL(hflux_adjoint_kernel, dim3((unsigned)nblk), dim3(FBLOCK), lds, c->stream, A);
     if (D.have_pl)
         hipLaunchKernelGGL(hflux_adjoint_pole, dim3((unsigned)((D.kall * kLallPl + 63) / 64)), dim3(64), 0, c->stream,
                            A);
