@@ -572,7 +572,7 @@ def test_gpu_autograd_multistep_checkpointed(gpu):
                 qn = step(*qn)
         J = cost(*qn)
         J.backward()
-        return float(J), x.grad.clone(), x_pl.grad.clone()
+        return float(J.detach()), x.grad.clone(), x_pl.grad.clone()
 
     J, g, g_pl = grad(0)
     Jc, gc, gc_pl = grad(6)
@@ -585,7 +585,7 @@ def test_gpu_autograd_multistep_checkpointed(gpu):
         for _ in range(24):
             qp, qm = step(*qp), step(*qm)
         fd = (float(cost(*qp)) - float(cost(*qm))) / (2 * h)
-    ad = float((g * v).sum() + (g_pl * v_pl).sum())
+    ad = float(((g * v).sum() + (g_pl * v_pl).sum()).detach())
     assert abs(fd - ad) <= 1e-4 * abs(ad), (fd, ad)
 
 
