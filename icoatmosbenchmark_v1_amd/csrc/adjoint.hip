@@ -119,6 +119,92 @@ __global__ void __launch_bounds__(BLOCK_VA) vi_adjoint_kernel(VaArgs A)
     }
 }
 
+// The same column with kall fixed at compile time (used at 42): the partial
+// solution and the elimination factors stay in registers (the loops are
+// unrolled), so the output arrays are written once and never re-read.  The
+// operations and their order are those of vi_adjoint_kernel.
+constexpr int BLOCK_VAR = 64;
+
+template <int KC>
+__global__ void __launch_bounds__(BLOCK_VAR) vi_adjoint_kernel_reg(VaArgs A)
+{
+    const long t = (long)blockIdx.x * BLOCK_VAR + threadIdx.x;
+    const long nreg = (long)A.gall * A.lall;
+    VaField F;
+    size_t base, s;
+    if (t < nreg) {
+        F = A.reg;
+        base = (size_t)(t / A.gall) * KC * A.gall + (size_t)(t % A.gall);
+        s = A.gall;
+    } else if (A.have_pl && t < nreg + (long)A.gall_pl * kLallPl) {
+        const long p = t - nreg;
+        F = A.pl;
+        base = (size_t)(p / A.gall_pl) * KC * A.gall_pl + (size_t)(p % A.gall_pl);
+        s = A.gall_pl;
+    } else {
+        return;
+    }
+    constexpr int kmin = 1, kmax = KC - 2;
+    const double dt = A.dt, CV = A.CVovRt2, grav = A.grav;
+    auto at = [&](int k) { return base + (size_t)k * s; };
+    double yp[KC], gm[KC];
+    double beta = F.Mc[at(kmin + 1)];
+    double ylast = (F.wb[at(kmin + 1)] * F.GSGAM2H[at(kmin + 1)]) / beta;
+    yp[kmin + 1] = ylast;
+#pragma unroll
+    for (int k = kmin + 2; k <= kmax; ++k) {
+        const double mu_km1 = F.Mu[at(k - 1)];
+        const double g = F.Ml[at(k)] / beta;
+        beta = F.Mc[at(k)] - mu_km1 * g;
+        ylast = (F.wb[at(k)] * F.GSGAM2H[at(k)] - mu_km1 * ylast) / beta;
+        yp[k] = ylast;
+        gm[k] = g;
+        if constexpr (KC > 64) __builtin_amdgcn_sched_barrier(0);
+    }
+    const double y_kmax = ylast;
+    double yn = 0.0;
+#pragma unroll
+    for (int k = KC - 1; k >= 0; --k) {
+        const bool in_k = k >= kmin + 1 && k <= kmax, in_n = k + 1 >= kmin + 1 && k + 1 <= kmax;
+        const size_t o = at(k);
+        double yk = 0.0;
+        if (in_k) yk = (k == kmax) ? yp[k] : yp[k] - gm[k + 1] * yn;
+        const double sb = in_k ? yk * CV : 0.0;
+        const double sbn = in_n ? yn * CV : 0.0;
+        double pb = 0.0, rb = 0.0;
+        if (in_k) {
+            pb = -(sb * dt * A.rdgzh[k]);
+            rb = -(sb * dt * grav * A.afact[k]);
+        }
+        if (in_n) {
+            pb = pb + sbn * dt * A.rdgzh[k + 1];
+            rb = rb - sbn * dt * grav * A.bfact[k + 1];
+        }
+        const double p0b = pb * F.RGSGAM2[o];
+        const double rg = F.RGAM[o];
+        const double r0b = rb * (rg * rg);
+        const double rgh = F.RGAMH[o];
+        const double sw = in_k ? sb * (rgh * rgh) : 0.0;
+        F.p0b[o] = p0b;
+        F.Spb[o] = dt * p0b;
+        F.r0b[o] = r0b;
+        F.Srb[o] = dt * r0b;
+        F.w0b[o] = sw * A.alpha;
+        F.Swb[o] = dt * sw;
+        if (k == kmin) {
+            const double bb = F.wb[o] * F.GSGAM2H[o] - F.Ml[at(kmin + 1)] * yn;
+            F.wb[o] = bb * F.RGSGAM2H[o];
+        } else if (k == kmax + 1) {
+            const double bt = F.wb[o] * F.GSGAM2H[o] - F.Mu[at(kmax)] * y_kmax;
+            F.wb[o] = bt * F.RGSGAM2H[o];
+        } else if (in_k) {
+            F.wb[o] = 0.0;
+        }
+        if (in_k) yn = yk;
+        if constexpr (KC > 64) __builtin_amdgcn_sched_barrier(0);
+    }
+}
+
 // ---- vertical_limiter_thuburn (dyn_vert_adv_limiter/src/mod_src_tracer.f90:41-243)
 //
 // At fixed ck and d the limited face value is q_h's input (bit clear in the
@@ -918,8 +1004,15 @@ int launch_vi_adjoint(nicam_ctx c, double *wb, double *wb_pl, double *w0b, doubl
     A.alpha = (double)c->alpha;
     A.grav = c->grav;
     A.ncol = (long)D.gall * D.lall + (D.have_pl ? (long)D.gall_pl * kLallPl : 0);
-    hipLaunchKernelGGL(vi_adjoint_kernel, dim3((unsigned)((A.ncol + BLOCK_VA - 1) / BLOCK_VA)), dim3(BLOCK_VA), 0,
-                       c->stream, A);
+    // kall = 42: the column in registers; at kall = 98 the two arrays spill (1.2 KB of scratch per lane), so
+    // the partial solution and the factors stay in the outputs (654 us at gl7)
+    if (D.kall == 42) {
+        hipLaunchKernelGGL(vi_adjoint_kernel_reg<42>, dim3((unsigned)((A.ncol + BLOCK_VAR - 1) / BLOCK_VAR)),
+                           dim3(BLOCK_VAR), 0, c->stream, A);
+    } else {
+        hipLaunchKernelGGL(vi_adjoint_kernel, dim3((unsigned)((A.ncol + BLOCK_VA - 1) / BLOCK_VA)), dim3(BLOCK_VA), 0,
+                           c->stream, A);
+    }
     NICAM_HIP(hipGetLastError());
     return NICAM_OK;
 }
