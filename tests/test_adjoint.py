@@ -605,7 +605,7 @@ def test_gpu_autograd_rhow_solve_dot_product(gpu):
     w, w_pl = autodiff.RhowSolve.apply(ctx, 60.0, *fields)
     u, u_pl = torch.randn_like(w), torch.randn_like(w_pl)
     ((w * u).sum() + (w_pl * u_pl).sum()).backward()
-    lhs = float((w * u).sum() + (w_pl * u_pl).sum())
+    lhs = float(((w * u).sum() + (w_pl * u_pl).sum()).detach())
     terms = [float((f.detach() * f.grad).sum()) for f in fields]
     scale = sum(float((f.detach() * f.grad).abs().sum()) for f in fields)
     assert abs(lhs - sum(terms)) <= 1e-12 * scale
