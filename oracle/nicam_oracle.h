@@ -17,6 +17,14 @@
  *                  GMTR_p_pl(gall_pl,1,2,8), GRD_xr_pl(gall_pl,1,2,3)
  * Indices in the restated loops are kept 1-based, as in the reference, and
  * converted to 0-based offsets only inside the accessor macros.
+ *
+ * The tangent-linear and adjoint functions (orc_*_tlm, orc_*_adjoint) have no
+ * counterpart in the reference.  They are pinned by what the mathematics
+ * fixes, in tests/test_adjoint.py:
+ *   - <TLM v, u> = <v, ADJ u> (and, for the linear solve, against the forward
+ *     solver itself);
+ *   - central differences of the forward kernels, which are themselves pinned
+ *     to the compiled reference.
  */
 #ifndef NICAM_ORACLE_H
 #include <stdint.h>
