@@ -759,14 +759,35 @@ __global__ void __launch_bounds__(FBLOCK) hflux_adjoint_kernel(FaArgs A)
         const int a0 = alo + s;
         const int i = a0 >= 0 ? a0 % iall : -1, j = a0 >= 0 ? a0 / iall : -1;
         const bool org = a0 >= 0 && a0 < gall && i <= iall - 2 && j <= iall - 2;
-        // the four triangles the origin's arcs use: TI(a0), TJ(a0), TJ(a0-iall) (AI), TI(a0-1) (AJ)
+        // the four triangles the origin's arcs use: TI(a0), TJ(a0), TJ(a0-iall) (AI), TI(a0-1) (AJ).
+        // They touch six cells (a0, its E, N, NE, S, W neighbours): each field is read there once, and
+        // the sums are formed in the forward's order.  The singular-point substitution (TI(0) = TJ(1))
+        // reaches only origins 0 and 1 and takes the general path.
         double tI[4], tJ[4], tJs[4], tIw[4];
+        if (org && !(sgp && a0 <= 1)) {
+            const bool hs = j >= 1, hw = i >= 1;
+            const double wi1 = W(a0, 0, 2), wi2 = W(a0, 0, 3), wi3 = W(a0, 0, 4);
+            const double wj1 = W(a0, 1, 2), wj2 = W(a0, 1, 3), wj3 = W(a0, 1, 4);
+            const int cs = a0 - iall, cw = a0 - 1;
+            const double ws1 = hs ? W(cs, 1, 2) : 0.0, ws2 = hs ? W(cs, 1, 3) : 0.0, ws3 = hs ? W(cs, 1, 4) : 0.0;
+            const double ww1 = hw ? W(cw, 0, 2) : 0.0, ww2 = hw ? W(cw, 0, 3) : 0.0, ww3 = hw ? W(cw, 0, 4) : 0.0;
 #pragma unroll
-        for (int f = 0; f < 4; ++f) {
-            tI[f] = org ? tri(f, 0, a0) : 0.0;
-            tJ[f] = org ? tri(f, 1, a0) : 0.0;
-            tJs[f] = (org && j >= 1) ? tri(f, 1, a0 - iall) : 0.0;
-            tIw[f] = (org && i >= 1) ? tri(f, 0, a0 - 1) : 0.0;
+            for (int f = 0; f < 4; ++f) {
+                const double x0 = X(f, a0), xe = X(f, a0 + 1), xn = X(f, a0 + iall), xne = X(f, a0 + iall + 1);
+                const double xs = hs ? X(f, cs) : 0.0, xw = hw ? X(f, cw) : 0.0;
+                tI[f] = x0 * wi1 + xe * wi2 + xne * wi3;
+                tJ[f] = x0 * wj1 + xne * wj2 + xn * wj3;
+                tJs[f] = hs ? xs * ws1 + xe * ws2 + x0 * ws3 : 0.0;
+                tIw[f] = hw ? xw * ww1 + x0 * ww2 + xn * ww3 : 0.0;
+            }
+        } else {
+#pragma unroll
+            for (int f = 0; f < 4; ++f) {
+                tI[f] = org ? tri(f, 0, a0) : 0.0;
+                tJ[f] = org ? tri(f, 1, a0) : 0.0;
+                tJs[f] = (org && j >= 1) ? tri(f, 1, a0 - iall) : 0.0;
+                tIw[f] = (org && i >= 1) ? tri(f, 0, a0 - 1) : 0.0;
+            }
         }
 #pragma unroll
         for (int a = 1; a <= 3; ++a) {
