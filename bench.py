@@ -41,6 +41,25 @@ def log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
+class _CallRecorder:
+    """Stands in for the library context and records which entry points a
+    step calls (the calls go through to the context)."""
+
+    def __init__(self, ctx):
+        self._ctx, self.names = ctx, []
+
+    def __getattr__(self, name):
+        f = getattr(self._ctx, name)
+        if not callable(f):
+            return f
+
+        def call(*a, **k):
+            self.names.append(name)
+            return f(*a, **k)
+
+        return call
+
+
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -211,7 +230,7 @@ def time_extras(ctx, spec, case, coef, stream, steps):
     return out
 
 
-def time_mixed(ctx, spec, case, stream, steps, k_vi, fp64):
+def time_mixed(ctx, spec, case, stream, steps, fp64):
     """The mixed-precision step: the FP32 flux and limiters (bit-identical to
     the reference built with -DSINGLE) with the FP64 VI solve, on the same
     inputs rounded to FP32; per-kernel times, algorithmic bytes, and each
@@ -550,41 +569,39 @@ def main() -> int:
     torch.cuda.synchronize()
     log(f"device memory in use: {torch.cuda.memory_allocated() / 1e9:.2f} GB + context {ctx.device_bytes() / 1e9:.2f} GB")
 
-    def k_flux():
-        ctx.horizontal_flux(F["flx_h"], F["flx_h_pl"], F["GRD_xc"], F["GRD_xc_pl"], *[F[k] for k in drivers.FLUX_IN],
-                            DT_LARGE)
+    # each headline call takes the library handle, so one pass can be checked
+    # through a recorder: the step must call exactly the four FP64 entry points
+    def k_flux(c):
+        c.horizontal_flux(F["flx_h"], F["flx_h_pl"], F["GRD_xc"], F["GRD_xc_pl"], *[F[k] for k in drivers.FLUX_IN],
+                          DT_LARGE)
 
-    def k_hlim():
-        ctx.horizontal_limiter_thuburn(H["q_a"], H["q_a_pl"], H["q"], H["q_pl"], H["d"], H["d_pl"], H["ch"],
-                                       H["ch_pl"], H["cmask"], H["cmask_pl"], H["Qout_prev"], H["Qout_prev_pl"],
-                                       H["Qout_post"], H["Qout_post_pl"])
+    def k_hlim(c):
+        c.horizontal_limiter_thuburn(H["q_a"], H["q_a_pl"], H["q"], H["q_pl"], H["d"], H["d_pl"], H["ch"],
+                                     H["ch_pl"], H["cmask"], H["cmask_pl"], H["Qout_prev"], H["Qout_prev_pl"],
+                                     H["Qout_post"], H["Qout_post_pl"])
 
-    def k_vlim():
-        ctx.vertical_limiter_thuburn(V["q_h"], V["q_h_pl"], V["q"], V["q_pl"], V["d"], V["d_pl"], V["ck"],
-                                     V["ck_pl"])
+    def k_vlim(c):
+        c.vertical_limiter_thuburn(V["q_h"], V["q_h_pl"], V["q"], V["q_pl"], V["d"], V["d_pl"], V["ck"],
+                                   V["ck_pl"])
 
-    def k_vi():
-        args = []
-        for k in drivers.VI_IN:
-            args += [W[k], W[k + "_pl"]]
-        ctx.vi_rhow_solver(W["rhogw"], W["rhogw_pl"], *args, DT_SMALL)
-
-    # the solve with FP32 state and sources, FP64 rho*w and elimination
-    vkeys = list(drivers.VI_IN)
-    W32 = {k + s: torch.from_numpy(np.ascontiguousarray(case["vi"][k + s], dtype=np.float32)).to(dev)
-           for k in vkeys for s in ("", "_pl")}
-    w64 = {s: torch.from_numpy(np.ascontiguousarray(case["vi"]["rhogw" + s])).to(dev) for s in ("", "_pl")}
-
-    def k_vi_mixed():
+    def k_vi(c):
         a = []
-        for k in vkeys:
-            a += [W32[k], W32[k + "_pl"]]
-        ctx.vi_rhow_solver_mixed(w64[""], w64["_pl"], *a, DT_SMALL)
+        for k in drivers.VI_IN:
+            a += [W[k], W[k + "_pl"]]
+        c.vi_rhow_solver(W["rhogw"], W["rhogw_pl"], *a, DT_SMALL)
 
-    calls = (k_flux, k_hlim, k_vlim, k_vi_mixed)
+    calls = (k_flux, k_hlim, k_vlim, k_vi)
+    rec = _CallRecorder(ctx)
+    for f in calls:
+        f(rec)
+    if rec.names != list(KERNELS):
+        raise AssertionError(f"the FP64 headline step must call {list(KERNELS)}, it called {rec.names}")
+    fields = [t for d in (F, H, V, W) for t in d.values() if t.is_floating_point()]
+    if any(t.dtype != torch.float64 for t in fields):
+        raise AssertionError("the FP64 headline step was handed a non-FP64 field")
     for _ in range(args.warmup):
         for f in calls:
-            f()
+            f(ctx)
     torch.cuda.synchronize()
 
     ev = [[torch.cuda.Event(enable_timing=True) for _ in range(len(calls) + 1)] for _ in range(args.steps)]
@@ -593,7 +610,7 @@ def main() -> int:
     for s in range(args.steps):
         ev[s][0].record(stream)
         for i, f in enumerate(calls):
-            f()
+            f(ctx)
             ev[s][i + 1].record(stream)
     torch.cuda.synchronize()
     t1 = time.perf_counter()
@@ -629,12 +646,12 @@ def main() -> int:
         # the FP64 outputs of one step from the original inputs, for the error figures
         H["q_a"].copy_(up(case["hlim"], ("q_a",))["q_a"])
         V["q_h"].copy_(up(case["vlim"], ("q_h",))["q_h"])
-        k_flux()
-        k_hlim()
-        k_vlim()
+        k_flux(ctx)
+        k_hlim(ctx)
+        k_vlim(ctx)
         torch.cuda.synchronize()
         fp64 = dict(flx_h=F["flx_h"], GRD_xc=F["GRD_xc"], q_a=H["q_a"], Qout_prev=H["Qout_prev"], q_h=V["q_h"])
-        mixed = time_mixed(ctx, spec, case, stream, args.steps, k_vi, fp64)
+        mixed = time_mixed(ctx, spec, case, stream, args.steps, fp64)
         mixed["speedup_vs_f64"] = round(ms_step / mixed["ms_per_step"], 3)
 
     batched = None
