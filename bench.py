@@ -35,6 +35,7 @@ sys.path.insert(0, str(REPO))
 METRIC = "NICAM adv+limiter+VI step: cell-levels/s & %HBM peak; adjoint/fwd ratio"
 KERNELS = ("horizontal_flux", "horizontal_limiter_thuburn", "vertical_limiter_thuburn", "vi_rhow_solver")
 DT_LARGE, DT_SMALL = 360.0, 60.0  # SET_dt_large / SET_dt_small (problem_size.inc:136-137)
+EPS64, FLT_MAX = 2.0 ** -52, 3.4028234663852886e38
 
 
 def log(msg: str) -> None:
@@ -230,11 +231,81 @@ def time_extras(ctx, spec, case, coef, stream, steps):
     return out
 
 
-def time_mixed(ctx, spec, case, stream, steps, fp64):
-    """The mixed-precision step: the FP32 flux and limiters (bit-identical to
-    the reference built with -DSINGLE) with the FP64 VI solve, on the same
-    inputs rounded to FP32; per-kernel times, algorithmic bytes, and each
-    field's error against the FP64 step (max |f32 - f64| / max |f64|)."""
+def time_chained(ctx, spec, case, stream, steps):
+    """One chained tracer step, FP64 and mixed (drivers.chain_step): the flux,
+    the horizontal limiter, the conservative horizontal update of rhogq, the
+    vertical limiter, the vertical update, the rho*w solve, and the FP64
+    global mass sum.  Reports each step's time, the conservation of the
+    global tracer mass, and each field's error of the mixed step against the
+    FP64 step: max |mixed - f64| / max |f64| over the field, and the share of
+    points where |mixed - f64| exceeds 1e-5 of that maximum."""
+    import numpy as np
+    import torch
+
+    from icoatmosbenchmark_v1_amd import drivers
+
+    ctx.comm_setup(1)
+    ctx.set_const_f32(EPS64, FLT_MAX)
+    gzh = case["vi"]["gzh"]
+    S = {m: drivers.chain_state(ctx, spec, case, mixed=m) for m in (False, True)}
+    m0 = drivers.total_mass(ctx, S[False]["rhogq"], S[False]["rhogq_pl"], gzh)
+    out = {}
+    for m in (False, True):
+        drivers.chain_step(ctx, S[m], DT_LARGE, DT_SMALL)
+        mass = drivers.total_mass(ctx, S[m]["rhogq"], S[m]["rhogq_pl"], gzh)
+        out["mixed" if m else "f64"] = dict(mass_rel_change=abs(mass - m0) / abs(m0))
+    torch.cuda.synchronize()
+    rho = torch.from_numpy(np.ascontiguousarray(case["flux"]["rho"])).to(S[False]["rhogq"].device)
+    inner = torch.zeros(spec.gall_1d, spec.gall_1d, dtype=torch.bool, device=rho.device)
+    inner[1:-1, 1:-1] = True
+    inner = inner.reshape(-1)
+    pairs = dict(flx_h=("flx_h", None), GRD_xc=("GRD_xc", None), q_a=("h_q_a", None), Qout_prev=("Qout_prev", None),
+                 q_h=("v_q_h", None), q=("rhogq", rho), rhogw=("rhogw", None))
+    errors, share = {}, {}
+    for name, (key, div) in pairs.items():
+        a, b = S[True][key].double(), S[False][key]
+        if div is not None:  # the updated mixing ratio on the inner cells
+            a, b = (a / div)[..., inner], (b / div)[..., inner]
+        scale = float(b.abs().max())
+        err = (a - b).abs() / scale
+        errors[name] = float(err.max())
+        share[name] = float((err > 1e-5).double().mean())
+    mass_part = torch.zeros(spec.kall, dtype=torch.float64, device=rho.device)
+
+    def step(m):
+        drivers.chain_step(ctx, S[m], DT_LARGE, DT_SMALL)
+        ctx.tracer_mass(S[m]["rhogq"], S[m]["rhogq_pl"], mass_part)
+
+    for m in (False, True):
+        for _ in range(2):
+            step(m)
+        e = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+        torch.cuda.synchronize()
+        e[0].record(stream)
+        for _ in range(steps):
+            step(m)
+        e[1].record(stream)
+        torch.cuda.synchronize()
+        out["mixed" if m else "f64"]["ms_per_step"] = round(e[0].elapsed_time(e[1]) / steps, 5)
+    out["speedup_mixed_vs_f64"] = round(out["f64"]["ms_per_step"] / out["mixed"]["ms_per_step"], 3)
+    out["rel_error_vs_f64"] = errors
+    out["share_above_1em5"] = share
+    out["steps"] = ("horizontal_flux, horizontal_limiter_thuburn, tracer_update_conservative, "
+                    "vertical_limiter_thuburn, tracer_update_vertical, vi_rhow_solver, tracer_mass")
+    del S
+    torch.cuda.empty_cache()
+    log(f"chained step: f64 {out['f64']['ms_per_step']} ms, mixed {out['mixed']['ms_per_step']} ms; "
+        f"mass change f64 {out['f64']['mass_rel_change']:.2e}, mixed {out['mixed']['mass_rel_change']:.2e}; "
+        f"q error {errors['q']:.2e}, rhogw error {errors['rhogw']:.2e}")
+    return out
+
+
+def time_mixed(ctx, spec, case, stream, steps):
+    """The mixed-precision step: the FP32 flux and limiters (with FP64's
+    CONST_EPS in the limiters' switches) with the mixed rho*w solve (FP32
+    state and sources, FP64 elimination), on the same inputs rounded to FP32;
+    per-kernel times and algorithmic bytes.  The errors against FP64 come from
+    the chained step (time_chained)."""
     import numpy as np
     import torch
 
@@ -248,14 +319,13 @@ def time_mixed(ctx, spec, case, stream, steps, fp64):
     def z(shape):
         return torch.zeros(shape, dtype=torch.float32, device=dev)
 
+    ctx.set_const_f32(EPS64, FLT_MAX)
     F = up32(case["flux"], drivers.FLUX_IN)
     F.update(flx_h=z((6,) + spec.shape3), flx_h_pl=z(spec.shape_pl), GRD_xc=z((3, 3) + spec.shape3),
              GRD_xc_pl=z((3,) + spec.shape_pl))
     H = up32(case["hlim"], drivers.HLIM_IN + ("q_a", "q_a_pl"))
-    qa0 = H["q_a"].clone()
     H.update(Qout_prev=z((2,) + spec.shape3), Qout_prev_pl=z((2,) + spec.shape_pl))
     V = up32(case["vlim"], drivers.VLIM_IN + ("q_h", "q_h_pl"))
-    qh0 = V["q_h"].clone()
 
     def k_flux():
         ctx.horizontal_flux_f32(F["flx_h"], F["flx_h_pl"], F["GRD_xc"], F["GRD_xc_pl"],
@@ -269,22 +339,6 @@ def time_mixed(ctx, spec, case, stream, steps, fp64):
     def k_vlim():
         ctx.vertical_limiter_thuburn_f32(V["q_h"], V["q_h_pl"], V["q"], V["q_pl"], V["d"], V["d_pl"], V["ck"],
                                          V["ck_pl"])
-
-    # errors of one step from the original inputs, against the FP64 step's outputs
-    k_flux()
-    k_hlim()
-    k_vlim()
-    torch.cuda.synchronize()
-    errors, share = {}, {}
-    for name, got in (("flx_h", F["flx_h"]), ("GRD_xc", F["GRD_xc"]), ("q_a", H["q_a"]),
-                      ("Qout_prev", H["Qout_prev"]), ("q_h", V["q_h"])):
-        want = fp64[name]
-        scale = float(want.abs().max())
-        err = (got.double() - want).abs() / scale if scale > 0 else (got.double() - want).abs()
-        errors[name] = float(err.max())
-        share[name] = float((err > 1e-5).double().mean())  # where a limiter switch flips in FP32
-    H["q_a"].copy_(qa0)
-    V["q_h"].copy_(qh0)
 
     # the solve with FP32 state and sources, FP64 rho*w and elimination
     vkeys = list(drivers.VI_IN)
@@ -326,7 +380,7 @@ def time_mixed(ctx, spec, case, stream, steps, fp64):
     return dict(dtype="f32 flux, limiters and VI sources; f64 rho*w and tridiagonal elimination", ms_per_step=ms_step,
                 value=spec.inner_cell_levels() / (ms_step * 1e-3), unit="cell-levels/s",
                 hbm_frac_step=round(step_gbs / perfmodel.HBM_PEAK_GBS, 4), step_alg_bytes=sum(kb.values()),
-                kernels=kernels, rel_error_vs_f64=errors, share_above_1em5=share)
+                kernels=kernels, eps="FP64 CONST_EPS (2^-52) in the FP32 limiters")
 
 
 def time_adjoint(ctx, spec, case, stream, steps, fwd_ms):
@@ -435,92 +489,97 @@ def time_adjoint(ctx, spec, case, stream, steps, fwd_ms):
                     (adj + sum(masked.values()) - sum(fwd_ms[k] for k in masked)) / fwd, 3))
 
 
-def time_batched(ctx, spec, case, stream, steps, ntr: int = 32):
-    """A step advancing ntr tracers: the flux and the rho*w solve once, the
-    two limiters once per tracer (shared ch, cmask, ck, d; each tracer its own
-    q, q_a, q_h, Qout), timed eagerly and as one captured HIP graph.  Each
-    tracer's fields are the base fields shifted along the cells, made on the
-    device."""
-    import numpy as np
+BATCH_GRIDS = ((5, 98), (6, 78), (7, 58), (8, 42))  # (glevel, kall) of the four rlevel-1 grids
+
+
+def time_batched(geo7, stream, steps, ntr: int = 32):
+    """ntr tracers on four grids of uneven size (glevel 5..8, rlevel 1, kall
+    98/78/58/42), FP32 flux and limiters with FP64's CONST_EPS, the mixed
+    rho*w solve: the group step (four launches over all grids and tracers,
+    launch shapes from the autotuner) timed eagerly and as its captured HIP
+    graph, against the same work as per-call launches (one grid and one
+    tracer per call, 4 x (2 + 2 ntr) launches).  Each tracer's fields are the
+    base fields shifted along the cells."""
+    import tempfile
+
     import torch
 
-    from icoatmosbenchmark_v1_amd import drivers, perfmodel
+    from icoatmosbenchmark_v1_amd import drivers, perfmodel, synth
+    from icoatmosbenchmark_v1_amd.group import Group, batched_state
 
-    dev = torch.device("cuda:0")
-
-    def up(d, keys):
-        return {k: torch.from_numpy(np.ascontiguousarray(d[k])).to(dev) for k in keys}
-
-    F = up(case["flux"], drivers.FLUX_IN)
-    F.update(flx_h=torch.zeros((6,) + spec.shape3, dtype=torch.float64, device=dev),
-             flx_h_pl=torch.zeros(spec.shape_pl, dtype=torch.float64, device=dev),
-             GRD_xc=torch.zeros((3, 3) + spec.shape3, dtype=torch.float64, device=dev),
-             GRD_xc_pl=torch.zeros((3,) + spec.shape_pl, dtype=torch.float64, device=dev))
-    W = up(case["vi"], [k for k in drivers.VI_IN] + [k + "_pl" for k in drivers.VI_IN] + ["rhogw", "rhogw_pl"])
-    H0 = up(case["hlim"], drivers.HLIM_IN + ("q_a", "q_a_pl"))
-    V0 = up(case["vlim"], drivers.VLIM_IN + ("q_h", "q_h_pl"))
-    Ht, Vt = [], []
-    for t in range(ntr):
-        sh = 7 * t
-        Ht.append(dict(q=torch.roll(H0["q"], sh, -1).contiguous(), q_a=torch.roll(H0["q_a"], sh, -1).contiguous(),
-                       Qp=torch.zeros((2,) + spec.shape3, dtype=torch.float64, device=dev),
-                       Qp_pl=torch.zeros((2,) + spec.shape_pl, dtype=torch.float64, device=dev),
-                       q_a_pl=H0["q_a_pl"].clone()))
-        Vt.append(dict(q=torch.roll(V0["q"], sh, -1).contiguous(), q_h=torch.roll(V0["q_h"], sh, -1).contiguous(),
-                       q_h_pl=V0["q_h_pl"].clone()))
+    t0 = time.perf_counter()
+    specs, ctxs, states = [], [], []
+    for gl, kall in BATCH_GRIDS:
+        geo = geo7 if gl == 7 else synth.icosahedral_geometry(gl, 1)[0]
+        spec = synth.GridSpec(2 ** (gl - 1) + 2, kall, 40, seed=100 + gl)
+        case = synth.make_case(spec, geo=geo)
+        c = drivers.setup_context(spec, geo, case["vi"])
+        c.set_stream(stream)
+        c.set_const_f32(EPS64, FLT_MAX)
+        specs.append(spec)
+        ctxs.append(c)
+        states.append(batched_state(spec, case, ntr))
+        del case
     torch.cuda.synchronize()
+    log(f"batched: {len(specs)} grids x {ntr} tracers set up in {time.perf_counter() - t0:.1f} s, "
+        f"device memory {torch.cuda.memory_allocated() / 1e9:.1f} GB")
+    grp = Group(ctxs)
+    grp.set_stream(stream)
+    with tempfile.TemporaryDirectory() as tmp:
+        n_timed = grp.autotune(ntr, states, DT_LARGE, DT_SMALL, cache_path=f"{tmp}/tune.txt")
+        tuned = open(f"{tmp}/tune.txt").read().strip().splitlines()
 
-    def step():
-        ctx.horizontal_flux(F["flx_h"], F["flx_h_pl"], F["GRD_xc"], F["GRD_xc_pl"], *[F[k] for k in drivers.FLUX_IN],
-                            DT_LARGE)
-        for h in Ht:
-            ctx.horizontal_limiter_thuburn(h["q_a"], h["q_a_pl"], h["q"], H0["q_pl"], H0["d"], H0["d_pl"], H0["ch"],
-                                           H0["ch_pl"], H0["cmask"], H0["cmask_pl"], h["Qp"], h["Qp_pl"],
-                                           H0["Qout_post"], H0["Qout_post_pl"])
-        for v in Vt:
-            ctx.vertical_limiter_thuburn(v["q_h"], v["q_h_pl"], v["q"], V0["q_pl"], V0["d"], V0["d_pl"], V0["ck"],
-                                         V0["ck_pl"])
-        a = []
-        for k in drivers.VI_IN:
-            a += [W[k], W[k + "_pl"]]
-        ctx.vi_rhow_solver(W["rhogw"], W["rhogw_pl"], *a, DT_SMALL)
-
-    def timed(fn, s):
+    def timed(fn):
         for _ in range(2):
             fn()
         e = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
-        e[0].record(s)
+        torch.cuda.synchronize()
+        e[0].record(stream)
         for _ in range(steps):
             fn()
-        e[1].record(s)
+        e[1].record(stream)
         torch.cuda.synchronize()
         return e[0].elapsed_time(e[1]) / steps
 
-    eager = timed(step, stream)
-    ref = [h["q_a"].clone() for h in Ht[:2]] + [v["q_h"].clone() for v in Vt[:2]]
-    g = torch.cuda.CUDAGraph()
-    side = torch.cuda.Stream()
-    side.wait_stream(torch.cuda.current_stream())
-    with torch.cuda.stream(side):
-        ctx.set_stream(side)
-        step()  # warm-up on the capture stream
-        torch.cuda.synchronize()
-        with torch.cuda.graph(g, stream=side):
-            step()
-    torch.cuda.synchronize()
-    ctx.set_stream(stream)
-    graph = timed(g.replay, stream)
-    # the limiters are idempotent at fixed bounds: every replay must reproduce the eager faces
-    same = all(torch.equal(a, b) for a, b in zip(ref, [h["q_a"] for h in Ht[:2]] + [v["q_h"] for v in Vt[:2]]))
-    kb = perfmodel.kernel_bytes(spec)
-    alg = kb["horizontal_flux"] + kb["vi_rhow_solver"] + ntr * (kb["horizontal_limiter_thuburn"] +
-                                                                 kb["vertical_limiter_thuburn"])
-    inner = spec.inner_cell_levels()
-    del Ht, Vt, g
-    return dict(tracers=ntr, eager_ms=round(eager, 4), graph_ms=round(graph, 4),
-                value=ntr * inner / (graph * 1e-3), unit="tracer cell-levels/s", alg_bytes=alg,
+    eager = timed(lambda: grp.tracer_step_f32(ntr, states, DT_LARGE, DT_SMALL, use_graph=False))
+    ref = [S[k].clone() for S in states for k in ("q_a", "q_h")]
+    graph = timed(lambda: grp.tracer_step_f32(ntr, states, DT_LARGE, DT_SMALL, use_graph=True))
+    # the limiters are idempotent at fixed bounds: the replays reproduce the eager faces
+    same = all(torch.equal(a, S[k]) for a, (S, k) in zip(ref, [(S, k) for S in states for k in ("q_a", "q_h")]))
+
+    def per_call():
+        for c, S in zip(ctxs, states):
+            c.horizontal_flux_f32(S["flx_h"], S["flx_h_pl"], S["GRD_xc"], S["GRD_xc_pl"], S["rho"], S["rho_pl"],
+                                  S["rhovx"], S["rhovx_pl"], S["rhovy"], S["rhovy_pl"], S["rhovz"], S["rhovz_pl"],
+                                  DT_LARGE)
+            for t in range(ntr):
+                c.horizontal_limiter_thuburn_f32(S["q_a"][t], S["q_a_pl"][t], S["q"][t], S["q_pl"][t], S["d"],
+                                                 S["d_pl"], S["ch"], S["ch_pl"], S["cmask"], S["cmask_pl"],
+                                                 S["Qout_prev"][t], S["Qout_prev_pl"][t], S["Qout_post"][t],
+                                                 S["Qout_post_pl"][t])
+                c.vertical_limiter_thuburn_f32(S["q_h"][t], S["q_h_pl"][t], S["qv"][t], S["qv_pl"][t], S["dv"],
+                                               S["dv_pl"], S["ck"], S["ck_pl"])
+            a = []
+            for k in drivers.VI_IN:
+                a += [S[k], S[k + "_pl"]]
+            c.vi_rhow_solver_mixed(S["rhogw"], S["rhogw_pl"], *a, DT_SMALL)
+
+    calls = timed(per_call)
+    alg = sum(sum(perfmodel.batched_bytes(sp, ntr).values()) for sp in specs)
+    work = sum(ntr * sp.inner_cell_levels() for sp in specs)
+    grp.close()
+    for c in ctxs:
+        c.close()
+    del states
+    torch.cuda.empty_cache()
+    return dict(tracers=ntr, grids=[dict(glevel=gl, rlevel=1, gall_1d=sp.gall_1d, kall=sp.kall, lall=sp.lall)
+                                    for (gl, _), sp in zip(BATCH_GRIDS, specs)],
+                dtype="f32 flux and limiters (FP64 CONST_EPS), mixed rho*w solve",
+                graph_ms=round(graph, 4), eager_ms=round(eager, 4), per_call_ms=round(calls, 4),
+                launches_per_step=4, per_call_launches=len(specs) * (2 + 2 * ntr),
+                value=work / (graph * 1e-3), unit="tracer cell-levels/s", alg_bytes=alg,
                 hbm_frac=round(alg / (graph * 1e-3) / 1e9 / perfmodel.HBM_PEAK_GBS, 4),
-                graph_matches_eager=bool(same), launches_per_step=2 + 2 * ntr)
+                graph_matches_eager=bool(same), autotuner=dict(candidates_timed=n_timed, choices=tuned))
 
 
 def main() -> int:
@@ -641,23 +700,18 @@ def main() -> int:
 
     extras = None if args.no_extras else time_extras(ctx, spec, case, coef, stream, args.steps)
 
-    mixed = None
+    mixed = chained = None
     if not args.no_mixed:
-        # the FP64 outputs of one step from the original inputs, for the error figures
-        H["q_a"].copy_(up(case["hlim"], ("q_a",))["q_a"])
-        V["q_h"].copy_(up(case["vlim"], ("q_h",))["q_h"])
-        k_flux(ctx)
-        k_hlim(ctx)
-        k_vlim(ctx)
-        torch.cuda.synchronize()
-        fp64 = dict(flx_h=F["flx_h"], GRD_xc=F["GRD_xc"], q_a=H["q_a"], Qout_prev=H["Qout_prev"], q_h=V["q_h"])
-        mixed = time_mixed(ctx, spec, case, stream, args.steps, fp64)
+        mixed = time_mixed(ctx, spec, case, stream, args.steps)
         mixed["speedup_vs_f64"] = round(ms_step / mixed["ms_per_step"], 3)
+        chained = time_chained(ctx, spec, case, stream, max(3, args.steps // 2))
 
     batched = None
     if not args.no_batched:
-        batched = time_batched(ctx, spec, case, stream, max(3, args.steps // 4))
-        log(f"batched {batched['tracers']} tracers: eager {batched['eager_ms']} ms, graph {batched['graph_ms']} ms")
+        batched = time_batched(geo, stream, max(3, args.steps // 4))
+        log(f"batched {batched['tracers']} tracers on {len(batched['grids'])} grids: graph {batched['graph_ms']} ms, "
+            f"eager {batched['eager_ms']} ms, per-call {batched['per_call_ms']} ms, "
+            f"{batched['hbm_frac']:.3f} of 8 TB/s")
 
     adjoint = None
     if not args.no_adjoint:
@@ -699,6 +753,7 @@ def main() -> int:
         kernels=kernels,
         extras=extras,
         mixed=mixed,
+        chained=chained,
         roofline=roof,
         cpu_baseline=cpu,
         host=platform.node(),

@@ -283,6 +283,90 @@ int build_halo_map(int rlevel, int gall_1d, int gall_pl, std::vector<HaloPoint> 
     return NICAM_OK;
 }
 
+// The shared faces of neighbouring regions.  Two regions sharing an edge each
+// compute the flux through every arc of that edge from their own copies of the
+// inputs and metrics, so the two values agree only to rounding.  For an update
+// that conserves mass exactly, each such face is given one value: the one of
+// the region with the lower index (regions before the poles).  For every
+// inner cell whose neighbour through face m (1..6) is a halo point, the partner
+// is the cell that halo point copies, and its face whose neighbour is a halo
+// point copied from this cell.  Codes in tab (6 per edge cell): -1 = the
+// region's own value, -2 = no flux (face 6 of a singular-point cell, which the
+// flux sets to zero), else (l' * gall + g') * 8 + m' = minus the partner's
+// outward transfer.  pole_tab[p * (gall_pl - 1) + v - 1]: the partner of the
+// pole centre's face toward ring point v, coded the same way.
+int build_arc_partners(int rlevel, int gall_1d, int gall_pl, const std::vector<HaloPoint> &dst,
+                       const std::vector<HaloPoint> &src, const std::vector<int> &have_sgp, std::vector<int> &slot,
+                       std::vector<int> &tab, std::vector<int> &pole_tab)
+{
+    const int gall = gall_1d * gall_1d, lall = 10 << (2 * rlevel), npl = gall_pl - 1;
+    const int lo = 1, hi = gall_1d - 2;  // inner range, 0-based
+    std::map<HaloPoint, HaloPoint> srcof;
+    for (size_t e = 0; e < dst.size(); ++e) srcof[dst[e]] = src[e];
+    const int di[7] = {0, 1, 1, 0, -1, -1, 0}, dj[7] = {0, 0, 1, 1, 0, -1, -1};
+    auto inner = [&](int i, int j) { return i >= lo && i <= hi && j >= lo && j <= hi; };
+    auto zero_face = [&](int l, int g, int m) { return m == 6 && have_sgp[l] && g == lo * gall_1d + lo; };
+    // the face of region cell (l, g) whose neighbour is a halo point copied from `from`
+    auto face_toward = [&](int l, int g, const HaloPoint &from, int &m_out) -> int {
+        const int i = g % gall_1d, j = g / gall_1d;
+        int found = 0;
+        for (int m = 1; m <= 6; ++m) {
+            const int ni = i + di[m], nj = j + dj[m];
+            if (inner(ni, nj) || zero_face(l, g, m)) continue;
+            auto it = srcof.find(HaloPoint{0, l, nj * gall_1d + ni});
+            if (it == srcof.end()) continue;
+            const HaloPoint &s = it->second;
+            if (s.kind == from.kind && s.l == from.l && s.g == from.g) {
+                m_out = m;
+                ++found;
+            }
+        }
+        return found;
+    };
+    slot.assign((size_t)lall * gall, -1);
+    tab.clear();
+    for (int l = 0; l < lall; ++l)
+        for (int j = lo; j <= hi; ++j)
+            for (int i = lo; i <= hi; ++i) {
+                if (i != lo && i != hi && j != lo && j != hi) continue;
+                const int g = j * gall_1d + i;
+                int codes[6];
+                for (int m = 1; m <= 6; ++m) {
+                    const int ni = i + di[m], nj = j + dj[m];
+                    codes[m - 1] = -1;
+                    if (inner(ni, nj)) continue;
+                    if (zero_face(l, g, m)) {
+                        codes[m - 1] = -2;
+                        continue;
+                    }
+                    auto it = srcof.find(HaloPoint{0, l, nj * gall_1d + ni});
+                    if (it == srcof.end()) return fail(NICAM_ERR_ARG, "arc partners: a halo neighbour has no source");
+                    const HaloPoint &s = it->second;
+                    if (s.kind != 0 || s.l > l) continue;  // this region's value is the one both sides use
+                    if (s.l == l) return fail(NICAM_ERR_ARG, "arc partners: a region borders itself");
+                    int mp = 0;
+                    if (face_toward(s.l, s.g, HaloPoint{0, l, g}, mp) != 1)
+                        return fail(NICAM_ERR_ARG, "arc partners: no unique partner face");
+                    codes[m - 1] = (s.l * gall + s.g) * 8 + mp;
+                }
+                slot[(size_t)l * gall + g] = (int)(tab.size() / 6);
+                tab.insert(tab.end(), codes, codes + 6);
+            }
+    pole_tab.assign(2 * npl, -1);
+    for (int p = 0; p < 2; ++p)
+        for (int v = 1; v <= npl; ++v) {
+            auto it = srcof.find(HaloPoint{1, p, v});
+            if (it == srcof.end() || it->second.kind != 0)
+                return fail(NICAM_ERR_ARG, "arc partners: a pole ring point has no region source");
+            const HaloPoint &s = it->second;
+            int mp = 0;
+            if (face_toward(s.l, s.g, HaloPoint{1, p, 0}, mp) != 1)
+                return fail(NICAM_ERR_ARG, "arc partners: no unique face toward the pole");
+            pole_tab[p * npl + v - 1] = (s.l * gall + s.g) * 8 + mp;
+        }
+    return NICAM_OK;
+}
+
 }  // namespace nicam
 
 using namespace nicam;

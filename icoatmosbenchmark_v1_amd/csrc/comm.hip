@@ -97,6 +97,18 @@ int comm_setup(nicam_ctx c, int rlevel)
     NICAM_HIP(hipMemcpy(c->halo_src, hs.data(), sizeof(int) * hs.size(), hipMemcpyHostToDevice));
     c->halo_n = (int)dst.size();
     c->rlevel = rlevel;
+
+    std::vector<int> slot, tab, pole_tab;
+    rc = build_arc_partners(rlevel, D.gall_1d, D.gall_pl, dst, src, sgp, slot, tab, pole_tab);
+    if (rc != NICAM_OK) return rc;
+    int **arr[] = {&c->arc_slot, &c->arc_tab, &c->arc_pole};
+    const std::vector<int> *vec[] = {&slot, &tab, &pole_tab};
+    for (int a = 0; a < 3; ++a) {
+        if (*arr[a]) (void)hipFree(*arr[a]);
+        *arr[a] = nullptr;
+        NICAM_HIP(hipMalloc(arr[a], sizeof(int) * std::max<size_t>(1, vec[a]->size())));
+        NICAM_HIP(hipMemcpy(*arr[a], vec[a]->data(), sizeof(int) * vec[a]->size(), hipMemcpyHostToDevice));
+    }
     return NICAM_OK;
 }
 

@@ -9,6 +9,8 @@
 #include <string>
 #include <vector>
 
+#include <type_traits>
+
 #include "internal.h"
 
 namespace nicam {
@@ -182,7 +184,8 @@ int nicam_ctx_destroy(nicam_ctx c)
                       &c->GMTR_t_pl, &c->GMTR_a, &c->GMTR_a_pl, &c->RGAM, &c->RGAM_pl,
                       &c->RGAMH, &c->RGAMH_pl, &c->RGSGAM2, &c->RGSGAM2_pl, &c->RGSGAM2H,
                       &c->RGSGAM2H_pl, &c->GSGAM2H, &c->GSGAM2H_pl, &c->rdgzh, &c->afact,
-                      &c->bfact, &c->Mc, &c->Mc_pl, &c->Ml, &c->Ml_pl, &c->Mu, &c->Mu_pl, &c->vi_gamma,
+                      &c->bfact, &c->Mc, &c->Mc_pl, &c->Ml, &c->Ml_pl, &c->Mu, &c->Mu_pl, &c->vi_beta,
+                      &c->vi_beta_pl, &c->vi_gam, &c->vi_gam_pl,
                       &c->RGSQRTH, &c->RGSQRTH_pl, &c->C2WfactGz, &c->C2WfactGz_pl, &c->rdgz,
                       &c->cfact, &c->dfact};
     for (double **p : all) dev_free(*p);
@@ -196,6 +199,9 @@ int nicam_ctx_destroy(nicam_ctx c)
     if (c->have_sgp) (void)hipFree(c->have_sgp);
     if (c->halo_dst) (void)hipFree(c->halo_dst);
     if (c->halo_src) (void)hipFree(c->halo_src);
+    for (int *p : {c->arc_slot, c->arc_tab, c->arc_pole})
+        if (p) (void)hipFree(p);
+    if (c->mass_part) (void)hipFree(c->mass_part);
     delete c;
     return NICAM_OK;
 }
@@ -226,7 +232,7 @@ int nicam_ctx_set_tuning(nicam_ctx c, int param, int value)
     case NICAM_TUNE_DIFF_KPER: c->tune.diff_kper = value; break;
     case NICAM_TUNE_DDMP_KPER: c->tune.ddmp_kper = value; break;
     case NICAM_TUNE_VI_VARIANT:
-        if (value > 3) return fail(NICAM_ERR_ARG, "NICAM_TUNE_VI_VARIANT must be 0..3");
+        if (value > 4) return fail(NICAM_ERR_ARG, "NICAM_TUNE_VI_VARIANT must be 0..4");
         c->tune.vi_variant = value;
         break;
     default: return fail(NICAM_ERR_ARG, "unknown tuning parameter");
@@ -250,6 +256,15 @@ int nicam_set_const(nicam_ctx c, double eps, double huge_val)
     if (!(eps >= 0.0) || !(huge_val > 0.0)) return fail(NICAM_ERR_ARG, "eps must be >= 0 and huge > 0");
     c->eps = eps;
     c->big = huge_val;
+    return NICAM_OK;
+}
+
+int nicam_set_const_f32(nicam_ctx c, float eps, float huge_val)
+{
+    NICAM_TRY(check_ctx(c));
+    if (!(eps >= 0.0f) || !(huge_val > 0.0f)) return fail(NICAM_ERR_ARG, "eps must be >= 0 and huge > 0");
+    c->eps32 = eps;
+    c->big32 = huge_val;
     return NICAM_OK;
 }
 
@@ -522,6 +537,19 @@ int nicam_set_vmetrics3d(nicam_ctx c, const double *RGSQRTH, const double *RGSQR
     return NICAM_OK;
 }
 
+// Allocates the factor arrays (set-up only; the solves allocate nothing) and
+// factors the coefficients just installed.
+static int factor_coef(nicam_ctx c)
+{
+    NICAM_TRY(dev_alloc(c, &c->vi_beta, c->d.nreg()));
+    NICAM_TRY(dev_alloc(c, &c->vi_gam, c->d.nreg()));
+    if (c->d.have_pl) {
+        NICAM_TRY(dev_alloc(c, &c->vi_beta_pl, c->d.npl()));
+        NICAM_TRY(dev_alloc(c, &c->vi_gam_pl, c->d.npl()));
+    }
+    return launch_vi_factor(c);
+}
+
 int nicam_vi_rhow_solver_putcoef(nicam_ctx c, const double *Mc, const double *Mc_pl, const double *Ml,
                                  const double *Ml_pl, const double *Mu, const double *Mu_pl)
 {
@@ -529,6 +557,7 @@ int nicam_vi_rhow_solver_putcoef(nicam_ctx c, const double *Mc, const double *Mc
     NICAM_TRY(set_pair(c, &c->Mc, &c->Mc_pl, Mc, Mc_pl, "Mc"));
     NICAM_TRY(set_pair(c, &c->Ml, &c->Ml_pl, Ml, Ml_pl, "Ml"));
     NICAM_TRY(set_pair(c, &c->Mu, &c->Mu_pl, Mu, Mu_pl, "Mu"));
+    NICAM_TRY(factor_coef(c));
     NICAM_HIP(hipStreamSynchronize(c->stream));
     c->have_coef = true;
     return NICAM_OK;
@@ -1230,6 +1259,31 @@ int nicam_tracer_update_vertical(nicam_ctx c, double *rhogq, double *rhogq_pl, c
     return stage_out(c, a, v);
 }
 
+int nicam_tracer_update_vertical_f32(nicam_ctx c, double *rhogq, double *rhogq_pl, const float *flx_v,
+                                     const float *flx_v_pl, const float *q_h, const float *q_h_pl)
+{
+    NICAM_TRY(check_ctx(c));
+    if (!c->have_rdgz) return fail(NICAM_ERR_STATE, "GRD_rdgz not set (nicam_grd_setup or nicam_set_rdgz)");
+    const void *req[] = {rhogq, flx_v, q_h};
+    const char *nm[] = {"rhogq", "flx_v", "q_h"};
+    for (int i = 0; i < 3; ++i) NICAM_TRY(need(req[i], nm[i]));
+    if (c->d.have_pl) {
+        const void *rp[] = {rhogq_pl, flx_v_pl, q_h_pl};
+        for (int i = 0; i < 3; ++i) NICAM_TRY(need(rp[i], "pole array"));
+    }
+    if (c->ptr_mode == NICAM_PTR_DEVICE)
+        return launch_vertical_update_f32(c, rhogq, PL(rhogq_pl), flx_v, PL(flx_v_pl), q_h, PL(q_h_pl));
+    const size_t R = c->d.nreg(), P = c->d.npl();
+    std::vector<HostArg> a = {{rhogq, R * 8, true},       {PL(rhogq_pl), P * 8, true}, {flx_v, R * 4, false},
+                              {PL(flx_v_pl), P * 4, false}, {q_h, R * 4, false},      {PL(q_h_pl), P * 4, false}};
+    std::vector<void *> v;
+    NICAM_TRY(stage_in(c, a, v));
+    NICAM_TRY(launch_vertical_update_f32(c, static_cast<double *>(v[0]), static_cast<double *>(v[1]),
+                                         static_cast<const float *>(v[2]), static_cast<const float *>(v[3]),
+                                         static_cast<const float *>(v[4]), static_cast<const float *>(v[5])));
+    return stage_out(c, a, v);
+}
+
 int nicam_vi_rhow_update_matrix(nicam_ctx c, const double *eth, const double *eth_pl, const double *g_tilde,
                                 const double *g_tilde_pl, double dt)
 {
@@ -1265,6 +1319,7 @@ int nicam_vi_rhow_update_matrix(nicam_ctx c, const double *eth, const double *et
         NICAM_TRY(launch_vi_matrix(c, D(0), D(1), D(2), D(3), dt));
         NICAM_TRY(stage_out(c, a, v));
     }
+    NICAM_TRY(factor_coef(c));
     NICAM_HIP(hipStreamSynchronize(c->stream));
     c->have_coef = true;
     return NICAM_OK;
@@ -1382,5 +1437,88 @@ int nicam_vertical_limiter_thuburn_f32(nicam_ctx c, float *q_h, float *q_h_pl, c
 }
 
 #undef PL
+
+}  // extern "C"
+
+// ---- FP64 global tracer-mass sums and the conservative update ---------------
+
+template <class T>
+static int tracer_mass_t(nicam_ctx c, const T *rhogq, const T *rhogq_pl, double *mass)
+{
+    NICAM_TRY(check_ctx(c));
+    if (!c->have_hmetrics) return fail(NICAM_ERR_STATE, "horizontal metrics not set (nicam_set_hmetrics)");
+    NICAM_TRY(need(rhogq, "rhogq"));
+    NICAM_TRY(need(mass, "mass"));
+    if (c->d.have_pl) NICAM_TRY(need(rhogq_pl, "pole array"));
+    const size_t np = (size_t)c->d.kall * c->d.lall;
+    if (!c->mass_part) {  // the partial sums: allocated on the first call, kept by the context
+        NICAM_HIP(hipMalloc(&c->mass_part, np * sizeof(double)));
+        c->device_bytes += np * sizeof(double);
+    }
+    if (c->ptr_mode == NICAM_PTR_DEVICE) {
+        if constexpr (std::is_same<T, float>::value) return launch_mass_f32(c, rhogq, (c->d.have_pl ? rhogq_pl : nullptr), mass, c->mass_part);
+        else return launch_mass(c, rhogq, (c->d.have_pl ? rhogq_pl : nullptr), mass, c->mass_part);
+    }
+    const size_t R = c->d.nreg() * sizeof(T), P = c->d.npl() * sizeof(T);
+    std::vector<HostArg> a = {{rhogq, R, false}, {(c->d.have_pl ? rhogq_pl : nullptr), P, false}, {mass, c->d.kall * sizeof(double), true}};
+    std::vector<void *> v;
+    NICAM_TRY(stage_in(c, a, v));
+    if constexpr (std::is_same<T, float>::value)
+        NICAM_TRY(launch_mass_f32(c, static_cast<float *>(v[0]), static_cast<float *>(v[1]),
+                                  static_cast<double *>(v[2]), c->mass_part));
+    else
+        NICAM_TRY(launch_mass(c, static_cast<double *>(v[0]), static_cast<double *>(v[1]),
+                              static_cast<double *>(v[2]), c->mass_part));
+    return stage_out(c, a, v);
+}
+
+template <class T>
+static int update_cons_t(nicam_ctx c, double *rhogq, double *rhogq_pl, const T *flx_h, const T *q_a)
+{
+    NICAM_TRY(check_ctx(c));
+    if (!c->have_hmetrics) return fail(NICAM_ERR_STATE, "horizontal metrics not set (nicam_set_hmetrics)");
+    if (!c->arc_slot) return fail(NICAM_ERR_STATE, "the conservative update needs the halo map (nicam_comm_setup)");
+    const void *req[] = {rhogq, flx_h, q_a};
+    const char *nm[] = {"rhogq", "flx_h", "q_a"};
+    for (int i = 0; i < 3; ++i) NICAM_TRY(need(req[i], nm[i]));
+    if (c->d.have_pl) NICAM_TRY(need(rhogq_pl, "pole array"));
+    auto launch = [&](double *r, double *rp, const T *f, const T *q) {
+        if constexpr (std::is_same<T, float>::value) return launch_update_cons_f32(c, r, rp, f, q);
+        else return launch_update_cons(c, r, rp, f, q);
+    };
+    if (c->ptr_mode == NICAM_PTR_DEVICE) return launch(rhogq, (c->d.have_pl ? rhogq_pl : nullptr), flx_h, q_a);
+    const size_t R = c->d.nreg(), P = c->d.npl();
+    std::vector<HostArg> a = {{rhogq, R * 8, true}, {(c->d.have_pl ? rhogq_pl : nullptr), P * 8, true}, {flx_h, 6 * R * sizeof(T), false},
+                              {q_a, 6 * R * sizeof(T), false}};
+    std::vector<void *> v;
+    NICAM_TRY(stage_in(c, a, v));
+    NICAM_TRY(launch(static_cast<double *>(v[0]), static_cast<double *>(v[1]), static_cast<const T *>(v[2]),
+                     static_cast<const T *>(v[3])));
+    return stage_out(c, a, v);
+}
+
+extern "C" {
+
+int nicam_tracer_mass(nicam_ctx c, const double *rhogq, const double *rhogq_pl, double *mass)
+{
+    return tracer_mass_t<double>(c, rhogq, rhogq_pl, mass);
+}
+
+int nicam_tracer_mass_f32(nicam_ctx c, const float *rhogq, const float *rhogq_pl, double *mass)
+{
+    return tracer_mass_t<float>(c, rhogq, rhogq_pl, mass);
+}
+
+int nicam_tracer_update_conservative(nicam_ctx c, double *rhogq, double *rhogq_pl, const double *flx_h,
+                                     const double *q_a)
+{
+    return update_cons_t<double>(c, rhogq, rhogq_pl, flx_h, q_a);
+}
+
+int nicam_tracer_update_conservative_f32(nicam_ctx c, double *rhogq, double *rhogq_pl, const float *flx_h,
+                                         const float *q_a)
+{
+    return update_cons_t<float>(c, rhogq, rhogq_pl, flx_h, q_a);
+}
 
 }  // extern "C"

@@ -97,8 +97,10 @@ __device__ void hflux_pole(const HfluxArgs<T> &A, int t)
     }
 }
 
+// Work-group b of one grid's launch (the grouped kernel of several grids
+// passes the index within the grid).
 template <int NST, class T>
-__global__ void __launch_bounds__(CHUNK) hflux_kernel(HfluxArgs<T> A)
+__device__ __forceinline__ void hflux_block(const HfluxArgs<T> &A, int b)
 {
     extern __shared__ __align__(16) unsigned char lds_raw[];
     T *lds = reinterpret_cast<T *>(lds_raw);
@@ -114,7 +116,6 @@ __global__ void __launch_bounds__(CHUNK) hflux_kernel(HfluxArgs<T> A)
     // Logical block -> (region, level range, chunk); the chunks of one region
     // and level range are consecutive and are kept on one XCD (hardware
     // dispatch sends block b to XCD b % 8) so their shared halo rows hit L2.
-    int b = blockIdx.x;
     if (b >= A.nmain) {  // pole work-groups
         hflux_pole(A, (b - A.nmain) * CHUNK + threadIdx.x);
         return;
@@ -323,14 +324,38 @@ __global__ void __launch_bounds__(CHUNK) hflux_kernel(HfluxArgs<T> A)
 #undef XR
 }
 
+template <int NST, class T>
+__global__ void __launch_bounds__(CHUNK) hflux_kernel(HfluxArgs<T> A)
+{
+    hflux_block<NST, T>(A, blockIdx.x);
+}
+
+// several grids in one launch: work-groups [off[n], off[n+1]) belong to grid n
 template <class T>
-int launch_hflux_t(nicam_ctx c, T *flx_h, T *flx_h_pl, T *xc, T *xc_pl, const T *rho, const T *rho_pl,
-                   const T *rvx, const T *rvx_pl, const T *rvy, const T *rvy_pl, const T *rvz, const T *rvz_pl,
-                   T dt, T eps, const HMetrics<T> &M)
+struct HfluxGroup {
+    HfluxArgs<T> a[kMaxGroup];
+    int off[kMaxGroup + 1];
+    int n;
+};
+
+template <int NST, class T>
+__global__ void __launch_bounds__(CHUNK) hflux_group_kernel(HfluxGroup<T> G)
+{
+    const int b = blockIdx.x;
+    int n = 0;
+    while (n + 1 < G.n && b >= G.off[n + 1]) ++n;
+    hflux_block<NST, T>(G.a[n], b - G.off[n]);
+}
+
+// The launch arguments of one grid: nblk work-groups, lds bytes, nst staged
+// cells per work-item.
+template <class T>
+int hflux_make_args(nicam_ctx c, HfluxArgs<T> &A, long &nblk, size_t &lds, int &nst, T *flx_h, T *flx_h_pl, T *xc,
+                    T *xc_pl, const T *rho, const T *rho_pl, const T *rvx, const T *rvx_pl, const T *rvy,
+                    const T *rvy_pl, const T *rvz, const T *rvz_pl, T dt, T eps, const HMetrics<T> &M)
 {
     const Dims &D = c->d;
     if (D.gall_1d > MAXI) return fail(NICAM_ERR_ARG, "gall_1d larger than the supported 160");
-    HfluxArgs<T> A;
     A.flx = flx_h;
     A.xc = xc;
     A.f[0] = rho;
@@ -372,14 +397,31 @@ int launch_hflux_t(nicam_ctx c, T *flx_h, T *flx_h_pl, T *xc, T *xc_pl, const T 
     A.ga_pl = M.ga_pl;
     A.gall_pl = D.gall_pl;
     const size_t n_t = CHUNK + D.gall_1d + 1;
-    const size_t lds = sizeof(T) * (4 * (CHUNK + 2 * (D.gall_1d + 1)) + 8 * n_t + (NICAM_HFLUX_WT_LDS ? 6 * n_t : 0));
-    const bool two = CHUNK * 2 >= CHUNK + 2 * (D.gall_1d + 1);  // two staged cells per work-item suffice
+    lds = sizeof(T) * (4 * (CHUNK + 2 * (D.gall_1d + 1)) + 8 * n_t + (NICAM_HFLUX_WT_LDS ? 6 * n_t : 0));
+    nst = CHUNK * 2 >= CHUNK + 2 * (D.gall_1d + 1) ? 2 : 3;  // two staged cells per work-item suffice
+    nblk = nmain + npole;
+    return NICAM_OK;
+}
+
+template <class T>
+int launch_hflux_t(nicam_ctx c, T *flx_h, T *flx_h_pl, T *xc, T *xc_pl, const T *rho, const T *rho_pl,
+                   const T *rvx, const T *rvx_pl, const T *rvy, const T *rvy_pl, const T *rvz, const T *rvz_pl,
+                   T dt, T eps, const HMetrics<T> &M)
+{
+    HfluxArgs<T> A;
+    long nblk = 0;
+    size_t lds = 0;
+    int nst = 2;
+    const int rc = hflux_make_args(c, A, nblk, lds, nst, flx_h, flx_h_pl, xc, xc_pl, rho, rho_pl, rvx, rvx_pl, rvy,
+                                   rvy_pl, rvz, rvz_pl, dt, eps, M);
+    if (rc != NICAM_OK) return rc;
+    const bool two = nst == 2;
     const void *fn = two ? (const void *)hflux_kernel<2, T> : (const void *)hflux_kernel<3, T>;
     if (lds > 64 * 1024) NICAM_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     if (two)
-        hipLaunchKernelGGL((hflux_kernel<2, T>), dim3((unsigned)(nmain + npole)), dim3(CHUNK), lds, c->stream, A);
+        hipLaunchKernelGGL((hflux_kernel<2, T>), dim3((unsigned)nblk), dim3(CHUNK), lds, c->stream, A);
     else
-        hipLaunchKernelGGL((hflux_kernel<3, T>), dim3((unsigned)(nmain + npole)), dim3(CHUNK), lds, c->stream, A);
+        hipLaunchKernelGGL((hflux_kernel<3, T>), dim3((unsigned)nblk), dim3(CHUNK), lds, c->stream, A);
     NICAM_HIP(hipGetLastError());
     return NICAM_OK;
 }
@@ -401,9 +443,46 @@ int launch_hflux_f32(nicam_ctx c, float *flx_h, float *flx_h_pl, float *xc, floa
                      const float *rho_pl, const float *rvx, const float *rvx_pl, const float *rvy,
                      const float *rvy_pl, const float *rvz, const float *rvz_pl, float dt)
 {
-    // CONST_EPS of the reference built with -DSINGLE: epsilon(0._SP)
+    // the FP32 CONST_EPS (nicam_set_const_f32; by default epsilon(0._SP), the -DSINGLE build's)
     return launch_hflux_t<float>(c, flx_h, flx_h_pl, xc, xc_pl, rho, rho_pl, rvx, rvx_pl, rvy, rvy_pl, rvz, rvz_pl,
-                                 dt, kEpsF32, c->hm32);
+                                 dt, c->eps32, c->hm32);
+}
+
+}  // namespace nicam
+
+namespace nicam {
+
+int launch_hflux_group_f32(const nicam_ctx *cs, int n, const nicam_tracer_step_f32 *g, float dt, hipStream_t stream)
+{
+    if (n < 1 || n > kMaxGroup) return fail(NICAM_ERR_ARG, "a group holds 1..8 grids");
+    HfluxGroup<float> G;
+    G.n = n;
+    G.off[0] = 0;
+    size_t lds_max = 0;
+    int nst_max = 2;
+    for (int i = 0; i < n; ++i) {
+        const nicam_tracer_step_f32 &p = g[i];
+        long nb = 0;
+        size_t lds = 0;
+        int nst = 2;
+        const int rc = hflux_make_args<float>(cs[i], G.a[i], nb, lds, nst, p.flx_h, p.flx_h_pl, p.GRD_xc, p.GRD_xc_pl,
+                                              p.rho, p.rho_pl, p.rhovx, p.rhovx_pl, p.rhovy, p.rhovy_pl, p.rhovz,
+                                              p.rhovz_pl, dt, cs[i]->eps32, cs[i]->hm32);
+        if (rc != NICAM_OK) return rc;
+        if ((long)G.off[i] + nb > 0x7fffffffL) return fail(NICAM_ERR_ARG, "group too large for one launch");
+        G.off[i + 1] = G.off[i] + (int)nb;
+        lds_max = std::max(lds_max, lds);
+        nst_max = std::max(nst_max, nst);
+    }
+    // the staging ring of the widest grid serves them all (three cells per work-item cover every width)
+    const void *fn = nst_max == 2 ? (const void *)hflux_group_kernel<2, float> : (const void *)hflux_group_kernel<3, float>;
+    if (lds_max > 64 * 1024) NICAM_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_max));
+    if (nst_max == 2)
+        hipLaunchKernelGGL((hflux_group_kernel<2, float>), dim3((unsigned)G.off[n]), dim3(CHUNK), lds_max, stream, G);
+    else
+        hipLaunchKernelGGL((hflux_group_kernel<3, float>), dim3((unsigned)G.off[n]), dim3(CHUNK), lds_max, stream, G);
+    NICAM_HIP(hipGetLastError());
+    return NICAM_OK;
 }
 
 }  // namespace nicam

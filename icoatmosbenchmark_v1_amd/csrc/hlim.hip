@@ -13,6 +13,13 @@
 // post-exchange range Qout_post (phase 2), writing both the arc's own face and
 // the neighbour's mirrored face.  With Qout_post given, the two phases are
 // independent, so one launch does both.
+//
+// Many tracers share d, ch and cmask (NICAM keeps its tracers as the last
+// dimension, q(gall,kall,lall,ntr)).  A work-item loads its cell's share of
+// those fields once, with the tracer-independent sums Cin, Cout and the
+// zerosw switch, and then limits every tracer in turn, restaging the tracer
+// tile in LDS for each.  One tracer is the same kernel with ntr = 1, and
+// every tracer's arithmetic is the single-tracer kernel's, in its order.
 #include "internal.h"
 
 namespace nicam {
@@ -61,6 +68,8 @@ struct HlimArgs {
     const int *have_sgp;
     int iall, gall, kall, lall;
     int phases;  // 1: allowable range only, 2: clamp only, 3: both
+    int ntr;     // tracers; tracer t's fields start t*ts (regular) and t*tps (pole) elements on,
+    long ts, tps;  // times 6 for q_a and 2 for Qout
     int nmain;   // work-groups of the regular regions; the rest do the poles
     T eps, big;
     // pole regions (HLIM:307-363, 367-373, 412-425)
@@ -73,10 +82,15 @@ struct HlimArgs {
 };
 
 template <class T>
-__device__ void hlim_pole(const HlimArgs<T> &A, int t);
+__device__ void hlim_pole(const HlimArgs<T> &A, int t, int tr);
 
-template <class T>
-__global__ void __launch_bounds__(CHUNK) hlim_kernel(HlimArgs<T> A)
+// MULTI: the tracer loop with the shared fields loaded once before it; the
+// single-tracer instance loads them where the tracer's own work needs them
+// (fewer registers live across the tile's barrier).
+// Work-group b of one grid's launch (the single-grid kernel passes its block
+// index; the grouped kernel of several grids passes the index within the grid).
+template <class T, bool MULTI>
+__device__ __forceinline__ void hlim_block(const HlimArgs<T> &A, int b)
 {
     // q for cells [g0 - iall - 1, g0 + CHUNK + iall + 2)
     constexpr int SPAN = CHUNK + 2 * MAXI + 3;
@@ -84,9 +98,8 @@ __global__ void __launch_bounds__(CHUNK) hlim_kernel(HlimArgs<T> A)
 
     const int iall = A.iall, gall = A.gall, kall = A.kall, lall = A.lall;
     const int nchunk = (gall + CHUNK - 1) / CHUNK;
-    int b = blockIdx.x;
     if (b >= A.nmain) {  // pole work-groups
-        hlim_pole(A, (b - A.nmain) * CHUNK + threadIdx.x);
+        for (int tr = 0; tr < (MULTI ? A.ntr : 1); ++tr) hlim_pole(A, (b - A.nmain) * CHUNK + threadIdx.x, tr);
         return;
     }
     {
@@ -102,156 +115,206 @@ __global__ void __launch_bounds__(CHUNK) hlim_kernel(HlimArgs<T> A)
     const int n_in = CHUNK + 2 * iall + 3;
     const size_t fb = ((size_t)l * kall + k) * gall;
     const int tid = threadIdx.x;
-    for (int s = tid; s < n_in; s += CHUNK) {
-        const int g = lo + s;
-        s_q[s] = (g >= 0 && g < gall) ? A.q[fb + g] : T(0);
-    }
-    __syncthreads();
-
     const int g = g0 + tid;
-    if (g >= gall) return;
+    const bool active = g < gall;
     const int i = g % iall, j = g / iall;
     const bool sgp = A.have_sgp[l] != 0;
     const size_t plane = (size_t)lall * kall * gall;
     const T BIG = A.big;
+    const bool own = active && i <= iall - 2 && j <= iall - 2;  // own arcs, origin g
+    const bool inner = active && i >= 1 && i <= iall - 2 && j >= 1 && j <= iall - 2;
+    const int org[3] = {g - 1, g - 1 - iall, g - iall};  // arcs at (i-1,j), (i-1,j-1), (i,j-1)
 
-    // tracer at absolute cell index p (must lie in the staged window)
-    auto Q = [&](int p) { return s_q[p - lo]; };
-
-    // min/max of the 4 tracer values around arc m (0=AI,1=AIJ,2=AJ) with
-    // origin p; HLIM:145-160 with the index clamps of :152-153 and the
-    // singular-point stencil of :197-203.
-    auto arc_minmax = [&](int m, int p, T &qmn, T &qmx) {
-        T a, b2, c2, d2;
-        if (m == 0) {
-            const int ijm1 = max(p - iall, 0);
-            a = Q(p); b2 = Q(ijm1); c2 = Q(p + 1); d2 = Q(p + iall + 1);
-        } else if (m == 1) {
-            if (sgp && p == 0) {
-                a = Q(p); b2 = Q(p + iall + 1); c2 = Q(p + iall + 2); d2 = Q(p + iall);
-            } else {
-                a = Q(p); b2 = Q(p + 1); c2 = Q(p + iall + 1); d2 = Q(p + iall);
-            }
-        } else {
-            const int im1j = max(p - 1, 0);
-            a = Q(p); b2 = Q(p + iall + 1); c2 = Q(p + iall); d2 = Q(im1j);
-        }
-        qmn = dmin(dmin(dmin(a, b2), c2), d2);
-        qmx = dmax(dmax(dmax(a, b2), c2), d2);
-    };
-
-    // own arcs, origin g (valid for i,j <= iall-2)
-    const bool own = (i <= iall - 2 && j <= iall - 2);
-    T omn[3], omx[3], ocm[3];
+    // ---- what every tracer shares: masks, Courant sums, zerosw, d ----
+    T ocm[3] = {T(0), T(0), T(0)}, ncm[3] = {T(0), T(0), T(0)}, chm[6];
+    T Cin_sum = T(0), Cout_sum = T(0), zerosw = T(0), dc = T(0);
     if (own) {
 #pragma unroll
-        for (int m = 0; m < 3; ++m) {
-            arc_minmax(m, g, omn[m], omx[m]);
-            ocm[m] = A.cmask[(size_t)m * plane + fb + g];  // re-read by the neighbours
-        }
+        for (int m = 0; m < 3; ++m) ocm[m] = A.cmask[(size_t)m * plane + fb + g];  // re-read by the neighbours
     }
-
-    // ---- phase 1: allowable range of the cell (HLIM:216-301) ----
-    T qout_min, qout_max;
-    const T qc = Q(g);
-    if ((A.phases & 1) && i >= 1 && i <= iall - 2 && j >= 1 && j <= iall - 2) {
-        T qinmn[6], qinmx[6];
+    const bool p1 = (A.phases & 1) && inner;
+    auto load_shared = [&]() {
+        if (!p1) return;
 #pragma unroll
-        for (int m = 0; m < 3; ++m) {  // faces 1..3: own side of own arcs (:162-187)
-            const T c = ocm[m];
-            qinmn[m] = (c)*omn[m] + (T(1) - c) * BIG;
-            qinmx[m] = (c)*omx[m] + (T(1) - c) * (-BIG);
-        }
-        // faces 4..6: neighbour side of the arcs at (i-1,j), (i-1,j-1), (i,j-1)
-        const int org[3] = {g - 1, g - 1 - iall, g - iall};
-#pragma unroll
-        for (int m = 0; m < 3; ++m) {
-            T mn, mx;
-            arc_minmax(m, org[m], mn, mx);
-            const T c = A.cmask[(size_t)m * plane + fb + org[m]];
-            qinmn[m + 3] = (c)*BIG + (T(1) - c) * mn;
-            qinmx[m + 3] = (c) * (-BIG) + (T(1) - c) * mx;
-        }
-        T qnext_min = qc, qnext_max = qc;
-#pragma unroll
-        for (int f = 0; f < 6; ++f) qnext_min = dmin(qnext_min, qinmn[f]);
-#pragma unroll
-        for (int f = 0; f < 6; ++f) qnext_max = dmax(qnext_max, qinmx[f]);
-        T chv[6], chm[6];
+        for (int m = 0; m < 3; ++m) ncm[m] = A.cmask[(size_t)m * plane + fb + org[m]];
+        T chv[6];
 #pragma unroll
         for (int f = 0; f < 6; ++f) {
             chv[f] = ld_once(&A.ch[(size_t)f * plane + fb + g]);
             chm[f] = dmin(chv[f], T(0));
         }
-        const T Cin_sum = chm[0] + chm[1] + chm[2] + chm[3] + chm[4] + chm[5];
-        const T Cout_sum = chv[0] - chm[0] + chv[1] - chm[1] + chv[2] - chm[2] + chv[3] - chm[3] +
-                                chv[4] - chm[4] + chv[5] - chm[5];
-        const T CQin_min_sum = chm[0] * qinmn[0] + chm[1] * qinmn[1] + chm[2] * qinmn[2] +
-                                    chm[3] * qinmn[3] + chm[4] * qinmn[4] + chm[5] * qinmn[5];
-        const T CQin_max_sum = chm[0] * qinmx[0] + chm[1] * qinmx[1] + chm[2] * qinmx[2] +
-                                    chm[3] * qinmx[3] + chm[4] * qinmx[4] + chm[5] * qinmx[5];
-        const T zerosw = T(0.5) - sign_half(tabs(Cout_sum) - A.eps);
-        const T dc = ld_once(&A.d[fb + g]);
-        qout_min = (qc - CQin_max_sum - qnext_max * (T(1) - Cin_sum - Cout_sum + dc)) / (Cout_sum + zerosw) *
-                       (T(1) - zerosw) + qc * zerosw;
-        qout_max = (qc - CQin_min_sum - qnext_min * (T(1) - Cin_sum - Cout_sum + dc)) / (Cout_sum + zerosw) *
-                       (T(1) - zerosw) + qc * zerosw;
-    } else {  // halo (:286-301)
-        qout_min = qc;
-        qout_max = qc;
-    }
-    if (A.phases & 1) {
-#if NICAM_HLIM_STORE == 2
-        A.Qprev[fb + g] = qout_min;
-        A.Qprev[plane + fb + g] = qout_max;
-#else
-        stream_store(&A.Qprev[fb + g], qout_min);
-        stream_store(&A.Qprev[plane + fb + g], qout_max);
-#endif
-    }
+        Cin_sum = chm[0] + chm[1] + chm[2] + chm[3] + chm[4] + chm[5];
+        Cout_sum = chv[0] - chm[0] + chv[1] - chm[1] + chv[2] - chm[2] + chv[3] - chm[3] + chv[4] - chm[4] +
+                   chv[5] - chm[5];
+        zerosw = T(0.5) - sign_half(tabs(Cout_sum) - A.eps);
+        dc = ld_once(&A.d[fb + g]);
+    };
+    if constexpr (MULTI) load_shared();
 
-    // ---- phase 2: clamp the face values of the own arcs (HLIM:377-410) ----
-    if (A.mask && (A.phases & 2) && !own) A.mask[fb + g] = 0;
-    if (own && (A.phases & 2)) {
-        uint32_t bits = 0;
-        const int nb[3] = {g + 1, g + iall + 1, g + iall};
-        const T qo_min_c = A.Qpost[fb + g], qo_max_c = A.Qpost[plane + fb + g];  // re-read by neighbours
+    for (int tr = 0; tr < (MULTI ? A.ntr : 1); ++tr) {
+        const T *q = A.q + tr * A.ts;
+        T *q_a = A.q_a + tr * 6 * A.ts;
+        T *Qprev = A.Qprev + tr * 2 * A.ts;
+        const T *Qpost = A.Qpost + tr * 2 * A.ts;
+        if (tr > 0) __syncthreads();  // the previous tracer's tile has been read
+        for (int s = tid; s < n_in; s += CHUNK) {
+            const int gg = lo + s;
+            s_q[s] = (gg >= 0 && gg < gall) ? q[fb + gg] : T(0);
+        }
+        __syncthreads();
+        if (!active) continue;
+
+        // tracer at absolute cell index p (must lie in the staged window)
+        auto Q = [&](int p) { return s_q[p - lo]; };
+
+        // min/max of the 4 tracer values around arc m (0=AI,1=AIJ,2=AJ) with
+        // origin p; HLIM:145-160 with the index clamps of :152-153 and the
+        // singular-point stencil of :197-203.
+        auto arc_minmax = [&](int m, int p, T &qmn, T &qmx) {
+            T a, b2, c2, d2;
+            if (m == 0) {
+                const int ijm1 = max(p - iall, 0);
+                a = Q(p); b2 = Q(ijm1); c2 = Q(p + 1); d2 = Q(p + iall + 1);
+            } else if (m == 1) {
+                if (sgp && p == 0) {
+                    a = Q(p); b2 = Q(p + iall + 1); c2 = Q(p + iall + 2); d2 = Q(p + iall);
+                } else {
+                    a = Q(p); b2 = Q(p + 1); c2 = Q(p + iall + 1); d2 = Q(p + iall);
+                }
+            } else {
+                const int im1j = max(p - 1, 0);
+                a = Q(p); b2 = Q(p + iall + 1); c2 = Q(p + iall); d2 = Q(im1j);
+            }
+            qmn = dmin(dmin(dmin(a, b2), c2), d2);
+            qmx = dmax(dmax(dmax(a, b2), c2), d2);
+        };
+
+        T omn[3], omx[3];
+        if (own) {
 #pragma unroll
-        for (int m = 0; m < 3; ++m) {
-            const T c = ocm[m];
-            const T in_mn_s = (c)*omn[m] + (T(1) - c) * BIG;
-            const T in_mx_s = (c)*omx[m] + (T(1) - c) * (-BIG);
-            const T in_mn_n = (c)*BIG + (T(1) - c) * omn[m];
-            const T in_mx_n = (c) * (-BIG) + (T(1) - c) * omx[m];
-            const T qo_min_n = A.Qpost[fb + nb[m]], qo_max_n = A.Qpost[plane + fb + nb[m]];
-            T qa = ld_once(&A.q_a[(size_t)m * plane + fb + g]);
-            const T qa_in = qa;
-            qa = (c)*dmin(dmax(qa, in_mn_s), in_mx_s) + (T(1) - c) * dmin(dmax(qa, in_mn_n), in_mx_n);
-            qa = (c)*dmax(dmin(qa, qo_max_n), qo_min_n) + (T(1) - c) * dmax(dmin(qa, qo_max_c), qo_min_c);
-            bits |= (qa != qa_in ? 1u : 0u) << m;
+            for (int m = 0; m < 3; ++m) arc_minmax(m, g, omn[m], omx[m]);
+        }
+
+        // ---- phase 1: allowable range of the cell (HLIM:216-301) ----
+        T qout_min, qout_max;
+        const T qc = Q(g);
+        if constexpr (!MULTI) load_shared();
+        if (p1) {
+            T qinmn[6], qinmx[6];
+#pragma unroll
+            for (int m = 0; m < 3; ++m) {  // faces 1..3: own side of own arcs (:162-187)
+                const T c = ocm[m];
+                qinmn[m] = (c)*omn[m] + (T(1) - c) * BIG;
+                qinmx[m] = (c)*omx[m] + (T(1) - c) * (-BIG);
+            }
+            // faces 4..6: neighbour side of the arcs at (i-1,j), (i-1,j-1), (i,j-1)
+#pragma unroll
+            for (int m = 0; m < 3; ++m) {
+                T mn, mx;
+                arc_minmax(m, org[m], mn, mx);
+                const T c = ncm[m];
+                qinmn[m + 3] = (c)*BIG + (T(1) - c) * mn;
+                qinmx[m + 3] = (c) * (-BIG) + (T(1) - c) * mx;
+            }
+            T qnext_min = qc, qnext_max = qc;
+#pragma unroll
+            for (int f = 0; f < 6; ++f) qnext_min = dmin(qnext_min, qinmn[f]);
+#pragma unroll
+            for (int f = 0; f < 6; ++f) qnext_max = dmax(qnext_max, qinmx[f]);
+            const T CQin_min_sum = chm[0] * qinmn[0] + chm[1] * qinmn[1] + chm[2] * qinmn[2] +
+                                   chm[3] * qinmn[3] + chm[4] * qinmn[4] + chm[5] * qinmn[5];
+            const T CQin_max_sum = chm[0] * qinmx[0] + chm[1] * qinmx[1] + chm[2] * qinmx[2] +
+                                   chm[3] * qinmx[3] + chm[4] * qinmx[4] + chm[5] * qinmx[5];
+            qout_min = (qc - CQin_max_sum - qnext_max * (T(1) - Cin_sum - Cout_sum + dc)) / (Cout_sum + zerosw) *
+                           (T(1) - zerosw) + qc * zerosw;
+            qout_max = (qc - CQin_min_sum - qnext_min * (T(1) - Cin_sum - Cout_sum + dc)) / (Cout_sum + zerosw) *
+                           (T(1) - zerosw) + qc * zerosw;
+        } else {  // halo (:286-301)
+            qout_min = qc;
+            qout_max = qc;
+        }
+        if (A.phases & 1) {
 #if NICAM_HLIM_STORE == 2
-            A.q_a[(size_t)m * plane + fb + g] = qa;
+            Qprev[fb + g] = qout_min;
+            Qprev[plane + fb + g] = qout_max;
 #else
-            stream_store(&A.q_a[(size_t)m * plane + fb + g], qa);
-#endif
-#if NICAM_HLIM_STORE >= 1
-            A.q_a[(size_t)(m + 3) * plane + fb + nb[m]] = qa;  // one element off the wave's alignment
-#else
-            stream_store(&A.q_a[(size_t)(m + 3) * plane + fb + nb[m]], qa);
+            stream_store(&Qprev[fb + g], qout_min);
+            stream_store(&Qprev[plane + fb + g], qout_max);
 #endif
         }
-        if (A.mask) A.mask[fb + g] = (uint8_t)bits;
+
+        // ---- phase 2: clamp the face values of the own arcs (HLIM:377-410) ----
+        if (A.mask && (A.phases & 2) && !own) A.mask[fb + g] = 0;
+        if (own && (A.phases & 2)) {
+            uint32_t bits = 0;
+            const int nb[3] = {g + 1, g + iall + 1, g + iall};
+            const T qo_min_c = Qpost[fb + g], qo_max_c = Qpost[plane + fb + g];  // re-read by neighbours
+#pragma unroll
+            for (int m = 0; m < 3; ++m) {
+                const T c = ocm[m];
+                const T in_mn_s = (c)*omn[m] + (T(1) - c) * BIG;
+                const T in_mx_s = (c)*omx[m] + (T(1) - c) * (-BIG);
+                const T in_mn_n = (c)*BIG + (T(1) - c) * omn[m];
+                const T in_mx_n = (c) * (-BIG) + (T(1) - c) * omx[m];
+                const T qo_min_n = Qpost[fb + nb[m]], qo_max_n = Qpost[plane + fb + nb[m]];
+                T qa = ld_once(&q_a[(size_t)m * plane + fb + g]);
+                const T qa_in = qa;
+                qa = (c)*dmin(dmax(qa, in_mn_s), in_mx_s) + (T(1) - c) * dmin(dmax(qa, in_mn_n), in_mx_n);
+                qa = (c)*dmax(dmin(qa, qo_max_n), qo_min_n) + (T(1) - c) * dmax(dmin(qa, qo_max_c), qo_min_c);
+                bits |= (qa != qa_in ? 1u : 0u) << m;
+#if NICAM_HLIM_STORE == 2
+                q_a[(size_t)m * plane + fb + g] = qa;
+#else
+                stream_store(&q_a[(size_t)m * plane + fb + g], qa);
+#endif
+#if NICAM_HLIM_STORE >= 1
+                q_a[(size_t)(m + 3) * plane + fb + nb[m]] = qa;  // one element off the wave's alignment
+#else
+                stream_store(&q_a[(size_t)(m + 3) * plane + fb + nb[m]], qa);
+#endif
+            }
+            if (A.mask) A.mask[fb + g] = (uint8_t)bits;
+        }
     }
+}
+
+template <class T, bool MULTI>
+__global__ void __launch_bounds__(CHUNK) hlim_kernel(HlimArgs<T> A)
+{
+    hlim_block<T, MULTI>(A, blockIdx.x);
+}
+
+// Several grids in one launch: work-groups [off[n], off[n+1]) belong to grid n.
+template <class T>
+struct HlimGroup {
+    HlimArgs<T> a[kMaxGroup];
+    int off[kMaxGroup + 1];
+    int n;
+};
+
+template <class T>
+__global__ void __launch_bounds__(CHUNK) hlim_group_kernel(HlimGroup<T> G)
+{
+    const int b = blockIdx.x;
+    int n = 0;
+    while (n + 1 < G.n && b >= G.off[n + 1]) ++n;
+    hlim_block<T, true>(G.a[n], b - G.off[n]);
 }
 
 // Pole regions: one work-item per (k, l) (HLIM:307-363, 367-373, 412-425).
 // Qin of ring point v depends only on q around it and its cmask, so it is
 // recomputed in each of the three passes over the ring instead of stored.
 template <class T>
-__device__ void hlim_pole(const HlimArgs<T> &A, int t)
+__device__ void hlim_pole(const HlimArgs<T> &A0, int t, int tr)
 {
-    const int kall = A.kall, npl = A.gall_pl;
+    const int kall = A0.kall, npl = A0.gall_pl;
     if (t >= kall * kLallPl) return;
+    // tracer tr's pole fields
+    HlimArgs<T> A = A0;
+    A.q_pl = A0.q_pl + tr * A0.tps;
+    A.q_a_pl = A0.q_a_pl ? A0.q_a_pl + tr * A0.tps : nullptr;
+    A.Qprev_pl = A0.Qprev_pl + tr * 2 * A0.tps;
+    A.Qpost_pl = A0.Qpost_pl ? A0.Qpost_pl + tr * 2 * A0.tps : nullptr;
     const int k = t % kall, l = t / kall;
     const size_t fb = ((size_t)l * kall + k) * npl;
     const size_t plane = (size_t)kLallPl * kall * npl;
@@ -337,15 +400,19 @@ int halo_fill(nicam_ctx c, float *v, float *v_pl, int kmax, int nvar)
     return launch_halo_fill_f32(c, v, v_pl, kmax, nvar, false);
 }
 
+// The launch arguments of one grid (nblocks: work-groups of one launch).
 template <class T>
-int launch_hlim_t(nicam_ctx c, T *q_a, T *q_a_pl, const T *q, const T *q_pl, const T *d, const T *d_pl,
-                  const T *ch, const T *ch_pl, const T *cmask, const T *cmask_pl, T *Qprev, T *Qprev_pl,
-                  const T *Qpost, const T *Qpost_pl, T eps, T big, uint8_t *mask = nullptr,
-                  uint8_t *mask_pl = nullptr)
+int hlim_make_args(nicam_ctx c, HlimArgs<T> &A, long &nblocks, T *q_a, T *q_a_pl, const T *q, const T *q_pl,
+                   const T *d, const T *d_pl, const T *ch, const T *ch_pl, const T *cmask, const T *cmask_pl,
+                   T *Qprev, T *Qprev_pl, const T *Qpost, const T *Qpost_pl, T eps, T big, uint8_t *mask,
+                   uint8_t *mask_pl, int ntr)
 {
     const Dims &D = c->d;
     if (D.gall_1d > MAXI) return fail(NICAM_ERR_ARG, "gall_1d larger than the LDS tiles (160)");
-    HlimArgs<T> A;
+    if (ntr < 1 || (mask && ntr != 1)) return fail(NICAM_ERR_ARG, "tracer count must be >= 1 (1 with a mask)");
+    A.ntr = ntr;
+    A.ts = (long)D.nreg();
+    A.tps = (long)D.npl();
     A.mask = mask;
     A.mask_pl = mask_pl;
     A.q_a = q_a;
@@ -375,9 +442,29 @@ int launch_hlim_t(nicam_ctx c, T *q_a, T *q_a_pl, const T *q, const T *q_pl, con
     A.Qpost_pl = Qpost_pl;
     A.gall_pl = D.gall_pl;
     A.have_pl = D.have_pl;
+    A.phases = 3;
+    nblocks = nmain + npole;
+    return NICAM_OK;
+}
+
+template <class T>
+int launch_hlim_t(nicam_ctx c, T *q_a, T *q_a_pl, const T *q, const T *q_pl, const T *d, const T *d_pl,
+                  const T *ch, const T *ch_pl, const T *cmask, const T *cmask_pl, T *Qprev, T *Qprev_pl,
+                  const T *Qpost, const T *Qpost_pl, T eps, T big, uint8_t *mask = nullptr,
+                  uint8_t *mask_pl = nullptr, int ntr = 1)
+{
+    const Dims &D = c->d;
+    HlimArgs<T> A;
+    long nblk = 0;
+    int rc0 = hlim_make_args(c, A, nblk, q_a, q_a_pl, q, q_pl, d, d_pl, ch, ch_pl, cmask, cmask_pl, Qprev, Qprev_pl,
+                             Qpost, Qpost_pl, eps, big, mask, mask_pl, ntr);
+    if (rc0 != NICAM_OK) return rc0;
     auto run = [&](int phases) -> int {
         A.phases = phases;
-        hipLaunchKernelGGL(hlim_kernel<T>, dim3((unsigned)(nmain + npole)), dim3(CHUNK), 0, c->stream, A);
+        if (ntr > 1)
+            hipLaunchKernelGGL((hlim_kernel<T, true>), dim3((unsigned)nblk), dim3(CHUNK), 0, c->stream, A);
+        else
+            hipLaunchKernelGGL((hlim_kernel<T, false>), dim3((unsigned)nblk), dim3(CHUNK), 0, c->stream, A);
         NICAM_HIP(hipGetLastError());
         return NICAM_OK;
     };
@@ -392,7 +479,7 @@ int launch_hlim_t(nicam_ctx c, T *q_a, T *q_a_pl, const T *q, const T *q_pl, con
     if (!Qprev_pl) return fail(NICAM_ERR_ARG, "Qout_prev_pl is needed to exchange Qout with the poles");
     int rc = run(1);
     if (rc != NICAM_OK) return rc;
-    rc = halo_fill(c, Qprev, Qprev_pl, D.kall, 2);
+    rc = halo_fill(c, Qprev, Qprev_pl, D.kall, 2 * ntr);
     if (rc != NICAM_OK) return rc;
     A.Qpost = Qprev;
     A.Qpost_pl = Qprev_pl;
@@ -424,9 +511,58 @@ int launch_hlim_f32(nicam_ctx c, float *q_a, float *q_a_pl, const float *q, cons
                     const float *cmask_pl, float *Qprev, float *Qprev_pl, const float *Qpost,
                     const float *Qpost_pl)
 {
-    // CONST_EPS and CONST_HUGE of the reference built with -DSINGLE
+    // the FP32 CONST_EPS and CONST_HUGE (nicam_set_const_f32; by default the -DSINGLE build's)
     return launch_hlim_t<float>(c, q_a, q_a_pl, q, q_pl, d, d_pl, ch, ch_pl, cmask, cmask_pl, Qprev, Qprev_pl, Qpost,
-                                Qpost_pl, kEpsF32, kHugeF32);
+                                Qpost_pl, c->eps32, c->big32);
+}
+
+}  // namespace nicam
+
+namespace nicam {
+
+int launch_hlim_batch(nicam_ctx c, int ntr, double *q_a, double *q_a_pl, const double *q, const double *q_pl,
+                      const double *d, const double *d_pl, const double *ch, const double *ch_pl,
+                      const double *cmask, const double *cmask_pl, double *Qprev, double *Qprev_pl,
+                      const double *Qpost, const double *Qpost_pl)
+{
+    return launch_hlim_t<double>(c, q_a, q_a_pl, q, q_pl, d, d_pl, ch, ch_pl, cmask, cmask_pl, Qprev, Qprev_pl, Qpost,
+                                 Qpost_pl, c->eps, c->big, nullptr, nullptr, ntr);
+}
+
+int launch_hlim_batch_f32(nicam_ctx c, int ntr, float *q_a, float *q_a_pl, const float *q, const float *q_pl,
+                          const float *d, const float *d_pl, const float *ch, const float *ch_pl,
+                          const float *cmask, const float *cmask_pl, float *Qprev, float *Qprev_pl,
+                          const float *Qpost, const float *Qpost_pl)
+{
+    return launch_hlim_t<float>(c, q_a, q_a_pl, q, q_pl, d, d_pl, ch, ch_pl, cmask, cmask_pl, Qprev, Qprev_pl, Qpost,
+                                Qpost_pl, c->eps32, c->big32, nullptr, nullptr, ntr);
+}
+
+}  // namespace nicam
+
+namespace nicam {
+
+// FP32 limiter of ntr tracers on n grids in one launch (Qout_post supplied)
+int launch_hlim_group_f32(const nicam_ctx *cs, int n, int ntr, const nicam_tracer_step_f32 *g, hipStream_t stream)
+{
+    if (n < 1 || n > kMaxGroup) return fail(NICAM_ERR_ARG, "a group holds 1..8 grids");
+    HlimGroup<float> G;
+    G.n = n;
+    G.off[0] = 0;
+    for (int i = 0; i < n; ++i) {
+        const nicam_tracer_step_f32 &p = g[i];
+        if (!p.Qout_post) return fail(NICAM_ERR_ARG, "the grouped limiter needs Qout_post");
+        long nb = 0;
+        const int rc = hlim_make_args<float>(cs[i], G.a[i], nb, p.q_a, p.q_a_pl, p.q, p.q_pl, p.d, p.d_pl, p.ch,
+                                             p.ch_pl, p.cmask, p.cmask_pl, p.Qout_prev, p.Qout_prev_pl, p.Qout_post,
+                                             p.Qout_post_pl, cs[i]->eps32, cs[i]->big32, nullptr, nullptr, ntr);
+        if (rc != NICAM_OK) return rc;
+        if ((long)G.off[i] + nb > 0x7fffffffL) return fail(NICAM_ERR_ARG, "group too large for one launch");
+        G.off[i + 1] = G.off[i] + (int)nb;
+    }
+    hipLaunchKernelGGL(hlim_group_kernel<float>, dim3((unsigned)G.off[n]), dim3(CHUNK), 0, stream, G);
+    NICAM_HIP(hipGetLastError());
+    return NICAM_OK;
 }
 
 }  // namespace nicam

@@ -19,6 +19,7 @@ constexpr int kGmtrPNmax = 8;  // GMTR_p_nmax
 constexpr int kGmtrTNmax = 5;  // GMTR_t_nmax
 constexpr int kGmtrANmax = 12; // GMTR_a_nmax
 constexpr int kGmtrANmaxPl = 18;
+constexpr int kMaxGroup = 8;  // grids in one grouped launch
 
 // Sizes of one grid, as the kernels see them.  All indices inside the
 // kernels are 0-based: g = j*gall_1d + i, halo at i or j = 0 and gall_1d-1.
@@ -39,7 +40,7 @@ struct Dims {
 struct nicam_tune {
     int hflux_kper = 0;   // levels per work-group in horizontal_flux
     int vlim_kchunk = 0;  // levels per work-item in vertical_limiter_thuburn
-    int vi_variant = 0;   // elimination-factor storage in vi_rhow_solver
+    int vi_variant = 0;   // elimination-factor storage in vi_rhow_solver (1..4)
     int diff_kper = 0;    // levels per work-group in OPRT_diffusion
     int ddmp_kper = 0;    // levels per work-group in OPRT3D_divdamp
 };
@@ -63,6 +64,8 @@ struct nicam_ctx_s {
     int *have_sgp = nullptr;  // device [lall]
 
     double eps = 0, big = 0;
+    // CONST_EPS / CONST_HUGE of the _f32 entry points (nicam_set_const_f32)
+    float eps32 = nicam::kEpsF32, big32 = nicam::kHugeF32;
     double grav = 9.80616, rdry = 287.0, cvdry = 717.5;
     int alpha = 1;
 
@@ -112,10 +115,15 @@ struct nicam_ctx_s {
     int *halo_dst = nullptr, *halo_src = nullptr;
     int halo_n = 0;
     int rlevel = -1;
+    // partners of the faces regions share (nicam_comm_setup; the conservative update)
+    int *arc_slot = nullptr, *arc_tab = nullptr, *arc_pole = nullptr;
 
-    // vi_rhow_solver work array for the elimination factors (variant "global")
-    double *vi_gamma = nullptr;
-    size_t vi_gamma_n = 0;
+    // partial sums of nicam_tracer_mass ([kall][lall]), allocated on its first call
+    double *mass_part = nullptr;
+
+    // the Thomas factorisation of Mc/Ml/Mu (beta(k), gamma(k)), formed on the
+    // device whenever the coefficients are set
+    double *vi_beta = nullptr, *vi_beta_pl = nullptr, *vi_gam = nullptr, *vi_gam_pl = nullptr;
 
     // staging buffers for NICAM_PTR_HOST, one per argument slot
     std::vector<void *> stage;
@@ -223,6 +231,10 @@ struct HaloPoint {
 int build_halo_map(int rlevel, int gall_1d, int gall_pl, std::vector<HaloPoint> &dst, std::vector<HaloPoint> &src,
                    std::vector<int> &have_sgp, HaloPoint pole_src[2]);
 int comm_setup(nicam_ctx c, int rlevel);
+// per edge cell 6 face codes (see comm.cpp), slot[l*gall+g] = edge-cell index or -1, pole faces
+int build_arc_partners(int rlevel, int gall_1d, int gall_pl, const std::vector<HaloPoint> &dst,
+                       const std::vector<HaloPoint> &src, const std::vector<int> &have_sgp, std::vector<int> &slot,
+                       std::vector<int> &tab, std::vector<int> &pole_tab);
 // corners W, N, E, S of the 10 diamonds -> icosahedron vertex class; returns the class count
 int diamond_corner_classes(int cls[10][4]);
 // var(gall,kmax,lall,nvar), var_pl(gall_pl,kmax,2,nvar); pole_fill: COMM_var
@@ -270,6 +282,30 @@ int launch_vi_adjoint(nicam_ctx c, double *wb, double *wb_pl, double *w0b, doubl
 int launch_vi_mixed(nicam_ctx c, double *w, double *w_pl, const float *w0, const float *w0_pl, const float *p0,
                     const float *p0_pl, const float *r0, const float *r0_pl, const float *Sr, const float *Sr_pl,
                     const float *Sw, const float *Sw_pl, const float *Sp, const float *Sp_pl, double dt);
+int launch_vi_factor(nicam_ctx c);
+// many tracers, several grids
+int launch_hlim_batch(nicam_ctx c, int ntr, double *q_a, double *q_a_pl, const double *q, const double *q_pl,
+                      const double *d, const double *d_pl, const double *ch, const double *ch_pl,
+                      const double *cmask, const double *cmask_pl, double *Qprev, double *Qprev_pl,
+                      const double *Qpost, const double *Qpost_pl);
+int launch_hlim_batch_f32(nicam_ctx c, int ntr, float *q_a, float *q_a_pl, const float *q, const float *q_pl,
+                          const float *d, const float *d_pl, const float *ch, const float *ch_pl,
+                          const float *cmask, const float *cmask_pl, float *Qprev, float *Qprev_pl,
+                          const float *Qpost, const float *Qpost_pl);
+int launch_vlim_batch(nicam_ctx c, int ntr, double *q_h, double *q_h_pl, const double *q, const double *q_pl,
+                      const double *d, const double *d_pl, const double *ck, const double *ck_pl);
+int launch_vlim_batch_f32(nicam_ctx c, int ntr, float *q_h, float *q_h_pl, const float *q, const float *q_pl,
+                          const float *d, const float *d_pl, const float *ck, const float *ck_pl);
+int launch_hflux_group_f32(const nicam_ctx *cs, int n, const nicam_tracer_step_f32 *g, float dt, hipStream_t s);
+int launch_hlim_group_f32(const nicam_ctx *cs, int n, int ntr, const nicam_tracer_step_f32 *g, hipStream_t s);
+int launch_vlim_group_f32(const nicam_ctx *cs, int n, int ntr, const nicam_tracer_step_f32 *g, hipStream_t s);
+int launch_vi_group_mixed(const nicam_ctx *cs, int n, const nicam_tracer_step_f32 *g, double dt, hipStream_t s);
+int launch_mass(nicam_ctx c, const double *rq, const double *rq_pl, double *mass, double *part);
+int launch_mass_f32(nicam_ctx c, const float *rq, const float *rq_pl, double *mass, double *part);
+int launch_update_cons(nicam_ctx c, double *rq, double *rq_pl, const double *flx, const double *qa);
+int launch_update_cons_f32(nicam_ctx c, double *rq, double *rq_pl, const float *flx, const float *qa);
+int launch_vertical_update_f32(nicam_ctx c, double *rq, double *rq_pl, const float *flx_v, const float *flx_v_pl,
+                               const float *q_h, const float *q_h_pl);
 int launch_vi(nicam_ctx c, double *w, double *w_pl, const double *w0, const double *w0_pl,
               const double *p0, const double *p0_pl, const double *r0, const double *r0_pl,
               const double *Sr, const double *Sr_pl, const double *Sw, const double *Sw_pl,

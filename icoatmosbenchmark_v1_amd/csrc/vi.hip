@@ -10,14 +10,22 @@
 // consecutive g at one level, so all global traffic is contiguous.  The pole
 // columns are appended as the last work-items.
 //
-// The back substitution needs the elimination factors gamma(k) and the
-// forward solution of every level.  Three storage variants are provided:
-//   1  gamma in LDS (one slot per lane and level), forward solution in rhogw
-//   2  gamma in a device work array laid out [level][column], forward
+// The elimination factors gamma(k) = Mu(k-1)/beta(k-1) and
+// beta(k) = Mc(k) - Ml(k)*gamma(k) depend on the coefficients only, not on the
+// source term.  nicam_vi_rhow_solver_putcoef (and the on-device matrix build)
+// therefore run the factorisation once, per column with the reference's
+// operations in its order (vi_factor_kernel), and keep beta and gamma beside
+// Mc/Ml/Mu.  A factored solve reads Ml, beta and gamma where the reference
+// reads Mc, Ml and Mu (the same traffic), and its dependency chain per level
+// holds one division instead of two.  Storage variants of the solve:
+//   1  unfactored, gamma in LDS (one slot per lane and level), forward
 //      solution in rhogw
-//   3  gamma and the forward solution in registers (fully unrolled levels;
-//      built for kall = 42 and 98)
-// All three perform the same operations in the same order.
+//   2  factored, forward solution in rhogw (any kall)
+//   3  unfactored, gamma and the forward solution in registers (fully
+//      unrolled levels; built for kall = 42 and 98)
+//   4  factored, forward solution in registers (kall = 42 and 98)
+// All of them produce the same bits: the factors are formed by the same
+// operations in the same order whether they are stored or not.
 #include <type_traits>
 
 #include "internal.h"
@@ -35,14 +43,14 @@ struct ViFieldT {
     const double *Mc, *Ml, *Mu;
     const Src *RGAM, *RGAMH, *RGSGAM2;
     const double *RGSGAM2H, *GSGAM2H;
+    const double *beta, *gam;  // the stored factorisation (factored variants)
 };
 
 template <class Src>
 struct ViArgsT {
     ViFieldT<Src> reg, pl;
     const double *rdgzh, *afact, *bfact;
-    double *gwork;  // variant 2: [kall][ncol]
-    long ncol;      // columns including the pole columns
+    long ncol;  // columns including the pole columns
     int gall, kall, lall, gall_pl, have_pl;
     double dt, CVovRt2, alpha, grav;
 };
@@ -54,11 +62,8 @@ struct LdsGamma {
     __device__ void set(int k, double v) { p[k * stride] = v; }
     __device__ double get(int k) const { return p[k * stride]; }
 };
-struct GlobalGamma {
-    double *p;  // work array at this column; stride = ncol
-    long stride;
-    __device__ void set(int k, double v) { p[k * stride] = v; }
-    __device__ double get(int k) const { return p[k * stride]; }
+struct NoGamma {  // factored variants: gamma is read from the stored factorisation
+    __device__ void set(int, double) {}
 };
 template <int N>
 struct RegArray {
@@ -69,8 +74,9 @@ struct RegArray {
 
 // One column.  KC > 0 fixes kall at compile time (loops fully unrolled, W
 // may then hold the forward solution in registers); KC == 0 reads kall at run
-// time and keeps the forward solution in rhogw.
-template <int KC, bool WREG, class G, class WS, class Src>
+// time and keeps the forward solution in rhogw.  FACT reads beta and gamma
+// from the stored factorisation instead of forming them.
+template <int KC, bool WREG, bool FACT, class G, class WS, class Src>
 __device__ __forceinline__ void vi_column(const ViFieldT<Src> &F, size_t base, size_t s, int kall_rt,
                                           const double *__restrict__ rdgzh, const double *__restrict__ afact,
                                           const double *__restrict__ bfact, double dt, double CVovRt2,
@@ -95,7 +101,7 @@ __device__ __forceinline__ void vi_column(const ViFieldT<Src> &F, size_t base, s
     // the inputs of one level; the next level's are loaded before this
     // level's stores are issued, so the loads overlap the elimination
     struct Lvl {
-        double rgh, rg, p0, Sp, rgs2, r0, Sr, w0, Sw, Mc, Ml, Mu;
+        double rgh, rg, p0, Sp, rgs2, r0, Sr, w0, Sw, Mc, Ml, Mu;  // Mc holds beta when FACT
     };
     auto load = [&](int k) {
         const size_t o = at(k);
@@ -109,9 +115,15 @@ __device__ __forceinline__ void vi_column(const ViFieldT<Src> &F, size_t base, s
         v.Sr = F.Sr[o];
         v.w0 = F.w0[o];
         v.Sw = F.Sw[o];
-        v.Mc = F.Mc[o];
-        v.Ml = F.Ml[o];
-        v.Mu = F.Mu[o];
+        if constexpr (FACT) {
+            v.Mc = F.beta[o];
+            v.Ml = F.Ml[o];
+            v.Mu = 0.0;
+        } else {
+            v.Mc = F.Mc[o];
+            v.Ml = F.Ml[o];
+            v.Mu = F.Mu[o];
+        }
         return v;
     };
     double beta = 0.0, w_km1 = 0.0, Mu_km1 = 0.0;
@@ -128,11 +140,13 @@ __device__ __forceinline__ void vi_column(const ViFieldT<Src> &F, size_t base, s
                     (R_k * afact[k] + R_km1 * bfact[k]) * dt * grav) *
                    CVovRt2;
         if (k == kmin + 1) S = S - cur.Ml * w_kmin;  // mod_vi.f90:211
-        if (k == kmax) S = S - cur.Mu * w_kmax1;     // mod_vi.f90:212
+        if (k == kmax) S = S - (FACT ? F.Mu[o] : cur.Mu) * w_kmax1;  // mod_vi.f90:212
         double w;
         if (k == kmin + 1) {  // mod_vi.f90:218-222
             beta = cur.Mc;
             w = S / beta;
+        } else if constexpr (FACT) {  // mod_vi.f90:225-231 with beta(k) stored
+            w = (S - cur.Ml * w_km1) / cur.Mc;
         } else {  // mod_vi.f90:225-231
             const double gk = Mu_km1 / beta;
             gam.set(k, gk);
@@ -155,7 +169,8 @@ __device__ __forceinline__ void vi_column(const ViFieldT<Src> &F, size_t base, s
     auto load_b = [&](int k, double &wf_k, double &g_k1, double &gs_k1) {
         if constexpr (WREG) wf_k = wf.get(k);
         else wf_k = F.w[at(k)];
-        g_k1 = gam.get(k + 1);
+        if constexpr (FACT) g_k1 = F.gam[at(k + 1)];
+        else g_k1 = gam.get(k + 1);
         gs_k1 = F.GSGAM2H[at(k + 1)];
     };
     double wf_c = 0.0, g_c = 0.0, gs_c = 0.0;
@@ -202,7 +217,7 @@ __device__ __forceinline__ bool vi_map(const ViArgsT<Src> &A, long t, const ViFi
 }
 
 constexpr int BLOCK_LDS = 64;
-constexpr int BLOCK_GLB = 256;
+constexpr int BLOCK_FAC = 256;
 constexpr int BLOCK_REG = 64;
 
 template <class Src>
@@ -215,53 +230,117 @@ __global__ void __launch_bounds__(BLOCK_LDS) vi_kernel_lds(ViArgsT<Src> A)
     if (!vi_map(A, t, F, base, s)) return;
     LdsGamma gam{s_gam + threadIdx.x, BLOCK_LDS};
     int unused = 0;
-    vi_column<0, false>(*F, base, s, A.kall, A.rdgzh, A.afact, A.bfact, A.dt, A.CVovRt2, A.alpha, A.grav, gam,
-                        unused);
+    vi_column<0, false, false>(*F, base, s, A.kall, A.rdgzh, A.afact, A.bfact, A.dt, A.CVovRt2, A.alpha, A.grav,
+                               gam, unused);
 }
 
+// factored, any kall: the forward solution goes through rhogw
 template <class Src>
-__global__ void __launch_bounds__(BLOCK_GLB) vi_kernel_glb(ViArgsT<Src> A)
+__global__ void __launch_bounds__(BLOCK_FAC) vi_kernel_fact(ViArgsT<Src> A)
 {
-    const long t = (long)blockIdx.x * BLOCK_GLB + threadIdx.x;
+    const long t = (long)blockIdx.x * BLOCK_FAC + threadIdx.x;
     const ViFieldT<Src> *F;
     size_t base, s;
     if (!vi_map(A, t, F, base, s)) return;
-    GlobalGamma gam{A.gwork + t, A.ncol};
+    NoGamma gam;
     int unused = 0;
-    vi_column<0, false>(*F, base, s, A.kall, A.rdgzh, A.afact, A.bfact, A.dt, A.CVovRt2, A.alpha, A.grav, gam,
-                        unused);
+    vi_column<0, false, true>(*F, base, s, A.kall, A.rdgzh, A.afact, A.bfact, A.dt, A.CVovRt2, A.alpha, A.grav,
+                              gam, unused);
 }
 
-template <int KC, bool WREG, class Src>
+// several grids in one launch (factored, any kall): work-groups
+// [off[n], off[n+1]) belong to grid n
+template <class Src>
+struct ViGroup {
+    ViArgsT<Src> a[kMaxGroup];
+    int off[kMaxGroup + 1];
+    int n;
+};
+
+template <class Src>
+__global__ void __launch_bounds__(BLOCK_FAC) vi_group_kernel(ViGroup<Src> G)
+{
+    const int b = blockIdx.x;
+    int n = 0;
+    while (n + 1 < G.n && b >= G.off[n + 1]) ++n;
+    const ViArgsT<Src> &A = G.a[n];
+    const long t = (long)(b - G.off[n]) * BLOCK_FAC + threadIdx.x;
+    const ViFieldT<Src> *F;
+    size_t base, s;
+    if (!vi_map(A, t, F, base, s)) return;
+    NoGamma gam;
+    int unused = 0;
+    vi_column<0, false, true>(*F, base, s, A.kall, A.rdgzh, A.afact, A.bfact, A.dt, A.CVovRt2, A.alpha, A.grav,
+                              gam, unused);
+}
+
+template <int KC, bool FACT, class Src>
 __global__ void __launch_bounds__(BLOCK_REG) vi_kernel_reg(ViArgsT<Src> A)
 {
     const long t = (long)blockIdx.x * BLOCK_REG + threadIdx.x;
     const ViFieldT<Src> *F;
     size_t base, s;
     if (!vi_map(A, t, F, base, s)) return;
-    RegArray<KC> gam;
-    if constexpr (WREG) {
-        RegArray<KC> wf;
-        vi_column<KC, true>(*F, base, s, KC, A.rdgzh, A.afact, A.bfact, A.dt, A.CVovRt2, A.alpha, A.grav, gam, wf);
+    RegArray<KC> wf;
+    if constexpr (FACT) {
+        NoGamma gam;
+        vi_column<KC, true, true>(*F, base, s, KC, A.rdgzh, A.afact, A.bfact, A.dt, A.CVovRt2, A.alpha, A.grav, gam,
+                                  wf);
     } else {
-        int unused = 0;
-        vi_column<KC, false>(*F, base, s, KC, A.rdgzh, A.afact, A.bfact, A.dt, A.CVovRt2, A.alpha, A.grav, gam,
-                             unused);
+        RegArray<KC> gam;
+        vi_column<KC, true, false>(*F, base, s, KC, A.rdgzh, A.afact, A.bfact, A.dt, A.CVovRt2, A.alpha, A.grav,
+                                   gam, wf);
+    }
+}
+
+// The Thomas factorisation of one column (mod_vi.f90:218-231 without the
+// right-hand side): beta(kmin+1) = Mc(kmin+1); gamma(k) = Mu(k-1)/beta(k-1),
+// beta(k) = Mc(k) - Ml(k)*gamma(k) for k = kmin+2..kmax.  Other levels are
+// left as they are; no solve reads them.
+struct FactorPart {
+    const double *Mc, *Ml, *Mu;
+    double *beta, *gam;
+    int gall, ncols;  // level stride and columns of this part
+};
+struct FactorArgs {
+    FactorPart reg, pl;
+    int kall;
+};
+
+__global__ void __launch_bounds__(BLOCK_FAC) vi_factor_kernel(FactorArgs A)
+{
+    long t = (long)blockIdx.x * BLOCK_FAC + threadIdx.x;
+    const FactorPart *P = &A.reg;
+    if (t >= A.reg.ncols) {
+        t -= A.reg.ncols;
+        P = &A.pl;
+        if (t >= A.pl.ncols) return;
+    }
+    const int kall = A.kall, kmin = 1, kmax = kall - 2;
+    const size_t base = (size_t)(t / P->gall) * kall * P->gall + (size_t)(t % P->gall), s = P->gall;
+    double beta = P->Mc[base + (kmin + 1) * s];
+    P->beta[base + (kmin + 1) * s] = beta;
+    for (int k = kmin + 2; k <= kmax; ++k) {
+        const size_t o = base + k * s;
+        const double g = P->Mu[o - s] / beta;
+        beta = P->Mc[o] - P->Ml[o] * g;
+        P->gam[o] = g;
+        P->beta[o] = beta;
     }
 }
 
 template <class Src>
-int launch_vi_t(nicam_ctx c, double *w, double *w_pl, const Src *w0, const Src *w0_pl, const Src *p0, const Src *p0_pl,
-                const Src *r0, const Src *r0_pl, const Src *Sr, const Src *Sr_pl, const Src *Sw, const Src *Sw_pl, const Src *Sp,
-                const Src *Sp_pl, double dt, const Src *RGAM, const Src *RGAM_pl, const Src *RGAMH, const Src *RGAMH_pl,
-                const Src *RGSGAM2, const Src *RGSGAM2_pl)
+void vi_make_args(nicam_ctx c, ViArgsT<Src> &A, double *w, double *w_pl, const Src *w0, const Src *w0_pl,
+                  const Src *p0, const Src *p0_pl, const Src *r0, const Src *r0_pl, const Src *Sr, const Src *Sr_pl,
+                  const Src *Sw, const Src *Sw_pl, const Src *Sp, const Src *Sp_pl, double dt, const Src *RGAM,
+                  const Src *RGAM_pl, const Src *RGAMH, const Src *RGAMH_pl, const Src *RGSGAM2,
+                  const Src *RGSGAM2_pl)
 {
     const Dims &D = c->d;
-    ViArgsT<Src> A;
     A.reg = ViFieldT<Src>{w, w0, p0, r0, Sr, Sw, Sp, c->Mc, c->Ml, c->Mu, RGAM, RGAMH, RGSGAM2, c->RGSGAM2H,
-                        c->GSGAM2H};
+                          c->GSGAM2H, c->vi_beta, c->vi_gam};
     A.pl = ViFieldT<Src>{w_pl, w0_pl, p0_pl, r0_pl, Sr_pl, Sw_pl, Sp_pl, c->Mc_pl, c->Ml_pl, c->Mu_pl, RGAM_pl,
-                       RGAMH_pl, RGSGAM2_pl, c->RGSGAM2H_pl, c->GSGAM2H_pl};
+                         RGAMH_pl, RGSGAM2_pl, c->RGSGAM2H_pl, c->GSGAM2H_pl, c->vi_beta_pl, c->vi_gam_pl};
     A.rdgzh = c->rdgzh;
     A.afact = c->afact;
     A.bfact = c->bfact;
@@ -275,15 +354,23 @@ int launch_vi_t(nicam_ctx c, double *w, double *w_pl, const Src *w0, const Src *
     A.alpha = (double)c->alpha;                   // mod_vi.f90:179
     A.grav = c->grav;
     A.ncol = (long)D.gall * D.lall + (D.have_pl ? (long)D.gall_pl * kLallPl : 0);
-    A.gwork = nullptr;
+}
+
+template <class Src>
+int launch_vi_t(nicam_ctx c, double *w, double *w_pl, const Src *w0, const Src *w0_pl, const Src *p0, const Src *p0_pl,
+                const Src *r0, const Src *r0_pl, const Src *Sr, const Src *Sr_pl, const Src *Sw, const Src *Sw_pl, const Src *Sp,
+                const Src *Sp_pl, double dt, const Src *RGAM, const Src *RGAM_pl, const Src *RGAMH, const Src *RGAMH_pl,
+                const Src *RGSGAM2, const Src *RGSGAM2_pl)
+{
+    const Dims &D = c->d;
+    ViArgsT<Src> A;
+    vi_make_args(c, A, w, w_pl, w0, w0_pl, p0, p0_pl, r0, r0_pl, Sr, Sr_pl, Sw, Sw_pl, Sp, Sp_pl, dt, RGAM, RGAM_pl,
+                 RGAMH, RGAMH_pl, RGSGAM2, RGSGAM2_pl);
 
     int variant = c->tune.vi_variant;
     const bool reg_ok = (D.kall == 42 || D.kall == 98);
-    if (variant == 3 && !reg_ok) variant = 0;
-    // measured at gl7 (40 x 66x66 x 98): FP64 inputs: registers 380-407 us, work array 525 us, LDS 600 us;
-    // FP32 inputs: work array 372 us, registers 562 us, LDS 566 us (the widening loads lengthen the
-    // register variant's one-wave-per-SIMD dependency chain)
-    if (variant == 0) variant = (reg_ok && !std::is_same<Src, float>::value) ? 3 : 2;
+    if ((variant == 3 || variant == 4) && !reg_ok) variant = 0;
+    if (variant == 0) variant = reg_ok ? 4 : 2;
 
     if (variant == 1) {
         const size_t lds = sizeof(double) * BLOCK_LDS * D.kall;
@@ -294,35 +381,41 @@ int launch_vi_t(nicam_ctx c, double *w, double *w_pl, const Src *w0, const Src *
         hipLaunchKernelGGL(vi_kernel_lds<Src>, dim3((unsigned)((A.ncol + BLOCK_LDS - 1) / BLOCK_LDS)), dim3(BLOCK_LDS), lds,
                            c->stream, A);
     } else if (variant == 2) {
-        const size_t need = (size_t)A.ncol * D.kall;
-        if (c->vi_gamma_n < need) {
-            if (c->vi_gamma) {
-                NICAM_HIP(hipStreamSynchronize(c->stream));
-                (void)hipFree(c->vi_gamma);
-                c->device_bytes -= c->vi_gamma_n * sizeof(double);
-                c->vi_gamma = nullptr;
-                c->vi_gamma_n = 0;
-            }
-            NICAM_HIP(hipMalloc(&c->vi_gamma, need * sizeof(double)));
-            c->vi_gamma_n = need;
-            c->device_bytes += need * sizeof(double);
-        }
-        A.gwork = c->vi_gamma;
-        hipLaunchKernelGGL(vi_kernel_glb<Src>, dim3((unsigned)((A.ncol + BLOCK_GLB - 1) / BLOCK_GLB)), dim3(BLOCK_GLB), 0,
-                           c->stream, A);
+        hipLaunchKernelGGL(vi_kernel_fact<Src>, dim3((unsigned)((A.ncol + BLOCK_FAC - 1) / BLOCK_FAC)), dim3(BLOCK_FAC),
+                           0, c->stream, A);
     } else {
         const dim3 grid((unsigned)((A.ncol + BLOCK_REG - 1) / BLOCK_REG));
-        // gamma and the forward solution of a whole column in registers: 252
-        // VGPRs at kall = 42, 256 VGPRs + 256 AGPRs (one wavefront per SIMD,
-        // no spills) at kall = 98
-        if (D.kall == 42) hipLaunchKernelGGL((vi_kernel_reg<42, true, Src>), grid, dim3(BLOCK_REG), 0, c->stream, A);
-        else hipLaunchKernelGGL((vi_kernel_reg<98, true, Src>), grid, dim3(BLOCK_REG), 0, c->stream, A);
+        // variant 3 holds gamma and the forward solution of a whole column in
+        // registers (252 VGPRs at kall = 42; 256 VGPRs + 256 AGPRs, one
+        // wavefront per SIMD, at kall = 98); variant 4 only the forward solution
+        if (variant == 3) {
+            if (D.kall == 42) hipLaunchKernelGGL((vi_kernel_reg<42, false, Src>), grid, dim3(BLOCK_REG), 0, c->stream, A);
+            else hipLaunchKernelGGL((vi_kernel_reg<98, false, Src>), grid, dim3(BLOCK_REG), 0, c->stream, A);
+        } else {
+            if (D.kall == 42) hipLaunchKernelGGL((vi_kernel_reg<42, true, Src>), grid, dim3(BLOCK_REG), 0, c->stream, A);
+            else hipLaunchKernelGGL((vi_kernel_reg<98, true, Src>), grid, dim3(BLOCK_REG), 0, c->stream, A);
+        }
     }
     NICAM_HIP(hipGetLastError());
     return NICAM_OK;
 }
 
 }  // namespace
+
+int launch_vi_factor(nicam_ctx c)
+{
+    const Dims &D = c->d;
+    FactorArgs A;
+    A.reg = FactorPart{c->Mc, c->Ml, c->Mu, c->vi_beta, c->vi_gam, D.gall, D.gall * D.lall};
+    A.pl = FactorPart{c->Mc_pl, c->Ml_pl, c->Mu_pl, c->vi_beta_pl, c->vi_gam_pl, D.gall_pl,
+                      D.have_pl ? D.gall_pl * kLallPl : 0};
+    A.kall = D.kall;
+    const long n = (long)A.reg.ncols + A.pl.ncols;
+    hipLaunchKernelGGL(vi_factor_kernel, dim3((unsigned)((n + BLOCK_FAC - 1) / BLOCK_FAC)), dim3(BLOCK_FAC), 0,
+                       c->stream, A);
+    NICAM_HIP(hipGetLastError());
+    return NICAM_OK;
+}
 
 int launch_vi(nicam_ctx c, double *w, double *w_pl, const double *w0, const double *w0_pl,
               const double *p0, const double *p0_pl, const double *r0, const double *r0_pl,
@@ -339,6 +432,27 @@ int launch_vi_mixed(nicam_ctx c, double *w, double *w_pl, const float *w0, const
 {
     return launch_vi_t<float>(c, w, w_pl, w0, w0_pl, p0, p0_pl, r0, r0_pl, Sr, Sr_pl, Sw, Sw_pl, Sp, Sp_pl, dt,
                               c->RGAM32, c->RGAM32_pl, c->RGAMH32, c->RGAMH32_pl, c->RGSGAM2_32, c->RGSGAM2_32_pl);
+}
+
+int launch_vi_group_mixed(const nicam_ctx *cs, int n, const nicam_tracer_step_f32 *g, double dt, hipStream_t stream)
+{
+    if (n < 1 || n > kMaxGroup) return fail(NICAM_ERR_ARG, "a group holds 1..8 grids");
+    ViGroup<float> G;
+    G.n = n;
+    G.off[0] = 0;
+    for (int i = 0; i < n; ++i) {
+        const nicam_ctx c = cs[i];
+        const nicam_tracer_step_f32 &p = g[i];
+        vi_make_args<float>(c, G.a[i], p.rhogw, p.rhogw_pl, p.rhogw0, p.rhogw0_pl, p.preg0, p.preg0_pl, p.rhog0,
+                            p.rhog0_pl, p.Srho, p.Srho_pl, p.Sw, p.Sw_pl, p.Spre, p.Spre_pl, dt, c->RGAM32,
+                            c->RGAM32_pl, c->RGAMH32, c->RGAMH32_pl, c->RGSGAM2_32, c->RGSGAM2_32_pl);
+        const long nb = (G.a[i].ncol + BLOCK_FAC - 1) / BLOCK_FAC;
+        if ((long)G.off[i] + nb > 0x7fffffffL) return fail(NICAM_ERR_ARG, "group too large for one launch");
+        G.off[i + 1] = G.off[i] + (int)nb;
+    }
+    hipLaunchKernelGGL(vi_group_kernel<float>, dim3((unsigned)G.off[n]), dim3(BLOCK_FAC), 0, stream, G);
+    NICAM_HIP(hipGetLastError());
+    return NICAM_OK;
 }
 
 }  // namespace nicam

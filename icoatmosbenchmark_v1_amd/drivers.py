@@ -136,3 +136,75 @@ def run_divdamp(ctx: Context, spec: GridSpec, f: Dict[str, np.ndarray], fill: fl
     a = _prep(ctx, {**outs, **{k: f[k] for k in DDMP_IN}}, device)
     ctx.oprt3d_divdamp(*[a[k] for k in DDMP_OUT], *[a[k] for k in DDMP_IN])
     return _back(ctx, {k: a[k] for k in outs})
+
+
+# ---- one chained tracer step, FP64 or mixed precision ------------------------
+def chain_state(ctx: Context, spec: GridSpec, case: Dict[str, Dict[str, np.ndarray]], mixed: bool, device="cuda:0"):
+    """Device tensors of one chained step on the case's inputs: the flux's
+    (rho, rho*v), the limiters' (q, d, ch, cmask, Qout_post, ck, pre-limit
+    q_a and q_h), the vertical mass flux flx_v (from the lower-face Courant
+    number: flx_v(k) = -ck(k,1) rhog(k) / rdgz(k)), the rho*w solve's, and the
+    tracer mass rhogq = rho*q.  With mixed=True every input of the flux, the
+    limiters and the solve's sources is rounded to FP32; rhogq and rhogw stay
+    FP64 (the accumulated state)."""
+    import torch
+
+    dt = torch.float32 if mixed else torch.float64
+
+    def up(x, typ=dt):
+        return torch.from_numpy(np.ascontiguousarray(x)).to(device=device, dtype=typ)
+
+    f, h, v, vi = case["flux"], case["hlim"], case["vlim"], case["vi"]
+    S = {k: up(f[k]) for k in FLUX_IN}
+    S.update({"h_" + k: up(h[k]) for k in HLIM_IN + ("q_a", "q_a_pl")})
+    S.update({"v_" + k: up(v[k]) for k in VLIM_IN + ("q_h", "q_h_pl")})
+    z = lambda shape, typ=dt: torch.zeros(shape, dtype=typ, device=device)  # noqa: E731
+    S.update(flx_h=z((6,) + spec.shape3), flx_h_pl=z(spec.shape_pl), GRD_xc=z((3, 3) + spec.shape3),
+             GRD_xc_pl=z((3,) + spec.shape_pl), Qout_prev=z((2,) + spec.shape3), Qout_prev_pl=z((2,) + spec.shape_pl))
+    rdgz = 1.0 / np.diff(vi["gzh"], append=2 * vi["gzh"][-1] - vi["gzh"][-2])  # 1/(gzh(k+1)-gzh(k))
+    fv = -v["ck"][0] * f["rho"] / rdgz[None, :, None]
+    fv_pl = -v["ck_pl"][0] * f["rho_pl"] / rdgz[None, :, None]
+    S.update(flx_v=up(fv), flx_v_pl=up(fv_pl))
+    S.update({k + s: up(vi[k + s]) for k in VI_IN for s in ("", "_pl")})
+    S.update(rhogw=up(vi["rhogw"], torch.float64), rhogw_pl=up(vi["rhogw_pl"], torch.float64))
+    S.update(rhogq=up(f["rho"] * h["q"], torch.float64), rhogq_pl=up(f["rho_pl"] * h["q_pl"], torch.float64))
+    S["mixed"] = mixed
+    return S
+
+
+def chain_step(ctx: Context, S, dt_large: float, dt_small: float) -> None:
+    """horizontal_flux -> horizontal_limiter_thuburn -> the conservative
+    horizontal update of rhogq -> vertical_limiter_thuburn -> the vertical
+    update of rhogq -> vi_rhow_solver, on the tensors of chain_state.  Mixed:
+    the _f32 flux and limiters, FP64 accumulation of rhogq (the _f32 updates)
+    and the mixed rho*w solve."""
+    m = S["mixed"]
+    sfx = "_f32" if m else ""
+    getattr(ctx, "horizontal_flux" + sfx)(S["flx_h"], S["flx_h_pl"], S["GRD_xc"], S["GRD_xc_pl"],
+                                          *[S[k] for k in FLUX_IN], dt_large)
+    getattr(ctx, "horizontal_limiter_thuburn" + sfx)(
+        S["h_q_a"], S["h_q_a_pl"], S["h_q"], S["h_q_pl"], S["h_d"], S["h_d_pl"], S["h_ch"], S["h_ch_pl"],
+        S["h_cmask"], S["h_cmask_pl"], S["Qout_prev"], S["Qout_prev_pl"], S["h_Qout_post"], S["h_Qout_post_pl"])
+    ctx.tracer_update_conservative(S["rhogq"], S["rhogq_pl"], S["flx_h"], S["h_q_a"], single=m)
+    getattr(ctx, "vertical_limiter_thuburn" + sfx)(S["v_q_h"], S["v_q_h_pl"], S["v_q"], S["v_q_pl"], S["v_d"],
+                                                   S["v_d_pl"], S["v_ck"], S["v_ck_pl"])
+    getattr(ctx, "tracer_update_vertical" + sfx)(S["rhogq"], S["rhogq_pl"], S["flx_v"], S["flx_v_pl"], S["v_q_h"],
+                                                 S["v_q_h_pl"])
+    args = []
+    for k in VI_IN:
+        args += [S[k], S[k + "_pl"]]
+    getattr(ctx, "vi_rhow_solver_mixed" if m else "vi_rhow_solver")(S["rhogw"], S["rhogw_pl"], *args, dt_small)
+
+
+def total_mass(ctx: Context, rhogq, rhogq_pl, gzh: np.ndarray) -> float:
+    """The global tracer mass sum_k dz(k) * M(k) over kmin..kmax, M(k) the
+    FP64 area sums of nicam_tracer_mass; the level sum exact (math.fsum)."""
+    import math
+
+    import torch
+
+    m = torch.zeros(ctx.kall, dtype=torch.float64, device=rhogq.device)
+    ctx.tracer_mass(rhogq, rhogq_pl, m)
+    mk = m.cpu().numpy()
+    dz = np.diff(gzh, append=2 * gzh[-1] - gzh[-2])
+    return math.fsum(float(mk[k]) * float(dz[k]) for k in range(1, ctx.kall - 1))

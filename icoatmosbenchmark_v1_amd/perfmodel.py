@@ -145,3 +145,36 @@ def operator_bytes(spec: GridSpec, word: int = 8) -> Dict[str, int]:
     glue = dict(tracer_courant=19 * cl * word, oprt_gradient=(4 * cl + 21 * gall * lall) * word,
                 horizontal_remap=(25 * cl + 3 * gall * lall) * word, tracer_update=14 * cl * word)
     return dict(oprt_diffusion=diff, oprt3d_divdamp=ddmp, **glue)
+
+
+def batched_bytes(spec: GridSpec, ntr: int, word: int = 4) -> Dict[str, int]:
+    """Algorithmic bytes of the batched step of one grid (ntr tracers): the
+    flux and the mixed rho*w solve once; the limiters' shared fields (d, ch
+    on the interior, cmask on the arc origins; d and the two ck planes) once,
+    and their per-tracer streams (kernel_bytes' other terms) ntr times."""
+    n, kall, lall, npl = spec.gall_1d, spec.kall, spec.lall, spec.gall_pl
+    gall, inner, arcs = n * n, (n - 2) ** 2, (n - 1) ** 2
+    ncol = gall * lall
+    pl = 2 if spec.have_pl else 0
+    ring = npl - 1
+    kb = kernel_bytes(spec, word)
+    h_shared = (inner * (1 + 6) + arcs * 3) * kall * lall * word + pl * (2 * npl * kall + ring * kall) * word
+    h_tracer = kb["horizontal_limiter_thuburn"] - h_shared
+    v_shared = ((kall - 2) + (kall - 1) + (kall - 2)) * (ncol + pl * npl) * word
+    v_tracer = kb["vertical_limiter_thuburn"] - v_shared
+    return dict(horizontal_flux=kb["horizontal_flux"], vi_rhow_solver=vi_mixed_bytes(spec),
+                horizontal_limiter_thuburn=h_shared + ntr * h_tracer,
+                vertical_limiter_thuburn=v_shared + ntr * v_tracer)
+
+
+def chain_bytes(spec: GridSpec, mixed: bool) -> Dict[str, int]:
+    """Algorithmic bytes of the chained step's kernels beyond the four of
+    kernel_bytes: the conservative horizontal update (reads rhogq, flx_h and
+    q_a faces 1..3, writes rhogq, plus AREA and RAREA per column), the
+    vertical update (reads rhogq, flx_v, q_h, writes rhogq) and the FP64 mass
+    sum (reads rhogq and AREA).  rhogq is FP64 in both precisions."""
+    w = 4 if mixed else 8
+    cl = spec.array_cell_levels() + (2 * spec.gall_pl * spec.kall if spec.have_pl else 0)
+    ncol = spec.gall * spec.lall
+    return dict(tracer_update_conservative=cl * (16 + 6 * w) + 2 * ncol * 8,
+                tracer_update_vertical=cl * (16 + 2 * w), tracer_mass=cl * 8 + ncol * 8)

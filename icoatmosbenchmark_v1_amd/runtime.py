@@ -66,6 +66,7 @@ SIGNATURES = {
     "nicam_const_setup": (c_int, [_P]),
     "nicam_set_const": (c_int, [_P, c_double, c_double]),
     "nicam_set_phys_const": (c_int, [_P, c_double, c_double, c_double, c_int]),
+    "nicam_set_const_f32": (c_int, [_P, c_float, c_float]),
     "nicam_grd_input_vgrid": (c_int, [c_char_p, c_int, _P, _P]),
     "nicam_grd_setup": (c_int, [_P, _P, _P, _P, _P, _P]),
     "nicam_set_vgrid_factors": (c_int, [_P, _P, _P, _P]),
@@ -114,6 +115,20 @@ SIGNATURES = {
     "nicam_horizontal_flux_adjoint": (c_int, [_P] * 21 + [c_double]),
     "nicam_horizontal_limiter_thuburn_adjoint": (c_int, [_P] * 23),
     "nicam_vertical_limiter_thuburn_adjoint": (c_int, [_P] * 15),
+    "nicam_tracer_mass": (c_int, [_P] * 4),
+    "nicam_tracer_mass_f32": (c_int, [_P] * 4),
+    "nicam_tracer_update_conservative": (c_int, [_P] * 5),
+    "nicam_tracer_update_conservative_f32": (c_int, [_P] * 5),
+    "nicam_tracer_update_vertical_f32": (c_int, [_P] * 7),
+    "nicam_horizontal_limiter_thuburn_batch": (c_int, [_P, c_int] + [_P] * 14),
+    "nicam_horizontal_limiter_thuburn_batch_f32": (c_int, [_P, c_int] + [_P] * 14),
+    "nicam_vertical_limiter_thuburn_batch": (c_int, [_P, c_int] + [_P] * 8),
+    "nicam_vertical_limiter_thuburn_batch_f32": (c_int, [_P, c_int] + [_P] * 8),
+    "nicam_group_create": (c_int, [_P, c_int, _P]),
+    "nicam_group_destroy": (c_int, [_P]),
+    "nicam_group_set_stream": (c_int, [_P, _P]),
+    "nicam_group_tracer_step_f32": (c_int, [_P, c_int, _P, c_float, c_double, c_int]),
+    "nicam_group_autotune": (c_int, [_P, c_int, _P, c_float, c_double, c_char_p, _P]),
 }
 
 
@@ -315,6 +330,10 @@ class Context:
 
     def set_const(self, eps: float, huge: float) -> None:
         _check(self.lib.nicam_set_const(self.h, eps, huge), "nicam_set_const")
+
+    def set_const_f32(self, eps: float, huge: float) -> None:
+        """CONST_EPS / CONST_HUGE of the _f32 entry points (default: the -DSINGLE build's)."""
+        _check(self.lib.nicam_set_const_f32(self.h, float(eps), float(huge)), "nicam_set_const_f32")
 
     def set_phys_const(self, grav=9.80616, rdry=287.0, cvdry=717.5, non_hydro_alpha=1) -> None:
         _check(self.lib.nicam_set_phys_const(self.h, grav, rdry, cvdry, non_hydro_alpha), "nicam_set_phys_const")
@@ -527,6 +546,23 @@ class Context:
                          (q_a_pl, "q_a_pl", P)])
         _check(self.lib.nicam_tracer_update(self.h, *a), "nicam_tracer_update")
 
+    def tracer_mass(self, rhogq, rhogq_pl, mass, single: bool = False) -> None:
+        """nicam_tracer_mass(_f32): mass[k] = FP64 sum of AREA*rhogq over the inner cells and pole centres."""
+        R, P = self.nreg, self.npl
+        a = self._args([(rhogq, "rhogq", R), (rhogq_pl, "rhogq_pl", P)], single=single)
+        a.append(_ptr(mass, self.mode, "mass", self.kall))
+        fn = "nicam_tracer_mass_f32" if single else "nicam_tracer_mass"
+        _check(getattr(self.lib, fn)(self.h, *a), fn)
+
+    def tracer_update_conservative(self, rhogq, rhogq_pl, flx_h, q_a, single: bool = False) -> None:
+        """nicam_tracer_update_conservative(_f32): the flux-form update with one transfer per arc."""
+        R, P = self.nreg, self.npl
+        a = [_ptr(rhogq, self.mode, "rhogq", R), None if not self.have_pl else _ptr(rhogq_pl, self.mode,
+                                                                                    "rhogq_pl", P)]
+        a += [_ptr(flx_h, self.mode, "flx_h", 6 * R, single), _ptr(q_a, self.mode, "q_a", 6 * R, single)]
+        fn = "nicam_tracer_update_conservative_f32" if single else "nicam_tracer_update_conservative"
+        _check(getattr(self.lib, fn)(self.h, *a), fn)
+
     def vertical_mass_flux(self, flx_v, flx_v_pl, rhogvx, rhogvx_pl, rhogvy, rhogvy_pl, rhogvz, rhogvz_pl,
                            rhogw, rhogw_pl, dt: float) -> None:
         """nicam_vertical_mass_flux: flx_v = rhogw_vm*dt on the interior half levels."""
@@ -556,6 +592,41 @@ class Context:
         a = self._args([(rhogq, "rhogq", R), (rhogq_pl, "rhogq_pl", P), (flx_v, "flx_v", R),
                         (flx_v_pl, "flx_v_pl", P), (q_h, "q_h", R), (q_h_pl, "q_h_pl", P)])
         _check(self.lib.nicam_tracer_update_vertical(self.h, *a), "nicam_tracer_update_vertical")
+
+    def horizontal_limiter_thuburn_batch(self, ntr, q_a, q_a_pl, q, q_pl, d, d_pl, ch, ch_pl, cmask, cmask_pl,
+                                         Qout_prev, Qout_prev_pl, Qout_post, Qout_post_pl, single: bool = False):
+        """nicam_horizontal_limiter_thuburn_batch(_f32): ntr tracers, tracer-last arrays, device pointers."""
+        R, P, n = self.nreg, self.npl, int(ntr)
+        spec = [(q_a, "q_a", 6 * R * n), (q_a_pl, "q_a_pl", P * n), (q, "q", R * n), (q_pl, "q_pl", P * n),
+                (d, "d", R), (d_pl, "d_pl", P), (ch, "ch", 6 * R), (ch_pl, "ch_pl", P), (cmask, "cmask", 6 * R),
+                (cmask_pl, "cmask_pl", P), (Qout_prev, "Qout_prev", 2 * R * n), (Qout_prev_pl, "Qout_prev_pl", 2 * P * n),
+                (Qout_post, "Qout_post", 2 * R * n), (Qout_post_pl, "Qout_post_pl", 2 * P * n)]
+        a = self._args(spec, single=single)
+        fn = "nicam_horizontal_limiter_thuburn_batch" + ("_f32" if single else "")
+        _check(getattr(self.lib, fn)(self.h, n, *a), fn)
+
+    def vertical_limiter_thuburn_batch(self, ntr, q_h, q_h_pl, q, q_pl, d, d_pl, ck, ck_pl, single: bool = False):
+        """nicam_vertical_limiter_thuburn_batch(_f32): ntr tracers, tracer-last arrays, device pointers."""
+        R, P, n = self.nreg, self.npl, int(ntr)
+        spec = [(q_h, "q_h", R * n), (q_h_pl, "q_h_pl", P * n), (q, "q", R * n), (q_pl, "q_pl", P * n),
+                (d, "d", R), (d_pl, "d_pl", P), (ck, "ck", 2 * R), (ck_pl, "ck_pl", 2 * P)]
+        a = self._args(spec, single=single)
+        fn = "nicam_vertical_limiter_thuburn_batch" + ("_f32" if single else "")
+        _check(getattr(self.lib, fn)(self.h, n, *a), fn)
+
+    def horizontal_limiter_thuburn_batch_f32(self, ntr, *a):
+        self.horizontal_limiter_thuburn_batch(ntr, *a, single=True)
+
+    def vertical_limiter_thuburn_batch_f32(self, ntr, *a):
+        self.vertical_limiter_thuburn_batch(ntr, *a, single=True)
+
+    def tracer_update_vertical_f32(self, rhogq, rhogq_pl, flx_v, flx_v_pl, q_h, q_h_pl) -> None:
+        """nicam_tracer_update_vertical_f32: FP32 flx_v and q_h, FP64 rhogq."""
+        R, P = self.nreg, self.npl
+        a = self._args([(rhogq, "rhogq", R), (rhogq_pl, "rhogq_pl", P)])
+        a += self._args([(flx_v, "flx_v", R), (flx_v_pl, "flx_v_pl", P), (q_h, "q_h", R), (q_h_pl, "q_h_pl", P)],
+                        single=True)
+        _check(self.lib.nicam_tracer_update_vertical_f32(self.h, *a), "nicam_tracer_update_vertical_f32")
 
     def vi_rhow_update_matrix(self, eth, eth_pl, g_tilde, g_tilde_pl, dt: float) -> None:
         """nicam_vi_rhow_update_matrix: Mc/Ml/Mu of the context from h and g~ at the half levels."""

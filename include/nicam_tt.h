@@ -76,9 +76,14 @@ int nicam_ctx_set_pointer_mode(nicam_ctx ctx, int mode);
 /* Launch-shape parameters (all default to 0 = library's choice):
  *   NICAM_TUNE_HFLUX_KPER   levels per work-group of horizontal_flux
  *   NICAM_TUNE_VLIM_KCHUNK  levels per work-item of the vertical limiter
- *   NICAM_TUNE_VI_VARIANT   storage of the Thomas elimination factors:
- *                           1 = LDS, 2 = device work array, 3 = registers
- *                           (kall = 42 or 98, else automatic), 0 = automatic
+ *   NICAM_TUNE_VI_VARIANT   the Thomas elimination of vi_rhow_solver:
+ *                           1 = factors formed in the solve, kept in LDS;
+ *                           2 = factors read from the factorisation that
+ *                           putcoef stores, forward solution through rhogw;
+ *                           3 = factors formed in the solve, column in
+ *                           registers; 4 = stored factors, forward solution
+ *                           in registers (3 and 4: kall = 42 or 98, else
+ *                           automatic); 0 = automatic
  *   NICAM_TUNE_DIFF_KPER    levels per work-group of nicam_oprt_diffusion
  *   NICAM_TUNE_DDMP_KPER    levels per work-group of nicam_oprt3d_divdamp
  * Results do not depend on these settings (every variant computes the same
@@ -99,6 +104,14 @@ size_t nicam_ctx_device_bytes(nicam_ctx ctx);
 int nicam_const_setup(nicam_ctx ctx);
 /* Direct assignment of CONST_EPS / CONST_HUGE (problem_size.inc:54-55). */
 int nicam_set_const(nicam_ctx ctx, double eps, double huge_val);
+/* CONST_EPS / CONST_HUGE of the _f32 entry points.  The default is what
+ * CONST_setup (mod_misc.f90:447-461) gives the reference built with -DSINGLE:
+ * epsilon(0._SP) = 2^-23 and huge(0._SP).  Setting eps to FP64's epsilon
+ * (2^-52, representable in FP32) makes the limiters' zerosw switches
+ * (dyn_horiz_adv_limiter/src/mod_src_tracer.f90:274,
+ * dyn_vert_adv_limiter/src/mod_src_tracer.f90:131,170,217) decide as the FP64
+ * path does, which the mixed-precision step uses. */
+int nicam_set_const_f32(nicam_ctx ctx, float eps, float huge_val);
 /* CONST_GRAV, CONST_Rdry, CONST_CVdry, NON_HYDRO_ALPHA (problem_size.inc:56-59). */
 int nicam_set_phys_const(nicam_ctx ctx, double grav, double rdry, double cvdry,
                          int non_hydro_alpha);
@@ -628,6 +641,117 @@ int nicam_vertical_limiter_thuburn_adjoint(nicam_ctx ctx,
                                            const double *d, const double *d_pl,
                                            const double *ck, const double *ck_pl,
                                            const uint32_t *mask, const uint32_t *mask_pl);
+
+/* ---- FP64 global tracer mass and the update that conserves it ----
+ * Neither has a counterpart in the reference, whose kernels stop at the face
+ * values; they complete the flux-form step the reference's kernels feed
+ * (flx_h of dyn_horiz_adv_flux/src/mod_src_tracer.f90:234-309, the limited
+ * q_a of dyn_horiz_adv_limiter/src/mod_src_tracer.f90:377-410). */
+
+/* mass[k] (kall doubles, device or host per the pointer mode) = the sum over
+ * the inner cells of every region of GMTR_p(AREA) * rhogq(k), plus both pole
+ * centres' AREA * rhogq_pl(1,k).  Products and sums in FP64, in a fixed order
+ * (the same result on every run).  The first call allocates a kall*lall
+ * partial-sum buffer that the context keeps.  The _f32 form reads FP32
+ * rhogq and widens it exactly. */
+int nicam_tracer_mass(nicam_ctx ctx, const double *rhogq, const double *rhogq_pl, double *mass);
+int nicam_tracer_mass_f32(nicam_ctx ctx, const float *rhogq, const float *rhogq_pl, double *mass);
+
+/* rhogq -= RAREA * (sum of the six outward face transfers), in FP64, on the
+ * inner cells and the pole centres (halos untouched).  Each arc's transfer
+ * AREA(o)*flx_h(o,a)*q_a(o,a) is taken from its origin o (a = 1..3) and used
+ * with opposite signs by its two cells; a face two regions share takes the
+ * lower-indexed region's transfer, and the pole centres take the transfers of
+ * the region faces that point at them, so the global sum of AREA*rhogq is
+ * conserved to the rounding of the FP64 operations.  flx_h and q_a are
+ * (gall,kall,lall,6); only faces 1-3 are read, and the pole arrays are not
+ * read.  Needs nicam_comm_setup (all regions on this GPU).  The _f32 form
+ * takes FP32 flx_h and q_a (the mixed step) and accumulates in FP64. */
+int nicam_tracer_update_conservative(nicam_ctx ctx, double *rhogq, double *rhogq_pl,
+                                     const double *flx_h, const double *q_a);
+int nicam_tracer_update_conservative_f32(nicam_ctx ctx, double *rhogq, double *rhogq_pl,
+                                         const float *flx_h, const float *q_a);
+
+/* nicam_tracer_update_vertical with FP32 flx_v and q_h and an FP64 rhogq:
+ * each half level's transfer flx_v*q_h is formed once in FP64 and used by the
+ * levels on both sides, so column mass is conserved to FP64 rounding. */
+int nicam_tracer_update_vertical_f32(nicam_ctx ctx, double *rhogq, double *rhogq_pl,
+                                     const float *flx_v, const float *flx_v_pl,
+                                     const float *q_h, const float *q_h_pl);
+
+/* ---- many tracers, several grids: the batched tracer step ----
+ * Batches the reference's per-call loop (dyn_horiz_adv_limiter/src/main.f90:208-214
+ * calls horizontal_limiter_thuburn for one tracer at a time): the limiters
+ * take ntr tracers in NICAM's tracer-last layout and read the fields the
+ * tracers share (d, ch, cmask; d, ck) once per cell-level. */
+
+/* horizontal_limiter_thuburn of ntr tracers: q_a(gall,kall,lall,6,ntr),
+ * q(gall,kall,lall,ntr), Qout_prev and Qout_post (gall,kall,lall,2,ntr), the
+ * pole arrays with the same trailing ntr dimension; d, ch, cmask as in the
+ * single-tracer call.  Each tracer's result is the single-tracer call's, bit
+ * for bit.  Qout_post = NULL exchanges all tracers' Qout on the device.
+ * Device pointers only. */
+int nicam_horizontal_limiter_thuburn_batch(nicam_ctx ctx, int ntr,
+                                           double *q_a, double *q_a_pl, const double *q, const double *q_pl,
+                                           const double *d, const double *d_pl, const double *ch, const double *ch_pl,
+                                           const double *cmask, const double *cmask_pl,
+                                           double *Qout_prev, double *Qout_prev_pl,
+                                           const double *Qout_post, const double *Qout_post_pl);
+int nicam_horizontal_limiter_thuburn_batch_f32(nicam_ctx ctx, int ntr,
+                                               float *q_a, float *q_a_pl, const float *q, const float *q_pl,
+                                               const float *d, const float *d_pl, const float *ch, const float *ch_pl,
+                                               const float *cmask, const float *cmask_pl,
+                                               float *Qout_prev, float *Qout_prev_pl,
+                                               const float *Qout_post, const float *Qout_post_pl);
+/* vertical_limiter_thuburn of ntr tracers: q_h and q (gall,kall,lall,ntr),
+ * pole arrays likewise; d and ck shared.  Device pointers only. */
+int nicam_vertical_limiter_thuburn_batch(nicam_ctx ctx, int ntr, double *q_h, double *q_h_pl,
+                                         const double *q, const double *q_pl, const double *d, const double *d_pl,
+                                         const double *ck, const double *ck_pl);
+int nicam_vertical_limiter_thuburn_batch_f32(nicam_ctx ctx, int ntr, float *q_h, float *q_h_pl,
+                                             const float *q, const float *q_pl, const float *d, const float *d_pl,
+                                             const float *ck, const float *ck_pl);
+
+/* The arrays of one grid's mixed-precision tracer step (device pointers):
+ * the FP32 flux, the FP32 limiters of ntr tracers (layouts as in the _batch
+ * calls; qv/dv are the vertical limiter's q and d) and the mixed rho*w solve
+ * (FP32 state and sources, FP64 rhogw). */
+typedef struct nicam_tracer_step_f32 {
+    float *flx_h, *flx_h_pl, *GRD_xc, *GRD_xc_pl;
+    const float *rho, *rho_pl, *rhovx, *rhovx_pl, *rhovy, *rhovy_pl, *rhovz, *rhovz_pl;
+    float *q_a, *q_a_pl;
+    const float *q, *q_pl, *d, *d_pl, *ch, *ch_pl, *cmask, *cmask_pl;
+    float *Qout_prev, *Qout_prev_pl;
+    const float *Qout_post, *Qout_post_pl;
+    float *q_h, *q_h_pl;
+    const float *qv, *qv_pl, *dv, *dv_pl, *ck, *ck_pl;
+    double *rhogw, *rhogw_pl;
+    const float *rhogw0, *rhogw0_pl, *preg0, *preg0_pl, *rhog0, *rhog0_pl;
+    const float *Srho, *Srho_pl, *Sw, *Sw_pl, *Spre, *Spre_pl;
+} nicam_tracer_step_f32;
+
+/* A group of 1..8 grids (contexts on the current device) advanced together:
+ * each kernel of the step runs once over every grid and tracer.  Launches
+ * go on the group's stream (set one; the null stream cannot be captured). */
+typedef struct nicam_group_s *nicam_group;
+int nicam_group_create(nicam_group *grp, int n, const nicam_ctx *ctxs);
+int nicam_group_destroy(nicam_group grp);
+int nicam_group_set_stream(nicam_group grp, void *hip_stream);
+/* One step on every grid: horizontal_flux_f32, the horizontal and vertical
+ * limiters of ntr tracers, vi_rhow_solver_mixed -- four launches in all.
+ * use_graph != 0: the first call captures the four launches into a HIP graph
+ * that later calls with the same arguments replay (different arguments
+ * recapture).  Each grid's outputs equal its single-grid calls bit for bit. */
+int nicam_group_tracer_step_f32(nicam_group grp, int ntr, const nicam_tracer_step_f32 *grids,
+                                float dt_large, double dt_small, int use_graph);
+/* Picks each grid's launch shapes (levels per work-group of the flux, levels
+ * per work-item of the vertical limiter, the rho*w elimination variant) by
+ * timing the candidates on the given arrays (their outputs are overwritten),
+ * and sets them with nicam_ctx_set_tuning.  With cache_path, the choices are
+ * read from that text file when it holds this device and shape, else timed
+ * and appended to it.  n_timed (may be NULL) returns the candidates timed. */
+int nicam_group_autotune(nicam_group grp, int ntr, const nicam_tracer_step_f32 *grids, float dt_large,
+                         double dt_small, const char *cache_path, int *n_timed);
 
 #ifdef __cplusplus
 }

@@ -653,6 +653,14 @@ class RefLibsSP:
         self.hlim.ref_horizontal_limiter_thuburn.argtypes = [c_int] * 5 + [P] * 15
         self.vlim.ref_vertical_limiter_thuburn.argtypes = [c_int] * 5 + [P] * 8
         self.hlim.ref_const_setup.argtypes = [POINTER(c_double), POINTER(c_double)]
+        for lib in (self.flux, self.hlim, self.vlim):
+            lib.ref_set_eps.argtypes = [c_double]
+
+    def set_eps(self, eps):
+        """CONST_EPS installed after each CONST_setup of the three SINGLE
+        builds (rounded to SP there); None restores epsilon(0._SP)."""
+        for lib in (self.flux, self.hlim, self.vlim):
+            lib.ref_set_eps(-1.0 if eps is None else float(eps))
 
     def const_setup(self):
         """CONST_EPS, CONST_HUGE of the SINGLE build."""

@@ -15,8 +15,11 @@ module ref_harness_common
   private
   public :: harness_set_sizes
   public :: ref_const_setup
+  public :: ref_set_eps
 
   logical, save :: log_opened = .false.
+  ! CONST_EPS to install after CONST_setup; negative = keep CONST_setup's
+  real(c_double), save :: eps_override = -1.0_c_double
 
 contains
 
@@ -46,7 +49,14 @@ contains
     ADM_have_sgp(1) = have_sgp
 
     call CONST_setup
+    if ( eps_override >= 0.0_c_double ) CONST_EPS = real( eps_override, RP )
   end subroutine harness_set_sizes
+
+  !> Replaces CONST_EPS after every CONST_setup (a negative value restores it)
+  subroutine ref_set_eps( eps ) bind(C, name='ref_set_eps')
+    real(c_double), value, intent(in) :: eps
+    eps_override = eps
+  end subroutine ref_set_eps
 
   !> CONST_EPS / CONST_HUGE as CONST_setup leaves them
   subroutine ref_const_setup( eps, big ) bind(C, name='ref_const_setup')

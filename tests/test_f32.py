@@ -153,11 +153,20 @@ def test_gpu_f32_host_pointer_mode(gpu, refsp):
     assert_parity(out, refsp.horizontal_flux(spec, case["geo"], case["flux"], 360.0), rtol=0.0)
 
 
+@pytest.fixture(scope="module")
+def gl8():
+    """The mixed-precision configuration's shape: the generated gl8/rl1 grid
+    (40 regions of 130x130, its GMTR metrics), 98 levels (62,914,752 inner
+    cell-levels per field), seeded fields."""
+    spec = synth.GridSpec(130, 98, 40, seed=2025)
+    geo = synth.icosahedral_geometry(8, 1)[0]
+    return spec, geo, synth.make_case(spec, geo=geo)
+
+
 @pytest.mark.gpu
-def test_gpu_mixed_gl8_sampled_regions(gpu, refsp):
-    """Mixed precision at glevel 8: the generated gl8/rl1 grid (40 regions of
-    130x130, its GMTR metrics), 10 levels.  The FP32 flux and limiters run on
-    the whole grid; three regions are checked bit for bit against the SINGLE
+def test_gpu_mixed_gl8_sampled_regions(gpu, refsp, gl8):
+    """The FP32 flux and limiters run on the whole gl8 grid at 98 levels;
+    three regions and the poles are checked bit for bit against the SINGLE
     reference run on just those regions, and the FP64 GPU path on the same
     grid (bit-identical to the FP64 oracle elsewhere) gives the error of every
     region and field."""
@@ -165,12 +174,10 @@ def test_gpu_mixed_gl8_sampled_regions(gpu, refsp):
 
     from parity import region_slice
 
-    spec = synth.GridSpec(130, 10, 40, seed=2025)
-    geo = synth.icosahedral_geometry(8, 1)[0]
-    case = synth.make_case(spec, geo=geo)
+    spec, geo, case = gl8
     ctx = drivers.setup_context(spec, geo)
     regions = [0, 21, 39]
-    sub = synth.GridSpec(130, 10, len(regions), seed=2025)
+    sub = synth.GridSpec(130, spec.kall, len(regions), seed=2025)
     pick = lambda d: region_slice(d, regions, pole=True)  # noqa: E731
     z = lambda shape: torch.zeros(shape, dtype=torch.float32, device=gpu)  # noqa: E731
     f = _dev32(None, case["flux"], gpu)
@@ -178,25 +185,69 @@ def test_gpu_mixed_gl8_sampled_regions(gpu, refsp):
                GRD_xc_pl=z((3,) + spec.shape_pl))
     ctx.horizontal_flux_f32(out["flx_h"], out["flx_h_pl"], out["GRD_xc"], out["GRD_xc_pl"],
                             *[f[k] for k in drivers.FLUX_IN], 360.0)
+    del f
     got = {k: v.cpu().numpy() for k, v in out.items()}
+    del out
     assert_parity(pick(got), refsp.horizontal_flux(sub, pick(geo), pick(case["flux"]), 360.0), rtol=0.0)
     dp = drivers.run_flux(ctx, spec, case["flux"], 360.0)
     assert _rel_err(got["flx_h"], dp["flx_h"]) < 5e-5 and _rel_err(got["GRD_xc"], dp["GRD_xc"]) < 1e-6
+    del got, dp
     h = _dev32(None, {k: case["hlim"][k] for k in drivers.HLIM_IN + ("q_a", "q_a_pl")}, gpu)
     h.update(Qout_prev=z((2,) + spec.shape3), Qout_prev_pl=z((2,) + spec.shape_pl))
     ctx.horizontal_limiter_thuburn_f32(h["q_a"], h["q_a_pl"], h["q"], h["q_pl"], h["d"], h["d_pl"], h["ch"],
                                        h["ch_pl"], h["cmask"], h["cmask_pl"], h["Qout_prev"], h["Qout_prev_pl"],
                                        h["Qout_post"], h["Qout_post_pl"])
     got = {k: h[k].cpu().numpy() for k in ("q_a", "q_a_pl", "Qout_prev", "Qout_prev_pl")}
+    del h
     assert_parity(pick(got), refsp.horizontal_limiter_thuburn(sub, pick(geo)["have_sgp"], pick(case["hlim"])),
                   rtol=0.0)
     dp = drivers.run_hlim(ctx, spec, case["hlim"])
     assert _rel_err(got["q_a"], dp["q_a"]) < 1e-5
+    del got, dp
     v = _dev32(None, {k: case["vlim"][k] for k in drivers.VLIM_IN + ("q_h", "q_h_pl")}, gpu)
     ctx.vertical_limiter_thuburn_f32(v["q_h"], v["q_h_pl"], v["q"], v["q_pl"], v["d"], v["d_pl"], v["ck"],
                                      v["ck_pl"])
     got = {k: v[k].cpu().numpy() for k in ("q_h", "q_h_pl")}
+    del v
     assert_parity(pick(got), refsp.vertical_limiter_thuburn(sub, pick(case["vlim"])), rtol=0.0)
+    ctx.close()
+    torch.cuda.empty_cache()
+
+
+@pytest.mark.gpu
+def test_gpu_mixed_gl8_chained_step_conserves_mass(gpu, gl8):
+    """One chained step at gl8 x 98 levels, FP64 and mixed (FP32 flux and
+    limiters with FP64's CONST_EPS, FP64 accumulation of rhogq, the mixed
+    rho*w solve): the FP64 global tracer mass is conserved to 1e-13 by both,
+    and the mixed step's updated tracer and rho*w lie within 1e-5 of the FP64
+    step's (max |mixed - f64| / max |f64| over each field)."""
+    import torch
+
+    spec, geo, case = gl8
+    ctx = drivers.setup_context(spec, geo, case["vi"])
+    ctx.comm_setup(1)
+    ctx.set_const_f32(EPS64, float(np.finfo(np.float32).max))
+    gzh = case["vi"]["gzh"]
+    res = {}
+    for mixed in (False, True):
+        S = drivers.chain_state(ctx, spec, case, mixed=mixed, device=gpu)
+        m0 = drivers.total_mass(ctx, S["rhogq"], S["rhogq_pl"], gzh)
+        drivers.chain_step(ctx, S, 360.0, 60.0)
+        m1 = drivers.total_mass(ctx, S["rhogq"], S["rhogq_pl"], gzh)
+        assert abs(m1 - m0) <= 1e-13 * abs(m0), (mixed, m0, m1)
+        res[mixed] = {k: S[k].double().cpu() for k in ("rhogq", "rhogw")}
+        del S
+        torch.cuda.empty_cache()
+    rho = torch.from_numpy(np.ascontiguousarray(case["flux"]["rho"]))
+    q = {m: res[m]["rhogq"] / rho for m in res}
+    inner = np.zeros((spec.gall_1d, spec.gall_1d), dtype=bool)
+    inner[1:-1, 1:-1] = True
+    inner = torch.from_numpy(inner.reshape(-1))
+    err_q = float(((q[True] - q[False])[..., inner]).abs().max() / q[False][..., inner].abs().max())
+    err_w = float((res[True]["rhogw"] - res[False]["rhogw"]).abs().max() / res[False]["rhogw"].abs().max())
+    print(f"gl8 mixed step: q error {err_q:.3e}, rho*w error {err_w:.3e}", flush=True)
+    assert err_q <= 1e-5 and err_w <= 1e-5
+    ctx.close()
 
 
 def test_f32_symbols_and_dtype_checks():
@@ -243,3 +294,85 @@ def test_gpu_vi_mixed_is_fp64_on_rounded_inputs(gpu, oracle, c):
         np.testing.assert_array_equal(w_pl.cpu().numpy(), want["rhogw_pl"])
     full = oracle.vi_rhow_solver(spec, vi, 60.0, g["rdgzh"], g["afact"], g["bfact"])
     assert _rel_err(w.cpu().numpy(), full["rhogw"]) < 1e-5
+
+
+# ---- the FP32 path with FP64's CONST_EPS (the mixed-precision step's mode) --
+EPS64 = float(np.finfo(np.float64).eps)
+
+
+def _tiny_outflow(case, spec):
+    """Cells whose only outflow face carries a Courant number of 1e-9: their
+    outflow sum lies between FP64's and FP32's epsilon, so the zerosw switch
+    (HLIM/src/mod_src_tracer.f90:274) decides by which epsilon is in force."""
+    h = dict(case["hlim"])
+    ch = h["ch"].copy()
+    n1 = spec.gall_1d
+    cells = [(0, 1, 3 * n1 + 4), (0, 2, 5 * n1 + 7), (1, 0, 6 * n1 + 3)]
+    q = h["q"].copy()
+    for c in cells:
+        q[c] = 0.5  # away from the tracer's exact zeros
+    h["q"] = q
+    for l, k, g in cells:
+        ch[:, l, k, g] = 0.0  # no inflow to cancel against: the outflow sum is exactly 1e-9 in FP32 too
+        ch[0, l, k, g] = 1e-9
+    h["ch"] = ch
+    h["cmask"] = np.where(ch < 0, 1.0, 0.0)  # the mask follows the sign (cmask = 1 on inflow)
+    return h, cells
+
+
+@pytest.mark.ref
+def test_fp64_eps_makes_the_single_reference_decide_as_double(refsp, reflibs):
+    spec, case = _case(CASES[1])
+    sgp = case["geo"]["have_sgp"]
+    h, cells = _tiny_outflow(case, spec)
+    dp = reflibs.horizontal_limiter_thuburn(spec, sgp, h)
+    try:
+        sp_default = refsp.horizontal_limiter_thuburn(spec, sgp, h)
+        refsp.set_eps(EPS64)
+        sp_eps64 = refsp.horizontal_limiter_thuburn(spec, sgp, h)
+    finally:
+        refsp.set_eps(None)
+    q = h["q"]
+    for l, k, g in cells:
+        # zerosw = 1 gives Qout = q exactly; zerosw = 0 divides by the tiny outflow
+        assert sp_default["Qout_prev"][0, l, k, g] == np.float32(q[l, k, g])
+        assert dp["Qout_prev"][0, l, k, g] != q[l, k, g]
+        assert sp_eps64["Qout_prev"][0, l, k, g] != np.float32(q[l, k, g])
+    # elsewhere the two FP32 modes agree: the switch only moves where the sums are that small
+    mask = np.ones(spec.shape3, dtype=bool)
+    for l, k, g in cells:
+        mask[l, k, g] = False
+    assert np.array_equal(sp_default["Qout_prev"][:, mask], sp_eps64["Qout_prev"][:, mask])
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("c", [CASES[1], CASES[2]], ids=[IDS[1], IDS[2]])
+def test_gpu_f32_fp64_eps_matches_the_single_reference(gpu, refsp, c):
+    """nicam_set_const_f32(2^-52, huge(0._SP)): the FP32 limiters equal the
+    SINGLE reference run with the same CONST_EPS, bit for bit."""
+    spec, case = _case(c)
+    sgp = case["geo"]["have_sgp"]
+    h_in, _ = _tiny_outflow(case, spec)
+    case = {**case, "hlim": {**case["hlim"], **h_in}}
+    ctx = drivers.setup_context(spec, case["geo"])
+    ctx.set_const_f32(EPS64, float(np.finfo(np.float32).max))
+    import torch
+
+    z = lambda shape: torch.zeros(shape, dtype=torch.float32, device=gpu)  # noqa: E731
+    h = _dev32(None, {**{k: h_in[k] for k in drivers.HLIM_IN}, "q_a": case["hlim"]["q_a"],
+                      "q_a_pl": case["hlim"]["q_a_pl"]}, gpu)
+    h.update(Qout_prev=z((2,) + spec.shape3), Qout_prev_pl=z((2,) + spec.shape_pl))
+    ctx.horizontal_limiter_thuburn_f32(h["q_a"], h["q_a_pl"], h["q"], h["q_pl"], h["d"], h["d_pl"], h["ch"],
+                                       h["ch_pl"], h["cmask"], h["cmask_pl"], h["Qout_prev"], h["Qout_prev_pl"],
+                                       h["Qout_post"], h["Qout_post_pl"])
+    v = _dev32(None, {k: case["vlim"][k] for k in drivers.VLIM_IN + ("q_h", "q_h_pl")}, gpu)
+    ctx.vertical_limiter_thuburn_f32(v["q_h"], v["q_h_pl"], v["q"], v["q_pl"], v["d"], v["d_pl"], v["ck"],
+                                     v["ck_pl"])
+    try:
+        refsp.set_eps(EPS64)
+        want_h = refsp.horizontal_limiter_thuburn(spec, sgp, {**case["hlim"], **h_in})
+        want_v = refsp.vertical_limiter_thuburn(spec, case["vlim"])
+    finally:
+        refsp.set_eps(None)
+    assert_parity({k: h[k].cpu().numpy() for k in ("q_a", "q_a_pl", "Qout_prev", "Qout_prev_pl")}, want_h, rtol=0.0)
+    assert_parity({k: v[k].cpu().numpy() for k in ("q_h", "q_h_pl")}, want_v, rtol=0.0)

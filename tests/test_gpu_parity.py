@@ -166,7 +166,7 @@ def test_icosahedral_gl5_all_regions(gpu, oracle):
 from icoatmosbenchmark_v1_amd.runtime import TUNE_HFLUX_KPER, TUNE_VI_VARIANT, TUNE_VLIM_KCHUNK  # noqa: E402
 
 
-@pytest.mark.parametrize("variant", [1, 2, 3])
+@pytest.mark.parametrize("variant", [1, 2, 3, 4])
 @pytest.mark.parametrize("c", [(34, 42, 2, 21, [1, 0], True), (18, 13, 2, 22, [0, 1], True),
                                (18, 98, 2, 25, [1, 0], True)], ids=["k42", "k13", "k98"])
 def test_vi_variants(gpu, oracle, variant, c):
