@@ -370,7 +370,10 @@ int launch_vi_t(nicam_ctx c, double *w, double *w_pl, const Src *w0, const Src *
     int variant = c->tune.vi_variant;
     const bool reg_ok = (D.kall == 42 || D.kall == 98);
     if ((variant == 3 || variant == 4) && !reg_ok) variant = 0;
-    if (variant == 0) variant = reg_ok ? 4 : 2;
+    // measured at gl7 (40 x 66x66 x 98): FP64 inputs: 4 (factored, registers) 366 us, 3 380, 2 462, 1 597;
+    // FP32 inputs: 2 (factored, through rhogw) 323 us, 3 and 4 557 (the widening loads lengthen the
+    // register variants' one-wave-per-SIMD chain), 1 566
+    if (variant == 0) variant = (reg_ok && !std::is_same<Src, float>::value) ? 4 : 2;
 
     if (variant == 1) {
         const size_t lds = sizeof(double) * BLOCK_LDS * D.kall;

@@ -23,6 +23,7 @@ ORACLE_SO = HERE / "liboracle.so"
 REF_DIR = HERE / "_ref"
 REF_LIBS = {k: REF_DIR / f"libref_{k}.so" for k in ("flux", "hlim", "vlim", "vi", "diff", "ddmp", "gmtr")}
 REF_LIBS_SP = {k: REF_DIR / f"libref_{k}_sp.so" for k in ("flux", "hlim", "vlim")}
+REF_LIBS_OMP = {k: REF_DIR / f"libref_{k}_omp.so" for k in ("flux", "hlim", "vlim", "vi")}
 
 
 # inputs of OPRT_diffusion in argument order (dyn_diffusion/src/mod_oprt.f90:41-46)
@@ -443,16 +444,21 @@ class RefLibs:
     call on a large-stack thread."""
 
     @staticmethod
-    def available() -> bool:
-        return all(p.exists() for p in REF_LIBS.values())
+    def available(omp: bool = False) -> bool:
+        return all(p.exists() for p in REF_LIBS.values()) and (not omp or all(p.exists() for p in
+                                                                               REF_LIBS_OMP.values()))
 
-    def __init__(self):
-        if not self.available():
+    def __init__(self, omp: bool = False):
+        """omp: the four hot-path kernels from the -fopenmp builds, whose own
+        OpenMP loops use set_threads(n) threads."""
+        if not self.available(omp):
             raise FileNotFoundError("oracle/_ref not built (make -C oracle ref, needs the reference tree)")
-        self.flux = ctypes.CDLL(str(REF_LIBS["flux"]))
-        self.hlim = ctypes.CDLL(str(REF_LIBS["hlim"]))
-        self.vlim = ctypes.CDLL(str(REF_LIBS["vlim"]))
-        self.vi = ctypes.CDLL(str(REF_LIBS["vi"]))
+        libs = REF_LIBS_OMP if omp else REF_LIBS
+        self.flux = ctypes.CDLL(str(libs["flux"]))
+        self.hlim = ctypes.CDLL(str(libs["hlim"]))
+        self.vlim = ctypes.CDLL(str(libs["vlim"]))
+        self.vi = ctypes.CDLL(str(libs["vi"]))
+        self.omp = omp
         P = c_void_p
         self.flux.ref_horizontal_flux.argtypes = [c_int] * 5 + [P] * 13 + [c_double] + [P] * 8
         self.hlim.ref_horizontal_limiter_thuburn.argtypes = [c_int] * 5 + [P] * 15
@@ -469,6 +475,17 @@ class RefLibs:
         self.gmtr.ref_oprt_diffusion_setup.argtypes = [c_int] * 4 + [P] * 11
         self.gmtr.ref_oprt_vector_setup.argtypes = [c_int] * 5 + [P] * 9
         self.gmtr.ref_oprt_laplacian_setup.argtypes = [c_int] * 4 + [P] * 9
+
+    def set_threads(self, n: int) -> None:
+        """OpenMP threads of the -fopenmp builds (omp_set_num_threads of the
+        OpenMP runtime they link)."""
+        if not self.omp:
+            raise RuntimeError("the single-thread builds have no OpenMP runtime")
+        for lib in (self.flux, self.hlim, self.vlim, self.vi):
+            ctypes.CDLL(None)  # noqa: the runtime is already loaded with the libraries
+        rt = ctypes.CDLL("libomp.so", mode=ctypes.RTLD_GLOBAL)
+        rt.omp_set_num_threads(int(n))
+        self._rt = rt
 
     def oprt_coef_setup(self, which, m, have_sgp, have_pl=True):
         """OPRT_{divergence,rotation,gradient,laplacian}_setup

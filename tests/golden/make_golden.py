@@ -2,7 +2,12 @@
 Fortran kernels (oracle/_ref, compiled from the reference tree by
 oracle/Makefile).
 
-    python tests/golden/make_golden.py
+    python tests/golden/make_golden.py [fp64] [f32] [oprt]
+
+fp64: the four hot-path kernels and GRD_setup in double precision; f32: the
+flux and both limiters of the reference built with -DSINGLE, with its own
+CONST_EPS and with FP64's (the library's mixed-precision setting); oprt:
+OPRT_diffusion and OPRT3D_divdamp.  With no argument all three are written.
 
 Each fixture holds the seeded inputs of one small case and the outputs the
 reference produced for them, for the four hot-path kernels and GRD_setup.
@@ -31,9 +36,99 @@ CASES = [
 VGRID = REPO / "tests" / "golden" / "vgrid40_600m_24km.dat"
 
 
+# FP32 cases: (name, gall_1d, kall, lall, seed, have_sgp, have_pl)
+CASES_F32 = [
+    ("f32_g6k5l2", 6, 5, 2, 1, [1, 0], True),
+    ("f32_g18k12l2", 18, 12, 2, 3, [0, 1], True),
+    ("f32_g10k6l2_nopl", 10, 6, 2, 7, [0, 0], False),
+]
+# operator cases: (name, gall_1d, kall, lall, seed, have_sgp, have_pl)
+CASES_OPRT = [
+    ("oprt_g10k8l3", 10, 8, 3, 2, [0, 1, 0], True),
+    ("oprt_g18k6l1_nopl", 18, 6, 1, 3, [1], False),
+]
+EPS64 = float(np.finfo(np.float64).eps)
+
+
+def tiny_outflow(case, spec):
+    """Three cells whose only outflow face carries a Courant number of 1e-9
+    (between FP64's and FP32's epsilon), with q = 0.5 there: the zerosw switch
+    decides by which epsilon is in force."""
+    h = dict(case["hlim"])
+    ch, q = h["ch"].copy(), h["q"].copy()
+    n1 = spec.gall_1d
+    for l, k, g in [(0, 1, 2 * n1 + 2), (0, min(2, spec.kall - 1), 2 * n1 + 3), (spec.lall - 1, 0, 3 * n1 + 2)]:
+        ch[:, l, k, g] = 0.0
+        ch[0, l, k, g] = 1e-9
+        q[l, k, g] = 0.5
+    h["ch"], h["q"] = ch, q
+    h["cmask"] = np.where(ch < 0, 1.0, 0.0)
+    return h
+
+
+def write_f32() -> None:
+    sp = binding.RefLibsSP()
+    for name, n, kall, lall, seed, sgp, pl in CASES_F32:
+        spec = synth.GridSpec(n, kall, lall, have_pl=pl, seed=seed)
+        case = synth.make_case(spec, have_sgp=sgp)
+        out = {"geo/" + k: v for k, v in case["geo"].items() if k != "points"}
+        f32 = lambda d: {k: np.asarray(v, dtype=np.float32) for k, v in d.items()}  # noqa: E731
+        h_eps = tiny_outflow(case, spec)
+        for kern, d in (("flux", case["flux"]), ("hlim", case["hlim"]), ("vlim", case["vlim"]), ("hlim_eps64", h_eps)):
+            for k, v in f32(d).items():
+                out[f"{kern}/in/{k}"] = v
+        res = {"flux": sp.horizontal_flux(spec, case["geo"], case["flux"], 360.0),
+               "hlim": sp.horizontal_limiter_thuburn(spec, case["geo"]["have_sgp"], case["hlim"]),
+               "vlim": sp.vertical_limiter_thuburn(spec, case["vlim"])}
+        try:
+            sp.set_eps(EPS64)
+            res["hlim_eps64"] = sp.horizontal_limiter_thuburn(spec, case["geo"]["have_sgp"], h_eps)
+            res["vlim_eps64"] = sp.vertical_limiter_thuburn(spec, case["vlim"])
+        finally:
+            sp.set_eps(None)
+        for kern, r in res.items():
+            for k, v in r.items():
+                out[f"{kern}/out/{k}"] = v
+        out["spec"] = np.array([n, kall, lall, seed, int(pl)], dtype=np.int64)
+        np.savez_compressed(HERE / f"{name}.npz", **out)
+        print(f"wrote {name}.npz ({len(out)} arrays)")
+
+
+def write_oprt() -> None:
+    ref = binding.RefLibs()
+    for name, n, kall, lall, seed, sgp, pl in CASES_OPRT:
+        spec = synth.GridSpec(n, kall, lall, have_pl=pl, seed=seed)
+        geo = synth.make_geometry(spec, sgp)
+        out = {}
+        f = synth.diffusion_inputs(spec, geo)
+        for k, v in f.items():
+            out["diff/in/" + k] = v
+        for k, v in ref.oprt_diffusion(spec, sgp, f, fill=7.0).items():
+            out["diff/out/" + k] = v
+        g = synth.divdamp_inputs(spec, geo)
+        for k, v in g.items():
+            out["ddmp/in/" + k] = v
+        for k, v in ref.oprt3d_divdamp(spec, sgp, g, fill=7.0).items():
+            out["ddmp/out/" + k] = v
+        out["sgp"] = np.asarray(sgp, dtype=np.int32)
+        out["spec"] = np.array([n, kall, lall, seed, int(pl)], dtype=np.int64)
+        np.savez_compressed(HERE / f"{name}.npz", **out)
+        print(f"wrote {name}.npz ({len(out)} arrays)")
+
+
 def main() -> None:
+    which = set(sys.argv[1:]) or {"fp64", "f32", "oprt"}
     if not binding.RefLibs.available():
         binding.build(with_ref=True)
+    if "f32" in which:
+        write_f32()
+    if "oprt" in which:
+        write_oprt()
+    if "fp64" in which:
+        write_fp64()
+
+
+def write_fp64() -> None:
     ref = binding.RefLibs()
     for name, n, kall, lall, seed, sgp, pl in CASES:
         spec = synth.GridSpec(n, kall, lall, have_pl=pl, seed=seed)

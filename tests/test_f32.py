@@ -376,3 +376,53 @@ def test_gpu_f32_fp64_eps_matches_the_single_reference(gpu, refsp, c):
         refsp.set_eps(None)
     assert_parity({k: h[k].cpu().numpy() for k in ("q_a", "q_a_pl", "Qout_prev", "Qout_prev_pl")}, want_h, rtol=0.0)
     assert_parity({k: v[k].cpu().numpy() for k in ("q_h", "q_h_pl")}, want_v, rtol=0.0)
+
+
+# ---- the same checks against committed outputs of the SINGLE reference ------
+# (tests/golden/make_golden.py f32): they need no oracle/_ref build
+F32_FIXTURES = ["f32_g6k5l2", "f32_g18k12l2", "f32_g10k6l2_nopl"]
+
+
+def _fixture(name):
+    from pathlib import Path
+
+    z = np.load(Path(__file__).resolve().parent / "golden" / f"{name}.npz")
+    n, kall, lall, seed, pl = (int(x) for x in z["spec"])
+    spec = synth.GridSpec(n, kall, lall, have_pl=bool(pl), seed=seed)
+    grab = lambda pre: {k[len(pre):]: z[k] for k in z.files if k.startswith(pre)}  # noqa: E731
+    return spec, grab
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", F32_FIXTURES)
+def test_gpu_f32_matches_single_reference_fixture(gpu, name):
+    import torch
+
+    spec, grab = _fixture(name)
+    ctx = drivers.setup_context(spec, grab("geo/"))
+    z = lambda shape: torch.zeros(shape, dtype=torch.float32, device=gpu)  # noqa: E731
+    f = _dev32(None, grab("flux/in/"), gpu)
+    out = dict(flx_h=z((6,) + spec.shape3), flx_h_pl=z(spec.shape_pl), GRD_xc=z((3, 3) + spec.shape3),
+               GRD_xc_pl=z((3,) + spec.shape_pl))
+    ctx.horizontal_flux_f32(out["flx_h"], out["flx_h_pl"], out["GRD_xc"], out["GRD_xc_pl"],
+                            *[f[k] for k in drivers.FLUX_IN], 360.0)
+    skip = () if spec.have_pl else ("flx_h_pl", "GRD_xc_pl")
+    assert_parity({k: v.cpu().numpy() for k, v in out.items()}, grab("flux/out/"), rtol=0.0, skip=skip)
+    for mode in ("", "_eps64"):
+        if mode:
+            ctx.set_const_f32(EPS64, float(np.finfo(np.float32).max))
+        h = _dev32(None, grab("hlim" + mode + "/in/"), gpu)
+        h.update(Qout_prev=z((2,) + spec.shape3), Qout_prev_pl=z((2,) + spec.shape_pl))
+        ctx.horizontal_limiter_thuburn_f32(h["q_a"], h["q_a_pl"], h["q"], h["q_pl"], h["d"], h["d_pl"], h["ch"],
+                                           h["ch_pl"], h["cmask"], h["cmask_pl"], h["Qout_prev"], h["Qout_prev_pl"],
+                                           h["Qout_post"], h["Qout_post_pl"])
+        got = {k: h[k].cpu().numpy() for k in ("q_a", "q_a_pl", "Qout_prev", "Qout_prev_pl")}
+        want = grab("hlim" + mode + "/out/")
+        if not spec.have_pl:
+            got["Qout_prev_pl"][..., 0] = want["Qout_prev_pl"][..., 0]
+        assert_parity(got, want, rtol=0.0)
+        v = _dev32(None, grab("vlim/in/"), gpu)
+        ctx.vertical_limiter_thuburn_f32(v["q_h"], v["q_h_pl"], v["q"], v["q_pl"], v["d"], v["d_pl"], v["ck"],
+                                         v["ck_pl"])
+        assert_parity({k: v[k].cpu().numpy() for k in ("q_h", "q_h_pl")}, grab("vlim" + mode + "/out/"), rtol=0.0)
+    ctx.close()

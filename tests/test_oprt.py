@@ -169,3 +169,38 @@ def test_divdamp_gpu_host_pointer_mode(gpu, oracle):
     ctx = drivers.setup_context(spec, geo, pointer_mode=PTR_HOST)
     drivers.setup_divdamp(ctx, f)
     assert_parity(drivers.run_divdamp(ctx, spec, f), oracle.oprt3d_divdamp(spec, sgp, f))
+
+
+# ---- against committed outputs of the compiled reference (make_golden.py oprt)
+OPRT_FIXTURES = ["oprt_g10k8l3", "oprt_g18k6l1_nopl"]
+
+
+def _oprt_fixture(name):
+    from pathlib import Path
+
+    z = np.load(Path(__file__).resolve().parent / "golden" / f"{name}.npz")
+    n, kall, lall, seed, pl = (int(x) for x in z["spec"])
+    spec = synth.GridSpec(n, kall, lall, have_pl=bool(pl), seed=seed)
+    grab = lambda pre: {k[len(pre):]: z[k] for k in z.files if k.startswith(pre)}  # noqa: E731
+    return spec, [int(x) for x in z["sgp"]], grab
+
+
+@pytest.mark.parametrize("name", OPRT_FIXTURES)
+def test_oracle_matches_reference_fixture(oracle, name):
+    """The oracle reproduces the reference's own outputs of both operators,
+    bit for bit, without the reference build."""
+    spec, sgp, grab = _oprt_fixture(name)
+    assert_parity(oracle.oprt_diffusion(spec, sgp, grab("diff/in/"), fill=7.0), grab("diff/out/"), rtol=0.0)
+    assert_parity(oracle.oprt3d_divdamp(spec, sgp, grab("ddmp/in/"), fill=7.0), grab("ddmp/out/"), rtol=0.0)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", OPRT_FIXTURES)
+def test_gpu_operators_match_reference_fixture(gpu, name):
+    spec, sgp, grab = _oprt_fixture(name)
+    geo = synth.make_geometry(spec, sgp)
+    ctx = drivers.setup_context(spec, geo)
+    assert_parity(drivers.run_diffusion(ctx, spec, grab("diff/in/"), fill=7.0), grab("diff/out/"), rtol=0.0)
+    f = grab("ddmp/in/")
+    drivers.setup_divdamp(ctx, f)
+    assert_parity(drivers.run_divdamp(ctx, spec, f, fill=7.0), grab("ddmp/out/"), rtol=0.0)
