@@ -20,10 +20,15 @@ namespace {
 // (:209-212, :242-246).  One work-item per column, as in the forward solver.
 // The forward sweep keeps its partial solution in w0b and its elimination
 // factors in Srb; the backward sweep reads each before it writes that level's
-// final cotangents, so no work array is needed.
+// final cotangents, so no work array is needed.  The factors of the
+// transposed system, g(k) = Ml(k)/beta(k-1) and beta(k) = Mc(k) - Mu(k-1) g(k),
+// depend on the coefficients only: the factored variant (FACT) reads them
+// from the factorisation nicam_vi_rhow_solver_putcoef stores, formed by the
+// same operations, so its dependency chain per level holds one division.
 struct VaField {
     double *wb, *w0b, *p0b, *r0b, *Srb, *Swb, *Spb;
     const double *Mc, *Ml, *Mu, *RGAM, *RGAMH, *RGSGAM2, *RGSGAM2H, *GSGAM2H;
+    const double *tbeta, *tgam;  // the transposed system's stored factors
 };
 
 struct VaArgs {
@@ -36,6 +41,7 @@ struct VaArgs {
 
 constexpr int BLOCK_VA = 256;
 
+template <bool FACT>
 __global__ void __launch_bounds__(BLOCK_VA) vi_adjoint_kernel(VaArgs A)
 {
     const long t = (long)blockIdx.x * BLOCK_VA + threadIdx.x;
@@ -59,15 +65,19 @@ __global__ void __launch_bounds__(BLOCK_VA) vi_adjoint_kernel(VaArgs A)
     auto at = [&](int k) { return base + (size_t)k * s; };
 
     // transposed elimination on kmin+1..kmax
-    double beta = F->Mc[at(kmin + 1)];
+    double beta = FACT ? F->tbeta[at(kmin + 1)] : F->Mc[at(kmin + 1)];
     double yp = (F->wb[at(kmin + 1)] * F->GSGAM2H[at(kmin + 1)]) / beta;
     F->w0b[at(kmin + 1)] = yp;
     for (int k = kmin + 2; k <= kmax; ++k) {
         const double mu_km1 = F->Mu[at(k - 1)];
-        const double g = F->Ml[at(k)] / beta;
-        beta = F->Mc[at(k)] - mu_km1 * g;
+        if constexpr (FACT) {
+            beta = F->tbeta[at(k)];
+        } else {
+            const double g = F->Ml[at(k)] / beta;
+            beta = F->Mc[at(k)] - mu_km1 * g;
+            F->Srb[at(k)] = g;
+        }
         yp = (F->wb[at(k)] * F->GSGAM2H[at(k)] - mu_km1 * yp) / beta;
-        F->Srb[at(k)] = g;
         F->w0b[at(k)] = yp;
     }
     const double y_kmax = yp;
@@ -81,7 +91,7 @@ __global__ void __launch_bounds__(BLOCK_VA) vi_adjoint_kernel(VaArgs A)
         if (in_k) {
             const double ypk = F->w0b[o];
             yk = (k == kmax) ? ypk : ypk - g_k1 * yn;
-            if (k >= kmin + 2) gk = F->Srb[o];
+            if (k >= kmin + 2) gk = FACT ? F->tgam[o] : F->Srb[o];
         }
         const double sb = in_k ? yk * CV : 0.0;
         const double sbn = in_n ? yn * CV : 0.0;
@@ -1009,9 +1019,10 @@ int launch_vi_adjoint(nicam_ctx c, double *wb, double *wb_pl, double *w0b, doubl
     const Dims &D = c->d;
     VaArgs A;
     A.reg = VaField{wb, w0b, p0b, r0b, Srb, Swb, Spb, c->Mc, c->Ml, c->Mu, c->RGAM, c->RGAMH, c->RGSGAM2,
-                    c->RGSGAM2H, c->GSGAM2H};
+                    c->RGSGAM2H, c->GSGAM2H, c->vi_tbeta, c->vi_tgam};
     A.pl = VaField{wb_pl, w0b_pl, p0b_pl, r0b_pl, Srb_pl, Swb_pl, Spb_pl, c->Mc_pl, c->Ml_pl, c->Mu_pl,
-                   c->RGAM_pl, c->RGAMH_pl, c->RGSGAM2_pl, c->RGSGAM2H_pl, c->GSGAM2H_pl};
+                   c->RGAM_pl, c->RGAMH_pl, c->RGSGAM2_pl, c->RGSGAM2H_pl, c->GSGAM2H_pl, c->vi_tbeta_pl,
+                   c->vi_tgam_pl};
     A.rdgzh = c->rdgzh;
     A.afact = c->afact;
     A.bfact = c->bfact;
@@ -1026,13 +1037,18 @@ int launch_vi_adjoint(nicam_ctx c, double *wb, double *wb_pl, double *w0b, doubl
     A.grav = c->grav;
     A.ncol = (long)D.gall * D.lall + (D.have_pl ? (long)D.gall_pl * kLallPl : 0);
     // kall = 42: the column in registers; at kall = 98 the two arrays spill (1.2 KB of scratch per lane), so
-    // the partial solution and the factors stay in the outputs (654 us at gl7)
-    if (D.kall == 42) {
+    // the partial solution stays in the outputs and the factors come from the stored factorisation
+    // (NICAM_TUNE_VI_VARIANT 1 or 3: formed in the solve, as before)
+    const int variant = c->tune.vi_variant;
+    if (D.kall == 42 && variant != 2 && variant != 4) {
         hipLaunchKernelGGL(vi_adjoint_kernel_reg<42>, dim3((unsigned)((A.ncol + BLOCK_VAR - 1) / BLOCK_VAR)),
                            dim3(BLOCK_VAR), 0, c->stream, A);
+    } else if (variant == 1 || variant == 3) {
+        hipLaunchKernelGGL(vi_adjoint_kernel<false>, dim3((unsigned)((A.ncol + BLOCK_VA - 1) / BLOCK_VA)),
+                           dim3(BLOCK_VA), 0, c->stream, A);
     } else {
-        hipLaunchKernelGGL(vi_adjoint_kernel, dim3((unsigned)((A.ncol + BLOCK_VA - 1) / BLOCK_VA)), dim3(BLOCK_VA), 0,
-                           c->stream, A);
+        hipLaunchKernelGGL(vi_adjoint_kernel<true>, dim3((unsigned)((A.ncol + BLOCK_VA - 1) / BLOCK_VA)),
+                           dim3(BLOCK_VA), 0, c->stream, A);
     }
     NICAM_HIP(hipGetLastError());
     return NICAM_OK;

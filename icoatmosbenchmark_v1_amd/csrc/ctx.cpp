@@ -185,7 +185,8 @@ int nicam_ctx_destroy(nicam_ctx c)
                       &c->RGAMH, &c->RGAMH_pl, &c->RGSGAM2, &c->RGSGAM2_pl, &c->RGSGAM2H,
                       &c->RGSGAM2H_pl, &c->GSGAM2H, &c->GSGAM2H_pl, &c->rdgzh, &c->afact,
                       &c->bfact, &c->Mc, &c->Mc_pl, &c->Ml, &c->Ml_pl, &c->Mu, &c->Mu_pl, &c->vi_beta,
-                      &c->vi_beta_pl, &c->vi_gam, &c->vi_gam_pl,
+                      &c->vi_beta_pl, &c->vi_gam, &c->vi_gam_pl, &c->vi_tbeta, &c->vi_tbeta_pl, &c->vi_tgam,
+                      &c->vi_tgam_pl,
                       &c->RGSQRTH, &c->RGSQRTH_pl, &c->C2WfactGz, &c->C2WfactGz_pl, &c->rdgz,
                       &c->cfact, &c->dfact};
     for (double **p : all) dev_free(*p);
@@ -541,12 +542,11 @@ int nicam_set_vmetrics3d(nicam_ctx c, const double *RGSQRTH, const double *RGSQR
 // factors the coefficients just installed.
 static int factor_coef(nicam_ctx c)
 {
-    NICAM_TRY(dev_alloc(c, &c->vi_beta, c->d.nreg()));
-    NICAM_TRY(dev_alloc(c, &c->vi_gam, c->d.nreg()));
-    if (c->d.have_pl) {
-        NICAM_TRY(dev_alloc(c, &c->vi_beta_pl, c->d.npl()));
-        NICAM_TRY(dev_alloc(c, &c->vi_gam_pl, c->d.npl()));
-    }
+    double **reg[] = {&c->vi_beta, &c->vi_gam, &c->vi_tbeta, &c->vi_tgam};
+    double **pl[] = {&c->vi_beta_pl, &c->vi_gam_pl, &c->vi_tbeta_pl, &c->vi_tgam_pl};
+    for (double **p : reg) NICAM_TRY(dev_alloc(c, p, c->d.nreg()));
+    if (c->d.have_pl)
+        for (double **p : pl) NICAM_TRY(dev_alloc(c, p, c->d.npl()));
     return launch_vi_factor(c);
 }
 

@@ -124,6 +124,7 @@ struct nicam_ctx_s {
     // the Thomas factorisation of Mc/Ml/Mu (beta(k), gamma(k)), formed on the
     // device whenever the coefficients are set
     double *vi_beta = nullptr, *vi_beta_pl = nullptr, *vi_gam = nullptr, *vi_gam_pl = nullptr;
+    double *vi_tbeta = nullptr, *vi_tbeta_pl = nullptr, *vi_tgam = nullptr, *vi_tgam_pl = nullptr;  // transposed
 
     // staging buffers for NICAM_PTR_HOST, one per argument slot
     std::vector<void *> stage;

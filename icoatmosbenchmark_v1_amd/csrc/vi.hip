@@ -299,8 +299,9 @@ __global__ void __launch_bounds__(BLOCK_REG) vi_kernel_reg(ViArgsT<Src> A)
 // left as they are; no solve reads them.
 struct FactorPart {
     const double *Mc, *Ml, *Mu;
-    double *beta, *gam;
-    int gall, ncols;  // level stride and columns of this part
+    double *beta, *gam;    // the system's factors
+    double *tbeta, *tgam;  // the transposed system's (the adjoint solve)
+    int gall, ncols;       // level stride and columns of this part
 };
 struct FactorArgs {
     FactorPart reg, pl;
@@ -326,6 +327,17 @@ __global__ void __launch_bounds__(BLOCK_FAC) vi_factor_kernel(FactorArgs A)
         beta = P->Mc[o] - P->Ml[o] * g;
         P->gam[o] = g;
         P->beta[o] = beta;
+    }
+    // the transposed system (diagonal Mc(k), sub-diagonal Mu(k-1), super-diagonal
+    // Ml(k+1)): g(k) = Ml(k)/beta(k-1), beta(k) = Mc(k) - Mu(k-1) g(k)
+    beta = P->Mc[base + (kmin + 1) * s];
+    P->tbeta[base + (kmin + 1) * s] = beta;
+    for (int k = kmin + 2; k <= kmax; ++k) {
+        const size_t o = base + k * s;
+        const double g = P->Ml[o] / beta;
+        beta = P->Mc[o] - P->Mu[o - s] * g;
+        P->tgam[o] = g;
+        P->tbeta[o] = beta;
     }
 }
 
@@ -409,9 +421,9 @@ int launch_vi_factor(nicam_ctx c)
 {
     const Dims &D = c->d;
     FactorArgs A;
-    A.reg = FactorPart{c->Mc, c->Ml, c->Mu, c->vi_beta, c->vi_gam, D.gall, D.gall * D.lall};
-    A.pl = FactorPart{c->Mc_pl, c->Ml_pl, c->Mu_pl, c->vi_beta_pl, c->vi_gam_pl, D.gall_pl,
-                      D.have_pl ? D.gall_pl * kLallPl : 0};
+    A.reg = FactorPart{c->Mc, c->Ml, c->Mu, c->vi_beta, c->vi_gam, c->vi_tbeta, c->vi_tgam, D.gall, D.gall * D.lall};
+    A.pl = FactorPart{c->Mc_pl, c->Ml_pl, c->Mu_pl, c->vi_beta_pl, c->vi_gam_pl, c->vi_tbeta_pl, c->vi_tgam_pl,
+                      D.gall_pl, D.have_pl ? D.gall_pl * kLallPl : 0};
     A.kall = D.kall;
     const long n = (long)A.reg.ncols + A.pl.ncols;
     hipLaunchKernelGGL(vi_factor_kernel, dim3((unsigned)((n + BLOCK_FAC - 1) / BLOCK_FAC)), dim3(BLOCK_FAC), 0,

@@ -117,7 +117,8 @@ class HaloExchange:
         out = flat.index_copy(-1, self.dst, flat.index_select(-1, self.src))
         a2 = out[..., :self.lall * self.gall].reshape(-1, kall, self.lall, self.gall).permute(0, 2, 1, 3)
         b2 = out[..., self.lall * self.gall:].reshape(-1, kall, 2, self.gall_pl).permute(0, 2, 1, 3)
-        return a2.reshape(lead + (self.lall, kall, self.gall)), b2.reshape(lead + (2, kall, self.gall_pl))
+        return (a2.reshape(lead + (self.lall, kall, self.gall)).contiguous(),
+                b2.reshape(lead + (2, kall, self.gall_pl)).contiguous())
 
 
 class TracerStep:
@@ -288,6 +289,7 @@ class TracerStep:
         from torch.utils.checkpoint import checkpoint
 
         ctx, ntr = self.ctx, q.shape[0]
+        rhog, rhog_pl = rhog.contiguous(), rhog_pl.contiguous()
         flx, flx_pl, xc, xc_pl = HorizontalFlux.apply(ctx, self.dt, rhog, rhog_pl, *self.rv)
         ch, ch_pl, cm, cm_pl = self.courant(flx, flx_pl, rhog, rhog_pl)
         chd, chd_pl = ch.detach().contiguous(), ch_pl.detach().contiguous()

@@ -83,8 +83,12 @@ int nicam_ctx_set_pointer_mode(nicam_ctx ctx, int mode);
  *                           3 = factors formed in the solve, column in
  *                           registers; 4 = stored factors, forward solution
  *                           in registers (3 and 4: kall = 42 or 98, else
- *                           automatic); 0 = automatic
- *   NICAM_TUNE_DIFF_KPER    levels per work-group of nicam_oprt_diffusion
+ *                           automatic); 0 = automatic.  The adjoint
+ *                           follows it: 1 and 3 form the transposed
+ *                           system's factors in the solve, 2, 4 and 0 read
+ *                           the ones putcoef stores (0 at kall = 42: the
+ *                           column in registers)
+ *   NICAM_TUNE_DIFF_KPER   levels per work-group of nicam_oprt_diffusion
  *   NICAM_TUNE_DDMP_KPER    levels per work-group of nicam_oprt3d_divdamp
  * Results do not depend on these settings (every variant computes the same
  * floating-point operations in the same order). */
