@@ -170,6 +170,11 @@ int nicam_ctx_create(nicam_ctx *out, int gall_1d, int kall, int lall, int gall_p
         nicam_ctx_destroy(c);
         return rc;
     }
+    const int rc_s = mass_scratch_init(c->device);
+    if (rc_s != NICAM_OK) {
+        nicam_ctx_destroy(c);
+        return rc_s;
+    }
     *out = c;
     return NICAM_OK;
 }

@@ -301,6 +301,7 @@ int launch_hflux_group_f32(const nicam_ctx *cs, int n, const nicam_tracer_step_f
 int launch_hlim_group_f32(const nicam_ctx *cs, int n, int ntr, const nicam_tracer_step_f32 *g, hipStream_t s);
 int launch_vlim_group_f32(const nicam_ctx *cs, int n, int ntr, const nicam_tracer_step_f32 *g, hipStream_t s);
 int launch_vi_group_mixed(const nicam_ctx *cs, int n, const nicam_tracer_step_f32 *g, double dt, hipStream_t s);
+int mass_scratch_init(int device);  // the per-device address of nicam_tracer_mass_noalloc's scratch
 int launch_mass(nicam_ctx c, const double *rq, const double *rq_pl, double *mass, double *part);
 int launch_mass_f32(nicam_ctx c, const float *rq, const float *rq_pl, double *mass, double *part);
 int launch_update_cons(nicam_ctx c, double *rq, double *rq_pl, const double *flx, const double *qa);

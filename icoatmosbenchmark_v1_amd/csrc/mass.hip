@@ -8,14 +8,25 @@
 // work-group adds each level's region partial sums in region order and the
 // two pole terms.  The result is therefore the same on every run and every
 // launch shape of the device, which a conservation check needs.
-// nicam_tracer_mass keeps its partial sums in a buffer the context allocates
-// on the first call.
+//
+// Two entry points share the kernels.  nicam_tracer_mass keeps its partial
+// sums in a buffer the context allocates on the first call.
+// nicam_tracer_mass_noalloc is for callers that cannot allocate device memory
+// (code under HIP graph capture from its first call, serving loops, harnesses
+// that forbid hipMalloc): its partial sums live in `scratch`, a file-scope
+// device array of kMassScratch doubles, one copy per GPU, whose device address
+// nicam_ctx_create looks up once per device (mass_scratch_init).
+#include <mutex>
+
 #include "internal.h"
 
 namespace nicam {
 namespace {
 
 constexpr int MBLK = 256;
+constexpr size_t kMassScratch = 1u << 16;  // partial sums: kall * lall <= 65536 (512 KiB per GPU)
+
+__device__ __align__(256) double scratch[kMassScratch];
 
 template <class T>
 struct MassArgs {
@@ -86,7 +97,29 @@ int launch_mass_t(nicam_ctx c, const T *rq, const T *rq_pl, double *mass, double
     return NICAM_OK;
 }
 
+// the device address of `scratch` on each GPU (the host-side name is not one)
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double *g_scratch_ptr[kMaxDevices] = {};
+
 }  // namespace
+
+// Once per device, with that device current: the address of its `scratch`.
+int mass_scratch_init(int device)
+{
+    if (device < 0 || device >= kMaxDevices) return fail(NICAM_ERR_ARG, "device index out of range");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch_ptr[device]) return NICAM_OK;
+    int cur = 0;
+    NICAM_HIP(hipGetDevice(&cur));
+    if (cur != device) NICAM_HIP(hipSetDevice(device));
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (cur != device) (void)hipSetDevice(cur);
+    if (e != hipSuccess) return hip_fail(e, "hipGetSymbolAddress(scratch)");
+    g_scratch_ptr[device] = static_cast<double *>(p);
+    return NICAM_OK;
+}
 
 int launch_mass(nicam_ctx c, const double *rq, const double *rq_pl, double *mass, double *part)
 {
@@ -99,3 +132,27 @@ int launch_mass_f32(nicam_ctx c, const float *rq, const float *rq_pl, double *ma
 }
 
 }  // namespace nicam
+
+using namespace nicam;
+
+// nicam_tracer_mass without any allocation: device pointers only.  Calls that
+// overlap on one GPU share its `scratch` and race; calls on one stream are
+// ordered, and each call writes every partial sum it reads.
+extern "C" int nicam_tracer_mass_noalloc(nicam_ctx c, const double *rhogq, const double *rhogq_pl, double *mass)
+{
+    if (!c) return fail(NICAM_ERR_ARG, "null context");
+    if (!rhogq || !mass) return fail(NICAM_ERR_ARG, "null argument");
+    if (c->d.have_pl && !rhogq_pl) return fail(NICAM_ERR_ARG, "null pole array");
+    if ((size_t)c->d.kall * c->d.lall > kMassScratch)
+        return fail(NICAM_ERR_ARG, "kall * lall exceeds the 65536 partial sums of the static scratch array");
+    if (c->ptr_mode != NICAM_PTR_DEVICE)
+        return fail(NICAM_ERR_STATE, "nicam_tracer_mass_noalloc takes device pointers only (NICAM_PTR_DEVICE)");
+    if (!c->have_hmetrics) return fail(NICAM_ERR_STATE, "horizontal metrics not set (nicam_set_hmetrics)");
+    double *p = nullptr;
+    {
+        std::lock_guard<std::mutex> lock(g_scratch_mu);
+        if (c->device >= 0 && c->device < kMaxDevices) p = g_scratch_ptr[c->device];
+    }
+    if (!p) return fail(NICAM_ERR_STATE, "the scratch address of this device is not set (nicam_ctx_create)");
+    return launch_mass(c, rhogq, c->d.have_pl ? rhogq_pl : nullptr, mass, p);
+}

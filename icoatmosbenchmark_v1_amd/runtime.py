@@ -117,6 +117,7 @@ SIGNATURES = {
     "nicam_vertical_limiter_thuburn_adjoint": (c_int, [_P] * 15),
     "nicam_tracer_mass": (c_int, [_P] * 4),
     "nicam_tracer_mass_f32": (c_int, [_P] * 4),
+    "nicam_tracer_mass_noalloc": (c_int, [_P] * 4),
     "nicam_tracer_update_conservative": (c_int, [_P] * 5),
     "nicam_tracer_update_conservative_f32": (c_int, [_P] * 5),
     "nicam_tracer_update_vertical_f32": (c_int, [_P] * 7),

@@ -88,7 +88,7 @@ int nicam_ctx_set_pointer_mode(nicam_ctx ctx, int mode);
  *                           system's factors in the solve, 2, 4 and 0 read
  *                           the ones putcoef stores (0 at kall = 42: the
  *                           column in registers)
- *   NICAM_TUNE_DIFF_KPER   levels per work-group of nicam_oprt_diffusion
+ *   NICAM_TUNE_DIFF_KPER    levels per work-group of nicam_oprt_diffusion
  *   NICAM_TUNE_DDMP_KPER    levels per work-group of nicam_oprt3d_divdamp
  * Results do not depend on these settings (every variant computes the same
  * floating-point operations in the same order). */
@@ -660,6 +660,15 @@ int nicam_vertical_limiter_thuburn_adjoint(nicam_ctx ctx,
  * rhogq and widens it exactly. */
 int nicam_tracer_mass(nicam_ctx ctx, const double *rhogq, const double *rhogq_pl, double *mass);
 int nicam_tracer_mass_f32(nicam_ctx ctx, const float *rhogq, const float *rhogq_pl, double *mass);
+/* nicam_tracer_mass with no allocation at all: no counterpart in the
+ * reference.  For callers that cannot allocate device memory (code under HIP
+ * graph capture from its first call, tight serving loops, harnesses that
+ * forbid hipMalloc).  The partial sums live in a static device array of
+ * 65536 doubles per GPU (kall * lall must not exceed 65536, else
+ * NICAM_ERR_ARG), whose address nicam_ctx_create looks up.  Device pointers
+ * only; the result equals nicam_tracer_mass's bit for bit.  Calls that overlap
+ * on one GPU share the array and race; calls on one stream are ordered. */
+int nicam_tracer_mass_noalloc(nicam_ctx ctx, const double *rhogq, const double *rhogq_pl, double *mass);
 
 /* rhogq -= RAREA * (sum of the six outward face transfers), in FP64, on the
  * inner cells and the pole centres (halos untouched).  Each arc's transfer
