@@ -548,11 +548,14 @@ def time_adjoint(ctx, spec, case, stream, steps, fwd_ms):
                              alg_bytes=ab[name], gbs=round(gbs, 1), frac=round(gbs / perfmodel.HBM_PEAK_GBS, 4))
     adj = sum(out.values())
     fwd = sum(fwd_ms[k] for k in out)
+    # the reverse pass of a checkpointed run recomputes the step's forward (the limiters masked) and then
+    # runs the adjoints: its cost over the plain forward's is the ratio the gradient pays per step
+    recompute = fwd + sum(masked.values()) - sum(fwd_ms[k] for k in masked)
     return dict(kernels=kernels,
                 masked_forward_ms={k: round(v, 5) for k, v in masked.items()},
                 covered=sorted(out), ratio_covered=round(adj / fwd, 3),
-                ratio_covered_with_masked_forward=round(
-                    (adj + sum(masked.values()) - sum(fwd_ms[k] for k in masked)) / fwd, 3))
+                recompute_ms=round(recompute, 5),
+                ratio_with_recompute=round((recompute + adj) / fwd, 3))
 
 
 BATCH_GRIDS = ((5, 98), (6, 78), (7, 58), (8, 42))  # (glevel, kall) of the four rlevel-1 grids
@@ -785,7 +788,8 @@ def main() -> int:
     adjoint = None
     if not args.no_adjoint:
         adjoint = time_adjoint(ctx, spec, case, stream, args.steps, kms)
-        log(f"adjoint/forward over {adjoint['covered']}: {adjoint['ratio_covered']}")
+        log(f"adjoint/forward over {adjoint['covered']}: {adjoint['ratio_covered']} (adjoints only), "
+            f"{adjoint['ratio_with_recompute']} (with the recomputed masked forward)")
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -816,7 +820,7 @@ def main() -> int:
                     tracers=1),
         hbm_frac_step=round(step_gbs / perfmodel.HBM_PEAK_GBS, 4),
         step_alg_bytes=step_bytes,
-        adjoint_fwd_ratio=None if adjoint is None else adjoint["ratio_covered"],
+        adjoint_fwd_ratio=None if adjoint is None else adjoint["ratio_with_recompute"],
         adjoint=adjoint,
         batched=batched,
         kernels=kernels,

@@ -19,7 +19,8 @@ from .runtime import Context
 
 
 class VerticalLimiter(torch.autograd.Function):
-    """q_h' = vertical_limiter_thuburn(q_h, q; d, ck) (VLIM:41-243)."""
+    """q_h' = vertical_limiter_thuburn(q_h, q, ck; d) (VLIM:41-243); the inflow
+    flags (signs of ck) and d are held fixed."""
 
     @staticmethod
     def forward(fctx, nctx: Context, q_h, q_h_pl, q, q_pl, d, d_pl, ck, ck_pl):
@@ -37,9 +38,10 @@ class VerticalLimiter(torch.autograd.Function):
         out, out_pl, q, q_pl, d, d_pl, ck, ck_pl, mask, mask_pl = fctx.saved_tensors
         qhb, qhb_pl = g.contiguous().clone(), g_pl.contiguous().clone()
         qb, qb_pl = torch.empty_like(q), torch.empty_like(q_pl)
+        ckb, ckb_pl = torch.empty_like(ck), torch.empty_like(ck_pl)
         fctx.nctx.vertical_limiter_thuburn_adjoint(qhb, qhb_pl, qb, qb_pl, out, out_pl, q, q_pl, d, d_pl, ck, ck_pl,
-                                                   mask, mask_pl)
-        return None, qhb, qhb_pl, qb, qb_pl, None, None, None, None
+                                                   mask, mask_pl, ckb, ckb_pl)
+        return None, qhb, qhb_pl, qb, qb_pl, None, None, ckb, ckb_pl
 
 
 class RhowSolve(torch.autograd.Function):

@@ -229,6 +229,8 @@ struct VlimLvl {
     double qmin, qmax;
     double n0, n1, n2;  // d Qout_min / d q(j-1), q(j), q(j+1)
     double x0, x1, x2;  // d Qout_max / d q(j-1), q(j), q(j+1)
+    double cn1, cn2;    // d Qout_min / d ck(j,1), ck(j,2)
+    double cx1, cx2;    // d Qout_max / d ck(j,1), ck(j,2)
 };
 
 __device__ __forceinline__ double sgn_half(double x) { return copysign(0.5, x); }
@@ -281,11 +283,17 @@ __device__ __forceinline__ VlimLvl vlim_level(double qkm1, double qk, double qkp
     rows(2, r.n2, r.x2);
     r.n1 = (r.n1 + fac) + zerosw;
     r.x1 = (r.x1 + fac) + zerosw;
+    // ck(j,1) enters Cin where inflagL = 1 (with CQin) and Cout elsewhere; ck(j,2) likewise with inflagU
+    const bool z = zerosw != 0.0;
+    r.cn1 = z ? 0.0 : (inflagL > 0.0 ? qnext_max - Qin_maxL : qnext_max - r.qmin) / Cout;
+    r.cn2 = z ? 0.0 : (inflagU > 0.0 ? qnext_max - Qin_maxU : qnext_max - r.qmin) / Cout;
+    r.cx1 = z ? 0.0 : (inflagL > 0.0 ? qnext_min - Qin_minL : qnext_min - r.qmax) / Cout;
+    r.cx2 = z ? 0.0 : (inflagU > 0.0 ? qnext_min - Qin_minU : qnext_min - r.qmax) / Cout;
     return r;
 }
 
 struct VlaField {
-    double *qhb, *qb;
+    double *qhb, *qb, *ckb;  // ckb: cotangent of ck (NULL: not formed)
     const double *q_h, *q, *d, *ck;
     const uint32_t *mask;
 };
@@ -324,6 +332,8 @@ __device__ __forceinline__ void vlim_adjoint_run(const VlaField &F, size_t base,
     double c1 = F.ck[at(kf - 1)], c1p = F.ck[at(kf)];
     VlimLvl lo = vlim_level<POLE>(qm, q0, qp, c1, c1p, F.ck[ck2 + at(kf - 1)], F.d[at(kf - 1)], EPS, BIG);
     double w0 = 0.0, w1 = 0.0, w2 = 0.0, w3 = 0.0;  // levels k-2 .. k+1
+    double cp1 = 0.0, cp2 = 0.0, cc1 = 0.0, cc2 = 0.0;  // ck cotangents of levels k-1 and k
+    const bool want_ck = F.ckb != nullptr;
     int wi = -1;
     uint32_t word = 0;
     for (int k = kf; k <= kl; ++k) {
@@ -357,6 +367,17 @@ __device__ __forceinline__ void vlim_adjoint_run(const VlaField &F, size_t base,
             row1 = from_lo ? e1 : e0;
             row2 = from_lo ? e2 : e1;
             row3 = from_lo ? 0.0 : e2;
+            if (want_ck) {
+                const double g1 = from_lo ? (use_min ? lo.cn1 : lo.cx1) : (use_min ? hi.cn1 : hi.cx1);
+                const double g2 = from_lo ? (use_min ? lo.cn2 : lo.cx2) : (use_min ? hi.cn2 : hi.cx2);
+                if (from_lo) {
+                    cp1 = cp1 + g1 * ob;
+                    cp2 = cp2 + g2 * ob;
+                } else {
+                    cc1 = cc1 + g1 * ob;
+                    cc2 = cc2 + g2 * ob;
+                }
+            }
         }
         if (p >= pb && p < pe) F.qhb[at(k)] = cqh * ob;
         w0 = w0 + row0 * ob;
@@ -368,11 +389,32 @@ __device__ __forceinline__ void vlim_adjoint_run(const VlaField &F, size_t base,
         w1 = w2;
         w2 = w3;
         w3 = 0.0;
+        if (want_ck) {  // faces k-1 and k reach level k-1's ck; none later
+            if (k - 1 >= mb && k - 1 < me) {
+                F.ckb[at(k - 1)] = cp1;
+                F.ckb[ck2 + at(k - 1)] = cp2;
+            }
+            cp1 = cc1;
+            cp2 = cc2;
+            cc1 = cc2 = 0.0;
+        }
         lo = hi;
     }
     if (kl - 1 >= mb && kl - 1 < me) F.qb[at(kl - 1)] = w0;
     if (kl >= mb && kl < me) F.qb[at(kl)] = w1;
     if (kl + 1 >= mb && kl + 1 < me) F.qb[at(kl + 1)] = w2;
+    if (want_ck) {
+        if (kl >= mb && kl < me) {
+            F.ckb[at(kl)] = cp1;
+            F.ckb[ck2 + at(kl)] = cp2;
+        }
+        // levels that no face reaches: below the first face's lower level and above the last face
+        for (int j = mb; j < me; ++j)
+            if (j < kmin || j > kmax) {
+                F.ckb[at(j)] = 0.0;
+                F.ckb[ck2 + at(j)] = 0.0;
+            }
+    }
 }
 
 __global__ void __launch_bounds__(BLOCK_VLA) vlim_adjoint_kernel(VlaArgs A)
@@ -426,26 +468,36 @@ __global__ void __launch_bounds__(BLOCK_VLA) vlim_adjoint_kernel(VlaArgs A)
 // changed; a changed value is attributed, in this order, to Qout_post min,
 // Qout_post max, Qin min, Qin max, by equality with the stored output).
 //
-// One work-group per CHUNK cells of one (k, l) slab, as in the forward: q of
-// the window [g0-iall-1, g0+CHUNK+iall+2) staged in LDS, and the cotangents of
-// q and Qout_post over the same window accumulated in LDS (every scatter
-// target lies in it), then added to global memory once per window cell.
-// Neighbouring windows overlap, so those adds are atomic and q_b, Qout_post_b
-// are cleared first; sums of a cell's few terms may therefore round in any
-// order.  Faces of q_a are owned by one work-item each.
-constexpr int HCHUNK = 256;
+// One work-group per (k, l) slab, sweeping it in steps of HSW cells, with
+// no atomics: every cotangent is a gather in the order of a sequential sweep
+// over the slab.  A cell's q_b is the sum of the terms of the Qout rows that
+// name it, in cell order of the rows' owners (each row: bound mm = 0 then 1,
+// its terms in row order), followed by the face cotangents attributed to it,
+// in cell order of the faces' origins (arcs m = 0, 1, 2); Qout_post_b
+// likewise.  Those are the orders of the oracle's loops, so the results are
+// its bits.  An LDS ring of R = HSW + 2*iall + 3 cells holds q and, per cell,
+// what the gathers read: its row's coefficients and Qprev_b, the cell each
+// term names (one of the seven cells of the hexagon around the owner; the
+// inner rows' arcs never reach further or hit the index clamps of
+// HLIM:152-153), and the cotangents of its three own faces with their
+// attribution.  Each step loads q 2*iall+3 cells ahead, forms rows and faces
+// iall+1 cells ahead and gathers HSW cells; a face's q_a_b is read and
+// rewritten by its origin's work-item only.
+constexpr int HSW = 256;
 constexpr int HMAXI = 160;
+constexpr int HPOLE = 256;
 
 struct HaArgs {
     double *qab, *qb, *Qpostb;
+    double *chb;  // cotangent of ch (NULL: not formed)
     const double *Qprevb;
     const double *q_a, *q, *d, *ch, *cmask, *Qpost;
     const uint8_t *mask;
     const int *have_sgp;
-    int iall, gall, kall, lall, nmain;
+    int iall, gall, kall, lall, ring;
     double eps, big;
     // poles
-    double *qab_pl, *qb_pl, *Qpostb_pl;
+    double *qab_pl, *qb_pl, *Qpostb_pl, *chb_pl;
     const double *Qprevb_pl;
     const double *q_a_pl, *q_pl, *d_pl, *ch_pl, *cmask_pl, *Qpost_pl;
     const uint8_t *mask_pl;
@@ -468,46 +520,84 @@ __device__ __forceinline__ void ha_arc(int m, int p, int iall, bool sgp, int c[4
     }
 }
 
-__global__ void __launch_bounds__(HCHUNK) hlim_adjoint_pole(HaArgs A);
-
-__global__ void __launch_bounds__(HCHUNK) hlim_adjoint_kernel(HaArgs A)
+// index 0..6 of cell g + off in the hexagon {-iall-1, -iall, -1, 0, 1, iall, iall+1}
+__device__ __forceinline__ uint32_t hex_index(int off, int iall)
 {
-    constexpr int SPAN = HCHUNK + 2 * HMAXI + 3;
-    __shared__ double s_q[SPAN], s_b[SPAN], s_pn[SPAN], s_px[SPAN];
-    const int iall = A.iall, gall = A.gall, kall = A.kall, lall = A.lall;
-    const int nchunk = (gall + HCHUNK - 1) / HCHUNK;
-    const int b = blockIdx.x;
-    const int chunk = b % nchunk, kl = b / nchunk;
-    const int k = kl % kall, l = kl / kall;
-    const int g0 = chunk * HCHUNK, lo = g0 - iall - 1, n_in = HCHUNK + 2 * iall + 3;
+    return off < -1 ? (off == -iall - 1 ? 0u : 1u) : (off > 1 ? (off == iall ? 5u : 6u) : (uint32_t)(off + 3));
+}
+
+// Whether arc f (0..5, as ordered in a row) of a row's owner contains the
+// owner's hexagon cell w: f0 {0,-iall,1,iall+1}, f1 {0,1,iall+1,iall},
+// f2 {0,iall+1,iall,-1}, f3 {-1,-iall-1,0,iall}, f4 {-iall-1,-iall,0,-1},
+// f5 {-iall,1,0,-iall-1} (not the singular-point row, whose f4 is
+// {-iall-1,0,1,-1}).
+__host__ __device__ constexpr bool arc_has(int f, int w)
+{
+    return w == 3 || (f == 0 && (w == 1 || w == 4 || w == 6)) || (f == 1 && (w == 4 || w == 5 || w == 6)) ||
+           (f == 2 && (w == 2 || w == 5 || w == 6)) || (f == 3 && (w == 0 || w == 2 || w == 5)) ||
+           (f == 4 && (w == 0 || w == 1 || w == 2)) || (f == 5 && (w == 0 || w == 1 || w == 4));
+}
+
+// The slot (0..3, in ha_arc's order) of arc m's cell at offset d from its
+// origin, -1 if none; d by index: 0 iall+2, 1 iall+1, 2 iall, 3 1, 4 0, 5 -1,
+// 6 -iall (the index clamps reach cell 0 only, which takes the general path).
+__host__ __device__ constexpr int arc_slot(int m, int d)
+{
+    return d == 4 ? 0
+                  : (m == 0 ? (d == 6 ? 1 : d == 3 ? 2 : d == 1 ? 3 : -1)
+                            : (m == 1 ? (d == 3 ? 1 : d == 1 ? 2 : d == 2 ? 3 : -1)
+                                      : (d == 1 ? 1 : d == 2 ? 2 : d == 5 ? 3 : -1)));
+}
+// ... of the singular-point arc (m = 1, origin 0): {0, iall+1, iall+2, iall}
+__host__ __device__ constexpr int arc_slot_sgp(int d) { return d == 4 ? 0 : d == 1 ? 1 : d == 0 ? 2 : d == 2 ? 3 : -1; }
+
+__global__ void __launch_bounds__(HPOLE) hlim_adjoint_pole(HaArgs A);
+
+__global__ void __launch_bounds__(HSW) hlim_adjoint_kernel(HaArgs A)
+{
+    extern __shared__ double lds[];
+    const int iall = A.iall, gall = A.gall, kall = A.kall, lall = A.lall, R = A.ring;
+    double *r_q = lds;               // q
+    double *r_c0 = r_q + R;          // row term 0 (the owner), fac + zerosw
+    double *r_c1 = r_c0 + R;         // [2][R] row term 1 (qnext's selection), per bound
+    double *r_cf = r_c1 + 2 * R;     // [6][R] row terms of the six arcs
+    double *r_bq = r_cf + 6 * R;     // [2][R] Qprev_b
+    double *r_ob = r_bq + 2 * R;     // [3][R] face cotangents of the cell's own arcs
+    uint64_t *r_code = reinterpret_cast<uint64_t *>(r_ob + 3 * R);  // row: cells named by the terms
+    uint32_t *r_fc = reinterpret_cast<uint32_t *>(r_code + R);      // faces: attribution
+    const int k = blockIdx.x % kall, l = blockIdx.x / kall;
+    if (l >= lall) return;
     const size_t fb = ((size_t)l * kall + k) * gall;
     const size_t plane = (size_t)lall * kall * gall;
     const int tid = threadIdx.x;
-    for (int s = tid; s < n_in; s += HCHUNK) {
-        const int gg = lo + s;
-        s_q[s] = (gg >= 0 && gg < gall) ? A.q[fb + gg] : 0.0;
-        s_b[s] = 0.0;
-        s_pn[s] = 0.0;
-        s_px[s] = 0.0;
-    }
-    __syncthreads();
-    auto Q = [&](int p) { return s_q[p - lo]; };
-    auto addq = [&](int p, double v) { atomicAdd(&s_b[p - lo], v); };
-    const int g = g0 + tid;
-    if (g < gall) {
-        const int i = g % iall, j = g / iall;
-        const bool sgp = A.have_sgp[l] != 0;
-        const double BIG = A.big;
-        if (i < 1 || i > iall - 2 || j < 1 || j > iall - 2) {  // halo: Qout = q (HLIM:286-301)
-            addq(g, A.Qprevb[fb + g]);
-            addq(g, A.Qprevb[plane + fb + g]);
+    const bool sgp = A.have_sgp[l] != 0;
+    const double BIG = A.big;
+    constexpr uint64_t kInner = 1ull << 63;
+
+    // one cell's row (if inner) or Qprev_b pass-through (halo), and its faces
+    auto form = [&](int e) {
+        const int se = e % R;
+        auto Q = [&](int x) {
+            int s = se + (x - e);
+            s += s < 0 ? R : 0;
+            s -= s >= R ? R : 0;
+            return r_q[s];
+        };
+        const int i = e % iall, j = e / iall;
+        r_bq[se] = A.Qprevb[fb + e];
+        r_bq[R + se] = A.Qprevb[plane + fb + e];
+        if (i < 1 || i > iall - 2 || j < 1 || j > iall - 2) {
+            r_code[se] = 0;
+            if (A.chb)
+#pragma unroll
+                for (int f = 0; f < 6; ++f) A.chb[(size_t)f * plane + fb + e] = 0.0;
         } else {
             int cmn[6], cmx[6];
             double vmn[6], vmx[6], cc[6];
 #pragma unroll
             for (int f = 0; f < 6; ++f) {
                 const int m = f < 3 ? f : f - 3;
-                const int owner = f < 3 ? g : g - (m == 0 ? 1 : (m == 1 ? iall + 1 : iall));
+                const int owner = f < 3 ? e : e - (m == 0 ? 1 : (m == 1 ? iall + 1 : iall));
                 int c4[4];
                 ha_arc(m, owner, iall, sgp, c4);
                 double lo_v = Q(c4[0]), hi_v = lo_v;
@@ -530,7 +620,7 @@ __global__ void __launch_bounds__(HCHUNK) hlim_adjoint_kernel(HaArgs A)
                     vmx[f] = c * (-BIG) + (1.0 - c) * hi_v;
                 }
             }
-            const double qij = Q(g);
+            const double qij = Q(e);
             int smin = -1, smax = -1;
             double nmin = qij, nmax = qij;
 #pragma unroll
@@ -541,77 +631,221 @@ __global__ void __launch_bounds__(HCHUNK) hlim_adjoint_kernel(HaArgs A)
             double chv[6], chm[6];
 #pragma unroll
             for (int f = 0; f < 6; ++f) {
-                chv[f] = A.ch[(size_t)f * plane + fb + g];
+                chv[f] = A.ch[(size_t)f * plane + fb + e];
                 chm[f] = dmn(chv[f], 0.0);
             }
-            const double Cin = chm[0] + chm[1] + chm[2] + chm[3] + chm[4] + chm[5];
-            const double Cout = chv[0] - chm[0] + chv[1] - chm[1] + chv[2] - chm[2] + chv[3] - chm[3] + chv[4] -
-                                chm[4] + chv[5] - chm[5];
+            double Cin = chm[0], Cout = chv[0] - chm[0];
+#pragma unroll
+            for (int f = 1; f < 6; ++f) {
+                Cin = Cin + chm[f];
+                Cout = Cout + chv[f] - chm[f];
+            }
             const double zerosw = 0.5 - sgn_half(fabs(Cout) - A.eps);
-            const double E = 1.0 - Cin - Cout + A.d[fb + g];
+            const double E = 1.0 - Cin - Cout + A.d[fb + e];
             const double fac = (1.0 - zerosw) / (Cout + zerosw);
+            r_c0[se] = fac + zerosw;
+            if (A.chb) {
+                // Qout in ch: an inflow face (ch < 0) gives (qnext - Qin)/Cout, any other (qnext - Qout)/Cout
+                double CQmin = chm[0] * vmn[0], CQmax = chm[0] * vmx[0];
+#pragma unroll
+                for (int f = 1; f < 6; ++f) {
+                    CQmin = CQmin + chm[f] * vmn[f];
+                    CQmax = CQmax + chm[f] * vmx[f];
+                }
+                const double Qmin = (qij - CQmax - nmax * E) / (Cout + zerosw) * (1.0 - zerosw) + qij * zerosw;
+                const double Qmax = (qij - CQmin - nmin * E) / (Cout + zerosw) * (1.0 - zerosw) + qij * zerosw;
+                const double b0 = r_bq[se], b1 = r_bq[R + se];
+#pragma unroll
+                for (int f = 0; f < 6; ++f) {
+                    const double d0 = zerosw != 0.0 ? 0.0 : (chv[f] < 0.0 ? nmax - vmx[f] : nmax - Qmin) / Cout;
+                    const double d1 = zerosw != 0.0 ? 0.0 : (chv[f] < 0.0 ? nmin - vmn[f] : nmin - Qmax) / Cout;
+                    // a zero cotangent contributes nothing (0 * an overflowed bound must not give NaN)
+                    A.chb[(size_t)f * plane + fb + e] = (b0 != 0.0 ? b0 * d0 : 0.0) + (b1 != 0.0 ? b1 * d1 : 0.0);
+                }
+            }
+            uint64_t code = kInner;
 #pragma unroll
             for (int mm = 0; mm < 2; ++mm) {
-                const double bq = A.Qprevb[(size_t)mm * plane + fb + g];
                 const int sel = mm == 0 ? smax : smin;
-                const int *cx = mm == 0 ? cmx : cmn;
-                addq(g, (fac + zerosw) * bq);
-                addq(sel < 0 ? g : cx[sel], (-(fac * E) * (sel < 0 ? 1.0 : cc[sel])) * bq);
+                // the selected arc's cell and factor by selects (an indexed register array would live in scratch)
+                int cell_f[6];
+                uint32_t selw = 3u;
+                double selc = 1.0;
 #pragma unroll
-                for (int f = 0; f < 6; ++f) addq(cx[f], (-(fac * chm[f]) * cc[f]) * bq);
+                for (int f = 0; f < 6; ++f) {
+                    cell_f[f] = mm == 0 ? cmx[f] : cmn[f];
+                    const uint32_t w = hex_index(cell_f[f] - e, iall);
+                    selw = sel == f ? w : selw;
+                    selc = sel == f ? cc[f] : selc;
+                }
+                r_c1[mm * R + se] = -(fac * E) * selc;
+                uint64_t cm = selw;
+#pragma unroll
+                for (int f = 0; f < 6; ++f) cm |= (uint64_t)hex_index(cell_f[f] - e, iall) << (3 + 3 * f);
+                code |= cm << (21 * mm);
             }
+#pragma unroll
+            for (int f = 0; f < 6; ++f) r_cf[f * R + se] = -(fac * chm[f]) * cc[f];
+            r_code[se] = code;
         }
         // faces 1-3 of the own arcs (HLIM:377-410)
+        uint32_t fc = 0;
         if (i <= iall - 2 && j <= iall - 2) {
-            const uint32_t bits = A.mask[fb + g];
+            const uint32_t bits = A.mask[fb + e];
 #pragma unroll
             for (int m = 0; m < 3; ++m) {
-                const int nb = g + (m == 0 ? 1 : (m == 1 ? iall + 1 : iall));
-                const size_t fo = (size_t)m * plane + fb + g, fn = (size_t)(m + 3) * plane + fb + nb;
+                const int nb = e + (m == 0 ? 1 : (m == 1 ? iall + 1 : iall));
+                const size_t fo = (size_t)m * plane + fb + e, fn = (size_t)(m + 3) * plane + fb + nb;
                 const double c = A.cmask[fo];
-                const int T = c > 0.0 ? nb : g;
+                const int T = c > 0.0 ? nb : e;
                 const double ob = A.qab[fo] + A.qab[fn];
-                int kind = 0, cell = 0;
+                uint32_t kind = 0, slot = 0;
                 if ((bits >> m) & 1u) {
                     const double out = A.q_a[fo];
                     int c4[4];
-                    ha_arc(m, g, iall, sgp, c4);
+                    ha_arc(m, e, iall, sgp, c4);
                     double lo_v = Q(c4[0]), hi_v = lo_v;
-                    int il = c4[0], ih = c4[0];
+                    uint32_t il = 0, ih = 0;
 #pragma unroll
                     for (int t = 1; t < 4; ++t) {
                         const double v = Q(c4[t]);
-                        if (v < lo_v) { lo_v = v; il = c4[t]; }
-                        if (v > hi_v) { hi_v = v; ih = c4[t]; }
+                        if (v < lo_v) { lo_v = v; il = t; }
+                        if (v > hi_v) { hi_v = v; ih = t; }
                     }
                     if (out == A.Qpost[fb + T]) kind = 1;
                     else if (out == A.Qpost[plane + fb + T]) kind = 2;
-                    else if (out == lo_v) { kind = 3; cell = il; }
-                    else { kind = 4; cell = ih; }
+                    else if (out == lo_v) { kind = 3; slot = il; }
+                    else { kind = 4; slot = ih; }
                 }
                 A.qab[fn] = 0.0;
                 A.qab[fo] = kind == 0 ? ob : 0.0;
-                if (kind == 1) atomicAdd(&s_pn[T - lo], ob);
-                else if (kind == 2) atomicAdd(&s_px[T - lo], ob);
-                else if (kind >= 3) addq(cell, ob);
+                r_ob[m * R + se] = ob;
+                fc |= (kind | (slot << 3) | ((uint32_t)(T == nb) << 5)) << (6 * m);
             }
         }
-    }
+        r_fc[se] = fc;
+    };
+
+    // the cotangents of cell c: rows, then faces, in the sweep's order
+    auto gather = [&](int c) {
+        const int sc = c % R;
+        auto sl = [&](int x) {
+            int s = sc + (x - c);
+            s += s < 0 ? R : 0;
+            s -= s >= R ? R : 0;
+            return s;
+        };
+        const int hx[7] = {-iall - 1, -iall, -1, 0, 1, iall, iall + 1};
+        double acc = 0.0;
+#pragma unroll
+        for (int h = 0; h < 7; ++h) {
+            const int g = c + hx[h];
+            if (g < 0 || g >= gall) continue;
+            const int s = sl(g);
+            const uint64_t code = r_code[s];
+            const uint32_t want = 6 - h;  // c seen from g
+            if (!(code & kInner)) {
+                if (h == 3) {
+                    acc = acc + r_bq[s];
+                    acc = acc + r_bq[R + s];
+                }
+                continue;
+            }
+            // only arcs that contain c can name it (any arc, for the singular-point row)
+            const bool any_arc = sgp && g == iall + 1;
+#pragma unroll
+            for (int mm = 0; mm < 2; ++mm) {
+                const double b = r_bq[mm * R + s];
+                const uint64_t cm = code >> (21 * mm);
+                if (h == 3) acc = acc + r_c0[s] * b;
+                if ((uint32_t)(cm & 7u) == want) acc = acc + r_c1[mm * R + s] * b;
+#pragma unroll
+                for (int f = 0; f < 6; ++f)
+                    if ((arc_has(f, 6 - h) || any_arc) && (uint32_t)((cm >> (3 + 3 * f)) & 7u) == want)
+                        acc = acc + r_cf[f * R + s] * b;
+            }
+        }
+        double pn = 0.0, px = 0.0;
+        // faces of origin p; d: index of c - p (arc_slot's), known after unrolling
+        auto faces_at = [&](int p, int d) {
+            if (p < 0 || p >= gall) return;  // cells that are no origins carry no faces (code 0)
+            const int s = sl(p);
+            const uint32_t fc = r_fc[s];
+#pragma unroll
+            for (int m = 0; m < 3; ++m) {
+                const int ws = (m == 1 && sgp && p == 0) ? arc_slot_sgp(d) : arc_slot(m, d);
+                const bool t_own = d == 4, t_nb = d == (m == 0 ? 3 : (m == 1 ? 1 : 2));
+                if (ws < 0 && !t_own && !t_nb) continue;
+                const uint32_t code = (fc >> (6 * m)) & 63u, kind = code & 7u;
+                if (kind == 0) continue;
+                const double ob = r_ob[m * R + s];
+                if (kind >= 3) {
+                    if (ws >= 0 && ((code >> 3) & 3u) == (uint32_t)ws) acc = acc + ob;
+                } else if ((code >> 5) ? t_nb : t_own) {
+                    if (kind == 1) pn = pn + ob;
+                    else px = px + ob;
+                }
+            }
+        };
+        auto faces = [&](int p) {
+            if (p < 0 || p >= gall) return;
+            const int s = sl(p);
+            const uint32_t fc = r_fc[s];
+#pragma unroll
+            for (int m = 0; m < 3; ++m) {
+                const uint32_t code = (fc >> (6 * m)) & 63u, kind = code & 7u;
+                if (kind == 0) continue;
+                const double ob = r_ob[m * R + s];
+                if (kind >= 3) {
+                    int c4[4];
+                    ha_arc(m, p, iall, sgp, c4);
+                    const uint32_t sl4 = (code >> 3) & 3u;
+                    const int cell = sl4 == 0 ? c4[0] : (sl4 == 1 ? c4[1] : (sl4 == 2 ? c4[2] : c4[3]));
+                    if (cell == c) acc = acc + ob;
+                } else {
+                    const int T = (code >> 5) ? p + (m == 0 ? 1 : (m == 1 ? iall + 1 : iall)) : p;
+                    if (T == c) {
+                        if (kind == 1) pn = pn + ob;
+                        else px = px + ob;
+                    }
+                }
+            }
+        };
+        if (c == 0) {
+            // the clamps send arc cells of row 0's origins to cell 0
+            for (int p = 0; p <= iall; ++p) faces(p);
+        } else {
+            const int po[7] = {-iall - 2, -iall - 1, -iall, -1, 0, 1, iall};
+#pragma unroll
+            for (int h = 0; h < 7; ++h) faces_at(c + po[h], h);
+        }
+        A.qb[fb + c] = acc;
+        A.Qpostb[fb + c] = pn;
+        A.Qpostb[plane + fb + c] = px;
+    };
+
+    // prologue: q of cells [0, 2*iall+2], rows and faces of cells [0, iall]
+    for (int e = tid; e <= 2 * iall + 2 && e < gall; e += HSW) r_q[e % R] = A.q[fb + e];
     __syncthreads();
-    for (int s = tid; s < n_in; s += HCHUNK) {
-        const int gg = lo + s;
-        if (gg < 0 || gg >= gall) continue;
-        if (s_b[s] != 0.0) atomicAdd(&A.qb[fb + gg], s_b[s]);
-        if (s_pn[s] != 0.0) atomicAdd(&A.Qpostb[fb + gg], s_pn[s]);
-        if (s_px[s] != 0.0) atomicAdd(&A.Qpostb[plane + fb + gg], s_px[s]);
+    for (int e = tid; e <= iall && e < gall; e += HSW) form(e);
+    __syncthreads();
+    for (int b0 = 0; b0 < gall; b0 += HSW) {
+        const int eq = b0 + 2 * iall + 3 + tid;
+        if (eq < gall) r_q[eq % R] = A.q[fb + eq];
+        __syncthreads();
+        const int ef = b0 + iall + 1 + tid;
+        if (ef < gall) form(ef);
+        __syncthreads();
+        const int c = b0 + tid;
+        if (c < gall) gather(c);
     }
 }
 
 // Poles: one work-item per (k, l) slab, sequential as the oracle (HLIM:307-363,
 // 412-425); it owns the slab's pole arrays, so no atomics.
-__global__ void __launch_bounds__(HCHUNK) hlim_adjoint_pole(HaArgs A)
+__global__ void __launch_bounds__(HPOLE) hlim_adjoint_pole(HaArgs A)
 {
-    const int t = blockIdx.x * HCHUNK + threadIdx.x;
+    const int t = blockIdx.x * HPOLE + threadIdx.x;
     const int kall = A.kall, npl = A.gall_pl;
     if (!A.have_pl || t >= kall * kLallPl) return;
     const int k = t % kall, l = t / kall;
@@ -661,6 +895,33 @@ __global__ void __launch_bounds__(HCHUNK) hlim_adjoint_pole(HaArgs A)
     const double zerosw = 0.5 - sgn_half(fabs(Cout) - A.eps);
     const double E = 1.0 - Cin - Cout + A.d_pl[fb];
     const double fac = (1.0 - zerosw) / (Cout + zerosw);
+    if (A.chb_pl) {
+        // Qout in ch(v): cmask*ch enters Cin and CQin, ch - cmask*ch Cout
+        double CQmin = 0.0, CQmax = 0.0;
+        for (int v = 1; v < npl; ++v) {
+            int il, ih;
+            double lv, hv;
+            arc(v, il, ih, lv, hv);
+            const double chm = cms[v] * chs[v], c = cms[v];
+            CQmin = CQmin + chm * (c * lv + (1.0 - c) * BIG);
+            CQmax = CQmax + chm * (c * hv + (1.0 - c) * (-BIG));
+        }
+        const double Qmin = (qn - CQmax - nmax * E) / (Cout + zerosw) * (1.0 - zerosw) + qn * zerosw;
+        const double Qmax = (qn - CQmin - nmin * E) / (Cout + zerosw) * (1.0 - zerosw) + qn * zerosw;
+        const double b0 = A.Qprevb_pl[fb], b1 = A.Qprevb_pl[pplane + fb];
+        double *xc = A.chb_pl + fb;
+        xc[0] = 0.0;
+        for (int v = 1; v < npl; ++v) {
+            int il, ih;
+            double lv, hv;
+            arc(v, il, ih, lv, hv);
+            const double c = cms[v];
+            const double imx = c * hv + (1.0 - c) * (-BIG), imn = c * lv + (1.0 - c) * BIG;
+            const double d0 = zerosw != 0.0 ? 0.0 : ((nmax - c * imx) - (1.0 - c) * Qmin) / Cout;
+            const double d1 = zerosw != 0.0 ? 0.0 : ((nmin - c * imn) - (1.0 - c) * Qmax) / Cout;
+            xc[v] = (b0 != 0.0 ? b0 * d0 : 0.0) + (b1 != 0.0 ? b1 * d1 : 0.0);
+        }
+    }
     for (int mm = 0; mm < 2; ++mm) {
         const double bq = A.Qprevb_pl[(size_t)mm * pplane + fb];
         const int sel = mm == 0 ? smax : smin;
@@ -703,27 +964,21 @@ __global__ void __launch_bounds__(HCHUNK) hlim_adjoint_pole(HaArgs A)
     }
 }
 
-__global__ void fill_zero_kernel(double *p, size_t n)
-{
-    for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (size_t)gridDim.x * blockDim.x)
-        p[t] = 0.0;
-}
-
 // ---- horizontal_flux (dyn_horiz_adv_flux/src/mod_src_tracer.f90:42-379) -------
 //
 // Linear in rho*v; GRD_xc carries 1/max(rho_arc, EPS).  The arc cotangents
-// are gathered, not scattered: a work-group of cells of one slab
-// computes the cotangent of every arc whose triangles touch its cells
-// (origins [g0-iall-1, g0+cells+iall)) into LDS, then each work-item sums
+// are gathered, not scattered: one work-group per (k, l) slab sweeps it in
+// steps of FSW cells.  Each step computes the cotangents of the next FSW
+// arc origins (each origin's three arcs, its four triangles formed once)
+// into an LDS ring of R = FSW + 2*iall + 1 origins, then each work-item sums
 // its cell's six triangles in the order of a sequential sweep over triangle
-// origins, so the result is the oracle's, bit for bit, with no atomics.  The
-// tracer fields are read from HBM directly (neighbouring arcs hit the
-// caches), which keeps LDS to the arc cotangents.  (Staging them too, with
-// 256-cell chunks, ran at 3.05 ms at gl7; 512-cell chunks with two passes
-// over the origins at 1.91 ms.)
-// A work-group of FBLOCK work-items covers FBLOCK - (2*iall+1) cells, so the
-// arc phase (FBLOCK origins) takes one pass.
-constexpr int FBLOCK = 1024;
+// origins, so the result is the oracle's, bit for bit, with no atomics and
+// no arc computed twice.  The tracer fields are read from HBM directly
+// (neighbouring arcs hit the caches), which keeps LDS to the arc cotangents.
+// (Earlier forms: chunks of 1024 cells with their halo of arcs recomputed,
+// one work-group per CU for its 96 KiB of LDS, 1.25 ms at gl7; with the
+// tracer fields staged too, 256-cell chunks, 3.05 ms.)
+constexpr int FSW = 256;
 
 struct FaArgs {
     const double *flxb, *xcb;   // (gall,kall,lall,6), (gall,kall,lall,3,3)
@@ -731,7 +986,7 @@ struct FaArgs {
     const double *X[4];
     const double *gt, *ga, *gp;
     const int *have_sgp;
-    int iall, gall, kall, lall, cells;
+    int iall, gall, kall, lall, ring;
     double dt, eps;
     // poles
     const double *flxb_pl, *xcb_pl;
@@ -741,17 +996,13 @@ struct FaArgs {
     int gall_pl, have_pl;
 };
 
-__global__ void __launch_bounds__(FBLOCK) hflux_adjoint_kernel(FaArgs A)
+__global__ void __launch_bounds__(FSW) hflux_adjoint_kernel(FaArgs A)
 {
     extern __shared__ double lds[];
-    const int iall = A.iall, gall = A.gall, kall = A.kall, lall = A.lall, cells = A.cells;
-    const int nchunk = (gall + cells - 1) / cells;
-    const int b = blockIdx.x;
-    const int chunk = b % nchunk, kl = b / nchunk;
-    const int k = kl % kall, l = kl / kall;
-    const int g0 = chunk * cells;
-    const int alo = g0 - iall - 1, na = cells + 2 * iall + 1;
-    double *s_a = lds;  // [3 arcs][4 fields][na]
+    const int iall = A.iall, gall = A.gall, kall = A.kall, lall = A.lall, R = A.ring;
+    const int k = blockIdx.x % kall, l = blockIdx.x / kall;
+    if (l >= lall) return;
+    double *s_a = lds;  // [3 arcs][4 fields][R]
     const size_t fb = ((size_t)l * kall + k) * gall;
     const size_t plane = (size_t)lall * kall * gall;
     const size_t mstr = (size_t)lall * gall, mb = (size_t)l * gall;
@@ -765,10 +1016,11 @@ __global__ void __launch_bounds__(FBLOCK) hflux_adjoint_kernel(FaArgs A)
         return X(f, c) * W(c, 1, 2) + X(f, c + iall + 1) * W(c, 1, 3) + X(f, c + iall) * W(c, 1, 4);
     };
     const double dt = A.dt, EPS = A.eps;
-    for (int s = tid; s < na; s += FBLOCK) {
-        const int a0 = alo + s;
-        const int i = a0 >= 0 ? a0 % iall : -1, j = a0 >= 0 ? a0 / iall : -1;
-        const bool org = a0 >= 0 && a0 < gall && i <= iall - 2 && j <= iall - 2;
+    // the three arcs of origin a0 (zero where a0 is no origin)
+    auto arcs = [&](int a0) {
+        const int s = a0 % R;
+        const int i = a0 % iall, j = a0 / iall;
+        const bool org = a0 < gall && i <= iall - 2 && j <= iall - 2;
         // the four triangles the origin's arcs use: TI(a0), TJ(a0), TJ(a0-iall) (AI), TI(a0-1) (AJ).
         // They touch six cells (a0, its E, N, NE, S, W neighbours): each field is read there once, and
         // the sums are formed in the forward's order.  The singular-point substitution (TI(0) = TJ(1))
@@ -827,36 +1079,50 @@ __global__ void __launch_bounds__(FBLOCK) hflux_adjoint_kernel(FaArgs A)
                 bv[0] = live_rho ? r2b : 0.0;
             }
 #pragma unroll
-            for (int f = 0; f < 4; ++f) s_a[((a - 1) * 4 + f) * na + s] = bv[f];
+            for (int f = 0; f < 4; ++f) s_a[((a - 1) * 4 + f) * R + s] = bv[f];
         }
-    }
-    __syncthreads();
-    const int X0 = g0 + tid;
-    if (tid >= cells || X0 >= gall) return;
-    auto AB = [&](int a, int f, int c) { return s_a[((a - 1) * 4 + f) * na + c - alo]; };
-    // triangle origins of the cell's six triangles, and whether each is one (i, j in [0, iall-2])
-    const int i0 = X0 % iall, j0 = X0 / iall;
-    const bool o_sw = i0 >= 1 && j0 >= 1 && i0 - 1 <= iall - 2 && j0 - 1 <= iall - 2;  // X0-iall-1
-    const bool o_s = j0 >= 1 && i0 <= iall - 2 && j0 - 1 <= iall - 2;                  // X0-iall
-    const bool o_w = i0 >= 1 && i0 - 1 <= iall - 2 && j0 <= iall - 2;                  // X0-1
-    const bool o_c = i0 <= iall - 2 && j0 <= iall - 2;                                 // X0
-    const int c_sw = X0 - iall - 1, c_s = X0 - iall, c_w = X0 - 1;
-    const double w_sw_i = o_sw ? W(c_sw, 0, 4) : 0.0, w_sw_j = o_sw ? W(c_sw, 1, 3) : 0.0;
-    const double w_s_j = o_s ? W(c_s, 1, 4) : 0.0, w_w_i = o_w ? W(c_w, 0, 3) : 0.0;
-    const double w_c_i = o_c ? W(X0, 0, 2) : 0.0, w_c_j = o_c ? W(X0, 1, 2) : 0.0;
-    // triangle cotangents: TI(c) from AI(c), AIJ(c), AJ(c+1); TJ(c) from AI(c+iall), AIJ(c), AJ(c)
-    auto tib0 = [&](int f, int c) { return ((0.0 + AB(1, f, c)) + AB(2, f, c)) + AB(3, f, c + 1); };
-    auto tjb0 = [&](int f, int c) { return ((0.0 + AB(1, f, c + iall)) + AB(2, f, c)) + AB(3, f, c); };
-    auto tib = [&](int f, int c) { return (sgp && c == 0) ? 0.0 : tib0(f, c); };
-    auto tjb = [&](int f, int c) { return (sgp && c == 1) ? tjb0(f, 1) + tib0(f, 0) : tjb0(f, c); };
+    };
+    // the cell's four cotangents from its six triangles
+    auto cell = [&](int X0) {
+        const int sc = X0 % R;
+        auto AB = [&](int a, int f, int c) {
+            int q = sc + (c - X0);
+            q += q < 0 ? R : 0;
+            q -= q >= R ? R : 0;
+            return s_a[((a - 1) * 4 + f) * R + q];
+        };
+        // triangle origins of the cell's six triangles, and whether each is one (i, j in [0, iall-2])
+        const int i0 = X0 % iall, j0 = X0 / iall;
+        const bool o_sw = i0 >= 1 && j0 >= 1 && i0 - 1 <= iall - 2 && j0 - 1 <= iall - 2;  // X0-iall-1
+        const bool o_s = j0 >= 1 && i0 <= iall - 2 && j0 - 1 <= iall - 2;                  // X0-iall
+        const bool o_w = i0 >= 1 && i0 - 1 <= iall - 2 && j0 <= iall - 2;                  // X0-1
+        const bool o_c = i0 <= iall - 2 && j0 <= iall - 2;                                 // X0
+        const int c_sw = X0 - iall - 1, c_s = X0 - iall, c_w = X0 - 1;
+        const double w_sw_i = o_sw ? W(c_sw, 0, 4) : 0.0, w_sw_j = o_sw ? W(c_sw, 1, 3) : 0.0;
+        const double w_s_j = o_s ? W(c_s, 1, 4) : 0.0, w_w_i = o_w ? W(c_w, 0, 3) : 0.0;
+        const double w_c_i = o_c ? W(X0, 0, 2) : 0.0, w_c_j = o_c ? W(X0, 1, 2) : 0.0;
+        // triangle cotangents: TI(c) from AI(c), AIJ(c), AJ(c+1); TJ(c) from AI(c+iall), AIJ(c), AJ(c)
+        auto tib0 = [&](int f, int c) { return ((0.0 + AB(1, f, c)) + AB(2, f, c)) + AB(3, f, c + 1); };
+        auto tjb0 = [&](int f, int c) { return ((0.0 + AB(1, f, c + iall)) + AB(2, f, c)) + AB(3, f, c); };
+        auto tib = [&](int f, int c) { return (sgp && c == 0) ? 0.0 : tib0(f, c); };
+        auto tjb = [&](int f, int c) { return (sgp && c == 1) ? tjb0(f, 1) + tib0(f, 0) : tjb0(f, c); };
 #pragma unroll
-    for (int f = 0; f < 4; ++f) {
-        double y = 0.0;
-        if (o_sw) y = y + (tib(f, c_sw) * w_sw_i + tjb(f, c_sw) * w_sw_j);
-        if (o_s) y = y + tjb(f, c_s) * w_s_j;
-        if (o_w) y = y + tib(f, c_w) * w_w_i;
-        if (o_c) y = y + (tib(f, X0) * w_c_i + tjb(f, X0) * w_c_j);
-        A.Xb[f][fb + X0] = y;
+        for (int f = 0; f < 4; ++f) {
+            double y = 0.0;
+            if (o_sw) y = y + (tib(f, c_sw) * w_sw_i + tjb(f, c_sw) * w_sw_j);
+            if (o_s) y = y + tjb(f, c_s) * w_s_j;
+            if (o_w) y = y + tib(f, c_w) * w_w_i;
+            if (o_c) y = y + (tib(f, X0) * w_c_i + tjb(f, X0) * w_c_j);
+            A.Xb[f][fb + X0] = y;
+        }
+    };
+    // origins [b0 - iall - 1, b0 + FSW + iall - 1] are in the ring while cells [b0, b0 + FSW) gather
+    for (int e = tid; e < iall; e += FSW) arcs(e);
+    for (int b0 = 0; b0 < gall; b0 += FSW) {
+        arcs(b0 + iall + tid);
+        __syncthreads();
+        if (b0 + tid < gall) cell(b0 + tid);
+        __syncthreads();
     }
 }
 
@@ -933,15 +1199,15 @@ int launch_hflux_adjoint(nicam_ctx c, const double *flxb, const double *flxb_pl,
     A.iall = D.gall_1d; A.gall = D.gall; A.kall = D.kall; A.lall = D.lall;
     A.dt = dt; A.eps = c->eps;
     A.gall_pl = D.gall_pl; A.have_pl = D.have_pl;
-    const int iall = D.gall_1d;
-    A.cells = FBLOCK - (2 * iall + 1);
-    const long nblk = (long)((D.gall + A.cells - 1) / A.cells) * D.kall * D.lall;
+    // the ring covers the origins [b0 - iall - 1, b0 + FSW + iall - 1] a step reads
+    A.ring = FSW + 2 * D.gall_1d + 1;
+    const long nblk = (long)D.kall * D.lall;
     if (nblk > 0x7fffffffL) return fail(NICAM_ERR_ARG, "grid too large for one launch");
-    const size_t lds = sizeof(double) * 12 * (size_t)FBLOCK;
+    const size_t lds = sizeof(double) * 12 * (size_t)A.ring;
     if (lds > 64 * 1024)
         NICAM_HIP(hipFuncSetAttribute((const void *)hflux_adjoint_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
                                       (int)lds));
-    hipLaunchKernelGGL(hflux_adjoint_kernel, dim3((unsigned)nblk), dim3(FBLOCK), lds, c->stream, A);
+    hipLaunchKernelGGL(hflux_adjoint_kernel, dim3((unsigned)nblk), dim3(FSW), lds, c->stream, A);
     if (D.have_pl)
         hipLaunchKernelGGL(hflux_adjoint_pole, dim3((unsigned)((D.kall * kLallPl + 63) / 64)), dim3(64), 0, c->stream,
                            A);
@@ -954,12 +1220,12 @@ int launch_hlim_adjoint(nicam_ctx c, double *qab, double *qab_pl, double *qb, do
                         const double *q_a_pl, const double *q, const double *q_pl, const double *d,
                         const double *d_pl, const double *ch, const double *ch_pl, const double *cmask,
                         const double *cmask_pl, const double *Qpost, const double *Qpost_pl, const uint8_t *mask,
-                        const uint8_t *mask_pl)
+                        const uint8_t *mask_pl, double *chb, double *chb_pl)
 {
     const Dims &D = c->d;
     if (D.gall_1d > HMAXI) return fail(NICAM_ERR_ARG, "gall_1d larger than the LDS tiles (160)");
     HaArgs A;
-    A.qab = qab; A.qb = qb; A.Qpostb = Qpostb; A.Qprevb = Qprevb;
+    A.qab = qab; A.qb = qb; A.Qpostb = Qpostb; A.Qprevb = Qprevb; A.chb = chb; A.chb_pl = chb_pl;
     A.q_a = q_a; A.q = q; A.d = d; A.ch = ch; A.cmask = cmask; A.Qpost = Qpost; A.mask = mask;
     A.have_sgp = c->have_sgp;
     A.iall = D.gall_1d; A.gall = D.gall; A.kall = D.kall; A.lall = D.lall;
@@ -968,16 +1234,18 @@ int launch_hlim_adjoint(nicam_ctx c, double *qab, double *qab_pl, double *qb, do
     A.q_a_pl = q_a_pl; A.q_pl = q_pl; A.d_pl = d_pl; A.ch_pl = ch_pl; A.cmask_pl = cmask_pl;
     A.Qpost_pl = Qpost_pl; A.mask_pl = mask_pl;
     A.gall_pl = D.gall_pl; A.have_pl = D.have_pl;
-    const long nmain = (long)((D.gall + HCHUNK - 1) / HCHUNK) * D.kall * D.lall;
-    if (nmain > 0x7fffffffL) return fail(NICAM_ERR_ARG, "grid too large for one launch");
-    A.nmain = (int)nmain;
-    const size_t nreg = (size_t)D.gall * D.kall * D.lall;
-    hipLaunchKernelGGL(fill_zero_kernel, dim3(2048), dim3(256), 0, c->stream, qb, nreg);
-    hipLaunchKernelGGL(fill_zero_kernel, dim3(2048), dim3(256), 0, c->stream, Qpostb, 2 * nreg);
-    hipLaunchKernelGGL(hlim_adjoint_kernel, dim3((unsigned)nmain), dim3(HCHUNK), 0, c->stream, A);
+    // the ring covers the cells [b0 - iall - 2, b0 + HSW + iall] a step reads
+    A.ring = HSW + 2 * D.gall_1d + 3;
+    const long nslab = (long)D.kall * D.lall;
+    if (nslab > 0x7fffffffL) return fail(NICAM_ERR_ARG, "grid too large for one launch");
+    const size_t lds = (size_t)A.ring * (16 * sizeof(double) + sizeof(uint32_t));
+    if (lds > 64 * 1024)
+        NICAM_HIP(hipFuncSetAttribute((const void *)hlim_adjoint_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                      (int)lds));
+    hipLaunchKernelGGL(hlim_adjoint_kernel, dim3((unsigned)nslab), dim3(HSW), lds, c->stream, A);
     if (D.have_pl)
-        hipLaunchKernelGGL(hlim_adjoint_pole, dim3((unsigned)((D.kall * kLallPl + HCHUNK - 1) / HCHUNK)),
-                           dim3(HCHUNK), 0, c->stream, A);
+        hipLaunchKernelGGL(hlim_adjoint_pole, dim3((unsigned)((D.kall * kLallPl + HPOLE - 1) / HPOLE)), dim3(HPOLE),
+                           0, c->stream, A);
     NICAM_HIP(hipGetLastError());
     return NICAM_OK;
 }
@@ -985,13 +1253,13 @@ int launch_hlim_adjoint(nicam_ctx c, double *qab, double *qab_pl, double *qb, do
 int launch_vlim_adjoint(nicam_ctx c, double *qhb, double *qhb_pl, double *qb, double *qb_pl, const double *q_h,
                         const double *q_h_pl, const double *q, const double *q_pl, const double *d,
                         const double *d_pl, const double *ck, const double *ck_pl, const uint32_t *mask,
-                        const uint32_t *mask_pl)
+                        const uint32_t *mask_pl, double *ckb, double *ckb_pl)
 {
     const Dims &D = c->d;
     if (D.kall - 3 <= 0) return NICAM_OK;
     VlaArgs A;
-    A.reg = VlaField{qhb, qb, q_h, q, d, ck, mask};
-    A.pl = VlaField{qhb_pl, qb_pl, q_h_pl, q_pl, d_pl, ck_pl, mask_pl};
+    A.reg = VlaField{qhb, qb, ckb, q_h, q, d, ck, mask};
+    A.pl = VlaField{qhb_pl, qb_pl, ckb_pl, q_h_pl, q_pl, d_pl, ck_pl, mask_pl};
     A.gall = D.gall;
     A.kall = D.kall;
     A.lall = D.lall;

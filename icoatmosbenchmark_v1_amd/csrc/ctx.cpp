@@ -241,6 +241,10 @@ int nicam_ctx_set_tuning(nicam_ctx c, int param, int value)
         if (value > 4) return fail(NICAM_ERR_ARG, "NICAM_TUNE_VI_VARIANT must be 0..4");
         c->tune.vi_variant = value;
         break;
+    case NICAM_TUNE_HLIM_VARIANT:
+        if (value > 2) return fail(NICAM_ERR_ARG, "NICAM_TUNE_HLIM_VARIANT must be 0..2");
+        c->tune.hlim_variant = value;
+        break;
     default: return fail(NICAM_ERR_ARG, "unknown tuning parameter");
     }
     return NICAM_OK;
@@ -965,9 +969,10 @@ int nicam_horizontal_limiter_thuburn_adjoint(nicam_ctx c, double *q_a_b, double 
                                              const double *d, const double *d_pl, const double *ch,
                                              const double *ch_pl, const double *cmask, const double *cmask_pl,
                                              const double *Qpost, const double *Qpost_pl, const uint8_t *mask,
-                                             const uint8_t *mask_pl)
+                                             const uint8_t *mask_pl, double *ch_b, double *ch_b_pl)
 {
     NICAM_TRY(check_ctx(c));
+    if (ch_b && c->d.have_pl && !ch_b_pl) return fail(NICAM_ERR_ARG, "ch_b given without ch_b_pl");
     const void *req[] = {q_a_b, q_b, Qprev_b, Qpost_b, q_a, q, d, ch, cmask, Qpost, mask};
     const char *nm[] = {"q_a_b", "q_b", "Qout_prev_b", "Qout_post_b", "q_a", "q", "d", "ch", "cmask",
                         "Qout_post", "mask"};
@@ -980,7 +985,8 @@ int nicam_horizontal_limiter_thuburn_adjoint(nicam_ctx c, double *q_a_b, double 
     if (c->ptr_mode == NICAM_PTR_DEVICE)
         return launch_hlim_adjoint(c, q_a_b, PL(q_a_b_pl), q_b, PL(q_b_pl), Qprev_b, PL(Qprev_b_pl), Qpost_b,
                                    PL(Qpost_b_pl), q_a, PL(q_a_pl), q, PL(q_pl), d, PL(d_pl), ch, PL(ch_pl), cmask,
-                                   PL(cmask_pl), Qpost, PL(Qpost_pl), mask, PL(mask_pl));
+                                   PL(cmask_pl), Qpost, PL(Qpost_pl), mask, PL(mask_pl), ch_b,
+                                   ch_b ? PL(ch_b_pl) : nullptr);
     const size_t R = c->d.nreg() * 8, P = c->d.npl() * 8;
     std::vector<HostArg> a = {{q_a_b, R * 6, true},      {PL(q_a_b_pl), P, true},       {q_b, R, true},
                               {PL(q_b_pl), P, true},     {Qprev_b, R * 2, false},       {PL(Qprev_b_pl), P * 2, false},
@@ -989,13 +995,15 @@ int nicam_horizontal_limiter_thuburn_adjoint(nicam_ctx c, double *q_a_b, double 
                               {d, R, false},             {PL(d_pl), P, false},          {ch, R * 6, false},
                               {PL(ch_pl), P, false},     {cmask, R * 6, false},         {PL(cmask_pl), P, false},
                               {Qpost, R * 2, false},     {PL(Qpost_pl), P * 2, false},  {mask, R / 8, false},
-                              {PL(mask_pl), P / 8, false}};
+                              {PL(mask_pl), P / 8, false}, {ch_b, R * 6, true},
+                              {ch_b ? PL(ch_b_pl) : nullptr, P, true}};
     std::vector<void *> v;
     NICAM_TRY(stage_in(c, a, v));
     auto D = [&](int i) { return static_cast<double *>(v[i]); };
     NICAM_TRY(launch_hlim_adjoint(c, D(0), D(1), D(2), D(3), D(4), D(5), D(6), D(7), D(8), D(9), D(10), D(11),
                                   D(12), D(13), D(14), D(15), D(16), D(17), D(18), D(19),
-                                  static_cast<const uint8_t *>(v[20]), static_cast<const uint8_t *>(v[21])));
+                                  static_cast<const uint8_t *>(v[20]), static_cast<const uint8_t *>(v[21]), D(22),
+                                  D(23)));
     return stage_out(c, a, v);
 }
 
@@ -1037,9 +1045,11 @@ int nicam_vertical_limiter_thuburn_mask(nicam_ctx c, double *q_h, double *q_h_pl
 int nicam_vertical_limiter_thuburn_adjoint(nicam_ctx c, double *q_h_b, double *q_h_b_pl, double *q_b, double *q_b_pl,
                                            const double *q_h, const double *q_h_pl, const double *q,
                                            const double *q_pl, const double *d, const double *d_pl, const double *ck,
-                                           const double *ck_pl, const uint32_t *mask, const uint32_t *mask_pl)
+                                           const double *ck_pl, const uint32_t *mask, const uint32_t *mask_pl,
+                                           double *ck_b, double *ck_b_pl)
 {
     NICAM_TRY(check_ctx(c));
+    if (ck_b && c->d.have_pl && !ck_b_pl) return fail(NICAM_ERR_ARG, "ck_b given without ck_b_pl");
     const void *req[] = {q_h_b, q_b, q_h, q, d, ck, mask};
     const char *nm[] = {"q_h_b", "q_b", "q_h", "q", "d", "ck", "mask"};
     for (int i = 0; i < 7; ++i) NICAM_TRY(need(req[i], nm[i]));
@@ -1049,19 +1059,21 @@ int nicam_vertical_limiter_thuburn_adjoint(nicam_ctx c, double *q_h_b, double *q
     }
     if (c->ptr_mode == NICAM_PTR_DEVICE)
         return launch_vlim_adjoint(c, q_h_b, PL(q_h_b_pl), q_b, PL(q_b_pl), q_h, PL(q_h_pl), q, PL(q_pl), d,
-                                   PL(d_pl), ck, PL(ck_pl), mask, PL(mask_pl));
+                                   PL(d_pl), ck, PL(ck_pl), mask, PL(mask_pl), ck_b, ck_b ? PL(ck_b_pl) : nullptr);
     const size_t R = c->d.nreg() * 8, P = c->d.npl() * 8;
     const size_t M = vlim_mask_words(c->d, false) * 4, MP = vlim_mask_words(c->d, true) * 4;
     std::vector<HostArg> a = {{q_h_b, R, true},        {PL(q_h_b_pl), P, true},      {q_b, R, true},
                               {PL(q_b_pl), P, true},   {q_h, R, false},          {PL(q_h_pl), P, false},
                               {q, R, false},       {PL(q_pl), P, false},     {d, R, false},
                               {PL(d_pl), P, false}, {ck, 2 * R, false},      {PL(ck_pl), 2 * P, false},
-                              {mask, M, false},    {PL(mask_pl), MP, false}};
+                              {mask, M, false},    {PL(mask_pl), MP, false}, {ck_b, 2 * R, true},
+                              {ck_b ? PL(ck_b_pl) : nullptr, 2 * P, true}};
     std::vector<void *> v;
     NICAM_TRY(stage_in(c, a, v));
     auto D = [&](int i) { return static_cast<double *>(v[i]); };
     NICAM_TRY(launch_vlim_adjoint(c, D(0), D(1), D(2), D(3), D(4), D(5), D(6), D(7), D(8), D(9), D(10), D(11),
-                                  static_cast<const uint32_t *>(v[12]), static_cast<const uint32_t *>(v[13])));
+                                  static_cast<const uint32_t *>(v[12]), static_cast<const uint32_t *>(v[13]), D(14),
+                                  D(15)));
     return stage_out(c, a, v);
 }
 

@@ -86,10 +86,12 @@ __device__ void hlim_pole(const HlimArgs<T> &A, int t, int tr);
 
 // MULTI: the tracer loop with the shared fields loaded once before it; the
 // single-tracer instance loads them where the tracer's own work needs them
-// (fewer registers live across the tile's barrier).
+// (fewer registers live across the tile's barrier), unless PRE, which issues
+// every streamed load (ch, cmask, d, Qout_post, q_a) before the barrier so
+// that their latency overlaps the tile's.
 // Work-group b of one grid's launch (the single-grid kernel passes its block
 // index; the grouped kernel of several grids passes the index within the grid).
-template <class T, bool MULTI>
+template <class T, bool MULTI, bool PRE = false>
 __device__ __forceinline__ void hlim_block(const HlimArgs<T> &A, int b)
 {
     // q for cells [g0 - iall - 1, g0 + CHUNK + iall + 2)
@@ -149,7 +151,22 @@ __device__ __forceinline__ void hlim_block(const HlimArgs<T> &A, int b)
         zerosw = T(0.5) - sign_half(tabs(Cout_sum) - A.eps);
         dc = ld_once(&A.d[fb + g]);
     };
-    if constexpr (MULTI) load_shared();
+    if constexpr (MULTI || PRE) load_shared();
+    // PRE: the clamp's inputs too (one tracer)
+    T pre_qa[3] = {T(0), T(0), T(0)}, pre_qpn[3][2] = {}, pre_qpc[2] = {T(0), T(0)};
+    if constexpr (PRE) {
+        if (own && (A.phases & 2)) {
+            const int nb[3] = {g + 1, g + iall + 1, g + iall};
+            pre_qpc[0] = A.Qpost[fb + g];
+            pre_qpc[1] = A.Qpost[plane + fb + g];
+#pragma unroll
+            for (int m = 0; m < 3; ++m) {
+                pre_qpn[m][0] = A.Qpost[fb + nb[m]];
+                pre_qpn[m][1] = A.Qpost[plane + fb + nb[m]];
+                pre_qa[m] = ld_once(&A.q_a[(size_t)m * plane + fb + g]);
+            }
+        }
+    }
 
     for (int tr = 0; tr < (MULTI ? A.ntr : 1); ++tr) {
         const T *q = A.q + tr * A.ts;
@@ -198,7 +215,7 @@ __device__ __forceinline__ void hlim_block(const HlimArgs<T> &A, int b)
         // ---- phase 1: allowable range of the cell (HLIM:216-301) ----
         T qout_min, qout_max;
         const T qc = Q(g);
-        if constexpr (!MULTI) load_shared();
+        if constexpr (!MULTI && !PRE) load_shared();
         if (p1) {
             T qinmn[6], qinmx[6];
 #pragma unroll
@@ -248,7 +265,8 @@ __device__ __forceinline__ void hlim_block(const HlimArgs<T> &A, int b)
         if (own && (A.phases & 2)) {
             uint32_t bits = 0;
             const int nb[3] = {g + 1, g + iall + 1, g + iall};
-            const T qo_min_c = Qpost[fb + g], qo_max_c = Qpost[plane + fb + g];  // re-read by neighbours
+            const T qo_min_c = PRE ? pre_qpc[0] : Qpost[fb + g];  // re-read by neighbours
+            const T qo_max_c = PRE ? pre_qpc[1] : Qpost[plane + fb + g];
 #pragma unroll
             for (int m = 0; m < 3; ++m) {
                 const T c = ocm[m];
@@ -256,8 +274,9 @@ __device__ __forceinline__ void hlim_block(const HlimArgs<T> &A, int b)
                 const T in_mx_s = (c)*omx[m] + (T(1) - c) * (-BIG);
                 const T in_mn_n = (c)*BIG + (T(1) - c) * omn[m];
                 const T in_mx_n = (c) * (-BIG) + (T(1) - c) * omx[m];
-                const T qo_min_n = Qpost[fb + nb[m]], qo_max_n = Qpost[plane + fb + nb[m]];
-                T qa = ld_once(&q_a[(size_t)m * plane + fb + g]);
+                const T qo_min_n = PRE ? pre_qpn[m][0] : Qpost[fb + nb[m]];
+                const T qo_max_n = PRE ? pre_qpn[m][1] : Qpost[plane + fb + nb[m]];
+                T qa = PRE ? pre_qa[m] : ld_once(&q_a[(size_t)m * plane + fb + g]);
                 const T qa_in = qa;
                 qa = (c)*dmin(dmax(qa, in_mn_s), in_mx_s) + (T(1) - c) * dmin(dmax(qa, in_mn_n), in_mx_n);
                 qa = (c)*dmax(dmin(qa, qo_max_n), qo_min_n) + (T(1) - c) * dmax(dmin(qa, qo_max_c), qo_min_c);
@@ -278,10 +297,10 @@ __device__ __forceinline__ void hlim_block(const HlimArgs<T> &A, int b)
     }
 }
 
-template <class T, bool MULTI>
+template <class T, bool MULTI, bool PRE = false>
 __global__ void __launch_bounds__(CHUNK) hlim_kernel(HlimArgs<T> A)
 {
-    hlim_block<T, MULTI>(A, blockIdx.x);
+    hlim_block<T, MULTI, PRE>(A, blockIdx.x);
 }
 
 // Several grids in one launch: work-groups [off[n], off[n+1]) belong to grid n.
@@ -463,6 +482,8 @@ int launch_hlim_t(nicam_ctx c, T *q_a, T *q_a_pl, const T *q, const T *q_pl, con
         A.phases = phases;
         if (ntr > 1)
             hipLaunchKernelGGL((hlim_kernel<T, true>), dim3((unsigned)nblk), dim3(CHUNK), 0, c->stream, A);
+        else if (c->tune.hlim_variant == 2)
+            hipLaunchKernelGGL((hlim_kernel<T, false, true>), dim3((unsigned)nblk), dim3(CHUNK), 0, c->stream, A);
         else
             hipLaunchKernelGGL((hlim_kernel<T, false>), dim3((unsigned)nblk), dim3(CHUNK), 0, c->stream, A);
         NICAM_HIP(hipGetLastError());

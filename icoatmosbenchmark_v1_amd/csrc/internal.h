@@ -43,6 +43,7 @@ struct nicam_tune {
     int vi_variant = 0;   // elimination-factor storage in vi_rhow_solver (1..4)
     int diff_kper = 0;    // levels per work-group in OPRT_diffusion
     int ddmp_kper = 0;    // levels per work-group in OPRT3D_divdamp
+    int hlim_variant = 0; // horizontal limiter: 1 = streamed loads after the tile barrier, 2 = before it
 };
 
 namespace nicam {
@@ -269,14 +270,14 @@ int launch_hlim_adjoint(nicam_ctx c, double *qab, double *qab_pl, double *qb, do
                         const double *q_a_pl, const double *q, const double *q_pl, const double *d,
                         const double *d_pl, const double *ch, const double *ch_pl, const double *cmask,
                         const double *cmask_pl, const double *Qpost, const double *Qpost_pl, const uint8_t *mask,
-                        const uint8_t *mask_pl);
+                        const uint8_t *mask_pl, double *chb, double *chb_pl);
 int launch_vlim_mask(nicam_ctx c, double *q_h, double *q_h_pl, const double *q, const double *q_pl,
                      const double *d, const double *d_pl, const double *ck, const double *ck_pl, uint32_t *mask,
                      uint32_t *mask_pl);
 int launch_vlim_adjoint(nicam_ctx c, double *qhb, double *qhb_pl, double *qb, double *qb_pl, const double *q_h,
                         const double *q_h_pl, const double *q, const double *q_pl, const double *d,
                         const double *d_pl, const double *ck, const double *ck_pl, const uint32_t *mask,
-                        const uint32_t *mask_pl);
+                        const uint32_t *mask_pl, double *ckb, double *ckb_pl);
 int launch_vi_adjoint(nicam_ctx c, double *wb, double *wb_pl, double *w0b, double *w0b_pl, double *p0b,
                       double *p0b_pl, double *r0b, double *r0b_pl, double *Srb, double *Srb_pl, double *Swb,
                       double *Swb_pl, double *Spb, double *Spb_pl, double dt);

@@ -4,6 +4,19 @@
     rocprofv3 --pmc FETCH_SIZE      -d OUT -- python3 -m icoatmosbenchmark_v1_amd.profile_step
     rocprofv3 --pmc WRITE_SIZE      -d OUT -- python3 -m icoatmosbenchmark_v1_amd.profile_step
 
+--mode picks what each step launches:
+  step      the four FP64 kernels (default)
+  chained   the chained FP64 and mixed steps: the _f32 flux and limiters, the
+            mixed rho*w solve, the conservative and vertical updates, the
+            mass sums; the limiter with the library's own Qout exchange
+            (FP64 and FP32) and one COMM_data_transfer
+  vi        every rho*w solve variant (1-4) with FP64 and FP32 inputs, the
+            factorisation (putcoef) and both adjoint variants
+  batched   the grouped multi-tracer step on the gl5-gl8 grids (32 tracers)
+  extras    OPRT_diffusion, OPRT3D_divdamp, the horizontal and vertical step
+            glue and the Mc/Ml/Mu update
+  adjoint   the masked limiters and the four adjoints
+
 Before the step it launches one plain device-to-device copy of exactly
 --copy-mib MiB (torch's copy kernel), whose FETCH_SIZE / WRITE_SIZE readings
 calibrate the counters: bytes moved / counter reading gives the factor that
@@ -26,11 +39,14 @@ def main() -> int:
     ap.add_argument("--kall", type=int, default=98)
     ap.add_argument("--lall", type=int, default=40)
     ap.add_argument("--seed", type=int, default=2024)
-    ap.add_argument("--extras", action="store_true",
-                    help="profile OPRT_diffusion, OPRT3D_divdamp and the step glue instead of the step")
-    ap.add_argument("--adjoint", action="store_true",
-                    help="profile the masked forward limiters and the four adjoint kernels instead of the step")
+    ap.add_argument("--mode", default="step", choices=("step", "chained", "vi", "batched", "extras", "adjoint"))
+    ap.add_argument("--extras", action="store_true", help="same as --mode extras")
+    ap.add_argument("--adjoint", action="store_true", help="same as --mode adjoint")
     args = ap.parse_args()
+    if args.extras:
+        args.mode = "extras"
+    if args.adjoint:
+        args.mode = "adjoint"
 
     import numpy as np
     import torch
@@ -46,6 +62,8 @@ def main() -> int:
     torch.cuda.synchronize()
     print(f"[profile_step] copy of {args.copy_mib} MiB done", flush=True)
     del a, b
+    if args.mode == "batched":
+        return batched(args.steps)
 
     spec = synth.GridSpec(args.gall_1d, args.kall, args.lall, seed=args.seed)
     lv = synth.icosahedral_levels(spec.gall_1d, spec.lall)
@@ -66,10 +84,14 @@ def main() -> int:
     V = up(case["vlim"], drivers.VLIM_IN + ("q_h", "q_h_pl"))
     W = up(case["vi"], [k for k in drivers.VI_IN] + [k + "_pl" for k in drivers.VI_IN] + ["rhogw", "rhogw_pl"])
     torch.cuda.synchronize()
-    if args.extras:
+    if args.mode == "extras":
         return extras(ctx, spec, case, coef, args.steps, up, z)
-    if args.adjoint:
+    if args.mode == "adjoint":
         return adjoint(ctx, spec, args.steps, F, H, V, W, z)
+    if args.mode == "chained":
+        return chained(ctx, spec, case, args.steps, z)
+    if args.mode == "vi":
+        return vi_variants(ctx, spec, args.steps, W)
     for s in range(args.steps):
         ctx.horizontal_flux(F["flx_h"], F["flx_h_pl"], F["GRD_xc"], F["GRD_xc_pl"], *[F[k] for k in drivers.FLUX_IN],
                             360.0)
@@ -128,12 +150,117 @@ def adjoint(ctx, spec, steps, F, H, V, W, z) -> int:
     return 0
 
 
-def extras(ctx, spec, case, coef, steps, up, z) -> int:
-    """OPRT_diffusion, OPRT3D_divdamp and the step glue (courant, gradient,
-    remap, update), each launched once per step."""
+def chained(ctx, spec, case, steps, z) -> int:
+    """The chained FP64 and mixed steps with their mass sums, then the limiter
+    with the library's Qout exchange and one halo transfer, per step."""
+    import numpy as np
     import torch
 
     from icoatmosbenchmark_v1_amd import drivers, synth
+
+    lv = synth.icosahedral_levels(spec.gall_1d, spec.lall)
+    if not lv:
+        print("[profile_step] --mode chained needs an icosahedral grid size", file=sys.stderr)
+        return 2
+    ctx.comm_setup(lv[1])
+    ctx.set_const_f32(float(np.finfo(np.float64).eps), float(np.finfo(np.float32).max))
+    S64 = drivers.chain_state(ctx, spec, case, mixed=False)
+    S32 = drivers.chain_state(ctx, spec, case, mixed=True)
+    m = torch.zeros(spec.kall, dtype=torch.float64, device=S64["rho"].device)
+    var, var_pl = z(spec.shape3), z(spec.shape_pl)
+    for s in range(steps):
+        for S in (S64, S32):
+            drivers.chain_step(ctx, S, 360.0, 60.0)
+            ctx.tracer_mass(S["rhogq"], S["rhogq_pl"], m)
+        for S, single in ((S64, False), (S32, True)):
+            sfx = "_f32" if single else ""
+            getattr(ctx, "horizontal_limiter_thuburn" + sfx)(
+                S["h_q_a"], S["h_q_a_pl"], S["h_q"], S["h_q_pl"], S["h_d"], S["h_d_pl"], S["h_ch"], S["h_ch_pl"],
+                S["h_cmask"], S["h_cmask_pl"], S["Qout_prev"], S["Qout_prev_pl"], None, None)
+        ctx.comm_data_transfer(var, var_pl, 1)
+        ctx.tracer_mass(S32["h_q"], S32["h_q_pl"], m, single=True)
+        torch.cuda.synchronize()
+        print(f"[profile_step] chained {s + 1}/{steps} done", flush=True)
+    ctx.close()
+    return 0
+
+
+def vi_variants(ctx, spec, steps, W) -> int:
+    """Each rho*w solve variant, FP64 and FP32 inputs; putcoef's
+    factorisation; the adjoint forming its factors and reading them."""
+    import torch
+
+    from icoatmosbenchmark_v1_amd import drivers
+    from icoatmosbenchmark_v1_amd.runtime import TUNE_VI_VARIANT
+
+    W32 = {k: (v.float() if k not in ("rhogw", "rhogw_pl") else v) for k, v in W.items()}
+    a64, a32 = [], []
+    for k in drivers.VI_IN:
+        a64 += [W[k], W[k + "_pl"]]
+        a32 += [W32[k], W32[k + "_pl"]]
+    bufs = [torch.zeros_like(W["rhogw"] if i % 2 == 0 else W["rhogw_pl"]) for i in range(12)]
+    wb, wb_pl = torch.randn_like(W["rhogw"]), torch.randn_like(W["rhogw_pl"])
+    Mc, Mc_pl, Ml, Ml_pl, Mu, Mu_pl = (torch.empty_like(W["rhogw"] if i % 2 == 0 else W["rhogw_pl"])
+                                       for i in range(6))
+    ctx.vi_rhow_getcoef(Mc, Mc_pl, Ml, Ml_pl, Mu, Mu_pl)
+    for s in range(steps):
+        ctx.vi_rhow_solver_putcoef(Mc, Mc_pl, Ml, Ml_pl, Mu, Mu_pl)
+        for v in (1, 2, 3, 4):
+            ctx.set_tuning(TUNE_VI_VARIANT, v)
+            ctx.vi_rhow_solver(W["rhogw"], W["rhogw_pl"], *a64, 60.0)
+            ctx.vi_rhow_solver_mixed(W["rhogw"], W["rhogw_pl"], *a32, 60.0)
+        for v in (1, 2):
+            ctx.set_tuning(TUNE_VI_VARIANT, v)
+            ctx.vi_rhow_solver_adjoint(wb, wb_pl, *bufs, 60.0)
+        torch.cuda.synchronize()
+        print(f"[profile_step] vi variants {s + 1}/{steps} done", flush=True)
+    ctx.close()
+    return 0
+
+
+def batched(steps) -> int:
+    """The grouped step of 32 tracers on the gl5-gl8 rl1 grids (98, 78, 58, 42
+    levels), eager launches on a side stream."""
+    import numpy as np
+    import torch
+
+    from icoatmosbenchmark_v1_amd import drivers, group, synth
+
+    dev = torch.device("cuda:0")
+    stream = torch.cuda.Stream(dev)
+    grids = ((5, 98), (6, 78), (7, 58), (8, 42))
+    ctxs, states = [], []
+    for gl, kall in grids:
+        geo, _ = synth.icosahedral_geometry(gl, 1)
+        spec = synth.GridSpec(2 ** (gl - 1) + 2, kall, 40, seed=gl)
+        case = synth.make_case(spec, geo=geo)
+        ctx = drivers.setup_context(spec, geo, case["vi"])
+        ctx.set_stream(stream)
+        ctx.set_const_f32(float(np.finfo(np.float64).eps), float(np.finfo(np.float32).max))
+        ctxs.append(ctx)
+        states.append(group.batched_state(spec, case, 32, dev))
+    G = group.Group(ctxs)
+    G.set_stream(stream)
+    torch.cuda.synchronize()
+    with torch.cuda.stream(stream):
+        for s in range(steps):
+            G.tracer_step_f32(32, states, 360.0, 60.0, use_graph=False)
+            stream.synchronize()
+            print(f"[profile_step] batched {s + 1}/{steps} done", flush=True)
+    G.close()
+    for c in ctxs:
+        c.close()
+    return 0
+
+
+def extras(ctx, spec, case, coef, steps, up, z) -> int:
+    """The vertical glue (mass flux, Courant numbers, face values, update),
+    the Mc/Ml/Mu update, OPRT_diffusion, OPRT3D_divdamp and the horizontal
+    glue (courant, gradient, remap, update), each launched once per step."""
+    import torch
+
+    from icoatmosbenchmark_v1_amd import drivers, synth
+    from icoatmosbenchmark_v1_amd.drivers import DDMP_IN
 
     if coef is None or "coef_grad" not in case["geo"]:
         print("[profile_step] --extras needs an icosahedral grid size", file=sys.stderr)
@@ -156,8 +283,18 @@ def extras(ctx, spec, case, coef, steps, up, z) -> int:
     grad, grad_pl = z((3,) + spec.shape3), z((3,) + spec.shape_pl)
     qa, qa_pl = z((6,) + spec.shape3), z(spec.shape_pl)
     rq, rq_pl = F["rho"] * Q["q"], F["rho_pl"] * Q["q_pl"]
+    fv, fv_pl = z(spec.shape3), z(spec.shape_pl)
+    ck, ck_pl = z((2,) + spec.shape3), z((2,) + spec.shape_pl)
+    qh, qh_pl = z(spec.shape3), z(spec.shape_pl)
+    eth, eth_pl = torch.full_like(fv, 3.0e5), torch.full_like(fv_pl, 3.0e5)
+    gt, gt_pl = torch.full_like(fv, 9.80616), torch.full_like(fv_pl, 9.80616)
     torch.cuda.synchronize()
     for s in range(steps):
+        ctx.vertical_mass_flux(fv, fv_pl, *[G[k] for k in DDMP_IN[:8]], 360.0)
+        ctx.vertical_courant(ck, ck_pl, fv, fv_pl, F["rho"], F["rho_pl"])
+        ctx.vertical_face(qh, qh_pl, Q["q"], Q["q_pl"])
+        ctx.tracer_update_vertical(rq, rq_pl, fv, fv_pl, qh, qh_pl)
+        ctx.vi_rhow_update_matrix(eth, eth_pl, gt, gt_pl, 60.0)
         ctx.oprt_diffusion(D["dscl"], D["dscl_pl"], *[D[k] for k in drivers.DIFF_IN])
         ctx.oprt3d_divdamp(*[G[k] for k in drivers.DDMP_OUT], *[G[k] for k in drivers.DDMP_IN])
         ctx.tracer_courant(ch, ch_pl, cm, cm_pl, flx, flx_pl, F["rho"], F["rho_pl"])

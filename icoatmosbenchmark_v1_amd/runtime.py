@@ -41,6 +41,7 @@ PTR_HOST = 1
 TUNE_HFLUX_KPER = 1
 TUNE_VLIM_KCHUNK = 3
 TUNE_VI_VARIANT = 4
+TUNE_HLIM_VARIANT = 7
 TUNE_DIFF_KPER = 5
 TUNE_DDMP_KPER = 6
 
@@ -113,8 +114,8 @@ SIGNATURES = {
     "nicam_vertical_limiter_thuburn_mask": (c_int, [_P] * 11),
     "nicam_horizontal_limiter_thuburn_mask": (c_int, [_P] * 17),
     "nicam_horizontal_flux_adjoint": (c_int, [_P] * 21 + [c_double]),
-    "nicam_horizontal_limiter_thuburn_adjoint": (c_int, [_P] * 23),
-    "nicam_vertical_limiter_thuburn_adjoint": (c_int, [_P] * 15),
+    "nicam_horizontal_limiter_thuburn_adjoint": (c_int, [_P] * 25),
+    "nicam_vertical_limiter_thuburn_adjoint": (c_int, [_P] * 17),
     "nicam_tracer_mass": (c_int, [_P] * 4),
     "nicam_tracer_mass_f32": (c_int, [_P] * 4),
     "nicam_tracer_mass_noalloc": (c_int, [_P] * 4),
@@ -687,8 +688,10 @@ class Context:
 
     def horizontal_limiter_thuburn_adjoint(self, q_a_b, q_a_b_pl, q_b, q_b_pl, Qout_prev_b, Qout_prev_b_pl,
                                            Qout_post_b, Qout_post_b_pl, q_a, q_a_pl, q, q_pl, d, d_pl, ch, ch_pl,
-                                           cmask, cmask_pl, Qout_post, Qout_post_pl, mask, mask_pl) -> None:
-        """nicam_horizontal_limiter_thuburn_adjoint (Qout_post supplied)."""
+                                           cmask, cmask_pl, Qout_post, Qout_post_pl, mask, mask_pl, ch_b=None,
+                                           ch_b_pl=None) -> None:
+        """nicam_horizontal_limiter_thuburn_adjoint (Qout_post supplied); ch_b, ch_b_pl: the Courant
+        numbers' cotangent, formed when given."""
         R, P = self.nreg, self.npl
         a = self._args([(q_a_b, "q_a_b", 6 * R), (q_a_b_pl, "q_a_b_pl", P), (q_b, "q_b", R), (q_b_pl, "q_b_pl", P),
                         (Qout_prev_b, "Qout_prev_b", 2 * R), (Qout_prev_b_pl, "Qout_prev_b_pl", 2 * P),
@@ -699,6 +702,7 @@ class Context:
                         (Qout_post_pl, "Qout_post_pl", 2 * P)])
         a += [_ptr_u8(mask, self.mode, "mask", R), None if not self.have_pl else _ptr_u8(mask_pl, self.mode,
                                                                                          "mask_pl", P)]
+        a += self._args([(ch_b, "ch_b", 6 * R), (ch_b_pl, "ch_b_pl", P)])
         _check(self.lib.nicam_horizontal_limiter_thuburn_adjoint(self.h, *a),
                "nicam_horizontal_limiter_thuburn_adjoint")
 
@@ -718,8 +722,9 @@ class Context:
         _check(self.lib.nicam_vertical_limiter_thuburn_mask(self.h, *a), "nicam_vertical_limiter_thuburn_mask")
 
     def vertical_limiter_thuburn_adjoint(self, q_h_b, q_h_b_pl, q_b, q_b_pl, q_h, q_h_pl, q, q_pl, d, d_pl, ck,
-                                         ck_pl, mask, mask_pl) -> None:
-        """nicam_vertical_limiter_thuburn_adjoint: cotangents of q_h's input and of q."""
+                                         ck_pl, mask, mask_pl, ck_b=None, ck_b_pl=None) -> None:
+        """nicam_vertical_limiter_thuburn_adjoint: cotangents of q_h's input, of q and (when ck_b is
+        given) of ck."""
         R, P = self.nreg, self.npl
         M, MP = self.vlim_mask_words()
         a = self._args([(q_h_b, "q_h_b", R), (q_h_b_pl, "q_h_b_pl", P), (q_b, "q_b", R), (q_b_pl, "q_b_pl", P),
@@ -727,6 +732,7 @@ class Context:
                         (d_pl, "d_pl", P), (ck, "ck", 2 * R), (ck_pl, "ck_pl", 2 * P)])
         a += [_ptr_u32(mask, self.mode, "mask", M), None if not self.have_pl else _ptr_u32(mask_pl, self.mode,
                                                                                            "mask_pl", MP)]
+        a += self._args([(ck_b, "ck_b", 2 * R), (ck_b_pl, "ck_b_pl", 2 * P)])
         _check(self.lib.nicam_vertical_limiter_thuburn_adjoint(self.h, *a),
                "nicam_vertical_limiter_thuburn_adjoint")
 

@@ -20,9 +20,11 @@ nicam_*_adjoint entry points.  The glue between them, which the reference
 does not have (its kernels take ch, cmask, q_a, ck, q_h as inputs), is
 written here as torch operations on the same arrays in the oracle's layout
 and operation order, so its derivatives come from torch's autograd.  The
-limiters' adjoints hold ch, cmask, ck and d fixed: the dependence of the
-limiters' bounds on rhog through the Courant numbers is not differentiated,
-so the gradient in rhog is exact where no limiter clips.
+limiters' adjoints return the cotangents of the Courant numbers ch and ck,
+through which their bounds depend on rhog; what stays fixed is piecewise
+constant (cmask and the inflow flags, the signs of the Courant numbers) or
+zero here (d), so the gradient in q and rhog is the step's own wherever no
+switch sits exactly at its threshold.
 
 Arrays are C arrays with the Fortran dimensions reversed: q[t][l][k][g],
 q_pl[t][p][k][v]; rhog[l][k][g], rhog_pl[p][k][v].
@@ -33,6 +35,7 @@ from typing import Dict, List, Sequence, Tuple
 
 import numpy as np
 import torch
+import torch.nn.functional as F
 
 from .autodiff import RhowSolve, VerticalLimiter
 from .drivers import VI_IN
@@ -67,8 +70,8 @@ class HorizontalFlux(torch.autograd.Function):
 
 
 class HorizontalLimiter(torch.autograd.Function):
-    """(q_a', Qout_prev) = horizontal_limiter_thuburn(q_a, q, Qout_post; d, ch,
-    cmask) (HLIM:42-430) with Qout_post supplied; d, ch, cmask held fixed."""
+    """(q_a', Qout_prev) = horizontal_limiter_thuburn(q_a, q, Qout_post, ch; d,
+    cmask) (HLIM:42-430) with Qout_post supplied; d and cmask held fixed."""
 
     @staticmethod
     def forward(fctx, nctx: Context, q_a, q_a_pl, q, q_pl, post, post_pl, d, d_pl, ch, ch_pl, cm, cm_pl):
@@ -90,21 +93,50 @@ class HorizontalLimiter(torch.autograd.Function):
         prevb, prevb_pl = c(g_prev, post), c(g_prev_pl, post_pl)
         qb, qb_pl = torch.empty_like(q), torch.empty_like(q_pl)
         postb, postb_pl = torch.empty_like(post), torch.empty_like(post_pl)
+        chb, chb_pl = torch.empty_like(ch), torch.empty_like(ch_pl)
         fctx.nctx.horizontal_limiter_thuburn_adjoint(qab, qab_pl, qb, qb_pl, prevb, prevb_pl, postb, postb_pl, out,
                                                      out_pl, q, q_pl, d, d_pl, ch, ch_pl, cm, cm_pl, post, post_pl,
-                                                     mask, mask_pl)
-        return (None, qab, qab_pl, qb, qb_pl, postb, postb_pl) + (None,) * 6
+                                                     mask, mask_pl, chb, chb_pl)
+        return (None, qab, qab_pl, qb, qb_pl, postb, postb_pl, None, None, chb, chb_pl, None, None)
+
+
+class _HaloGather(torch.autograd.Function):
+    """flat[..., dst] = flat[..., src] with every source read before any
+    write.  Its transpose zeroes the destinations' cotangents and adds each
+    destination's to its source, in rounds whose sources are distinct, so
+    the sums do not depend on the order of atomic additions."""
+
+    @staticmethod
+    def forward(fctx, flat, dst, src, rounds):
+        fctx.dst, fctx.rounds = dst, rounds
+        return flat.index_copy(-1, dst, flat.index_select(-1, src))
+
+    @staticmethod
+    def backward(fctx, g):
+        out = g.contiguous().index_fill(-1, fctx.dst, 0.0)
+        for d, s in fctx.rounds:
+            out.index_add_(-1, s, g.index_select(-1, d))
+        return out, None, None, None
 
 
 class HaloExchange:
     """COMM_data_transfer as index operations: every halo point and pole ring
-    point takes the value of its source (the library's halo map); torch's
-    autograd turns the gather into the matching scatter-add."""
+    point takes the value of its source (the library's halo map)."""
 
     def __init__(self, hm, lall: int, gall: int, gall_pl: int, device):
         code = lambda kind, l, g: np.where(kind == 0, l * gall + g, lall * gall + l * gall_pl + g)  # noqa: E731
-        self.dst = torch.from_numpy(code(hm["dst_kind"], hm["dst_l"], hm["dst_g"]).astype(np.int64)).to(device)
-        self.src = torch.from_numpy(code(hm["src_kind"], hm["src_l"], hm["src_g"]).astype(np.int64)).to(device)
+        dst = code(hm["dst_kind"], hm["dst_l"], hm["dst_g"]).astype(np.int64)
+        src = code(hm["src_kind"], hm["src_l"], hm["src_g"]).astype(np.int64)
+        self.dst = torch.from_numpy(dst).to(device)
+        self.src = torch.from_numpy(src).to(device)
+        # the r-th pair of each source, in (source, destination) order, goes to round r
+        order = np.lexsort((dst, src))
+        ss, dd = src[order], dst[order]
+        first = np.r_[True, ss[1:] != ss[:-1]]
+        start = np.maximum.accumulate(np.where(first, np.arange(len(ss)), 0))
+        rank = np.arange(len(ss)) - start
+        self.rounds = [(torch.from_numpy(dd[rank == r]).to(device), torch.from_numpy(ss[rank == r]).to(device))
+                       for r in range(int(rank.max()) + 1 if len(ss) else 0)]
         self.lall, self.gall, self.gall_pl = lall, gall, gall_pl
 
     def __call__(self, x, x_pl):
@@ -114,7 +146,7 @@ class HaloExchange:
         a = x.reshape(-1, self.lall, kall, self.gall).permute(0, 2, 1, 3).reshape(-1, kall, self.lall * self.gall)
         b = x_pl.reshape(-1, 2, kall, self.gall_pl).permute(0, 2, 1, 3).reshape(-1, kall, 2 * self.gall_pl)
         flat = torch.cat([a, b], dim=-1)
-        out = flat.index_copy(-1, self.dst, flat.index_select(-1, self.src))
+        out = _HaloGather.apply(flat, self.dst, self.src, self.rounds)
         a2 = out[..., :self.lall * self.gall].reshape(-1, kall, self.lall, self.gall).permute(0, 2, 1, 3)
         b2 = out[..., self.lall * self.gall:].reshape(-1, kall, 2, self.gall_pl).permute(0, 2, 1, 3)
         return (a2.reshape(lead + (self.lall, kall, self.gall)).contiguous(),
@@ -126,8 +158,8 @@ class TracerStep:
     (q, q_pl, rhog, rhog_pl) and get the same four back, one step on."""
 
     def __init__(self, ctx: Context, spec, geo, case, hm, fac, dt: float = 360.0, dt_vi: float = 60.0,
-                 courant_v: float = 0.3):
-        dev = torch.device("cuda:0")
+                 courant_v: float = 0.3, device="cuda:0"):
+        dev = torch.device(device)
         T = lambda a: torch.from_numpy(np.ascontiguousarray(a, dtype=np.float64)).to(dev)  # noqa: E731
         self.ctx, self.spec, self.dt, self.dt_vi = ctx, spec, dt, dt_vi
         n1, kall = spec.gall_1d, spec.kall
@@ -147,11 +179,8 @@ class TracerStep:
         inner = np.zeros((n1, n1), dtype=bool)
         inner[1:-1, 1:-1] = True
         self.inner = torch.from_numpy(inner.reshape(-1)).to(dev)
-        self.gidx = torch.arange(n1 + 1, spec.gall - n1 - 1, device=dev)  # the flat range gmin..gmax
+        self.glo, self.ghi = n1 + 1, spec.gall - n1 - 1  # the flat range gmin..gmax
         self.goff = (0, 1, n1 + 1, n1, -1, -n1 - 1, -n1)
-        ij = [(j * n1 + i) for j in range(n1 - 1) for i in range(n1 - 1)]  # arc origins i, j in [gmin-1, gmax]
-        self.oidx = torch.tensor(ij, device=dev)
-        self.noff = (1, n1 + 1, n1)
         kk = torch.arange(kall, device=dev)
         self.kin = ((kk >= 1) & (kk <= kall - 2)).to(torch.float64)[:, None]        # kmin..kmax
         self.kface = ((kk >= 2) & (kk <= kall - 2)).to(torch.float64)[:, None]      # interior half levels
@@ -182,15 +211,18 @@ class TracerStep:
 
     def gradient(self, q, q_pl):
         """q [l][k][g] -> grad [3][l][k][g] on the flat range gmin..gmax (zero
-        elsewhere); the pole centre from its ring."""
-        gi = self.gidx
+        elsewhere); the pole centre from its ring.  Slices and padding, not
+        index tensors: their transposes are gathers, so the gradient's sums do
+        not depend on the order of atomic additions."""
+        lo, hi = self.glo, self.ghi
         parts = []
         for d in range(3):
             s = None
             for m in range(7):
-                t = self.cg[:, d, m, gi][:, None, :] * q[:, :, gi + self.goff[m]]
+                o = self.goff[m]
+                t = self.cg[:, d, m, lo:hi][:, None, :] * q[:, :, lo + o:hi + o]
                 s = t if s is None else s + t
-            parts.append(torch.zeros_like(q).index_copy(2, gi, s))
+            parts.append(F.pad(s, (lo, self.spec.gall - hi)))
         gp = []
         for d in range(3):
             s = (self.cg_pl[:, d, :][:, None, :] * q_pl).sum(-1)
@@ -199,24 +231,30 @@ class TracerStep:
 
     def remap(self, q, q_pl, grad, grad_pl, cm, cm_pl, xc, xc_pl):
         """Upwind face values q + grad.(GRD_xc - GRD_x) of every arc from the
-        origins i, j in [gmin-1, gmax]; the faces no arc reaches stay 0."""
-        oi = self.oidx
+        origins i, j in [gmin-1, gmax]; the faces no arc reaches stay 0.  The
+        origins and their neighbours are slices of the (j, i) view."""
+        n = self.n1
+        sq = lambda x: x.reshape(x.shape[:-1] + (n, n))  # noqa: E731  [..., j, i]
+        org = lambda x: sq(x)[..., :n - 1, :n - 1]  # noqa: E731
+        # neighbour of arc a: (i+1, j), (i+1, j+1), (i, j+1); the padding that puts it back in place
+        nbs = (lambda x: sq(x)[..., :n - 1, 1:], lambda x: sq(x)[..., 1:, 1:], lambda x: sq(x)[..., 1:, :n - 1])
+        pads = ((1, 0, 0, 1), (1, 0, 1, 0), (0, 1, 1, 0))
+        flat = lambda x: x.reshape(x.shape[:-2] + (n * n,))  # noqa: E731
         planes = [None] * 6
         for a in range(3):
-            nb = oi + self.noff[a]
-            xa = xc[:, a][:, :, :, oi]  # [d][l][k][n]
+            xa = xc[:, a]  # [d][l][k][g]
 
-            def rec(idx):
+            def rec(sel):
                 r = None
                 for d in range(3):
-                    t = grad[d][:, :, idx] * (xa[d] - self.X[d][:, None, idx])
+                    t = sel(grad[d]) * (org(xa[d]) - sel(self.X[d])[:, None])
                     r = t if r is None else r + t
-                return q[:, :, idx] + r
+                return sel(q) + r
 
-            qs, qn = rec(oi), rec(nb)
-            cs, cn = cm[a][:, :, oi], cm[a + 3][:, :, nb]
-            planes[a] = torch.zeros_like(q).index_copy(2, oi, cs * qn + (1.0 - cs) * qs)
-            planes[a + 3] = torch.zeros_like(q).index_copy(2, nb, cn * qs + (1.0 - cn) * qn)
+            qs, qn = rec(org), rec(nbs[a])
+            cs, cn = org(cm[a]), nbs[a](cm[a + 3])
+            planes[a] = flat(F.pad(cs * qn + (1.0 - cs) * qs, (0, 1, 0, 1)))
+            planes[a + 3] = flat(F.pad(cn * qs + (1.0 - cn) * qn, pads[a]))
         q_a = torch.stack(planes)
         rec_pl = []
         for v in range(1, self.spec.gall_pl):
@@ -243,10 +281,9 @@ class TracerStep:
         rd = self.rdgz[:, None]
         nxt = lambda a: torch.cat([a[..., 1:, :], torch.zeros_like(a[..., :1, :])], dim=-2)  # noqa: E731  k+1
         prv = lambda a: torch.cat([torch.zeros_like(a[..., :1, :]), a[..., :-1, :]], dim=-2)  # noqa: E731  k-1
-        with torch.no_grad():
-            ck = torch.stack([-flx_v / rhog * rd * self.kface, nxt(flx_v) / rhog * rd * nxt(self.kface) * self.kin])
-            ck_pl = torch.stack([-flx_v_pl / rhog_pl * rd * self.kface,
-                                 nxt(flx_v_pl) / rhog_pl * rd * nxt(self.kface) * self.kin])
+        ck = torch.stack([-flx_v / rhog * rd * self.kface, nxt(flx_v) / rhog * rd * nxt(self.kface) * self.kin])
+        ck_pl = torch.stack([-flx_v_pl / rhog_pl * rd * self.kface,
+                             nxt(flx_v_pl) / rhog_pl * rd * nxt(self.kface) * self.kin])
         q_h = (self.afact[:, None] * q + self.bfact[:, None] * prv(q)) * self.kface
         q_h_pl = (self.afact[:, None] * q_pl + self.bfact[:, None] * prv(q_pl)) * self.kface
         outs, outs_pl = [], []
@@ -266,7 +303,7 @@ class TracerStep:
         return rq, rq_pl, rhog, rhog_pl
 
     # -- the step ----------------------------------------------------------------
-    def face_values(self, q, q_pl, xc, xc_pl, cm, cm_pl, chd, chd_pl):
+    def face_values(self, q, q_pl, xc, xc_pl, cm, cm_pl, ch, ch_pl):
         """One tracer's limited face values: gradient, exchange, remap, the
         limiter's allowable range, its exchange, the clamp."""
         ctx = self.ctx
@@ -276,10 +313,10 @@ class TracerStep:
         zero, zero_pl = q.new_zeros((2,) + q.shape), q.new_zeros((2,) + q_pl.shape)
         q, q_pl, q_a, q_a_pl = q.contiguous(), q_pl.contiguous(), q_a.contiguous(), q_a_pl.contiguous()
         _, _, prev, prev_pl = HorizontalLimiter.apply(ctx, q_a, q_a_pl, q, q_pl, zero, zero_pl, self.d, self.d_pl,
-                                                      chd, chd_pl, cm, cm_pl)
+                                                      ch, ch_pl, cm, cm_pl)
         post, post_pl = self.halo(prev, prev_pl)
         q_a, q_a_pl, _, _ = HorizontalLimiter.apply(ctx, q_a, q_a_pl, q, q_pl, post.contiguous(),
-                                                    post_pl.contiguous(), self.d, self.d_pl, chd, chd_pl, cm, cm_pl)
+                                                    post_pl.contiguous(), self.d, self.d_pl, ch, ch_pl, cm, cm_pl)
         return q_a, q_a_pl
 
     def __call__(self, q, q_pl, rhog, rhog_pl, recompute_tracers: bool = False):
@@ -292,10 +329,10 @@ class TracerStep:
         rhog, rhog_pl = rhog.contiguous(), rhog_pl.contiguous()
         flx, flx_pl, xc, xc_pl = HorizontalFlux.apply(ctx, self.dt, rhog, rhog_pl, *self.rv)
         ch, ch_pl, cm, cm_pl = self.courant(flx, flx_pl, rhog, rhog_pl)
-        chd, chd_pl = ch.detach().contiguous(), ch_pl.detach().contiguous()
+        ch, ch_pl = ch.contiguous(), ch_pl.contiguous()
         qas, qas_pl = [], []
         for t in range(ntr):
-            args = (q[t], q_pl[t], xc, xc_pl, cm, cm_pl, chd, chd_pl)
+            args = (q[t], q_pl[t], xc, xc_pl, cm, cm_pl, ch, ch_pl)
             if recompute_tracers:
                 q_a, q_a_pl = checkpoint(self.face_values, *args, use_reentrant=False)
             else:

@@ -90,6 +90,9 @@ int nicam_ctx_set_pointer_mode(nicam_ctx ctx, int mode);
  *                           column in registers)
  *   NICAM_TUNE_DIFF_KPER    levels per work-group of nicam_oprt_diffusion
  *   NICAM_TUNE_DDMP_KPER    levels per work-group of nicam_oprt3d_divdamp
+ *   NICAM_TUNE_HLIM_VARIANT horizontal limiter, one tracer: 1 = the streamed
+ *                           inputs loaded after the tracer tile's barrier,
+ *                           2 = before it; 0 = automatic
  * Results do not depend on these settings (every variant computes the same
  * floating-point operations in the same order). */
 enum nicam_tuning {
@@ -97,7 +100,8 @@ enum nicam_tuning {
     NICAM_TUNE_VLIM_KCHUNK = 3,
     NICAM_TUNE_VI_VARIANT = 4,
     NICAM_TUNE_DIFF_KPER = 5,
-    NICAM_TUNE_DDMP_KPER = 6
+    NICAM_TUNE_DDMP_KPER = 6,
+    NICAM_TUNE_HLIM_VARIANT = 7
 };
 int nicam_ctx_set_tuning(nicam_ctx ctx, int param, int value);
 
@@ -588,8 +592,8 @@ int nicam_horizontal_limiter_thuburn_mask(nicam_ctx ctx,
                                           const double *Qout_post, const double *Qout_post_pl,
                                           uint8_t *mask, uint8_t *mask_pl);
 
-/* Adjoint of horizontal_limiter_thuburn with Qout_post supplied, at fixed d,
- * ch and cmask (dyn_horiz_adv_limiter/src/mod_src_tracer.f90:42-430):
+/* Adjoint of horizontal_limiter_thuburn with Qout_post supplied, at fixed d
+ * and cmask (dyn_horiz_adv_limiter/src/mod_src_tracer.f90:42-430):
  *   inout q_a_b        in: cotangent of the limited q_a; out: cotangent of
  *                      q_a's input (faces 4-6 the kernel overwrites get 0;
  *                      faces it does not touch pass through)
@@ -599,10 +603,14 @@ int nicam_horizontal_limiter_thuburn_mask(nicam_ctx ctx,
  *   out   Qout_post_b  cotangent of Qout_post
  *   in    q_a          the forward's output; q, d, ch, cmask, Qout_post its
  *                      inputs; mask its mask
+ *   out   ch_b         cotangent of ch (NULL: not formed; with poles ch_b_pl
+ *                      too): Qout of a cell depends on its six Courant
+ *                      numbers through Cin, Cout and the inflow sums
+ *                      (HLIM:216-284); at ch = 0 the right-hand derivative
  * A changed face value is attributed, in this order, to Qout_post min,
- * Qout_post max, the arc's Qin min, Qin max, by equality with q_a.  The
- * regions' q_b and Qout_post_b are sums of a few terms added atomically, so
- * they may differ from a sequential sum by rounding. */
+ * Qout_post max, the arc's Qin min, Qin max, by equality with q_a.  Every
+ * cotangent is a sum in a fixed order (the order of a sequential sweep over
+ * the cells), so the results do not vary between runs. */
 int nicam_horizontal_limiter_thuburn_adjoint(nicam_ctx ctx,
                                              double *q_a_b, double *q_a_b_pl,
                                              double *q_b, double *q_b_pl,
@@ -614,7 +622,8 @@ int nicam_horizontal_limiter_thuburn_adjoint(nicam_ctx ctx,
                                              const double *ch, const double *ch_pl,
                                              const double *cmask, const double *cmask_pl,
                                              const double *Qout_post, const double *Qout_post_pl,
-                                             const uint8_t *mask, const uint8_t *mask_pl);
+                                             const uint8_t *mask, const uint8_t *mask_pl,
+                                             double *ch_b, double *ch_b_pl);
 
 /* vertical_limiter_thuburn that also records its branch mask: the same
  * arguments and results as nicam_vertical_limiter_thuburn, plus one bit per
@@ -629,12 +638,15 @@ int nicam_vertical_limiter_thuburn_mask(nicam_ctx ctx,
                                         const double *ck, const double *ck_pl,
                                         uint32_t *mask, uint32_t *mask_pl);
 
-/* Adjoint of vertical_limiter_thuburn with respect to q_h's input and q, at
- * fixed ck and d (dyn_vert_adv_limiter/src/mod_src_tracer.f90:41-243):
+/* Adjoint of vertical_limiter_thuburn with respect to q_h's input, q and ck,
+ * at fixed d and inflow flags (dyn_vert_adv_limiter/src/mod_src_tracer.f90:41-243):
  *   inout q_h_b  in: cotangent of the limited q_h; out: cotangent of q_h's
  *                input (faces outside kmin+1..kmax pass through)
  *   out   q_b    cotangent of q (every level written)
  *   in    q_h    the forward's output; q, d, ck its inputs; mask its mask
+ *   out   ck_b   cotangent of ck(g,k,l,2) (NULL: not formed; with poles
+ *                ck_b_pl too; every level written): the bounds of level k
+ *                depend on ck(k,1) and ck(k,2) through Cin, Cout and CQin
  * A clipped face takes the lower bound when its value equals Qout_min of the
  * upwind level, else the upper bound. */
 int nicam_vertical_limiter_thuburn_adjoint(nicam_ctx ctx,
@@ -644,7 +656,8 @@ int nicam_vertical_limiter_thuburn_adjoint(nicam_ctx ctx,
                                            const double *q, const double *q_pl,
                                            const double *d, const double *d_pl,
                                            const double *ck, const double *ck_pl,
-                                           const uint32_t *mask, const uint32_t *mask_pl);
+                                           const uint32_t *mask, const uint32_t *mask_pl,
+                                           double *ck_b, double *ck_b_pl);
 
 /* ---- FP64 global tracer mass and the update that conserves it ----
  * Neither has a counterpart in the reference, whose kernels stop at the face

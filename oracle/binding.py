@@ -85,12 +85,12 @@ class Oracle:
         L.orc_vertical_limiter_thuburn.argtypes = [POINTER(OrcDims)] + [P] * 8
         L.orc_vi_rhow_solver_adjoint.argtypes = ([POINTER(OrcDims)] + [P] * 14 + [c_double] + [P] * 16 + [P] * 3 +
                                                  [c_double, c_double, c_double, c_int])
-        L.orc_vertical_limiter_thuburn_tlm.argtypes = [POINTER(OrcDims)] + [P] * 16
-        L.orc_horizontal_limiter_thuburn_tlm.argtypes = [POINTER(OrcDims), P] + [P] * 22
+        L.orc_vertical_limiter_thuburn_tlm.argtypes = [POINTER(OrcDims)] + [P] * 18
+        L.orc_horizontal_limiter_thuburn_tlm.argtypes = [POINTER(OrcDims), P] + [P] * 24
         L.orc_horizontal_flux_tlm.argtypes = [POINTER(OrcDims), P] + [P] * 20 + [c_double] + [P] * 6
         L.orc_horizontal_flux_adjoint.argtypes = [POINTER(OrcDims), P] + [P] * 20 + [c_double] + [P] * 6
-        L.orc_horizontal_limiter_thuburn_adjoint.argtypes = [POINTER(OrcDims), P] + [P] * 22
-        L.orc_vertical_limiter_thuburn_adjoint.argtypes = [POINTER(OrcDims)] + [P] * 14
+        L.orc_horizontal_limiter_thuburn_adjoint.argtypes = [POINTER(OrcDims), P] + [P] * 24
+        L.orc_vertical_limiter_thuburn_adjoint.argtypes = [POINTER(OrcDims)] + [P] * 16
         L.orc_vi_rhow_solver.argtypes = ([POINTER(OrcDims)] + [P] * 14 + [c_double] + [P] * 16 + [P] * 3 +
                                          [c_double, c_double, c_double, c_int])
         L.orc_oprt_diffusion.argtypes = [POINTER(OrcDims), P] + [P] * 10
@@ -269,9 +269,9 @@ class Oracle:
         return out
 
     def vertical_limiter_thuburn_lin(self, spec, f, q_h_out, mask, mask_pl, adjoint, a, eps=None, big=None):
-        """TLM (adjoint=False; a = dict(dq_h_in, dq_h_in_pl, dq, dq_pl) -> dq_h, dq_h_pl) or adjoint
-        (adjoint=True; a = dict(q_h_b, q_h_b_pl) -> q_h_b, q_h_b_pl, q_b, q_b_pl) of the vertical limiter
-        at the forward state f (its inputs), q_h_out (its output) and mask."""
+        """TLM (adjoint=False; a = dict(dq_h_in, dq_h_in_pl, dq, dq_pl[, dck, dck_pl]) -> dq_h, dq_h_pl) or
+        adjoint (adjoint=True; a = dict(q_h_b, q_h_b_pl) -> q_h_b, q_h_b_pl, q_b, q_b_pl, ck_b, ck_b_pl) of
+        the vertical limiter at the forward state f (its inputs), q_h_out (its output) and mask."""
         D = self.dims(spec, eps=eps, big=big)
         ins = {k: _f64(f[k]) for k in ("q", "q_pl", "d", "d_pl", "ck", "ck_pl")}
         qh = {k: _f64(v) for k, v in q_h_out.items()}
@@ -282,14 +282,17 @@ class Oracle:
         if not adjoint:
             din = {k: _f64(v) for k, v in a.items()}
             out = dict(dq_h=din["dq_h_in"].copy(), dq_h_pl=din["dq_h_in_pl"].copy())
+            dck = [_p(din["dck"]), _p(din["dck_pl"])] if "dck" in din else [None, None]
             self.lib.orc_vertical_limiter_thuburn_tlm(ctypes.byref(D), _p(out["dq_h"]), _p(out["dq_h_pl"]),
                                                       _p(din["dq_h_in"]), _p(din["dq_h_in_pl"]), _p(din["dq"]),
-                                                      _p(din["dq_pl"]), *fw)
+                                                      _p(din["dq_pl"]), *fw, *dck)
             return out
         out = dict(q_h_b=_f64(a["q_h_b"]), q_h_b_pl=_f64(a["q_h_b_pl"]), q_b=np.zeros_like(ins["q"]),
-                   q_b_pl=np.zeros_like(ins["q_pl"]))
+                   q_b_pl=np.zeros_like(ins["q_pl"]), ck_b=np.zeros_like(ins["ck"]),
+                   ck_b_pl=np.zeros_like(ins["ck_pl"]))
         self.lib.orc_vertical_limiter_thuburn_adjoint(ctypes.byref(D), _p(out["q_h_b"]), _p(out["q_h_b_pl"]),
-                                                      _p(out["q_b"]), _p(out["q_b_pl"]), *fw)
+                                                      _p(out["q_b"]), _p(out["q_b_pl"]), *fw, _p(out["ck_b"]),
+                                                      _p(out["ck_b_pl"]))
         return out
 
     def horizontal_flux_lin(self, spec, geo, f, dt, adjoint, a):
@@ -328,9 +331,9 @@ class Oracle:
         return out
 
     def horizontal_limiter_thuburn_lin(self, spec, have_sgp, f, q_a_out, mask, mask_pl, adjoint, a):
-        """TLM (adjoint=False; a = dq_a_in, dq, dQpost (+_pl) -> dq_a, dQprev) or adjoint (adjoint=True;
-        a = q_a_b, Qprev_b (+_pl) -> q_a_b, q_b, Qpost_b) of the horizontal limiter with Qout_post
-        supplied, at the forward state f (inputs), q_a_out (its output) and its mask."""
+        """TLM (adjoint=False; a = dq_a_in, dq, dQpost[, dch] (+_pl) -> dq_a, dQprev) or adjoint
+        (adjoint=True; a = q_a_b, Qprev_b (+_pl) -> q_a_b, q_b, Qpost_b, ch_b) of the horizontal limiter
+        with Qout_post supplied, at the forward state f (inputs), q_a_out (its output) and its mask."""
         D = self.dims(spec)
         sgp_a = np.ascontiguousarray(have_sgp, dtype=np.int32)
         sgp = sgp_a.ctypes.data
@@ -350,15 +353,18 @@ class Oracle:
             self.lib.orc_horizontal_limiter_thuburn_tlm(ctypes.byref(D), sgp, _p(out["dq_a"]), _p(out["dq_a_pl"]),
                                                         _p(x["dq"]), _p(x["dq_pl"]), _p(out["dQprev"]),
                                                         _p(out["dQprev_pl"]), _p(x["dQpost"]), _p(x["dQpost_pl"]),
-                                                        *fw)
+                                                        *fw, *([_p(x["dch"]), _p(x["dch_pl"])] if "dch" in x
+                                                               else [None, None]))
             return out
         out = dict(q_a_b=x["q_a_b"].copy(), q_a_b_pl=x["q_a_b_pl"].copy(), q_b=np.zeros_like(ins["q"]),
                    q_b_pl=np.zeros_like(ins["q_pl"]), Qpost_b=np.zeros_like(ins["Qout_post"]),
-                   Qpost_b_pl=np.zeros_like(ins["Qout_post_pl"]))
+                   Qpost_b_pl=np.zeros_like(ins["Qout_post_pl"]), ch_b=np.zeros_like(ins["ch"]),
+                   ch_b_pl=np.zeros_like(ins["ch_pl"]))
         self.lib.orc_horizontal_limiter_thuburn_adjoint(ctypes.byref(D), sgp, _p(out["q_a_b"]), _p(out["q_a_b_pl"]),
                                                         _p(out["q_b"]), _p(out["q_b_pl"]), _p(x["Qprev_b"]),
                                                         _p(x["Qprev_b_pl"]), _p(out["Qpost_b"]),
-                                                        _p(out["Qpost_b_pl"]), *fw)
+                                                        _p(out["Qpost_b_pl"]), *fw, _p(out["ch_b"]),
+                                                        _p(out["ch_b_pl"]))
         return out
 
     def vi_rhow_solver_adjoint(self, spec, f, b, dt, rdgzh, afact, bfact, grav=9.80616, rdry=287.0, cvdry=717.5,

@@ -1371,6 +1371,7 @@ void orc_vi_rhow_solver_adjoint(const orc_dims *D, double *wb, double *wb_pl, do
 typedef struct {
     double qmin, qmax;  /* Qout_min, Qout_max */
     double rmin[3], rmax[3];
+    double cmin[2], cmax[2];  /* derivatives in ck(j,1), ck(j,2) */
 } vlim_lvl;
 
 static void vlim_level(const double *q, const double *d, const double *ck, size_t o, size_t s, size_t ck2,
@@ -1416,41 +1417,60 @@ static void vlim_level(const double *q, const double *d, const double *ck, size_
     }
     r->rmin[1] = (r->rmin[1] + fac) + zerosw;
     r->rmax[1] = (r->rmax[1] + fac) + zerosw;
+    /* ck(j,1) enters Cin where inflagL = 1 (with CQin) and Cout elsewhere; ck(j,2) likewise with inflagU:
+     * (qnext - Qin)/Cout or (qnext - Qout)/Cout; zero where zerosw = 1 */
+    const int z = zerosw != 0.0;
+    r->cmin[0] = z ? 0.0 : (inflagL > 0.0 ? qnext_max - Qin_maxL : qnext_max - r->qmin) / Cout;
+    r->cmin[1] = z ? 0.0 : (inflagU > 0.0 ? qnext_max - Qin_maxU : qnext_max - r->qmin) / Cout;
+    r->cmax[0] = z ? 0.0 : (inflagL > 0.0 ? qnext_min - Qin_minL : qnext_min - r->qmax) / Cout;
+    r->cmax[1] = z ? 0.0 : (inflagU > 0.0 ? qnext_min - Qin_minU : qnext_min - r->qmax) / Cout;
 }
 
-/* Row of the limited face k: coefficient of q_h_in (*cqh) and of q(k-2..k+1)
- * (row[0..3]). */
+/* Row of the limited face k: coefficient of q_h_in (*cqh), of q(k-2..k+1)
+ * (row[0..3]) and of ck(j,1), ck(j,2) (crow) of the bound's level j (*lev:
+ * 0 = k-1, 1 = k). */
 static void vlim_face_row(const vlim_lvl *lo, const vlim_lvl *hi, double inflagL, int clipped, double out,
-                          double *cqh, double row[4])
+                          double *cqh, double row[4], double crow[2], int *lev)
 {
     row[0] = row[1] = row[2] = row[3] = 0.0;
+    crow[0] = crow[1] = 0.0;
+    *lev = 0;
     *cqh = clipped ? 0.0 : 1.0;
     if (!clipped) return;
     const vlim_lvl *L = (inflagL > 0.0) ? lo : hi; /* level k-1 or k */
     const int off = (inflagL > 0.0) ? 0 : 1;        /* row index of q(j-1) */
-    const double *r = (out == L->qmin) ? L->rmin : L->rmax;
+    const int use_min = out == L->qmin;
+    const double *r = use_min ? L->rmin : L->rmax;
     for (int m = 0; m < 3; ++m) row[off + m] = r[m];
+    crow[0] = use_min ? L->cmin[0] : L->cmax[0];
+    crow[1] = use_min ? L->cmin[1] : L->cmax[1];
+    *lev = off;
 }
 
 /* One column: TLM (dqh_out from dqh_in, dq) or adjoint (qhb, qb) */
 static void vlim_lin_column(int kall, size_t base, size_t s, size_t ck2, const uint32_t *mask, size_t mbase,
                             size_t mstride, const double *q_h, const double *q, const double *d, const double *ck,
                             double EPS, double BIG, int pole, int adjoint, double *a_qh, const double *a_dqh_in,
-                            double *a_q)
+                            double *a_q, double *a_ck)
 {
     const int kmin = 2, kmax = kall - 1;
 #define AT(k) (base + (size_t)((k) - 1) * s)
     vlim_lvl lo, hi;
     if (adjoint)
-        for (int k = 1; k <= kall; ++k) a_q[AT(k)] = 0.0;
+        for (int k = 1; k <= kall; ++k) {
+            a_q[AT(k)] = 0.0;
+            if (a_ck) a_ck[AT(k)] = a_ck[ck2 + AT(k)] = 0.0;
+        }
     vlim_level(q, d, ck, AT(kmin), s, ck2, EPS, BIG, pole, &lo);
     for (int k = kmin + 1; k <= kmax; ++k) {
         vlim_level(q, d, ck, AT(k), s, ck2, EPS, BIG, pole, &hi);
         const int p = k - (kmin + 1);
         const int clipped = (mask[mbase + (size_t)(p / 32) * mstride] >> (p % 32)) & 1u;
         const double inflagL = 0.5 - fsign(0.5, ck[AT(k)]);
-        double cqh, row[4];
-        vlim_face_row(&lo, &hi, inflagL, clipped, q_h[AT(k)], &cqh, row);
+        double cqh, row[4], crow[2];
+        int lev;
+        vlim_face_row(&lo, &hi, inflagL, clipped, q_h[AT(k)], &cqh, row, crow, &lev);
+        const int j = k - 1 + lev;
         if (adjoint) {
             const double ob = a_qh[AT(k)];
             a_qh[AT(k)] = cqh * ob;
@@ -1458,12 +1478,17 @@ static void vlim_lin_column(int kall, size_t base, size_t s, size_t ck2, const u
                 const int kk = k - 2 + m;
                 if (kk >= 1 && kk <= kall) a_q[AT(kk)] = a_q[AT(kk)] + row[m] * ob;
             }
+            if (a_ck && clipped) {
+                a_ck[AT(j)] = a_ck[AT(j)] + crow[0] * ob;
+                a_ck[ck2 + AT(j)] = a_ck[ck2 + AT(j)] + crow[1] * ob;
+            }
         } else {
             double v = cqh * a_dqh_in[AT(k)];
             for (int m = 0; m < 4; ++m) {
                 const int kk = k - 2 + m;
                 if (kk >= 1 && kk <= kall) v = v + row[m] * a_q[AT(kk)];
             }
+            if (a_ck && clipped) v = (v + crow[0] * a_ck[AT(j)]) + crow[1] * a_ck[ck2 + AT(j)];
             a_qh[AT(k)] = v;
         }
         lo = hi;
@@ -1474,7 +1499,8 @@ static void vlim_lin_column(int kall, size_t base, size_t s, size_t ck2, const u
 static void vlim_lin(const orc_dims *D, int adjoint, double *a_qh, double *a_qh_pl, const double *a_dqh_in,
                      const double *a_dqh_in_pl, double *a_q, double *a_q_pl, const double *q_h, const double *q_h_pl,
                      const double *q, const double *q_pl, const double *d, const double *d_pl, const double *ck,
-                     const double *ck_pl, const uint32_t *mask, const uint32_t *mask_pl)
+                     const double *ck_pl, const uint32_t *mask, const uint32_t *mask_pl, double *a_ck,
+                     double *a_ck_pl)
 {
     const int gall = D->gall_1d * D->gall_1d, kall = D->kall, nword = (kall - 3 + 31) / 32;
     if (kall - 3 <= 0) return;
@@ -1483,32 +1509,35 @@ static void vlim_lin(const orc_dims *D, int adjoint, double *a_qh, double *a_qh_
         for (int g = 0; g < gall; ++g)
             vlim_lin_column(kall, (size_t)l * kall * gall + g, gall, (size_t)gall * kall * D->lall, mask,
                             (size_t)l * nword * gall + g, gall, q_h, q, d, ck, D->eps, D->big, 0, adjoint, a_qh,
-                            a_dqh_in, a_q);
+                            a_dqh_in, a_q, a_ck);
     if (D->have_pl)
         for (int l = 0; l < LALL_PL; ++l)
             for (int g = 0; g < D->gall_pl; ++g)
                 vlim_lin_column(kall, (size_t)l * kall * D->gall_pl + g, D->gall_pl,
                                 (size_t)D->gall_pl * kall * LALL_PL, mask_pl, (size_t)l * nword * D->gall_pl + g,
                                 D->gall_pl, q_h_pl, q_pl, d_pl, ck_pl, D->eps, D->big, 1, adjoint, a_qh_pl,
-                                a_dqh_in_pl, a_q_pl);
+                                a_dqh_in_pl, a_q_pl, a_ck_pl);
 }
 
 void orc_vertical_limiter_thuburn_tlm(const orc_dims *D, double *dqh, double *dqh_pl, const double *dqh_in,
                                       const double *dqh_in_pl, const double *dq, const double *dq_pl,
                                       const double *q_h, const double *q_h_pl, const double *q, const double *q_pl,
                                       const double *d, const double *d_pl, const double *ck, const double *ck_pl,
-                                      const uint32_t *mask, const uint32_t *mask_pl)
+                                      const uint32_t *mask, const uint32_t *mask_pl, const double *dck,
+                                      const double *dck_pl)
 {
     vlim_lin(D, 0, dqh, dqh_pl, dqh_in, dqh_in_pl, (double *)dq, (double *)dq_pl, q_h, q_h_pl, q, q_pl, d, d_pl,
-             ck, ck_pl, mask, mask_pl);
+             ck, ck_pl, mask, mask_pl, (double *)dck, (double *)dck_pl);
 }
 
 void orc_vertical_limiter_thuburn_adjoint(const orc_dims *D, double *qhb, double *qhb_pl, double *qb, double *qb_pl,
                                           const double *q_h, const double *q_h_pl, const double *q,
                                           const double *q_pl, const double *d, const double *d_pl, const double *ck,
-                                          const double *ck_pl, const uint32_t *mask, const uint32_t *mask_pl)
+                                          const double *ck_pl, const uint32_t *mask, const uint32_t *mask_pl,
+                                          double *ck_b, double *ck_b_pl)
 {
-    vlim_lin(D, 1, qhb, qhb_pl, NULL, NULL, qb, qb_pl, q_h, q_h_pl, q, q_pl, d, d_pl, ck, ck_pl, mask, mask_pl);
+    vlim_lin(D, 1, qhb, qhb_pl, NULL, NULL, qb, qb_pl, q_h, q_h_pl, q, q_pl, d, d_pl, ck, ck_pl, mask, mask_pl,
+             ck_b, ck_b_pl);
 }
 
 /* ------------------------------------------------------------------------- */
@@ -1567,7 +1596,7 @@ static int hl_off(int m, int iall) { return m == 1 ? 1 : (m == 2 ? iall + 1 : ia
 static void hl_slab(int iall, int sgp, const double *qs, const double *ds, const double *chs, const double *cms,
                     size_t cplane, const double *qas_out, const double *Qpost_s, size_t qplane,
                     const uint8_t *mk, double EPS, double BIG, int adjoint, double *xa, size_t aplane,
-                    double *xq, double *xQprev, double *xQpost)
+                    double *xq, double *xQprev, double *xQpost, double *xch)
 {
     const int gall = iall * iall, gmin = 2, gmax = iall - 1;
     const int sgp_cell = sgp ? (gmin - 1 - 1) * iall + (gmin - 1) : -1;
@@ -1581,6 +1610,8 @@ static void hl_slab(int iall, int sgp, const double *qs, const double *ds, const
                 if (adjoint) xq[g - 1] = xq[g - 1] + xQprev[(size_t)mm * qplane + g - 1];
                 else xQprev[(size_t)mm * qplane + g - 1] = xq[g - 1];
             }
+            if (adjoint && xch)
+                for (int f = 1; f <= 6; ++f) xch[(size_t)(f - 1) * cplane + g - 1] = 0.0;
             continue;
         }
         /* the six arcs of g: f=1..3 its own, f=4..6 the neighbours' (m = f-3) */
@@ -1612,6 +1643,28 @@ static void hl_slab(int iall, int sgp, const double *qs, const double *ds, const
         const double zerosw = 0.5 - fsign(0.5, fabs(Cout) - EPS);
         const double E = 1.0 - Cin - Cout + ds[g - 1];
         const double fac = (1.0 - zerosw) / (Cout + zerosw);
+        /* derivatives of Qout min (mm=0) and max (mm=1) in ch(f): min(ch, 0) enters Cin and CQin, ch - min(ch, 0)
+         * Cout, so an inflow face (ch < 0) gives (qnext - Qin)/Cout and any other (qnext - Qout)/Cout (the
+         * right-hand derivative at ch = 0); zero where zerosw = 1, where Qout = q */
+        double dch[2][7];
+        {
+            const double CQmin = chm[1] * vmn[1] + chm[2] * vmn[2] + chm[3] * vmn[3] + chm[4] * vmn[4] + chm[5] * vmn[5] + chm[6] * vmn[6];
+            const double CQmax = chm[1] * vmx[1] + chm[2] * vmx[2] + chm[3] * vmx[3] + chm[4] * vmx[4] + chm[5] * vmx[5] + chm[6] * vmx[6];
+            const double Qmin = (qij - CQmax - nmax * E) / (Cout + zerosw) * (1.0 - zerosw) + qij * zerosw;
+            const double Qmax = (qij - CQmin - nmin * E) / (Cout + zerosw) * (1.0 - zerosw) + qij * zerosw;
+            for (int f = 1; f <= 6; ++f) {
+                dch[0][f] = zerosw != 0.0 ? 0.0 : (chv[f] < 0.0 ? nmax - vmx[f] : nmax - Qmin) / Cout;
+                dch[1][f] = zerosw != 0.0 ? 0.0 : (chv[f] < 0.0 ? nmin - vmn[f] : nmin - Qmax) / Cout;
+            }
+        }
+        /* a bound whose cotangent (direction) is zero contributes nothing: where an inflow face meets the
+         * sentinel of an arc whose upwind side is the cell itself, the bound is huge and its derivative may
+         * overflow, and 0 * inf must not turn into NaN */
+        if (adjoint && xch) {
+            const double b0 = xQprev[g - 1], b1 = xQprev[qplane + g - 1];
+            for (int f = 1; f <= 6; ++f)
+                xch[(size_t)(f - 1) * cplane + g - 1] = (b0 != 0.0 ? b0 * dch[0][f] : 0.0) + (b1 != 0.0 ? b1 * dch[1][f] : 0.0);
+        }
         /* Qout min (mm=0) uses the max side, Qout max (mm=1) the min side */
         for (int mm = 0; mm < 2; ++mm) {
             const int sel = mm == 0 ? smax : smin;
@@ -1630,6 +1683,11 @@ static void hl_slab(int iall, int sgp, const double *qs, const double *ds, const
             } else {
                 double v = 0.0;
                 for (int t = 0; t < n; ++t) v = v + coef[t] * xq[cells[t] - 1];
+                if (xch)
+                    for (int f = 1; f <= 6; ++f) {
+                        const double dc = xch[(size_t)(f - 1) * cplane + g - 1];
+                        if (dc != 0.0) v = v + dch[mm][f] * dc;
+                    }
                 xQprev[o] = v;
             }
         }
@@ -1682,7 +1740,7 @@ static void hl_slab(int iall, int sgp, const double *qs, const double *ds, const
 /* one (k, l) slab of a pole region (HLIM:307-363, 412-425); arrays at the slab */
 static void hl_pole_slab(int npl, const double *qs, double dn, const double *chs, const double *cms,
                          const double *qas_out, const double *Qpost_s, size_t pplane, const uint8_t *mk, double EPS,
-                         double BIG, int adjoint, double *xa, double *xq, double *xQprev, double *xQpost)
+                         double BIG, int adjoint, double *xa, double *xq, double *xQprev, double *xQpost, double *xch)
 {
     /* ring point v (1-based 2..npl): arc {centre, v, v-1, v+1} */
     int amn[16], amx[16];
@@ -1703,15 +1761,37 @@ static void hl_pole_slab(int npl, const double *qs, double dn, const double *chs
         if (mn1 < nmin) { nmin = mn1; smin = v; }
         if (mx1 > nmax) { nmax = mx1; smax = v; }
     }
-    double Cin = 0.0, Cout = 0.0;
+    double Cin = 0.0, Cout = 0.0, CQmin = 0.0, CQmax = 0.0;
     for (int v = 2; v <= npl; ++v) {
         const double chm = cms[v - 1] * chs[v - 1];
+        const double c = cms[v - 1];
         Cin = Cin + chm;
         Cout = Cout - chm + chs[v - 1];
+        CQmin = CQmin + chm * (c * vmn[v] + (1.0 - c) * BIG);
+        CQmax = CQmax + chm * (c * vmx[v] + (1.0 - c) * (-BIG));
     }
     const double zerosw = 0.5 - fsign(0.5, fabs(Cout) - EPS);
     const double E = 1.0 - Cin - Cout + dn;
     const double fac = (1.0 - zerosw) / (Cout + zerosw);
+    /* derivatives in ch(v): cmask*ch enters Cin and CQin, ch - cmask*ch Cout:
+     * ((qnext - c Qin) - (1 - c) Qout) / Cout, zero where zerosw = 1 */
+    double dch[2][16];
+    {
+        const double Qmin = (qn - CQmax - nmax * E) / (Cout + zerosw) * (1.0 - zerosw) + qn * zerosw;
+        const double Qmax = (qn - CQmin - nmin * E) / (Cout + zerosw) * (1.0 - zerosw) + qn * zerosw;
+        for (int v = 2; v <= npl; ++v) {
+            const double c = cms[v - 1];
+            const double imx = c * vmx[v] + (1.0 - c) * (-BIG), imn = c * vmn[v] + (1.0 - c) * BIG;
+            dch[0][v - 1] = zerosw != 0.0 ? 0.0 : ((nmax - c * imx) - (1.0 - c) * Qmin) / Cout;
+            dch[1][v - 1] = zerosw != 0.0 ? 0.0 : ((nmin - c * imn) - (1.0 - c) * Qmax) / Cout;
+        }
+    }
+    if (adjoint && xch) {
+        xch[0] = 0.0;
+        const double b0 = xQprev[0], b1 = xQprev[pplane];
+        for (int v = 2; v <= npl; ++v)
+            xch[v - 1] = (b0 != 0.0 ? b0 * dch[0][v - 1] : 0.0) + (b1 != 0.0 ? b1 * dch[1][v - 1] : 0.0);
+    }
     for (int mm = 0; mm < 2; ++mm) {
         const int sel = mm == 0 ? smax : smin;
         const int *cx = mm == 0 ? amx : amn;
@@ -1728,6 +1808,9 @@ static void hl_pole_slab(int npl, const double *qs, double dn, const double *chs
         } else {
             double v = 0.0;
             for (int t = 0; t < n; ++t) v = v + coef[t] * xq[cells[t] - 1];
+            if (xch)
+                for (int r = 2; r <= npl; ++r)
+                    if (xch[r - 1] != 0.0) v = v + dch[mm][r - 1] * xch[r - 1];
             xQprev[o] = v;
             for (int r = 2; r <= npl; ++r) xQprev[o + r - 1] = 0.0; /* the ring is zeroed (HLIM:367-370) */
         }
@@ -1765,7 +1848,7 @@ static void hlim_lin(const orc_dims *D, const int *have_sgp, int adjoint, double
                      const double *q_a, const double *q_a_pl, const double *q, const double *q_pl, const double *d,
                      const double *d_pl, const double *ch, const double *ch_pl, const double *cmask,
                      const double *cmask_pl, const double *Qpost, const double *Qpost_pl, const uint8_t *mask,
-                     const uint8_t *mask_pl)
+                     const uint8_t *mask_pl, double *xch, double *xch_pl)
 {
     const int iall = D->gall_1d, gall = iall * iall, kall = D->kall, lall = D->lall, npl = D->gall_pl;
     const size_t nreg = (size_t)gall * kall * lall, npole = (size_t)npl * kall * LALL_PL;
@@ -1782,7 +1865,7 @@ static void hlim_lin(const orc_dims *D, const int *have_sgp, int adjoint, double
         for (int k = 0; k < kall; ++k) {
             const size_t s = ((size_t)l * kall + k) * gall;
             hl_slab(iall, have_sgp[l], q + s, d + s, ch + s, cmask + s, nreg, q_a + s, Qpost + s, nreg, mask + s,
-                    D->eps, D->big, adjoint, xa + s, nreg, xq + s, xQprev + s, xQpost + s);
+                    D->eps, D->big, adjoint, xa + s, nreg, xq + s, xQprev + s, xQpost + s, xch ? xch + s : NULL);
         }
     if (D->have_pl)
         for (int l = 0; l < LALL_PL; ++l)
@@ -1790,7 +1873,7 @@ static void hlim_lin(const orc_dims *D, const int *have_sgp, int adjoint, double
                 const size_t s = ((size_t)l * kall + k) * npl;
                 hl_pole_slab(npl, q_pl + s, d_pl[s], ch_pl + s, cmask_pl + s, q_a_pl + s, Qpost_pl + s, npole,
                              mask_pl + s, D->eps, D->big, adjoint, xa_pl + s, xq_pl + s, xQprev_pl + s,
-                             xQpost_pl + s);
+                             xQpost_pl + s, xch_pl ? xch_pl + s : NULL);
             }
 }
 
@@ -1800,11 +1883,12 @@ void orc_horizontal_limiter_thuburn_tlm(const orc_dims *D, const int *have_sgp, 
                                         const double *q_a_pl, const double *q, const double *q_pl, const double *d,
                                         const double *d_pl, const double *ch, const double *ch_pl,
                                         const double *cmask, const double *cmask_pl, const double *Qpost,
-                                        const double *Qpost_pl, const uint8_t *mask, const uint8_t *mask_pl)
+                                        const double *Qpost_pl, const uint8_t *mask, const uint8_t *mask_pl,
+                                        const double *dch, const double *dch_pl)
 {
     hlim_lin(D, have_sgp, 0, dq_a, dq_a_pl, (double *)dq, (double *)dq_pl, dQprev, dQprev_pl, (double *)dQpost,
              (double *)dQpost_pl, q_a, q_a_pl, q, q_pl, d, d_pl, ch, ch_pl, cmask, cmask_pl, Qpost, Qpost_pl, mask,
-             mask_pl);
+             mask_pl, (double *)dch, (double *)dch_pl);
 }
 
 void orc_horizontal_limiter_thuburn_adjoint(const orc_dims *D, const int *have_sgp, double *q_a_b,
@@ -1814,10 +1898,12 @@ void orc_horizontal_limiter_thuburn_adjoint(const orc_dims *D, const int *have_s
                                             const double *q_pl, const double *d, const double *d_pl,
                                             const double *ch, const double *ch_pl, const double *cmask,
                                             const double *cmask_pl, const double *Qpost, const double *Qpost_pl,
-                                            const uint8_t *mask, const uint8_t *mask_pl)
+                                            const uint8_t *mask, const uint8_t *mask_pl, double *ch_b,
+                                            double *ch_b_pl)
 {
     hlim_lin(D, have_sgp, 1, q_a_b, q_a_b_pl, q_b, q_b_pl, (double *)Qprev_b, (double *)Qprev_b_pl, Qpost_b,
-             Qpost_b_pl, q_a, q_a_pl, q, q_pl, d, d_pl, ch, ch_pl, cmask, cmask_pl, Qpost, Qpost_pl, mask, mask_pl);
+             Qpost_b_pl, q_a, q_a_pl, q, q_pl, d, d_pl, ch, ch_pl, cmask, cmask_pl, Qpost, Qpost_pl, mask, mask_pl,
+             ch_b, ch_b_pl);
 }
 
 /* ------------------------------------------------------------------------- */

@@ -209,35 +209,42 @@ void orc_vi_rhow_solver_adjoint(const orc_dims *D, double *wb, double *wb_pl, do
                                 const double *afact, const double *bfact, double grav, double rdry, double cvdry,
                                 int non_hydro_alpha);
 
-/* Tangent-linear and adjoint of orc_vertical_limiter_thuburn at fixed ck and
- * d.  mask: one bit per limited face (set where the forward clipped it),
+/* Tangent-linear and adjoint of orc_vertical_limiter_thuburn at fixed d (and
+ * the signs of ck).  mask: one bit per limited face (set where the forward clipped it),
  * word ((l*nword + p/32)*gall + g), bit p%32, p = k-(kmin+1), nword =
- * ceil((kall-3)/32); q_h is the forward's output.  TLM: dqh from dqh_in and
- * dq.  Adjoint: qhb in = output cotangent, out = cotangent of q_h's input;
- * qb written on every level. */
+ * ceil((kall-3)/32); q_h is the forward's output.  TLM: dqh from dqh_in, dq
+ * and dck (NULL = zero).  Adjoint: qhb in = output cotangent, out =
+ * cotangent of q_h's input; qb and ck_b (if not NULL, both planes) written
+ * on every level. */
 void orc_vertical_limiter_thuburn_tlm(const orc_dims *D, double *dqh, double *dqh_pl, const double *dqh_in,
                                       const double *dqh_in_pl, const double *dq, const double *dq_pl,
                                       const double *q_h, const double *q_h_pl, const double *q, const double *q_pl,
                                       const double *d, const double *d_pl, const double *ck, const double *ck_pl,
-                                      const uint32_t *mask, const uint32_t *mask_pl);
+                                      const uint32_t *mask, const uint32_t *mask_pl, const double *dck,
+                                      const double *dck_pl);
 void orc_vertical_limiter_thuburn_adjoint(const orc_dims *D, double *qhb, double *qhb_pl, double *qb, double *qb_pl,
                                           const double *q_h, const double *q_h_pl, const double *q,
                                           const double *q_pl, const double *d, const double *d_pl, const double *ck,
-                                          const double *ck_pl, const uint32_t *mask, const uint32_t *mask_pl);
+                                          const double *ck_pl, const uint32_t *mask, const uint32_t *mask_pl,
+                                          double *ck_b, double *ck_b_pl);
 
 /* Tangent-linear and adjoint of orc_horizontal_limiter_thuburn with
- * Qout_post supplied, at fixed d, ch and cmask.  mask: uint8 per cell-level,
+ * Qout_post supplied, at fixed d and cmask.  mask: uint8 per cell-level,
  * bit m-1 set where face m (1-3) of a cell in [gmin-1,gmax]^2 changed; pole:
  * uint8 per point, bit 0 for ring faces.  q_a is the forward's output.
- * TLM: dq_a in = dq_a_in, out = dq_a; dQprev written from dq and dQpost.
- * Adjoint: q_a_b in/out; q_b, Qpost_b written from q_a_b and Qprev_b. */
+ * TLM: dq_a in = dq_a_in, out = dq_a; dQprev written from dq, dQpost and
+ * dch (the Courant numbers' direction; NULL = zero).
+ * Adjoint: q_a_b in/out; q_b, Qpost_b and ch_b (if not NULL; every plane and
+ * level written) from q_a_b and Qprev_b.  The derivative of Qout in ch at
+ * ch = 0 is the right-hand one. */
 void orc_horizontal_limiter_thuburn_tlm(const orc_dims *D, const int *have_sgp, double *dq_a, double *dq_a_pl,
                                         const double *dq, const double *dq_pl, double *dQprev, double *dQprev_pl,
                                         const double *dQpost, const double *dQpost_pl, const double *q_a,
                                         const double *q_a_pl, const double *q, const double *q_pl, const double *d,
                                         const double *d_pl, const double *ch, const double *ch_pl,
                                         const double *cmask, const double *cmask_pl, const double *Qpost,
-                                        const double *Qpost_pl, const uint8_t *mask, const uint8_t *mask_pl);
+                                        const double *Qpost_pl, const uint8_t *mask, const uint8_t *mask_pl,
+                                        const double *dch, const double *dch_pl);
 void orc_horizontal_limiter_thuburn_adjoint(const orc_dims *D, const int *have_sgp, double *q_a_b,
                                             double *q_a_b_pl, double *q_b, double *q_b_pl, const double *Qprev_b,
                                             const double *Qprev_b_pl, double *Qpost_b, double *Qpost_b_pl,
@@ -245,7 +252,8 @@ void orc_horizontal_limiter_thuburn_adjoint(const orc_dims *D, const int *have_s
                                             const double *q_pl, const double *d, const double *d_pl,
                                             const double *ch, const double *ch_pl, const double *cmask,
                                             const double *cmask_pl, const double *Qpost, const double *Qpost_pl,
-                                            const uint8_t *mask, const uint8_t *mask_pl);
+                                            const uint8_t *mask, const uint8_t *mask_pl, double *ch_b,
+                                            double *ch_b_pl);
 
 /* Tangent-linear and adjoint of orc_horizontal_flux in (rho, rhovx, rhovy,
  * rhovz), metrics and dt fixed.  TLM: dflx (6 faces), dxc (3 arcs x 3)

@@ -1,0 +1,126 @@
+"""A stand-in for runtime.Context on CPU tensors whose kernels are the
+oracle's C restatement: the flux, both limiters (masked forwards and
+adjoints, with the Courant numbers' cotangents) and the rho*w solve with its
+adjoint.  It lets the tests run tracer_ad.TracerStep, the end-to-end
+gradient harness, on the CPU, and compare the GPU harness with it."""
+import numpy as np
+import torch
+
+from test_adjoint import pack_hlim_mask, pack_vlim_mask
+
+FLUX = ("rho", "rho_pl", "rhovx", "rhovx_pl", "rhovy", "rhovy_pl", "rhovz", "rhovz_pl")
+VI = ("rhogw", "rhogw0", "preg0", "rhog0", "Srho", "Sw", "Spre")
+
+
+def _np(t):
+    return t.detach().numpy()
+
+
+def _put(t, a):
+    t.copy_(torch.from_numpy(np.ascontiguousarray(a, dtype=np.float64)).reshape(t.shape))
+
+
+class OracleContext:
+    def __init__(self, oracle, spec, geo, vi, fac):
+        self.orc, self.spec, self.geo = oracle, spec, geo
+        self.sgp = [int(x) for x in geo["have_sgp"]]
+        self.vi = vi
+        self.fac = (fac["rdgzh"], fac["afact"], fac["bfact"])
+        self.kall, self.gall, self.lall, self.gall_pl = spec.kall, spec.gall, spec.lall, spec.gall_pl
+        self.nreg, self.npl = spec.gall * spec.kall * spec.lall, spec.gall_pl * spec.kall * 2
+        self.hmask = {}
+
+    def close(self):
+        pass
+
+    # ---- horizontal_flux --------------------------------------------------------
+    def horizontal_flux(self, flx, flx_pl, xc, xc_pl, *X_dt):
+        X, dt = X_dt[:-1], X_dt[-1]
+        f = {k: _np(v) for k, v in zip(FLUX, X)}
+        out = self.orc.horizontal_flux(self.spec, self.geo, f, dt)
+        for t, k in ((flx, "flx_h"), (flx_pl, "flx_h_pl"), (xc, "GRD_xc"), (xc_pl, "GRD_xc_pl")):
+            _put(t, out[k])
+
+    def horizontal_flux_adjoint(self, fb, fb_pl, xb, xb_pl, *rest):
+        Xb, X, dt = rest[:8], rest[8:16], rest[16]
+        f = {k: _np(v) for k, v in zip(FLUX, X)}
+        a = dict(flx_h_b=_np(fb), flx_h_b_pl=_np(fb_pl), GRD_xc_b=_np(xb), GRD_xc_b_pl=_np(xb_pl))
+        out = self.orc.horizontal_flux_lin(self.spec, self.geo, f, dt, True, a)
+        for t, k in zip(Xb, ("rho", "rho_pl", "rhovx", "rhovx_pl", "rhovy", "rhovy_pl", "rhovz", "rhovz_pl")):
+            _put(t, out[k.replace("_pl", "") + "_b" + ("_pl" if k.endswith("_pl") else "")])
+
+    # ---- horizontal limiter -----------------------------------------------------
+    def horizontal_limiter_thuburn_mask(self, q_a, q_a_pl, q, q_pl, d, d_pl, ch, ch_pl, cm, cm_pl, prev, prev_pl,
+                                        post, post_pl, mask, mask_pl):
+        f = dict(q_a=_np(q_a).copy(), q_a_pl=_np(q_a_pl).copy(), q=_np(q), q_pl=_np(q_pl), d=_np(d),
+                 d_pl=_np(d_pl), ch=_np(ch), ch_pl=_np(ch_pl), cmask=_np(cm), cmask_pl=_np(cm_pl),
+                 Qout_post=_np(post), Qout_post_pl=_np(post_pl))
+        out = self.orc.horizontal_limiter_thuburn(self.spec, self.sgp, f)
+        m, mp = pack_hlim_mask(self.spec, f, out)
+        _put(q_a, out["q_a"])
+        _put(q_a_pl, out["q_a_pl"])
+        _put(prev, out["Qout_prev"])
+        _put(prev_pl, out["Qout_prev_pl"])
+        mask.copy_(torch.from_numpy(m.reshape(mask.shape)))
+        mask_pl.copy_(torch.from_numpy(mp.reshape(mask_pl.shape)))
+
+    def horizontal_limiter_thuburn_adjoint(self, qab, qab_pl, qb, qb_pl, prevb, prevb_pl, postb, postb_pl, q_a,
+                                           q_a_pl, q, q_pl, d, d_pl, ch, ch_pl, cm, cm_pl, post, post_pl, mask,
+                                           mask_pl, ch_b=None, ch_b_pl=None):
+        f = dict(q=_np(q), q_pl=_np(q_pl), d=_np(d), d_pl=_np(d_pl), ch=_np(ch), ch_pl=_np(ch_pl), cmask=_np(cm),
+                 cmask_pl=_np(cm_pl), Qout_post=_np(post), Qout_post_pl=_np(post_pl))
+        u = dict(q_a_b=_np(qab), q_a_b_pl=_np(qab_pl), Qprev_b=_np(prevb), Qprev_b_pl=_np(prevb_pl))
+        out = self.orc.horizontal_limiter_thuburn_lin(self.spec, self.sgp, f, dict(q_a=_np(q_a), q_a_pl=_np(q_a_pl)),
+                                                      _np(mask).reshape(self.lall, self.kall, self.gall),
+                                                      _np(mask_pl), True, u)
+        for t, k in ((qab, "q_a_b"), (qab_pl, "q_a_b_pl"), (qb, "q_b"), (qb_pl, "q_b_pl"), (postb, "Qpost_b"),
+                     (postb_pl, "Qpost_b_pl"), (ch_b, "ch_b"), (ch_b_pl, "ch_b_pl")):
+            if t is not None:
+                _put(t, out[k])
+
+    # ---- vertical limiter -------------------------------------------------------
+    def vlim_mask_words(self):
+        nword = (self.kall - 3 + 31) // 32 if self.kall > 3 else 0
+        return nword * self.gall * self.lall, nword * self.gall_pl * 2
+
+    def vertical_limiter_thuburn_mask(self, q_h, q_h_pl, q, q_pl, d, d_pl, ck, ck_pl, mask, mask_pl):
+        f = dict(q_h=_np(q_h).copy(), q_h_pl=_np(q_h_pl).copy(), q=_np(q), q_pl=_np(q_pl), d=_np(d),
+                 d_pl=_np(d_pl), ck=_np(ck), ck_pl=_np(ck_pl))
+        out = self.orc.vertical_limiter_thuburn(self.spec, f)
+        m, mp = pack_vlim_mask(self.spec, {"q_h": f["q_h"], "q_h_pl": f["q_h_pl"]}, out)
+        _put(q_h, out["q_h"])
+        _put(q_h_pl, out["q_h_pl"])
+        mask.copy_(torch.from_numpy(m.reshape(-1).view(np.int32)))
+        mask_pl.copy_(torch.from_numpy(mp.reshape(-1).view(np.int32)))
+
+    def vertical_limiter_thuburn_adjoint(self, qhb, qhb_pl, qb, qb_pl, q_h, q_h_pl, q, q_pl, d, d_pl, ck, ck_pl,
+                                         mask, mask_pl, ck_b=None, ck_b_pl=None):
+        f = dict(q=_np(q), q_pl=_np(q_pl), d=_np(d), d_pl=_np(d_pl), ck=_np(ck), ck_pl=_np(ck_pl))
+        nword = (self.kall - 3 + 31) // 32
+        m = _np(mask).view(np.uint32).reshape(self.lall, nword, self.gall)
+        mp = _np(mask_pl).view(np.uint32).reshape(2, nword, self.gall_pl)
+        u = dict(q_h_b=_np(qhb), q_h_b_pl=_np(qhb_pl))
+        out = self.orc.vertical_limiter_thuburn_lin(self.spec, f, dict(q_h=_np(q_h), q_h_pl=_np(q_h_pl)), m, mp,
+                                                    True, u)
+        for t, k in ((qhb, "q_h_b"), (qhb_pl, "q_h_b_pl"), (qb, "q_b"), (qb_pl, "q_b_pl"), (ck_b, "ck_b"),
+                     (ck_b_pl, "ck_b_pl")):
+            if t is not None:
+                _put(t, out[k])
+
+    # ---- rho*w solve ------------------------------------------------------------
+    def vi_rhow_solver(self, w, w_pl, *rest):
+        ins, dt = rest[:-1], rest[-1]
+        f = dict(self.vi, rhogw=_np(w), rhogw_pl=_np(w_pl))
+        for i, n in enumerate(VI[1:]):
+            f[n], f[n + "_pl"] = _np(ins[2 * i]), _np(ins[2 * i + 1])
+        out = self.orc.vi_rhow_solver(self.spec, f, dt, *self.fac)
+        _put(w, out["rhogw"])
+        _put(w_pl, out["rhogw_pl"])
+
+    def vi_rhow_solver_adjoint(self, *rest):
+        bufs, dt = rest[:-1], rest[-1]
+        out = self.orc.vi_rhow_solver_adjoint(self.spec, self.vi, {"rhogw": _np(bufs[0]), "rhogw_pl": _np(bufs[1])},
+                                              dt, *self.fac)
+        for i, n in enumerate(VI):
+            _put(bufs[2 * i], out[n])
+            _put(bufs[2 * i + 1], out[n + "_pl"])

@@ -129,7 +129,8 @@ def test_oracle_vlim_adjoint_dot_product(oracle, c):
     assert m.any(), "no face was clipped: the case does not exercise the limiter"
     rng = np.random.default_rng(seed)
     dx = {"dq_h_in": rng.standard_normal(f["q_h"].shape), "dq_h_in_pl": rng.standard_normal(f["q_h_pl"].shape),
-          "dq": rng.standard_normal(f["q"].shape), "dq_pl": rng.standard_normal(f["q_pl"].shape)}
+          "dq": rng.standard_normal(f["q"].shape), "dq_pl": rng.standard_normal(f["q_pl"].shape),
+          "dck": rng.standard_normal(f["ck"].shape), "dck_pl": rng.standard_normal(f["ck_pl"].shape)}
     u = {"q_h_b": rng.standard_normal(f["q_h"].shape), "q_h_b_pl": rng.standard_normal(f["q_h_pl"].shape)}
     if not pl:
         for d in (dx, u):
@@ -139,8 +140,10 @@ def test_oracle_vlim_adjoint_dot_product(oracle, c):
     t = oracle.vertical_limiter_thuburn_lin(spec, f, out, m, mp, False, dx)
     a = oracle.vertical_limiter_thuburn_lin(spec, f, out, m, mp, True, u)
     lhs, s1 = _dot({"x": t["dq_h"], "x_pl": t["dq_h_pl"]}, {"x": u["q_h_b"], "x_pl": u["q_h_b_pl"]}, ["x"])
-    rhs, s2 = _dot({"a": dx["dq_h_in"], "a_pl": dx["dq_h_in_pl"], "b": dx["dq"], "b_pl": dx["dq_pl"]},
-                   {"a": a["q_h_b"], "a_pl": a["q_h_b_pl"], "b": a["q_b"], "b_pl": a["q_b_pl"]}, ["a", "b"])
+    rhs, s2 = _dot({"a": dx["dq_h_in"], "a_pl": dx["dq_h_in_pl"], "b": dx["dq"], "b_pl": dx["dq_pl"],
+                    "c": dx["dck"], "c_pl": dx["dck_pl"]},
+                   {"a": a["q_h_b"], "a_pl": a["q_h_b_pl"], "b": a["q_b"], "b_pl": a["q_b_pl"], "c": a["ck_b"],
+                    "c_pl": a["ck_b_pl"]}, ["a", "b", "c"])
     assert abs(lhs - rhs) <= 1e-13 * max(s1, s2), (lhs, rhs)
 
 
@@ -275,14 +278,24 @@ def test_gpu_vlim_mask_and_adjoint_match_oracle(gpu, oracle, c):
     qhb_pl = torch.from_numpy(u["q_h_b_pl"].copy()).to(gpu)
     qb = torch.full_like(dev["q"], np.nan)
     qb_pl = torch.full_like(dev["q_pl"], np.nan)
+    ckb, ckb_pl = torch.full_like(dev["ck"], np.nan), torch.full_like(dev["ck_pl"], np.nan)
     ctx.vertical_limiter_thuburn_adjoint(qhb, qhb_pl, qb, qb_pl, dev["q_h"], dev["q_h_pl"], dev["q"], dev["q_pl"],
-                                         dev["d"], dev["d_pl"], dev["ck"], dev["ck_pl"], m, mp)
+                                         dev["d"], dev["d_pl"], dev["ck"], dev["ck_pl"], m, mp, ckb, ckb_pl)
     want = oracle.vertical_limiter_thuburn_lin(spec, f, out, wm, wmp, True, u)
     np.testing.assert_array_equal(qhb.cpu().numpy(), want["q_h_b"])
     np.testing.assert_array_equal(qb.cpu().numpy(), want["q_b"])
+    np.testing.assert_array_equal(ckb.cpu().numpy(), want["ck_b"])
     if pl:
         np.testing.assert_array_equal(qhb_pl.cpu().numpy(), want["q_h_b_pl"])
         np.testing.assert_array_equal(qb_pl.cpu().numpy(), want["q_b_pl"])
+        np.testing.assert_array_equal(ckb_pl.cpu().numpy(), want["ck_b_pl"])
+    # without ck_b the other cotangents are the same
+    qhb2 = torch.from_numpy(u["q_h_b"].copy()).to(gpu)
+    qhb2_pl = torch.from_numpy(u["q_h_b_pl"].copy()).to(gpu)
+    qb2, qb2_pl = torch.full_like(qb, np.nan), torch.full_like(qb_pl, np.nan)
+    ctx.vertical_limiter_thuburn_adjoint(qhb2, qhb2_pl, qb2, qb2_pl, dev["q_h"], dev["q_h_pl"], dev["q"],
+                                         dev["q_pl"], dev["d"], dev["d_pl"], dev["ck"], dev["ck_pl"], m, mp)
+    assert torch.equal(qhb2, qhb) and torch.equal(qb2, qb)
 
 
 @pytest.mark.gpu
@@ -305,10 +318,12 @@ def test_gpu_vlim_adjoint_host_pointer_mode(gpu, oracle):
     u = {"q_h_b": rng.standard_normal(f["q_h"].shape), "q_h_b_pl": rng.standard_normal(f["q_h_pl"].shape)}
     qhb, qhb_pl = u["q_h_b"].copy(), u["q_h_b_pl"].copy()
     qb, qb_pl = np.zeros_like(f["q"]), np.zeros_like(f["q_pl"])
+    ckb, ckb_pl = np.full_like(f["ck"], np.nan), np.full_like(f["ck_pl"], np.nan)
     ctx.vertical_limiter_thuburn_adjoint(qhb, qhb_pl, qb, qb_pl, qh, qh_pl, f["q"], f["q_pl"], f["d"], f["d_pl"],
-                                         f["ck"], f["ck_pl"], m, mp)
+                                         f["ck"], f["ck_pl"], m, mp, ckb, ckb_pl)
     want = oracle.vertical_limiter_thuburn_lin(spec, f, out, wm, wmp, True, u)
-    for k, got in (("q_h_b", qhb), ("q_h_b_pl", qhb_pl), ("q_b", qb), ("q_b_pl", qb_pl)):
+    for k, got in (("q_h_b", qhb), ("q_h_b_pl", qhb_pl), ("q_b", qb), ("q_b_pl", qb_pl), ("ck_b", ckb),
+                   ("ck_b_pl", ckb_pl)):
         np.testing.assert_array_equal(got, want[k], err_msg=k)
 
 
@@ -349,6 +364,7 @@ def test_oracle_hlim_adjoint_dot_product(oracle, c):
     assert m.any() and mp.any()
     rng = np.random.default_rng(seed)
     dx = _hlim_dx(rng, f)
+    dx.update(dch=rng.standard_normal(f["ch"].shape), dch_pl=rng.standard_normal(f["ch_pl"].shape))
     u = {"q_a_b": rng.standard_normal(f["q_a"].shape), "q_a_b_pl": rng.standard_normal(f["q_a_pl"].shape),
          "Qprev_b": rng.standard_normal(f["Qout_post"].shape),
          "Qprev_b_pl": rng.standard_normal(f["Qout_post_pl"].shape)}
@@ -358,9 +374,9 @@ def test_oracle_hlim_adjoint_dot_product(oracle, c):
     lhs, s1 = _dot({"a": t["dq_a"], "a_pl": t["dq_a_pl"], "b": t["dQprev"], "b_pl": t["dQprev_pl"]},
                    {"a": u["q_a_b"], "a_pl": u["q_a_b_pl"], "b": u["Qprev_b"], "b_pl": u["Qprev_b_pl"]}, ["a", "b"])
     rhs, s2 = _dot({"a": dx["dq_a_in"], "a_pl": dx["dq_a_in_pl"], "b": dx["dq"], "b_pl": dx["dq_pl"],
-                    "c": dx["dQpost"], "c_pl": dx["dQpost_pl"]},
+                    "c": dx["dQpost"], "c_pl": dx["dQpost_pl"], "e": dx["dch"], "e_pl": dx["dch_pl"]},
                    {"a": a["q_a_b"], "a_pl": a["q_a_b_pl"], "b": a["q_b"], "b_pl": a["q_b_pl"], "c": a["Qpost_b"],
-                    "c_pl": a["Qpost_b_pl"]}, ["a", "b", "c"])
+                    "c_pl": a["Qpost_b_pl"], "e": a["ch_b"], "e_pl": a["ch_b_pl"]}, ["a", "b", "c", "e"])
     assert abs(lhs - rhs) <= 1e-13 * max(s1, s2), (lhs, rhs)
 
 
@@ -396,14 +412,92 @@ def test_oracle_hlim_tlm_matches_finite_differences(oracle):
         assert np.median(err) < 1e-7, k
 
 
-HLIM_GPU = [(6, 4, 2, 91, [1, 0]), (18, 6, 3, 92, [0, 1, 1]), (66, 12, 2, 93, [1, 0]), (130, 4, 1, 94, [0])]
+def test_oracle_courant_derivatives_match_finite_differences(oracle):
+    """The limiters' derivatives in the Courant numbers (ch, ck), alone, against
+    central differences of the forward kernels with cmask fixed; directions are
+    zero where a Courant number is an exact zero (the inflow flags and the
+    Cin/Cout split switch there).  The bounds move with ch, ck, so every
+    clipped face carries a derivative."""
+    # horizontal
+    spec = synth.GridSpec(12, 5, 2, have_pl=True, seed=83)
+    sgp = [1, 0]
+    f = dict(synth.make_case(spec, have_sgp=sgp)["hlim"])
+    rng = np.random.default_rng(83)
+    out = oracle.horizontal_limiter_thuburn(spec, sgp, f)
+    m, mp = pack_hlim_mask(spec, f, out)
+    zero = lambda a: {k: np.zeros_like(v) for k, v in a.items()}  # noqa: E731
+    dx = zero(_hlim_dx(rng, f))
+    dx.update(dch=rng.standard_normal(f["ch"].shape) * (f["ch"] != 0.0),
+              dch_pl=rng.standard_normal(f["ch_pl"].shape) * (f["ch_pl"] != 0.0))
+    t = oracle.horizontal_limiter_thuburn_lin(spec, sgp, f, out, m, mp, False, dx)
+    h = 1e-8
+    # (entries whose direction is zero stay as they are: -0.0 + 0.0 would flip a sign)
+    mv = lambda a, d, s: np.where(d != 0.0, a + s * h * d, a)  # noqa: E731
+    op = oracle.horizontal_limiter_thuburn(spec, sgp, {**f, "ch": mv(f["ch"], dx["dch"], 1),
+                                                       "ch_pl": mv(f["ch_pl"], dx["dch_pl"], 1)})
+    om = oracle.horizontal_limiter_thuburn(spec, sgp, {**f, "ch": mv(f["ch"], dx["dch"], -1),
+                                                       "ch_pl": mv(f["ch_pl"], dx["dch_pl"], -1)})
+    moved = 0
+    for k, tk in (("q_a", "dq_a"), ("Qout_prev", "dQprev"), ("q_a_pl", "dq_a_pl"), ("Qout_prev_pl", "dQprev_pl")):
+        fd = (op[k] - om[k]) / (2 * h)
+        err = np.abs(fd - t[tk])
+        bad = err > 1e-5 * (1.0 + np.abs(t[tk]))
+        assert bad.mean() < 0.01, (k, int(bad.sum()), bad.size)
+        moved += int((np.abs(t[tk]) > 1e-3).sum())
+    assert moved > 100
+    # vertical
+    spec = synth.GridSpec(12, 20, 2, have_pl=True, seed=31)
+    f = _smooth_vlim(spec, 31)
+    out, (m, mp) = _vlim_fwd(oracle, spec, f)
+    dx = {"dq_h_in": np.zeros_like(f["q_h"]), "dq_h_in_pl": np.zeros_like(f["q_h_pl"]), "dq": np.zeros_like(f["q"]),
+          "dq_pl": np.zeros_like(f["q_pl"])}
+    dck = rng.standard_normal(f["ck"].shape) * (f["ck"] != 0.0)
+    dck_pl = rng.standard_normal(f["ck_pl"].shape) * (f["ck_pl"] != 0.0)
+    dx.update(dck=dck, dck_pl=dck_pl)
+    t = oracle.vertical_limiter_thuburn_lin(spec, f, out, m, mp, False, dx)
+    op = oracle.vertical_limiter_thuburn(spec, {**f, "ck": mv(f["ck"], dck, 1), "ck_pl": mv(f["ck_pl"], dck_pl, 1)})
+    om = oracle.vertical_limiter_thuburn(spec, {**f, "ck": mv(f["ck"], dck, -1), "ck_pl": mv(f["ck_pl"], dck_pl, -1)})
+    for k in ("q_h", "q_h_pl"):
+        fd = (op[k] - om[k]) / (2 * h)
+        tl = t["d" + k]
+        err = np.abs(fd - tl)[..., KMIN0 + 1:spec.kall - 1, :]
+        bad = err > 1e-5 * (1.0 + np.abs(tl[..., KMIN0 + 1:spec.kall - 1, :]))
+        assert bad.mean() < 0.01, (k, int(bad.sum()), bad.size)
+        assert int((np.abs(tl) > 1e-3).sum()) > (5 if k.endswith("_pl") else 50), k
+
+
+def test_oracle_courant_derivative_fails_a_planted_slip(oracle):
+    """The dot-product test of the Courant directions is sharp: an adjoint that
+    drops the outflow faces' term (qnext - Qout)/Cout fails it."""
+    spec = synth.GridSpec(10, 5, 2, have_pl=True, seed=72)
+    sgp = [0, 1]
+    f = dict(synth.make_case(spec, have_sgp=sgp)["hlim"])
+    out = oracle.horizontal_limiter_thuburn(spec, sgp, f)
+    m, mp = pack_hlim_mask(spec, f, out)
+    rng = np.random.default_rng(72)
+    dx = {k: np.zeros_like(v) for k, v in _hlim_dx(rng, f).items()}
+    dx.update(dch=rng.standard_normal(f["ch"].shape), dch_pl=rng.standard_normal(f["ch_pl"].shape))
+    u = {"q_a_b": np.zeros_like(f["q_a"]), "q_a_b_pl": np.zeros_like(f["q_a_pl"]),
+         "Qprev_b": rng.standard_normal(f["Qout_post"].shape), "Qprev_b_pl": np.zeros_like(f["Qout_post_pl"])}
+    t = oracle.horizontal_limiter_thuburn_lin(spec, sgp, f, out, m, mp, False, dx)
+    a = oracle.horizontal_limiter_thuburn_lin(spec, sgp, f, out, m, mp, True, u)
+    lhs = float((t["dQprev"] * u["Qprev_b"]).sum())
+    good = float((dx["dch"] * a["ch_b"]).sum())
+    bad = float((dx["dch"] * a["ch_b"] * (f["ch"] < 0.0)).sum())  # only the inflow faces' terms
+    assert abs(lhs - good) <= 1e-12 * abs(lhs)
+    assert abs(lhs - bad) > 1e-3 * abs(lhs)
+
+
+HLIM_GPU = [(6, 4, 2, 91, [1, 0]), (18, 6, 3, 92, [0, 1, 1]), (66, 12, 2, 93, [1, 0]), (130, 4, 1, 94, [0]),
+            (160, 4, 1, 95, [1])]
 
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("c", HLIM_GPU, ids=[f"g{c[0]}k{c[1]}" for c in HLIM_GPU])
 def test_gpu_hlim_mask_and_adjoint_match_oracle(gpu, oracle, c):
-    """Mask and q_a_b bit for bit; q_b and Qout_post_b (summed atomically on
-    the GPU) to 1e-13 of the largest value; the pole arrays bit for bit."""
+    """Mask, q_a_b, q_b and Qout_post_b bit for bit (the GPU gathers every
+    cotangent in the oracle's order), regions and poles; gall_1d up to 160,
+    the widest region the kernels take."""
     import torch
 
     n, kall, lall, seed, sgp = c
@@ -434,17 +528,19 @@ def test_gpu_hlim_mask_and_adjoint_match_oracle(gpu, oracle, c):
     g = {k: T(v) for k, v in u.items()}
     qb, qb_pl = torch.full_like(dv["q"], np.nan), torch.full_like(dv["q_pl"], np.nan)
     pb, pb_pl = torch.full_like(qp, np.nan), torch.full_like(qp_pl, np.nan)
+    chb, chb_pl = torch.full_like(dv["ch"], np.nan), torch.full_like(dv["ch_pl"], np.nan)
     ctx.horizontal_limiter_thuburn_adjoint(g["q_a_b"], g["q_a_b_pl"], qb, qb_pl, g["Qprev_b"], g["Qprev_b_pl"], pb,
                                            pb_pl, dv["q_a"], dv["q_a_pl"], dv["q"], dv["q_pl"], dv["d"], dv["d_pl"],
                                            dv["ch"], dv["ch_pl"], dv["cmask"], dv["cmask_pl"], dv["Qout_post"],
-                                           dv["Qout_post_pl"], m, mp)
+                                           dv["Qout_post_pl"], m, mp, chb, chb_pl)
     np.testing.assert_array_equal(g["q_a_b"].cpu().numpy(), want["q_a_b"])
     np.testing.assert_array_equal(g["q_a_b_pl"].cpu().numpy(), want["q_a_b_pl"])
     np.testing.assert_array_equal(qb_pl.cpu().numpy(), want["q_b_pl"])
     np.testing.assert_array_equal(pb_pl.cpu().numpy(), want["Qpost_b_pl"])
-    for got, k in ((qb, "q_b"), (pb, "Qpost_b")):
-        w = want[k]
-        np.testing.assert_allclose(got.cpu().numpy(), w, rtol=0, atol=1e-13 * np.abs(w).max(), err_msg=k)
+    np.testing.assert_array_equal(qb.cpu().numpy(), want["q_b"])
+    np.testing.assert_array_equal(pb.cpu().numpy(), want["Qpost_b"])
+    np.testing.assert_array_equal(chb.cpu().numpy(), want["ch_b"])
+    np.testing.assert_array_equal(chb_pl.cpu().numpy(), want["ch_b_pl"])
 
 
 # ---- horizontal flux ------------------------------------------------------------
@@ -502,7 +598,8 @@ def test_oracle_flux_tlm_matches_finite_differences(oracle):
     assert np.abs(t["dflx_h"]).max() > 0
 
 
-FLUX_GPU = [(6, 4, 2, 121, [1, 0]), (18, 4, 3, 122, [0, 1, 1]), (66, 5, 2, 123, [1, 0]), (130, 4, 1, 124, [1])]
+FLUX_GPU = [(6, 4, 2, 121, [1, 0]), (18, 4, 3, 122, [0, 1, 1]), (66, 5, 2, 123, [1, 0]), (130, 4, 1, 124, [1]),
+            (160, 4, 1, 125, [1])]
 
 
 @pytest.mark.gpu

@@ -1,0 +1,274 @@
+"""The end-to-end adjoint: gradients of a quadratic cost in the final
+tracers, through several chained tracer steps (flux, limiters, rho*w solve
+and the glue between them, tracer_ad.TracerStep), with respect to the
+initial tracers and density.
+
+What pins the gradient:
+* Euler's identity.  At fixed density every stage of the step is linear in
+  the tracers or, for the limiters, piecewise linear and positively
+  homogeneous of degree one in them (the bounds are minima, maxima and
+  linear combinations of tracer values; the switches depend on the Courant
+  numbers only).  So a step F satisfies J_q(q) q = F(q), and for any
+  cotangent u, <F(q), u> = <q, J_q^T u>: a dot-product test of the composed
+  step's hand-written and torch transposes along the direction q, to the
+  rounding of the two FP64 computations.  Over n steps, J(q0) = 1/2
+  |q_n|^2 gives <q0, dJ/dq0> = 2 J.  A wrong transpose anywhere (a dropped
+  term, a wrong sign, a face attributed to the wrong bound) breaks both.
+* Central finite differences in the tracers and in the density on smooth
+  fields: the density reaches the limiters' bounds through the Courant
+  numbers, whose cotangents the limiter adjoints return.
+* Determinism: the same run twice gives the same bits; checkpointed
+  segments (recomputed in the backward pass) give the stored run's
+  gradient to rounding (autograd may sum a tensor's gradient terms in
+  another order when part of the graph is rebuilt).
+"""
+import numpy as np
+import pytest
+
+from icoatmosbenchmark_v1_amd import drivers, runtime, synth
+
+
+def _setup(gl, rl, kall, seed, ntr):
+    import torch
+
+    from icoatmosbenchmark_v1_amd.tracer_ad import TracerStep
+    from oracle import binding
+
+    geo, _ = synth.icosahedral_geometry(gl, rl)
+    n1 = 2 ** (gl - rl) + 2
+    spec = synth.GridSpec(n1, kall, 10 * 4 ** rl, seed=seed)
+    case = synth.make_case(spec, geo=geo)
+    ctx = drivers.setup_context(spec, geo, case["vi"])
+    hm = runtime.halo_map(rl, n1)
+    fac = binding.Oracle().grd_setup(case["vi"]["gz"], case["vi"]["gzh"])
+    step = TracerStep(ctx, spec, geo, case, hm, fac)
+    dev = torch.device("cuda:0")
+    # smooth tracers on the sphere: low-order functions of the cell positions
+    X = geo["GRD_x"][:, :, 0, :] / synth.RADIUS  # [d][l][g]
+    X_pl = geo["GRD_x_pl"][:, :, 0, :] / synth.RADIUS
+    z = np.linspace(0.0, 1.0, kall)[None, :, None]
+    qs, qs_pl = [], []
+    for t in range(ntr):
+        a = 0.3 + 0.1 * t
+        qs.append(0.5 + 0.2 * np.sin(2.0 * X[0] + a)[:, None, :] * np.cos(X[1] - a)[:, None, :] + 0.1 * z)
+        qs_pl.append(0.5 + 0.2 * np.sin(2.0 * X_pl[0] + a)[:, None, :] * np.cos(X_pl[1] - a)[:, None, :] + 0.1 * z)
+    T = lambda a: torch.from_numpy(np.ascontiguousarray(a, dtype=np.float64)).to(dev)  # noqa: E731
+    q0, q0_pl = step.halo(T(np.stack(qs)), T(np.stack(qs_pl)))
+    rho, rho_pl = step.halo(T(case["flux"]["rho"]), T(case["flux"]["rho_pl"]))
+    return ctx, step, q0, q0_pl, rho, rho_pl
+
+
+def _cpu_setup(oracle, gl, rl, kall, seed, ntr):
+    """The same harness on the CPU, its kernels the oracle's (oracle_context)."""
+    import torch
+
+    from icoatmosbenchmark_v1_amd.tracer_ad import TracerStep
+    from oracle_context import OracleContext
+
+    geo, _ = synth.icosahedral_geometry(gl, rl)
+    n1 = 2 ** (gl - rl) + 2
+    spec = synth.GridSpec(n1, kall, 10 * 4 ** rl, seed=seed)
+    case = synth.make_case(spec, geo=geo)
+    fac = oracle.grd_setup(case["vi"]["gz"], case["vi"]["gzh"])
+    ctx = OracleContext(oracle, spec, geo, case["vi"], fac)
+    step = TracerStep(ctx, spec, geo, case, runtime.halo_map(rl, n1), fac, device="cpu")
+    X = geo["GRD_x"][:, :, 0, :] / synth.RADIUS
+    X_pl = geo["GRD_x_pl"][:, :, 0, :] / synth.RADIUS
+    z = np.linspace(0.0, 1.0, kall)[None, :, None]
+    qs, qs_pl = [], []
+    for t in range(ntr):
+        a = 0.3 + 0.1 * t
+        qs.append(0.5 + 0.2 * np.sin(2.0 * X[0] + a)[:, None, :] * np.cos(X[1] - a)[:, None, :] + 0.1 * z)
+        qs_pl.append(0.5 + 0.2 * np.sin(2.0 * X_pl[0] + a)[:, None, :] * np.cos(X_pl[1] - a)[:, None, :] + 0.1 * z)
+    T = lambda a: torch.from_numpy(np.ascontiguousarray(a, dtype=np.float64))  # noqa: E731
+    q0, q0_pl = step.halo(T(np.stack(qs)), T(np.stack(qs_pl)))
+    rho, rho_pl = step.halo(T(case["flux"]["rho"]), T(case["flux"]["rho_pl"]))
+    return ctx, step, q0, q0_pl, rho, rho_pl
+
+
+def _run(step, q0, q0_pl, r0, r0_pl, nsteps, every, recompute_tracers=False):
+    from icoatmosbenchmark_v1_amd import autodiff
+
+    x = [q0.clone().requires_grad_(True), q0_pl.clone().requires_grad_(True), r0.clone().requires_grad_(True),
+         r0_pl.clone().requires_grad_(True)]
+    fn = lambda *s: step(*s, recompute_tracers=recompute_tracers)  # noqa: E731
+    if every:
+        s = autodiff.checkpointed(fn, x, nsteps, every)
+    else:
+        s = tuple(x)
+        for _ in range(nsteps):
+            s = fn(*s)
+    J = step.cost(s[0], s[1])
+    J.backward()
+    return float(J.detach()), [v.grad.detach().clone() for v in x]
+
+
+def _rel(a, b):
+    return max(float((x - y).abs().max()) / max(float(y.abs().max()), 1e-300) for x, y in zip(a, b))
+
+
+# ---- on the CPU, with the oracle's kernels ----------------------------------------
+def test_cpu_tracer_step_dot_product_along_q(oracle):
+    import torch
+
+    ctx, step, q0, q0_pl, r0, r0_pl = _cpu_setup(oracle, 3, 0, 6, 81, 2)
+    gen = torch.Generator().manual_seed(3)
+    for nsteps in (1, 2):
+        x = [q0.clone().requires_grad_(True), q0_pl.clone().requires_grad_(True), r0.clone(), r0_pl.clone()]
+        s = tuple(x)
+        for _ in range(nsteps):
+            s = step(*s)
+        u = [torch.randn(t.shape, dtype=torch.float64, generator=gen) for t in s[:2]]
+        lhs = sum(float((a.detach() * b).sum()) for a, b in zip(s[:2], u))
+        torch.autograd.backward(s[:2], u)
+        rhs = float((q0 * x[0].grad).sum() + (q0_pl * x[1].grad).sum())
+        scale = sum(float((a.detach().abs() * b.abs()).sum()) for a, b in zip(s[:2], u))
+        assert abs(lhs - rhs) <= 1e-12 * scale, (nsteps, lhs, rhs)
+
+
+def test_cpu_tracer_gradient_euler_and_finite_differences(oracle):
+    import torch
+
+    ctx, step, q0, q0_pl, r0, r0_pl = _cpu_setup(oracle, 3, 0, 6, 82, 2)
+    J, g = _run(step, q0, q0_pl, r0, r0_pl, 3, 0)
+    assert all(bool(torch.isfinite(x).all()) for x in g)
+    lhs = float((q0 * g[0]).sum() + (q0_pl * g[1]).sum())
+    assert abs(lhs - 2 * J) <= 1e-12 * abs(2 * J), (lhs, 2 * J)
+    gen = torch.Generator().manual_seed(5)
+    for which in (0, 2):
+        v = torch.randn((q0 if which == 0 else r0).shape, dtype=torch.float64, generator=gen)
+        v_pl = torch.randn((q0_pl if which == 0 else r0_pl).shape, dtype=torch.float64, generator=gen)
+        scale = 1.0 if which == 0 else float(r0.abs().max())
+        v, v_pl = step.halo(v * scale * 1e-2, v_pl * scale * 1e-2)
+        # the density's direction needs a longer step: at 1e-6 the difference of J drowns in its rounding
+        h = 1e-6 if which == 0 else 1e-3
+        with torch.no_grad():
+            def J_at(sign):
+                s = [q0, q0_pl, r0, r0_pl]
+                s[which] = s[which] + sign * h * v
+                s[which + 1] = s[which + 1] + sign * h * v_pl
+                for _ in range(3):
+                    s = step(*s)
+                return float(step.cost(s[0], s[1]))
+            fd = (J_at(1) - J_at(-1)) / (2 * h)
+        ad = float((g[which] * v).sum() + (g[which + 1] * v_pl).sum())
+        assert abs(fd - ad) <= 1e-5 * abs(ad), (which, fd, ad)
+
+
+def test_cpu_tracer_gradient_checkpointed(oracle):
+    ctx, step, q0, q0_pl, r0, r0_pl = _cpu_setup(oracle, 3, 0, 6, 83, 2)
+    J, g = _run(step, q0, q0_pl, r0, r0_pl, 3, 0)
+    for every, rec in ((2, False), (1, True)):
+        Jc, gc = _run(step, q0, q0_pl, r0, r0_pl, 3, every, recompute_tracers=rec)
+        assert Jc == J and _rel(gc, g) <= 1e-12, (every, rec)
+
+
+# ---- on the GPU ----------------------------------------------------------------------
+@pytest.mark.gpu
+def test_gpu_tracer_gradient_equals_the_oracle_harness(gpu, oracle):
+    """The GPU harness and the CPU one on the oracle's kernels: the kernels
+    agree bit for bit, torch's glue on the two devices to rounding."""
+    ctx_c, step_c, *xc = _cpu_setup(oracle, 3, 0, 6, 84, 2)
+    ctx, step, *xg = _setup(3, 0, 6, 84, 2)
+    Jc, gc = _run(step_c, *xc, 3, 0)
+    Jg, gg = _run(step, *xg, 3, 0)
+    assert abs(Jg - Jc) <= 1e-13 * abs(Jc)
+    assert _rel([x.cpu() for x in gg], gc) <= 1e-11
+    ctx.close()
+
+
+@pytest.mark.gpu
+def test_gpu_tracer_gradient_deterministic_and_checkpointed(gpu):
+    import torch
+
+    ctx, step, q0, q0_pl, r0, r0_pl = _setup(4, 0, 8, 71, 3)
+    J, g = _run(step, q0, q0_pl, r0, r0_pl, 4, 0)
+    J2, g2 = _run(step, q0, q0_pl, r0, r0_pl, 4, 0)
+    assert J == J2 and all(torch.equal(a, b) for a, b in zip(g, g2))
+    for every, rec in ((2, False), (1, True)):
+        Jc, gc = _run(step, q0, q0_pl, r0, r0_pl, 4, every, recompute_tracers=rec)
+        Jc2, gc2 = _run(step, q0, q0_pl, r0, r0_pl, 4, every, recompute_tracers=rec)
+        assert Jc == J and Jc2 == J
+        assert all(torch.equal(a, b) for a, b in zip(gc, gc2))
+        assert _rel(gc, g) <= 1e-12, (every, rec, _rel(gc, g))
+    ctx.close()
+
+
+@pytest.mark.gpu
+def test_gpu_tracer_step_dot_product_along_q(gpu):
+    """<F(q), u> = <q, J_q^T u> for one step and for four, random u on every
+    output (tracers and density)."""
+    import torch
+
+    ctx, step, q0, q0_pl, r0, r0_pl = _setup(4, 0, 8, 72, 2)
+    gen = torch.Generator(device=gpu).manual_seed(3)
+    for nsteps in (1, 4):
+        x = [q0.clone().requires_grad_(True), q0_pl.clone().requires_grad_(True), r0.clone(), r0_pl.clone()]
+        s = tuple(x)
+        for _ in range(nsteps):
+            s = step(*s)
+        u = [torch.randn(t.shape, dtype=torch.float64, device=gpu, generator=gen) for t in s[:2]]
+        lhs = sum(float((a.detach() * b).sum()) for a, b in zip(s[:2], u))
+        torch.autograd.backward(s[:2], u)
+        terms = [q0 * x[0].grad, q0_pl * x[1].grad]
+        rhs = sum(float(t.sum()) for t in terms)
+        scale = sum(float((a.detach().abs() * b.abs()).sum()) for a, b in zip(s[:2], u))
+        print(f"dot product, {nsteps} step(s): <F(q),u> {lhs:.15e} <q,J^T u> {rhs:.15e}", flush=True)
+        assert abs(lhs - rhs) <= 1e-12 * scale, (nsteps, lhs, rhs, scale)
+    ctx.close()
+
+
+@pytest.mark.gpu
+def test_gpu_tracer_gradient_euler_and_finite_differences(gpu):
+    import torch
+
+    ctx, step, q0, q0_pl, r0, r0_pl = _setup(4, 0, 8, 71, 3)
+    J, g = _run(step, q0, q0_pl, r0, r0_pl, 4, 0)
+    lhs = float((q0 * g[0]).sum() + (q0_pl * g[1]).sum())
+    assert abs(lhs - 2 * J) <= 1e-12 * abs(2 * J), (lhs, 2 * J)
+    assert float(g[2].abs().max()) > 0.0
+    gen = torch.Generator(device=gpu).manual_seed(5)
+    for which in (0, 2):
+        v = torch.randn(q0.shape if which == 0 else r0.shape, dtype=torch.float64, device=gpu, generator=gen)
+        v_pl = torch.randn(q0_pl.shape if which == 0 else r0_pl.shape, dtype=torch.float64, device=gpu,
+                           generator=gen)
+        scale = 1.0 if which == 0 else float(r0.abs().max())
+        v, v_pl = step.halo(v * scale * 1e-2, v_pl * scale * 1e-2)
+        # the density's direction needs a longer step: at 1e-6 the difference of J drowns in its rounding
+        h = 1e-6 if which == 0 else 1e-3
+        with torch.no_grad():
+            def J_at(sign):
+                s = [q0, q0_pl, r0, r0_pl]
+                s[which] = s[which] + sign * h * v
+                s[which + 1] = s[which + 1] + sign * h * v_pl
+                for _ in range(4):
+                    s = step(*s)
+                return float(step.cost(s[0], s[1]))
+            fd = (J_at(1) - J_at(-1)) / (2 * h)
+        ad = float((g[which] * v).sum() + (g[which + 1] * v_pl).sum())
+        print(f"finite differences ({'q' if which == 0 else 'rhog'}): fd {fd:.12e} ad {ad:.12e}", flush=True)
+        assert abs(fd - ad) <= 1e-5 * abs(ad), (which, fd, ad)
+    ctx.close()
+
+
+@pytest.mark.gpu
+def test_gpu_tracer_gradient_gl7_16_tracers_24_steps(gpu):
+    """The adjoint configuration: glevel 7 (40 regions of 66x66, 98 levels),
+    16 tracers, 24 steps, checkpoints every 6 steps (and each step's tracers
+    recomputed in turn), the gradient in the initial tracers and density."""
+    import time
+
+    import torch
+
+    ctx, step, q0, q0_pl, r0, r0_pl = _setup(7, 1, 98, 77, 16)
+    torch.cuda.reset_peak_memory_stats()
+    t0 = time.perf_counter()
+    J, g = _run(step, q0, q0_pl, r0, r0_pl, 24, 6, recompute_tracers=True)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    lhs = float((q0 * g[0]).sum() + (q0_pl * g[1]).sum())
+    print(f"gl7 16 tracers 24 steps: J {J:.12e}, <q0,dJ/dq0> {lhs:.12e}, gradient in {dt:.1f} s, "
+          f"peak memory {torch.cuda.max_memory_allocated() / 1e9:.1f} GB", flush=True)
+    assert abs(lhs - 2 * J) <= 1e-12 * abs(2 * J)
+    assert torch.isfinite(g[2]).all() and float(g[2].abs().max()) > 0.0
+    ctx.close()
