@@ -469,20 +469,20 @@ __global__ void __launch_bounds__(BLOCK_VLA) vlim_adjoint_kernel(VlaArgs A)
 // Qout_post max, Qin min, Qin max, by equality with the stored output).
 //
 // One work-group per (k, l) slab, sweeping it in steps of HSW cells, with
-// no atomics: every cotangent is a gather in the order of a sequential sweep
-// over the slab.  A cell's q_b is the sum of the terms of the Qout rows that
-// name it, in cell order of the rows' owners (each row: bound mm = 0 then 1,
-// its terms in row order), followed by the face cotangents attributed to it,
-// in cell order of the faces' origins (arcs m = 0, 1, 2); Qout_post_b
-// likewise.  Those are the orders of the oracle's loops, so the results are
-// its bits.  An LDS ring of R = HSW + 2*iall + 3 cells holds q and, per cell,
-// what the gathers read: its row's coefficients and Qprev_b, the cell each
-// term names (one of the seven cells of the hexagon around the owner; the
-// inner rows' arcs never reach further or hit the index clamps of
-// HLIM:152-153), and the cotangents of its three own faces with their
-// attribution.  Each step loads q 2*iall+3 cells ahead, forms rows and faces
-// iall+1 cells ahead and gathers HSW cells; a face's q_a_b is read and
-// rewritten by its origin's work-item only.
+// no atomics: every cotangent is a gather in a fixed order, the oracle's.
+// Every cell a Qout row names lies in the hexagon of the row's owner
+// {g-iall-1, g-iall, g-1, g, g+1, g+iall, g+iall+1} (the inner rows' arcs never
+// reach further or hit the index clamps of HLIM:152-153), so each row sums its
+// terms (bound mm = 0 then 1, each bound's terms in order) per hexagon cell
+// into seven partial sums, and a cell's q_b is the sum of the partial sums of
+// the rows of its hexagon, in cell order of the rows' owners, followed by the
+// face cotangents attributed to it, in cell order of the faces' origins (arcs
+// m = 0, 1, 2); Qout_post_b likewise.  An LDS ring of R = HSW + 2*iall + 3
+// cells holds q and, per cell, its row's seven partial sums and the
+// cotangents of its three own faces with their attribution (92 bytes).  Each
+// step loads q 2*iall+3 cells ahead, forms rows and faces iall+1 cells ahead
+// and gathers HSW cells; a face's q_a_b is read and rewritten by its origin's
+// work-item only.
 constexpr int HSW = 256;
 constexpr int HMAXI = 160;
 constexpr int HPOLE = 256;
@@ -553,18 +553,16 @@ __host__ __device__ constexpr int arc_slot_sgp(int d) { return d == 4 ? 0 : d ==
 
 __global__ void __launch_bounds__(HPOLE) hlim_adjoint_pole(HaArgs A);
 
-__global__ void __launch_bounds__(HSW) hlim_adjoint_kernel(HaArgs A)
+// four waves per SIMD (the LDS ring allows four work-groups per CU): 1.57 ms at gl7, against 1.87 ms at
+// three and 2.25 ms at two, despite the registers the cap spills
+__global__ void __launch_bounds__(HSW) __attribute__((amdgpu_waves_per_eu(4, 8))) hlim_adjoint_kernel(HaArgs A)
 {
     extern __shared__ double lds[];
     const int iall = A.iall, gall = A.gall, kall = A.kall, lall = A.lall, R = A.ring;
     double *r_q = lds;               // q
-    double *r_c0 = r_q + R;          // row term 0 (the owner), fac + zerosw
-    double *r_c1 = r_c0 + R;         // [2][R] row term 1 (qnext's selection), per bound
-    double *r_cf = r_c1 + 2 * R;     // [6][R] row terms of the six arcs
-    double *r_bq = r_cf + 6 * R;     // [2][R] Qprev_b
-    double *r_ob = r_bq + 2 * R;     // [3][R] face cotangents of the cell's own arcs
-    uint64_t *r_code = reinterpret_cast<uint64_t *>(r_ob + 3 * R);  // row: cells named by the terms
-    uint32_t *r_fc = reinterpret_cast<uint32_t *>(r_code + R);      // faces: attribution
+    double *r_p = r_q + R;           // [7][R] the row's partial sums per hexagon cell
+    double *r_ob = r_p + 7 * R;      // [3][R] face cotangents of the cell's own arcs
+    uint32_t *r_fc = reinterpret_cast<uint32_t *>(r_ob + 3 * R);  // faces: attribution
     const int k = blockIdx.x % kall, l = blockIdx.x / kall;
     if (l >= lall) return;
     const size_t fb = ((size_t)l * kall + k) * gall;
@@ -572,7 +570,6 @@ __global__ void __launch_bounds__(HSW) hlim_adjoint_kernel(HaArgs A)
     const int tid = threadIdx.x;
     const bool sgp = A.have_sgp[l] != 0;
     const double BIG = A.big;
-    constexpr uint64_t kInner = 1ull << 63;
 
     // one cell's row (if inner) or Qprev_b pass-through (halo), and its faces
     auto form = [&](int e) {
@@ -584,10 +581,10 @@ __global__ void __launch_bounds__(HSW) hlim_adjoint_kernel(HaArgs A)
             return r_q[s];
         };
         const int i = e % iall, j = e / iall;
-        r_bq[se] = A.Qprevb[fb + e];
-        r_bq[R + se] = A.Qprevb[plane + fb + e];
+        const double b0 = A.Qprevb[fb + e], b1 = A.Qprevb[plane + fb + e];
+        double part[7] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
         if (i < 1 || i > iall - 2 || j < 1 || j > iall - 2) {
-            r_code[se] = 0;
+            part[3] = (0.0 + b0) + b1;  // halo: Qout = q
             if (A.chb)
 #pragma unroll
                 for (int f = 0; f < 6; ++f) A.chb[(size_t)f * plane + fb + e] = 0.0;
@@ -643,7 +640,6 @@ __global__ void __launch_bounds__(HSW) hlim_adjoint_kernel(HaArgs A)
             const double zerosw = 0.5 - sgn_half(fabs(Cout) - A.eps);
             const double E = 1.0 - Cin - Cout + A.d[fb + e];
             const double fac = (1.0 - zerosw) / (Cout + zerosw);
-            r_c0[se] = fac + zerosw;
             if (A.chb) {
                 // Qout in ch: an inflow face (ch < 0) gives (qnext - Qin)/Cout, any other (qnext - Qout)/Cout
                 double CQmin = chm[0] * vmn[0], CQmax = chm[0] * vmx[0];
@@ -654,7 +650,6 @@ __global__ void __launch_bounds__(HSW) hlim_adjoint_kernel(HaArgs A)
                 }
                 const double Qmin = (qij - CQmax - nmax * E) / (Cout + zerosw) * (1.0 - zerosw) + qij * zerosw;
                 const double Qmax = (qij - CQmin - nmin * E) / (Cout + zerosw) * (1.0 - zerosw) + qij * zerosw;
-                const double b0 = r_bq[se], b1 = r_bq[R + se];
 #pragma unroll
                 for (int f = 0; f < 6; ++f) {
                     const double d0 = zerosw != 0.0 ? 0.0 : (chv[f] < 0.0 ? nmax - vmx[f] : nmax - Qmin) / Cout;
@@ -663,31 +658,40 @@ __global__ void __launch_bounds__(HSW) hlim_adjoint_kernel(HaArgs A)
                     A.chb[(size_t)f * plane + fb + e] = (b0 != 0.0 ? b0 * d0 : 0.0) + (b1 != 0.0 ? b1 * d1 : 0.0);
                 }
             }
-            uint64_t code = kInner;
+            // the row's terms, each into the partial sum of the hexagon cell it names (selects, not
+            // indexing: an indexed register array would live in scratch)
+            double cf[6];
+#pragma unroll
+            for (int f = 0; f < 6; ++f) cf[f] = -(fac * chm[f]) * cc[f];
+            const double c0 = fac + zerosw;
 #pragma unroll
             for (int mm = 0; mm < 2; ++mm) {
                 const int sel = mm == 0 ? smax : smin;
-                // the selected arc's cell and factor by selects (an indexed register array would live in scratch)
-                int cell_f[6];
+                const double b = mm == 0 ? b0 : b1;
+                uint32_t wf[6];
                 uint32_t selw = 3u;
                 double selc = 1.0;
 #pragma unroll
                 for (int f = 0; f < 6; ++f) {
-                    cell_f[f] = mm == 0 ? cmx[f] : cmn[f];
-                    const uint32_t w = hex_index(cell_f[f] - e, iall);
-                    selw = sel == f ? w : selw;
+                    wf[f] = hex_index((mm == 0 ? cmx[f] : cmn[f]) - e, iall);
+                    selw = sel == f ? wf[f] : selw;
                     selc = sel == f ? cc[f] : selc;
                 }
-                r_c1[mm * R + se] = -(fac * E) * selc;
-                uint64_t cm = selw;
+                part[3] = part[3] + c0 * b;
+                const double t1 = (-(fac * E) * selc) * b;
 #pragma unroll
-                for (int f = 0; f < 6; ++f) cm |= (uint64_t)hex_index(cell_f[f] - e, iall) << (3 + 3 * f);
-                code |= cm << (21 * mm);
+                for (int w = 0; w < 7; ++w) part[w] = selw == (uint32_t)w ? part[w] + t1 : part[w];
+#pragma unroll
+                for (int f = 0; f < 6; ++f) {
+                    const double t = cf[f] * b;
+#pragma unroll
+                    for (int w = 0; w < 7; ++w)  // arc f's cells (any, for the singular-point row's f = 4)
+                        if (arc_has(f, w) || f == 4) part[w] = wf[f] == (uint32_t)w ? part[w] + t : part[w];
+                }
             }
-#pragma unroll
-            for (int f = 0; f < 6; ++f) r_cf[f * R + se] = -(fac * chm[f]) * cc[f];
-            r_code[se] = code;
         }
+#pragma unroll
+        for (int w = 0; w < 7; ++w) r_p[w * R + se] = part[w];
         // faces 1-3 of the own arcs (HLIM:377-410)
         uint32_t fc = 0;
         if (i <= iall - 2 && j <= iall - 2) {
@@ -738,32 +742,9 @@ __global__ void __launch_bounds__(HSW) hlim_adjoint_kernel(HaArgs A)
         const int hx[7] = {-iall - 1, -iall, -1, 0, 1, iall, iall + 1};
         double acc = 0.0;
 #pragma unroll
-        for (int h = 0; h < 7; ++h) {
+        for (int h = 0; h < 7; ++h) {  // the rows of c's hexagon, in cell order: their partial sums for c
             const int g = c + hx[h];
-            if (g < 0 || g >= gall) continue;
-            const int s = sl(g);
-            const uint64_t code = r_code[s];
-            const uint32_t want = 6 - h;  // c seen from g
-            if (!(code & kInner)) {
-                if (h == 3) {
-                    acc = acc + r_bq[s];
-                    acc = acc + r_bq[R + s];
-                }
-                continue;
-            }
-            // only arcs that contain c can name it (any arc, for the singular-point row)
-            const bool any_arc = sgp && g == iall + 1;
-#pragma unroll
-            for (int mm = 0; mm < 2; ++mm) {
-                const double b = r_bq[mm * R + s];
-                const uint64_t cm = code >> (21 * mm);
-                if (h == 3) acc = acc + r_c0[s] * b;
-                if ((uint32_t)(cm & 7u) == want) acc = acc + r_c1[mm * R + s] * b;
-#pragma unroll
-                for (int f = 0; f < 6; ++f)
-                    if ((arc_has(f, 6 - h) || any_arc) && (uint32_t)((cm >> (3 + 3 * f)) & 7u) == want)
-                        acc = acc + r_cf[f * R + s] * b;
-            }
+            if (g >= 0 && g < gall) acc = acc + r_p[(6 - h) * R + sl(g)];
         }
         double pn = 0.0, px = 0.0;
         // faces of origin p; d: index of c - p (arc_slot's), known after unrolling
@@ -1238,7 +1219,7 @@ int launch_hlim_adjoint(nicam_ctx c, double *qab, double *qab_pl, double *qb, do
     A.ring = HSW + 2 * D.gall_1d + 3;
     const long nslab = (long)D.kall * D.lall;
     if (nslab > 0x7fffffffL) return fail(NICAM_ERR_ARG, "grid too large for one launch");
-    const size_t lds = (size_t)A.ring * (16 * sizeof(double) + sizeof(uint32_t));
+    const size_t lds = (size_t)A.ring * (11 * sizeof(double) + sizeof(uint32_t));
     if (lds > 64 * 1024)
         NICAM_HIP(hipFuncSetAttribute((const void *)hlim_adjoint_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
                                       (int)lds));

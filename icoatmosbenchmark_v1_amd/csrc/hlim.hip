@@ -482,7 +482,9 @@ int launch_hlim_t(nicam_ctx c, T *q_a, T *q_a_pl, const T *q, const T *q_pl, con
         A.phases = phases;
         if (ntr > 1)
             hipLaunchKernelGGL((hlim_kernel<T, true>), dim3((unsigned)nblk), dim3(CHUNK), 0, c->stream, A);
-        else if (c->tune.hlim_variant == 2)
+        // FP32: the loads before the barrier (gl7: 344 against 359-369 us); FP64: no difference (673-684 us),
+        // the fewer registers
+        else if (c->tune.hlim_variant == 2 || (c->tune.hlim_variant == 0 && std::is_same<T, float>::value))
             hipLaunchKernelGGL((hlim_kernel<T, false, true>), dim3((unsigned)nblk), dim3(CHUNK), 0, c->stream, A);
         else
             hipLaunchKernelGGL((hlim_kernel<T, false>), dim3((unsigned)nblk), dim3(CHUNK), 0, c->stream, A);

@@ -1602,13 +1602,29 @@ static void hl_slab(int iall, int sgp, const double *qs, const double *ds, const
     const int sgp_cell = sgp ? (gmin - 1 - 1) * iall + (gmin - 1) : -1;
 #define CM(g, f) cms[(size_t)((f) - 1) * cplane + (g) - 1]
 #define CH(g, f) chs[(size_t)((f) - 1) * cplane + (g) - 1]
+    /* The adjoint's order: each row's terms are first summed per cell of the
+     * owner's hexagon {g-iall-1, g-iall, g-1, g, g+1, g+iall, g+iall+1} (every
+     * cell a row names lies there), in row order (bound min, then max; each
+     * bound's terms in order), into seven partial sums; each cell then adds
+     * the partial sums of the rows of its hexagon in cell order. */
+    const int hexo[7] = {-iall - 1, -iall, -1, 0, 1, iall, iall + 1};
+#define HEXADD(part)                                                              \
+    do {                                                                          \
+        for (int w = 0; w < 7; ++w) {                                             \
+            const int tg = g + hexo[w];                                           \
+            if (tg >= 1 && tg <= gall) xq[tg - 1] = xq[tg - 1] + (part)[w];       \
+        }                                                                         \
+    } while (0)
     /* Qout of each cell: coefficient rows (up to 14 terms) */
     for (int g = 1; g <= gall; ++g) {
         const int i = (g - 1) % iall + 1, j = (g - 1) / iall + 1;
         if (i < gmin || i > gmax || j < gmin || j > gmax) { /* halo: Qout = q (HLIM:286-301) */
-            for (int mm = 0; mm < 2; ++mm) {
-                if (adjoint) xq[g - 1] = xq[g - 1] + xQprev[(size_t)mm * qplane + g - 1];
-                else xQprev[(size_t)mm * qplane + g - 1] = xq[g - 1];
+            if (adjoint) {
+                double part[7] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+                for (int mm = 0; mm < 2; ++mm) part[3] = part[3] + xQprev[(size_t)mm * qplane + g - 1];
+                HEXADD(part);
+            } else {
+                for (int mm = 0; mm < 2; ++mm) xQprev[(size_t)mm * qplane + g - 1] = xq[g - 1];
             }
             if (adjoint && xch)
                 for (int f = 1; f <= 6; ++f) xch[(size_t)(f - 1) * cplane + g - 1] = 0.0;
@@ -1666,6 +1682,7 @@ static void hl_slab(int iall, int sgp, const double *qs, const double *ds, const
                 xch[(size_t)(f - 1) * cplane + g - 1] = (b0 != 0.0 ? b0 * dch[0][f] : 0.0) + (b1 != 0.0 ? b1 * dch[1][f] : 0.0);
         }
         /* Qout min (mm=0) uses the max side, Qout max (mm=1) the min side */
+        double part[7] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
         for (int mm = 0; mm < 2; ++mm) {
             const int sel = mm == 0 ? smax : smin;
             const int *cx = mm == 0 ? cmx : cmn;
@@ -1679,7 +1696,12 @@ static void hl_slab(int iall, int sgp, const double *qs, const double *ds, const
             const size_t o = (size_t)mm * qplane + g - 1;
             if (adjoint) {
                 const double b = xQprev[o];
-                for (int t = 0; t < n; ++t) xq[cells[t] - 1] = xq[cells[t] - 1] + coef[t] * b;
+                for (int t = 0; t < n; ++t) {
+                    int w = 0;
+                    while (w < 7 && g + hexo[w] != cells[t]) ++w;
+                    if (w == 7) abort(); /* a row names only cells of its owner's hexagon */
+                    part[w] = part[w] + coef[t] * b;
+                }
             } else {
                 double v = 0.0;
                 for (int t = 0; t < n; ++t) v = v + coef[t] * xq[cells[t] - 1];
@@ -1691,7 +1713,9 @@ static void hl_slab(int iall, int sgp, const double *qs, const double *ds, const
                 xQprev[o] = v;
             }
         }
+        if (adjoint) HEXADD(part);
     }
+#undef HEXADD
     /* faces 1-3 of [gmin-1, gmax]^2 (HLIM:377-410) */
     for (int j = gmin - 1; j <= gmax; ++j)
         for (int i = gmin - 1; i <= gmax; ++i) {
