@@ -555,6 +555,8 @@ __global__ void __launch_bounds__(HPOLE) hlim_adjoint_pole(HaArgs A);
 
 // four waves per SIMD (the LDS ring allows four work-groups per CU): 1.57 ms at gl7, against 1.87 ms at
 // three and 2.25 ms at two, despite the registers the cap spills
+// CHB: also the cotangent of ch
+template <bool CHB>
 __global__ void __launch_bounds__(HSW) __attribute__((amdgpu_waves_per_eu(4, 8))) hlim_adjoint_kernel(HaArgs A)
 {
     extern __shared__ double lds[];
@@ -581,16 +583,88 @@ __global__ void __launch_bounds__(HSW) __attribute__((amdgpu_waves_per_eu(4, 8))
             return r_q[s];
         };
         const int i = e % iall, j = e / iall;
+        const bool inner = i >= 1 && i <= iall - 2 && j >= 1 && j <= iall - 2;
+        const bool own = i <= iall - 2 && j <= iall - 2;  // origin of three arcs
+        // ---- every global load of the cell before its first store: a load issued after a store
+        // to memory it may alias is ordered behind it
         const double b0 = A.Qprevb[fb + e], b1 = A.Qprevb[plane + fb + e];
-        double part[7] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-        if (i < 1 || i > iall - 2 || j < 1 || j > iall - 2) {
-            part[3] = (0.0 + b0) + b1;  // halo: Qout = q
-            if (A.chb)
+        uint32_t bits = 0;
+        double cmo[3] = {0.0, 0.0, 0.0}, ob[3] = {0.0, 0.0, 0.0};  // own arcs' cmask, face cotangents
+        if (own) {
+            bits = A.mask[fb + e];
 #pragma unroll
-                for (int f = 0; f < 6; ++f) A.chb[(size_t)f * plane + fb + e] = 0.0;
+            for (int m = 0; m < 3; ++m) {
+                const int nb = e + (m == 0 ? 1 : (m == 1 ? iall + 1 : iall));
+                cmo[m] = A.cmask[(size_t)m * plane + fb + e];
+                ob[m] = A.qab[(size_t)m * plane + fb + e] + A.qab[(size_t)(m + 3) * plane + fb + nb];
+            }
+        }
+        double cmw[3] = {0.0, 0.0, 0.0}, chv[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0}, dc = 0.0;
+        if (inner) {
+#pragma unroll
+            for (int m = 0; m < 3; ++m)  // the arcs of (i-1,j), (i-1,j-1), (i,j-1)
+                cmw[m] = A.cmask[(size_t)m * plane + fb + e - (m == 0 ? 1 : (m == 1 ? iall + 1 : iall))];
+#pragma unroll
+            for (int f = 0; f < 6; ++f) chv[f] = A.ch[(size_t)f * plane + fb + e];
+            dc = A.d[fb + e];
+        }
+        double fout[3] = {0.0, 0.0, 0.0}, fq0[3] = {0.0, 0.0, 0.0}, fq1[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+        for (int m = 0; m < 3; ++m) {
+            if ((bits >> m) & 1u) {  // a changed face: the forward's output and the bounds it may equal
+                const int nb = e + (m == 0 ? 1 : (m == 1 ? iall + 1 : iall));
+                const int T = cmo[m] > 0.0 ? nb : e;
+                fout[m] = A.q_a[(size_t)m * plane + fb + e];
+                fq0[m] = A.Qpost[fb + T];
+                fq1[m] = A.Qpost[plane + fb + T];
+            }
+        }
+
+        // ---- faces 1-3 of the own arcs (HLIM:377-410): attribution
+        uint32_t fc = 0;
+        if (own) {
+#pragma unroll
+            for (int m = 0; m < 3; ++m) {
+                const int nb = e + (m == 0 ? 1 : (m == 1 ? iall + 1 : iall));
+                const int T = cmo[m] > 0.0 ? nb : e;
+                uint32_t kind = 0, slot = 0;
+                if ((bits >> m) & 1u) {
+                    int c4[4];
+                    ha_arc(m, e, iall, sgp, c4);
+                    double lo_v = Q(c4[0]), hi_v = lo_v;
+                    uint32_t il = 0, ih = 0;
+#pragma unroll
+                    for (int t = 1; t < 4; ++t) {
+                        const double v = Q(c4[t]);
+                        if (v < lo_v) { lo_v = v; il = t; }
+                        if (v > hi_v) { hi_v = v; ih = t; }
+                    }
+                    if (fout[m] == fq0[m]) kind = 1;
+                    else if (fout[m] == fq1[m]) kind = 2;
+                    else if (fout[m] == lo_v) { kind = 3; slot = il; }
+                    else { kind = 4; slot = ih; }
+                }
+                A.qab[(size_t)(m + 3) * plane + fb + nb] = 0.0;
+                A.qab[(size_t)m * plane + fb + e] = kind == 0 ? ob[m] : 0.0;
+                r_ob[m * R + se] = ob[m];
+                fc |= (kind | (slot << 3) | ((uint32_t)(T == nb) << 5)) << (6 * m);
+            }
+        }
+        r_fc[se] = fc;
+
+        // ---- the row of Qout (HLIM:216-301)
+        double part[7] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+        double chb[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+        if (!inner) {
+            part[3] = (0.0 + b0) + b1;  // halo: Qout = q
         } else {
-            int cmn[6], cmx[6];
-            double vmn[6], vmx[6], cc[6];
+            // the six arcs' bounds; the running extremes qnext, in face order from q(e), and the
+            // hexagon cells (3 bits per face) that hold each arc's min and max
+            const double qij = Q(e);
+            double nmin = qij, nmax = qij, cc[6];
+            double vmn[CHB ? 6 : 1], vmx[CHB ? 6 : 1];
+            int smin = -1, smax = -1;
+            uint32_t wmn = 0, wmx = 0;
 #pragma unroll
             for (int f = 0; f < 6; ++f) {
                 const int m = f < 3 ? f : f - 3;
@@ -605,32 +679,28 @@ __global__ void __launch_bounds__(HSW) __attribute__((amdgpu_waves_per_eu(4, 8))
                     if (v < lo_v) { lo_v = v; il = c4[t]; }
                     if (v > hi_v) { hi_v = v; ih = c4[t]; }
                 }
-                const double c = A.cmask[(size_t)m * plane + fb + owner];
-                cmn[f] = il;
-                cmx[f] = ih;
+                const double c = f < 3 ? cmo[m] : cmw[m];
                 cc[f] = f < 3 ? c : 1.0 - c;
+                double mn, mx;
                 if (f < 3) {
-                    vmn[f] = c * lo_v + (1.0 - c) * BIG;
-                    vmx[f] = c * hi_v + (1.0 - c) * (-BIG);
+                    mn = c * lo_v + (1.0 - c) * BIG;
+                    mx = c * hi_v + (1.0 - c) * (-BIG);
                 } else {
-                    vmn[f] = c * BIG + (1.0 - c) * lo_v;
-                    vmx[f] = c * (-BIG) + (1.0 - c) * hi_v;
+                    mn = c * BIG + (1.0 - c) * lo_v;
+                    mx = c * (-BIG) + (1.0 - c) * hi_v;
+                }
+                if (mn < nmin) { nmin = mn; smin = f; }
+                if (mx > nmax) { nmax = mx; smax = f; }
+                wmn |= hex_index(il - e, iall) << (3 * f);
+                wmx |= hex_index(ih - e, iall) << (3 * f);
+                if constexpr (CHB) {
+                    vmn[f] = mn;
+                    vmx[f] = mx;
                 }
             }
-            const double qij = Q(e);
-            int smin = -1, smax = -1;
-            double nmin = qij, nmax = qij;
+            double chm[6];
 #pragma unroll
-            for (int f = 0; f < 6; ++f) {
-                if (vmn[f] < nmin) { nmin = vmn[f]; smin = f; }
-                if (vmx[f] > nmax) { nmax = vmx[f]; smax = f; }
-            }
-            double chv[6], chm[6];
-#pragma unroll
-            for (int f = 0; f < 6; ++f) {
-                chv[f] = A.ch[(size_t)f * plane + fb + e];
-                chm[f] = dmn(chv[f], 0.0);
-            }
+            for (int f = 0; f < 6; ++f) chm[f] = dmn(chv[f], 0.0);
             double Cin = chm[0], Cout = chv[0] - chm[0];
 #pragma unroll
             for (int f = 1; f < 6; ++f) {
@@ -638,9 +708,9 @@ __global__ void __launch_bounds__(HSW) __attribute__((amdgpu_waves_per_eu(4, 8))
                 Cout = Cout + chv[f] - chm[f];
             }
             const double zerosw = 0.5 - sgn_half(fabs(Cout) - A.eps);
-            const double E = 1.0 - Cin - Cout + A.d[fb + e];
+            const double E = 1.0 - Cin - Cout + dc;
             const double fac = (1.0 - zerosw) / (Cout + zerosw);
-            if (A.chb) {
+            if constexpr (CHB) {
                 // Qout in ch: an inflow face (ch < 0) gives (qnext - Qin)/Cout, any other (qnext - Qout)/Cout
                 double CQmin = chm[0] * vmn[0], CQmax = chm[0] * vmx[0];
 #pragma unroll
@@ -655,79 +725,41 @@ __global__ void __launch_bounds__(HSW) __attribute__((amdgpu_waves_per_eu(4, 8))
                     const double d0 = zerosw != 0.0 ? 0.0 : (chv[f] < 0.0 ? nmax - vmx[f] : nmax - Qmin) / Cout;
                     const double d1 = zerosw != 0.0 ? 0.0 : (chv[f] < 0.0 ? nmin - vmn[f] : nmin - Qmax) / Cout;
                     // a zero cotangent contributes nothing (0 * an overflowed bound must not give NaN)
-                    A.chb[(size_t)f * plane + fb + e] = (b0 != 0.0 ? b0 * d0 : 0.0) + (b1 != 0.0 ? b1 * d1 : 0.0);
+                    chb[f] = (b0 != 0.0 ? b0 * d0 : 0.0) + (b1 != 0.0 ? b1 * d1 : 0.0);
                 }
             }
             // the row's terms, each into the partial sum of the hexagon cell it names (selects, not
             // indexing: an indexed register array would live in scratch)
-            double cf[6];
-#pragma unroll
-            for (int f = 0; f < 6; ++f) cf[f] = -(fac * chm[f]) * cc[f];
             const double c0 = fac + zerosw;
 #pragma unroll
             for (int mm = 0; mm < 2; ++mm) {
                 const int sel = mm == 0 ? smax : smin;
                 const double b = mm == 0 ? b0 : b1;
-                uint32_t wf[6];
-                uint32_t selw = 3u;
+                const uint32_t wb = mm == 0 ? wmx : wmn;
+                const uint32_t selw = sel < 0 ? 3u : (wb >> (3 * sel)) & 7u;
                 double selc = 1.0;
 #pragma unroll
-                for (int f = 0; f < 6; ++f) {
-                    wf[f] = hex_index((mm == 0 ? cmx[f] : cmn[f]) - e, iall);
-                    selw = sel == f ? wf[f] : selw;
-                    selc = sel == f ? cc[f] : selc;
-                }
+                for (int f = 0; f < 6; ++f) selc = sel == f ? cc[f] : selc;
                 part[3] = part[3] + c0 * b;
                 const double t1 = (-(fac * E) * selc) * b;
 #pragma unroll
                 for (int w = 0; w < 7; ++w) part[w] = selw == (uint32_t)w ? part[w] + t1 : part[w];
 #pragma unroll
                 for (int f = 0; f < 6; ++f) {
-                    const double t = cf[f] * b;
+                    const double t = (-(fac * chm[f]) * cc[f]) * b;
+                    const uint32_t wf = (wb >> (3 * f)) & 7u;
 #pragma unroll
                     for (int w = 0; w < 7; ++w)  // arc f's cells (any, for the singular-point row's f = 4)
-                        if (arc_has(f, w) || f == 4) part[w] = wf[f] == (uint32_t)w ? part[w] + t : part[w];
+                        if (arc_has(f, w) || f == 4) part[w] = wf == (uint32_t)w ? part[w] + t : part[w];
                 }
             }
         }
 #pragma unroll
         for (int w = 0; w < 7; ++w) r_p[w * R + se] = part[w];
-        // faces 1-3 of the own arcs (HLIM:377-410)
-        uint32_t fc = 0;
-        if (i <= iall - 2 && j <= iall - 2) {
-            const uint32_t bits = A.mask[fb + e];
+        if constexpr (CHB) {
 #pragma unroll
-            for (int m = 0; m < 3; ++m) {
-                const int nb = e + (m == 0 ? 1 : (m == 1 ? iall + 1 : iall));
-                const size_t fo = (size_t)m * plane + fb + e, fn = (size_t)(m + 3) * plane + fb + nb;
-                const double c = A.cmask[fo];
-                const int T = c > 0.0 ? nb : e;
-                const double ob = A.qab[fo] + A.qab[fn];
-                uint32_t kind = 0, slot = 0;
-                if ((bits >> m) & 1u) {
-                    const double out = A.q_a[fo];
-                    int c4[4];
-                    ha_arc(m, e, iall, sgp, c4);
-                    double lo_v = Q(c4[0]), hi_v = lo_v;
-                    uint32_t il = 0, ih = 0;
-#pragma unroll
-                    for (int t = 1; t < 4; ++t) {
-                        const double v = Q(c4[t]);
-                        if (v < lo_v) { lo_v = v; il = t; }
-                        if (v > hi_v) { hi_v = v; ih = t; }
-                    }
-                    if (out == A.Qpost[fb + T]) kind = 1;
-                    else if (out == A.Qpost[plane + fb + T]) kind = 2;
-                    else if (out == lo_v) { kind = 3; slot = il; }
-                    else { kind = 4; slot = ih; }
-                }
-                A.qab[fn] = 0.0;
-                A.qab[fo] = kind == 0 ? ob : 0.0;
-                r_ob[m * R + se] = ob;
-                fc |= (kind | (slot << 3) | ((uint32_t)(T == nb) << 5)) << (6 * m);
-            }
+            for (int f = 0; f < 6; ++f) A.chb[(size_t)f * plane + fb + e] = chb[f];
         }
-        r_fc[se] = fc;
     };
 
     // the cotangents of cell c: rows, then faces, in the sweep's order
@@ -1220,10 +1252,13 @@ int launch_hlim_adjoint(nicam_ctx c, double *qab, double *qab_pl, double *qb, do
     const long nslab = (long)D.kall * D.lall;
     if (nslab > 0x7fffffffL) return fail(NICAM_ERR_ARG, "grid too large for one launch");
     const size_t lds = (size_t)A.ring * (11 * sizeof(double) + sizeof(uint32_t));
+    const void *kern = chb ? (const void *)hlim_adjoint_kernel<true> : (const void *)hlim_adjoint_kernel<false>;
     if (lds > 64 * 1024)
-        NICAM_HIP(hipFuncSetAttribute((const void *)hlim_adjoint_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                      (int)lds));
-    hipLaunchKernelGGL(hlim_adjoint_kernel, dim3((unsigned)nslab), dim3(HSW), lds, c->stream, A);
+        NICAM_HIP(hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    if (chb)
+        hipLaunchKernelGGL(hlim_adjoint_kernel<true>, dim3((unsigned)nslab), dim3(HSW), lds, c->stream, A);
+    else
+        hipLaunchKernelGGL(hlim_adjoint_kernel<false>, dim3((unsigned)nslab), dim3(HSW), lds, c->stream, A);
     if (D.have_pl)
         hipLaunchKernelGGL(hlim_adjoint_pole, dim3((unsigned)((D.kall * kLallPl + HPOLE - 1) / HPOLE)), dim3(HPOLE),
                            0, c->stream, A);

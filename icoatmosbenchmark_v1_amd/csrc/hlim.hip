@@ -95,8 +95,11 @@ template <class T, bool MULTI, bool PRE = false>
 __device__ __forceinline__ void hlim_block(const HlimArgs<T> &A, int b)
 {
     // q for cells [g0 - iall - 1, g0 + CHUNK + iall + 2)
+    // MULTI keeps two tiles: tracer t+1's tile is loaded into registers while
+    // tracer t is limited, and written to the other tile after it
     constexpr int SPAN = CHUNK + 2 * MAXI + 3;
-    __shared__ T s_q[SPAN];
+    constexpr int NLD = (SPAN + CHUNK - 1) / CHUNK;
+    __shared__ T s_q[MULTI ? 2 : 1][SPAN];
 
     const int iall = A.iall, gall = A.gall, kall = A.kall, lall = A.lall;
     const int nchunk = (gall + CHUNK - 1) / CHUNK;
@@ -168,21 +171,37 @@ __device__ __forceinline__ void hlim_block(const HlimArgs<T> &A, int b)
         }
     }
 
+    T nxt[NLD];  // the next tracer's share of its tile
     for (int tr = 0; tr < (MULTI ? A.ntr : 1); ++tr) {
-        const T *q = A.q + tr * A.ts;
         T *q_a = A.q_a + tr * 6 * A.ts;
         T *Qprev = A.Qprev + tr * 2 * A.ts;
         const T *Qpost = A.Qpost + tr * 2 * A.ts;
-        if (tr > 0) __syncthreads();  // the previous tracer's tile has been read
-        for (int s = tid; s < n_in; s += CHUNK) {
-            const int gg = lo + s;
-            s_q[s] = (gg >= 0 && gg < gall) ? q[fb + gg] : T(0);
+        T *sq = s_q[MULTI ? (tr & 1) : 0];
+        // tile tr-2, the last one read from sq, was read before the previous barrier
+        if (tr == 0) {
+            const T *q = A.q;
+            for (int s = tid; s < n_in; s += CHUNK) {
+                const int gg = lo + s;
+                sq[s] = (gg >= 0 && gg < gall) ? q[fb + gg] : T(0);
+            }
+        } else {
+#pragma unroll
+            for (int u = 0; u < NLD; ++u)
+                if (tid + u * CHUNK < n_in) sq[tid + u * CHUNK] = nxt[u];
         }
         __syncthreads();
+        if (MULTI && tr + 1 < A.ntr) {
+            const T *qn = A.q + (tr + 1) * A.ts;
+#pragma unroll
+            for (int u = 0; u < NLD; ++u) {
+                const int gg = lo + tid + u * CHUNK;
+                nxt[u] = (tid + u * CHUNK < n_in && gg >= 0 && gg < gall) ? qn[fb + gg] : T(0);
+            }
+        }
         if (!active) continue;
 
         // tracer at absolute cell index p (must lie in the staged window)
-        auto Q = [&](int p) { return s_q[p - lo]; };
+        auto Q = [&](int p) { return sq[p - lo]; };
 
         // min/max of the 4 tracer values around arc m (0=AI,1=AIJ,2=AJ) with
         // origin p; HLIM:145-160 with the index clamps of :152-153 and the
