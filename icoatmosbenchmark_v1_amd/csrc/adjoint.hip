@@ -557,7 +557,7 @@ __global__ void __launch_bounds__(HPOLE) hlim_adjoint_pole(HaArgs A);
 // three and 2.25 ms at two, despite the registers the cap spills
 // CHB: also the cotangent of ch
 template <bool CHB>
-__global__ void __launch_bounds__(HSW) __attribute__((amdgpu_waves_per_eu(4, 8))) hlim_adjoint_kernel(HaArgs A)
+__global__ void __launch_bounds__(HSW) __attribute__((amdgpu_waves_per_eu(CHB ? 3 : 4, 8))) hlim_adjoint_kernel(HaArgs A)
 {
     extern __shared__ double lds[];
     const int iall = A.iall, gall = A.gall, kall = A.kall, lall = A.lall, R = A.ring;
@@ -654,25 +654,25 @@ __global__ void __launch_bounds__(HSW) __attribute__((amdgpu_waves_per_eu(4, 8))
 
         // ---- the row of Qout (HLIM:216-301)
         double part[7] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-        double chb[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
         if (!inner) {
             part[3] = (0.0 + b0) + b1;  // halo: Qout = q
+            if constexpr (CHB) {
+#pragma unroll
+                for (int f = 0; f < 6; ++f) A.chb[(size_t)f * plane + fb + e] = 0.0;
+            }
         } else {
             // the six arcs' bounds; the running extremes qnext, in face order from q(e), and the
             // hexagon cells (3 bits per face) that hold each arc's min and max
             const double qij = Q(e);
-            double nmin = qij, nmax = qij, cc[6];
-            double vmn[CHB ? 6 : 1], vmx[CHB ? 6 : 1];
-            int smin = -1, smax = -1;
-            uint32_t wmn = 0, wmx = 0;
-#pragma unroll
-            for (int f = 0; f < 6; ++f) {
+            auto ccf = [&](int f) { return f < 3 ? cmo[f] : 1.0 - cmw[f - 3]; };
+            auto bounds = [&](int f, double &mn, double &mx, int &il, int &ih) {
                 const int m = f < 3 ? f : f - 3;
                 const int owner = f < 3 ? e : e - (m == 0 ? 1 : (m == 1 ? iall + 1 : iall));
                 int c4[4];
                 ha_arc(m, owner, iall, sgp, c4);
                 double lo_v = Q(c4[0]), hi_v = lo_v;
-                int il = c4[0], ih = c4[0];
+                il = c4[0];
+                ih = c4[0];
 #pragma unroll
                 for (int t = 1; t < 4; ++t) {
                     const double v = Q(c4[t]);
@@ -680,8 +680,6 @@ __global__ void __launch_bounds__(HSW) __attribute__((amdgpu_waves_per_eu(4, 8))
                     if (v > hi_v) { hi_v = v; ih = c4[t]; }
                 }
                 const double c = f < 3 ? cmo[m] : cmw[m];
-                cc[f] = f < 3 ? c : 1.0 - c;
-                double mn, mx;
                 if (f < 3) {
                     mn = c * lo_v + (1.0 - c) * BIG;
                     mx = c * hi_v + (1.0 - c) * (-BIG);
@@ -689,18 +687,27 @@ __global__ void __launch_bounds__(HSW) __attribute__((amdgpu_waves_per_eu(4, 8))
                     mn = c * BIG + (1.0 - c) * lo_v;
                     mx = c * (-BIG) + (1.0 - c) * hi_v;
                 }
+            };
+            double chm[6];
+#pragma unroll
+            for (int f = 0; f < 6; ++f) chm[f] = dmn(chv[f], 0.0);
+            double nmin = qij, nmax = qij, CQmin = 0.0, CQmax = 0.0;
+            int smin = -1, smax = -1;
+            uint32_t wmn = 0, wmx = 0;
+#pragma unroll
+            for (int f = 0; f < 6; ++f) {
+                double mn, mx;
+                int il, ih;
+                bounds(f, mn, mx, il, ih);
                 if (mn < nmin) { nmin = mn; smin = f; }
                 if (mx > nmax) { nmax = mx; smax = f; }
                 wmn |= hex_index(il - e, iall) << (3 * f);
                 wmx |= hex_index(ih - e, iall) << (3 * f);
                 if constexpr (CHB) {
-                    vmn[f] = mn;
-                    vmx[f] = mx;
+                    CQmin = f == 0 ? chm[0] * mn : CQmin + chm[f] * mn;
+                    CQmax = f == 0 ? chm[0] * mx : CQmax + chm[f] * mx;
                 }
             }
-            double chm[6];
-#pragma unroll
-            for (int f = 0; f < 6; ++f) chm[f] = dmn(chv[f], 0.0);
             double Cin = chm[0], Cout = chv[0] - chm[0];
 #pragma unroll
             for (int f = 1; f < 6; ++f) {
@@ -711,21 +718,19 @@ __global__ void __launch_bounds__(HSW) __attribute__((amdgpu_waves_per_eu(4, 8))
             const double E = 1.0 - Cin - Cout + dc;
             const double fac = (1.0 - zerosw) / (Cout + zerosw);
             if constexpr (CHB) {
-                // Qout in ch: an inflow face (ch < 0) gives (qnext - Qin)/Cout, any other (qnext - Qout)/Cout
-                double CQmin = chm[0] * vmn[0], CQmax = chm[0] * vmx[0];
-#pragma unroll
-                for (int f = 1; f < 6; ++f) {
-                    CQmin = CQmin + chm[f] * vmn[f];
-                    CQmax = CQmax + chm[f] * vmx[f];
-                }
+                // Qout in ch: an inflow face (ch < 0) gives (qnext - Qin)/Cout, any other (qnext - Qout)/Cout;
+                // the arcs' bounds formed again from the tile rather than held
                 const double Qmin = (qij - CQmax - nmax * E) / (Cout + zerosw) * (1.0 - zerosw) + qij * zerosw;
                 const double Qmax = (qij - CQmin - nmin * E) / (Cout + zerosw) * (1.0 - zerosw) + qij * zerosw;
 #pragma unroll
                 for (int f = 0; f < 6; ++f) {
-                    const double d0 = zerosw != 0.0 ? 0.0 : (chv[f] < 0.0 ? nmax - vmx[f] : nmax - Qmin) / Cout;
-                    const double d1 = zerosw != 0.0 ? 0.0 : (chv[f] < 0.0 ? nmin - vmn[f] : nmin - Qmax) / Cout;
+                    double mn, mx;
+                    int il, ih;
+                    bounds(f, mn, mx, il, ih);
+                    const double d0 = zerosw != 0.0 ? 0.0 : (chv[f] < 0.0 ? nmax - mx : nmax - Qmin) / Cout;
+                    const double d1 = zerosw != 0.0 ? 0.0 : (chv[f] < 0.0 ? nmin - mn : nmin - Qmax) / Cout;
                     // a zero cotangent contributes nothing (0 * an overflowed bound must not give NaN)
-                    chb[f] = (b0 != 0.0 ? b0 * d0 : 0.0) + (b1 != 0.0 ? b1 * d1 : 0.0);
+                    A.chb[(size_t)f * plane + fb + e] = (b0 != 0.0 ? b0 * d0 : 0.0) + (b1 != 0.0 ? b1 * d1 : 0.0);
                 }
             }
             // the row's terms, each into the partial sum of the hexagon cell it names (selects, not
@@ -739,14 +744,14 @@ __global__ void __launch_bounds__(HSW) __attribute__((amdgpu_waves_per_eu(4, 8))
                 const uint32_t selw = sel < 0 ? 3u : (wb >> (3 * sel)) & 7u;
                 double selc = 1.0;
 #pragma unroll
-                for (int f = 0; f < 6; ++f) selc = sel == f ? cc[f] : selc;
+                for (int f = 0; f < 6; ++f) selc = sel == f ? ccf(f) : selc;
                 part[3] = part[3] + c0 * b;
                 const double t1 = (-(fac * E) * selc) * b;
 #pragma unroll
                 for (int w = 0; w < 7; ++w) part[w] = selw == (uint32_t)w ? part[w] + t1 : part[w];
 #pragma unroll
                 for (int f = 0; f < 6; ++f) {
-                    const double t = (-(fac * chm[f]) * cc[f]) * b;
+                    const double t = (-(fac * chm[f]) * ccf(f)) * b;
                     const uint32_t wf = (wb >> (3 * f)) & 7u;
 #pragma unroll
                     for (int w = 0; w < 7; ++w)  // arc f's cells (any, for the singular-point row's f = 4)
@@ -756,10 +761,6 @@ __global__ void __launch_bounds__(HSW) __attribute__((amdgpu_waves_per_eu(4, 8))
         }
 #pragma unroll
         for (int w = 0; w < 7; ++w) r_p[w * R + se] = part[w];
-        if constexpr (CHB) {
-#pragma unroll
-            for (int f = 0; f < 6; ++f) A.chb[(size_t)f * plane + fb + e] = chb[f];
-        }
     };
 
     // the cotangents of cell c: rows, then faces, in the sweep's order
@@ -795,8 +796,9 @@ __global__ void __launch_bounds__(HSW) __attribute__((amdgpu_waves_per_eu(4, 8))
                 if (kind >= 3) {
                     if (ws >= 0 && ((code >> 3) & 3u) == (uint32_t)ws) acc = acc + ob;
                 } else if ((code >> 5) ? t_nb : t_own) {
-                    if (kind == 1) pn = pn + ob;
-                    else px = px + ob;
+                    // selects: a branch on which sum to add to would keep the sums in scratch
+                    pn = kind == 1 ? pn + ob : pn;
+                    px = kind == 1 ? px : px + ob;
                 }
             }
         };
@@ -818,8 +820,8 @@ __global__ void __launch_bounds__(HSW) __attribute__((amdgpu_waves_per_eu(4, 8))
                 } else {
                     const int T = (code >> 5) ? p + (m == 0 ? 1 : (m == 1 ? iall + 1 : iall)) : p;
                     if (T == c) {
-                        if (kind == 1) pn = pn + ob;
-                        else px = px + ob;
+                        pn = kind == 1 ? pn + ob : pn;
+                        px = kind == 1 ? px : px + ob;
                     }
                 }
             }
