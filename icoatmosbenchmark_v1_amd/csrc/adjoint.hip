@@ -561,8 +561,11 @@ __global__ void __launch_bounds__(HSW) __attribute__((amdgpu_waves_per_eu(CHB ? 
 {
     extern __shared__ double lds[];
     const int iall = A.iall, gall = A.gall, kall = A.kall, lall = A.lall, R = A.ring;
-    double *r_q = lds;               // q
-    double *r_p = r_q + R;           // [7][R] the row's partial sums per hexagon cell
+    // q, with the first and last QP slots repeated past either end, so that a read within QP of a
+    // cell needs no wrap
+    const int QP = iall + 2;
+    double *r_q = lds + QP;
+    double *r_p = lds + R + 2 * QP;  // [7][R] the row's partial sums per hexagon cell
     double *r_ob = r_p + 7 * R;      // [3][R] face cotangents of the cell's own arcs
     uint32_t *r_fc = reinterpret_cast<uint32_t *>(r_ob + 3 * R);  // faces: attribution
     const int k = blockIdx.x % kall, l = blockIdx.x / kall;
@@ -573,15 +576,18 @@ __global__ void __launch_bounds__(HSW) __attribute__((amdgpu_waves_per_eu(CHB ? 
     const bool sgp = A.have_sgp[l] != 0;
     const double BIG = A.big;
 
+    // the hexagon index (3 bits per slot, in ha_arc's slot order) of the four cells of arc f of an
+    // inner cell's row: f0..2 its own arcs, f3 the arc AI of (i-1,j), f4 AIJ of (i-1,j-1), f5 AJ of
+    // (i,j-1); f4 of the singular-point row is {-iall-1, 0, 1, -1}
+    constexpr uint32_t HEXTAB[6] = {3u | 1u << 3 | 4u << 6 | 6u << 9, 3u | 4u << 3 | 6u << 6 | 5u << 9,
+                                    3u | 6u << 3 | 5u << 6 | 2u << 9, 2u | 0u << 3 | 3u << 6 | 5u << 9,
+                                    0u | 1u << 3 | 3u << 6 | 2u << 9, 1u | 4u << 3 | 3u << 6 | 0u << 9};
+    constexpr uint32_t HEXTAB4_SGP = 0u | 3u << 3 | 4u << 6 | 2u << 9;
+
     // one cell's row (if inner) or Qprev_b pass-through (halo), and its faces
     auto form = [&](int e) {
         const int se = e % R;
-        auto Q = [&](int x) {
-            int s = se + (x - e);
-            s += s < 0 ? R : 0;
-            s -= s >= R ? R : 0;
-            return r_q[s];
-        };
+        auto Q = [&](int x) { return r_q[se + (x - e)]; };  // x within QP of e
         const int i = e % iall, j = e / iall;
         const bool inner = i >= 1 && i <= iall - 2 && j >= 1 && j <= iall - 2;
         const bool own = i <= iall - 2 && j <= iall - 2;  // origin of three arcs
@@ -620,6 +626,32 @@ __global__ void __launch_bounds__(HSW) __attribute__((amdgpu_waves_per_eu(CHB ? 
             }
         }
 
+        // min, max of the four cells of arc f (in the forward's chain order) and the slot of each,
+        // the first in that order on a tie
+        auto arc = [&](int f, double &lo, double &hi, uint32_t &sl, uint32_t &sh) {
+            const int m = f < 3 ? f : f - 3;
+            const int owner = f < 3 ? e : e - (m == 0 ? 1 : (m == 1 ? iall + 1 : iall));
+            int c4[4];
+            ha_arc(m, owner, iall, sgp, c4);
+            lo = Q(c4[0]);
+            hi = lo;
+            sl = 0;
+            sh = 0;
+#pragma unroll
+            for (int t = 1; t < 4; ++t) {
+                const double v = Q(c4[t]);
+                if (v < lo) { lo = v; sl = t; }
+                if (v > hi) { hi = v; sh = t; }
+            }
+        };
+        double olo[3], ohi[3];  // the own arcs'
+        uint32_t osl[3], osh[3];
+#pragma unroll
+        for (int m = 0; m < 3; ++m) {
+            olo[m] = 0.0; ohi[m] = 0.0; osl[m] = 0; osh[m] = 0;
+            if (own) arc(m, olo[m], ohi[m], osl[m], osh[m]);
+        }
+
         // ---- faces 1-3 of the own arcs (HLIM:377-410): attribution
         uint32_t fc = 0;
         if (own) {
@@ -629,20 +661,10 @@ __global__ void __launch_bounds__(HSW) __attribute__((amdgpu_waves_per_eu(CHB ? 
                 const int T = cmo[m] > 0.0 ? nb : e;
                 uint32_t kind = 0, slot = 0;
                 if ((bits >> m) & 1u) {
-                    int c4[4];
-                    ha_arc(m, e, iall, sgp, c4);
-                    double lo_v = Q(c4[0]), hi_v = lo_v;
-                    uint32_t il = 0, ih = 0;
-#pragma unroll
-                    for (int t = 1; t < 4; ++t) {
-                        const double v = Q(c4[t]);
-                        if (v < lo_v) { lo_v = v; il = t; }
-                        if (v > hi_v) { hi_v = v; ih = t; }
-                    }
                     if (fout[m] == fq0[m]) kind = 1;
                     else if (fout[m] == fq1[m]) kind = 2;
-                    else if (fout[m] == lo_v) { kind = 3; slot = il; }
-                    else { kind = 4; slot = ih; }
+                    else if (fout[m] == olo[m]) { kind = 3; slot = osl[m]; }
+                    else { kind = 4; slot = osh[m]; }
                 }
                 A.qab[(size_t)(m + 3) * plane + fb + nb] = 0.0;
                 A.qab[(size_t)m * plane + fb + e] = kind == 0 ? ob[m] : 0.0;
@@ -652,115 +674,104 @@ __global__ void __launch_bounds__(HSW) __attribute__((amdgpu_waves_per_eu(CHB ? 
         }
         r_fc[se] = fc;
 
-        // ---- the row of Qout (HLIM:216-301)
-        double part[7] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+        // ---- the row of Qout (HLIM:216-301), summed per hexagon cell into the cell's seven slots of
+        // r_p; the slots belong to this work-item alone, and its LDS adds are done in program order
+        double *rp = r_p + se;
         if (!inner) {
-            part[3] = (0.0 + b0) + b1;  // halo: Qout = q
+#pragma unroll
+            for (int w = 0; w < 7; ++w) rp[w * R] = w == 3 ? (0.0 + b0) + b1 : 0.0;  // halo: Qout = q
             if constexpr (CHB) {
 #pragma unroll
                 for (int f = 0; f < 6; ++f) A.chb[(size_t)f * plane + fb + e] = 0.0;
             }
-        } else {
-            // the six arcs' bounds; the running extremes qnext, in face order from q(e), and the
-            // hexagon cells (3 bits per face) that hold each arc's min and max
-            const double qij = Q(e);
-            auto ccf = [&](int f) { return f < 3 ? cmo[f] : 1.0 - cmw[f - 3]; };
-            auto bounds = [&](int f, double &mn, double &mx, int &il, int &ih) {
-                const int m = f < 3 ? f : f - 3;
-                const int owner = f < 3 ? e : e - (m == 0 ? 1 : (m == 1 ? iall + 1 : iall));
-                int c4[4];
-                ha_arc(m, owner, iall, sgp, c4);
-                double lo_v = Q(c4[0]), hi_v = lo_v;
-                il = c4[0];
-                ih = c4[0];
+            return;
+        }
 #pragma unroll
-                for (int t = 1; t < 4; ++t) {
-                    const double v = Q(c4[t]);
-                    if (v < lo_v) { lo_v = v; il = c4[t]; }
-                    if (v > hi_v) { hi_v = v; ih = c4[t]; }
-                }
-                const double c = f < 3 ? cmo[m] : cmw[m];
-                if (f < 3) {
-                    mn = c * lo_v + (1.0 - c) * BIG;
-                    mx = c * hi_v + (1.0 - c) * (-BIG);
-                } else {
-                    mn = c * BIG + (1.0 - c) * lo_v;
-                    mx = c * (-BIG) + (1.0 - c) * hi_v;
-                }
-            };
-            double chm[6];
+        for (int w = 0; w < 7; ++w) rp[w * R] = 0.0;
+        const bool sgp_row = sgp && e == iall + 1;  // its f4 is the singular-point arc
+        // the six arcs' bounds (Qin); the running extremes qnext, in face order from q(e)
+        const double qij = Q(e);
+        auto ccf = [&](int f) { return f < 3 ? cmo[f] : 1.0 - cmw[f - 3]; };
+        auto bounds = [&](int f, double &mn, double &mx, uint32_t &wl, uint32_t &wh) {
+            double lo, hi;
+            uint32_t sl, sh;
+            if (f < 3) {
+                lo = olo[f]; hi = ohi[f]; sl = osl[f]; sh = osh[f];
+            } else {
+                arc(f, lo, hi, sl, sh);
+            }
+            const uint32_t tab = (f == 4 && sgp_row) ? HEXTAB4_SGP : HEXTAB[f];
+            wl = (tab >> (3 * sl)) & 7u;
+            wh = (tab >> (3 * sh)) & 7u;
+            const double c = f < 3 ? cmo[f] : cmw[f - 3];
+            if (f < 3) {
+                mn = c * lo + (1.0 - c) * BIG;
+                mx = c * hi + (1.0 - c) * (-BIG);
+            } else {
+                mn = c * BIG + (1.0 - c) * lo;
+                mx = c * (-BIG) + (1.0 - c) * hi;
+            }
+        };
+        double chm[6];
 #pragma unroll
-            for (int f = 0; f < 6; ++f) chm[f] = dmn(chv[f], 0.0);
-            double nmin = qij, nmax = qij, CQmin = 0.0, CQmax = 0.0;
-            int smin = -1, smax = -1;
-            uint32_t wmn = 0, wmx = 0;
+        for (int f = 0; f < 6; ++f) chm[f] = dmn(chv[f], 0.0);
+        double nmin = qij, nmax = qij, CQmin = 0.0, CQmax = 0.0;
+        int smin = -1, smax = -1;
+        uint32_t wmn = 0, wmx = 0;  // 3 bits per face: the hexagon cell of each arc's min, max
+#pragma unroll
+        for (int f = 0; f < 6; ++f) {
+            double mn, mx;
+            uint32_t wl, wh;
+            bounds(f, mn, mx, wl, wh);
+            if (mn < nmin) { nmin = mn; smin = f; }
+            if (mx > nmax) { nmax = mx; smax = f; }
+            wmn |= wl << (3 * f);
+            wmx |= wh << (3 * f);
+            if constexpr (CHB) {
+                CQmin = f == 0 ? chm[0] * mn : CQmin + chm[f] * mn;
+                CQmax = f == 0 ? chm[0] * mx : CQmax + chm[f] * mx;
+            }
+        }
+        double Cin = chm[0], Cout = chv[0] - chm[0];
+#pragma unroll
+        for (int f = 1; f < 6; ++f) {
+            Cin = Cin + chm[f];
+            Cout = Cout + chv[f] - chm[f];
+        }
+        const double zerosw = 0.5 - sgn_half(fabs(Cout) - A.eps);
+        const double E = 1.0 - Cin - Cout + dc;
+        const double fac = (1.0 - zerosw) / (Cout + zerosw);
+        if constexpr (CHB) {
+            // Qout in ch: an inflow face (ch < 0) gives (qnext - Qin)/Cout, any other (qnext - Qout)/Cout
+            const double Qmin = (qij - CQmax - nmax * E) / (Cout + zerosw) * (1.0 - zerosw) + qij * zerosw;
+            const double Qmax = (qij - CQmin - nmin * E) / (Cout + zerosw) * (1.0 - zerosw) + qij * zerosw;
 #pragma unroll
             for (int f = 0; f < 6; ++f) {
                 double mn, mx;
-                int il, ih;
-                bounds(f, mn, mx, il, ih);
-                if (mn < nmin) { nmin = mn; smin = f; }
-                if (mx > nmax) { nmax = mx; smax = f; }
-                wmn |= hex_index(il - e, iall) << (3 * f);
-                wmx |= hex_index(ih - e, iall) << (3 * f);
-                if constexpr (CHB) {
-                    CQmin = f == 0 ? chm[0] * mn : CQmin + chm[f] * mn;
-                    CQmax = f == 0 ? chm[0] * mx : CQmax + chm[f] * mx;
-                }
-            }
-            double Cin = chm[0], Cout = chv[0] - chm[0];
-#pragma unroll
-            for (int f = 1; f < 6; ++f) {
-                Cin = Cin + chm[f];
-                Cout = Cout + chv[f] - chm[f];
-            }
-            const double zerosw = 0.5 - sgn_half(fabs(Cout) - A.eps);
-            const double E = 1.0 - Cin - Cout + dc;
-            const double fac = (1.0 - zerosw) / (Cout + zerosw);
-            if constexpr (CHB) {
-                // Qout in ch: an inflow face (ch < 0) gives (qnext - Qin)/Cout, any other (qnext - Qout)/Cout;
-                // the arcs' bounds formed again from the tile rather than held
-                const double Qmin = (qij - CQmax - nmax * E) / (Cout + zerosw) * (1.0 - zerosw) + qij * zerosw;
-                const double Qmax = (qij - CQmin - nmin * E) / (Cout + zerosw) * (1.0 - zerosw) + qij * zerosw;
-#pragma unroll
-                for (int f = 0; f < 6; ++f) {
-                    double mn, mx;
-                    int il, ih;
-                    bounds(f, mn, mx, il, ih);
-                    const double d0 = zerosw != 0.0 ? 0.0 : (chv[f] < 0.0 ? nmax - mx : nmax - Qmin) / Cout;
-                    const double d1 = zerosw != 0.0 ? 0.0 : (chv[f] < 0.0 ? nmin - mn : nmin - Qmax) / Cout;
-                    // a zero cotangent contributes nothing (0 * an overflowed bound must not give NaN)
-                    A.chb[(size_t)f * plane + fb + e] = (b0 != 0.0 ? b0 * d0 : 0.0) + (b1 != 0.0 ? b1 * d1 : 0.0);
-                }
-            }
-            // the row's terms, each into the partial sum of the hexagon cell it names (selects, not
-            // indexing: an indexed register array would live in scratch)
-            const double c0 = fac + zerosw;
-#pragma unroll
-            for (int mm = 0; mm < 2; ++mm) {
-                const int sel = mm == 0 ? smax : smin;
-                const double b = mm == 0 ? b0 : b1;
-                const uint32_t wb = mm == 0 ? wmx : wmn;
-                const uint32_t selw = sel < 0 ? 3u : (wb >> (3 * sel)) & 7u;
-                double selc = 1.0;
-#pragma unroll
-                for (int f = 0; f < 6; ++f) selc = sel == f ? ccf(f) : selc;
-                part[3] = part[3] + c0 * b;
-                const double t1 = (-(fac * E) * selc) * b;
-#pragma unroll
-                for (int w = 0; w < 7; ++w) part[w] = selw == (uint32_t)w ? part[w] + t1 : part[w];
-#pragma unroll
-                for (int f = 0; f < 6; ++f) {
-                    const double t = (-(fac * chm[f]) * ccf(f)) * b;
-                    const uint32_t wf = (wb >> (3 * f)) & 7u;
-#pragma unroll
-                    for (int w = 0; w < 7; ++w)  // arc f's cells (any, for the singular-point row's f = 4)
-                        if (arc_has(f, w) || f == 4) part[w] = wf == (uint32_t)w ? part[w] + t : part[w];
-                }
+                uint32_t wl, wh;
+                bounds(f, mn, mx, wl, wh);
+                const double d0 = zerosw != 0.0 ? 0.0 : (chv[f] < 0.0 ? nmax - mx : nmax - Qmin) / Cout;
+                const double d1 = zerosw != 0.0 ? 0.0 : (chv[f] < 0.0 ? nmin - mn : nmin - Qmax) / Cout;
+                // a zero cotangent contributes nothing (0 * an overflowed bound must not give NaN)
+                A.chb[(size_t)f * plane + fb + e] = (b0 != 0.0 ? b0 * d0 : 0.0) + (b1 != 0.0 ? b1 * d1 : 0.0);
             }
         }
+        // the row's terms, each added to the slot of the hexagon cell it names
+        const double c0 = fac + zerosw;
 #pragma unroll
-        for (int w = 0; w < 7; ++w) r_p[w * R + se] = part[w];
+        for (int mm = 0; mm < 2; ++mm) {
+            const int sel = mm == 0 ? smax : smin;
+            const double b = mm == 0 ? b0 : b1;
+            const uint32_t wb = mm == 0 ? wmx : wmn;
+            const uint32_t selw = sel < 0 ? 3u : (wb >> (3 * sel)) & 7u;
+            double selc = 1.0;
+#pragma unroll
+            for (int f = 0; f < 6; ++f) selc = sel == f ? ccf(f) : selc;
+            atomicAdd(&rp[3 * R], c0 * b);
+            atomicAdd(&rp[selw * R], (-(fac * E) * selc) * b);
+#pragma unroll
+            for (int f = 0; f < 6; ++f) atomicAdd(&rp[((wb >> (3 * f)) & 7u) * R], (-(fac * chm[f]) * ccf(f)) * b);
+        }
     };
 
     // the cotangents of cell c: rows, then faces, in the sweep's order
@@ -839,14 +850,21 @@ __global__ void __launch_bounds__(HSW) __attribute__((amdgpu_waves_per_eu(CHB ? 
         A.Qpostb[plane + fb + c] = px;
     };
 
+    // q of cell e into its slot, and into the slot's copy past the ring's other end
+    auto qput = [&](int e, double v) {
+        const int se = e % R;
+        r_q[se] = v;
+        if (se < QP) r_q[se + R] = v;
+        if (se >= R - QP) r_q[se - R] = v;
+    };
     // prologue: q of cells [0, 2*iall+2], rows and faces of cells [0, iall]
-    for (int e = tid; e <= 2 * iall + 2 && e < gall; e += HSW) r_q[e % R] = A.q[fb + e];
+    for (int e = tid; e <= 2 * iall + 2 && e < gall; e += HSW) qput(e, A.q[fb + e]);
     __syncthreads();
     for (int e = tid; e <= iall && e < gall; e += HSW) form(e);
     __syncthreads();
     for (int b0 = 0; b0 < gall; b0 += HSW) {
         const int eq = b0 + 2 * iall + 3 + tid;
-        if (eq < gall) r_q[eq % R] = A.q[fb + eq];
+        if (eq < gall) qput(eq, A.q[fb + eq]);
         __syncthreads();
         const int ef = b0 + iall + 1 + tid;
         if (ef < gall) form(ef);
@@ -1017,23 +1035,29 @@ __global__ void __launch_bounds__(FSW) hflux_adjoint_kernel(FaArgs A)
     const int iall = A.iall, gall = A.gall, kall = A.kall, lall = A.lall, R = A.ring;
     const int k = blockIdx.x % kall, l = blockIdx.x / kall;
     if (l >= lall) return;
-    double *s_a = lds;  // [3 arcs][4 fields][R]
+    double *s_a = lds;           // [3 arcs][4 fields][R] arc cotangents of origins [b0 - iall - 1, b0 + FSW + iall - 1]
+    double *s_x = lds + 12 * R;  // [4 fields][R] rho, rho*v of cells [b0, b0 + FSW + 2 iall + 1)
     const size_t fb = ((size_t)l * kall + k) * gall;
     const size_t plane = (size_t)lall * kall * gall;
     const size_t mstr = (size_t)lall * gall, mb = (size_t)l * gall;
     const bool sgp = A.have_sgp[l] != 0;
     const int tid = threadIdx.x;
     auto W = [&](int c, int t, int w) { return A.gt[(size_t)(w * 2 + t) * mstr + mb + c]; };  // w: 2,3,4 = W1..W3
-    auto X = [&](int f, int p) { return A.X[f][fb + p]; };
-    auto tri = [&](int f, int t, int c) {  // forward triangle sums (FLX:139-201)
-        if (t == 0 && sgp && c == 0) t = 1, c = 1;
-        if (t == 0) return X(f, c) * W(c, 0, 2) + X(f, c + 1) * W(c, 0, 3) + X(f, c + iall + 1) * W(c, 0, 4);
-        return X(f, c) * W(c, 1, 2) + X(f, c + iall + 1) * W(c, 1, 3) + X(f, c + iall) * W(c, 1, 4);
-    };
     const double dt = A.dt, EPS = A.eps;
     // the three arcs of origin a0 (zero where a0 is no origin)
     auto arcs = [&](int a0) {
-        const int s = a0 % R;
+        const int s = a0 % R;  // a0's slot in both rings
+        auto X = [&](int f, int p) {  // p within [a0 - iall, a0 + iall + 1]
+            int q = s + (p - a0);
+            q += q < 0 ? R : 0;
+            q -= q >= R ? R : 0;
+            return s_x[f * R + q];
+        };
+        auto tri = [&](int f, int t, int c) {  // forward triangle sums (FLX:139-201)
+            if (t == 0 && sgp && c == 0) t = 1, c = 1;
+            if (t == 0) return X(f, c) * W(c, 0, 2) + X(f, c + 1) * W(c, 0, 3) + X(f, c + iall + 1) * W(c, 0, 4);
+            return X(f, c) * W(c, 1, 2) + X(f, c + iall + 1) * W(c, 1, 3) + X(f, c + iall) * W(c, 1, 4);
+        };
         const int i = a0 % iall, j = a0 / iall;
         const bool org = a0 < gall && i <= iall - 2 && j <= iall - 2;
         // the four triangles the origin's arcs use: TI(a0), TJ(a0), TJ(a0-iall) (AI), TI(a0-1) (AJ).
@@ -1097,8 +1121,29 @@ __global__ void __launch_bounds__(FSW) hflux_adjoint_kernel(FaArgs A)
             for (int f = 0; f < 4; ++f) s_a[((a - 1) * 4 + f) * R + s] = bv[f];
         }
     };
+    // triangle origins of cell X0's six triangles, and whether each is one (i, j in [0, iall-2])
+    auto origins = [&](int X0, bool &o_sw, bool &o_s, bool &o_w, bool &o_c) {
+        const int i0 = X0 % iall, j0 = X0 / iall;
+        o_sw = i0 >= 1 && j0 >= 1 && i0 - 1 <= iall - 2 && j0 - 1 <= iall - 2;  // X0-iall-1
+        o_s = j0 >= 1 && i0 <= iall - 2 && j0 - 1 <= iall - 2;                  // X0-iall
+        o_w = i0 >= 1 && i0 - 1 <= iall - 2 && j0 <= iall - 2;                  // X0-1
+        o_c = i0 <= iall - 2 && j0 <= iall - 2;                                 // X0
+    };
+    // the weights of the cell in its six triangles (loaded a step ahead of their use)
+    auto cellw = [&](int X0, double w[6]) {
+        bool o_sw, o_s, o_w, o_c;
+        origins(X0, o_sw, o_s, o_w, o_c);
+        const bool in = X0 < gall;
+        const int c_sw = X0 - iall - 1, c_s = X0 - iall, c_w = X0 - 1;
+        w[0] = in && o_sw ? W(c_sw, 0, 4) : 0.0;
+        w[1] = in && o_sw ? W(c_sw, 1, 3) : 0.0;
+        w[2] = in && o_s ? W(c_s, 1, 4) : 0.0;
+        w[3] = in && o_w ? W(c_w, 0, 3) : 0.0;
+        w[4] = in && o_c ? W(X0, 0, 2) : 0.0;
+        w[5] = in && o_c ? W(X0, 1, 2) : 0.0;
+    };
     // the cell's four cotangents from its six triangles
-    auto cell = [&](int X0) {
+    auto cell = [&](int X0, const double w[6]) {
         const int sc = X0 % R;
         auto AB = [&](int a, int f, int c) {
             int q = sc + (c - X0);
@@ -1106,16 +1151,10 @@ __global__ void __launch_bounds__(FSW) hflux_adjoint_kernel(FaArgs A)
             q -= q >= R ? R : 0;
             return s_a[((a - 1) * 4 + f) * R + q];
         };
-        // triangle origins of the cell's six triangles, and whether each is one (i, j in [0, iall-2])
-        const int i0 = X0 % iall, j0 = X0 / iall;
-        const bool o_sw = i0 >= 1 && j0 >= 1 && i0 - 1 <= iall - 2 && j0 - 1 <= iall - 2;  // X0-iall-1
-        const bool o_s = j0 >= 1 && i0 <= iall - 2 && j0 - 1 <= iall - 2;                  // X0-iall
-        const bool o_w = i0 >= 1 && i0 - 1 <= iall - 2 && j0 <= iall - 2;                  // X0-1
-        const bool o_c = i0 <= iall - 2 && j0 <= iall - 2;                                 // X0
+        bool o_sw, o_s, o_w, o_c;
+        origins(X0, o_sw, o_s, o_w, o_c);
         const int c_sw = X0 - iall - 1, c_s = X0 - iall, c_w = X0 - 1;
-        const double w_sw_i = o_sw ? W(c_sw, 0, 4) : 0.0, w_sw_j = o_sw ? W(c_sw, 1, 3) : 0.0;
-        const double w_s_j = o_s ? W(c_s, 1, 4) : 0.0, w_w_i = o_w ? W(c_w, 0, 3) : 0.0;
-        const double w_c_i = o_c ? W(X0, 0, 2) : 0.0, w_c_j = o_c ? W(X0, 1, 2) : 0.0;
+        const double w_sw_i = w[0], w_sw_j = w[1], w_s_j = w[2], w_w_i = w[3], w_c_i = w[4], w_c_j = w[5];
         // triangle cotangents: TI(c) from AI(c), AIJ(c), AJ(c+1); TJ(c) from AI(c+iall), AIJ(c), AJ(c)
         auto tib0 = [&](int f, int c) { return ((0.0 + AB(1, f, c)) + AB(2, f, c)) + AB(3, f, c + 1); };
         auto tjb0 = [&](int f, int c) { return ((0.0 + AB(1, f, c + iall)) + AB(2, f, c)) + AB(3, f, c); };
@@ -1131,12 +1170,35 @@ __global__ void __launch_bounds__(FSW) hflux_adjoint_kernel(FaArgs A)
             A.Xb[f][fb + X0] = y;
         }
     };
-    // origins [b0 - iall - 1, b0 + FSW + iall - 1] are in the ring while cells [b0, b0 + FSW) gather
+    // rho, rho*v of cell p into its slot of the ring (zero beyond the region)
+    auto xload = [&](int p, double v[4]) {
+#pragma unroll
+        for (int f = 0; f < 4; ++f) v[f] = p < gall ? A.X[f][fb + p] : 0.0;
+    };
+    auto xstore = [&](int p, const double v[4]) {
+        const int sp = p % R;
+#pragma unroll
+        for (int f = 0; f < 4; ++f) s_x[f * R + sp] = v[f];
+    };
+    for (int p = tid; p < R; p += FSW) {
+        double v[4];
+        xload(p, v);
+        xstore(p, v);
+    }
+    __syncthreads();
+    // origins [b0 - iall - 1, b0 + FSW + iall - 1] are in the arc ring while cells [b0, b0 + FSW) gather;
+    // each step replaces the cells [b0, b0 + FSW) of the cell ring, read by its arcs only, with the next
+    // step's, loaded while the arcs are formed
     for (int e = tid; e < iall; e += FSW) arcs(e);
     for (int b0 = 0; b0 < gall; b0 += FSW) {
+        double xn[4], w[6];
+        const int pn = b0 + R + tid;
+        xload(pn, xn);
+        cellw(b0 + tid, w);
         arcs(b0 + iall + tid);
         __syncthreads();
-        if (b0 + tid < gall) cell(b0 + tid);
+        if (b0 + tid < gall) cell(b0 + tid, w);
+        xstore(pn, xn);
         __syncthreads();
     }
 }
@@ -1218,7 +1280,7 @@ int launch_hflux_adjoint(nicam_ctx c, const double *flxb, const double *flxb_pl,
     A.ring = FSW + 2 * D.gall_1d + 1;
     const long nblk = (long)D.kall * D.lall;
     if (nblk > 0x7fffffffL) return fail(NICAM_ERR_ARG, "grid too large for one launch");
-    const size_t lds = sizeof(double) * 12 * (size_t)A.ring;
+    const size_t lds = sizeof(double) * 16 * (size_t)A.ring;
     if (lds > 64 * 1024)
         NICAM_HIP(hipFuncSetAttribute((const void *)hflux_adjoint_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
                                       (int)lds));
@@ -1253,7 +1315,8 @@ int launch_hlim_adjoint(nicam_ctx c, double *qab, double *qab_pl, double *qb, do
     A.ring = HSW + 2 * D.gall_1d + 3;
     const long nslab = (long)D.kall * D.lall;
     if (nslab > 0x7fffffffL) return fail(NICAM_ERR_ARG, "grid too large for one launch");
-    const size_t lds = (size_t)A.ring * (11 * sizeof(double) + sizeof(uint32_t));
+    const size_t lds = (size_t)(A.ring + 2 * (D.gall_1d + 2)) * sizeof(double) +
+                       (size_t)A.ring * (10 * sizeof(double) + sizeof(uint32_t));
     const void *kern = chb ? (const void *)hlim_adjoint_kernel<true> : (const void *)hlim_adjoint_kernel<false>;
     if (lds > 64 * 1024)
         NICAM_HIP(hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
