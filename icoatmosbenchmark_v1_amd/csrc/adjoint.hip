@@ -784,32 +784,34 @@ __global__ void __launch_bounds__(HSW) __attribute__((amdgpu_waves_per_eu(CHB ? 
             return s;
         };
         const int hx[7] = {-iall - 1, -iall, -1, 0, 1, iall, iall + 1};
+        // Sums start at +0.0 and so never hold -0.0: adding +0.0 in place of a term that does not
+        // apply leaves them unchanged, which lets every term below be a select rather than a branch.
         double acc = 0.0;
 #pragma unroll
         for (int h = 0; h < 7; ++h) {  // the rows of c's hexagon, in cell order: their partial sums for c
             const int g = c + hx[h];
-            if (g >= 0 && g < gall) acc = acc + r_p[(6 - h) * R + sl(g)];
+            const double v = r_p[(6 - h) * R + sl(g)];
+            acc = acc + ((g >= 0 && g < gall) ? v : 0.0);
         }
         double pn = 0.0, px = 0.0;
         // faces of origin p; d: index of c - p (arc_slot's), known after unrolling
         auto faces_at = [&](int p, int d) {
-            if (p < 0 || p >= gall) return;  // cells that are no origins carry no faces (code 0)
             const int s = sl(p);
-            const uint32_t fc = r_fc[s];
+            const uint32_t fc = (p >= 0 && p < gall) ? r_fc[s] : 0u;  // no origin, no faces (code 0)
 #pragma unroll
             for (int m = 0; m < 3; ++m) {
-                const int ws = (m == 1 && sgp && p == 0) ? arc_slot_sgp(d) : arc_slot(m, d);
+                const int ws0 = arc_slot(m, d);
+                const int ws = (m == 1 && arc_slot_sgp(d) != ws0) ? ((sgp && p == 0) ? arc_slot_sgp(d) : ws0) : ws0;
                 const bool t_own = d == 4, t_nb = d == (m == 0 ? 3 : (m == 1 ? 1 : 2));
-                if (ws < 0 && !t_own && !t_nb) continue;
+                if ((m != 1 || arc_slot_sgp(d) < 0) && ws0 < 0 && !t_own && !t_nb) continue;
                 const uint32_t code = (fc >> (6 * m)) & 63u, kind = code & 7u;
-                if (kind == 0) continue;
                 const double ob = r_ob[m * R + s];
-                if (kind >= 3) {
-                    if (ws >= 0 && ((code >> 3) & 3u) == (uint32_t)ws) acc = acc + ob;
-                } else if ((code >> 5) ? t_nb : t_own) {
-                    // selects: a branch on which sum to add to would keep the sums in scratch
-                    pn = kind == 1 ? pn + ob : pn;
-                    px = kind == 1 ? px : px + ob;
+                const bool hit = kind >= 3 && ws >= 0 && ((code >> 3) & 3u) == (uint32_t)ws;
+                acc = acc + (hit ? ob : 0.0);
+                if (t_own || t_nb) {
+                    const bool at = (code >> 5) ? t_nb : t_own;
+                    pn = pn + ((kind == 1 && at) ? ob : 0.0);
+                    px = px + ((kind == 2 && at) ? ob : 0.0);
                 }
             }
         };
