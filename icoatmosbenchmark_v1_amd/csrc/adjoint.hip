@@ -44,89 +44,211 @@ constexpr int BLOCK_VA = 256;
 template <bool FACT>
 __global__ void __launch_bounds__(BLOCK_VA) vi_adjoint_kernel(VaArgs A)
 {
-    const long t = (long)blockIdx.x * BLOCK_VA + threadIdx.x;
-    const long nreg = (long)A.gall * A.lall;
-    const VaField *F;
-    size_t base, s;
-    if (t < nreg) {
-        F = &A.reg;
-        base = (size_t)(t / A.gall) * A.kall * A.gall + (size_t)(t % A.gall);
-        s = A.gall;
-    } else if (A.have_pl && t < nreg + (long)A.gall_pl * kLallPl) {
-        const long p = t - nreg;
-        F = &A.pl;
-        base = (size_t)(p / A.gall_pl) * A.kall * A.gall_pl + (size_t)(p % A.gall_pl);
-        s = A.gall_pl;
-    } else {
-        return;
-    }
-    const int kall = A.kall, kmin = 1, kmax = kall - 2;  // 0-based
-    const double dt = A.dt, CV = A.CVovRt2, grav = A.grav;
-    auto at = [&](int k) { return base + (size_t)k * s; };
+    column_run<BLOCK_VA>(A, blockIdx.x, [&](const VaField &F, size_t base, size_t s) {
+        const int kall = A.kall, kmin = 1, kmax = kall - 2;  // 0-based
+        const double dt = A.dt, CV = A.CVovRt2, grav = A.grav;
+        auto at = [&](int k) { return base + (size_t)k * s; };
 
-    // transposed elimination on kmin+1..kmax
-    double beta = FACT ? F->tbeta[at(kmin + 1)] : F->Mc[at(kmin + 1)];
-    double yp = (F->wb[at(kmin + 1)] * F->GSGAM2H[at(kmin + 1)]) / beta;
-    F->w0b[at(kmin + 1)] = yp;
-    for (int k = kmin + 2; k <= kmax; ++k) {
-        const double mu_km1 = F->Mu[at(k - 1)];
-        if constexpr (FACT) {
-            beta = F->tbeta[at(k)];
-        } else {
-            const double g = F->Ml[at(k)] / beta;
-            beta = F->Mc[at(k)] - mu_km1 * g;
-            F->Srb[at(k)] = g;
+        // transposed elimination on kmin+1..kmax
+        double beta = FACT ? F.tbeta[at(kmin + 1)] : F.Mc[at(kmin + 1)];
+        double yp = (F.wb[at(kmin + 1)] * F.GSGAM2H[at(kmin + 1)]) / beta;
+        F.w0b[at(kmin + 1)] = yp;
+        for (int k = kmin + 2; k <= kmax; ++k) {
+            const double mu_km1 = F.Mu[at(k - 1)];
+            if constexpr (FACT) {
+                beta = F.tbeta[at(k)];
+            } else {
+                const double g = F.Ml[at(k)] / beta;
+                beta = F.Mc[at(k)] - mu_km1 * g;
+                F.Srb[at(k)] = g;
+            }
+            yp = (F.wb[at(k)] * F.GSGAM2H[at(k)] - mu_km1 * yp) / beta;
+            F.w0b[at(k)] = yp;
         }
-        yp = (F->wb[at(k)] * F->GSGAM2H[at(k)] - mu_km1 * yp) / beta;
-        F->w0b[at(k)] = yp;
-    }
-    const double y_kmax = yp;
+        const double y_kmax = yp;
 
-    // back substitution fused with the cotangents of each level, top down
-    double yn = 0.0, g_k1 = 0.0;
-    for (int k = kall - 1; k >= 0; --k) {
-        const bool in_k = k >= kmin + 1 && k <= kmax, in_n = k + 1 >= kmin + 1 && k + 1 <= kmax;
-        const size_t o = at(k);
-        double yk = 0.0, gk = 0.0;
-        if (in_k) {
-            const double ypk = F->w0b[o];
-            yk = (k == kmax) ? ypk : ypk - g_k1 * yn;
-            if (k >= kmin + 2) gk = FACT ? F->tgam[o] : F->Srb[o];
+        // back substitution fused with the cotangents of each level, top down
+        double yn = 0.0, g_k1 = 0.0;
+        for (int k = kall - 1; k >= 0; --k) {
+            const bool in_k = k >= kmin + 1 && k <= kmax, in_n = k + 1 >= kmin + 1 && k + 1 <= kmax;
+            const size_t o = at(k);
+            double yk = 0.0, gk = 0.0;
+            if (in_k) {
+                const double ypk = F.w0b[o];
+                yk = (k == kmax) ? ypk : ypk - g_k1 * yn;
+                if (k >= kmin + 2) gk = FACT ? F.tgam[o] : F.Srb[o];
+            }
+            const double sb = in_k ? yk * CV : 0.0;
+            const double sbn = in_n ? yn * CV : 0.0;
+            double pb = 0.0, rb = 0.0;
+            if (in_k) {
+                pb = -(sb * dt * A.rdgzh[k]);
+                rb = -(sb * dt * grav * A.afact[k]);
+            }
+            if (in_n) {
+                pb = pb + sbn * dt * A.rdgzh[k + 1];
+                rb = rb - sbn * dt * grav * A.bfact[k + 1];
+            }
+            const double p0b = pb * F.RGSGAM2[o];
+            const double rg = F.RGAM[o];
+            const double r0b = rb * (rg * rg);
+            const double rgh = F.RGAMH[o];
+            const double sw = in_k ? sb * (rgh * rgh) : 0.0;
+            F.p0b[o] = p0b;
+            F.Spb[o] = dt * p0b;
+            F.r0b[o] = r0b;
+            F.Srb[o] = dt * r0b;
+            F.w0b[o] = sw * A.alpha;
+            F.Swb[o] = dt * sw;
+            if (k == kmin) {
+                const double bb = F.wb[o] * F.GSGAM2H[o] - F.Ml[at(kmin + 1)] * yn;
+                F.wb[o] = bb * F.RGSGAM2H[o];
+            } else if (k == kmax + 1) {
+                const double bt = F.wb[o] * F.GSGAM2H[o] - F.Mu[at(kmax)] * y_kmax;
+                F.wb[o] = bt * F.RGSGAM2H[o];
+            } else if (in_k) {
+                F.wb[o] = 0.0;
+            }
+            yn = yk;
+            g_k1 = gk;
         }
-        const double sb = in_k ? yk * CV : 0.0;
-        const double sbn = in_n ? yn * CV : 0.0;
-        double pb = 0.0, rb = 0.0;
-        if (in_k) {
-            pb = -(sb * dt * A.rdgzh[k]);
-            rb = -(sb * dt * grav * A.afact[k]);
+    });
+}
+
+// The factored column in chunks of VC levels: each chunk's inputs are loaded
+// before the previous chunk's outputs are stored, so a column waits for
+// memory once per chunk rather than once per level (a launch holds about
+// three waves per SIMD, too few to hide a load per level).  The operations
+// and their order are those of vi_adjoint_kernel<true>.
+constexpr int VC = 8;
+
+__global__ void __launch_bounds__(BLOCK_VA) vi_adjoint_kernel_chunked(VaArgs A)
+{
+    column_run<BLOCK_VA>(A, blockIdx.x, [&](const VaField &F, size_t base, size_t s) {
+        const int kall = A.kall, kmin = 1, kmax = kall - 2;  // 0-based
+        const double dt = A.dt, CV = A.CVovRt2, grav = A.grav;
+        auto at = [&](int k) { return base + (size_t)k * s; };
+
+        // transposed elimination on kmin+1..kmax
+        double beta = F.tbeta[at(kmin + 1)];
+        double yp = (F.wb[at(kmin + 1)] * F.GSGAM2H[at(kmin + 1)]) / beta;
+        F.w0b[at(kmin + 1)] = yp;
+        {
+            double mu[VC], tb[VC], wv[VC], gv[VC];
+            auto load = [&](int k0, double *m_, double *t_, double *w_, double *g_) {
+    #pragma unroll
+                for (int c = 0; c < VC; ++c) {
+                    const int k = k0 + c;
+                    if (k <= kmax) {
+                        m_[c] = F.Mu[at(k - 1)];
+                        t_[c] = F.tbeta[at(k)];
+                        w_[c] = F.wb[at(k)];
+                        g_[c] = F.GSGAM2H[at(k)];
+                    }
+                }
+            };
+            load(kmin + 2, mu, tb, wv, gv);
+            for (int k0 = kmin + 2; k0 <= kmax; k0 += VC) {
+                double nmu[VC], ntb[VC], nwv[VC], ngv[VC];
+                if (k0 + VC <= kmax) load(k0 + VC, nmu, ntb, nwv, ngv);
+    #pragma unroll
+                for (int c = 0; c < VC; ++c) {
+                    const int k = k0 + c;
+                    if (k <= kmax) {
+                        yp = (wv[c] * gv[c] - mu[c] * yp) / tb[c];
+                        F.w0b[at(k)] = yp;
+                    }
+                }
+    #pragma unroll
+                for (int c = 0; c < VC; ++c) {
+                    mu[c] = nmu[c];
+                    tb[c] = ntb[c];
+                    wv[c] = nwv[c];
+                    gv[c] = ngv[c];
+                }
+            }
         }
-        if (in_n) {
-            pb = pb + sbn * dt * A.rdgzh[k + 1];
-            rb = rb - sbn * dt * grav * A.bfact[k + 1];
+        const double y_kmax = yp;
+
+        // back substitution fused with the cotangents of each level, top down, VC levels a chunk
+        double yn = 0.0, g_k1 = 0.0;
+        double w0[VC], tg[VC], rs[VC], rgv[VC], rhv[VC], rz[VC], rzn[VC], af[VC], bfn[VC];
+        auto load = [&](int k0, double *w0_, double *tg_, double *rs_, double *rg_, double *rh_, double *rz_,
+                        double *rzn_, double *af_, double *bfn_) {
+    #pragma unroll
+            for (int c = 0; c < VC; ++c) {
+                const int k = k0 - c;
+                if (k >= 0) {
+                    const size_t o = at(k);
+                    const bool in_k = k >= kmin + 1 && k <= kmax, in_n = k + 1 >= kmin + 1 && k + 1 <= kmax;
+                    w0_[c] = in_k ? F.w0b[o] : 0.0;
+                    tg_[c] = (in_k && k >= kmin + 2) ? F.tgam[o] : 0.0;
+                    rs_[c] = F.RGSGAM2[o];
+                    rg_[c] = F.RGAM[o];
+                    rh_[c] = F.RGAMH[o];
+                    rz_[c] = in_k ? A.rdgzh[k] : 0.0;
+                    af_[c] = in_k ? A.afact[k] : 0.0;
+                    rzn_[c] = in_n ? A.rdgzh[k + 1] : 0.0;
+                    bfn_[c] = in_n ? A.bfact[k + 1] : 0.0;
+                }
+            }
+        };
+        load(kall - 1, w0, tg, rs, rgv, rhv, rz, rzn, af, bfn);
+        for (int k0 = kall - 1; k0 >= 0; k0 -= VC) {
+            double nw0[VC], ntg[VC], nrs[VC], nrg[VC], nrh[VC], nrz[VC], nrzn[VC], naf[VC], nbfn[VC];
+            if (k0 - VC >= 0) load(k0 - VC, nw0, ntg, nrs, nrg, nrh, nrz, nrzn, naf, nbfn);
+    #pragma unroll
+            for (int c = 0; c < VC; ++c) {
+                const int k = k0 - c;
+                if (k < 0) break;
+                const bool in_k = k >= kmin + 1 && k <= kmax, in_n = k + 1 >= kmin + 1 && k + 1 <= kmax;
+                const size_t o = at(k);
+                double yk = 0.0, gk = 0.0;
+                if (in_k) {
+                    yk = (k == kmax) ? w0[c] : w0[c] - g_k1 * yn;
+                    if (k >= kmin + 2) gk = tg[c];
+                }
+                const double sb = in_k ? yk * CV : 0.0;
+                const double sbn = in_n ? yn * CV : 0.0;
+                double pb = 0.0, rb = 0.0;
+                if (in_k) {
+                    pb = -(sb * dt * rz[c]);
+                    rb = -(sb * dt * grav * af[c]);
+                }
+                if (in_n) {
+                    pb = pb + sbn * dt * rzn[c];
+                    rb = rb - sbn * dt * grav * bfn[c];
+                }
+                const double p0b = pb * rs[c];
+                const double rg = rgv[c];
+                const double r0b = rb * (rg * rg);
+                const double rgh = rhv[c];
+                const double sw = in_k ? sb * (rgh * rgh) : 0.0;
+                F.p0b[o] = p0b;
+                F.Spb[o] = dt * p0b;
+                F.r0b[o] = r0b;
+                F.Srb[o] = dt * r0b;
+                F.w0b[o] = sw * A.alpha;
+                F.Swb[o] = dt * sw;
+                if (k == kmin) {
+                    const double bb = F.wb[o] * F.GSGAM2H[o] - F.Ml[at(kmin + 1)] * yn;
+                    F.wb[o] = bb * F.RGSGAM2H[o];
+                } else if (k == kmax + 1) {
+                    const double bt = F.wb[o] * F.GSGAM2H[o] - F.Mu[at(kmax)] * y_kmax;
+                    F.wb[o] = bt * F.RGSGAM2H[o];
+                } else if (in_k) {
+                    F.wb[o] = 0.0;
+                }
+                yn = yk;
+                g_k1 = gk;
+            }
+    #pragma unroll
+            for (int c = 0; c < VC; ++c) {
+                w0[c] = nw0[c]; tg[c] = ntg[c]; rs[c] = nrs[c]; rgv[c] = nrg[c]; rhv[c] = nrh[c];
+                rz[c] = nrz[c]; rzn[c] = nrzn[c]; af[c] = naf[c]; bfn[c] = nbfn[c];
+            }
         }
-        const double p0b = pb * F->RGSGAM2[o];
-        const double rg = F->RGAM[o];
-        const double r0b = rb * (rg * rg);
-        const double rgh = F->RGAMH[o];
-        const double sw = in_k ? sb * (rgh * rgh) : 0.0;
-        F->p0b[o] = p0b;
-        F->Spb[o] = dt * p0b;
-        F->r0b[o] = r0b;
-        F->Srb[o] = dt * r0b;
-        F->w0b[o] = sw * A.alpha;
-        F->Swb[o] = dt * sw;
-        if (k == kmin) {
-            const double bb = F->wb[o] * F->GSGAM2H[o] - F->Ml[at(kmin + 1)] * yn;
-            F->wb[o] = bb * F->RGSGAM2H[o];
-        } else if (k == kmax + 1) {
-            const double bt = F->wb[o] * F->GSGAM2H[o] - F->Mu[at(kmax)] * y_kmax;
-            F->wb[o] = bt * F->RGSGAM2H[o];
-        } else if (in_k) {
-            F->wb[o] = 0.0;
-        }
-        yn = yk;
-        g_k1 = gk;
-    }
+    });
 }
 
 // The same column with kall fixed at compile time (used at 42): the partial
@@ -138,81 +260,67 @@ constexpr int BLOCK_VAR = 64;
 template <int KC>
 __global__ void __launch_bounds__(BLOCK_VAR) vi_adjoint_kernel_reg(VaArgs A)
 {
-    const long t = (long)blockIdx.x * BLOCK_VAR + threadIdx.x;
-    const long nreg = (long)A.gall * A.lall;
-    VaField F;
-    size_t base, s;
-    if (t < nreg) {
-        F = A.reg;
-        base = (size_t)(t / A.gall) * KC * A.gall + (size_t)(t % A.gall);
-        s = A.gall;
-    } else if (A.have_pl && t < nreg + (long)A.gall_pl * kLallPl) {
-        const long p = t - nreg;
-        F = A.pl;
-        base = (size_t)(p / A.gall_pl) * KC * A.gall_pl + (size_t)(p % A.gall_pl);
-        s = A.gall_pl;
-    } else {
-        return;
-    }
-    constexpr int kmin = 1, kmax = KC - 2;
-    const double dt = A.dt, CV = A.CVovRt2, grav = A.grav;
-    auto at = [&](int k) { return base + (size_t)k * s; };
-    double yp[KC], gm[KC];
-    double beta = F.Mc[at(kmin + 1)];
-    double ylast = (F.wb[at(kmin + 1)] * F.GSGAM2H[at(kmin + 1)]) / beta;
-    yp[kmin + 1] = ylast;
-#pragma unroll
-    for (int k = kmin + 2; k <= kmax; ++k) {
-        const double mu_km1 = F.Mu[at(k - 1)];
-        const double g = F.Ml[at(k)] / beta;
-        beta = F.Mc[at(k)] - mu_km1 * g;
-        ylast = (F.wb[at(k)] * F.GSGAM2H[at(k)] - mu_km1 * ylast) / beta;
-        yp[k] = ylast;
-        gm[k] = g;
-        if constexpr (KC > 64) __builtin_amdgcn_sched_barrier(0);
-    }
-    const double y_kmax = ylast;
-    double yn = 0.0;
-#pragma unroll
-    for (int k = KC - 1; k >= 0; --k) {
-        const bool in_k = k >= kmin + 1 && k <= kmax, in_n = k + 1 >= kmin + 1 && k + 1 <= kmax;
-        const size_t o = at(k);
-        double yk = 0.0;
-        if (in_k) yk = (k == kmax) ? yp[k] : yp[k] - gm[k + 1] * yn;
-        const double sb = in_k ? yk * CV : 0.0;
-        const double sbn = in_n ? yn * CV : 0.0;
-        double pb = 0.0, rb = 0.0;
-        if (in_k) {
-            pb = -(sb * dt * A.rdgzh[k]);
-            rb = -(sb * dt * grav * A.afact[k]);
+    column_run<BLOCK_VAR>(A, blockIdx.x, [&](const VaField &F, size_t base, size_t s) {
+        constexpr int kmin = 1, kmax = KC - 2;
+        const double dt = A.dt, CV = A.CVovRt2, grav = A.grav;
+        auto at = [&](int k) { return base + (size_t)k * s; };
+        double yp[KC], gm[KC];
+        double beta = F.Mc[at(kmin + 1)];
+        double ylast = (F.wb[at(kmin + 1)] * F.GSGAM2H[at(kmin + 1)]) / beta;
+        yp[kmin + 1] = ylast;
+    #pragma unroll
+        for (int k = kmin + 2; k <= kmax; ++k) {
+            const double mu_km1 = F.Mu[at(k - 1)];
+            const double g = F.Ml[at(k)] / beta;
+            beta = F.Mc[at(k)] - mu_km1 * g;
+            ylast = (F.wb[at(k)] * F.GSGAM2H[at(k)] - mu_km1 * ylast) / beta;
+            yp[k] = ylast;
+            gm[k] = g;
+            if constexpr (KC > 64) __builtin_amdgcn_sched_barrier(0);
         }
-        if (in_n) {
-            pb = pb + sbn * dt * A.rdgzh[k + 1];
-            rb = rb - sbn * dt * grav * A.bfact[k + 1];
+        const double y_kmax = ylast;
+        double yn = 0.0;
+    #pragma unroll
+        for (int k = KC - 1; k >= 0; --k) {
+            const bool in_k = k >= kmin + 1 && k <= kmax, in_n = k + 1 >= kmin + 1 && k + 1 <= kmax;
+            const size_t o = at(k);
+            double yk = 0.0;
+            if (in_k) yk = (k == kmax) ? yp[k] : yp[k] - gm[k + 1] * yn;
+            const double sb = in_k ? yk * CV : 0.0;
+            const double sbn = in_n ? yn * CV : 0.0;
+            double pb = 0.0, rb = 0.0;
+            if (in_k) {
+                pb = -(sb * dt * A.rdgzh[k]);
+                rb = -(sb * dt * grav * A.afact[k]);
+            }
+            if (in_n) {
+                pb = pb + sbn * dt * A.rdgzh[k + 1];
+                rb = rb - sbn * dt * grav * A.bfact[k + 1];
+            }
+            const double p0b = pb * F.RGSGAM2[o];
+            const double rg = F.RGAM[o];
+            const double r0b = rb * (rg * rg);
+            const double rgh = F.RGAMH[o];
+            const double sw = in_k ? sb * (rgh * rgh) : 0.0;
+            F.p0b[o] = p0b;
+            F.Spb[o] = dt * p0b;
+            F.r0b[o] = r0b;
+            F.Srb[o] = dt * r0b;
+            F.w0b[o] = sw * A.alpha;
+            F.Swb[o] = dt * sw;
+            if (k == kmin) {
+                const double bb = F.wb[o] * F.GSGAM2H[o] - F.Ml[at(kmin + 1)] * yn;
+                F.wb[o] = bb * F.RGSGAM2H[o];
+            } else if (k == kmax + 1) {
+                const double bt = F.wb[o] * F.GSGAM2H[o] - F.Mu[at(kmax)] * y_kmax;
+                F.wb[o] = bt * F.RGSGAM2H[o];
+            } else if (in_k) {
+                F.wb[o] = 0.0;
+            }
+            if (in_k) yn = yk;
+            if constexpr (KC > 64) __builtin_amdgcn_sched_barrier(0);
         }
-        const double p0b = pb * F.RGSGAM2[o];
-        const double rg = F.RGAM[o];
-        const double r0b = rb * (rg * rg);
-        const double rgh = F.RGAMH[o];
-        const double sw = in_k ? sb * (rgh * rgh) : 0.0;
-        F.p0b[o] = p0b;
-        F.Spb[o] = dt * p0b;
-        F.r0b[o] = r0b;
-        F.Srb[o] = dt * r0b;
-        F.w0b[o] = sw * A.alpha;
-        F.Swb[o] = dt * sw;
-        if (k == kmin) {
-            const double bb = F.wb[o] * F.GSGAM2H[o] - F.Ml[at(kmin + 1)] * yn;
-            F.wb[o] = bb * F.RGSGAM2H[o];
-        } else if (k == kmax + 1) {
-            const double bt = F.wb[o] * F.GSGAM2H[o] - F.Mu[at(kmax)] * y_kmax;
-            F.wb[o] = bt * F.RGSGAM2H[o];
-        } else if (in_k) {
-            F.wb[o] = 0.0;
-        }
-        if (in_k) yn = yk;
-        if constexpr (KC > 64) __builtin_amdgcn_sched_barrier(0);
-    }
+    });
 }
 
 // ---- vertical_limiter_thuburn (dyn_vert_adv_limiter/src/mod_src_tracer.f90:41-243)
@@ -1392,13 +1500,13 @@ int launch_vi_adjoint(nicam_ctx c, double *wb, double *wb_pl, double *w0b, doubl
     // (NICAM_TUNE_VI_VARIANT 1 or 3: formed in the solve, as before)
     const int variant = c->tune.vi_variant;
     if (D.kall == 42 && variant != 2 && variant != 4) {
-        hipLaunchKernelGGL(vi_adjoint_kernel_reg<42>, dim3((unsigned)((A.ncol + BLOCK_VAR - 1) / BLOCK_VAR)),
+        hipLaunchKernelGGL(vi_adjoint_kernel_reg<42>, dim3((unsigned)column_blocks<BLOCK_VAR>(D.gall, D.lall, D.gall_pl, D.have_pl)),
                            dim3(BLOCK_VAR), 0, c->stream, A);
     } else if (variant == 1 || variant == 3) {
-        hipLaunchKernelGGL(vi_adjoint_kernel<false>, dim3((unsigned)((A.ncol + BLOCK_VA - 1) / BLOCK_VA)),
+        hipLaunchKernelGGL(vi_adjoint_kernel<false>, dim3((unsigned)column_blocks<BLOCK_VA>(D.gall, D.lall, D.gall_pl, D.have_pl)),
                            dim3(BLOCK_VA), 0, c->stream, A);
     } else {
-        hipLaunchKernelGGL(vi_adjoint_kernel<true>, dim3((unsigned)((A.ncol + BLOCK_VA - 1) / BLOCK_VA)),
+        hipLaunchKernelGGL(vi_adjoint_kernel_chunked, dim3((unsigned)column_blocks<BLOCK_VA>(D.gall, D.lall, D.gall_pl, D.have_pl)),
                            dim3(BLOCK_VA), 0, c->stream, A);
     }
     NICAM_HIP(hipGetLastError());

@@ -21,6 +21,35 @@ constexpr int kGmtrANmax = 12; // GMTR_a_nmax
 constexpr int kGmtrANmaxPl = 18;
 constexpr int kMaxGroup = 8;  // grids in one grouped launch
 
+// Columns of a grid in work-groups of BLOCK work-items: the regular regions'
+// columns first, then, from a work-group of their own, the pole columns, so
+// that the field set (A.reg or A.pl) a work-group works on is uniform across
+// it and its pointers stay in scalar registers.  Args has reg, pl, gall, lall,
+// kall, gall_pl and have_pl; body(F, base, s) gets the field set, the
+// column's first element and the level stride.
+template <int BLOCK>
+inline long column_blocks(long gall, long lall, long gall_pl, bool have_pl)
+{
+    return (gall * lall + BLOCK - 1) / BLOCK + (have_pl ? (gall_pl * kLallPl + BLOCK - 1) / BLOCK : 0);
+}
+template <int BLOCK, class Args, class Body>
+__device__ __forceinline__ void column_run(const Args &A, long blk, Body &&body)
+{
+    const long nreg = (long)A.gall * A.lall;
+    const long nbr = (nreg + BLOCK - 1) / BLOCK;
+    if (blk < nbr) {
+        const long t = blk * BLOCK + threadIdx.x;
+        if (t >= nreg) return;
+        const int g = (int)(t % A.gall), l = (int)(t / A.gall);
+        body(A.reg, (size_t)l * A.kall * A.gall + g, (size_t)A.gall);
+    } else if (A.have_pl) {
+        const long p = (blk - nbr) * BLOCK + threadIdx.x;
+        if (p >= (long)A.gall_pl * kLallPl) return;
+        const int g = (int)(p % A.gall_pl), l = (int)(p / A.gall_pl);
+        body(A.pl, (size_t)l * A.kall * A.gall_pl + g, (size_t)A.gall_pl);
+    }
+}
+
 // Sizes of one grid, as the kernels see them.  All indices inside the
 // kernels are 0-based: g = j*gall_1d + i, halo at i or j = 0 and gall_1d-1.
 struct Dims {

@@ -193,29 +193,6 @@ __device__ __forceinline__ void vi_column(const ViFieldT<Src> &F, size_t base, s
     F.w[at(kmax + 1)] = w_kmax1 * F.GSGAM2H[at(kmax + 1)];
 }
 
-// work-item t -> (field set, column base, level stride); false if idle
-template <class Src>
-__device__ __forceinline__ bool vi_map(const ViArgsT<Src> &A, long t, const ViFieldT<Src> *&F, size_t &base, size_t &s)
-{
-    const long nreg = (long)A.gall * A.lall;
-    if (t < nreg) {
-        const int g = (int)(t % A.gall), l = (int)(t / A.gall);
-        F = &A.reg;
-        base = (size_t)l * A.kall * A.gall + g;
-        s = A.gall;
-        return true;
-    }
-    if (A.have_pl && t < nreg + (long)A.gall_pl * kLallPl) {
-        const int p = (int)(t - nreg);
-        const int g = p % A.gall_pl, l = p / A.gall_pl;
-        F = &A.pl;
-        base = (size_t)l * A.kall * A.gall_pl + g;
-        s = A.gall_pl;
-        return true;
-    }
-    return false;
-}
-
 constexpr int BLOCK_LDS = 64;
 constexpr int BLOCK_FAC = 256;
 constexpr int BLOCK_REG = 64;
@@ -224,28 +201,24 @@ template <class Src>
 __global__ void __launch_bounds__(BLOCK_LDS) vi_kernel_lds(ViArgsT<Src> A)
 {
     extern __shared__ double s_gam[];
-    const long t = (long)blockIdx.x * BLOCK_LDS + threadIdx.x;
-    const ViFieldT<Src> *F;
-    size_t base, s;
-    if (!vi_map(A, t, F, base, s)) return;
-    LdsGamma gam{s_gam + threadIdx.x, BLOCK_LDS};
-    int unused = 0;
-    vi_column<0, false, false>(*F, base, s, A.kall, A.rdgzh, A.afact, A.bfact, A.dt, A.CVovRt2, A.alpha, A.grav,
-                               gam, unused);
+    column_run<BLOCK_LDS>(A, blockIdx.x, [&](const ViFieldT<Src> &F, size_t base, size_t s) {
+        LdsGamma gam{s_gam + threadIdx.x, BLOCK_LDS};
+        int unused = 0;
+        vi_column<0, false, false>(F, base, s, A.kall, A.rdgzh, A.afact, A.bfact, A.dt, A.CVovRt2, A.alpha, A.grav,
+                                   gam, unused);
+    });
 }
 
 // factored, any kall: the forward solution goes through rhogw
 template <class Src>
 __global__ void __launch_bounds__(BLOCK_FAC) vi_kernel_fact(ViArgsT<Src> A)
 {
-    const long t = (long)blockIdx.x * BLOCK_FAC + threadIdx.x;
-    const ViFieldT<Src> *F;
-    size_t base, s;
-    if (!vi_map(A, t, F, base, s)) return;
-    NoGamma gam;
-    int unused = 0;
-    vi_column<0, false, true>(*F, base, s, A.kall, A.rdgzh, A.afact, A.bfact, A.dt, A.CVovRt2, A.alpha, A.grav,
-                              gam, unused);
+    column_run<BLOCK_FAC>(A, blockIdx.x, [&](const ViFieldT<Src> &F, size_t base, size_t s) {
+        NoGamma gam;
+        int unused = 0;
+        vi_column<0, false, true>(F, base, s, A.kall, A.rdgzh, A.afact, A.bfact, A.dt, A.CVovRt2, A.alpha, A.grav,
+                                  gam, unused);
+    });
 }
 
 // several grids in one launch (factored, any kall): work-groups
@@ -264,32 +237,52 @@ __global__ void __launch_bounds__(BLOCK_FAC) vi_group_kernel(ViGroup<Src> G)
     int n = 0;
     while (n + 1 < G.n && b >= G.off[n + 1]) ++n;
     const ViArgsT<Src> &A = G.a[n];
-    const long t = (long)(b - G.off[n]) * BLOCK_FAC + threadIdx.x;
-    const ViFieldT<Src> *F;
-    size_t base, s;
-    if (!vi_map(A, t, F, base, s)) return;
-    NoGamma gam;
-    int unused = 0;
-    vi_column<0, false, true>(*F, base, s, A.kall, A.rdgzh, A.afact, A.bfact, A.dt, A.CVovRt2, A.alpha, A.grav,
-                              gam, unused);
+    column_run<BLOCK_FAC>(A, b - G.off[n], [&](const ViFieldT<Src> &F, size_t base, size_t s) {
+        NoGamma gam;
+        int unused = 0;
+        vi_column<0, false, true>(F, base, s, A.kall, A.rdgzh, A.afact, A.bfact, A.dt, A.CVovRt2, A.alpha, A.grav,
+                                  gam, unused);
+    });
 }
 
 template <int KC, bool FACT, class Src>
 __global__ void __launch_bounds__(BLOCK_REG) vi_kernel_reg(ViArgsT<Src> A)
 {
-    const long t = (long)blockIdx.x * BLOCK_REG + threadIdx.x;
-    const ViFieldT<Src> *F;
-    size_t base, s;
-    if (!vi_map(A, t, F, base, s)) return;
-    RegArray<KC> wf;
-    if constexpr (FACT) {
-        NoGamma gam;
-        vi_column<KC, true, true>(*F, base, s, KC, A.rdgzh, A.afact, A.bfact, A.dt, A.CVovRt2, A.alpha, A.grav, gam,
-                                  wf);
+    auto column = [&](const ViFieldT<Src> &F, size_t base, size_t s) {
+        RegArray<KC> wf;
+        if constexpr (FACT) {
+            NoGamma gam;
+            vi_column<KC, true, true>(F, base, s, KC, A.rdgzh, A.afact, A.bfact, A.dt, A.CVovRt2, A.alpha, A.grav,
+                                      gam, wf);
+        } else {
+            RegArray<KC> gam;
+            vi_column<KC, true, false>(F, base, s, KC, A.rdgzh, A.afact, A.bfact, A.dt, A.CVovRt2, A.alpha, A.grav,
+                                       gam, wf);
+        }
+    };
+    if constexpr (std::is_same<Src, double>::value) {
+        // FP64 sources: the field set chosen per work-item, its pointers in vector registers, schedules
+        // better here (gl7, variant 4: 368 us against 401 us with the work-group's)
+        // (one copy of the column, on a per-work-item pointer)
+        const long t = (long)blockIdx.x * BLOCK_REG + threadIdx.x;
+        const long nreg = (long)A.gall * A.lall;
+        const ViFieldT<Src> *F;
+        size_t base, s;
+        if (t < nreg) {
+            F = &A.reg;
+            base = (size_t)(t / A.gall) * A.kall * A.gall + (size_t)(t % A.gall);
+            s = A.gall;
+        } else {
+            // the grid is column_blocks' (the pole columns from a work-group of their own)
+            const long p = t - ((nreg + BLOCK_REG - 1) / BLOCK_REG) * BLOCK_REG;
+            if (!A.have_pl || p < 0 || p >= (long)A.gall_pl * kLallPl) return;
+            F = &A.pl;
+            base = (size_t)(p / A.gall_pl) * A.kall * A.gall_pl + (size_t)(p % A.gall_pl);
+            s = A.gall_pl;
+        }
+        column(*F, base, s);
     } else {
-        RegArray<KC> gam;
-        vi_column<KC, true, false>(*F, base, s, KC, A.rdgzh, A.afact, A.bfact, A.dt, A.CVovRt2, A.alpha, A.grav,
-                                   gam, wf);
+        column_run<BLOCK_REG>(A, blockIdx.x, column);  // FP32 sources: 309 us against 550 us per work-item
     }
 }
 
@@ -382,10 +375,9 @@ int launch_vi_t(nicam_ctx c, double *w, double *w_pl, const Src *w0, const Src *
     int variant = c->tune.vi_variant;
     const bool reg_ok = (D.kall == 42 || D.kall == 98);
     if ((variant == 3 || variant == 4) && !reg_ok) variant = 0;
-    // measured at gl7 (40 x 66x66 x 98): FP64 inputs: 4 (factored, registers) 366 us, 3 380, 2 462, 1 597;
-    // FP32 inputs: 2 (factored, through rhogw) 323 us, 3 and 4 557 (the widening loads lengthen the
-    // register variants' one-wave-per-SIMD chain), 1 566
-    if (variant == 0) variant = (reg_ok && !std::is_same<Src, float>::value) ? 4 : 2;
+    // measured at gl7 (40 x 66x66 x 98): FP64 inputs: 4 (factored, registers) 368 us, 3 379, 2 469, 1 623;
+    // FP32 inputs: 4 309 us, 2 (factored, through rhogw) 324, 3 759, 1 633
+    if (variant == 0) variant = reg_ok ? 4 : 2;
 
     if (variant == 1) {
         const size_t lds = sizeof(double) * BLOCK_LDS * D.kall;
@@ -393,13 +385,13 @@ int launch_vi_t(nicam_ctx c, double *w, double *w_pl, const Src *w0, const Src *
         if (lds > 64 * 1024)
             NICAM_HIP(hipFuncSetAttribute((const void *)vi_kernel_lds<Src>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                           (int)lds));
-        hipLaunchKernelGGL(vi_kernel_lds<Src>, dim3((unsigned)((A.ncol + BLOCK_LDS - 1) / BLOCK_LDS)), dim3(BLOCK_LDS), lds,
-                           c->stream, A);
+        hipLaunchKernelGGL(vi_kernel_lds<Src>, dim3((unsigned)column_blocks<BLOCK_LDS>(D.gall, D.lall, D.gall_pl, D.have_pl)),
+                           dim3(BLOCK_LDS), lds, c->stream, A);
     } else if (variant == 2) {
-        hipLaunchKernelGGL(vi_kernel_fact<Src>, dim3((unsigned)((A.ncol + BLOCK_FAC - 1) / BLOCK_FAC)), dim3(BLOCK_FAC),
-                           0, c->stream, A);
+        hipLaunchKernelGGL(vi_kernel_fact<Src>, dim3((unsigned)column_blocks<BLOCK_FAC>(D.gall, D.lall, D.gall_pl, D.have_pl)),
+                           dim3(BLOCK_FAC), 0, c->stream, A);
     } else {
-        const dim3 grid((unsigned)((A.ncol + BLOCK_REG - 1) / BLOCK_REG));
+        const dim3 grid((unsigned)column_blocks<BLOCK_REG>(D.gall, D.lall, D.gall_pl, D.have_pl));
         // variant 3 holds gamma and the forward solution of a whole column in
         // registers (252 VGPRs at kall = 42; 256 VGPRs + 256 AGPRs, one
         // wavefront per SIMD, at kall = 98); variant 4 only the forward solution
@@ -461,7 +453,7 @@ int launch_vi_group_mixed(const nicam_ctx *cs, int n, const nicam_tracer_step_f3
         vi_make_args<float>(c, G.a[i], p.rhogw, p.rhogw_pl, p.rhogw0, p.rhogw0_pl, p.preg0, p.preg0_pl, p.rhog0,
                             p.rhog0_pl, p.Srho, p.Srho_pl, p.Sw, p.Sw_pl, p.Spre, p.Spre_pl, dt, c->RGAM32,
                             c->RGAM32_pl, c->RGAMH32, c->RGAMH32_pl, c->RGSGAM2_32, c->RGSGAM2_32_pl);
-        const long nb = (G.a[i].ncol + BLOCK_FAC - 1) / BLOCK_FAC;
+        const long nb = column_blocks<BLOCK_FAC>(G.a[i].gall, G.a[i].lall, G.a[i].gall_pl, G.a[i].have_pl);
         if ((long)G.off[i] + nb > 0x7fffffffL) return fail(NICAM_ERR_ARG, "group too large for one launch");
         G.off[i + 1] = G.off[i] + (int)nb;
     }
