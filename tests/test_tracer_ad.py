@@ -261,14 +261,26 @@ def test_gpu_tracer_gradient_gl7_16_tracers_24_steps(gpu):
     import torch
 
     ctx, step, q0, q0_pl, r0, r0_pl = _setup(7, 1, 98, 77, 16)
+    # the forward alone, for the cost of the gradient against it
+    with torch.no_grad():
+        s = (q0, q0_pl, r0, r0_pl)
+        s = step(*s)  # warm-up
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        s = (q0, q0_pl, r0, r0_pl)
+        for _ in range(24):
+            s = step(*s)
+        torch.cuda.synchronize()
+        t_fwd = time.perf_counter() - t0
+        del s
     torch.cuda.reset_peak_memory_stats()
     t0 = time.perf_counter()
     J, g = _run(step, q0, q0_pl, r0, r0_pl, 24, 6, recompute_tracers=True)
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
     lhs = float((q0 * g[0]).sum() + (q0_pl * g[1]).sum())
-    print(f"gl7 16 tracers 24 steps: J {J:.12e}, <q0,dJ/dq0> {lhs:.12e}, gradient in {dt:.1f} s, "
-          f"peak memory {torch.cuda.max_memory_allocated() / 1e9:.1f} GB", flush=True)
+    print(f"gl7 16 tracers 24 steps: J {J:.12e}, <q0,dJ/dq0> {lhs:.12e}, forward {t_fwd:.2f} s, gradient in "
+          f"{dt:.2f} s ({dt / t_fwd:.2f}x), peak memory {torch.cuda.max_memory_allocated() / 1e9:.1f} GB", flush=True)
     assert abs(lhs - 2 * J) <= 1e-12 * abs(2 * J)
     assert torch.isfinite(g[2]).all() and float(g[2].abs().max()) > 0.0
     ctx.close()
