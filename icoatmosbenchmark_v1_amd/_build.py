@@ -20,7 +20,7 @@ LIB_NAME = "libnicam_tt.so"
 LIB_PATH = PKG_DIR / LIB_NAME
 OBJ_DIR = PKG_DIR / "_build"
 
-SOURCES = ["ctx.cpp", "comm.cpp", "grid.cpp", "gmtr.cpp", "oprt_setup.cpp", "hflux.hip", "hlim.hip", "vlim.hip", "vi.hip", "comm.hip", "oprt.hip", "glue.hip", "vglue.hip", "f32.hip", "adjoint.hip", "mass.hip", "conserve.hip", "group.cpp"]
+SOURCES = ["ctx.cpp", "comm.cpp", "grid.cpp", "gmtr.cpp", "oprt_setup.cpp", "hflux.hip", "hlim.hip", "vlim.hip", "vi.hip", "comm.hip", "oprt.hip", "glue.hip", "vglue.hip", "f32.hip", "adjoint.hip", "mass.hip", "conserve.hip", "group.cpp", "glue_adjoint.hip"]
 
 HIPCC_FLAGS = [
     "-x", "hip",

@@ -97,6 +97,8 @@ int comm_setup(nicam_ctx c, int rlevel)
     NICAM_HIP(hipMemcpy(c->halo_src, hs.data(), sizeof(int) * hs.size(), hipMemcpyHostToDevice));
     c->halo_n = (int)dst.size();
     c->rlevel = rlevel;
+    rc = halo_adjoint_setup(c, std::vector<int>(hd.begin() + 2, hd.end()), std::vector<int>(hs.begin() + 2, hs.end()));
+    if (rc != NICAM_OK) return rc;
 
     std::vector<int> slot, tab, pole_tab;
     rc = build_arc_partners(rlevel, D.gall_1d, D.gall_pl, dst, src, sgp, slot, tab, pole_tab);

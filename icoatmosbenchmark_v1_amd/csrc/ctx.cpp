@@ -7,6 +7,7 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <functional>
 #include <vector>
 
 #include <type_traits>
@@ -205,7 +206,7 @@ int nicam_ctx_destroy(nicam_ctx c)
     if (c->have_sgp) (void)hipFree(c->have_sgp);
     if (c->halo_dst) (void)hipFree(c->halo_dst);
     if (c->halo_src) (void)hipFree(c->halo_src);
-    for (int *p : {c->arc_slot, c->arc_tab, c->arc_pole})
+    for (int *p : {c->arc_slot, c->arc_tab, c->arc_pole, c->hadj_src, c->hadj_off, c->hadj_dst})
         if (p) (void)hipFree(p);
     if (c->mass_part) (void)hipFree(c->mass_part);
     delete c;
@@ -606,6 +607,23 @@ static int halo_entry(nicam_ctx c, double *var, double *var_pl, int kmax, int vm
 int nicam_comm_data_transfer(nicam_ctx c, double *var, double *var_pl, int kmax, int vmax)
 {
     return halo_entry(c, var, var_pl, kmax, vmax, false);
+}
+
+int nicam_comm_data_transfer_adjoint(nicam_ctx c, double *var_b, double *var_b_pl, int kmax, int vmax)
+{
+    NICAM_TRY(check_ctx(c));
+    if (kmax < 1 || vmax < 1) return fail(NICAM_ERR_ARG, "kmax and vmax must be positive");
+    NICAM_TRY(need(var_b, "var_b"));
+    NICAM_TRY(need(var_b_pl, "var_b_pl"));
+    if (!c->hadj_nsrc) return fail(NICAM_ERR_STATE, "halo map not set (nicam_comm_setup)");
+    if (c->ptr_mode == NICAM_PTR_DEVICE) return launch_halo_adjoint(c, var_b, var_b_pl, kmax, vmax);
+    const size_t n = (size_t)kmax * vmax * 8;
+    const size_t R = (size_t)c->d.gall * c->d.lall * n, P = (size_t)c->d.gall_pl * kLallPl * n;
+    std::vector<HostArg> a = {{var_b, R, true}, {var_b_pl, P, true}};
+    std::vector<void *> v;
+    NICAM_TRY(stage_in(c, a, v));
+    NICAM_TRY(launch_halo_adjoint(c, static_cast<double *>(v[0]), static_cast<double *>(v[1]), kmax, vmax));
+    return stage_out(c, a, v);
 }
 
 int nicam_comm_var(nicam_ctx c, double *var, double *var_pl, int kmax, int vmax)
@@ -1177,6 +1195,95 @@ int nicam_tracer_update(nicam_ctx c, double *rhogq, double *rhogq_pl, const doub
     auto D = [&](int i) { return static_cast<double *>(v[i]); };
     NICAM_TRY(launch_update(c, D(0), D(1), D(2), D(3), D(4), D(5)));
     return stage_out(c, a, v);
+}
+
+// ---- adjoints of the horizontal glue ------------------------------------------
+// Shared by the four below: check the arrays (pole arrays when the context has
+// poles), then launch directly (device pointers) or stage host arrays.
+static int glue_adjoint_call(nicam_ctx c, std::vector<HostArg> a, const char *const *names,
+                             const std::function<int(const std::vector<void *> &)> &launch)
+{
+    NICAM_TRY(check_ctx(c));
+    for (size_t i = 0; i < a.size(); i += 2) {
+        NICAM_TRY(need(a[i].host, names[i / 2]));
+        if (c->d.have_pl) NICAM_TRY(need(a[i + 1].host, "pole array"));
+        else a[i + 1].host = nullptr;
+    }
+    std::vector<void *> v;
+    if (c->ptr_mode == NICAM_PTR_DEVICE) {
+        for (const HostArg &h : a) v.push_back(const_cast<void *>(h.host));
+        return launch(v);
+    }
+    NICAM_TRY(stage_in(c, a, v));
+    NICAM_TRY(launch(v));
+    return stage_out(c, a, v);
+}
+
+int nicam_oprt_gradient_adjoint(nicam_ctx c, double *scl_b, double *scl_b_pl, const double *grad_b,
+                                const double *grad_b_pl, const double *coef_grad, const double *coef_grad_pl)
+{
+    const size_t R = c ? c->d.nreg() * 8 : 0, P = c ? c->d.npl() * 8 : 0;
+    const size_t CG = c ? (size_t)c->d.gall * c->d.lall * 21 * 8 : 0, CGP = c ? (size_t)c->d.gall_pl * 3 * 2 * 8 : 0;
+    const char *nm[] = {"scl_b", "grad_b", "coef_grad"};
+    return glue_adjoint_call(c, {{scl_b, R, true}, {scl_b_pl, P, true}, {grad_b, 3 * R, false},
+                                 {grad_b_pl, 3 * P, false}, {coef_grad, CG, false}, {coef_grad_pl, CGP, false}},
+                             nm, [&](const std::vector<void *> &v) {
+                                 auto D = [&](int i) { return static_cast<double *>(v[i]); };
+                                 return launch_gradient_adjoint(c, D(0), D(1), D(2), D(3), D(4), D(5));
+                             });
+}
+
+int nicam_tracer_courant_adjoint(nicam_ctx c, double *flx_h_b, double *flx_h_b_pl, double *rhog_b,
+                                 double *rhog_b_pl, const double *ch_b, const double *ch_b_pl, const double *flx_h,
+                                 const double *flx_h_pl, const double *rhog, const double *rhog_pl)
+{
+    const size_t R = c ? c->d.nreg() * 8 : 0, P = c ? c->d.npl() * 8 : 0;
+    const char *nm[] = {"flx_h_b", "rhog_b", "ch_b", "flx_h", "rhog"};
+    return glue_adjoint_call(c, {{flx_h_b, 6 * R, true}, {flx_h_b_pl, P, true}, {rhog_b, R, true},
+                                 {rhog_b_pl, P, true}, {ch_b, 6 * R, false}, {ch_b_pl, P, false},
+                                 {flx_h, 6 * R, false}, {flx_h_pl, P, false}, {rhog, R, false}, {rhog_pl, P, false}},
+                             nm, [&](const std::vector<void *> &v) {
+                                 auto D = [&](int i) { return static_cast<double *>(v[i]); };
+                                 return launch_courant_adjoint(c, D(0), D(1), D(2), D(3), D(4), D(5), D(6), D(7),
+                                                               D(8), D(9));
+                             });
+}
+
+int nicam_horizontal_remap_adjoint(nicam_ctx c, double *q_b, double *q_b_pl, double *grad_b, double *grad_b_pl,
+                                   double *GRD_xc_b, double *GRD_xc_b_pl, const double *q_a_b, const double *q_a_b_pl,
+                                   const double *grad, const double *grad_pl, const double *cmask,
+                                   const double *cmask_pl, const double *GRD_xc, const double *GRD_xc_pl,
+                                   const double *GRD_x, const double *GRD_x_pl)
+{
+    const size_t R = c ? c->d.nreg() * 8 : 0, P = c ? c->d.npl() * 8 : 0;
+    const size_t X = c ? (size_t)c->d.gall * c->d.lall * 3 * 8 : 0, XP = c ? (size_t)c->d.gall_pl * 2 * 3 * 8 : 0;
+    const char *nm[] = {"q_b", "grad_b", "GRD_xc_b", "q_a_b", "grad", "cmask", "GRD_xc", "GRD_x"};
+    return glue_adjoint_call(c, {{q_b, R, true}, {q_b_pl, P, true}, {grad_b, 3 * R, true}, {grad_b_pl, 3 * P, true},
+                                 {GRD_xc_b, 9 * R, true}, {GRD_xc_b_pl, 3 * P, true}, {q_a_b, 6 * R, false},
+                                 {q_a_b_pl, P, false}, {grad, 3 * R, false}, {grad_pl, 3 * P, false},
+                                 {cmask, 6 * R, false}, {cmask_pl, P, false}, {GRD_xc, 9 * R, false},
+                                 {GRD_xc_pl, 3 * P, false}, {GRD_x, X, false}, {GRD_x_pl, XP, false}},
+                             nm, [&](const std::vector<void *> &v) {
+                                 auto D = [&](int i) { return static_cast<double *>(v[i]); };
+                                 return launch_remap_adjoint(c, D(0), D(1), D(2), D(3), D(4), D(5), D(6), D(7), D(8),
+                                                             D(9), D(10), D(11), D(12), D(13), D(14), D(15));
+                             });
+}
+
+int nicam_tracer_update_adjoint(nicam_ctx c, double *flx_h_b, double *flx_h_b_pl, double *q_a_b, double *q_a_b_pl,
+                                const double *rhogq_b, const double *rhogq_b_pl, const double *flx_h,
+                                const double *flx_h_pl, const double *q_a, const double *q_a_pl)
+{
+    const size_t R = c ? c->d.nreg() * 8 : 0, P = c ? c->d.npl() * 8 : 0;
+    const char *nm[] = {"flx_h_b", "q_a_b", "rhogq_b", "flx_h", "q_a"};
+    return glue_adjoint_call(c, {{flx_h_b, 6 * R, true}, {flx_h_b_pl, P, true}, {q_a_b, 6 * R, true},
+                                 {q_a_b_pl, P, true}, {rhogq_b, R, false}, {rhogq_b_pl, P, false},
+                                 {flx_h, 6 * R, false}, {flx_h_pl, P, false}, {q_a, 6 * R, false}, {q_a_pl, P, false}},
+                             nm, [&](const std::vector<void *> &v) {
+                                 auto D = [&](int i) { return static_cast<double *>(v[i]); };
+                                 return launch_update_adjoint(c, D(0), D(1), D(2), D(3), D(4), D(5), D(6), D(7), D(8),
+                                                              D(9));
+                             });
 }
 
 int nicam_vertical_mass_flux(nicam_ctx c, double *flx_v, double *flx_v_pl, const double *rhogvx,

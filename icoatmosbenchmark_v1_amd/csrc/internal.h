@@ -147,6 +147,10 @@ struct nicam_ctx_s {
     int rlevel = -1;
     // partners of the faces regions share (nicam_comm_setup; the conservative update)
     int *arc_slot = nullptr, *arc_tab = nullptr, *arc_pole = nullptr;
+    // the halo fill's transpose (nicam_comm_setup): the sources in ascending
+    // order, each with its destinations (CSR, ascending codes)
+    int *hadj_src = nullptr, *hadj_off = nullptr, *hadj_dst = nullptr;
+    int hadj_nsrc = 0;
 
     // partial sums of nicam_tracer_mass ([kall][lall]), allocated on its first call
     double *mass_part = nullptr;
@@ -270,6 +274,21 @@ int build_arc_partners(int rlevel, int gall_1d, int gall_pl, const std::vector<H
 int diamond_corner_classes(int cls[10][4]);
 // var(gall,kmax,lall,nvar), var_pl(gall_pl,kmax,2,nvar); pole_fill: COMM_var
 int launch_halo_fill(nicam_ctx c, double *var, double *var_pl, int kmax, int nvar, bool pole_fill);
+// adjoints of the glue and of the halo fill (glue_adjoint.hip)
+int halo_adjoint_setup(nicam_ctx c, const std::vector<int> &dst, const std::vector<int> &src);
+int launch_halo_adjoint(nicam_ctx c, double *var, double *var_pl, int kmax, int nvar);
+int launch_gradient_adjoint(nicam_ctx c, double *scl_b, double *scl_b_pl, const double *grad_b,
+                            const double *grad_b_pl, const double *coef_grad, const double *coef_grad_pl);
+int launch_courant_adjoint(nicam_ctx c, double *flx_b, double *flx_b_pl, double *rhog_b, double *rhog_b_pl,
+                           const double *ch_b, const double *ch_b_pl, const double *flx_h, const double *flx_h_pl,
+                           const double *rhog, const double *rhog_pl);
+int launch_remap_adjoint(nicam_ctx c, double *q_b, double *q_b_pl, double *grad_b, double *grad_b_pl, double *xc_b,
+                         double *xc_b_pl, const double *q_a_b, const double *q_a_b_pl, const double *grad,
+                         const double *grad_pl, const double *cmask, const double *cmask_pl, const double *xc,
+                         const double *xc_pl, const double *x, const double *x_pl);
+int launch_update_adjoint(nicam_ctx c, double *flx_b, double *flx_b_pl, double *q_a_b, double *q_a_b_pl,
+                          const double *rhogq_b, const double *rhogq_b_pl, const double *flx_h,
+                          const double *flx_h_pl, const double *q_a, const double *q_a_pl);
 int launch_halo_fill_f32(nicam_ctx c, float *var, float *var_pl, int kmax, int nvar, bool pole_fill);
 int launch_hlim_f32(nicam_ctx c, float *q_a, float *q_a_pl, const float *q, const float *q_pl, const float *d,
                     const float *d_pl, const float *ch, const float *ch_pl, const float *cmask,

@@ -96,6 +96,11 @@ SIGNATURES = {
     "nicam_tracer_courant": (c_int, [_P] * 9),
     "nicam_horizontal_remap": (c_int, [_P] * 13),
     "nicam_tracer_update": (c_int, [_P] * 7),
+    "nicam_oprt_gradient_adjoint": (c_int, [_P] * 7),
+    "nicam_tracer_courant_adjoint": (c_int, [_P] * 11),
+    "nicam_horizontal_remap_adjoint": (c_int, [_P] * 17),
+    "nicam_tracer_update_adjoint": (c_int, [_P] * 11),
+    "nicam_comm_data_transfer_adjoint": (c_int, [_P, _P, _P, c_int, c_int]),
     "nicam_vertical_mass_flux": (c_int, [_P] * 11 + [c_double]),
     "nicam_vertical_courant": (c_int, [_P] * 7),
     "nicam_vertical_face": (c_int, [_P] * 5),
@@ -362,6 +367,12 @@ class Context:
         _check(self.lib.nicam_comm_data_transfer(self.h, *self._halo_args(var, var_pl, kmax, vmax)),
                "nicam_comm_data_transfer")
 
+    def comm_data_transfer_adjoint(self, var_b, var_b_pl, vmax: int = 1, kmax: Optional[int] = None) -> None:
+        """The transpose of comm_data_transfer, in place: each halo destination's cotangent is added to its
+        source's and cleared."""
+        _check(self.lib.nicam_comm_data_transfer_adjoint(self.h, *self._halo_args(var_b, var_b_pl, kmax, vmax)),
+               "nicam_comm_data_transfer_adjoint")
+
     def comm_var(self, var, var_pl, vmax: int = 1, kmax: Optional[int] = None) -> None:
         """COMM_var: pole centres from their regions, then COMM_data_transfer."""
         _check(self.lib.nicam_comm_var(self.h, *self._halo_args(var, var_pl, kmax, vmax)), "nicam_comm_var")
@@ -547,6 +558,45 @@ class Context:
         a += self._args([(flx_h, "flx_h", 6 * R), (flx_h_pl, "flx_h_pl", P), (q_a, "q_a", 6 * R),
                          (q_a_pl, "q_a_pl", P)])
         _check(self.lib.nicam_tracer_update(self.h, *a), "nicam_tracer_update")
+
+    # adjoints of the horizontal glue: outputs (the *_b arrays first in each list) overwritten
+    def oprt_gradient_adjoint(self, scl_b, scl_b_pl, grad_b, grad_b_pl, coef_grad, coef_grad_pl) -> None:
+        R, P = self.nreg, self.npl
+        a = [_ptr(scl_b, self.mode, "scl_b", R), _ptr(scl_b_pl, self.mode, "scl_b_pl", P)]
+        a += self._args([(grad_b, "grad_b", 3 * R), (grad_b_pl, "grad_b_pl", 3 * P),
+                         (coef_grad, "coef_grad", self.gall * self.lall * 21),
+                         (coef_grad_pl, "coef_grad_pl", self.gall_pl * 6)])
+        _check(self.lib.nicam_oprt_gradient_adjoint(self.h, *a), "nicam_oprt_gradient_adjoint")
+
+    def tracer_courant_adjoint(self, flx_h_b, flx_h_b_pl, rhog_b, rhog_b_pl, ch_b, ch_b_pl, flx_h, flx_h_pl, rhog,
+                               rhog_pl) -> None:
+        R, P = self.nreg, self.npl
+        a = [_ptr(flx_h_b, self.mode, "flx_h_b", 6 * R), _ptr(flx_h_b_pl, self.mode, "flx_h_b_pl", P),
+             _ptr(rhog_b, self.mode, "rhog_b", R), _ptr(rhog_b_pl, self.mode, "rhog_b_pl", P)]
+        a += self._args([(ch_b, "ch_b", 6 * R), (ch_b_pl, "ch_b_pl", P), (flx_h, "flx_h", 6 * R),
+                         (flx_h_pl, "flx_h_pl", P), (rhog, "rhog", R), (rhog_pl, "rhog_pl", P)])
+        _check(self.lib.nicam_tracer_courant_adjoint(self.h, *a), "nicam_tracer_courant_adjoint")
+
+    def horizontal_remap_adjoint(self, q_b, q_b_pl, grad_b, grad_b_pl, GRD_xc_b, GRD_xc_b_pl, q_a_b, q_a_b_pl,
+                                 grad, grad_pl, cmask, cmask_pl, GRD_xc, GRD_xc_pl, GRD_x, GRD_x_pl) -> None:
+        R, P = self.nreg, self.npl
+        a = [_ptr(q_b, self.mode, "q_b", R), _ptr(q_b_pl, self.mode, "q_b_pl", P),
+             _ptr(grad_b, self.mode, "grad_b", 3 * R), _ptr(grad_b_pl, self.mode, "grad_b_pl", 3 * P),
+             _ptr(GRD_xc_b, self.mode, "GRD_xc_b", 9 * R), _ptr(GRD_xc_b_pl, self.mode, "GRD_xc_b_pl", 3 * P)]
+        a += self._args([(q_a_b, "q_a_b", 6 * R), (q_a_b_pl, "q_a_b_pl", P), (grad, "grad", 3 * R),
+                         (grad_pl, "grad_pl", 3 * P), (cmask, "cmask", 6 * R), (cmask_pl, "cmask_pl", P),
+                         (GRD_xc, "GRD_xc", 9 * R), (GRD_xc_pl, "GRD_xc_pl", 3 * P),
+                         (GRD_x, "GRD_x", 3 * self.gall * self.lall), (GRD_x_pl, "GRD_x_pl", 6 * self.gall_pl)])
+        _check(self.lib.nicam_horizontal_remap_adjoint(self.h, *a), "nicam_horizontal_remap_adjoint")
+
+    def tracer_update_adjoint(self, flx_h_b, flx_h_b_pl, q_a_b, q_a_b_pl, rhogq_b, rhogq_b_pl, flx_h, flx_h_pl, q_a,
+                              q_a_pl) -> None:
+        R, P = self.nreg, self.npl
+        a = [_ptr(flx_h_b, self.mode, "flx_h_b", 6 * R), _ptr(flx_h_b_pl, self.mode, "flx_h_b_pl", P),
+             _ptr(q_a_b, self.mode, "q_a_b", 6 * R), _ptr(q_a_b_pl, self.mode, "q_a_b_pl", P)]
+        a += self._args([(rhogq_b, "rhogq_b", R), (rhogq_b_pl, "rhogq_b_pl", P), (flx_h, "flx_h", 6 * R),
+                         (flx_h_pl, "flx_h_pl", P), (q_a, "q_a", 6 * R), (q_a_pl, "q_a_pl", P)])
+        _check(self.lib.nicam_tracer_update_adjoint(self.h, *a), "nicam_tracer_update_adjoint")
 
     def tracer_mass(self, rhogq, rhogq_pl, mass, single: bool = False) -> None:
         """nicam_tracer_mass(_f32): mass[k] = FP64 sum of AREA*rhogq over the inner cells and pole centres."""

@@ -5,26 +5,28 @@ One step advances ntr tracers q and the density rhog of an icosahedral grid
 (all regions and both poles on one GPU) over fixed winds:
 
     horizontal_flux(rhog, rho*v)              -> flx_h, GRD_xc    [HIP; adjoint HIP]
-    ch = flx_h / rhog, cmask = upwind mask                          [glue]
-    grad = gradient(q), halo exchange; q_a = upwind remap            [glue]
+    ch = flx_h / rhog, cmask = upwind mask                          [HIP; adjoint HIP]
+    grad = gradient(q), halo exchange; q_a = upwind remap            [HIP; adjoint HIP]
     Qout = phase 1 of the limiter, halo exchange, q_a limited        [HIP; adjoint HIP]
-    rhog*q and rhog updated by the face fluxes, q = rhog*q / rhog    [glue]
+    rhog*q and rhog updated by the face fluxes, q = rhog*q / rhog    [HIP; adjoint HIP]
     rho*w = vi_rhow_solver(..., rhog0 = rhog, ...)                   [HIP; adjoint HIP]
     flx_v = alpha * rho*w * dt; ck; q_h = half-level interpolation   [glue]
     q_h limited by vertical_limiter_thuburn                          [HIP; adjoint HIP]
     rhog*q and rhog updated by the vertical fluxes; halo exchange    [glue]
 
-The four reference kernels run as torch.autograd.Functions over the
-library's forward kernels (the masked forwards for the limiters) and their
-nicam_*_adjoint entry points.  The glue between them, which the reference
-does not have (its kernels take ch, cmask, q_a, ck, q_h as inputs), is
-written here as torch operations on the same arrays in the oracle's layout
-and operation order, so its derivatives come from torch's autograd.  The
-limiters' adjoints return the cotangents of the Courant numbers ch and ck,
-through which their bounds depend on rhog; what stays fixed is piecewise
-constant (cmask and the inflow flags, the signs of the Courant numbers) or
-zero here (d), so the gradient in q and rhog is the step's own wherever no
-switch sits exactly at its threshold.
+The four reference kernels, the horizontal glue (Courant numbers and masks,
+gradient, remap, update) and the halo exchange run as
+torch.autograd.Functions over the context's forward entry points (the
+masked forwards for the limiters) and their *_adjoint entry points, each
+deterministic; the context is the library (runtime.Context, which needs its
+halo map, nicam_comm_setup) or, in the tests, the oracle.  The vertical glue
+is torch operations on the same arrays in the oracle's layout and operation
+order, so its derivatives come from torch's autograd.  The limiters'
+adjoints return the cotangents of the Courant numbers ch and ck, through
+which their bounds depend on rhog; what stays fixed is piecewise constant
+(cmask and the inflow flags, the signs of the Courant numbers) or zero here
+(d), so the gradient in q and rhog is the step's own wherever no switch sits
+exactly at its threshold.
 
 Arrays are C arrays with the Fortran dimensions reversed: q[t][l][k][g],
 q_pl[t][p][k][v]; rhog[l][k][g], rhog_pl[p][k][v].
@@ -35,7 +37,6 @@ from typing import Dict, List, Sequence, Tuple
 
 import numpy as np
 import torch
-import torch.nn.functional as F
 
 from .autodiff import RhowSolve, VerticalLimiter
 from .drivers import VI_IN
@@ -100,64 +101,139 @@ class HorizontalLimiter(torch.autograd.Function):
         return (None, qab, qab_pl, qb, qb_pl, postb, postb_pl, None, None, chb, chb_pl, None, None)
 
 
-class _HaloGather(torch.autograd.Function):
-    """flat[..., dst] = flat[..., src] with every source read before any
-    write.  Its transpose zeroes the destinations' cotangents and adds each
-    destination's to its source, in rounds whose sources are distinct, so
-    the sums do not depend on the order of atomic additions."""
+def _c(g, ref):
+    """A cotangent autograd may give as None, as a contiguous tensor."""
+    return torch.zeros_like(ref) if g is None else g.contiguous()
+
+
+class Halo(torch.autograd.Function):
+    """COMM_data_transfer of x [vmax][l][k][g], x_pl [vmax][p][k][v] (the
+    library's on-device halo fill, on a copy); its transpose
+    (nicam_comm_data_transfer_adjoint) adds every destination's cotangent to
+    its source's in a fixed order and clears the destination's."""
 
     @staticmethod
-    def forward(fctx, flat, dst, src, rounds):
-        fctx.dst, fctx.rounds = dst, rounds
-        return flat.index_copy(-1, dst, flat.index_select(-1, src))
+    def forward(fctx, nctx: Context, vmax: int, x, x_pl):
+        y, y_pl = x.contiguous().clone(), x_pl.contiguous().clone()
+        nctx.comm_data_transfer(y, y_pl, vmax=vmax, kmax=x.shape[-2])
+        fctx.nctx, fctx.vmax = nctx, vmax
+        return y, y_pl
 
     @staticmethod
-    def backward(fctx, g):
-        out = g.contiguous().index_fill(-1, fctx.dst, 0.0)
-        for d, s in fctx.rounds:
-            out.index_add_(-1, s, g.index_select(-1, d))
-        return out, None, None, None
+    def backward(fctx, g, g_pl):
+        b, b_pl = _c(g, g).clone(), _c(g_pl, g_pl).clone()
+        fctx.nctx.comm_data_transfer_adjoint(b, b_pl, vmax=fctx.vmax, kmax=b.shape[-2])
+        return None, None, b, b_pl
+
+
+class Courant(torch.autograd.Function):
+    """ch = flx_h / rhog and the upwind masks (nicam_tracer_courant); the masks
+    are piecewise constant and carry no cotangent."""
+
+    @staticmethod
+    def forward(fctx, nctx: Context, flx, flx_pl, rhog, rhog_pl):
+        ch, ch_pl = torch.empty_like(flx), torch.empty_like(flx_pl)
+        cm, cm_pl = torch.empty_like(flx), torch.empty_like(flx_pl)
+        nctx.tracer_courant(ch, ch_pl, cm, cm_pl, flx, flx_pl, rhog, rhog_pl)
+        fctx.nctx = nctx
+        fctx.save_for_backward(flx, flx_pl, rhog, rhog_pl)
+        fctx.mark_non_differentiable(cm, cm_pl)
+        return ch, ch_pl, cm, cm_pl
+
+    @staticmethod
+    def backward(fctx, g_ch, g_ch_pl, g_cm, g_cm_pl):
+        flx, flx_pl, rhog, rhog_pl = fctx.saved_tensors
+        fb, fb_pl = torch.empty_like(flx), torch.empty_like(flx_pl)
+        rb, rb_pl = torch.empty_like(rhog), torch.empty_like(rhog_pl)
+        fctx.nctx.tracer_courant_adjoint(fb, fb_pl, rb, rb_pl, _c(g_ch, flx), _c(g_ch_pl, flx_pl), flx, flx_pl, rhog,
+                                         rhog_pl)
+        return None, fb, fb_pl, rb, rb_pl
+
+
+class Gradient(torch.autograd.Function):
+    """grad = the 7-point stencil of coef_grad on q (nicam_oprt_gradient)."""
+
+    @staticmethod
+    def forward(fctx, nctx: Context, cg, cg_pl, q, q_pl):
+        grad = q.new_empty((3,) + q.shape)
+        grad_pl = q_pl.new_empty((3,) + q_pl.shape)
+        nctx.oprt_gradient(grad, grad_pl, q, q_pl, cg, cg_pl)
+        fctx.nctx, fctx.cg, fctx.cg_pl = nctx, cg, cg_pl
+        return grad, grad_pl
+
+    @staticmethod
+    def backward(fctx, g, g_pl):
+        b, b_pl = g.new_empty(g.shape[1:]), g_pl.new_empty(g_pl.shape[1:])
+        fctx.nctx.oprt_gradient_adjoint(b, b_pl, _c(g, g), _c(g_pl, g_pl), fctx.cg, fctx.cg_pl)
+        return None, None, None, b, b_pl
+
+
+class Remap(torch.autograd.Function):
+    """Upwind face values q + grad . (GRD_xc - GRD_x) (nicam_horizontal_remap)
+    into zeroed arrays; cmask and GRD_x held fixed."""
+
+    @staticmethod
+    def forward(fctx, nctx: Context, cm, cm_pl, X, X_pl, q, q_pl, grad, grad_pl, xc, xc_pl):
+        q_a, q_a_pl = q.new_zeros((6,) + q.shape), q_pl.new_zeros(q_pl.shape)
+        nctx.horizontal_remap(q_a, q_a_pl, q, q_pl, grad, grad_pl, cm, cm_pl, xc, xc_pl, X, X_pl)
+        fctx.nctx = nctx
+        fctx.save_for_backward(grad, grad_pl, cm, cm_pl, xc, xc_pl, X, X_pl)
+        return q_a, q_a_pl
+
+    @staticmethod
+    def backward(fctx, g, g_pl):
+        grad, grad_pl, cm, cm_pl, xc, xc_pl, X, X_pl = fctx.saved_tensors
+        qb, qb_pl = grad.new_empty(grad.shape[1:]), grad_pl.new_empty(grad_pl.shape[1:])
+        gb, gb_pl = torch.empty_like(grad), torch.empty_like(grad_pl)
+        xb, xb_pl = torch.empty_like(xc), torch.empty_like(xc_pl)
+        g = qb.new_zeros((6,) + qb.shape) if g is None else g.contiguous()
+        fctx.nctx.horizontal_remap_adjoint(qb, qb_pl, gb, gb_pl, xb, xb_pl, g, _c(g_pl, qb_pl), grad, grad_pl, cm,
+                                           cm_pl, xc, xc_pl, X, X_pl)
+        return None, None, None, None, None, qb, qb_pl, gb, gb_pl, xb, xb_pl
+
+
+class Update(torch.autograd.Function):
+    """rhogq - the sum over the faces of flx_h * q_a on the inner cells and the
+    pole centres (nicam_tracer_update), for one tracer."""
+
+    @staticmethod
+    def forward(fctx, nctx: Context, rq, rq_pl, flx, flx_pl, q_a, q_a_pl):
+        out, out_pl = rq.contiguous().clone(), rq_pl.contiguous().clone()
+        nctx.tracer_update(out, out_pl, flx, flx_pl, q_a, q_a_pl)
+        fctx.nctx = nctx
+        fctx.save_for_backward(flx, flx_pl, q_a, q_a_pl)
+        return out, out_pl
+
+    @staticmethod
+    def backward(fctx, g, g_pl):
+        flx, flx_pl, q_a, q_a_pl = fctx.saved_tensors
+        g, g_pl = _c(g, flx[0]), _c(g_pl, flx_pl)
+        fb, fb_pl = torch.empty_like(flx), torch.empty_like(flx_pl)
+        qb, qb_pl = torch.empty_like(q_a), torch.empty_like(q_a_pl)
+        fctx.nctx.tracer_update_adjoint(fb, fb_pl, qb, qb_pl, g, g_pl, flx, flx_pl, q_a, q_a_pl)
+        return None, g, g_pl, fb, fb_pl, qb, qb_pl
 
 
 class HaloExchange:
-    """COMM_data_transfer as index operations: every halo point and pole ring
-    point takes the value of its source (the library's halo map)."""
+    """COMM_data_transfer through the context: x [..., lall, kall, gall],
+    x_pl [..., 2, kall, gall_pl], the leading dimensions as variables."""
 
-    def __init__(self, hm, lall: int, gall: int, gall_pl: int, device):
-        code = lambda kind, l, g: np.where(kind == 0, l * gall + g, lall * gall + l * gall_pl + g)  # noqa: E731
-        dst = code(hm["dst_kind"], hm["dst_l"], hm["dst_g"]).astype(np.int64)
-        src = code(hm["src_kind"], hm["src_l"], hm["src_g"]).astype(np.int64)
-        self.dst = torch.from_numpy(dst).to(device)
-        self.src = torch.from_numpy(src).to(device)
-        # the r-th pair of each source, in (source, destination) order, goes to round r
-        order = np.lexsort((dst, src))
-        ss, dd = src[order], dst[order]
-        first = np.r_[True, ss[1:] != ss[:-1]]
-        start = np.maximum.accumulate(np.where(first, np.arange(len(ss)), 0))
-        rank = np.arange(len(ss)) - start
-        self.rounds = [(torch.from_numpy(dd[rank == r]).to(device), torch.from_numpy(ss[rank == r]).to(device))
-                       for r in range(int(rank.max()) + 1 if len(ss) else 0)]
-        self.lall, self.gall, self.gall_pl = lall, gall, gall_pl
+    def __init__(self, nctx: Context):
+        self.nctx = nctx
 
     def __call__(self, x, x_pl):
-        """x [..., lall, kall, gall], x_pl [..., 2, kall, gall_pl]."""
         lead = x.shape[:-3]
-        kall = x.shape[-2]
-        a = x.reshape(-1, self.lall, kall, self.gall).permute(0, 2, 1, 3).reshape(-1, kall, self.lall * self.gall)
-        b = x_pl.reshape(-1, 2, kall, self.gall_pl).permute(0, 2, 1, 3).reshape(-1, kall, 2 * self.gall_pl)
-        flat = torch.cat([a, b], dim=-1)
-        out = _HaloGather.apply(flat, self.dst, self.src, self.rounds)
-        a2 = out[..., :self.lall * self.gall].reshape(-1, kall, self.lall, self.gall).permute(0, 2, 1, 3)
-        b2 = out[..., self.lall * self.gall:].reshape(-1, kall, 2, self.gall_pl).permute(0, 2, 1, 3)
-        return (a2.reshape(lead + (self.lall, kall, self.gall)).contiguous(),
-                b2.reshape(lead + (2, kall, self.gall_pl)).contiguous())
+        vmax = int(np.prod(lead)) if lead else 1
+        y, y_pl = Halo.apply(self.nctx, vmax, x.contiguous().reshape((vmax,) + x.shape[-3:]),
+                             x_pl.contiguous().reshape((vmax,) + x_pl.shape[-3:]))
+        return y.reshape(x.shape), y_pl.reshape(x_pl.shape)
 
 
 class TracerStep:
     """The step of the module docstring for one grid; call it with
     (q, q_pl, rhog, rhog_pl) and get the same four back, one step on."""
 
-    def __init__(self, ctx: Context, spec, geo, case, hm, fac, dt: float = 360.0, dt_vi: float = 60.0,
+    def __init__(self, ctx: Context, spec, geo, case, fac, dt: float = 360.0, dt_vi: float = 60.0,
                  courant_v: float = 0.3, device="cuda:0"):
         dev = torch.device(device)
         T = lambda a: torch.from_numpy(np.ascontiguousarray(a, dtype=np.float64)).to(dev)  # noqa: E731
@@ -171,7 +247,7 @@ class TracerStep:
         self.dv, self.dv_pl = T(case["vlim"]["d"]) * 0.0, T(case["vlim"]["d_pl"]) * 0.0
         self.cg, self.cg_pl = T(geo["coef_grad"]), T(geo["coef_grad_pl"])
         self.X, self.X_pl = T(geo["GRD_x"][:, :, 0, :]), T(geo["GRD_x_pl"][:, :, 0, :])  # [d][l][g], [d][p][v]
-        self.halo = HaloExchange(hm, spec.lall, spec.gall, spec.gall_pl, dev)
+        self.halo = HaloExchange(ctx)  # the context holds the halo map (nicam_comm_setup)
         vi = case["vi"]
         self.vi = {k + s: T(vi[k + s]) for k in VI_IN for s in ("", "_pl")}
         self.rhogw_in, self.rhogw_in_pl = T(vi["rhogw"]), T(vi["rhogw_pl"])
@@ -179,8 +255,6 @@ class TracerStep:
         inner = np.zeros((n1, n1), dtype=bool)
         inner[1:-1, 1:-1] = True
         self.inner = torch.from_numpy(inner.reshape(-1)).to(dev)
-        self.glo, self.ghi = n1 + 1, spec.gall - n1 - 1  # the flat range gmin..gmax
-        self.goff = (0, 1, n1 + 1, n1, -1, -n1 - 1, -n1)
         kk = torch.arange(kall, device=dev)
         self.kin = ((kk >= 1) & (kk <= kall - 2)).to(torch.float64)[:, None]        # kmin..kmax
         self.kface = ((kk >= 2) & (kk <= kall - 2)).to(torch.float64)[:, None]      # interior half levels
@@ -202,78 +276,22 @@ class TracerStep:
         return a
 
     def courant(self, flx, flx_pl, rhog, rhog_pl):
-        ch = flx / rhog
-        ch_pl = torch.cat([torch.zeros_like(rhog_pl[..., :1]), flx_pl[..., 1:] / rhog_pl[..., :1]], dim=-1)
-        cm = (0.5 - torch.copysign(torch.full_like(ch, 0.5), ch - self.eps)).detach()
-        cm_pl = (0.5 - torch.copysign(torch.full_like(ch_pl, 0.5), ch_pl - self.eps)).detach()
-        cm_pl[..., 0] = 0.0
-        return ch, ch_pl, cm, cm_pl
+        return Courant.apply(self.ctx, flx, flx_pl, rhog, rhog_pl)
 
     def gradient(self, q, q_pl):
         """q [l][k][g] -> grad [3][l][k][g] on the flat range gmin..gmax (zero
-        elsewhere); the pole centre from its ring.  Slices and padding, not
-        index tensors: their transposes are gathers, so the gradient's sums do
-        not depend on the order of atomic additions."""
-        lo, hi = self.glo, self.ghi
-        parts = []
-        for d in range(3):
-            s = None
-            for m in range(7):
-                o = self.goff[m]
-                t = self.cg[:, d, m, lo:hi][:, None, :] * q[:, :, lo + o:hi + o]
-                s = t if s is None else s + t
-            parts.append(F.pad(s, (lo, self.spec.gall - hi)))
-        gp = []
-        for d in range(3):
-            s = (self.cg_pl[:, d, :][:, None, :] * q_pl).sum(-1)
-            gp.append(torch.cat([s[..., None], torch.zeros_like(q_pl[..., 1:])], dim=-1))
-        return torch.stack(parts), torch.stack(gp)
+        elsewhere); the pole centre from its ring."""
+        return Gradient.apply(self.ctx, self.cg, self.cg_pl, q.contiguous(), q_pl.contiguous())
 
     def remap(self, q, q_pl, grad, grad_pl, cm, cm_pl, xc, xc_pl):
         """Upwind face values q + grad.(GRD_xc - GRD_x) of every arc from the
-        origins i, j in [gmin-1, gmax]; the faces no arc reaches stay 0.  The
-        origins and their neighbours are slices of the (j, i) view."""
-        n = self.n1
-        sq = lambda x: x.reshape(x.shape[:-1] + (n, n))  # noqa: E731  [..., j, i]
-        org = lambda x: sq(x)[..., :n - 1, :n - 1]  # noqa: E731
-        # neighbour of arc a: (i+1, j), (i+1, j+1), (i, j+1); the padding that puts it back in place
-        nbs = (lambda x: sq(x)[..., :n - 1, 1:], lambda x: sq(x)[..., 1:, 1:], lambda x: sq(x)[..., 1:, :n - 1])
-        pads = ((1, 0, 0, 1), (1, 0, 1, 0), (0, 1, 1, 0))
-        flat = lambda x: x.reshape(x.shape[:-2] + (n * n,))  # noqa: E731
-        planes = [None] * 6
-        for a in range(3):
-            xa = xc[:, a]  # [d][l][k][g]
-
-            def rec(sel):
-                r = None
-                for d in range(3):
-                    t = sel(grad[d]) * (org(xa[d]) - sel(self.X[d])[:, None])
-                    r = t if r is None else r + t
-                return sel(q) + r
-
-            qs, qn = rec(org), rec(nbs[a])
-            cs, cn = org(cm[a]), nbs[a](cm[a + 3])
-            planes[a] = flat(F.pad(cs * qn + (1.0 - cs) * qs, (0, 1, 0, 1)))
-            planes[a + 3] = flat(F.pad(cn * qs + (1.0 - cn) * qn, pads[a]))
-        q_a = torch.stack(planes)
-        rec_pl = []
-        for v in range(1, self.spec.gall_pl):
-            x = xc_pl[:, :, :, v]  # [d][p][k]
-            qs = q_pl[:, :, 0] + sum(grad_pl[d][:, :, 0] * (x[d] - self.X_pl[d][:, None, 0]) for d in range(3))
-            qn = q_pl[:, :, v] + sum(grad_pl[d][:, :, v] * (x[d] - self.X_pl[d][:, None, v]) for d in range(3))
-            c = cm_pl[:, :, v]
-            rec_pl.append(c * qn + (1.0 - c) * qs)
-        q_a_pl = torch.stack([torch.zeros_like(q_pl[..., 0])] + rec_pl, dim=-1)
-        return q_a, q_a_pl
+        origins i, j in [gmin-1, gmax]; the faces no arc reaches stay 0."""
+        return Remap.apply(self.ctx, cm, cm_pl, self.X, self.X_pl, q.contiguous(), q_pl.contiguous(),
+                           grad.contiguous(), grad_pl.contiguous(), xc, xc_pl)
 
     def hupdate(self, rq, rq_pl, flx, flx_pl, q_a, q_a_pl):
-        s = flx[0] * q_a[0]
-        for m in range(1, 6):
-            s = s + flx[m] * q_a[m]
-        out = torch.where(self.inner, rq - s, rq)
-        sp = (flx_pl[..., 1:] * q_a_pl[..., 1:]).sum(-1)
-        out_pl = torch.cat([(rq_pl[..., 0] - sp)[..., None], rq_pl[..., 1:]], dim=-1)
-        return out, out_pl
+        """rq [l][k][g] (one tracer) less the sum of flx * q_a over the faces."""
+        return Update.apply(self.ctx, rq, rq_pl, flx, flx_pl, q_a.contiguous(), q_a_pl.contiguous())
 
     def vertical(self, q, q_pl, rq, rq_pl, rhog, rhog_pl, flx_v, flx_v_pl):
         """ck, q_h, the vertical limiter, the flux-form update of rho*q and rho."""
@@ -339,7 +357,8 @@ class TracerStep:
                 q_a, q_a_pl = self.face_values(*args)
             qas.append(q_a)
             qas_pl.append(q_a_pl)
-        rq, rq_pl = self.hupdate(rhog * q, rhog_pl * q_pl, flx, flx_pl, torch.stack(qas, 1), torch.stack(qas_pl, 0))
+        rqs = [self.hupdate(rhog * q[t], rhog_pl * q_pl[t], flx, flx_pl, qas[t], qas_pl[t]) for t in range(ntr)]
+        rq, rq_pl = torch.stack([a for a, _ in rqs]), torch.stack([b for _, b in rqs])
         ones, ones_pl = torch.ones_like(flx), torch.ones_like(flx_pl)
         rhog1, rhog1_pl = self.hupdate(rhog, rhog_pl, flx, flx_pl, ones, ones_pl)
         q1, q1_pl = rq / rhog1, rq_pl / rhog1_pl

@@ -438,6 +438,34 @@ int nicam_horizontal_remap(nicam_ctx ctx, double *q_a, double *q_a_pl, const dou
 int nicam_tracer_update(nicam_ctx ctx, double *rhogq, double *rhogq_pl, const double *flx_h, const double *flx_h_pl,
                         const double *q_a, const double *q_a_pl);
 
+/* Adjoints of the four above and of the halo fill, for the differentiable
+ * tracer step.  No reference counterpart (the reference has no glue).  Each
+ * overwrites its outputs (the *_b arrays it writes) with the transpose of its
+ * forward's derivative applied to the given cotangents, cmask held fixed;
+ * every output is summed by one work-item in a fixed order, so the results are
+ * deterministic.  nicam_oprt_gradient_adjoint: scl_b from grad_b.
+ * nicam_tracer_courant_adjoint: flx_h_b = ch_b / rhog and rhog_b =
+ * -(sum of ch_b * ch) / rhog.  nicam_horizontal_remap_adjoint: q_b, grad_b and
+ * GRD_xc_b from q_a_b (entries the forward does not write get none).
+ * nicam_tracer_update_adjoint: flx_h_b and q_a_b from rhogq_b (rhogq's own
+ * cotangent passes through unchanged).  nicam_comm_data_transfer_adjoint, in
+ * place: every halo destination's cotangent is added to its source's and then
+ * cleared (nicam_comm_setup first). */
+int nicam_oprt_gradient_adjoint(nicam_ctx ctx, double *scl_b, double *scl_b_pl, const double *grad_b,
+                                const double *grad_b_pl, const double *coef_grad, const double *coef_grad_pl);
+int nicam_tracer_courant_adjoint(nicam_ctx ctx, double *flx_h_b, double *flx_h_b_pl, double *rhog_b,
+                                 double *rhog_b_pl, const double *ch_b, const double *ch_b_pl, const double *flx_h,
+                                 const double *flx_h_pl, const double *rhog, const double *rhog_pl);
+int nicam_horizontal_remap_adjoint(nicam_ctx ctx, double *q_b, double *q_b_pl, double *grad_b, double *grad_b_pl,
+                                   double *GRD_xc_b, double *GRD_xc_b_pl, const double *q_a_b, const double *q_a_b_pl,
+                                   const double *grad, const double *grad_pl, const double *cmask,
+                                   const double *cmask_pl, const double *GRD_xc, const double *GRD_xc_pl,
+                                   const double *GRD_x, const double *GRD_x_pl);
+int nicam_tracer_update_adjoint(nicam_ctx ctx, double *flx_h_b, double *flx_h_b_pl, double *q_a_b, double *q_a_b_pl,
+                                const double *rhogq_b, const double *rhogq_b_pl, const double *flx_h,
+                                const double *flx_h_pl, const double *q_a, const double *q_a_pl);
+int nicam_comm_data_transfer_adjoint(nicam_ctx ctx, double *var_b, double *var_b_pl, int kmax, int vmax);
+
 /* FP32 variants of the flux and the two limiters: the same arguments as the
  * FP64 entry points above, every array and dt in single precision.  They
  * reproduce, bit for bit, the reference's own kernels built with -DSINGLE

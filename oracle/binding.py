@@ -99,6 +99,10 @@ class Oracle:
         L.orc_tracer_courant.argtypes = [POINTER(OrcDims)] + [P] * 8
         L.orc_horizontal_remap.argtypes = [POINTER(OrcDims)] + [P] * 12
         L.orc_tracer_update.argtypes = [POINTER(OrcDims)] + [P] * 6
+        L.orc_oprt_gradient_adjoint.argtypes = [POINTER(OrcDims)] + [P] * 6
+        L.orc_tracer_courant_adjoint.argtypes = [POINTER(OrcDims)] + [P] * 10
+        L.orc_horizontal_remap_adjoint.argtypes = [POINTER(OrcDims)] + [P] * 16
+        L.orc_tracer_update_adjoint.argtypes = [POINTER(OrcDims)] + [P] * 10
         L.orc_vertical_mass_flux.argtypes = [POINTER(OrcDims)] + [P] * 16 + [c_double]
         L.orc_vertical_courant.argtypes = [POINTER(OrcDims)] + [P] * 7
         L.orc_vertical_face.argtypes = [POINTER(OrcDims)] + [P] * 6
@@ -169,6 +173,44 @@ class Oracle:
         out = dict(rhogq=_f64(rhogq), rhogq_pl=_f64(rhogq_pl))
         ins = [_f64(a) for a in (flx_h, flx_h_pl, q_a, q_a_pl)]
         self.lib.orc_tracer_update(ctypes.byref(D), _p(out["rhogq"]), _p(out["rhogq_pl"]), *[_p(a) for a in ins])
+        return out
+
+    # adjoints of the horizontal glue (nicam_oracle.c orc_*_adjoint): outputs overwritten
+    def oprt_gradient_adjoint(self, spec, grad_b, grad_b_pl, coef_grad, coef_grad_pl):
+        D = self.dims(spec)
+        out = dict(scl_b=np.zeros(spec.shape3), scl_b_pl=np.zeros(spec.shape_pl))
+        ins = [_f64(a) for a in (grad_b, grad_b_pl, coef_grad, coef_grad_pl)]
+        self.lib.orc_oprt_gradient_adjoint(ctypes.byref(D), _p(out["scl_b"]), _p(out["scl_b_pl"]),
+                                           *[_p(a) for a in ins])
+        return out
+
+    def tracer_courant_adjoint(self, spec, ch_b, ch_b_pl, flx_h, flx_h_pl, rhog, rhog_pl):
+        D = self.dims(spec)
+        out = dict(flx_b=np.zeros((6,) + spec.shape3), flx_b_pl=np.zeros(spec.shape_pl),
+                   rhog_b=np.zeros(spec.shape3), rhog_b_pl=np.zeros(spec.shape_pl))
+        ins = [_f64(a) for a in (ch_b, ch_b_pl, flx_h, flx_h_pl, rhog, rhog_pl)]
+        self.lib.orc_tracer_courant_adjoint(ctypes.byref(D), _p(out["flx_b"]), _p(out["flx_b_pl"]),
+                                            _p(out["rhog_b"]), _p(out["rhog_b_pl"]), *[_p(a) for a in ins])
+        return out
+
+    def horizontal_remap_adjoint(self, spec, qa_b, qa_b_pl, grad, grad_pl, cmask, cmask_pl, xc, xc_pl, x, x_pl):
+        D = self.dims(spec)
+        out = dict(q_b=np.zeros(spec.shape3), q_b_pl=np.zeros(spec.shape_pl), grad_b=np.zeros((3,) + spec.shape3),
+                   grad_b_pl=np.zeros((3,) + spec.shape_pl), xc_b=np.zeros((3, 3) + spec.shape3),
+                   xc_b_pl=np.zeros((3,) + spec.shape_pl))
+        ins = [_f64(a) for a in (qa_b, qa_b_pl, grad, grad_pl, cmask, cmask_pl, xc, xc_pl, x, x_pl)]
+        self.lib.orc_horizontal_remap_adjoint(ctypes.byref(D), *[_p(out[k]) for k in ("q_b", "q_b_pl", "grad_b",
+                                                                                      "grad_b_pl", "xc_b", "xc_b_pl")],
+                                              *[_p(a) for a in ins])
+        return out
+
+    def tracer_update_adjoint(self, spec, rhogq_b, rhogq_b_pl, flx_h, flx_h_pl, q_a, q_a_pl):
+        D = self.dims(spec)
+        out = dict(flx_b=np.zeros((6,) + spec.shape3), flx_b_pl=np.zeros(spec.shape_pl),
+                   qa_b=np.zeros((6,) + spec.shape3), qa_b_pl=np.zeros(spec.shape_pl))
+        ins = [_f64(a) for a in (rhogq_b, rhogq_b_pl, flx_h, flx_h_pl, q_a, q_a_pl)]
+        self.lib.orc_tracer_update_adjoint(ctypes.byref(D), _p(out["flx_b"]), _p(out["flx_b_pl"]), _p(out["qa_b"]),
+                                           _p(out["qa_b_pl"]), *[_p(a) for a in ins])
         return out
 
     def vertical_mass_flux(self, spec, f, m, dt):

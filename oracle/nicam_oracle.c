@@ -1073,6 +1073,168 @@ void orc_tracer_update(const orc_dims *D, double *rhogq, double *rhogq_pl, const
             P3(rhogq_pl, GSLF_PL, k, l) = P3(rhogq_pl, GSLF_PL, k, l) - s;
         }
 }
+/* Adjoints of the horizontal glue, for the differentiable step.  Each
+ * overwrites its outputs with the transpose of its forward's derivative
+ * applied to the given cotangents, summed in the order written here (the
+ * order the GPU kernels reproduce).  Parity unpinned against the reference
+ * (it has no glue); pinned by dot-product tests against the forwards above. */
+
+/* scl_b(g) = sum over d = 1..3, m = 0..6 of coef_grad(g', m, d) grad_b(g', d),
+ * g' = g - offset(m) in [gmin, gmax]; pole: scl_b_pl(v) = sum over d of
+ * coef_grad_pl(v-1, d) grad_b_pl(centre, d). */
+void orc_oprt_gradient_adjoint(const orc_dims *D, double *scl_b, double *scl_b_pl, const double *grad_b,
+                               const double *grad_b_pl, const double *coef_grad, const double *coef_grad_pl)
+{
+    const int iall = D->gall_1d, gall = iall * iall, kall = D->kall, lall = D->lall, gall_pl = D->gall_pl;
+    const int gmin = (2 - 1) * iall + 2, gmax = (iall - 1 - 1) * iall + iall - 1;
+    const int off[7] = {0, 1, iall + 1, iall, -1, -iall - 1, -iall};
+    for (int l = 1; l <= lall; ++l)
+        for (int k = 1; k <= kall; ++k)
+            for (int g = 1; g <= gall; ++g) {
+                double s = 0.0;
+                for (int d = 1; d <= 3; ++d)
+                    for (int m = 0; m < 7; ++m) {
+                        const int gp = g - off[m];
+                        if (gp >= gmin && gp <= gmax) s = s + CG(gp, m, d, l) * F4(grad_b, gp, k, l, d);
+                    }
+                F3(scl_b, g, k, l) = s;
+            }
+    for (int l = 1; l <= LALL_PL; ++l)
+        for (int k = 1; k <= kall; ++k)
+            for (int v = 1; v <= gall_pl; ++v) {
+                double s = 0.0;
+                if (D->have_pl)
+                    for (int d = 1; d <= 3; ++d) s = s + CG_PL(v - 1, d, l) * P4(grad_b_pl, GSLF_PL, k, l, d);
+                P3(scl_b_pl, v, k, l) = s;
+            }
+}
+
+/* flx_b = ch_b / rhog; rhog_b = -(sum over faces of ch_b * (flx/rhog)) / rhog;
+ * the pole ring against the centre's rhog_pl (cmask is piecewise constant). */
+void orc_tracer_courant_adjoint(const orc_dims *D, double *flx_b, double *flx_b_pl, double *rhog_b, double *rhog_b_pl,
+                                const double *ch_b, const double *ch_b_pl, const double *flx_h, const double *flx_h_pl,
+                                const double *rhog, const double *rhog_pl)
+{
+    const int iall = D->gall_1d, gall = iall * iall, kall = D->kall, lall = D->lall, gall_pl = D->gall_pl;
+    for (int l = 1; l <= lall; ++l)
+        for (int k = 1; k <= kall; ++k)
+            for (int g = 1; g <= gall; ++g) {
+                const double r = F3(rhog, g, k, l);
+                double t = 0.0;
+                for (int m = 1; m <= 6; ++m) {
+                    const double b = F4(ch_b, g, k, l, m);
+                    F4(flx_b, g, k, l, m) = b / r;
+                    t = t + b * (F4(flx_h, g, k, l, m) / r);
+                }
+                F3(rhog_b, g, k, l) = -t / r;
+            }
+    for (int l = 1; l <= LALL_PL; ++l)
+        for (int k = 1; k <= kall; ++k) {
+            for (int v = 1; v <= gall_pl; ++v) {
+                P3(flx_b_pl, v, k, l) = 0.0;
+                P3(rhog_b_pl, v, k, l) = 0.0;
+            }
+            if (!D->have_pl) continue;
+            const double r = P3(rhog_pl, GSLF_PL, k, l);
+            double t = 0.0;
+            for (int v = GMIN_PL; v <= gall_pl; ++v) {
+                const double b = P3(ch_b_pl, v, k, l);
+                P3(flx_b_pl, v, k, l) = b / r;
+                t = t + b * (P3(flx_h_pl, v, k, l) / r);
+            }
+            P3(rhog_b_pl, GSLF_PL, k, l) = -t / r;
+        }
+}
+
+/* The transpose of orc_horizontal_remap in q, grad and GRD_xc (cmask fixed):
+ * for every arc, in the forward's loop order, the cotangents of its two faces
+ * fa (the origin's face a) and fb (the neighbour's face a+3) give those of its
+ * two nodes, qs_b = (1-cs) fa + cn fb and qn_b = cs fa + (1-cn) fb, which are
+ * added to q_b and grad_b of the origin and the neighbour; GRD_xc_b of the arc
+ * is qs_b grad(origin) + qn_b grad(neighbour).  Other entries 0. */
+void orc_horizontal_remap_adjoint(const orc_dims *D, double *q_b, double *q_b_pl, double *grad_b, double *grad_b_pl,
+                                  double *xc_b, double *xc_b_pl, const double *qa_b, const double *qa_b_pl,
+                                  const double *grad, const double *grad_pl, const double *cmask,
+                                  const double *cmask_pl, const double *GRD_xc, const double *GRD_xc_pl,
+                                  const double *GRD_x, const double *GRD_x_pl)
+{
+    const int iall = D->gall_1d, gall = iall * iall, kall = D->kall, lall = D->lall, gall_pl = D->gall_pl;
+    const size_t n3 = (size_t)gall * kall * lall, n3p = (size_t)gall_pl * kall * LALL_PL;
+    for (size_t x = 0; x < n3; ++x) q_b[x] = 0.0;
+    for (size_t x = 0; x < 3 * n3; ++x) grad_b[x] = 0.0;
+    for (size_t x = 0; x < 9 * n3; ++x) xc_b[x] = 0.0;
+    for (size_t x = 0; x < n3p; ++x) q_b_pl[x] = 0.0;
+    for (size_t x = 0; x < 3 * n3p; ++x) grad_b_pl[x] = 0.0;
+    for (size_t x = 0; x < 3 * n3p; ++x) xc_b_pl[x] = 0.0;
+    for (int l = 1; l <= lall; ++l)
+        for (int k = 1; k <= kall; ++k)
+            for (int j = 1; j <= iall - 1; ++j)
+                for (int i = 1; i <= iall - 1; ++i) {
+                    const int ij = (j - 1) * iall + i;
+                    const int nbs[3] = {ij + 1, ij + iall + 1, ij + iall};
+                    for (int a = 1; a <= 3; ++a) {
+                        const int nb = nbs[a - 1];
+                        const double fa = F4(qa_b, ij, k, l, a), fb = F4(qa_b, nb, k, l, a + 3);
+                        const double cs = F4(cmask, ij, k, l, a), cn = F4(cmask, nb, k, l, a + 3);
+                        const double qs_b = (1.0 - cs) * fa + cn * fb;
+                        const double qn_b = cs * fa + (1.0 - cn) * fb;
+                        F3(q_b, ij, k, l) = F3(q_b, ij, k, l) + qs_b;
+                        F3(q_b, nb, k, l) = F3(q_b, nb, k, l) + qn_b;
+                        for (int d = 1; d <= 3; ++d) {
+                            const double x = F5(GRD_xc, ij, k, l, a, d);
+                            F4(grad_b, ij, k, l, d) = F4(grad_b, ij, k, l, d) + qs_b * (x - GX(ij, l, d));
+                            F4(grad_b, nb, k, l, d) = F4(grad_b, nb, k, l, d) + qn_b * (x - GX(nb, l, d));
+                            F5(xc_b, ij, k, l, a, d) = qs_b * F4(grad, ij, k, l, d) + qn_b * F4(grad, nb, k, l, d);
+                        }
+                    }
+                }
+    if (!D->have_pl) return;
+    for (int l = 1; l <= LALL_PL; ++l)
+        for (int k = 1; k <= kall; ++k)
+            for (int v = GMIN_PL; v <= gall_pl; ++v) {
+                const int n = GSLF_PL;
+                const double f = P3(qa_b_pl, v, k, l), c = P3(cmask_pl, v, k, l);
+                const double qs_b = (1.0 - c) * f, qn_b = c * f;
+                P3(q_b_pl, n, k, l) = P3(q_b_pl, n, k, l) + qs_b;
+                P3(q_b_pl, v, k, l) = P3(q_b_pl, v, k, l) + qn_b;
+                for (int d = 1; d <= 3; ++d) {
+                    const double x = P4(GRD_xc_pl, v, k, l, d);
+                    P4(grad_b_pl, n, k, l, d) = P4(grad_b_pl, n, k, l, d) + qs_b * (x - GX_PL(n, l, d));
+                    P4(grad_b_pl, v, k, l, d) = P4(grad_b_pl, v, k, l, d) + qn_b * (x - GX_PL(v, l, d));
+                    P4(xc_b_pl, v, k, l, d) = qs_b * P4(grad_pl, n, k, l, d) + qn_b * P4(grad_pl, v, k, l, d);
+                }
+            }
+}
+
+/* The transpose of orc_tracer_update in flx_h and q_a (rhogq's cotangent passes
+ * through unchanged): on the inner cells flx_b = -(rhogq_b q_a), q_a_b =
+ * -(rhogq_b flx_h), 0 elsewhere; the pole ring against the centre's rhogq_b. */
+void orc_tracer_update_adjoint(const orc_dims *D, double *flx_b, double *flx_b_pl, double *qa_b, double *qa_b_pl,
+                               const double *rhogq_b, const double *rhogq_b_pl, const double *flx_h,
+                               const double *flx_h_pl, const double *q_a, const double *q_a_pl)
+{
+    const int iall = D->gall_1d, gall = iall * iall, kall = D->kall, lall = D->lall, gall_pl = D->gall_pl;
+    for (int m = 1; m <= 6; ++m)
+        for (int l = 1; l <= lall; ++l)
+            for (int k = 1; k <= kall; ++k)
+                for (int j = 1; j <= iall; ++j)
+                    for (int i = 1; i <= iall; ++i) {
+                        const int g = (j - 1) * iall + i;
+                        const int inner = i >= 2 && i <= iall - 1 && j >= 2 && j <= iall - 1;
+                        const double b = F3(rhogq_b, g, k, l);
+                        F4(flx_b, g, k, l, m) = inner ? -(b * F4(q_a, g, k, l, m)) : 0.0;
+                        F4(qa_b, g, k, l, m) = inner ? -(b * F4(flx_h, g, k, l, m)) : 0.0;
+                    }
+    for (int l = 1; l <= LALL_PL; ++l)
+        for (int k = 1; k <= kall; ++k) {
+            const double b = D->have_pl ? P3(rhogq_b_pl, GSLF_PL, k, l) : 0.0;
+            for (int v = 1; v <= gall_pl; ++v) {
+                const int ring = D->have_pl && v >= GMIN_PL;
+                P3(flx_b_pl, v, k, l) = ring ? -(b * P3(q_a_pl, v, k, l)) : 0.0;
+                P3(qa_b_pl, v, k, l) = ring ? -(b * P3(flx_h_pl, v, k, l)) : 0.0;
+            }
+        }
+}
 #undef CG
 #undef CG_PL
 #undef GX

@@ -163,6 +163,21 @@ void orc_horizontal_remap(const orc_dims *D, double *q_a, double *q_a_pl, const 
                           const double *GRD_x_pl);
 void orc_tracer_update(const orc_dims *D, double *rhogq, double *rhogq_pl, const double *flx_h,
                        const double *flx_h_pl, const double *q_a, const double *q_a_pl);
+/* Adjoints of the four above (the transposes of their derivatives; cmask
+ * fixed), for the differentiable step; outputs overwritten. */
+void orc_oprt_gradient_adjoint(const orc_dims *D, double *scl_b, double *scl_b_pl, const double *grad_b,
+                               const double *grad_b_pl, const double *coef_grad, const double *coef_grad_pl);
+void orc_tracer_courant_adjoint(const orc_dims *D, double *flx_b, double *flx_b_pl, double *rhog_b, double *rhog_b_pl,
+                                const double *ch_b, const double *ch_b_pl, const double *flx_h, const double *flx_h_pl,
+                                const double *rhog, const double *rhog_pl);
+void orc_horizontal_remap_adjoint(const orc_dims *D, double *q_b, double *q_b_pl, double *grad_b, double *grad_b_pl,
+                                  double *xc_b, double *xc_b_pl, const double *qa_b, const double *qa_b_pl,
+                                  const double *grad, const double *grad_pl, const double *cmask,
+                                  const double *cmask_pl, const double *GRD_xc, const double *GRD_xc_pl,
+                                  const double *GRD_x, const double *GRD_x_pl);
+void orc_tracer_update_adjoint(const orc_dims *D, double *flx_b, double *flx_b_pl, double *qa_b, double *qa_b_pl,
+                               const double *rhogq_b, const double *rhogq_b_pl, const double *flx_h,
+                               const double *flx_h_pl, const double *q_a, const double *q_a_pl);
 
 int orc_num_threads(void);
 

@@ -1,11 +1,13 @@
 """A stand-in for runtime.Context on CPU tensors whose kernels are the
 oracle's C restatement: the flux, both limiters (masked forwards and
-adjoints, with the Courant numbers' cotangents) and the rho*w solve with its
-adjoint.  It lets the tests run tracer_ad.TracerStep, the end-to-end
+adjoints, with the Courant numbers' cotangents), the rho*w solve, the
+horizontal glue, each with its adjoint, and the halo fill of a halo map
+(halo_sim) with its transpose.  It lets the tests run tracer_ad.TracerStep, the end-to-end
 gradient harness, on the CPU, and compare the GPU harness with it."""
 import numpy as np
 import torch
 
+from halo_sim import apply_map, apply_map_adjoint
 from test_adjoint import pack_hlim_mask, pack_vlim_mask
 
 FLUX = ("rho", "rho_pl", "rhovx", "rhovx_pl", "rhovy", "rhovy_pl", "rhovz", "rhovz_pl")
@@ -21,8 +23,8 @@ def _put(t, a):
 
 
 class OracleContext:
-    def __init__(self, oracle, spec, geo, vi, fac):
-        self.orc, self.spec, self.geo = oracle, spec, geo
+    def __init__(self, oracle, spec, geo, vi, fac, hm):
+        self.orc, self.spec, self.geo, self.hm = oracle, spec, geo, hm
         self.sgp = [int(x) for x in geo["have_sgp"]]
         self.vi = vi
         self.fac = (fac["rdgzh"], fac["afact"], fac["bfact"])
@@ -32,6 +34,65 @@ class OracleContext:
 
     def close(self):
         pass
+
+    # ---- halo fill and the horizontal glue ----------------------------------------
+    def comm_data_transfer(self, var, var_pl, vmax=1, kmax=None):
+        a, b = apply_map(self.hm, _np(var).reshape((vmax,) + var.shape[-3:]),
+                         _np(var_pl).reshape((vmax,) + var_pl.shape[-3:]))
+        _put(var, a)
+        _put(var_pl, b)
+
+    def comm_data_transfer_adjoint(self, var_b, var_b_pl, vmax=1, kmax=None):
+        a, b = apply_map_adjoint(self.hm, _np(var_b).reshape((vmax,) + var_b.shape[-3:]),
+                                 _np(var_b_pl).reshape((vmax,) + var_b_pl.shape[-3:]))
+        _put(var_b, a)
+        _put(var_b_pl, b)
+
+    def tracer_courant(self, ch, ch_pl, cm, cm_pl, flx, flx_pl, rhog, rhog_pl):
+        out = self.orc.tracer_courant(self.spec, _np(flx), _np(flx_pl), _np(rhog), _np(rhog_pl))
+        for t, k in ((ch, "ch"), (ch_pl, "ch_pl"), (cm, "cmask"), (cm_pl, "cmask_pl")):
+            _put(t, out[k])
+
+    def tracer_courant_adjoint(self, fb, fb_pl, rb, rb_pl, chb, chb_pl, flx, flx_pl, rhog, rhog_pl):
+        out = self.orc.tracer_courant_adjoint(self.spec, _np(chb), _np(chb_pl), _np(flx), _np(flx_pl), _np(rhog),
+                                              _np(rhog_pl))
+        for t, k in ((fb, "flx_b"), (fb_pl, "flx_b_pl"), (rb, "rhog_b"), (rb_pl, "rhog_b_pl")):
+            _put(t, out[k])
+
+    def oprt_gradient(self, grad, grad_pl, q, q_pl, cg, cg_pl):
+        out = self.orc.oprt_gradient(self.spec, _np(q), _np(q_pl), _np(cg), _np(cg_pl))
+        _put(grad, out["grad"])
+        _put(grad_pl, out["grad_pl"])
+
+    def oprt_gradient_adjoint(self, qb, qb_pl, gb, gb_pl, cg, cg_pl):
+        out = self.orc.oprt_gradient_adjoint(self.spec, _np(gb), _np(gb_pl), _np(cg), _np(cg_pl))
+        _put(qb, out["scl_b"])
+        _put(qb_pl, out["scl_b_pl"])
+
+    def horizontal_remap(self, q_a, q_a_pl, q, q_pl, grad, grad_pl, cm, cm_pl, xc, xc_pl, X, X_pl):
+        out = self.orc.horizontal_remap(self.spec, _np(q_a), _np(q_a_pl), _np(q), _np(q_pl), _np(grad), _np(grad_pl),
+                                        _np(cm), _np(cm_pl), _np(xc), _np(xc_pl), _np(X), _np(X_pl))
+        _put(q_a, out["q_a"])
+        _put(q_a_pl, out["q_a_pl"])
+
+    def horizontal_remap_adjoint(self, qb, qb_pl, gb, gb_pl, xb, xb_pl, qab, qab_pl, grad, grad_pl, cm, cm_pl, xc,
+                                 xc_pl, X, X_pl):
+        out = self.orc.horizontal_remap_adjoint(self.spec, _np(qab), _np(qab_pl), _np(grad), _np(grad_pl), _np(cm),
+                                                _np(cm_pl), _np(xc), _np(xc_pl), _np(X), _np(X_pl))
+        for t, k in ((qb, "q_b"), (qb_pl, "q_b_pl"), (gb, "grad_b"), (gb_pl, "grad_b_pl"), (xb, "xc_b"),
+                     (xb_pl, "xc_b_pl")):
+            _put(t, out[k])
+
+    def tracer_update(self, rq, rq_pl, flx, flx_pl, q_a, q_a_pl):
+        out = self.orc.tracer_update(self.spec, _np(rq), _np(rq_pl), _np(flx), _np(flx_pl), _np(q_a), _np(q_a_pl))
+        _put(rq, out["rhogq"])
+        _put(rq_pl, out["rhogq_pl"])
+
+    def tracer_update_adjoint(self, fb, fb_pl, qab, qab_pl, rqb, rqb_pl, flx, flx_pl, q_a, q_a_pl):
+        out = self.orc.tracer_update_adjoint(self.spec, _np(rqb), _np(rqb_pl), _np(flx), _np(flx_pl), _np(q_a),
+                                             _np(q_a_pl))
+        for t, k in ((fb, "flx_b"), (fb_pl, "flx_b_pl"), (qab, "qa_b"), (qab_pl, "qa_b_pl")):
+            _put(t, out[k])
 
     # ---- horizontal_flux --------------------------------------------------------
     def horizontal_flux(self, flx, flx_pl, xc, xc_pl, *X_dt):

@@ -39,9 +39,9 @@ def _setup(gl, rl, kall, seed, ntr):
     spec = synth.GridSpec(n1, kall, 10 * 4 ** rl, seed=seed)
     case = synth.make_case(spec, geo=geo)
     ctx = drivers.setup_context(spec, geo, case["vi"])
-    hm = runtime.halo_map(rl, n1)
+    ctx.comm_setup(rl)
     fac = binding.Oracle().grd_setup(case["vi"]["gz"], case["vi"]["gzh"])
-    step = TracerStep(ctx, spec, geo, case, hm, fac)
+    step = TracerStep(ctx, spec, geo, case, fac)
     dev = torch.device("cuda:0")
     # smooth tracers on the sphere: low-order functions of the cell positions
     X = geo["GRD_x"][:, :, 0, :] / synth.RADIUS  # [d][l][g]
@@ -70,8 +70,8 @@ def _cpu_setup(oracle, gl, rl, kall, seed, ntr):
     spec = synth.GridSpec(n1, kall, 10 * 4 ** rl, seed=seed)
     case = synth.make_case(spec, geo=geo)
     fac = oracle.grd_setup(case["vi"]["gz"], case["vi"]["gzh"])
-    ctx = OracleContext(oracle, spec, geo, case["vi"], fac)
-    step = TracerStep(ctx, spec, geo, case, runtime.halo_map(rl, n1), fac, device="cpu")
+    ctx = OracleContext(oracle, spec, geo, case["vi"], fac, runtime.halo_map(rl, n1))
+    step = TracerStep(ctx, spec, geo, case, fac, device="cpu")
     X = geo["GRD_x"][:, :, 0, :] / synth.RADIUS
     X_pl = geo["GRD_x_pl"][:, :, 0, :] / synth.RADIUS
     z = np.linspace(0.0, 1.0, kall)[None, :, None]
