@@ -649,11 +649,16 @@ __global__ void __launch_bounds__(HPOLE) hlim_adjoint_pole(HaArgs A);
 
 // four waves per SIMD (the LDS ring allows four work-groups per CU): 1.57 ms at gl7, against 1.87 ms at
 // three and 2.25 ms at two, despite the registers the cap spills
-// CHB: also the cotangent of ch
-template <bool CHB>
+// CHB: also the cotangent of ch.  PASSES: 1 the rows only (the cotangents of
+// Qout_prev; no faces), 2 the faces only (of q_a; Qout_prev_b taken as zero),
+// 3 both.  A pass left out adds nothing, so the two halves give the full
+// pass's results, each bit for bit (its sums start at +0.0 and the other
+// half's terms are +0.0 or -0.0).
+template <bool CHB, int PASSES>
 __global__ void __launch_bounds__(HSW) __attribute__((amdgpu_waves_per_eu(CHB ? 3 : 4, 8))) hlim_adjoint_kernel(HaArgs A)
 {
     extern __shared__ double lds[];
+    constexpr bool ROWS = (PASSES & 1) != 0, FACES = (PASSES & 2) != 0;
     const int iall = A.iall, gall = A.gall, kall = A.kall, lall = A.lall, R = A.ring;
     // q, with the first and last QP slots repeated past either end, so that a read within QP of a
     // cell needs no wrap
@@ -687,20 +692,21 @@ __global__ void __launch_bounds__(HSW) __attribute__((amdgpu_waves_per_eu(CHB ? 
         const bool own = i <= iall - 2 && j <= iall - 2;  // origin of three arcs
         // ---- every global load of the cell before its first store: a load issued after a store
         // to memory it may alias is ordered behind it
-        const double b0 = A.Qprevb[fb + e], b1 = A.Qprevb[plane + fb + e];
+        const double b0 = ROWS ? A.Qprevb[fb + e] : 0.0, b1 = ROWS ? A.Qprevb[plane + fb + e] : 0.0;
         uint32_t bits = 0;
         double cmo[3] = {0.0, 0.0, 0.0}, ob[3] = {0.0, 0.0, 0.0};  // own arcs' cmask, face cotangents
         if (own) {
-            bits = A.mask[fb + e];
+            if constexpr (FACES) bits = A.mask[fb + e];
 #pragma unroll
             for (int m = 0; m < 3; ++m) {
                 const int nb = e + (m == 0 ? 1 : (m == 1 ? iall + 1 : iall));
                 cmo[m] = A.cmask[(size_t)m * plane + fb + e];
-                ob[m] = A.qab[(size_t)m * plane + fb + e] + A.qab[(size_t)(m + 3) * plane + fb + nb];
+                if constexpr (FACES)
+                    ob[m] = A.qab[(size_t)m * plane + fb + e] + A.qab[(size_t)(m + 3) * plane + fb + nb];
             }
         }
         double cmw[3] = {0.0, 0.0, 0.0}, chv[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0}, dc = 0.0;
-        if (inner) {
+        if (ROWS && inner) {
 #pragma unroll
             for (int m = 0; m < 3; ++m)  // the arcs of (i-1,j), (i-1,j-1), (i,j-1)
                 cmw[m] = A.cmask[(size_t)m * plane + fb + e - (m == 0 ? 1 : (m == 1 ? iall + 1 : iall))];
@@ -748,7 +754,7 @@ __global__ void __launch_bounds__(HSW) __attribute__((amdgpu_waves_per_eu(CHB ? 
 
         // ---- faces 1-3 of the own arcs (HLIM:377-410): attribution
         uint32_t fc = 0;
-        if (own) {
+        if (FACES && own) {
 #pragma unroll
             for (int m = 0; m < 3; ++m) {
                 const int nb = e + (m == 0 ? 1 : (m == 1 ? iall + 1 : iall));
@@ -771,6 +777,9 @@ __global__ void __launch_bounds__(HSW) __attribute__((amdgpu_waves_per_eu(CHB ? 
         // ---- the row of Qout (HLIM:216-301), summed per hexagon cell into the cell's seven slots of
         // r_p; the slots belong to this work-item alone, and its LDS adds are done in program order
         double *rp = r_p + se;
+        if constexpr (!ROWS) {
+            return;  // no partial sums: the gather reads none
+        }
         if (!inner) {
 #pragma unroll
             for (int w = 0; w < 7; ++w) rp[w * R] = w == 3 ? (0.0 + b0) + b1 : 0.0;  // halo: Qout = q
@@ -881,11 +890,13 @@ __global__ void __launch_bounds__(HSW) __attribute__((amdgpu_waves_per_eu(CHB ? 
         // Sums start at +0.0 and so never hold -0.0: adding +0.0 in place of a term that does not
         // apply leaves them unchanged, which lets every term below be a select rather than a branch.
         double acc = 0.0;
+        if constexpr (ROWS) {
 #pragma unroll
-        for (int h = 0; h < 7; ++h) {  // the rows of c's hexagon, in cell order: their partial sums for c
-            const int g = c + hx[h];
-            const double v = r_p[(6 - h) * R + sl(g)];
-            acc = acc + ((g >= 0 && g < gall) ? v : 0.0);
+            for (int h = 0; h < 7; ++h) {  // the rows of c's hexagon, in cell order: their partial sums for c
+                const int g = c + hx[h];
+                const double v = r_p[(6 - h) * R + sl(g)];
+                acc = acc + ((g >= 0 && g < gall) ? v : 0.0);
+            }
         }
         double pn = 0.0, px = 0.0;
         // faces of origin p; d: index of c - p (arc_slot's), known after unrolling
@@ -933,7 +944,9 @@ __global__ void __launch_bounds__(HSW) __attribute__((amdgpu_waves_per_eu(CHB ? 
                 }
             }
         };
-        if (c == 0) {
+        if (!FACES) {
+            // no faces: q_b is the rows' alone, Qout_post_b (if asked for) zero
+        } else if (c == 0) {
             // the clamps send arc cells of row 0's origins to cell 0
             for (int p = 0; p <= iall; ++p) faces(p);
         } else {
@@ -942,8 +955,10 @@ __global__ void __launch_bounds__(HSW) __attribute__((amdgpu_waves_per_eu(CHB ? 
             for (int h = 0; h < 7; ++h) faces_at(c + po[h], h);
         }
         A.qb[fb + c] = acc;
-        A.Qpostb[fb + c] = pn;
-        A.Qpostb[plane + fb + c] = px;
+        if (A.Qpostb) {
+            A.Qpostb[fb + c] = pn;
+            A.Qpostb[plane + fb + c] = px;
+        }
     };
 
     // q of cell e into its slot, and into the slot's copy past the ring's other end
@@ -981,11 +996,14 @@ __global__ void __launch_bounds__(HPOLE) hlim_adjoint_pole(HaArgs A)
     const size_t fb = ((size_t)l * kall + k) * npl;
     const size_t pplane = (size_t)kLallPl * kall * npl;
     const double *qs = A.q_pl + fb, *chs = A.ch_pl + fb, *cms = A.cmask_pl + fb;
-    double *xq = A.qb_pl + fb, *xa = A.qab_pl + fb, *xp = A.Qpostb_pl + fb;
+    // the rows run when Qout_prev_b is given, the faces when q_a_b is
+    double *xq = A.qb_pl + fb, *xa = A.qab_pl ? A.qab_pl + fb : nullptr, *xp = A.Qpostb_pl ? A.Qpostb_pl + fb : nullptr;
     for (int v = 0; v < npl; ++v) {
         xq[v] = 0.0;
-        xp[v] = 0.0;
-        xp[pplane + v] = 0.0;
+        if (xp) {
+            xp[v] = 0.0;
+            xp[pplane + v] = 0.0;
+        }
     }
     const double BIG = A.big;
     auto arc = [&](int v, int &il, int &ih, double &lo_v, double &hi_v) {
@@ -1051,7 +1069,7 @@ __global__ void __launch_bounds__(HPOLE) hlim_adjoint_pole(HaArgs A)
             xc[v] = (b0 != 0.0 ? b0 * d0 : 0.0) + (b1 != 0.0 ? b1 * d1 : 0.0);
         }
     }
-    for (int mm = 0; mm < 2; ++mm) {
+    for (int mm = 0; mm < 2 && A.Qprevb_pl; ++mm) {
         const double bq = A.Qprevb_pl[(size_t)mm * pplane + fb];
         const int sel = mm == 0 ? smax : smin;
         xq[0] = xq[0] + (fac + zerosw) * bq;
@@ -1071,7 +1089,7 @@ __global__ void __launch_bounds__(HPOLE) hlim_adjoint_pole(HaArgs A)
             xq[cx] = xq[cx] + (-(fac * (cms[v] * chs[v])) * cms[v]) * bq;
         }
     }
-    for (int v = 1; v < npl; ++v) {
+    for (int v = 1; v < npl && xa; ++v) {
         const double c = cms[v];
         const int T = c > 0.0 ? v : 0;
         int kind = 0, cell = 0;
@@ -1413,13 +1431,25 @@ int launch_hlim_adjoint(nicam_ctx c, double *qab, double *qab_pl, double *qb, do
     if (nslab > 0x7fffffffL) return fail(NICAM_ERR_ARG, "grid too large for one launch");
     const size_t lds = (size_t)(A.ring + 2 * (D.gall_1d + 2)) * sizeof(double) +
                        (size_t)A.ring * (10 * sizeof(double) + sizeof(uint32_t));
-    const void *kern = chb ? (const void *)hlim_adjoint_kernel<true> : (const void *)hlim_adjoint_kernel<false>;
+    // the rows run when Qout_prev_b is given (ch_b comes with them), the faces when q_a_b is
+    const int passes = (Qprevb ? 1 : 0) | (qab ? 2 : 0);
+    const bool wch = chb != nullptr;
+    const void *kern = nullptr;
+    if (passes == 3) kern = wch ? (const void *)hlim_adjoint_kernel<true, 3> : (const void *)hlim_adjoint_kernel<false, 3>;
+    else if (passes == 1) kern = wch ? (const void *)hlim_adjoint_kernel<true, 1> : (const void *)hlim_adjoint_kernel<false, 1>;
+    else kern = (const void *)hlim_adjoint_kernel<false, 2>;
     if (lds > 64 * 1024)
         NICAM_HIP(hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    if (chb)
-        hipLaunchKernelGGL(hlim_adjoint_kernel<true>, dim3((unsigned)nslab), dim3(HSW), lds, c->stream, A);
+    if (passes == 3 && wch)
+        hipLaunchKernelGGL((hlim_adjoint_kernel<true, 3>), dim3((unsigned)nslab), dim3(HSW), lds, c->stream, A);
+    else if (passes == 3)
+        hipLaunchKernelGGL((hlim_adjoint_kernel<false, 3>), dim3((unsigned)nslab), dim3(HSW), lds, c->stream, A);
+    else if (passes == 1 && wch)
+        hipLaunchKernelGGL((hlim_adjoint_kernel<true, 1>), dim3((unsigned)nslab), dim3(HSW), lds, c->stream, A);
+    else if (passes == 1)
+        hipLaunchKernelGGL((hlim_adjoint_kernel<false, 1>), dim3((unsigned)nslab), dim3(HSW), lds, c->stream, A);
     else
-        hipLaunchKernelGGL(hlim_adjoint_kernel<false>, dim3((unsigned)nslab), dim3(HSW), lds, c->stream, A);
+        hipLaunchKernelGGL((hlim_adjoint_kernel<false, 2>), dim3((unsigned)nslab), dim3(HSW), lds, c->stream, A);
     if (D.have_pl)
         hipLaunchKernelGGL(hlim_adjoint_pole, dim3((unsigned)((D.kall * kLallPl + HPOLE - 1) / HPOLE)), dim3(HPOLE),
                            0, c->stream, A);

@@ -990,16 +990,29 @@ int nicam_horizontal_limiter_thuburn_adjoint(nicam_ctx c, double *q_a_b, double 
                                              const uint8_t *mask_pl, double *ch_b, double *ch_b_pl)
 {
     NICAM_TRY(check_ctx(c));
+    // the faces run when q_a_b is given, the rows when Qout_prev_b is; ch_b comes from the rows
+    if (!q_a_b && !Qprev_b) return fail(NICAM_ERR_ARG, "neither q_a_b nor Qout_prev_b given");
+    if (ch_b && !Qprev_b) return fail(NICAM_ERR_ARG, "ch_b needs Qout_prev_b (it comes from the rows)");
     if (ch_b && c->d.have_pl && !ch_b_pl) return fail(NICAM_ERR_ARG, "ch_b given without ch_b_pl");
-    const void *req[] = {q_a_b, q_b, Qprev_b, Qpost_b, q_a, q, d, ch, cmask, Qpost, mask};
-    const char *nm[] = {"q_a_b", "q_b", "Qout_prev_b", "Qout_post_b", "q_a", "q", "d", "ch", "cmask",
-                        "Qout_post", "mask"};
-    for (int i = 0; i < 11; ++i) NICAM_TRY(need(req[i], nm[i]));
-    if (c->d.have_pl) {
-        const void *rp[] = {q_a_b_pl, q_b_pl, Qprev_b_pl, Qpost_b_pl, q_a_pl, q_pl, d_pl, ch_pl, cmask_pl,
-                            Qpost_pl, mask_pl};
-        for (int i = 0; i < 11; ++i) NICAM_TRY(need(rp[i], "pole array"));
+    const void *req[] = {q_b, q, d, ch, cmask};
+    const char *nm[] = {"q_b", "q", "d", "ch", "cmask"};
+    for (int i = 0; i < 5; ++i) NICAM_TRY(need(req[i], nm[i]));
+    if (q_a_b) {
+        const void *fr[] = {Qpost_b, q_a, Qpost, mask};
+        const char *fn[] = {"Qout_post_b", "q_a", "Qout_post", "mask"};
+        for (int i = 0; i < 4; ++i) NICAM_TRY(need(fr[i], fn[i]));
     }
+    if (c->d.have_pl) {
+        const void *rp[] = {q_b_pl, q_pl, d_pl, ch_pl, cmask_pl};
+        for (int i = 0; i < 5; ++i) NICAM_TRY(need(rp[i], "pole array"));
+        if (q_a_b) {
+            const void *fp[] = {q_a_b_pl, Qpost_b_pl, q_a_pl, Qpost_pl, mask_pl};
+            for (int i = 0; i < 5; ++i) NICAM_TRY(need(fp[i], "pole array"));
+        }
+        if (Qprev_b) NICAM_TRY(need(Qprev_b_pl, "pole array"));
+    }
+    if (!q_a_b) q_a_b_pl = Qpost_b = Qpost_b_pl = nullptr;
+    if (!Qprev_b) Qprev_b_pl = nullptr;
     if (c->ptr_mode == NICAM_PTR_DEVICE)
         return launch_hlim_adjoint(c, q_a_b, PL(q_a_b_pl), q_b, PL(q_b_pl), Qprev_b, PL(Qprev_b_pl), Qpost_b,
                                    PL(Qpost_b_pl), q_a, PL(q_a_pl), q, PL(q_pl), d, PL(d_pl), ch, PL(ch_pl), cmask,

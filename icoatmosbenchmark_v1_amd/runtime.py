@@ -741,7 +741,8 @@ class Context:
                                            cmask, cmask_pl, Qout_post, Qout_post_pl, mask, mask_pl, ch_b=None,
                                            ch_b_pl=None) -> None:
         """nicam_horizontal_limiter_thuburn_adjoint (Qout_post supplied); ch_b, ch_b_pl: the Courant
-        numbers' cotangent, formed when given."""
+        numbers' cotangent, formed when given.  q_a_b None: the rows alone (Qout_post_b, q_a, Qout_post and
+        mask may be None too); Qout_prev_b None: the faces alone (no ch_b)."""
         R, P = self.nreg, self.npl
         a = self._args([(q_a_b, "q_a_b", 6 * R), (q_a_b_pl, "q_a_b_pl", P), (q_b, "q_b", R), (q_b_pl, "q_b_pl", P),
                         (Qout_prev_b, "Qout_prev_b", 2 * R), (Qout_prev_b_pl, "Qout_prev_b_pl", 2 * P),

@@ -7,7 +7,7 @@ One step advances ntr tracers q and the density rhog of an icosahedral grid
     horizontal_flux(rhog, rho*v)              -> flx_h, GRD_xc    [HIP; adjoint HIP]
     ch = flx_h / rhog, cmask = upwind mask                          [HIP; adjoint HIP]
     grad = gradient(q), halo exchange; q_a = upwind remap            [HIP; adjoint HIP]
-    Qout = phase 1 of the limiter, halo exchange, q_a limited        [HIP; adjoint HIP]
+    Qout = phase 1 of the limiter, halo exchange, q_a limited        [HIP, one call; adjoint HIP]
     rhog*q and rhog updated by the face fluxes, q = rhog*q / rhog    [HIP; adjoint HIP]
     rho*w = vi_rhow_solver(..., rhog0 = rhog, ...)                   [HIP; adjoint HIP]
     flx_v = alpha * rho*w * dt; ck; q_h = half-level interpolation   [glue]
@@ -71,34 +71,41 @@ class HorizontalFlux(torch.autograd.Function):
 
 
 class HorizontalLimiter(torch.autograd.Function):
-    """(q_a', Qout_prev) = horizontal_limiter_thuburn(q_a, q, Qout_post, ch; d,
-    cmask) (HLIM:42-430) with Qout_post supplied; d and cmask held fixed."""
+    """q_a' = horizontal_limiter_thuburn(q_a, q, ch; d, cmask) (HLIM:42-430) in
+    the library's exchange mode: the allowable range Qout_prev, its halo
+    exchange (Qout_post) and the clamp, one masked call; d and cmask held fixed.
+    Its transpose: the faces' half pass of the limiter's adjoint (q_a_b ->
+    q_a's, q's and Qout_post's cotangents), the halo exchange's transpose
+    (Qout_post's -> Qout_prev's) and the rows' half pass (Qout_prev's -> q's and
+    ch's)."""
 
     @staticmethod
-    def forward(fctx, nctx: Context, q_a, q_a_pl, q, q_pl, post, post_pl, d, d_pl, ch, ch_pl, cm, cm_pl):
+    def forward(fctx, nctx: Context, q_a, q_a_pl, q, q_pl, d, d_pl, ch, ch_pl, cm, cm_pl):
         out, out_pl = q_a.clone(), q_a_pl.clone()
-        prev, prev_pl = post.new_zeros(post.shape), post_pl.new_zeros(post_pl.shape)
+        post, post_pl = q.new_zeros((2,) + q.shape), q_pl.new_zeros((2,) + q_pl.shape)
         mask = torch.empty(q.shape, dtype=torch.uint8, device=q.device)
         mask_pl = torch.empty(q_pl.shape, dtype=torch.uint8, device=q.device)
-        nctx.horizontal_limiter_thuburn_mask(out, out_pl, q, q_pl, d, d_pl, ch, ch_pl, cm, cm_pl, prev, prev_pl,
-                                             post, post_pl, mask, mask_pl)
+        nctx.horizontal_limiter_thuburn_mask(out, out_pl, q, q_pl, d, d_pl, ch, ch_pl, cm, cm_pl, post, post_pl,
+                                             None, None, mask, mask_pl)
         fctx.nctx = nctx
         fctx.save_for_backward(out, out_pl, q, q_pl, post, post_pl, d, d_pl, ch, ch_pl, cm, cm_pl, mask, mask_pl)
-        return out, out_pl, prev, prev_pl
+        return out, out_pl
 
     @staticmethod
-    def backward(fctx, g_qa, g_qa_pl, g_prev, g_prev_pl):
+    def backward(fctx, g_qa, g_qa_pl):
         out, out_pl, q, q_pl, post, post_pl, d, d_pl, ch, ch_pl, cm, cm_pl, mask, mask_pl = fctx.saved_tensors
-        c = lambda g, ref: torch.zeros_like(ref) if g is None else g.contiguous().clone()  # noqa: E731
-        qab, qab_pl = c(g_qa, out), c(g_qa_pl, out_pl)
-        prevb, prevb_pl = c(g_prev, post), c(g_prev_pl, post_pl)
+        nctx = fctx.nctx
+        qab = torch.zeros_like(out) if g_qa is None else g_qa.contiguous().clone()
+        qab_pl = torch.zeros_like(out_pl) if g_qa_pl is None else g_qa_pl.contiguous().clone()
         qb, qb_pl = torch.empty_like(q), torch.empty_like(q_pl)
-        postb, postb_pl = torch.empty_like(post), torch.empty_like(post_pl)
+        pb, pb_pl = torch.empty_like(post), torch.empty_like(post_pl)
+        fixed = (out, out_pl, q, q_pl, d, d_pl, ch, ch_pl, cm, cm_pl, post, post_pl, mask, mask_pl)
+        nctx.horizontal_limiter_thuburn_adjoint(qab, qab_pl, qb, qb_pl, None, None, pb, pb_pl, *fixed)
+        nctx.comm_data_transfer_adjoint(pb, pb_pl, vmax=2, kmax=q.shape[-2])
+        qb2, qb2_pl = torch.empty_like(q), torch.empty_like(q_pl)
         chb, chb_pl = torch.empty_like(ch), torch.empty_like(ch_pl)
-        fctx.nctx.horizontal_limiter_thuburn_adjoint(qab, qab_pl, qb, qb_pl, prevb, prevb_pl, postb, postb_pl, out,
-                                                     out_pl, q, q_pl, d, d_pl, ch, ch_pl, cm, cm_pl, post, post_pl,
-                                                     mask, mask_pl, chb, chb_pl)
-        return (None, qab, qab_pl, qb, qb_pl, postb, postb_pl, None, None, chb, chb_pl, None, None)
+        nctx.horizontal_limiter_thuburn_adjoint(None, None, qb2, qb2_pl, pb, pb_pl, None, None, *fixed, chb, chb_pl)
+        return (None, qab, qab_pl, qb + qb2, qb_pl + qb2_pl, None, None, chb, chb_pl, None, None)
 
 
 def _c(g, ref):
@@ -328,14 +335,8 @@ class TracerStep:
         grad, grad_pl = self.gradient(q, q_pl)
         grad, grad_pl = self.halo(grad, grad_pl)
         q_a, q_a_pl = self.remap(q, q_pl, grad, grad_pl, cm, cm_pl, xc, xc_pl)
-        zero, zero_pl = q.new_zeros((2,) + q.shape), q.new_zeros((2,) + q_pl.shape)
         q, q_pl, q_a, q_a_pl = q.contiguous(), q_pl.contiguous(), q_a.contiguous(), q_a_pl.contiguous()
-        _, _, prev, prev_pl = HorizontalLimiter.apply(ctx, q_a, q_a_pl, q, q_pl, zero, zero_pl, self.d, self.d_pl,
-                                                      ch, ch_pl, cm, cm_pl)
-        post, post_pl = self.halo(prev, prev_pl)
-        q_a, q_a_pl, _, _ = HorizontalLimiter.apply(ctx, q_a, q_a_pl, q, q_pl, post.contiguous(),
-                                                    post_pl.contiguous(), self.d, self.d_pl, ch, ch_pl, cm, cm_pl)
-        return q_a, q_a_pl
+        return HorizontalLimiter.apply(ctx, q_a, q_a_pl, q, q_pl, self.d, self.d_pl, ch, ch_pl, cm, cm_pl)
 
     def __call__(self, q, q_pl, rhog, rhog_pl, recompute_tracers: bool = False):
         """One step.  recompute_tracers: each tracer's face values are

@@ -638,7 +638,13 @@ int nicam_horizontal_limiter_thuburn_mask(nicam_ctx ctx,
  * A changed face value is attributed, in this order, to Qout_post min,
  * Qout_post max, the arc's Qin min, Qin max, by equality with q_a.  Every
  * cotangent is a sum in a fixed order (the order of a sequential sweep over
- * the cells), so the results do not vary between runs. */
+ * the cells), so the results do not vary between runs.
+ * Half passes: with q_a_b NULL only the rows run (Qout_prev_b's terms; q_a_b,
+ * Qout_post_b, q_a, Qout_post and mask may be NULL); with Qout_prev_b NULL only
+ * the faces (q_a_b's; ch_b must then be NULL).  Each gives, bit for bit, the
+ * full pass with the other cotangent zero; a caller that exchanges Qout
+ * between the passes (the exchange mode's transpose) runs the faces, the
+ * transpose of the exchange on Qout_post_b, then the rows. */
 int nicam_horizontal_limiter_thuburn_adjoint(nicam_ctx ctx,
                                              double *q_a_b, double *q_a_b_pl,
                                              double *q_b, double *q_b_pl,

@@ -113,10 +113,16 @@ class OracleContext:
     # ---- horizontal limiter -----------------------------------------------------
     def horizontal_limiter_thuburn_mask(self, q_a, q_a_pl, q, q_pl, d, d_pl, ch, ch_pl, cm, cm_pl, prev, prev_pl,
                                         post, post_pl, mask, mask_pl):
+        exchange = post is None  # the library's exchange mode: Qout_prev exchanged, then the clamp on it
+        z, z_pl = np.zeros((2,) + tuple(q.shape)), np.zeros((2,) + tuple(q_pl.shape))
         f = dict(q_a=_np(q_a).copy(), q_a_pl=_np(q_a_pl).copy(), q=_np(q), q_pl=_np(q_pl), d=_np(d),
                  d_pl=_np(d_pl), ch=_np(ch), ch_pl=_np(ch_pl), cmask=_np(cm), cmask_pl=_np(cm_pl),
-                 Qout_post=_np(post), Qout_post_pl=_np(post_pl))
+                 Qout_post=z if exchange else _np(post), Qout_post_pl=z_pl if exchange else _np(post_pl))
         out = self.orc.horizontal_limiter_thuburn(self.spec, self.sgp, f)
+        if exchange:
+            f["Qout_post"], f["Qout_post_pl"] = apply_map(self.hm, out["Qout_prev"], out["Qout_prev_pl"])
+            out = self.orc.horizontal_limiter_thuburn(self.spec, self.sgp, f)
+            out["Qout_prev"], out["Qout_prev_pl"] = f["Qout_post"], f["Qout_post_pl"]
         m, mp = pack_hlim_mask(self.spec, f, out)
         _put(q_a, out["q_a"])
         _put(q_a_pl, out["q_a_pl"])
@@ -128,9 +134,14 @@ class OracleContext:
     def horizontal_limiter_thuburn_adjoint(self, qab, qab_pl, qb, qb_pl, prevb, prevb_pl, postb, postb_pl, q_a,
                                            q_a_pl, q, q_pl, d, d_pl, ch, ch_pl, cm, cm_pl, post, post_pl, mask,
                                            mask_pl, ch_b=None, ch_b_pl=None):
+        # a half pass (q_a_b or Qout_prev_b None) is the full one with that cotangent zero
         f = dict(q=_np(q), q_pl=_np(q_pl), d=_np(d), d_pl=_np(d_pl), ch=_np(ch), ch_pl=_np(ch_pl), cmask=_np(cm),
                  cmask_pl=_np(cm_pl), Qout_post=_np(post), Qout_post_pl=_np(post_pl))
-        u = dict(q_a_b=_np(qab), q_a_b_pl=_np(qab_pl), Qprev_b=_np(prevb), Qprev_b_pl=_np(prevb_pl))
+        P2, P2_pl = (2,) + tuple(q.shape), (2,) + tuple(q_pl.shape)
+        u = dict(q_a_b=np.zeros((6,) + tuple(q.shape)) if qab is None else _np(qab),
+                 q_a_b_pl=np.zeros(tuple(q_pl.shape)) if qab_pl is None else _np(qab_pl),
+                 Qprev_b=np.zeros(P2) if prevb is None else _np(prevb),
+                 Qprev_b_pl=np.zeros(P2_pl) if prevb_pl is None else _np(prevb_pl))
         out = self.orc.horizontal_limiter_thuburn_lin(self.spec, self.sgp, f, dict(q_a=_np(q_a), q_a_pl=_np(q_a_pl)),
                                                       _np(mask).reshape(self.lall, self.kall, self.gall),
                                                       _np(mask_pl), True, u)
