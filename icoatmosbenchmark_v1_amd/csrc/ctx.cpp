@@ -1299,6 +1299,54 @@ int nicam_tracer_update_adjoint(nicam_ctx c, double *flx_h_b, double *flx_h_b_pl
                              });
 }
 
+// ---- adjoints of the vertical glue ---------------------------------------------
+int nicam_vertical_courant_adjoint(nicam_ctx c, double *flx_v_b, double *flx_v_b_pl, double *rhog_b,
+                                   double *rhog_b_pl, const double *ck_b, const double *ck_b_pl, const double *flx_v,
+                                   const double *flx_v_pl, const double *rhog, const double *rhog_pl)
+{
+    if (c && !c->have_rdgz) return fail(NICAM_ERR_STATE, "GRD_rdgz not set (nicam_grd_setup or nicam_set_rdgz)");
+    const size_t R = c ? c->d.nreg() * 8 : 0, P = c ? c->d.npl() * 8 : 0;
+    const char *nm[] = {"flx_v_b", "rhog_b", "ck_b", "flx_v", "rhog"};
+    return glue_adjoint_call(c, {{flx_v_b, R, true}, {flx_v_b_pl, P, true}, {rhog_b, R, true}, {rhog_b_pl, P, true},
+                                 {ck_b, 2 * R, false}, {ck_b_pl, 2 * P, false}, {flx_v, R, false},
+                                 {flx_v_pl, P, false}, {rhog, R, false}, {rhog_pl, P, false}},
+                             nm, [&](const std::vector<void *> &v) {
+                                 auto D = [&](int i) { return static_cast<double *>(v[i]); };
+                                 return launch_vertical_courant_adjoint(c, D(0), D(1), D(2), D(3), D(4), D(5), D(6),
+                                                                        D(7), D(8), D(9));
+                             });
+}
+
+int nicam_vertical_face_adjoint(nicam_ctx c, double *q_b, double *q_b_pl, const double *q_h_b, const double *q_h_b_pl)
+{
+    if (c && !c->have_vgrid) return fail(NICAM_ERR_STATE, "vertical grid not set (nicam_grd_setup)");
+    const size_t R = c ? c->d.nreg() * 8 : 0, P = c ? c->d.npl() * 8 : 0;
+    const char *nm[] = {"q_b", "q_h_b"};
+    return glue_adjoint_call(c, {{q_b, R, true}, {q_b_pl, P, true}, {q_h_b, R, false}, {q_h_b_pl, P, false}}, nm,
+                             [&](const std::vector<void *> &v) {
+                                 auto D = [&](int i) { return static_cast<double *>(v[i]); };
+                                 return launch_vertical_face_adjoint(c, D(0), D(1), D(2), D(3));
+                             });
+}
+
+int nicam_tracer_update_vertical_adjoint(nicam_ctx c, double *flx_v_b, double *flx_v_b_pl, double *q_h_b,
+                                         double *q_h_b_pl, const double *rhogq_b, const double *rhogq_b_pl,
+                                         const double *flx_v, const double *flx_v_pl, const double *q_h,
+                                         const double *q_h_pl)
+{
+    if (c && !c->have_rdgz) return fail(NICAM_ERR_STATE, "GRD_rdgz not set (nicam_grd_setup or nicam_set_rdgz)");
+    const size_t R = c ? c->d.nreg() * 8 : 0, P = c ? c->d.npl() * 8 : 0;
+    const char *nm[] = {"flx_v_b", "q_h_b", "rhogq_b", "flx_v", "q_h"};
+    return glue_adjoint_call(c, {{flx_v_b, R, true}, {flx_v_b_pl, P, true}, {q_h_b, R, true}, {q_h_b_pl, P, true},
+                                 {rhogq_b, R, false}, {rhogq_b_pl, P, false}, {flx_v, R, false},
+                                 {flx_v_pl, P, false}, {q_h, R, false}, {q_h_pl, P, false}},
+                             nm, [&](const std::vector<void *> &v) {
+                                 auto D = [&](int i) { return static_cast<double *>(v[i]); };
+                                 return launch_vertical_update_adjoint(c, D(0), D(1), D(2), D(3), D(4), D(5), D(6),
+                                                                       D(7), D(8), D(9));
+                             });
+}
+
 int nicam_vertical_mass_flux(nicam_ctx c, double *flx_v, double *flx_v_pl, const double *rhogvx,
                              const double *rhogvx_pl, const double *rhogvy, const double *rhogvy_pl,
                              const double *rhogvz, const double *rhogvz_pl, const double *rhogw,

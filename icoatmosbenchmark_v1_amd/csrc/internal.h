@@ -234,6 +234,13 @@ int launch_vertical_mass_flux(nicam_ctx c, double *flx_v, double *flx_v_pl, cons
 int launch_vertical_courant(nicam_ctx c, double *ck, double *ck_pl, const double *flx_v, const double *flx_v_pl,
                             const double *rhog, const double *rhog_pl);
 int launch_vertical_face(nicam_ctx c, double *q_h, double *q_h_pl, const double *q, const double *q_pl);
+int launch_vertical_courant_adjoint(nicam_ctx c, double *flx_b, double *flx_b_pl, double *rhog_b, double *rhog_b_pl,
+                                    const double *ck_b, const double *ck_b_pl, const double *flx_v,
+                                    const double *flx_v_pl, const double *rhog, const double *rhog_pl);
+int launch_vertical_face_adjoint(nicam_ctx c, double *q_b, double *q_b_pl, const double *q_h_b, const double *q_h_b_pl);
+int launch_vertical_update_adjoint(nicam_ctx c, double *flx_b, double *flx_b_pl, double *q_h_b, double *q_h_b_pl,
+                                   const double *rq_b, const double *rq_b_pl, const double *flx_v,
+                                   const double *flx_v_pl, const double *q_h, const double *q_h_pl);
 int launch_vi_matrix(nicam_ctx c, const double *eth, const double *eth_pl, const double *g_tilde,
                      const double *g_tilde_pl, double dt);
 int launch_vertical_update(nicam_ctx c, double *rhogq, double *rhogq_pl, const double *flx_v, const double *flx_v_pl,

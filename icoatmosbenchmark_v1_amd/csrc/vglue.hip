@@ -30,6 +30,7 @@ constexpr int BLK = 256;
 // One of the two parts of a launch: the regular regions or the poles.
 struct VPart {
     double *out;                   // flx_v, ck, q_h or rhogq
+    double *out2;                  // the adjoints' second output
     const double *in[4];           // per operation, see the kernels
     const double *c2w, *rgamh, *rgsqrth;  // vertical metrics (mass flux only)
     int gall, lall;
@@ -42,7 +43,7 @@ struct VCommon {
     double dt;
 };
 
-enum { V_FLUX = 0, V_COURANT = 1, V_FACE = 2, V_UPDATE = 3 };
+enum { V_FLUX = 0, V_COURANT = 1, V_FACE = 2, V_UPDATE = 3, V_COURANT_ADJ = 4, V_FACE_ADJ = 5, V_UPDATE_ADJ = 6 };
 
 // Work-groups [0, nR) cover the regular regions, the rest the poles.  The
 // levels above and below a cell are gall cells away, a few work-groups on:
@@ -93,13 +94,43 @@ __global__ void __launch_bounds__(BLK) vglue_kernel(VPart R, VPart P, VCommon C)
         // in: q; q_h kept outside kmin+1..kmax
         if (k >= C.kmin + 1 && k <= C.kmax)
             stream_store(&A.out[t], C.afact[k] * A.in[0][t] + C.bfact[k] * A.in[0][t - dn]);
-    } else {
+    } else if (OP == V_UPDATE) {
         // in: flx_v, q_h; rhogq updated on kmin..kmax
         if (k >= C.kmin && k <= C.kmax) {
             const double up = (k + 1 <= C.kmax) ? A.in[0][t + dn] * A.in[1][t + dn] : 0.0;
             const double lo = (k >= C.kmin + 1) ? A.in[0][t] * A.in[1][t] : 0.0;
             A.out[t] = A.out[t] - (up - lo) * C.rdgz[k];
         }
+    } else if (OP == V_COURANT_ADJ) {
+        // in: ck_b (two planes A.n apart), flx_v, rhog; out: flx_v_b, rhog_b.  flx_v(k) enters the
+        // lower face of k and the upper face of k-1, rhog(k) both faces of k (-ck/rhog each)
+        const double *cb = A.in[0], *fv = A.in[1], *rh = A.in[2];
+        const double r = rh[t];
+        double f = 0.0, s = 0.0;
+        if (k > C.kmin && k <= C.kmax) f = f + -cb[t] / r * C.rdgz[k];
+        if (k - 1 >= C.kmin && k - 1 < C.kmax) f = f + cb[A.n + t - dn] / rh[t - dn] * C.rdgz[k - 1];
+        if (k > C.kmin && k <= C.kmax) s = s + cb[t] * (-fv[t] / r * C.rdgz[k]);
+        if (k >= C.kmin && k < C.kmax) s = s + cb[A.n + t] * (fv[t + dn] / r * C.rdgz[k]);
+        A.out[t] = f;
+        A.out2[t] = -s / r;
+    } else if (OP == V_FACE_ADJ) {
+        // in: q_h_b; out: q_b (q(k) enters q_h(k) and q_h(k+1))
+        const double *hb = A.in[0];
+        double s = 0.0;
+        if (k >= C.kmin + 1 && k <= C.kmax) s = s + C.afact[k] * hb[t];
+        if (k + 1 >= C.kmin + 1 && k + 1 <= C.kmax) s = s + C.bfact[k + 1] * hb[t + dn];
+        A.out[t] = s;
+    } else {
+        // V_UPDATE_ADJ, in: rhogq_b, flx_v, q_h; out: flx_v_b, q_h_b.  The flux through the interior face
+        // k leaves level k-1 and enters level k.
+        double a = 0.0, b = 0.0;
+        if (k >= C.kmin + 1 && k <= C.kmax) {
+            const double Fb = A.in[0][t] * C.rdgz[k] - A.in[0][t - dn] * C.rdgz[k - 1];
+            a = Fb * A.in[2][t];
+            b = Fb * A.in[1][t];
+        }
+        A.out[t] = a;
+        A.out2[t] = b;
     }
 }
 
@@ -260,6 +291,38 @@ int launch_vertical_update(nicam_ctx c, double *rhogq, double *rhogq_pl, const d
     const VPart R = part(D.gall, D.lall, D.kall, true, rhogq, {flx_v, q_h});
     const VPart P = part(D.gall_pl, kLallPl, D.kall, D.have_pl && rhogq_pl, rhogq_pl, {flx_v_pl, q_h_pl});
     return launch_v<V_UPDATE>(c, R, P, 0.0);
+}
+
+int launch_vertical_courant_adjoint(nicam_ctx c, double *flx_b, double *flx_b_pl, double *rhog_b, double *rhog_b_pl,
+                                    const double *ck_b, const double *ck_b_pl, const double *flx_v,
+                                    const double *flx_v_pl, const double *rhog, const double *rhog_pl)
+{
+    const Dims &D = c->d;
+    VPart R = part(D.gall, D.lall, D.kall, true, flx_b, {ck_b, flx_v, rhog});
+    VPart P = part(D.gall_pl, kLallPl, D.kall, flx_b_pl != nullptr, flx_b_pl, {ck_b_pl, flx_v_pl, rhog_pl});
+    R.out2 = rhog_b;
+    P.out2 = rhog_b_pl;
+    return launch_v<V_COURANT_ADJ>(c, R, P, 0.0);
+}
+
+int launch_vertical_face_adjoint(nicam_ctx c, double *q_b, double *q_b_pl, const double *q_h_b, const double *q_h_b_pl)
+{
+    const Dims &D = c->d;
+    VPart R = part(D.gall, D.lall, D.kall, true, q_b, {q_h_b});
+    VPart P = part(D.gall_pl, kLallPl, D.kall, q_b_pl != nullptr, q_b_pl, {q_h_b_pl});
+    return launch_v<V_FACE_ADJ>(c, R, P, 0.0);
+}
+
+int launch_vertical_update_adjoint(nicam_ctx c, double *flx_b, double *flx_b_pl, double *q_h_b, double *q_h_b_pl,
+                                   const double *rq_b, const double *rq_b_pl, const double *flx_v,
+                                   const double *flx_v_pl, const double *q_h, const double *q_h_pl)
+{
+    const Dims &D = c->d;
+    VPart R = part(D.gall, D.lall, D.kall, true, flx_b, {rq_b, flx_v, q_h});
+    VPart P = part(D.gall_pl, kLallPl, D.kall, flx_b_pl != nullptr, flx_b_pl, {rq_b_pl, flx_v_pl, q_h_pl});
+    R.out2 = q_h_b;
+    P.out2 = q_h_b_pl;
+    return launch_v<V_UPDATE_ADJ>(c, R, P, 0.0);
 }
 
 }  // namespace nicam

@@ -101,6 +101,9 @@ SIGNATURES = {
     "nicam_horizontal_remap_adjoint": (c_int, [_P] * 17),
     "nicam_tracer_update_adjoint": (c_int, [_P] * 11),
     "nicam_comm_data_transfer_adjoint": (c_int, [_P, _P, _P, c_int, c_int]),
+    "nicam_vertical_courant_adjoint": (c_int, [_P] * 11),
+    "nicam_vertical_face_adjoint": (c_int, [_P] * 5),
+    "nicam_tracer_update_vertical_adjoint": (c_int, [_P] * 11),
     "nicam_vertical_mass_flux": (c_int, [_P] * 11 + [c_double]),
     "nicam_vertical_courant": (c_int, [_P] * 7),
     "nicam_vertical_face": (c_int, [_P] * 5),
@@ -597,6 +600,30 @@ class Context:
         a += self._args([(rhogq_b, "rhogq_b", R), (rhogq_b_pl, "rhogq_b_pl", P), (flx_h, "flx_h", 6 * R),
                          (flx_h_pl, "flx_h_pl", P), (q_a, "q_a", 6 * R), (q_a_pl, "q_a_pl", P)])
         _check(self.lib.nicam_tracer_update_adjoint(self.h, *a), "nicam_tracer_update_adjoint")
+
+    def vertical_courant_adjoint(self, flx_v_b, flx_v_b_pl, rhog_b, rhog_b_pl, ck_b, ck_b_pl, flx_v, flx_v_pl, rhog,
+                                 rhog_pl) -> None:
+        R, P = self.nreg, self.npl
+        a = [_ptr(flx_v_b, self.mode, "flx_v_b", R), _ptr(flx_v_b_pl, self.mode, "flx_v_b_pl", P),
+             _ptr(rhog_b, self.mode, "rhog_b", R), _ptr(rhog_b_pl, self.mode, "rhog_b_pl", P)]
+        a += self._args([(ck_b, "ck_b", 2 * R), (ck_b_pl, "ck_b_pl", 2 * P), (flx_v, "flx_v", R),
+                         (flx_v_pl, "flx_v_pl", P), (rhog, "rhog", R), (rhog_pl, "rhog_pl", P)])
+        _check(self.lib.nicam_vertical_courant_adjoint(self.h, *a), "nicam_vertical_courant_adjoint")
+
+    def vertical_face_adjoint(self, q_b, q_b_pl, q_h_b, q_h_b_pl) -> None:
+        R, P = self.nreg, self.npl
+        a = [_ptr(q_b, self.mode, "q_b", R), _ptr(q_b_pl, self.mode, "q_b_pl", P)]
+        a += self._args([(q_h_b, "q_h_b", R), (q_h_b_pl, "q_h_b_pl", P)])
+        _check(self.lib.nicam_vertical_face_adjoint(self.h, *a), "nicam_vertical_face_adjoint")
+
+    def tracer_update_vertical_adjoint(self, flx_v_b, flx_v_b_pl, q_h_b, q_h_b_pl, rhogq_b, rhogq_b_pl, flx_v,
+                                       flx_v_pl, q_h, q_h_pl) -> None:
+        R, P = self.nreg, self.npl
+        a = [_ptr(flx_v_b, self.mode, "flx_v_b", R), _ptr(flx_v_b_pl, self.mode, "flx_v_b_pl", P),
+             _ptr(q_h_b, self.mode, "q_h_b", R), _ptr(q_h_b_pl, self.mode, "q_h_b_pl", P)]
+        a += self._args([(rhogq_b, "rhogq_b", R), (rhogq_b_pl, "rhogq_b_pl", P), (flx_v, "flx_v", R),
+                         (flx_v_pl, "flx_v_pl", P), (q_h, "q_h", R), (q_h_pl, "q_h_pl", P)])
+        _check(self.lib.nicam_tracer_update_vertical_adjoint(self.h, *a), "nicam_tracer_update_vertical_adjoint")
 
     def tracer_mass(self, rhogq, rhogq_pl, mass, single: bool = False) -> None:
         """nicam_tracer_mass(_f32): mass[k] = FP64 sum of AREA*rhogq over the inner cells and pole centres."""

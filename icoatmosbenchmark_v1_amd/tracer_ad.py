@@ -10,23 +10,24 @@ One step advances ntr tracers q and the density rhog of an icosahedral grid
     Qout = phase 1 of the limiter, halo exchange, q_a limited        [HIP, one call; adjoint HIP]
     rhog*q and rhog updated by the face fluxes, q = rhog*q / rhog    [HIP; adjoint HIP]
     rho*w = vi_rhow_solver(..., rhog0 = rhog, ...)                   [HIP; adjoint HIP]
-    flx_v = alpha * rho*w * dt; ck; q_h = half-level interpolation   [glue]
+    flx_v = alpha * rho*w * dt                                       [torch]
+    ck; q_h = half-level interpolation                               [HIP; adjoint HIP]
     q_h limited by vertical_limiter_thuburn                          [HIP; adjoint HIP]
-    rhog*q and rhog updated by the vertical fluxes; halo exchange    [glue]
+    rhog*q and rhog updated by the vertical fluxes; halo exchange    [HIP; adjoint HIP]
 
-The four reference kernels, the horizontal glue (Courant numbers and masks,
-gradient, remap, update) and the halo exchange run as
-torch.autograd.Functions over the context's forward entry points (the
-masked forwards for the limiters) and their *_adjoint entry points, each
-deterministic; the context is the library (runtime.Context, which needs its
-halo map, nicam_comm_setup) or, in the tests, the oracle.  The vertical glue
-is torch operations on the same arrays in the oracle's layout and operation
-order, so its derivatives come from torch's autograd.  The limiters'
-adjoints return the cotangents of the Courant numbers ch and ck, through
-which their bounds depend on rhog; what stays fixed is piecewise constant
-(cmask and the inflow flags, the signs of the Courant numbers) or zero here
-(d), so the gradient in q and rhog is the step's own wherever no switch sits
-exactly at its threshold.
+The four reference kernels, the horizontal and vertical glue (Courant
+numbers and masks, gradient, remap, half-level values, updates) and the
+halo exchange run as torch.autograd.Functions over the context's forward
+entry points (the masked forwards for the limiters) and their *_adjoint
+entry points, each deterministic; the context is the library
+(runtime.Context, which needs its halo map, nicam_comm_setup) or, in the
+tests, the oracle.  What is left to torch's autograd is elementwise: the
+products rhog*q, the quotients rhog*q / rhog and the scaling of rho*w into
+flx_v.  The limiters' adjoints return the cotangents of the Courant
+numbers ch and ck, through which their bounds depend on rhog; what stays
+fixed is piecewise constant (cmask and the inflow flags, the signs of the
+Courant numbers) or zero here (d), so the gradient in q and rhog is the
+step's own wherever no switch sits exactly at its threshold.
 
 Arrays are C arrays with the Fortran dimensions reversed: q[t][l][k][g],
 q_pl[t][p][k][v]; rhog[l][k][g], rhog_pl[p][k][v].
@@ -221,6 +222,71 @@ class Update(torch.autograd.Function):
         return None, g, g_pl, fb, fb_pl, qb, qb_pl
 
 
+class VCourant(torch.autograd.Function):
+    """ck [2][l][k][g] of the lower and upper faces (nicam_vertical_courant);
+    its transpose gives the cotangents of flx_v and rhog."""
+
+    @staticmethod
+    def forward(fctx, nctx: Context, flx_v, flx_v_pl, rhog, rhog_pl):
+        ck, ck_pl = flx_v.new_empty((2,) + flx_v.shape), flx_v_pl.new_empty((2,) + flx_v_pl.shape)
+        nctx.vertical_courant(ck, ck_pl, flx_v, flx_v_pl, rhog, rhog_pl)
+        fctx.nctx = nctx
+        fctx.save_for_backward(flx_v, flx_v_pl, rhog, rhog_pl)
+        return ck, ck_pl
+
+    @staticmethod
+    def backward(fctx, g, g_pl):
+        flx_v, flx_v_pl, rhog, rhog_pl = fctx.saved_tensors
+        fb, fb_pl = torch.empty_like(flx_v), torch.empty_like(flx_v_pl)
+        rb, rb_pl = torch.empty_like(rhog), torch.empty_like(rhog_pl)
+        g = flx_v.new_zeros((2,) + flx_v.shape) if g is None else g.contiguous()
+        g_pl = flx_v_pl.new_zeros((2,) + flx_v_pl.shape) if g_pl is None else g_pl.contiguous()
+        fctx.nctx.vertical_courant_adjoint(fb, fb_pl, rb, rb_pl, g, g_pl, flx_v, flx_v_pl, rhog, rhog_pl)
+        return None, fb, fb_pl, rb, rb_pl
+
+
+class VFace(torch.autograd.Function):
+    """q_h = afact q(k) + bfact q(k-1) on the interior half levels, 0 on the
+    others (nicam_vertical_face into a zeroed array), for one tracer."""
+
+    @staticmethod
+    def forward(fctx, nctx: Context, q, q_pl):
+        q_h, q_h_pl = torch.zeros_like(q), torch.zeros_like(q_pl)
+        nctx.vertical_face(q_h, q_h_pl, q, q_pl)
+        fctx.nctx = nctx
+        fctx.save_for_backward(q, q_pl)
+        return q_h, q_h_pl
+
+    @staticmethod
+    def backward(fctx, g, g_pl):
+        q, q_pl = fctx.saved_tensors
+        qb, qb_pl = torch.empty_like(q), torch.empty_like(q_pl)
+        fctx.nctx.vertical_face_adjoint(qb, qb_pl, _c(g, q), _c(g_pl, q_pl))
+        return None, qb, qb_pl
+
+
+class VUpdate(torch.autograd.Function):
+    """rhogq less the vertical flux divergence of flx_v * q_h on kmin..kmax
+    (nicam_tracer_update_vertical), for one tracer; with q_h = 1 the density."""
+
+    @staticmethod
+    def forward(fctx, nctx: Context, rq, rq_pl, flx_v, flx_v_pl, q_h, q_h_pl):
+        out, out_pl = rq.contiguous().clone(), rq_pl.contiguous().clone()
+        nctx.tracer_update_vertical(out, out_pl, flx_v, flx_v_pl, q_h, q_h_pl)
+        fctx.nctx = nctx
+        fctx.save_for_backward(flx_v, flx_v_pl, q_h, q_h_pl)
+        return out, out_pl
+
+    @staticmethod
+    def backward(fctx, g, g_pl):
+        flx_v, flx_v_pl, q_h, q_h_pl = fctx.saved_tensors
+        g, g_pl = _c(g, flx_v), _c(g_pl, flx_v_pl)
+        fb, fb_pl = torch.empty_like(flx_v), torch.empty_like(flx_v_pl)
+        qb, qb_pl = torch.empty_like(q_h), torch.empty_like(q_h_pl)
+        fctx.nctx.tracer_update_vertical_adjoint(fb, fb_pl, qb, qb_pl, g, g_pl, flx_v, flx_v_pl, q_h, q_h_pl)
+        return None, g, g_pl, fb, fb_pl, qb, qb_pl
+
+
 class HaloExchange:
     """COMM_data_transfer through the context: x [..., lall, kall, gall],
     x_pl [..., 2, kall, gall_pl], the leading dimensions as variables."""
@@ -258,7 +324,7 @@ class TracerStep:
         vi = case["vi"]
         self.vi = {k + s: T(vi[k + s]) for k in VI_IN for s in ("", "_pl")}
         self.rhogw_in, self.rhogw_in_pl = T(vi["rhogw"]), T(vi["rhogw_pl"])
-        self.rdgz, self.afact, self.bfact = T(fac["rdgz"]), T(fac["afact"]), T(fac["bfact"])
+        self.rdgz = T(fac["rdgz"])
         inner = np.zeros((n1, n1), dtype=bool)
         inner[1:-1, 1:-1] = True
         self.inner = torch.from_numpy(inner.reshape(-1)).to(dev)
@@ -302,30 +368,21 @@ class TracerStep:
 
     def vertical(self, q, q_pl, rq, rq_pl, rhog, rhog_pl, flx_v, flx_v_pl):
         """ck, q_h, the vertical limiter, the flux-form update of rho*q and rho."""
-        kall = self.kall
-        rd = self.rdgz[:, None]
-        nxt = lambda a: torch.cat([a[..., 1:, :], torch.zeros_like(a[..., :1, :])], dim=-2)  # noqa: E731  k+1
-        prv = lambda a: torch.cat([torch.zeros_like(a[..., :1, :]), a[..., :-1, :]], dim=-2)  # noqa: E731  k-1
-        ck = torch.stack([-flx_v / rhog * rd * self.kface, nxt(flx_v) / rhog * rd * nxt(self.kface) * self.kin])
-        ck_pl = torch.stack([-flx_v_pl / rhog_pl * rd * self.kface,
-                             nxt(flx_v_pl) / rhog_pl * rd * nxt(self.kface) * self.kin])
-        q_h = (self.afact[:, None] * q + self.bfact[:, None] * prv(q)) * self.kface
-        q_h_pl = (self.afact[:, None] * q_pl + self.bfact[:, None] * prv(q_pl)) * self.kface
+        ctx = self.ctx
+        flx_v, flx_v_pl, rhog, rhog_pl = flx_v.contiguous(), flx_v_pl.contiguous(), rhog.contiguous(), rhog_pl.contiguous()
+        ck, ck_pl = VCourant.apply(ctx, flx_v, flx_v_pl, rhog, rhog_pl)
+        ck, ck_pl = ck.contiguous(), ck_pl.contiguous()
         outs, outs_pl = [], []
         for t in range(q.shape[0]):
-            a, b = VerticalLimiter.apply(self.ctx, q_h[t].contiguous(), q_h_pl[t].contiguous(), q[t].contiguous(),
-                                         q_pl[t].contiguous(), self.dv, self.dv_pl, ck.contiguous(),
-                                         ck_pl.contiguous())
+            qt, qt_pl = q[t].contiguous(), q_pl[t].contiguous()
+            q_h, q_h_pl = VFace.apply(ctx, qt, qt_pl)
+            a, b = VerticalLimiter.apply(ctx, q_h, q_h_pl, qt, qt_pl, self.dv, self.dv_pl, ck, ck_pl)
+            a, b = VUpdate.apply(ctx, rq[t], rq_pl[t], flx_v, flx_v_pl, a.contiguous(), b.contiguous())
             outs.append(a)
             outs_pl.append(b)
-        q_h, q_h_pl = torch.stack(outs), torch.stack(outs_pl)
-        F, F_pl = flx_v * q_h * self.kface, flx_v_pl * q_h_pl * self.kface
-        rq = rq - (nxt(F) - F) * rd * self.kin
-        rq_pl = rq_pl - (nxt(F_pl) - F_pl) * rd * self.kin
-        fv, fv_pl = flx_v * self.kface, flx_v_pl * self.kface
-        rhog = rhog - (nxt(fv) - fv) * rd * self.kin
-        rhog_pl = rhog_pl - (nxt(fv_pl) - fv_pl) * rd * self.kin
-        return rq, rq_pl, rhog, rhog_pl
+        ones, ones_pl = torch.ones_like(rhog), torch.ones_like(rhog_pl)
+        rhog, rhog_pl = VUpdate.apply(ctx, rhog, rhog_pl, flx_v, flx_v_pl, ones, ones_pl)
+        return torch.stack(outs), torch.stack(outs_pl), rhog, rhog_pl
 
     # -- the step ----------------------------------------------------------------
     def face_values(self, q, q_pl, xc, xc_pl, cm, cm_pl, ch, ch_pl):

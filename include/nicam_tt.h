@@ -521,6 +521,22 @@ int nicam_vertical_face(nicam_ctx ctx, double *q_h, double *q_h_pl, const double
 int nicam_tracer_update_vertical(nicam_ctx ctx, double *rhogq, double *rhogq_pl, const double *flx_v,
                                  const double *flx_v_pl, const double *q_h, const double *q_h_pl);
 
+/* Adjoints of the vertical glue above, for the differentiable tracer step
+ * (no reference counterpart; outputs overwritten, each a fixed-order sum per
+ * cell-level).  nicam_vertical_courant_adjoint: flx_v_b and rhog_b from ck_b
+ * over the interior faces the forward forms; nicam_vertical_face_adjoint: q_b
+ * from q_h_b; nicam_tracer_update_vertical_adjoint: flx_v_b and q_h_b from
+ * rhogq_b (rhogq's own cotangent passes through unchanged). */
+int nicam_vertical_courant_adjoint(nicam_ctx ctx, double *flx_v_b, double *flx_v_b_pl, double *rhog_b,
+                                   double *rhog_b_pl, const double *ck_b, const double *ck_b_pl, const double *flx_v,
+                                   const double *flx_v_pl, const double *rhog, const double *rhog_pl);
+int nicam_vertical_face_adjoint(nicam_ctx ctx, double *q_b, double *q_b_pl, const double *q_h_b,
+                                const double *q_h_b_pl);
+int nicam_tracer_update_vertical_adjoint(nicam_ctx ctx, double *flx_v_b, double *flx_v_b_pl, double *q_h_b,
+                                         double *q_h_b_pl, const double *rhogq_b, const double *rhogq_b_pl,
+                                         const double *flx_v, const double *flx_v_pl, const double *q_h,
+                                         const double *q_h_pl);
+
 /* Coefficients of the tridiagonal rho*w system that nicam_vi_rhow_solver
  * reads, computed on the device and installed in the context (what
  * nicam_vi_rhow_solver_putcoef installs from arrays).  No reference

@@ -103,6 +103,9 @@ class Oracle:
         L.orc_tracer_courant_adjoint.argtypes = [POINTER(OrcDims)] + [P] * 10
         L.orc_horizontal_remap_adjoint.argtypes = [POINTER(OrcDims)] + [P] * 16
         L.orc_tracer_update_adjoint.argtypes = [POINTER(OrcDims)] + [P] * 10
+        L.orc_vertical_courant_adjoint.argtypes = [POINTER(OrcDims)] + [P] * 11
+        L.orc_vertical_face_adjoint.argtypes = [POINTER(OrcDims)] + [P] * 6
+        L.orc_tracer_update_vertical_adjoint.argtypes = [POINTER(OrcDims)] + [P] * 11
         L.orc_vertical_mass_flux.argtypes = [POINTER(OrcDims)] + [P] * 16 + [c_double]
         L.orc_vertical_courant.argtypes = [POINTER(OrcDims)] + [P] * 7
         L.orc_vertical_face.argtypes = [POINTER(OrcDims)] + [P] * 6
@@ -211,6 +214,33 @@ class Oracle:
         ins = [_f64(a) for a in (rhogq_b, rhogq_b_pl, flx_h, flx_h_pl, q_a, q_a_pl)]
         self.lib.orc_tracer_update_adjoint(ctypes.byref(D), _p(out["flx_b"]), _p(out["flx_b_pl"]), _p(out["qa_b"]),
                                            _p(out["qa_b_pl"]), *[_p(a) for a in ins])
+        return out
+
+    # adjoints of the vertical glue (orc_*_adjoint): outputs overwritten
+    def vertical_courant_adjoint(self, spec, rdgz, ck_b, ck_b_pl, flx_v, flx_v_pl, rhog, rhog_pl):
+        D = self.dims(spec)
+        out = dict(flx_b=np.zeros(spec.shape3), flx_b_pl=np.zeros(spec.shape_pl), rhog_b=np.zeros(spec.shape3),
+                   rhog_b_pl=np.zeros(spec.shape_pl))
+        ins = [_f64(a) for a in (rdgz, ck_b, ck_b_pl, flx_v, flx_v_pl, rhog, rhog_pl)]
+        self.lib.orc_vertical_courant_adjoint(ctypes.byref(D), _p(ins[0]), *[_p(out[k]) for k in (
+            "flx_b", "flx_b_pl", "rhog_b", "rhog_b_pl")], *[_p(a) for a in ins[1:]])
+        return out
+
+    def vertical_face_adjoint(self, spec, afact, bfact, q_h_b, q_h_b_pl):
+        D = self.dims(spec)
+        out = dict(q_b=np.zeros(spec.shape3), q_b_pl=np.zeros(spec.shape_pl))
+        ins = [_f64(a) for a in (afact, bfact, q_h_b, q_h_b_pl)]
+        self.lib.orc_vertical_face_adjoint(ctypes.byref(D), _p(ins[0]), _p(ins[1]), _p(out["q_b"]), _p(out["q_b_pl"]),
+                                           _p(ins[2]), _p(ins[3]))
+        return out
+
+    def tracer_update_vertical_adjoint(self, spec, rdgz, rq_b, rq_b_pl, flx_v, flx_v_pl, q_h, q_h_pl):
+        D = self.dims(spec)
+        out = dict(flx_b=np.zeros(spec.shape3), flx_b_pl=np.zeros(spec.shape_pl), q_h_b=np.zeros(spec.shape3),
+                   q_h_b_pl=np.zeros(spec.shape_pl))
+        ins = [_f64(a) for a in (rdgz, rq_b, rq_b_pl, flx_v, flx_v_pl, q_h, q_h_pl)]
+        self.lib.orc_tracer_update_vertical_adjoint(ctypes.byref(D), _p(ins[0]), *[_p(out[k]) for k in (
+            "flx_b", "flx_b_pl", "q_h_b", "q_h_b_pl")], *[_p(a) for a in ins[1:]])
         return out
 
     def vertical_mass_flux(self, spec, f, m, dt):

@@ -1372,6 +1372,106 @@ void orc_tracer_update_vertical(const orc_dims *D, const double *rdgz, double *r
             }
 }
 
+/* Adjoints of the three above, for the differentiable step (outputs
+ * overwritten; parity unpinned against the reference, which has no glue;
+ * pinned by dot-product tests against the forwards).  Sums in the order
+ * written. */
+
+/* flx_b(k) = lower face k's term (-ck_b(k,1)/rhog(k)*rdgz(k)) + upper face
+ * k-1's (ck_b(k-1,2)/rhog(k-1)*rdgz(k-1)); rhog_b(k) = -(ck_b(k,1) ck(k,1) +
+ * ck_b(k,2) ck(k,2))/rhog(k), over the interior faces the forward forms. */
+void orc_vertical_courant_adjoint(const orc_dims *D, const double *rdgz, double *flx_b, double *flx_b_pl,
+                                  double *rhog_b, double *rhog_b_pl, const double *ck_b, const double *ck_b_pl,
+                                  const double *flx_v, const double *flx_v_pl, const double *rhog,
+                                  const double *rhog_pl)
+{
+    const int iall = D->gall_1d, gall = iall * iall, kall = D->kall, lall = D->lall, gall_pl = D->gall_pl;
+    const int kmin = 2, kmax = kall - 1;
+#define VCA(X3, X4, fb, rb, cb, fv, rh, g, l)                                                               \
+    for (int k = 1; k <= kall; ++k) {                                                                        \
+        double f = 0.0, t = 0.0;                                                                             \
+        if (k > kmin && k <= kmax) f = f + -X4(cb, g, k, l, 1) / X3(rh, g, k, l) * rdgz[k - 1];              \
+        if (k - 1 >= kmin && k - 1 < kmax) f = f + X4(cb, g, k - 1, l, 2) / X3(rh, g, k - 1, l) * rdgz[k - 2]; \
+        if (k > kmin && k <= kmax) t = t + X4(cb, g, k, l, 1) * (-X3(fv, g, k, l) / X3(rh, g, k, l) * rdgz[k - 1]); \
+        if (k >= kmin && k < kmax)                                                                           \
+            t = t + X4(cb, g, k, l, 2) * (X3(fv, g, k + 1, l) / X3(rh, g, k, l) * rdgz[k - 1]);              \
+        X3(fb, g, k, l) = f;                                                                                 \
+        X3(rb, g, k, l) = -t / X3(rh, g, k, l);                                                              \
+    }
+    for (int l = 1; l <= lall; ++l)
+        for (int g = 1; g <= gall; ++g) VCA(F3, F4, flx_b, rhog_b, ck_b, flx_v, rhog, g, l)
+    for (int l = 1; l <= LALL_PL; ++l)
+        for (int v = 1; v <= gall_pl; ++v) {
+            if (D->have_pl) {
+                VCA(P3, P4, flx_b_pl, rhog_b_pl, ck_b_pl, flx_v_pl, rhog_pl, v, l)
+            } else {
+                for (int k = 1; k <= kall; ++k) P3(flx_b_pl, v, k, l) = P3(rhog_b_pl, v, k, l) = 0.0;
+            }
+        }
+#undef VCA
+}
+
+/* q_b(k) = afact(k) q_h_b(k) [k in kmin+1..kmax] + bfact(k+1) q_h_b(k+1)
+ * [k+1 in kmin+1..kmax]. */
+void orc_vertical_face_adjoint(const orc_dims *D, const double *afact, const double *bfact, double *q_b,
+                               double *q_b_pl, const double *q_h_b, const double *q_h_b_pl)
+{
+    const int iall = D->gall_1d, gall = iall * iall, kall = D->kall, lall = D->lall, gall_pl = D->gall_pl;
+    const int kmin = 2, kmax = kall - 1;
+#define VFA(X3, qb, hb, g, l)                                                                  \
+    for (int k = 1; k <= kall; ++k) {                                                           \
+        double s = 0.0;                                                                         \
+        if (k >= kmin + 1 && k <= kmax) s = s + afact[k - 1] * X3(hb, g, k, l);                 \
+        if (k + 1 >= kmin + 1 && k + 1 <= kmax) s = s + bfact[k] * X3(hb, g, k + 1, l);         \
+        X3(qb, g, k, l) = s;                                                                    \
+    }
+    for (int l = 1; l <= lall; ++l)
+        for (int g = 1; g <= gall; ++g) VFA(F3, q_b, q_h_b, g, l)
+    for (int l = 1; l <= LALL_PL; ++l)
+        for (int v = 1; v <= gall_pl; ++v) {
+            if (D->have_pl) {
+                VFA(P3, q_b_pl, q_h_b_pl, v, l)
+            } else {
+                for (int k = 1; k <= kall; ++k) P3(q_b_pl, v, k, l) = 0.0;
+            }
+        }
+#undef VFA
+}
+
+/* For the interior faces k (kmin+1..kmax), Fb = rhogq_b(k) rdgz(k) -
+ * rhogq_b(k-1) rdgz(k-1); flx_b(k) = Fb q_h(k), q_h_b(k) = Fb flx_v(k); other
+ * levels 0 (rhogq's cotangent passes through unchanged). */
+void orc_tracer_update_vertical_adjoint(const orc_dims *D, const double *rdgz, double *flx_b, double *flx_b_pl,
+                                        double *q_h_b, double *q_h_b_pl, const double *rq_b, const double *rq_b_pl,
+                                        const double *flx_v, const double *flx_v_pl, const double *q_h,
+                                        const double *q_h_pl)
+{
+    const int iall = D->gall_1d, gall = iall * iall, kall = D->kall, lall = D->lall, gall_pl = D->gall_pl;
+    const int kmin = 2, kmax = kall - 1;
+#define UVA(X3, fb, hb, rb, fv, qh, g, l)                                                      \
+    for (int k = 1; k <= kall; ++k) {                                                           \
+        double a = 0.0, b = 0.0;                                                                \
+        if (k >= kmin + 1 && k <= kmax) {                                                       \
+            const double Fb = X3(rb, g, k, l) * rdgz[k - 1] - X3(rb, g, k - 1, l) * rdgz[k - 2]; \
+            a = Fb * X3(qh, g, k, l);                                                           \
+            b = Fb * X3(fv, g, k, l);                                                           \
+        }                                                                                       \
+        X3(fb, g, k, l) = a;                                                                    \
+        X3(hb, g, k, l) = b;                                                                    \
+    }
+    for (int l = 1; l <= lall; ++l)
+        for (int g = 1; g <= gall; ++g) UVA(F3, flx_b, q_h_b, rq_b, flx_v, q_h, g, l)
+    for (int l = 1; l <= LALL_PL; ++l)
+        for (int v = 1; v <= gall_pl; ++v) {
+            if (D->have_pl) {
+                UVA(P3, flx_b_pl, q_h_b_pl, rq_b_pl, flx_v_pl, q_h_pl, v, l)
+            } else {
+                for (int k = 1; k <= kall; ++k) P3(flx_b_pl, v, k, l) = P3(q_h_b_pl, v, k, l) = 0.0;
+            }
+        }
+#undef UVA
+}
+
 /* ---------------------------------------------------------------------------
  * Mc/Ml/Mu of the vertically implicit rho*w system (no reference code: the
  * reference reads them from its input file).  Collected from the discretised

@@ -201,6 +201,17 @@ void orc_vertical_face(const orc_dims *D, const double *afact, const double *bfa
                        const double *q, const double *q_pl);
 void orc_tracer_update_vertical(const orc_dims *D, const double *rdgz, double *rhogq, double *rhogq_pl,
                                 const double *flx_v, const double *flx_v_pl, const double *q_h, const double *q_h_pl);
+/* Adjoints of the three above (outputs overwritten), for the differentiable step. */
+void orc_vertical_courant_adjoint(const orc_dims *D, const double *rdgz, double *flx_b, double *flx_b_pl,
+                                  double *rhog_b, double *rhog_b_pl, const double *ck_b, const double *ck_b_pl,
+                                  const double *flx_v, const double *flx_v_pl, const double *rhog,
+                                  const double *rhog_pl);
+void orc_vertical_face_adjoint(const orc_dims *D, const double *afact, const double *bfact, double *q_b,
+                               double *q_b_pl, const double *q_h_b, const double *q_h_b_pl);
+void orc_tracer_update_vertical_adjoint(const orc_dims *D, const double *rdgz, double *flx_b, double *flx_b_pl,
+                                        double *q_h_b, double *q_h_b_pl, const double *rq_b, const double *rq_b_pl,
+                                        const double *flx_v, const double *flx_v_pl, const double *q_h,
+                                        const double *q_h_pl);
 
 /* Mc/Ml/Mu of the rho*w system from h (eth) and g~ at the half levels, the
  * VMTR metrics and the GRD factors (no reference code). */

@@ -1,7 +1,7 @@
 """A stand-in for runtime.Context on CPU tensors whose kernels are the
 oracle's C restatement: the flux, both limiters (masked forwards and
 adjoints, with the Courant numbers' cotangents), the rho*w solve, the
-horizontal glue, each with its adjoint, and the halo fill of a halo map
+horizontal and vertical glue, each with its adjoint, and the halo fill of a halo map
 (halo_sim) with its transpose.  It lets the tests run tracer_ad.TracerStep, the end-to-end
 gradient harness, on the CPU, and compare the GPU harness with it."""
 import numpy as np
@@ -28,6 +28,7 @@ class OracleContext:
         self.sgp = [int(x) for x in geo["have_sgp"]]
         self.vi = vi
         self.fac = (fac["rdgzh"], fac["afact"], fac["bfact"])
+        self.rdgz = fac["rdgz"]
         self.kall, self.gall, self.lall, self.gall_pl = spec.kall, spec.gall, spec.lall, spec.gall_pl
         self.nreg, self.npl = spec.gall * spec.kall * spec.lall, spec.gall_pl * spec.kall * 2
         self.hmask = {}
@@ -92,6 +93,40 @@ class OracleContext:
         out = self.orc.tracer_update_adjoint(self.spec, _np(rqb), _np(rqb_pl), _np(flx), _np(flx_pl), _np(q_a),
                                              _np(q_a_pl))
         for t, k in ((fb, "flx_b"), (fb_pl, "flx_b_pl"), (qab, "qa_b"), (qab_pl, "qa_b_pl")):
+            _put(t, out[k])
+
+    # ---- the vertical glue ----------------------------------------------------------
+    def vertical_courant(self, ck, ck_pl, flx_v, flx_v_pl, rhog, rhog_pl):
+        out = self.orc.vertical_courant(self.spec, self.rdgz, _np(flx_v), _np(flx_v_pl), _np(rhog), _np(rhog_pl))
+        _put(ck, out["ck"])
+        _put(ck_pl, out["ck_pl"])
+
+    def vertical_courant_adjoint(self, fb, fb_pl, rb, rb_pl, ckb, ckb_pl, flx_v, flx_v_pl, rhog, rhog_pl):
+        out = self.orc.vertical_courant_adjoint(self.spec, self.rdgz, _np(ckb), _np(ckb_pl), _np(flx_v),
+                                                _np(flx_v_pl), _np(rhog), _np(rhog_pl))
+        for t, k in ((fb, "flx_b"), (fb_pl, "flx_b_pl"), (rb, "rhog_b"), (rb_pl, "rhog_b_pl")):
+            _put(t, out[k])
+
+    def vertical_face(self, q_h, q_h_pl, q, q_pl):
+        out = self.orc.vertical_face(self.spec, self.fac[1], self.fac[2], _np(q_h), _np(q_h_pl), _np(q), _np(q_pl))
+        _put(q_h, out["q_h"])
+        _put(q_h_pl, out["q_h_pl"])
+
+    def vertical_face_adjoint(self, qb, qb_pl, qhb, qhb_pl):
+        out = self.orc.vertical_face_adjoint(self.spec, self.fac[1], self.fac[2], _np(qhb), _np(qhb_pl))
+        _put(qb, out["q_b"])
+        _put(qb_pl, out["q_b_pl"])
+
+    def tracer_update_vertical(self, rq, rq_pl, flx_v, flx_v_pl, q_h, q_h_pl):
+        out = self.orc.tracer_update_vertical(self.spec, self.rdgz, _np(rq), _np(rq_pl), _np(flx_v), _np(flx_v_pl),
+                                              _np(q_h), _np(q_h_pl))
+        _put(rq, out["rhogq"])
+        _put(rq_pl, out["rhogq_pl"])
+
+    def tracer_update_vertical_adjoint(self, fb, fb_pl, qhb, qhb_pl, rqb, rqb_pl, flx_v, flx_v_pl, q_h, q_h_pl):
+        out = self.orc.tracer_update_vertical_adjoint(self.spec, self.rdgz, _np(rqb), _np(rqb_pl), _np(flx_v),
+                                                      _np(flx_v_pl), _np(q_h), _np(q_h_pl))
+        for t, k in ((fb, "flx_b"), (fb_pl, "flx_b_pl"), (qhb, "q_h_b"), (qhb_pl, "q_h_b_pl")):
             _put(t, out[k])
 
     # ---- horizontal_flux --------------------------------------------------------
