@@ -1143,10 +1143,30 @@ struct FaArgs {
     int gall_pl, have_pl;
 };
 
-__global__ void __launch_bounds__(FSW) hflux_adjoint_kernel(FaArgs A)
+// The cotangents of one origin's arcs that HBM supplies: flx_h_b of its own
+// face and of the neighbour's mirrored face, and GRD_xc_b (loaded a step
+// ahead of their use).
+#ifndef NICAM_HFLUXADJ_PREFETCH
+#define NICAM_HFLUXADJ_PREFETCH 1  // 1: the next step's arc cotangents are loaded before the gather
+#endif
+#ifndef NICAM_HFLUXADJ_METRICS
+#define NICAM_HFLUXADJ_METRICS 0  // 1: the arcs' metrics (triangle weights, normals, areas) are loaded a step ahead too
+#endif
+struct ArcIn {
+    double fa[3], fn[3], xb[3][3];
+#if NICAM_HFLUXADJ_METRICS
+    double w[12], hn[3][3], ra, rn[3];  // W1..W3 of TI(a0), TJ(a0), TJ(a0-iall), TI(a0-1); GMTR_a; GMTR_p * dt
+#endif
+};
+
+// R: the ring size, fixed at compile time so that a ring slot is one wrapped
+// index per neighbour and every (arc, field) plane is an immediate offset from
+// it.  R >= FSW + 2 * iall + 1.
+template <int R>
+__global__ void __launch_bounds__(FSW) __attribute__((amdgpu_waves_per_eu(2, 8))) hflux_adjoint_kernel(FaArgs A)
 {
     extern __shared__ double lds[];
-    const int iall = A.iall, gall = A.gall, kall = A.kall, lall = A.lall, R = A.ring;
+    const int iall = A.iall, gall = A.gall, kall = A.kall, lall = A.lall;
     const int k = blockIdx.x % kall, l = blockIdx.x / kall;
     if (l >= lall) return;
     double *s_a = lds;           // [3 arcs][4 fields][R] arc cotangents of origins [b0 - iall - 1, b0 + FSW + iall - 1]
@@ -1158,44 +1178,82 @@ __global__ void __launch_bounds__(FSW) hflux_adjoint_kernel(FaArgs A)
     const int tid = threadIdx.x;
     auto W = [&](int c, int t, int w) { return A.gt[(size_t)(w * 2 + t) * mstr + mb + c]; };  // w: 2,3,4 = W1..W3
     const double dt = A.dt, EPS = A.eps;
-    // the three arcs of origin a0 (zero where a0 is no origin)
-    auto arcs = [&](int a0) {
-        const int s = a0 % R;  // a0's slot in both rings
-        auto X = [&](int f, int p) {  // p within [a0 - iall, a0 + iall + 1]
-            int q = s + (p - a0);
-            q += q < 0 ? R : 0;
-            q -= q >= R ? R : 0;
-            return s_x[f * R + q];
-        };
-        auto tri = [&](int f, int t, int c) {  // forward triangle sums (FLX:139-201)
-            if (t == 0 && sgp && c == 0) t = 1, c = 1;
-            if (t == 0) return X(f, c) * W(c, 0, 2) + X(f, c + 1) * W(c, 0, 3) + X(f, c + iall + 1) * W(c, 0, 4);
-            return X(f, c) * W(c, 1, 2) + X(f, c + iall + 1) * W(c, 1, 3) + X(f, c + iall) * W(c, 1, 4);
-        };
-        const int i = a0 % iall, j = a0 / iall;
-        const bool org = a0 < gall && i <= iall - 2 && j <= iall - 2;
-        // the four triangles the origin's arcs use: TI(a0), TJ(a0), TJ(a0-iall) (AI), TI(a0-1) (AJ).
-        // They touch six cells (a0, its E, N, NE, S, W neighbours): each field is read there once, and
-        // the sums are formed in the forward's order.  The singular-point substitution (TI(0) = TJ(1))
-        // reaches only origins 0 and 1 and takes the general path.
+    auto up = [](int q) { return q >= R ? q - R : q; };
+    auto dn = [](int q) { return q < 0 ? q + R : q; };
+    auto AB = [&](int a, int f, int q) { return s_a[((a - 1) * 4 + f) * R + q]; };
+    auto valid = [&](int a, int i, int j) { return j >= (a == 1 ? 1 : 0) && i >= (a == 3 ? 1 : 0); };
+    // is (i, j) an arc origin (i, j in [0, iall-2])
+    auto is_org = [&](int i, int j) { return i <= iall - 2 && j <= iall - 2; };
+    // loads of region-relative cells clamped into the region: lanes past its end or off its
+    // origins read a cell of the region and discard what they read
+    auto cl = [&](int c) { return c < 0 ? 0 : (c > gall - 1 ? gall - 1 : c); };
+    auto arcin = [&](int a0, ArcIn &in) {
+#pragma unroll
+        for (int a = 1; a <= 3; ++a) {
+            const int g = cl(a0), nb = cl(a0 + (a == 1 ? 1 : (a == 2 ? iall + 1 : iall)));
+            in.fa[a - 1] = A.flxb[(size_t)(a - 1) * plane + fb + g];
+            in.fn[a - 1] = A.flxb[(size_t)(a + 2) * plane + fb + nb];
+#pragma unroll
+            for (int dd = 0; dd < 3; ++dd) in.xb[a - 1][dd] = A.xcb[(size_t)(dd * 3 + (a - 1)) * plane + fb + g];
+#if NICAM_HFLUXADJ_METRICS
+#pragma unroll
+            for (int dd = 0; dd < 3; ++dd) in.hn[a - 1][dd] = A.ga[(size_t)(dd * 3 + (a - 1)) * mstr + mb + g];
+            in.rn[a - 1] = A.gp[1 * mstr + mb + nb] * dt;
+#endif
+        }
+#if NICAM_HFLUXADJ_METRICS
+        const int c0 = cl(a0), cs = cl(a0 - iall), cw = cl(a0 - 1);
+#pragma unroll
+        for (int w = 0; w < 3; ++w) {
+            in.w[w] = W(c0, 0, 2 + w);
+            in.w[3 + w] = W(c0, 1, 2 + w);
+            in.w[6 + w] = W(cs, 1, 2 + w);
+            in.w[9 + w] = W(cw, 0, 2 + w);
+        }
+        in.ra = A.gp[1 * mstr + mb + c0] * dt;
+#endif
+    };
+    // the three arcs of origin a0 (slot s, position (i, j); zero where a0 is no origin).  GEN: a0 may
+    // be 0 or 1 (the singular-point substitution TI(0) = TJ(1)) or lie in row 0 (no TJ(a0-iall));
+    // otherwise a0 >= iall.
+    auto arcs = [&](auto gen_tag, int a0, int s, int i, int j, const ArcIn &in) {
+        constexpr bool GEN = decltype(gen_tag)::value;
+        const bool org = is_org(i, j);
         double tI[4], tJ[4], tJs[4], tIw[4];
-        if (org && !(sgp && a0 <= 1)) {
-            const bool hs = j >= 1, hw = i >= 1;
-            const double wi1 = W(a0, 0, 2), wi2 = W(a0, 0, 3), wi3 = W(a0, 0, 4);
-            const double wj1 = W(a0, 1, 2), wj2 = W(a0, 1, 3), wj3 = W(a0, 1, 4);
-            const int cs = a0 - iall, cw = a0 - 1;
-            const double ws1 = hs ? W(cs, 1, 2) : 0.0, ws2 = hs ? W(cs, 1, 3) : 0.0, ws3 = hs ? W(cs, 1, 4) : 0.0;
-            const double ww1 = hw ? W(cw, 0, 2) : 0.0, ww2 = hw ? W(cw, 0, 3) : 0.0, ww3 = hw ? W(cw, 0, 4) : 0.0;
+        if (!GEN || (org && !(sgp && a0 <= 1))) {
+            // the four triangles the origin's arcs use: TI(a0), TJ(a0), TJ(a0-iall) (AI), TI(a0-1) (AJ).
+            // They touch six cells (a0, its E, N, NE, S, W neighbours): each field is read there once,
+            // and the sums are formed in the forward's order (FLX:139-201).
+            const bool hs = !GEN || j >= 1, hw = i >= 1;
+#if NICAM_HFLUXADJ_METRICS
+            const double wi1 = in.w[0], wi2 = in.w[1], wi3 = in.w[2];
+            const double wj1 = in.w[3], wj2 = in.w[4], wj3 = in.w[5];
+            const double ws1 = in.w[6], ws2 = in.w[7], ws3 = in.w[8];
+            const double ww1 = in.w[9], ww2 = in.w[10], ww3 = in.w[11];
+#else
+            const int c0 = cl(a0), cs = cl(a0 - iall), cw = cl(a0 - 1);
+            const double wi1 = W(c0, 0, 2), wi2 = W(c0, 0, 3), wi3 = W(c0, 0, 4);
+            const double wj1 = W(c0, 1, 2), wj2 = W(c0, 1, 3), wj3 = W(c0, 1, 4);
+            const double ws1 = W(cs, 1, 2), ws2 = W(cs, 1, 3), ws3 = W(cs, 1, 4);
+            const double ww1 = W(cw, 0, 2), ww2 = W(cw, 0, 3), ww3 = W(cw, 0, 4);
+#endif
+            const int qe = up(s + 1), qn = up(s + iall), qne = up(s + iall + 1), qs = dn(s - iall), qw = dn(s - 1);
 #pragma unroll
             for (int f = 0; f < 4; ++f) {
-                const double x0 = X(f, a0), xe = X(f, a0 + 1), xn = X(f, a0 + iall), xne = X(f, a0 + iall + 1);
-                const double xs = hs ? X(f, cs) : 0.0, xw = hw ? X(f, cw) : 0.0;
+                const double *x = s_x + f * R;
+                const double x0 = x[s], xe = x[qe], xn = x[qn], xne = x[qne], xs = x[qs], xw = x[qw];
                 tI[f] = x0 * wi1 + xe * wi2 + xne * wi3;
                 tJ[f] = x0 * wj1 + xne * wj2 + xn * wj3;
                 tJs[f] = hs ? xs * ws1 + xe * ws2 + x0 * ws3 : 0.0;
                 tIw[f] = hw ? xw * ww1 + x0 * ww2 + xn * ww3 : 0.0;
             }
         } else {
+            auto X = [&](int f, int p) { return s_x[f * R + dn(up(s + (p - a0)))]; };
+            auto tri = [&](int f, int t, int c) {
+                if (t == 0 && sgp && c == 0) t = 1, c = 1;
+                if (t == 0) return X(f, c) * W(c, 0, 2) + X(f, c + 1) * W(c, 0, 3) + X(f, c + iall + 1) * W(c, 0, 4);
+                return X(f, c) * W(c, 1, 2) + X(f, c + iall + 1) * W(c, 1, 3) + X(f, c + iall) * W(c, 1, 4);
+            };
 #pragma unroll
             for (int f = 0; f < 4; ++f) {
                 tI[f] = org ? tri(f, 0, a0) : 0.0;
@@ -1204,93 +1262,126 @@ __global__ void __launch_bounds__(FSW) hflux_adjoint_kernel(FaArgs A)
                 tIw[f] = (org && i >= 1) ? tri(f, 0, a0 - 1) : 0.0;
             }
         }
+#if NICAM_HFLUXADJ_METRICS
+        const double ra = in.ra;
+#else
+        const double ra = A.gp[1 * mstr + mb + cl(a0)] * dt;
+#endif
 #pragma unroll
         for (int a = 1; a <= 3; ++a) {
-            const bool valid = org && j >= (a == 1 ? 1 : 0) && i >= (a == 3 ? 1 : 0);
-            double bv[4] = {0.0, 0.0, 0.0, 0.0};
-            if (valid) {
-                const int g = a0;
-                const int nb = g + (a == 1 ? 1 : (a == 2 ? iall + 1 : iall));
-                double s2[4];
+            // every lane forms the arc; those off the origins store zeros
+            const bool ok = org && valid(a, i, j);
+            const int g = cl(a0);
+            const int nb = a0 + (a == 1 ? 1 : (a == 2 ? iall + 1 : iall));
+            double s2[4];
 #pragma unroll
-                for (int f = 0; f < 4; ++f)  // FLX:234-309: the arc's two triangles, in the forward's order
-                    s2[f] = a == 1 ? tJs[f] + tI[f] : (a == 2 ? tI[f] + tJ[f] : tJ[f] + tIw[f]);
-                const double rr = 1.0 / dmx(s2[0], EPS);
-                const bool live_rho = !(EPS > s2[0]);
-                const double ra = A.gp[1 * mstr + mb + g] * dt, rn = A.gp[1 * mstr + mb + nb] * dt;
-                const bool sgp_face = sgp && a == 3 && nb == iall + 1;
-                const double fb_nb = sgp_face ? 0.0 : A.flxb[(size_t)(a + 2) * plane + fb + nb];
-                const double fluxb = A.flxb[(size_t)(a - 1) * plane + fb + g] * ra - fb_nb * rn;
-                double r2b = 0.0;
+            for (int f = 0; f < 4; ++f)  // FLX:234-309: the arc's two triangles, in the forward's order
+                s2[f] = a == 1 ? tJs[f] + tI[f] : (a == 2 ? tI[f] + tJ[f] : tJ[f] + tIw[f]);
+            const double rr = 1.0 / dmx(s2[0], EPS);
+            const bool live_rho = !(EPS > s2[0]);
+#if NICAM_HFLUXADJ_METRICS
+            const double rn = in.rn[a - 1];
+#else
+            const double rn = A.gp[1 * mstr + mb + cl(nb)] * dt;
+#endif
+            const bool sgp_face = sgp && a == 3 && nb == iall + 1;
+            const double fb_nb = sgp_face ? 0.0 : in.fn[a - 1];
+            const double fluxb = in.fa[a - 1] * ra - fb_nb * rn;
+            double bv[4], r2b = 0.0;
 #pragma unroll
-                for (int dd = 0; dd < 3; ++dd) {
-                    const double hn = A.ga[(size_t)(dd * 3 + (a - 1)) * mstr + mb + g];
-                    const double xb = A.xcb[(size_t)(dd * 3 + (a - 1)) * plane + fb + g];
-                    bv[1 + dd] = 0.5 * hn * fluxb - xb * rr * dt * 0.5;
-                    r2b = r2b + xb * s2[1 + dd] * dt * 0.5 * rr * rr;
-                }
-                bv[0] = live_rho ? r2b : 0.0;
+            for (int dd = 0; dd < 3; ++dd) {
+#if NICAM_HFLUXADJ_METRICS
+                const double hn = in.hn[a - 1][dd];
+#else
+                const double hn = A.ga[(size_t)(dd * 3 + (a - 1)) * mstr + mb + g];
+#endif
+                const double xb = in.xb[a - 1][dd];
+                bv[1 + dd] = 0.5 * hn * fluxb - xb * rr * dt * 0.5;
+                r2b = r2b + xb * s2[1 + dd] * dt * 0.5 * rr * rr;
             }
+            bv[0] = live_rho ? r2b : 0.0;
 #pragma unroll
-            for (int f = 0; f < 4; ++f) s_a[((a - 1) * 4 + f) * R + s] = bv[f];
+            for (int f = 0; f < 4; ++f) s_a[((a - 1) * 4 + f) * R + s] = ok ? bv[f] : 0.0;
         }
     };
-    // triangle origins of cell X0's six triangles, and whether each is one (i, j in [0, iall-2])
-    auto origins = [&](int X0, bool &o_sw, bool &o_s, bool &o_w, bool &o_c) {
-        const int i0 = X0 % iall, j0 = X0 / iall;
+    // triangle origins of the cell at (i0, j0): its six triangles, and whether each is one
+    auto origins = [&](int i0, int j0, bool &o_sw, bool &o_s, bool &o_w, bool &o_c) {
         o_sw = i0 >= 1 && j0 >= 1 && i0 - 1 <= iall - 2 && j0 - 1 <= iall - 2;  // X0-iall-1
         o_s = j0 >= 1 && i0 <= iall - 2 && j0 - 1 <= iall - 2;                  // X0-iall
         o_w = i0 >= 1 && i0 - 1 <= iall - 2 && j0 <= iall - 2;                  // X0-1
         o_c = i0 <= iall - 2 && j0 <= iall - 2;                                 // X0
     };
     // the weights of the cell in its six triangles (loaded a step ahead of their use)
+    // (a weight is read where its triangle is no origin too, and then not used)
     auto cellw = [&](int X0, double w[6]) {
-        bool o_sw, o_s, o_w, o_c;
-        origins(X0, o_sw, o_s, o_w, o_c);
-        const bool in = X0 < gall;
-        const int c_sw = X0 - iall - 1, c_s = X0 - iall, c_w = X0 - 1;
-        w[0] = in && o_sw ? W(c_sw, 0, 4) : 0.0;
-        w[1] = in && o_sw ? W(c_sw, 1, 3) : 0.0;
-        w[2] = in && o_s ? W(c_s, 1, 4) : 0.0;
-        w[3] = in && o_w ? W(c_w, 0, 3) : 0.0;
-        w[4] = in && o_c ? W(X0, 0, 2) : 0.0;
-        w[5] = in && o_c ? W(X0, 1, 2) : 0.0;
+        const int c_sw = cl(X0 - iall - 1), c_s = cl(X0 - iall), c_w = cl(X0 - 1), c_c = cl(X0);
+        w[0] = W(c_sw, 0, 4);
+        w[1] = W(c_sw, 1, 3);
+        w[2] = W(c_s, 1, 4);
+        w[3] = W(c_w, 0, 3);
+        w[4] = W(c_c, 0, 2);
+        w[5] = W(c_c, 1, 2);
     };
-    // the cell's four cotangents from its six triangles
-    auto cell = [&](int X0, const double w[6]) {
-        const int sc = X0 % R;
-        auto AB = [&](int a, int f, int c) {
-            int q = sc + (c - X0);
-            q += q < 0 ? R : 0;
-            q -= q >= R ? R : 0;
-            return s_a[((a - 1) * 4 + f) * R + q];
-        };
+    // the cell's four cotangents from its six triangles.  Triangle cotangents: TI(c) from AI(c),
+    // AIJ(c), AJ(c+1); TJ(c) from AI(c+iall), AIJ(c), AJ(c); the six triangles use the twelve arcs
+    // of origins c_sw, c_s, c_w, X0, X0+1, X0+iall.
+    auto cell = [&](int X0, int sc, int i0, int j0, const double w[6]) {
         bool o_sw, o_s, o_w, o_c;
-        origins(X0, o_sw, o_s, o_w, o_c);
-        const int c_sw = X0 - iall - 1, c_s = X0 - iall, c_w = X0 - 1;
+        origins(i0, j0, o_sw, o_s, o_w, o_c);
         const double w_sw_i = w[0], w_sw_j = w[1], w_s_j = w[2], w_w_i = w[3], w_c_i = w[4], w_c_j = w[5];
-        // triangle cotangents: TI(c) from AI(c), AIJ(c), AJ(c+1); TJ(c) from AI(c+iall), AIJ(c), AJ(c)
-        auto tib0 = [&](int f, int c) { return ((0.0 + AB(1, f, c)) + AB(2, f, c)) + AB(3, f, c + 1); };
-        auto tjb0 = [&](int f, int c) { return ((0.0 + AB(1, f, c + iall)) + AB(2, f, c)) + AB(3, f, c); };
-        auto tib = [&](int f, int c) { return (sgp && c == 0) ? 0.0 : tib0(f, c); };
-        auto tjb = [&](int f, int c) { return (sgp && c == 1) ? tjb0(f, 1) + tib0(f, 0) : tjb0(f, c); };
+        if (sgp && (X0 <= 1 || X0 == iall + 1 || X0 == iall + 2)) {
+            // the cells whose triangles include TI(0) or TJ(1): the singular-point substitution moves
+            // TI(0)'s cotangent onto TJ(1)
+            const int c_sw = X0 - iall - 1, c_s = X0 - iall, c_w = X0 - 1;
+            auto ABg = [&](int a, int f, int c) { return AB(a, f, dn(up(sc + (c - X0)))); };
+            auto tib0 = [&](int f, int c) { return ((0.0 + ABg(1, f, c)) + ABg(2, f, c)) + ABg(3, f, c + 1); };
+            auto tjb0 = [&](int f, int c) { return ((0.0 + ABg(1, f, c + iall)) + ABg(2, f, c)) + ABg(3, f, c); };
+            auto tib = [&](int f, int c) { return (c == 0) ? 0.0 : tib0(f, c); };
+            auto tjb = [&](int f, int c) { return (c == 1) ? tjb0(f, 1) + tib0(f, 0) : tjb0(f, c); };
+#pragma unroll
+            for (int f = 0; f < 4; ++f) {
+                double y = 0.0;
+                if (o_sw) y = y + (tib(f, c_sw) * w_sw_i + tjb(f, c_sw) * w_sw_j);
+                if (o_s) y = y + tjb(f, c_s) * w_s_j;
+                if (o_w) y = y + tib(f, c_w) * w_w_i;
+                if (o_c) y = y + (tib(f, X0) * w_c_i + tjb(f, X0) * w_c_j);
+                A.Xb[f][fb + X0] = y;
+            }
+            return;
+        }
+        const int q_c = sc, q_e = up(sc + 1), q_n = up(sc + iall);
+        const int q_w = dn(sc - 1), q_s = dn(sc - iall), q_sw = dn(sc - iall - 1);
 #pragma unroll
         for (int f = 0; f < 4; ++f) {
+            const double a1sw = AB(1, f, q_sw), a2sw = AB(2, f, q_sw), a3sw = AB(3, f, q_sw);
+            const double a2s = AB(2, f, q_s), a3s = AB(3, f, q_s);
+            const double a1w = AB(1, f, q_w), a2w = AB(2, f, q_w);
+            const double a1c = AB(1, f, q_c), a2c = AB(2, f, q_c), a3c = AB(3, f, q_c);
+            const double a3e = AB(3, f, q_e), a1n = AB(1, f, q_n);
+            // a triangle that is no origin leaves the sum as it is (a select, not an added zero, which
+            // would turn a -0.0 into +0.0); the terms are formed by every lane, from whatever the ring
+            // and the clamped weights hold there
             double y = 0.0;
-            if (o_sw) y = y + (tib(f, c_sw) * w_sw_i + tjb(f, c_sw) * w_sw_j);
-            if (o_s) y = y + tjb(f, c_s) * w_s_j;
-            if (o_w) y = y + tib(f, c_w) * w_w_i;
-            if (o_c) y = y + (tib(f, X0) * w_c_i + tjb(f, X0) * w_c_j);
+            const double t_sw = (((0.0 + a1sw) + a2sw) + a3s) * w_sw_i + (((0.0 + a1w) + a2sw) + a3sw) * w_sw_j;
+            y = o_sw ? y + t_sw : y;
+            const double t_s = (((0.0 + a1c) + a2s) + a3s) * w_s_j;
+            y = o_s ? y + t_s : y;
+            const double t_w = (((0.0 + a1w) + a2w) + a3c) * w_w_i;
+            y = o_w ? y + t_w : y;
+            const double t_c = (((0.0 + a1c) + a2c) + a3e) * w_c_i + (((0.0 + a1n) + a2c) + a3c) * w_c_j;
+            y = o_c ? y + t_c : y;
             A.Xb[f][fb + X0] = y;
         }
     };
     // rho, rho*v of cell p into its slot of the ring (zero beyond the region)
     auto xload = [&](int p, double v[4]) {
 #pragma unroll
-        for (int f = 0; f < 4; ++f) v[f] = p < gall ? A.X[f][fb + p] : 0.0;
+        for (int f = 0; f < 4; ++f) {
+            const double t = A.X[f][fb + cl(p)];
+            v[f] = p < gall ? t : 0.0;
+        }
     };
-    auto xstore = [&](int p, const double v[4]) {
-        const int sp = p % R;
+    auto xstore = [&](int sp, const double v[4]) {
 #pragma unroll
         for (int f = 0; f < 4; ++f) s_x[f * R + sp] = v[f];
     };
@@ -1300,20 +1391,52 @@ __global__ void __launch_bounds__(FSW) hflux_adjoint_kernel(FaArgs A)
         xstore(p, v);
     }
     __syncthreads();
+    for (int e = tid; e < iall; e += FSW) {  // origins [0, iall): row j = 0
+        ArcIn in;
+        arcin(e, in);
+        arcs(std::true_type{}, e, e, e, 0, in);
+    }
     // origins [b0 - iall - 1, b0 + FSW + iall - 1] are in the arc ring while cells [b0, b0 + FSW) gather;
     // each step replaces the cells [b0, b0 + FSW) of the cell ring, read by its arcs only, with the next
-    // step's, loaded while the arcs are formed
-    for (int e = tid; e < iall; e += FSW) arcs(e);
+    // step's, loaded while the arcs are formed.  Positions advance by FSW cells a step: di columns and
+    // dj rows.
+    const int di = FSW % iall, dj = FSW / iall;
+    auto adv = [&](int &i, int &j) {
+        i += di;
+        j += dj;
+        if (i >= iall) i -= iall, j += 1;
+    };
+    int ia = tid % iall, ja = tid / iall + 1;  // origin iall + tid
+    int ic = tid % iall, jc = tid / iall;      // cell tid
+    int sa = up(iall + tid), sc = tid;         // their slots
+#if NICAM_HFLUXADJ_PREFETCH
+    ArcIn cur;
+    arcin(iall + tid, cur);
+#endif
     for (int b0 = 0; b0 < gall; b0 += FSW) {
+        const int a0 = b0 + iall + tid;
         double xn[4], w[6];
-        const int pn = b0 + R + tid;
-        xload(pn, xn);
+        xload(b0 + R + tid, xn);
         cellw(b0 + tid, w);
-        arcs(b0 + iall + tid);
+#if NICAM_HFLUXADJ_PREFETCH
+        arcs(std::false_type{}, a0, sa, ia, ja, cur);
+        adv(ia, ja);
+        arcin(a0 + FSW, cur);
+#else
+        {
+            ArcIn in;
+            arcin(a0, in);
+            arcs(std::false_type{}, a0, sa, ia, ja, in);
+        }
+        adv(ia, ja);
+#endif
         __syncthreads();
-        if (b0 + tid < gall) cell(b0 + tid, w);
-        xstore(pn, xn);
+        if (b0 + tid < gall) cell(b0 + tid, sc, ic, jc, w);
+        xstore(sc, xn);  // cell b0 + R + tid has cell b0 + tid's slot
         __syncthreads();
+        adv(ic, jc);
+        sa = up(sa + FSW);
+        sc = up(sc + FSW);
     }
 }
 
@@ -1390,15 +1513,25 @@ int launch_hflux_adjoint(nicam_ctx c, const double *flxb, const double *flxb_pl,
     A.iall = D.gall_1d; A.gall = D.gall; A.kall = D.kall; A.lall = D.lall;
     A.dt = dt; A.eps = c->eps;
     A.gall_pl = D.gall_pl; A.have_pl = D.have_pl;
-    // the ring covers the origins [b0 - iall - 1, b0 + FSW + iall - 1] a step reads
-    A.ring = FSW + 2 * D.gall_1d + 1;
     const long nblk = (long)D.kall * D.lall;
     if (nblk > 0x7fffffffL) return fail(NICAM_ERR_ARG, "grid too large for one launch");
-    const size_t lds = sizeof(double) * 16 * (size_t)A.ring;
-    if (lds > 64 * 1024)
-        NICAM_HIP(hipFuncSetAttribute((const void *)hflux_adjoint_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                      (int)lds));
-    hipLaunchKernelGGL(hflux_adjoint_kernel, dim3((unsigned)nblk), dim3(FSW), lds, c->stream, A);
+    // the ring covers the origins [b0 - iall - 1, b0 + FSW + iall - 1] a step reads: the smallest
+    // compiled size of at least FSW + 2 iall + 1
+    auto launch = [&](auto kern, int R) -> int {
+        A.ring = R;
+        const size_t lds = sizeof(double) * 16 * (size_t)R;
+        if (lds > 64 * 1024)
+            NICAM_HIP(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        hipLaunchKernelGGL(kern, dim3((unsigned)nblk), dim3(FSW), lds, c->stream, A);
+        return NICAM_OK;
+    };
+    const int need = FSW + 2 * D.gall_1d + 1;
+    int rc;
+    if (need <= FSW + 2 * 34 + 1) rc = launch(hflux_adjoint_kernel<FSW + 2 * 34 + 1>, FSW + 2 * 34 + 1);
+    else if (need <= FSW + 2 * 66 + 1) rc = launch(hflux_adjoint_kernel<FSW + 2 * 66 + 1>, FSW + 2 * 66 + 1);
+    else if (need <= FSW + 2 * 130 + 1) rc = launch(hflux_adjoint_kernel<FSW + 2 * 130 + 1>, FSW + 2 * 130 + 1);
+    else rc = launch(hflux_adjoint_kernel<FSW + 2 * 160 + 1>, FSW + 2 * 160 + 1);
+    if (rc != NICAM_OK) return rc;
     if (D.have_pl)
         hipLaunchKernelGGL(hflux_adjoint_pole, dim3((unsigned)((D.kall * kLallPl + 63) / 64)), dim3(64), 0, c->stream,
                            A);
