@@ -26,10 +26,11 @@ namespace nicam {
 namespace {
 
 constexpr int CHUNK = 256;  // output cells per work-group
-constexpr int MAXI = 160;   // largest gall_1d supported (gl8/rl1 regions are 130)
-constexpr int NTP = 2;      // TI/TJ points per work-item: CHUNK*NTP >= CHUNK + MAXI + 1
-// Staged cells per work-item, NST: CHUNK*NST >= CHUNK + 2*(gall_1d+1); the
-// launcher picks 2 up to gall_1d = 127 and 3 above, to spare registers.
+constexpr int MAXI = 514;   // largest gall_1d supported (gl8/rl1 regions are 130, gl9/rl1 258, gl9/rl0 514)
+// Per work-item: NTP TI/TJ points, CHUNK*NTP >= CHUNK + gall_1d + 1, and NST
+// staged cells, CHUNK*NST >= CHUNK + 2*(gall_1d+1).  The launcher picks the
+// smallest pair that covers the region width, to spare registers: (2, 2) up to
+// gall_1d = 127, (3, 2) to 255, (4, 3) to 383, (6, 4) to 514.
 #ifndef NICAM_HFLUX_WT_LDS
 #define NICAM_HFLUX_WT_LDS 0  // 1: triangle weights in LDS (154 VGPRs, 3 waves per SIMD) measured no faster at gl7
 #endif
@@ -99,7 +100,7 @@ __device__ void hflux_pole(const HfluxArgs<T> &A, int t)
 
 // Work-group b of one grid's launch (the grouped kernel of several grids
 // passes the index within the grid).
-template <int NST, class T>
+template <int NST, int NTP, class T>
 __device__ __forceinline__ void hflux_block(const HfluxArgs<T> &A, int b)
 {
     extern __shared__ __align__(16) unsigned char lds_raw[];
@@ -324,10 +325,10 @@ __device__ __forceinline__ void hflux_block(const HfluxArgs<T> &A, int b)
 #undef XR
 }
 
-template <int NST, class T>
+template <int NST, int NTP, class T>
 __global__ void __launch_bounds__(CHUNK) hflux_kernel(HfluxArgs<T> A)
 {
-    hflux_block<NST, T>(A, blockIdx.x);
+    hflux_block<NST, NTP, T>(A, blockIdx.x);
 }
 
 // several grids in one launch: work-groups [off[n], off[n+1]) belong to grid n
@@ -338,24 +339,39 @@ struct HfluxGroup {
     int n;
 };
 
-template <int NST, class T>
+template <int NST, int NTP, class T>
 __global__ void __launch_bounds__(CHUNK) hflux_group_kernel(HfluxGroup<T> G)
 {
     const int b = blockIdx.x;
     int n = 0;
     while (n + 1 < G.n && b >= G.off[n + 1]) ++n;
-    hflux_block<NST, T>(G.a[n], b - G.off[n]);
+    hflux_block<NST, NTP, T>(G.a[n], b - G.off[n]);
 }
 
-// The launch arguments of one grid: nblk work-groups, lds bytes, nst staged
-// cells per work-item.
+// The (NST, NTP) pair for a region width: 0 (2, 2), 1 (3, 2), 2 (4, 3), 3 (6, 4).
+inline int hflux_variant(int iall) { return iall <= 127 ? 0 : (iall <= 255 ? 1 : (iall <= 383 ? 2 : 3)); }
+
+// One launch of variant v of the four instances k[0..3].
+template <class Arg>
+int hflux_launch(void (*const k[4])(Arg), int v, long nblk, size_t lds, hipStream_t stream, const Arg &A)
+{
+    void (*fn)(Arg) = k[v];
+    if (lds > 64 * 1024)
+        NICAM_HIP(hipFuncSetAttribute((const void *)fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    hipLaunchKernelGGL(fn, dim3((unsigned)nblk), dim3(CHUNK), lds, stream, A);
+    NICAM_HIP(hipGetLastError());
+    return NICAM_OK;
+}
+
+// The launch arguments of one grid: nblk work-groups, lds bytes, nst the
+// variant (hflux_variant) its width needs.
 template <class T>
 int hflux_make_args(nicam_ctx c, HfluxArgs<T> &A, long &nblk, size_t &lds, int &nst, T *flx_h, T *flx_h_pl, T *xc,
                     T *xc_pl, const T *rho, const T *rho_pl, const T *rvx, const T *rvx_pl, const T *rvy,
                     const T *rvy_pl, const T *rvz, const T *rvz_pl, T dt, T eps, const HMetrics<T> &M)
 {
     const Dims &D = c->d;
-    if (D.gall_1d > MAXI) return fail(NICAM_ERR_ARG, "gall_1d larger than the supported 160");
+    if (D.gall_1d > MAXI) return fail(NICAM_ERR_ARG, "gall_1d larger than the supported 514");
     A.flx = flx_h;
     A.xc = xc;
     A.f[0] = rho;
@@ -398,7 +414,7 @@ int hflux_make_args(nicam_ctx c, HfluxArgs<T> &A, long &nblk, size_t &lds, int &
     A.gall_pl = D.gall_pl;
     const size_t n_t = CHUNK + D.gall_1d + 1;
     lds = sizeof(T) * (4 * (CHUNK + 2 * (D.gall_1d + 1)) + 8 * n_t + (NICAM_HFLUX_WT_LDS ? 6 * n_t : 0));
-    nst = CHUNK * 2 >= CHUNK + 2 * (D.gall_1d + 1) ? 2 : 3;  // two staged cells per work-item suffice
+    nst = hflux_variant(D.gall_1d);
     nblk = nmain + npole;
     return NICAM_OK;
 }
@@ -415,15 +431,9 @@ int launch_hflux_t(nicam_ctx c, T *flx_h, T *flx_h_pl, T *xc, T *xc_pl, const T 
     const int rc = hflux_make_args(c, A, nblk, lds, nst, flx_h, flx_h_pl, xc, xc_pl, rho, rho_pl, rvx, rvx_pl, rvy,
                                    rvy_pl, rvz, rvz_pl, dt, eps, M);
     if (rc != NICAM_OK) return rc;
-    const bool two = nst == 2;
-    const void *fn = two ? (const void *)hflux_kernel<2, T> : (const void *)hflux_kernel<3, T>;
-    if (lds > 64 * 1024) NICAM_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    if (two)
-        hipLaunchKernelGGL((hflux_kernel<2, T>), dim3((unsigned)nblk), dim3(CHUNK), lds, c->stream, A);
-    else
-        hipLaunchKernelGGL((hflux_kernel<3, T>), dim3((unsigned)nblk), dim3(CHUNK), lds, c->stream, A);
-    NICAM_HIP(hipGetLastError());
-    return NICAM_OK;
+    void (*const k[4])(HfluxArgs<T>) = {hflux_kernel<2, 2, T>, hflux_kernel<3, 2, T>, hflux_kernel<4, 3, T>,
+                                         hflux_kernel<6, 4, T>};
+    return hflux_launch(k, nst, nblk, lds, c->stream, A);
 }
 
 }  // namespace
@@ -459,12 +469,12 @@ int launch_hflux_group_f32(const nicam_ctx *cs, int n, const nicam_tracer_step_f
     G.n = n;
     G.off[0] = 0;
     size_t lds_max = 0;
-    int nst_max = 2;
+    int nst_max = 0;
     for (int i = 0; i < n; ++i) {
         const nicam_tracer_step_f32 &p = g[i];
         long nb = 0;
         size_t lds = 0;
-        int nst = 2;
+        int nst = 0;
         const int rc = hflux_make_args<float>(cs[i], G.a[i], nb, lds, nst, p.flx_h, p.flx_h_pl, p.GRD_xc, p.GRD_xc_pl,
                                               p.rho, p.rho_pl, p.rhovx, p.rhovx_pl, p.rhovy, p.rhovy_pl, p.rhovz,
                                               p.rhovz_pl, dt, cs[i]->eps32, cs[i]->hm32);
@@ -474,15 +484,10 @@ int launch_hflux_group_f32(const nicam_ctx *cs, int n, const nicam_tracer_step_f
         lds_max = std::max(lds_max, lds);
         nst_max = std::max(nst_max, nst);
     }
-    // the staging ring of the widest grid serves them all (three cells per work-item cover every width)
-    const void *fn = nst_max == 2 ? (const void *)hflux_group_kernel<2, float> : (const void *)hflux_group_kernel<3, float>;
-    if (lds_max > 64 * 1024) NICAM_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_max));
-    if (nst_max == 2)
-        hipLaunchKernelGGL((hflux_group_kernel<2, float>), dim3((unsigned)G.off[n]), dim3(CHUNK), lds_max, stream, G);
-    else
-        hipLaunchKernelGGL((hflux_group_kernel<3, float>), dim3((unsigned)G.off[n]), dim3(CHUNK), lds_max, stream, G);
-    NICAM_HIP(hipGetLastError());
-    return NICAM_OK;
+    // the staging ring and TI/TJ points of the widest grid serve them all
+    void (*const k[4])(HfluxGroup<float>) = {hflux_group_kernel<2, 2, float>, hflux_group_kernel<3, 2, float>,
+                                             hflux_group_kernel<4, 3, float>, hflux_group_kernel<6, 4, float>};
+    return hflux_launch(k, nst_max, G.off[n], lds_max, stream, G);
 }
 
 }  // namespace nicam

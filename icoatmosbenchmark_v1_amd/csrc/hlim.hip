@@ -35,7 +35,8 @@ namespace {
 #define NICAM_HLIM_STORE 0  // 0: nontemporal stores (668 us at gl7), 1: plain mirrored-face stores (697), 2: plain throughout (710)
 #endif
 constexpr int CHUNK = NICAM_HLIM_CHUNK;  // cells (work-items) per work-group
-constexpr int MAXI = 160;
+constexpr int MAXI_TILE = 160;  // widest region of the common instances' q tile (gl8/rl1 regions are 130)
+constexpr int MAXI = 514;       // widest region of all (gl9/rl0, gl10/rl1), with the wide instances' tile
 
 // A load of a streamed input read once (optionally with the nontemporal hint).
 template <class T>
@@ -91,13 +92,14 @@ __device__ void hlim_pole(const HlimArgs<T> &A, int t, int tr);
 // that their latency overlaps the tile's.
 // Work-group b of one grid's launch (the single-grid kernel passes its block
 // index; the grouped kernel of several grids passes the index within the grid).
-template <class T, bool MULTI, bool PRE = false>
+// TW: the widest region the q tile holds.
+template <class T, bool MULTI, bool PRE = false, int TW = MAXI_TILE>
 __device__ __forceinline__ void hlim_block(const HlimArgs<T> &A, int b)
 {
     // q for cells [g0 - iall - 1, g0 + CHUNK + iall + 2)
     // MULTI keeps two tiles: tracer t+1's tile is loaded into registers while
     // tracer t is limited, and written to the other tile after it
-    constexpr int SPAN = CHUNK + 2 * MAXI + 3;
+    constexpr int SPAN = CHUNK + 2 * TW + 3;
     constexpr int NLD = (SPAN + CHUNK - 1) / CHUNK;
     __shared__ T s_q[MULTI ? 2 : 1][SPAN];
 
@@ -316,10 +318,10 @@ __device__ __forceinline__ void hlim_block(const HlimArgs<T> &A, int b)
     }
 }
 
-template <class T, bool MULTI, bool PRE = false>
+template <class T, bool MULTI, bool PRE = false, int TW = MAXI_TILE>
 __global__ void __launch_bounds__(CHUNK) hlim_kernel(HlimArgs<T> A)
 {
-    hlim_block<T, MULTI, PRE>(A, blockIdx.x);
+    hlim_block<T, MULTI, PRE, TW>(A, blockIdx.x);
 }
 
 // Several grids in one launch: work-groups [off[n], off[n+1]) belong to grid n.
@@ -446,7 +448,7 @@ int hlim_make_args(nicam_ctx c, HlimArgs<T> &A, long &nblocks, T *q_a, T *q_a_pl
                    uint8_t *mask_pl, int ntr)
 {
     const Dims &D = c->d;
-    if (D.gall_1d > MAXI) return fail(NICAM_ERR_ARG, "gall_1d larger than the LDS tiles (160)");
+    if (D.gall_1d > MAXI) return fail(NICAM_ERR_ARG, "gall_1d larger than the LDS tiles (514)");
     if (ntr < 1 || (mask && ntr != 1)) return fail(NICAM_ERR_ARG, "tracer count must be >= 1 (1 with a mask)");
     A.ntr = ntr;
     A.ts = (long)D.nreg();
@@ -497,16 +499,24 @@ int launch_hlim_t(nicam_ctx c, T *q_a, T *q_a_pl, const T *q, const T *q_pl, con
     int rc0 = hlim_make_args(c, A, nblk, q_a, q_a_pl, q, q_pl, d, d_pl, ch, ch_pl, cmask, cmask_pl, Qprev, Qprev_pl,
                              Qpost, Qpost_pl, eps, big, mask, mask_pl, ntr);
     if (rc0 != NICAM_OK) return rc0;
+    // regions wider than the common tile take the wide instances (a larger static tile)
+    const bool wide = D.gall_1d > MAXI_TILE;
     auto run = [&](int phases) -> int {
         A.phases = phases;
-        if (ntr > 1)
-            hipLaunchKernelGGL((hlim_kernel<T, true>), dim3((unsigned)nblk), dim3(CHUNK), 0, c->stream, A);
+        const dim3 grid((unsigned)nblk), blk(CHUNK);
+        if (ntr > 1) {
+            if (wide) hipLaunchKernelGGL((hlim_kernel<T, true, false, MAXI>), grid, blk, 0, c->stream, A);
+            else hipLaunchKernelGGL((hlim_kernel<T, true>), grid, blk, 0, c->stream, A);
+        }
         // FP32: the loads before the barrier (gl7: 344 against 359-369 us); FP64: no difference (673-684 us),
         // the fewer registers
-        else if (c->tune.hlim_variant == 2 || (c->tune.hlim_variant == 0 && std::is_same<T, float>::value))
-            hipLaunchKernelGGL((hlim_kernel<T, false, true>), dim3((unsigned)nblk), dim3(CHUNK), 0, c->stream, A);
-        else
-            hipLaunchKernelGGL((hlim_kernel<T, false>), dim3((unsigned)nblk), dim3(CHUNK), 0, c->stream, A);
+        else if (c->tune.hlim_variant == 2 || (c->tune.hlim_variant == 0 && std::is_same<T, float>::value)) {
+            if (wide) hipLaunchKernelGGL((hlim_kernel<T, false, true, MAXI>), grid, blk, 0, c->stream, A);
+            else hipLaunchKernelGGL((hlim_kernel<T, false, true>), grid, blk, 0, c->stream, A);
+        } else {
+            if (wide) hipLaunchKernelGGL((hlim_kernel<T, false, false, MAXI>), grid, blk, 0, c->stream, A);
+            else hipLaunchKernelGGL((hlim_kernel<T, false>), grid, blk, 0, c->stream, A);
+        }
         NICAM_HIP(hipGetLastError());
         return NICAM_OK;
     };
@@ -594,6 +604,7 @@ int launch_hlim_group_f32(const nicam_ctx *cs, int n, int ntr, const nicam_trace
     for (int i = 0; i < n; ++i) {
         const nicam_tracer_step_f32 &p = g[i];
         if (!p.Qout_post) return fail(NICAM_ERR_ARG, "the grouped limiter needs Qout_post");
+        if (cs[i]->d.gall_1d > MAXI_TILE) return fail(NICAM_ERR_ARG, "gall_1d larger than the grouped limiter's tiles (160)");
         long nb = 0;
         const int rc = hlim_make_args<float>(cs[i], G.a[i], nb, p.q_a, p.q_a_pl, p.q, p.q_pl, p.d, p.d_pl, p.ch,
                                              p.ch_pl, p.cmask, p.cmask_pl, p.Qout_prev, p.Qout_prev_pl, p.Qout_post,

@@ -18,6 +18,7 @@ CASES = [
     (66, 16, 4, 5, [1, 0, 0, 1], True),
     (130, 6, 2, 6, [1, 0], True),
     (10, 6, 2, 7, [0, 0], False),
+    (258, 4, 1, 9, [1], True),  # gl9/rl1 region width: the wide instances
 ]
 IDS = [f"g{c[0]}k{c[1]}l{c[2]}{'' if c[5] else '-nopl'}" for c in CASES]
 
