@@ -19,6 +19,10 @@
   glue_adjoint  the adjoints of the horizontal and vertical glue and of the
             halo fill (COMM_data_transfer), and the h-limiter adjoint's
             two half passes
+  coverage  what the other modes leave out: the flux and h-limiter
+            instances for wide regions (gall_1d 258, 322, 514; FP64 and
+            FP32, one and two tracers), the rho*w adjoint with its column in
+            registers (kall 42) and the multi-tracer vertical limiter
 
 Before the step it launches one plain device-to-device copy of exactly
 --copy-mib MiB (torch's copy kernel), whose FETCH_SIZE / WRITE_SIZE readings
@@ -43,7 +47,7 @@ def main() -> int:
     ap.add_argument("--lall", type=int, default=40)
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--mode", default="step", choices=("step", "chained", "vi", "batched", "extras", "adjoint",
-                                                         "glue_adjoint"))
+                                                         "glue_adjoint", "coverage"))
     ap.add_argument("--extras", action="store_true", help="same as --mode extras")
     ap.add_argument("--adjoint", action="store_true", help="same as --mode adjoint")
     args = ap.parse_args()
@@ -68,6 +72,8 @@ def main() -> int:
     del a, b
     if args.mode == "batched":
         return batched(args.steps)
+    if args.mode == "coverage":
+        return coverage(args.steps)
 
     spec = synth.GridSpec(args.gall_1d, args.kall, args.lall, seed=args.seed)
     lv = synth.icosahedral_levels(spec.gall_1d, spec.lall)
@@ -229,6 +235,70 @@ def glue_adjoint(ctx, spec, case, steps, F, H, up, z) -> int:
         torch.cuda.synchronize()
         print(f"[profile_step] glue adjoint {s + 1}/{steps} done", flush=True)
     ctx.close()
+    return 0
+
+
+def coverage(steps) -> int:
+    """Per step: flux and h-limiter (FP64 and FP32, the limiter for one and
+    for two tracers) on regions 258, 322 and 514 wide, the rho*w adjoint at
+    kall 42 (the register column), the vertical limiter for four tracers."""
+    import numpy as np
+    import torch
+
+    from icoatmosbenchmark_v1_amd import drivers, synth
+
+    dev = torch.device("cuda:0")
+    T = lambda a, dt=torch.float64: torch.from_numpy(np.ascontiguousarray(a)).to(dev, dt)  # noqa: E731
+    grids = []
+    for n, kall, lall in ((258, 16, 8), (322, 16, 4), (514, 16, 1)):
+        spec = synth.GridSpec(n, kall, lall, seed=n)
+        case = synth.make_case(spec)
+        ctx = drivers.setup_context(spec, case["geo"])
+        ctx.set_stream(torch.cuda.current_stream(dev))
+        f = {k: T(case["flux"][k]) for k in drivers.FLUX_IN}
+        h = {k: T(case["hlim"][k]) for k in drivers.HLIM_IN + ("q_a", "q_a_pl")}
+        grids.append((spec, ctx, f, h))
+    specv = synth.GridSpec(66, 42, 10, seed=42)
+    casev = synth.make_case(specv)
+    ctxv = drivers.setup_context(specv, casev["geo"], casev["vi"])
+    ctxv.set_stream(torch.cuda.current_stream(dev))
+    r = lambda shape: torch.randn(shape, dtype=torch.float64, device=dev)  # noqa: E731
+    wb, wb_pl = r(specv.shape3), r(specv.shape_pl)
+    bufs = [torch.zeros(specv.shape3 if i % 2 == 0 else specv.shape_pl, dtype=torch.float64, device=dev)
+            for i in range(12)]
+    ntr = 4
+    v = casev["vlim"]
+    tr = lambda a: T(np.stack([a] * ntr, axis=0))  # Fortran's tracer-last: the outermost C index  # noqa: E731
+    vq_h, vq_h_pl, vq, vq_pl = tr(v["q_h"]), tr(v["q_h_pl"]), tr(v["q"]), tr(v["q_pl"])
+    vd, vd_pl, vck, vck_pl = T(v["d"]), T(v["d_pl"]), T(v["ck"]), T(v["ck_pl"])
+    torch.cuda.synchronize()
+    for s in range(steps):
+        for spec, ctx, f, h in grids:
+            z = lambda shape, dt=torch.float64: torch.zeros(shape, dtype=dt, device=dev)  # noqa: E731
+            for dt in (torch.float64, torch.float32):
+                x = {k: t.to(dt) for k, t in f.items()}
+                flx, flx_pl = z((6,) + spec.shape3, dt), z(spec.shape_pl, dt)
+                xc, xc_pl = z((3, 3) + spec.shape3, dt), z((3,) + spec.shape_pl, dt)
+                run = ctx.horizontal_flux if dt == torch.float64 else ctx.horizontal_flux_f32
+                run(flx, flx_pl, xc, xc_pl, *[x[k] for k in drivers.FLUX_IN], 360.0)
+                y = {k: t.to(dt) for k, t in h.items()}
+                qp, qp_pl = z((2,) + spec.shape3, dt), z((2,) + spec.shape_pl, dt)
+                run = ctx.horizontal_limiter_thuburn if dt == torch.float64 else ctx.horizontal_limiter_thuburn_f32
+                run(y["q_a"], y["q_a_pl"], y["q"], y["q_pl"], y["d"], y["d_pl"], y["ch"], y["ch_pl"], y["cmask"],
+                    y["cmask_pl"], qp, qp_pl, y["Qout_post"], y["Qout_post_pl"])
+            # two tracers (Fortran's tracer-last layout: the outermost C index)
+            two = lambda a: torch.stack([a, a], dim=0).contiguous()  # noqa: E731
+            qp2, qp2_pl = z((2, 2) + spec.shape3), z((2, 2) + spec.shape_pl)
+            ctx.horizontal_limiter_thuburn_batch(2, two(h["q_a"]), two(h["q_a_pl"]), two(h["q"]), two(h["q_pl"]),
+                                                 h["d"], h["d_pl"], h["ch"], h["ch_pl"], h["cmask"], h["cmask_pl"],
+                                                 qp2, qp2_pl, two(h["Qout_post"]), two(h["Qout_post_pl"]))
+        ctxv.vi_rhow_solver_adjoint(wb, wb_pl, *bufs, 60.0)
+        ctxv.vertical_limiter_thuburn_batch(ntr, vq_h, vq_h_pl, vq, vq_pl, vd, vd_pl, vck, vck_pl)
+        torch.cuda.synchronize()
+        print(f"[profile_step] coverage {s + 1}/{steps} done", flush=True)
+    for _, ctx, _, _ in grids:
+        ctx.close()
+    ctxv.close()
     return 0
 
 
