@@ -23,6 +23,9 @@ namespace nicam {
 namespace {
 
 constexpr int BLOCK = 256;
+#ifndef NICAM_VLIM_KC
+#define NICAM_VLIM_KC 4  // levels per run of the single-tracer kernel (0: runtime-length runs of 4); gl7: 4 164 us, 8 168, 16 168, runtime-length 169 (FP64)
+#endif
 
 template <class T>
 __device__ __forceinline__ T dmin(T a, T b) { return (b < a) ? b : a; }
@@ -107,6 +110,71 @@ __device__ __forceinline__ void vlim_run(T *__restrict__ q_h, const T *__restric
         ck1k = ck1kp1;
     }
     if constexpr (MASK) *mask_word = bits;
+}
+
+// vlim_run with the run's length fixed at KC levels at compile time: every
+// input of the run (q on kb-2..ke, ck on kb-1..ke, d on kb-1..ke-1, q_h on
+// kb..ke-1) is loaded before the first level is formed, so a work-item waits
+// for memory once per run rather than once per level, and runs can be long
+// (fewer levels re-read at run edges) without losing loads in flight.  A
+// shorter last run (ke - kb < KC) reads level ke in place of the levels past
+// it and stores nothing there.  Same operations, in the same order, as
+// vlim_run.
+template <bool POLE, class T, int KC>
+__device__ __forceinline__ void vlim_run_kc(T *__restrict__ q_h, const T *__restrict__ q, const T *__restrict__ d,
+                                            const T *__restrict__ ck, size_t base, size_t s, size_t ck2, int kb,
+                                            int ke, T EPS, T BIG)
+{
+    auto at = [&](int k) { return base + (size_t)(k < ke ? k : ke) * s; };
+    T qv[KC + 3], c1[KC + 2], c2[KC + 1], dv[KC + 1], qh[KC];
+#pragma unroll
+    for (int t = 0; t < KC + 3; ++t) qv[t] = q[at(kb - 2 + t)];
+#pragma unroll
+    for (int t = 0; t < KC + 2; ++t) c1[t] = ck[at(kb - 1 + t)];
+#pragma unroll
+    for (int t = 0; t < KC + 1; ++t) {
+        c2[t] = ck[ck2 + at(kb - 1 + t)];
+        dv[t] = d[at(kb - 1 + t)];
+    }
+#pragma unroll
+    for (int t = 0; t < KC; ++t) qh[t] = q_h[at(kb + t)];
+    T Qmin_km1 = T(0), Qmax_km1 = T(0);
+#pragma unroll
+    for (int t = 0; t <= KC; ++t) {  // level k = kb - 1 + t
+        const int k = kb - 1 + t;
+        const T qkm1 = qv[t], qk = qv[t + 1], qkp1 = qv[t + 2];
+        const T ck1k = c1[t], ck1kp1 = c1[t + 1], ck2k = c2[t], dk = dv[t];
+        const T inflagL = T(0.5) - sign_half(ck1k);
+        const T inflagU = T(0.5) + sign_half(ck1kp1);
+        const T Qin_minL = dmin(qk, qkm1) + (T(1) - inflagL) * BIG;
+        const T Qin_minU = dmin(qk, qkp1) + (T(1) - inflagU) * BIG;
+        const T Qin_maxL = dmax(qk, qkm1) - (T(1) - inflagL) * BIG;
+        const T Qin_maxU = dmax(qk, qkp1) - (T(1) - inflagU) * BIG;
+        const T qnext_min = dmin(dmin(Qin_minL, Qin_minU), qk);
+        const T qnext_max = dmax(dmax(Qin_maxL, Qin_maxU), qk);
+        const T Cin = (inflagL)*ck1k + (inflagU)*ck2k;
+        const T Cout = (T(1) - inflagL) * ck1k + (T(1) - inflagU) * ck2k;
+        T CQin_min, CQin_max;
+        if constexpr (POLE) {
+            CQin_max = (inflagL) * (ck1k * Qin_maxL) + (inflagU) * (ck2k * Qin_maxU);
+            CQin_min = (inflagL) * (ck1k * Qin_minL) + (inflagU) * (ck2k * Qin_minU);
+        } else {
+            CQin_min = (inflagL)*ck1k * Qin_minL + (inflagU)*ck2k * Qin_minU;
+            CQin_max = (inflagL)*ck1k * Qin_maxL + (inflagU)*ck2k * Qin_maxU;
+        }
+        const T zerosw = T(0.5) - sign_half(tabs(Cout) - EPS);
+        const T Qmin_k = ((qk - qnext_max) + qnext_max * (Cin + Cout - dk) - CQin_max) / (Cout + zerosw) *
+                                  (T(1) - zerosw) + qk * zerosw;
+        const T Qmax_k = ((qk - qnext_min) + qnext_min * (Cin + Cout - dk) - CQin_min) / (Cout + zerosw) *
+                                  (T(1) - zerosw) + qk * zerosw;
+        if (t >= 1 && k < ke) {  // VLIM:179-180
+            const T qhv = qh[t - 1];
+            const T out = (inflagL)*dmax(dmin(qhv, Qmax_km1), Qmin_km1) + (T(1) - inflagL) * dmax(dmin(qhv, Qmax_k), Qmin_k);
+            stream_store(&q_h[base + (size_t)k * s], out);
+        }
+        Qmin_km1 = Qmin_k;
+        Qmax_km1 = Qmax_k;
+    }
 }
 
 // vlim_run for up to TB tracers (nb of them) of one column run; tracer b's
@@ -267,6 +335,35 @@ __global__ void __launch_bounds__(BLOCK) vlim_kernel(VlimArgs<T> A)
     }
 }
 
+// vlim_kernel with KC-level runs (vlim_run_kc); no mask.
+template <class T, int KC>
+__global__ void __launch_bounds__(BLOCK) vlim_kernel_kc(VlimArgs<T> A)
+{
+    const long ncol_reg = (long)A.gall * A.lall;
+    const long ncol = ncol_reg + (A.have_pl ? (long)A.gall_pl * kLallPl : 0);
+    const long ncb = (ncol + BLOCK - 1) / BLOCK;  // column blocks
+    const long L = xcd_contiguous(blockIdx.x, ncb * A.nrun);
+    const int run = (int)(L % A.nrun);
+    const long col = (L / A.nrun) * BLOCK + threadIdx.x;
+    if (col >= ncol) return;
+    const int kmin = 1, kmax = A.kall - 2;  // 0-based
+    const int kb = kmin + 1 + run * KC;
+    const int ke = min(kmax + 1, kb + KC);
+    if (kb >= ke) return;
+    if (col < ncol_reg) {
+        const int g = (int)(col % A.gall), l = (int)(col / A.gall);
+        const size_t base = (size_t)l * A.kall * A.gall + g;
+        const size_t ck2 = (size_t)A.gall * A.kall * A.lall;
+        vlim_run_kc<false, T, KC>(A.q_h, A.q, A.d, A.ck, base, A.gall, ck2, kb, ke, A.eps, A.big);
+    } else {
+        const int p = (int)(col - ncol_reg);
+        const int g = p % A.gall_pl, l = p / A.gall_pl;
+        const size_t base = (size_t)l * A.kall * A.gall_pl + g;
+        const size_t ck2 = (size_t)A.gall_pl * A.kall * kLallPl;
+        vlim_run_kc<true, T, KC>(A.q_h_pl, A.q_pl, A.d_pl, A.ck_pl, base, A.gall_pl, ck2, kb, ke, A.eps, A.big);
+    }
+}
+
 }  // namespace
 
 namespace {
@@ -301,8 +398,9 @@ int vlim_make_args(nicam_ctx c, VlimArgs<T> &A, long &nblk, T *q_h, T *q_h_pl, c
     nblk = 0;
     const int nlev = D.kall - 3;  // limited levels kmin+1..kmax
     if (nlev <= 0) return NICAM_OK;
-    // short runs give more independent work-items (measured best at gl7: 4)
-    int kc = c->tune.vlim_kchunk > 0 ? c->tune.vlim_kchunk : 4;
+    // runtime-length runs: short runs give more independent work-items (measured best at gl7: 4); the
+    // single-tracer kernel without a mask takes compile-time runs of NICAM_VLIM_KC levels instead
+    int kc = c->tune.vlim_kchunk > 0 ? c->tune.vlim_kchunk : (!multi && !mask && NICAM_VLIM_KC > 0 ? NICAM_VLIM_KC : 4);
     if (mask) kc = 32;  // one mask word per run
     kc = std::min(kc, nlev);
     A.kc = kc;
@@ -326,6 +424,9 @@ int launch_vlim_t(nicam_ctx c, T *q_h, T *q_h_pl, const T *q, const T *q_pl, con
     if (ntr > 1)
         hipLaunchKernelGGL(vlim_multi_kernel<T>, dim3((unsigned)nblk), dim3(BLOCK), 0, c->stream, A);
     else if (mask) hipLaunchKernelGGL((vlim_kernel<T, true>), dim3((unsigned)nblk), dim3(BLOCK), 0, c->stream, A);
+    else if (NICAM_VLIM_KC > 0 && A.kc == NICAM_VLIM_KC)
+        hipLaunchKernelGGL((vlim_kernel_kc<T, (NICAM_VLIM_KC > 0 ? NICAM_VLIM_KC : 1)>), dim3((unsigned)nblk),
+                           dim3(BLOCK), 0, c->stream, A);
     else hipLaunchKernelGGL((vlim_kernel<T, false>), dim3((unsigned)nblk), dim3(BLOCK), 0, c->stream, A);
     NICAM_HIP(hipGetLastError());
     return NICAM_OK;
