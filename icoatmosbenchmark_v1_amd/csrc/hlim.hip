@@ -157,24 +157,40 @@ __device__ __forceinline__ void hlim_block(const HlimArgs<T> &A, int b)
         dc = ld_once(&A.d[fb + g]);
     };
     if constexpr (MULTI || PRE) load_shared();
-    // PRE: the clamp's inputs too (one tracer)
+    // PRE: the clamp's inputs too (one tracer); MULTI: tracer tr+1's clamp inputs are loaded with its
+    // tile, while tracer tr is limited
+    constexpr bool PF = MULTI || PRE;
     T pre_qa[3] = {T(0), T(0), T(0)}, pre_qpn[3][2] = {}, pre_qpc[2] = {T(0), T(0)};
-    if constexpr (PRE) {
+    auto load_clamp = [&](int tr, T (&qa)[3], T (&qpn)[3][2], T (&qpc)[2]) {
         if (own && (A.phases & 2)) {
             const int nb[3] = {g + 1, g + iall + 1, g + iall};
-            pre_qpc[0] = A.Qpost[fb + g];
-            pre_qpc[1] = A.Qpost[plane + fb + g];
+            const T *Qp = A.Qpost + tr * 2 * A.ts;
+            const T *qa_t = A.q_a + tr * 6 * A.ts;
+            qpc[0] = Qp[fb + g];
+            qpc[1] = Qp[plane + fb + g];
 #pragma unroll
             for (int m = 0; m < 3; ++m) {
-                pre_qpn[m][0] = A.Qpost[fb + nb[m]];
-                pre_qpn[m][1] = A.Qpost[plane + fb + nb[m]];
-                pre_qa[m] = ld_once(&A.q_a[(size_t)m * plane + fb + g]);
+                qpn[m][0] = Qp[fb + nb[m]];
+                qpn[m][1] = Qp[plane + fb + nb[m]];
+                qa[m] = ld_once(&qa_t[(size_t)m * plane + fb + g]);
             }
         }
-    }
+    };
+    if constexpr (PF) load_clamp(0, pre_qa, pre_qpn, pre_qpc);
+    T nx_qa[3] = {T(0), T(0), T(0)}, nx_qpn[3][2] = {}, nx_qpc[2] = {T(0), T(0)};
 
     T nxt[NLD];  // the next tracer's share of its tile
     for (int tr = 0; tr < (MULTI ? A.ntr : 1); ++tr) {
+        if (MULTI && tr > 0) {  // this tracer's clamp inputs, loaded during the previous one
+#pragma unroll
+            for (int m = 0; m < 3; ++m) {
+                pre_qa[m] = nx_qa[m];
+                pre_qpn[m][0] = nx_qpn[m][0];
+                pre_qpn[m][1] = nx_qpn[m][1];
+            }
+            pre_qpc[0] = nx_qpc[0];
+            pre_qpc[1] = nx_qpc[1];
+        }
         T *q_a = A.q_a + tr * 6 * A.ts;
         T *Qprev = A.Qprev + tr * 2 * A.ts;
         const T *Qpost = A.Qpost + tr * 2 * A.ts;
@@ -199,6 +215,7 @@ __device__ __forceinline__ void hlim_block(const HlimArgs<T> &A, int b)
                 const int gg = lo + tid + u * CHUNK;
                 nxt[u] = (tid + u * CHUNK < n_in && gg >= 0 && gg < gall) ? qn[fb + gg] : T(0);
             }
+            load_clamp(tr + 1, nx_qa, nx_qpn, nx_qpc);
         }
         if (!active) continue;
 
@@ -286,8 +303,8 @@ __device__ __forceinline__ void hlim_block(const HlimArgs<T> &A, int b)
         if (own && (A.phases & 2)) {
             uint32_t bits = 0;
             const int nb[3] = {g + 1, g + iall + 1, g + iall};
-            const T qo_min_c = PRE ? pre_qpc[0] : Qpost[fb + g];  // re-read by neighbours
-            const T qo_max_c = PRE ? pre_qpc[1] : Qpost[plane + fb + g];
+            const T qo_min_c = PF ? pre_qpc[0] : Qpost[fb + g];  // re-read by neighbours
+            const T qo_max_c = PF ? pre_qpc[1] : Qpost[plane + fb + g];
 #pragma unroll
             for (int m = 0; m < 3; ++m) {
                 const T c = ocm[m];
@@ -295,9 +312,9 @@ __device__ __forceinline__ void hlim_block(const HlimArgs<T> &A, int b)
                 const T in_mx_s = (c)*omx[m] + (T(1) - c) * (-BIG);
                 const T in_mn_n = (c)*BIG + (T(1) - c) * omn[m];
                 const T in_mx_n = (c) * (-BIG) + (T(1) - c) * omx[m];
-                const T qo_min_n = PRE ? pre_qpn[m][0] : Qpost[fb + nb[m]];
-                const T qo_max_n = PRE ? pre_qpn[m][1] : Qpost[plane + fb + nb[m]];
-                T qa = PRE ? pre_qa[m] : ld_once(&q_a[(size_t)m * plane + fb + g]);
+                const T qo_min_n = PF ? pre_qpn[m][0] : Qpost[fb + nb[m]];
+                const T qo_max_n = PF ? pre_qpn[m][1] : Qpost[plane + fb + nb[m]];
+                T qa = PF ? pre_qa[m] : ld_once(&q_a[(size_t)m * plane + fb + g]);
                 const T qa_in = qa;
                 qa = (c)*dmin(dmax(qa, in_mn_s), in_mx_s) + (T(1) - c) * dmin(dmax(qa, in_mn_n), in_mx_n);
                 qa = (c)*dmax(dmin(qa, qo_max_n), qo_min_n) + (T(1) - c) * dmax(dmin(qa, qo_max_c), qo_min_c);
