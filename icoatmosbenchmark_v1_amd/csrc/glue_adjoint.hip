@@ -37,7 +37,8 @@ struct GradAdjArgs {
 
 __global__ void __launch_bounds__(BLK) gradient_adjoint_kernel(GradAdjArgs A)
 {
-    const long t = (long)blockIdx.x * BLK + threadIdx.x;
+    // consecutive work-groups on one XCD: the rows above and below and the region's coefficients stay in its L2
+    const long t = xcd_contiguous(blockIdx.x, gridDim.x) * BLK + threadIdx.x;
     const int iall = A.iall, gall = A.gall, kall = A.kall;
     if (t < A.nreg) {
         const int g = (int)(t % gall);
@@ -120,7 +121,8 @@ struct RemapAdjArgs {
 
 __global__ void __launch_bounds__(BLK) remap_adjoint_kernel(RemapAdjArgs A)
 {
-    const long t = (long)blockIdx.x * BLK + threadIdx.x;
+    // consecutive work-groups on one XCD: the row below, read by the neighbour arcs, stays in its L2
+    const long t = xcd_contiguous(blockIdx.x, gridDim.x) * BLK + threadIdx.x;
     const int iall = A.iall, gall = A.gall, kall = A.kall, lall = A.lall;
     const long n = A.nreg;
     if (t < n) {
@@ -169,13 +171,13 @@ __global__ void __launch_bounds__(BLK) remap_adjoint_kernel(RemapAdjArgs A)
                 }
             }
         }
-        A.qb[t] = qb;
+        stream_store(&A.qb[t], qb);
 #pragma unroll
-        for (int d = 0; d < 3; ++d) A.gb[d * n + t] = gb[d];
+        for (int d = 0; d < 3; ++d) stream_store(&A.gb[d * n + t], gb[d]);
 #pragma unroll
         for (int a = 0; a < 3; ++a)
 #pragma unroll
-            for (int d = 0; d < 3; ++d) A.xb[(d * 3 + a) * n + t] = xb[a][d];
+            for (int d = 0; d < 3; ++d) stream_store(&A.xb[(d * 3 + a) * n + t], xb[a][d]);
         return;
     }
     const long p = t - n;  // one work-item per pole level, in the oracle's order
