@@ -44,7 +44,7 @@ constexpr int BLOCK_VA = 256;
 template <bool FACT>
 __global__ void __launch_bounds__(BLOCK_VA) vi_adjoint_kernel(VaArgs A)
 {
-    column_run<BLOCK_VA>(A, blockIdx.x, [&](const VaField &F, size_t base, size_t s) {
+    column_run<BLOCK_VA>(A, xcd_contiguous(blockIdx.x, gridDim.x), [&](const VaField &F, size_t base, size_t s) {
         const int kall = A.kall, kmin = 1, kmax = kall - 2;  // 0-based
         const double dt = A.dt, CV = A.CVovRt2, grav = A.grav;
         auto at = [&](int k) { return base + (size_t)k * s; };
@@ -124,7 +124,7 @@ constexpr int VC = 8;
 
 __global__ void __launch_bounds__(BLOCK_VA) vi_adjoint_kernel_chunked(VaArgs A)
 {
-    column_run<BLOCK_VA>(A, blockIdx.x, [&](const VaField &F, size_t base, size_t s) {
+    column_run<BLOCK_VA>(A, xcd_contiguous(blockIdx.x, gridDim.x), [&](const VaField &F, size_t base, size_t s) {
         const int kall = A.kall, kmin = 1, kmax = kall - 2;  // 0-based
         const double dt = A.dt, CV = A.CVovRt2, grav = A.grav;
         auto at = [&](int k) { return base + (size_t)k * s; };
@@ -260,7 +260,7 @@ constexpr int BLOCK_VAR = 64;
 template <int KC>
 __global__ void __launch_bounds__(BLOCK_VAR) vi_adjoint_kernel_reg(VaArgs A)
 {
-    column_run<BLOCK_VAR>(A, blockIdx.x, [&](const VaField &F, size_t base, size_t s) {
+    column_run<BLOCK_VAR>(A, xcd_contiguous(blockIdx.x, gridDim.x), [&](const VaField &F, size_t base, size_t s) {
         constexpr int kmin = 1, kmax = KC - 2;
         const double dt = A.dt, CV = A.CVovRt2, grav = A.grav;
         auto at = [&](int k) { return base + (size_t)k * s; };

@@ -32,6 +32,9 @@ inline long column_blocks(long gall, long lall, long gall_pl, bool have_pl)
 {
     return (gall * lall + BLOCK - 1) / BLOCK + (have_pl ? (gall_pl * kLallPl + BLOCK - 1) / BLOCK : 0);
 }
+// blk: the work-group's index in its grid; single-grid launches pass
+// xcd_contiguous(blockIdx.x, gridDim.x), so that neighbouring column blocks,
+// which share the cache lines at each level row's ends, run on one XCD.
 template <int BLOCK, class Args, class Body>
 __device__ __forceinline__ void column_run(const Args &A, long blk, Body &&body)
 {

@@ -201,7 +201,7 @@ template <class Src>
 __global__ void __launch_bounds__(BLOCK_LDS) vi_kernel_lds(ViArgsT<Src> A)
 {
     extern __shared__ double s_gam[];
-    column_run<BLOCK_LDS>(A, blockIdx.x, [&](const ViFieldT<Src> &F, size_t base, size_t s) {
+    column_run<BLOCK_LDS>(A, xcd_contiguous(blockIdx.x, gridDim.x), [&](const ViFieldT<Src> &F, size_t base, size_t s) {
         LdsGamma gam{s_gam + threadIdx.x, BLOCK_LDS};
         int unused = 0;
         vi_column<0, false, false>(F, base, s, A.kall, A.rdgzh, A.afact, A.bfact, A.dt, A.CVovRt2, A.alpha, A.grav,
@@ -213,7 +213,7 @@ __global__ void __launch_bounds__(BLOCK_LDS) vi_kernel_lds(ViArgsT<Src> A)
 template <class Src>
 __global__ void __launch_bounds__(BLOCK_FAC) vi_kernel_fact(ViArgsT<Src> A)
 {
-    column_run<BLOCK_FAC>(A, blockIdx.x, [&](const ViFieldT<Src> &F, size_t base, size_t s) {
+    column_run<BLOCK_FAC>(A, xcd_contiguous(blockIdx.x, gridDim.x), [&](const ViFieldT<Src> &F, size_t base, size_t s) {
         NoGamma gam;
         int unused = 0;
         vi_column<0, false, true>(F, base, s, A.kall, A.rdgzh, A.afact, A.bfact, A.dt, A.CVovRt2, A.alpha, A.grav,
@@ -264,7 +264,7 @@ __global__ void __launch_bounds__(BLOCK_REG) vi_kernel_reg(ViArgsT<Src> A)
         // FP64 sources: the field set chosen per work-item, its pointers in vector registers, schedules
         // better here (gl7, variant 4: 368 us against 401 us with the work-group's)
         // (one copy of the column, on a per-work-item pointer)
-        const long t = (long)blockIdx.x * BLOCK_REG + threadIdx.x;
+        const long t = xcd_contiguous(blockIdx.x, gridDim.x) * BLOCK_REG + threadIdx.x;
         const long nreg = (long)A.gall * A.lall;
         const ViFieldT<Src> *F;
         size_t base, s;
@@ -282,7 +282,7 @@ __global__ void __launch_bounds__(BLOCK_REG) vi_kernel_reg(ViArgsT<Src> A)
         }
         column(*F, base, s);
     } else {
-        column_run<BLOCK_REG>(A, blockIdx.x, column);  // FP32 sources: 309 us against 550 us per work-item
+        column_run<BLOCK_REG>(A, xcd_contiguous(blockIdx.x, gridDim.x), column);  // FP32 sources: 309 us against 550 us per work-item
     }
 }
 

@@ -256,3 +256,35 @@ def test_widths_beyond_the_tiles_are_refused(gpu):
         assert e.value.status == NICAM_ERR_ARG and "514" in str(e.value)
     got = drivers.run_vlim(ctx, spec, case["vlim"])  # no width limit
     assert np.isfinite(got["q_h"]).all()
+
+
+def test_adjoint_widths_beyond_their_tiles_are_refused(gpu):
+    """The flux and h-limiter adjoints keep 160-cell tiles: at gall_1d = 258,
+    which the forwards take, they fail with NICAM_ERR_ARG before launching
+    anything."""
+    import torch
+
+    from icoatmosbenchmark_v1_amd.runtime import NICAM_ERR_ARG, NicamError
+
+    spec = synth.GridSpec(258, 4, 1, seed=3)
+    case = synth.make_case(spec, have_sgp=[0])
+    ctx = drivers.setup_context(spec, case["geo"])
+    z = lambda sh: torch.zeros(sh, dtype=torch.float64, device=gpu)  # noqa: E731
+    T = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(gpu)  # noqa: E731
+    F = {k: T(case["flux"][k]) for k in drivers.FLUX_IN}
+    fb = [z((6,) + spec.shape3), z(spec.shape_pl), z((3, 3) + spec.shape3), z((3,) + spec.shape_pl)]
+    xb = [z(spec.shape3 if i % 2 == 0 else spec.shape_pl) for i in range(8)]
+    H = {k: T(case["hlim"][k]) for k in drivers.HLIM_IN + ("q_a", "q_a_pl")}
+    hm = torch.zeros(ctx.nreg, dtype=torch.uint8, device=gpu)
+    hm_pl = torch.zeros(ctx.npl, dtype=torch.uint8, device=gpu)
+    for run in (lambda: ctx.horizontal_flux_adjoint(*fb, *xb, *[F[k] for k in drivers.FLUX_IN], 360.0),
+                lambda: ctx.horizontal_limiter_thuburn_adjoint(
+                    z((6,) + spec.shape3), z(spec.shape_pl), z(spec.shape3), z(spec.shape_pl),
+                    z((2,) + spec.shape3), z((2,) + spec.shape_pl), z((2,) + spec.shape3), z((2,) + spec.shape_pl),
+                    H["q_a"], H["q_a_pl"], H["q"], H["q_pl"], H["d"], H["d_pl"], H["ch"], H["ch_pl"], H["cmask"],
+                    H["cmask_pl"], H["Qout_post"], H["Qout_post_pl"], hm, hm_pl)):
+        with pytest.raises(NicamError) as e:
+            run()
+        assert e.value.status == NICAM_ERR_ARG and "160" in str(e.value)
+    got = drivers.run_flux(ctx, spec, case["flux"], 360.0)  # the forward takes the width
+    assert np.isfinite(got["flx_h"]).all()
