@@ -1146,17 +1146,8 @@ struct FaArgs {
 // The cotangents of one origin's arcs that HBM supplies: flx_h_b of its own
 // face and of the neighbour's mirrored face, and GRD_xc_b (loaded a step
 // ahead of their use).
-#ifndef NICAM_HFLUXADJ_PREFETCH
-#define NICAM_HFLUXADJ_PREFETCH 1  // 1: the next step's arc cotangents are loaded before the gather
-#endif
-#ifndef NICAM_HFLUXADJ_METRICS
-#define NICAM_HFLUXADJ_METRICS 0  // 1: the arcs' metrics (triangle weights, normals, areas) are loaded a step ahead too
-#endif
 struct ArcIn {
     double fa[3], fn[3], xb[3][3];
-#if NICAM_HFLUXADJ_METRICS
-    double w[12], hn[3][3], ra, rn[3];  // W1..W3 of TI(a0), TJ(a0), TJ(a0-iall), TI(a0-1); GMTR_a; GMTR_p * dt
-#endif
 };
 
 // R: the ring size, fixed at compile time so that a ring slot is one wrapped
@@ -1195,23 +1186,7 @@ __global__ void __launch_bounds__(FSW) __attribute__((amdgpu_waves_per_eu(2, 8))
             in.fn[a - 1] = A.flxb[(size_t)(a + 2) * plane + fb + nb];
 #pragma unroll
             for (int dd = 0; dd < 3; ++dd) in.xb[a - 1][dd] = A.xcb[(size_t)(dd * 3 + (a - 1)) * plane + fb + g];
-#if NICAM_HFLUXADJ_METRICS
-#pragma unroll
-            for (int dd = 0; dd < 3; ++dd) in.hn[a - 1][dd] = A.ga[(size_t)(dd * 3 + (a - 1)) * mstr + mb + g];
-            in.rn[a - 1] = A.gp[1 * mstr + mb + nb] * dt;
-#endif
         }
-#if NICAM_HFLUXADJ_METRICS
-        const int c0 = cl(a0), cs = cl(a0 - iall), cw = cl(a0 - 1);
-#pragma unroll
-        for (int w = 0; w < 3; ++w) {
-            in.w[w] = W(c0, 0, 2 + w);
-            in.w[3 + w] = W(c0, 1, 2 + w);
-            in.w[6 + w] = W(cs, 1, 2 + w);
-            in.w[9 + w] = W(cw, 0, 2 + w);
-        }
-        in.ra = A.gp[1 * mstr + mb + c0] * dt;
-#endif
     };
     // the three arcs of origin a0 (slot s, position (i, j); zero where a0 is no origin).  GEN: a0 may
     // be 0 or 1 (the singular-point substitution TI(0) = TJ(1)) or lie in row 0 (no TJ(a0-iall));
@@ -1225,18 +1200,11 @@ __global__ void __launch_bounds__(FSW) __attribute__((amdgpu_waves_per_eu(2, 8))
             // They touch six cells (a0, its E, N, NE, S, W neighbours): each field is read there once,
             // and the sums are formed in the forward's order (FLX:139-201).
             const bool hs = !GEN || j >= 1, hw = i >= 1;
-#if NICAM_HFLUXADJ_METRICS
-            const double wi1 = in.w[0], wi2 = in.w[1], wi3 = in.w[2];
-            const double wj1 = in.w[3], wj2 = in.w[4], wj3 = in.w[5];
-            const double ws1 = in.w[6], ws2 = in.w[7], ws3 = in.w[8];
-            const double ww1 = in.w[9], ww2 = in.w[10], ww3 = in.w[11];
-#else
             const int c0 = cl(a0), cs = cl(a0 - iall), cw = cl(a0 - 1);
             const double wi1 = W(c0, 0, 2), wi2 = W(c0, 0, 3), wi3 = W(c0, 0, 4);
             const double wj1 = W(c0, 1, 2), wj2 = W(c0, 1, 3), wj3 = W(c0, 1, 4);
             const double ws1 = W(cs, 1, 2), ws2 = W(cs, 1, 3), ws3 = W(cs, 1, 4);
             const double ww1 = W(cw, 0, 2), ww2 = W(cw, 0, 3), ww3 = W(cw, 0, 4);
-#endif
             const int qe = up(s + 1), qn = up(s + iall), qne = up(s + iall + 1), qs = dn(s - iall), qw = dn(s - 1);
 #pragma unroll
             for (int f = 0; f < 4; ++f) {
@@ -1262,11 +1230,7 @@ __global__ void __launch_bounds__(FSW) __attribute__((amdgpu_waves_per_eu(2, 8))
                 tIw[f] = (org && i >= 1) ? tri(f, 0, a0 - 1) : 0.0;
             }
         }
-#if NICAM_HFLUXADJ_METRICS
-        const double ra = in.ra;
-#else
         const double ra = A.gp[1 * mstr + mb + cl(a0)] * dt;
-#endif
 #pragma unroll
         for (int a = 1; a <= 3; ++a) {
             // every lane forms the arc; those off the origins store zeros
@@ -1279,22 +1243,14 @@ __global__ void __launch_bounds__(FSW) __attribute__((amdgpu_waves_per_eu(2, 8))
                 s2[f] = a == 1 ? tJs[f] + tI[f] : (a == 2 ? tI[f] + tJ[f] : tJ[f] + tIw[f]);
             const double rr = 1.0 / dmx(s2[0], EPS);
             const bool live_rho = !(EPS > s2[0]);
-#if NICAM_HFLUXADJ_METRICS
-            const double rn = in.rn[a - 1];
-#else
             const double rn = A.gp[1 * mstr + mb + cl(nb)] * dt;
-#endif
             const bool sgp_face = sgp && a == 3 && nb == iall + 1;
             const double fb_nb = sgp_face ? 0.0 : in.fn[a - 1];
             const double fluxb = in.fa[a - 1] * ra - fb_nb * rn;
             double bv[4], r2b = 0.0;
 #pragma unroll
             for (int dd = 0; dd < 3; ++dd) {
-#if NICAM_HFLUXADJ_METRICS
-                const double hn = in.hn[a - 1][dd];
-#else
                 const double hn = A.ga[(size_t)(dd * 3 + (a - 1)) * mstr + mb + g];
-#endif
                 const double xb = in.xb[a - 1][dd];
                 bv[1 + dd] = 0.5 * hn * fluxb - xb * rr * dt * 0.5;
                 r2b = r2b + xb * s2[1 + dd] * dt * 0.5 * rr * rr;
@@ -1409,27 +1365,16 @@ __global__ void __launch_bounds__(FSW) __attribute__((amdgpu_waves_per_eu(2, 8))
     int ia = tid % iall, ja = tid / iall + 1;  // origin iall + tid
     int ic = tid % iall, jc = tid / iall;      // cell tid
     int sa = up(iall + tid), sc = tid;         // their slots
-#if NICAM_HFLUXADJ_PREFETCH
-    ArcIn cur;
+    ArcIn cur;  // the arc cotangents of this step's origins, loaded during the previous step
     arcin(iall + tid, cur);
-#endif
     for (int b0 = 0; b0 < gall; b0 += FSW) {
         const int a0 = b0 + iall + tid;
         double xn[4], w[6];
         xload(b0 + R + tid, xn);
         cellw(b0 + tid, w);
-#if NICAM_HFLUXADJ_PREFETCH
         arcs(std::false_type{}, a0, sa, ia, ja, cur);
         adv(ia, ja);
         arcin(a0 + FSW, cur);
-#else
-        {
-            ArcIn in;
-            arcin(a0, in);
-            arcs(std::false_type{}, a0, sa, ia, ja, in);
-        }
-        adv(ia, ja);
-#endif
         __syncthreads();
         if (b0 + tid < gall) cell(b0 + tid, sc, ic, jc, w);
         xstore(sc, xn);  // cell b0 + R + tid has cell b0 + tid's slot
