@@ -46,12 +46,13 @@ def test_group_rejects_bad_sizes():
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("single", [False, True])
-def test_gpu_batched_limiters_equal_single_tracer_calls(gpu, single):
+@pytest.mark.parametrize("grid", [GRIDS[2], (258, 4, 1, 64, [1], True)], ids=["g34", "g258-wide-tiles"])
+def test_gpu_batched_limiters_equal_single_tracer_calls(gpu, single, grid):
     import torch
 
     dt = torch.float32 if single else torch.float64
     sfx = "_f32" if single else ""
-    spec, case = _grid(GRIDS[2])
+    spec, case = _grid(grid)
     ctx = drivers.setup_context(spec, case["geo"])
     up = lambda x: torch.from_numpy(np.ascontiguousarray(x)).to(gpu, dt)  # noqa: E731
     h, v = case["hlim"], case["vlim"]
