@@ -174,6 +174,12 @@ int nicam_set_vmetrics3d(nicam_ctx ctx,
 
 /* ---- hot path ---------------------------------------------------------- */
 
+/* Region widths: the flux and the h-limiter (and their _f32 and batched
+ * forms) take gall_1d up to 514 (gl9/rl0, gl10/rl1); the v-limiter and the
+ * rho*w solve have no limit; the step glue's remap, the grouped limiter and
+ * the flux and h-limiter adjoints take gall_1d up to 160, the operators up to
+ * 255.  Wider regions fail with NICAM_ERR_ARG before anything is launched. */
+
 /* horizontal_flux (dyn_horiz_adv_flux/src/mod_src_tracer.f90:42-379).
  *   out flx_h(gall,kall,lall,6), flx_h_pl(gall_pl,kall,2)
  *   out GRD_xc(gall,kall,lall,3,3), GRD_xc_pl(gall_pl,kall,2,3)
