@@ -118,8 +118,8 @@ def _mp_worker(regions):
 
 
 def cpu_baseline(spec, case, regions: int, reps: int = 3):
-    """The reference's own Fortran kernels (oracle/_ref) on the GPU box's
-    host, on all 40 regions: one thread (the reference's documented runs), its
+    """The reference's own Fortran kernels (oracle/_ref) on the host cores of
+    the machine the GPU sits in, on all 40 regions: one thread (the reference's documented runs), its
     -fopenmp build on every host core, and the regions split over one process
     per core (as NICAM's MPI decomposition splits them), each running the
     single-thread build.  The reported value is the fastest; the C
