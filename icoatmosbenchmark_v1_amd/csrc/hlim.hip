@@ -525,9 +525,9 @@ int launch_hlim_t(nicam_ctx c, T *q_a, T *q_a_pl, const T *q, const T *q_pl, con
             if (wide) hipLaunchKernelGGL((hlim_kernel<T, true, false, MAXI>), grid, blk, 0, c->stream, A);
             else hipLaunchKernelGGL((hlim_kernel<T, true>), grid, blk, 0, c->stream, A);
         }
-        // FP32: the loads before the barrier (gl7: 344 against 359-369 us); FP64: no difference (673-684 us),
-        // the fewer registers
-        else if (c->tune.hlim_variant == 2 || (c->tune.hlim_variant == 0 && std::is_same<T, float>::value)) {
+        // the loads before the barrier (gl7 FP32: 344 against 359-369 us; FP64, in the step after the flux:
+        // 710 against 719 us); variant 1 keeps them after it (fewer registers live across the barrier)
+        else if (c->tune.hlim_variant != 1) {
             if (wide) hipLaunchKernelGGL((hlim_kernel<T, false, true, MAXI>), grid, blk, 0, c->stream, A);
             else hipLaunchKernelGGL((hlim_kernel<T, false, true>), grid, blk, 0, c->stream, A);
         } else {

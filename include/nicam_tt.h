@@ -92,7 +92,7 @@ int nicam_ctx_set_pointer_mode(nicam_ctx ctx, int mode);
  *   NICAM_TUNE_DDMP_KPER    levels per work-group of nicam_oprt3d_divdamp
  *   NICAM_TUNE_HLIM_VARIANT horizontal limiter, one tracer: 1 = the streamed
  *                           inputs loaded after the tracer tile's barrier,
- *                           2 = before it; 0 = automatic
+ *                           2 = before it; 0 = automatic (2)
  * Results do not depend on these settings (every variant computes the same
  * floating-point operations in the same order). */
 enum nicam_tuning {
