@@ -46,15 +46,19 @@ __global__ void __launch_bounds__(BLK) gradient_adjoint_kernel(GradAdjArgs A)
         const int l = (int)(lk / kall);
         const int gmin = iall + 1, gmax = gall - iall - 2;  // 0-based flat range of the forward
         const int off[7] = {0, 1, iall + 1, iall, -1, -iall - 1, -iall};
+        // every term is loaded (addresses clamped into the range) and the ones outside it add +0.0: the
+        // sum starts at +0.0 and so never holds -0.0, and adding +0.0 leaves it as it is
         double s = 0.0;
 #pragma unroll
         for (int d = 0; d < 3; ++d)
 #pragma unroll
             for (int m = 0; m < 7; ++m) {
                 const int gp = g - off[m];
-                if (gp >= gmin && gp <= gmax)
-                    s = s + A.cg[(((size_t)l * 3 + d) * 7 + m) * gall + gp] *
-                                A.gb[(size_t)d * A.nreg + (size_t)lk * gall + gp];
+                const bool ok = gp >= gmin && gp <= gmax;
+                const int gc = gp < gmin ? gmin : (gp > gmax ? gmax : gp);
+                const double term = A.cg[(((size_t)l * 3 + d) * 7 + m) * gall + gc] *
+                                    A.gb[(size_t)d * A.nreg + (size_t)lk * gall + gc];
+                s = s + (ok ? term : 0.0);
             }
         A.sb[t] = s;
         return;
